@@ -1,0 +1,240 @@
+/*
+ * vitrum_hip.h -- C ABI of the MI355X path-tracing backend (libvthip.so).
+ *
+ * The reference's plug-in boundary is the TypeScript `Engine` contract
+ * (packages/core/src/engine/index.ts:109-486) created by an `EngineFactory`
+ * (packages/core/src/engine/factory.ts) and registered in BACKEND_CONSTRUCTORS
+ * (packages/engine/src/createEngine.ts:99-103).  These entry points are what a
+ * Node N-API binding of that contract calls; each cites the reference
+ * interface it replaces.
+ *
+ * Conventions
+ *  - Every function returns vt_status (0 = success, negative = error).  The
+ *    message of the last failure on an engine is available through
+ *    vt_last_error(); engine-less functions report through vt_last_error(NULL).
+ *  - Host pointers are borrowed for the duration of the call only.
+ *  - "_device" entry points take device pointers and enqueue work on the
+ *    engine's HIP stream without synchronising; everything else is blocking.
+ *  - Arguments are validated before any state changes (the reference throws
+ *    TypeError/RangeError before mutation, buildArrayBvh.ts:348-408).
+ *  - One engine per thread at a time; calls on one engine must not overlap.
+ */
+#ifndef VITRUM_HIP_H
+#define VITRUM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t vt_status;
+enum {
+  VT_OK = 0,
+  VT_ERR_INVALID_ARGUMENT = -1, /* TypeError in the reference */
+  VT_ERR_RANGE = -2,            /* RangeError in the reference */
+  VT_ERR_BUILD = -3,            /* Error thrown by a builder/validator */
+  VT_ERR_HIP = -4,              /* EngineError{kind:'gpu-internal'} */
+  VT_ERR_NO_DEVICE = -5,        /* EngineError{kind:'device-lost'} at creation */
+  VT_ERR_STATE = -6,            /* call made in the wrong engine state */
+  VT_ERR_OUT_OF_MEMORY = -7
+};
+
+typedef struct vt_engine vt_engine;
+
+/* ---- rays and hits ------------------------------------------------------- */
+
+/* 32-byte ray: origin, tmin, unit direction, tmax. */
+typedef struct vt_ray {
+  float o[3];
+  float tmin;
+  float d[3];
+  float tmax;
+} vt_ray;
+
+/* 20-byte closest-hit record.  prim is the global packed (BVH-reordered)
+ * triangle index and inst the packed instance index, the pair
+ * SceneHit{triIndex, instanceIndex} of
+ * packages/pt-webgpu/src/wgsl/pathTrace/intersectionCore.wgsl.ts:33-60;
+ * u, v are its baryVW.  A miss has inst == 0xFFFFFFFF and t == +inf. */
+typedef struct vt_hit {
+  uint32_t prim;
+  uint32_t inst;
+  float t;
+  float u;
+  float v;
+} vt_hit;
+
+/* ---- scene description (packages/core/src/scene/index.ts:42) ----------- */
+
+enum { VT_PRIM_MESH = 0, VT_PRIM_INSTANCED_MESH = 1 };
+
+/* Mesh / instanced mesh primitive (packages/core/src/scene/primitives.ts:
+ * 167-227).  positions/normals are xyz float triples; indices are optional
+ * (NULL = generated 0..vertexCount-1); transforms are column-major mat4s (a
+ * mesh takes 0 or 1, an instanced mesh one per instance). */
+typedef struct vt_primitive {
+  uint32_t kind;
+  uint32_t material;
+  const float *positions;
+  uint32_t position_floats;
+  const float *normals;
+  uint32_t normal_floats;
+  const uint32_t *indices;
+  uint32_t index_count;
+  const float *transforms;
+  uint32_t transform_count;
+} vt_primitive;
+
+/* MaterialSpec subset rendered by the Lambert + GGX base BSDF
+ * (packages/core/src/scene/material.ts:167-458). */
+typedef struct vt_material {
+  float base_color[3];
+  float roughness;
+  float metallic;
+  float emissive[3];
+  float emissive_intensity;
+  uint32_t double_sided;
+} vt_material;
+
+/* Environment (packages/core/src/scene/environment.ts:16-37): kind 0 = none,
+ * 1 = constant radiance (a constant-texel HDRI). */
+typedef struct vt_environment {
+  uint32_t kind;
+  float radiance[3];
+} vt_environment;
+
+typedef struct vt_scene_desc {
+  const vt_primitive *primitives;
+  uint32_t primitive_count;
+  const vt_material *materials;
+  uint32_t material_count;
+  vt_environment environment;
+  float scene_radius; /* ray-origin bias scale (sceneScalePolicy.ts:83-123);
+                         0 = derive from the scene bounds */
+} vt_scene_desc;
+
+/* ---- engine lifecycle ---------------------------------------------------- */
+
+/* EngineOptions (packages/core/src/engine/factory.ts) + the pt-webgpu
+ * options this backend honours (packages/pt-webgpu/src/index.ts:219-364). */
+typedef struct vt_options {
+  uint32_t max_bounces;           /* 0 = default 8 */
+  uint32_t max_samples_per_pixel; /* 0 = unlimited */
+  uint32_t sampling;              /* 0 = pcg, 1 = sobol */
+  uint32_t reserved[5];
+} vt_options;
+
+/* createPTEngine_WebGPU (packages/pt-webgpu/src/index.ts:2363).  The device
+ * ordinal is chosen by the host; the engine never resets the device. */
+vt_status vt_create(int32_t device_ordinal, const vt_options *opts, vt_engine **out);
+/* Engine.dispose (packages/core/src/engine/index.ts:348).  NULL is a no-op. */
+vt_status vt_destroy(vt_engine *eng);
+/* Copies the last error message (NUL-terminated, truncated to len). */
+vt_status vt_last_error(const vt_engine *eng, char *buf, size_t len);
+
+/* Engine.setScene (packages/core/src/engine/index.ts:120;
+ * packages/pt-webgpu/src/index.ts:1120).  Packs and builds on the host (BLAS
+ * per primitive, TLAS over instances), converts to the device layout and
+ * uploads.  Transactional: on failure the previous scene stays bound. */
+vt_status vt_set_scene(vt_engine *eng, const vt_scene_desc *scene);
+
+/* ---- traversal (the hot path) -------------------------------------------- */
+
+/* traceTlasClosest (packages/shared-bvh/src/wgsl/tlasSceneHitTraversal.wgsl.ts:
+ * 185) with the pt-webgpu instance augmentation (packages/pt-webgpu/src/wgsl/
+ * pathTrace/intersection.wgsl.ts:31-152).  Host buffers; blocking. */
+vt_status vt_trace_closest(vt_engine *eng, const vt_ray *rays, uint32_t n, vt_hit *hits);
+/* traceTlasAny (tlasSceneHitTraversal.wgsl.ts:227): occluded[i] = 0 or 1. */
+vt_status vt_trace_any(vt_engine *eng, const vt_ray *rays, uint32_t n, uint32_t *occluded);
+/* Device-pointer variants, asynchronous on the engine stream. */
+vt_status vt_trace_closest_device(vt_engine *eng, const vt_ray *d_rays, uint32_t n,
+                                  vt_hit *d_hits);
+vt_status vt_trace_any_device(vt_engine *eng, const vt_ray *d_rays, uint32_t n,
+                              uint32_t *d_occluded);
+/* Work counters of the last closest-hit launch made with counting enabled
+ * (vt_set_counting).  counts: 4 u32 per ray (BLAS boxes tested, triangles
+ * tested, TLAS boxes tested, instances entered), host buffer. */
+vt_status vt_set_counting(vt_engine *eng, int32_t enabled);
+vt_status vt_trace_closest_counts(vt_engine *eng, const vt_ray *rays, uint32_t n, vt_hit *hits,
+                                  uint32_t *counts);
+
+/* Traversal scheduling knobs (no counterpart in the reference): a wave is
+ * refilled once `refill_min` (1..64) of its lanes are idle, and lanes run up
+ * to `steps` traversal steps between refill checks. */
+vt_status vt_set_trace_tuning(vt_engine *eng, uint32_t refill_min, uint32_t steps);
+
+/* Test hook (no counterpart in the reference): q = a / b and r = sqrt(a)
+ * computed on the device, to check that the device's binary32 divide and
+ * square root round exactly like the host's. Host buffers. */
+vt_status vt_selftest_fp32(vt_engine *eng, const float *a, const float *b, float *q, float *r,
+                           uint32_t n);
+
+/* ---- device memory and stream plumbing ----------------------------------- */
+
+vt_status vt_device_alloc(vt_engine *eng, size_t bytes, void **d_ptr);
+vt_status vt_device_free(vt_engine *eng, void *d_ptr);
+vt_status vt_memcpy_h2d(vt_engine *eng, void *d_dst, const void *src, size_t bytes);
+vt_status vt_memcpy_d2h(vt_engine *eng, void *dst, const void *d_src, size_t bytes);
+vt_status vt_synchronize(vt_engine *eng);
+/* The engine's hipStream_t, for callers that order their own work with it. */
+vt_status vt_get_stream(vt_engine *eng, void **stream);
+
+/* Kernel timing: milliseconds of the last traversal launch measured with HIP
+ * events recorded on the engine stream around that launch. */
+typedef struct vt_frame_stats {
+  float last_trace_ms;
+  float last_frame_ms;
+  uint32_t samples_accumulated;
+  uint32_t last_ray_count;
+  uint64_t device_bytes;
+} vt_frame_stats;
+/* FrameStats telemetry (packages/core/src/engine/telemetry.ts). */
+vt_status vt_stats(vt_engine *eng, vt_frame_stats *out);
+
+/* ---- host-side builders (no device needed) ------------------------------- */
+
+/* buildArrayBvh (packages/shared-bvh/src/buildArrayBvh.ts:342).  Same
+ * arguments and outputs; out_node_words holds max(1, 2*triCount-1)*8 words. */
+vt_status vt_build_blas(const float *positions, uint32_t position_floats,
+                        uint32_t position_stride, const uint32_t *indices,
+                        uint32_t index_words, uint32_t index_stride,
+                        const uint32_t *tri_material_ids, uint32_t max_leaf,
+                        uint32_t num_bins, uint32_t max_depth, uint32_t *out_node_words,
+                        uint32_t *out_node_count, uint32_t *out_reordered_indices,
+                        uint32_t *out_reordered_mat, uint32_t *out_reordered_to_source);
+
+/* buildTlas (packages/shared-bvh/src/tlas.ts:859).  AABBs are binary64, as
+ * the reference's JS numbers; out_nodes holds (2n-1)*8 words. */
+vt_status vt_build_tlas(uint32_t n, const uint32_t *blas_ids, const double *aabb_min,
+                        const double *aabb_max, const float *world_to_local,
+                        uint32_t max_leaf, uint32_t num_bins, uint32_t max_depth,
+                        uint32_t *out_nodes, uint32_t *out_node_count,
+                        uint32_t *out_instance_indices, uint32_t *out_blas_roots,
+                        float *out_transforms);
+
+/* packSceneFromCore (packages/shared-bvh/src/scenePack.ts:1104) into a
+ * host-side pack whose fields can be read back for inspection. */
+typedef struct vt_pack vt_pack;
+enum {
+  VT_PACK_POSITIONS = 0, VT_PACK_NORMALS, VT_PACK_INDICES, VT_PACK_TRI_MATERIAL,
+  VT_PACK_TRI_SOURCE, VT_PACK_TRI_PRIMITIVE, VT_PACK_BVH_NODES, VT_PACK_TLAS_NODES,
+  VT_PACK_TLAS_INSTANCE_INDICES, VT_PACK_TLAS_BLAS_ROOTS, VT_PACK_TLAS_W2L,
+  VT_PACK_TLAS_L2W, VT_PACK_INSTANCE_PRIMITIVE, VT_PACK_INSTANCE_SOURCE,
+  VT_PACK_FIELD_COUNT
+};
+vt_status vt_pack_scene(const vt_primitive *prims, uint32_t prim_count, uint32_t threads,
+                        vt_pack **out);
+/* Field size in 32-bit words (negative on a bad field). */
+int64_t vt_pack_field_words(const vt_pack *p, int32_t field);
+vt_status vt_pack_copy_field(const vt_pack *p, int32_t field, void *dst, uint64_t dst_words);
+void vt_pack_free(vt_pack *p);
+
+/* Library version string, e.g. "vitrum-hip 0.1.0 gfx950". */
+const char *vt_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

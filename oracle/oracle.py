@@ -1,0 +1,288 @@
+"""ctypes binding of the CPU oracle (oracle/liborc.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+U32P = C.POINTER(C.c_uint32)
+F32P = C.POINTER(C.c_float)
+F64P = C.POINTER(C.c_double)
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+class Primitive(C.Structure):
+    _fields_ = [
+        ("kind", C.c_uint32),
+        ("material_id", C.c_uint32),
+        ("positions", F32P),
+        ("position_floats", C.c_uint32),
+        ("normals", F32P),
+        ("normal_floats", C.c_uint32),
+        ("indices", U32P),
+        ("index_count", C.c_uint32),
+        ("transforms", F32P),
+        ("transform_count", C.c_uint32),
+    ]
+
+
+class Scene(C.Structure):
+    _fields_ = [
+        ("positions", F32P),
+        ("vertex_count", C.c_uint32),
+        ("indices", U32P),
+        ("triangle_count", C.c_uint32),
+        ("bvh_nodes", U32P),
+        ("bvh_node_count", C.c_uint32),
+        ("tlas_nodes", U32P),
+        ("tlas_node_count", C.c_uint32),
+        ("instance_indices", U32P),
+        ("instance_index_count", C.c_uint32),
+        ("blas_roots", U32P),
+        ("w2l", F32P),
+        ("l2w", F32P),
+        ("instance_count", C.c_uint32),
+    ]
+
+
+FIELDS = [
+    "positions", "normals", "indices", "tri_material", "tri_source", "tri_primitive",
+    "bvh_nodes", "tlas_nodes", "tlas_instance_indices", "tlas_blas_roots", "tlas_w2l",
+    "tlas_l2w", "instance_primitive", "instance_source", "bindings", "binding_aabb",
+]
+FLOAT_FIELDS = {"positions", "normals", "tlas_w2l", "tlas_l2w", "binding_aabb"}
+
+RAY_DTYPE = np.dtype([("o", "<f4", 3), ("tmin", "<f4"), ("d", "<f4", 3), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("prim", "<u4"), ("inst", "<u4"), ("t", "<f4"), ("u", "<f4"), ("v", "<f4")])
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liborc.so")
+        if not os.path.exists(path):
+            raise OracleError(f"oracle library not built: {path} (run make -C oracle)")
+        L = C.CDLL(path)
+        L.orc_last_error.restype = C.c_char_p
+        L.orc_pack_field_size.restype = C.c_int64
+        L.orc_pack_field_size.argtypes = [C.c_void_p, C.c_int]
+        L.orc_pack_copy_field.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_uint64]
+        L.orc_pack_free.argtypes = [C.c_void_p]
+        L.orc_pack_warning_count.argtypes = [C.c_void_p]
+        L.orc_pack_warning_count.restype = C.c_uint32
+        L.orc_pcg_init.restype = C.c_uint32
+        L.orc_pcg_next.restype = C.c_uint32
+        L.orc_rand_f32.restype = C.c_float
+        L.orc_pcg_init.argtypes = [C.c_uint32] * 3
+        L.orc_pcg_next.argtypes = [U32P]
+        L.orc_rand_f32.argtypes = [U32P]
+        L.orc_watertight.argtypes = [F32P, F32P, F32P, F32P, F32P, C.c_float, F32P]
+        L.orc_validate_bvh.argtypes = [U32P, C.c_uint32, U32P, C.c_int32, C.c_int64, C.c_uint32, U32P]
+        L.orc_invert_mat4.argtypes = [F32P, F32P]
+        _LIB = L
+    return _LIB
+
+
+def _err():
+    return lib().orc_last_error().decode()
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+def build_blas(positions, indices, tri_material_ids=None, *, position_stride=4, index_stride=4,
+               max_leaf=4, num_bins=16, max_depth=58):
+    """buildArrayBvh (packages/shared-bvh/src/buildArrayBvh.ts:342)."""
+    positions = np.ascontiguousarray(positions, dtype=np.float32).ravel()
+    indices = np.ascontiguousarray(indices, dtype=np.uint32).ravel()
+    tri_count = indices.size // index_stride
+    mats = (np.zeros(tri_count, np.uint32) if tri_material_ids is None
+            else np.ascontiguousarray(tri_material_ids, dtype=np.uint32))
+    cap = max(1, 2 * tri_count - 1)
+    nodes = np.zeros(cap * 8, np.uint32)
+    count = C.c_uint32(0)
+    ridx = np.zeros(max(indices.size, 1), np.uint32)
+    rmat = np.zeros(max(tri_count, 1), np.uint32)
+    rsrc = np.zeros(max(tri_count, 1), np.uint32)
+    rc = lib().orc_build_blas(
+        _p(positions, F32P), C.c_uint32(positions.size), C.c_uint32(position_stride),
+        _p(indices, U32P), C.c_uint32(indices.size), C.c_uint32(index_stride), _p(mats, U32P),
+        C.c_uint32(max_leaf), C.c_uint32(num_bins), C.c_uint32(max_depth), _p(nodes, U32P),
+        C.byref(count), _p(ridx, U32P), _p(rmat, U32P), _p(rsrc, U32P))
+    if rc != 0:
+        raise OracleError(_err())
+    n = count.value
+    return {
+        "nodes": nodes[: n * 8].copy(),
+        "node_count": n,
+        "reordered_indices": ridx[: indices.size].copy(),
+        "reordered_mat": rmat[:tri_count].copy(),
+        "reordered_to_source": rsrc[:tri_count].copy(),
+    }
+
+
+def validate_bvh(node_words, node_count, roots=None, triangle_count=None, stack_capacity=60):
+    node_words = np.ascontiguousarray(node_words, dtype=np.uint32)
+    r = None if roots is None else np.ascontiguousarray(roots, dtype=np.uint32)
+    proof = np.zeros(7, np.uint32)
+    rc = lib().orc_validate_bvh(_p(node_words, U32P), node_count, _p(r, U32P),
+                                -1 if r is None else r.size,
+                                -1 if triangle_count is None else triangle_count,
+                                stack_capacity, _p(proof, U32P))
+    if rc != 0:
+        raise OracleError(_err())
+    return dict(zip(["nodeCount", "rootCount", "interiorNodeCount", "leafNodeCount", "maxDepth",
+                     "maxTraversalStackEntries", "traversalStackCapacity"], proof.tolist()))
+
+
+def build_tlas(blas_ids, aabb_min, aabb_max, w2l, *, max_leaf=1, num_bins=16, max_depth=58):
+    """buildTlas (packages/shared-bvh/src/tlas.ts:859)."""
+    blas_ids = np.ascontiguousarray(blas_ids, dtype=np.uint32)
+    n = blas_ids.size
+    mn = np.ascontiguousarray(aabb_min, dtype=np.float64).ravel()
+    mx = np.ascontiguousarray(aabb_max, dtype=np.float64).ravel()
+    xf = np.ascontiguousarray(w2l, dtype=np.float32).ravel()
+    nodes = np.zeros(max(1, 2 * n - 1) * 8, np.uint32)
+    count = C.c_uint32(0)
+    perm = np.zeros(max(n, 1), np.uint32)
+    roots = np.zeros(max(n, 1), np.uint32)
+    tf = np.zeros(max(n, 1) * 16, np.float32)
+    fb = np.zeros(4, np.uint32)
+    rc = lib().orc_build_tlas(
+        C.c_uint32(n), _p(blas_ids, U32P), _p(mn, F64P), _p(mx, F64P), _p(xf, F32P),
+        C.c_uint32(max_leaf), C.c_uint32(num_bins), C.c_uint32(max_depth), _p(nodes, U32P),
+        C.byref(count), _p(perm, U32P), _p(roots, U32P), _p(tf, F32P), _p(fb, U32P))
+    if rc != 0:
+        raise OracleError(_err())
+    c = count.value
+    return {"nodes": nodes[: c * 8].copy(), "node_count": c, "instance_indices": perm[:n].copy(),
+            "blas_roots": roots[:n].copy(), "transforms": tf[: n * 16].copy(),
+            "fallbacks": fb.tolist()}
+
+
+def invert_mat4(m):
+    m = np.ascontiguousarray(m, dtype=np.float32).ravel()
+    out = np.zeros(16, np.float32)
+    ok = lib().orc_invert_mat4(_p(m, F32P), _p(out, F32P))
+    return out if ok else None
+
+
+def pack_scene(primitives):
+    """packSceneFromCore (packages/shared-bvh/src/scenePack.ts:1104).
+
+    primitives: list of dicts with keys kind ('mesh'|'instanced-mesh'),
+    positions, normals, indices (optional), transform (mesh, optional) or
+    instances (instanced-mesh), material_id.
+    """
+    keep = []
+    arr = (Primitive * max(1, len(primitives)))()
+    for i, p in enumerate(primitives):
+        pos = np.ascontiguousarray(p["positions"], dtype=np.float32).ravel()
+        nrm = np.ascontiguousarray(p["normals"], dtype=np.float32).ravel()
+        idx = p.get("indices")
+        idx = None if idx is None else np.ascontiguousarray(idx, dtype=np.uint32).ravel()
+        if p["kind"] == "instanced-mesh":
+            xf = np.ascontiguousarray(np.asarray(p["instances"], np.float32).reshape(-1, 16))
+            kind, ntx = 1, xf.shape[0]
+        else:
+            t = p.get("transform")
+            xf = None if t is None else np.ascontiguousarray(np.asarray(t, np.float32).reshape(16))
+            kind, ntx = 0, (0 if xf is None else 1)
+        keep += [pos, nrm, idx, xf]
+        arr[i] = Primitive(kind, p.get("material_id", 0), _p(pos, F32P), pos.size, _p(nrm, F32P),
+                           nrm.size, _p(idx, U32P), 0 if idx is None else idx.size,
+                           _p(xf, F32P), ntx)
+    handle = C.c_void_p()
+    rc = lib().orc_pack_scene(arr, len(primitives), C.byref(handle))
+    if rc != 0:
+        raise OracleError(_err())
+    out = {}
+    try:
+        for fid, name in enumerate(FIELDS):
+            n = lib().orc_pack_field_size(handle, fid)
+            buf = np.zeros(max(n, 1), np.float32 if name in FLOAT_FIELDS else np.uint32)
+            lib().orc_pack_copy_field(handle, fid, buf.ctypes.data, C.c_uint64(buf.size))
+            out[name] = buf[:n]
+        out["warnings"] = lib().orc_pack_warning_count(handle)
+    finally:
+        lib().orc_pack_free(handle)
+    return out
+
+
+class PackedScene:
+    """Holds numpy views of a packed scene and the orc_scene struct over them."""
+
+    def __init__(self, pack):
+        self.pack = pack
+        self._keep = {k: np.ascontiguousarray(pack[k]) for k in (
+            "positions", "indices", "bvh_nodes", "tlas_nodes", "tlas_instance_indices",
+            "tlas_blas_roots", "tlas_w2l", "tlas_l2w")}
+        k = self._keep
+        self.struct = Scene(
+            _p(k["positions"], F32P), k["positions"].size // 4,
+            _p(k["indices"], U32P), k["indices"].size // 4,
+            _p(k["bvh_nodes"], U32P), k["bvh_nodes"].size // 8,
+            _p(k["tlas_nodes"], U32P), k["tlas_nodes"].size // 8,
+            _p(k["tlas_instance_indices"], U32P), k["tlas_instance_indices"].size,
+            _p(k["tlas_blas_roots"], U32P), _p(k["tlas_w2l"], F32P), _p(k["tlas_l2w"], F32P),
+            k["tlas_blas_roots"].size)
+
+
+def trace_closest(scene: PackedScene, rays, *, threads=None, counts=False):
+    rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+    n = rays.size
+    hits = np.zeros(n, HIT_DTYPE)
+    cnt = np.zeros(n * 4, np.uint32) if counts else None
+    threads = threads or min(16, os.cpu_count() or 1)
+    lib().orc_trace_closest(C.byref(scene.struct), rays.ctypes.data_as(C.c_void_p), C.c_uint32(n),
+                            hits.ctypes.data_as(C.c_void_p), _p(cnt, U32P), C.c_uint32(threads))
+    return (hits, cnt.reshape(n, 4)) if counts else hits
+
+
+def trace_any(scene: PackedScene, rays, *, threads=None):
+    rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+    n = rays.size
+    out = np.zeros(n, np.uint32)
+    threads = threads or min(16, os.cpu_count() or 1)
+    lib().orc_trace_any(C.byref(scene.struct), rays.ctypes.data_as(C.c_void_p), C.c_uint32(n),
+                        _p(out, U32P), C.c_uint32(threads))
+    return out
+
+
+def fp64_hit_params(scene: PackedScene, rays, hits):
+    rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+    hits = np.ascontiguousarray(hits, dtype=HIT_DTYPE)
+    out = np.zeros((rays.size, 3), np.float64)
+    lib().orc_fp64_hit_params(C.byref(scene.struct), rays.ctypes.data_as(C.c_void_p),
+                              hits.ctypes.data_as(C.c_void_p), C.c_uint32(rays.size),
+                              out.ctypes.data_as(F64P))
+    return out
+
+
+def watertight(o, d, a, b, c, exclusive_min_t=0.0):
+    args = [np.ascontiguousarray(v, dtype=np.float32) for v in (o, d, a, b, c)]
+    out = np.zeros(10, np.float32)
+    lib().orc_watertight(*[_p(v, F32P) for v in args], C.c_float(exclusive_min_t), _p(out, F32P))
+    return {"hit": bool(out[0]), "t": out[1], "point": out[2:5].copy(), "bary": out[5:8].copy(),
+            "side": out[8], "zero_edge_mask": int(out[9])}
+
+
+def pcg_init(px, py, seed):
+    return lib().orc_pcg_init(px, py, seed)
+
+
+def pcg_stream(px, py, seed, n):
+    s = C.c_uint32(pcg_init(px, py, seed))
+    return [lib().orc_pcg_next(C.byref(s)) for _ in range(n)]

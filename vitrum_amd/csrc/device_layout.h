@@ -1,0 +1,59 @@
+// device_layout.h -- HBM layout of a packed scene for the traversal kernels.
+//
+// Node64 (64 bytes, one L1 line): both child boxes live in the parent, so one
+// 64-byte load tests two children.  It is derived losslessly from the
+// reference's 32-byte preorder nodes (packages/shared-bvh/src/strides.ts:24-31):
+// every reference interior node becomes one Node64 holding its two children,
+// and every BLAS/TLAS root gets a "wrapper" Node64 whose left child is the
+// root and whose right child is empty, so the root box test is an ordinary
+// child test.
+//
+//   words 0-2  left child box min      words 6-8   right child box min
+//   words 3-5  left child box max      words 9-11  right child box max
+//   word 12    left child ref          word 13     right child ref
+//   words 14-15 reserved (0)
+//
+// Child ref: bit 31 clear -> interior, value = Node64 index.
+//            bit 31 set   -> leaf, bits 27-30 = primitive count (1..15),
+//                            bits 0-26 = first primitive (triangle index for
+//                            a BLAS, instance slot for the TLAS).
+//            kEmptyRef    -> no child (its box is empty: +inf..-inf).
+//
+// Triangle record (48 bytes): a.xyz b.x | b.yz c.xy | c.z 0 0 0, indexed by
+// the global packed (BVH-reordered) triangle index.
+//
+// Instance record (144 bytes, TLAS permutation order): world-to-local
+// columns (4 x float4), local-to-world columns (4 x float4), then
+// {blas wrapper node, packed instance index, 0, 0}.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace vt {
+
+struct Pack;
+
+constexpr uint32_t kRefLeaf = 0x80000000u;
+constexpr uint32_t kEmptyRef = 0x80000000u;  // leaf with zero primitives
+constexpr uint32_t kMaxLeafCountInRef = 15;
+constexpr uint32_t kMaxLeafOffsetInRef = (1u << 27) - 1;
+constexpr uint32_t kNode64Words = 16;
+constexpr uint32_t kTriWords = 12;
+constexpr uint32_t kInstWords = 36;
+
+struct DeviceLayout {
+  std::vector<uint32_t> blas_nodes;   // Node64 words
+  std::vector<uint32_t> tlas_nodes;   // Node64 words
+  std::vector<float> tris;            // kTriWords per triangle
+  std::vector<uint32_t> instances;    // kInstWords per instance slot
+  uint32_t tlas_root = 0;             // wrapper Node64 index in tlas_nodes
+  uint32_t blas_node_count = 0, tlas_node_count = 0, tri_count = 0, inst_count = 0;
+};
+
+// Returns an empty string on success, else the reason the pack cannot be
+// represented (e.g. a leaf larger than 15 primitives).
+std::string build_device_layout(const Pack& p, DeviceLayout& out);
+
+}  // namespace vt

@@ -1,0 +1,540 @@
+// engine.cpp -- the C ABI of libvthip.so (declared in include/vitrum_hip.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/vitrum_hip.h"
+#include "builder.h"
+#include "device_layout.h"
+#include "engine_internal.h"
+#include "traverse.h"
+
+namespace {
+
+thread_local std::string g_global_error;
+
+vt_status set_err(vt_engine* e, vt_status code, const std::string& msg) {
+  if (e)
+    e->last_error = msg;
+  else
+    g_global_error = msg;
+  return code;
+}
+
+#define HIP_TRY(eng, expr)                                                               \
+  do {                                                                                   \
+    hipError_t _e = (expr);                                                              \
+    if (_e != hipSuccess)                                                                \
+      return set_err((eng), VT_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+vt::PrimitiveIn to_prim(const vt_primitive& p) {
+  return vt::PrimitiveIn{p.kind,      p.material,       p.positions,   p.position_floats,
+                         p.normals,   p.normal_floats,  p.indices,     p.index_count,
+                         p.transforms, p.transform_count};
+}
+
+vt_status check_prims(vt_engine* e, const vt_primitive* prims, uint32_t n) {
+  if (n && !prims) return set_err(e, VT_ERR_INVALID_ARGUMENT, "primitives is NULL");
+  for (uint32_t i = 0; i < n; ++i) {
+    const vt_primitive& p = prims[i];
+    if (p.kind > 1) return set_err(e, VT_ERR_INVALID_ARGUMENT, "unknown primitive kind");
+    if (p.position_floats && !p.positions)
+      return set_err(e, VT_ERR_INVALID_ARGUMENT, "primitive positions is NULL");
+    if (p.normal_floats && !p.normals)
+      return set_err(e, VT_ERR_INVALID_ARGUMENT, "primitive normals is NULL");
+    if (p.index_count && !p.indices)
+      return set_err(e, VT_ERR_INVALID_ARGUMENT, "primitive indices is NULL");
+    if (p.transform_count && !p.transforms)
+      return set_err(e, VT_ERR_INVALID_ARGUMENT, "primitive transforms is NULL");
+    if (p.kind == VT_PRIM_MESH && p.transform_count > 1)
+      return set_err(e, VT_ERR_INVALID_ARGUMENT, "a mesh takes at most one transform");
+  }
+  return VT_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Device buffer helpers
+// ---------------------------------------------------------------------------
+
+vt_status DeviceBuffer::ensure(vt_engine* e, size_t want) {
+  if (want <= bytes) return VT_OK;
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+  if (want == 0) return VT_OK;
+  hipError_t r = hipMalloc(&ptr, want);
+  if (r != hipSuccess) {
+    ptr = nullptr;
+    return set_err(e, VT_ERR_OUT_OF_MEMORY,
+                   "hipMalloc(" + std::to_string(want) + "): " + hipGetErrorString(r));
+  }
+  bytes = want;
+  return VT_OK;
+}
+
+void DeviceBuffer::release() {
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+}
+
+vt_status engine_upload(vt_engine* e, DeviceBuffer& dst, const void* src, size_t bytes) {
+  vt_status s = dst.ensure(e, std::max<size_t>(bytes, 16));
+  if (s) return s;
+  if (bytes) HIP_TRY(e, hipMemcpyAsync(dst.ptr, src, bytes, hipMemcpyHostToDevice, e->stream));
+  return VT_OK;
+}
+
+void SceneBuffers::release() {
+  blas.release();
+  tlas.release();
+  tris.release();
+  instances.release();
+  positions.release();
+  normals.release();
+  indices.release();
+  tri_material.release();
+  materials.release();
+}
+
+// ---------------------------------------------------------------------------
+// Lifecycle
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+const char* vt_version(void) { return "vitrum-hip 0.1.0 gfx950"; }
+
+vt_status vt_last_error(const vt_engine* eng, char* buf, size_t len) {
+  if (!buf || len == 0) return VT_ERR_INVALID_ARGUMENT;
+  const std::string& m = eng ? eng->last_error : g_global_error;
+  std::snprintf(buf, len, "%s", m.c_str());
+  return VT_OK;
+}
+
+vt_status vt_create(int32_t device_ordinal, const vt_options* opts, vt_engine** out) {
+  if (!out) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "out is NULL");
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+    return set_err(nullptr, VT_ERR_NO_DEVICE, "no HIP device is available");
+  if (device_ordinal < 0 || device_ordinal >= count)
+    return set_err(nullptr, VT_ERR_RANGE, "device ordinal out of range");
+  if (opts && opts->sampling > 1)
+    return set_err(nullptr, VT_ERR_RANGE, "sampling must be 0 (pcg) or 1 (sobol)");
+  std::unique_ptr<vt_engine> e(new vt_engine());
+  e->device = device_ordinal;
+  if (opts) e->opts = *opts;
+  if (e->opts.max_bounces == 0) e->opts.max_bounces = 8;
+  HIP_TRY(nullptr, hipSetDevice(device_ordinal));
+  HIP_TRY(nullptr, hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+  HIP_TRY(nullptr, hipEventCreate(&e->ev_start));
+  HIP_TRY(nullptr, hipEventCreate(&e->ev_stop));
+  hipDeviceProp_t prop;
+  HIP_TRY(nullptr, hipGetDeviceProperties(&prop, device_ordinal));
+  e->cu_count = (uint32_t)prop.multiProcessorCount;
+  int per_cu = 0;
+  HIP_TRY(nullptr, vtk::trace_occupancy(false, &per_cu));
+  e->trace_blocks_per_cu = std::max(1, per_cu);
+  e->trace_grid = e->cu_count * e->trace_blocks_per_cu;
+  vt_status s = e->counter.ensure(e.get(), 64);
+  if (!s) s = e->error_word.ensure(e.get(), 64);
+  const size_t spill = (size_t)(vtk::kStackCapacity - vtk::kLdsDepth) * e->trace_grid *
+                       vtk::kTraceBlock * sizeof(uint2);
+  if (!s) s = e->spill.ensure(e.get(), spill);
+  if (s) return set_err(nullptr, s, e->last_error);
+  HIP_TRY(nullptr, vtk::launch_zero((uint32_t*)e->error_word.ptr, 16, e->stream));
+  HIP_TRY(nullptr, hipStreamSynchronize(e->stream));
+  *out = e.release();
+  return VT_OK;
+}
+
+vt_status vt_destroy(vt_engine* eng) {
+  if (!eng) return VT_OK;
+  (void)hipSetDevice(eng->device);
+  (void)hipStreamSynchronize(eng->stream);
+  eng->scene.release();
+  eng->counter.release();
+  eng->error_word.release();
+  eng->spill.release();
+  eng->tmp_rays.release();
+  eng->tmp_out.release();
+  eng->tmp_counts.release();
+  for (auto& kv : eng->user_allocs) (void)hipFree(kv);
+  engine_release_render(eng);
+  (void)hipEventDestroy(eng->ev_start);
+  (void)hipEventDestroy(eng->ev_stop);
+  (void)hipStreamDestroy(eng->stream);
+  delete eng;
+  return VT_OK;
+}
+
+vt_status vt_set_scene(vt_engine* eng, const vt_scene_desc* scene) {
+  if (!eng) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "engine is NULL");
+  if (!scene) return set_err(eng, VT_ERR_INVALID_ARGUMENT, "scene is NULL");
+  if (vt_status s = check_prims(eng, scene->primitives, scene->primitive_count)) return s;
+  for (uint32_t i = 0; i < scene->primitive_count; ++i)
+    if (scene->primitives[i].material >= std::max<uint32_t>(1, scene->material_count) &&
+        scene->material_count > 0)
+      return set_err(eng, VT_ERR_RANGE, "primitive material index out of range");
+  std::vector<vt::PrimitiveIn> prims(scene->primitive_count);
+  for (uint32_t i = 0; i < scene->primitive_count; ++i) prims[i] = to_prim(scene->primitives[i]);
+  auto pack = std::make_unique<vt::Pack>();
+  const uint32_t threads = std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+  vt::BuildError be = vt::pack_scene(prims.data(), (uint32_t)prims.size(), threads, *pack);
+  if (be) return set_err(eng, be.code, be.message);
+  auto layout = std::make_unique<vt::DeviceLayout>();
+  std::string le = vt::build_device_layout(*pack, *layout);
+  if (!le.empty()) return set_err(eng, VT_ERR_RANGE, le);
+  HIP_TRY(eng, hipSetDevice(eng->device));
+  // Upload into fresh buffers, then swap: a failure leaves the old scene bound.
+  SceneBuffers nb;
+  vt_status s = VT_OK;
+  s = s ? s : engine_upload(eng, nb.blas, layout->blas_nodes.data(), layout->blas_nodes.size() * 4);
+  s = s ? s : engine_upload(eng, nb.tlas, layout->tlas_nodes.data(), layout->tlas_nodes.size() * 4);
+  s = s ? s : engine_upload(eng, nb.tris, layout->tris.data(), layout->tris.size() * 4);
+  s = s ? s : engine_upload(eng, nb.instances, layout->instances.data(), layout->instances.size() * 4);
+  s = s ? s : engine_upload(eng, nb.positions, pack->positions.data(), pack->positions.size() * 4);
+  s = s ? s : engine_upload(eng, nb.normals, pack->normals.data(), pack->normals.size() * 4);
+  s = s ? s : engine_upload(eng, nb.indices, pack->indices.data(), pack->indices.size() * 4);
+  s = s ? s : engine_upload(eng, nb.tri_material, pack->tri_material.data(), pack->tri_material.size() * 4);
+  if (!s) {
+    hipError_t r = hipStreamSynchronize(eng->stream);
+    if (r != hipSuccess) s = set_err(eng, VT_ERR_HIP, hipGetErrorString(r));
+  }
+  if (s) {
+    nb.release();
+    return s;
+  }
+  eng->scene.release();
+  eng->scene = nb;
+  nb = SceneBuffers();
+  eng->layout = std::move(layout);
+  eng->pack = std::move(pack);
+  eng->scene_desc_env = scene->environment;
+  eng->materials.assign(scene->materials, scene->materials + scene->material_count);
+  eng->scene_radius = scene->scene_radius;
+  eng->has_scene = true;
+  return engine_on_scene_changed(eng);
+}
+
+vt_status vt_device_alloc(vt_engine* eng, size_t bytes, void** d_ptr) {
+  if (!eng || !d_ptr) return set_err(eng, VT_ERR_INVALID_ARGUMENT, "NULL argument");
+  HIP_TRY(eng, hipSetDevice(eng->device));
+  HIP_TRY(eng, hipMalloc(d_ptr, std::max<size_t>(bytes, 16)));
+  eng->user_allocs.push_back(*d_ptr);
+  return VT_OK;
+}
+
+vt_status vt_device_free(vt_engine* eng, void* d_ptr) {
+  if (!eng) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "engine is NULL");
+  auto it = std::find(eng->user_allocs.begin(), eng->user_allocs.end(), d_ptr);
+  if (it == eng->user_allocs.end())
+    return set_err(eng, VT_ERR_INVALID_ARGUMENT, "pointer was not allocated by vt_device_alloc");
+  eng->user_allocs.erase(it);
+  HIP_TRY(eng, hipStreamSynchronize(eng->stream));
+  HIP_TRY(eng, hipFree(d_ptr));
+  return VT_OK;
+}
+
+vt_status vt_memcpy_h2d(vt_engine* eng, void* d_dst, const void* src, size_t bytes) {
+  if (!eng || (!src && bytes) || (!d_dst && bytes))
+    return set_err(eng, VT_ERR_INVALID_ARGUMENT, "NULL argument");
+  HIP_TRY(eng, hipMemcpyAsync(d_dst, src, bytes, hipMemcpyHostToDevice, eng->stream));
+  HIP_TRY(eng, hipStreamSynchronize(eng->stream));
+  return VT_OK;
+}
+
+vt_status vt_memcpy_d2h(vt_engine* eng, void* dst, const void* d_src, size_t bytes) {
+  if (!eng || (!dst && bytes) || (!d_src && bytes))
+    return set_err(eng, VT_ERR_INVALID_ARGUMENT, "NULL argument");
+  HIP_TRY(eng, hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, eng->stream));
+  HIP_TRY(eng, hipStreamSynchronize(eng->stream));
+  return VT_OK;
+}
+
+vt_status vt_synchronize(vt_engine* eng) {
+  if (!eng) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "engine is NULL");
+  HIP_TRY(eng, hipStreamSynchronize(eng->stream));
+  return VT_OK;
+}
+
+vt_status vt_get_stream(vt_engine* eng, void** stream) {
+  if (!eng || !stream) return set_err(eng, VT_ERR_INVALID_ARGUMENT, "NULL argument");
+  *stream = (void*)eng->stream;
+  return VT_OK;
+}
+
+vt_status vt_stats(vt_engine* eng, vt_frame_stats* out) {
+  if (!eng || !out) return set_err(eng, VT_ERR_INVALID_ARGUMENT, "NULL argument");
+  *out = vt_frame_stats{};
+  if (eng->trace_timed) {
+    HIP_TRY(eng, hipEventSynchronize(eng->ev_stop));
+    float ms = 0;
+    HIP_TRY(eng, hipEventElapsedTime(&ms, eng->ev_start, eng->ev_stop));
+    out->last_trace_ms = ms;
+  }
+  out->last_frame_ms = eng->last_frame_ms;
+  out->samples_accumulated = eng->samples_accumulated;
+  out->last_ray_count = eng->last_ray_count;
+  out->device_bytes = eng->scene.blas.bytes + eng->scene.tlas.bytes + eng->scene.tris.bytes +
+                      eng->scene.instances.bytes + eng->scene.positions.bytes +
+                      eng->scene.normals.bytes + eng->scene.indices.bytes;
+  return VT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Traversal
+// ---------------------------------------------------------------------------
+
+static vt_status trace_device(vt_engine* eng, const vt_ray* d_rays, uint32_t n, void* d_out,
+                              bool any, uint4* d_counts) {
+  if (!eng) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "engine is NULL");
+  if (!eng->has_scene) return set_err(eng, VT_ERR_STATE, "no scene is set");
+  if (n && (!d_rays || !d_out)) return set_err(eng, VT_ERR_INVALID_ARGUMENT, "NULL buffer");
+  if (n > 0x7FFFFFFFu) return set_err(eng, VT_ERR_RANGE, "too many rays");
+  HIP_TRY(eng, hipSetDevice(eng->device));
+  vtk::TraceParams p{};
+  const vt::DeviceLayout& L = *eng->layout;
+  p.scene.blas_nodes = (const float4*)eng->scene.blas.ptr;
+  p.scene.tlas_nodes = (const float4*)eng->scene.tlas.ptr;
+  p.scene.tris = (const float4*)eng->scene.tris.ptr;
+  p.scene.instances = (const float4*)eng->scene.instances.ptr;
+  p.scene.tlas_root = L.tlas_root;
+  p.scene.tlas_node_count = L.tlas_node_count;
+  p.scene.tri_count = L.tri_count;
+  p.scene.inst_count = L.inst_count;
+  p.rays = (const float4*)d_rays;
+  p.hits = any ? nullptr : (uint32_t*)d_out;
+  p.occluded = any ? (uint32_t*)d_out : nullptr;
+  p.counts = d_counts;
+  p.n = n;
+  p.counter = (uint32_t*)eng->counter.ptr;
+  p.error = (uint32_t*)eng->error_word.ptr;
+  p.spill = (uint2*)eng->spill.ptr;
+  p.refill_min = eng->refill_min;
+  p.steps = eng->trace_steps;
+  eng->last_ray_count = n;
+  if (n == 0) return VT_OK;
+  HIP_TRY(eng, hipEventRecord(eng->ev_start, eng->stream));
+  HIP_TRY(eng, vtk::launch_trace(p, any, d_counts != nullptr, eng->trace_grid, eng->stream));
+  HIP_TRY(eng, hipEventRecord(eng->ev_stop, eng->stream));
+  eng->trace_timed = true;
+  return VT_OK;
+}
+
+static vt_status check_error_word(vt_engine* eng) {
+  uint32_t err = 0;
+  HIP_TRY(eng, hipMemcpyAsync(&err, eng->error_word.ptr, 4, hipMemcpyDeviceToHost, eng->stream));
+  HIP_TRY(eng, hipStreamSynchronize(eng->stream));
+  if (err) {
+    (void)vtk::launch_zero((uint32_t*)eng->error_word.ptr, 1, eng->stream);
+    return set_err(eng, VT_ERR_HIP, "traversal stack capacity exceeded");
+  }
+  return VT_OK;
+}
+
+static vt_status trace_host(vt_engine* eng, const vt_ray* rays, uint32_t n, void* out,
+                            size_t out_stride, bool any, uint32_t* counts) {
+  if (!eng) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "engine is NULL");
+  if (n && (!rays || !out)) return set_err(eng, VT_ERR_INVALID_ARGUMENT, "NULL buffer");
+  if (n == 0) return VT_OK;
+  HIP_TRY(eng, hipSetDevice(eng->device));
+  vt_status s = eng->tmp_rays.ensure(eng, (size_t)n * sizeof(vt_ray));
+  if (!s) s = eng->tmp_out.ensure(eng, (size_t)n * out_stride);
+  if (!s && counts) s = eng->tmp_counts.ensure(eng, (size_t)n * 16);
+  if (s) return s;
+  HIP_TRY(eng, hipMemcpyAsync(eng->tmp_rays.ptr, rays, (size_t)n * sizeof(vt_ray),
+                              hipMemcpyHostToDevice, eng->stream));
+  s = trace_device(eng, (const vt_ray*)eng->tmp_rays.ptr, n, eng->tmp_out.ptr, any,
+                   counts ? (uint4*)eng->tmp_counts.ptr : nullptr);
+  if (s) return s;
+  HIP_TRY(eng, hipMemcpyAsync(out, eng->tmp_out.ptr, (size_t)n * out_stride, hipMemcpyDeviceToHost,
+                              eng->stream));
+  if (counts)
+    HIP_TRY(eng, hipMemcpyAsync(counts, eng->tmp_counts.ptr, (size_t)n * 16, hipMemcpyDeviceToHost,
+                                eng->stream));
+  HIP_TRY(eng, hipStreamSynchronize(eng->stream));
+  return check_error_word(eng);
+}
+
+vt_status vt_trace_closest(vt_engine* eng, const vt_ray* rays, uint32_t n, vt_hit* hits) {
+  return trace_host(eng, rays, n, hits, sizeof(vt_hit), false, nullptr);
+}
+
+vt_status vt_trace_any(vt_engine* eng, const vt_ray* rays, uint32_t n, uint32_t* occluded) {
+  return trace_host(eng, rays, n, occluded, 4, true, nullptr);
+}
+
+vt_status vt_trace_closest_counts(vt_engine* eng, const vt_ray* rays, uint32_t n, vt_hit* hits,
+                                  uint32_t* counts) {
+  if (!counts) return set_err(eng, VT_ERR_INVALID_ARGUMENT, "counts is NULL");
+  return trace_host(eng, rays, n, hits, sizeof(vt_hit), false, counts);
+}
+
+vt_status vt_set_counting(vt_engine* eng, int32_t enabled) {
+  if (!eng) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "engine is NULL");
+  eng->counting = enabled != 0;
+  return VT_OK;
+}
+
+vt_status vt_trace_closest_device(vt_engine* eng, const vt_ray* d_rays, uint32_t n, vt_hit* d_hits) {
+  return trace_device(eng, d_rays, n, d_hits, false, nullptr);
+}
+
+vt_status vt_trace_any_device(vt_engine* eng, const vt_ray* d_rays, uint32_t n, uint32_t* d_occluded) {
+  return trace_device(eng, d_rays, n, d_occluded, true, nullptr);
+}
+
+vt_status vt_set_trace_tuning(vt_engine* eng, uint32_t refill_min, uint32_t steps) {
+  if (!eng) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "engine is NULL");
+  if (refill_min < 1 || refill_min > 64 || steps < 1)
+    return set_err(eng, VT_ERR_RANGE, "refill_min must be in [1, 64] and steps >= 1");
+  eng->refill_min = refill_min;
+  eng->trace_steps = steps;
+  return VT_OK;
+}
+
+vt_status vt_selftest_fp32(vt_engine* eng, const float* a, const float* b, float* q, float* r,
+                           uint32_t n) {
+  if (!eng || (n && (!a || !b || !q || !r)))
+    return set_err(eng, VT_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (n == 0) return VT_OK;
+  HIP_TRY(eng, hipSetDevice(eng->device));
+  DeviceBuffer buf;
+  if (vt_status s = buf.ensure(eng, (size_t)n * 16)) return s;
+  float* d = (float*)buf.ptr;
+  HIP_TRY(eng, hipMemcpyAsync(d, a, (size_t)n * 4, hipMemcpyHostToDevice, eng->stream));
+  HIP_TRY(eng, hipMemcpyAsync(d + n, b, (size_t)n * 4, hipMemcpyHostToDevice, eng->stream));
+  HIP_TRY(eng, vtk::launch_fp32_selftest(d, d + n, d + 2 * (size_t)n, d + 3 * (size_t)n, n, eng->stream));
+  HIP_TRY(eng, hipMemcpyAsync(q, d + 2 * (size_t)n, (size_t)n * 4, hipMemcpyDeviceToHost, eng->stream));
+  HIP_TRY(eng, hipMemcpyAsync(r, d + 3 * (size_t)n, (size_t)n * 4, hipMemcpyDeviceToHost, eng->stream));
+  HIP_TRY(eng, hipStreamSynchronize(eng->stream));
+  buf.release();
+  return VT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Host-side builders
+// ---------------------------------------------------------------------------
+
+vt_status vt_build_blas(const float* positions, uint32_t position_floats, uint32_t position_stride,
+                        const uint32_t* indices, uint32_t index_words, uint32_t index_stride,
+                        const uint32_t* tri_material_ids, uint32_t max_leaf, uint32_t num_bins,
+                        uint32_t max_depth, uint32_t* out_node_words, uint32_t* out_node_count,
+                        uint32_t* out_reordered_indices, uint32_t* out_reordered_mat,
+                        uint32_t* out_reordered_to_source) {
+  if ((position_floats && !positions) || (index_words && !indices) || !out_node_words ||
+      !out_node_count || (index_words && !out_reordered_indices))
+    return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "NULL argument");
+  vt::BlasOut o;
+  vt::BuildError e = vt::build_blas(positions, position_floats, position_stride, indices, index_words,
+                                    index_stride, tri_material_ids, max_leaf, num_bins, max_depth, o);
+  if (e) return set_err(nullptr, e.code, e.message);
+  std::memcpy(out_node_words, o.nodes.data(), o.nodes.size() * 4);
+  *out_node_count = (uint32_t)(o.nodes.size() / 8);
+  if (!o.reordered_indices.empty())
+    std::memcpy(out_reordered_indices, o.reordered_indices.data(), o.reordered_indices.size() * 4);
+  if (out_reordered_mat && !o.reordered_mat.empty())
+    std::memcpy(out_reordered_mat, o.reordered_mat.data(), o.reordered_mat.size() * 4);
+  if (out_reordered_to_source && !o.reordered_to_source.empty())
+    std::memcpy(out_reordered_to_source, o.reordered_to_source.data(),
+                o.reordered_to_source.size() * 4);
+  return VT_OK;
+}
+
+vt_status vt_build_tlas(uint32_t n, const uint32_t* blas_ids, const double* aabb_min,
+                        const double* aabb_max, const float* world_to_local, uint32_t max_leaf,
+                        uint32_t num_bins, uint32_t max_depth, uint32_t* out_nodes,
+                        uint32_t* out_node_count, uint32_t* out_instance_indices,
+                        uint32_t* out_blas_roots, float* out_transforms) {
+  if (n && (!blas_ids || !aabb_min || !aabb_max || !world_to_local))
+    return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!out_nodes || !out_node_count || !out_instance_indices || !out_blas_roots || !out_transforms)
+    return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "NULL output");
+  vt::TlasOut o;
+  vt::BuildError e = vt::build_tlas(n, blas_ids, aabb_min, aabb_max, world_to_local, max_leaf,
+                                    num_bins, max_depth, o);
+  if (e) return set_err(nullptr, e.code, e.message);
+  std::memcpy(out_nodes, o.nodes.data(), o.nodes.size() * 4);
+  *out_node_count = (uint32_t)(o.nodes.size() / 8);
+  std::memcpy(out_instance_indices, o.instance_indices.data(), o.instance_indices.size() * 4);
+  std::memcpy(out_blas_roots, o.blas_roots.data(), o.blas_roots.size() * 4);
+  std::memcpy(out_transforms, o.transforms.data(), o.transforms.size() * 4);
+  return VT_OK;
+}
+
+struct vt_pack {
+  vt::Pack p;
+};
+
+vt_status vt_pack_scene(const vt_primitive* prims, uint32_t prim_count, uint32_t threads,
+                        vt_pack** out) {
+  if (!out) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "out is NULL");
+  *out = nullptr;
+  if (vt_status s = check_prims(nullptr, prims, prim_count)) return s;
+  std::vector<vt::PrimitiveIn> v(prim_count);
+  for (uint32_t i = 0; i < prim_count; ++i) v[i] = to_prim(prims[i]);
+  auto pk = std::make_unique<vt_pack>();
+  vt::BuildError e = vt::pack_scene(v.data(), prim_count, threads, pk->p);
+  if (e) return set_err(nullptr, e.code, e.message);
+  *out = pk.release();
+  return VT_OK;
+}
+
+static const void* pack_field(const vt::Pack& p, int32_t f, size_t* words) {
+  auto ret = [&](const auto& vec) -> const void* {
+    *words = vec.size();
+    return vec.data();
+  };
+  switch (f) {
+    case VT_PACK_POSITIONS: return ret(p.positions);
+    case VT_PACK_NORMALS: return ret(p.normals);
+    case VT_PACK_INDICES: return ret(p.indices);
+    case VT_PACK_TRI_MATERIAL: return ret(p.tri_material);
+    case VT_PACK_TRI_SOURCE: return ret(p.tri_source);
+    case VT_PACK_TRI_PRIMITIVE: return ret(p.tri_primitive);
+    case VT_PACK_BVH_NODES: return ret(p.bvh_nodes);
+    case VT_PACK_TLAS_NODES: return ret(p.tlas_nodes);
+    case VT_PACK_TLAS_INSTANCE_INDICES: return ret(p.tlas_instance_indices);
+    case VT_PACK_TLAS_BLAS_ROOTS: return ret(p.tlas_blas_roots);
+    case VT_PACK_TLAS_W2L: return ret(p.tlas_w2l);
+    case VT_PACK_TLAS_L2W: return ret(p.tlas_l2w);
+    case VT_PACK_INSTANCE_PRIMITIVE: return ret(p.instance_primitive);
+    case VT_PACK_INSTANCE_SOURCE: return ret(p.instance_source);
+    default: *words = 0; return nullptr;
+  }
+}
+
+int64_t vt_pack_field_words(const vt_pack* p, int32_t field) {
+  if (!p || field < 0 || field >= VT_PACK_FIELD_COUNT) return -1;
+  size_t w = 0;
+  pack_field(p->p, field, &w);
+  return (int64_t)w;
+}
+
+vt_status vt_pack_copy_field(const vt_pack* p, int32_t field, void* dst, uint64_t dst_words) {
+  if (!p || field < 0 || field >= VT_PACK_FIELD_COUNT)
+    return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "bad pack or field");
+  size_t w = 0;
+  const void* src = pack_field(p->p, field, &w);
+  if (w > dst_words) return set_err(nullptr, VT_ERR_RANGE, "destination too small");
+  if (w && !dst) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "dst is NULL");
+  if (w) std::memcpy(dst, src, w * 4);
+  return VT_OK;
+}
+
+void vt_pack_free(vt_pack* p) { delete p; }
+
+}  // extern "C"

@@ -1,0 +1,62 @@
+// engine_internal.h -- the vt_engine object behind the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/vitrum_hip.h"
+#include "builder.h"
+#include "device_layout.h"
+
+struct DeviceBuffer {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  // Grows the allocation to at least `want` bytes (contents are not kept).
+  vt_status ensure(vt_engine* e, size_t want);
+  void release();
+};
+
+struct SceneBuffers {
+  DeviceBuffer blas, tlas, tris, instances;          // traversal layout
+  DeviceBuffer positions, normals, indices, tri_material, materials;  // shading data
+  void release();
+};
+
+struct RenderState;  // integrator state (render.hip)
+
+struct vt_engine {
+  int device = 0;
+  vt_options opts{};
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+  bool trace_timed = false;
+  uint32_t cu_count = 0;
+  uint32_t trace_blocks_per_cu = 1;
+  uint32_t trace_grid = 1;
+  uint32_t refill_min = 16;
+  uint32_t trace_steps = 48;
+  bool counting = false;
+  DeviceBuffer counter, error_word, spill;
+  DeviceBuffer tmp_rays, tmp_out, tmp_counts;
+  std::vector<void*> user_allocs;
+  SceneBuffers scene;
+  std::unique_ptr<vt::Pack> pack;
+  std::unique_ptr<vt::DeviceLayout> layout;
+  std::vector<vt_material> materials;
+  vt_environment scene_desc_env{};
+  float scene_radius = 0;
+  bool has_scene = false;
+  float last_frame_ms = 0;
+  uint32_t samples_accumulated = 0;
+  uint32_t last_ray_count = 0;
+  RenderState* render = nullptr;
+  std::string last_error;
+};
+
+vt_status engine_upload(vt_engine* e, DeviceBuffer& dst, const void* src, size_t bytes);
+// Hooks into the integrator (render.cpp): rebuild per-scene render data, free it.
+vt_status engine_on_scene_changed(vt_engine* e);
+void engine_release_render(vt_engine* e);

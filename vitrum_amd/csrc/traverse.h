@@ -1,0 +1,45 @@
+// traverse.h -- launch interface of the traversal kernels (traverse.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace vtk {
+
+constexpr uint32_t kTraceBlock = 128;     // threads per block (2 waves)
+constexpr uint32_t kLdsDepth = 24;        // stack entries per lane kept in LDS
+constexpr uint32_t kStackCapacity = 128;  // TLAS + BLAS entries per lane (>= 2 * 58 + 2)
+
+struct DeviceScene {
+  const float4* blas_nodes;  // Node64, 4 x float4 each
+  const float4* tlas_nodes;
+  const float4* tris;        // 3 x float4 per triangle
+  const float4* instances;   // 9 x float4 per instance slot
+  uint32_t tlas_root;
+  uint32_t tlas_node_count;
+  uint32_t tri_count;
+  uint32_t inst_count;
+};
+
+struct TraceParams {
+  DeviceScene scene;
+  const float4* rays;  // 2 x float4 per ray
+  uint32_t* hits;      // 5 words per ray (closest)
+  uint32_t* occluded;  // 1 word per ray (any)
+  uint4* counts;       // per ray (counting mode)
+  uint32_t n;
+  uint32_t* counter;   // work counter, zeroed by the launcher
+  uint32_t* error;     // set to non-zero on stack overflow
+  uint2* spill;        // (kStackCapacity - kLdsDepth) x grid threads
+  uint32_t refill_min; // refill a wave once this many lanes are idle
+  uint32_t steps;      // traversal steps between refill checks
+};
+
+hipError_t launch_trace(const TraceParams& p, bool any, bool count, uint32_t grid, hipStream_t s);
+hipError_t trace_occupancy(bool any, int* blocks_per_cu);
+hipError_t launch_zero(uint32_t* p, uint32_t n, hipStream_t s);
+hipError_t launch_fp32_selftest(const float* a, const float* b, float* q, float* r, uint32_t n,
+                                hipStream_t s);
+
+}  // namespace vtk

@@ -1,0 +1,557 @@
+// traverse.hip -- closest-hit and any-hit TLAS->BLAS traversal for gfx950.
+//
+// Semantics are those of the reference's traversal, so hit records are
+// bit-identical to it:
+//   traceTlasClosest / traceTlasAny  packages/shared-bvh/src/wgsl/tlasSceneHitTraversal.wgsl.ts:92-269
+//   pt-webgpu instance augmentation  packages/pt-webgpu/src/wgsl/pathTrace/intersection.wgsl.ts:31-152
+//   traceMeshBvh                     packages/pt-webgpu/src/wgsl/pathTrace/intersectionCore.wgsl.ts:461-596
+//   watertight triangle test         packages/shared-bvh/src/opticalWatertightTriangle.ts:98-220
+//   safeInvDir / slab test           packages/shared-bvh/src/wgsl/bvhIntersect.wgsl.ts:86-111,
+//                                    intersectionCore.wgsl.ts:24-31
+// Every float operation rounds once (built with -ffp-contract=off and
+// correctly rounded division and square root).
+//
+// Design (MI355X):
+//  * Node64 BVH2 nodes: the two child boxes sit in the parent, so a node
+//    expansion is one 64-byte load and two slab tests.  The traversal order
+//    is the reference's left-first depth-first order; a child's entry
+//    distance is kept on the stack and re-checked against the current best
+//    when it is popped, which is exactly the reference's pop-time cull.
+//  * Persistent waves: each wave pulls rays from a global counter.  Lanes that
+//    finish are refilled together once enough of the wave is idle: a ballot
+//    of idle lanes, one atomicAdd per refill, and a prefix count (mbcnt) to
+//    hand out consecutive ray indices.
+//  * One combined TLAS+BLAS stack per lane in LDS (entry-major so a wave's
+//    accesses at equal depth hit 64 distinct banks), spilling the rare deep
+//    entries to a per-thread global area.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "traverse.h"
+
+namespace vtk {
+
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kEmpty = 0x80000000u;
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr uint32_t kMiss = 0xFFFFFFFFu;
+
+__device__ __forceinline__ float u2f(uint32_t u) { return __uint_as_float(u); }
+__device__ __forceinline__ uint32_t f2u(float f) { return __float_as_uint(f); }
+__device__ __forceinline__ float safe_max() { return __uint_as_float(0x7F7FFFFDu); }  // 3.402823e38
+__device__ __forceinline__ float inf_const() { return __uint_as_float(0x7F7FFFFFu); }  // 3.402823466e38
+
+__device__ __forceinline__ float fabs_(float x) { return __builtin_fabsf(x); }
+__device__ __forceinline__ bool finite_(float x) { return __builtin_isfinite(x); }
+__device__ __forceinline__ bool nan_(float x) { return __builtin_isnan(x); }
+// Math.max: NaN-propagating (the TS mirror's semantics).
+__device__ __forceinline__ float jmax(float a, float b) {
+  return (nan_(a) || nan_(b)) ? __builtin_nanf("") : (a > b ? a : b);
+}
+__device__ __forceinline__ float max3abs(float x, float y, float z) {
+  return jmax(fabs_(x), jmax(fabs_(y), fabs_(z)));
+}
+__device__ __forceinline__ float sel3(int k, float x, float y, float z) {
+  return k == 0 ? x : (k == 1 ? y : z);
+}
+
+struct V3 {
+  float x, y, z;
+};
+
+__device__ __forceinline__ V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+__device__ __forceinline__ V3 sub(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ float dot3(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+
+// ---------------------------------------------------------------------------
+// Watertight triangle test (opticalWatertightTriangle.ts:15-220)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ float edge_fn(float ax, float ay, float bx, float by) {
+  if (ax == bx && ay == by) return 0.0f;
+  const bool ordered = ax < bx || (ax == bx && ay < by);
+  const float fx = ordered ? ax : bx, fy = ordered ? ay : by;
+  const float sx = ordered ? bx : ax, sy = ordered ? by : ay;
+  const float canonical = fx * sy - fy * sx;
+  return ordered ? canonical : -canonical;
+}
+
+__device__ __noinline__ float ray_param(V3 o, V3 dir, V3 p) {
+  const V3 delta = sub(p, o);
+  const float ds = max3abs(delta.x, delta.y, delta.z);
+  const float rs = max3abs(dir.x, dir.y, dir.z);
+  if (!(rs > 0.0f) || !finite_(rs) || !finite_(ds)) return -1.0f;
+  if (!(ds > 0.0f)) return 0.0f;
+  const V3 d = v3(dir.x / rs, dir.y / rs, dir.z / rs);
+  const V3 q = v3(delta.x / ds, delta.y / ds, delta.z / ds);
+  const float num = (q.x * d.x + q.y * d.y) + q.z * d.z;
+  const float den = (d.x * d.x + d.y * d.y) + d.z * d.z;
+  return (num / den) * (ds / rs);
+}
+
+__device__ __forceinline__ bool lex_before(V3 a, V3 b) {
+  return a.x < b.x || (a.x == b.x && (a.y < b.y || (a.y == b.y && a.z < b.z)));
+}
+
+__device__ __noinline__ float edge_ray_param(V3 o, V3 dir, V3 fi, V3 si, float fpx, float fpy,
+                                             float spx, float spy, V3* point) {
+  const bool ordered = lex_before(fi, si);
+  const V3 f = ordered ? fi : si, s = ordered ? si : fi;
+  const float ax = ordered ? fpx : spx, ay = ordered ? fpy : spy;
+  const float bx = ordered ? spx : fpx, by = ordered ? spy : fpy;
+  const float ddx = bx - ax, ddy = by - ay;
+  float interp;
+  if (fabs_(ddx) >= fabs_(ddy)) {
+    if (ddx == 0.0f) {
+      *point = v3(0, 0, 0);
+      return -1.0f;
+    }
+    interp = -ax / ddx;
+  } else {
+    if (ddy == 0.0f) {
+      *point = v3(0, 0, 0);
+      return -1.0f;
+    }
+    interp = -ay / ddy;
+  }
+  *point = v3(f.x + interp * (s.x - f.x), f.y + interp * (s.y - f.y), f.z + interp * (s.z - f.z));
+  return ray_param(o, dir, *point);
+}
+
+struct TriHit {
+  float t;
+  float u, v;  // bary.y, bary.z
+  V3 point;
+};
+
+// Returns true and fills h when the triangle is hit with lo < t < hi, where
+// lo = tmin (also the predicate's exclusive minimum) and hi the current bound.
+// Work that can only turn an out-of-window hit into a miss is skipped for
+// out-of-window hits, which leaves the accepted set unchanged.
+__device__ __forceinline__ bool tri_test(V3 o, V3 dir, V3 a, V3 b, V3 c, float tmin, float hi,
+                                         TriHit& h) {
+  const float ds = max3abs(dir.x, dir.y, dir.z);
+  if (!(ds > 0.0f) || !finite_(ds) || nan_(tmin)) return false;
+  const V3 d = v3(dir.x / ds, dir.y / ds, dir.z / ds);
+  int kz = fabs_(d.y) > fabs_(d.x) ? 1 : 0;
+  if (fabs_(d.z) > fabs_(sel3(kz, d.x, d.y, d.z))) kz = 2;
+  int kx = kz == 2 ? 0 : kz + 1;
+  int ky = kx == 2 ? 0 : kx + 1;
+  const float dkz = sel3(kz, d.x, d.y, d.z);
+  if (dkz < 0.0f) {
+    const int t = kx;
+    kx = ky;
+    ky = t;
+  }
+  const float sx = sel3(kx, d.x, d.y, d.z) / dkz;
+  const float sy = sel3(ky, d.x, d.y, d.z) / dkz;
+  const float sz = 1.0f / dkz;
+  const V3 pa = sub(a, o), pb = sub(b, o), pc = sub(c, o);
+  const float paz = sel3(kz, pa.x, pa.y, pa.z), pbz = sel3(kz, pb.x, pb.y, pb.z),
+              pcz = sel3(kz, pc.x, pc.y, pc.z);
+  const float apx = sel3(kx, pa.x, pa.y, pa.z) - sx * paz, apy = sel3(ky, pa.x, pa.y, pa.z) - sy * paz;
+  const float bpx = sel3(kx, pb.x, pb.y, pb.z) - sx * pbz, bpy = sel3(ky, pb.x, pb.y, pb.z) - sy * pbz;
+  const float cpx = sel3(kx, pc.x, pc.y, pc.z) - sx * pcz, cpy = sel3(ky, pc.x, pc.y, pc.z) - sy * pcz;
+  const float e0 = edge_fn(bpx, bpy, cpx, cpy);
+  const float e1 = edge_fn(cpx, cpy, apx, apy);
+  const float e2 = edge_fn(apx, apy, bpx, bpy);
+  const float det = (e0 + e1) + e2;
+  if (det == 0.0f || !finite_(det)) return false;
+  const bool positive = det > 0.0f;
+  const float n0 = positive ? e0 : -e0, n1 = positive ? e1 : -e1, n2 = positive ? e2 : -e2;
+  if (n0 < 0.0f || n1 < 0.0f || n2 < 0.0f) return false;
+  const float scaled_t = (e0 * (paz * sz) + e1 * (pbz * sz)) + e2 * (pcz * sz);
+  float t = (scaled_t / det) / ds;
+  V3 point = v3(0, 0, 0);
+  const int zc = (n0 == 0.0f) + (n1 == 0.0f) + (n2 == 0.0f);
+  if (zc != 0) {
+    if (zc >= 2) {
+      if (n0 != 0.0f) point = a, t = ray_param(o, dir, point);
+      if (n1 != 0.0f) point = b, t = ray_param(o, dir, point);
+      if (n2 != 0.0f) point = c, t = ray_param(o, dir, point);
+    } else if (n0 == 0.0f) {
+      t = edge_ray_param(o, dir, b, c, bpx, bpy, cpx, cpy, &point);
+    } else if (n1 == 0.0f) {
+      t = edge_ray_param(o, dir, c, a, cpx, cpy, apx, apy, &point);
+    } else {
+      t = edge_ray_param(o, dir, a, b, apx, apy, bpx, bpy, &point);
+    }
+  }
+  const float lo = tmin > 0.0f ? tmin : 0.0f;
+  if (!(t > lo) || !finite_(t)) return false;
+  if (!(t > tmin && t < hi)) return false;  // traversal acceptance window
+  const V3 e1r = sub(b, a), e2r = sub(c, a);
+  const float es = jmax(max3abs(e1r.x, e1r.y, e1r.z), max3abs(e2r.x, e2r.y, e2r.z));
+  if (!(es > 0.0f) || !finite_(es)) return false;
+  const V3 s1 = v3(e1r.x / es, e1r.y / es, e1r.z / es);
+  const V3 s2 = v3(e2r.x / es, e2r.y / es, e2r.z / es);
+  const V3 nr = v3(s1.y * s2.z - s1.z * s2.y, s1.z * s2.x - s1.x * s2.z, s1.x * s2.y - s1.y * s2.x);
+  const float ns = max3abs(nr.x, nr.y, nr.z);
+  if (!(ns > 0.0f) || !finite_(ns)) return false;
+  const V3 nd = v3(nr.x / ns, nr.y / ns, nr.z / ns);
+  const float nl = __builtin_sqrtf((nd.x * nd.x + nd.y * nd.y) + nd.z * nd.z);
+  if (!(nl > 0.0f) || !finite_(nl)) return false;
+  const V3 n = v3(nd.x / nl, nd.y / nl, nd.z / nl);
+  const float side = -((d.x * n.x + d.y * n.y) + d.z * n.z);
+  if (side == 0.0f || !finite_(side)) return false;
+  h.t = t;
+  h.u = e1 / det;
+  h.v = e2 / det;
+  if (zc == 0) {
+    point = v3(a.x + (h.u * (b.x - a.x) + h.v * (c.x - a.x)),
+               a.y + (h.u * (b.y - a.y) + h.v * (c.y - a.y)),
+               a.z + (h.u * (b.z - a.z) + h.v * (c.z - a.z)));
+  }
+  h.point = point;
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// Slab test, transforms (bvhIntersect.wgsl.ts:86-111; intersectionCore.wgsl.ts:24-83)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ float safe_inv(float v) {
+  const float m = safe_max();
+  if (v == 0.0f) return v >= 0.0f ? m : -m;
+  const float r = 1.0f / v;
+  return r < -m ? -m : (r > m ? m : r);
+}
+
+// Computes tNear and returns the bound-independent part of the test:
+// !(tNear > tFar || tFar < tMin).
+__device__ __forceinline__ bool slab(V3 o, V3 inv, float mnx, float mny, float mnz, float mxx,
+                                     float mxy, float mxz, float tmin, float& tnear) {
+  const float t1x = (mnx - o.x) * inv.x, t1y = (mny - o.y) * inv.y, t1z = (mnz - o.z) * inv.z;
+  const float t2x = (mxx - o.x) * inv.x, t2y = (mxy - o.y) * inv.y, t2z = (mxz - o.z) * inv.z;
+  const float tn = fmaxf(fmaxf(fminf(t1x, t2x), fminf(t1y, t2y)), fminf(t1z, t2z));
+  const float tf = fminf(fminf(fmaxf(t1x, t2x), fmaxf(t1y, t2y)), fmaxf(t1z, t2z));
+  tnear = tn;
+  return !(tn > tf || tf < tmin);
+}
+
+__device__ __forceinline__ V3 xform_point(float4 c0, float4 c1, float4 c2, float4 c3, V3 p) {
+  const float rx = ((c0.x * p.x + c1.x * p.y) + c2.x * p.z) + c3.x;
+  const float ry = ((c0.y * p.x + c1.y * p.y) + c2.y * p.z) + c3.y;
+  const float rz = ((c0.z * p.x + c1.z * p.y) + c2.z * p.z) + c3.z;
+  const float rw = ((c0.w * p.x + c1.w * p.y) + c2.w * p.z) + c3.w;
+  const float s = fmaxf(fmaxf(fabs_(rx), fabs_(ry)), fmaxf(fabs_(rz), fabs_(rw)));
+  if (!(s > 0.0f) || s > safe_max()) return v3(0, 0, 0);
+  const float qx = rx / s, qy = ry / s, qz = rz / s, qw = rw / s;
+  if (qw == 0.0f) return v3(0, 0, 0);
+  const V3 q = v3(qx / qw, qy / qw, qz / qw);
+  if (nan_(q.x) || nan_(q.y) || nan_(q.z) || fabs_(q.x) > safe_max() || fabs_(q.y) > safe_max() ||
+      fabs_(q.z) > safe_max())
+    return v3(0, 0, 0);
+  return q;
+}
+
+__device__ __forceinline__ V3 safe_normalize(V3 v) {
+  const float s = fmaxf(fabs_(v.x), fmaxf(fabs_(v.y), fabs_(v.z)));
+  if (!(s > 0.0f) || s > safe_max()) return v3(0, 1, 0);
+  const V3 q = v3(v.x / s, v.y / s, v.z / s);
+  const float len = __builtin_sqrtf((q.x * q.x + q.y * q.y) + q.z * q.z);
+  return v3(q.x / len, q.y / len, q.z / len);
+}
+
+__device__ __forceinline__ V3 xform_dir(float4 c0, float4 c1, float4 c2, V3 d) {
+  return safe_normalize(v3((c0.x * d.x + c1.x * d.y) + c2.x * d.z, (c0.y * d.x + c1.y * d.y) + c2.y * d.z,
+                           (c0.z * d.x + c1.z * d.y) + c2.z * d.z));
+}
+
+// ---------------------------------------------------------------------------
+// Persistent traversal kernel
+// ---------------------------------------------------------------------------
+
+template <uint32_t BLOCK, uint32_t LDS_DEPTH>
+struct LaneStack {
+  uint2* lds;
+  uint2* spill;
+  uint32_t spill_stride, tid, gtid;
+  __device__ __forceinline__ void store(uint32_t i, uint2 v) {
+    if (i < LDS_DEPTH)
+      lds[i * BLOCK + tid] = v;
+    else
+      spill[(size_t)(i - LDS_DEPTH) * spill_stride + gtid] = v;
+  }
+  __device__ __forceinline__ uint2 load(uint32_t i) const {
+    return i < LDS_DEPTH ? lds[i * BLOCK + tid] : spill[(size_t)(i - LDS_DEPTH) * spill_stride + gtid];
+  }
+};
+
+template <bool ANY, bool COUNT, uint32_t BLOCK, uint32_t LDS_DEPTH>
+__global__ __launch_bounds__(BLOCK) void trace_kernel(TraceParams p) {
+  __shared__ uint2 lds_stack[LDS_DEPTH * BLOCK];
+  LaneStack<BLOCK, LDS_DEPTH> stk{lds_stack, p.spill, gridDim.x * BLOCK, threadIdx.x,
+                                  blockIdx.x * BLOCK + threadIdx.x};
+  const float4* __restrict__ blas = p.scene.blas_nodes;
+  const float4* __restrict__ tlas = p.scene.tlas_nodes;
+  const float4* __restrict__ tris = p.scene.tris;
+  const float4* __restrict__ insts = p.scene.instances;
+  const uint32_t lane = __lane_id();
+  const uint64_t lane_below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+
+  bool active = false;
+  uint32_t ray_idx = 0;
+  // World ray and best hit.
+  V3 ro{}, rd{}, rinv{};
+  float tmin = 0, tmax = 0, best = 0, bu = 0, bv = 0;
+  uint32_t bprim = 0, binst = kMiss;
+  // Instance-local state.
+  V3 lo{}, ld{}, linv{}, lpos{};
+  float ltmin = 0, lbest = 0, lu = 0, lv = 0;
+  uint32_t lprim = 0, inst_id = 0;
+  bool lhit = false;
+  uint32_t slot = 0, slot_end = 0;
+  uint32_t level = 0, cur = kNone, sp = 0, blas_base = 0;
+  uint32_t c_blas = 0, c_tri = 0, c_tlas = 0, c_inst = 0;
+
+  auto enter_instance = [&](uint32_t s) {
+    const float4* ir = insts + 9 * (size_t)s;
+    const float4 c0 = ir[0], c1 = ir[1], c2 = ir[2], c3 = ir[3];
+    const uint4 meta = *reinterpret_cast<const uint4*>(ir + 8);
+    lo = xform_point(c0, c1, c2, c3, ro);
+    ld = xform_dir(c0, c1, c2, rd);
+    linv = v3(safe_inv(ld.x), safe_inv(ld.y), safe_inv(ld.z));
+    const V3 start = v3(ro.x + rd.x * tmin, ro.y + rd.y * tmin, ro.z + rd.z * tmin);
+    ltmin = fmaxf(dot3(sub(xform_point(c0, c1, c2, c3, start), lo), ld), 0.0f);
+    // Closest: localTMax starts at INFINITY; any-hit: at the ray's tMax
+    // (tlasSceneHitTraversal.wgsl.ts:103-108 and :168-173).
+    const float bound = ANY ? tmax : best;
+    float ltmax = ANY ? tmax : inf_const();
+    if (bound < inf_const() * 0.5f) {
+      const V3 end = v3(ro.x + rd.x * bound, ro.y + rd.y * bound, ro.z + rd.z * bound);
+      ltmax = fmaxf(dot3(sub(xform_point(c0, c1, c2, c3, end), lo), ld), ltmin);
+    }
+    lbest = ltmax;
+    lhit = false;
+    inst_id = meta.y;
+    level = 1;
+    blas_base = sp;
+    cur = meta.x;
+    if (COUNT) ++c_inst;
+  };
+
+  auto finalize_instance = [&]() {
+    if (!ANY && lhit && lbest > ltmin) {
+      const float4* ir = insts + 9 * (size_t)slot + 4;
+      const V3 w = xform_point(ir[0], ir[1], ir[2], ir[3], lpos);
+      const float wd = dot3(sub(w, ro), rd);
+      if (wd > tmin && wd < best) {
+        best = wd;
+        bprim = lprim;
+        binst = inst_id;
+        bu = lu;
+        bv = lv;
+      }
+    }
+  };
+
+  auto finish = [&](bool occluded) {
+    if (ANY) {
+      p.occluded[ray_idx] = occluded ? 1u : 0u;
+    } else {
+      const bool hit = binst != kMiss;
+      uint32_t* h = p.hits + 5 * (size_t)ray_idx;
+      h[0] = hit ? bprim : kMiss;
+      h[1] = hit ? binst : kMiss;
+      h[2] = hit ? f2u(best) : 0x7F800000u;
+      h[3] = hit ? f2u(bu) : 0u;
+      h[4] = hit ? f2u(bv) : 0u;
+    }
+    if (COUNT) p.counts[ray_idx] = make_uint4(c_blas, c_tri, c_tlas, c_inst);
+    active = false;
+  };
+
+  auto push = [&](uint32_t ref, float tn) {
+    if (sp >= kStackCapacity) {
+      atomicOr(p.error, 1u);
+      return false;
+    }
+    stk.store(sp++, make_uint2(ref, f2u(tn)));
+    return true;
+  };
+
+  for (;;) {
+    // ---- wave-level refill: ballot idle lanes, one atomic, prefix ranks ----
+    const uint64_t idle = __ballot(!active);
+    const uint32_t n_idle = (uint32_t)__popcll(idle);
+    if (n_idle >= p.refill_min || idle == __ballot(1)) {
+      uint32_t base = 0;
+      if (lane == 0) base = atomicAdd(p.counter, n_idle);
+      base = __shfl(base, 0);
+      if (!active) {
+        const uint32_t idx = base + (uint32_t)__popcll(idle & lane_below);
+        if (idx < p.n) {
+          ray_idx = idx;
+          const float4 a = p.rays[2 * (size_t)idx], b = p.rays[2 * (size_t)idx + 1];
+          ro = v3(a.x, a.y, a.z);
+          tmin = a.w;
+          rd = v3(b.x, b.y, b.z);
+          tmax = b.w;
+          rinv = v3(safe_inv(rd.x), safe_inv(rd.y), safe_inv(rd.z));
+          best = tmax;
+          binst = kMiss;
+          bprim = 0;
+          bu = bv = 0;
+          level = 0;
+          sp = 0;
+          cur = p.scene.tlas_node_count ? p.scene.tlas_root : kNone;
+          if (COUNT) c_blas = c_tri = c_tlas = c_inst = 0;
+          active = true;
+          if (cur == kNone) finish(false);  // no TLAS: fail closed
+        }
+      }
+    }
+    if (!__any(active)) break;
+
+    for (uint32_t step = 0; active && step < p.steps; ++step) {
+      if (cur == kNone) {
+        // ---- pop with the reference's pop-time cull; unwind levels ----
+        for (;;) {
+          if (level == 1) {
+            if (sp > blas_base) {
+              const uint2 e = stk.load(--sp);
+              if (!(u2f(e.y) > lbest)) {
+                cur = e.x;
+                break;
+              }
+              continue;
+            }
+            finalize_instance();
+            level = 0;
+            if (++slot < slot_end) {
+              enter_instance(slot);
+              break;
+            }
+            continue;
+          }
+          if (sp > 0) {
+            const uint2 e = stk.load(--sp);
+            if (!(u2f(e.y) > (ANY ? tmax : best))) {
+              cur = e.x;
+              break;
+            }
+            continue;
+          }
+          finish(false);
+          break;
+        }
+        if (!active) break;
+      }
+      if ((cur & kLeafBit) == 0) {
+        // ---- interior: one Node64 load, two child boxes ----
+        const float4* nd = (level ? blas : tlas) + 4 * (size_t)cur;
+        const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2], q3 = nd[3];
+        const uint32_t ref_l = f2u(q3.x), ref_r = f2u(q3.y);
+        const V3 o = level ? lo : ro;
+        const V3 inv = level ? linv : rinv;
+        const float tm = level ? ltmin : tmin;
+        const float bound = level ? lbest : (ANY ? tmax : best);
+        float tn_l, tn_r;
+        const bool pass_l = ref_l != kEmpty && slab(o, inv, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, tm, tn_l);
+        const bool pass_r = ref_r != kEmpty && slab(o, inv, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, tm, tn_r);
+        if (COUNT) {
+          const uint32_t tested = (ref_l != kEmpty) + (ref_r != kEmpty);
+          if (level) c_blas += tested; else c_tlas += tested;
+        }
+        if (pass_l && !(tn_l > bound)) {
+          if (pass_r && !push(ref_r, tn_r)) {
+            finish(false);
+            break;
+          }
+          cur = ref_l;
+        } else if (pass_r && !(tn_r > bound)) {
+          cur = ref_r;
+        } else {
+          cur = kNone;
+        }
+        continue;
+      }
+      // ---- leaf ----
+      const uint32_t count = (cur >> 27) & 15u, first = cur & 0x07FFFFFFu;
+      cur = kNone;
+      if (level == 0) {
+        slot = first;
+        slot_end = first + count;
+        enter_instance(slot);
+        continue;
+      }
+      for (uint32_t i = 0; i < count; ++i) {
+        const uint32_t t = first + i;
+        const float4* tr = tris + 3 * (size_t)t;
+        const float4 w0 = tr[0], w1 = tr[1], w2 = tr[2];
+        if (COUNT) ++c_tri;
+        TriHit h;
+        if (tri_test(lo, ld, v3(w0.x, w0.y, w0.z), v3(w0.w, w1.x, w1.y), v3(w1.z, w1.w, w2.x), ltmin,
+                     lbest, h)) {
+          if (ANY) {
+            finish(true);
+            break;
+          }
+          lhit = true;
+          lbest = h.t;
+          lprim = t;
+          lu = h.u;
+          lv = h.v;
+          lpos = h.point;
+        }
+      }
+    }
+  }
+}
+
+__global__ void zero_u32_kernel(uint32_t* p, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+__global__ void fp32_selftest_kernel(const float* a, const float* b, float* q, float* r, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    q[i] = a[i] / b[i];
+    r[i] = __builtin_sqrtf(a[i]);
+  }
+}
+
+}  // namespace vtk
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+
+namespace vtk {
+
+template <bool ANY, bool COUNT>
+static hipError_t launch_one(const TraceParams& p, uint32_t grid, hipStream_t s) {
+  hipLaunchKernelGGL((trace_kernel<ANY, COUNT, kTraceBlock, kLdsDepth>), dim3(grid),
+                     dim3(kTraceBlock), 0, s, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_trace(const TraceParams& p, bool any, bool count, uint32_t grid, hipStream_t s) {
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, p.counter, 1u);
+  if (any) return launch_one<true, false>(p, grid, s);
+  return count ? launch_one<false, true>(p, grid, s) : launch_one<false, false>(p, grid, s);
+}
+
+hipError_t trace_occupancy(bool any, int* blocks_per_cu) {
+  if (any)
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        blocks_per_cu, trace_kernel<true, false, kTraceBlock, kLdsDepth>, kTraceBlock, 0);
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      blocks_per_cu, trace_kernel<false, false, kTraceBlock, kLdsDepth>, kTraceBlock, 0);
+}
+
+hipError_t launch_zero(uint32_t* p, uint32_t n, hipStream_t s) {
+  hipLaunchKernelGGL(zero_u32_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_fp32_selftest(const float* a, const float* b, float* q, float* r, uint32_t n,
+                                hipStream_t s) {
+  hipLaunchKernelGGL(fp32_selftest_kernel, dim3((n + 255) / 256), dim3(256), 0, s, a, b, q, r, n);
+  return hipGetLastError();
+}
+
+}  // namespace vtk

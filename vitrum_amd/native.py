@@ -1,0 +1,310 @@
+"""ctypes binding of libvthip.so (the C ABI declared in include/vitrum_hip.h).
+
+Used by the Python tests and the benchmark.  Loading fails loudly when the
+library has not been built: there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .workloads import HIT_DTYPE, RAY_DTYPE
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libvthip.so")
+_LIB = None
+
+U32P = C.POINTER(C.c_uint32)
+F32P = C.POINTER(C.c_float)
+F64P = C.POINTER(C.c_double)
+
+STATUS = {0: "VT_OK", -1: "VT_ERR_INVALID_ARGUMENT", -2: "VT_ERR_RANGE", -3: "VT_ERR_BUILD",
+          -4: "VT_ERR_HIP", -5: "VT_ERR_NO_DEVICE", -6: "VT_ERR_STATE", -7: "VT_ERR_OUT_OF_MEMORY"}
+
+EXPORTED = [
+    "vt_create", "vt_destroy", "vt_last_error", "vt_set_scene", "vt_trace_closest", "vt_trace_any",
+    "vt_trace_closest_device", "vt_trace_any_device", "vt_set_counting", "vt_trace_closest_counts",
+    "vt_set_trace_tuning", "vt_selftest_fp32", "vt_device_alloc", "vt_device_free",
+    "vt_memcpy_h2d", "vt_memcpy_d2h", "vt_synchronize", "vt_get_stream", "vt_stats",
+    "vt_build_blas", "vt_build_tlas", "vt_pack_scene", "vt_pack_field_words",
+    "vt_pack_copy_field", "vt_pack_free", "vt_version",
+]
+
+
+class VtError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{STATUS.get(status, status)}: {message}")
+        self.status = status
+
+
+class Primitive(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("material", C.c_uint32), ("positions", F32P),
+                ("position_floats", C.c_uint32), ("normals", F32P), ("normal_floats", C.c_uint32),
+                ("indices", U32P), ("index_count", C.c_uint32), ("transforms", F32P),
+                ("transform_count", C.c_uint32)]
+
+
+class Material(C.Structure):
+    _fields_ = [("base_color", C.c_float * 3), ("roughness", C.c_float), ("metallic", C.c_float),
+                ("emissive", C.c_float * 3), ("emissive_intensity", C.c_float),
+                ("double_sided", C.c_uint32)]
+
+
+class Environment(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("radiance", C.c_float * 3)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("primitives", C.POINTER(Primitive)), ("primitive_count", C.c_uint32),
+                ("materials", C.POINTER(Material)), ("material_count", C.c_uint32),
+                ("environment", Environment), ("scene_radius", C.c_float)]
+
+
+class Options(C.Structure):
+    _fields_ = [("max_bounces", C.c_uint32), ("max_samples_per_pixel", C.c_uint32),
+                ("sampling", C.c_uint32), ("reserved", C.c_uint32 * 5)]
+
+
+class FrameStats(C.Structure):
+    _fields_ = [("last_trace_ms", C.c_float), ("last_frame_ms", C.c_float),
+                ("samples_accumulated", C.c_uint32), ("last_ray_count", C.c_uint32),
+                ("device_bytes", C.c_uint64)]
+
+
+PACK_FIELDS = ["positions", "normals", "indices", "tri_material", "tri_source", "tri_primitive",
+               "bvh_nodes", "tlas_nodes", "tlas_instance_indices", "tlas_blas_roots", "tlas_w2l",
+               "tlas_l2w", "instance_primitive", "instance_source"]
+FLOAT_FIELDS = {"positions", "normals", "tlas_w2l", "tlas_l2w"}
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() or "
+                              "make -C vitrum_amd/csrc")
+        L = C.CDLL(LIB_PATH)
+        L.vt_version.restype = C.c_char_p
+        L.vt_pack_field_words.restype = C.c_int64
+        L.vt_pack_field_words.argtypes = [C.c_void_p, C.c_int32]
+        L.vt_pack_copy_field.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_uint64]
+        L.vt_pack_free.argtypes = [C.c_void_p]
+        L.vt_device_alloc.argtypes = [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]
+        L.vt_device_free.argtypes = [C.c_void_p, C.c_void_p]
+        L.vt_memcpy_h2d.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
+        L.vt_memcpy_d2h.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
+        for name in ("vt_trace_closest", "vt_trace_any", "vt_trace_closest_device",
+                     "vt_trace_any_device"):
+            getattr(L, name).argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]
+        L.vt_trace_closest_counts.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
+                                              C.c_void_p]
+        _LIB = L
+    return _LIB
+
+
+def last_error(eng=None) -> str:
+    buf = C.create_string_buffer(1024)
+    lib().vt_last_error(eng, buf, C.c_size_t(len(buf)))
+    return buf.value.decode(errors="replace")
+
+
+def check(status, eng=None):
+    if status != 0:
+        raise VtError(status, last_error(eng))
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def _prim_structs(primitives):
+    keep = []
+    arr = (Primitive * max(1, len(primitives)))()
+    for i, p in enumerate(primitives):
+        pos = np.ascontiguousarray(p["positions"], np.float32).ravel()
+        nrm = np.ascontiguousarray(p["normals"], np.float32).ravel()
+        idx = p.get("indices")
+        idx = None if idx is None else np.ascontiguousarray(idx, np.uint32).ravel()
+        if p["kind"] == "instanced-mesh":
+            xf = np.ascontiguousarray(np.asarray(p["instances"], np.float32).reshape(-1, 16))
+            kind, ntx = 1, xf.shape[0]
+        else:
+            t = p.get("transform")
+            xf = None if t is None else np.ascontiguousarray(np.asarray(t, np.float32).reshape(16))
+            kind, ntx = 0, (0 if xf is None else 1)
+        keep += [pos, nrm, idx, xf]
+        arr[i] = Primitive(kind, p.get("material_id", i), _p(pos, F32P), pos.size, _p(nrm, F32P),
+                           nrm.size, _p(idx, U32P), 0 if idx is None else idx.size, _p(xf, F32P), ntx)
+    return arr, keep
+
+
+def pack_scene(primitives, threads=0):
+    """packSceneFromCore through the product builder (vt_pack_scene)."""
+    arr, _keep = _prim_structs(primitives)
+    handle = C.c_void_p()
+    check(lib().vt_pack_scene(arr, C.c_uint32(len(primitives)), C.c_uint32(threads), C.byref(handle)))
+    out = {}
+    try:
+        for fid, name in enumerate(PACK_FIELDS):
+            n = lib().vt_pack_field_words(handle, fid)
+            buf = np.zeros(max(n, 1), np.float32 if name in FLOAT_FIELDS else np.uint32)
+            check(lib().vt_pack_copy_field(handle, fid, buf.ctypes.data, C.c_uint64(buf.size)))
+            out[name] = buf[:n]
+    finally:
+        lib().vt_pack_free(handle)
+    return out
+
+
+def build_blas(positions, indices, tri_material_ids=None, *, position_stride=4, index_stride=4,
+               max_leaf=4, num_bins=16, max_depth=58):
+    """buildArrayBvh through the product builder (vt_build_blas)."""
+    positions = np.ascontiguousarray(positions, np.float32).ravel()
+    indices = np.ascontiguousarray(indices, np.uint32).ravel()
+    n = indices.size // index_stride
+    mats = np.zeros(n, np.uint32) if tri_material_ids is None else np.ascontiguousarray(
+        tri_material_ids, np.uint32)
+    nodes = np.zeros(max(1, 2 * n - 1) * 8, np.uint32)
+    count = C.c_uint32()
+    ridx = np.zeros(max(indices.size, 1), np.uint32)
+    rmat = np.zeros(max(n, 1), np.uint32)
+    rsrc = np.zeros(max(n, 1), np.uint32)
+    check(lib().vt_build_blas(_p(positions, F32P), C.c_uint32(positions.size),
+                              C.c_uint32(position_stride), _p(indices, U32P),
+                              C.c_uint32(indices.size), C.c_uint32(index_stride), _p(mats, U32P),
+                              C.c_uint32(max_leaf), C.c_uint32(num_bins), C.c_uint32(max_depth),
+                              _p(nodes, U32P), C.byref(count), _p(ridx, U32P), _p(rmat, U32P),
+                              _p(rsrc, U32P)))
+    c = count.value
+    return {"nodes": nodes[: c * 8].copy(), "node_count": c,
+            "reordered_indices": ridx[: indices.size].copy(), "reordered_mat": rmat[:n].copy(),
+            "reordered_to_source": rsrc[:n].copy()}
+
+
+def build_tlas(blas_ids, aabb_min, aabb_max, w2l, *, max_leaf=1, num_bins=16, max_depth=58):
+    blas_ids = np.ascontiguousarray(blas_ids, np.uint32)
+    n = blas_ids.size
+    mn = np.ascontiguousarray(aabb_min, np.float64).ravel()
+    mx = np.ascontiguousarray(aabb_max, np.float64).ravel()
+    xf = np.ascontiguousarray(w2l, np.float32).ravel()
+    nodes = np.zeros(max(1, 2 * n - 1) * 8, np.uint32)
+    count = C.c_uint32()
+    perm = np.zeros(max(n, 1), np.uint32)
+    roots = np.zeros(max(n, 1), np.uint32)
+    tf = np.zeros(max(n, 1) * 16, np.float32)
+    check(lib().vt_build_tlas(C.c_uint32(n), _p(blas_ids, U32P), _p(mn, F64P), _p(mx, F64P),
+                              _p(xf, F32P), C.c_uint32(max_leaf), C.c_uint32(num_bins),
+                              C.c_uint32(max_depth), _p(nodes, U32P), C.byref(count),
+                              _p(perm, U32P), _p(roots, U32P), _p(tf, F32P)))
+    c = count.value
+    return {"nodes": nodes[: c * 8].copy(), "node_count": c, "instance_indices": perm[:n].copy(),
+            "blas_roots": roots[:n].copy(), "transforms": tf[: n * 16].copy()}
+
+
+def _material(m) -> Material:
+    em = m.get("emissive", [0.0, 0.0, 0.0])
+    return Material((C.c_float * 3)(*m.get("baseColor", [0.8, 0.8, 0.8])),
+                    float(m.get("roughness", 1.0)), float(m.get("metallic", 0.0)),
+                    (C.c_float * 3)(*em), float(m.get("emissiveIntensity", 1.0 if any(em) else 0.0)),
+                    1 if m.get("doubleSided", False) else 0)
+
+
+class Engine:
+    """Thin owner of a vt_engine (one per GPU device)."""
+
+    def __init__(self, device: int = 0, max_bounces: int = 8, sampling: int = 0):
+        self._h = C.c_void_p()
+        opts = Options(max_bounces, 0, sampling)
+        check(lib().vt_create(C.c_int32(device), C.byref(opts), C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().vt_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_scene(self, scene):
+        prims = []
+        mats = []
+        for i, p in enumerate(scene["primitives"]):
+            prims.append(dict(p, material_id=i))
+            mats.append(_material(p.get("material", {})))
+        arr, _keep = _prim_structs(prims)
+        marr = (Material * max(1, len(mats)))(*mats)
+        sky = scene.get("sky", [0.0, 0.0, 0.0])
+        env = Environment(1 if any(sky) else 0, (C.c_float * 3)(*sky))
+        desc = SceneDesc(arr, len(prims), marr, len(mats), env, float(scene.get("scene_radius", 0.0)))
+        check(lib().vt_set_scene(self._h, C.byref(desc)), self._h)
+
+    def trace_closest(self, rays):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        hits = np.zeros(rays.size, HIT_DTYPE)
+        check(lib().vt_trace_closest(self._h, rays.ctypes.data, rays.size, hits.ctypes.data), self._h)
+        return hits
+
+    def trace_closest_counts(self, rays):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        hits = np.zeros(rays.size, HIT_DTYPE)
+        counts = np.zeros((rays.size, 4), np.uint32)
+        check(lib().vt_trace_closest_counts(self._h, rays.ctypes.data, rays.size, hits.ctypes.data,
+                                            counts.ctypes.data), self._h)
+        return hits, counts
+
+    def trace_any(self, rays):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        out = np.zeros(rays.size, np.uint32)
+        check(lib().vt_trace_any(self._h, rays.ctypes.data, rays.size, out.ctypes.data), self._h)
+        return out
+
+    def set_trace_tuning(self, refill_min, steps):
+        check(lib().vt_set_trace_tuning(self._h, C.c_uint32(refill_min), C.c_uint32(steps)), self._h)
+
+    def selftest_fp32(self, a, b):
+        a = np.ascontiguousarray(a, np.float32)
+        b = np.ascontiguousarray(b, np.float32)
+        q = np.zeros_like(a)
+        r = np.zeros_like(a)
+        check(lib().vt_selftest_fp32(self._h, _p(a, F32P), _p(b, F32P), _p(q, F32P), _p(r, F32P),
+                                     C.c_uint32(a.size)), self._h)
+        return q, r
+
+    # device-resident plumbing
+    def alloc(self, nbytes):
+        ptr = C.c_void_p()
+        check(lib().vt_device_alloc(self._h, C.c_size_t(nbytes), C.byref(ptr)), self._h)
+        return ptr.value
+
+    def free(self, ptr):
+        check(lib().vt_device_free(self._h, C.c_void_p(ptr)), self._h)
+
+    def h2d(self, dptr, arr):
+        arr = np.ascontiguousarray(arr)
+        check(lib().vt_memcpy_h2d(self._h, C.c_void_p(dptr), arr.ctypes.data, arr.nbytes), self._h)
+
+    def d2h(self, arr, dptr):
+        check(lib().vt_memcpy_d2h(self._h, arr.ctypes.data, C.c_void_p(dptr), arr.nbytes), self._h)
+
+    def trace_closest_device(self, d_rays, n, d_hits):
+        check(lib().vt_trace_closest_device(self._h, C.c_void_p(d_rays), n, C.c_void_p(d_hits)),
+              self._h)
+
+    def trace_any_device(self, d_rays, n, d_out):
+        check(lib().vt_trace_any_device(self._h, C.c_void_p(d_rays), n, C.c_void_p(d_out)), self._h)
+
+    def synchronize(self):
+        check(lib().vt_synchronize(self._h), self._h)
+
+    def stats(self):
+        s = FrameStats()
+        check(lib().vt_stats(self._h, C.byref(s)), self._h)
+        return {k: getattr(s, k) for k, _ in FrameStats._fields_}
