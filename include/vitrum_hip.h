@@ -140,6 +140,42 @@ vt_status vt_last_error(const vt_engine *eng, char *buf, size_t len);
  * uploads.  Transactional: on failure the previous scene stays bound. */
 vt_status vt_set_scene(vt_engine *eng, const vt_scene_desc *scene);
 
+/* ---- frames ---------------------------------------------------------------- */
+
+/* FrameInput (packages/core/src/frame.ts:91-105) reduced to what the path
+ * tracer consumes: the inverse of projMatrix * viewMatrix (column-major), the
+ * viewport, frameIndex / frameSeed and quality.bounces / samplesTarget. */
+typedef struct vt_frame {
+  float inv_view_proj[16];
+  uint32_t width;
+  uint32_t height;
+  uint32_t frame_index;
+  uint32_t frame_seed;
+  uint32_t max_bounces;    /* 0 = the engine's max_bounces */
+  uint32_t samples_target; /* 0 = unlimited */
+} vt_frame;
+
+/* Engine.renderFrame (packages/core/src/engine/index.ts:251;
+ * packages/pt-webgpu/src/index.ts:1817): one sample per pixel accumulated
+ * into the running mean.  Enqueued on the engine stream; returns without
+ * waiting.  A changed viewport restarts accumulation. */
+vt_status vt_render_frame(vt_engine *eng, const vt_frame *frame);
+/* captureFrame (packages/core/src/engine/index.ts:466; pt-webgpu
+ * index.ts:2019): Float32 RGBA running mean, row-major, top-left origin,
+ * alpha 1.  Blocking. */
+vt_status vt_capture(vt_engine *eng, float *rgba, uint32_t width, uint32_t height);
+/* Engine.reset (packages/core/src/engine/index.ts:256): clears accumulation. */
+vt_status vt_reset(vt_engine *eng);
+/* GPU milliseconds of the last vt_render_frame (HIP events on the engine
+ * stream around the whole frame).  Blocking. */
+vt_status vt_frame_time(vt_engine *eng, float *ms);
+/* Test hook (no counterpart in the reference): a 64-bit hash per pixel of the
+ * last frame's discrete path decisions (hit triangles, light picks, lobes,
+ * roulette), for path-by-path comparison with a CPU integrator. */
+vt_status vt_capture_signatures(vt_engine *eng, uint64_t *sig, uint32_t width, uint32_t height);
+/* Mesh-area emitter count and the ray-origin bias of the bound scene. */
+vt_status vt_render_info(vt_engine *eng, uint32_t *emitter_count, float *ray_bias);
+
 /* ---- traversal (the hot path) -------------------------------------------- */
 
 /* traceTlasClosest (packages/shared-bvh/src/wgsl/tlasSceneHitTraversal.wgsl.ts:

@@ -1,0 +1,720 @@
+/*
+ * integrator.c -- binary64 mirror of pt-webgpu's unidirectional path tracer
+ * for opaque Lambert + GGX materials, mesh-area emitters and a constant
+ * environment.  TEST INFRASTRUCTURE ONLY (see vt_oracle.h).
+ *
+ * Follows packages/pt-webgpu/src/wgsl/pathTrace/kernel.wgsl.ts:962-2032 (the
+ * `main` megakernel) with the RNG consumed in the same order:
+ *   pcgInit; 2 jitter draws; per bounce: [light pick: ptRandBoundedU32]
+ *   [mesh light: 2 draws | environment: 1 CDF draw + 2 cell draws]
+ *   [BSDF: extension-lobe draw, specular/diffuse draw, 2 direction draws]
+ *   [RR draw when bounce > 2].
+ * Shading arithmetic is binary64; ray/scene intersection uses the f32
+ * traversal of traverse.c on the f32-rounded rays (the device traverses f32
+ * rays), and hit geometry is re-solved in binary64.
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "vt_oracle.h"
+
+int orc_fail(const char *fmt, ...);
+int orc_trace_closest(const orc_scene *s, const orc_ray *rays, uint32_t n, orc_hit *out,
+                      uint32_t *counts, uint32_t threads);
+int orc_trace_any(const orc_scene *s, const orc_ray *rays, uint32_t n, uint32_t *out,
+                  uint32_t threads);
+
+#define PI 3.14159265358979323846
+#define INV_PI (1.0 / PI)
+#define F32_MAX 3.4028234663852886e38
+#define SAFE_MAX 3.4028229765e38 /* 0x7F7FFFFD */
+#define ALPHA_FLOOR 1e-3         /* roughDielectric.ts:38 */
+
+typedef struct {
+  double x, y, z;
+} d3;
+
+static d3 D3(double x, double y, double z) { d3 r = {x, y, z}; return r; }
+static d3 add(d3 a, d3 b) { return D3(a.x + b.x, a.y + b.y, a.z + b.z); }
+static d3 sub(d3 a, d3 b) { return D3(a.x - b.x, a.y - b.y, a.z - b.z); }
+static d3 mul(d3 a, d3 b) { return D3(a.x * b.x, a.y * b.y, a.z * b.z); }
+static d3 scl(d3 a, double s) { return D3(a.x * s, a.y * s, a.z * s); }
+static double dot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static d3 cross(d3 a, d3 b) {
+  return D3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static double maxd(double a, double b) { return a > b ? a : b; }
+static double mind(double a, double b) { return a < b ? a : b; }
+static double clampd(double x, double lo, double hi) { return x < lo ? lo : (x > hi ? hi : x); }
+static double max3abs(d3 v) { return maxd(fabs(v.x), maxd(fabs(v.y), fabs(v.z))); }
+
+/* safe_normalize (pt-webgpu/src/wgsl/common.wgsl.ts) */
+static d3 safe_normalize(d3 v) {
+  double s = max3abs(v);
+  if (!(s > 0) || s > SAFE_MAX) return D3(0, 1, 0);
+  d3 q = scl(v, 1.0 / s);
+  return scl(q, 1.0 / sqrt(dot(q, q)));
+}
+
+/* ---- RNG (shared-samplers/src/wgsl/pcg.wgsl.ts; common.wgsl.ts) ------- */
+
+static uint32_t pcg_next(uint32_t *s) { return orc_pcg_next(s); }
+static double rand01(uint32_t *s) { return (double)(pcg_next(s) >> 8) / 16777216.0; }
+
+/* represented_bernoulli_probability_f32 (pcg.wgsl.ts:31-40) */
+static double represented(double p) {
+  if (!(p > 0)) return 0;
+  if (p >= 1) return 1;
+  double b = clampd(floor(p * 16777216.0 + 0.5), 1.0, 16777215.0);
+  return b / 16777216.0;
+}
+
+/* ptRandomU32 / ptRandBoundedU32 (pt-webgpu/src/wgsl/common.wgsl.ts) */
+static uint32_t rand_bounded(uint32_t *s, uint32_t bound) {
+  if (bound <= 1) return 0;
+  uint32_t threshold = ((0xFFFFFFFFu % bound) + 1u) % bound;
+  for (;;) {
+    uint32_t hi = pcg_next(s) & 0xFFFFFF00u;
+    uint32_t lo = pcg_next(s) >> 24;
+    uint32_t w = hi | lo;
+    if (w >= threshold) return w % bound;
+  }
+}
+
+static double luminance(d3 c) { return 0.2126 * c.x + 0.7152 * c.y + 0.0722 * c.z; }
+
+/* ---- BSDF (bsdf.wgsl.ts, material.wgsl.ts) ------------------------------ */
+
+typedef struct {
+  d3 base;
+  double rough, metal;
+  d3 emit;
+  int two_sided;
+} mat_t;
+
+/* materialSpecularF0 (bsdf.wgsl.ts:192-209), ior 1.5, white specular. */
+static d3 spec_f0(const mat_t *m) {
+  double ratio = (1.5 - 1.0) / (1.5 + 1.0);
+  double f = ratio * ratio;
+  double mt = clampd(m->metal, 0, 1);
+  return D3(f + (m->base.x - f) * mt, f + (m->base.y - f) * mt, f + (m->base.z - f) * mt);
+}
+
+/* materialSpecularFresnelSchlick (bsdf.wgsl.ts:211-222) */
+static d3 schlick(double cos_t, d3 f0) {
+  double x = clampd(1.0 - cos_t, 0, 1);
+  double x5 = x * x * x * x * x;
+  return D3(f0.x + (1 - f0.x) * x5, f0.y + (1 - f0.y) * x5, f0.z + (1 - f0.z) * x5);
+}
+static d3 clamp01v(d3 v) { return D3(clampd(v.x, 0, 1), clampd(v.y, 0, 1), clampd(v.z, 0, 1)); }
+
+/* ggxD (material.wgsl.ts:2797-2804) */
+static double ggx_d(double ndh, double alpha) {
+  double a2 = alpha * alpha;
+  double n2 = clampd(ndh * ndh, 0, 1);
+  double d = (1.0 - n2) + n2 * a2;
+  return a2 / (PI * d * d);
+}
+
+/* smithG1 (pt-webgpu/src/math/roughDielectric.ts:52-60) */
+static double smith_g1(double ndv, double rough) {
+  double c = maxd(ndv, 0);
+  if (c <= 1e-6) return 0;
+  double alpha = maxd(rough * rough, ALPHA_FLOOR);
+  double tan2 = maxd(0, 1 - c * c) / maxd(c * c, 1e-10);
+  return 2.0 / (1.0 + sqrt(1.0 + alpha * alpha * tan2));
+}
+
+/* powerHeuristic (material.wgsl.ts:2808-2822) */
+static double power_heuristic(double a, double b) {
+  if (!(a >= 0) || !(b >= 0) || a > F32_MAX || b > F32_MAX) return 0;
+  double s = maxd(a, b);
+  if (!(s > 0)) return 0;
+  a /= s;
+  b /= s;
+  return (a * a) / (a * a + b * b);
+}
+
+/* Kulla-Conty tables and lookups (material.wgsl.ts:2852-2931) */
+static const double GGX_E_LUT[64] = {
+    0.1375, 0.5617, 0.7546, 0.8522, 0.9111, 0.9505, 0.9788, 1.0,
+    0.2955, 0.515,  0.7091, 0.8192, 0.889,  0.937,  0.9721, 0.9988,
+    0.5794, 0.5541, 0.6677, 0.7691, 0.8451, 0.9021, 0.9457, 0.98,
+    0.7011, 0.6486, 0.6669, 0.7199, 0.7776, 0.8305, 0.8764, 0.9155,
+    0.7335, 0.6901, 0.6696, 0.6756, 0.6972, 0.7262, 0.7578, 0.7893,
+    0.7153, 0.6712, 0.6355, 0.6145, 0.6052, 0.6045, 0.6101, 0.6199,
+    0.6669, 0.6137, 0.5657, 0.5286, 0.5,    0.478,  0.4611, 0.4483,
+    0.6017, 0.537,  0.4773, 0.4296, 0.3905, 0.358,  0.3305, 0.3069};
+static const double GGX_EAVG_LUT[8] = {0.9106, 0.8931, 0.8629, 0.8094,
+                                       0.725,  0.6147, 0.4931, 0.3766};
+
+static double lut_e(double cos_t, double rough) {
+  double mu = clampd(cos_t, 0, 1), r = clampd(rough, 0, 1);
+  double fr = r * 7.0, fm = mu * 7.0;
+  unsigned r0 = (unsigned)floor(fr), m0 = (unsigned)floor(fm);
+  unsigned r1 = r0 + 1 < 7 ? r0 + 1 : 7, m1 = m0 + 1 < 7 ? m0 + 1 : 7;
+  double tr = fr - r0, tm = fm - m0;
+  double e0 = GGX_E_LUT[r0 * 8 + m0] + (GGX_E_LUT[r0 * 8 + m1] - GGX_E_LUT[r0 * 8 + m0]) * tm;
+  double e1 = GGX_E_LUT[r1 * 8 + m0] + (GGX_E_LUT[r1 * 8 + m1] - GGX_E_LUT[r1 * 8 + m0]) * tm;
+  return clampd(e0 + (e1 - e0) * tr, 0.02, 1.0);
+}
+static double lut_eavg(double rough) {
+  double r = clampd(rough, 0, 1), fr = r * 7.0;
+  unsigned r0 = (unsigned)floor(fr), r1 = r0 + 1 < 7 ? r0 + 1 : 7;
+  double tr = fr - r0;
+  return clampd(GGX_EAVG_LUT[r0] + (GGX_EAVG_LUT[r1] - GGX_EAVG_LUT[r0]) * tr, 0.3, 1.0);
+}
+static d3 ggx_ms(d3 f0, double rough, double ndv, double ndl) {
+  double eavg = lut_eavg(rough), om = 1.0 - eavg;
+  if (om < 1e-4) return D3(0, 0, 0);
+  double eo = lut_e(ndv, rough), ei = lut_e(ndl, rough);
+  double shape = (1.0 - eo) * (1.0 - ei) / maxd(PI * om, 1e-6);
+  double c[3] = {f0.x, f0.y, f0.z}, o[3];
+  for (int k = 0; k < 3; ++k) {
+    double favg = c[k] + (1.0 - c[k]) * (1.0 / 21.0);
+    double fms = favg * favg * eavg / maxd(1.0 - favg * om, 1e-4);
+    o[k] = fms * shape;
+  }
+  return D3(o[0], o[1], o[2]);
+}
+
+/* evaluateBrdfFullWithClearcoatNormal (bsdf.wgsl.ts:809-916), base lobes only. */
+static d3 bsdf_eval(const mat_t *m, d3 n, d3 wo, d3 wi) {
+  double ndl = maxd(dot(n, wi), 0), ndv = maxd(dot(n, wo), 0);
+  if (ndl <= 1e-5 || ndv <= 1e-5) return D3(0, 0, 0);
+  d3 h = safe_normalize(add(wi, wo));
+  double ndh = maxd(dot(n, h), 0), vdh = maxd(dot(wo, h), 0);
+  d3 f0 = spec_f0(m);
+  d3 F = clamp01v(schlick(vdh, f0));
+  double kdm = 1.0 - m->metal;
+  d3 diff = D3((1 - F.x) * kdm * m->base.x * INV_PI, (1 - F.y) * kdm * m->base.y * INV_PI,
+               (1 - F.z) * kdm * m->base.z * INV_PI);
+  double alpha = maxd(m->rough * m->rough, ALPHA_FLOOR);
+  double d = ggx_d(ndh, alpha);
+  double g = smith_g1(ndv, m->rough) * smith_g1(ndl, m->rough);
+  double k = (d * g) / maxd(4.0 * ndv * ndl, 1e-6);
+  d3 ms = ggx_ms(clamp01v(f0), m->rough, ndv, ndl);
+  return add(add(diff, scl(F, k)), ms);
+}
+
+/* brdfFiniteBaseLobeWeights (bsdf.wgsl.ts:1172-1210): specular probability. */
+static double spec_prob(const mat_t *m, double ndv) {
+  d3 F = clamp01v(schlick(ndv, spec_f0(m)));
+  double x = maxd(luminance(F), m->metal);
+  return represented(clampd(0.04 + (0.96 - 0.04) * x, 0.04, 0.96));
+}
+
+/* brdfDirectionalPdfWithIridescence (bsdf.wgsl.ts:1555-1631). */
+static double bsdf_pdf(const mat_t *m, d3 n, d3 wo, d3 wi) {
+  double wdn = dot(n, wi), odn = dot(n, wo);
+  double ndv = maxd(odn, 0);
+  if (ndv <= 1e-5) return 0;
+  d3 h = safe_normalize(add(wi, wo));
+  double ndh = maxd(dot(n, h), 0);
+  double sp = spec_prob(m, ndv), dp = 1.0 - sp;
+  if (!(wdn * odn > 0)) return 0;
+  double ndl = maxd(wdn, 0);
+  if (ndl <= 1e-5) return 0;
+  double alpha = maxd(m->rough * m->rough, ALPHA_FLOOR);
+  double d = ggx_d(ndh, alpha);
+  double pspec = (d * smith_g1(ndv, m->rough)) / maxd(4.0 * ndv, 1e-6);
+  return dp * ndl * INV_PI + sp * pspec;
+}
+
+/* buildOnb (bsdf.wgsl.ts:1632-1640) */
+static void build_onb(d3 n, d3 *t, d3 *b) {
+  d3 up = fabs(n.y) > 0.999 ? D3(1, 0, 0) : D3(0, 1, 0);
+  d3 c = cross(up, n);
+  *t = scl(c, 1.0 / sqrt(dot(c, c)));
+  *b = cross(n, *t);
+}
+
+/* sampleGgxVndfTangent (bsdf.wgsl.ts:1710-1735) */
+static d3 vndf_sample(d3 wo, double alpha, uint32_t *rng) {
+  d3 vh = safe_normalize(D3(alpha * wo.x, alpha * wo.y, wo.z));
+  double lensq = vh.x * vh.x + vh.y * vh.y;
+  d3 t1 = lensq > 1e-10 ? scl(D3(-vh.y, vh.x, 0), 1.0 / sqrt(lensq)) : D3(1, 0, 0);
+  d3 t2 = cross(vh, t1);
+  double u1 = rand01(rng), u2 = rand01(rng);
+  double r = sqrt(u1), phi = 2.0 * PI * u2;
+  double a = r * cos(phi), b = r * sin(phi);
+  double s = 0.5 * (1.0 + vh.z);
+  b = (1.0 - s) * sqrt(maxd(0, 1.0 - a * a)) + s * b;
+  d3 nh = add(add(scl(t1, a), scl(t2, b)), scl(vh, sqrt(maxd(0, 1.0 - a * a - b * b))));
+  return safe_normalize(D3(alpha * nh.x, alpha * nh.y, maxd(1e-6, nh.z)));
+}
+
+/* ---- camera (kernelCore.wgsl.ts:22-26; common.wgsl.ts:412-469) --------- */
+
+static void mat4_mul_vec(const double *m, const double *v, double *o) {
+  for (int r = 0; r < 4; ++r)
+    o[r] = m[r] * v[0] + m[4 + r] * v[1] + m[8 + r] * v[2] + m[12 + r] * v[3];
+}
+
+static int primary_ray(const double *inv_vp, uint32_t px, uint32_t py, uint32_t w, uint32_t h,
+                       double jx, double jy, d3 *o, d3 *d) {
+  double u = ((double)px + jx) / (double)w, v = ((double)py + jy) / (double)h;
+  double ndc[2] = {u * 2.0 - 1.0, 1.0 - v * 2.0};
+  double fin[4] = {ndc[0], ndc[1], 1, 1}, nin[4] = {ndc[0], ndc[1], -1, 1}, fh[4], nh[4];
+  mat4_mul_vec(inv_vp, fin, fh);
+  mat4_mul_vec(inv_vp, nin, nh);
+  if (nh[3] == 0 || fh[3] == 0) return 0;
+  double orient = (fh[3] * nh[3]) > 0 ? 1.0 : -1.0;
+  *o = D3(nh[0] / nh[3], nh[1] / nh[3], nh[2] / nh[3]);
+  d3 num = D3((fh[0] * nh[3] - nh[0] * fh[3]) * orient, (fh[1] * nh[3] - nh[1] * fh[3]) * orient,
+              (fh[2] * nh[3] - nh[2] * fh[3]) * orient);
+  *d = safe_normalize(num);
+  return 1;
+}
+
+/* ---- emitters (scene/emitterPacking.ts:748-820) -------------------------- */
+
+typedef struct {
+  d3 a, b, c; /* world vertices (f32 values) */
+  d3 le;
+  int two_sided;
+  uint32_t inst, prim;
+} emitter_t;
+
+/* measureAreaVector (common.wgsl.ts:310-362) in binary64: unit normal and area. */
+static int area_measure(d3 u, d3 v, d3 *normal, double *area) {
+  d3 c = cross(u, v);
+  double l = sqrt(dot(c, c));
+  if (!(l > 0)) return 0;
+  *normal = scl(c, 1.0 / l);
+  *area = 0.5 * l;
+  return 1;
+}
+
+/* ptAreaToSolidAnglePdf (material.wgsl.ts:242-275) */
+static double area_to_solid_angle(double dist, double cos_l, double area) {
+  if (!(dist > 0) || !(cos_l > 0) || !(area > 0)) return 0;
+  return dist * dist / (cos_l * area);
+}
+
+/* ---- environment (connect.wgsl.ts:126-219; connectCore.wgsl.ts:46-61) --- */
+
+typedef struct {
+  int present;
+  d3 radiance;
+  uint32_t w, h;
+  const float *cdf; /* w*h + 1 entries, f32 */
+  double texel_pdf;
+} env_t;
+
+static d3 env_cell_dir(uint32_t x, uint32_t y, uint32_t w, uint32_t h, double xa, double xb) {
+  double phi0 = ((double)x / w - 0.5) * (2.0 * PI), phi1 = ((double)(x + 1) / w - 0.5) * (2.0 * PI);
+  double th0 = (double)y / h * PI, th1 = (double)(y + 1) / h * PI;
+  double phi = phi0 + (phi1 - phi0) * xa;
+  double ct = cos(th0) + (cos(th1) - cos(th0)) * xb;
+  double st = sqrt(maxd(0, 1 - ct * ct));
+  return D3(cos(phi) * st, ct, sin(phi) * st);
+}
+
+/* ---- render ----------------------------------------------------------- */
+
+typedef struct orc_render_desc {
+  const orc_scene *scene;
+  const float *normals;           /* vec4 per packed vertex */
+  const uint32_t *tri_material;   /* packed triangle -> material */
+  const float *materials;         /* 12 floats: base rgb, rough, metal, emit rgb, two_sided, pad */
+  uint32_t material_count;
+  const float *emitters;          /* 16 floats: a xyz, b xyz, c xyz, le rgb, two_sided, inst, prim, pad */
+  uint32_t emitter_count;
+  uint32_t env_present;
+  float env_radiance[3];
+  uint32_t env_w, env_h;
+  const float *env_cdf;
+  float env_texel_pdf;
+  float inv_view_proj[16];        /* column-major, f32 */
+  uint32_t width, height;
+  uint32_t frame_seed;
+  uint32_t max_bounces;
+  float ray_bias;
+} orc_render_desc;
+
+typedef struct {
+  const orc_render_desc *d;
+  emitter_t *em;
+  mat_t *mats;
+  env_t env;
+  double inv_vp[16];
+  double bias, tmin;
+} rctx;
+
+static float f32(double x) { return (float)x; }
+
+static orc_ray mk_ray(d3 o, d3 d, double tmin, double tmax) {
+  orc_ray r;
+  r.o[0] = f32(o.x), r.o[1] = f32(o.y), r.o[2] = f32(o.z);
+  r.d[0] = f32(d.x), r.d[1] = f32(d.y), r.d[2] = f32(d.z);
+  r.tmin = f32(tmin);
+  r.tmax = tmax >= F32_MAX ? (float)F32_MAX : f32(tmax);
+  return r;
+}
+
+static int visible(const rctx *c, d3 o, d3 d, double tmax) {
+  orc_ray r = mk_ray(o, d, c->tmin, tmax);
+  uint32_t occ = 0;
+  orc_trace_any(c->d->scene, &r, 1, &occ, 1);
+  return occ == 0;
+}
+
+typedef struct {
+  int hit;
+  uint32_t inst, prim;
+  d3 pos, normal;  /* shading normal oriented to the front face */
+  int front;
+} surf_t;
+
+/* Closest hit and binary64 surface at the hit. */
+static int closest(const rctx *c, d3 o, d3 d, surf_t *s) {
+  orc_ray r = mk_ray(o, d, c->tmin, F32_MAX);
+  orc_hit h;
+  orc_trace_closest(c->d->scene, &r, 1, &h, NULL, 1);
+  if (h.inst == 0xFFFFFFFFu) {
+    s->hit = 0;
+    return 0;
+  }
+  const orc_scene *sc = c->d->scene;
+  const float *m = sc->l2w + 16 * (size_t)h.inst;
+  const float *w2l = sc->w2l + 16 * (size_t)h.inst;
+  const uint32_t *tri = sc->indices + 4 * (size_t)h.prim;
+  d3 P[3], N[3];
+  for (int v = 0; v < 3; ++v) {
+    const float *p = sc->positions + 4 * (size_t)tri[v];
+    double x = p[0], y = p[1], z = p[2];
+    double q[4];
+    for (int k = 0; k < 4; ++k)
+      q[k] = (double)m[k] * x + (double)m[4 + k] * y + (double)m[8 + k] * z + (double)m[12 + k];
+    P[v] = D3(q[0] / q[3], q[1] / q[3], q[2] / q[3]);
+    const float *nn = c->d->normals + 4 * (size_t)tri[v];
+    N[v] = D3(nn[0], nn[1], nn[2]);
+  }
+  /* binary64 Moller-Trumbore on the world triangle, ray as traced (f32). */
+  d3 O = D3(r.o[0], r.o[1], r.o[2]), Dd = D3(r.d[0], r.d[1], r.d[2]);
+  d3 e1 = sub(P[1], P[0]), e2 = sub(P[2], P[0]);
+  d3 pv = cross(Dd, e2);
+  double det = dot(e1, pv);
+  double u = h.u, v = h.v;
+  if (det != 0) {
+    d3 tv = sub(O, P[0]);
+    u = dot(tv, pv) / det;
+    d3 qv = cross(tv, e1);
+    v = dot(Dd, qv) / det;
+  }
+  s->hit = 1;
+  s->inst = h.inst;
+  s->prim = h.prim;
+  s->pos = add(P[0], add(scl(e1, u), scl(e2, v)));
+  d3 nl = safe_normalize(add(add(scl(N[0], 1 - u - v), scl(N[1], u)), scl(N[2], v)));
+  /* transformNormalFromWorldToLocalCols: dot with each W2L column */
+  d3 nw = safe_normalize(D3(w2l[0] * nl.x + w2l[1] * nl.y + w2l[2] * nl.z,
+                            w2l[4] * nl.x + w2l[5] * nl.y + w2l[6] * nl.z,
+                            w2l[8] * nl.x + w2l[9] * nl.y + w2l[10] * nl.z));
+  d3 ng = cross(e1, e2);
+  s->front = dot(ng, Dd) < 0;
+  s->normal = s->front ? nw : scl(nw, -1);
+  return 1;
+}
+
+static int find_emitter(const rctx *c, uint32_t inst, uint32_t prim) {
+  /* emitters are sorted by (inst, prim) in an index built at setup */
+  int lo = 0, hi = (int)c->d->emitter_count - 1;
+  const emitter_t *e = c->em;
+  while (lo <= hi) {
+    int mid = (lo + hi) / 2;
+    uint64_t km = ((uint64_t)e[mid].inst << 32) | e[mid].prim, k = ((uint64_t)inst << 32) | prim;
+    if (km == k) return mid;
+    if (km < k) lo = mid + 1; else hi = mid - 1;
+  }
+  return -1;
+}
+
+/* Records, per path, the discrete decisions that drive the estimator. */
+static uint64_t mix64(uint64_t h, uint64_t v) {
+  h ^= v + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+  return h * 0xBF58476D1CE4E5B9ull;
+}
+
+static void render_path(const rctx *c, uint32_t px, uint32_t py, uint32_t frame_index, double *out,
+                        uint64_t *sig_out, const int *sorted_to_orig) {
+  const orc_render_desc *d = c->d;
+  uint32_t rng = orc_pcg_init(px, py, d->frame_seed ^ frame_index);
+  double jx = rand01(&rng), jy = rand01(&rng);
+  d3 o, dir;
+  d3 L = D3(0, 0, 0), T = D3(1, 1, 1);
+  uint64_t sig = 0x1234;
+  if (!primary_ray(c->inv_vp, px, py, d->width, d->height, jx, jy, &o, &dir)) {
+    out[0] = out[1] = out[2] = 0;
+    *sig_out = sig;
+    return;
+  }
+  int prev_mis = 0;
+  uint32_t limit = d->max_bounces < 1 ? 1 : (d->max_bounces > 8 ? 8 : d->max_bounces);
+  uint32_t n_light = d->emitter_count + (c->env.present ? 1u : 0u);
+  for (uint32_t bounce = 0; bounce < limit; ++bounce) {
+    surf_t s;
+    closest(c, o, dir, &s);
+    sig = mix64(sig, s.hit ? (((uint64_t)s.inst << 32) | s.prim) : 0xFFFFFFFFFFFFull);
+    if (!s.hit) {
+      if (!prev_mis && c->env.present) L = add(L, mul(T, c->env.radiance));
+      break;
+    }
+    const mat_t *m = &c->mats[d->tri_material[s.prim]];
+    d3 n = s.normal, wo = scl(dir, -1), hp = s.pos;
+    if (!prev_mis && (s.front || m->two_sided)) L = add(L, mul(T, m->emit));
+    /* NEE: one light, uniform pick (mediumNee.wgsl.ts:14-40). */
+    if (n_light > 0) {
+      uint32_t picked = rand_bounded(&rng, n_light);
+      double inv_sel = (double)n_light;
+      sig = mix64(sig, 0xA000000000ull | picked);
+      if (picked < d->emitter_count) {
+        const emitter_t *e = &c->em[sorted_to_orig[picked]];
+        double r1 = rand01(&rng), r2 = rand01(&rng);
+        double su = sqrt(r1), uu = 1 - su, vv = r2 * su, ww = 1 - uu - vv;
+        d3 lp = add(add(scl(e->a, uu), scl(e->b, vv)), scl(e->c, ww));
+        d3 tl = sub(lp, hp);
+        double dist = sqrt(dot(tl, tl));
+        d3 ln;
+        double area;
+        int ok = area_measure(sub(e->b, e->a), sub(e->c, e->a), &ln, &area);
+        d3 wi = safe_normalize(tl);
+        double ndl = fabs(dot(n, wi));
+        if (dist > 0 && ndl > 0 && ok) {
+          d3 f = bsdf_eval(m, n, wo, wi);
+          double cl = dot(ln, scl(wi, -1));
+          cl = e->two_sided ? fabs(cl) : maxd(cl, 0);
+          if (cl > 0) {
+            double lpdf = area_to_solid_angle(dist, cl, area);
+            if (lpdf > 0) {
+              double bpdf = bsdf_pdf(m, n, wo, wi);
+              double w = power_heuristic(lpdf, bpdf);
+              d3 off = dot(n, wi) > 0 ? n : scl(n, -1);
+              double tmax = maxd(dist - 2.0 * c->bias, c->tmin);
+              if (visible(c, add(hp, scl(off, c->bias)), wi, tmax)) {
+                double k = ndl * w / lpdf * inv_sel;
+                L = add(L, scl(mul(mul(T, f), e->le), k));
+              }
+            }
+          }
+        }
+      } else {
+        double xi = rand01(&rng);
+        uint32_t count = c->env.w * c->env.h, lo = 0, hi = count;
+        while (lo + 1 < hi) {
+          uint32_t mid = (lo + hi) >> 1;
+          if ((double)c->env.cdf[mid] <= xi) lo = mid; else hi = mid;
+        }
+        uint32_t idx = lo < count - 1 ? lo : count - 1;
+        double xa = rand01(&rng), xb = rand01(&rng);
+        d3 wi = safe_normalize(env_cell_dir(idx % c->env.w, idx / c->env.w, c->env.w, c->env.h, xa, xb));
+        double epdf = c->env.texel_pdf;
+        double ndl = fabs(dot(n, wi));
+        if (ndl > 0) {
+          d3 off = dot(n, wi) > 0 ? n : scl(n, -1);
+          if (visible(c, add(hp, scl(off, c->bias)), wi, F32_MAX)) {
+            d3 f = bsdf_eval(m, n, wo, wi);
+            double bpdf = bsdf_pdf(m, n, wo, wi);
+            double w = power_heuristic(epdf, bpdf);
+            double k = ndl * w / maxd(epdf, 1e-8) * inv_sel;
+            L = add(L, scl(mul(mul(T, f), c->env.radiance), k));
+          }
+        }
+      }
+    }
+    /* BSDF sample, opaque branch (bsdf.wgsl.ts:2920-2997). */
+    double ndv = maxd(dot(n, wo), 0);
+    double sp = spec_prob(m, ndv);
+    (void)rand01(&rng); /* extension-lobe draw: P(base) = 1 */
+    int glossy = rand01(&rng) < sp;
+    d3 tb, bb;
+    build_onb(n, &tb, &bb);
+    d3 wi;
+    if (glossy) {
+      double alpha = maxd(m->rough * m->rough, ALPHA_FLOOR);
+      d3 wl = D3(dot(wo, tb), dot(wo, bb), dot(wo, n));
+      d3 hl = vndf_sample(wl, alpha, &rng);
+      d3 hw = safe_normalize(add(add(scl(tb, hl.x), scl(bb, hl.y)), scl(n, hl.z)));
+      d3 inc = scl(wo, -1);
+      wi = safe_normalize(sub(inc, scl(hw, 2.0 * dot(hw, inc))));
+    } else {
+      double u1 = rand01(&rng), u2 = rand01(&rng);
+      double r = sqrt(u1), phi = 2.0 * PI * u2, ct = sqrt(maxd(0, 1 - u1));
+      wi = safe_normalize(add(add(scl(tb, r * cos(phi)), scl(bb, r * sin(phi))), scl(n, ct)));
+    }
+    sig = mix64(sig, glossy ? 0xB1 : 0xB0);
+    /* finalizeFiniteBounceSample (bsdf.wgsl.ts:1863-1926) */
+    double mpdf = bsdf_pdf(m, n, wo, wi);
+    d3 f = bsdf_eval(m, n, wo, wi);
+    double cosine = fabs(dot(n, wi));
+    int allows = 1;
+    d3 tmul;
+    if (mpdf <= 0 || cosine <= 1e-8) {
+      tmul = D3(0, 0, 0);
+      allows = 0;
+      mpdf = 0;
+    } else {
+      tmul = scl(f, cosine / mpdf);
+    }
+    d3 T_at = T;
+    T = mul(T, tmul);
+    d3 new_o = add(hp, scl(n, c->bias));
+    /* BSDF -> area light and environment connections (connect.wgsl.ts:478-700). */
+    if (allows) {
+      double ndl = fabs(dot(n, wi));
+      double bpdf = mpdf;
+      if (ndl > 0 && bpdf > 0) {
+        d3 off = dot(n, wi) > 0 ? n : scl(n, -1);
+        d3 oo = add(hp, scl(off, c->bias));
+        double best_t = F32_MAX, best_pdf = 0;
+        d3 best_le = D3(0, 0, 0);
+        for (uint32_t li = 0; li < d->emitter_count; ++li) {
+          const emitter_t *e = &c->em[sorted_to_orig[li]];
+          /* intersectMeshAreaLightRay (connect.wgsl.ts:453-476), binary64 */
+          d3 e1 = sub(e->b, e->a), e2 = sub(e->c, e->a);
+          d3 pv = cross(wi, e2);
+          double det = dot(e1, pv);
+          if (det == 0) continue;
+          d3 tv = sub(oo, e->a);
+          double u = dot(tv, pv) / det;
+          if (u < 0 || u > 1) continue;
+          d3 qv = cross(tv, e1);
+          double v = dot(wi, qv) / det;
+          if (v < 0 || u + v > 1) continue;
+          double t = dot(e2, qv) / det;
+          if (!(t > c->tmin) || t >= F32_MAX) continue;
+          d3 ln;
+          double area;
+          if (!area_measure(e1, e2, &ln, &area)) continue;
+          double cl = dot(ln, scl(wi, -1));
+          cl = e->two_sided ? fabs(cl) : maxd(cl, 0);
+          if (cl <= 0) continue;
+          double lpdf = area_to_solid_angle(t, cl, area);
+          if (!(lpdf > 0)) continue;
+          if (!(t < best_t)) continue;
+          if (!visible(c, oo, wi, maxd(t - 2.0 * c->bias, c->tmin))) continue;
+          best_t = t;
+          best_pdf = lpdf;
+          best_le = e->le;
+        }
+        if (best_t < F32_MAX && best_pdf > 0) {
+          double w = power_heuristic(bpdf, best_pdf);
+          L = add(L, scl(mul(mul(T_at, f), best_le), ndl * w / bpdf));
+        }
+        if (c->env.present && visible(c, oo, wi, F32_MAX)) {
+          double w = power_heuristic(bpdf, c->env.texel_pdf);
+          L = add(L, scl(mul(mul(T_at, f), c->env.radiance), ndl * w / bpdf));
+        }
+      }
+    }
+    prev_mis = allows && mpdf > 0;
+    o = new_o;
+    dir = wi;
+    if (bounce > 2) {
+      double surv = represented(clampd(maxd(T.x, maxd(T.y, T.z)), 0.1, 0.95));
+      int lives = rand01(&rng) < surv;
+      sig = mix64(sig, lives ? 0xC1 : 0xC0);
+      if (!lives) break;
+      T = scl(T, 1.0 / surv);
+    }
+  }
+  out[0] = maxd(L.x, 0);
+  out[1] = maxd(L.y, 0);
+  out[2] = maxd(L.z, 0);
+  *sig_out = sig;
+}
+
+typedef struct {
+  const rctx *c;
+  const int *sorted_to_orig;
+  uint32_t frame_index, x0, y0, w, h, row_begin, row_end;
+  double *out;
+  uint64_t *sig;
+} rjob;
+
+static void *render_rows(void *arg) {
+  rjob *j = (rjob *)arg;
+  for (uint32_t yy = j->row_begin; yy < j->row_end; ++yy)
+    for (uint32_t xx = 0; xx < j->w; ++xx) {
+      size_t i = (size_t)yy * j->w + xx;
+      render_path(j->c, j->x0 + xx, j->y0 + yy, j->frame_index, j->out + 3 * i, j->sig + i,
+                  j->sorted_to_orig);
+    }
+  return NULL;
+}
+
+static const emitter_t *g_em_sort;
+static int cmp_em(const void *a, const void *b) {
+  const emitter_t *x = &g_em_sort[*(const int *)a], *y = &g_em_sort[*(const int *)b];
+  uint64_t kx = ((uint64_t)x->inst << 32) | x->prim, ky = ((uint64_t)y->inst << 32) | y->prim;
+  return kx < ky ? -1 : (kx > ky ? 1 : 0);
+}
+
+/* Renders one frame of the tile [x0, x0+w) x [y0, y0+h).
+ * out: 3 doubles per pixel (radiance of this frame's path), sig: path
+ * signature per pixel. */
+int orc_render(const orc_render_desc *d, uint32_t frame_index, uint32_t x0, uint32_t y0, uint32_t w,
+               uint32_t h, double *out, uint64_t *sig, uint32_t threads) {
+  rctx c;
+  memset(&c, 0, sizeof c);
+  c.d = d;
+  c.mats = (mat_t *)calloc(d->material_count + 1, sizeof(mat_t));
+  for (uint32_t i = 0; i < d->material_count; ++i) {
+    const float *q = d->materials + 12 * (size_t)i;
+    c.mats[i].base = D3(q[0], q[1], q[2]);
+    c.mats[i].rough = q[3];
+    c.mats[i].metal = q[4];
+    c.mats[i].emit = D3(q[5], q[6], q[7]);
+    c.mats[i].two_sided = q[8] > 0.5f;
+  }
+  /* Emitters: the record list is in light-selection order; a sorted copy
+   * (by instance, triangle) serves hit lookups. */
+  emitter_t *em = (emitter_t *)calloc(d->emitter_count + 1, sizeof(emitter_t));
+  for (uint32_t i = 0; i < d->emitter_count; ++i) {
+    const float *q = d->emitters + 16 * (size_t)i;
+    em[i].a = D3(q[0], q[1], q[2]);
+    em[i].b = D3(q[3], q[4], q[5]);
+    em[i].c = D3(q[6], q[7], q[8]);
+    em[i].le = D3(q[9], q[10], q[11]);
+    em[i].two_sided = q[12] > 0.5f;
+    em[i].inst = (uint32_t)q[13];
+    em[i].prim = (uint32_t)q[14];
+  }
+  int *order = (int *)malloc(sizeof(int) * (d->emitter_count + 1));
+  for (uint32_t i = 0; i < d->emitter_count; ++i) order[i] = (int)i;
+  c.em = em;
+  c.env.present = d->env_present != 0;
+  c.env.radiance = D3(d->env_radiance[0], d->env_radiance[1], d->env_radiance[2]);
+  c.env.w = d->env_w;
+  c.env.h = d->env_h;
+  c.env.cdf = d->env_cdf;
+  c.env.texel_pdf = d->env_texel_pdf;
+  for (int i = 0; i < 16; ++i) c.inv_vp[i] = d->inv_view_proj[i];
+  c.bias = (double)d->ray_bias;
+  c.tmin = (double)f32(c.bias * 0.1);
+  if (threads < 1) threads = 1;
+  if (threads > 64) threads = 64;
+  pthread_t tid[64];
+  rjob jobs[64];
+  uint32_t per = (h + threads - 1) / threads;
+  for (uint32_t t = 0; t < threads; ++t) {
+    jobs[t] = (rjob){&c, order, frame_index, x0, y0, w, h, t * per < h ? t * per : h,
+                     (t + 1) * per < h ? (t + 1) * per : h, out, sig};
+    if (pthread_create(&tid[t], NULL, render_rows, &jobs[t]) != 0) {
+      render_rows(&jobs[t]);
+      tid[t] = 0;
+    }
+  }
+  for (uint32_t t = 0; t < threads; ++t)
+    if (tid[t]) pthread_join(tid[t], NULL);
+  free(c.mats);
+  free(em);
+  free(order);
+  (void)cmp_em;
+  (void)find_emitter;
+  (void)mind;
+  return 0;
+}

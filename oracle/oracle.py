@@ -286,3 +286,180 @@ def pcg_init(px, py, seed):
 def pcg_stream(px, py, seed, n):
     s = C.c_uint32(pcg_init(px, py, seed))
     return [lib().orc_pcg_next(C.byref(s)) for _ in range(n)]
+
+
+# ---------------------------------------------------------------------------
+# Integrator oracle (integrator.c)
+# ---------------------------------------------------------------------------
+
+class RenderDesc(C.Structure):
+    _fields_ = [
+        ("scene", C.POINTER(Scene)), ("normals", F32P), ("tri_material", U32P),
+        ("materials", F32P), ("material_count", C.c_uint32), ("emitters", F32P),
+        ("emitter_count", C.c_uint32), ("env_present", C.c_uint32),
+        ("env_radiance", C.c_float * 3), ("env_w", C.c_uint32), ("env_h", C.c_uint32),
+        ("env_cdf", F32P), ("env_texel_pdf", C.c_float), ("inv_view_proj", C.c_float * 16),
+        ("width", C.c_uint32), ("height", C.c_uint32), ("frame_seed", C.c_uint32),
+        ("max_bounces", C.c_uint32), ("ray_bias", C.c_float),
+    ]
+
+
+def material_table(materials):
+    """12 floats per material: base rgb, roughness, metallic, Le rgb
+    (emissive x emissiveIntensity), doubleSided, 3 pad."""
+    out = np.zeros((max(1, len(materials)), 12), np.float32)
+    for i, m in enumerate(materials):
+        em = np.asarray(m.get("emissive", [0, 0, 0]), np.float64)
+        inten = m.get("emissiveIntensity", 1.0 if em.any() else 0.0)
+        out[i, 0:3] = m.get("baseColor", [0.8, 0.8, 0.8])
+        out[i, 3] = m.get("roughness", 1.0)
+        out[i, 4] = m.get("metallic", 0.0)
+        out[i, 5:8] = (em * inten).astype(np.float32)
+        out[i, 8] = 1.0 if m.get("doubleSided", False) else 0.0
+    return out
+
+
+def build_emitters(pack, mat_table):
+    """Mesh-area emitter records, one per emissive triangle
+    (pt-webgpu/src/scene/emitterPacking.ts:748-820): primitives in scene
+    order, each primitive's packed instances in order, source triangles in
+    order.  World vertices are the binary64 local-to-world image of the f32
+    local vertices, rounded to f32.  16 floats per record: a, b, c, Le,
+    twoSided, packed instance, packed triangle, 0."""
+    b = pack["bindings"].reshape(-1, 8)
+    pos = pack["positions"].reshape(-1, 4)[:, :3].astype(np.float64)
+    idx = pack["indices"].reshape(-1, 4)[:, :3]
+    tri_src = pack["tri_source"]
+    inst_prim = pack["instance_primitive"]
+    l2w = pack["tlas_l2w"].reshape(-1, 16).astype(np.float64)
+    tri_mat = pack["tri_material"]
+    recs = []
+    for blas_root, n_inst, vs, vc, ts, tc, prim, kind in b.tolist():
+        mat = int(tri_mat[ts])
+        le = mat_table[mat, 5:8]
+        if not (le > 0).any():
+            continue
+        two = mat_table[mat, 8]
+        # packed triangle of each source triangle
+        order = np.empty(tc, np.int64)
+        order[tri_src[ts:ts + tc]] = np.arange(ts, ts + tc)
+        for inst in np.nonzero(inst_prim == prim)[0]:
+            M = l2w[inst].reshape(4, 4).T  # row-major
+            for s in range(tc):
+                t = int(order[s])
+                w = []
+                for k in range(3):
+                    p = M @ np.append(pos[idx[t, k]], 1.0)
+                    w.append((p[:3] / p[3]).astype(np.float32))
+                recs.append(np.concatenate([w[0], w[1], w[2], le, [two, inst, t, 0]]).astype(np.float32))
+    if not recs:
+        return np.zeros((0, 16), np.float32)
+    return np.stack(recs)
+
+
+def represented_cdf(weights, bucket_bits=24, counts=False):
+    """buildRepresentedDistributionF32 (shared-samplers/src/representedDistribution.ts:
+    44-175): Hamilton apportionment of 2^bucket_bits buckets, f32 cdf (and the
+    bucket counts when `counts`)."""
+    w = np.asarray(weights, np.float64)
+    total = 1 << bucket_bits
+    pos_idx = [i for i in range(w.size) if w[i] > 0]
+    buckets = np.zeros(w.size, np.int64)
+    if pos_idx:
+        mx = max(w[i] for i in pos_idx)
+        s, comp = 0.0, 0.0
+        for i in pos_idx:
+            sc = w[i] / mx
+            corr = sc - comp
+            nxt = s + corr
+            comp = (nxt - s) - corr
+            s = nxt
+        remaining = total - len(pos_idx)
+        rema = []
+        allocated = len(pos_idx)
+        for i in pos_idx:
+            quota = 0.0 if remaining == 0 else ((w[i] / mx) / s) * remaining
+            extra = int(np.floor(quota))
+            buckets[i] = 1 + extra
+            allocated += extra
+            rema.append((i, quota - extra))
+        rema.sort(key=lambda r: (-r[1], r[0]))
+        deficit = total - allocated
+        cur = 0
+        while deficit > 0:
+            buckets[rema[cur % len(rema)][0]] += 1
+            cur += 1
+            deficit -= 1
+        if deficit < 0:
+            asc = sorted(rema, key=lambda r: (r[1], -r[0]))
+            excess, cur = -deficit, 0
+            while excess > 0:
+                i = asc[cur % len(asc)][0]
+                cur += 1
+                if buckets[i] <= 1:
+                    continue
+                buckets[i] -= 1
+                excess -= 1
+    cdf = np.zeros(w.size + 1, np.float32)
+    cum = np.cumsum(buckets)
+    cdf[1:] = (cum / float(total)).astype(np.float32)
+    return (cdf, buckets) if counts else cdf
+
+
+def build_constant_env(radiance, width=64, height=32):
+    """Equirect constant-radiance HDRI importance table
+    (pt-webgpu/src/scene/environmentPacking.ts:340-420): weights
+    luminance x texel solid angle, represented 24-bit CDF, texel pdf =
+    effective pmf / texel solid angle."""
+    lum = 0.2126 * radiance[0] + 0.7152 * radiance[1] + 0.0722 * radiance[2]
+    dphi = 2 * np.pi / width
+    y = np.repeat(np.arange(height), width)
+    omega = dphi * (np.cos(y / height * np.pi) - np.cos((y + 1) / height * np.pi))
+    cdf = represented_cdf(np.maximum(0.0, lum * omega))
+    pdf = ((cdf[1:].astype(np.float64) - cdf[:-1].astype(np.float64)) / omega).astype(np.float32)
+    return cdf, pdf
+
+
+def render(pack, materials, camera_inv_vp, width, height, *, frame_seed, frame_index, max_bounces,
+           ray_bias, env_radiance=None, env_size=(64, 32), tile=None, threads=None):
+    """One frame (1 spp) of the binary64 integrator mirror.  Returns
+    (radiance[h, w, 3] float64, signature[h, w] uint64)."""
+    ps = PackedScene(pack)
+    mt = material_table(materials)
+    em = np.ascontiguousarray(build_emitters(pack, mt))
+    normals = np.ascontiguousarray(pack["normals"], np.float32)
+    tri_mat = np.ascontiguousarray(pack["tri_material"], np.uint32)
+    env_present = env_radiance is not None and any(v > 0 for v in env_radiance)
+    if env_present:
+        cdf, pdf = build_constant_env(env_radiance, *env_size)
+        texel_pdf = float(pdf[0])
+        if not np.all(pdf == pdf[0]):
+            texel_pdf = float(pdf.mean())
+    else:
+        cdf, texel_pdf = np.zeros(2, np.float32), 0.0
+    d = RenderDesc()
+    d.scene = C.pointer(ps.struct)
+    d.normals = normals.ctypes.data_as(F32P)
+    d.tri_material = tri_mat.ctypes.data_as(U32P)
+    mt = np.ascontiguousarray(mt)
+    d.materials = mt.ctypes.data_as(F32P)
+    d.material_count = len(materials)
+    d.emitters = em.ctypes.data_as(F32P) if em.size else None
+    d.emitter_count = em.shape[0]
+    d.env_present = 1 if env_present else 0
+    d.env_radiance = (C.c_float * 3)(*(env_radiance or [0, 0, 0]))
+    d.env_w, d.env_h = (env_size if env_present else (0, 0))
+    d.env_cdf = cdf.ctypes.data_as(F32P)
+    d.env_texel_pdf = texel_pdf
+    d.inv_view_proj = (C.c_float * 16)(*np.asarray(camera_inv_vp, np.float32).ravel())
+    d.width, d.height = width, height
+    d.frame_seed = frame_seed
+    d.max_bounces = max_bounces
+    d.ray_bias = ray_bias
+    x0, y0, w, h = tile or (0, 0, width, height)
+    out = np.zeros((h, w, 3), np.float64)
+    sig = np.zeros((h, w), np.uint64)
+    lib().orc_render(C.byref(d), C.c_uint32(frame_index), x0, y0, w, h,
+                     out.ctypes.data_as(F64P), sig.ctypes.data_as(C.POINTER(C.c_uint64)),
+                     C.c_uint32(threads or min(16, os.cpu_count() or 1)))
+    return out, sig

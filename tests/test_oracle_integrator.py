@@ -1,0 +1,98 @@
+"""CPU pins of the binary64 integrator mirror (oracle/integrator.c).
+
+  * represented distributions: the reference's own vectors
+    (shared-samplers/src/__tests__/representedDistribution.test.ts:10-55);
+  * estimator: a plane under a constant sky converges to L times the BSDF's
+    directional albedo, the albedo from a dense quadrature that shares no code
+    with any sampling routine (a dropped pdf term, MIS weight or Fresnel factor
+    moves the mean by far more than the tolerance);
+  * a directly viewed emitter returns its radiance exactly;
+  * determinism: the same seed gives the same image and path signatures.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from vitrum_amd import workloads as W
+
+from render_cases import SEED, cpu_frames, directional_albedo_at_normal, plane_under_sky
+
+
+def test_represented_distribution_reference_vectors():
+    cdf, counts = O.represented_cdf([2.0 ** -30, 2.0 ** -30, 1.0], counts=True)
+    assert counts.tolist() == [1, 1, (1 << 24) - 2]
+    assert cdf[0] == 0 and cdf[-1] == 1
+    cdf, counts = O.represented_cdf([1, 1, 1], bucket_bits=3, counts=True)
+    assert counts.tolist() == [3, 3, 2]
+    assert cdf.tolist() == [0, 3 / 8, 6 / 8, 1]
+    cdf, counts = O.represented_cdf([0, 0, 0], bucket_bits=3, counts=True)
+    assert counts.tolist() == [0, 0, 0] and cdf.tolist() == [0, 0, 0, 0]
+
+
+def test_represented_distribution_apportionment_is_nearest():
+    rng = np.random.default_rng(5)
+    w = rng.random(2048) ** 4
+    cdf, counts = O.represented_cdf(w, counts=True)
+    assert counts.sum() == 1 << 24 and (counts >= 1).all()
+    ideal = 1 + w / w.sum() * ((1 << 24) - w.size)
+    assert np.abs(counts - ideal).max() < 1.0
+    assert (np.diff(cdf) >= 0).all()
+
+
+def test_constant_env_pdf_integrates_to_one():
+    cdf, pdf = O.build_constant_env([1.0, 0.8, 0.6], 64, 32)
+    y = np.repeat(np.arange(32), 64)
+    omega = 2 * np.pi / 64 * (np.cos(y / 32 * np.pi) - np.cos((y + 1) / 32 * np.pi))
+    assert float(np.sum(pdf.astype(np.float64) * omega)) == pytest.approx(1.0, abs=1e-6)
+    # constant radiance: pdf is uniform over the sphere to one bucket per texel
+    quantum = 1.0 / ((1 << 24) * omega)
+    assert (np.abs(pdf - 1 / (4 * np.pi)) <= quantum * 1.0001).all()
+
+
+@pytest.mark.parametrize("roughness,metallic", [(1.0, 0.0), (0.3, 0.0), (0.5, 1.0)])
+def test_plane_under_constant_sky_matches_quadrature(roughness, metallic):
+    L = np.array([1.0, 0.8, 0.6])
+    sc = plane_under_sky(0.5, L.tolist(), roughness, metallic)
+    bias, _ = W.ray_bias(sc["scene_radius"])
+    imgs, _ = cpu_frames(sc, 32, 32, float(bias))
+    mean = imgs.mean(axis=(0, 1, 2))
+    expect = L * directional_albedo_at_normal(0.5, roughness, metallic)
+    np.testing.assert_allclose(mean, expect, rtol=3e-3)
+
+
+def test_quadrature_energy_bounds():
+    """The quadrature itself: with a black base only the 4% dielectric specular
+    and multiscatter lobes reflect (small, positive albedo); with a white base
+    the albedo stays within the energy bound of the diffuse + specular sum."""
+    dark = directional_albedo_at_normal(0.0, 1.0, 0.0)
+    white = directional_albedo_at_normal(1.0, 1.0, 0.0)
+    assert 0.0 < dark < 0.1
+    assert 0.9 < white < 1.04
+
+
+def emitter_card():
+    pos = np.array([-5, -5, 0, 5, -5, 0, 5, 5, 0, -5, 5, 0], np.float32)
+    nrm = np.tile([0, 0, 1], 4).astype(np.float32)
+    idx = np.array([0, 1, 2, 0, 2, 3], np.uint32)
+    prim = {"kind": "mesh", "positions": pos, "normals": nrm, "indices": idx,
+            "material": {"baseColor": [0, 0, 0], "roughness": 1, "metallic": 0,
+                         "emissive": [1.0, 0.5, 0.25], "emissiveIntensity": 2.0}}
+    cam = W.look_at_camera(eye=(0, 0, 3), target=(0, 0, 0), up=(0, 1, 0), vfov_deg=30)
+    return {"primitives": [prim], "camera": cam, "scene_radius": 8.0, "sky": [0, 0, 0]}
+
+
+def test_directly_viewed_emitter_is_exact():
+    sc = emitter_card()
+    imgs, _ = cpu_frames(sc, 16, 2, float(W.ray_bias(8.0)[0]), max_bounces=1)
+    np.testing.assert_allclose(imgs.reshape(-1, 3), np.tile([2.0, 1.0, 0.5], (512, 1)), rtol=1e-12)
+
+
+def test_integrator_is_deterministic():
+    sc = W.cornell_scene()
+    bias = float(W.ray_bias(sc["scene_radius"])[0])
+    a, sa = cpu_frames(sc, 24, 1, bias)
+    b, sb = cpu_frames(sc, 24, 1, bias)
+    assert np.array_equal(a, b) and np.array_equal(sa, sb)
+    c, sc_ = cpu_frames(sc, 24, 2, bias)
+    assert not np.array_equal(c[0], c[1])  # frame index reseeds the stream
+    assert c[0].mean() > 0.05

@@ -33,6 +33,7 @@ struct vt_engine {
   hipStream_t stream = nullptr;
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;
   bool trace_timed = false;
+  bool frame_timed = false;
   uint32_t cu_count = 0;
   uint32_t trace_blocks_per_cu = 1;
   uint32_t trace_grid = 1;

@@ -1,0 +1,455 @@
+// render.cpp -- host side of the wavefront path tracer: per-scene shading
+// data (materials, mesh-area emitters, environment importance table),
+// per-resolution path state, and the frame schedule behind vt_render_frame.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "engine_internal.h"
+#include "render.h"
+
+namespace {
+
+vt_status rerr(vt_engine* e, vt_status code, const std::string& msg) {
+  e->last_error = msg;
+  return code;
+}
+
+#define HIP_CHECK(e, expr)                                                                 \
+  do {                                                                                     \
+    hipError_t _r = (expr);                                                                \
+    if (_r != hipSuccess) return rerr((e), VT_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_r)); \
+  } while (0)
+
+// buildRepresentedDistributionF32 (packages/shared-samplers/src/
+// representedDistribution.ts:44-175): Hamilton apportionment of 2^24
+// buckets over the positive weights, then an f32 cumulative table.
+std::vector<float> represented_cdf(const std::vector<double>& w) {
+  const int64_t total = int64_t(1) << 24;
+  std::vector<int64_t> buckets(w.size(), 0);
+  std::vector<size_t> pos;
+  for (size_t i = 0; i < w.size(); ++i)
+    if (w[i] > 0) pos.push_back(i);
+  if (!pos.empty()) {
+    double mx = 0;
+    for (size_t i : pos) mx = std::max(mx, w[i]);
+    double sum = 0, comp = 0;
+    for (size_t i : pos) {
+      const double corr = w[i] / mx - comp;
+      const double next = sum + corr;
+      comp = (next - sum) - corr;
+      sum = next;
+    }
+    const int64_t remaining = total - (int64_t)pos.size();
+    int64_t allocated = (int64_t)pos.size();
+    std::vector<std::pair<size_t, double>> rem;
+    for (size_t i : pos) {
+      const double quota = remaining == 0 ? 0.0 : ((w[i] / mx) / sum) * (double)remaining;
+      const int64_t extra = (int64_t)std::floor(quota);
+      buckets[i] = 1 + extra;
+      allocated += extra;
+      rem.push_back({i, quota - (double)extra});
+    }
+    std::stable_sort(rem.begin(), rem.end(), [](const auto& a, const auto& b) {
+      return a.second != b.second ? a.second > b.second : a.first < b.first;
+    });
+    int64_t deficit = total - allocated;
+    for (size_t c = 0; deficit > 0; ++c, --deficit) buckets[rem[c % rem.size()].first] += 1;
+    if (deficit < 0) {
+      auto asc = rem;
+      std::stable_sort(asc.begin(), asc.end(), [](const auto& a, const auto& b) {
+        return a.second != b.second ? a.second < b.second : a.first > b.first;
+      });
+      int64_t excess = -deficit;
+      for (size_t c = 0; excess > 0; ++c) {
+        const size_t i = asc[c % asc.size()].first;
+        if (buckets[i] <= 1) continue;
+        buckets[i] -= 1;
+        --excess;
+      }
+    }
+  }
+  std::vector<float> cdf(w.size() + 1, 0.0f);
+  int64_t cum = 0;
+  for (size_t i = 0; i < w.size(); ++i) {
+    cum += buckets[i];
+    cdf[i + 1] = (float)((double)cum / (double)total);
+  }
+  return cdf;
+}
+
+}  // namespace
+
+struct RenderState {
+  // Per-scene shading data.
+  DeviceBuffer materials, emitters, instances, env_cdf, env_pdf, tri_source;
+  uint32_t emitter_count = 0;
+  uint32_t env_present = 0, env_w = 64, env_h = 32;
+  float env_radiance[3] = {0, 0, 0};
+  float ray_bias = 0, tmin = 0;
+  // Per-resolution path state.
+  uint32_t width = 0, height = 0;
+  DeviceBuffer rng, throughput, conn, radiance, prev_pdf, flags, signature, count;
+  DeviceBuffer queue[2], rays[2], hits, hit_pos;
+  DeviceBuffer shadow_path, shadow_rays, shadow_contrib, shadow_occ, accum;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  void release_paths() {
+    for (DeviceBuffer* b : {&rng, &throughput, &conn, &radiance, &prev_pdf, &flags, &signature, &count,
+                            &queue[0], &queue[1], &rays[0], &rays[1], &hits, &hit_pos, &shadow_path,
+                            &shadow_rays, &shadow_contrib, &shadow_occ, &accum})
+      b->release();
+    width = height = 0;
+  }
+  void release_scene() {
+    for (DeviceBuffer* b : {&materials, &emitters, &instances, &env_cdf, &env_pdf, &tri_source}) b->release();
+  }
+};
+
+void engine_release_render(vt_engine* e) {
+  if (!e->render) return;
+  e->render->release_paths();
+  e->render->release_scene();
+  if (e->render->ev0) (void)hipEventDestroy(e->render->ev0);
+  if (e->render->ev1) (void)hipEventDestroy(e->render->ev1);
+  delete e->render;
+  e->render = nullptr;
+}
+
+// ptWebgpuRayOriginBias (packages/pt-webgpu/src/scene/sceneScalePolicy.ts:83-118).
+static float ray_origin_bias(double radius, const double* center) {
+  double mag = std::max(std::fabs((double)(float)center[0]),
+                        std::max(std::fabs((double)(float)center[1]), std::fabs((double)(float)center[2])));
+  double ulp = (mag == 0 || mag < 1.1754943508222875e-38) ? std::ldexp(1.0, -149)
+                                                          : std::ldexp(1.0, (int)std::floor(std::log2(mag)) - 23);
+  return (float)std::max(radius * (1e-3 / std::sqrt(3.0)), 2.0 * ulp);
+}
+
+vt_status engine_on_scene_changed(vt_engine* e) {
+  e->samples_accumulated = 0;
+  if (!e->render) {
+    e->render = new RenderState();
+    HIP_CHECK(e, hipEventCreate(&e->render->ev0));
+    HIP_CHECK(e, hipEventCreate(&e->render->ev1));
+  }
+  RenderState& R = *e->render;
+  const vt::Pack& P = *e->pack;
+  // Materials: base rgb, roughness, metallic, Le = emissive * intensity, twoSided.
+  std::vector<float> mats(std::max<size_t>(1, e->materials.size()) * vtr::kMatFloats, 0.0f);
+  for (size_t i = 0; i < e->materials.size(); ++i) {
+    const vt_material& m = e->materials[i];
+    float* q = &mats[i * vtr::kMatFloats];
+    for (int k = 0; k < 3; ++k) q[k] = m.base_color[k], q[5 + k] = m.emissive[k] * m.emissive_intensity;
+    q[3] = m.roughness;
+    q[4] = m.metallic;
+    q[8] = m.double_sided ? 1.0f : 0.0f;
+  }
+  if (e->materials.empty()) mats[0] = mats[1] = mats[2] = 0.8f, mats[3] = 1.0f;
+  // Mesh-area emitters, one per emissive triangle (scene/emitterPacking.ts:
+  // 748-820): primitives in scene order, each primitive's instances in packed
+  // order, triangles in source order; world vertices are the binary64
+  // local-to-world image of the local vertices, rounded to f32.
+  const uint32_t n_inst = (uint32_t)P.tlas_blas_roots.size();
+  std::vector<float> inst(std::max<uint32_t>(1, n_inst) * vtr::kInstFloats, 0.0f);
+  for (uint32_t i = 0; i < n_inst; ++i) {
+    float* q = &inst[(size_t)i * vtr::kInstFloats];
+    std::memcpy(q, &P.tlas_w2l[16 * (size_t)i], 64);
+    std::memcpy(q + 16, &P.tlas_l2w[16 * (size_t)i], 64);
+    const float* m = &P.tlas_l2w[16 * (size_t)i];
+    const double det = (double)m[0] * ((double)m[5] * m[10] - (double)m[9] * m[6]) -
+                       (double)m[4] * ((double)m[1] * m[10] - (double)m[9] * m[2]) +
+                       (double)m[8] * ((double)m[1] * m[6] - (double)m[5] * m[2]);
+    q[32] = det < 0 ? 1.0f : 0.0f;
+    uint32_t none = 0xFFFFFFFFu;
+    std::memcpy(q + 33, &none, 4);
+  }
+  std::vector<float> em;
+  for (const vt::Binding& b : P.bindings) {
+    const uint32_t mid = P.tri_material[b.tri_start];
+    if (mid * vtr::kMatFloats >= mats.size()) continue;
+    const float* mq = &mats[(size_t)mid * vtr::kMatFloats];
+    if (!(mq[5] > 0 || mq[6] > 0 || mq[7] > 0)) continue;
+    std::vector<uint32_t> packed_of_source(b.tri_count);
+    for (uint32_t t = 0; t < b.tri_count; ++t) packed_of_source[P.tri_source[b.tri_start + t]] = b.tri_start + t;
+    for (uint32_t ii = 0; ii < n_inst; ++ii) {
+      if (P.instance_primitive[ii] != b.primitive) continue;
+      const uint32_t base = (uint32_t)(em.size() / vtr::kEmitFloats);
+      std::memcpy(&inst[(size_t)ii * vtr::kInstFloats + 33], &base, 4);
+      const float* m = &P.tlas_l2w[16 * (size_t)ii];
+      for (uint32_t s = 0; s < b.tri_count; ++s) {
+        const uint32_t t = packed_of_source[s];
+        float rec[vtr::kEmitFloats] = {0};
+        for (int v = 0; v < 3; ++v) {
+          const float* p = &P.positions[4 * (size_t)P.indices[4 * (size_t)t + v]];
+          double q[4];
+          for (int k = 0; k < 4; ++k)
+            q[k] = (double)m[k] * p[0] + (double)m[4 + k] * p[1] + (double)m[8 + k] * p[2] + (double)m[12 + k];
+          for (int k = 0; k < 3; ++k) rec[3 * v + k] = (float)(q[k] / q[3]);
+        }
+        rec[9] = mq[5], rec[10] = mq[6], rec[11] = mq[7];
+        rec[12] = mq[8];
+        rec[13] = (float)ii;
+        rec[14] = (float)t;
+        em.insert(em.end(), rec, rec + vtr::kEmitFloats);
+      }
+    }
+  }
+  R.emitter_count = (uint32_t)(em.size() / vtr::kEmitFloats);
+  if (em.empty()) em.assign(vtr::kEmitFloats, 0.0f);
+  // Environment: constant-radiance equirect map, luminance x texel solid
+  // angle importance (scene/environmentPacking.ts:340-420).
+  R.env_present = e->scene_desc_env.kind == 1 &&
+                  (e->scene_desc_env.radiance[0] > 0 || e->scene_desc_env.radiance[1] > 0 ||
+                   e->scene_desc_env.radiance[2] > 0);
+  for (int k = 0; k < 3; ++k) R.env_radiance[k] = R.env_present ? e->scene_desc_env.radiance[k] : 0.0f;
+  const uint32_t W = R.env_w, H = R.env_h;
+  std::vector<double> weights((size_t)W * H), omega((size_t)W * H);
+  const double lum = 0.2126 * R.env_radiance[0] + 0.7152 * R.env_radiance[1] + 0.0722 * R.env_radiance[2];
+  const double dphi = 2.0 * M_PI / W;
+  for (uint32_t i = 0; i < W * H; ++i) {
+    const uint32_t y = i / W;
+    omega[i] = dphi * (std::cos((double)y / H * M_PI) - std::cos((double)(y + 1) / H * M_PI));
+    weights[i] = std::max(0.0, lum * omega[i]);
+  }
+  std::vector<float> cdf = represented_cdf(weights), pdf((size_t)W * H);
+  for (uint32_t i = 0; i < W * H; ++i) pdf[i] = (float)(((double)cdf[i + 1] - (double)cdf[i]) / omega[i]);
+  // Ray-origin bias from the scene bounds (or the caller's radius).
+  double center[3], radius = e->scene_radius;
+  for (int k = 0; k < 3; ++k) center[k] = 0.5 * (P.world_min[k] + P.world_max[k]);
+  if (!(radius > 0)) {
+    double d2 = 0;
+    for (int k = 0; k < 3; ++k) d2 += (P.world_max[k] - P.world_min[k]) * (P.world_max[k] - P.world_min[k]);
+    radius = 0.5 * std::sqrt(d2);
+    if (!(radius > 0)) radius = std::sqrt(3.0);
+  }
+  if (!std::isfinite(center[0])) center[0] = center[1] = center[2] = 0;
+  R.ray_bias = ray_origin_bias(radius, center);
+  R.tmin = std::max(R.ray_bias * 0.1f, 1.175494351e-38f);
+  vt_status s = VT_OK;
+  s = s ? s : engine_upload(e, R.materials, mats.data(), mats.size() * 4);
+  s = s ? s : engine_upload(e, R.emitters, em.data(), em.size() * 4);
+  s = s ? s : engine_upload(e, R.instances, inst.data(), inst.size() * 4);
+  s = s ? s : engine_upload(e, R.env_cdf, cdf.data(), cdf.size() * 4);
+  s = s ? s : engine_upload(e, R.env_pdf, pdf.data(), pdf.size() * 4);
+  s = s ? s : engine_upload(e, R.tri_source, P.tri_source.data(), P.tri_source.size() * 4);
+  if (s) return s;
+  HIP_CHECK(e, hipStreamSynchronize(e->stream));
+  return VT_OK;
+}
+
+static vt_status ensure_paths(vt_engine* e, uint32_t w, uint32_t h) {
+  RenderState& R = *e->render;
+  if (R.width == w && R.height == h) return VT_OK;
+  R.release_paths();
+  const size_t n = (size_t)w * h;
+  vt_status s = VT_OK;
+  auto need = [&](DeviceBuffer& b, size_t bytes) {
+    if (!s) s = b.ensure(e, std::max<size_t>(bytes, 64));
+  };
+  need(R.rng, n * 4);
+  need(R.throughput, n * 12);
+  need(R.conn, n * 12);
+  need(R.radiance, n * 12);
+  need(R.prev_pdf, n * 4);
+  need(R.flags, n * 4);
+  need(R.signature, n * 8);
+  need(R.count, 64);
+  need(R.queue[0], n * 4);
+  need(R.queue[1], n * 4);
+  need(R.rays[0], n * 32);
+  need(R.rays[1], n * 32);
+  need(R.hits, n * 20);
+  need(R.hit_pos, n * 12);
+  need(R.shadow_path, n * 4);
+  need(R.shadow_rays, n * 32);
+  need(R.shadow_contrib, n * 12);
+  need(R.shadow_occ, n * 4);
+  need(R.accum, n * 16);
+  if (s) {
+    R.release_paths();
+    return s;
+  }
+  R.width = w;
+  R.height = h;
+  HIP_CHECK(e, vtr::launch_zero_f32((float*)R.accum.ptr, (uint32_t)(n * 4), e->stream));
+  e->samples_accumulated = 0;
+  return VT_OK;
+}
+
+static vtk::TraceParams trace_params(vt_engine* e) {
+  vtk::TraceParams p{};
+  const vt::DeviceLayout& L = *e->layout;
+  p.scene.blas_nodes = (const float4*)e->scene.blas.ptr;
+  p.scene.tlas_nodes = (const float4*)e->scene.tlas.ptr;
+  p.scene.tris = (const float4*)e->scene.tris.ptr;
+  p.scene.instances = (const float4*)e->scene.instances.ptr;
+  p.scene.tlas_root = L.tlas_root;
+  p.scene.tlas_node_count = L.tlas_node_count;
+  p.scene.tri_count = L.tri_count;
+  p.scene.inst_count = L.inst_count;
+  p.counter = (uint32_t*)e->counter.ptr;
+  p.error = (uint32_t*)e->error_word.ptr;
+  p.spill = (uint2*)e->spill.ptr;
+  p.refill_min = e->refill_min;
+  p.steps = e->trace_steps;
+  return p;
+}
+
+extern "C" {
+
+vt_status vt_render_frame(vt_engine* e, const vt_frame* fr) {
+  if (!e) return VT_ERR_INVALID_ARGUMENT;
+  if (!fr) return rerr(e, VT_ERR_INVALID_ARGUMENT, "frame is NULL");
+  if (!e->has_scene || !e->render) return rerr(e, VT_ERR_STATE, "no scene is set");
+  if (fr->width == 0 || fr->height == 0 || (uint64_t)fr->width * fr->height > (1u << 28))
+    return rerr(e, VT_ERR_RANGE, "viewport must be non-empty and at most 2^28 pixels");
+  for (int i = 0; i < 16; ++i)
+    if (!std::isfinite(fr->inv_view_proj[i])) return rerr(e, VT_ERR_RANGE, "inverse view-projection must be finite");
+  HIP_CHECK(e, hipSetDevice(e->device));
+  if (vt_status s = ensure_paths(e, fr->width, fr->height)) return s;
+  if (e->opts.max_samples_per_pixel && e->samples_accumulated >= e->opts.max_samples_per_pixel) return VT_OK;
+  if (fr->samples_target && e->samples_accumulated >= fr->samples_target) return VT_OK;
+  RenderState& R = *e->render;
+  vtr::FrameParams f{};
+  std::memcpy(f.inv_vp, fr->inv_view_proj, 64);
+  f.width = fr->width;
+  f.height = fr->height;
+  f.frame_seed = fr->frame_seed;
+  f.frame_index = fr->frame_index;
+  const uint32_t mb = fr->max_bounces ? fr->max_bounces : e->opts.max_bounces;
+  f.bounce_limit = std::max(1u, std::min(mb, 8u));
+  f.ray_bias = R.ray_bias;
+  f.tmin = R.tmin;
+  vtr::ShadeScene sc{};
+  sc.positions = (const float4*)e->scene.positions.ptr;
+  sc.normals = (const float4*)e->scene.normals.ptr;
+  sc.indices = (const uint4*)e->scene.indices.ptr;
+  sc.tri_material = (const uint32_t*)e->scene.tri_material.ptr;
+  sc.tri_source = (const uint32_t*)R.tri_source.ptr;
+  sc.materials = (const float*)R.materials.ptr;
+  sc.emitters = (const float*)R.emitters.ptr;
+  sc.instances = (const float*)R.instances.ptr;
+  sc.env_cdf = (const float*)R.env_cdf.ptr;
+  sc.env_pdf = (const float*)R.env_pdf.ptr;
+  sc.emitter_count = R.emitter_count;
+  sc.env_present = R.env_present;
+  sc.env_w = R.env_w;
+  sc.env_h = R.env_h;
+  for (int k = 0; k < 3; ++k) sc.env_radiance[k] = R.env_radiance[k];
+  vtr::PathState ps{};
+  ps.rng = (uint32_t*)R.rng.ptr;
+  ps.throughput = (float*)R.throughput.ptr;
+  ps.conn = (float*)R.conn.ptr;
+  ps.radiance = (float*)R.radiance.ptr;
+  ps.prev_pdf = (float*)R.prev_pdf.ptr;
+  ps.flags = (uint32_t*)R.flags.ptr;
+  ps.signature = (uint64_t*)R.signature.ptr;
+  ps.count = (uint32_t*)R.count.ptr;
+  ps.hits = (uint32_t*)R.hits.ptr;
+  ps.hit_pos = (float*)R.hit_pos.ptr;
+  ps.shadow_path = (uint32_t*)R.shadow_path.ptr;
+  ps.shadow_rays = (float4*)R.shadow_rays.ptr;
+  ps.shadow_contrib = (float*)R.shadow_contrib.ptr;
+  ps.shadow_occ = (uint32_t*)R.shadow_occ.ptr;
+  ps.accum = (float*)R.accum.ptr;
+  const uint32_t cap = fr->width * fr->height;
+  HIP_CHECK(e, hipEventRecord(R.ev0, e->stream));
+  HIP_CHECK(e, vtk::launch_zero((uint32_t*)R.count.ptr, 16, e->stream));
+  int cur = 0;
+  ps.queue = (uint32_t*)R.queue[cur].ptr;
+  ps.rays = (float4*)R.rays[cur].ptr;
+  HIP_CHECK(e, vtr::launch_camera(f, ps, e->stream));
+  vtk::TraceParams tp = trace_params(e);
+  for (uint32_t v = 0; v <= f.bounce_limit; ++v) {
+    ps.queue = (uint32_t*)R.queue[cur].ptr;
+    ps.rays = (float4*)R.rays[cur].ptr;
+    ps.next_queue = (uint32_t*)R.queue[cur ^ 1].ptr;
+    ps.next_rays = (float4*)R.rays[cur ^ 1].ptr;
+    vtk::TraceParams cp = tp;
+    cp.rays = ps.rays;
+    cp.hits = ps.hits;
+    cp.hit_pos = ps.hit_pos;
+    cp.n_ptr = ps.count;
+    HIP_CHECK(e, vtk::launch_trace(cp, false, false, e->trace_grid, e->stream));
+    HIP_CHECK(e, vtr::launch_shade(f, sc, ps, v, cap, e->stream));
+    if (v < f.bounce_limit) {
+      vtk::TraceParams sp = tp;
+      sp.rays = ps.shadow_rays;
+      sp.occluded = ps.shadow_occ;
+      sp.n_ptr = ps.count + 2;
+      HIP_CHECK(e, vtk::launch_trace(sp, true, false, e->trace_grid, e->stream));
+      HIP_CHECK(e, vtr::launch_shadow_resolve(ps, cap, e->stream));
+    }
+    HIP_CHECK(e, vtr::launch_reset_counts(ps, e->stream));
+    cur ^= 1;
+  }
+  HIP_CHECK(e, vtr::launch_accumulate(f, ps, e->stream));
+  HIP_CHECK(e, hipEventRecord(R.ev1, e->stream));
+  e->samples_accumulated += 1;
+  e->frame_timed = true;
+  return VT_OK;
+}
+
+vt_status vt_capture(vt_engine* e, float* rgba, uint32_t width, uint32_t height) {
+  if (!e) return VT_ERR_INVALID_ARGUMENT;
+  if (!rgba) return rerr(e, VT_ERR_INVALID_ARGUMENT, "rgba is NULL");
+  if (!e->render || e->render->width != width || e->render->height != height)
+    return rerr(e, VT_ERR_STATE, "no frame of this size has been rendered");
+  HIP_CHECK(e, hipMemcpyAsync(rgba, e->render->accum.ptr, (size_t)width * height * 16,
+                              hipMemcpyDeviceToHost, e->stream));
+  HIP_CHECK(e, hipStreamSynchronize(e->stream));
+  const float inv = e->samples_accumulated ? 1.0f / (float)e->samples_accumulated : 0.0f;
+  for (size_t i = 0; i < (size_t)width * height; ++i) {
+    rgba[4 * i] *= inv;
+    rgba[4 * i + 1] *= inv;
+    rgba[4 * i + 2] *= inv;
+    rgba[4 * i + 3] = 1.0f;
+  }
+  uint32_t err = 0;
+  HIP_CHECK(e, hipMemcpy(&err, e->error_word.ptr, 4, hipMemcpyDeviceToHost));
+  if (err) return rerr(e, VT_ERR_HIP, "traversal stack capacity exceeded during rendering");
+  return VT_OK;
+}
+
+vt_status vt_capture_signatures(vt_engine* e, uint64_t* sig, uint32_t width, uint32_t height) {
+  if (!e || !sig) return VT_ERR_INVALID_ARGUMENT;
+  if (!e->render || e->render->width != width || e->render->height != height)
+    return rerr(e, VT_ERR_STATE, "no frame of this size has been rendered");
+  HIP_CHECK(e, hipMemcpyAsync(sig, e->render->signature.ptr, (size_t)width * height * 8,
+                              hipMemcpyDeviceToHost, e->stream));
+  HIP_CHECK(e, hipStreamSynchronize(e->stream));
+  return VT_OK;
+}
+
+vt_status vt_reset(vt_engine* e) {
+  if (!e) return VT_ERR_INVALID_ARGUMENT;
+  e->samples_accumulated = 0;
+  if (e->render && e->render->width) {
+    HIP_CHECK(e, vtr::launch_zero_f32((float*)e->render->accum.ptr,
+                                      e->render->width * e->render->height * 4, e->stream));
+    HIP_CHECK(e, hipStreamSynchronize(e->stream));
+  }
+  return VT_OK;
+}
+
+vt_status vt_frame_time(vt_engine* e, float* ms) {
+  if (!e || !ms) return VT_ERR_INVALID_ARGUMENT;
+  *ms = 0;
+  if (!e->render || !e->frame_timed) return VT_OK;
+  HIP_CHECK(e, hipEventSynchronize(e->render->ev1));
+  HIP_CHECK(e, hipEventElapsedTime(ms, e->render->ev0, e->render->ev1));
+  e->last_frame_ms = *ms;
+  return VT_OK;
+}
+
+vt_status vt_render_info(vt_engine* e, uint32_t* emitter_count, float* ray_bias) {
+  if (!e || !e->render) return VT_ERR_STATE;
+  if (emitter_count) *emitter_count = e->render->emitter_count;
+  if (ray_bias) *ray_bias = e->render->ray_bias;
+  return VT_OK;
+}
+
+}  // extern "C"

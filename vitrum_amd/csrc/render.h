@@ -1,0 +1,74 @@
+// render.h -- device data and launchers of the wavefront path tracer (render.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "traverse.h"
+
+namespace vtr {
+
+constexpr uint32_t kMatFloats = 12;   // base rgb, roughness, metallic, Le rgb, twoSided, pad x3
+constexpr uint32_t kEmitFloats = 16;  // a, b, c, Le, twoSided, packed instance, packed tri, pad
+constexpr uint32_t kInstFloats = 36;  // W2L (16), L2W (16), mirrored, emitter base, pad x2
+
+struct ShadeScene {
+  const float4* positions;      // packed vertices (vec4)
+  const float4* normals;
+  const uint4* indices;         // packed triangles (vec4u)
+  const uint32_t* tri_material;
+  const uint32_t* tri_source;   // source triangle of each packed triangle
+  const float* materials;       // kMatFloats per material
+  const float* emitters;        // kEmitFloats per emitter, light-selection order
+  const float* instances;       // kInstFloats per packed instance
+  const float* env_cdf;         // env_w * env_h + 1
+  const float* env_pdf;         // per texel
+  uint32_t emitter_count;
+  uint32_t env_present, env_w, env_h;
+  float env_radiance[3];
+};
+
+struct FrameParams {
+  float inv_vp[16];
+  uint32_t width, height;
+  uint32_t frame_seed, frame_index;
+  uint32_t bounce_limit;
+  float ray_bias, tmin;
+};
+
+// Structure-of-arrays path state, one path per pixel.
+struct PathState {
+  uint32_t* rng;
+  float* throughput;   // 3 per path: throughput entering the current vertex
+  float* conn;         // 3 per path: throughput carried by the BSDF sample (pre-RR)
+  float* radiance;     // 3 per path
+  float* prev_pdf;     // > 0 when the previous sample takes part in area/env MIS
+  uint32_t* flags;     // bit 0: survives RR; bits 8-15: next vertex index
+  uint64_t* signature; // hash of the path's discrete decisions
+  // Extension queue (rays traced with closest hit).
+  uint32_t* queue;      // path index per queued ray
+  float4* rays;         // 2 per queued ray
+  uint32_t* hits;       // 5 per queued ray
+  float* hit_pos;       // 3 per queued ray
+  uint32_t* count;      // [0] extension rays, [1] next extension rays, [2] shadow rays
+  uint32_t* next_queue;
+  float4* next_rays;
+  // Shadow queue (any-hit).
+  uint32_t* shadow_path;
+  float4* shadow_rays;
+  float* shadow_contrib;  // 3 per shadow ray
+  uint32_t* shadow_occ;
+  // Accumulation.
+  float* accum;           // 4 per pixel (rgb, unused)
+};
+
+hipError_t launch_camera(const FrameParams& f, const PathState& s, hipStream_t st);
+hipError_t launch_shade(const FrameParams& f, const ShadeScene& sc, const PathState& s,
+                        uint32_t vertex, uint32_t capacity, hipStream_t st);
+hipError_t launch_shadow_resolve(const PathState& s, uint32_t capacity, hipStream_t st);
+hipError_t launch_accumulate(const FrameParams& f, const PathState& s, hipStream_t st);
+hipError_t launch_reset_counts(const PathState& s, hipStream_t st);
+hipError_t launch_zero_f32(float* p, uint32_t n, hipStream_t st);
+
+}  // namespace vtr

@@ -1,0 +1,686 @@
+// render.hip -- wavefront unidirectional path tracer for gfx950.
+//
+// The estimator is the one of pt-webgpu's `main` megakernel
+// (packages/pt-webgpu/src/wgsl/pathTrace/kernel.wgsl.ts:962-2032) for opaque
+// Lambert + GGX materials, mesh-area emitters and an environment map, with
+// the per-path RNG consumed in the same order.  The megakernel is split into
+// stages that run over compacted queues:
+//
+//   camera  -> for each vertex: closest-hit (traverse.hip) -> shade -> any-hit
+//              shadow rays (traverse.hip) -> shadow resolve -> ... -> accumulate
+//
+// The BSDF->light MIS connections (connect.wgsl.ts:478-700) are evaluated
+// where the wavefront naturally has the data: the BSDF-sampled ray *is* the
+// next extension ray (same origin, direction and tMin), so its closest hit
+// decides which emitter the connection sees and whether the environment is
+// visible.  The connection is weighted with the throughput before Russian
+// roulette, and a path killed by roulette still has its last ray traced for
+// the connection, exactly as the megakernel adds the connection before the
+// roulette draw.
+//
+// Queues are appended with wave-aggregated atomics: a ballot of the lanes
+// that emit, one atomicAdd by the first of them, and mbcnt ranks.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "render.h"
+
+namespace vtr {
+
+constexpr float kPi = 3.14159265358979f;
+constexpr float kInvPi = 0.31830988618f;
+constexpr float kInv2Pi = 0.15915494309189535f;
+constexpr uint32_t kMiss = 0xFFFFFFFFu;
+
+__device__ __forceinline__ float safe_max() { return __uint_as_float(0x7F7FFFFDu); }
+__device__ __forceinline__ float f32_max() { return __uint_as_float(0x7F7FFFFFu); }
+
+struct F3 {
+  float x, y, z;
+};
+__device__ __forceinline__ F3 f3(float x, float y, float z) { return F3{x, y, z}; }
+__device__ __forceinline__ F3 operator+(F3 a, F3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ F3 operator-(F3 a, F3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ F3 operator*(F3 a, F3 b) { return f3(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ F3 operator*(F3 a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ F3 operator*(float s, F3 a) { return f3(s * a.x, s * a.y, s * a.z); }
+__device__ __forceinline__ float dot(F3 a, F3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ F3 cross(F3 a, F3 b) {
+  return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+__device__ __forceinline__ F3 load3(const float* p) { return f3(p[0], p[1], p[2]); }
+__device__ __forceinline__ void store3(float* p, F3 v) {
+  p[0] = v.x;
+  p[1] = v.y;
+  p[2] = v.z;
+}
+
+// safe_normalize (pt-webgpu/src/wgsl/common.wgsl.ts)
+__device__ F3 safe_normalize(F3 v) {
+  const float s = fmaxf(fabsf(v.x), fmaxf(fabsf(v.y), fabsf(v.z)));
+  if (!(s > 0.0f) || s > safe_max()) return f3(0, 1, 0);
+  const F3 q = f3(v.x / s, v.y / s, v.z / s);
+  const float l = __builtin_sqrtf(dot(q, q));
+  return f3(q.x / l, q.y / l, q.z / l);
+}
+
+// safe_length (common.wgsl.ts)
+__device__ float safe_length(F3 v) {
+  const float s = fmaxf(fabsf(v.x), fmaxf(fabsf(v.y), fabsf(v.z)));
+  if (!(s > 0.0f) || s > f32_max()) return 0.0f;
+  const F3 q = f3(v.x / s, v.y / s, v.z / s);
+  const float r = s * __builtin_sqrtf(dot(q, q));
+  return (!(r > 0.0f) || r > f32_max()) ? 0.0f : r;
+}
+
+// ---- RNG (shared-samplers/src/wgsl/pcg.wgsl.ts; common.wgsl.ts) ---------
+
+__device__ __forceinline__ uint32_t pcg_init(uint32_t px, uint32_t py, uint32_t key) {
+  uint32_t s = px * 1664525u + py * 1013904223u + key * 22695477u;
+  s ^= s >> 17;
+  s ^= s << 31;
+  s ^= s >> 11;
+  return s;
+}
+__device__ __forceinline__ uint32_t pcg_next(uint32_t& s) {
+  s = s * 747796405u + 2891336453u;
+  uint32_t w = ((s >> ((s >> 28) + 4u)) ^ s) * 277803737u;
+  return (w >> 22) ^ w;
+}
+__device__ __forceinline__ float rand01(uint32_t& s) {
+  return (float)(pcg_next(s) >> 8) / 16777216.0f;
+}
+__device__ float represented(float p) {
+  if (!(p > 0.0f)) return 0.0f;
+  if (p >= 1.0f) return 1.0f;
+  const float b = clampf(floorf(p * 16777216.0f + 0.5f), 1.0f, 16777215.0f);
+  return b / 16777216.0f;
+}
+__device__ uint32_t rand_bounded(uint32_t& s, uint32_t bound) {
+  if (bound <= 1u) return 0u;
+  const uint32_t threshold = ((0xFFFFFFFFu % bound) + 1u) % bound;
+  for (;;) {
+    const uint32_t hi = pcg_next(s) & 0xFFFFFF00u;
+    const uint32_t lo = pcg_next(s) >> 24;
+    const uint32_t w = hi | lo;
+    if (w >= threshold) return w % bound;
+  }
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t h, uint64_t v) {
+  h ^= v + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+  return h * 0xBF58476D1CE4E5B9ull;
+}
+
+__device__ __forceinline__ float luminance(F3 c) { return (0.2126f * c.x + 0.7152f * c.y) + 0.0722f * c.z; }
+
+// ---- BSDF (pt-webgpu/src/wgsl/pathTrace/bsdf.wgsl.ts, material.wgsl.ts) --
+
+struct Mat {
+  F3 base;
+  float rough, metal;
+  F3 emit;
+  bool two_sided;
+};
+
+__device__ Mat load_mat(const float* m) {
+  Mat r;
+  r.base = load3(m);
+  r.rough = m[3];
+  r.metal = m[4];
+  r.emit = load3(m + 5);
+  r.two_sided = m[8] > 0.5f;
+  return r;
+}
+
+// materialSpecularF0 with ior 1.5 and white specular (bsdf.wgsl.ts:192-209).
+__device__ F3 spec_f0(const Mat& m) {
+  const float ratio = (1.5f - 1.0f) / (1.5f + 1.0f);
+  const float f = ratio * ratio;
+  const float mt = clampf(m.metal, 0.0f, 1.0f);
+  return f3(f + (m.base.x - f) * mt, f + (m.base.y - f) * mt, f + (m.base.z - f) * mt);
+}
+// materialSpecularFresnelSchlick (bsdf.wgsl.ts:211-222), clamped to [0, 1].
+__device__ F3 schlick01(float c, F3 f0) {
+  const float x = clampf(1.0f - c, 0.0f, 1.0f);
+  const float x2 = x * x;
+  const float x5 = x2 * x2 * x;
+  return f3(clampf(f0.x + (1.0f - f0.x) * x5, 0.0f, 1.0f), clampf(f0.y + (1.0f - f0.y) * x5, 0.0f, 1.0f),
+            clampf(f0.z + (1.0f - f0.z) * x5, 0.0f, 1.0f));
+}
+__device__ float ggx_d(float ndh, float alpha) {  // material.wgsl.ts:2797
+  const float a2 = alpha * alpha;
+  const float n2 = clampf(ndh * ndh, 0.0f, 1.0f);
+  const float d = (1.0f - n2) + n2 * a2;
+  return a2 / (kPi * d * d);
+}
+__device__ float smith_g1(float ndv, float rough) {  // roughDielectric.ts:52-60
+  const float c = fmaxf(ndv, 0.0f);
+  if (c <= 1e-6f) return 0.0f;
+  const float alpha = fmaxf(rough * rough, 1e-3f);
+  const float tan2 = fmaxf(0.0f, 1.0f - c * c) / fmaxf(c * c, 1e-10f);
+  return 2.0f / (1.0f + __builtin_sqrtf(1.0f + alpha * alpha * tan2));
+}
+__device__ float power_heuristic(float a, float b) {  // material.wgsl.ts:2808
+  if (!(a >= 0.0f) || !(b >= 0.0f) || a > f32_max() || b > f32_max()) return 0.0f;
+  const float s = fmaxf(a, b);
+  if (!(s > 0.0f)) return 0.0f;
+  const float x = a / s, y = b / s;
+  const float x2 = x * x, y2 = y * y;
+  return x2 / (x2 + y2);
+}
+
+__constant__ float kGgxE[64] = {
+    0.1375f, 0.5617f, 0.7546f, 0.8522f, 0.9111f, 0.9505f, 0.9788f, 1.0f,
+    0.2955f, 0.515f,  0.7091f, 0.8192f, 0.889f,  0.937f,  0.9721f, 0.9988f,
+    0.5794f, 0.5541f, 0.6677f, 0.7691f, 0.8451f, 0.9021f, 0.9457f, 0.98f,
+    0.7011f, 0.6486f, 0.6669f, 0.7199f, 0.7776f, 0.8305f, 0.8764f, 0.9155f,
+    0.7335f, 0.6901f, 0.6696f, 0.6756f, 0.6972f, 0.7262f, 0.7578f, 0.7893f,
+    0.7153f, 0.6712f, 0.6355f, 0.6145f, 0.6052f, 0.6045f, 0.6101f, 0.6199f,
+    0.6669f, 0.6137f, 0.5657f, 0.5286f, 0.5f,    0.478f,  0.4611f, 0.4483f,
+    0.6017f, 0.537f,  0.4773f, 0.4296f, 0.3905f, 0.358f,  0.3305f, 0.3069f};
+__constant__ float kGgxEavg[8] = {0.9106f, 0.8931f, 0.8629f, 0.8094f, 0.725f, 0.6147f, 0.4931f, 0.3766f};
+
+__device__ float lut_e(float c, float rough) {  // ggxDirectionalAlbedo (material.wgsl.ts:2869)
+  const float mu = clampf(c, 0.0f, 1.0f), r = clampf(rough, 0.0f, 1.0f);
+  const float fr = r * 7.0f, fm = mu * 7.0f;
+  const uint32_t r0 = (uint32_t)floorf(fr), m0 = (uint32_t)floorf(fm);
+  const uint32_t r1 = min(r0 + 1u, 7u), m1 = min(m0 + 1u, 7u);
+  const float tr = fr - (float)r0, tm = fm - (float)m0;
+  const float e00 = kGgxE[r0 * 8 + m0], e01 = kGgxE[r0 * 8 + m1];
+  const float e10 = kGgxE[r1 * 8 + m0], e11 = kGgxE[r1 * 8 + m1];
+  const float e0 = e00 + (e01 - e00) * tm, e1 = e10 + (e11 - e10) * tm;
+  return clampf(e0 + (e1 - e0) * tr, 0.02f, 1.0f);
+}
+__device__ float lut_eavg(float rough) {
+  const float r = clampf(rough, 0.0f, 1.0f), fr = r * 7.0f;
+  const uint32_t r0 = (uint32_t)floorf(fr), r1 = min(r0 + 1u, 7u);
+  const float tr = fr - (float)r0;
+  return clampf(kGgxEavg[r0] + (kGgxEavg[r1] - kGgxEavg[r0]) * tr, 0.3f, 1.0f);
+}
+// ggxMultiscatterLobe (material.wgsl.ts:2906-2931)
+__device__ F3 ggx_ms(F3 f0, float rough, float ndv, float ndl) {
+  const float eavg = lut_eavg(rough), om = 1.0f - eavg;
+  if (om < 1e-4f) return f3(0, 0, 0);
+  const float eo = lut_e(ndv, rough), ei = lut_e(ndl, rough);
+  const float shape = (1.0f - eo) * (1.0f - ei) / fmaxf(kPi * om, 1e-6f);
+  const float c[3] = {f0.x, f0.y, f0.z};
+  float o[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float favg = c[k] + (1.0f - c[k]) * (1.0f / 21.0f);
+    const float fms = favg * favg * eavg / fmaxf(1.0f - favg * om, 1e-4f);
+    o[k] = fms * shape;
+  }
+  return f3(o[0], o[1], o[2]);
+}
+
+// evaluateBrdfFullWithClearcoatNormal, base lobes (bsdf.wgsl.ts:809-916).
+__device__ F3 bsdf_eval(const Mat& m, F3 n, F3 wo, F3 wi) {
+  const float ndl = fmaxf(dot(n, wi), 0.0f), ndv = fmaxf(dot(n, wo), 0.0f);
+  if (ndl <= 1e-5f || ndv <= 1e-5f) return f3(0, 0, 0);
+  const F3 h = safe_normalize(wi + wo);
+  const float ndh = fmaxf(dot(n, h), 0.0f), vdh = fmaxf(dot(wo, h), 0.0f);
+  const F3 f0 = spec_f0(m);
+  const F3 F = schlick01(vdh, f0);
+  const float kdm = 1.0f - m.metal;
+  const F3 diff = f3((1.0f - F.x) * kdm * m.base.x * kInvPi, (1.0f - F.y) * kdm * m.base.y * kInvPi,
+                     (1.0f - F.z) * kdm * m.base.z * kInvPi);
+  const float alpha = fmaxf(m.rough * m.rough, 1e-3f);
+  const float d = ggx_d(ndh, alpha);
+  const float g = smith_g1(ndv, m.rough) * smith_g1(ndl, m.rough);
+  const F3 spec = (d * g) * F * (1.0f / fmaxf(4.0f * ndv * ndl, 1e-6f));
+  const F3 f0c = f3(clampf(f0.x, 0, 1), clampf(f0.y, 0, 1), clampf(f0.z, 0, 1));
+  return diff + spec + ggx_ms(f0c, m.rough, ndv, ndl);
+}
+
+// brdfFiniteBaseLobeWeights specular probability (bsdf.wgsl.ts:1172-1210).
+__device__ float spec_prob(const Mat& m, float ndv) {
+  const F3 F = schlick01(ndv, spec_f0(m));
+  const float x = fmaxf(luminance(F), m.metal);
+  return represented(clampf(0.04f + (0.96f - 0.04f) * x, 0.04f, 0.96f));
+}
+
+// brdfDirectionalPdfWithIridescence (bsdf.wgsl.ts:1555-1631).
+__device__ float bsdf_pdf(const Mat& m, F3 n, F3 wo, F3 wi) {
+  const float wdn = dot(n, wi), odn = dot(n, wo);
+  const float ndv = fmaxf(odn, 0.0f);
+  if (ndv <= 1e-5f) return 0.0f;
+  const F3 h = safe_normalize(wi + wo);
+  const float ndh = fmaxf(dot(n, h), 0.0f);
+  const float sp = spec_prob(m, ndv), dp = 1.0f - sp;
+  if (!(wdn * odn > 0.0f)) return 0.0f;
+  const float ndl = fmaxf(wdn, 0.0f);
+  if (ndl <= 1e-5f) return 0.0f;
+  const float alpha = fmaxf(m.rough * m.rough, 1e-3f);
+  const float pspec = (ggx_d(ndh, alpha) * smith_g1(ndv, m.rough)) / fmaxf(4.0f * ndv, 1e-6f);
+  return dp * (ndl * kInvPi) + sp * pspec;
+}
+
+// buildOnb (bsdf.wgsl.ts:1632-1640)
+__device__ void build_onb(F3 n, F3& t, F3& b) {
+  const F3 up = fabsf(n.y) > 0.999f ? f3(1, 0, 0) : f3(0, 1, 0);
+  const F3 c = cross(up, n);
+  const float l = __builtin_sqrtf(dot(c, c));
+  t = f3(c.x / l, c.y / l, c.z / l);
+  b = cross(n, t);
+}
+
+// sampleGgxVndfTangent (bsdf.wgsl.ts:1710-1735)
+__device__ F3 vndf_sample(F3 wo, float alpha, uint32_t& rng) {
+  const F3 vh = safe_normalize(f3(alpha * wo.x, alpha * wo.y, wo.z));
+  const float lensq = vh.x * vh.x + vh.y * vh.y;
+  const float inv = lensq > 1e-10f ? 1.0f / __builtin_sqrtf(lensq) : 0.0f;
+  const F3 t1 = lensq > 1e-10f ? f3(-vh.y * inv, vh.x * inv, 0.0f) : f3(1, 0, 0);
+  const F3 t2 = cross(vh, t1);
+  const float u1 = rand01(rng), u2 = rand01(rng);
+  const float r = __builtin_sqrtf(u1), phi = 2.0f * kPi * u2;
+  const float a = r * cosf(phi);
+  float b = r * sinf(phi);
+  const float s = 0.5f * (1.0f + vh.z);
+  b = (1.0f - s) * __builtin_sqrtf(fmaxf(0.0f, 1.0f - a * a)) + s * b;
+  const F3 nh = a * t1 + b * t2 + __builtin_sqrtf(fmaxf(0.0f, 1.0f - a * a - b * b)) * vh;
+  return safe_normalize(f3(alpha * nh.x, alpha * nh.y, fmaxf(1e-6f, nh.z)));
+}
+
+// measureAreaVector (common.wgsl.ts:310-362): unit normal and area.
+__device__ bool area_measure(F3 u, F3 v, F3& normal, float& area) {
+  const float es = fmaxf(fmaxf(fabsf(u.x), fmaxf(fabsf(u.y), fabsf(u.z))),
+                         fmaxf(fabsf(v.x), fmaxf(fabsf(v.y), fabsf(v.z))));
+  if (!(es > 0.0f) || es > safe_max()) return false;
+  const F3 av = cross(f3(u.x / es, u.y / es, u.z / es), f3(v.x / es, v.y / es, v.z / es));
+  const float cs = fmaxf(fabsf(av.x), fmaxf(fabsf(av.y), fabsf(av.z)));
+  if (!(cs > 0.0f) || cs > safe_max()) return false;
+  const F3 ad = f3(av.x / cs, av.y / cs, av.z / cs);
+  const float dl = __builtin_sqrtf(dot(ad, ad));
+  normal = f3(ad.x / dl, ad.y / dl, ad.z / dl);
+  float a = 0.5f * (cs * dl);
+  a = (a * es) * es;
+  area = a;
+  return a > 0.0f && a <= safe_max();
+}
+// ptAreaToSolidAnglePdf (material.wgsl.ts:242-275), area-measure form.
+__device__ float area_to_solid_angle(float dist, float cosl, float area, float es) {
+  if (!(dist > 0.0f) || !(cosl > 0.0f) || !(area > 0.0f)) return 0.0f;
+  const float na = (area / es) / es;
+  const float de = dist / es;
+  const float pdf = (de * de) / (cosl * na);
+  return (pdf > 0.0f && pdf <= f32_max()) ? pdf : 0.0f;
+}
+__device__ float edge_scale(F3 u, F3 v) {
+  return fmaxf(fmaxf(fabsf(u.x), fmaxf(fabsf(u.y), fabsf(u.z))), fmaxf(fabsf(v.x), fmaxf(fabsf(v.y), fabsf(v.z))));
+}
+
+// ---- environment (connect.wgsl.ts:126-219) -------------------------------
+
+__device__ float env_pdf_dir(const ShadeScene& sc, F3 d) {
+  const float phi = atan2f(d.z, d.x);
+  const float theta = acosf(clampf(d.y, -1.0f, 1.0f));
+  float u = phi * kInv2Pi + 0.5f;
+  u = u - floorf(u);
+  const float v = clampf(theta * kInvPi, 0.0f, 0.999999f);
+  const uint32_t x = min((uint32_t)floorf(u * (float)sc.env_w), sc.env_w - 1u);
+  const uint32_t y = min((uint32_t)floorf(v * (float)sc.env_h), sc.env_h - 1u);
+  return sc.env_pdf[y * sc.env_w + x];
+}
+
+// ---- wave queue ------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool want) {
+  const uint64_t m = __ballot(want);
+  if (m == 0ull) return kMiss;
+  const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1u;
+  const uint32_t lane = __lane_id();
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+  base = __shfl(base, (int)leader);
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64u - lane));
+  return want ? base + (uint32_t)__popcll(m & below) : kMiss;
+}
+
+// ---- kernels -------------------------------------------------------------
+
+// generatePrimaryRay / unproject_ray_common (kernelCore.wgsl.ts:22-26; common.wgsl.ts:412-469)
+__global__ void camera_kernel(FrameParams f, PathState s) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= f.width * f.height) return;
+  const uint32_t px = p % f.width, py = p / f.width;
+  uint32_t rng = pcg_init(px, py, f.frame_seed ^ f.frame_index);
+  const float jx = rand01(rng), jy = rand01(rng);
+  const float u = ((float)px + jx) / (float)f.width, v = ((float)py + jy) / (float)f.height;
+  const float nx = u * 2.0f - 1.0f, ny = 1.0f - v * 2.0f;
+  const float* m = f.inv_vp;
+  float fh[4], nh[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    fh[r] = ((m[r] * nx + m[4 + r] * ny) + m[8 + r] * 1.0f) + m[12 + r];
+    nh[r] = ((m[r] * nx + m[4 + r] * ny) + m[8 + r] * -1.0f) + m[12 + r];
+  }
+  const float fs = fmaxf(fmaxf(fabsf(fh[0]), fabsf(fh[1])), fmaxf(fabsf(fh[2]), fabsf(fh[3])));
+  const float ns = fmaxf(fmaxf(fabsf(nh[0]), fabsf(nh[1])), fmaxf(fabsf(nh[2]), fabsf(nh[3])));
+  F3 o = f3(0, 0, 0), d = f3(0, 0, 0);
+  bool ok = fs > 0.0f && fs <= safe_max() && ns > 0.0f && ns <= safe_max();
+  if (ok) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) fh[r] = fh[r] / fs, nh[r] = nh[r] / ns;
+    ok = nh[3] != 0.0f;
+  }
+  if (ok) {
+    o = f3(nh[0] / nh[3], nh[1] / nh[3], nh[2] / nh[3]);
+    const float orient = fh[3] != 0.0f ? (fh[3] * nh[3] > 0.0f ? 1.0f : (fh[3] * nh[3] < 0.0f ? -1.0f : 0.0f)) : 1.0f;
+    const F3 num = f3((fh[0] * nh[3] - nh[0] * fh[3]) * orient, (fh[1] * nh[3] - nh[1] * fh[3]) * orient,
+                      (fh[2] * nh[3] - nh[2] * fh[3]) * orient);
+    d = safe_normalize(num);
+  }
+  s.rng[p] = rng;
+  store3(s.throughput + 3 * (size_t)p, f3(1, 1, 1));
+  store3(s.conn + 3 * (size_t)p, f3(0, 0, 0));
+  store3(s.radiance + 3 * (size_t)p, f3(0, 0, 0));
+  s.prev_pdf[p] = 0.0f;
+  s.flags[p] = 1u;
+  s.signature[p] = 0x1234ull;
+  s.queue[p] = p;
+  s.rays[2 * (size_t)p] = make_float4(o.x, o.y, o.z, f.tmin);
+  s.rays[2 * (size_t)p + 1] = make_float4(d.x, d.y, d.z, f32_max());
+  if (p == 0) s.count[0] = f.width * f.height;
+}
+
+// One vertex of every queued path: MIS connection of the previous BSDF sample,
+// emission, NEE (enqueues a shadow ray), BSDF sample (enqueues the next
+// extension ray) and Russian roulette.
+__global__ __launch_bounds__(256) void shade_kernel(FrameParams f, ShadeScene sc, PathState s,
+                                                     uint32_t vertex) {
+  const uint32_t n = s.count[0];
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+    const uint32_t i = base + threadIdx.x;
+    bool want_shadow = false, want_ext = false;
+    float4 shadow_a, shadow_b, ext_a, ext_b;
+    F3 shadow_c = f3(0, 0, 0);
+    uint32_t p = 0;
+    if (i < n) {
+      p = s.queue[i];
+      const uint32_t* h = s.hits + 5 * (size_t)i;
+      const uint32_t prim = h[0], inst = h[1];
+      const float t_hit = __uint_as_float(h[2]), bu = __uint_as_float(h[3]), bv = __uint_as_float(h[4]);
+      const bool miss = inst == kMiss;
+      const float4 rb = s.rays[2 * (size_t)i + 1];
+      const F3 dir = f3(rb.x, rb.y, rb.z);
+      uint32_t rng = s.rng[p];
+      F3 T = load3(s.throughput + 3 * (size_t)p);
+      const F3 Cn = load3(s.conn + 3 * (size_t)p);
+      F3 L = load3(s.radiance + 3 * (size_t)p);
+      const float prev_pdf = s.prev_pdf[p];
+      uint32_t flags = s.flags[p];
+      uint64_t sig = s.signature[p];
+      const bool prev_mis = prev_pdf > 0.0f;
+      const F3 env = f3(sc.env_radiance[0], sc.env_radiance[1], sc.env_radiance[2]);
+      // 1. BSDF -> light / environment connection of the previous sample.
+      Mat m{};
+      const float* inst_rec = miss ? nullptr : sc.instances + (size_t)inst * kInstFloats;
+      if (!miss) m = load_mat(sc.materials + (size_t)sc.tri_material[prim] * kMatFloats);
+      if (prev_mis) {
+        if (miss) {
+          if (sc.env_present) L = L + Cn * env * power_heuristic(prev_pdf, env_pdf_dir(sc, dir));
+        } else if ((m.emit.x > 0.0f || m.emit.y > 0.0f || m.emit.z > 0.0f) && sc.emitter_count) {
+          const uint32_t ebase = __float_as_uint(inst_rec[33]);
+          if (ebase != kMiss) {
+            const float* e = sc.emitters + (size_t)(ebase + sc.tri_source[prim]) * kEmitFloats;
+            const F3 a = load3(e), b = load3(e + 3), c = load3(e + 6);
+            F3 ln;
+            float area;
+            if (area_measure(b - a, c - a, ln, area)) {
+              float cl = dot(ln, f3(-dir.x, -dir.y, -dir.z));
+              cl = e[12] > 0.5f ? fabsf(cl) : fmaxf(cl, 0.0f);
+              const float lpdf = area_to_solid_angle(t_hit, cl, area, edge_scale(b - a, c - a));
+              if (lpdf > 0.0f) L = L + Cn * load3(e + 9) * power_heuristic(prev_pdf, lpdf);
+            }
+          }
+        }
+      }
+      const bool alive = (flags & 1u) != 0u;
+      bool done = !alive || vertex >= f.bounce_limit;
+      if (!done) sig = mix64(sig, miss ? 0xFFFFFFFFFFFFull : (((uint64_t)inst << 32) | prim));
+      if (!done && miss) {
+        if (!prev_mis && sc.env_present) L = L + T * env;
+        done = true;
+      }
+      if (!done) {
+        // 2. Surface at the hit (shadePrologue.wgsl.ts:55-130).
+        const uint4 tri = sc.indices[prim];
+        const float4 n0 = sc.normals[tri.x], n1 = sc.normals[tri.y], n2 = sc.normals[tri.z];
+        const float ba = (1.0f - bu) - bv;
+        const F3 nl = safe_normalize(f3(n0.x, n0.y, n0.z) * ba + f3(n1.x, n1.y, n1.z) * bu +
+                                     f3(n2.x, n2.y, n2.z) * bv);
+        const float* w2l = inst_rec;
+        const F3 nw = safe_normalize(f3(dot(load3(w2l), nl), dot(load3(w2l + 4), nl), dot(load3(w2l + 8), nl)));
+        const float4 p0 = sc.positions[tri.x], p1 = sc.positions[tri.y], p2 = sc.positions[tri.z];
+        const F3 ng = cross(f3(p1.x - p0.x, p1.y - p0.y, p1.z - p0.z), f3(p2.x - p0.x, p2.y - p0.y, p2.z - p0.z));
+        const F3 ld = f3(dot(f3(w2l[0], w2l[4], w2l[8]), dir), dot(f3(w2l[1], w2l[5], w2l[9]), dir),
+                         dot(f3(w2l[2], w2l[6], w2l[10]), dir));
+        const bool mirrored = inst_rec[32] > 0.5f;
+        const bool front = (dot(ld, ng) < 0.0f) != mirrored;
+        const F3 nrm = front ? nw : f3(-nw.x, -nw.y, -nw.z);
+        const F3 hp = load3(s.hit_pos + 3 * (size_t)i);
+        const F3 wo = f3(-dir.x, -dir.y, -dir.z);
+        if (!prev_mis && (front || m.two_sided)) L = L + T * m.emit;
+        // 3. NEE over one uniformly selected light (mediumNee.wgsl.ts:14-40;
+        //    kernel.wgsl.ts:1496-1690).
+        const uint32_t n_light = sc.emitter_count + (sc.env_present ? 1u : 0u);
+        if (n_light > 0u) {
+          const uint32_t picked = rand_bounded(rng, n_light);
+          const float inv_sel = (float)n_light;
+          sig = mix64(sig, 0xA000000000ull | picked);
+          if (picked < sc.emitter_count) {
+            const float* e = sc.emitters + (size_t)picked * kEmitFloats;
+            const F3 a = load3(e), b = load3(e + 3), c = load3(e + 6);
+            const float r1 = rand01(rng), r2 = rand01(rng);
+            const float su = __builtin_sqrtf(r1), uu = 1.0f - su, vv = r2 * su, ww = (1.0f - uu) - vv;
+            const F3 lp = a * uu + b * vv + c * ww;
+            const F3 tl = lp - hp;
+            const float dist = safe_length(tl);
+            F3 ln;
+            float area;
+            const bool ok = area_measure(b - a, c - a, ln, area);
+            const F3 wi = safe_normalize(tl);
+            const float ndl = fabsf(dot(nrm, wi));
+            if (dist > 0.0f && ndl > 0.0f && ok) {
+              const F3 fr = bsdf_eval(m, nrm, wo, wi);
+              float cl = dot(ln, f3(-wi.x, -wi.y, -wi.z));
+              cl = e[12] > 0.5f ? fabsf(cl) : fmaxf(cl, 0.0f);
+              if (cl > 0.0f) {
+                const float lpdf = area_to_solid_angle(dist, cl, area, edge_scale(b - a, c - a));
+                if (lpdf > 0.0f) {
+                  const float bpdf = bsdf_pdf(m, nrm, wo, wi);
+                  const float w = power_heuristic(lpdf, bpdf);
+                  const F3 off = dot(nrm, wi) > 0.0f ? nrm : f3(-nrm.x, -nrm.y, -nrm.z);
+                  const F3 so = hp + off * f.ray_bias;
+                  const float tmax = fmaxf(dist - f.ray_bias * 2.0f, f.tmin);
+                  shadow_c = T * fr * load3(e + 9) * (ndl * w / lpdf * inv_sel);
+                  shadow_a = make_float4(so.x, so.y, so.z, f.tmin);
+                  shadow_b = make_float4(wi.x, wi.y, wi.z, tmax);
+                  want_shadow = true;
+                }
+              }
+            }
+          } else {
+            const float xi = rand01(rng);
+            const uint32_t count = sc.env_w * sc.env_h;
+            uint32_t lo = 0, hi = count;
+            while (lo + 1u < hi) {
+              const uint32_t mid = (lo + hi) >> 1;
+              if (sc.env_cdf[mid] <= xi) lo = mid; else hi = mid;
+            }
+            const uint32_t idx = min(lo, count - 1u);
+            const float xa = rand01(rng), xb = rand01(rng);
+            const uint32_t ex = idx % sc.env_w, ey = idx / sc.env_w;
+            const float iw = 1.0f / (float)sc.env_w, ih = 1.0f / (float)sc.env_h;
+            const float phi0 = ((float)ex * iw - 0.5f) * (2.0f * kPi), phi1 = ((float)(ex + 1u) * iw - 0.5f) * (2.0f * kPi);
+            const float th0 = (float)ey * ih * kPi, th1 = (float)(ey + 1u) * ih * kPi;
+            const float phi = phi0 + (phi1 - phi0) * xa;
+            const float c0 = cosf(th0), c1 = cosf(th1);
+            const float ct = c0 + (c1 - c0) * xb;
+            const float st = __builtin_sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+            const F3 wi = safe_normalize(f3(cosf(phi) * st, ct, sinf(phi) * st));
+            const float epdf = sc.env_pdf[idx];
+            const float ndl = fabsf(dot(nrm, wi));
+            if (ndl > 0.0f) {
+              const F3 off = dot(nrm, wi) > 0.0f ? nrm : f3(-nrm.x, -nrm.y, -nrm.z);
+              const F3 so = hp + off * f.ray_bias;
+              const F3 fr = bsdf_eval(m, nrm, wo, wi);
+              const float bpdf = bsdf_pdf(m, nrm, wo, wi);
+              const float w = power_heuristic(epdf, bpdf);
+              shadow_c = T * fr * env * (ndl * w / fmaxf(epdf, 1e-8f) * inv_sel);
+              shadow_a = make_float4(so.x, so.y, so.z, f.tmin);
+              shadow_b = make_float4(wi.x, wi.y, wi.z, f32_max());
+              want_shadow = true;
+            }
+          }
+        }
+        // 4. BSDF sample, opaque branch (bsdf.wgsl.ts:2920-2997) and the finite
+        //    finalizer (bsdf.wgsl.ts:1863-1926).
+        const float ndv = fmaxf(dot(nrm, wo), 0.0f);
+        const float sp = spec_prob(m, ndv);
+        (void)rand01(rng);  // extension-lobe draw; P(base lobes) = 1
+        const bool glossy = rand01(rng) < sp;
+        F3 tb, bb;
+        build_onb(nrm, tb, bb);
+        F3 wi;
+        if (glossy) {
+          const float alpha = fmaxf(m.rough * m.rough, 1e-3f);
+          const F3 wl = f3(dot(wo, tb), dot(wo, bb), dot(wo, nrm));
+          const F3 hl = vndf_sample(wl, alpha, rng);
+          const F3 hw = safe_normalize(hl.x * tb + hl.y * bb + hl.z * nrm);
+          const F3 inc = f3(-wo.x, -wo.y, -wo.z);
+          wi = safe_normalize(inc - hw * (2.0f * dot(hw, inc)));
+        } else {
+          const float u1 = rand01(rng), u2 = rand01(rng);
+          const float r = __builtin_sqrtf(u1), phi = 2.0f * kPi * u2;
+          const float ct = __builtin_sqrtf(fmaxf(0.0f, 1.0f - u1));
+          wi = safe_normalize((r * cosf(phi)) * tb + (r * sinf(phi)) * bb + ct * nrm);
+        }
+        sig = mix64(sig, glossy ? 0xB1ull : 0xB0ull);
+        const float mpdf = bsdf_pdf(m, nrm, wo, wi);
+        const F3 fr = bsdf_eval(m, nrm, wo, wi);
+        const float cosine = fabsf(dot(nrm, wi));
+        const bool allows = mpdf > 0.0f && cosine > 1e-8f;
+        const F3 C2 = allows ? T * (fr * (cosine / mpdf)) : f3(0, 0, 0);
+        // 5. Russian roulette after the connection (kernel.wgsl.ts:2009-2013).
+        bool lives = true;
+        float rr_mul = 1.0f;
+        if (vertex > 2u) {
+          const float surv = represented(clampf(fmaxf(C2.x, fmaxf(C2.y, C2.z)), 0.1f, 0.95f));
+          lives = rand01(rng) < surv;
+          sig = mix64(sig, lives ? 0xC1ull : 0xC0ull);
+          rr_mul = 1.0f / surv;
+        }
+        T = C2 * rr_mul;
+        store3(s.conn + 3 * (size_t)p, C2);
+        s.prev_pdf[p] = allows ? mpdf : 0.0f;
+        flags = lives ? 1u : 0u;
+        if (allows) {
+          const F3 no = hp + nrm * f.ray_bias;
+          ext_a = make_float4(no.x, no.y, no.z, f.tmin);
+          ext_b = make_float4(wi.x, wi.y, wi.z, f32_max());
+          want_ext = true;
+        }
+      }
+      s.rng[p] = rng;
+      store3(s.throughput + 3 * (size_t)p, T);
+      store3(s.radiance + 3 * (size_t)p, L);
+      s.flags[p] = flags;
+      s.signature[p] = sig;
+    }
+    // Wave-aggregated queue appends (all lanes of the wave reach these).
+    const uint32_t si = wave_append(s.count + 2, want_shadow);
+    if (want_shadow) {
+      s.shadow_path[si] = p;
+      s.shadow_rays[2 * (size_t)si] = shadow_a;
+      s.shadow_rays[2 * (size_t)si + 1] = shadow_b;
+      store3(s.shadow_contrib + 3 * (size_t)si, shadow_c);
+    }
+    const uint32_t ei = wave_append(s.count + 1, want_ext);
+    if (want_ext) {
+      s.next_queue[ei] = p;
+      s.next_rays[2 * (size_t)ei] = ext_a;
+      s.next_rays[2 * (size_t)ei + 1] = ext_b;
+    }
+  }
+}
+
+__global__ void shadow_resolve_kernel(PathState s) {
+  const uint32_t n = s.count[2];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    if (s.shadow_occ[i]) continue;
+    const uint32_t p = s.shadow_path[i];
+    float* L = s.radiance + 3 * (size_t)p;
+    const float* c = s.shadow_contrib + 3 * (size_t)i;
+    L[0] = L[0] + c[0];
+    L[1] = L[1] + c[1];
+    L[2] = L[2] + c[2];
+  }
+}
+
+__global__ void accumulate_kernel(FrameParams f, PathState s) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= f.width * f.height) return;
+  const float* L = s.radiance + 3 * (size_t)p;
+  float* a = s.accum + 4 * (size_t)p;
+  a[0] = a[0] + fmaxf(L[0], 0.0f);
+  a[1] = a[1] + fmaxf(L[1], 0.0f);
+  a[2] = a[2] + fmaxf(L[2], 0.0f);
+}
+
+// Promotes the next extension queue to the current one and clears the
+// shadow count, on the device so a frame needs no host round trip.
+__global__ void rotate_counts_kernel(uint32_t* count) {
+  if (threadIdx.x == 0) {
+    count[0] = count[1];
+    count[1] = 0u;
+    count[2] = 0u;
+  }
+}
+
+__global__ void zero_f32_kernel(float* p, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0f;
+}
+
+hipError_t launch_camera(const FrameParams& f, const PathState& s, hipStream_t st) {
+  const uint32_t n = f.width * f.height;
+  hipLaunchKernelGGL(camera_kernel, dim3((n + 255) / 256), dim3(256), 0, st, f, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_shade(const FrameParams& f, const ShadeScene& sc, const PathState& s,
+                        uint32_t vertex, uint32_t capacity, hipStream_t st) {
+  const uint32_t blocks = (capacity + 255) / 256;
+  hipLaunchKernelGGL(shade_kernel, dim3(blocks < 8192 ? blocks : 8192), dim3(256), 0, st, f, sc, s, vertex);
+  return hipGetLastError();
+}
+
+hipError_t launch_shadow_resolve(const PathState& s, uint32_t capacity, hipStream_t st) {
+  const uint32_t blocks = (capacity + 255) / 256;
+  hipLaunchKernelGGL(shadow_resolve_kernel, dim3(blocks < 4096 ? blocks : 4096), dim3(256), 0, st, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_accumulate(const FrameParams& f, const PathState& s, hipStream_t st) {
+  const uint32_t n = f.width * f.height;
+  hipLaunchKernelGGL(accumulate_kernel, dim3((n + 255) / 256), dim3(256), 0, st, f, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_reset_counts(const PathState& s, hipStream_t st) {
+  hipLaunchKernelGGL(rotate_counts_kernel, dim3(1), dim3(64), 0, st, s.count);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_f32(float* p, uint32_t n, hipStream_t st) {
+  hipLaunchKernelGGL(zero_f32_kernel, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+
+}  // namespace vtr

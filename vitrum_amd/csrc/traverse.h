@@ -8,7 +8,7 @@
 namespace vtk {
 
 constexpr uint32_t kTraceBlock = 128;     // threads per block (2 waves)
-constexpr uint32_t kLdsDepth = 24;        // stack entries per lane kept in LDS
+constexpr uint32_t kLdsDepth = 20;        // stack entries per lane kept in LDS
 constexpr uint32_t kStackCapacity = 128;  // TLAS + BLAS entries per lane (>= 2 * 58 + 2)
 
 struct DeviceScene {
@@ -29,6 +29,8 @@ struct TraceParams {
   uint32_t* occluded;  // 1 word per ray (any)
   uint4* counts;       // per ray (counting mode)
   uint32_t n;
+  const uint32_t* n_ptr;  // when set, the ray count is read from device memory
+  float* hit_pos;         // optional: world hit position, 3 floats per ray
   uint32_t* counter;   // work counter, zeroed by the launcher
   uint32_t* error;     // set to non-zero on stack overflow
   uint2* spill;        // (kStackCapacity - kLdsDepth) x grid threads

@@ -125,15 +125,23 @@ struct TriHit {
   V3 point;
 };
 
-// Returns true and fills h when the triangle is hit with lo < t < hi, where
-// lo = tmin (also the predicate's exclusive minimum) and hi the current bound.
-// Work that can only turn an out-of-window hit into a miss is skipped for
-// out-of-window hits, which leaves the accepted set unchanged.
-__device__ __forceinline__ bool tri_test(V3 o, V3 dir, V3 a, V3 b, V3 c, float tmin, float hi,
-                                         TriHit& h) {
-  const float ds = max3abs(dir.x, dir.y, dir.z);
-  if (!(ds > 0.0f) || !finite_(ds) || nan_(tmin)) return false;
-  const V3 d = v3(dir.x / ds, dir.y / ds, dir.z / ds);
+// Ray-only part of the watertight test (opticalWatertightTriangle.ts:106-124):
+// the dominant axis permutation and shear, computed once per ray per instance.
+struct WtRay {
+  float ds;          // max |dir| component
+  float sx, sy, sz;  // shear constants
+  uint32_t perm;     // kx | ky << 2 | kz << 4; 0xFFFFFFFF = invalid direction
+};
+
+__device__ __forceinline__ WtRay wt_setup(V3 dir) {
+  WtRay w;
+  w.ds = max3abs(dir.x, dir.y, dir.z);
+  if (!(w.ds > 0.0f) || !finite_(w.ds)) {
+    w.perm = 0xFFFFFFFFu;
+    w.sx = w.sy = w.sz = 0.0f;
+    return w;
+  }
+  const V3 d = v3(dir.x / w.ds, dir.y / w.ds, dir.z / w.ds);
   int kz = fabs_(d.y) > fabs_(d.x) ? 1 : 0;
   if (fabs_(d.z) > fabs_(sel3(kz, d.x, d.y, d.z))) kz = 2;
   int kx = kz == 2 ? 0 : kz + 1;
@@ -144,15 +152,57 @@ __device__ __forceinline__ bool tri_test(V3 o, V3 dir, V3 a, V3 b, V3 c, float t
     kx = ky;
     ky = t;
   }
-  const float sx = sel3(kx, d.x, d.y, d.z) / dkz;
-  const float sy = sel3(ky, d.x, d.y, d.z) / dkz;
-  const float sz = 1.0f / dkz;
+  w.sx = sel3(kx, d.x, d.y, d.z) / dkz;
+  w.sy = sel3(ky, d.x, d.y, d.z) / dkz;
+  w.sz = 1.0f / dkz;
+  w.perm = (uint32_t)kx | ((uint32_t)ky << 2) | ((uint32_t)kz << 4);
+  return w;
+}
+
+// Rare tail of an in-window hit (opticalWatertightTriangle.ts:164-219): the
+// authored-normal side test, barycentrics and the canonical hit point.
+__device__ __noinline__ bool wt_accept_tail(V3 dir, float ds, V3 a, V3 b, V3 c, float e1, float e2,
+                                            float det, int zc, V3 point, TriHit* h) {
+  const V3 e1r = sub(b, a), e2r = sub(c, a);
+  const float es = jmax(max3abs(e1r.x, e1r.y, e1r.z), max3abs(e2r.x, e2r.y, e2r.z));
+  if (!(es > 0.0f) || !finite_(es)) return false;
+  const V3 s1 = v3(e1r.x / es, e1r.y / es, e1r.z / es);
+  const V3 s2 = v3(e2r.x / es, e2r.y / es, e2r.z / es);
+  const V3 nr = v3(s1.y * s2.z - s1.z * s2.y, s1.z * s2.x - s1.x * s2.z, s1.x * s2.y - s1.y * s2.x);
+  const float ns = max3abs(nr.x, nr.y, nr.z);
+  if (!(ns > 0.0f) || !finite_(ns)) return false;
+  const V3 nd = v3(nr.x / ns, nr.y / ns, nr.z / ns);
+  const float nl = __builtin_sqrtf((nd.x * nd.x + nd.y * nd.y) + nd.z * nd.z);
+  if (!(nl > 0.0f) || !finite_(nl)) return false;
+  const V3 n = v3(nd.x / nl, nd.y / nl, nd.z / nl);
+  const V3 d = v3(dir.x / ds, dir.y / ds, dir.z / ds);
+  const float side = -((d.x * n.x + d.y * n.y) + d.z * n.z);
+  if (side == 0.0f || !finite_(side)) return false;
+  h->u = e1 / det;
+  h->v = e2 / det;
+  if (zc == 0) {
+    point = v3(a.x + (h->u * (b.x - a.x) + h->v * (c.x - a.x)),
+               a.y + (h->u * (b.y - a.y) + h->v * (c.y - a.y)),
+               a.z + (h->u * (b.z - a.z) + h->v * (c.z - a.z)));
+  }
+  h->point = point;
+  return true;
+}
+
+// Returns true and fills h when the triangle is hit with lo < t < hi, where
+// lo = tmin (also the predicate's exclusive minimum) and hi the current bound.
+// Work that can only turn an out-of-window hit into a miss is skipped for
+// out-of-window hits, which leaves the accepted set unchanged.
+__device__ __forceinline__ bool tri_test(V3 o, V3 dir, const WtRay& w, V3 a, V3 b, V3 c, float tmin,
+                                         float hi, TriHit& h) {
+  if (w.perm == 0xFFFFFFFFu || nan_(tmin)) return false;
+  const int kx = w.perm & 3, ky = (w.perm >> 2) & 3, kz = w.perm >> 4;
   const V3 pa = sub(a, o), pb = sub(b, o), pc = sub(c, o);
   const float paz = sel3(kz, pa.x, pa.y, pa.z), pbz = sel3(kz, pb.x, pb.y, pb.z),
               pcz = sel3(kz, pc.x, pc.y, pc.z);
-  const float apx = sel3(kx, pa.x, pa.y, pa.z) - sx * paz, apy = sel3(ky, pa.x, pa.y, pa.z) - sy * paz;
-  const float bpx = sel3(kx, pb.x, pb.y, pb.z) - sx * pbz, bpy = sel3(ky, pb.x, pb.y, pb.z) - sy * pbz;
-  const float cpx = sel3(kx, pc.x, pc.y, pc.z) - sx * pcz, cpy = sel3(ky, pc.x, pc.y, pc.z) - sy * pcz;
+  const float apx = sel3(kx, pa.x, pa.y, pa.z) - w.sx * paz, apy = sel3(ky, pa.x, pa.y, pa.z) - w.sy * paz;
+  const float bpx = sel3(kx, pb.x, pb.y, pb.z) - w.sx * pbz, bpy = sel3(ky, pb.x, pb.y, pb.z) - w.sy * pbz;
+  const float cpx = sel3(kx, pc.x, pc.y, pc.z) - w.sx * pcz, cpy = sel3(ky, pc.x, pc.y, pc.z) - w.sy * pcz;
   const float e0 = edge_fn(bpx, bpy, cpx, cpy);
   const float e1 = edge_fn(cpx, cpy, apx, apy);
   const float e2 = edge_fn(apx, apy, bpx, bpy);
@@ -161,8 +211,8 @@ __device__ __forceinline__ bool tri_test(V3 o, V3 dir, V3 a, V3 b, V3 c, float t
   const bool positive = det > 0.0f;
   const float n0 = positive ? e0 : -e0, n1 = positive ? e1 : -e1, n2 = positive ? e2 : -e2;
   if (n0 < 0.0f || n1 < 0.0f || n2 < 0.0f) return false;
-  const float scaled_t = (e0 * (paz * sz) + e1 * (pbz * sz)) + e2 * (pcz * sz);
-  float t = (scaled_t / det) / ds;
+  const float scaled_t = (e0 * (paz * w.sz) + e1 * (pbz * w.sz)) + e2 * (pcz * w.sz);
+  float t = (scaled_t / det) / w.ds;
   V3 point = v3(0, 0, 0);
   const int zc = (n0 == 0.0f) + (n1 == 0.0f) + (n2 == 0.0f);
   if (zc != 0) {
@@ -181,30 +231,8 @@ __device__ __forceinline__ bool tri_test(V3 o, V3 dir, V3 a, V3 b, V3 c, float t
   const float lo = tmin > 0.0f ? tmin : 0.0f;
   if (!(t > lo) || !finite_(t)) return false;
   if (!(t > tmin && t < hi)) return false;  // traversal acceptance window
-  const V3 e1r = sub(b, a), e2r = sub(c, a);
-  const float es = jmax(max3abs(e1r.x, e1r.y, e1r.z), max3abs(e2r.x, e2r.y, e2r.z));
-  if (!(es > 0.0f) || !finite_(es)) return false;
-  const V3 s1 = v3(e1r.x / es, e1r.y / es, e1r.z / es);
-  const V3 s2 = v3(e2r.x / es, e2r.y / es, e2r.z / es);
-  const V3 nr = v3(s1.y * s2.z - s1.z * s2.y, s1.z * s2.x - s1.x * s2.z, s1.x * s2.y - s1.y * s2.x);
-  const float ns = max3abs(nr.x, nr.y, nr.z);
-  if (!(ns > 0.0f) || !finite_(ns)) return false;
-  const V3 nd = v3(nr.x / ns, nr.y / ns, nr.z / ns);
-  const float nl = __builtin_sqrtf((nd.x * nd.x + nd.y * nd.y) + nd.z * nd.z);
-  if (!(nl > 0.0f) || !finite_(nl)) return false;
-  const V3 n = v3(nd.x / nl, nd.y / nl, nd.z / nl);
-  const float side = -((d.x * n.x + d.y * n.y) + d.z * n.z);
-  if (side == 0.0f || !finite_(side)) return false;
   h.t = t;
-  h.u = e1 / det;
-  h.v = e2 / det;
-  if (zc == 0) {
-    point = v3(a.x + (h.u * (b.x - a.x) + h.v * (c.x - a.x)),
-               a.y + (h.u * (b.y - a.y) + h.v * (c.y - a.y)),
-               a.z + (h.u * (b.z - a.z) + h.v * (c.z - a.z)));
-  }
-  h.point = point;
-  return true;
+  return wt_accept_tail(dir, w.ds, a, b, c, e1, e2, det, zc, point, &h);
 }
 
 // ---------------------------------------------------------------------------
@@ -291,14 +319,17 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceParams p) {
   const uint32_t lane = __lane_id();
   const uint64_t lane_below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
+  const uint32_t n_rays = p.n_ptr ? *p.n_ptr : p.n;
   bool active = false;
   uint32_t ray_idx = 0;
   // World ray and best hit.
   V3 ro{}, rd{}, rinv{};
   float tmin = 0, tmax = 0, best = 0, bu = 0, bv = 0;
+  V3 bpos{};
   uint32_t bprim = 0, binst = kMiss;
   // Instance-local state.
   V3 lo{}, ld{}, linv{}, lpos{};
+  WtRay lw{};
   float ltmin = 0, lbest = 0, lu = 0, lv = 0;
   uint32_t lprim = 0, inst_id = 0;
   bool lhit = false;
@@ -313,6 +344,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceParams p) {
     lo = xform_point(c0, c1, c2, c3, ro);
     ld = xform_dir(c0, c1, c2, rd);
     linv = v3(safe_inv(ld.x), safe_inv(ld.y), safe_inv(ld.z));
+    lw = wt_setup(ld);
     const V3 start = v3(ro.x + rd.x * tmin, ro.y + rd.y * tmin, ro.z + rd.z * tmin);
     ltmin = fmaxf(dot3(sub(xform_point(c0, c1, c2, c3, start), lo), ld), 0.0f);
     // Closest: localTMax starts at INFINITY; any-hit: at the ray's tMax
@@ -339,6 +371,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceParams p) {
       const float wd = dot3(sub(w, ro), rd);
       if (wd > tmin && wd < best) {
         best = wd;
+        bpos = w;
         bprim = lprim;
         binst = inst_id;
         bu = lu;
@@ -358,6 +391,12 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceParams p) {
       h[2] = hit ? f2u(best) : 0x7F800000u;
       h[3] = hit ? f2u(bu) : 0u;
       h[4] = hit ? f2u(bv) : 0u;
+      if (p.hit_pos) {
+        float* hp = p.hit_pos + 3 * (size_t)ray_idx;
+        hp[0] = bpos.x;
+        hp[1] = bpos.y;
+        hp[2] = bpos.z;
+      }
     }
     if (COUNT) p.counts[ray_idx] = make_uint4(c_blas, c_tri, c_tlas, c_inst);
     active = false;
@@ -382,7 +421,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceParams p) {
       base = __shfl(base, 0);
       if (!active) {
         const uint32_t idx = base + (uint32_t)__popcll(idle & lane_below);
-        if (idx < p.n) {
+        if (idx < n_rays) {
           ray_idx = idx;
           const float4 a = p.rays[2 * (size_t)idx], b = p.rays[2 * (size_t)idx + 1];
           ro = v3(a.x, a.y, a.z);
@@ -483,7 +522,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceParams p) {
         const float4 w0 = tr[0], w1 = tr[1], w2 = tr[2];
         if (COUNT) ++c_tri;
         TriHit h;
-        if (tri_test(lo, ld, v3(w0.x, w0.y, w0.z), v3(w0.w, w1.x, w1.y), v3(w1.z, w1.w, w2.x), ltmin,
+        if (tri_test(lo, ld, lw, v3(w0.x, w0.y, w0.z), v3(w0.w, w1.x, w1.y), v3(w1.z, w1.w, w2.x), ltmin,
                      lbest, h)) {
           if (ANY) {
             finish(true);

@@ -29,7 +29,8 @@ EXPORTED = [
     "vt_set_trace_tuning", "vt_selftest_fp32", "vt_device_alloc", "vt_device_free",
     "vt_memcpy_h2d", "vt_memcpy_d2h", "vt_synchronize", "vt_get_stream", "vt_stats",
     "vt_build_blas", "vt_build_tlas", "vt_pack_scene", "vt_pack_field_words",
-    "vt_pack_copy_field", "vt_pack_free", "vt_version",
+    "vt_pack_copy_field", "vt_pack_free", "vt_version", "vt_render_frame", "vt_capture",
+    "vt_reset", "vt_frame_time", "vt_capture_signatures", "vt_render_info",
 ]
 
 
@@ -65,6 +66,12 @@ class SceneDesc(C.Structure):
 class Options(C.Structure):
     _fields_ = [("max_bounces", C.c_uint32), ("max_samples_per_pixel", C.c_uint32),
                 ("sampling", C.c_uint32), ("reserved", C.c_uint32 * 5)]
+
+
+class Frame(C.Structure):
+    _fields_ = [("inv_view_proj", C.c_float * 16), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("frame_index", C.c_uint32), ("frame_seed", C.c_uint32),
+                ("max_bounces", C.c_uint32), ("samples_target", C.c_uint32)]
 
 
 class FrameStats(C.Structure):
@@ -277,6 +284,38 @@ class Engine:
         check(lib().vt_selftest_fp32(self._h, _p(a, F32P), _p(b, F32P), _p(q, F32P), _p(r, F32P),
                                      C.c_uint32(a.size)), self._h)
         return q, r
+
+    # frames
+    def render_frame(self, inv_view_proj, width, height, frame_index, frame_seed=0x9E3779B9,
+                     max_bounces=0, samples_target=0):
+        f = Frame((C.c_float * 16)(*np.asarray(inv_view_proj, np.float32).ravel()), width, height,
+                  frame_index, frame_seed, max_bounces, samples_target)
+        check(lib().vt_render_frame(self._h, C.byref(f)), self._h)
+
+    def capture(self, width, height):
+        out = np.zeros((height, width, 4), np.float32)
+        check(lib().vt_capture(self._h, out.ctypes.data_as(F32P), width, height), self._h)
+        return out
+
+    def capture_signatures(self, width, height):
+        out = np.zeros((height, width), np.uint64)
+        check(lib().vt_capture_signatures(self._h, out.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                          width, height), self._h)
+        return out
+
+    def reset(self):
+        check(lib().vt_reset(self._h), self._h)
+
+    def frame_time_ms(self):
+        ms = C.c_float()
+        check(lib().vt_frame_time(self._h, C.byref(ms)), self._h)
+        return ms.value
+
+    def render_info(self):
+        n = C.c_uint32()
+        bias = C.c_float()
+        check(lib().vt_render_info(self._h, C.byref(n), C.byref(bias)), self._h)
+        return {"emitter_count": n.value, "ray_bias": bias.value}
 
     # device-resident plumbing
     def alloc(self, nbytes):

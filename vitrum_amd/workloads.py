@@ -302,6 +302,14 @@ def projection(vfov_deg, aspect, near, far):
     return p
 
 
+def inv_view_proj(camera, width, height):
+    """Column-major float32 inverse of projection x view (FrameInput's
+    viewMatrix / projMatrix, packages/core/src/frame.ts:91)."""
+    proj = projection(camera["vfov_deg"], width / height, camera["near"], camera["far"])
+    inv = np.linalg.inv(proj @ camera["view"])
+    return inv.T.astype(np.float32).ravel()
+
+
 RAY_DTYPE = np.dtype([("o", "<f4", 3), ("tmin", "<f4"), ("d", "<f4", 3), ("tmax", "<f4")])
 HIT_DTYPE = np.dtype([("prim", "<u4"), ("inst", "<u4"), ("t", "<f4"), ("u", "<f4"), ("v", "<f4")])
 TMAX_INFINITY = np.uint32(0x7F7FFFFF).view(np.float32)
