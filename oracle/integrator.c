@@ -248,24 +248,45 @@ static d3 vndf_sample(d3 wo, double alpha, uint32_t *rng) {
 
 /* ---- camera (kernelCore.wgsl.ts:22-26; common.wgsl.ts:412-469) --------- */
 
-static void mat4_mul_vec(const double *m, const double *v, double *o) {
-  for (int r = 0; r < 4; ++r)
-    o[r] = m[r] * v[0] + m[4 + r] * v[1] + m[8 + r] * v[2] + m[12 + r] * v[3];
-}
 
-static int primary_ray(const double *inv_vp, uint32_t px, uint32_t py, uint32_t w, uint32_t h,
-                       double jx, double jy, d3 *o, d3 *d) {
-  double u = ((double)px + jx) / (double)w, v = ((double)py + jy) / (double)h;
-  double ndc[2] = {u * 2.0 - 1.0, 1.0 - v * 2.0};
-  double fin[4] = {ndc[0], ndc[1], 1, 1}, nin[4] = {ndc[0], ndc[1], -1, 1}, fh[4], nh[4];
-  mat4_mul_vec(inv_vp, fin, fh);
-  mat4_mul_vec(inv_vp, nin, nh);
-  if (nh[3] == 0 || fh[3] == 0) return 0;
-  double orient = (fh[3] * nh[3]) > 0 ? 1.0 : -1.0;
+/* generatePrimaryRay / unproject_ray_common (kernelCore.wgsl.ts:22-26;
+ * common.wgsl.ts:412-469).  Camera rays are generated in f32, as the reference
+ * shader does: the near/far unprojection cancels heavily in f32 when
+ * far/near is large, so a binary64 camera would trace different rays.  The
+ * matrix-vector products sum the columns in order 0..3; divisions and square
+ * roots round once (IEEE binary32). */
+static float f32max3abs(float a, float b, float c) {
+  float x = fabsf(a), y = fabsf(b), z = fabsf(c);
+  return x > y ? (x > z ? x : z) : (y > z ? y : z);
+}
+static int primary_ray(const float *m, uint32_t px, uint32_t py, uint32_t w, uint32_t h, float jx,
+                       float jy, d3 *o, d3 *d) {
+  const float u = ((float)px + jx) / (float)w, v = ((float)py + jy) / (float)h;
+  const float nx = u * 2.0f - 1.0f, ny = 1.0f - v * 2.0f;
+  float fh[4], nh[4];
+  for (int r = 0; r < 4; ++r) {
+    fh[r] = ((m[r] * nx + m[4 + r] * ny) + m[8 + r] * 1.0f) + m[12 + r];
+    nh[r] = ((m[r] * nx + m[4 + r] * ny) + m[8 + r] * -1.0f) + m[12 + r];
+  }
+  const float fs = fmaxf(fmaxf(fabsf(fh[0]), fabsf(fh[1])), fmaxf(fabsf(fh[2]), fabsf(fh[3])));
+  const float ns = fmaxf(fmaxf(fabsf(nh[0]), fabsf(nh[1])), fmaxf(fabsf(nh[2]), fabsf(nh[3])));
+  const float safe = 3.4028229765e38f; /* 0x7F7FFFFD */
+  if (!(fs > 0.0f && fs <= safe && ns > 0.0f && ns <= safe)) return 0;
+  for (int r = 0; r < 4; ++r) fh[r] = fh[r] / fs, nh[r] = nh[r] / ns;
+  if (nh[3] == 0.0f) return 0;
   *o = D3(nh[0] / nh[3], nh[1] / nh[3], nh[2] / nh[3]);
-  d3 num = D3((fh[0] * nh[3] - nh[0] * fh[3]) * orient, (fh[1] * nh[3] - nh[1] * fh[3]) * orient,
-              (fh[2] * nh[3] - nh[2] * fh[3]) * orient);
-  *d = safe_normalize(num);
+  const float orient = fh[3] != 0.0f ? (fh[3] * nh[3] > 0.0f ? 1.0f : (fh[3] * nh[3] < 0.0f ? -1.0f : 0.0f)) : 1.0f;
+  float q[3] = {(fh[0] * nh[3] - nh[0] * fh[3]) * orient, (fh[1] * nh[3] - nh[1] * fh[3]) * orient,
+                (fh[2] * nh[3] - nh[2] * fh[3]) * orient};
+  /* safe_normalize in f32 (common.wgsl.ts) */
+  const float sc = f32max3abs(q[0], q[1], q[2]);
+  if (!(sc > 0.0f) || sc > safe) {
+    *d = D3(0, 1, 0);
+    return 1;
+  }
+  for (int k = 0; k < 3; ++k) q[k] = q[k] / sc;
+  const float l = sqrtf((q[0] * q[0] + q[1] * q[1]) + q[2] * q[2]);
+  *d = D3(q[0] / l, q[1] / l, q[2] / l);
   return 1;
 }
 
@@ -301,8 +322,22 @@ typedef struct {
   d3 radiance;
   uint32_t w, h;
   const float *cdf; /* w*h + 1 entries, f32 */
-  double texel_pdf;
+  const float *pdf; /* per texel, solid-angle density of the sampler */
 } env_t;
+
+/* environmentLookup's texel pdf (connect.wgsl.ts:126-153) in binary64. */
+static double env_pdf_dir(const env_t *e, d3 d) {
+  double phi = atan2(d.z, d.x);
+  double theta = acos(d.y < -1 ? -1 : (d.y > 1 ? 1 : d.y));
+  double u = phi * (0.5 / PI) + 0.5;
+  u = u - floor(u);
+  double v = theta / PI;
+  v = v < 0 ? 0 : (v > 0.999999 ? 0.999999 : v);
+  uint32_t x = (uint32_t)floor(u * e->w), y = (uint32_t)floor(v * e->h);
+  if (x > e->w - 1) x = e->w - 1;
+  if (y > e->h - 1) y = e->h - 1;
+  return (double)e->pdf[y * e->w + x];
+}
 
 static d3 env_cell_dir(uint32_t x, uint32_t y, uint32_t w, uint32_t h, double xa, double xb) {
   double phi0 = ((double)x / w - 0.5) * (2.0 * PI), phi1 = ((double)(x + 1) / w - 0.5) * (2.0 * PI);
@@ -327,7 +362,7 @@ typedef struct orc_render_desc {
   float env_radiance[3];
   uint32_t env_w, env_h;
   const float *env_cdf;
-  float env_texel_pdf;
+  const float *env_pdf;
   float inv_view_proj[16];        /* column-major, f32 */
   uint32_t width, height;
   uint32_t frame_seed;
@@ -340,7 +375,6 @@ typedef struct {
   emitter_t *em;
   mat_t *mats;
   env_t env;
-  double inv_vp[16];
   double bias, tmin;
 } rctx;
 
@@ -439,15 +473,23 @@ static uint64_t mix64(uint64_t h, uint64_t v) {
   return h * 0xBF58476D1CE4E5B9ull;
 }
 
+/* Mixes one decision into the signature; with `ev` non-NULL also logs it
+   (ev[0] = count, ev[1..ORC_EVENT_SLOTS-1] = decisions). */
+#define ORC_EVENT_SLOTS 40
+static uint64_t mixlog(uint64_t h, uint64_t v, uint64_t *ev) {
+  if (ev && ev[0] + 1 < ORC_EVENT_SLOTS) ev[1 + ev[0]++] = v;
+  return mix64(h, v);
+}
+
 static void render_path(const rctx *c, uint32_t px, uint32_t py, uint32_t frame_index, double *out,
-                        uint64_t *sig_out, const int *sorted_to_orig) {
+                        uint64_t *sig_out, const int *sorted_to_orig, uint64_t *ev) {
   const orc_render_desc *d = c->d;
   uint32_t rng = orc_pcg_init(px, py, d->frame_seed ^ frame_index);
-  double jx = rand01(&rng), jy = rand01(&rng);
+  const float jx = (float)rand01(&rng), jy = (float)rand01(&rng);
   d3 o, dir;
   d3 L = D3(0, 0, 0), T = D3(1, 1, 1);
   uint64_t sig = 0x1234;
-  if (!primary_ray(c->inv_vp, px, py, d->width, d->height, jx, jy, &o, &dir)) {
+  if (!primary_ray(d->inv_view_proj, px, py, d->width, d->height, jx, jy, &o, &dir)) {
     out[0] = out[1] = out[2] = 0;
     *sig_out = sig;
     return;
@@ -458,7 +500,7 @@ static void render_path(const rctx *c, uint32_t px, uint32_t py, uint32_t frame_
   for (uint32_t bounce = 0; bounce < limit; ++bounce) {
     surf_t s;
     closest(c, o, dir, &s);
-    sig = mix64(sig, s.hit ? (((uint64_t)s.inst << 32) | s.prim) : 0xFFFFFFFFFFFFull);
+    sig = mixlog(sig, s.hit ? (((uint64_t)s.inst << 32) | s.prim) : 0xFFFFFFFFFFFFull, ev);
     if (!s.hit) {
       if (!prev_mis && c->env.present) L = add(L, mul(T, c->env.radiance));
       break;
@@ -470,7 +512,7 @@ static void render_path(const rctx *c, uint32_t px, uint32_t py, uint32_t frame_
     if (n_light > 0) {
       uint32_t picked = rand_bounded(&rng, n_light);
       double inv_sel = (double)n_light;
-      sig = mix64(sig, 0xA000000000ull | picked);
+      sig = mixlog(sig, 0xA000000000ull | picked, ev);
       if (picked < d->emitter_count) {
         const emitter_t *e = &c->em[sorted_to_orig[picked]];
         double r1 = rand01(&rng), r2 = rand01(&rng);
@@ -511,7 +553,7 @@ static void render_path(const rctx *c, uint32_t px, uint32_t py, uint32_t frame_
         uint32_t idx = lo < count - 1 ? lo : count - 1;
         double xa = rand01(&rng), xb = rand01(&rng);
         d3 wi = safe_normalize(env_cell_dir(idx % c->env.w, idx / c->env.w, c->env.w, c->env.h, xa, xb));
-        double epdf = c->env.texel_pdf;
+        double epdf = (double)c->env.pdf[idx];
         double ndl = fabs(dot(n, wi));
         if (ndl > 0) {
           d3 off = dot(n, wi) > 0 ? n : scl(n, -1);
@@ -545,7 +587,7 @@ static void render_path(const rctx *c, uint32_t px, uint32_t py, uint32_t frame_
       double r = sqrt(u1), phi = 2.0 * PI * u2, ct = sqrt(maxd(0, 1 - u1));
       wi = safe_normalize(add(add(scl(tb, r * cos(phi)), scl(bb, r * sin(phi))), scl(n, ct)));
     }
-    sig = mix64(sig, glossy ? 0xB1 : 0xB0);
+    sig = mixlog(sig, glossy ? 0xB1 : 0xB0, ev);
     /* finalizeFiniteBounceSample (bsdf.wgsl.ts:1863-1926) */
     double mpdf = bsdf_pdf(m, n, wo, wi);
     d3 f = bsdf_eval(m, n, wo, wi);
@@ -605,18 +647,22 @@ static void render_path(const rctx *c, uint32_t px, uint32_t py, uint32_t frame_
           L = add(L, scl(mul(mul(T_at, f), best_le), ndl * w / bpdf));
         }
         if (c->env.present && visible(c, oo, wi, F32_MAX)) {
-          double w = power_heuristic(bpdf, c->env.texel_pdf);
+          double w = power_heuristic(bpdf, env_pdf_dir(&c->env, wi));
           L = add(L, scl(mul(mul(T_at, f), c->env.radiance), ndl * w / bpdf));
         }
       }
     }
     prev_mis = allows && mpdf > 0;
+    /* A rejected sample leaves zero throughput.  The reference keeps tracing
+       such a path (kernel.wgsl.ts:1993-2013), but every later term it adds is
+       multiplied by that zero, so ending it here changes no radiance value. */
+    if (!allows) break;
     o = new_o;
     dir = wi;
     if (bounce > 2) {
       double surv = represented(clampd(maxd(T.x, maxd(T.y, T.z)), 0.1, 0.95));
       int lives = rand01(&rng) < surv;
-      sig = mix64(sig, lives ? 0xC1 : 0xC0);
+      sig = mixlog(sig, lives ? 0xC1 : 0xC0, ev);
       if (!lives) break;
       T = scl(T, 1.0 / surv);
     }
@@ -633,6 +679,7 @@ typedef struct {
   uint32_t frame_index, x0, y0, w, h, row_begin, row_end;
   double *out;
   uint64_t *sig;
+  uint64_t *events; /* NULL, or ORC_EVENT_SLOTS per pixel */
 } rjob;
 
 static void *render_rows(void *arg) {
@@ -641,7 +688,7 @@ static void *render_rows(void *arg) {
     for (uint32_t xx = 0; xx < j->w; ++xx) {
       size_t i = (size_t)yy * j->w + xx;
       render_path(j->c, j->x0 + xx, j->y0 + yy, j->frame_index, j->out + 3 * i, j->sig + i,
-                  j->sorted_to_orig);
+                  j->sorted_to_orig, j->events ? j->events + ORC_EVENT_SLOTS * i : NULL);
     }
   return NULL;
 }
@@ -655,9 +702,10 @@ static int cmp_em(const void *a, const void *b) {
 
 /* Renders one frame of the tile [x0, x0+w) x [y0, y0+h).
  * out: 3 doubles per pixel (radiance of this frame's path), sig: path
- * signature per pixel. */
+ * signature per pixel, events: NULL or ORC_EVENT_SLOTS words per pixel (count,
+ * then each decision mixed into the signature, in order). */
 int orc_render(const orc_render_desc *d, uint32_t frame_index, uint32_t x0, uint32_t y0, uint32_t w,
-               uint32_t h, double *out, uint64_t *sig, uint32_t threads) {
+               uint32_t h, double *out, uint64_t *sig, uint64_t *events, uint32_t threads) {
   rctx c;
   memset(&c, 0, sizeof c);
   c.d = d;
@@ -691,8 +739,7 @@ int orc_render(const orc_render_desc *d, uint32_t frame_index, uint32_t x0, uint
   c.env.w = d->env_w;
   c.env.h = d->env_h;
   c.env.cdf = d->env_cdf;
-  c.env.texel_pdf = d->env_texel_pdf;
-  for (int i = 0; i < 16; ++i) c.inv_vp[i] = d->inv_view_proj[i];
+  c.env.pdf = d->env_pdf;
   c.bias = (double)d->ray_bias;
   c.tmin = (double)f32(c.bias * 0.1);
   if (threads < 1) threads = 1;
@@ -702,7 +749,7 @@ int orc_render(const orc_render_desc *d, uint32_t frame_index, uint32_t x0, uint
   uint32_t per = (h + threads - 1) / threads;
   for (uint32_t t = 0; t < threads; ++t) {
     jobs[t] = (rjob){&c, order, frame_index, x0, y0, w, h, t * per < h ? t * per : h,
-                     (t + 1) * per < h ? (t + 1) * per : h, out, sig};
+                     (t + 1) * per < h ? (t + 1) * per : h, out, sig, events};
     if (pthread_create(&tid[t], NULL, render_rows, &jobs[t]) != 0) {
       render_rows(&jobs[t]);
       tid[t] = 0;

@@ -298,7 +298,7 @@ class RenderDesc(C.Structure):
         ("materials", F32P), ("material_count", C.c_uint32), ("emitters", F32P),
         ("emitter_count", C.c_uint32), ("env_present", C.c_uint32),
         ("env_radiance", C.c_float * 3), ("env_w", C.c_uint32), ("env_h", C.c_uint32),
-        ("env_cdf", F32P), ("env_texel_pdf", C.c_float), ("inv_view_proj", C.c_float * 16),
+        ("env_cdf", F32P), ("env_pdf", F32P), ("inv_view_proj", C.c_float * 16),
         ("width", C.c_uint32), ("height", C.c_uint32), ("frame_seed", C.c_uint32),
         ("max_bounces", C.c_uint32), ("ray_bias", C.c_float),
     ]
@@ -421,9 +421,11 @@ def build_constant_env(radiance, width=64, height=32):
 
 
 def render(pack, materials, camera_inv_vp, width, height, *, frame_seed, frame_index, max_bounces,
-           ray_bias, env_radiance=None, env_size=(64, 32), tile=None, threads=None):
+           ray_bias, env_radiance=None, env_size=(64, 32), tile=None, threads=None, events=False):
     """One frame (1 spp) of the binary64 integrator mirror.  Returns
-    (radiance[h, w, 3] float64, signature[h, w] uint64)."""
+    (radiance[h, w, 3] float64, signature[h, w] uint64), plus with `events`
+    the decisions mixed into each signature ([h, w, 40] uint64: count, then
+    hit keys, light picks, lobes and roulette outcomes in path order)."""
     ps = PackedScene(pack)
     mt = material_table(materials)
     em = np.ascontiguousarray(build_emitters(pack, mt))
@@ -432,11 +434,9 @@ def render(pack, materials, camera_inv_vp, width, height, *, frame_seed, frame_i
     env_present = env_radiance is not None and any(v > 0 for v in env_radiance)
     if env_present:
         cdf, pdf = build_constant_env(env_radiance, *env_size)
-        texel_pdf = float(pdf[0])
-        if not np.all(pdf == pdf[0]):
-            texel_pdf = float(pdf.mean())
     else:
-        cdf, texel_pdf = np.zeros(2, np.float32), 0.0
+        cdf, pdf = np.zeros(2, np.float32), np.zeros(1, np.float32)
+    cdf, pdf = np.ascontiguousarray(cdf), np.ascontiguousarray(pdf)
     d = RenderDesc()
     d.scene = C.pointer(ps.struct)
     d.normals = normals.ctypes.data_as(F32P)
@@ -450,7 +450,7 @@ def render(pack, materials, camera_inv_vp, width, height, *, frame_seed, frame_i
     d.env_radiance = (C.c_float * 3)(*(env_radiance or [0, 0, 0]))
     d.env_w, d.env_h = (env_size if env_present else (0, 0))
     d.env_cdf = cdf.ctypes.data_as(F32P)
-    d.env_texel_pdf = texel_pdf
+    d.env_pdf = pdf.ctypes.data_as(F32P)
     d.inv_view_proj = (C.c_float * 16)(*np.asarray(camera_inv_vp, np.float32).ravel())
     d.width, d.height = width, height
     d.frame_seed = frame_seed
@@ -459,7 +459,9 @@ def render(pack, materials, camera_inv_vp, width, height, *, frame_seed, frame_i
     x0, y0, w, h = tile or (0, 0, width, height)
     out = np.zeros((h, w, 3), np.float64)
     sig = np.zeros((h, w), np.uint64)
+    ev = np.zeros((h, w, 40), np.uint64) if events else None
     lib().orc_render(C.byref(d), C.c_uint32(frame_index), x0, y0, w, h,
                      out.ctypes.data_as(F64P), sig.ctypes.data_as(C.POINTER(C.c_uint64)),
+                     ev.ctypes.data_as(C.POINTER(C.c_uint64)) if events else None,
                      C.c_uint32(threads or min(16, os.cpu_count() or 1)))
-    return out, sig
+    return (out, sig, ev) if events else (out, sig)

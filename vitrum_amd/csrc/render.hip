@@ -379,7 +379,7 @@ __global__ void camera_kernel(FrameParams f, PathState s) {
   store3(s.conn + 3 * (size_t)p, f3(0, 0, 0));
   store3(s.radiance + 3 * (size_t)p, f3(0, 0, 0));
   s.prev_pdf[p] = 0.0f;
-  s.flags[p] = 1u;
+  s.flags[p] = ok ? 1u : 0u;  // a failed unprojection ends the path with no radiance
   s.signature[p] = 0x1234ull;
   s.queue[p] = p;
   s.rays[2 * (size_t)p] = make_float4(o.x, o.y, o.z, f.tmin);
@@ -568,9 +568,12 @@ __global__ __launch_bounds__(256) void shade_kernel(FrameParams f, ShadeScene sc
         const bool allows = mpdf > 0.0f && cosine > 1e-8f;
         const F3 C2 = allows ? T * (fr * (cosine / mpdf)) : f3(0, 0, 0);
         // 5. Russian roulette after the connection (kernel.wgsl.ts:2009-2013).
-        bool lives = true;
+        //    A rejected sample leaves zero throughput: the reference keeps
+        //    tracing that path, but all it adds later is multiplied by zero,
+        //    so it ends here.
+        bool lives = allows;
         float rr_mul = 1.0f;
-        if (vertex > 2u) {
+        if (allows && vertex > 2u) {
           const float surv = represented(clampf(fmaxf(C2.x, fmaxf(C2.y, C2.z)), 0.1f, 0.95f));
           lives = rand01(rng) < surv;
           sig = mix64(sig, lives ? 0xC1ull : 0xC0ull);
