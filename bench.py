@@ -61,6 +61,10 @@ def main():
     ap.add_argument("--height", type=int, default=1024)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="run one device-to-device copy of this size first (HBM counter calibration)")
+    ap.add_argument("--refill-min", type=int, default=0, help="traversal wave refill threshold (0: default)")
+    ap.add_argument("--steps-per-refill", type=int, default=0, help="traversal steps between refills (0: default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -70,6 +74,15 @@ def main():
     from vitrum_amd import native, workloads as W
 
     eng = native.Engine(0)
+    if args.refill_min or args.steps_per_refill:
+        eng.set_trace_tuning(args.refill_min or 16, args.steps_per_refill or 48)
+    if args.calibrate_copy_mib:
+        nb = args.calibrate_copy_mib << 20
+        a, b = eng.alloc(nb), eng.alloc(nb)
+        eng.d2d(b, a, nb)
+        eng.free(a)
+        eng.free(b)
+        log(f"calibration copy: {nb} bytes read + {nb} bytes written by copy16_kernel")
     t0 = time.perf_counter()
     sc = W.spheres_scene()
     log(f"scene generated in {time.perf_counter() - t0:.1f} s")
@@ -89,7 +102,8 @@ def main():
     _, counts = eng.trace_closest_counts(rays)
     c = counts.astype(np.float64).sum(axis=0)
     box_tests, tri_tests, tlas_tests, inst_entries = c
-    # Reference layout (SURVEY 32 B per binary node tested, 64 B per triangle):
+    # Reference layout: 32 B per binary node tested, 64 B per triangle tested
+    # (index vec4u + 3 vertex vec4f; shared-bvh/src/strides.ts:24-71).
     ref_bytes = 32.0 * (box_tests + tlas_tests) + 64.0 * tri_tests
     # This kernel's layout: one 64 B Node64 per expansion (two boxes), 48 B per
     # triangle, 80 B per instance entry (W2L + metadata), 32 B ray + 20 B hit.

@@ -213,6 +213,10 @@ vt_status vt_device_alloc(vt_engine *eng, size_t bytes, void **d_ptr);
 vt_status vt_device_free(vt_engine *eng, void *d_ptr);
 vt_status vt_memcpy_h2d(vt_engine *eng, void *d_dst, const void *src, size_t bytes);
 vt_status vt_memcpy_d2h(vt_engine *eng, void *dst, const void *d_src, size_t bytes);
+/* Device-to-device copy on the engine stream by the library's own streaming
+ * kernel (16-byte vector loads and stores; 16-byte aligned pointers and
+ * sizes), so its HBM counters are those of a plain copy of known size. */
+vt_status vt_memcpy_d2d(vt_engine *eng, void *d_dst, const void *d_src, size_t bytes);
 vt_status vt_synchronize(vt_engine *eng);
 /* The engine's hipStream_t, for callers that order their own work with it. */
 vt_status vt_get_stream(vt_engine *eng, void **stream);

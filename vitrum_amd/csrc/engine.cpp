@@ -264,6 +264,17 @@ vt_status vt_memcpy_d2h(vt_engine* eng, void* dst, const void* d_src, size_t byt
   return VT_OK;
 }
 
+vt_status vt_memcpy_d2d(vt_engine* eng, void* d_dst, const void* d_src, size_t bytes) {
+  if (!eng || (!d_dst && bytes) || (!d_src && bytes))
+    return set_err(eng, VT_ERR_INVALID_ARGUMENT, "NULL argument");
+  if ((bytes | (uintptr_t)d_dst | (uintptr_t)d_src) & 15u)
+    return set_err(eng, VT_ERR_INVALID_ARGUMENT, "pointers and size must be multiples of 16 bytes");
+  HIP_TRY(eng, hipSetDevice(eng->device));
+  HIP_TRY(eng, vtk::launch_copy16((uint4*)d_dst, (const uint4*)d_src, bytes / 16, eng->stream));
+  HIP_TRY(eng, hipStreamSynchronize(eng->stream));
+  return VT_OK;
+}
+
 vt_status vt_synchronize(vt_engine* eng) {
   if (!eng) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "engine is NULL");
   HIP_TRY(eng, hipStreamSynchronize(eng->stream));

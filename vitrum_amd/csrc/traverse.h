@@ -41,6 +41,8 @@ struct TraceParams {
 hipError_t launch_trace(const TraceParams& p, bool any, bool count, uint32_t grid, hipStream_t s);
 hipError_t trace_occupancy(bool any, int* blocks_per_cu);
 hipError_t launch_zero(uint32_t* p, uint32_t n, hipStream_t s);
+// Streaming copy of n16 16-byte words.
+hipError_t launch_copy16(uint4* dst, const uint4* src, size_t n16, hipStream_t s);
 hipError_t launch_fp32_selftest(const float* a, const float* b, float* q, float* r, uint32_t n,
                                 hipStream_t s);
 

@@ -545,6 +545,11 @@ __global__ void zero_u32_kernel(uint32_t* p, uint32_t n) {
   if (i < n) p[i] = 0u;
 }
 
+__global__ void copy16_kernel(uint4* __restrict__ dst, const uint4* __restrict__ src, size_t n16) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
 __global__ void fp32_selftest_kernel(const float* a, const float* b, float* q, float* r, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
@@ -584,6 +589,14 @@ hipError_t trace_occupancy(bool any, int* blocks_per_cu) {
 
 hipError_t launch_zero(uint32_t* p, uint32_t n, hipStream_t s) {
   hipLaunchKernelGGL(zero_u32_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy16(uint4* dst, const uint4* src, size_t n16, hipStream_t s) {
+  if (n16 == 0) return hipSuccess;
+  const size_t want = (n16 + 255) / 256;
+  const uint32_t blocks = (uint32_t)(want < 8192 ? want : 8192);
+  hipLaunchKernelGGL(copy16_kernel, dim3(blocks), dim3(256), 0, s, dst, src, n16);
   return hipGetLastError();
 }
 

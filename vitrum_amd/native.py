@@ -27,7 +27,7 @@ EXPORTED = [
     "vt_create", "vt_destroy", "vt_last_error", "vt_set_scene", "vt_trace_closest", "vt_trace_any",
     "vt_trace_closest_device", "vt_trace_any_device", "vt_set_counting", "vt_trace_closest_counts",
     "vt_set_trace_tuning", "vt_selftest_fp32", "vt_device_alloc", "vt_device_free",
-    "vt_memcpy_h2d", "vt_memcpy_d2h", "vt_synchronize", "vt_get_stream", "vt_stats",
+    "vt_memcpy_h2d", "vt_memcpy_d2h", "vt_memcpy_d2d", "vt_synchronize", "vt_get_stream", "vt_stats",
     "vt_build_blas", "vt_build_tlas", "vt_pack_scene", "vt_pack_field_words",
     "vt_pack_copy_field", "vt_pack_free", "vt_version", "vt_render_frame", "vt_capture",
     "vt_reset", "vt_frame_time", "vt_capture_signatures", "vt_render_info",
@@ -102,6 +102,7 @@ def lib():
         L.vt_device_free.argtypes = [C.c_void_p, C.c_void_p]
         L.vt_memcpy_h2d.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
         L.vt_memcpy_d2h.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
+        L.vt_memcpy_d2d.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
         for name in ("vt_trace_closest", "vt_trace_any", "vt_trace_closest_device",
                      "vt_trace_any_device"):
             getattr(L, name).argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]
@@ -329,6 +330,9 @@ class Engine:
     def h2d(self, dptr, arr):
         arr = np.ascontiguousarray(arr)
         check(lib().vt_memcpy_h2d(self._h, C.c_void_p(dptr), arr.ctypes.data, arr.nbytes), self._h)
+
+    def d2d(self, dst, src, nbytes):
+        check(lib().vt_memcpy_d2d(self._h, C.c_void_p(dst), C.c_void_p(src), C.c_size_t(nbytes)), self._h)
 
     def d2h(self, arr, dptr):
         check(lib().vt_memcpy_d2h(self._h, arr.ctypes.data, C.c_void_p(dptr), arr.nbytes), self._h)
