@@ -197,9 +197,11 @@ vt_status vt_trace_closest_counts(vt_engine *eng, const vt_ray *rays, uint32_t n
                                   uint32_t *counts);
 
 /* Traversal scheduling knobs (no counterpart in the reference): a wave is
- * refilled once `refill_min` (1..64) of its lanes are idle, and lanes run up
- * to `steps` traversal steps between refill checks. */
-vt_status vt_set_trace_tuning(vt_engine *eng, uint32_t refill_min, uint32_t steps);
+ * refilled once `refill_min` (1..64) of its lanes are idle; each round of the
+ * traversal loop runs up to `steps` node iterations and ends its node phase
+ * early once fewer than `interior_min` (1..64) lanes still have node work. */
+vt_status vt_set_trace_tuning(vt_engine *eng, uint32_t refill_min, uint32_t steps,
+                              uint32_t interior_min);
 
 /* Test hook (no counterpart in the reference): q = a / b and r = sqrt(a)
  * computed on the device, to check that the device's binary32 divide and

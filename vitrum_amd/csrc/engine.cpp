@@ -336,6 +336,7 @@ static vt_status trace_device(vt_engine* eng, const vt_ray* d_rays, uint32_t n, 
   p.spill = (uint2*)eng->spill.ptr;
   p.refill_min = eng->refill_min;
   p.steps = eng->trace_steps;
+  p.interior_min = eng->interior_min;
   eng->last_ray_count = n;
   if (n == 0) return VT_OK;
   HIP_TRY(eng, hipEventRecord(eng->ev_start, eng->stream));
@@ -408,12 +409,15 @@ vt_status vt_trace_any_device(vt_engine* eng, const vt_ray* d_rays, uint32_t n, 
   return trace_device(eng, d_rays, n, d_occluded, true, nullptr);
 }
 
-vt_status vt_set_trace_tuning(vt_engine* eng, uint32_t refill_min, uint32_t steps) {
+vt_status vt_set_trace_tuning(vt_engine* eng, uint32_t refill_min, uint32_t steps,
+                              uint32_t interior_min) {
   if (!eng) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "engine is NULL");
-  if (refill_min < 1 || refill_min > 64 || steps < 1)
-    return set_err(eng, VT_ERR_RANGE, "refill_min must be in [1, 64] and steps >= 1");
+  if (refill_min < 1 || refill_min > 64 || steps < 1 || interior_min < 1 || interior_min > 64)
+    return set_err(eng, VT_ERR_RANGE,
+                   "refill_min and interior_min must be in [1, 64] and steps >= 1");
   eng->refill_min = refill_min;
   eng->trace_steps = steps;
+  eng->interior_min = interior_min;
   return VT_OK;
 }
 

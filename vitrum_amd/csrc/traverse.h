@@ -34,8 +34,9 @@ struct TraceParams {
   uint32_t* counter;   // work counter, zeroed by the launcher
   uint32_t* error;     // set to non-zero on stack overflow
   uint2* spill;        // (kStackCapacity - kLdsDepth) x grid threads
-  uint32_t refill_min; // refill a wave once this many lanes are idle
-  uint32_t steps;      // traversal steps between refill checks
+  uint32_t refill_min;    // refill a wave once this many lanes are idle
+  uint32_t steps;         // most node/pop iterations per round
+  uint32_t interior_min;  // a round's node phase ends when fewer lanes have node work
 };
 
 hipError_t launch_trace(const TraceParams& p, bool any, bool count, uint32_t grid, hipStream_t s);

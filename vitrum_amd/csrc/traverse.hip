@@ -307,6 +307,21 @@ struct LaneStack {
   }
 };
 
+// Lane modes of the phased traversal loop.
+constexpr uint32_t kModeIdle = 0;   // no ray
+constexpr uint32_t kModeNode = 1;   // pop / expand interior nodes at the current level
+constexpr uint32_t kModeLeaf = 2;   // a BLAS leaf's triangles are pending
+constexpr uint32_t kModeEnter = 3;  // enter instance `slot`
+constexpr uint32_t kModeExit = 4;   // the current instance's BLAS is exhausted
+constexpr uint32_t kModeDone = 5;   // write the result
+
+// One round of the loop runs four wave-uniform phases: (1) up to `steps`
+// node iterations (a pop with the pop-time cull, then one Node64 expansion)
+// until fewer than `interior_min` lanes still have node work; (2) the pending
+// BLAS leaves' triangles; (3) instance exit and entry; (4) results.  The
+// expensive phases run once for every lane that reached them instead of
+// once per node step, and each lane still performs exactly the reference's
+// sequence of node, triangle and instance operations.
 template <bool ANY, bool COUNT, uint32_t BLOCK, uint32_t LDS_DEPTH>
 __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceParams p) {
   __shared__ uint2 lds_stack[LDS_DEPTH * BLOCK];
@@ -320,7 +335,8 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceParams p) {
   const uint64_t lane_below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
   const uint32_t n_rays = p.n_ptr ? *p.n_ptr : p.n;
-  bool active = false;
+  uint32_t mode = kModeIdle;
+  bool occluded = false;
   uint32_t ray_idx = 0;
   // World ray and best hit.
   V3 ro{}, rd{}, rinv{};
@@ -333,93 +349,19 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceParams p) {
   float ltmin = 0, lbest = 0, lu = 0, lv = 0;
   uint32_t lprim = 0, inst_id = 0;
   bool lhit = false;
-  uint32_t slot = 0, slot_end = 0;
+  uint32_t slot = 0, slot_end = 0, leaf_first = 0, leaf_count = 0;
   uint32_t level = 0, cur = kNone, sp = 0, blas_base = 0;
   uint32_t c_blas = 0, c_tri = 0, c_tlas = 0, c_inst = 0;
 
-  auto enter_instance = [&](uint32_t s) {
-    const float4* ir = insts + 9 * (size_t)s;
-    const float4 c0 = ir[0], c1 = ir[1], c2 = ir[2], c3 = ir[3];
-    const uint4 meta = *reinterpret_cast<const uint4*>(ir + 8);
-    lo = xform_point(c0, c1, c2, c3, ro);
-    ld = xform_dir(c0, c1, c2, rd);
-    linv = v3(safe_inv(ld.x), safe_inv(ld.y), safe_inv(ld.z));
-    lw = wt_setup(ld);
-    const V3 start = v3(ro.x + rd.x * tmin, ro.y + rd.y * tmin, ro.z + rd.z * tmin);
-    ltmin = fmaxf(dot3(sub(xform_point(c0, c1, c2, c3, start), lo), ld), 0.0f);
-    // Closest: localTMax starts at INFINITY; any-hit: at the ray's tMax
-    // (tlasSceneHitTraversal.wgsl.ts:103-108 and :168-173).
-    const float bound = ANY ? tmax : best;
-    float ltmax = ANY ? tmax : inf_const();
-    if (bound < inf_const() * 0.5f) {
-      const V3 end = v3(ro.x + rd.x * bound, ro.y + rd.y * bound, ro.z + rd.z * bound);
-      ltmax = fmaxf(dot3(sub(xform_point(c0, c1, c2, c3, end), lo), ld), ltmin);
-    }
-    lbest = ltmax;
-    lhit = false;
-    inst_id = meta.y;
-    level = 1;
-    blas_base = sp;
-    cur = meta.x;
-    if (COUNT) ++c_inst;
-  };
-
-  auto finalize_instance = [&]() {
-    if (!ANY && lhit && lbest > ltmin) {
-      const float4* ir = insts + 9 * (size_t)slot + 4;
-      const V3 w = xform_point(ir[0], ir[1], ir[2], ir[3], lpos);
-      const float wd = dot3(sub(w, ro), rd);
-      if (wd > tmin && wd < best) {
-        best = wd;
-        bpos = w;
-        bprim = lprim;
-        binst = inst_id;
-        bu = lu;
-        bv = lv;
-      }
-    }
-  };
-
-  auto finish = [&](bool occluded) {
-    if (ANY) {
-      p.occluded[ray_idx] = occluded ? 1u : 0u;
-    } else {
-      const bool hit = binst != kMiss;
-      uint32_t* h = p.hits + 5 * (size_t)ray_idx;
-      h[0] = hit ? bprim : kMiss;
-      h[1] = hit ? binst : kMiss;
-      h[2] = hit ? f2u(best) : 0x7F800000u;
-      h[3] = hit ? f2u(bu) : 0u;
-      h[4] = hit ? f2u(bv) : 0u;
-      if (p.hit_pos) {
-        float* hp = p.hit_pos + 3 * (size_t)ray_idx;
-        hp[0] = bpos.x;
-        hp[1] = bpos.y;
-        hp[2] = bpos.z;
-      }
-    }
-    if (COUNT) p.counts[ray_idx] = make_uint4(c_blas, c_tri, c_tlas, c_inst);
-    active = false;
-  };
-
-  auto push = [&](uint32_t ref, float tn) {
-    if (sp >= kStackCapacity) {
-      atomicOr(p.error, 1u);
-      return false;
-    }
-    stk.store(sp++, make_uint2(ref, f2u(tn)));
-    return true;
-  };
-
   for (;;) {
-    // ---- wave-level refill: ballot idle lanes, one atomic, prefix ranks ----
-    const uint64_t idle = __ballot(!active);
+    // ---- refill: ballot idle lanes, one atomic, prefix ranks ----
+    const uint64_t idle = __ballot(mode == kModeIdle);
     const uint32_t n_idle = (uint32_t)__popcll(idle);
     if (n_idle >= p.refill_min || idle == __ballot(1)) {
       uint32_t base = 0;
       if (lane == 0) base = atomicAdd(p.counter, n_idle);
       base = __shfl(base, 0);
-      if (!active) {
+      if (mode == kModeIdle) {
         const uint32_t idx = base + (uint32_t)__popcll(idle & lane_below);
         if (idx < n_rays) {
           ray_idx = idx;
@@ -433,108 +375,185 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceParams p) {
           binst = kMiss;
           bprim = 0;
           bu = bv = 0;
+          occluded = false;
           level = 0;
           sp = 0;
           cur = p.scene.tlas_node_count ? p.scene.tlas_root : kNone;
           if (COUNT) c_blas = c_tri = c_tlas = c_inst = 0;
-          active = true;
-          if (cur == kNone) finish(false);  // no TLAS: fail closed
+          mode = cur == kNone ? kModeDone : kModeNode;  // no TLAS: fail closed
         }
       }
     }
-    if (!__any(active)) break;
+    if (!__any(mode != kModeIdle)) break;
 
-    for (uint32_t step = 0; active && step < p.steps; ++step) {
+    // ---- (1) node phase ----
+    for (uint32_t k = 0; k < p.steps; ++k) {
+      const uint32_t n_node = (uint32_t)__popcll(__ballot(mode == kModeNode));
+      if (n_node == 0 || (k > 0 && n_node < p.interior_min)) break;
+      if (mode != kModeNode) continue;
       if (cur == kNone) {
-        // ---- pop with the reference's pop-time cull; unwind levels ----
-        for (;;) {
-          if (level == 1) {
-            if (sp > blas_base) {
-              const uint2 e = stk.load(--sp);
-              if (!(u2f(e.y) > lbest)) {
-                cur = e.x;
-                break;
-              }
-              continue;
-            }
-            finalize_instance();
-            level = 0;
-            if (++slot < slot_end) {
-              enter_instance(slot);
-              break;
-            }
-            continue;
-          }
-          if (sp > 0) {
+        // Pop with the reference's pop-time cull.
+        if (level == 1) {
+          if (sp > blas_base) {
             const uint2 e = stk.load(--sp);
-            if (!(u2f(e.y) > (ANY ? tmax : best))) {
-              cur = e.x;
-              break;
-            }
+            if (!(u2f(e.y) > lbest)) cur = e.x;
+          } else {
+            mode = kModeExit;
+          }
+        } else if (sp > 0) {
+          const uint2 e = stk.load(--sp);
+          if (!(u2f(e.y) > (ANY ? tmax : best))) cur = e.x;
+        } else {
+          mode = kModeDone;
+        }
+        if (cur == kNone) continue;
+      }
+      if (cur & kLeafBit) {
+        const uint32_t count = (cur >> 27) & 15u, first = cur & 0x07FFFFFFu;
+        cur = kNone;
+        if (level == 0) {
+          slot = first;
+          slot_end = first + count;
+          mode = count ? kModeEnter : kModeNode;
+        } else {
+          leaf_first = first;
+          leaf_count = count;
+          mode = kModeLeaf;
+        }
+        continue;
+      }
+      // Interior: one Node64 load, two child boxes.
+      const float4* nd = (level ? blas : tlas) + 4 * (size_t)cur;
+      const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2], q3 = nd[3];
+      const uint32_t ref_l = f2u(q3.x), ref_r = f2u(q3.y);
+      const V3 o = level ? lo : ro;
+      const V3 inv = level ? linv : rinv;
+      const float tm = level ? ltmin : tmin;
+      const float bound = level ? lbest : (ANY ? tmax : best);
+      float tn_l, tn_r;
+      const bool pass_l = ref_l != kEmpty && slab(o, inv, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, tm, tn_l);
+      const bool pass_r = ref_r != kEmpty && slab(o, inv, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, tm, tn_r);
+      if (COUNT) {
+        const uint32_t tested = (ref_l != kEmpty) + (ref_r != kEmpty);
+        if (level) c_blas += tested; else c_tlas += tested;
+      }
+      if (pass_l && !(tn_l > bound)) {
+        if (pass_r) {
+          if (sp >= kStackCapacity) {
+            atomicOr(p.error, 1u);
+            mode = kModeDone;
             continue;
           }
-          finish(false);
-          break;
+          stk.store(sp++, make_uint2(ref_r, f2u(tn_r)));
         }
-        if (!active) break;
+        cur = ref_l;
+      } else if (pass_r && !(tn_r > bound)) {
+        cur = ref_r;
+      } else {
+        cur = kNone;
       }
-      if ((cur & kLeafBit) == 0) {
-        // ---- interior: one Node64 load, two child boxes ----
-        const float4* nd = (level ? blas : tlas) + 4 * (size_t)cur;
-        const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2], q3 = nd[3];
-        const uint32_t ref_l = f2u(q3.x), ref_r = f2u(q3.y);
-        const V3 o = level ? lo : ro;
-        const V3 inv = level ? linv : rinv;
-        const float tm = level ? ltmin : tmin;
-        const float bound = level ? lbest : (ANY ? tmax : best);
-        float tn_l, tn_r;
-        const bool pass_l = ref_l != kEmpty && slab(o, inv, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, tm, tn_l);
-        const bool pass_r = ref_r != kEmpty && slab(o, inv, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, tm, tn_r);
-        if (COUNT) {
-          const uint32_t tested = (ref_l != kEmpty) + (ref_r != kEmpty);
-          if (level) c_blas += tested; else c_tlas += tested;
-        }
-        if (pass_l && !(tn_l > bound)) {
-          if (pass_r && !push(ref_r, tn_r)) {
-            finish(false);
-            break;
+    }
+
+    // ---- (2) leaf phase: the pending BLAS leaf's triangles, in order ----
+    if (__any(mode == kModeLeaf)) {
+      if (mode == kModeLeaf) {
+        mode = kModeNode;
+        for (uint32_t i = 0; i < leaf_count; ++i) {
+          const uint32_t t = leaf_first + i;
+          const float4* tr = tris + 3 * (size_t)t;
+          const float4 w0 = tr[0], w1 = tr[1], w2 = tr[2];
+          if (COUNT) ++c_tri;
+          TriHit h;
+          if (tri_test(lo, ld, lw, v3(w0.x, w0.y, w0.z), v3(w0.w, w1.x, w1.y), v3(w1.z, w1.w, w2.x),
+                       ltmin, lbest, h)) {
+            if (ANY) {
+              occluded = true;
+              mode = kModeDone;
+              break;
+            }
+            lhit = true;
+            lbest = h.t;
+            lprim = t;
+            lu = h.u;
+            lv = h.v;
+            lpos = h.point;
           }
-          cur = ref_l;
-        } else if (pass_r && !(tn_r > bound)) {
-          cur = ref_r;
+        }
+      }
+    }
+
+    // ---- (3) instance exit (closest: merge the local hit) and entry ----
+    if (__any(mode == kModeExit)) {
+      if (mode == kModeExit) {
+        if (!ANY && lhit && lbest > ltmin) {
+          const float4* ir = insts + 9 * (size_t)slot + 4;
+          const V3 w = xform_point(ir[0], ir[1], ir[2], ir[3], lpos);
+          const float wd = dot3(sub(w, ro), rd);
+          if (wd > tmin && wd < best) {
+            best = wd;
+            bpos = w;
+            bprim = lprim;
+            binst = inst_id;
+            bu = lu;
+            bv = lv;
+          }
+        }
+        level = 0;
+        mode = ++slot < slot_end ? kModeEnter : kModeNode;
+      }
+    }
+    if (__any(mode == kModeEnter)) {
+      if (mode == kModeEnter) {
+        const float4* ir = insts + 9 * (size_t)slot;
+        const float4 c0 = ir[0], c1 = ir[1], c2 = ir[2], c3 = ir[3];
+        const uint4 meta = *reinterpret_cast<const uint4*>(ir + 8);
+        lo = xform_point(c0, c1, c2, c3, ro);
+        ld = xform_dir(c0, c1, c2, rd);
+        linv = v3(safe_inv(ld.x), safe_inv(ld.y), safe_inv(ld.z));
+        lw = wt_setup(ld);
+        const V3 start = v3(ro.x + rd.x * tmin, ro.y + rd.y * tmin, ro.z + rd.z * tmin);
+        ltmin = fmaxf(dot3(sub(xform_point(c0, c1, c2, c3, start), lo), ld), 0.0f);
+        // Closest: localTMax starts at INFINITY; any-hit: at the ray's tMax
+        // (tlasSceneHitTraversal.wgsl.ts:103-108 and :168-173).
+        const float bnd = ANY ? tmax : best;
+        float ltmax = ANY ? tmax : inf_const();
+        if (bnd < inf_const() * 0.5f) {
+          const V3 end = v3(ro.x + rd.x * bnd, ro.y + rd.y * bnd, ro.z + rd.z * bnd);
+          ltmax = fmaxf(dot3(sub(xform_point(c0, c1, c2, c3, end), lo), ld), ltmin);
+        }
+        lbest = ltmax;
+        lhit = false;
+        inst_id = meta.y;
+        level = 1;
+        blas_base = sp;
+        cur = meta.x;
+        if (COUNT) ++c_inst;
+        mode = kModeNode;
+      }
+    }
+
+    // ---- (4) results ----
+    if (__any(mode == kModeDone)) {
+      if (mode == kModeDone) {
+        if (ANY) {
+          p.occluded[ray_idx] = occluded ? 1u : 0u;
         } else {
-          cur = kNone;
-        }
-        continue;
-      }
-      // ---- leaf ----
-      const uint32_t count = (cur >> 27) & 15u, first = cur & 0x07FFFFFFu;
-      cur = kNone;
-      if (level == 0) {
-        slot = first;
-        slot_end = first + count;
-        enter_instance(slot);
-        continue;
-      }
-      for (uint32_t i = 0; i < count; ++i) {
-        const uint32_t t = first + i;
-        const float4* tr = tris + 3 * (size_t)t;
-        const float4 w0 = tr[0], w1 = tr[1], w2 = tr[2];
-        if (COUNT) ++c_tri;
-        TriHit h;
-        if (tri_test(lo, ld, lw, v3(w0.x, w0.y, w0.z), v3(w0.w, w1.x, w1.y), v3(w1.z, w1.w, w2.x), ltmin,
-                     lbest, h)) {
-          if (ANY) {
-            finish(true);
-            break;
+          const bool hit = binst != kMiss;
+          uint32_t* h = p.hits + 5 * (size_t)ray_idx;
+          h[0] = hit ? bprim : kMiss;
+          h[1] = hit ? binst : kMiss;
+          h[2] = hit ? f2u(best) : 0x7F800000u;
+          h[3] = hit ? f2u(bu) : 0u;
+          h[4] = hit ? f2u(bv) : 0u;
+          if (p.hit_pos) {
+            float* hp = p.hit_pos + 3 * (size_t)ray_idx;
+            hp[0] = bpos.x;
+            hp[1] = bpos.y;
+            hp[2] = bpos.z;
           }
-          lhit = true;
-          lbest = h.t;
-          lprim = t;
-          lu = h.u;
-          lv = h.v;
-          lpos = h.point;
         }
+        if (COUNT) p.counts[ray_idx] = make_uint4(c_blas, c_tri, c_tlas, c_inst);
+        mode = kModeIdle;
       }
     }
   }
