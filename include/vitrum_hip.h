@@ -273,6 +273,13 @@ int64_t vt_pack_field_words(const vt_pack *p, int32_t field);
 vt_status vt_pack_copy_field(const vt_pack *p, int32_t field, void *dst, uint64_t dst_words);
 void vt_pack_free(vt_pack *p);
 
+/* invertMat4 (packages/shared-bvh/src/mathUtils.ts:21-110): binary64
+ * cofactor inverse of a column-major mat4 rounded to binary32, with the
+ * reciprocal-product check.  VT_ERR_RANGE when the matrix is not invertible.
+ * The frame driver uses it for inverse(projMatrix * viewMatrix)
+ * (packages/pt-webgpu/src/frameParamsPacker.ts:117-118). */
+vt_status vt_invert_mat4(const float m[16], float out[16]);
+
 /* Library version string, e.g. "vitrum-hip 0.1.0 gfx950". */
 const char *vt_version(void);
 

@@ -114,6 +114,12 @@ void SceneBuffers::release() {
 
 extern "C" {
 
+vt_status vt_invert_mat4(const float m[16], float out[16]) {
+  if (!m || !out) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!vt::invert_mat4(m, out)) return set_err(nullptr, VT_ERR_RANGE, "matrix is not invertible");
+  return VT_OK;
+}
+
 const char* vt_version(void) { return "vitrum-hip 0.1.0 gfx950"; }
 
 vt_status vt_last_error(const vt_engine* eng, char* buf, size_t len) {

@@ -1,0 +1,358 @@
+'use strict';
+// pt-hip: the MI355X path-tracing backend behind vitrum's Engine contract.
+//
+// createPTEngine_HIP(opts) is the EngineFactory of this backend (the pt-webgpu
+// one is createPTEngine_WebGPU, packages/pt-webgpu/src/index.ts:2363), and
+// constructPathTracerHipForDispatch has the BackendConstructor signature of
+// packages/engine/src/createEngineInternals.ts:452 so it can be registered in
+// BACKEND_CONSTRUCTORS (packages/engine/src/createEngine.ts:99-103).
+//
+// The returned object implements packages/core/src/engine/index.ts:109-486:
+// state, capabilities, setScene, getScene, renderFrame, reset, pause, resume,
+// dispose, onFrame/onProgress/onError/onWarning and captureFrame.  All device
+// work goes through the N-API addon (../napi/vitrum_hip.node) into
+// libvthip.so; there is no JavaScript fallback renderer.
+
+const path = require('path');
+
+let addon = null;
+function loadAddon() {
+  if (addon) return addon;
+  const file = path.join(__dirname, '..', 'napi', 'vitrum_hip.node');
+  try {
+    addon = require(file);
+  } catch (err) {
+    const e = new Error(`pt-hip: native addon could not be loaded (${file}): ${err.message}`);
+    e.code = 'VT_ERR_NO_ADDON';
+    throw e;
+  }
+  return addon;
+}
+
+// multiplyMat4 (packages/pt-webgpu/src/math/mat4.ts:4-18): column-major
+// a * b, each element summed in binary64 and stored as binary32.
+function multiplyMat4(a, b) {
+  const out = new Float32Array(16);
+  for (let col = 0; col < 4; col += 1) {
+    for (let row = 0; row < 4; row += 1) {
+      out[col * 4 + row] =
+        (a[row] || 0) * (b[col * 4] || 0) +
+        (a[4 + row] || 0) * (b[col * 4 + 1] || 0) +
+        (a[8 + row] || 0) * (b[col * 4 + 2] || 0) +
+        (a[12 + row] || 0) * (b[col * 4 + 3] || 0);
+    }
+  }
+  return out;
+}
+
+function isMat4(m) {
+  return m != null && typeof m.length === 'number' && m.length === 16;
+}
+
+function toF32(v, label) {
+  if (v instanceof Float32Array) return v;
+  if (v == null || typeof v.length !== 'number') throw new TypeError(`${label} must be a Float32Array or array`);
+  return Float32Array.from(v);
+}
+
+function toU32(v, label) {
+  if (v == null) return null;
+  if (v instanceof Uint32Array) return v;
+  if (v instanceof Uint16Array || Array.isArray(v)) return Uint32Array.from(v);
+  throw new TypeError(`${label} must be a Uint32Array, Uint16Array or array`);
+}
+
+// Constant-radiance environments: an HDRI whose texels are all equal (the
+// importance table of such a map is what the device samples).
+function environmentOf(env) {
+  if (env == null || env.kind === 'none') return { kind: 0, radiance: [0, 0, 0] };
+  if (env.kind !== 'hdri') {
+    throw new RangeError(`pt-hip renders 'none' and constant 'hdri' environments, not '${env.kind}'`);
+  }
+  const map = env.hdri || {};
+  const intensity = env.intensity == null ? 1 : env.intensity;
+  if (!Number.isFinite(intensity) || intensity < 0) throw new RangeError('environment.intensity must be finite and >= 0');
+  if (env.rotationY && env.rotationY !== 0) {
+    // A constant map is rotation invariant.
+  }
+  let rgb;
+  if (Array.isArray(map.constant) || map.constant instanceof Float32Array) {
+    rgb = [map.constant[0], map.constant[1], map.constant[2]];
+  } else if (map.data && map.width > 0 && map.height > 0) {
+    const d = map.data;
+    const ch = d.length / (map.width * map.height);
+    if (ch !== 3 && ch !== 4) throw new TypeError('environment hdri data must be RGB or RGBA');
+    rgb = [d[0], d[1], d[2]];
+    for (let i = 0; i < d.length; i += ch) {
+      if (d[i] !== rgb[0] || d[i + 1] !== rgb[1] || d[i + 2] !== rgb[2]) {
+        throw new RangeError('pt-hip renders constant-radiance environment maps only');
+      }
+    }
+  } else {
+    throw new TypeError('environment.hdri must carry {constant:[r,g,b]} or {width,height,data}');
+  }
+  const radiance = rgb.map((c) => c * intensity);
+  if (!radiance.every((c) => Number.isFinite(c) && c >= 0)) throw new RangeError('environment radiance must be finite and >= 0');
+  return { kind: radiance.some((c) => c > 0) ? 1 : 0, radiance };
+}
+
+// Scene (packages/core/src/scene/index.ts:42) -> the addon's scene blob.
+function sceneBlob(scene, opts) {
+  if (scene == null || !Array.isArray(scene.primitives)) throw new TypeError('scene.primitives must be an array');
+  const primitives = [];
+  const materials = [];
+  scene.primitives.forEach((p, i) => {
+    if (p.kind !== 'mesh' && p.kind !== 'instanced-mesh') {
+      throw new RangeError(`pt-hip supports 'mesh' and 'instanced-mesh' primitives, not '${p.kind}' (primitive ${i})`);
+    }
+    const m = p.material || {};
+    materials.push({
+      baseColor: m.baseColor || [0.8, 0.8, 0.8],
+      roughness: m.roughness == null ? 0.5 : m.roughness,
+      metallic: m.metallic == null ? 0 : m.metallic,
+      emissive: m.emissive || [0, 0, 0],
+      emissiveIntensity: m.emissiveIntensity == null ? 1 : m.emissiveIntensity,
+      doubleSided: !!m.doubleSided,
+    });
+    let transforms = null;
+    if (p.kind === 'mesh') {
+      if (p.transform != null) {
+        if (!isMat4(p.transform)) throw new TypeError(`primitive ${i}: transform must be a mat4`);
+        transforms = toF32(p.transform, 'transform');
+      }
+    } else {
+      if (!Array.isArray(p.instances)) throw new TypeError(`primitive ${i}: instances must be an array of mat4`);
+      transforms = new Float32Array(16 * p.instances.length);
+      p.instances.forEach((t, k) => {
+        if (!isMat4(t)) throw new TypeError(`primitive ${i}: instance ${k} must be a mat4`);
+        transforms.set(t, 16 * k);
+      });
+    }
+    primitives.push({
+      kind: p.kind === 'mesh' ? 0 : 1,
+      material: i,
+      positions: toF32(p.positions, 'positions'),
+      normals: p.normals == null ? null : toF32(p.normals, 'normals'),
+      indices: toU32(p.indices, 'indices'),
+      transforms,
+    });
+  });
+  return {
+    primitives,
+    materials,
+    environment: environmentOf(scene.environment),
+    sceneRadius: opts && opts.sceneRadius ? opts.sceneRadius : 0,
+  };
+}
+
+const SKIPPED = Object.freeze({ kind: 'skipped', samplesAccumulated: 0, isConverged: false });
+
+class HipPathTracerEngine {
+  constructor(native, handle, opts) {
+    this._n = native;
+    this._h = handle;
+    this._opts = opts;
+    this._scene = null;
+    this._size = null;
+    this._samples = 0;
+    this._listeners = { frame: new Set(), progress: new Set(), error: new Set(), warning: new Set() };
+    this.state = 'ready';
+    this.backendId = 'pt-hip';
+    this.capabilities = Object.freeze({
+      supportsIncrementalScene: false,
+      supportsAddRemovePrimitive: false,
+      supportsAuxBuffers: false,
+      accumulates: true,
+      maxSamplesPerPixel: opts.maxSamplesPerPixel || Infinity,
+    });
+  }
+
+  _live() {
+    if (this.state === 'disposed') throw new Error('pt-hip: engine has been disposed');
+    if (this.state === 'error') throw new Error('pt-hip: engine is in the error state; dispose and recreate it');
+  }
+
+  _emit(kind, value) {
+    for (const cb of this._listeners[kind]) {
+      try {
+        cb(value);
+      } catch (_) {
+        // listener failures never reach the renderer
+      }
+    }
+  }
+
+  _fail(err) {
+    const fatal = err && (err.code === 'VT_ERR_HIP' || err.code === 'VT_ERR_NO_DEVICE');
+    if (fatal) this.state = 'error';
+    this._emit('error', { kind: fatal ? 'gpu-internal' : 'render', message: String(err && err.message), fatal, raw: err });
+  }
+
+  // Engine.setScene (core/src/engine/index.ts:120): validates and converts
+  // before any state changes; the previous scene stays bound on failure.
+  setScene(scene) {
+    this._live();
+    const blob = sceneBlob(scene, this._opts);
+    if (scene.emitters && scene.emitters.length) {
+      this._emit('warning', {
+        code: 'unsupported-emitter', backend: 'pt-hip', phase: 'setScene',
+        message: 'pt-hip lights scenes with emissive meshes and the environment; scene.emitters are ignored',
+      });
+    }
+    this._n.setScene(this._h, blob);
+    this._scene = scene;
+    this._samples = 0;
+  }
+
+  getScene() {
+    return this._scene;
+  }
+
+  // Engine.renderFrame (core/src/engine/index.ts:251): one sample per pixel,
+  // accumulated into the running mean until quality.samplesTarget.
+  renderFrame(input) {
+    this._live();
+    if (this.state === 'paused' || this._scene == null) return SKIPPED;
+    if (input == null || !isMat4(input.viewMatrix) || !isMat4(input.projMatrix)) {
+      throw new TypeError('FrameInput.viewMatrix and projMatrix must be mat4s');
+    }
+    const vp = input.viewport || {};
+    const width = vp.width | 0, height = vp.height | 0;
+    if (!(width > 0 && height > 0)) throw new RangeError('FrameInput.viewport must have positive width and height');
+    const inv = this._n.invertMat4(multiplyMat4(input.projMatrix, input.viewMatrix));
+    if (inv == null) throw new RangeError('projMatrix * viewMatrix is not invertible');
+    const q = input.quality || {};
+    const target = q.samplesTarget == null ? 0 : q.samplesTarget;
+    if (this._size && (this._size[0] !== width || this._size[1] !== height)) this._samples = 0;
+    this._size = [width, height];
+    if (target && this._samples >= target) {
+      const out = { kind: 'rendered', primaryRadiance: this._texture(), samplesAccumulated: this._samples, isConverged: true };
+      return out;
+    }
+    try {
+      this._n.renderFrame(this._h, {
+        invViewProj: inv,
+        width,
+        height,
+        frameIndex: input.frameIndex >>> 0,
+        frameSeed: input.frameSeed >>> 0,
+        maxBounces: q.bounces == null ? 0 : q.bounces,
+        samplesTarget: target,
+      });
+    } catch (err) {
+      this._fail(err);
+      throw err;
+    }
+    this._samples += 1;
+    const out = {
+      kind: 'rendered',
+      primaryRadiance: this._texture(),
+      samplesAccumulated: this._samples,
+      isConverged: target > 0 && this._samples >= target,
+    };
+    this._emit('frame', out);
+    this._emit('progress', { samplesAccumulated: this._samples, samplesTarget: target });
+    return out;
+  }
+
+  // BackendTexture branded 'hip': the engine's accumulation image.
+  _texture() {
+    return Object.freeze({ backend: 'hip', engine: this, width: this._size[0], height: this._size[1], format: 'rgba32float' });
+  }
+
+  reset() {
+    this._live();
+    this._n.reset(this._h);
+    this._samples = 0;
+  }
+
+  pause() {
+    if (this.state === 'ready') this.state = 'paused';
+  }
+
+  resume() {
+    if (this.state === 'paused') this.state = 'ready';
+  }
+
+  dispose() {
+    if (this.state === 'disposed') return;
+    this._n.destroy(this._h);
+    this._h = null;
+    this.state = 'disposed';
+    for (const k of Object.keys(this._listeners)) this._listeners[k].clear();
+  }
+
+  _on(kind, cb) {
+    if (typeof cb !== 'function') throw new TypeError('listener must be a function');
+    this._listeners[kind].add(cb);
+    return () => this._listeners[kind].delete(cb);
+  }
+
+  onFrame(cb) { return this._on('frame', cb); }
+  onProgress(cb) { return this._on('progress', cb); }
+  onError(cb) { return this._on('error', cb); }
+  onWarning(cb) { return this._on('warning', cb); }
+
+  // captureFrame (core/src/engine/index.ts:466): row-major, top-left origin,
+  // linear RGBA float32 running mean.
+  captureFrame(opts) {
+    return new Promise((resolve, reject) => {
+      try {
+        this._live();
+        if (opts && opts.colorSpace && opts.colorSpace !== 'linear') {
+          throw new RangeError("pt-hip captures the linear accumulation only (colorSpace 'linear')");
+        }
+        if (!this._size || this._samples === 0) {
+          resolve(null);
+          return;
+        }
+        const [w, h] = this._size;
+        resolve({ width: w, height: h, rgba: this._n.capture(this._h, w, h) });
+      } catch (err) {
+        reject(err);
+      }
+    });
+  }
+
+  stats() {
+    this._live();
+    return this._n.stats(this._h);
+  }
+}
+
+// EngineFactory (packages/core/src/engine/factory.ts): opts.device is the HIP
+// device ordinal chosen by the host (default 0).
+async function createPTEngine_HIP(opts) {
+  const o = Object.assign({}, opts || {});
+  const known = new Set(['device', 'maxBounces', 'maxSamplesPerPixel', 'sampling', 'sceneRadius', 'debug',
+    'onWarning', 'denoiser', 'extensions']);
+  for (const k of Object.keys(o)) {
+    if (!known.has(k)) throw new TypeError(`pt-hip: unknown option '${k}'`);
+  }
+  if (o.denoiser && o.denoiser !== 'none') throw new RangeError("pt-hip has no denoiser (use denoiser: 'none')");
+  const native = loadAddon();
+  const handle = native.create(o.device == null ? 0 : o.device, {
+    maxBounces: o.maxBounces || 0,
+    maxSamplesPerPixel: Number.isFinite(o.maxSamplesPerPixel) ? o.maxSamplesPerPixel : 0,
+    sampling: o.sampling === 'sobol' ? 1 : 0,
+  });
+  const engine = new HipPathTracerEngine(native, handle, o);
+  if (typeof o.onWarning === 'function') engine.onWarning(o.onWarning);
+  return engine;
+}
+
+// BackendConstructor (createEngineInternals.ts:452): (opts, scene, aabb,
+// needsTlas) -> engine carrying backendId.  The TLAS is always built here.
+async function constructPathTracerHipForDispatch(opts, scene /* , aabb, needsTlas */) {
+  const engine = await createPTEngine_HIP(opts);
+  if (scene) engine.setScene(scene);
+  return engine;
+}
+
+module.exports = {
+  createPTEngine_HIP,
+  constructPathTracerHipForDispatch,
+  multiplyMat4,
+  sceneBlob,
+  loadAddon,
+};

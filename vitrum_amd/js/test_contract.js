@@ -1,0 +1,83 @@
+'use strict';
+// Contract checks of the pt-hip JavaScript backend that need no GPU: the
+// addon loads and exports its functions, scene and option validation throw
+// the reference's error classes before any device work, and multiplyMat4
+// matches the binary64-sum-then-binary32 rule.  With VITRUM_HIP_GPU=1 it also
+// renders a small scene on device 0 and checks the capture contract.
+const assert = require('assert');
+const pt = require('./index.js');
+
+const addon = pt.loadAddon();
+for (const k of ['create', 'destroy', 'setScene', 'renderFrame', 'capture', 'reset', 'stats',
+  'traceClosest', 'traceAny', 'invertMat4', 'version']) {
+  assert.strictEqual(typeof addon[k], 'function', `addon.${k}`);
+}
+assert.ok(/gfx950/.test(addon.version()));
+
+const I = [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1];
+const T = [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 2, 3, 4, 1];
+const m = pt.multiplyMat4(T, T);
+assert.deepStrictEqual(Array.from(m.slice(12)), [4, 6, 8, 1]);
+const inv = addon.invertMat4(new Float32Array(T));
+assert.deepStrictEqual(Array.from(inv.slice(12)), [-2, -3, -4, 1]);
+assert.strictEqual(addon.invertMat4(new Float32Array(16)), null);
+assert.throws(() => addon.invertMat4(new Float32Array(3)), TypeError);
+
+// Scene conversion: unsupported kinds and bad arrays are rejected up front.
+const quad = {
+  kind: 'mesh', id: 'q',
+  positions: new Float32Array([0, 0, 0, 1, 0, 0, 1, 1, 0]),
+  normals: new Float32Array([0, 0, 1, 0, 0, 1, 0, 0, 1]),
+  indices: new Uint16Array([0, 1, 2]),
+  material: { baseColor: [0.5, 0.5, 0.5], roughness: 1, metallic: 0 },
+};
+const blob = pt.sceneBlob({ primitives: [quad], emitters: [], environment: { kind: 'none' } });
+assert.ok(blob.primitives[0].indices instanceof Uint32Array);
+assert.strictEqual(blob.environment.kind, 0);
+assert.throws(() => pt.sceneBlob({ primitives: [{ kind: 'sphere' }] }), RangeError);
+assert.throws(() => pt.sceneBlob({ primitives: [Object.assign({}, quad, { transform: [1, 2] })] }), TypeError);
+const sky = pt.sceneBlob({ primitives: [], environment: { kind: 'hdri', hdri: { constant: [1, 0.5, 0.25] }, intensity: 2 } });
+assert.deepStrictEqual(sky.environment, { kind: 1, radiance: [2, 1, 0.5] });
+assert.throws(() => pt.sceneBlob({ primitives: [], environment: { kind: 'hdri', hdri: { width: 2, height: 1, data: new Float32Array([1, 1, 1, 1, 0, 0, 0, 1]) } } }), RangeError);
+
+(async () => {
+  await assert.rejects(pt.createPTEngine_HIP({ bogus: 1 }), TypeError);
+  await assert.rejects(pt.createPTEngine_HIP({ denoiser: 'oidn-final' }), RangeError);
+  if (process.env.VITRUM_HIP_GPU !== '1') {
+    await assert.rejects(pt.createPTEngine_HIP({}), (e) => e.code === 'VT_ERR_NO_DEVICE' || e.code === 'VT_ERR_HIP');
+    console.log('pt-hip contract ok (no device)');
+    return;
+  }
+  const eng = await pt.createPTEngine_HIP({ device: 0 });
+  assert.strictEqual(eng.state, 'ready');
+  const frames = [];
+  const off = eng.onFrame((f) => frames.push(f));
+  eng.setScene({ primitives: [quad], emitters: [], environment: { kind: 'hdri', hdri: { constant: [1, 1, 1] } } });
+  const view = [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, -0.5, -0.5, -3, 1];
+  const f = 1 / Math.tan(Math.PI / 12), near = 0.01, far = 100;
+  const proj = [f, 0, 0, 0, 0, f, 0, 0, 0, 0, (far + near) / (near - far), -1, 0, 0, (2 * far * near) / (near - far), 0];
+  for (let i = 0; i < 4; i += 1) {
+    const out = eng.renderFrame({ viewMatrix: view, projMatrix: proj, viewport: { width: 16, height: 16, devicePixelRatio: 1 },
+      frameIndex: i, frameSeed: 0x9e3779b9, quality: { samplesTarget: 3, bounces: 4 } });
+    assert.strictEqual(out.kind, 'rendered');
+    assert.strictEqual(out.samplesAccumulated, Math.min(i + 1, 3));
+    assert.strictEqual(out.isConverged, i >= 2);
+  }
+  off();
+  assert.strictEqual(frames.length, 3);
+  const cap = await eng.captureFrame();
+  assert.strictEqual(cap.width, 16);
+  assert.strictEqual(cap.rgba.length, 16 * 16 * 4);
+  const center = cap.rgba[(8 * 16 + 8) * 4];
+  assert.ok(center > 0 && center < 1.5, `center radiance ${center}`);
+  // traceClosest through the addon: a ray straight at the triangle.
+  const hits = new Float32Array(addon.traceClosest(eng._h, new Float32Array([0.6, 0.3, 1, 0, 0, 0, -1, 1e30])));
+  assert.ok(Math.abs(hits[2] - 1) < 1e-6, `t = ${hits[2]}`);
+  eng.dispose();
+  assert.strictEqual(eng.state, 'disposed');
+  assert.throws(() => eng.setScene({ primitives: [] }), Error);
+  console.log('pt-hip contract ok (device 0)');
+})().catch((e) => {
+  console.error(e);
+  process.exit(1);
+});

@@ -1,0 +1,501 @@
+// addon.cpp -- Node N-API binding of libvthip.so.
+//
+// The JavaScript backend (vitrum_amd/js/index.js) implements the reference's
+// Engine contract (packages/core/src/engine/index.ts:109-486) on top of these
+// functions; each one forwards to one entry point of include/vitrum_hip.h.
+// Typed-array backing stores are borrowed for the duration of a call only.
+// Library status codes become the JavaScript error classes the reference
+// throws: VT_ERR_INVALID_ARGUMENT -> TypeError, VT_ERR_RANGE -> RangeError,
+// anything else -> Error with `code` = "VT_ERR_*".
+#include <node_api.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/vitrum_hip.h"
+
+namespace {
+
+struct Handle {
+  vt_engine* eng = nullptr;
+};
+
+const char* status_code(vt_status s) {
+  switch (s) {
+    case VT_ERR_INVALID_ARGUMENT: return "VT_ERR_INVALID_ARGUMENT";
+    case VT_ERR_RANGE: return "VT_ERR_RANGE";
+    case VT_ERR_BUILD: return "VT_ERR_BUILD";
+    case VT_ERR_HIP: return "VT_ERR_HIP";
+    case VT_ERR_NO_DEVICE: return "VT_ERR_NO_DEVICE";
+    case VT_ERR_STATE: return "VT_ERR_STATE";
+    case VT_ERR_OUT_OF_MEMORY: return "VT_ERR_OUT_OF_MEMORY";
+    default: return "VT_ERR_UNKNOWN";
+  }
+}
+
+// Throws the JavaScript error for a failed call; returns true when it threw.
+bool throw_status(napi_env env, const vt_engine* eng, vt_status s) {
+  if (s == VT_OK) return false;
+  char msg[1024] = {0};
+  vt_last_error(eng, msg, sizeof msg);
+  if (!msg[0]) std::snprintf(msg, sizeof msg, "vitrum-hip call failed (%s)", status_code(s));
+  if (s == VT_ERR_INVALID_ARGUMENT)
+    napi_throw_type_error(env, status_code(s), msg);
+  else if (s == VT_ERR_RANGE)
+    napi_throw_range_error(env, status_code(s), msg);
+  else
+    napi_throw_error(env, status_code(s), msg);
+  return true;
+}
+
+bool throw_type(napi_env env, const char* msg) {
+  napi_throw_type_error(env, "VT_ERR_INVALID_ARGUMENT", msg);
+  return true;
+}
+
+#define NAPI_OK(call)                                              \
+  do {                                                             \
+    if ((call) != napi_ok) {                                       \
+      napi_throw_error(env, nullptr, "N-API call failed: " #call); \
+      return nullptr;                                              \
+    }                                                              \
+  } while (0)
+
+std::vector<napi_value> args_of(napi_env env, napi_callback_info info, size_t n) {
+  std::vector<napi_value> v(n, nullptr);
+  size_t argc = n;
+  napi_get_cb_info(env, info, &argc, v.data(), nullptr, nullptr);
+  for (size_t i = argc; i < n; ++i) napi_get_undefined(env, &v[i]);
+  return v;
+}
+
+bool is_nullish(napi_env env, napi_value v) {
+  napi_valuetype t;
+  if (napi_typeof(env, v, &t) != napi_ok) return true;
+  return t == napi_undefined || t == napi_null;
+}
+
+bool get_u32(napi_env env, napi_value obj, const char* key, uint32_t def, uint32_t* out) {
+  napi_value v;
+  bool has = false;
+  napi_has_named_property(env, obj, key, &has);
+  if (!has) {
+    *out = def;
+    return true;
+  }
+  napi_get_named_property(env, obj, key, &v);
+  if (is_nullish(env, v)) {
+    *out = def;
+    return true;
+  }
+  double d;
+  if (napi_get_value_double(env, v, &d) != napi_ok || !(d >= 0) || d > 4294967295.0 || d != (double)(uint32_t)d) {
+    std::string m = std::string("'") + key + "' must be an unsigned 32-bit integer";
+    throw_type(env, m.c_str());
+    return false;
+  }
+  *out = (uint32_t)d;
+  return true;
+}
+
+bool get_f32(napi_env env, napi_value obj, const char* key, float def, float* out) {
+  napi_value v;
+  bool has = false;
+  napi_has_named_property(env, obj, key, &has);
+  if (!has) {
+    *out = def;
+    return true;
+  }
+  napi_get_named_property(env, obj, key, &v);
+  if (is_nullish(env, v)) {
+    *out = def;
+    return true;
+  }
+  double d;
+  if (napi_get_value_double(env, v, &d) != napi_ok) {
+    std::string m = std::string("'") + key + "' must be a number";
+    throw_type(env, m.c_str());
+    return false;
+  }
+  *out = (float)d;
+  return true;
+}
+
+// Reads `n` numbers from a JS array or typed array property into out[].
+bool get_vec(napi_env env, napi_value obj, const char* key, uint32_t n, const float* def, float* out) {
+  napi_value v;
+  bool has = false;
+  napi_has_named_property(env, obj, key, &has);
+  if (has) napi_get_named_property(env, obj, key, &v);
+  if (!has || is_nullish(env, v)) {
+    for (uint32_t i = 0; i < n; ++i) out[i] = def[i];
+    return true;
+  }
+  for (uint32_t i = 0; i < n; ++i) {
+    napi_value e;
+    double d;
+    if (napi_get_element(env, v, i, &e) != napi_ok || napi_get_value_double(env, e, &d) != napi_ok) {
+      std::string m = std::string("'") + key + "' must hold " + std::to_string(n) + " numbers";
+      throw_type(env, m.c_str());
+      return false;
+    }
+    out[i] = (float)d;
+  }
+  return true;
+}
+
+// Borrows the backing store of a typed-array property of the given type.
+bool get_typed(napi_env env, napi_value obj, const char* key, napi_typedarray_type want, bool optional,
+               void** data, size_t* length) {
+  napi_value v;
+  bool has = false;
+  napi_has_named_property(env, obj, key, &has);
+  if (has) napi_get_named_property(env, obj, key, &v);
+  if (!has || is_nullish(env, v)) {
+    *data = nullptr;
+    *length = 0;
+    if (optional) return true;
+    std::string m = std::string("'") + key + "' is required";
+    throw_type(env, m.c_str());
+    return false;
+  }
+  bool is_ta = false;
+  napi_is_typedarray(env, v, &is_ta);
+  napi_typedarray_type t;
+  napi_value ab;
+  size_t off;
+  if (!is_ta || napi_get_typedarray_info(env, v, &t, length, data, &ab, &off) != napi_ok || t != want) {
+    std::string m = std::string("'") + key + "' has the wrong typed-array type";
+    throw_type(env, m.c_str());
+    return false;
+  }
+  return true;
+}
+
+Handle* handle_of(napi_env env, napi_value v) {
+  void* p = nullptr;
+  if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
+    throw_type(env, "expected an engine handle");
+    return nullptr;
+  }
+  Handle* h = static_cast<Handle*>(p);
+  if (!h->eng) {
+    napi_throw_error(env, "VT_ERR_STATE", "engine has been disposed");
+    return nullptr;
+  }
+  return h;
+}
+
+void finalize_handle(napi_env, void* data, void*) {
+  Handle* h = static_cast<Handle*>(data);
+  if (h->eng) vt_destroy(h->eng);
+  delete h;
+}
+
+// create(deviceOrdinal, {maxBounces, maxSamplesPerPixel, sampling}) -> handle
+napi_value Create(napi_env env, napi_callback_info info) {
+  auto a = args_of(env, info, 2);
+  int32_t dev = 0;
+  if (!is_nullish(env, a[0]) && napi_get_value_int32(env, a[0], &dev) != napi_ok) {
+    throw_type(env, "device ordinal must be an integer");
+    return nullptr;
+  }
+  vt_options o{};
+  if (!is_nullish(env, a[1])) {
+    if (!get_u32(env, a[1], "maxBounces", 0, &o.max_bounces)) return nullptr;
+    if (!get_u32(env, a[1], "maxSamplesPerPixel", 0, &o.max_samples_per_pixel)) return nullptr;
+    if (!get_u32(env, a[1], "sampling", 0, &o.sampling)) return nullptr;
+  }
+  vt_engine* eng = nullptr;
+  if (throw_status(env, nullptr, vt_create(dev, &o, &eng))) return nullptr;
+  Handle* h = new Handle{eng};
+  napi_value out;
+  NAPI_OK(napi_create_external(env, h, finalize_handle, nullptr, &out));
+  return out;
+}
+
+napi_value Destroy(napi_env env, napi_callback_info info) {
+  auto a = args_of(env, info, 1);
+  void* p = nullptr;
+  if (napi_get_value_external(env, a[0], &p) == napi_ok && p) {
+    Handle* h = static_cast<Handle*>(p);
+    if (h->eng) vt_destroy(h->eng);
+    h->eng = nullptr;
+  }
+  return nullptr;
+}
+
+// setScene(handle, {primitives, materials, environment, sceneRadius})
+napi_value SetScene(napi_env env, napi_callback_info info) {
+  auto a = args_of(env, info, 2);
+  Handle* h = handle_of(env, a[0]);
+  if (!h) return nullptr;
+  napi_value desc = a[1], prims, mats;
+  bool is_arr = false;
+  if (is_nullish(env, desc)) {
+    throw_type(env, "scene description is required");
+    return nullptr;
+  }
+  NAPI_OK(napi_get_named_property(env, desc, "primitives", &prims));
+  napi_is_array(env, prims, &is_arr);
+  if (!is_arr) {
+    throw_type(env, "'primitives' must be an array");
+    return nullptr;
+  }
+  uint32_t np = 0, nm = 0;
+  napi_get_array_length(env, prims, &np);
+  std::vector<vt_primitive> vp(np);
+  for (uint32_t i = 0; i < np; ++i) {
+    napi_value pr;
+    NAPI_OK(napi_get_element(env, prims, i, &pr));
+    vt_primitive& q = vp[i];
+    std::memset(&q, 0, sizeof q);
+    if (!get_u32(env, pr, "kind", VT_PRIM_MESH, &q.kind)) return nullptr;
+    if (!get_u32(env, pr, "material", i, &q.material)) return nullptr;
+    void* d;
+    size_t n;
+    if (!get_typed(env, pr, "positions", napi_float32_array, false, &d, &n)) return nullptr;
+    q.positions = static_cast<const float*>(d);
+    q.position_floats = (uint32_t)n;
+    if (!get_typed(env, pr, "normals", napi_float32_array, true, &d, &n)) return nullptr;
+    q.normals = static_cast<const float*>(d);
+    q.normal_floats = (uint32_t)n;
+    if (!get_typed(env, pr, "indices", napi_uint32_array, true, &d, &n)) return nullptr;
+    q.indices = static_cast<const uint32_t*>(d);
+    q.index_count = (uint32_t)n;
+    if (!get_typed(env, pr, "transforms", napi_float32_array, true, &d, &n)) return nullptr;
+    q.transforms = static_cast<const float*>(d);
+    q.transform_count = (uint32_t)(n / 16);
+    if (n % 16) {
+      throw_type(env, "'transforms' must hold whole column-major mat4s");
+      return nullptr;
+    }
+  }
+  NAPI_OK(napi_get_named_property(env, desc, "materials", &mats));
+  napi_is_array(env, mats, &is_arr);
+  if (!is_arr) {
+    throw_type(env, "'materials' must be an array");
+    return nullptr;
+  }
+  napi_get_array_length(env, mats, &nm);
+  std::vector<vt_material> vm(nm);
+  const float white[3] = {0.8f, 0.8f, 0.8f}, zero[3] = {0, 0, 0};
+  for (uint32_t i = 0; i < nm; ++i) {
+    napi_value m;
+    NAPI_OK(napi_get_element(env, mats, i, &m));
+    vt_material& q = vm[i];
+    std::memset(&q, 0, sizeof q);
+    if (!get_vec(env, m, "baseColor", 3, white, q.base_color)) return nullptr;
+    if (!get_f32(env, m, "roughness", 0.5f, &q.roughness)) return nullptr;
+    if (!get_f32(env, m, "metallic", 0.0f, &q.metallic)) return nullptr;
+    if (!get_vec(env, m, "emissive", 3, zero, q.emissive)) return nullptr;
+    if (!get_f32(env, m, "emissiveIntensity", 1.0f, &q.emissive_intensity)) return nullptr;
+    bool ds = false;
+    napi_value v;
+    bool has = false;
+    napi_has_named_property(env, m, "doubleSided", &has);
+    if (has && napi_get_named_property(env, m, "doubleSided", &v) == napi_ok) napi_get_value_bool(env, v, &ds);
+    q.double_sided = ds ? 1u : 0u;
+  }
+  vt_scene_desc sd{};
+  sd.primitives = vp.data();
+  sd.primitive_count = np;
+  sd.materials = vm.data();
+  sd.material_count = nm;
+  napi_value envv;
+  bool has_env = false;
+  napi_has_named_property(env, desc, "environment", &has_env);
+  if (has_env && napi_get_named_property(env, desc, "environment", &envv) == napi_ok && !is_nullish(env, envv)) {
+    if (!get_u32(env, envv, "kind", 0, &sd.environment.kind)) return nullptr;
+    if (!get_vec(env, envv, "radiance", 3, zero, sd.environment.radiance)) return nullptr;
+  }
+  if (!get_f32(env, desc, "sceneRadius", 0.0f, &sd.scene_radius)) return nullptr;
+  if (throw_status(env, h->eng, vt_set_scene(h->eng, &sd))) return nullptr;
+  return nullptr;
+}
+
+// renderFrame(handle, {invViewProj, width, height, frameIndex, frameSeed, maxBounces, samplesTarget})
+napi_value RenderFrame(napi_env env, napi_callback_info info) {
+  auto a = args_of(env, info, 2);
+  Handle* h = handle_of(env, a[0]);
+  if (!h) return nullptr;
+  if (is_nullish(env, a[1])) {
+    throw_type(env, "frame description is required");
+    return nullptr;
+  }
+  vt_frame f{};
+  void* d;
+  size_t n;
+  if (!get_typed(env, a[1], "invViewProj", napi_float32_array, false, &d, &n)) return nullptr;
+  if (n != 16) {
+    throw_type(env, "'invViewProj' must be a Float32Array(16)");
+    return nullptr;
+  }
+  std::memcpy(f.inv_view_proj, d, 64);
+  if (!get_u32(env, a[1], "width", 0, &f.width)) return nullptr;
+  if (!get_u32(env, a[1], "height", 0, &f.height)) return nullptr;
+  if (!get_u32(env, a[1], "frameIndex", 0, &f.frame_index)) return nullptr;
+  if (!get_u32(env, a[1], "frameSeed", 0, &f.frame_seed)) return nullptr;
+  if (!get_u32(env, a[1], "maxBounces", 0, &f.max_bounces)) return nullptr;
+  if (!get_u32(env, a[1], "samplesTarget", 0, &f.samples_target)) return nullptr;
+  if (throw_status(env, h->eng, vt_render_frame(h->eng, &f))) return nullptr;
+  return nullptr;
+}
+
+// capture(handle, width, height) -> Float32Array (row-major RGBA, top-left origin)
+napi_value Capture(napi_env env, napi_callback_info info) {
+  auto a = args_of(env, info, 3);
+  Handle* h = handle_of(env, a[0]);
+  if (!h) return nullptr;
+  uint32_t w = 0, hh = 0;
+  if (napi_get_value_uint32(env, a[1], &w) != napi_ok || napi_get_value_uint32(env, a[2], &hh) != napi_ok) {
+    throw_type(env, "width and height must be unsigned integers");
+    return nullptr;
+  }
+  const size_t floats = (size_t)w * hh * 4;
+  void* data = nullptr;
+  napi_value ab, out;
+  NAPI_OK(napi_create_arraybuffer(env, floats * 4, &data, &ab));
+  if (throw_status(env, h->eng, vt_capture(h->eng, static_cast<float*>(data), w, hh))) return nullptr;
+  NAPI_OK(napi_create_typedarray(env, napi_float32_array, floats, ab, 0, &out));
+  return out;
+}
+
+napi_value Reset(napi_env env, napi_callback_info info) {
+  auto a = args_of(env, info, 1);
+  Handle* h = handle_of(env, a[0]);
+  if (!h) return nullptr;
+  throw_status(env, h->eng, vt_reset(h->eng));
+  return nullptr;
+}
+
+napi_value Stats(napi_env env, napi_callback_info info) {
+  auto a = args_of(env, info, 1);
+  Handle* h = handle_of(env, a[0]);
+  if (!h) return nullptr;
+  vt_frame_stats s{};
+  if (throw_status(env, h->eng, vt_stats(h->eng, &s))) return nullptr;
+  float ms = 0;
+  if (throw_status(env, h->eng, vt_frame_time(h->eng, &ms))) return nullptr;
+  napi_value o, v;
+  NAPI_OK(napi_create_object(env, &o));
+  napi_create_double(env, s.last_trace_ms, &v);
+  napi_set_named_property(env, o, "lastTraceMs", v);
+  napi_create_double(env, ms, &v);
+  napi_set_named_property(env, o, "lastFrameMs", v);
+  napi_create_uint32(env, s.samples_accumulated, &v);
+  napi_set_named_property(env, o, "samplesAccumulated", v);
+  napi_create_uint32(env, s.last_ray_count, &v);
+  napi_set_named_property(env, o, "lastRayCount", v);
+  napi_create_double(env, (double)s.device_bytes, &v);
+  napi_set_named_property(env, o, "deviceBytes", v);
+  return o;
+}
+
+// traceClosest(handle, rays: Float32Array(8 n)) -> ArrayBuffer of n 20-byte hits
+napi_value TraceClosest(napi_env env, napi_callback_info info) {
+  auto a = args_of(env, info, 2);
+  Handle* h = handle_of(env, a[0]);
+  if (!h) return nullptr;
+  bool is_ta = false;
+  napi_is_typedarray(env, a[1], &is_ta);
+  napi_typedarray_type t;
+  size_t len = 0, off = 0;
+  void* d = nullptr;
+  napi_value ab0;
+  if (!is_ta || napi_get_typedarray_info(env, a[1], &t, &len, &d, &ab0, &off) != napi_ok ||
+      t != napi_float32_array || len % 8) {
+    throw_type(env, "rays must be a Float32Array of 8 floats per ray");
+    return nullptr;
+  }
+  const uint32_t n = (uint32_t)(len / 8);
+  void* out = nullptr;
+  napi_value ab;
+  NAPI_OK(napi_create_arraybuffer(env, (size_t)n * sizeof(vt_hit), &out, &ab));
+  if (throw_status(env, h->eng, vt_trace_closest(h->eng, static_cast<const vt_ray*>(d), n,
+                                                 static_cast<vt_hit*>(out))))
+    return nullptr;
+  return ab;
+}
+
+// traceAny(handle, rays: Float32Array(8 n)) -> Uint32Array(n) of 0/1
+napi_value TraceAny(napi_env env, napi_callback_info info) {
+  auto a = args_of(env, info, 2);
+  Handle* h = handle_of(env, a[0]);
+  if (!h) return nullptr;
+  bool is_ta = false;
+  napi_is_typedarray(env, a[1], &is_ta);
+  napi_typedarray_type t;
+  size_t len = 0, off = 0;
+  void* d = nullptr;
+  napi_value ab0;
+  if (!is_ta || napi_get_typedarray_info(env, a[1], &t, &len, &d, &ab0, &off) != napi_ok ||
+      t != napi_float32_array || len % 8) {
+    throw_type(env, "rays must be a Float32Array of 8 floats per ray");
+    return nullptr;
+  }
+  const uint32_t n = (uint32_t)(len / 8);
+  void* out = nullptr;
+  napi_value ab, arr;
+  NAPI_OK(napi_create_arraybuffer(env, (size_t)n * 4, &out, &ab));
+  if (throw_status(env, h->eng, vt_trace_any(h->eng, static_cast<const vt_ray*>(d), n,
+                                             static_cast<uint32_t*>(out))))
+    return nullptr;
+  NAPI_OK(napi_create_typedarray(env, napi_uint32_array, n, ab, 0, &arr));
+  return arr;
+}
+
+// invertMat4(Float32Array(16)) -> Float32Array(16) | null
+napi_value InvertMat4(napi_env env, napi_callback_info info) {
+  auto a = args_of(env, info, 1);
+  bool is_ta = false;
+  napi_is_typedarray(env, a[0], &is_ta);
+  napi_typedarray_type t;
+  size_t len = 0, off = 0;
+  void* d = nullptr;
+  napi_value ab0;
+  if (!is_ta || napi_get_typedarray_info(env, a[0], &t, &len, &d, &ab0, &off) != napi_ok ||
+      t != napi_float32_array || len != 16) {
+    throw_type(env, "expected a Float32Array(16)");
+    return nullptr;
+  }
+  void* out = nullptr;
+  napi_value ab, arr;
+  NAPI_OK(napi_create_arraybuffer(env, 64, &out, &ab));
+  if (vt_invert_mat4(static_cast<const float*>(d), static_cast<float*>(out)) != VT_OK) {
+    napi_get_null(env, &arr);
+    return arr;
+  }
+  NAPI_OK(napi_create_typedarray(env, napi_float32_array, 16, ab, 0, &arr));
+  return arr;
+}
+
+napi_value Version(napi_env env, napi_callback_info) {
+  napi_value v;
+  NAPI_OK(napi_create_string_utf8(env, vt_version(), NAPI_AUTO_LENGTH, &v));
+  return v;
+}
+
+napi_value Init(napi_env env, napi_value exports) {
+  const struct {
+    const char* name;
+    napi_callback cb;
+  } fns[] = {{"create", Create},         {"destroy", Destroy},
+             {"setScene", SetScene},     {"renderFrame", RenderFrame},
+             {"capture", Capture},       {"reset", Reset},
+             {"stats", Stats},           {"traceClosest", TraceClosest},
+             {"traceAny", TraceAny},     {"invertMat4", InvertMat4},
+             {"version", Version}};
+  for (const auto& f : fns) {
+    napi_value fn;
+    if (napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.cb, nullptr, &fn) != napi_ok) return nullptr;
+    if (napi_set_named_property(env, exports, f.name, fn) != napi_ok) return nullptr;
+  }
+  return exports;
+}
+
+}  // namespace
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, Init)
