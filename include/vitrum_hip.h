@@ -273,6 +273,42 @@ int64_t vt_pack_field_words(const vt_pack *p, int32_t field);
 vt_status vt_pack_copy_field(const vt_pack *p, int32_t field, void *dst, uint64_t dst_words);
 void vt_pack_free(vt_pack *p);
 
+/* ---- neural radiance cache ------------------------------------------------
+ * The online-trained cache of walkaround-hybrid's NRC subsystem
+ * (packages/walkaround-hybrid/src/neural/nrc/nrcSubsystem.ts,
+ * fusedMlpTrainer.ts, wgsl/fusedMlp.wgsl.ts): a 64-wide MLP with 4 hidden
+ * ReLU layers and RGB output, bf16 weights and activations, FP32 accumulation,
+ * FP32 master weights trained by Adam on the relative-L2 loss.
+ * Samples are 16-float feature records: position normalised to [0,1]^3,
+ * direction and normal as (theta/pi, phi/(2 pi)), roughness, diffuse albedo
+ * (3), specular albedo (3), 2 zero floats; targets are RGB. */
+typedef struct vt_nrc_options {
+  float learning_rate; /* 1e-2 */
+  float beta1;         /* 0.9 */
+  float beta2;         /* 0.999 */
+  float epsilon;       /* 1e-8 */
+  uint32_t seed;       /* initial weights */
+  uint32_t reserved[3];
+} vt_nrc_options;
+/* Number of FP32 master weights (W0..W3 64x64 then W4 3x64, row-major [out][in]). */
+vt_status vt_nrc_param_count(uint32_t *count);
+/* Re-initialises weights (from opts->seed) and Adam state; NULL keeps options. */
+vt_status vt_nrc_reset(vt_engine *eng, const vt_nrc_options *opts);
+vt_status vt_nrc_set_weights(vt_engine *eng, const float *weights, uint32_t count);
+vt_status vt_nrc_get_weights(vt_engine *eng, float *weights, uint32_t count);
+/* RGB prediction (3 floats per sample). Host buffers / device pointers. */
+vt_status vt_nrc_infer(vt_engine *eng, const float *features, uint32_t n, float *rgb);
+vt_status vt_nrc_infer_device(vt_engine *eng, const float *d_features, uint32_t n, float *d_rgb);
+/* One Adam step on the batch (fusedMlpTrainer.ts train step); loss may be NULL. */
+vt_status vt_nrc_train_step(vt_engine *eng, const float *features, const float *targets, uint32_t n,
+                            float *loss);
+vt_status vt_nrc_train_step_device(vt_engine *eng, const float *d_features, const float *d_targets,
+                                   uint32_t n);
+vt_status vt_nrc_last_loss(vt_engine *eng, float *loss);
+/* d loss / d weights of the batch (no update), for gradient checks. */
+vt_status vt_nrc_gradient(vt_engine *eng, const float *features, const float *targets, uint32_t n,
+                          float *grad, float *loss);
+
 /* invertMat4 (packages/shared-bvh/src/mathUtils.ts:21-110): binary64
  * cofactor inverse of a column-major mat4 rounded to binary32, with the
  * reciprocal-product check.  VT_ERR_RANGE when the matrix is not invertible.

@@ -179,6 +179,7 @@ vt_status vt_destroy(vt_engine* eng) {
   eng->tmp_counts.release();
   for (auto& kv : eng->user_allocs) (void)hipFree(kv);
   engine_release_render(eng);
+  engine_release_nrc(eng);
   (void)hipEventDestroy(eng->ev_start);
   (void)hipEventDestroy(eng->ev_stop);
   (void)hipStreamDestroy(eng->stream);

@@ -26,6 +26,7 @@ struct SceneBuffers {
 };
 
 struct RenderState;  // integrator state (render.hip)
+struct NrcState;     // neural radiance cache (nrc.hip)
 
 struct vt_engine {
   int device = 0;
@@ -55,6 +56,7 @@ struct vt_engine {
   uint32_t samples_accumulated = 0;
   uint32_t last_ray_count = 0;
   RenderState* render = nullptr;
+  NrcState* nrc = nullptr;
   std::string last_error;
 };
 
@@ -62,3 +64,4 @@ vt_status engine_upload(vt_engine* e, DeviceBuffer& dst, const void* src, size_t
 // Hooks into the integrator (render.cpp): rebuild per-scene render data, free it.
 vt_status engine_on_scene_changed(vt_engine* e);
 void engine_release_render(vt_engine* e);
+void engine_release_nrc(vt_engine* e);

@@ -1,0 +1,250 @@
+// nrc.cpp -- C ABI of the neural radiance cache (kernels in nrc.hip).
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "engine_internal.h"
+#include "nrc.h"
+
+struct NrcState {
+  DeviceBuffer master, m, v, grad, frags, loss;
+  DeviceBuffer feats, targets, out;  // staging for the host-buffer entry points
+  vt_nrc_options opts{};
+  uint32_t step = 0;
+  void release() {
+    for (DeviceBuffer* b : {&master, &m, &v, &grad, &frags, &loss, &feats, &targets, &out}) b->release();
+  }
+};
+
+void engine_release_nrc(vt_engine* e) {
+  if (!e->nrc) return;
+  e->nrc->release();
+  delete e->nrc;
+  e->nrc = nullptr;
+}
+
+static vt_status nrc_err(vt_engine* e, vt_status s, const char* msg) {
+  e->last_error = msg;
+  return s;
+}
+
+#define NRC_HIP(e, call)                                        \
+  do {                                                          \
+    hipError_t _err = (call);                                   \
+    if (_err != hipSuccess) {                                   \
+      (e)->last_error = std::string(#call ": ") + hipGetErrorString(_err); \
+      return VT_ERR_HIP;                                        \
+    }                                                           \
+  } while (0)
+
+// Xavier-uniform initial weights from a PCG stream (deterministic per seed).
+static void init_weights(uint32_t seed, std::vector<float>& w) {
+  w.assign(vtn::kParams, 0.0f);
+  uint64_t st = 0x853c49e6748fea9bull ^ ((uint64_t)seed << 1);
+  auto next = [&]() {
+    st = st * 6364136223846793005ull + 1442695040888963407ull;
+    const uint32_t x = (uint32_t)(((st >> 18) ^ st) >> 27), rot = (uint32_t)(st >> 59);
+    return (x >> rot) | (x << ((32u - rot) & 31u));
+  };
+  auto fill = [&](size_t off, uint32_t rows, uint32_t cols) {
+    const double a = std::sqrt(6.0 / (double)(rows + cols));
+    for (size_t i = 0; i < (size_t)rows * cols; ++i)
+      w[off + i] = (float)((((double)next() / 4294967296.0) * 2.0 - 1.0) * a);
+  };
+  for (uint32_t l = 0; l < vtn::kHidden; ++l) fill((size_t)l * 4096, 64, 64);
+  fill(16384, vtn::kOut, 64);
+}
+
+static vt_status nrc_ensure(vt_engine* e) {
+  if (e->nrc) return VT_OK;
+  NrcState* s = new NrcState();
+  s->opts.learning_rate = 1e-2f;
+  s->opts.beta1 = 0.9f;
+  s->opts.beta2 = 0.999f;
+  s->opts.epsilon = 1e-8f;
+  s->opts.seed = 1;
+  e->nrc = s;
+  vt_status st = VT_OK;
+  const size_t pb = vtn::kParams * sizeof(float);
+  for (DeviceBuffer* b : {&s->master, &s->m, &s->v, &s->grad}) st = st ? st : b->ensure(e, pb);
+  st = st ? st : s->frags.ensure(e, vtn::kFrags * 16);
+  st = st ? st : s->loss.ensure(e, 64);
+  if (st) {
+    engine_release_nrc(e);
+    return st;
+  }
+  return vt_nrc_reset(e, nullptr);
+}
+
+static vt_status upload_master(vt_engine* e, const float* w) {
+  NrcState& s = *e->nrc;
+  NRC_HIP(e, hipSetDevice(e->device));
+  NRC_HIP(e, hipMemcpyAsync(s.master.ptr, w, vtn::kParams * sizeof(float), hipMemcpyHostToDevice, e->stream));
+  NRC_HIP(e, vtn::launch_fill((float*)s.m.ptr, vtn::kParams, 0.0f, e->stream));
+  NRC_HIP(e, vtn::launch_fill((float*)s.v.ptr, vtn::kParams, 0.0f, e->stream));
+  NRC_HIP(e, vtn::launch_fill((float*)s.grad.ptr, vtn::kParams, 0.0f, e->stream));
+  NRC_HIP(e, vtn::launch_pack((const float*)s.master.ptr, (uint4*)s.frags.ptr, e->stream));
+  NRC_HIP(e, hipStreamSynchronize(e->stream));
+  s.step = 0;
+  return VT_OK;
+}
+
+static bool finite_features(const float* f, size_t n) {
+  for (size_t i = 0; i < n; ++i)
+    if (!std::isfinite(f[i])) return false;
+  return true;
+}
+
+extern "C" {
+
+vt_status vt_nrc_param_count(uint32_t* count) {
+  if (!count) return VT_ERR_INVALID_ARGUMENT;
+  *count = vtn::kParams;
+  return VT_OK;
+}
+
+vt_status vt_nrc_reset(vt_engine* e, const vt_nrc_options* opts) {
+  if (!e) return VT_ERR_INVALID_ARGUMENT;
+  if (!e->nrc) {
+    vt_status s = nrc_ensure(e);  // allocates, then re-enters with defaults
+    if (s || !opts) return s;
+  }
+  NrcState& s = *e->nrc;
+  if (opts) {
+    if (!(opts->learning_rate > 0) || !(opts->beta1 >= 0 && opts->beta1 < 1) ||
+        !(opts->beta2 >= 0 && opts->beta2 < 1) || !(opts->epsilon > 0))
+      return nrc_err(e, VT_ERR_RANGE, "learning_rate > 0, 0 <= beta1, beta2 < 1 and epsilon > 0 are required");
+    s.opts = *opts;
+  }
+  std::vector<float> w;
+  init_weights(s.opts.seed, w);
+  return upload_master(e, w.data());
+}
+
+vt_status vt_nrc_set_weights(vt_engine* e, const float* w, uint32_t count) {
+  if (!e || !w) return VT_ERR_INVALID_ARGUMENT;
+  if (count != vtn::kParams) return nrc_err(e, VT_ERR_RANGE, "weight count must equal vt_nrc_param_count()");
+  if (!finite_features(w, count)) return nrc_err(e, VT_ERR_RANGE, "weights must be finite");
+  if (vt_status s = nrc_ensure(e)) return s;
+  return upload_master(e, w);
+}
+
+vt_status vt_nrc_get_weights(vt_engine* e, float* w, uint32_t count) {
+  if (!e || !w) return VT_ERR_INVALID_ARGUMENT;
+  if (count != vtn::kParams) return nrc_err(e, VT_ERR_RANGE, "weight count must equal vt_nrc_param_count()");
+  if (vt_status s = nrc_ensure(e)) return s;
+  NRC_HIP(e, hipMemcpyAsync(w, e->nrc->master.ptr, count * sizeof(float), hipMemcpyDeviceToHost, e->stream));
+  NRC_HIP(e, hipStreamSynchronize(e->stream));
+  return VT_OK;
+}
+
+vt_status vt_nrc_infer_device(vt_engine* e, const float* d_features, uint32_t n, float* d_rgb) {
+  if (!e || (n && (!d_features || !d_rgb))) return VT_ERR_INVALID_ARGUMENT;
+  if (vt_status s = nrc_ensure(e)) return s;
+  NRC_HIP(e, hipSetDevice(e->device));
+  NRC_HIP(e, vtn::launch_infer(d_features, n, (const uint4*)e->nrc->frags.ptr, d_rgb, e->stream));
+  return VT_OK;
+}
+
+vt_status vt_nrc_infer(vt_engine* e, const float* features, uint32_t n, float* rgb) {
+  if (!e || (n && (!features || !rgb))) return VT_ERR_INVALID_ARGUMENT;
+  if (!finite_features(features, (size_t)n * vtn::kFeat)) return nrc_err(e, VT_ERR_RANGE, "features must be finite");
+  if (vt_status s = nrc_ensure(e)) return s;
+  NrcState& s = *e->nrc;
+  if (vt_status st = s.feats.ensure(e, (size_t)n * vtn::kFeat * 4 + 64)) return st;
+  if (vt_status st = s.out.ensure(e, (size_t)n * 12 + 64)) return st;
+  NRC_HIP(e, hipMemcpyAsync(s.feats.ptr, features, (size_t)n * vtn::kFeat * 4, hipMemcpyHostToDevice, e->stream));
+  if (vt_status st = vt_nrc_infer_device(e, (const float*)s.feats.ptr, n, (float*)s.out.ptr)) return st;
+  NRC_HIP(e, hipMemcpyAsync(rgb, s.out.ptr, (size_t)n * 12, hipMemcpyDeviceToHost, e->stream));
+  NRC_HIP(e, hipStreamSynchronize(e->stream));
+  return VT_OK;
+}
+
+// Accumulates the batch gradient into the engine's gradient buffer and the
+// batch loss into loss[0] (both device memory of the engine).
+static vt_status nrc_accumulate(vt_engine* e, const float* d_features, const float* d_targets, uint32_t n) {
+  NrcState& s = *e->nrc;
+  NRC_HIP(e, hipSetDevice(e->device));
+  NRC_HIP(e, vtn::launch_fill((float*)s.loss.ptr, 1, 0.0f, e->stream));
+  NRC_HIP(e, vtn::launch_train(d_features, d_targets, n, (const uint4*)s.frags.ptr, (float*)s.grad.ptr,
+                               (float*)s.loss.ptr, e->stream));
+  return VT_OK;
+}
+
+static vt_status nrc_apply_adam(vt_engine* e) {
+  NrcState& s = *e->nrc;
+  s.step += 1;
+  vtn::AdamParams p{};
+  p.lr = s.opts.learning_rate;
+  p.beta1 = s.opts.beta1;
+  p.beta2 = s.opts.beta2;
+  p.eps = s.opts.epsilon;
+  p.bc1 = (float)(1.0 - std::pow((double)s.opts.beta1, (double)s.step));
+  p.bc2 = (float)(1.0 - std::pow((double)s.opts.beta2, (double)s.step));
+  NRC_HIP(e, vtn::launch_adam((float*)s.master.ptr, (float*)s.grad.ptr, (float*)s.m.ptr, (float*)s.v.ptr, p,
+                              e->stream));
+  NRC_HIP(e, vtn::launch_pack((const float*)s.master.ptr, (uint4*)s.frags.ptr, e->stream));
+  return VT_OK;
+}
+
+vt_status vt_nrc_train_step_device(vt_engine* e, const float* d_features, const float* d_targets, uint32_t n) {
+  if (!e || (n && (!d_features || !d_targets))) return VT_ERR_INVALID_ARGUMENT;
+  if (n == 0) return VT_OK;
+  if (vt_status s = nrc_ensure(e)) return s;
+  if (vt_status s = nrc_accumulate(e, d_features, d_targets, n)) return s;
+  return nrc_apply_adam(e);
+}
+
+vt_status vt_nrc_last_loss(vt_engine* e, float* loss) {
+  if (!e || !loss) return VT_ERR_INVALID_ARGUMENT;
+  if (vt_status s = nrc_ensure(e)) return s;
+  NRC_HIP(e, hipMemcpyAsync(loss, e->nrc->loss.ptr, 4, hipMemcpyDeviceToHost, e->stream));
+  NRC_HIP(e, hipStreamSynchronize(e->stream));
+  return VT_OK;
+}
+
+static vt_status stage_batch(vt_engine* e, const float* features, const float* targets, uint32_t n) {
+  if (!finite_features(features, (size_t)n * vtn::kFeat) || !finite_features(targets, (size_t)n * 3))
+    return nrc_err(e, VT_ERR_RANGE, "features and targets must be finite");
+  NrcState& s = *e->nrc;
+  if (vt_status st = s.feats.ensure(e, (size_t)n * vtn::kFeat * 4 + 64)) return st;
+  if (vt_status st = s.targets.ensure(e, (size_t)n * 12 + 64)) return st;
+  NRC_HIP(e, hipMemcpyAsync(s.feats.ptr, features, (size_t)n * vtn::kFeat * 4, hipMemcpyHostToDevice, e->stream));
+  NRC_HIP(e, hipMemcpyAsync(s.targets.ptr, targets, (size_t)n * 12, hipMemcpyHostToDevice, e->stream));
+  return VT_OK;
+}
+
+vt_status vt_nrc_train_step(vt_engine* e, const float* features, const float* targets, uint32_t n, float* loss) {
+  if (!e || (n && (!features || !targets))) return VT_ERR_INVALID_ARGUMENT;
+  if (vt_status s = nrc_ensure(e)) return s;
+  if (n == 0) return VT_OK;
+  if (vt_status s = stage_batch(e, features, targets, n)) return s;
+  NrcState& s = *e->nrc;
+  if (vt_status st = vt_nrc_train_step_device(e, (const float*)s.feats.ptr, (const float*)s.targets.ptr, n))
+    return st;
+  if (loss) return vt_nrc_last_loss(e, loss);
+  NRC_HIP(e, hipStreamSynchronize(e->stream));
+  return VT_OK;
+}
+
+vt_status vt_nrc_gradient(vt_engine* e, const float* features, const float* targets, uint32_t n, float* grad,
+                          float* loss) {
+  if (!e || !grad || (n && (!features || !targets))) return VT_ERR_INVALID_ARGUMENT;
+  if (vt_status s = nrc_ensure(e)) return s;
+  NrcState& s = *e->nrc;
+  NRC_HIP(e, hipSetDevice(e->device));
+  NRC_HIP(e, vtn::launch_fill((float*)s.grad.ptr, vtn::kParams, 0.0f, e->stream));
+  if (n) {
+    if (vt_status st = stage_batch(e, features, targets, n)) return st;
+    if (vt_status st = nrc_accumulate(e, (const float*)s.feats.ptr, (const float*)s.targets.ptr, n)) return st;
+  } else {
+    NRC_HIP(e, vtn::launch_fill((float*)s.loss.ptr, 1, 0.0f, e->stream));
+  }
+  NRC_HIP(e, hipMemcpyAsync(grad, s.grad.ptr, vtn::kParams * 4, hipMemcpyDeviceToHost, e->stream));
+  if (loss) NRC_HIP(e, hipMemcpyAsync(loss, s.loss.ptr, 4, hipMemcpyDeviceToHost, e->stream));
+  NRC_HIP(e, vtn::launch_fill((float*)s.grad.ptr, vtn::kParams, 0.0f, e->stream));
+  NRC_HIP(e, hipStreamSynchronize(e->stream));
+  return VT_OK;
+}
+
+}  // extern "C"

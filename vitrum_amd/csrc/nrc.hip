@@ -1,0 +1,441 @@
+// nrc.hip -- fused BF16 MFMA kernels of the neural radiance cache.
+//
+// Network (Mueller et al. 2021, Real-time Neural Radiance Caching): encoded
+// input x (64) -> H1..H4 = ReLU(W_l H_{l-1}) (64 wide) -> y = W4 H4 (RGB).
+// Loss: relative L2 of walkaround-hybrid/src/neural/nrc/wgsl/fusedMlp.wgsl.ts
+// :65-120 (luminance of the prediction as a stop-gradient denominator,
+// epsilon 1e-2); Adam as in walkaround-hybrid/src/neural/nrc/
+// fusedMlpTrainer.ts:30-84 (FP32 master weights, bf16 copies for the MFMAs).
+//
+// MFMA formulation.  Every layer is computed transposed, H_out^T = W H_in^T,
+// with v_mfma_f32_16x16x32_bf16: A = 16 output neurons x 32 inputs of W,
+// B = 32 inputs x 16 samples of H_in^T, C = 16 neurons x 16 samples.  A lane
+// (g = lane / 16, r = lane % 16) of C holds neurons 4g..4g+3 of sample r.  The
+// 32-wide K of each MFMA step kb is permuted so that lane (g, r) supplies
+// inputs {32kb + 4g + j} and {32kb + 16 + 4g + j}, j < 4 -- exactly the values
+// the same lane holds in the C tiles of neuron blocks 2kb and 2kb + 1 of the
+// previous layer.  Activations therefore pass from layer to layer in
+// registers (ReLU, bf16 rounding), with no LDS round trip; the weight
+// fragments are packed in the same permuted order.  Backward uses W^T the same
+// way.  Weight gradients dW = dH^T H need the sample dimension as K; those
+// operands are staged transposed ([neuron][sample]) in LDS.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "nrc.h"
+
+namespace vtn {
+
+typedef __bf16 bf16x8 __attribute__((__vector_size__(8 * sizeof(__bf16))));
+typedef float f32x4 __attribute__((__vector_size__(4 * sizeof(float))));
+
+__device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// Round to nearest even (NaN -> quiet NaN).
+__host__ __device__ __forceinline__ uint32_t f2bf(float f) {
+  uint32_t u;
+  __builtin_memcpy(&u, &f, 4);
+  if ((u & 0x7FFFFFFFu) > 0x7F800000u) return 0x7FC0u;
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return u >> 16;
+}
+
+__device__ __forceinline__ bf16x8 frag_of(uint4 q) { return __builtin_bit_cast(bf16x8, q); }
+
+__device__ __forceinline__ uint4 pack8(const float* v) {
+  uint4 q;
+  q.x = f2bf(v[0]) | (f2bf(v[1]) << 16);
+  q.y = f2bf(v[2]) | (f2bf(v[3]) << 16);
+  q.z = f2bf(v[4]) | (f2bf(v[5]) << 16);
+  q.w = f2bf(v[6]) | (f2bf(v[7]) << 16);
+  return q;
+}
+
+// Input neuron of element t (0..7) of lane group g in K step kb.
+__host__ __device__ __forceinline__ uint32_t nmap(uint32_t kb, uint32_t g, uint32_t t) {
+  return (2u * kb + (t >> 2)) * 16u + g * 4u + (t & 3u);
+}
+
+// Encoded input dimension d of a raw feature record (see nrc.h).
+//   0-35  sin(pi * 2^k * p_a), axis a = d / 12, k = d % 12 (12 frequencies)
+//   36-51 one-blob, 4 Gaussian bins (sigma = 1/4) for each of the 4 angles
+//   52    1 - exp(-roughness); 53-55 diffuse albedo; 56-58 specular albedo
+//   63    1 (bias input); others 0
+__device__ __forceinline__ float encode_dim(uint32_t d, const float* f) {
+  if (d < 36u) {
+    const float t = f[d / 12u] * (float)(1u << (d % 12u));
+    const float r = t - 2.0f * floorf(t * 0.5f);
+    return sinpif(r);
+  }
+  if (d < 52u) {
+    const uint32_t q = (d - 36u) >> 2, b = (d - 36u) & 3u;
+    const float e = (f[3u + q] - ((float)b + 0.5f) * 0.25f) * 4.0f;
+    return expf(-0.5f * (e * e));
+  }
+  if (d == 52u) return 1.0f - expf(-f[7]);
+  if (d < 56u) return f[8u + d - 53u];
+  if (d < 59u) return f[11u + d - 56u];
+  return d == 63u ? 1.0f : 0.0f;
+}
+
+// ---- packing: FP32 master weights -> permuted bf16 fragments --------------
+
+__global__ void pack_kernel(const float* __restrict__ w, uint4* __restrict__ frags) {
+  const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;  // one fragment (lane) each
+  if (id >= kFrags) return;
+  const uint32_t lane = id & 63u, g = lane >> 4, r = lane & 15u;
+  float v[8];
+  if (id < kHidden * kFragsHidden) {
+    // forward, hidden layer l: A = W_l, row o = ob*16 + r, k = nmap(kb, g, t)
+    const uint32_t l = id / kFragsHidden, rem = id % kFragsHidden;
+    const uint32_t ob = rem / 128u, kb = (rem / 64u) & 1u;
+    for (uint32_t t = 0; t < 8; ++t) v[t] = w[l * 4096u + (ob * 16u + r) * 64u + nmap(kb, g, t)];
+  } else if (id < kFragsFwd) {
+    // forward, output layer: rows 0..2 of W4, zero padding to 16 rows
+    const uint32_t kb = ((id - kHidden * kFragsHidden) / 64u) & 1u;
+    for (uint32_t t = 0; t < 8; ++t) v[t] = r < kOut ? w[16384u + r * 64u + nmap(kb, g, t)] : 0.0f;
+  } else {
+    // backward through W_m (m = 1..4): A = W_m^T, row i = ib*16 + r, k = output neuron
+    const uint32_t b = id - kFragsFwd;
+    const uint32_t m = 1u + b / kFragsHidden, rem = b % kFragsHidden;
+    const uint32_t ib = rem / 128u, kb = (rem / 64u) & 1u;
+    const uint32_t i = ib * 16u + r;
+    for (uint32_t t = 0; t < 8; ++t) {
+      const uint32_t o = nmap(kb, g, t);
+      if (m < 4u)
+        v[t] = w[m * 4096u + o * 64u + i];
+      else
+        v[t] = (kb == 0u && o < kOut) ? w[16384u + o * 64u + i] : 0.0f;
+    }
+  }
+  frags[id] = pack8(v);
+}
+
+// ---- shared forward ------------------------------------------------------------
+
+// Builds the B fragments of the encoded inputs of 4 sample blocks.
+__device__ __forceinline__ void encode_tile(const float* __restrict__ feats, uint32_t n, uint32_t base,
+                                            uint32_t g, uint32_t r, uint4 (&B)[4][2]) {
+#pragma unroll
+  for (uint32_t sb = 0; sb < 4; ++sb) {
+    const uint32_t s = base + sb * 16u + r;
+    float f[kFeat];
+    if (s < n) {
+      const float4* fp = reinterpret_cast<const float4*>(feats + (size_t)s * kFeat);
+#pragma unroll
+      for (uint32_t q = 0; q < 4; ++q) {
+        const float4 x = fp[q];
+        f[4 * q] = x.x, f[4 * q + 1] = x.y, f[4 * q + 2] = x.z, f[4 * q + 3] = x.w;
+      }
+    } else {
+#pragma unroll
+      for (uint32_t q = 0; q < kFeat; ++q) f[q] = 0.0f;
+    }
+#pragma unroll
+    for (uint32_t kb = 0; kb < 2; ++kb) {
+      float v[8];
+#pragma unroll
+      for (uint32_t t = 0; t < 8; ++t) v[t] = s < n ? encode_dim(nmap(kb, g, t), f) : 0.0f;
+      B[sb][kb] = pack8(v);
+    }
+  }
+}
+
+// One hidden layer: B (inputs) -> B (ReLU outputs, bf16); returns the
+// positive-output bit mask (bit sb*16 + kb*8 + t).
+__device__ __forceinline__ uint64_t hidden_layer(const uint4* __restrict__ frags, uint32_t lane,
+                                                 uint4 (&B)[4][2]) {
+  f32x4 C[4][4];
+#pragma unroll
+  for (uint32_t ob = 0; ob < 4; ++ob)
+#pragma unroll
+    for (uint32_t sb = 0; sb < 4; ++sb) C[ob][sb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (uint32_t ob = 0; ob < 4; ++ob)
+#pragma unroll
+    for (uint32_t kb = 0; kb < 2; ++kb) {
+      const bf16x8 a = frag_of(frags[(ob * 2u + kb) * 64u + lane]);
+#pragma unroll
+      for (uint32_t sb = 0; sb < 4; ++sb) C[ob][sb] = mfma(a, frag_of(B[sb][kb]), C[ob][sb]);
+    }
+  uint64_t mask = 0;
+#pragma unroll
+  for (uint32_t sb = 0; sb < 4; ++sb)
+#pragma unroll
+    for (uint32_t kb = 0; kb < 2; ++kb) {
+      float v[8];
+#pragma unroll
+      for (uint32_t t = 0; t < 8; ++t) {
+        const float x = C[2u * kb + (t >> 2)][sb][t & 3u];
+        v[t] = x > 0.0f ? x : 0.0f;
+        if (x > 0.0f) mask |= 1ull << (sb * 16u + kb * 8u + t);
+      }
+      B[sb][kb] = pack8(v);
+    }
+  return mask;
+}
+
+__device__ __forceinline__ void output_layer(const uint4* __restrict__ frags, uint32_t lane,
+                                             const uint4 (&B)[4][2], f32x4 (&Y)[4]) {
+#pragma unroll
+  for (uint32_t sb = 0; sb < 4; ++sb) Y[sb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (uint32_t kb = 0; kb < 2; ++kb) {
+    const bf16x8 a = frag_of(frags[kHidden * kFragsHidden + kb * 64u + lane]);
+#pragma unroll
+    for (uint32_t sb = 0; sb < 4; ++sb) Y[sb] = mfma(a, frag_of(B[sb][kb]), Y[sb]);
+  }
+}
+
+// ---- inference: one wave = 64 samples -------------------------------------------
+
+__global__ __launch_bounds__(256) void infer_kernel(const float* __restrict__ feats, uint32_t n,
+                                                    const uint4* __restrict__ frags, float* __restrict__ rgb) {
+  const uint32_t lane = threadIdx.x & 63u, g = lane >> 4, r = lane & 15u;
+  const uint32_t base = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64u;
+  if (base >= n) return;  // whole wave past the end
+  uint4 B[4][2];
+  encode_tile(feats, n, base, g, r, B);
+#pragma unroll
+  for (uint32_t l = 0; l < kHidden; ++l) (void)hidden_layer(frags + l * kFragsHidden, lane, B);
+  f32x4 Y[4];
+  output_layer(frags, lane, B, Y);
+  if (g == 0u) {
+#pragma unroll
+    for (uint32_t sb = 0; sb < 4; ++sb) {
+      const uint32_t s = base + sb * 16u + r;
+      if (s < n) {
+        rgb[3 * (size_t)s] = Y[sb][0];
+        rgb[3 * (size_t)s + 1] = Y[sb][1];
+        rgb[3 * (size_t)s + 2] = Y[sb][2];
+      }
+    }
+  }
+}
+
+// ---- training: forward, relative-L2 gradient, backward, dW ------------------------
+
+constexpr uint32_t kTile = 64u * 64u;  // one transposed [neuron][sample] bf16 tile
+
+// Stores the B fragments of a layer input transposed: T[n * 64 + s].
+__device__ __forceinline__ void store_t(uint16_t* T, uint32_t g, uint32_t r, const uint4 (&B)[4][2]) {
+#pragma unroll
+  for (uint32_t sb = 0; sb < 4; ++sb)
+#pragma unroll
+    for (uint32_t kb = 0; kb < 2; ++kb) {
+      const uint32_t w[4] = {B[sb][kb].x, B[sb][kb].y, B[sb][kb].z, B[sb][kb].w};
+#pragma unroll
+      for (uint32_t t = 0; t < 8; ++t)
+        T[nmap(kb, g, t) * 64u + sb * 16u + r] = (uint16_t)((w[t >> 1] >> ((t & 1u) * 16u)) & 0xFFFFu);
+    }
+}
+
+// dW (rows: `rows` output neurons in 16-blocks) += D^T H over the 64 samples,
+// with D = DT ([neuron][sample]) and H = HT; accumulated into grad[o*64 + i].
+__device__ __forceinline__ void weight_grad(const uint16_t* DT, const uint16_t* HT, uint32_t row_blocks,
+                                            uint32_t rows, uint32_t g, uint32_t r, float* __restrict__ grad) {
+  for (uint32_t ob = 0; ob < row_blocks; ++ob) {
+    f32x4 C[4];
+#pragma unroll
+    for (uint32_t ib = 0; ib < 4; ++ib) C[ib] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (uint32_t ks = 0; ks < 2; ++ks) {
+      const uint4 a = *reinterpret_cast<const uint4*>(DT + (ob * 16u + r) * 64u + ks * 32u + g * 8u);
+#pragma unroll
+      for (uint32_t ib = 0; ib < 4; ++ib) {
+        const uint4 b = *reinterpret_cast<const uint4*>(HT + (ib * 16u + r) * 64u + ks * 32u + g * 8u);
+        C[ib] = mfma(frag_of(a), frag_of(b), C[ib]);
+      }
+    }
+#pragma unroll
+    for (uint32_t ib = 0; ib < 4; ++ib)
+#pragma unroll
+      for (uint32_t j = 0; j < 4; ++j) {
+        const uint32_t o = ob * 16u + g * 4u + j;
+        if (o < rows) unsafeAtomicAdd(grad + o * 64u + ib * 16u + r, C[ib][j]);
+      }
+  }
+}
+
+// Backward through W_m^T: dH (C layout) = W_m^T dNext, masked by ReLU'.
+__device__ __forceinline__ void backward_layer(const uint4* __restrict__ fr, uint32_t lane, uint32_t kbs,
+                                               uint64_t mask, uint4 (&B)[4][2]) {
+  f32x4 C[4][4];
+#pragma unroll
+  for (uint32_t ib = 0; ib < 4; ++ib)
+#pragma unroll
+    for (uint32_t sb = 0; sb < 4; ++sb) C[ib][sb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (uint32_t ib = 0; ib < 4; ++ib)
+    for (uint32_t kb = 0; kb < kbs; ++kb) {
+      const bf16x8 a = frag_of(fr[(ib * 2u + kb) * 64u + lane]);
+#pragma unroll
+      for (uint32_t sb = 0; sb < 4; ++sb) C[ib][sb] = mfma(a, frag_of(B[sb][kb]), C[ib][sb]);
+    }
+#pragma unroll
+  for (uint32_t sb = 0; sb < 4; ++sb)
+#pragma unroll
+    for (uint32_t kb = 0; kb < 2; ++kb) {
+      float v[8];
+#pragma unroll
+      for (uint32_t t = 0; t < 8; ++t) {
+        const bool on = (mask >> (sb * 16u + kb * 8u + t)) & 1ull;
+        v[t] = on ? C[2u * kb + (t >> 2)][sb][t & 3u] : 0.0f;
+      }
+      B[sb][kb] = pack8(v);
+    }
+}
+
+__global__ __launch_bounds__(64) void train_kernel(const float* __restrict__ feats, const float* __restrict__ targets,
+                                                   uint32_t n, const uint4* __restrict__ frags,
+                                                   float* __restrict__ grad, float* __restrict__ loss) {
+  __shared__ __align__(16) uint16_t HT[5 * kTile];  // inputs of W0..W4, transposed
+  __shared__ __align__(16) uint16_t DT[kTile];      // deltas, transposed
+  const uint32_t lane = threadIdx.x, g = lane >> 4, r = lane & 15u;
+  const uint32_t base = blockIdx.x * 64u;
+  uint4 B[4][2];
+  uint64_t mask[kHidden];
+  encode_tile(feats, n, base, g, r, B);
+  store_t(HT, g, r, B);
+#pragma unroll
+  for (uint32_t l = 0; l < kHidden; ++l) {
+    mask[l] = hidden_layer(frags + l * kFragsHidden, lane, B);
+    store_t(HT + (l + 1u) * kTile, g, r, B);
+  }
+  f32x4 Y[4];
+  output_layer(frags, lane, B, Y);
+
+  // Relative L2 (fusedMlp.wgsl.ts:65-120): d/dy = 2 (y - t) / (3 n (L^2 + eps)),
+  // L = luminance of the prediction, normalised by its largest component.
+  float part = 0.0f;
+  for (uint32_t i = lane; i < 16u * 64u; i += 64u) DT[i] = 0;  // rows 0..15 of dY^T
+#pragma unroll
+  for (uint32_t sb = 0; sb < 4; ++sb) {
+    float d[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const uint32_t s = base + sb * 16u + r;
+    if (g == 0u && s < n) {
+      const float p0 = Y[sb][0], p1 = Y[sb][1], p2 = Y[sb][2];
+      const float t0 = targets[3 * (size_t)s], t1 = targets[3 * (size_t)s + 1], t2 = targets[3 * (size_t)s + 2];
+      const float sc = fmaxf(fabsf(p0), fmaxf(fabsf(p1), fabsf(p2)));
+      float lum = 0.0f;
+      if (sc > 0.0f) {
+        const float nl = (0.299f * (p0 / sc) + 0.587f * (p1 / sc)) + 0.114f * (p2 / sc);
+        lum = fminf(fmaxf(nl, -1.0f), 1.0f) * sc;
+      }
+      const float bl = fminf(fabsf(lum), 1e19f);
+      const float den = bl * bl + 1e-2f;
+      const float inv = 2.0f / (float)n / 3.0f / den;
+      const float p[3] = {p0, p1, p2}, tg[3] = {t0, t1, t2};
+#pragma unroll
+      for (uint32_t c = 0; c < 3; ++c) {
+        const float e = p[c] - tg[c];
+        float x = e * inv;
+        x = x != x ? 0.0f : fminf(fmaxf(x, -65504.0f), 65504.0f);
+        d[c] = x;
+        if (e == e && den == den) part += e * e / den;
+      }
+    }
+    B[sb][0] = pack8(d);
+    const float z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    B[sb][1] = pack8(z);
+  }
+  __syncthreads();
+  if (g == 0u) {
+#pragma unroll
+    for (uint32_t sb = 0; sb < 4; ++sb) {
+      const uint32_t w[2] = {B[sb][0].x, B[sb][0].y};
+      DT[0u * 64u + sb * 16u + r] = (uint16_t)(w[0] & 0xFFFFu);
+      DT[1u * 64u + sb * 16u + r] = (uint16_t)(w[0] >> 16);
+      DT[2u * 64u + sb * 16u + r] = (uint16_t)(w[1] & 0xFFFFu);
+    }
+  }
+  // loss: wave sum, one atomic
+  for (uint32_t off = 32; off > 0; off >>= 1) part += __shfl_xor(part, (int)off);
+  if (lane == 0) unsafeAtomicAdd(loss, part / (3.0f * (float)n));
+  __syncthreads();
+  // dW4 (3 x 64) and dH4
+  weight_grad(DT, HT + 4u * kTile, 1u, kOut, g, r, grad + 16384u);
+  __syncthreads();
+  const uint4* bw = frags + kFragsFwd;
+  backward_layer(bw + 3u * kFragsHidden, lane, 1u, mask[3], B);
+  for (uint32_t m = 3; m >= 1; --m) {
+    store_t(DT, g, r, B);  // dH_{m+1} (pre-activation delta of layer m's output)
+    __syncthreads();
+    weight_grad(DT, HT + m * kTile, 4u, 64u, g, r, grad + m * 4096u);
+    __syncthreads();
+    backward_layer(bw + (m - 1u) * kFragsHidden, lane, 2u, mask[m - 1u], B);
+  }
+  store_t(DT, g, r, B);  // dH1
+  __syncthreads();
+  weight_grad(DT, HT, 4u, 64u, g, r, grad);
+}
+
+// ---- Adam (fusedMlpTrainer.ts:30-84) -------------------------------------------
+
+__device__ __forceinline__ bool finite_(float x) { return x == x && fabsf(x) <= 3.402823e38f; }
+
+__global__ void adam_kernel(float* __restrict__ w, float* __restrict__ grad, float* __restrict__ m,
+                            float* __restrict__ v, AdamParams p) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= kParams) return;
+  const float cur = w[i], om = m[i], ov = v[i], gr = grad[i];
+  grad[i] = 0.0f;
+  if (!finite_(cur) || !finite_(om) || !finite_(ov) || ov < 0.0f) {
+    w[i] = finite_(cur) ? cur : 0.0f;
+    m[i] = 0.0f;
+    v[i] = 0.0f;
+    return;
+  }
+  if (!finite_(gr)) return;
+  const float mi = p.beta1 * om + (1.0f - p.beta1) * gr;
+  const float vi = p.beta2 * ov + (1.0f - p.beta2) * gr * gr;
+  const float mhat = mi / p.bc1, vhat = vi / p.bc2;
+  if (!finite_(mi) || !finite_(vi) || !finite_(mhat) || !finite_(vhat) || vi < 0.0f || vhat < 0.0f) return;
+  const float upd = cur - p.lr * mhat / (sqrtf(vhat) + p.eps);
+  if (!finite_(upd)) return;
+  m[i] = mi;
+  v[i] = vi;
+  w[i] = fminf(fmaxf(upd, -65504.0f), 65504.0f);
+}
+
+__global__ void fill_kernel(float* p, uint32_t n, float value) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = value;
+}
+
+// ---- launchers ----------------------------------------------------------------
+
+hipError_t launch_pack(const float* master, uint4* frags, hipStream_t s) {
+  hipLaunchKernelGGL(pack_kernel, dim3((kFrags + 255) / 256), dim3(256), 0, s, master, frags);
+  return hipGetLastError();
+}
+
+hipError_t launch_infer(const float* feats, uint32_t n, const uint4* frags, float* rgb, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const uint32_t waves = (n + 63u) / 64u;
+  hipLaunchKernelGGL(infer_kernel, dim3((waves + 3u) / 4u), dim3(256), 0, s, feats, n, frags, rgb);
+  return hipGetLastError();
+}
+
+hipError_t launch_train(const float* feats, const float* targets, uint32_t n, const uint4* frags, float* grad,
+                        float* loss, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(train_kernel, dim3((n + 63u) / 64u), dim3(64), 0, s, feats, targets, n, frags, grad, loss);
+  return hipGetLastError();
+}
+
+hipError_t launch_adam(float* master, float* grad, float* m, float* v, AdamParams p, hipStream_t s) {
+  hipLaunchKernelGGL(adam_kernel, dim3((kParams + 255) / 256), dim3(256), 0, s, master, grad, m, v, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(float* p, uint32_t n, float value, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n, value);
+  return hipGetLastError();
+}
+
+}  // namespace vtn

@@ -30,8 +30,19 @@ EXPORTED = [
     "vt_memcpy_h2d", "vt_memcpy_d2h", "vt_memcpy_d2d", "vt_synchronize", "vt_get_stream", "vt_stats",
     "vt_build_blas", "vt_build_tlas", "vt_pack_scene", "vt_pack_field_words",
     "vt_pack_copy_field", "vt_pack_free", "vt_version", "vt_render_frame", "vt_capture",
-    "vt_reset", "vt_frame_time", "vt_capture_signatures", "vt_render_info",
+    "vt_reset", "vt_frame_time", "vt_capture_signatures", "vt_render_info", "vt_invert_mat4",
+    "vt_nrc_param_count", "vt_nrc_reset", "vt_nrc_set_weights", "vt_nrc_get_weights", "vt_nrc_infer",
+    "vt_nrc_infer_device", "vt_nrc_train_step", "vt_nrc_train_step_device", "vt_nrc_last_loss",
+    "vt_nrc_gradient",
 ]
+
+
+class NrcOptions(C.Structure):
+    _fields_ = [("learning_rate", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float),
+                ("epsilon", C.c_float), ("seed", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+
+
+NRC_FEATURES = 16
 
 
 class VtError(RuntimeError):
@@ -312,6 +323,61 @@ class Engine:
         ms = C.c_float()
         check(lib().vt_frame_time(self._h, C.byref(ms)), self._h)
         return ms.value
+
+    # ---- neural radiance cache ----
+    def nrc_param_count(self):
+        n = C.c_uint32()
+        check(lib().vt_nrc_param_count(C.byref(n)))
+        return n.value
+
+    def nrc_reset(self, learning_rate=1e-2, beta1=0.9, beta2=0.999, epsilon=1e-8, seed=1):
+        o = NrcOptions(learning_rate, beta1, beta2, epsilon, seed)
+        check(lib().vt_nrc_reset(self._h, C.byref(o)), self._h)
+
+    def nrc_set_weights(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        check(lib().vt_nrc_set_weights(self._h, w.ctypes.data_as(F32P), C.c_uint32(w.size)), self._h)
+
+    def nrc_get_weights(self):
+        w = np.zeros(self.nrc_param_count(), np.float32)
+        check(lib().vt_nrc_get_weights(self._h, w.ctypes.data_as(F32P), C.c_uint32(w.size)), self._h)
+        return w
+
+    def nrc_infer(self, features):
+        f = np.ascontiguousarray(features, np.float32).reshape(-1, NRC_FEATURES)
+        out = np.zeros((f.shape[0], 3), np.float32)
+        check(lib().vt_nrc_infer(self._h, f.ctypes.data_as(F32P), C.c_uint32(f.shape[0]),
+                                 out.ctypes.data_as(F32P)), self._h)
+        return out
+
+    def nrc_train_step(self, features, targets):
+        f = np.ascontiguousarray(features, np.float32).reshape(-1, NRC_FEATURES)
+        t = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
+        loss = C.c_float()
+        check(lib().vt_nrc_train_step(self._h, f.ctypes.data_as(F32P), t.ctypes.data_as(F32P),
+                                      C.c_uint32(f.shape[0]), C.byref(loss)), self._h)
+        return loss.value
+
+    def nrc_gradient(self, features, targets):
+        f = np.ascontiguousarray(features, np.float32).reshape(-1, NRC_FEATURES)
+        t = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
+        g = np.zeros(self.nrc_param_count(), np.float32)
+        loss = C.c_float()
+        check(lib().vt_nrc_gradient(self._h, f.ctypes.data_as(F32P), t.ctypes.data_as(F32P),
+                                    C.c_uint32(f.shape[0]), g.ctypes.data_as(F32P), C.byref(loss)), self._h)
+        return loss.value, g
+
+    def nrc_infer_device(self, d_features, n, d_rgb):
+        check(lib().vt_nrc_infer_device(self._h, C.c_void_p(d_features), C.c_uint32(n), C.c_void_p(d_rgb)), self._h)
+
+    def nrc_train_step_device(self, d_features, d_targets, n):
+        check(lib().vt_nrc_train_step_device(self._h, C.c_void_p(d_features), C.c_void_p(d_targets),
+                                             C.c_uint32(n)), self._h)
+
+    def nrc_last_loss(self):
+        loss = C.c_float()
+        check(lib().vt_nrc_last_loss(self._h, C.byref(loss)), self._h)
+        return loss.value
 
     def render_info(self):
         n = C.c_uint32()
