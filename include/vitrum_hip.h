@@ -309,6 +309,14 @@ vt_status vt_nrc_last_loss(vt_engine *eng, float *loss);
 vt_status vt_nrc_gradient(vt_engine *eng, const float *features, const float *targets, uint32_t n,
                           float *grad, float *loss);
 
+/* Path termination by the cache in vt_render_frame (nrcSubsystem.ts): with
+ * `enabled`, each image path ends at the vertex after its first
+ * diffuse-dominant vertex (specular lobe probability < 1/2) and adds
+ * throughput x cache prediction there; `train_paths` extra self-training
+ * paths per frame (pseudo-random pixels, full length) record up to 4 vertices
+ * each and drive one Adam step per frame.  Resets accumulation. */
+vt_status vt_nrc_render_config(vt_engine *eng, int32_t enabled, uint32_t train_paths);
+
 /* invertMat4 (packages/shared-bvh/src/mathUtils.ts:21-110): binary64
  * cofactor inverse of a column-major mat4 rounded to binary32, with the
  * reciprocal-product check.  VT_ERR_RANGE when the matrix is not invertible.

@@ -1,0 +1,75 @@
+"""The cache inside the path tracer: online training from self-training paths
+and the gain over the plain estimator.
+
+Relative RMSE is the luminance RMSE (Rec. 709 weights) of an image against a
+high-spp plain reference, divided by the reference's mean luminance.
+"""
+import numpy as np
+import pytest
+
+from vitrum_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+SEED = 0x9E3779B9
+
+
+def lum(img):
+    return 0.2126 * img[..., 0] + 0.7152 * img[..., 1] + 0.0722 * img[..., 2]
+
+
+def rel_rmse(img, ref):
+    return float(np.sqrt(np.mean((lum(img) - lum(ref)) ** 2)) / np.mean(lum(ref)))
+
+
+def render(engine, ivp, res, frames, start=0):
+    engine.reset()
+    for f in range(frames):
+        engine.render_frame(ivp, res, res, start + f, SEED, 8)
+    return engine.capture(res, res)[..., :3].astype(np.float64)
+
+
+@pytest.fixture(scope="module")
+def cornell(engine):
+    sc = W.cornell_scene()
+    engine.set_scene(sc)
+    res = 96
+    ivp = W.inv_view_proj(sc["camera"], res, res)
+    engine.nrc_render_config(False)
+    ref = render(engine, ivp, res, 1024, start=100000)
+    return sc, res, ivp, ref
+
+
+def test_plain_path_unchanged_when_cache_disabled(engine, cornell):
+    sc, res, ivp, ref = cornell
+    engine.nrc_render_config(False)
+    a = render(engine, ivp, res, 4)
+    engine.nrc_render_config(True, 4096)
+    engine.nrc_render_config(False)
+    b = render(engine, ivp, res, 4)
+    assert np.array_equal(a, b)
+
+
+def test_cache_trains_online_and_beats_plain_at_equal_spp(engine, cornell):
+    sc, res, ivp, ref = cornell
+    engine.nrc_render_config(False)
+    plain = render(engine, ivp, res, 4, start=500)
+    engine.nrc_reset(seed=3)
+    engine.nrc_render_config(True, 8192)
+    losses = []
+    engine.reset()
+    for f in range(96):
+        engine.render_frame(ivp, res, res, 1000 + f, SEED, 8)
+        if f in (0, 1, 2, 93, 94, 95):
+            losses.append(engine.nrc_last_loss())
+    assert np.all(np.isfinite(losses))
+    assert np.mean(losses[3:]) < 0.5 * np.mean(losses[:3]), losses
+    nrc = render(engine, ivp, res, 4, start=2000)
+    assert np.all(np.isfinite(nrc)) and nrc.min() >= 0
+    e_plain, e_nrc = rel_rmse(plain, ref), rel_rmse(nrc, ref)
+    # the cache replaces the noisy path suffix: lower error at equal spp, with
+    # a bias small enough not to dominate at 4 spp
+    assert e_nrc < e_plain, (e_nrc, e_plain)
+    bias = abs(np.mean(lum(nrc)) - np.mean(lum(ref))) / np.mean(lum(ref))
+    assert bias < 0.1, bias
+    engine.nrc_render_config(False)

@@ -65,3 +65,9 @@ vt_status engine_upload(vt_engine* e, DeviceBuffer& dst, const void* src, size_t
 vt_status engine_on_scene_changed(vt_engine* e);
 void engine_release_render(vt_engine* e);
 void engine_release_nrc(vt_engine* e);
+// Cache hooks for the integrator (nrc.cpp): counts are device words.
+vt_status engine_nrc_ensure(vt_engine* e);
+vt_status engine_nrc_infer_counted(vt_engine* e, const float* d_feat, const uint32_t* d_count, float* d_rgb,
+                                   uint32_t capacity);
+vt_status engine_nrc_train_counted(vt_engine* e, const float* d_feat, const float* d_tgt, const uint32_t* d_count,
+                                   uint32_t capacity);

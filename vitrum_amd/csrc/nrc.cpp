@@ -95,6 +95,27 @@ static bool finite_features(const float* f, size_t n) {
   return true;
 }
 
+static vt_status nrc_apply_adam(vt_engine* e);
+static vt_status nrc_accumulate(vt_engine* e, const float* d_features, const float* d_targets, uint32_t n,
+                                const uint32_t* d_count = nullptr);
+
+vt_status engine_nrc_ensure(vt_engine* e) { return nrc_ensure(e); }
+
+vt_status engine_nrc_infer_counted(vt_engine* e, const float* d_feat, const uint32_t* d_count, float* d_rgb,
+                                   uint32_t capacity) {
+  if (vt_status s = nrc_ensure(e)) return s;
+  NRC_HIP(e, vtn::launch_infer(d_feat, capacity, d_count, (const uint4*)e->nrc->frags.ptr, d_rgb, e->stream));
+  return VT_OK;
+}
+
+vt_status engine_nrc_train_counted(vt_engine* e, const float* d_feat, const float* d_tgt, const uint32_t* d_count,
+                                   uint32_t capacity) {
+  if (vt_status s = nrc_ensure(e)) return s;
+  if (capacity == 0) return VT_OK;
+  if (vt_status s = nrc_accumulate(e, d_feat, d_tgt, capacity, d_count)) return s;
+  return nrc_apply_adam(e);
+}
+
 extern "C" {
 
 vt_status vt_nrc_param_count(uint32_t* count) {
@@ -142,7 +163,7 @@ vt_status vt_nrc_infer_device(vt_engine* e, const float* d_features, uint32_t n,
   if (!e || (n && (!d_features || !d_rgb))) return VT_ERR_INVALID_ARGUMENT;
   if (vt_status s = nrc_ensure(e)) return s;
   NRC_HIP(e, hipSetDevice(e->device));
-  NRC_HIP(e, vtn::launch_infer(d_features, n, (const uint4*)e->nrc->frags.ptr, d_rgb, e->stream));
+  NRC_HIP(e, vtn::launch_infer(d_features, n, nullptr, (const uint4*)e->nrc->frags.ptr, d_rgb, e->stream));
   return VT_OK;
 }
 
@@ -162,11 +183,12 @@ vt_status vt_nrc_infer(vt_engine* e, const float* features, uint32_t n, float* r
 
 // Accumulates the batch gradient into the engine's gradient buffer and the
 // batch loss into loss[0] (both device memory of the engine).
-static vt_status nrc_accumulate(vt_engine* e, const float* d_features, const float* d_targets, uint32_t n) {
+static vt_status nrc_accumulate(vt_engine* e, const float* d_features, const float* d_targets, uint32_t n,
+                                const uint32_t* d_count) {
   NrcState& s = *e->nrc;
   NRC_HIP(e, hipSetDevice(e->device));
   NRC_HIP(e, vtn::launch_fill((float*)s.loss.ptr, 1, 0.0f, e->stream));
-  NRC_HIP(e, vtn::launch_train(d_features, d_targets, n, (const uint4*)s.frags.ptr, (float*)s.grad.ptr,
+  NRC_HIP(e, vtn::launch_train(d_features, d_targets, n, d_count, (const uint4*)s.frags.ptr, (float*)s.grad.ptr,
                                (float*)s.loss.ptr, e->stream));
   return VT_OK;
 }

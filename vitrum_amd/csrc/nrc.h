@@ -33,10 +33,12 @@ struct AdamParams {
 };
 
 hipError_t launch_pack(const float* master, uint4* frags, hipStream_t s);
-hipError_t launch_infer(const float* feats, uint32_t n, const uint4* frags, float* rgb, hipStream_t s);
+// When n_ptr is set the sample count is read from device memory (at most n).
+hipError_t launch_infer(const float* feats, uint32_t n, const uint32_t* n_ptr, const uint4* frags, float* rgb,
+                        hipStream_t s);
 // grad += d loss / d W over the batch; loss[0] += relative-L2 loss of the batch.
-hipError_t launch_train(const float* feats, const float* targets, uint32_t n, const uint4* frags,
-                        float* grad, float* loss, hipStream_t s);
+hipError_t launch_train(const float* feats, const float* targets, uint32_t n, const uint32_t* n_ptr,
+                        const uint4* frags, float* grad, float* loss, hipStream_t s);
 hipError_t launch_adam(float* master, float* grad, float* m, float* v, AdamParams p, hipStream_t s);
 hipError_t launch_fill(float* p, uint32_t n, float value, hipStream_t s);
 

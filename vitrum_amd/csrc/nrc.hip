@@ -192,8 +192,10 @@ __device__ __forceinline__ void output_layer(const uint4* __restrict__ frags, ui
 
 // ---- inference: one wave = 64 samples -------------------------------------------
 
-__global__ __launch_bounds__(256) void infer_kernel(const float* __restrict__ feats, uint32_t n,
+__global__ __launch_bounds__(256) void infer_kernel(const float* __restrict__ feats, uint32_t cap,
+                                                    const uint32_t* __restrict__ n_ptr,
                                                     const uint4* __restrict__ frags, float* __restrict__ rgb) {
+  const uint32_t n = n_ptr ? min(*n_ptr, cap) : cap;
   const uint32_t lane = threadIdx.x & 63u, g = lane >> 4, r = lane & 15u;
   const uint32_t base = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64u;
   if (base >= n) return;  // whole wave past the end
@@ -290,8 +292,11 @@ __device__ __forceinline__ void backward_layer(const uint4* __restrict__ fr, uin
 }
 
 __global__ __launch_bounds__(64) void train_kernel(const float* __restrict__ feats, const float* __restrict__ targets,
-                                                   uint32_t n, const uint4* __restrict__ frags,
+                                                   uint32_t cap, const uint32_t* __restrict__ n_ptr,
+                                                   const uint4* __restrict__ frags,
                                                    float* __restrict__ grad, float* __restrict__ loss) {
+  const uint32_t n = n_ptr ? min(*n_ptr, cap) : cap;
+  if (blockIdx.x * 64u >= n) return;  // whole block past the end (uniform)
   __shared__ __align__(16) uint16_t HT[5 * kTile];  // inputs of W0..W4, transposed
   __shared__ __align__(16) uint16_t DT[kTile];      // deltas, transposed
   const uint32_t lane = threadIdx.x, g = lane >> 4, r = lane & 15u;
@@ -413,17 +418,19 @@ hipError_t launch_pack(const float* master, uint4* frags, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_infer(const float* feats, uint32_t n, const uint4* frags, float* rgb, hipStream_t s) {
+hipError_t launch_infer(const float* feats, uint32_t n, const uint32_t* n_ptr, const uint4* frags, float* rgb,
+                        hipStream_t s) {
   if (n == 0) return hipSuccess;
   const uint32_t waves = (n + 63u) / 64u;
-  hipLaunchKernelGGL(infer_kernel, dim3((waves + 3u) / 4u), dim3(256), 0, s, feats, n, frags, rgb);
+  hipLaunchKernelGGL(infer_kernel, dim3((waves + 3u) / 4u), dim3(256), 0, s, feats, n, n_ptr, frags, rgb);
   return hipGetLastError();
 }
 
-hipError_t launch_train(const float* feats, const float* targets, uint32_t n, const uint4* frags, float* grad,
-                        float* loss, hipStream_t s) {
+hipError_t launch_train(const float* feats, const float* targets, uint32_t n, const uint32_t* n_ptr,
+                        const uint4* frags, float* grad, float* loss, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(train_kernel, dim3((n + 63u) / 64u), dim3(64), 0, s, feats, targets, n, frags, grad, loss);
+  hipLaunchKernelGGL(train_kernel, dim3((n + 63u) / 64u), dim3(64), 0, s, feats, targets, n, n_ptr, frags, grad,
+                     loss);
   return hipGetLastError();
 }
 

@@ -92,17 +92,23 @@ struct RenderState {
   float env_radiance[3] = {0, 0, 0};
   float ray_bias = 0, tmin = 0;
   // Per-resolution path state.
-  uint32_t width = 0, height = 0;
+  uint32_t width = 0, height = 0, paths = 0;
   DeviceBuffer rng, throughput, conn, radiance, prev_pdf, flags, signature, count;
   DeviceBuffer queue[2], rays[2], hits, hit_pos;
   DeviceBuffer shadow_path, shadow_rays, shadow_contrib, shadow_occ, accum;
+  // Neural radiance cache (used when nrc_enabled).
+  bool nrc_enabled = false;
+  uint32_t nrc_train_paths = 65536;
+  float scene_min[3] = {0, 0, 0}, scene_inv_extent[3] = {1, 1, 1};
+  DeviceBuffer nrc_count, q_feat, q_thr, q_path, q_rgb, train_rec, train_n, train_feat, train_tgt;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   void release_paths() {
     for (DeviceBuffer* b : {&rng, &throughput, &conn, &radiance, &prev_pdf, &flags, &signature, &count,
                             &queue[0], &queue[1], &rays[0], &rays[1], &hits, &hit_pos, &shadow_path,
-                            &shadow_rays, &shadow_contrib, &shadow_occ, &accum})
+                            &shadow_rays, &shadow_contrib, &shadow_occ, &accum, &nrc_count, &q_feat, &q_thr,
+                            &q_path, &q_rgb, &train_rec, &train_n, &train_feat, &train_tgt})
       b->release();
-    width = height = 0;
+    width = height = paths = 0;
   }
   void release_scene() {
     for (DeviceBuffer* b : {&materials, &emitters, &instances, &env_cdf, &env_pdf, &tri_source}) b->release();
@@ -226,6 +232,12 @@ vt_status engine_on_scene_changed(vt_engine* e) {
     if (!(radius > 0)) radius = std::sqrt(3.0);
   }
   if (!std::isfinite(center[0])) center[0] = center[1] = center[2] = 0;
+  for (int k = 0; k < 3; ++k) {
+    const double lo = P.world_min[k], hi = P.world_max[k];
+    const bool ok = std::isfinite(lo) && std::isfinite(hi) && hi > lo;
+    R.scene_min[k] = ok ? (float)lo : 0.0f;
+    R.scene_inv_extent[k] = ok ? (float)(1.0 / (hi - lo)) : 1.0f;
+  }
   R.ray_bias = ray_origin_bias(radius, center);
   R.tmin = std::max(R.ray_bias * 0.1f, 1.175494351e-38f);
   vt_status s = VT_OK;
@@ -242,9 +254,12 @@ vt_status engine_on_scene_changed(vt_engine* e) {
 
 static vt_status ensure_paths(vt_engine* e, uint32_t w, uint32_t h) {
   RenderState& R = *e->render;
-  if (R.width == w && R.height == h) return VT_OK;
+  const uint32_t train = R.nrc_enabled ? R.nrc_train_paths : 0u;
+  const uint32_t paths = w * h + train;
+  if (R.width == w && R.height == h && R.paths == paths) return VT_OK;
   R.release_paths();
-  const size_t n = (size_t)w * h;
+  const size_t n = (size_t)paths;
+  const size_t pix = (size_t)w * h;
   vt_status s = VT_OK;
   auto need = [&](DeviceBuffer& b, size_t bytes) {
     if (!s) s = b.ensure(e, std::max<size_t>(bytes, 64));
@@ -267,14 +282,26 @@ static vt_status ensure_paths(vt_engine* e, uint32_t w, uint32_t h) {
   need(R.shadow_rays, n * 32);
   need(R.shadow_contrib, n * 12);
   need(R.shadow_occ, n * 4);
-  need(R.accum, n * 16);
+  need(R.accum, pix * 16);
+  if (train) {
+    need(R.nrc_count, 64);
+    need(R.q_feat, pix * 64);
+    need(R.q_thr, pix * 12);
+    need(R.q_path, pix * 4);
+    need(R.q_rgb, pix * 12);
+    need(R.train_rec, (size_t)train * vtr::kTrainSlots * vtr::kTrainRec * 4);
+    need(R.train_n, (size_t)train * 4);
+    need(R.train_feat, (size_t)train * vtr::kTrainSlots * 64);
+    need(R.train_tgt, (size_t)train * vtr::kTrainSlots * 12);
+  }
   if (s) {
     R.release_paths();
     return s;
   }
   R.width = w;
   R.height = h;
-  HIP_CHECK(e, vtr::launch_zero_f32((float*)R.accum.ptr, (uint32_t)(n * 4), e->stream));
+  R.paths = paths;
+  HIP_CHECK(e, vtr::launch_zero_f32((float*)R.accum.ptr, (uint32_t)(pix * 4), e->stream));
   e->samples_accumulated = 0;
   return VT_OK;
 }
@@ -324,6 +351,10 @@ vt_status vt_render_frame(vt_engine* e, const vt_frame* fr) {
   f.bounce_limit = std::max(1u, std::min(mb, 8u));
   f.ray_bias = R.ray_bias;
   f.tmin = R.tmin;
+  f.pixels = fr->width * fr->height;
+  f.nrc = R.nrc_enabled ? 1u : 0u;
+  f.train_paths = R.nrc_enabled ? R.nrc_train_paths : 0u;
+  for (int k = 0; k < 3; ++k) f.scene_min[k] = R.scene_min[k], f.scene_inv_extent[k] = R.scene_inv_extent[k];
   vtr::ShadeScene sc{};
   sc.positions = (const float4*)e->scene.positions.ptr;
   sc.normals = (const float4*)e->scene.normals.ptr;
@@ -356,9 +387,22 @@ vt_status vt_render_frame(vt_engine* e, const vt_frame* fr) {
   ps.shadow_contrib = (float*)R.shadow_contrib.ptr;
   ps.shadow_occ = (uint32_t*)R.shadow_occ.ptr;
   ps.accum = (float*)R.accum.ptr;
-  const uint32_t cap = fr->width * fr->height;
+  ps.nrc_count = (uint32_t*)R.nrc_count.ptr;
+  ps.q_feat = (float*)R.q_feat.ptr;
+  ps.q_thr = (float*)R.q_thr.ptr;
+  ps.q_path = (uint32_t*)R.q_path.ptr;
+  ps.q_rgb = (float*)R.q_rgb.ptr;
+  ps.train_rec = (float*)R.train_rec.ptr;
+  ps.train_n = (uint32_t*)R.train_n.ptr;
+  ps.train_feat = (float*)R.train_feat.ptr;
+  ps.train_tgt = (float*)R.train_tgt.ptr;
+  const uint32_t cap = R.paths;
+  if (f.nrc) {
+    if (vt_status s = engine_nrc_ensure(e)) return s;
+  }
   HIP_CHECK(e, hipEventRecord(R.ev0, e->stream));
   HIP_CHECK(e, vtk::launch_zero((uint32_t*)R.count.ptr, 16, e->stream));
+  if (f.nrc) HIP_CHECK(e, vtk::launch_zero((uint32_t*)R.nrc_count.ptr, 16, e->stream));
   int cur = 0;
   ps.queue = (uint32_t*)R.queue[cur].ptr;
   ps.rays = (float4*)R.rays[cur].ptr;
@@ -387,7 +431,19 @@ vt_status vt_render_frame(vt_engine* e, const vt_frame* fr) {
     HIP_CHECK(e, vtr::launch_reset_counts(ps, e->stream));
     cur ^= 1;
   }
+  if (f.nrc) {
+    // Cache queries of the image paths, then the online training step on the
+    // self-training paths' recorded vertices.
+    if (vt_status s = engine_nrc_infer_counted(e, ps.q_feat, ps.nrc_count, ps.q_rgb, f.pixels)) return s;
+    HIP_CHECK(e, vtr::launch_nrc_apply(ps, f.pixels, e->stream));
+  }
   HIP_CHECK(e, vtr::launch_accumulate(f, ps, e->stream));
+  if (f.nrc) {
+    HIP_CHECK(e, vtr::launch_nrc_targets(f, ps, e->stream));
+    if (vt_status s = engine_nrc_train_counted(e, ps.train_feat, ps.train_tgt, ps.nrc_count + 1,
+                                               f.train_paths * vtr::kTrainSlots))
+      return s;
+  }
   HIP_CHECK(e, hipEventRecord(R.ev1, e->stream));
   e->samples_accumulated += 1;
   e->frame_timed = true;
@@ -443,6 +499,19 @@ vt_status vt_frame_time(vt_engine* e, float* ms) {
   HIP_CHECK(e, hipEventSynchronize(e->render->ev1));
   HIP_CHECK(e, hipEventElapsedTime(ms, e->render->ev0, e->render->ev1));
   e->last_frame_ms = *ms;
+  return VT_OK;
+}
+
+vt_status vt_nrc_render_config(vt_engine* e, int32_t enabled, uint32_t train_paths) {
+  if (!e) return VT_ERR_INVALID_ARGUMENT;
+  if (enabled && (train_paths == 0 || train_paths > (1u << 22)))
+    return rerr(e, VT_ERR_RANGE, "train_paths must be in [1, 2^22]");
+  if (!e->render) return rerr(e, VT_ERR_STATE, "no scene is set");
+  RenderState& R = *e->render;
+  R.nrc_enabled = enabled != 0;
+  if (enabled) R.nrc_train_paths = train_paths;
+  R.width = R.height = 0;  // re-size path state on the next frame
+  e->samples_accumulated = 0;
   return VT_OK;
 }
 

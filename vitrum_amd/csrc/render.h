@@ -29,12 +29,20 @@ struct ShadeScene {
   float env_radiance[3];
 };
 
+constexpr uint32_t kTrainSlots = 4;   // recorded vertices per self-training path
+constexpr uint32_t kTrainRec = 22;     // 16 feature floats + throughput (3) + radiance (3)
+
 struct FrameParams {
   float inv_vp[16];
   uint32_t width, height;
   uint32_t frame_seed, frame_index;
   uint32_t bounce_limit;
   float ray_bias, tmin;
+  // Neural radiance cache: paths [0, pixels) are image paths, the next
+  // train_paths are self-training paths started at pseudo-random pixels.
+  uint32_t nrc;
+  uint32_t pixels, train_paths;
+  float scene_min[3], scene_inv_extent[3];
 };
 
 // Structure-of-arrays path state, one path per pixel.
@@ -61,6 +69,16 @@ struct PathState {
   uint32_t* shadow_occ;
   // Accumulation.
   float* accum;           // 4 per pixel (rgb, unused)
+  // Neural radiance cache.
+  uint32_t* nrc_count;    // [0] cache queries, [1] training samples
+  float* q_feat;          // 16 per query: features of the query vertex
+  float* q_thr;           // 3 per query: throughput arriving at it
+  uint32_t* q_path;       // path of each query
+  float* q_rgb;           // 3 per query: cache prediction
+  float* train_rec;       // kTrainSlots x kTrainRec per training path
+  uint32_t* train_n;      // recorded vertices per training path
+  float* train_feat;      // 16 per training sample (compacted)
+  float* train_tgt;       // 3 per training sample
 };
 
 hipError_t launch_camera(const FrameParams& f, const PathState& s, hipStream_t st);
@@ -68,6 +86,10 @@ hipError_t launch_shade(const FrameParams& f, const ShadeScene& sc, const PathSt
                         uint32_t vertex, uint32_t capacity, hipStream_t st);
 hipError_t launch_shadow_resolve(const PathState& s, uint32_t capacity, hipStream_t st);
 hipError_t launch_accumulate(const FrameParams& f, const PathState& s, hipStream_t st);
+// Adds throughput x cache prediction to every queried path.
+hipError_t launch_nrc_apply(const PathState& s, uint32_t capacity, hipStream_t st);
+// Turns the recorded vertices of the training paths into (features, target) samples.
+hipError_t launch_nrc_targets(const FrameParams& f, const PathState& s, hipStream_t st);
 hipError_t launch_reset_counts(const PathState& s, hipStream_t st);
 hipError_t launch_zero_f32(float* p, uint32_t n, hipStream_t st);
 

@@ -340,14 +340,64 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool want) {
   return want ? base + (uint32_t)__popcll(m & below) : kMiss;
 }
 
+// Same, callable under divergent control flow: the lanes that reach the call
+// take part, the first wanting lane does the atomic.
+__device__ __forceinline__ uint32_t wave_append_active(uint32_t* counter, bool want) {
+  const uint64_t m = __ballot(want);
+  if (m == 0ull) return kMiss;
+  const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1u;
+  const uint32_t lane = __lane_id();
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+  base = __shfl(base, (int)leader);
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64u - lane));
+  return want ? base + (uint32_t)__popcll(m & below) : kMiss;
+}
+
+// (theta / pi, phi / (2 pi) + 1/2) of a unit vector.
+__device__ __forceinline__ void sph01(F3 d, float& a, float& b) {
+  a = acosf(clampf(d.y, -1.0f, 1.0f)) * kInvPi;
+  b = atan2f(d.z, d.x) * kInv2Pi + 0.5f;
+}
+
+// Cache input record of a surface vertex (layout in nrc.h).
+__device__ void nrc_features(const FrameParams& f, F3 hp, F3 wo, F3 nrm, const Mat& m, float* out) {
+  out[0] = clampf((hp.x - f.scene_min[0]) * f.scene_inv_extent[0], 0.0f, 1.0f);
+  out[1] = clampf((hp.y - f.scene_min[1]) * f.scene_inv_extent[1], 0.0f, 1.0f);
+  out[2] = clampf((hp.z - f.scene_min[2]) * f.scene_inv_extent[2], 0.0f, 1.0f);
+  sph01(wo, out[3], out[4]);
+  sph01(nrm, out[5], out[6]);
+  out[7] = m.rough;
+  const float kd = 1.0f - m.metal;
+  out[8] = m.base.x * kd;
+  out[9] = m.base.y * kd;
+  out[10] = m.base.z * kd;
+  const F3 f0 = spec_f0(m);
+  out[11] = f0.x;
+  out[12] = f0.y;
+  out[13] = f0.z;
+  out[14] = 0.0f;
+  out[15] = 0.0f;
+}
+
 // ---- kernels -------------------------------------------------------------
 
 // generatePrimaryRay / unproject_ray_common (kernelCore.wgsl.ts:22-26; common.wgsl.ts:412-469)
 __global__ void camera_kernel(FrameParams f, PathState s) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= f.width * f.height) return;
-  const uint32_t px = p % f.width, py = p / f.width;
-  uint32_t rng = pcg_init(px, py, f.frame_seed ^ f.frame_index);
+  const uint32_t total = f.pixels + (f.nrc ? f.train_paths : 0u);
+  if (p >= total) return;
+  uint32_t pix = p, key = f.frame_seed ^ f.frame_index;
+  const bool train = p >= f.pixels;
+  if (train) {
+    // Self-training path: a pseudo-random pixel and its own PCG stream.
+    uint32_t h = pcg_init(p - f.pixels, f.frame_index, f.frame_seed ^ 0x5BD1E995u);
+    pix = pcg_next(h) % f.pixels;
+    key ^= 0xA511E9B3u;
+    s.train_n[p - f.pixels] = 0u;
+  }
+  const uint32_t px = pix % f.width, py = pix / f.width;
+  uint32_t rng = pcg_init(px, py, key);
   const float jx = rand01(rng), jy = rand01(rng);
   const float u = ((float)px + jx) / (float)f.width, v = ((float)py + jy) / (float)f.height;
   const float nx = u * 2.0f - 1.0f, ny = 1.0f - v * 2.0f;
@@ -379,12 +429,12 @@ __global__ void camera_kernel(FrameParams f, PathState s) {
   store3(s.conn + 3 * (size_t)p, f3(0, 0, 0));
   store3(s.radiance + 3 * (size_t)p, f3(0, 0, 0));
   s.prev_pdf[p] = 0.0f;
-  s.flags[p] = ok ? 1u : 0u;  // a failed unprojection ends the path with no radiance
+  s.flags[p] = ok ? (train ? 5u : 1u) : 0u;  // bit 0 alive, bit 2 training path
   s.signature[p] = 0x1234ull;
   s.queue[p] = p;
   s.rays[2 * (size_t)p] = make_float4(o.x, o.y, o.z, f.tmin);
   s.rays[2 * (size_t)p + 1] = make_float4(d.x, d.y, d.z, f32_max());
-  if (p == 0) s.count[0] = f.width * f.height;
+  if (p == 0) s.count[0] = total;
 }
 
 // One vertex of every queued path: MIS connection of the previous BSDF sample,
@@ -466,6 +516,30 @@ __global__ __launch_bounds__(256) void shade_kernel(FrameParams f, ShadeScene sc
         const F3 hp = load3(s.hit_pos + 3 * (size_t)i);
         const F3 wo = f3(-dir.x, -dir.y, -dir.z);
         if (!prev_mis && (front || m.two_sided)) L = L + T * m.emit;
+        // Neural radiance cache: an image path that has passed a
+        // diffuse-dominant vertex ends here with a cache query (its
+        // prediction times T is added after the bounce loop); a training path
+        // records the vertex (features, T, radiance so far) for its target.
+        bool query = false;
+        if (f.nrc) {
+          const bool train = (flags & 4u) != 0u;
+          query = !train && (flags & 2u) != 0u;
+          if (train && vertex < kTrainSlots) {
+            float* rec = s.train_rec + ((size_t)(p - f.pixels) * kTrainSlots + vertex) * kTrainRec;
+            nrc_features(f, hp, wo, nrm, m, rec);
+            store3(rec + 16, T);
+            store3(rec + 19, L);
+            s.train_n[p - f.pixels] = vertex + 1u;
+          }
+          const uint32_t qi = wave_append_active(s.nrc_count, query);
+          if (query) {
+            nrc_features(f, hp, wo, nrm, m, s.q_feat + 16 * (size_t)qi);
+            store3(s.q_thr + 3 * (size_t)qi, T);
+            s.q_path[qi] = p;
+            flags = 0u;
+          }
+        }
+        if (!query) {
         // 3. NEE over one uniformly selected light (mediumNee.wgsl.ts:14-40;
         //    kernel.wgsl.ts:1496-1690).
         const uint32_t n_light = sc.emitter_count + (sc.env_present ? 1u : 0u);
@@ -582,13 +656,15 @@ __global__ __launch_bounds__(256) void shade_kernel(FrameParams f, ShadeScene sc
         T = C2 * rr_mul;
         store3(s.conn + 3 * (size_t)p, C2);
         s.prev_pdf[p] = allows ? mpdf : 0.0f;
-        flags = lives ? 1u : 0u;
+        // bit 0 alive, bit 1 a diffuse-dominant vertex has been passed, bit 2 training path
+        flags = (flags & 4u) | (lives ? 1u : 0u) | (((flags & 2u) != 0u || sp < 0.5f) ? 2u : 0u);
         if (allows) {
           const F3 no = hp + nrm * f.ray_bias;
           ext_a = make_float4(no.x, no.y, no.z, f.tmin);
           ext_b = make_float4(wi.x, wi.y, wi.z, f32_max());
           want_ext = true;
         }
+        }  // !query
       }
       s.rng[p] = rng;
       store3(s.throughput + 3 * (size_t)p, T);
@@ -626,6 +702,49 @@ __global__ void shadow_resolve_kernel(PathState s) {
   }
 }
 
+__global__ void nrc_apply_kernel(PathState s) {
+  const uint32_t n = s.nrc_count[0];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t p = s.q_path[i];
+    const F3 t = load3(s.q_thr + 3 * (size_t)i), y = load3(s.q_rgb + 3 * (size_t)i);
+    const F3 c = f3(fmaxf(y.x, 0.0f), fmaxf(y.y, 0.0f), fmaxf(y.z, 0.0f));
+    store3(s.radiance + 3 * (size_t)p, load3(s.radiance + 3 * (size_t)p) + t * c);
+  }
+}
+
+// Target of a recorded vertex: (L_final - L_at_vertex) / T_at_vertex, the
+// radiance scattered at that vertex toward the previous one.
+__global__ void nrc_targets_kernel(FrameParams f, PathState s) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  bool any = false;
+  uint32_t nrec = 0;
+  F3 L = f3(0, 0, 0);
+  if (t < f.train_paths) {
+    nrec = s.train_n[t];
+    L = load3(s.radiance + 3 * (size_t)(f.pixels + t));
+    any = nrec > 0u;
+  }
+  for (uint32_t k = 0; k < kTrainSlots; ++k) {
+    bool want = false;
+    const float* rec = nullptr;
+    F3 tg = f3(0, 0, 0);
+    if (any && k < nrec) {
+      rec = s.train_rec + ((size_t)t * kTrainSlots + k) * kTrainRec;
+      const F3 T = load3(rec + 16), Lr = load3(rec + 19);
+      if (T.x > 0.0f && T.y > 0.0f && T.z > 0.0f) {
+        tg = f3(fmaxf(L.x - Lr.x, 0.0f) / T.x, fmaxf(L.y - Lr.y, 0.0f) / T.y, fmaxf(L.z - Lr.z, 0.0f) / T.z);
+        want = tg.x <= 3.0e38f && tg.y <= 3.0e38f && tg.z <= 3.0e38f;
+      }
+    }
+    const uint32_t j = wave_append(s.nrc_count + 1, want);
+    if (want) {
+      float* out = s.train_feat + 16 * (size_t)j;
+      for (uint32_t q = 0; q < 16; ++q) out[q] = rec[q];
+      store3(s.train_tgt + 3 * (size_t)j, tg);
+    }
+  }
+}
+
 __global__ void accumulate_kernel(FrameParams f, PathState s) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= f.width * f.height) return;
@@ -652,7 +771,7 @@ __global__ void zero_f32_kernel(float* p, uint32_t n) {
 }
 
 hipError_t launch_camera(const FrameParams& f, const PathState& s, hipStream_t st) {
-  const uint32_t n = f.width * f.height;
+  const uint32_t n = f.pixels + (f.nrc ? f.train_paths : 0u);
   hipLaunchKernelGGL(camera_kernel, dim3((n + 255) / 256), dim3(256), 0, st, f, s);
   return hipGetLastError();
 }
@@ -667,6 +786,18 @@ hipError_t launch_shade(const FrameParams& f, const ShadeScene& sc, const PathSt
 hipError_t launch_shadow_resolve(const PathState& s, uint32_t capacity, hipStream_t st) {
   const uint32_t blocks = (capacity + 255) / 256;
   hipLaunchKernelGGL(shadow_resolve_kernel, dim3(blocks < 4096 ? blocks : 4096), dim3(256), 0, st, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_nrc_apply(const PathState& s, uint32_t capacity, hipStream_t st) {
+  const uint32_t blocks = (capacity + 255) / 256;
+  hipLaunchKernelGGL(nrc_apply_kernel, dim3(blocks < 4096 ? (blocks ? blocks : 1) : 4096), dim3(256), 0, st, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_nrc_targets(const FrameParams& f, const PathState& s, hipStream_t st) {
+  if (f.train_paths == 0) return hipSuccess;
+  hipLaunchKernelGGL(nrc_targets_kernel, dim3((f.train_paths + 255) / 256), dim3(256), 0, st, f, s);
   return hipGetLastError();
 }
 

@@ -33,7 +33,7 @@ EXPORTED = [
     "vt_reset", "vt_frame_time", "vt_capture_signatures", "vt_render_info", "vt_invert_mat4",
     "vt_nrc_param_count", "vt_nrc_reset", "vt_nrc_set_weights", "vt_nrc_get_weights", "vt_nrc_infer",
     "vt_nrc_infer_device", "vt_nrc_train_step", "vt_nrc_train_step_device", "vt_nrc_last_loss",
-    "vt_nrc_gradient",
+    "vt_nrc_gradient", "vt_nrc_render_config",
 ]
 
 
@@ -373,6 +373,9 @@ class Engine:
     def nrc_train_step_device(self, d_features, d_targets, n):
         check(lib().vt_nrc_train_step_device(self._h, C.c_void_p(d_features), C.c_void_p(d_targets),
                                              C.c_uint32(n)), self._h)
+
+    def nrc_render_config(self, enabled, train_paths=65536):
+        check(lib().vt_nrc_render_config(self._h, C.c_int32(1 if enabled else 0), C.c_uint32(train_paths)), self._h)
 
     def nrc_last_loss(self):
         loss = C.c_float()
