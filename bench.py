@@ -64,8 +64,8 @@ def main():
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this size first (HBM counter calibration)")
     ap.add_argument("--refill-min", type=int, default=0, help="traversal wave refill threshold (0: default)")
-    ap.add_argument("--steps-per-refill", type=int, default=0, help="node iterations per round (0: default)")
-    ap.add_argument("--interior-min", type=int, default=0, help="node phase ends below this many lanes (0: default)")
+    ap.add_argument("--steps-per-refill", type=int, default=0, help="traversal steps between refills (0: default)")
+    ap.add_argument("--near-first", type=int, default=-1, help="1: nearer child first, 0: left first (-1: default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -75,8 +75,8 @@ def main():
     from vitrum_amd import native, workloads as W
 
     eng = native.Engine(0)
-    if args.refill_min or args.steps_per_refill or args.interior_min:
-        eng.set_trace_tuning(args.refill_min or 16, args.steps_per_refill or 64, args.interior_min or 24)
+    if args.refill_min or args.steps_per_refill or args.near_first >= 0:
+        eng.set_trace_tuning(args.refill_min or 16, args.steps_per_refill or 48, max(args.near_first, 0))
     if args.calibrate_copy_mib:
         nb = args.calibrate_copy_mib << 20
         a, b = eng.alloc(nb), eng.alloc(nb)

@@ -197,11 +197,13 @@ vt_status vt_trace_closest_counts(vt_engine *eng, const vt_ray *rays, uint32_t n
                                   uint32_t *counts);
 
 /* Traversal scheduling knobs (no counterpart in the reference): a wave is
- * refilled once `refill_min` (1..64) of its lanes are idle; each round of the
- * traversal loop runs up to `steps` node iterations and ends its node phase
- * early once fewer than `interior_min` (1..64) lanes still have node work. */
-vt_status vt_set_trace_tuning(vt_engine *eng, uint32_t refill_min, uint32_t steps,
-                              uint32_t interior_min);
+ * refilled once `refill_min` (1..64) of its lanes are idle, lanes run up to
+ * `steps` traversal steps between refill checks, and `near_first` (0/1) makes
+ * closest-hit traversal visit the nearer child first; ties between equal
+ * distances are still resolved as the reference's left-first order resolves
+ * them (lowest reordered triangle, lowest TLAS slot), so hit records are the
+ * same. */
+vt_status vt_set_trace_tuning(vt_engine *eng, uint32_t refill_min, uint32_t steps, uint32_t near_first);
 
 /* Test hook (no counterpart in the reference): q = a / b and r = sqrt(a)
  * computed on the device, to check that the device's binary32 divide and

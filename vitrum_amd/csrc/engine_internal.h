@@ -39,8 +39,8 @@ struct vt_engine {
   uint32_t trace_blocks_per_cu = 1;
   uint32_t trace_grid = 1;
   uint32_t refill_min = 16;
-  uint32_t trace_steps = 64;
-  uint32_t interior_min = 24;
+  uint32_t trace_steps = 48;
+  uint32_t near_first = 0;
   bool counting = false;
   DeviceBuffer counter, error_word, spill;
   DeviceBuffer tmp_rays, tmp_out, tmp_counts;

@@ -322,7 +322,7 @@ static vtk::TraceParams trace_params(vt_engine* e) {
   p.spill = (uint2*)e->spill.ptr;
   p.refill_min = e->refill_min;
   p.steps = e->trace_steps;
-  p.interior_min = e->interior_min;
+  p.near_first = e->near_first;
   return p;
 }
 

@@ -8,7 +8,7 @@
 namespace vtk {
 
 constexpr uint32_t kTraceBlock = 128;     // threads per block (2 waves)
-constexpr uint32_t kLdsDepth = 20;        // stack entries per lane kept in LDS
+constexpr uint32_t kLdsDepth = 24;        // stack entries per lane kept in LDS
 constexpr uint32_t kStackCapacity = 128;  // TLAS + BLAS entries per lane (>= 2 * 58 + 2)
 
 struct DeviceScene {
@@ -35,8 +35,8 @@ struct TraceParams {
   uint32_t* error;     // set to non-zero on stack overflow
   uint2* spill;        // (kStackCapacity - kLdsDepth) x grid threads
   uint32_t refill_min;    // refill a wave once this many lanes are idle
-  uint32_t steps;         // most node/pop iterations per round
-  uint32_t interior_min;  // a round's node phase ends when fewer lanes have node work
+  uint32_t steps;         // traversal steps between refill checks
+  uint32_t near_first;    // closest hit: visit the nearer child first (ties as the reference)
 };
 
 hipError_t launch_trace(const TraceParams& p, bool any, bool count, uint32_t grid, hipStream_t s);

@@ -56,9 +56,10 @@ __device__ __forceinline__ float sel3(int k, float x, float y, float z) {
   return k == 0 ? x : (k == 1 ? y : z);
 }
 
-struct V3 {
-  float x, y, z;
-};
+// Clang vector types travel in registers through the (rare) out-of-line
+// calls below; plain structs would be passed through scratch memory.
+typedef float V3 __attribute__((ext_vector_type(3)));
+typedef float F4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ V3 v3(float x, float y, float z) { return V3{x, y, z}; }
 __device__ __forceinline__ V3 sub(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
@@ -94,8 +95,9 @@ __device__ __forceinline__ bool lex_before(V3 a, V3 b) {
   return a.x < b.x || (a.x == b.x && (a.y < b.y || (a.y == b.y && a.z < b.z)));
 }
 
-__device__ __noinline__ float edge_ray_param(V3 o, V3 dir, V3 fi, V3 si, float fpx, float fpy,
-                                             float spx, float spy, V3* point) {
+// Returns (t, point) of an edge hit in registers.
+__device__ __noinline__ F4 edge_ray_param(V3 o, V3 dir, V3 fi, V3 si, float fpx, float fpy, float spx,
+                                          float spy) {
   const bool ordered = lex_before(fi, si);
   const V3 f = ordered ? fi : si, s = ordered ? si : fi;
   const float ax = ordered ? fpx : spx, ay = ordered ? fpy : spy;
@@ -103,20 +105,14 @@ __device__ __noinline__ float edge_ray_param(V3 o, V3 dir, V3 fi, V3 si, float f
   const float ddx = bx - ax, ddy = by - ay;
   float interp;
   if (fabs_(ddx) >= fabs_(ddy)) {
-    if (ddx == 0.0f) {
-      *point = v3(0, 0, 0);
-      return -1.0f;
-    }
+    if (ddx == 0.0f) return F4{-1.0f, 0.0f, 0.0f, 0.0f};
     interp = -ax / ddx;
   } else {
-    if (ddy == 0.0f) {
-      *point = v3(0, 0, 0);
-      return -1.0f;
-    }
+    if (ddy == 0.0f) return F4{-1.0f, 0.0f, 0.0f, 0.0f};
     interp = -ay / ddy;
   }
-  *point = v3(f.x + interp * (s.x - f.x), f.y + interp * (s.y - f.y), f.z + interp * (s.z - f.z));
-  return ray_param(o, dir, *point);
+  const V3 point = v3(f.x + interp * (s.x - f.x), f.y + interp * (s.y - f.y), f.z + interp * (s.z - f.z));
+  return F4{ray_param(o, dir, point), point.x, point.y, point.z};
 }
 
 struct TriHit {
@@ -159,10 +155,9 @@ __device__ __forceinline__ WtRay wt_setup(V3 dir) {
   return w;
 }
 
-// Rare tail of an in-window hit (opticalWatertightTriangle.ts:164-219): the
-// authored-normal side test, barycentrics and the canonical hit point.
-__device__ __noinline__ bool wt_accept_tail(V3 dir, float ds, V3 a, V3 b, V3 c, float e1, float e2,
-                                            float det, int zc, V3 point, TriHit* h) {
+// Rare tail of an in-window hit (opticalWatertightTriangle.ts:164-200): the
+// authored-normal side test.
+__device__ __noinline__ bool wt_side_ok(V3 dir, float ds, V3 a, V3 b, V3 c) {
   const V3 e1r = sub(b, a), e2r = sub(c, a);
   const float es = jmax(max3abs(e1r.x, e1r.y, e1r.z), max3abs(e2r.x, e2r.y, e2r.z));
   if (!(es > 0.0f) || !finite_(es)) return false;
@@ -177,24 +172,16 @@ __device__ __noinline__ bool wt_accept_tail(V3 dir, float ds, V3 a, V3 b, V3 c, 
   const V3 n = v3(nd.x / nl, nd.y / nl, nd.z / nl);
   const V3 d = v3(dir.x / ds, dir.y / ds, dir.z / ds);
   const float side = -((d.x * n.x + d.y * n.y) + d.z * n.z);
-  if (side == 0.0f || !finite_(side)) return false;
-  h->u = e1 / det;
-  h->v = e2 / det;
-  if (zc == 0) {
-    point = v3(a.x + (h->u * (b.x - a.x) + h->v * (c.x - a.x)),
-               a.y + (h->u * (b.y - a.y) + h->v * (c.y - a.y)),
-               a.z + (h->u * (b.z - a.z) + h->v * (c.z - a.z)));
-  }
-  h->point = point;
-  return true;
+  return !(side == 0.0f || !finite_(side));
 }
 
-// Returns true and fills h when the triangle is hit with lo < t < hi, where
-// lo = tmin (also the predicate's exclusive minimum) and hi the current bound.
-// Work that can only turn an out-of-window hit into a miss is skipped for
-// out-of-window hits, which leaves the accepted set unchanged.
+// Returns true and fills h when the triangle is hit with lo < t < hi (or
+// t <= hi when `incl_hi`), where lo = tmin (also the predicate's exclusive
+// minimum) and hi the current bound.  Work that can only turn an
+// out-of-window hit into a miss is skipped for out-of-window hits, which
+// leaves the accepted set unchanged.
 __device__ __forceinline__ bool tri_test(V3 o, V3 dir, const WtRay& w, V3 a, V3 b, V3 c, float tmin,
-                                         float hi, TriHit& h) {
+                                         float hi, bool incl_hi, TriHit& h) {
   if (w.perm == 0xFFFFFFFFu || nan_(tmin)) return false;
   const int kx = w.perm & 3, ky = (w.perm >> 2) & 3, kz = w.perm >> 4;
   const V3 pa = sub(a, o), pb = sub(b, o), pc = sub(c, o);
@@ -220,19 +207,33 @@ __device__ __forceinline__ bool tri_test(V3 o, V3 dir, const WtRay& w, V3 a, V3 
       if (n0 != 0.0f) point = a, t = ray_param(o, dir, point);
       if (n1 != 0.0f) point = b, t = ray_param(o, dir, point);
       if (n2 != 0.0f) point = c, t = ray_param(o, dir, point);
-    } else if (n0 == 0.0f) {
-      t = edge_ray_param(o, dir, b, c, bpx, bpy, cpx, cpy, &point);
-    } else if (n1 == 0.0f) {
-      t = edge_ray_param(o, dir, c, a, cpx, cpy, apx, apy, &point);
     } else {
-      t = edge_ray_param(o, dir, a, b, apx, apy, bpx, bpy, &point);
+      F4 ep;
+      if (n0 == 0.0f)
+        ep = edge_ray_param(o, dir, b, c, bpx, bpy, cpx, cpy);
+      else if (n1 == 0.0f)
+        ep = edge_ray_param(o, dir, c, a, cpx, cpy, apx, apy);
+      else
+        ep = edge_ray_param(o, dir, a, b, apx, apy, bpx, bpy);
+      t = ep.x;
+      point = v3(ep.y, ep.z, ep.w);
     }
   }
   const float lo = tmin > 0.0f ? tmin : 0.0f;
   if (!(t > lo) || !finite_(t)) return false;
-  if (!(t > tmin && t < hi)) return false;  // traversal acceptance window
+  if (!(t > tmin && (t < hi || (incl_hi && t == hi)))) return false;  // traversal acceptance window
+  // opticalWatertightTriangle.ts:164-219: authored-normal side test, then
+  // barycentrics and the canonical hit point.
+  if (!wt_side_ok(dir, w.ds, a, b, c)) return false;
   h.t = t;
-  return wt_accept_tail(dir, w.ds, a, b, c, e1, e2, det, zc, point, &h);
+  h.u = e1 / det;
+  h.v = e2 / det;
+  if (zc == 0) {
+    point = v3(a.x + (h.u * (b.x - a.x) + h.v * (c.x - a.x)), a.y + (h.u * (b.y - a.y) + h.v * (c.y - a.y)),
+               a.z + (h.u * (b.z - a.z) + h.v * (c.z - a.z)));
+  }
+  h.point = point;
+  return true;
 }
 
 // ---------------------------------------------------------------------------
@@ -291,77 +292,155 @@ __device__ __forceinline__ V3 xform_dir(float4 c0, float4 c1, float4 c2, V3 d) {
 // Persistent traversal kernel
 // ---------------------------------------------------------------------------
 
-template <uint32_t BLOCK, uint32_t LDS_DEPTH>
-struct LaneStack {
-  uint2* lds;
-  uint2* spill;
-  uint32_t spill_stride, tid, gtid;
-  __device__ __forceinline__ void store(uint32_t i, uint2 v) {
-    if (i < LDS_DEPTH)
-      lds[i * BLOCK + tid] = v;
-    else
-      spill[(size_t)(i - LDS_DEPTH) * spill_stride + gtid] = v;
-  }
-  __device__ __forceinline__ uint2 load(uint32_t i) const {
-    return i < LDS_DEPTH ? lds[i * BLOCK + tid] : spill[(size_t)(i - LDS_DEPTH) * spill_stride + gtid];
-  }
-};
-
-// Lane modes of the phased traversal loop.
-constexpr uint32_t kModeIdle = 0;   // no ray
-constexpr uint32_t kModeNode = 1;   // pop / expand interior nodes at the current level
-constexpr uint32_t kModeLeaf = 2;   // a BLAS leaf's triangles are pending
-constexpr uint32_t kModeEnter = 3;  // enter instance `slot`
-constexpr uint32_t kModeExit = 4;   // the current instance's BLAS is exhausted
-constexpr uint32_t kModeDone = 5;   // write the result
-
-// One round of the loop runs four wave-uniform phases: (1) up to `steps`
-// node iterations (a pop with the pop-time cull, then one Node64 expansion)
-// until fewer than `interior_min` lanes still have node work; (2) the pending
-// BLAS leaves' triangles; (3) instance exit and entry; (4) results.  The
-// expensive phases run once for every lane that reached them instead of
-// once per node step, and each lane still performs exactly the reference's
-// sequence of node, triangle and instance operations.
+// Single-step persistent loop: each iteration a lane pops (with the
+// reference's pop-time cull), expands one Node64 (two child boxes), tests
+// one leaf's triangles or enters / leaves one instance.
+//
+// Child order.  The reference visits the left child first.  With
+// `near_first` (closest hit only) the child with the smaller entry distance
+// is visited first and ties are broken as the reference's order breaks them:
+// among triangles with equal t the lower reordered index wins, and among
+// instances with equal world distance the lower TLAS slot (DFS rank) wins.
+// Node culling stays `tNear > best` (equal distances are still visited), and
+// an instance whose slot is below the current best's gets its local bound
+// widened by 2^-18 so that an equal-distance hit in it is still found.  The
+// counting variant always uses the reference order, so its work counts are
+// the reference's.
 template <bool ANY, bool COUNT, uint32_t BLOCK, uint32_t LDS_DEPTH>
 __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceParams p) {
+  // Per-lane stack: LDS_DEPTH entries in LDS (entry-major, so a wave's
+  // accesses at equal depth touch 64 distinct banks), deeper entries in a
+  // per-thread global spill area.  The shared array is indexed directly so the
+  // compiler emits ds_* instructions, and the spill area is accessed as 64-bit
+  // words, a different type, so the two branches cannot be merged into one
+  // generic (flat) access with the latency of global memory.
   __shared__ uint2 lds_stack[LDS_DEPTH * BLOCK];
-  LaneStack<BLOCK, LDS_DEPTH> stk{lds_stack, p.spill, gridDim.x * BLOCK, threadIdx.x,
-                                  blockIdx.x * BLOCK + threadIdx.x};
+  const uint32_t tid = threadIdx.x;
+  const size_t spill_stride = (size_t)gridDim.x * BLOCK, gtid = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+  unsigned long long* __restrict__ spill = reinterpret_cast<unsigned long long*>(p.spill);
+  auto stk_store = [&](uint32_t i, uint2 v) {
+    if (i < LDS_DEPTH)
+      lds_stack[i * BLOCK + tid] = v;
+    else
+      spill[(i - LDS_DEPTH) * spill_stride + gtid] = (unsigned long long)v.x | ((unsigned long long)v.y << 32);
+  };
+  auto stk_load = [&](uint32_t i) -> uint2 {
+    if (i < LDS_DEPTH) return lds_stack[i * BLOCK + tid];
+    const unsigned long long w = spill[(i - LDS_DEPTH) * spill_stride + gtid];
+    return make_uint2((uint32_t)w, (uint32_t)(w >> 32));
+  };
   const float4* __restrict__ blas = p.scene.blas_nodes;
   const float4* __restrict__ tlas = p.scene.tlas_nodes;
   const float4* __restrict__ tris = p.scene.tris;
   const float4* __restrict__ insts = p.scene.instances;
   const uint32_t lane = __lane_id();
   const uint64_t lane_below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const bool near = !ANY && !COUNT && p.near_first != 0u;
 
   const uint32_t n_rays = p.n_ptr ? *p.n_ptr : p.n;
-  uint32_t mode = kModeIdle;
-  bool occluded = false;
+  bool active = false;
   uint32_t ray_idx = 0;
   // World ray and best hit.
   V3 ro{}, rd{}, rinv{};
   float tmin = 0, tmax = 0, best = 0, bu = 0, bv = 0;
   V3 bpos{};
-  uint32_t bprim = 0, binst = kMiss;
+  uint32_t bprim = 0, binst = kMiss, bslot = 0;
   // Instance-local state.
   V3 lo{}, ld{}, linv{}, lpos{};
   WtRay lw{};
+  bool lw_ready = false;
   float ltmin = 0, lbest = 0, lu = 0, lv = 0;
   uint32_t lprim = 0, inst_id = 0;
   bool lhit = false;
-  uint32_t slot = 0, slot_end = 0, leaf_first = 0, leaf_count = 0;
+  uint32_t slot = 0, slot_end = 0;
   uint32_t level = 0, cur = kNone, sp = 0, blas_base = 0;
   uint32_t c_blas = 0, c_tri = 0, c_tlas = 0, c_inst = 0;
 
+  auto enter_instance = [&](uint32_t s) {
+    const float4* ir = insts + 9 * (size_t)s;
+    const float4 c0 = ir[0], c1 = ir[1], c2 = ir[2], c3 = ir[3];
+    const uint4 meta = *reinterpret_cast<const uint4*>(ir + 8);
+    lo = xform_point(c0, c1, c2, c3, ro);
+    ld = xform_dir(c0, c1, c2, rd);
+    linv = v3(safe_inv(ld.x), safe_inv(ld.y), safe_inv(ld.z));
+    lw_ready = false;  // the watertight set-up waits for the first leaf
+    const V3 start = v3(ro.x + rd.x * tmin, ro.y + rd.y * tmin, ro.z + rd.z * tmin);
+    ltmin = fmaxf(dot3(sub(xform_point(c0, c1, c2, c3, start), lo), ld), 0.0f);
+    // Closest: localTMax starts at INFINITY; any-hit: at the ray's tMax
+    // (tlasSceneHitTraversal.wgsl.ts:103-108 and :168-173).
+    const float bound = ANY ? tmax : best;
+    float ltmax = ANY ? tmax : inf_const();
+    if (bound < inf_const() * 0.5f) {
+      const V3 end = v3(ro.x + rd.x * bound, ro.y + rd.y * bound, ro.z + rd.z * bound);
+      ltmax = fmaxf(dot3(sub(xform_point(c0, c1, c2, c3, end), lo), ld), ltmin);
+      if (near && binst != kMiss && s < bslot) ltmax = ltmax * (1.0f + 0x1p-18f);
+    }
+    lbest = ltmax;
+    lhit = false;
+    inst_id = meta.y;
+    level = 1;
+    blas_base = sp;
+    cur = meta.x;
+    if (COUNT) ++c_inst;
+  };
+
+  auto finalize_instance = [&]() {
+    if (!ANY && lhit && lbest > ltmin) {
+      const float4* ir = insts + 9 * (size_t)slot + 4;
+      const V3 w = xform_point(ir[0], ir[1], ir[2], ir[3], lpos);
+      const float wd = dot3(sub(w, ro), rd);
+      if (wd > tmin && (wd < best || (near && wd == best && binst != kMiss && slot < bslot))) {
+        best = wd;
+        bpos = w;
+        bprim = lprim;
+        binst = inst_id;
+        bslot = slot;
+        bu = lu;
+        bv = lv;
+      }
+    }
+  };
+
+  auto finish = [&](bool occluded) {
+    if (ANY) {
+      p.occluded[ray_idx] = occluded ? 1u : 0u;
+    } else {
+      const bool hit = binst != kMiss;
+      uint32_t* h = p.hits + 5 * (size_t)ray_idx;
+      h[0] = hit ? bprim : kMiss;
+      h[1] = hit ? binst : kMiss;
+      h[2] = hit ? f2u(best) : 0x7F800000u;
+      h[3] = hit ? f2u(bu) : 0u;
+      h[4] = hit ? f2u(bv) : 0u;
+      if (p.hit_pos) {
+        float* hp = p.hit_pos + 3 * (size_t)ray_idx;
+        hp[0] = bpos.x;
+        hp[1] = bpos.y;
+        hp[2] = bpos.z;
+      }
+    }
+    if (COUNT) p.counts[ray_idx] = make_uint4(c_blas, c_tri, c_tlas, c_inst);
+    active = false;
+  };
+
+  auto push = [&](uint32_t ref, float tn) {
+    if (sp >= kStackCapacity) {
+      atomicOr(p.error, 1u);
+      return false;
+    }
+    stk_store(sp++, make_uint2(ref, f2u(tn)));
+    return true;
+  };
+
   for (;;) {
-    // ---- refill: ballot idle lanes, one atomic, prefix ranks ----
-    const uint64_t idle = __ballot(mode == kModeIdle);
+    // ---- wave-level refill: ballot idle lanes, one atomic, prefix ranks ----
+    const uint64_t idle = __ballot(!active);
     const uint32_t n_idle = (uint32_t)__popcll(idle);
     if (n_idle >= p.refill_min || idle == __ballot(1)) {
       uint32_t base = 0;
       if (lane == 0) base = atomicAdd(p.counter, n_idle);
       base = __shfl(base, 0);
-      if (mode == kModeIdle) {
+      if (!active) {
         const uint32_t idx = base + (uint32_t)__popcll(idle & lane_below);
         if (idx < n_rays) {
           ray_idx = idx;
@@ -374,186 +453,115 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceParams p) {
           best = tmax;
           binst = kMiss;
           bprim = 0;
+          bslot = 0;
           bu = bv = 0;
-          occluded = false;
           level = 0;
           sp = 0;
           cur = p.scene.tlas_node_count ? p.scene.tlas_root : kNone;
           if (COUNT) c_blas = c_tri = c_tlas = c_inst = 0;
-          mode = cur == kNone ? kModeDone : kModeNode;  // no TLAS: fail closed
+          active = true;
+          if (cur == kNone) finish(false);  // no TLAS: fail closed
         }
       }
     }
-    if (!__any(mode != kModeIdle)) break;
+    if (!__any(active)) break;
 
-    // ---- (1) node phase ----
-    for (uint32_t k = 0; k < p.steps; ++k) {
-      const uint32_t n_node = (uint32_t)__popcll(__ballot(mode == kModeNode));
-      if (n_node == 0 || (k > 0 && n_node < p.interior_min)) break;
-      if (mode != kModeNode) continue;
+    for (uint32_t step = 0; active && step < p.steps; ++step) {
       if (cur == kNone) {
-        // Pop with the reference's pop-time cull.
-        if (level == 1) {
-          if (sp > blas_base) {
-            const uint2 e = stk.load(--sp);
-            if (!(u2f(e.y) > lbest)) cur = e.x;
-          } else {
-            mode = kModeExit;
+        // ---- pop with the reference's pop-time cull; unwind levels ----
+        for (;;) {
+          if (level == 1) {
+            if (sp > blas_base) {
+              const uint2 e = stk_load(--sp);
+              if (!(u2f(e.y) > lbest)) {
+                cur = e.x;
+                break;
+              }
+              continue;
+            }
+            finalize_instance();
+            level = 0;
+            if (++slot < slot_end) {
+              enter_instance(slot);
+              break;
+            }
+            continue;
           }
-        } else if (sp > 0) {
-          const uint2 e = stk.load(--sp);
-          if (!(u2f(e.y) > (ANY ? tmax : best))) cur = e.x;
-        } else {
-          mode = kModeDone;
+          if (sp > 0) {
+            const uint2 e = stk_load(--sp);
+            if (!(u2f(e.y) > (ANY ? tmax : best))) {
+              cur = e.x;
+              break;
+            }
+            continue;
+          }
+          finish(false);
+          break;
         }
-        if (cur == kNone) continue;
+        if (!active) break;
       }
-      if (cur & kLeafBit) {
-        const uint32_t count = (cur >> 27) & 15u, first = cur & 0x07FFFFFFu;
-        cur = kNone;
-        if (level == 0) {
-          slot = first;
-          slot_end = first + count;
-          mode = count ? kModeEnter : kModeNode;
+      if ((cur & kLeafBit) == 0) {
+        // ---- interior: one Node64 load, two child boxes ----
+        const float4* nd = (level ? blas : tlas) + 4 * (size_t)cur;
+        const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2], q3 = nd[3];
+        const uint32_t ref_l = f2u(q3.x), ref_r = f2u(q3.y);
+        const V3 o = level ? lo : ro;
+        const V3 inv = level ? linv : rinv;
+        const float tm = level ? ltmin : tmin;
+        const float bound = level ? lbest : (ANY ? tmax : best);
+        float tn_l, tn_r;
+        const bool pass_l = ref_l != kEmpty && slab(o, inv, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, tm, tn_l);
+        const bool pass_r = ref_r != kEmpty && slab(o, inv, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, tm, tn_r);
+        if (COUNT) {
+          const uint32_t tested = (ref_l != kEmpty) + (ref_r != kEmpty);
+          if (level) c_blas += tested; else c_tlas += tested;
+        }
+        const bool take_l = pass_l && !(tn_l > bound), take_r = pass_r && !(tn_r > bound);
+        if (take_l && take_r) {
+          const bool right_first = near && tn_r < tn_l;
+          if (!push(right_first ? ref_l : ref_r, right_first ? tn_l : tn_r)) {
+            finish(false);
+            break;
+          }
+          cur = right_first ? ref_r : ref_l;
         } else {
-          leaf_first = first;
-          leaf_count = count;
-          mode = kModeLeaf;
+          cur = take_l ? ref_l : (take_r ? ref_r : kNone);
         }
         continue;
       }
-      // Interior: one Node64 load, two child boxes.
-      const float4* nd = (level ? blas : tlas) + 4 * (size_t)cur;
-      const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2], q3 = nd[3];
-      const uint32_t ref_l = f2u(q3.x), ref_r = f2u(q3.y);
-      const V3 o = level ? lo : ro;
-      const V3 inv = level ? linv : rinv;
-      const float tm = level ? ltmin : tmin;
-      const float bound = level ? lbest : (ANY ? tmax : best);
-      float tn_l, tn_r;
-      const bool pass_l = ref_l != kEmpty && slab(o, inv, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, tm, tn_l);
-      const bool pass_r = ref_r != kEmpty && slab(o, inv, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, tm, tn_r);
-      if (COUNT) {
-        const uint32_t tested = (ref_l != kEmpty) + (ref_r != kEmpty);
-        if (level) c_blas += tested; else c_tlas += tested;
+      // ---- leaf ----
+      const uint32_t count = (cur >> 27) & 15u, first = cur & 0x07FFFFFFu;
+      cur = kNone;
+      if (level == 0) {
+        slot = first;
+        slot_end = first + count;
+        enter_instance(slot);
+        continue;
       }
-      if (pass_l && !(tn_l > bound)) {
-        if (pass_r) {
-          if (sp >= kStackCapacity) {
-            atomicOr(p.error, 1u);
-            mode = kModeDone;
-            continue;
-          }
-          stk.store(sp++, make_uint2(ref_r, f2u(tn_r)));
-        }
-        cur = ref_l;
-      } else if (pass_r && !(tn_r > bound)) {
-        cur = ref_r;
-      } else {
-        cur = kNone;
-      }
-    }
-
-    // ---- (2) leaf phase: the pending BLAS leaf's triangles, in order ----
-    if (__any(mode == kModeLeaf)) {
-      if (mode == kModeLeaf) {
-        mode = kModeNode;
-        for (uint32_t i = 0; i < leaf_count; ++i) {
-          const uint32_t t = leaf_first + i;
-          const float4* tr = tris + 3 * (size_t)t;
-          const float4 w0 = tr[0], w1 = tr[1], w2 = tr[2];
-          if (COUNT) ++c_tri;
-          TriHit h;
-          if (tri_test(lo, ld, lw, v3(w0.x, w0.y, w0.z), v3(w0.w, w1.x, w1.y), v3(w1.z, w1.w, w2.x),
-                       ltmin, lbest, h)) {
-            if (ANY) {
-              occluded = true;
-              mode = kModeDone;
-              break;
-            }
-            lhit = true;
-            lbest = h.t;
-            lprim = t;
-            lu = h.u;
-            lv = h.v;
-            lpos = h.point;
-          }
-        }
-      }
-    }
-
-    // ---- (3) instance exit (closest: merge the local hit) and entry ----
-    if (__any(mode == kModeExit)) {
-      if (mode == kModeExit) {
-        if (!ANY && lhit && lbest > ltmin) {
-          const float4* ir = insts + 9 * (size_t)slot + 4;
-          const V3 w = xform_point(ir[0], ir[1], ir[2], ir[3], lpos);
-          const float wd = dot3(sub(w, ro), rd);
-          if (wd > tmin && wd < best) {
-            best = wd;
-            bpos = w;
-            bprim = lprim;
-            binst = inst_id;
-            bu = lu;
-            bv = lv;
-          }
-        }
-        level = 0;
-        mode = ++slot < slot_end ? kModeEnter : kModeNode;
-      }
-    }
-    if (__any(mode == kModeEnter)) {
-      if (mode == kModeEnter) {
-        const float4* ir = insts + 9 * (size_t)slot;
-        const float4 c0 = ir[0], c1 = ir[1], c2 = ir[2], c3 = ir[3];
-        const uint4 meta = *reinterpret_cast<const uint4*>(ir + 8);
-        lo = xform_point(c0, c1, c2, c3, ro);
-        ld = xform_dir(c0, c1, c2, rd);
-        linv = v3(safe_inv(ld.x), safe_inv(ld.y), safe_inv(ld.z));
+      if (!lw_ready) {
         lw = wt_setup(ld);
-        const V3 start = v3(ro.x + rd.x * tmin, ro.y + rd.y * tmin, ro.z + rd.z * tmin);
-        ltmin = fmaxf(dot3(sub(xform_point(c0, c1, c2, c3, start), lo), ld), 0.0f);
-        // Closest: localTMax starts at INFINITY; any-hit: at the ray's tMax
-        // (tlasSceneHitTraversal.wgsl.ts:103-108 and :168-173).
-        const float bnd = ANY ? tmax : best;
-        float ltmax = ANY ? tmax : inf_const();
-        if (bnd < inf_const() * 0.5f) {
-          const V3 end = v3(ro.x + rd.x * bnd, ro.y + rd.y * bnd, ro.z + rd.z * bnd);
-          ltmax = fmaxf(dot3(sub(xform_point(c0, c1, c2, c3, end), lo), ld), ltmin);
-        }
-        lbest = ltmax;
-        lhit = false;
-        inst_id = meta.y;
-        level = 1;
-        blas_base = sp;
-        cur = meta.x;
-        if (COUNT) ++c_inst;
-        mode = kModeNode;
+        lw_ready = true;
       }
-    }
-
-    // ---- (4) results ----
-    if (__any(mode == kModeDone)) {
-      if (mode == kModeDone) {
-        if (ANY) {
-          p.occluded[ray_idx] = occluded ? 1u : 0u;
-        } else {
-          const bool hit = binst != kMiss;
-          uint32_t* h = p.hits + 5 * (size_t)ray_idx;
-          h[0] = hit ? bprim : kMiss;
-          h[1] = hit ? binst : kMiss;
-          h[2] = hit ? f2u(best) : 0x7F800000u;
-          h[3] = hit ? f2u(bu) : 0u;
-          h[4] = hit ? f2u(bv) : 0u;
-          if (p.hit_pos) {
-            float* hp = p.hit_pos + 3 * (size_t)ray_idx;
-            hp[0] = bpos.x;
-            hp[1] = bpos.y;
-            hp[2] = bpos.z;
+      for (uint32_t i = 0; i < count; ++i) {
+        const uint32_t t = first + i;
+        const float4* tr = tris + 3 * (size_t)t;
+        const float4 w0 = tr[0], w1 = tr[1], w2 = tr[2];
+        if (COUNT) ++c_tri;
+        TriHit h;
+        if (tri_test(lo, ld, lw, v3(w0.x, w0.y, w0.z), v3(w0.w, w1.x, w1.y), v3(w1.z, w1.w, w2.x), ltmin,
+                     lbest, near && lhit, h)) {
+          if (ANY) {
+            finish(true);
+            break;
           }
+          if (h.t == lbest && lhit && !(t < lprim)) continue;  // equal t: lower index wins
+          lhit = true;
+          lbest = h.t;
+          lprim = t;
+          lu = h.u;
+          lv = h.v;
+          lpos = h.point;
         }
-        if (COUNT) p.counts[ray_idx] = make_uint4(c_blas, c_tri, c_tlas, c_inst);
-        mode = kModeIdle;
       }
     }
   }
