@@ -285,7 +285,7 @@ class Engine:
         check(lib().vt_trace_any(self._h, rays.ctypes.data, rays.size, out.ctypes.data), self._h)
         return out
 
-    def set_trace_tuning(self, refill_min, steps, near_first=0):
+    def set_trace_tuning(self, refill_min, steps, near_first=1):
         check(lib().vt_set_trace_tuning(self._h, C.c_uint32(refill_min), C.c_uint32(steps),
                                         C.c_uint32(near_first)), self._h)
 
