@@ -76,7 +76,7 @@ def main():
 
     eng = native.Engine(0)
     if args.refill_min or args.steps_per_refill or args.near_first >= 0:
-        eng.set_trace_tuning(args.refill_min or 8, args.steps_per_refill or 16,
+        eng.set_trace_tuning(args.refill_min or 16, args.steps_per_refill or 16,
                              1 if args.near_first < 0 else args.near_first)
     if args.calibrate_copy_mib:
         nb = args.calibrate_copy_mib << 20

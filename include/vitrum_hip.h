@@ -198,11 +198,13 @@ vt_status vt_trace_closest_counts(vt_engine *eng, const vt_ray *rays, uint32_t n
 
 /* Traversal scheduling knobs (no counterpart in the reference): a wave is
  * refilled once `refill_min` (1..64) of its lanes are idle, lanes run up to
- * `steps` traversal steps between refill checks, and `near_first` (0/1) makes
- * closest-hit traversal visit the nearer child first; ties between equal
- * distances are still resolved as the reference's left-first order resolves
- * them (lowest reordered triangle, lowest TLAS slot), so hit records are the
- * same. */
+ * `steps` traversal steps between refill checks, and `near_first` orders the
+ * closest-hit traversal: 0 = the reference's left-first order; 1 (default) =
+ * nearer child first inside each BLAS, equal triangle distances resolved as
+ * the reference resolves them (lowest reordered index), instances in the
+ * reference's TLAS order -- hit records identical to the reference's; 2 = also
+ * nearest-first across instances (can differ from the reference on coplanar
+ * geometry in different instances; for measurement). */
 vt_status vt_set_trace_tuning(vt_engine *eng, uint32_t refill_min, uint32_t steps, uint32_t near_first);
 
 /* Test hook (no counterpart in the reference): q = a / b and r = sqrt(a)

@@ -418,8 +418,8 @@ vt_status vt_trace_any_device(vt_engine* eng, const vt_ray* d_rays, uint32_t n, 
 
 vt_status vt_set_trace_tuning(vt_engine* eng, uint32_t refill_min, uint32_t steps, uint32_t near_first) {
   if (!eng) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "engine is NULL");
-  if (refill_min < 1 || refill_min > 64 || steps < 1 || near_first > 1)
-    return set_err(eng, VT_ERR_RANGE, "refill_min must be in [1, 64], steps >= 1 and near_first 0 or 1");
+  if (refill_min < 1 || refill_min > 64 || steps < 1 || near_first > 2)
+    return set_err(eng, VT_ERR_RANGE, "refill_min must be in [1, 64], steps >= 1 and near_first 0, 1 or 2");
   eng->refill_min = refill_min;
   eng->trace_steps = steps;
   eng->near_first = near_first;

@@ -297,15 +297,18 @@ __device__ __forceinline__ V3 xform_dir(float4 c0, float4 c1, float4 c2, V3 d) {
 // one leaf's triangles or enters / leaves one instance.
 //
 // Child order.  The reference visits the left child first.  With
-// `near_first` (closest hit only) the child with the smaller entry distance
-// is visited first and ties are broken as the reference's order breaks them:
-// among triangles with equal t the lower reordered index wins, and among
-// instances with equal world distance the lower TLAS slot (DFS rank) wins.
-// Node culling stays `tNear > best` (equal distances are still visited), and
-// an instance whose slot is below the current best's gets its local bound
-// widened by 2^-18 so that an equal-distance hit in it is still found.  The
-// counting variant always uses the reference order, so its work counts are
-// the reference's.
+// near_first = 1 (closest hit only) the child with the smaller entry distance
+// is visited first inside each BLAS, and equal triangle distances are
+// resolved as the reference's order resolves them (the lower reordered index
+// wins); instances are still visited in the reference's TLAS order, because
+// with coplanar geometry in different instances the reference's choice
+// depends on that order through the rounding of the instance transforms.
+// near_first = 2 also orders the TLAS by distance (lower TLAS slot wins equal
+// world distances; an instance below the current best's slot gets its local
+// bound widened by 2^-18); it can differ from the reference on such coplanar
+// instance geometry and exists for measurement only.  Node culling stays
+// `tNear > best` (equal distances are still visited).  The counting variant
+// always uses the reference order, so its work counts are the reference's.
 template <bool ANY, bool COUNT, uint32_t BLOCK, uint32_t LDS_DEPTH>
 __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceParams p) {
   // Per-lane stack: LDS_DEPTH entries in LDS (entry-major, so a wave's
@@ -335,7 +338,8 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceParams p) {
   const float4* __restrict__ insts = p.scene.instances;
   const uint32_t lane = __lane_id();
   const uint64_t lane_below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const bool near = !ANY && !COUNT && p.near_first != 0u;
+  const bool near = !ANY && !COUNT && p.near_first != 0u;        // inside BLASes
+  const bool near_tlas = !ANY && !COUNT && p.near_first == 2u;  // also across instances
 
   const uint32_t n_rays = p.n_ptr ? *p.n_ptr : p.n;
   bool active = false;
@@ -373,7 +377,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceParams p) {
     if (bound < inf_const() * 0.5f) {
       const V3 end = v3(ro.x + rd.x * bound, ro.y + rd.y * bound, ro.z + rd.z * bound);
       ltmax = fmaxf(dot3(sub(xform_point(c0, c1, c2, c3, end), lo), ld), ltmin);
-      if (near && binst != kMiss && s < bslot) ltmax = ltmax * (1.0f + 0x1p-18f);
+      if (near_tlas && binst != kMiss && s < bslot) ltmax = ltmax * (1.0f + 0x1p-18f);
     }
     lbest = ltmax;
     lhit = false;
@@ -389,7 +393,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceParams p) {
       const float4* ir = insts + 9 * (size_t)slot + 4;
       const V3 w = xform_point(ir[0], ir[1], ir[2], ir[3], lpos);
       const float wd = dot3(sub(w, ro), rd);
-      if (wd > tmin && (wd < best || (near && wd == best && binst != kMiss && slot < bslot))) {
+      if (wd > tmin && (wd < best || (near_tlas && wd == best && binst != kMiss && slot < bslot))) {
         best = wd;
         bpos = w;
         bprim = lprim;
@@ -518,7 +522,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceParams p) {
         }
         const bool take_l = pass_l && !(tn_l > bound), take_r = pass_r && !(tn_r > bound);
         if (take_l && take_r) {
-          const bool right_first = near && tn_r < tn_l;
+          const bool right_first = (level ? near : near_tlas) && tn_r < tn_l;
           if (!push(right_first ? ref_l : ref_r, right_first ? tn_l : tn_r)) {
             finish(false);
             break;
