@@ -309,6 +309,10 @@ vt_status vt_nrc_train_step(vt_engine *eng, const float *features, const float *
 vt_status vt_nrc_train_step_device(vt_engine *eng, const float *d_features, const float *d_targets,
                                    uint32_t n);
 vt_status vt_nrc_last_loss(vt_engine *eng, float *loss);
+/* Milliseconds of the last inference kernel and of the last training
+ * (forward + backward + weight-gradient) kernel, from HIP events recorded on
+ * the engine stream around them; 0 when not run yet. */
+vt_status vt_nrc_timing(vt_engine *eng, float *infer_ms, float *train_ms);
 /* d loss / d weights of the batch (no update), for gradient checks. */
 vt_status vt_nrc_gradient(vt_engine *eng, const float *features, const float *targets, uint32_t n,
                           float *grad, float *loss);

@@ -11,8 +11,13 @@ struct NrcState {
   DeviceBuffer feats, targets, out;  // staging for the host-buffer entry points
   vt_nrc_options opts{};
   uint32_t step = 0;
+  // HIP events around the last inference and the last training launch.
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  bool timed_infer = false, timed_train = false;
   void release() {
     for (DeviceBuffer* b : {&master, &m, &v, &grad, &frags, &loss, &feats, &targets, &out}) b->release();
+    for (hipEvent_t& x : ev)
+      if (x) (void)hipEventDestroy(x), x = nullptr;
   }
 };
 
@@ -69,6 +74,8 @@ static vt_status nrc_ensure(vt_engine* e) {
   for (DeviceBuffer* b : {&s->master, &s->m, &s->v, &s->grad}) st = st ? st : b->ensure(e, pb);
   st = st ? st : s->frags.ensure(e, vtn::kFrags * 16);
   st = st ? st : s->loss.ensure(e, 64);
+  for (hipEvent_t& x : s->ev)
+    if (!st && hipEventCreate(&x) != hipSuccess) st = VT_ERR_HIP;
   if (st) {
     engine_release_nrc(e);
     return st;
@@ -163,7 +170,10 @@ vt_status vt_nrc_infer_device(vt_engine* e, const float* d_features, uint32_t n,
   if (!e || (n && (!d_features || !d_rgb))) return VT_ERR_INVALID_ARGUMENT;
   if (vt_status s = nrc_ensure(e)) return s;
   NRC_HIP(e, hipSetDevice(e->device));
+  NRC_HIP(e, hipEventRecord(e->nrc->ev[0], e->stream));
   NRC_HIP(e, vtn::launch_infer(d_features, n, nullptr, (const uint4*)e->nrc->frags.ptr, d_rgb, e->stream));
+  NRC_HIP(e, hipEventRecord(e->nrc->ev[1], e->stream));
+  e->nrc->timed_infer = true;
   return VT_OK;
 }
 
@@ -188,8 +198,11 @@ static vt_status nrc_accumulate(vt_engine* e, const float* d_features, const flo
   NrcState& s = *e->nrc;
   NRC_HIP(e, hipSetDevice(e->device));
   NRC_HIP(e, vtn::launch_fill((float*)s.loss.ptr, 1, 0.0f, e->stream));
+  NRC_HIP(e, hipEventRecord(s.ev[2], e->stream));
   NRC_HIP(e, vtn::launch_train(d_features, d_targets, n, d_count, (const uint4*)s.frags.ptr, (float*)s.grad.ptr,
                                (float*)s.loss.ptr, e->stream));
+  NRC_HIP(e, hipEventRecord(s.ev[3], e->stream));
+  s.timed_train = true;
   return VT_OK;
 }
 
@@ -215,6 +228,22 @@ vt_status vt_nrc_train_step_device(vt_engine* e, const float* d_features, const 
   if (vt_status s = nrc_ensure(e)) return s;
   if (vt_status s = nrc_accumulate(e, d_features, d_targets, n)) return s;
   return nrc_apply_adam(e);
+}
+
+vt_status vt_nrc_timing(vt_engine* e, float* infer_ms, float* train_ms) {
+  if (!e || !infer_ms || !train_ms) return VT_ERR_INVALID_ARGUMENT;
+  *infer_ms = *train_ms = 0.0f;
+  if (!e->nrc) return VT_OK;
+  NrcState& s = *e->nrc;
+  if (s.timed_infer) {
+    NRC_HIP(e, hipEventSynchronize(s.ev[1]));
+    NRC_HIP(e, hipEventElapsedTime(infer_ms, s.ev[0], s.ev[1]));
+  }
+  if (s.timed_train) {
+    NRC_HIP(e, hipEventSynchronize(s.ev[3]));
+    NRC_HIP(e, hipEventElapsedTime(train_ms, s.ev[2], s.ev[3]));
+  }
+  return VT_OK;
 }
 
 vt_status vt_nrc_last_loss(vt_engine* e, float* loss) {

@@ -33,7 +33,7 @@ EXPORTED = [
     "vt_reset", "vt_frame_time", "vt_capture_signatures", "vt_render_info", "vt_invert_mat4",
     "vt_nrc_param_count", "vt_nrc_reset", "vt_nrc_set_weights", "vt_nrc_get_weights", "vt_nrc_infer",
     "vt_nrc_infer_device", "vt_nrc_train_step", "vt_nrc_train_step_device", "vt_nrc_last_loss",
-    "vt_nrc_gradient", "vt_nrc_render_config",
+    "vt_nrc_gradient", "vt_nrc_render_config", "vt_nrc_timing",
 ]
 
 
@@ -376,6 +376,11 @@ class Engine:
 
     def nrc_render_config(self, enabled, train_paths=65536):
         check(lib().vt_nrc_render_config(self._h, C.c_int32(1 if enabled else 0), C.c_uint32(train_paths)), self._h)
+
+    def nrc_timing(self):
+        a, b = C.c_float(), C.c_float()
+        check(lib().vt_nrc_timing(self._h, C.byref(a), C.byref(b)), self._h)
+        return a.value, b.value
 
     def nrc_last_loss(self):
         loss = C.c_float()
