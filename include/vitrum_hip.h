@@ -325,6 +325,38 @@ vt_status vt_nrc_gradient(vt_engine *eng, const float *features, const float *ta
  * each and drive one Adam step per frame.  Resets accumulation. */
 vt_status vt_nrc_render_config(vt_engine *eng, int32_t enabled, uint32_t train_paths);
 
+/* ---- batches: many scenes in one persistent launch --------------------------
+ * No single counterpart in the reference (it renders one scene per engine);
+ * a batch carries several scenes' traversal buffers so that the closest-hit
+ * rays of all of them run in one load-balanced persistent launch, which can
+ * be captured once into a HIP graph and replayed.  ray_offsets (host, scene
+ * count + 1 entries, ray_offsets[0] = 0) gives each scene's ray range in the
+ * ray array; hit records are per scene (that scene's primitive/instance
+ * indices), identical to vt_trace_closest on that scene alone. */
+typedef struct vt_batch vt_batch;
+typedef struct vt_tuning {
+  uint32_t refill_min;    /* wave refill threshold */
+  uint32_t steps;         /* traversal steps between refills */
+  uint32_t near_first;    /* 0 left-first, 1 nearest-first inside BLASes */
+  uint32_t blocks_per_cu; /* resident 128-thread blocks per CU */
+  float ms;               /* measured median launch time (autotuner output) */
+} vt_tuning;
+vt_status vt_batch_create(vt_engine *eng, vt_batch **out);
+vt_status vt_batch_destroy(vt_batch *b);
+vt_status vt_batch_add_scene(vt_batch *b, const vt_scene_desc *scene, uint32_t *index);
+vt_status vt_batch_scene_count(const vt_batch *b, uint32_t *count);
+/* One batched closest-hit launch (device pointers); *ms, if given, receives
+ * the event-timed launch duration (and the call then waits for it). */
+vt_status vt_batch_trace_closest_device(vt_batch *b, const vt_ray *d_rays, const uint32_t *ray_offsets,
+                                        vt_hit *d_hits, float *ms);
+/* Captures the launch into a HIP graph; vt_batch_replay re-runs it. */
+vt_status vt_batch_capture(vt_batch *b, const vt_ray *d_rays, const uint32_t *ray_offsets, vt_hit *d_hits);
+vt_status vt_batch_replay(vt_batch *b, float *ms);
+/* Measures candidate launch parameters on these rays, keeps the fastest. */
+vt_status vt_batch_autotune(vt_batch *b, const vt_ray *d_rays, const uint32_t *ray_offsets, vt_hit *d_hits,
+                            uint32_t repeats, vt_tuning *chosen);
+vt_status vt_batch_set_tuning(vt_batch *b, const vt_tuning *t);
+
 /* invertMat4 (packages/shared-bvh/src/mathUtils.ts:21-110): binary64
  * cofactor inverse of a column-major mat4 rounded to binary32, with the
  * reciprocal-product check.  VT_ERR_RANGE when the matrix is not invertible.

@@ -112,6 +112,34 @@ void SceneBuffers::release() {
 // Lifecycle
 // ---------------------------------------------------------------------------
 
+// Packs, builds and uploads the traversal buffers of one scene (no shading
+// data); used by the batched launch.
+vt_status engine_build_trace_scene(vt_engine* eng, const vt_scene_desc* scene, SceneBuffers& out,
+                                   vt::DeviceLayout& layout) {
+  if (!scene) return set_err(eng, VT_ERR_INVALID_ARGUMENT, "scene is NULL");
+  if (vt_status s = check_prims(eng, scene->primitives, scene->primitive_count)) return s;
+  std::vector<vt::PrimitiveIn> prims(scene->primitive_count);
+  for (uint32_t i = 0; i < scene->primitive_count; ++i) prims[i] = to_prim(scene->primitives[i]);
+  vt::Pack pack;
+  const uint32_t threads = std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+  vt::BuildError be = vt::pack_scene(prims.data(), (uint32_t)prims.size(), threads, pack);
+  if (be) return set_err(eng, be.code, be.message);
+  std::string le = vt::build_device_layout(pack, layout);
+  if (!le.empty()) return set_err(eng, VT_ERR_RANGE, le);
+  HIP_TRY(eng, hipSetDevice(eng->device));
+  vt_status s = VT_OK;
+  s = s ? s : engine_upload(eng, out.blas, layout.blas_nodes.data(), layout.blas_nodes.size() * 4);
+  s = s ? s : engine_upload(eng, out.tlas, layout.tlas_nodes.data(), layout.tlas_nodes.size() * 4);
+  s = s ? s : engine_upload(eng, out.tris, layout.tris.data(), layout.tris.size() * 4);
+  s = s ? s : engine_upload(eng, out.instances, layout.instances.data(), layout.instances.size() * 4);
+  if (!s) {
+    hipError_t r = hipStreamSynchronize(eng->stream);
+    if (r != hipSuccess) s = set_err(eng, VT_ERR_HIP, hipGetErrorString(r));
+  }
+  if (s) out.release();
+  return s;
+}
+
 extern "C" {
 
 vt_status vt_invert_mat4(const float m[16], float out[16]) {

@@ -61,6 +61,9 @@ struct vt_engine {
 };
 
 vt_status engine_upload(vt_engine* e, DeviceBuffer& dst, const void* src, size_t bytes);
+// Packs, builds and uploads one scene's traversal buffers (engine.cpp).
+vt_status engine_build_trace_scene(vt_engine* e, const vt_scene_desc* scene, SceneBuffers& out,
+                                   vt::DeviceLayout& layout);
 // Hooks into the integrator (render.cpp): rebuild per-scene render data, free it.
 vt_status engine_on_scene_changed(vt_engine* e);
 void engine_release_render(vt_engine* e);

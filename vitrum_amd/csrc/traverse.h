@@ -37,10 +37,17 @@ struct TraceParams {
   uint32_t refill_min;    // refill a wave once this many lanes are idle
   uint32_t steps;         // traversal steps between refill checks
   uint32_t near_first;    // closest hit: visit the nearer child first (ties as the reference)
+  // Batched launch: rays [offsets[k], offsets[k+1]) belong to scenes[k].
+  const DeviceScene* scenes;
+  const uint32_t* scene_offsets;
+  uint32_t scene_count;
 };
 
 hipError_t launch_trace(const TraceParams& p, bool any, bool count, uint32_t grid, hipStream_t s);
+// Closest hit over a batch of scenes (p.scenes / p.scene_offsets / p.scene_count).
+hipError_t launch_trace_batch(const TraceParams& p, uint32_t grid, hipStream_t s);
 hipError_t trace_occupancy(bool any, int* blocks_per_cu);
+hipError_t trace_occupancy_batch(int* blocks_per_cu);
 hipError_t launch_zero(uint32_t* p, uint32_t n, hipStream_t s);
 // Streaming copy of n16 16-byte words.
 hipError_t launch_copy16(uint4* dst, const uint4* src, size_t n16, hipStream_t s);

@@ -309,7 +309,7 @@ __device__ __forceinline__ V3 xform_dir(float4 c0, float4 c1, float4 c2, V3 d) {
 // instance geometry and exists for measurement only.  Node culling stays
 // `tNear > best` (equal distances are still visited).  The counting variant
 // always uses the reference order, so its work counts are the reference's.
-template <bool ANY, bool COUNT, uint32_t BLOCK, uint32_t LDS_DEPTH>
+template <bool ANY, bool COUNT, bool MULTI, uint32_t BLOCK, uint32_t LDS_DEPTH>
 __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceParams p) {
   // Per-lane stack: LDS_DEPTH entries in LDS (entry-major, so a wave's
   // accesses at equal depth touch 64 distinct banks), deeper entries in a
@@ -332,10 +332,13 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceParams p) {
     const unsigned long long w = spill[(i - LDS_DEPTH) * spill_stride + gtid];
     return make_uint2((uint32_t)w, (uint32_t)(w >> 32));
   };
+  // Scene of the current ray: the launch's scene, or (batched) the one whose
+  // ray range holds the ray, looked up when the lane takes a new ray.
   const float4* __restrict__ blas = p.scene.blas_nodes;
   const float4* __restrict__ tlas = p.scene.tlas_nodes;
   const float4* __restrict__ tris = p.scene.tris;
   const float4* __restrict__ insts = p.scene.instances;
+  uint32_t tlas_root = p.scene.tlas_root, tlas_count = p.scene.tlas_node_count;
   const uint32_t lane = __lane_id();
   const uint64_t lane_below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const bool near = !ANY && !COUNT && p.near_first != 0u;        // inside BLASes
@@ -448,6 +451,20 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceParams p) {
         const uint32_t idx = base + (uint32_t)__popcll(idle & lane_below);
         if (idx < n_rays) {
           ray_idx = idx;
+          if (MULTI) {
+            uint32_t lo_s = 0, hi_s = p.scene_count;
+            while (lo_s + 1u < hi_s) {
+              const uint32_t mid = (lo_s + hi_s) >> 1;
+              if (p.scene_offsets[mid] <= idx) lo_s = mid; else hi_s = mid;
+            }
+            const DeviceScene& S = p.scenes[lo_s];
+            blas = S.blas_nodes;
+            tlas = S.tlas_nodes;
+            tris = S.tris;
+            insts = S.instances;
+            tlas_root = S.tlas_root;
+            tlas_count = S.tlas_node_count;
+          }
           const float4 a = p.rays[2 * (size_t)idx], b = p.rays[2 * (size_t)idx + 1];
           ro = v3(a.x, a.y, a.z);
           tmin = a.w;
@@ -461,7 +478,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceParams p) {
           bu = bv = 0;
           level = 0;
           sp = 0;
-          cur = p.scene.tlas_node_count ? p.scene.tlas_root : kNone;
+          cur = tlas_count ? tlas_root : kNone;
           if (COUNT) c_blas = c_tri = c_tlas = c_inst = 0;
           active = true;
           if (cur == kNone) finish(false);  // no TLAS: fail closed
@@ -597,11 +614,16 @@ __global__ void fp32_selftest_kernel(const float* a, const float* b, float* q, f
 
 namespace vtk {
 
-template <bool ANY, bool COUNT>
+template <bool ANY, bool COUNT, bool MULTI = false>
 static hipError_t launch_one(const TraceParams& p, uint32_t grid, hipStream_t s) {
-  hipLaunchKernelGGL((trace_kernel<ANY, COUNT, kTraceBlock, kLdsDepth>), dim3(grid),
+  hipLaunchKernelGGL((trace_kernel<ANY, COUNT, MULTI, kTraceBlock, kLdsDepth>), dim3(grid),
                      dim3(kTraceBlock), 0, s, p);
   return hipGetLastError();
+}
+
+hipError_t launch_trace_batch(const TraceParams& p, uint32_t grid, hipStream_t s) {
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, p.counter, 1u);
+  return launch_one<false, false, true>(p, grid, s);
 }
 
 hipError_t launch_trace(const TraceParams& p, bool any, bool count, uint32_t grid, hipStream_t s) {
@@ -610,12 +632,17 @@ hipError_t launch_trace(const TraceParams& p, bool any, bool count, uint32_t gri
   return count ? launch_one<false, true>(p, grid, s) : launch_one<false, false>(p, grid, s);
 }
 
+hipError_t trace_occupancy_batch(int* blocks_per_cu) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      blocks_per_cu, trace_kernel<false, false, true, kTraceBlock, kLdsDepth>, kTraceBlock, 0);
+}
+
 hipError_t trace_occupancy(bool any, int* blocks_per_cu) {
   if (any)
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        blocks_per_cu, trace_kernel<true, false, kTraceBlock, kLdsDepth>, kTraceBlock, 0);
+        blocks_per_cu, trace_kernel<true, false, false, kTraceBlock, kLdsDepth>, kTraceBlock, 0);
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      blocks_per_cu, trace_kernel<false, false, kTraceBlock, kLdsDepth>, kTraceBlock, 0);
+      blocks_per_cu, trace_kernel<false, false, false, kTraceBlock, kLdsDepth>, kTraceBlock, 0);
 }
 
 hipError_t launch_zero(uint32_t* p, uint32_t n, hipStream_t s) {

@@ -33,8 +33,15 @@ EXPORTED = [
     "vt_reset", "vt_frame_time", "vt_capture_signatures", "vt_render_info", "vt_invert_mat4",
     "vt_nrc_param_count", "vt_nrc_reset", "vt_nrc_set_weights", "vt_nrc_get_weights", "vt_nrc_infer",
     "vt_nrc_infer_device", "vt_nrc_train_step", "vt_nrc_train_step_device", "vt_nrc_last_loss",
-    "vt_nrc_gradient", "vt_nrc_render_config", "vt_nrc_timing",
+    "vt_nrc_gradient", "vt_nrc_render_config", "vt_nrc_timing", "vt_batch_create", "vt_batch_destroy",
+    "vt_batch_add_scene", "vt_batch_scene_count", "vt_batch_trace_closest_device", "vt_batch_capture",
+    "vt_batch_replay", "vt_batch_autotune", "vt_batch_set_tuning",
 ]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("refill_min", C.c_uint32), ("steps", C.c_uint32), ("near_first", C.c_uint32),
+                ("blocks_per_cu", C.c_uint32), ("ms", C.c_float)]
 
 
 class NrcOptions(C.Structure):
@@ -426,3 +433,73 @@ class Engine:
         s = FrameStats()
         check(lib().vt_stats(self._h, C.byref(s)), self._h)
         return {k: getattr(s, k) for k, _ in FrameStats._fields_}
+
+
+def _scene_desc(scene):
+    """(SceneDesc, keep-alive objects) of a scene dict."""
+    prims = []
+    mats = []
+    for i, p in enumerate(scene["primitives"]):
+        prims.append(dict(p, material_id=i))
+        mats.append(_material(p.get("material", {})))
+    arr, keep = _prim_structs(prims)
+    marr = (Material * max(1, len(mats)))(*mats)
+    sky = scene.get("sky", [0.0, 0.0, 0.0])
+    env = Environment(1 if any(sky) else 0, (C.c_float * 3)(*sky))
+    desc = SceneDesc(arr, len(prims), marr, len(mats), env, float(scene.get("scene_radius", 0.0)))
+    return desc, (arr, keep, marr)
+
+
+class Batch:
+    """Several scenes traced in one persistent launch (vt_batch_*)."""
+
+    def __init__(self, engine):
+        self._eng = engine
+        self._h = C.c_void_p()
+        check(lib().vt_batch_create(engine.handle, C.byref(self._h)), engine.handle)
+
+    def close(self):
+        if self._h:
+            lib().vt_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_scene(self, scene):
+        desc, _keep = _scene_desc(scene)
+        idx = C.c_uint32()
+        check(lib().vt_batch_add_scene(self._h, C.byref(desc), C.byref(idx)), self._eng.handle)
+        return idx.value
+
+    @staticmethod
+    def _offsets(offsets):
+        o = np.ascontiguousarray(offsets, np.uint32)
+        return o, o.ctypes.data_as(U32P)
+
+    def trace_closest_device(self, d_rays, offsets, d_hits, timed=True):
+        o, op = self._offsets(offsets)
+        ms = C.c_float()
+        check(lib().vt_batch_trace_closest_device(self._h, C.c_void_p(d_rays), op, C.c_void_p(d_hits),
+                                                  C.byref(ms) if timed else None), self._eng.handle)
+        return ms.value
+
+    def capture(self, d_rays, offsets, d_hits):
+        o, op = self._offsets(offsets)
+        check(lib().vt_batch_capture(self._h, C.c_void_p(d_rays), op, C.c_void_p(d_hits)), self._eng.handle)
+
+    def replay(self):
+        ms = C.c_float()
+        check(lib().vt_batch_replay(self._h, C.byref(ms)), self._eng.handle)
+        return ms.value
+
+    def autotune(self, d_rays, offsets, d_hits, repeats=3):
+        o, op = self._offsets(offsets)
+        t = Tuning()
+        check(lib().vt_batch_autotune(self._h, C.c_void_p(d_rays), op, C.c_void_p(d_hits), C.c_uint32(repeats),
+                                      C.byref(t)), self._eng.handle)
+        return {"refill_min": t.refill_min, "steps": t.steps, "near_first": t.near_first,
+                "blocks_per_cu": t.blocks_per_cu, "ms": t.ms}
