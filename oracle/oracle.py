@@ -465,3 +465,44 @@ def render(pack, materials, camera_inv_vp, width, height, *, frame_seed, frame_i
                      ev.ctypes.data_as(C.POINTER(C.c_uint64)) if events else None,
                      C.c_uint32(threads or min(16, os.cpu_count() or 1)))
     return (out, sig, ev) if events else (out, sig)
+
+
+# ---------------------------------------------------------------------------
+# Owen-scrambled Sobol streams (sobol.c)
+# ---------------------------------------------------------------------------
+
+def _sobol_lib():
+    L = lib()
+    for name in ("orc_rev32", "orc_sobol_hash"):
+        getattr(L, name).restype = C.c_uint32
+        getattr(L, name).argtypes = [C.c_uint32]
+    for name in ("orc_sobol_combine", "orc_laine_karras", "orc_nested_scramble", "orc_sobol_component",
+                 "orc_sobol_rotation", "orc_sobol_frame_key"):
+        getattr(L, name).restype = C.c_uint32
+        getattr(L, name).argtypes = [C.c_uint32, C.c_uint32]
+    L.orc_sobol_owen_u32.restype = C.c_uint32
+    L.orc_sobol_owen_u32.argtypes = [C.c_uint32] * 4
+    L.orc_sobol_streams.argtypes = [U32P, U32P, C.c_uint32, C.c_uint32, C.c_uint32, U32P]
+    L.orc_sobol_stream_state.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, U32P]
+    return L
+
+
+def sobol(name, *args):
+    """Call one scalar Sobol primitive of the oracle by name (e.g. 'sobol_hash')."""
+    return getattr(_sobol_lib(), "orc_" + name)(*[C.c_uint32(a & 0xFFFFFFFF) for a in args])
+
+
+def sobol_streams(px, py, key, draws):
+    px = np.ascontiguousarray(px, np.uint32)
+    py = np.ascontiguousarray(py, np.uint32)
+    out = np.zeros((px.size, draws), np.uint32)
+    _sobol_lib().orc_sobol_streams(px.ctypes.data_as(U32P), py.ctypes.data_as(U32P), px.size, key, draws,
+                                   out.ctypes.data_as(U32P))
+    return out
+
+
+def sobol_stream_state(px, py, key):
+    s = np.zeros(7, np.uint32)
+    _sobol_lib().orc_sobol_stream_state(px, py, key, s.ctypes.data_as(U32P))
+    return dict(zip(["sampleIndex", "dimension", "pixelX", "pixelY", "sequenceKey", "rotationTile",
+                     "fallbackState"], [int(x) for x in s]))
