@@ -60,8 +60,28 @@ static d3 safe_normalize(d3 v) {
 
 /* ---- RNG (shared-samplers/src/wgsl/pcg.wgsl.ts; common.wgsl.ts) ------- */
 
-static uint32_t pcg_next(uint32_t *s) { return orc_pcg_next(s); }
-static double rand01(uint32_t *s) { return (double)(pcg_next(s) >> 8) / 16777216.0; }
+/* Sampler: PCG, or the Owen-scrambled Sobol stream of sobol.c (pt-webgpu's
+ * Sobol mode, common.wgsl.ts:22-197) continuing with PCG after dimension
+ * 511; dim == ORC_PCG_ONLY selects PCG. */
+#define ORC_PCG_ONLY 0xFFFFFFFFu
+typedef struct {
+  uint32_t s, dim, sample, tile, seed;
+} orng;
+uint32_t orc_sobol_owen_u32(uint32_t path, uint32_t dim, uint32_t tile, uint32_t stream_seed);
+uint32_t orc_sobol_hash(uint32_t x);
+uint32_t orc_sobol_combine(uint32_t s, uint32_t v);
+uint32_t orc_sobol_frame_key(uint32_t seed, uint32_t frame);
+
+static uint32_t pcg_next(orng *r) {
+  if (r->dim < 512u) {
+    const uint32_t v = orc_sobol_owen_u32(r->sample, r->dim, r->tile, r->seed);
+    r->dim += 1u;
+    return v;
+  }
+  if (r->dim != ORC_PCG_ONLY) r->dim += 1u;
+  return orc_pcg_next(&r->s);
+}
+static double rand01(orng *s) { return (double)(pcg_next(s) >> 8) / 16777216.0; }
 
 /* represented_bernoulli_probability_f32 (pcg.wgsl.ts:31-40) */
 static double represented(double p) {
@@ -72,7 +92,7 @@ static double represented(double p) {
 }
 
 /* ptRandomU32 / ptRandBoundedU32 (pt-webgpu/src/wgsl/common.wgsl.ts) */
-static uint32_t rand_bounded(uint32_t *s, uint32_t bound) {
+static uint32_t rand_bounded(orng *s, uint32_t bound) {
   if (bound <= 1) return 0;
   uint32_t threshold = ((0xFFFFFFFFu % bound) + 1u) % bound;
   for (;;) {
@@ -232,7 +252,7 @@ static void build_onb(d3 n, d3 *t, d3 *b) {
 }
 
 /* sampleGgxVndfTangent (bsdf.wgsl.ts:1710-1735) */
-static d3 vndf_sample(d3 wo, double alpha, uint32_t *rng) {
+static d3 vndf_sample(d3 wo, double alpha, orng *rng) {
   d3 vh = safe_normalize(D3(alpha * wo.x, alpha * wo.y, wo.z));
   double lensq = vh.x * vh.x + vh.y * vh.y;
   d3 t1 = lensq > 1e-10 ? scl(D3(-vh.y, vh.x, 0), 1.0 / sqrt(lensq)) : D3(1, 0, 0);
@@ -368,6 +388,7 @@ typedef struct orc_render_desc {
   uint32_t frame_seed;
   uint32_t max_bounces;
   float ray_bias;
+  uint32_t sampling; /* 0 = PCG, 1 = Owen-scrambled Sobol */
 } orc_render_desc;
 
 typedef struct {
@@ -484,7 +505,22 @@ static uint64_t mixlog(uint64_t h, uint64_t v, uint64_t *ev) {
 static void render_path(const rctx *c, uint32_t px, uint32_t py, uint32_t frame_index, double *out,
                         uint64_t *sig_out, const int *sorted_to_orig, uint64_t *ev) {
   const orc_render_desc *d = c->d;
-  uint32_t rng = orc_pcg_init(px, py, d->frame_seed ^ frame_index);
+  orng rng;
+  memset(&rng, 0, sizeof rng);
+  if (d->sampling == 1) {
+    /* pcgInit of the Sobol mode (common.wgsl.ts:130-150) */
+    const uint32_t key = orc_sobol_frame_key(d->frame_seed, frame_index);
+    const uint32_t pixel_seed = orc_sobol_hash(orc_sobol_combine(orc_sobol_hash(px), py));
+    rng.sample = key & 0xFFFFu;
+    rng.dim = 0;
+    rng.tile = (px & 7u) | ((py & 7u) << 3);
+    rng.seed = orc_sobol_hash(orc_sobol_combine(pixel_seed, key >> 16));
+    rng.s = orc_sobol_hash(orc_sobol_combine(
+        orc_sobol_combine(orc_sobol_combine(pixel_seed, key >> 16), key & 0xFFFFu), 512u));
+  } else {
+    rng.dim = ORC_PCG_ONLY;
+    rng.s = orc_pcg_init(px, py, d->frame_seed ^ frame_index);
+  }
   const float jx = (float)rand01(&rng), jy = (float)rand01(&rng);
   d3 o, dir;
   d3 L = D3(0, 0, 0), T = D3(1, 1, 1);

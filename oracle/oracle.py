@@ -300,7 +300,7 @@ class RenderDesc(C.Structure):
         ("env_radiance", C.c_float * 3), ("env_w", C.c_uint32), ("env_h", C.c_uint32),
         ("env_cdf", F32P), ("env_pdf", F32P), ("inv_view_proj", C.c_float * 16),
         ("width", C.c_uint32), ("height", C.c_uint32), ("frame_seed", C.c_uint32),
-        ("max_bounces", C.c_uint32), ("ray_bias", C.c_float),
+        ("max_bounces", C.c_uint32), ("ray_bias", C.c_float), ("sampling", C.c_uint32),
     ]
 
 
@@ -421,7 +421,8 @@ def build_constant_env(radiance, width=64, height=32):
 
 
 def render(pack, materials, camera_inv_vp, width, height, *, frame_seed, frame_index, max_bounces,
-           ray_bias, env_radiance=None, env_size=(64, 32), tile=None, threads=None, events=False):
+           ray_bias, env_radiance=None, env_size=(64, 32), tile=None, threads=None, events=False,
+           sampling=0):
     """One frame (1 spp) of the binary64 integrator mirror.  Returns
     (radiance[h, w, 3] float64, signature[h, w] uint64), plus with `events`
     the decisions mixed into each signature ([h, w, 40] uint64: count, then
@@ -456,6 +457,7 @@ def render(pack, materials, camera_inv_vp, width, height, *, frame_seed, frame_i
     d.frame_seed = frame_seed
     d.max_bounces = max_bounces
     d.ray_bias = ray_bias
+    d.sampling = sampling
     x0, y0, w, h = tile or (0, 0, width, height)
     out = np.zeros((h, w, 3), np.float64)
     sig = np.zeros((h, w), np.uint64)

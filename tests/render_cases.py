@@ -11,7 +11,7 @@ def materials_of(scene):
     return [p.get("material", {}) for p in scene["primitives"]]
 
 
-def cpu_frames(scene, res, frames, bias, max_bounces=8):
+def cpu_frames(scene, res, frames, bias, max_bounces=8, sampling=0):
     prims = [dict(p, material_id=i) for i, p in enumerate(scene["primitives"])]
     pack = O.pack_scene(prims)
     ivp = W.inv_view_proj(scene["camera"], res, res)
@@ -20,7 +20,7 @@ def cpu_frames(scene, res, frames, bias, max_bounces=8):
     for f in range(frames):
         img, sig = O.render(pack, materials_of(scene), ivp, res, res, frame_seed=SEED, frame_index=f,
                             max_bounces=max_bounces, ray_bias=bias,
-                            env_radiance=sky if any(sky) else None)
+                            env_radiance=sky if any(sky) else None, sampling=sampling)
         out.append(img)
         sigs.append(sig)
     return np.stack(out), np.stack(sigs)

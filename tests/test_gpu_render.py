@@ -28,12 +28,12 @@ def gpu_frames(engine, scene, res, frames, max_bounces=8):
     return engine.capture(res, res)[..., :3].astype(np.float64), engine.capture_signatures(res, res)
 
 
-def check_single_frame(engine, scene, res):
+def check_single_frame(engine, scene, res, sampling=0):
     gpu, gsig = gpu_frames(engine, scene, res, 1)
     info = engine.render_info()
     bias, _ = W.ray_bias(scene["scene_radius"])
     assert info["ray_bias"] == pytest.approx(float(bias), rel=1e-6)
-    cpu, csig = cpu_frames(scene, res, 1, info["ray_bias"])
+    cpu, csig = cpu_frames(scene, res, 1, info["ray_bias"], sampling=sampling)
     cpu, csig = cpu[0], csig[0]
     match = gsig == csig
     rate = match.mean()
@@ -55,6 +55,24 @@ def test_spheres_paths_match_fp64_integrator(engine):
     sc = W.spheres_scene(instances=48, lights=4)
     gpu, cpu, rate = check_single_frame(engine, sc, 64)
     assert gpu.mean() > 0.01
+
+
+@pytest.fixture(scope="module")
+def sobol_engine():
+    from vitrum_amd import native
+    eng = native.Engine(0, sampling=1)
+    yield eng
+    eng.close()
+
+
+def test_sobol_sampler_paths_match_fp64_integrator(sobol_engine):
+    """Owen-scrambled Sobol mode: the same path-matching contract; the
+    sampler's words are those of the oracle's pinned Sobol streams."""
+    sc = W.cornell_scene()
+    gpu, cpu, rate = check_single_frame(sobol_engine, sc, 64, sampling=1)
+    assert gpu.mean() > 0.05
+    sc = W.spheres_scene(instances=48, lights=4)
+    check_single_frame(sobol_engine, sc, 48, sampling=1)
 
 
 def test_cornell_multiframe_mean_within_statistical_bound(engine):

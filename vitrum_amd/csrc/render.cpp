@@ -11,6 +11,7 @@
 
 #include "engine_internal.h"
 #include "render.h"
+#include "sobol.h"
 
 namespace {
 
@@ -87,13 +88,14 @@ std::vector<float> represented_cdf(const std::vector<double>& w) {
 struct RenderState {
   // Per-scene shading data.
   DeviceBuffer materials, emitters, instances, env_cdf, env_pdf, tri_source;
+  DeviceBuffer sobol_dirs;  // 512 x 16 Sobol direction components (built once)
   uint32_t emitter_count = 0;
   uint32_t env_present = 0, env_w = 64, env_h = 32;
   float env_radiance[3] = {0, 0, 0};
   float ray_bias = 0, tmin = 0;
   // Per-resolution path state.
   uint32_t width = 0, height = 0, paths = 0;
-  DeviceBuffer rng, throughput, conn, radiance, prev_pdf, flags, signature, count;
+  DeviceBuffer rng, rng_dim, throughput, conn, radiance, prev_pdf, flags, signature, count;
   DeviceBuffer queue[2], rays[2], hits, hit_pos;
   DeviceBuffer shadow_path, shadow_rays, shadow_contrib, shadow_occ, accum;
   // Neural radiance cache (used when nrc_enabled).
@@ -103,7 +105,7 @@ struct RenderState {
   DeviceBuffer nrc_count, q_feat, q_thr, q_path, q_rgb, train_rec, train_n, train_feat, train_tgt;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   void release_paths() {
-    for (DeviceBuffer* b : {&rng, &throughput, &conn, &radiance, &prev_pdf, &flags, &signature, &count,
+    for (DeviceBuffer* b : {&rng, &rng_dim, &throughput, &conn, &radiance, &prev_pdf, &flags, &signature, &count,
                             &queue[0], &queue[1], &rays[0], &rays[1], &hits, &hit_pos, &shadow_path,
                             &shadow_rays, &shadow_contrib, &shadow_occ, &accum, &nrc_count, &q_feat, &q_thr,
                             &q_path, &q_rgb, &train_rec, &train_n, &train_feat, &train_tgt})
@@ -111,7 +113,8 @@ struct RenderState {
     width = height = paths = 0;
   }
   void release_scene() {
-    for (DeviceBuffer* b : {&materials, &emitters, &instances, &env_cdf, &env_pdf, &tri_source}) b->release();
+    for (DeviceBuffer* b : {&materials, &emitters, &instances, &env_cdf, &env_pdf, &tri_source, &sobol_dirs})
+      b->release();
   }
 };
 
@@ -247,6 +250,11 @@ vt_status engine_on_scene_changed(vt_engine* e) {
   s = s ? s : engine_upload(e, R.env_cdf, cdf.data(), cdf.size() * 4);
   s = s ? s : engine_upload(e, R.env_pdf, pdf.data(), pdf.size() * 4);
   s = s ? s : engine_upload(e, R.tri_source, P.tri_source.data(), P.tri_source.size() * 4);
+  if (!s && !R.sobol_dirs.ptr) {
+    std::vector<uint16_t> dirs(vts::kDims * vts::kBits);
+    vts::build_directions(dirs.data());
+    s = engine_upload(e, R.sobol_dirs, dirs.data(), dirs.size() * 2);
+  }
   if (s) return s;
   HIP_CHECK(e, hipStreamSynchronize(e->stream));
   return VT_OK;
@@ -265,6 +273,7 @@ static vt_status ensure_paths(vt_engine* e, uint32_t w, uint32_t h) {
     if (!s) s = b.ensure(e, std::max<size_t>(bytes, 64));
   };
   need(R.rng, n * 4);
+  need(R.rng_dim, n * 4);
   need(R.throughput, n * 12);
   need(R.conn, n * 12);
   need(R.radiance, n * 12);
@@ -352,6 +361,9 @@ vt_status vt_render_frame(vt_engine* e, const vt_frame* fr) {
   f.ray_bias = R.ray_bias;
   f.tmin = R.tmin;
   f.pixels = fr->width * fr->height;
+  f.sobol = e->opts.sampling == 1 ? 1u : 0u;
+  f.sobol_key = vts::frame_key(fr->frame_seed, fr->frame_index);
+  f.sobol_dirs = (const uint16_t*)R.sobol_dirs.ptr;
   f.nrc = R.nrc_enabled ? 1u : 0u;
   f.train_paths = R.nrc_enabled ? R.nrc_train_paths : 0u;
   for (int k = 0; k < 3; ++k) f.scene_min[k] = R.scene_min[k], f.scene_inv_extent[k] = R.scene_inv_extent[k];
@@ -373,6 +385,7 @@ vt_status vt_render_frame(vt_engine* e, const vt_frame* fr) {
   for (int k = 0; k < 3; ++k) sc.env_radiance[k] = R.env_radiance[k];
   vtr::PathState ps{};
   ps.rng = (uint32_t*)R.rng.ptr;
+  ps.rng_dim = (uint32_t*)R.rng_dim.ptr;
   ps.throughput = (float*)R.throughput.ptr;
   ps.conn = (float*)R.conn.ptr;
   ps.radiance = (float*)R.radiance.ptr;

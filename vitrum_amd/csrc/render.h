@@ -43,11 +43,16 @@ struct FrameParams {
   uint32_t nrc;
   uint32_t pixels, train_paths;
   float scene_min[3], scene_inv_extent[3];
+  // Sampler: PCG, or Owen-scrambled Sobol (frame key, direction table).
+  uint32_t sobol;
+  uint32_t sobol_key;
+  const uint16_t* sobol_dirs;
 };
 
 // Structure-of-arrays path state, one path per pixel.
 struct PathState {
-  uint32_t* rng;
+  uint32_t* rng;       // sampler state word
+  uint32_t* rng_dim;   // next Sobol dimension (0xFFFFFFFF: PCG)
   float* throughput;   // 3 per path: throughput entering the current vertex
   float* conn;         // 3 per path: throughput carried by the BSDF sample (pre-RR)
   float* radiance;     // 3 per path

@@ -25,6 +25,7 @@
 #include <cstdint>
 
 #include "render.h"
+#include "sobol.h"
 
 namespace vtr {
 
@@ -89,21 +90,39 @@ __device__ __forceinline__ uint32_t pcg_next(uint32_t& s) {
   uint32_t w = ((s >> ((s >> 28) + 4u)) ^ s) * 277803737u;
   return (w >> 22) ^ w;
 }
-__device__ __forceinline__ float rand01(uint32_t& s) {
-  return (float)(pcg_next(s) >> 8) / 16777216.0f;
+// Per-path sampler: PCG (dim == kPcgOnly) or pt-webgpu's Owen-scrambled
+// Sobol stream, which continues with PCG after its last dimension.
+constexpr uint32_t kPcgOnly = 0xFFFFFFFFu;
+struct Rng {
+  uint32_t s;     // PCG state / Sobol continuation state
+  uint32_t dim;   // next Sobol dimension, kPcgOnly for PCG
+  uint32_t seed;  // Sobol stream seed (pixel and sequence key)
+  uint32_t tile;  // Sobol rotation tile
+  uint32_t sample;
+  const uint16_t* dirs;
+};
+__device__ __forceinline__ uint32_t rng_next(Rng& r) {
+  if (r.dim >= vts::kDims) {
+    if (r.dim != kPcgOnly) ++r.dim;
+    return pcg_next(r.s);
+  }
+  const uint32_t v = vts::owen_u32(r.dirs, r.sample, r.dim, r.tile, r.seed);
+  ++r.dim;
+  return v;
 }
+__device__ __forceinline__ float rand01(Rng& r) { return (float)(rng_next(r) >> 8) / 16777216.0f; }
 __device__ float represented(float p) {
   if (!(p > 0.0f)) return 0.0f;
   if (p >= 1.0f) return 1.0f;
   const float b = clampf(floorf(p * 16777216.0f + 0.5f), 1.0f, 16777215.0f);
   return b / 16777216.0f;
 }
-__device__ uint32_t rand_bounded(uint32_t& s, uint32_t bound) {
+__device__ uint32_t rand_bounded(Rng& s, uint32_t bound) {
   if (bound <= 1u) return 0u;
   const uint32_t threshold = ((0xFFFFFFFFu % bound) + 1u) % bound;
   for (;;) {
-    const uint32_t hi = pcg_next(s) & 0xFFFFFF00u;
-    const uint32_t lo = pcg_next(s) >> 24;
+    const uint32_t hi = rng_next(s) & 0xFFFFFF00u;
+    const uint32_t lo = rng_next(s) >> 24;
     const uint32_t w = hi | lo;
     if (w >= threshold) return w % bound;
   }
@@ -269,7 +288,7 @@ __device__ void build_onb(F3 n, F3& t, F3& b) {
 }
 
 // sampleGgxVndfTangent (bsdf.wgsl.ts:1710-1735)
-__device__ F3 vndf_sample(F3 wo, float alpha, uint32_t& rng) {
+__device__ F3 vndf_sample(F3 wo, float alpha, Rng& rng) {
   const F3 vh = safe_normalize(f3(alpha * wo.x, alpha * wo.y, wo.z));
   const float lensq = vh.x * vh.x + vh.y * vh.y;
   const float inv = lensq > 1e-10f ? 1.0f / __builtin_sqrtf(lensq) : 0.0f;
@@ -382,6 +401,30 @@ __device__ void nrc_features(const FrameParams& f, F3 hp, F3 wo, F3 nrm, const M
 
 // ---- kernels -------------------------------------------------------------
 
+// Pixel of self-training path p (a pseudo-random pixel per frame).
+__device__ __forceinline__ uint32_t train_pixel(const FrameParams& f, uint32_t p) {
+  uint32_t h = pcg_init(p - f.pixels, f.frame_index, f.frame_seed ^ 0x5BD1E995u);
+  return pcg_next(h) % f.pixels;
+}
+
+// Sampler of a path (the state word `s` is set by the caller): Sobol for
+// image paths in Sobol mode (pcgInit of common.wgsl.ts:130-150), else PCG.
+__device__ __forceinline__ Rng path_rng(const FrameParams& f, uint32_t px, uint32_t py, bool train) {
+  Rng r;
+  r.s = 0;
+  r.dirs = f.sobol_dirs;
+  r.sample = f.sobol_key & 0xFFFFu;
+  if (f.sobol && !train) {
+    r.dim = 0;
+    r.seed = vts::hash(vts::combine(vts::pixel_seed(px, py), f.sobol_key >> 16));
+    r.tile = (px & 7u) | ((py & 7u) << 3);
+  } else {
+    r.dim = kPcgOnly;
+    r.seed = r.tile = 0;
+  }
+  return r;
+}
+
 // generatePrimaryRay / unproject_ray_common (kernelCore.wgsl.ts:22-26; common.wgsl.ts:412-469)
 __global__ void camera_kernel(FrameParams f, PathState s) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -391,13 +434,13 @@ __global__ void camera_kernel(FrameParams f, PathState s) {
   const bool train = p >= f.pixels;
   if (train) {
     // Self-training path: a pseudo-random pixel and its own PCG stream.
-    uint32_t h = pcg_init(p - f.pixels, f.frame_index, f.frame_seed ^ 0x5BD1E995u);
-    pix = pcg_next(h) % f.pixels;
+    pix = train_pixel(f, p);
     key ^= 0xA511E9B3u;
     s.train_n[p - f.pixels] = 0u;
   }
   const uint32_t px = pix % f.width, py = pix / f.width;
-  uint32_t rng = pcg_init(px, py, key);
+  Rng rng = path_rng(f, px, py, train);
+  rng.s = train || !f.sobol ? pcg_init(px, py, key) : vts::fallback_init(px, py, f.sobol_key);
   const float jx = rand01(rng), jy = rand01(rng);
   const float u = ((float)px + jx) / (float)f.width, v = ((float)py + jy) / (float)f.height;
   const float nx = u * 2.0f - 1.0f, ny = 1.0f - v * 2.0f;
@@ -424,7 +467,8 @@ __global__ void camera_kernel(FrameParams f, PathState s) {
                       (fh[2] * nh[3] - nh[2] * fh[3]) * orient);
     d = safe_normalize(num);
   }
-  s.rng[p] = rng;
+  s.rng[p] = rng.s;
+  s.rng_dim[p] = rng.dim;
   store3(s.throughput + 3 * (size_t)p, f3(1, 1, 1));
   store3(s.conn + 3 * (size_t)p, f3(0, 0, 0));
   store3(s.radiance + 3 * (size_t)p, f3(0, 0, 0));
@@ -458,7 +502,14 @@ __global__ __launch_bounds__(256) void shade_kernel(FrameParams f, ShadeScene sc
       const bool miss = inst == kMiss;
       const float4 rb = s.rays[2 * (size_t)i + 1];
       const F3 dir = f3(rb.x, rb.y, rb.z);
-      uint32_t rng = s.rng[p];
+      Rng rng;
+      {
+        const bool train = p >= f.pixels;
+        const uint32_t pix = train ? train_pixel(f, p) : p;
+        rng = path_rng(f, pix % f.width, pix / f.width, train);
+        rng.s = s.rng[p];
+        rng.dim = s.rng_dim[p];
+      }
       F3 T = load3(s.throughput + 3 * (size_t)p);
       const F3 Cn = load3(s.conn + 3 * (size_t)p);
       F3 L = load3(s.radiance + 3 * (size_t)p);
@@ -666,7 +717,8 @@ __global__ __launch_bounds__(256) void shade_kernel(FrameParams f, ShadeScene sc
         }
         }  // !query
       }
-      s.rng[p] = rng;
+      s.rng[p] = rng.s;
+      s.rng_dim[p] = rng.dim;
       store3(s.throughput + 3 * (size_t)p, T);
       store3(s.radiance + 3 * (size_t)p, L);
       s.flags[p] = flags;
