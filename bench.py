@@ -52,6 +52,104 @@ def cpu_baseline(scene, rays, seconds):
                       f"{dt:.1f} s on {threads} threads"}
 
 
+# MLP FLOPs per sample: forward 2 (4*64*64 + 64*3); backward through
+# W4..W1 2 (3*64 + 3*64*64); weight gradients 2 (4*64*64 + 3*64).
+NRC_FWD_FLOPS = 2 * (4 * 64 * 64 + 64 * 3)
+NRC_TRAIN_FLOPS = NRC_FWD_FLOPS + 2 * (3 * 64 + 3 * 64 * 64) + 2 * (4 * 64 * 64 + 3 * 64)
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def nrc_mlp(eng, reps=10):
+    """Cache MLP kernels on their own: 1920x1080 inference queries and one
+    training step of 65,536 paths x 4 recorded vertices, HIP-event timed."""
+    rng = np.random.default_rng(5)
+    n_inf, n_tr = 1920 * 1080, 65536 * 4
+    f = np.zeros((n_inf, 16), np.float32)
+    f[:, :14] = rng.random((n_inf, 14), dtype=np.float32)
+    t = rng.random((n_tr, 3), dtype=np.float32)
+    d_f, d_o, d_t = eng.alloc(f.nbytes), eng.alloc(n_inf * 12), eng.alloc(t.nbytes)
+    try:
+        eng.h2d(d_f, f)
+        eng.h2d(d_t, t)
+        eng.nrc_reset(seed=3)
+        inf_ms, tr_ms = [], []
+        for k in range(reps + 2):
+            eng.nrc_infer_device(d_f, n_inf, d_o)
+            eng.nrc_train_step_device(d_f, d_t, n_tr)
+            a, b = eng.nrc_timing()
+            if k >= 2:
+                inf_ms.append(a)
+                tr_ms.append(b)
+    finally:
+        for d in (d_f, d_o, d_t):
+            eng.free(d)
+    im, tm = float(np.median(inf_ms)), float(np.median(tr_ms))
+    itf = n_inf * NRC_FWD_FLOPS / (im * 1e-3) / 1e12
+    ttf = n_tr * NRC_TRAIN_FLOPS / (tm * 1e-3) / 1e12
+    return {"infer_samples": n_inf, "infer_ms": round(im, 4), "infer_tflops": round(itf, 2),
+            "train_samples": n_tr, "train_ms": round(tm, 4), "train_tflops": round(ttf, 2),
+            "flops_per_sample": {"infer": NRC_FWD_FLOPS, "train": NRC_TRAIN_FLOPS},
+            "mfma_util_infer": round(itf / BF16_PEAK_TFLOPS, 4),
+            "mfma_util_train": round(ttf / BF16_PEAK_TFLOPS, 4),
+            "peak_tflops": BF16_PEAK_TFLOPS, "dtype": "bf16 (fp32 accumulate)"}
+
+
+def lum(img):
+    return 0.2126 * img[..., 0] + 0.7152 * img[..., 1] + 0.0722 * img[..., 2]
+
+
+def rel_rmse(img, ref):
+    """Luminance RMSE (Rec. 709 weights) over the mean reference luminance."""
+    return float(np.sqrt(np.mean((lum(img) - lum(ref)) ** 2)) / np.mean(lum(ref)))
+
+
+def time_to_rmse(W, width, height, ref_spp, target, train_paths, max_frames):
+    """Plain vs cache-terminated path tracer on the Cornell scene: GPU time
+    (sum of HIP-event frame times) until the relative RMSE against a
+    ref_spp-sample plain reference first reaches `target`."""
+    from vitrum_amd import native
+    seed = 0x9E3779B9
+    eng = native.Engine(0)
+    sc = W.cornell_scene()
+    eng.set_scene(sc)
+    ivp = W.inv_view_proj(sc["camera"], width, height)
+    eng.nrc_render_config(False)
+    eng.reset()
+    t0 = time.perf_counter()
+    for f in range(ref_spp):
+        eng.render_frame(ivp, width, height, 50_000_000 + f, seed, 8)
+        if f % 256 == 255:
+            eng.synchronize()
+            log(f"reference {f + 1}/{ref_spp} spp, {time.perf_counter() - t0:.0f} s")
+    ref = eng.capture(width, height)[..., :3].astype(np.float64)
+    out = {"width": width, "height": height, "reference_spp": ref_spp, "target_rel_rmse": target,
+           "train_paths_per_frame": train_paths}
+    for name, nrc in (("plain", False), ("nrc", True)):
+        eng.nrc_render_config(nrc, train_paths)
+        if nrc:
+            eng.nrc_reset(seed=11)
+        eng.reset()
+        gpu_ms, reached, curve = 0.0, None, []
+        check = 1
+        for f in range(max_frames):
+            eng.render_frame(ivp, width, height, f, seed, 8)
+            gpu_ms += eng.frame_time_ms()
+            if f + 1 == check or f + 1 == max_frames:
+                e = rel_rmse(eng.capture(width, height)[..., :3].astype(np.float64), ref)
+                curve.append([f + 1, round(gpu_ms, 2), round(e, 5)])
+                log(f"{name}: {f + 1} spp, {gpu_ms:.0f} ms, rel RMSE {e:.4f}")
+                if e <= target:
+                    reached = gpu_ms
+                    break
+                check = max(check + 1, int(check * 1.25))
+        out[name] = {"time_to_target_ms": None if reached is None else round(reached, 2),
+                     "frames": curve[-1][0], "curve": curve}
+    p, n = out["plain"]["time_to_target_ms"], out["nrc"]["time_to_target_ms"]
+    out["nrc_speedup"] = round(p / n, 3) if p and n else None
+    eng.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,6 +164,13 @@ def main():
     ap.add_argument("--refill-min", type=int, default=0, help="traversal wave refill threshold (0: default)")
     ap.add_argument("--steps-per-refill", type=int, default=0, help="traversal steps between refills (0: default)")
     ap.add_argument("--near-first", type=int, default=-1, help="1: nearer child first, 0: left first (-1: default)")
+    ap.add_argument("--no-nrc-mlp", action="store_true", help="skip the cache MLP kernel measurement")
+    ap.add_argument("--render-rmse", action="store_true",
+                    help="also measure plain vs cache time to a relative RMSE (minutes of GPU time)")
+    ap.add_argument("--rmse-size", default="1920x1080")
+    ap.add_argument("--rmse-ref-spp", type=int, default=4096)
+    ap.add_argument("--rmse-target", type=float, default=1e-2)
+    ap.add_argument("--rmse-max-frames", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -172,9 +277,16 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if not args.no_nrc_mlp:
+        result["nrc_mlp"] = nrc_mlp(eng)
+        log(f"cache MLP: {result['nrc_mlp']}")
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(sc, rays, args.cpu_seconds)
     eng.close()
+    if args.render_rmse:
+        w, h = (int(x) for x in args.rmse_size.split("x"))
+        result["time_to_rmse"] = time_to_rmse(W, w, h, args.rmse_ref_spp, args.rmse_target, 65536,
+                                              args.rmse_max_frames)
     print(json.dumps(result), flush=True)
 
 
