@@ -28,6 +28,21 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def measured_traffic(kernel_key):
+    """HBM bytes per launch of the closest-hit kernel from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE in KiB per dispatch,
+    each scaled by the reading of a plain copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "trace_closest_spheres1M.traffic.json")
+    try:
+        with open(path) as fh:
+            t = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    if t.get("kernel_key") != kernel_key:
+        return None
+    return int(round((t["fetch_kib"] * t["fetch_scale"] + t["write_kib"] * t["write_scale"]) * 1024))
+
+
 def cpu_baseline(scene, rays, seconds):
     """The oracle (a CPU port of the reference traversal) on a bounded sample."""
     from oracle import oracle as O
@@ -57,6 +72,9 @@ def cpu_baseline(scene, rays, seconds):
 NRC_FWD_FLOPS = 2 * (4 * 64 * 64 + 64 * 3)
 NRC_TRAIN_FLOPS = NRC_FWD_FLOPS + 2 * (3 * 64 + 3 * 64 * 64) + 2 * (4 * 64 * 64 + 3 * 64)
 BF16_PEAK_TFLOPS = 2500.0
+# Identifies the traversal kernel configuration the committed traffic
+# measurement belongs to (child order, refill threshold, steps per refill).
+KERNEL_KEY = "trace_kernel<closest> near_first=1 refill=16 steps=16 defer=16 lds=24"
 
 
 def nrc_mlp(eng, reps=10):
@@ -164,6 +182,8 @@ def main():
     ap.add_argument("--refill-min", type=int, default=0, help="traversal wave refill threshold (0: default)")
     ap.add_argument("--steps-per-refill", type=int, default=0, help="traversal steps between refills (0: default)")
     ap.add_argument("--near-first", type=int, default=-1, help="1: nearer child first, 0: left first (-1: default)")
+    ap.add_argument("--defer-min", type=int, default=0,
+                    help="lanes that must need a leaf/instance operation before a wave runs it (0: default)")
     ap.add_argument("--no-nrc-mlp", action="store_true", help="skip the cache MLP kernel measurement")
     ap.add_argument("--render-rmse", action="store_true",
                     help="also measure plain vs cache time to a relative RMSE (minutes of GPU time)")
@@ -180,9 +200,9 @@ def main():
     from vitrum_amd import native, workloads as W
 
     eng = native.Engine(0)
-    if args.refill_min or args.steps_per_refill or args.near_first >= 0:
+    if args.refill_min or args.steps_per_refill or args.near_first >= 0 or args.defer_min:
         eng.set_trace_tuning(args.refill_min or 16, args.steps_per_refill or 16,
-                             1 if args.near_first < 0 else args.near_first)
+                             1 if args.near_first < 0 else args.near_first, args.defer_min or 16)
     if args.calibrate_copy_mib:
         nb = args.calibrate_copy_mib << 20
         a, b = eng.alloc(nb), eng.alloc(nb)
@@ -272,7 +292,7 @@ def main():
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(KERNEL_KEY),
             "bytes_per_launch": int(ref_bytes),
         },
         "cpu_baseline": None,

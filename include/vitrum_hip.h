@@ -204,8 +204,13 @@ vt_status vt_trace_closest_counts(vt_engine *eng, const vt_ray *rays, uint32_t n
  * the reference resolves them (lowest reordered index), instances in the
  * reference's TLAS order -- hit records identical to the reference's; 2 = also
  * nearest-first across instances (can differ from the reference on coplanar
- * geometry in different instances; for measurement). */
-vt_status vt_set_trace_tuning(vt_engine *eng, uint32_t refill_min, uint32_t steps, uint32_t near_first);
+ * geometry in different instances; for measurement).  A lane whose next
+ * operation is a BLAS leaf's triangle tests, an instance entry or an instance
+ * exit waits until `defer_min` (1..64) lanes of its wave need the same
+ * operation (or no lane has node work left), so the wave runs it once for
+ * many lanes; every lane's own sequence of operations is unchanged. */
+vt_status vt_set_trace_tuning(vt_engine *eng, uint32_t refill_min, uint32_t steps, uint32_t near_first,
+                              uint32_t defer_min);
 
 /* Test hook (no counterpart in the reference): q = a / b and r = sqrt(a)
  * computed on the device, to check that the device's binary32 divide and
@@ -338,6 +343,7 @@ typedef struct vt_tuning {
   uint32_t refill_min;    /* wave refill threshold */
   uint32_t steps;         /* traversal steps between refills */
   uint32_t near_first;    /* 0 left-first, 1 nearest-first inside BLASes */
+  uint32_t defer_min;     /* lanes that must need a leaf/instance operation */
   uint32_t blocks_per_cu; /* resident 128-thread blocks per CU */
   float ms;               /* measured median launch time (autotuner output) */
 } vt_tuning;

@@ -24,7 +24,7 @@ struct vt_batch {
   std::vector<uint32_t> host_offsets;
   bool table_dirty = true;
   // Launch parameters (the autotuner's choice).
-  uint32_t refill_min = 16, steps = 16, near_first = 1, blocks_per_cu = 1;
+  uint32_t refill_min = 16, steps = 16, near_first = 1, defer_min = 16, blocks_per_cu = 1;
   // Captured graph of (counter reset, batched trace).
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -99,6 +99,7 @@ static vtk::TraceParams batch_params(vt_batch* b, const vt_ray* d_rays, vt_hit* 
   p.refill_min = b->refill_min;
   p.steps = b->steps;
   p.near_first = b->near_first;
+  p.defer_min = b->defer_min;
   p.scenes = (const vtk::DeviceScene*)b->table.ptr;
   p.scene_offsets = (const uint32_t*)b->offsets.ptr;
   p.scene_count = (uint32_t)b->scenes.size();
@@ -239,30 +240,33 @@ vt_status vt_batch_autotune(vt_batch* b, const vt_ray* d_rays, const uint32_t* r
   if (n == 0 || !d_rays || !d_hits) return berr(b, VT_ERR_INVALID_ARGUMENT, "empty batch or NULL buffer");
   drop_graph(b);
   const uint32_t max_bpc = b->blocks_per_cu;
-  const uint32_t refills[] = {8, 16, 32}, steps[] = {16, 48}, orders[] = {1, 0};
+  const uint32_t refills[] = {8, 16, 32}, steps[] = {16, 48}, orders[] = {1, 0}, defers[] = {1, 16};
   vt_tuning best{};
   best.ms = -1.0f;
   for (uint32_t bpc = max_bpc; bpc >= 1 && bpc + 2 > max_bpc; --bpc)
     for (uint32_t r : refills)
       for (uint32_t st : steps)
-        for (uint32_t o : orders) {
-          b->refill_min = r;
-          b->steps = st;
-          b->near_first = o;
-          b->blocks_per_cu = bpc;
-          std::vector<float> t;
-          for (uint32_t k = 0; k < repeats; ++k) {
-            float ms = 0;
-            if (vt_status s = vt_batch_trace_closest_device(b, d_rays, ray_offsets, d_hits, &ms)) return s;
-            t.push_back(ms);
+        for (uint32_t o : orders)
+          for (uint32_t d : defers) {
+            b->refill_min = r;
+            b->steps = st;
+            b->near_first = o;
+            b->defer_min = d;
+            b->blocks_per_cu = bpc;
+            std::vector<float> t;
+            for (uint32_t k = 0; k < repeats; ++k) {
+              float ms = 0;
+              if (vt_status s = vt_batch_trace_closest_device(b, d_rays, ray_offsets, d_hits, &ms)) return s;
+              t.push_back(ms);
+            }
+            std::nth_element(t.begin(), t.begin() + t.size() / 2, t.end());
+            const float med = t[t.size() / 2];
+            if (best.ms < 0 || med < best.ms) best = vt_tuning{r, st, o, d, bpc, med};
           }
-          std::nth_element(t.begin(), t.begin() + t.size() / 2, t.end());
-          const float med = t[t.size() / 2];
-          if (best.ms < 0 || med < best.ms) best = vt_tuning{r, st, o, bpc, med};
-        }
   b->refill_min = best.refill_min;
   b->steps = best.steps;
   b->near_first = best.near_first;
+  b->defer_min = best.defer_min;
   b->blocks_per_cu = best.blocks_per_cu;
   *chosen = best;
   return VT_OK;
@@ -270,12 +274,14 @@ vt_status vt_batch_autotune(vt_batch* b, const vt_ray* d_rays, const uint32_t* r
 
 vt_status vt_batch_set_tuning(vt_batch* b, const vt_tuning* t) {
   if (!b || !t) return VT_ERR_INVALID_ARGUMENT;
-  if (t->refill_min < 1 || t->refill_min > 64 || t->steps < 1 || t->near_first > 1 || t->blocks_per_cu < 1 ||
+  if (t->refill_min < 1 || t->refill_min > 64 || t->steps < 1 || t->near_first > 1 || t->defer_min < 1 ||
+      t->defer_min > 64 || t->blocks_per_cu < 1 ||
       t->blocks_per_cu > b->eng->trace_blocks_per_cu)
     return berr(b, VT_ERR_RANGE, "tuning out of range");
   b->refill_min = t->refill_min;
   b->steps = t->steps;
   b->near_first = t->near_first;
+  b->defer_min = t->defer_min;
   b->blocks_per_cu = t->blocks_per_cu;
   drop_graph(b);
   return VT_OK;

@@ -372,6 +372,7 @@ static vt_status trace_device(vt_engine* eng, const vt_ray* d_rays, uint32_t n, 
   p.refill_min = eng->refill_min;
   p.steps = eng->trace_steps;
   p.near_first = eng->near_first;
+  p.defer_min = eng->defer_min;
   eng->last_ray_count = n;
   if (n == 0) return VT_OK;
   HIP_TRY(eng, hipEventRecord(eng->ev_start, eng->stream));
@@ -444,13 +445,16 @@ vt_status vt_trace_any_device(vt_engine* eng, const vt_ray* d_rays, uint32_t n, 
   return trace_device(eng, d_rays, n, d_occluded, true, nullptr);
 }
 
-vt_status vt_set_trace_tuning(vt_engine* eng, uint32_t refill_min, uint32_t steps, uint32_t near_first) {
+vt_status vt_set_trace_tuning(vt_engine* eng, uint32_t refill_min, uint32_t steps, uint32_t near_first,
+                              uint32_t defer_min) {
   if (!eng) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "engine is NULL");
-  if (refill_min < 1 || refill_min > 64 || steps < 1 || near_first > 2)
-    return set_err(eng, VT_ERR_RANGE, "refill_min must be in [1, 64], steps >= 1 and near_first 0, 1 or 2");
+  if (refill_min < 1 || refill_min > 64 || steps < 1 || near_first > 2 || defer_min < 1 || defer_min > 64)
+    return set_err(eng, VT_ERR_RANGE,
+                   "refill_min and defer_min must be in [1, 64], steps >= 1 and near_first 0, 1 or 2");
   eng->refill_min = refill_min;
   eng->trace_steps = steps;
   eng->near_first = near_first;
+  eng->defer_min = defer_min;
   return VT_OK;
 }
 

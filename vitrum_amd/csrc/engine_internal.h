@@ -41,6 +41,7 @@ struct vt_engine {
   uint32_t refill_min = 16;
   uint32_t trace_steps = 16;
   uint32_t near_first = 1;
+  uint32_t defer_min = 16;
   bool counting = false;
   DeviceBuffer counter, error_word, spill;
   DeviceBuffer tmp_rays, tmp_out, tmp_counts;

@@ -332,6 +332,7 @@ static vtk::TraceParams trace_params(vt_engine* e) {
   p.refill_min = e->refill_min;
   p.steps = e->trace_steps;
   p.near_first = e->near_first;
+  p.defer_min = e->defer_min;
   return p;
 }
 

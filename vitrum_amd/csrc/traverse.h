@@ -37,6 +37,7 @@ struct TraceParams {
   uint32_t refill_min;    // refill a wave once this many lanes are idle
   uint32_t steps;         // traversal steps between refill checks
   uint32_t near_first;    // closest hit: visit the nearer child first (ties as the reference)
+  uint32_t defer_min;     // run a leaf / instance operation once this many lanes need it
   // Batched launch: rays [offsets[k], offsets[k+1]) belong to scenes[k].
   const DeviceScene* scenes;
   const uint32_t* scene_offsets;

@@ -36,6 +36,8 @@ constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kEmpty = 0x80000000u;
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr uint32_t kMiss = 0xFFFFFFFFu;
+// Deferred per-lane operations of the traversal loop.
+constexpr uint32_t kPendNone = 0, kPendLeaf = 1, kPendEnter = 2, kPendExit = 3;
 
 __device__ __forceinline__ float u2f(uint32_t u) { return __uint_as_float(u); }
 __device__ __forceinline__ uint32_t f2u(float f) { return __float_as_uint(f); }
@@ -359,8 +361,9 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceParams p) {
   float ltmin = 0, lbest = 0, lu = 0, lv = 0;
   uint32_t lprim = 0, inst_id = 0;
   bool lhit = false;
-  uint32_t slot = 0, slot_end = 0;
+  uint32_t slot = 0, slot_end = 0, enter_slot = 0;
   uint32_t level = 0, cur = kNone, sp = 0, blas_base = 0;
+  uint32_t pend = 0;  // pending deferred operation (kPend*)
   uint32_t c_blas = 0, c_tri = 0, c_tlas = 0, c_inst = 0;
 
   auto enter_instance = [&](uint32_t s) {
@@ -478,6 +481,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceParams p) {
           bu = bv = 0;
           level = 0;
           sp = 0;
+          pend = kPendNone;
           cur = tlas_count ? tlas_root : kNone;
           if (COUNT) c_blas = c_tri = c_tlas = c_inst = 0;
           active = true;
@@ -487,9 +491,68 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceParams p) {
     }
     if (!__any(active)) break;
 
-    for (uint32_t step = 0; active && step < p.steps; ++step) {
+    for (uint32_t step = 0; step < p.steps; ++step) {
+      // ---- deferred expensive work, shared by the lanes that need it ----
+      // A lane whose next operation is an instance exit (merge of the local
+      // hit), an instance entry (ray transform) or a BLAS leaf (triangle
+      // tests) waits until `defer_min` lanes of the wave need the same
+      // operation, or until no lane has cheap node work left; each lane still
+      // performs exactly its own sequence of operations.
+      const uint64_t m_exit = __ballot(active && pend == kPendExit);
+      const uint64_t m_enter = __ballot(active && pend == kPendEnter);
+      const uint64_t m_leaf = __ballot(active && pend == kPendLeaf);
+      const uint64_t m_cheap = __ballot(active && pend == kPendNone);
+      if ((m_exit | m_enter | m_leaf | m_cheap) == 0ull) break;
+      const bool all = m_cheap == 0ull;
+      const uint32_t dm = p.defer_min;
+      if (m_exit && (all || (uint32_t)__popcll(m_exit) >= dm) && active && pend == kPendExit) {
+        finalize_instance();
+        level = 0;
+        if (++slot < slot_end) {
+          enter_slot = slot;
+          pend = kPendEnter;
+        } else {
+          pend = kPendNone;
+        }
+      }
+      if (m_enter && (all || (uint32_t)__popcll(m_enter) >= dm) && active && pend == kPendEnter) {
+        slot = enter_slot;
+        enter_instance(slot);
+        pend = kPendNone;
+      }
+      if (m_leaf && (all || (uint32_t)__popcll(m_leaf) >= dm) && active && pend == kPendLeaf) {
+        const uint32_t count = (cur >> 27) & 15u, first = cur & 0x07FFFFFFu;
+        cur = kNone;
+        pend = kPendNone;
+        if (!lw_ready) {
+          lw = wt_setup(ld);
+          lw_ready = true;
+        }
+        for (uint32_t i = 0; i < count; ++i) {
+          const uint32_t t = first + i;
+          const float4* tr = tris + 3 * (size_t)t;
+          const float4 w0 = tr[0], w1 = tr[1], w2 = tr[2];
+          if (COUNT) ++c_tri;
+          TriHit h;
+          if (tri_test(lo, ld, lw, v3(w0.x, w0.y, w0.z), v3(w0.w, w1.x, w1.y), v3(w1.z, w1.w, w2.x), ltmin,
+                       lbest, near && lhit, h)) {
+            if (ANY) {
+              finish(true);
+              break;
+            }
+            if (h.t == lbest && lhit && !(t < lprim)) continue;  // equal t: lower index wins
+            lhit = true;
+            lbest = h.t;
+            lprim = t;
+            lu = h.u;
+            lv = h.v;
+            lpos = h.point;
+          }
+        }
+      }
+      if (!active || pend != kPendNone) continue;
+      // ---- cheap work: pop with the reference's pop-time cull, expand ----
       if (cur == kNone) {
-        // ---- pop with the reference's pop-time cull; unwind levels ----
         for (;;) {
           if (level == 1) {
             if (sp > blas_base) {
@@ -500,13 +563,8 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceParams p) {
               }
               continue;
             }
-            finalize_instance();
-            level = 0;
-            if (++slot < slot_end) {
-              enter_instance(slot);
-              break;
-            }
-            continue;
+            pend = kPendExit;  // BLAS exhausted
+            break;
           }
           if (sp > 0) {
             const uint2 e = stk_load(--sp);
@@ -519,7 +577,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceParams p) {
           finish(false);
           break;
         }
-        if (!active) break;
+        if (!active || pend != kPendNone) continue;
       }
       if ((cur & kLeafBit) == 0) {
         // ---- interior: one Node64 load, two child boxes ----
@@ -542,46 +600,22 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceParams p) {
           const bool right_first = (level ? near : near_tlas) && tn_r < tn_l;
           if (!push(right_first ? ref_l : ref_r, right_first ? tn_l : tn_r)) {
             finish(false);
-            break;
+            continue;
           }
           cur = right_first ? ref_r : ref_l;
         } else {
           cur = take_l ? ref_l : (take_r ? ref_r : kNone);
         }
-        continue;
       }
-      // ---- leaf ----
-      const uint32_t count = (cur >> 27) & 15u, first = cur & 0x07FFFFFFu;
-      cur = kNone;
-      if (level == 0) {
-        slot = first;
-        slot_end = first + count;
-        enter_instance(slot);
-        continue;
-      }
-      if (!lw_ready) {
-        lw = wt_setup(ld);
-        lw_ready = true;
-      }
-      for (uint32_t i = 0; i < count; ++i) {
-        const uint32_t t = first + i;
-        const float4* tr = tris + 3 * (size_t)t;
-        const float4 w0 = tr[0], w1 = tr[1], w2 = tr[2];
-        if (COUNT) ++c_tri;
-        TriHit h;
-        if (tri_test(lo, ld, lw, v3(w0.x, w0.y, w0.z), v3(w0.w, w1.x, w1.y), v3(w1.z, w1.w, w2.x), ltmin,
-                     lbest, near && lhit, h)) {
-          if (ANY) {
-            finish(true);
-            break;
-          }
-          if (h.t == lbest && lhit && !(t < lprim)) continue;  // equal t: lower index wins
-          lhit = true;
-          lbest = h.t;
-          lprim = t;
-          lu = h.u;
-          lv = h.v;
-          lpos = h.point;
+      // A leaf reached by the pop or the expansion becomes pending work.
+      if (cur != kNone && (cur & kLeafBit)) {
+        if (level == 0) {
+          enter_slot = cur & 0x07FFFFFFu;
+          slot_end = enter_slot + ((cur >> 27) & 15u);
+          cur = kNone;
+          pend = enter_slot < slot_end ? kPendEnter : kPendNone;
+        } else {
+          pend = kPendLeaf;
         }
       }
     }

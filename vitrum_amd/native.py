@@ -41,7 +41,7 @@ EXPORTED = [
 
 class Tuning(C.Structure):
     _fields_ = [("refill_min", C.c_uint32), ("steps", C.c_uint32), ("near_first", C.c_uint32),
-                ("blocks_per_cu", C.c_uint32), ("ms", C.c_float)]
+                ("defer_min", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("ms", C.c_float)]
 
 
 class NrcOptions(C.Structure):
@@ -292,9 +292,9 @@ class Engine:
         check(lib().vt_trace_any(self._h, rays.ctypes.data, rays.size, out.ctypes.data), self._h)
         return out
 
-    def set_trace_tuning(self, refill_min, steps, near_first=1):
+    def set_trace_tuning(self, refill_min, steps, near_first=1, defer_min=16):
         check(lib().vt_set_trace_tuning(self._h, C.c_uint32(refill_min), C.c_uint32(steps),
-                                        C.c_uint32(near_first)), self._h)
+                                        C.c_uint32(near_first), C.c_uint32(defer_min)), self._h)
 
     def selftest_fp32(self, a, b):
         a = np.ascontiguousarray(a, np.float32)
@@ -496,10 +496,14 @@ class Batch:
         check(lib().vt_batch_replay(self._h, C.byref(ms)), self._eng.handle)
         return ms.value
 
+    def set_tuning(self, refill_min, steps, near_first, defer_min, blocks_per_cu):
+        t = Tuning(refill_min, steps, near_first, defer_min, blocks_per_cu, 0.0)
+        check(lib().vt_batch_set_tuning(self._h, C.byref(t)), self._eng.handle)
+
     def autotune(self, d_rays, offsets, d_hits, repeats=3):
         o, op = self._offsets(offsets)
         t = Tuning()
         check(lib().vt_batch_autotune(self._h, C.c_void_p(d_rays), op, C.c_void_p(d_hits), C.c_uint32(repeats),
                                       C.byref(t)), self._eng.handle)
         return {"refill_min": t.refill_min, "steps": t.steps, "near_first": t.near_first,
-                "blocks_per_cu": t.blocks_per_cu, "ms": t.ms}
+                "defer_min": t.defer_min, "blocks_per_cu": t.blocks_per_cu, "ms": t.ms}
