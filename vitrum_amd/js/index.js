@@ -153,6 +153,7 @@ class HipPathTracerEngine {
     this._h = handle;
     this._opts = opts;
     this._scene = null;
+    this._nrc = null;
     this._size = null;
     this._samples = 0;
     this._listeners = { frame: new Set(), progress: new Set(), error: new Set(), warning: new Set() };
@@ -201,6 +202,28 @@ class HipPathTracerEngine {
     }
     this._n.setScene(this._h, blob);
     this._scene = scene;
+    this._samples = 0;
+    if (this._nrc) {
+      this._n.nrcConfig(this._h, this._nrc);
+      this._nrc = { enabled: 1, trainPaths: this._nrc.trainPaths };  // seed / rate apply once
+    }
+  }
+
+  // Radiance cache of the path tracer (the reference's nrcEnabled/nrcConfig,
+  // WalkaroundGPUPipeline.ts:1860-1862).  nrcConfig.recordCap is the number
+  // of self-training records gathered per frame (4 vertices per training
+  // path); learningRate and seed re-initialise the cache.  Kept across
+  // setScene; resets accumulation.
+  setNeuralCache(enabled, config) {
+    this._live();
+    const nrc = nrcSettings(enabled, config);
+    if (this._scene != null) {
+      this._n.nrcConfig(this._h, nrc);
+      // seed / learning rate apply once; later scenes keep the trained cache
+      this._nrc = enabled ? { enabled: 1, trainPaths: nrc.trainPaths } : null;
+    } else {
+      this._nrc = enabled ? nrc : null;  // applied by setScene
+    }
     this._samples = 0;
   }
 
@@ -320,16 +343,38 @@ class HipPathTracerEngine {
   }
 }
 
+// Validated addon settings of the radiance cache from nrcEnabled/nrcConfig.
+function nrcSettings(enabled, config) {
+  const c = Object.assign({}, config || {});
+  for (const k of Object.keys(c)) {
+    if (k !== 'recordCap' && k !== 'learningRate' && k !== 'seed') {
+      throw new TypeError(`pt-hip: nrcConfig.${k} is not supported (recordCap, learningRate, seed)`);
+    }
+  }
+  const cap = c.recordCap == null ? 4 * 65536 : c.recordCap;
+  if (!(Number.isInteger(cap) && cap >= 4 && cap <= 4 * 4194304)) {
+    throw new RangeError('pt-hip: nrcConfig.recordCap must be an integer in [4, 2^24]');
+  }
+  if (c.learningRate != null && !(typeof c.learningRate === 'number' && c.learningRate > 0)) {
+    throw new RangeError('pt-hip: nrcConfig.learningRate must be a positive number');
+  }
+  const nrc = { enabled: enabled ? 1 : 0, trainPaths: Math.ceil(cap / 4) };
+  if (c.learningRate != null) nrc.learningRate = c.learningRate;
+  if (c.seed != null) nrc.seed = c.seed;
+  return nrc;
+}
+
 // EngineFactory (packages/core/src/engine/factory.ts): opts.device is the HIP
 // device ordinal chosen by the host (default 0).
 async function createPTEngine_HIP(opts) {
   const o = Object.assign({}, opts || {});
   const known = new Set(['device', 'maxBounces', 'maxSamplesPerPixel', 'sampling', 'sceneRadius', 'debug',
-    'onWarning', 'denoiser', 'extensions']);
+    'onWarning', 'denoiser', 'extensions', 'nrcEnabled', 'nrcConfig']);
   for (const k of Object.keys(o)) {
     if (!known.has(k)) throw new TypeError(`pt-hip: unknown option '${k}'`);
   }
   if (o.denoiser && o.denoiser !== 'none') throw new RangeError("pt-hip has no denoiser (use denoiser: 'none')");
+  if (o.nrcEnabled) nrcSettings(true, o.nrcConfig);  // validated before any device work
   const native = loadAddon();
   const handle = native.create(o.device == null ? 0 : o.device, {
     maxBounces: o.maxBounces || 0,
@@ -338,6 +383,7 @@ async function createPTEngine_HIP(opts) {
   });
   const engine = new HipPathTracerEngine(native, handle, o);
   if (typeof o.onWarning === 'function') engine.onWarning(o.onWarning);
+  if (o.nrcEnabled) engine.setNeuralCache(true, o.nrcConfig);
   return engine;
 }
 

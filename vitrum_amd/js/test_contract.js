@@ -9,7 +9,7 @@ const pt = require('./index.js');
 
 const addon = pt.loadAddon();
 for (const k of ['create', 'destroy', 'setScene', 'renderFrame', 'capture', 'reset', 'stats',
-  'traceClosest', 'traceAny', 'invertMat4', 'version']) {
+  'traceClosest', 'traceAny', 'invertMat4', 'nrcConfig', 'version']) {
   assert.strictEqual(typeof addon[k], 'function', `addon.${k}`);
 }
 assert.ok(/gfx950/.test(addon.version()));
@@ -43,6 +43,8 @@ assert.throws(() => pt.sceneBlob({ primitives: [], environment: { kind: 'hdri', 
 (async () => {
   await assert.rejects(pt.createPTEngine_HIP({ bogus: 1 }), TypeError);
   await assert.rejects(pt.createPTEngine_HIP({ denoiser: 'oidn-final' }), RangeError);
+  await assert.rejects(pt.createPTEngine_HIP({ nrcEnabled: true, nrcConfig: { levels: 16 } }), TypeError);
+  await assert.rejects(pt.createPTEngine_HIP({ nrcEnabled: true, nrcConfig: { recordCap: 2 } }), RangeError);
   if (process.env.VITRUM_HIP_GPU !== '1') {
     await assert.rejects(pt.createPTEngine_HIP({}), (e) => e.code === 'VT_ERR_NO_DEVICE' || e.code === 'VT_ERR_HIP');
     console.log('pt-hip contract ok (no device)');
@@ -73,6 +75,17 @@ assert.throws(() => pt.sceneBlob({ primitives: [], environment: { kind: 'hdri', 
   // traceClosest through the addon: a ray straight at the triangle.
   const hits = new Float32Array(addon.traceClosest(eng._h, new Float32Array([0.6, 0.3, 1, 0, 0, 0, -1, 1e30])));
   assert.ok(Math.abs(hits[2] - 1) < 1e-6, `t = ${hits[2]}`);
+  // The radiance cache (nrcEnabled): trains online, finite non-negative output.
+  const nrc = await pt.createPTEngine_HIP({ device: 0, nrcEnabled: true, nrcConfig: { recordCap: 4096, seed: 7 } });
+  nrc.setScene({ primitives: [quad], emitters: [], environment: { kind: 'hdri', hdri: { constant: [1, 1, 1] } } });
+  for (let i = 0; i < 4; i += 1) {
+    nrc.renderFrame({ viewMatrix: view, projMatrix: proj, viewport: { width: 16, height: 16, devicePixelRatio: 1 },
+      frameIndex: i, frameSeed: 7, quality: { bounces: 4 } });
+  }
+  const nc = await nrc.captureFrame();
+  assert.ok(nc.rgba.every((x) => Number.isFinite(x) && x >= 0), 'cache output finite and non-negative');
+  nrc.setNeuralCache(false);
+  nrc.dispose();
   eng.dispose();
   assert.strictEqual(eng.state, 'disposed');
   assert.throws(() => eng.setScene({ primitives: [] }), Error);

@@ -472,6 +472,38 @@ napi_value InvertMat4(napi_env env, napi_callback_info info) {
   return arr;
 }
 
+// nrcConfig(handle, {enabled, trainPaths, learningRate, seed}): the radiance
+// cache of the path tracer (nrcEnabled / nrcConfig of the reference's
+// WalkaroundGPUPipeline.ts:1860-1862).  A learningRate or seed re-initialises
+// the cache's weights and Adam state; needs a scene.
+napi_value NrcConfig(napi_env env, napi_callback_info info) {
+  auto a = args_of(env, info, 2);
+  Handle* h = handle_of(env, a[0]);
+  if (!h) return nullptr;
+  uint32_t enabled = 0, train_paths = 65536, seed = 1;
+  float lr = 1e-2f;
+  bool has_lr = false, has_seed = false;
+  if (!is_nullish(env, a[1])) {
+    if (!get_u32(env, a[1], "enabled", 0, &enabled)) return nullptr;
+    if (!get_u32(env, a[1], "trainPaths", 65536, &train_paths)) return nullptr;
+    napi_has_named_property(env, a[1], "learningRate", &has_lr);
+    napi_has_named_property(env, a[1], "seed", &has_seed);
+    if (!get_f32(env, a[1], "learningRate", 1e-2f, &lr)) return nullptr;
+    if (!get_u32(env, a[1], "seed", 1, &seed)) return nullptr;
+  }
+  if (has_lr || has_seed) {
+    vt_nrc_options o{};
+    o.learning_rate = lr;
+    o.beta1 = 0.9f;
+    o.beta2 = 0.999f;
+    o.epsilon = 1e-8f;
+    o.seed = seed;
+    if (throw_status(env, h->eng, vt_nrc_reset(h->eng, &o))) return nullptr;
+  }
+  throw_status(env, h->eng, vt_nrc_render_config(h->eng, enabled ? 1 : 0, train_paths));
+  return nullptr;
+}
+
 napi_value Version(napi_env env, napi_callback_info) {
   napi_value v;
   NAPI_OK(napi_create_string_utf8(env, vt_version(), NAPI_AUTO_LENGTH, &v));
@@ -487,7 +519,7 @@ napi_value Init(napi_env env, napi_value exports) {
              {"capture", Capture},       {"reset", Reset},
              {"stats", Stats},           {"traceClosest", TraceClosest},
              {"traceAny", TraceAny},     {"invertMat4", InvertMat4},
-             {"version", Version}};
+             {"nrcConfig", NrcConfig},   {"version", Version}};
   for (const auto& f : fns) {
     napi_value fn;
     if (napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.cb, nullptr, &fn) != napi_ok) return nullptr;
