@@ -261,6 +261,106 @@ def trace_any(scene: PackedScene, rays, *, threads=None):
     return out
 
 
+HIT64_DTYPE = np.dtype([("hit", "<u4"), ("prim", "<u4"), ("inst", "<u4"), ("pad", "<u4"),
+                        ("t", "<f8"), ("bary", "<f8", 3)])
+
+
+class Cwbvh:
+    """8-wide compressed BVH forest (buildCompressedWideBvhFromArrayBvh,
+    packages/shared-bvh/src/compressedWideBvh.ts:248) over the binary trees at
+    `roots` of a node array; arrays are copied out as numpy views."""
+
+    def __init__(self, node_words, roots=(0,), tri_count=None):
+        L = lib()
+        L.orc_cwbvh_node_count.restype = C.c_uint32
+        L.orc_cwbvh_node_count.argtypes = [C.c_void_p]
+        L.orc_cwbvh_free.argtypes = [C.c_void_p]
+        L.orc_cwbvh_stack_entries.restype = C.c_double
+        words = np.ascontiguousarray(node_words, np.uint32).ravel()
+        roots = np.ascontiguousarray(roots, np.uint32)
+        if tri_count is None:
+            tri_count = 1
+        h = C.c_void_p()
+        rc = L.orc_cwbvh_build(_p(words, U32P), C.c_uint32(words.size // 8), _p(roots, U32P),
+                               C.c_uint32(roots.size), C.c_uint32(tri_count), C.byref(h))
+        if rc != 0:
+            raise OracleError(_err())
+        self._h = h
+        n = L.orc_cwbvh_node_count(h)
+        self.node_count = n
+        self.node_bounds = np.zeros((n, 6), np.float32)
+        self.child_bounds = np.zeros((n, 8, 6), np.uint16)
+        self.child_meta = np.zeros((n, 8, 3), np.uint32)
+        self.child_count = np.zeros(n, np.uint32)
+        self.roots = np.zeros(roots.size, np.uint32)
+        self.status = np.zeros(roots.size, np.uint32)
+        self.stack_entries = np.zeros(roots.size, np.float64)
+        L.orc_cwbvh_copy(h, self.node_bounds.ctypes.data_as(F32P), self.child_bounds.ctypes.data_as(C.c_void_p),
+                         self.child_meta.ctypes.data_as(U32P), self.child_count.ctypes.data_as(U32P),
+                         self.roots.ctypes.data_as(U32P), self.status.ctypes.data_as(U32P),
+                         self.stack_entries.ctypes.data_as(F64P))
+
+    def close(self):
+        if self._h:
+            lib().orc_cwbvh_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def child_box(self, node, slot):
+        """cwbvhChildBounds (compressedWideBvh.ts:481): binary64 decode."""
+        pmn, pmx = self.node_bounds[node, :3].astype(np.float64), self.node_bounds[node, 3:].astype(np.float64)
+        q = self.child_bounds[node, slot].astype(np.float64)
+
+        def deq(qv, mn, mx):
+            if mx == mn or qv <= 0:
+                return mn
+            if qv >= 65535:
+                return mx
+            t = qv / 65535
+            return mn * (1 - t) + mx * t
+
+        return ([deq(q[a], pmn[a], pmx[a]) for a in range(3)], [deq(q[3 + a], pmn[a], pmx[a]) for a in range(3)])
+
+
+def cwbvh_stack_entries(child_meta, child_count, root=0):
+    """requiredCwbvhTraversalStackEntries (compressedWideBvh.ts:60)."""
+    L = lib()
+    L.orc_cwbvh_stack_entries.restype = C.c_double
+    m = np.ascontiguousarray(child_meta, np.uint32).ravel()
+    c = np.ascontiguousarray(child_count, np.uint32).ravel()
+    return L.orc_cwbvh_stack_entries(_p(m, U32P), _p(c, U32P), C.c_uint32(c.size), C.c_uint32(root))
+
+
+def scene_cwbvh(scene: PackedScene):
+    """The wide forest of a packed scene: one tree per distinct BLAS root;
+    returns (Cwbvh, wide root per packed instance)."""
+    blas_roots = np.asarray(scene.pack["tlas_blas_roots"], np.uint32)
+    uniq = np.unique(blas_roots)
+    cw = Cwbvh(scene.pack["bvh_nodes"], uniq, tri_count=scene.pack["indices"].size // 4)
+    lut = dict(zip(uniq.tolist(), cw.roots.tolist()))
+    return cw, np.array([lut[int(r)] for r in blas_roots], np.uint32)
+
+
+def cwbvh_trace_closest(scene: PackedScene, cw, inst_roots, rays, *, threads=None):
+    """intersectCompressedWideBvhFirstHit (compressedWideBvh.ts:754) per
+    tlasIntersect candidate (tlas.ts:1223), binary64."""
+    rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+    out = np.zeros(rays.size, HIT64_DTYPE)
+    roots = np.ascontiguousarray(inst_roots, np.uint32)
+    threads = threads or min(16, os.cpu_count() or 1)
+    rc = lib().orc_cwbvh_trace_closest(C.byref(scene.struct), cw._h, _p(roots, U32P),
+                                       rays.ctypes.data_as(C.c_void_p), C.c_uint32(rays.size),
+                                       out.ctypes.data_as(C.c_void_p), C.c_uint32(threads))
+    if rc != 0:
+        raise OracleError(_err())
+    return out
+
+
 def fp64_hit_params(scene: PackedScene, rays, hits):
     rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
     hits = np.ascontiguousarray(hits, dtype=HIT_DTYPE)

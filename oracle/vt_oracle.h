@@ -172,6 +172,44 @@ int orc_trace_any(const orc_scene *s, const orc_ray *rays, uint32_t n,
 int orc_fp64_hit_params(const orc_scene *s, const orc_ray *rays,
                         const orc_hit *hits, uint32_t n, double *out);
 
+/* ---- The reference's CPU closest hit: 8-wide compressed BVH -------------
+ * buildCompressedWideBvhFromArrayBvh (packages/shared-bvh/src/
+ * compressedWideBvh.ts:248-418) over the binary trees at `roots` of a node
+ * array, concatenated into one forest (roots[] -> wide roots), and
+ * intersectCompressedWideBvhFirstHit (:754-860, binary64 Moller-Trumbore)
+ * composed with tlasIntersect (packages/shared-bvh/src/tlas.ts:1223).
+ * status per root: 0 wide, 1 empty, 2 binary-fallback.                     */
+typedef struct orc_cwbvh orc_cwbvh;
+typedef struct orc_hit64 {
+  uint32_t hit;
+  uint32_t prim;  /* packed (BVH-reordered) triangle index */
+  uint32_t inst;  /* packed instance index */
+  uint32_t pad;
+  double t;
+  double bary[3]; /* weights of vertices a, b, c */
+} orc_hit64;
+int orc_cwbvh_build(const uint32_t *bvh_words, uint32_t node_count,
+                    const uint32_t *roots, uint32_t root_count,
+                    uint32_t tri_count, orc_cwbvh **out);
+void orc_cwbvh_free(orc_cwbvh *c);
+uint32_t orc_cwbvh_node_count(const orc_cwbvh *c);
+/* Any output pointer may be NULL.  Sizes: node_bounds 6/node, child_bounds
+ * 48/node (u16), child_meta 24/node, child_count 1/node; roots, status and
+ * stack_entries one per binary root. */
+int orc_cwbvh_copy(const orc_cwbvh *c, float *node_bounds,
+                   uint16_t *child_bounds, uint32_t *child_meta,
+                   uint32_t *child_count, uint32_t *roots, uint32_t *status,
+                   double *stack_entries);
+/* requiredCwbvhTraversalStackEntries (:60-92) of a given topology. */
+double orc_cwbvh_stack_entries(const uint32_t *child_meta,
+                               const uint32_t *child_count,
+                               uint32_t node_count, uint32_t root);
+/* blas_wide_root: wide root of each packed instance's BLAS. */
+int orc_cwbvh_trace_closest(const orc_scene *s, const orc_cwbvh *c,
+                            const uint32_t *blas_wide_root,
+                            const orc_ray *rays, uint32_t n, orc_hit64 *out,
+                            uint32_t threads);
+
 /* ---- PCG32: packages/shared-samplers/src/wgsl/pcg.wgsl.ts:5-22 ---------- */
 uint32_t orc_pcg_init(uint32_t px, uint32_t py, uint32_t frame_seed);
 uint32_t orc_pcg_next(uint32_t *state);

@@ -172,3 +172,56 @@ def test_spheres_full_frame_bit_exact(engine):
     bounce = W.bounce_rays(pack, rays, ref, st, sc["scene_radius"])
     assert_hits_equal(engine.trace_closest(bounce), O.trace_closest(ps, bounce), "spheres bounce")
     np.testing.assert_array_equal(engine.trace_any(bounce), O.trace_any(ps, bounce))
+
+
+def walk_agreement(got, ts):
+    """Rays whose GPU hit record names the same (triangle, instance) -- or the
+    same miss -- as the reference's binary64 wide-BVH CPU walk."""
+    hw = got["inst"] != 0xFFFFFFFF
+    ht = ts["hit"] == 1
+    return (hw == ht) & (~hw | ((got["prim"] == ts["prim"]) & (got["inst"] == ts["inst"])))
+
+
+def test_cornell_ids_match_reference_cpu_walk(engine, cornell):
+    """Hit records against the reference's own CPU closest hit
+    (intersectCompressedWideBvhFirstHit over the collapsed wide BVH,
+    compressedWideBvh.ts:754, composed with tlasIntersect): identical
+    (triangle, instance) ids for at least 99.9% of the 512x512 primary +
+    bounce rays (edge-grazing rays may differ: the walk's 1e-6 barycentric
+    tolerance admits both neighbours of a shared edge), and t within 1e-6
+    relative of the walk's binary64 t for 99% of primary rays."""
+    sc, pack, ps, rays, hits, bounce = cornell
+    engine.set_scene(sc)
+    allr = np.concatenate([rays, bounce])
+    got = engine.trace_closest(allr)
+    cw, roots = O.scene_cwbvh(ps)
+    ts = O.cwbvh_trace_closest(ps, cw, roots, allr)
+    same = walk_agreement(got, ts)
+    print(f"cornell: {same.mean():.6f} of {allr.size} rays agree with the CPU walk")
+    assert same.mean() >= 0.999, int((~same).sum())
+    prim = np.zeros(allr.size, bool)
+    prim[:rays.size] = True
+    m = prim & same & (ts["hit"] == 1)
+    rel = np.abs(got["t"][m].astype(np.float64) - ts["t"][m]) / np.abs(ts["t"][m])
+    assert np.quantile(rel, 0.99) <= 1e-6
+
+
+def test_spheres_ids_match_reference_cpu_walk(engine):
+    """The 1M-triangle, 256-instance scene: a 262,144-ray sample of the
+    1024x1024 primary + bounce rays against the CPU wide-BVH walk."""
+    sc = W.spheres_scene()
+    engine.set_scene(sc)
+    pack, ps = oracle_scene(sc)
+    rays, st = W.primary_rays(sc["camera"], 1024, 1024, scene_radius=sc["scene_radius"])
+    ph = engine.trace_closest(rays)
+    bounce = W.bounce_rays(pack, rays, ph, st, sc["scene_radius"])
+    allr = np.concatenate([rays, bounce])
+    pick = np.random.default_rng(11).choice(allr.size, 262144, replace=False)
+    sample = allr[np.sort(pick)]
+    got = engine.trace_closest(sample)
+    cw, roots = O.scene_cwbvh(ps)
+    assert np.all(cw.status == 0)
+    ts = O.cwbvh_trace_closest(ps, cw, roots, sample)
+    same = walk_agreement(got, ts)
+    print(f"spheres: {same.mean():.6f} of {sample.size} rays agree with the CPU walk")
+    assert same.mean() >= 0.999, int((~same).sum())
