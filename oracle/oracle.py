@@ -361,6 +361,95 @@ def cwbvh_trace_closest(scene: PackedScene, cw, inst_roots, rays, *, threads=Non
     return out
 
 
+LT_FLOATS = 16
+LT_BUCKETS = 1 << 24
+
+
+def light_tree_build(powers, centroids, aabb_min, aabb_max, cones=None):
+    """buildLightTree + packLightTreeForGPU (packages/shared-samplers/src/
+    lightTree.ts:492, :611).  cones: per emitter None (full sphere) or
+    (axis, theta_o, theta_e).  Returns the packed (nodes, 16) float32 array."""
+    n = len(powers)
+    pw = np.ascontiguousarray(powers, np.float64)
+    c = np.ascontiguousarray(centroids, np.float64).reshape(n, 3)
+    mn = np.ascontiguousarray(aabb_min, np.float64).reshape(n, 3)
+    mx = np.ascontiguousarray(aabb_max, np.float64).reshape(n, 3)
+    has = np.zeros(n, np.uint8)
+    axis = np.zeros((n, 3), np.float64)
+    to = np.zeros(n, np.float64)
+    te = np.zeros(n, np.float64)
+    for i, cn in enumerate(cones or [None] * n):
+        if cn is not None:
+            has[i] = 1
+            axis[i] = cn[0]
+            to[i] = cn[1] if len(cn) > 1 and cn[1] is not None else 0.0
+            te[i] = cn[2] if len(cn) > 2 and cn[2] is not None else np.pi / 2
+    out = np.zeros((max(1, 2 * n - 1), LT_FLOATS), np.float32)
+    count = C.c_uint32(0)
+    rc = lib().orc_light_tree_build(C.c_uint32(n), _p(pw, F64P), _p(c, F64P), _p(mn, F64P), _p(mx, F64P),
+                                    has.ctypes.data_as(C.c_void_p), _p(axis, F64P), _p(to, F64P), _p(te, F64P),
+                                    _p(out, F32P), C.byref(count))
+    if rc != 0:
+        raise OracleError(_err())
+    return out[:count.value].copy()
+
+
+def light_tree_pack_nodes(nodes):
+    """packLightTreeForGPU over explicit nodes: dicts with emitter, power,
+    left, right, aabb_min, aabb_max and optional axis/theta_o/theta_e."""
+    n = len(nodes)
+    em = np.array([d["emitter"] for d in nodes], np.int32)
+    pw = np.array([d["power"] for d in nodes], np.float64)
+    lf = np.array([d["left"] for d in nodes], np.int32)
+    rt = np.array([d["right"] for d in nodes], np.int32)
+    mn = np.array([d.get("aabb_min", (0, 0, 0)) for d in nodes], np.float64)
+    mx = np.array([d.get("aabb_max", (0, 0, 0)) for d in nodes], np.float64)
+    ax = np.array([d.get("axis", (0, 0, 0)) for d in nodes], np.float64)
+    to = np.array([d.get("theta_o", np.pi) for d in nodes], np.float64)
+    te = np.array([d.get("theta_e", np.pi) for d in nodes], np.float64)
+    out = np.zeros((n, LT_FLOATS), np.float32)
+    rc = lib().orc_light_tree_pack_nodes(C.c_uint32(n), em.ctypes.data_as(C.c_void_p), _p(pw, F64P),
+                                         lf.ctypes.data_as(C.c_void_p), rt.ctypes.data_as(C.c_void_p),
+                                         _p(mn, F64P), _p(mx, F64P), _p(ax, F64P), _p(to, F64P), _p(te, F64P),
+                                         _p(out, F32P))
+    if rc != 0:
+        raise OracleError(_err())
+    return out
+
+
+def _lt_setup():
+    L = lib()
+    L.orc_light_tree_importance.restype = C.c_float
+    L.orc_light_tree_importance.argtypes = [F32P, C.c_uint32, F32P, C.c_float]
+    L.orc_light_tree_sample.restype = C.c_int32
+    L.orc_light_tree_sample.argtypes = [F32P, C.c_uint32, F32P, C.c_float, C.c_uint32, F32P]
+    L.orc_light_tree_pdf.restype = C.c_float
+    L.orc_light_tree_pdf.argtypes = [F32P, C.c_uint32, F32P, C.c_float, C.c_int32]
+    return L
+
+
+def light_tree_importance(packed, node, p, floor_d2):
+    pk = np.ascontiguousarray(packed, np.float32)
+    pt = np.ascontiguousarray(p, np.float32)
+    return _lt_setup().orc_light_tree_importance(_p(pk, F32P), node, _p(pt, F32P), floor_d2)
+
+
+def light_tree_sample(packed, p, floor_d2, bucket):
+    """samplePackedLightTreeCPU with the 24-bit root draw `bucket`."""
+    pk = np.ascontiguousarray(packed, np.float32)
+    pt = np.ascontiguousarray(p, np.float32)
+    pdf = np.zeros(1, np.float32)
+    e = _lt_setup().orc_light_tree_sample(_p(pk, F32P), pk.shape[0], _p(pt, F32P), floor_d2, int(bucket),
+                                          _p(pdf, F32P))
+    return int(e), float(pdf[0])
+
+
+def light_tree_pdf(packed, p, floor_d2, emitter):
+    pk = np.ascontiguousarray(packed, np.float32)
+    pt = np.ascontiguousarray(p, np.float32)
+    return float(_lt_setup().orc_light_tree_pdf(_p(pk, F32P), pk.shape[0], _p(pt, F32P), floor_d2, emitter))
+
+
 def fp64_hit_params(scene: PackedScene, rays, hits):
     rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
     hits = np.ascontiguousarray(hits, dtype=HIT_DTYPE)

@@ -210,6 +210,29 @@ int orc_cwbvh_trace_closest(const orc_scene *s, const orc_cwbvh *c,
                             const orc_ray *rays, uint32_t n, orc_hit64 *out,
                             uint32_t threads);
 
+/* ---- Light tree: packages/shared-samplers/src/lightTree.ts --------------
+ * buildLightTree + packLightTreeForGPU (16 f32 per node; out capacity
+ * (2n-1)*16), packed importance, samplePackedLightTreeCPU (bucket = the
+ * 24-bit root draw) and packedLightTreePdfCPU.  has_cone may be NULL.      */
+int orc_light_tree_build(uint32_t n, const double *power, const double *centroid,
+                         const double *aabb_min, const double *aabb_max,
+                         const uint8_t *has_cone, const double *axis,
+                         const double *theta_o, const double *theta_e,
+                         float *out, uint32_t *out_nodes);
+int orc_light_tree_pack_nodes(uint32_t count, const int32_t *emitter,
+                              const double *power, const int32_t *left,
+                              const int32_t *right, const double *aabb_min,
+                              const double *aabb_max, const double *axis,
+                              const double *theta_o, const double *theta_e,
+                              float *out);
+float orc_light_tree_importance(const float *packed, uint32_t node,
+                                const float *p, float floor_d2);
+int32_t orc_light_tree_sample(const float *packed, uint32_t nodes,
+                              const float *p, float floor_d2, uint32_t bucket,
+                              float *pdf);
+float orc_light_tree_pdf(const float *packed, uint32_t nodes, const float *p,
+                         float floor_d2, int32_t emitter);
+
 /* ---- PCG32: packages/shared-samplers/src/wgsl/pcg.wgsl.ts:5-22 ---------- */
 uint32_t orc_pcg_init(uint32_t px, uint32_t py, uint32_t frame_seed);
 uint32_t orc_pcg_next(uint32_t *state);
