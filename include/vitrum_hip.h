@@ -123,7 +123,11 @@ typedef struct vt_options {
   uint32_t max_bounces;           /* 0 = default 8 */
   uint32_t max_samples_per_pixel; /* 0 = unlimited */
   uint32_t sampling;              /* 0 = pcg, 1 = sobol */
-  uint32_t reserved[5];
+  /* NEE light selection: 0 = the spatial light tree when the scene has two
+   * or more lights (lightTreeImportanceSampling, the reference's default on a
+   * full-tier device), 1 = uniform (lightTreeImportanceSampling: false). */
+  uint32_t light_selection;
+  uint32_t reserved[4];
 } vt_options;
 
 /* createPTEngine_WebGPU (packages/pt-webgpu/src/index.ts:2363).  The device
@@ -175,6 +179,11 @@ vt_status vt_frame_time(vt_engine *eng, float *ms);
 vt_status vt_capture_signatures(vt_engine *eng, uint64_t *sig, uint32_t width, uint32_t height);
 /* Mesh-area emitter count and the ray-origin bias of the bound scene. */
 vt_status vt_render_info(vt_engine *eng, uint32_t *emitter_count, float *ray_bias);
+/* The bound scene's NEE light tree as packLightTreeForGPU lays it out
+ * (packages/shared-samplers/src/lightTree.ts:611-737; 16 floats per node,
+ * leaves = mesh emitters in order, then the environment); *count = 0 when
+ * selection is uniform.  nodes may be NULL to query the count. */
+vt_status vt_light_tree(vt_engine *eng, float *nodes, uint32_t capacity, uint32_t *count);
 
 /* ---- traversal (the hot path) -------------------------------------------- */
 

@@ -389,6 +389,8 @@ typedef struct orc_render_desc {
   uint32_t max_bounces;
   float ray_bias;
   uint32_t sampling; /* 0 = PCG, 1 = Owen-scrambled Sobol */
+  const float *light_tree;        /* packed light tree (light_tree.c), or NULL */
+  uint32_t light_tree_nodes;      /* 0 = uniform light selection */
 } orc_render_desc;
 
 typedef struct {
@@ -544,10 +546,28 @@ static void render_path(const rctx *c, uint32_t px, uint32_t py, uint32_t frame_
     const mat_t *m = &c->mats[d->tri_material[s.prim]];
     d3 n = s.normal, wo = scl(dir, -1), hp = s.pos;
     if (!prev_mis && (s.front || m->two_sided)) L = add(L, mul(T, m->emit));
-    /* NEE: one light, uniform pick (mediumNee.wgsl.ts:14-40). */
+    /* NEE: one light picked by the light tree (one 24-bit draw when the
+     * root is interior), or uniformly when the tree is off or declines
+     * (sampleCanonicalDirectLight, mediumNee.wgsl.ts:14-38). */
     if (n_light > 0) {
-      uint32_t picked = rand_bounded(&rng, n_light);
-      double inv_sel = (double)n_light;
+      uint32_t picked = 0xFFFFFFFFu;
+      double inv_sel = 0;
+      if (d->light_tree_nodes) {
+        const float *t = d->light_tree;
+        uint32_t bucket = 0;
+        if ((int)t[2] >= 0 && (int)t[3] >= 0) bucket = pcg_next(&rng) >> 8;
+        const float p[3] = {(float)hp.x, (float)hp.y, (float)hp.z};
+        float pdf;
+        const int32_t e = orc_light_tree_sample(t, d->light_tree_nodes, p, 1e-3f, bucket, &pdf);
+        if (e >= 0 && pdf > 0 && (uint32_t)e < n_light) {
+          picked = (uint32_t)e;
+          inv_sel = 1.0 / (double)pdf;
+        }
+      }
+      if (picked == 0xFFFFFFFFu) {
+        picked = rand_bounded(&rng, n_light);
+        inv_sel = (double)n_light;
+      }
       sig = mixlog(sig, 0xA000000000ull | picked, ev);
       if (picked < d->emitter_count) {
         const emitter_t *e = &c->em[sorted_to_orig[picked]];

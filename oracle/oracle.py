@@ -490,6 +490,7 @@ class RenderDesc(C.Structure):
         ("env_cdf", F32P), ("env_pdf", F32P), ("inv_view_proj", C.c_float * 16),
         ("width", C.c_uint32), ("height", C.c_uint32), ("frame_seed", C.c_uint32),
         ("max_bounces", C.c_uint32), ("ray_bias", C.c_float), ("sampling", C.c_uint32),
+        ("light_tree", F32P), ("light_tree_nodes", C.c_uint32),
     ]
 
 
@@ -544,6 +545,102 @@ def build_emitters(pack, mat_table):
     if not recs:
         return np.zeros((0, 16), np.float32)
     return np.stack(recs)
+
+
+def _hypot3_js(a, b, c):
+    """Math.hypot (V8): max-scaled, compensated sum of squares."""
+    v = [abs(float(a)), abs(float(b)), abs(float(c))]
+    m = max(v)
+    if m == 0 or np.isinf(m):
+        return m
+    s = comp = 0.0
+    for x in v:
+        n = x / m
+        t = n * n - comp
+        nxt = s + t
+        comp = (nxt - s) - t
+        s = nxt
+    return np.sqrt(s) * m
+
+
+def _triangle_area_f32(a, b, c):
+    """measureTriangleAreaF32 (pt-webgpu/src/scene/areaEmitterGeometry.ts:
+    37-173): equilibrated f32 cross product, one rounding per operation;
+    0 when degenerate."""
+    f = np.float32
+    u = [f(b[k]) - f(a[k]) for k in range(3)]
+    v = [f(c[k]) - f(a[k]) for k in range(3)]
+    es = max(abs(x) for x in u + v)
+    if not (es > 0 and np.isfinite(es)):
+        return f(0)
+    su = [x / es for x in u]
+    sv = [x / es for x in v]
+    cx = su[1] * sv[2] - su[2] * sv[1]
+    cy = su[2] * sv[0] - su[0] * sv[2]
+    cz = su[0] * sv[1] - su[1] * sv[0]
+    cs = max(abs(cx), abs(cy), abs(cz))
+    if not (cs > 0 and np.isfinite(cs)):
+        return f(0)
+    dl = f(_hypot3_js(cx / cs, cy / cs, cz / cs))
+    if not (dl > 0):
+        return f(0)
+    area = ((f(0.5) * (cs * dl)) * es) * es
+    with np.errstate(divide="ignore", over="ignore"):
+        inv = f(1) / area
+    return area if (area > 0 and np.isfinite(area) and inv > 0 and np.isfinite(inv)) else f(0)
+
+
+def scene_light_tree(emitters, env_radiance=None, env_size=(64, 32)):
+    """The scene's NEE light tree (emitterPacking.ts:1005-1230 leaves, then
+    light_tree_build): one leaf per mesh emitter record in order -- power =
+    f32 luminance of its radiance x its f32 area, bounds of its vertices,
+    an outward hemisphere cone unless two-sided -- then the environment
+    (power = luminance x solid angle summed over the map, bounds = the union
+    of the emitters' bounds).  None below two lights."""
+    em = np.asarray(emitters, np.float32).reshape(-1, 16)
+    env_present = env_radiance is not None and any(v > 0 for v in env_radiance)
+    n = em.shape[0] + (1 if env_present else 0)
+    if n < 2:
+        return None
+    powers, cents, lo, hi, cones = [], [], [], [], []
+    for r in em:
+        a, b, c = r[0:3], r[3:6], r[6:9]
+        le = r[9:12]
+        lum = np.float32(max(0.0, 0.2126 * float(le[0]) + 0.7152 * float(le[1]) + 0.0722 * float(le[2])))
+        area = _triangle_area_f32(a, b, c)
+        powers.append(float(np.float32(float(lum) * float(area))))
+        A, B, Cc = (x.astype(np.float64) for x in (a, b, c))
+        cents.append([(A[k] + B[k] + Cc[k]) / 3 for k in range(3)])
+        lo.append(np.minimum(np.minimum(A, B), Cc))
+        hi.append(np.maximum(np.maximum(A, B), Cc))
+        if r[12] > 0.5:
+            cones.append(None)
+        else:
+            e1, e2 = B - A, Cc - A
+            nrm = [e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]]
+            ln = _hypot3_js(*nrm)
+            cones.append(([x / ln for x in nrm] if ln > 1e-12 else [0.0, 0.0, 0.0], 0.0, np.pi / 2))
+    if env_present:
+        W, H = env_size
+        lum = 0.2126 * float(np.float32(env_radiance[0])) + 0.7152 * float(np.float32(env_radiance[1])) + \
+            0.0722 * float(np.float32(env_radiance[2]))
+        total = 0.0
+        dphi = 2.0 * np.pi / W
+        for i in range(W * H):
+            y = i // W
+            om = dphi * (np.cos(y / H * np.pi) - np.cos((y + 1) / H * np.pi))
+            total += max(0.0, lum * om)
+        pw = float(np.float32(total))
+        powers.append((pw if pw > 0 else 2.0 ** -149) if total > 0 else 0.0)
+        if em.shape[0]:
+            ulo, uhi = np.min(np.array(lo), axis=0), np.max(np.array(hi), axis=0)
+        else:
+            ulo = uhi = np.zeros(3)
+        lo.append(ulo)
+        hi.append(uhi)
+        cents.append([(ulo[k] + uhi[k]) / 2 for k in range(3)])
+        cones.append(None)
+    return light_tree_build(powers, cents, lo, hi, cones)
 
 
 def represented_cdf(weights, bucket_bits=24, counts=False):
@@ -611,7 +708,7 @@ def build_constant_env(radiance, width=64, height=32):
 
 def render(pack, materials, camera_inv_vp, width, height, *, frame_seed, frame_index, max_bounces,
            ray_bias, env_radiance=None, env_size=(64, 32), tile=None, threads=None, events=False,
-           sampling=0):
+           sampling=0, light_tree=True):
     """One frame (1 spp) of the binary64 integrator mirror.  Returns
     (radiance[h, w, 3] float64, signature[h, w] uint64), plus with `events`
     the decisions mixed into each signature ([h, w, 40] uint64: count, then
@@ -647,6 +744,11 @@ def render(pack, materials, camera_inv_vp, width, height, *, frame_seed, frame_i
     d.max_bounces = max_bounces
     d.ray_bias = ray_bias
     d.sampling = sampling
+    tree = scene_light_tree(em, env_radiance, env_size) if light_tree else None
+    if tree is not None:
+        tree = np.ascontiguousarray(tree, np.float32)
+        d.light_tree = tree.ctypes.data_as(F32P)
+        d.light_tree_nodes = tree.shape[0]
     x0, y0, w, h = tile or (0, 0, width, height)
     out = np.zeros((h, w, 3), np.float64)
     sig = np.zeros((h, w), np.uint64)

@@ -11,16 +11,17 @@ def materials_of(scene):
     return [p.get("material", {}) for p in scene["primitives"]]
 
 
-def cpu_frames(scene, res, frames, bias, max_bounces=8, sampling=0):
+def cpu_frames(scene, res, frames, bias, max_bounces=8, sampling=0, light_tree=True, first_frame=0):
     prims = [dict(p, material_id=i) for i, p in enumerate(scene["primitives"])]
     pack = O.pack_scene(prims)
     ivp = W.inv_view_proj(scene["camera"], res, res)
     sky = scene.get("sky", [0, 0, 0])
     out, sigs = [], []
-    for f in range(frames):
+    for f in range(first_frame, first_frame + frames):
         img, sig = O.render(pack, materials_of(scene), ivp, res, res, frame_seed=SEED, frame_index=f,
                             max_bounces=max_bounces, ray_bias=bias,
-                            env_radiance=sky if any(sky) else None, sampling=sampling)
+                            env_radiance=sky if any(sky) else None, sampling=sampling,
+                            light_tree=light_tree)
         out.append(img)
         sigs.append(sig)
     return np.stack(out), np.stack(sigs)

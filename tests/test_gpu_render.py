@@ -28,12 +28,12 @@ def gpu_frames(engine, scene, res, frames, max_bounces=8):
     return engine.capture(res, res)[..., :3].astype(np.float64), engine.capture_signatures(res, res)
 
 
-def check_single_frame(engine, scene, res, sampling=0):
+def check_single_frame(engine, scene, res, sampling=0, light_tree=True):
     gpu, gsig = gpu_frames(engine, scene, res, 1)
     info = engine.render_info()
     bias, _ = W.ray_bias(scene["scene_radius"])
     assert info["ray_bias"] == pytest.approx(float(bias), rel=1e-6)
-    cpu, csig = cpu_frames(scene, res, 1, info["ray_bias"], sampling=sampling)
+    cpu, csig = cpu_frames(scene, res, 1, info["ray_bias"], sampling=sampling, light_tree=light_tree)
     cpu, csig = cpu[0], csig[0]
     match = gsig == csig
     rate = match.mean()
@@ -73,6 +73,40 @@ def test_sobol_sampler_paths_match_fp64_integrator(sobol_engine):
     assert gpu.mean() > 0.05
     sc = W.spheres_scene(instances=48, lights=4)
     check_single_frame(sobol_engine, sc, 48, sampling=1)
+
+
+def test_light_tree_bytes_equal_oracle(engine):
+    """The product's packed light tree (built on the host at setScene) equals
+    the oracle's buildLightTree + packLightTreeForGPU over the same lights,
+    bit for bit: Cornell (quad light, no sky: two triangle leaves) and the
+    sphere scene (light-sphere triangles plus the sky leaf)."""
+    from oracle import oracle as O
+    from render_cases import materials_of
+    for sc in (W.cornell_scene(), W.spheres_scene(instances=16, lights=4)):
+        engine.set_scene(sc)
+        got = engine.light_tree()
+        pack = O.pack_scene([dict(p, material_id=i) for i, p in enumerate(sc["primitives"])])
+        em = O.build_emitters(pack, O.material_table(materials_of(sc)))
+        sky = sc.get("sky", [0, 0, 0])
+        want = O.scene_light_tree(em, sky if any(sky) else None)
+        assert want is not None and got.shape == want.shape
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+@pytest.fixture(scope="module")
+def uniform_engine():
+    from vitrum_amd import native
+    eng = native.Engine(0, light_selection=1)
+    yield eng
+    eng.close()
+
+
+def test_uniform_light_selection_paths_match(uniform_engine):
+    """lightTreeImportanceSampling: false -- uniform light selection: the
+    same path-matching contract against the integrator in uniform mode."""
+    sc = W.spheres_scene(instances=48, lights=4)
+    check_single_frame(uniform_engine, sc, 48, light_tree=False)
+    assert uniform_engine.light_tree().shape[0] == 0
 
 
 def test_cornell_multiframe_mean_within_statistical_bound(engine):

@@ -167,6 +167,8 @@ vt_status vt_create(int32_t device_ordinal, const vt_options* opts, vt_engine** 
     return set_err(nullptr, VT_ERR_RANGE, "device ordinal out of range");
   if (opts && opts->sampling > 1)
     return set_err(nullptr, VT_ERR_RANGE, "sampling must be 0 (pcg) or 1 (sobol)");
+  if (opts && opts->light_selection > 1)
+    return set_err(nullptr, VT_ERR_RANGE, "light_selection must be 0 (light tree) or 1 (uniform)");
   std::unique_ptr<vt_engine> e(new vt_engine());
   e->device = device_ordinal;
   if (opts) e->opts = *opts;

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "engine_internal.h"
+#include "light_tree.h"
 #include "render.h"
 #include "sobol.h"
 
@@ -87,7 +88,9 @@ std::vector<float> represented_cdf(const std::vector<double>& w) {
 
 struct RenderState {
   // Per-scene shading data.
-  DeviceBuffer materials, emitters, instances, env_cdf, env_pdf, tri_source;
+  DeviceBuffer materials, emitters, instances, env_cdf, env_pdf, tri_source, light_tree;
+  uint32_t light_tree_nodes = 0;  // 0: uniform light selection
+  std::vector<float> light_tree_host;
   DeviceBuffer sobol_dirs;  // 512 x 16 Sobol direction components (built once)
   uint32_t emitter_count = 0;
   uint32_t env_present = 0, env_w = 64, env_h = 32;
@@ -113,7 +116,7 @@ struct RenderState {
     width = height = paths = 0;
   }
   void release_scene() {
-    for (DeviceBuffer* b : {&materials, &emitters, &instances, &env_cdf, &env_pdf, &tri_source, &sobol_dirs})
+    for (DeviceBuffer* b : {&materials, &emitters, &instances, &env_cdf, &env_pdf, &tri_source, &sobol_dirs, &light_tree})
       b->release();
   }
 };
@@ -225,6 +228,51 @@ vt_status engine_on_scene_changed(vt_engine* e) {
   }
   std::vector<float> cdf = represented_cdf(weights), pdf((size_t)W * H);
   for (uint32_t i = 0; i < W * H; ++i) pdf[i] = (float)(((double)cdf[i + 1] - (double)cdf[i]) / omega[i]);
+  // Light tree over the light slots -- mesh emitters in order, then the
+  // environment -- built when there are two or more lights and the engine
+  // uses tree selection (the reference's default; uploadSceneBuffers.ts:
+  // 1850-1862, emitterPacking.ts:1005-1230).
+  std::vector<float> tree;
+  const uint32_t n_lights = R.emitter_count + (R.env_present ? 1u : 0u);
+  if (e->opts.light_selection == 0 && n_lights >= 2) {
+    std::vector<vtl::Leaf> leaves;
+    leaves.reserve(n_lights);
+    double ulo[3] = {INFINITY, INFINITY, INFINITY}, uhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = 0; i < R.emitter_count; ++i) {
+      const float* q = &em[(size_t)i * vtr::kEmitFloats];
+      leaves.push_back(vtl::triangle_leaf(q, q + 3, q + 6, q + 9, q[12] > 0.5f));
+      for (int k = 0; k < 3; ++k) {
+        ulo[k] = std::min(ulo[k], leaves.back().lo[k]);
+        uhi[k] = std::max(uhi[k], leaves.back().hi[k]);
+      }
+    }
+    if (R.env_present) {
+      // environment leaf: luminance x solid angle over the map, rounded to
+      // f32 (environmentPacking.ts:183-202), bounds = the union of the
+      // positional lights' bounds (the origin when there are none)
+      double total = 0;
+      for (uint32_t i = 0; i < W * H; ++i) total += weights[i];
+      vtl::Leaf L{};
+      const float pw = (float)total;
+      L.power = total > 0 ? (pw > 0 ? pw : 0x1p-149f) : 0.0;
+      for (int k = 0; k < 3; ++k) {
+        const bool ok = std::isfinite(ulo[k]);
+        L.lo[k] = ok ? ulo[k] : 0.0;
+        L.hi[k] = ok ? uhi[k] : 0.0;
+        L.centroid[k] = (L.lo[k] + L.hi[k]) / 2;
+      }
+      L.oriented = false;
+      leaves.push_back(L);
+    }
+    try {
+      tree = vtl::build_packed(leaves);
+    } catch (const std::exception& ex) {
+      return rerr(e, VT_ERR_RANGE, ex.what());
+    }
+  }
+  R.light_tree_nodes = (uint32_t)(tree.size() / vtl::kNodeFloats);
+  R.light_tree_host = tree;
+  if (tree.empty()) tree.assign(vtl::kNodeFloats, 0.0f);
   // Ray-origin bias from the scene bounds (or the caller's radius).
   double center[3], radius = e->scene_radius;
   for (int k = 0; k < 3; ++k) center[k] = 0.5 * (P.world_min[k] + P.world_max[k]);
@@ -250,6 +298,7 @@ vt_status engine_on_scene_changed(vt_engine* e) {
   s = s ? s : engine_upload(e, R.env_cdf, cdf.data(), cdf.size() * 4);
   s = s ? s : engine_upload(e, R.env_pdf, pdf.data(), pdf.size() * 4);
   s = s ? s : engine_upload(e, R.tri_source, P.tri_source.data(), P.tri_source.size() * 4);
+  s = s ? s : engine_upload(e, R.light_tree, tree.data(), tree.size() * 4);
   if (!s && !R.sobol_dirs.ptr) {
     std::vector<uint16_t> dirs(vts::kDims * vts::kBits);
     vts::build_directions(dirs.data());
@@ -384,6 +433,8 @@ vt_status vt_render_frame(vt_engine* e, const vt_frame* fr) {
   sc.env_w = R.env_w;
   sc.env_h = R.env_h;
   for (int k = 0; k < 3; ++k) sc.env_radiance[k] = R.env_radiance[k];
+  sc.light_tree = (const float*)R.light_tree.ptr;
+  sc.light_tree_nodes = R.light_tree_nodes;
   vtr::PathState ps{};
   ps.rng = (uint32_t*)R.rng.ptr;
   ps.rng_dim = (uint32_t*)R.rng_dim.ptr;
@@ -526,6 +577,18 @@ vt_status vt_nrc_render_config(vt_engine* e, int32_t enabled, uint32_t train_pat
   if (enabled) R.nrc_train_paths = train_paths;
   R.width = R.height = 0;  // re-size path state on the next frame
   e->samples_accumulated = 0;
+  return VT_OK;
+}
+
+vt_status vt_light_tree(vt_engine* e, float* nodes, uint32_t capacity, uint32_t* count) {
+  if (!e || !count) return VT_ERR_INVALID_ARGUMENT;
+  if (!e->render) return rerr(e, VT_ERR_STATE, "no scene is set");
+  const RenderState& R = *e->render;
+  *count = R.light_tree_nodes;
+  if (nodes && capacity) {
+    if (capacity < R.light_tree_nodes) return rerr(e, VT_ERR_RANGE, "capacity below the light tree's node count");
+    std::memcpy(nodes, R.light_tree_host.data(), (size_t)R.light_tree_nodes * vtl::kNodeFloats * sizeof(float));
+  }
   return VT_OK;
 }
 

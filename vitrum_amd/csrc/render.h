@@ -27,6 +27,9 @@ struct ShadeScene {
   uint32_t emitter_count;
   uint32_t env_present, env_w, env_h;
   float env_radiance[3];
+  // Light tree over the light slots (16 floats per node); 0 nodes = uniform.
+  const float* light_tree;
+  uint32_t light_tree_nodes;
 };
 
 constexpr uint32_t kTrainSlots = 4;   // recorded vertices per self-training path

@@ -128,6 +128,121 @@ __device__ uint32_t rand_bounded(Rng& s, uint32_t bound) {
   }
 }
 
+// ---- light tree (packages/shared-samplers/src/wgsl/lightTree.wgsl.ts) -----
+// Represented importance of a packed node for shading point p in binary32,
+// one rounding per operation; log2/exp2 are evaluated in binary64 and
+// rounded once.
+constexpr float kF32MinNormal = 1.175494351e-38f;
+constexpr float kF32Max = 3.402823466e38f;
+__device__ __forceinline__ float lt_log2(float x) { return (float)log2((double)x); }
+__device__ __forceinline__ float lt_exp2(float x) { return (float)exp2((double)x); }
+__device__ __forceinline__ float lt_dot(F3 a, F3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ float max3abs(F3 v) { return fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fabsf(v.z)); }
+
+__device__ float lt_dist2(F3 p, F3 lo, F3 hi) {
+  const F3 h = f3(fmaxf(fmaxf(0.5f * lo.x - 0.5f * p.x, 0.0f), 0.5f * p.x - 0.5f * hi.x),
+                  fmaxf(fmaxf(0.5f * lo.y - 0.5f * p.y, 0.0f), 0.5f * p.y - 0.5f * hi.y),
+                  fmaxf(fmaxf(0.5f * lo.z - 0.5f * p.z, 0.0f), 0.5f * p.z - 0.5f * hi.z));
+  const float hs = max3abs(h);
+  if (!(hs >= kF32MinNormal)) return 0.0f;
+  if (hs > 0.5f * kF32Max) return kF32Max;
+  const F3 sc = f3(h.x / hs, h.y / hs, h.z / hs);
+  const float sq = lt_dot(sc, sc);
+  const float scale = 2.0f * hs;
+  if (scale >= __builtin_sqrtf(kF32Max / sq)) return kF32Max;
+  return fminf((scale * scale) * sq, kF32Max);
+}
+
+__device__ float lt_cone(F3 axis, float cos_o, float cos_oe, F3 p, F3 lo, F3 hi) {
+  const float as = max3abs(axis);
+  if (!(as > 0.0f) || as > kF32Max) return 1.0f;
+  const F3 sa = f3(axis.x / as, axis.y / as, axis.z / as);
+  const float al = __builtin_sqrtf(lt_dot(sa, sa));
+  const F3 dv = f3((0.5f * p.x - 0.25f * lo.x) - 0.25f * hi.x, (0.5f * p.y - 0.25f * lo.y) - 0.25f * hi.y,
+                   (0.5f * p.z - 0.25f * lo.z) - 0.25f * hi.z);
+  const F3 rv = f3(0.25f * hi.x - 0.25f * lo.x, 0.25f * hi.y - 0.25f * lo.y, 0.25f * hi.z - 0.25f * lo.z);
+  const float ds = max3abs(dv);
+  if (!(ds >= kF32MinNormal) || ds > kF32Max) return 1.0f;
+  const F3 sd = f3(dv.x / ds, dv.y / ds, dv.z / ds);
+  const float sdl = __builtin_sqrtf(lt_dot(sd, sd));
+  const float rs = max3abs(rv);
+  float r_over_d = 0.0f;
+  if (rs >= kF32MinNormal) {
+    const F3 sr = f3(rv.x / rs, rv.y / rs, rv.z / rs);
+    const float srl = __builtin_sqrtf(lt_dot(sr, sr));
+    const float lr = ((lt_log2(rs) + lt_log2(srl)) - lt_log2(ds)) - lt_log2(sdl);
+    if (lr >= 0.0f) return 1.0f;
+    r_over_d = lt_exp2(fmaxf(lr, -126.0f));
+  }
+  const F3 d = f3(sd.x / sdl, sd.y / sdl, sd.z / sdl);
+  const F3 a = f3(sa.x / al, sa.y / al, sa.z / al);
+  const float cos_t = clampf(lt_dot(a, d), -1.0f, 1.0f);
+  const float sin_u = clampf(r_over_d, 0.0f, 1.0f);
+  const float cos_u = __builtin_sqrtf(fmaxf(0.0f, 1.0f - sin_u * sin_u));
+  float cos_adj = 1.0f;
+  if (cos_t < cos_u) {
+    const float sin_t = __builtin_sqrtf(fmaxf(0.0f, 1.0f - cos_t * cos_t));
+    cos_adj = clampf(cos_t * cos_u + sin_t * sin_u, -1.0f, 1.0f);
+  }
+  if (cos_adj < cos_oe) return 0.0f;
+  if (cos_adj >= cos_o) return 1.0f;
+  const float span = cos_o - cos_oe;
+  if (!(span > 0.0f)) return 1.0f;
+  return clampf((cos_adj - cos_oe) / span, kF32MinNormal, 1.0f);
+}
+
+__device__ float lt_importance(const float* node, F3 p) {
+  const float power = node[1];
+  if (!(power > 0.0f)) return 0.0f;
+  const F3 lo = f3(node[4], node[5], node[6]), hi = f3(node[7], node[8], node[9]);
+  const float d2 = fmaxf(fmaxf(lt_dist2(p, lo, hi), 1e-3f), kF32MinNormal);
+  const float c = lt_cone(f3(node[10], node[11], node[12]), node[13], node[14], p, lo, hi);
+  if (!(c > 0.0f)) return 0.0f;
+  const float li = (lt_log2(power) - lt_log2(d2)) + lt_log2(fmaxf(c, kF32MinNormal));
+  if (li >= 127.99999f) return kF32Max;
+  if (li <= -126.0f) return kF32MinNormal;
+  return clampf(lt_exp2(li), kF32MinNormal, kF32Max);
+}
+
+// sampleLightTree: one 24-bit root draw apportioned down the tree (every leaf
+// keeps one bucket, the spare buckets follow the represented importances).
+// Returns the light slot, or -1; *pdf = leaf buckets / 2^24.
+__device__ int lt_sample(const float* tree, uint32_t nodes, F3 p, Rng& rng, float* pdf) {
+  *pdf = 0.0f;
+  uint32_t node = 0, cur = 0x01000000u, local = 0;
+  if ((int)tree[2] >= 0 && (int)tree[3] >= 0) local = min(0x00FFFFFFu, rng_next(rng) >> 8);
+  for (uint32_t guard = 0; guard < nodes + 1u; ++guard) {
+    const float* b = tree + 16 * (size_t)node;
+    const int l = (int)b[2], r = (int)b[3];
+    if (l < 0 || r < 0) {
+      *pdf = (float)cur / 16777216.0f;
+      return (int)b[0];
+    }
+    const float* lb = tree + 16 * (size_t)l;
+    const float* rb = tree + 16 * (size_t)r;
+    const uint32_t ll = (uint32_t)lb[15], rl = (uint32_t)rb[15];
+    const float li = lt_importance(lb, p), ri = lt_importance(rb, p);
+    const uint32_t remaining = cur - (ll + rl);
+    float p_left = 0.5f;
+    const float s = fmaxf(li, ri);
+    if (s > 0.0f) {
+      const float lf = li / s, rf = ri / s;
+      p_left = lf / (lf + rf);
+    }
+    const float rounded = floorf((float)remaining * p_left + 0.5f);
+    const uint32_t left = ll + min(remaining, (uint32_t)fmaxf(0.0f, rounded));
+    if (local < left) {
+      cur = left;
+      node = (uint32_t)l;
+    } else {
+      local -= left;
+      cur -= left;
+      node = (uint32_t)r;
+    }
+  }
+  return -1;
+}
+
 __device__ __forceinline__ uint64_t mix64(uint64_t h, uint64_t v) {
   h ^= v + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
   return h * 0xBF58476D1CE4E5B9ull;
@@ -591,12 +706,25 @@ __global__ __launch_bounds__(256) void shade_kernel(FrameParams f, ShadeScene sc
           }
         }
         if (!query) {
-        // 3. NEE over one uniformly selected light (mediumNee.wgsl.ts:14-40;
+        // 3. NEE over one light picked by the light tree, or uniformly when
+        //    the tree is off or declines (mediumNee.wgsl.ts:14-38;
         //    kernel.wgsl.ts:1496-1690).
         const uint32_t n_light = sc.emitter_count + (sc.env_present ? 1u : 0u);
         if (n_light > 0u) {
-          const uint32_t picked = rand_bounded(rng, n_light);
-          const float inv_sel = (float)n_light;
+          uint32_t picked = 0xFFFFFFFFu;
+          float inv_sel = 0.0f;
+          if (sc.light_tree_nodes) {
+            float pdf;
+            const int e = lt_sample(sc.light_tree, sc.light_tree_nodes, hp, rng, &pdf);
+            if (e >= 0 && pdf > 0.0f && (uint32_t)e < n_light) {
+              picked = (uint32_t)e;
+              inv_sel = 1.0f / pdf;
+            }
+          }
+          if (picked == 0xFFFFFFFFu) {
+            picked = rand_bounded(rng, n_light);
+            inv_sel = (float)n_light;
+          }
           sig = mix64(sig, 0xA000000000ull | picked);
           if (picked < sc.emitter_count) {
             const float* e = sc.emitters + (size_t)picked * kEmitFloats;

@@ -369,7 +369,7 @@ function nrcSettings(enabled, config) {
 async function createPTEngine_HIP(opts) {
   const o = Object.assign({}, opts || {});
   const known = new Set(['device', 'maxBounces', 'maxSamplesPerPixel', 'sampling', 'sceneRadius', 'debug',
-    'onWarning', 'denoiser', 'extensions', 'nrcEnabled', 'nrcConfig']);
+    'onWarning', 'denoiser', 'extensions', 'nrcEnabled', 'nrcConfig', 'lightTreeImportanceSampling']);
   for (const k of Object.keys(o)) {
     if (!known.has(k)) throw new TypeError(`pt-hip: unknown option '${k}'`);
   }
@@ -380,6 +380,7 @@ async function createPTEngine_HIP(opts) {
     maxBounces: o.maxBounces || 0,
     maxSamplesPerPixel: Number.isFinite(o.maxSamplesPerPixel) ? o.maxSamplesPerPixel : 0,
     sampling: o.sampling === 'sobol' ? 1 : 0,
+    lightSelection: o.lightTreeImportanceSampling === false ? 1 : 0,
   });
   const engine = new HipPathTracerEngine(native, handle, o);
   if (typeof o.onWarning === 'function') engine.onWarning(o.onWarning);

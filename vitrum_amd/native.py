@@ -30,7 +30,7 @@ EXPORTED = [
     "vt_memcpy_h2d", "vt_memcpy_d2h", "vt_memcpy_d2d", "vt_synchronize", "vt_get_stream", "vt_stats",
     "vt_build_blas", "vt_build_tlas", "vt_pack_scene", "vt_pack_field_words",
     "vt_pack_copy_field", "vt_pack_free", "vt_version", "vt_render_frame", "vt_capture",
-    "vt_reset", "vt_frame_time", "vt_capture_signatures", "vt_render_info", "vt_invert_mat4",
+    "vt_reset", "vt_frame_time", "vt_capture_signatures", "vt_render_info", "vt_light_tree", "vt_invert_mat4",
     "vt_nrc_param_count", "vt_nrc_reset", "vt_nrc_set_weights", "vt_nrc_get_weights", "vt_nrc_infer",
     "vt_nrc_infer_device", "vt_nrc_train_step", "vt_nrc_train_step_device", "vt_nrc_last_loss",
     "vt_nrc_gradient", "vt_nrc_render_config", "vt_nrc_timing", "vt_batch_create", "vt_batch_destroy",
@@ -83,7 +83,7 @@ class SceneDesc(C.Structure):
 
 class Options(C.Structure):
     _fields_ = [("max_bounces", C.c_uint32), ("max_samples_per_pixel", C.c_uint32),
-                ("sampling", C.c_uint32), ("reserved", C.c_uint32 * 5)]
+                ("sampling", C.c_uint32), ("light_selection", C.c_uint32), ("reserved", C.c_uint32 * 4)]
 
 
 class Frame(C.Structure):
@@ -239,9 +239,11 @@ def _material(m) -> Material:
 class Engine:
     """Thin owner of a vt_engine (one per GPU device)."""
 
-    def __init__(self, device: int = 0, max_bounces: int = 8, sampling: int = 0):
+    def __init__(self, device: int = 0, max_bounces: int = 8, sampling: int = 0, light_selection: int = 0):
+        """sampling: 0 PCG, 1 Owen-scrambled Sobol; light_selection: 0 light
+        tree (two or more lights), 1 uniform."""
         self._h = C.c_void_p()
-        opts = Options(max_bounces, 0, sampling)
+        opts = Options(max_bounces, 0, sampling, light_selection)
         check(lib().vt_create(C.c_int32(device), C.byref(opts), C.byref(self._h)))
 
     def close(self):
@@ -399,6 +401,15 @@ class Engine:
         bias = C.c_float()
         check(lib().vt_render_info(self._h, C.byref(n), C.byref(bias)), self._h)
         return {"emitter_count": n.value, "ray_bias": bias.value}
+
+    def light_tree(self):
+        """The bound scene's packed NEE light tree, (nodes, 16) float32."""
+        n = C.c_uint32()
+        check(lib().vt_light_tree(self._h, None, 0, C.byref(n)), self._h)
+        out = np.zeros((n.value, 16), np.float32)
+        if n.value:
+            check(lib().vt_light_tree(self._h, out.ctypes.data_as(C.POINTER(C.c_float)), n, C.byref(n)), self._h)
+        return out
 
     # device-resident plumbing
     def alloc(self, nbytes):
