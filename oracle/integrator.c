@@ -424,13 +424,15 @@ typedef struct {
   uint32_t inst, prim;
   d3 pos, normal;  /* shading normal oriented to the front face */
   int front;
+  float tpos[3];   /* the traversal's f32 world hit position */
 } surf_t;
 
 /* Closest hit and binary64 surface at the hit. */
 static int closest(const rctx *c, d3 o, d3 d, surf_t *s) {
   orc_ray r = mk_ray(o, d, c->tmin, F32_MAX);
   orc_hit h;
-  orc_trace_closest(c->d->scene, &r, 1, &h, NULL, 1);
+  float wpos[3];
+  orc_trace_closest_pos(c->d->scene, &r, 1, &h, wpos, 1);
   if (h.inst == 0xFFFFFFFFu) {
     s->hit = 0;
     return 0;
@@ -465,6 +467,9 @@ static int closest(const rctx *c, d3 o, d3 d, surf_t *s) {
   s->hit = 1;
   s->inst = h.inst;
   s->prim = h.prim;
+  s->tpos[0] = wpos[0];
+  s->tpos[1] = wpos[1];
+  s->tpos[2] = wpos[2];
   s->pos = add(P[0], add(scl(e1, u), scl(e2, v)));
   d3 nl = safe_normalize(add(add(scl(N[0], 1 - u - v), scl(N[1], u)), scl(N[2], v)));
   /* transformNormalFromWorldToLocalCols: dot with each W2L column */
@@ -556,7 +561,9 @@ static void render_path(const rctx *c, uint32_t px, uint32_t py, uint32_t frame_
         const float *t = d->light_tree;
         uint32_t bucket = 0;
         if ((int)t[2] >= 0 && (int)t[3] >= 0) bucket = pcg_next(&rng) >> 8;
-        const float p[3] = {(float)hp.x, (float)hp.y, (float)hp.z};
+        /* the reference evaluates the tree at the traversal's f32 hit
+         * position (hit.position of traceClosest) */
+        const float *p = s.tpos;
         float pdf;
         const int32_t e = orc_light_tree_sample(t, d->light_tree_nodes, p, 1e-3f, bucket, &pdf);
         if (e >= 0 && pdf > 0 && (uint32_t)e < n_light) {

@@ -6,6 +6,7 @@ cpu_baseline leg of bench.py, never by the product package.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -625,10 +626,11 @@ def scene_light_tree(emitters, env_radiance=None, env_size=(64, 32)):
         lum = 0.2126 * float(np.float32(env_radiance[0])) + 0.7152 * float(np.float32(env_radiance[1])) + \
             0.0722 * float(np.float32(env_radiance[2]))
         total = 0.0
-        dphi = 2.0 * np.pi / W
+        dphi = 2.0 * math.pi / W
         for i in range(W * H):
             y = i // W
-            om = dphi * (np.cos(y / H * np.pi) - np.cos((y + 1) / H * np.pi))
+            # libm cosine (the host code's), not numpy's vectorised one
+            om = dphi * (math.cos(y / H * math.pi) - math.cos((y + 1) / H * math.pi))
             total += max(0.0, lum * om)
         pw = float(np.float32(total))
         powers.append((pw if pw > 0 else 2.0 ** -149) if total > 0 else 0.0)
@@ -697,10 +699,13 @@ def build_constant_env(radiance, width=64, height=32):
     (pt-webgpu/src/scene/environmentPacking.ts:340-420): weights
     luminance x texel solid angle, represented 24-bit CDF, texel pdf =
     effective pmf / texel solid angle."""
-    lum = 0.2126 * radiance[0] + 0.7152 * radiance[1] + 0.0722 * radiance[2]
-    dphi = 2 * np.pi / width
-    y = np.repeat(np.arange(height), width)
-    omega = dphi * (np.cos(y / height * np.pi) - np.cos((y + 1) / height * np.pi))
+    r = [float(np.float32(v)) for v in radiance]  # the engine stores f32 radiance
+    lum = 0.2126 * r[0] + 0.7152 * r[1] + 0.0722 * r[2]
+    dphi = 2 * math.pi / width
+    # libm cosines, as the host code's
+    rows = np.array([dphi * (math.cos(j / height * math.pi) - math.cos((j + 1) / height * math.pi))
+                     for j in range(height)])
+    omega = np.repeat(rows, width)
     cdf = represented_cdf(np.maximum(0.0, lum * omega))
     pdf = ((cdf[1:].astype(np.float64) - cdf[:-1].astype(np.float64)) / omega).astype(np.float32)
     return cdf, pdf

@@ -345,6 +345,7 @@ typedef struct {
   float dist;
   uint32_t tri, inst;
   float u, v;
+  v3 pos; /* world hit position, L2W * local hit point (f32) */
 } world_hit;
 
 static void tlas_instance_closest(const orc_scene *s, uint32_t inst, const ray3 *r, float tmin,
@@ -378,6 +379,7 @@ static void tlas_instance_closest(const orc_scene *s, uint32_t inst, const ray3 
       hit->inst = inst;
       hit->u = lh.u;
       hit->v = lh.v;
+      hit->pos = wpos;
     }
   }
 }
@@ -488,6 +490,7 @@ typedef struct {
   orc_hit *hits;
   uint32_t *counts;
   uint32_t *any;
+  float *pos;
 } job_t;
 
 static void *closest_job(void *arg) {
@@ -510,6 +513,12 @@ static void *closest_job(void *arg) {
       o->inst = 0xFFFFFFFFu;
       o->t = INFINITY;
       o->u = o->v = 0;
+    }
+    if (j->pos) {
+      float *pp = j->pos + 3 * (size_t)i;
+      pp[0] = h.did_hit ? h.pos.x : 0.0f;
+      pp[1] = h.did_hit ? h.pos.y : 0.0f;
+      pp[2] = h.did_hit ? h.pos.z : 0.0f;
     }
     if (j->counts) {
       uint32_t *cc = j->counts + 4 * (size_t)i;
@@ -561,13 +570,19 @@ static int run_jobs(void *(*fn)(void *), job_t proto, uint32_t n, uint32_t threa
 
 int orc_trace_closest(const orc_scene *s, const orc_ray *rays, uint32_t n, orc_hit *out,
                       uint32_t *counts, uint32_t threads) {
-  job_t proto = {s, rays, 0, 0, out, counts, NULL};
+  job_t proto = {s, rays, 0, 0, out, counts, NULL, NULL};
+  return run_jobs(closest_job, proto, n, threads);
+}
+
+int orc_trace_closest_pos(const orc_scene *s, const orc_ray *rays, uint32_t n, orc_hit *out, float *pos,
+                          uint32_t threads) {
+  job_t proto = {s, rays, 0, 0, out, NULL, NULL, pos};
   return run_jobs(closest_job, proto, n, threads);
 }
 
 int orc_trace_any(const orc_scene *s, const orc_ray *rays, uint32_t n, uint32_t *out,
                   uint32_t threads) {
-  job_t proto = {s, rays, 0, 0, NULL, NULL, out};
+  job_t proto = {s, rays, 0, 0, NULL, NULL, out, NULL};
   return run_jobs(any_job, proto, n, threads);
 }
 

@@ -160,6 +160,10 @@ void orc_watertight(const float *origin, const float *direction,
  * TLAS nodes slab-tested, instances entered.                                */
 int orc_trace_closest(const orc_scene *s, const orc_ray *rays, uint32_t n,
                       orc_hit *out, uint32_t *counts, uint32_t threads);
+/* Closest hit plus the world hit position (3 f32 per ray, the traversal's
+ * L2W * local hit point, tlasSceneHitTraversal.wgsl.ts:145-152). */
+int orc_trace_closest_pos(const orc_scene *s, const orc_ray *rays, uint32_t n,
+                          orc_hit *out, float *pos, uint32_t threads);
 /* Any hit: traceTlasAny (tlasSceneHitTraversal.wgsl.ts:227). */
 int orc_trace_any(const orc_scene *s, const orc_ray *rays, uint32_t n,
                   uint32_t *out, uint32_t threads);

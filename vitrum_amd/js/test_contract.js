@@ -83,6 +83,14 @@ assert.throws(() => pt.sceneBlob({ primitives: [], environment: { kind: 'hdri', 
       frameIndex: i, frameSeed: 7, quality: { bounces: 4 } });
   }
   const nc = await nrc.captureFrame();
+  // uniform light selection (lightTreeImportanceSampling: false) renders too
+  const uni = await pt.createPTEngine_HIP({ device: 0, lightTreeImportanceSampling: false });
+  uni.setScene({ primitives: [quad], emitters: [], environment: { kind: 'hdri', hdri: { constant: [1, 1, 1] } } });
+  uni.renderFrame({ viewMatrix: view, projMatrix: proj, viewport: { width: 16, height: 16, devicePixelRatio: 1 },
+    frameIndex: 0, frameSeed: 7, quality: { bounces: 4 } });
+  const uc = await uni.captureFrame();
+  assert.ok(uc.rgba.every((x) => Number.isFinite(x) && x >= 0), 'uniform-selection output finite');
+  uni.dispose();
   assert.ok(nc.rgba.every((x) => Number.isFinite(x) && x >= 0), 'cache output finite and non-negative');
   nrc.setNeuralCache(false);
   nrc.dispose();
