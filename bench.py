@@ -91,7 +91,7 @@ NRC_TRAIN_FLOPS = NRC_FWD_FLOPS + 2 * (3 * 64 + 3 * 64 * 64) + 2 * (4 * 64 * 64 
 BF16_PEAK_TFLOPS = 2500.0
 # Identifies the traversal kernel configuration the committed traffic
 # measurement belongs to (child order, refill threshold, steps per refill).
-KERNEL_KEY = "trace_kernel<closest> near_first=1 refill=16 steps=16 defer=16 lds=24"
+KERNEL_KEY = "trace_kernel<closest> near_first=1 refill=8 steps=16 defer=16 lds=20 waves=4"
 
 
 def nrc_mlp(eng, reps=10):
@@ -218,7 +218,7 @@ def main():
 
     eng = native.Engine(0)
     if args.refill_min or args.steps_per_refill or args.near_first >= 0 or args.defer_min:
-        eng.set_trace_tuning(args.refill_min or 16, args.steps_per_refill or 16,
+        eng.set_trace_tuning(args.refill_min or 8, args.steps_per_refill or 16,
                              1 if args.near_first < 0 else args.near_first, args.defer_min or 16)
     if args.calibrate_copy_mib:
         nb = args.calibrate_copy_mib << 20

@@ -500,8 +500,10 @@ def material_table(materials):
     (emissive x emissiveIntensity), doubleSided, 3 pad."""
     out = np.zeros((max(1, len(materials)), 12), np.float32)
     for i, m in enumerate(materials):
-        em = np.asarray(m.get("emissive", [0, 0, 0]), np.float64)
-        inten = m.get("emissiveIntensity", 1.0 if em.any() else 0.0)
+        # the engine receives f32 emissive and intensity; Le is their binary64
+        # product rounded once to f32
+        em = np.asarray(m.get("emissive", [0, 0, 0]), np.float32).astype(np.float64)
+        inten = float(np.float32(m.get("emissiveIntensity", 1.0 if em.any() else 0.0)))
         out[i, 0:3] = m.get("baseColor", [0.8, 0.8, 0.8])
         out[i, 3] = m.get("roughness", 1.0)
         out[i, 4] = m.get("metallic", 0.0)
@@ -522,7 +524,7 @@ def build_emitters(pack, mat_table):
     idx = pack["indices"].reshape(-1, 4)[:, :3]
     tri_src = pack["tri_source"]
     inst_prim = pack["instance_primitive"]
-    l2w = pack["tlas_l2w"].reshape(-1, 16).astype(np.float64)
+    l2w = pack["tlas_l2w"].reshape(-1, 16).astype(np.float64).tolist()
     tri_mat = pack["tri_material"]
     recs = []
     for blas_root, n_inst, vs, vc, ts, tc, prim, kind in b.tolist():
@@ -535,13 +537,18 @@ def build_emitters(pack, mat_table):
         order = np.empty(tc, np.int64)
         order[tri_src[ts:ts + tc]] = np.arange(ts, ts + tc)
         for inst in np.nonzero(inst_prim == prim)[0]:
-            M = l2w[inst].reshape(4, 4).T  # row-major
+            m = l2w[inst]
             for s in range(tc):
                 t = int(order[s])
                 w = []
                 for k in range(3):
-                    p = M @ np.append(pos[idx[t, k]], 1.0)
-                    w.append((p[:3] / p[3]).astype(np.float32))
+                    # transformPoint (pt-webgpu/src/math/mat4.ts:20-33), left
+                    # to right in binary64, divided only when |w| > 1e-8
+                    x, y, z = (float(v) for v in pos[idx[t, k]])
+                    q = [m[j] * x + m[4 + j] * y + m[8 + j] * z + m[12 + j] for j in range(4)]
+                    if abs(q[3]) > 1e-8:
+                        q = [q[0] / q[3], q[1] / q[3], q[2] / q[3]]
+                    w.append(np.array(q[:3], np.float64).astype(np.float32))
                 recs.append(np.concatenate([w[0], w[1], w[2], le, [two, inst, t, 0]]).astype(np.float32))
     if not recs:
         return np.zeros((0, 16), np.float32)

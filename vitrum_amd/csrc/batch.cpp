@@ -24,7 +24,7 @@ struct vt_batch {
   std::vector<uint32_t> host_offsets;
   bool table_dirty = true;
   // Launch parameters (the autotuner's choice).
-  uint32_t refill_min = 16, steps = 16, near_first = 1, defer_min = 16, blocks_per_cu = 1;
+  uint32_t refill_min = 8, steps = 16, near_first = 1, defer_min = 16, blocks_per_cu = 1;
   // Captured graph of (counter reset, batched trace).
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;

@@ -38,7 +38,7 @@ struct vt_engine {
   uint32_t cu_count = 0;
   uint32_t trace_blocks_per_cu = 1;
   uint32_t trace_grid = 1;
-  uint32_t refill_min = 16;
+  uint32_t refill_min = 8;
   uint32_t trace_steps = 16;
   uint32_t near_first = 1;
   uint32_t defer_min = 16;

@@ -154,7 +154,10 @@ vt_status engine_on_scene_changed(vt_engine* e) {
   for (size_t i = 0; i < e->materials.size(); ++i) {
     const vt_material& m = e->materials[i];
     float* q = &mats[i * vtr::kMatFloats];
-    for (int k = 0; k < 3; ++k) q[k] = m.base_color[k], q[5 + k] = m.emissive[k] * m.emissive_intensity;
+    // Le = emissive x emissiveIntensity in binary64, rounded once to f32 (the
+    // reference multiplies JavaScript numbers and packs a Float32Array)
+    for (int k = 0; k < 3; ++k)
+      q[k] = m.base_color[k], q[5 + k] = (float)((double)m.emissive[k] * (double)m.emissive_intensity);
     q[3] = m.roughness;
     q[4] = m.metallic;
     q[8] = m.double_sided ? 1.0f : 0.0f;
@@ -196,10 +199,13 @@ vt_status engine_on_scene_changed(vt_engine* e) {
         float rec[vtr::kEmitFloats] = {0};
         for (int v = 0; v < 3; ++v) {
           const float* p = &P.positions[4 * (size_t)P.indices[4 * (size_t)t + v]];
+          // transformPoint (pt-webgpu/src/math/mat4.ts:20-33): binary64,
+          // divided by w only when |w| > 1e-8
           double q[4];
           for (int k = 0; k < 4; ++k)
             q[k] = (double)m[k] * p[0] + (double)m[4 + k] * p[1] + (double)m[8 + k] * p[2] + (double)m[12 + k];
-          for (int k = 0; k < 3; ++k) rec[3 * v + k] = (float)(q[k] / q[3]);
+          const bool proj = std::fabs(q[3]) > 1e-8;
+          for (int k = 0; k < 3; ++k) rec[3 * v + k] = (float)(proj ? q[k] / q[3] : q[k]);
         }
         rec[9] = mq[5], rec[10] = mq[6], rec[11] = mq[7];
         rec[12] = mq[8];

@@ -8,7 +8,7 @@
 namespace vtk {
 
 constexpr uint32_t kTraceBlock = 128;     // threads per block (2 waves)
-constexpr uint32_t kLdsDepth = 24;        // stack entries per lane kept in LDS
+constexpr uint32_t kLdsDepth = 20;        // stack entries per lane kept in LDS
 constexpr uint32_t kStackCapacity = 128;  // TLAS + BLAS entries per lane (>= 2 * 58 + 2)
 
 struct DeviceScene {

@@ -312,7 +312,9 @@ __device__ __forceinline__ V3 xform_dir(float4 c0, float4 c1, float4 c2, V3 d) {
 // `tNear > best` (equal distances are still visited).  The counting variant
 // always uses the reference order, so its work counts are the reference's.
 template <bool ANY, bool COUNT, bool MULTI, uint32_t BLOCK, uint32_t LDS_DEPTH>
-__global__ __launch_bounds__(BLOCK) void trace_kernel(TraceParams p) {
+// Register budget: four waves per SIMD (128 VGPRs; with the 20-entry LDS
+// stack, 8 blocks of 128 fit a CU), which beats three waves without spills.
+__global__ __launch_bounds__(BLOCK, 4) void trace_kernel(TraceParams p) {
   // Per-lane stack: LDS_DEPTH entries in LDS (entry-major, so a wave's
   // accesses at equal depth touch 64 distinct banks), deeper entries in a
   // per-thread global spill area.  The shared array is indexed directly so the
@@ -442,14 +444,19 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceParams p) {
     return true;
   };
 
+  // Once a refill hands out indices past the last ray the supply is gone for
+  // good; the wave then only finishes its active lanes (no further atomics on
+  // the shared counter, which every finishing wave would otherwise hit).
+  bool exhausted = false;
   for (;;) {
     // ---- wave-level refill: ballot idle lanes, one atomic, prefix ranks ----
     const uint64_t idle = __ballot(!active);
     const uint32_t n_idle = (uint32_t)__popcll(idle);
-    if (n_idle >= p.refill_min || idle == __ballot(1)) {
+    if (!exhausted && (n_idle >= p.refill_min || idle == __ballot(1))) {
       uint32_t base = 0;
       if (lane == 0) base = atomicAdd(p.counter, n_idle);
       base = __shfl(base, 0);
+      exhausted = base + n_idle >= n_rays;
       if (!active) {
         const uint32_t idx = base + (uint32_t)__popcll(idle & lane_below);
         if (idx < n_rays) {
@@ -495,16 +502,18 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceParams p) {
       // ---- deferred expensive work, shared by the lanes that need it ----
       // A lane whose next operation is an instance exit (merge of the local
       // hit), an instance entry (ray transform) or a BLAS leaf (triangle
-      // tests) waits until `defer_min` lanes of the wave need the same
-      // operation, or until no lane has cheap node work left; each lane still
-      // performs exactly its own sequence of operations.
+      // tests) waits until `defer_min` lanes of the wave -- or half of its
+      // active lanes, when fewer are left -- need the same operation, or
+      // until no lane has cheap node work left; each lane still performs
+      // exactly its own sequence of operations.
       const uint64_t m_exit = __ballot(active && pend == kPendExit);
       const uint64_t m_enter = __ballot(active && pend == kPendEnter);
       const uint64_t m_leaf = __ballot(active && pend == kPendLeaf);
       const uint64_t m_cheap = __ballot(active && pend == kPendNone);
       if ((m_exit | m_enter | m_leaf | m_cheap) == 0ull) break;
       const bool all = m_cheap == 0ull;
-      const uint32_t dm = p.defer_min;
+      const uint32_t n_act = (uint32_t)__popcll(m_exit | m_enter | m_leaf | m_cheap);
+      const uint32_t dm = min(p.defer_min, (n_act + 1u) >> 1);
       if (m_exit && (all || (uint32_t)__popcll(m_exit) >= dm) && active && pend == kPendExit) {
         finalize_instance();
         level = 0;
