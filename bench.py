@@ -138,10 +138,28 @@ def rel_rmse(img, ref):
     return float(np.sqrt(np.mean((lum(img) - lum(ref)) ** 2)) / np.mean(lum(ref)))
 
 
+def _accumulate(eng, ivp, width, height, first_frame, frames, seed):
+    eng.reset()
+    for f in range(frames):
+        eng.render_frame(ivp, width, height, first_frame + f, seed, 8)
+        if f % 256 == 255:
+            eng.synchronize()
+    return eng.capture(width, height)[..., :3].astype(np.float64)
+
+
 def time_to_rmse(W, width, height, ref_spp, target, train_paths, max_frames):
-    """Plain vs cache-terminated path tracer on the Cornell scene: GPU time
-    (sum of HIP-event frame times) until the relative RMSE against a
-    ref_spp-sample plain reference first reaches `target`."""
+    """Plain vs cache-terminated path tracer on the Cornell scene: GPU time to
+    a relative RMSE `target` against the converged image.
+
+    The reference is two independent plain halves A and B (ref_spp/2 samples
+    each); its own noise is removed from the error: with R = (A + B)/2,
+    E[(x - R)^2] = MSE(x) + Var(R) and Var(R) = E[(A - B)^2] / 4, so
+    MSE(x) = mean((x - R)^2) - mean((A - B)^2) / 4 (per-pixel luminance).
+    Each estimator is rendered for up to max_frames samples; its measured
+    curve e^2(N) is fitted by b^2 + v / N (b = bias, v = per-sample variance,
+    least squares over N >= 8) and the time to `target` is
+    N* x (mean GPU frame time), N* = v / (target^2 - b^2); None when the bias
+    alone exceeds the target.  Frame times are HIP events around each frame."""
     from vitrum_amd import native
     seed = 0x9E3779B9
     eng = native.Engine(0)
@@ -149,37 +167,53 @@ def time_to_rmse(W, width, height, ref_spp, target, train_paths, max_frames):
     eng.set_scene(sc)
     ivp = W.inv_view_proj(sc["camera"], width, height)
     eng.nrc_render_config(False)
-    eng.reset()
     t0 = time.perf_counter()
-    for f in range(ref_spp):
-        eng.render_frame(ivp, width, height, 50_000_000 + f, seed, 8)
-        if f % 256 == 255:
-            eng.synchronize()
-            log(f"reference {f + 1}/{ref_spp} spp, {time.perf_counter() - t0:.0f} s")
-    ref = eng.capture(width, height)[..., :3].astype(np.float64)
-    out = {"width": width, "height": height, "reference_spp": ref_spp, "target_rel_rmse": target,
-           "train_paths_per_frame": train_paths}
+    half = max(1, ref_spp // 2)
+    ref_a = _accumulate(eng, ivp, width, height, 50_000_000, half, seed)
+    ref_b = _accumulate(eng, ivp, width, height, 60_000_000, half, seed)
+    log(f"reference {2 * half} spp in {time.perf_counter() - t0:.0f} s")
+    ref = 0.5 * (ref_a + ref_b)
+    mean_lum = float(np.mean(lum(ref)))
+    ref_var = float(np.mean((lum(ref_a) - lum(ref_b)) ** 2) / 4.0)
+    out = {"scene": "cornell (config-1 box, 32,768 tris)", "width": width, "height": height,
+           "reference_spp": 2 * half, "target_rel_rmse": target, "train_paths_per_frame": train_paths,
+           "reference_rel_rmse": round(float(np.sqrt(ref_var)) / mean_lum, 5),
+           "method": "noise-corrected MSE vs split-half reference; fit e^2 = b^2 + v/N"}
     for name, nrc in (("plain", False), ("nrc", True)):
         eng.nrc_render_config(nrc, train_paths)
         if nrc:
             eng.nrc_reset(seed=11)
         eng.reset()
-        gpu_ms, reached, curve = 0.0, None, []
+        gpu_ms, curve = 0.0, []
         check = 1
         for f in range(max_frames):
             eng.render_frame(ivp, width, height, f, seed, 8)
             gpu_ms += eng.frame_time_ms()
             if f + 1 == check or f + 1 == max_frames:
-                e = rel_rmse(eng.capture(width, height)[..., :3].astype(np.float64), ref)
+                img = eng.capture(width, height)[..., :3].astype(np.float64)
+                mse = float(np.mean((lum(img) - lum(ref)) ** 2)) - ref_var
+                e = float(np.sqrt(max(mse, 0.0))) / mean_lum
                 curve.append([f + 1, round(gpu_ms, 2), round(e, 5)])
                 log(f"{name}: {f + 1} spp, {gpu_ms:.0f} ms, rel RMSE {e:.4f}")
-                if e <= target:
-                    reached = gpu_ms
-                    break
-                check = max(check + 1, int(check * 1.25))
-        out[name] = {"time_to_target_ms": None if reached is None else round(reached, 2),
-                     "frames": curve[-1][0], "curve": curve}
-    p, n = out["plain"]["time_to_target_ms"], out["nrc"]["time_to_target_ms"]
+                check = max(check + 1, int(check * 1.5))
+        pts = np.array([[n, e] for n, _, e in curve if n >= 8], np.float64)
+        frame_ms = gpu_ms / max_frames
+        entry = {"frames": max_frames, "frame_ms": round(frame_ms, 3), "curve": curve}
+        if pts.shape[0] >= 2:
+            A = np.stack([np.ones(pts.shape[0]), 1.0 / pts[:, 0]], axis=1)
+            (b2, v), *_ = np.linalg.lstsq(A, pts[:, 1] ** 2, rcond=None)
+            b2 = max(float(b2), 0.0)
+            entry["fit_bias"] = round(float(np.sqrt(b2)), 5)
+            entry["fit_sample_variance"] = round(float(v), 5)
+            if target ** 2 > b2 and v > 0:
+                n_star = v / (target ** 2 - b2)
+                entry["spp_to_target"] = round(float(n_star), 1)
+                entry["time_to_target_ms"] = round(float(n_star * frame_ms), 1)
+            else:
+                entry["spp_to_target"] = None
+                entry["time_to_target_ms"] = None
+        out[name] = entry
+    p, n = out["plain"].get("time_to_target_ms"), out["nrc"].get("time_to_target_ms")
     out["nrc_speedup"] = round(p / n, 3) if p and n else None
     eng.close()
     return out
@@ -202,12 +236,12 @@ def main():
     ap.add_argument("--defer-min", type=int, default=0,
                     help="lanes that must need a leaf/instance operation before a wave runs it (0: default)")
     ap.add_argument("--no-nrc-mlp", action="store_true", help="skip the cache MLP kernel measurement")
-    ap.add_argument("--render-rmse", action="store_true",
-                    help="also measure plain vs cache time to a relative RMSE (minutes of GPU time)")
+    ap.add_argument("--no-render-rmse", action="store_true",
+                    help="skip the plain vs cache time-to-RMSE measurement (about a minute of GPU time)")
     ap.add_argument("--rmse-size", default="1920x1080")
-    ap.add_argument("--rmse-ref-spp", type=int, default=4096)
+    ap.add_argument("--rmse-ref-spp", type=int, default=1024)
     ap.add_argument("--rmse-target", type=float, default=1e-2)
-    ap.add_argument("--rmse-max-frames", type=int, default=4096)
+    ap.add_argument("--rmse-max-frames", type=int, default=192)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -320,10 +354,13 @@ def main():
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(sc, rays, args.cpu_seconds)
     eng.close()
-    if args.render_rmse:
+    if not args.no_render_rmse:
         w, h = (int(x) for x in args.rmse_size.split("x"))
-        result["time_to_rmse"] = time_to_rmse(W, w, h, args.rmse_ref_spp, args.rmse_target, 65536,
-                                              args.rmse_max_frames)
+        try:
+            result["time_to_rmse"] = time_to_rmse(W, w, h, args.rmse_ref_spp, args.rmse_target, 65536,
+                                                  args.rmse_max_frames)
+        except Exception as exc:  # the traversal line above stays valid
+            result["time_to_rmse"] = {"error": repr(exc)}
     print(json.dumps(result), flush=True)
 
 
