@@ -127,7 +127,11 @@ typedef struct vt_options {
    * or more lights (lightTreeImportanceSampling, the reference's default on a
    * full-tier device), 1 = uniform (lightTreeImportanceSampling: false). */
   uint32_t light_selection;
-  uint32_t reserved[4];
+  /* Closest-hit traversal: 0 = the binary BVH (pt-webgpu's default), 1 =
+   * bvhTraversal 'cwbvh-closest' (8-wide compressed BVH with a
+   * Moller-Trumbore proposal re-verified by the watertight test). */
+  uint32_t bvh_traversal;
+  uint32_t reserved[3];
 } vt_options;
 
 /* createPTEngine_WebGPU (packages/pt-webgpu/src/index.ts:2363).  The device
@@ -220,6 +224,13 @@ vt_status vt_trace_closest_counts(vt_engine *eng, const vt_ray *rays, uint32_t n
  * many lanes; every lane's own sequence of operations is unchanged. */
 vt_status vt_set_trace_tuning(vt_engine *eng, uint32_t refill_min, uint32_t steps, uint32_t near_first,
                               uint32_t defer_min);
+
+/* bvhTraversal (packages/pt-webgpu/src/index.ts option; opt-in
+ * 'cwbvh-closest', packages/pt-webgpu/src/wgsl/pathTrace/intersection.wgsl.ts:
+ * 576-838): 0 = binary, 1 = wide.  Applies to vt_trace_closest* and to the
+ * path tracer's extension rays; any-hit rays stay binary, as in the
+ * reference. */
+vt_status vt_set_bvh_traversal(vt_engine *eng, uint32_t mode);
 
 /* Test hook (no counterpart in the reference): q = a / b and r = sqrt(a)
  * computed on the device, to check that the device's binary32 divide and

@@ -362,6 +362,27 @@ def cwbvh_trace_closest(scene: PackedScene, cw, inst_roots, rays, *, threads=Non
     return out
 
 
+class CwbvhView(C.Structure):
+    _fields_ = [("node_bounds", F32P), ("child_bounds", C.c_void_p), ("child_meta", U32P),
+                ("child_count", U32P), ("node_count", C.c_uint32), ("inst_root", U32P)]
+
+
+def trace_closest_cwbvh(scene: PackedScene, cw, inst_roots, rays, *, threads=None):
+    """pt-webgpu's `cwbvh-closest` mode in binary32 (intersection.wgsl.ts:
+    576-838): Moller-Trumbore proposal in the wide forest, watertight
+    re-verification, binary fallback.  Same hit record as trace_closest."""
+    rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+    hits = np.zeros(rays.size, HIT_DTYPE)
+    roots = np.ascontiguousarray(inst_roots, np.uint32)
+    keep = [np.ascontiguousarray(a) for a in (cw.node_bounds, cw.child_bounds, cw.child_meta, cw.child_count)]
+    v = CwbvhView(keep[0].ctypes.data_as(F32P), keep[1].ctypes.data_as(C.c_void_p), keep[2].ctypes.data_as(U32P),
+                  keep[3].ctypes.data_as(U32P), cw.node_count, roots.ctypes.data_as(U32P))
+    threads = threads or min(16, os.cpu_count() or 1)
+    lib().orc_trace_closest_cwbvh(C.byref(scene.struct), C.byref(v), rays.ctypes.data_as(C.c_void_p),
+                                  C.c_uint32(rays.size), hits.ctypes.data_as(C.c_void_p), C.c_uint32(threads))
+    return hits
+
+
 LT_FLOATS = 16
 LT_BUCKETS = 1 << 24
 

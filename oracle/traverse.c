@@ -648,3 +648,311 @@ uint32_t orc_pcg_next(uint32_t *state) {
 }
 
 float orc_rand_f32(uint32_t *state) { return (float)(orc_pcg_next(state) >> 8) / 16777216.0f; }
+
+/* ---- pt-webgpu's opt-in `cwbvh-closest` traversal (binary32) -------------
+ * cwbvhIntersectFirstHitRangeFromRoot (shared-bvh/src/wgsl/
+ * cwbvhIntersect.wgsl.ts:197-336) over the 8-wide forest, mollerTrumboreCore
+ * (bvhIntersect.wgsl.ts:160-238), and pt-webgpu's traceMeshCwbvhClosest /
+ * traceTlasClosestCwbvh (pathTrace/intersection.wgsl.ts:576-838): the wide
+ * walk proposes a Moller-Trumbore hit, the watertight test re-verifies it
+ * (bit-identical t required), and any disagreement, corrupt layout or stack
+ * overflow falls back to the binary traceMeshBvh for that BLAS. */
+
+#define CW_STACK 64
+#define CW_INF 3.402823e38f
+
+typedef struct {
+  int status; /* 0 complete, 1 overflow, 2 invalid layout */
+  int did_hit;
+  float dist;
+  uint32_t tri;
+} cw_hit;
+
+static float cw_decode(uint32_t q, float mn, float mx) {
+  if (!(mx > mn)) return mn;
+  if (q == 0) return mn;
+  if (q >= 65535u) return mx;
+  const float t = (float)q / 65535.0f;
+  return mn * (1.0f - t) + mx * t;
+}
+
+static int cw_bounds_valid(v3 mn, v3 mx) {
+  return mn.x <= mx.x && mn.y <= mx.y && mn.z <= mx.z && fabsf_(mn.x) <= CW_INF && fabsf_(mn.y) <= CW_INF &&
+         fabsf_(mn.z) <= CW_INF && fabsf_(mx.x) <= CW_INF && fabsf_(mx.y) <= CW_INF && fabsf_(mx.z) <= CW_INF;
+}
+
+static int cw_aabb_entry(v3 o, v3 inv, v3 bmin, v3 bmax, float tmax) {
+  v3 t0 = {(bmin.x - o.x) * inv.x, (bmin.y - o.y) * inv.y, (bmin.z - o.z) * inv.z};
+  v3 t1 = {(bmax.x - o.x) * inv.x, (bmax.y - o.y) * inv.y, (bmax.z - o.z) * inv.z};
+  float tn = fmax2(fmax2(fmin2(t0.x, t1.x), fmin2(t0.y, t1.y)), fmin2(t0.z, t1.z));
+  float tf = fmin2(fmin2(fmax2(t0.x, t1.x), fmax2(t0.y, t1.y)), fmax2(t0.z, t1.z));
+  return !(tn > tf || tf < 0.0f || tn > tmax);
+}
+
+static v3 cross3(v3 a, v3 b) {
+  return (v3){a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+static float dot3l(v3 a, v3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+
+/* mollerTrumboreCore: equilibrated edges and direction, angular determinant
+ * rejection at 1e-7, barycentric tolerance 1e-6. */
+static int mt_core(v3 o, v3 dir, v3 a, v3 b, v3 c, float tmin, float *t_out) {
+  v3 e1r = sub(b, a), e2r = sub(c, a);
+  float es = fmax2(fmax2(fmax2(fabsf_(e1r.x), fabsf_(e1r.y)), fabsf_(e1r.z)),
+                   fmax2(fmax2(fabsf_(e2r.x), fabsf_(e2r.y)), fabsf_(e2r.z)));
+  float ds = fmax2(fabsf_(dir.x), fmax2(fabsf_(dir.y), fabsf_(dir.z)));
+  if (!(es > 0.0f) || es > CW_INF || !(ds > 0.0f) || ds > CW_INF) return 0;
+  v3 e1 = {e1r.x / es, e1r.y / es, e1r.z / es};
+  v3 e2 = {e2r.x / es, e2r.y / es, e2r.z / es};
+  v3 dn = {dir.x / ds, dir.y / ds, dir.z / ds};
+  v3 ao0 = sub(o, a);
+  v3 ao = {ao0.x / es, ao0.y / es, ao0.z / es};
+  v3 n = cross3(e1, e2);
+  float ns = fmax2(fabsf_(n.x), fmax2(fabsf_(n.y), fabsf_(n.z)));
+  if (!(ns > 0.0f) || ns > CW_INF) return 0;
+  v3 nd = {n.x / ns, n.y / ns, n.z / ns};
+  float nl = sqrtf(dot3l(nd, nd));
+  float dl = sqrtf(dot3l(dn, dn));
+  v3 rd = {dn.x / dl, dn.y / dl, dn.z / dl};
+  float det = -dot3l(dn, n);
+  v3 nn = {nd.x / nl, nd.y / nl, nd.z / nl};
+  float ang = fabsf_(dot3l(rd, nn));
+  if (!(ang > 1e-7f) || det == 0.0f) return 0;
+  float inv_det = 1.0f / det;
+  v3 dao = cross3(ao, dn);
+  float u = dot3l(e2, dao) * inv_det;
+  float v = -dot3l(e1, dao) * inv_det;
+  float st = dot3l(ao, n) * inv_det;
+  float t = (st * es) / ds;
+  float w = (1.0f - u) - v;
+  if (isnan(u) || isnan(v) || isnan(w) || isnan(t) || fabsf_(u) > CW_INF || fabsf_(v) > CW_INF ||
+      fabsf_(w) > CW_INF || fabsf_(t) > CW_INF || u < -1e-6f || v < -1e-6f || w < -1e-6f ||
+      t < fmax2(tmin, 0.0f))
+    return 0;
+  *t_out = t;
+  return 1;
+}
+
+static cw_hit cw_walk(const orc_scene *s, const orc_cwbvh_view *cw, v3 o, v3 d, float tmin, float tmax,
+                      uint32_t root) {
+  cw_hit best = {0, 0, tmax, 0};
+  const uint32_t nc = cw->node_count;
+  if (nc == 0) return best;
+  if (root >= nc) {
+    best.status = 2;
+    return best;
+  }
+  uint32_t stack[CW_STACK];
+  uint32_t sp = 0;
+  stack[sp++] = root;
+  const v3 inv = {safe_inv_comp(d.x), safe_inv_comp(d.y), safe_inv_comp(d.z)};
+  while (sp > 0) {
+    const uint32_t node = stack[--sp];
+    if (node >= nc) {
+      best.status = 2;
+      return best;
+    }
+    const uint32_t count = cw->child_count[node];
+    if (count > 8) {
+      best.status = 2;
+      return best;
+    }
+    const float *pb = cw->node_bounds + 6 * (size_t)node;
+    const v3 pmn = {pb[0], pb[1], pb[2]}, pmx = {pb[3], pb[4], pb[5]};
+    if (!cw_bounds_valid(pmn, pmx)) {
+      best.status = 2;
+      return best;
+    }
+    for (uint32_t slot = 0; slot < count; ++slot) {
+      const uint32_t *m = cw->child_meta + ((size_t)node * 8 + slot) * 3;
+      if (m[0] == 0) {
+        best.status = 2;
+        return best;
+      }
+      const uint16_t *q = cw->child_bounds + ((size_t)node * 8 + slot) * 6;
+      const v3 bmn = {cw_decode(q[0], pmn.x, pmx.x), cw_decode(q[1], pmn.y, pmx.y), cw_decode(q[2], pmn.z, pmx.z)};
+      const v3 bmx = {cw_decode(q[3], pmn.x, pmx.x), cw_decode(q[4], pmn.y, pmx.y), cw_decode(q[5], pmn.z, pmx.z)};
+      if (!cw_bounds_valid(bmn, bmx)) {
+        best.status = 2;
+        return best;
+      }
+      if (!cw_aabb_entry(o, inv, bmn, bmx, best.dist)) continue;
+      if (m[0] == 1) {
+        if (m[1] >= nc) {
+          best.status = 2;
+          return best;
+        }
+        if (sp >= CW_STACK) {
+          best.status = 1;
+          return best;
+        }
+        stack[sp++] = m[1];
+      } else if (m[0] == 2) {
+        if (m[2] == 0 || m[1] > s->triangle_count || m[2] > s->triangle_count - m[1]) {
+          best.status = 2;
+          return best;
+        }
+        for (uint32_t i = 0; i < m[2]; ++i) {
+          const uint32_t tri = m[1] + i;
+          const uint32_t *ix = s->indices + 4 * (size_t)tri;
+          if (ix[0] >= s->vertex_count || ix[1] >= s->vertex_count || ix[2] >= s->vertex_count) {
+            best.status = 2;
+            return best;
+          }
+          float t;
+          if (mt_core(o, d, vload(s, ix[0]), vload(s, ix[1]), vload(s, ix[2]), tmin, &t) && t > tmin &&
+              t < best.dist) {
+            best.did_hit = 1;
+            best.dist = t;
+            best.tri = tri;
+          }
+        }
+      } else {
+        best.status = 2;
+        return best;
+      }
+    }
+  }
+  return best;
+}
+
+/* traceMeshCwbvhClosest */
+static int trace_mesh_cwbvh(const orc_scene *s, const orc_cwbvh_view *cw, const ray3 *r, float tmin, float tmax,
+                            local_hit *hit, uint32_t wide_root, uint32_t binary_root, counters *cnt) {
+  hit->did_hit = 0;
+  hit->dist = tmax;
+  hit->position = (v3){0, 0, 0};
+  hit->tri = 0;
+  hit->u = hit->v = 0;
+  if (cw->node_count == 0 || wide_root >= cw->node_count)
+    return trace_mesh_bvh(s, r, tmin, tmax, 1, hit, binary_root, cnt);
+  const cw_hit c = cw_walk(s, cw, r->o, r->d, tmin, tmax, wide_root);
+  if (c.status != 0) return trace_mesh_bvh(s, r, tmin, tmax, 1, hit, binary_root, cnt);
+  if (!c.did_hit || c.dist <= tmin || c.dist >= tmax) return 0;
+  if (c.tri >= s->triangle_count) return trace_mesh_bvh(s, r, tmin, tmax, 1, hit, binary_root, cnt);
+  const uint32_t *ix = s->indices + 4 * (size_t)c.tri;
+  const wt_hit e = watertight(r->o, r->d, vload(s, ix[0]), vload(s, ix[1]), vload(s, ix[2]), tmin);
+  uint32_t eb, cb;
+  memcpy(&eb, &e.t, 4);
+  memcpy(&cb, &c.dist, 4);
+  if (!e.hit || eb != cb) return trace_mesh_bvh(s, r, tmin, tmax, 1, hit, binary_root, cnt);
+  hit->did_hit = 1;
+  hit->dist = c.dist;
+  hit->position = e.point;
+  hit->tri = c.tri;
+  hit->u = e.bary.y;
+  hit->v = e.bary.z;
+  return 1;
+}
+
+/* traceTlasClosestCwbvh: the reference TLAS order, cwbvh BLAS walks. */
+static void trace_tlas_closest_cwbvh(const orc_scene *s, const orc_cwbvh_view *cw, const ray3 *r, float tmin,
+                                     float tmax, world_hit *hit) {
+  hit->did_hit = 0;
+  hit->dist = tmax;
+  hit->tri = 0;
+  hit->inst = 0xFFFFFFFFu;
+  hit->u = hit->v = 0;
+  if (s->tlas_node_count == 0 || s->instance_index_count == 0) {
+    trace_tlas_closest(s, r, tmin, tmax, hit, NULL);
+    return;
+  }
+  uint32_t stack[TLAS_STACK_DEPTH];
+  uint32_t sp = 0;
+  stack[sp++] = 0;
+  while (sp > 0) {
+    const uint32_t ni = stack[--sp];
+    if (ni >= s->tlas_node_count) continue;
+    const uint32_t *node = s->tlas_nodes + 8 * (size_t)ni;
+    if (!slab(r, node, tmin, hit->dist, NULL)) continue;
+    const uint32_t soc = node[7];
+    if ((soc & LEAF_FLAG) == LEAF_FLAG) {
+      const uint32_t count = soc & 0xFFFFu, start = node[6];
+      for (uint32_t i = 0; i < count; ++i) {
+        const uint32_t pi = start + i;
+        if (pi >= s->instance_index_count) continue;
+        const uint32_t inst = s->instance_indices[pi];
+        if (inst >= s->instance_count) continue;
+        const float *w2l = s->w2l + 16 * (size_t)inst;
+        const float *l2w = s->l2w + 16 * (size_t)inst;
+        const v3 lo = xform_point_cols(w2l, r->o);
+        const v3 ld = xform_dir_cols(w2l, r->d);
+        const ray3 lr = make_ray(lo, ld);
+        const v3 start_p = {r->o.x + r->d.x * tmin, r->o.y + r->d.y * tmin, r->o.z + r->d.z * tmin};
+        const float ltmin = fmax2(dot3(sub(xform_point_cols(w2l, start_p), lo), ld), 0.0f);
+        float ltmax = k_infinity();
+        if (hit->dist < k_infinity() * 0.5f) {
+          const v3 end = {r->o.x + r->d.x * hit->dist, r->o.y + r->d.y * hit->dist, r->o.z + r->d.z * hit->dist};
+          ltmax = fmax2(dot3(sub(xform_point_cols(w2l, end), lo), ld), ltmin);
+        }
+        local_hit lh;
+        trace_mesh_cwbvh(s, cw, &lr, ltmin, ltmax, &lh, cw->inst_root[inst], s->blas_roots[inst], NULL);
+        if (lh.did_hit && lh.dist > ltmin) {
+          const v3 wpos = xform_point_cols(l2w, lh.position);
+          const float wdist = dot3(sub(wpos, r->o), r->d);
+          if (wdist <= tmin || wdist >= hit->dist) continue;
+          hit->did_hit = 1;
+          hit->dist = wdist;
+          hit->tri = lh.tri;
+          hit->inst = inst;
+          hit->u = lh.u;
+          hit->v = lh.v;
+          hit->pos = wpos;
+        }
+      }
+    } else {
+      const uint32_t left = ni + 1, right = ni + node[6];
+      if (sp + 2 <= TLAS_STACK_DEPTH) {
+        stack[sp++] = right;
+        stack[sp++] = left;
+      } else {
+        trace_tlas_closest(s, r, tmin, tmax, hit, NULL);
+        return;
+      }
+    }
+  }
+}
+
+typedef struct {
+  const orc_scene *s;
+  const orc_cwbvh_view *cw;
+  const orc_ray *rays;
+  orc_hit *out;
+  uint32_t begin, end;
+} cw_job;
+
+static void *cw_job_fn(void *arg) {
+  cw_job *j = arg;
+  for (uint32_t i = j->begin; i < j->end; ++i) {
+    const orc_ray *q = &j->rays[i];
+    const ray3 r = make_ray((v3){q->o[0], q->o[1], q->o[2]}, (v3){q->d[0], q->d[1], q->d[2]});
+    world_hit h;
+    trace_tlas_closest_cwbvh(j->s, j->cw, &r, q->tmin, q->tmax, &h);
+    orc_hit *o = &j->out[i];
+    o->prim = h.did_hit ? h.tri : 0xFFFFFFFFu;
+    o->inst = h.did_hit ? h.inst : 0xFFFFFFFFu;
+    o->t = h.did_hit ? h.dist : INFINITY;
+    o->u = h.did_hit ? h.u : 0.0f;
+    o->v = h.did_hit ? h.v : 0.0f;
+  }
+  return NULL;
+}
+
+int orc_trace_closest_cwbvh(const orc_scene *s, const orc_cwbvh_view *cw, const orc_ray *rays, uint32_t n,
+                            orc_hit *out, uint32_t threads) {
+  if (threads < 1) threads = 1;
+  if (threads > 256) threads = 256;
+  pthread_t tid[256];
+  cw_job jobs[256];
+  const uint32_t per = (n + threads - 1) / threads;
+  for (uint32_t t = 0; t < threads; ++t) {
+    jobs[t] = (cw_job){s, cw, rays, out, t * per < n ? t * per : n, (t + 1) * per < n ? (t + 1) * per : n};
+    if (pthread_create(&tid[t], NULL, cw_job_fn, &jobs[t]) != 0) {
+      cw_job_fn(&jobs[t]);
+      tid[t] = 0;
+    }
+  }
+  for (uint32_t t = 0; t < threads; ++t)
+    if (tid[t]) pthread_join(tid[t], NULL);
+  return 0;
+}

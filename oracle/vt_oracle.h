@@ -208,6 +208,21 @@ int orc_cwbvh_copy(const orc_cwbvh *c, float *node_bounds,
 double orc_cwbvh_stack_entries(const uint32_t *child_meta,
                                const uint32_t *child_count,
                                uint32_t node_count, uint32_t root);
+/* pt-webgpu's opt-in `cwbvh-closest` traversal in binary32
+ * (pathTrace/intersection.wgsl.ts:576-838 over cwbvhIntersect.wgsl.ts:197-336):
+ * Moller-Trumbore proposal in the wide forest, watertight re-verification,
+ * binary fallback per BLAS.  The view points at an orc_cwbvh's arrays. */
+typedef struct orc_cwbvh_view {
+  const float *node_bounds;
+  const uint16_t *child_bounds;
+  const uint32_t *child_meta;
+  const uint32_t *child_count;
+  uint32_t node_count;
+  const uint32_t *inst_root; /* wide root per packed instance */
+} orc_cwbvh_view;
+int orc_trace_closest_cwbvh(const orc_scene *s, const orc_cwbvh_view *cw,
+                            const orc_ray *rays, uint32_t n, orc_hit *out,
+                            uint32_t threads);
 /* blas_wide_root: wide root of each packed instance's BLAS. */
 int orc_cwbvh_trace_closest(const orc_scene *s, const orc_cwbvh *c,
                             const uint32_t *blas_wide_root,

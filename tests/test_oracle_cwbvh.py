@@ -320,3 +320,129 @@ def test_watertight_traversal_agrees_with_reference_cpu_walk(cornell_rays):
     f = O.fp64_hit_params(ps, rays, wt)
     rel64 = np.abs(f[same & both, 0] - ts["t"][same & both]) / np.abs(ts["t"][same & both])
     assert rel64.max() <= 1e-6
+
+
+# ---- pt-webgpu's `cwbvh-closest` mode (binary32) ---------------------------
+
+
+class _Forest:
+    """A Cwbvh's arrays, copied so a test can corrupt them."""
+
+    def __init__(self, cw):
+        self.node_bounds = cw.node_bounds.copy()
+        self.child_bounds = cw.child_bounds.copy()
+        self.child_meta = cw.child_meta.copy()
+        self.child_count = cw.child_count.copy()
+        self.node_count = cw.node_count
+
+
+def test_f32_mode_known_answer_single_triangle():
+    """An axis-aligned ray through (0.25, 0.25) of a unit right triangle at
+    z = 2: Moller-Trumbore and the watertight test both give t = 2 exactly,
+    so the proposal is verified and the record is the binary mode's."""
+    pos = np.array([0, 0, 2, 0, 1, 0, 2, 0, 0, 1, 2, 0], np.float32)
+    idx = np.array([0, 1, 2, 0], np.uint32)
+    ps = single_instance_scene(pos, idx)
+    cw, roots = O.scene_cwbvh(ps)
+    r = rays_of([[0.25, 0.25, 0.0], [0.25, 0.25, 3.0], [2.0, 2.0, 0.0]],
+                [[0, 0, 1], [0, 0, 1], [0, 0, 1]])
+    h = O.trace_closest_cwbvh(ps, cw, roots, r)
+    assert h["prim"][0] == 0 and h["t"][0] == np.float32(2.0)
+    assert h["u"][0] == np.float32(0.25) and h["v"][0] == np.float32(0.25)
+    assert h["inst"][1] == 0xFFFFFFFF and h["inst"][2] == 0xFFFFFFFF  # behind / beside
+    assert_records_equal(h, O.trace_closest(ps, r))
+
+
+def test_f32_mode_rejects_grazing_ray():
+    """A ray in the triangle's plane: the angular determinant test
+    (|dot(rd, n)| <= 1e-7) rejects the proposal, and the watertight binary
+    traversal misses as well."""
+    pos = np.array([0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0], np.float32)
+    idx = np.array([0, 1, 2, 0], np.uint32)
+    ps = single_instance_scene(pos, idx)
+    cw, roots = O.scene_cwbvh(ps)
+    r = rays_of([[-1.0, 0.25, 0.0]], [[1, 0, 0]])
+    assert O.trace_closest_cwbvh(ps, cw, roots, r)["inst"][0] == 0xFFFFFFFF
+
+
+def assert_records_equal(a, b):
+    for k in ("prim", "inst", "t", "u", "v"):
+        np.testing.assert_array_equal(a[k].view(np.uint32), b[k].view(np.uint32), err_msg=k)
+
+
+def test_f32_mode_agrees_with_binary_mode(cornell_rays):
+    """Every proposal that survives the bit-exact watertight re-check is a
+    watertight hit, so the mode returns the binary traversal's record except
+    where a Moller-Trumbore proposal beats the exact closest triangle's
+    (then its t can only be larger) -- measured: no such ray among the
+    32,768 primary + bounce rays."""
+    ps, rays, _ = cornell_rays
+    cw, roots = O.scene_cwbvh(ps)
+    got = O.trace_closest_cwbvh(ps, cw, roots, rays)
+    ref = O.trace_closest(ps, rays)
+    same = np.ones(rays.size, bool)
+    for k in ("prim", "inst", "t", "u", "v"):
+        same &= got[k].view(np.uint32) == ref[k].view(np.uint32)
+    assert same.mean() >= 0.999, int((~same).sum())
+    both = ~same & (got["inst"] != 0xFFFFFFFF) & (ref["inst"] != 0xFFFFFFFF)
+    assert np.all(got["t"][both] >= ref["t"][both])
+
+
+def test_f32_mode_falls_back_on_invalid_forest(cornell_rays):
+    """intersection.wgsl.ts:576-700: an invalid wide layout (child count above
+    8, a child of kind 0, a wide root outside the forest) answers the BLAS
+    with the binary traversal, whose records are returned unchanged."""
+    ps, rays, _ = cornell_rays
+    rays = rays[:4096]
+    cw, roots = O.scene_cwbvh(ps)
+    ref = O.trace_closest(ps, rays)
+    bad = _Forest(cw)
+    bad.child_count[:] = 0xFFFFFFFF
+    assert_records_equal(O.trace_closest_cwbvh(ps, bad, roots, rays), ref)
+    bad = _Forest(cw)
+    bad.child_meta[:, :, 0] = 0
+    assert_records_equal(O.trace_closest_cwbvh(ps, bad, roots, rays), ref)
+    far = np.full_like(roots, cw.node_count + 5)
+    assert_records_equal(O.trace_closest_cwbvh(ps, cw, far, rays), ref)
+
+
+def test_f32_mode_tmin_tmax_window():
+    """cwbvhTMinParity.test.ts semantics in the f32 mode: hits at t <= tMin
+    or t >= tMax are not reported, the next surface beyond tMin is."""
+    pos, idx, mats = make_grid(4)
+    # a second layer at z = 1 (translated copy of the grid)
+    p2 = pos.reshape(-1, 4).copy()
+    p2[:, 2] = 1.0
+    pos2 = np.concatenate([pos.reshape(-1, 4), p2]).ravel()
+    idx2 = np.concatenate([idx.reshape(-1, 4), idx.reshape(-1, 4) + np.array([25, 25, 25, 0], np.uint32)]).ravel()
+    ps = single_instance_scene(pos2.astype(np.float32), idx2.astype(np.uint32))
+    cw, roots = O.scene_cwbvh(ps)
+    o = [[1.3, 2.7, -1.0]] * 4
+    d = [[0, 0, 1]] * 4
+    r = rays_of(o, d)
+    r["tmin"] = np.array([0.0, 1.0, 1.5, 0.0], np.float32)
+    r["tmax"] = np.array([W.TMAX_INFINITY, W.TMAX_INFINITY, W.TMAX_INFINITY, 1.0], np.float32)
+    h = O.trace_closest_cwbvh(ps, cw, roots, r)
+    # layers at t = 1 and t = 2: (0, inf) -> 1, (1, inf) -> 2, (1.5, inf) -> 2, (0, 1) -> miss
+    assert h["t"][:3].tolist() == [1.0, 2.0, 2.0]
+    assert h["inst"][3] == 0xFFFFFFFF
+    assert_records_equal(h, O.trace_closest(ps, r))
+
+
+def test_f32_mode_rechecks_tolerance_hits():
+    """A ray passing 4e-7 outside a triangle's edge: Moller-Trumbore's 1e-6
+    barycentric tolerance proposes the triangle, the watertight re-check
+    rejects it, and the binary fallback reports the miss (a mode that kept
+    the unverified proposal, with its own hit point, would report t = 1).  The second triangle widens
+    the wide node, so the first one's quantized box (rounded outward by one
+    step of 1/65535 of the parent's extent) contains the ray."""
+    pos = np.array([0, 0, 1, 0, 1, 0, 1, 0, 0, 1, 1, 0,
+                    2, -1, 1, 0, 3, -1, 1, 0, 2, -0.5, 1, 0], np.float32)
+    idx = np.array([0, 1, 2, 0, 3, 4, 5, 0], np.uint32)
+    ps = single_instance_scene(pos, idx)
+    cw, roots = O.scene_cwbvh(ps)
+    r = rays_of([[0.5, -4e-7, 0.0], [0.5, 4e-7, 0.0]], [[0, 0, 1], [0, 0, 1]])
+    h = O.trace_closest_cwbvh(ps, cw, roots, r)
+    assert h["inst"][0] == 0xFFFFFFFF
+    assert h["inst"][1] == 0 and h["t"][1] == np.float32(1.0)
+    assert_records_equal(h, O.trace_closest(ps, r))

@@ -169,6 +169,8 @@ vt_status vt_create(int32_t device_ordinal, const vt_options* opts, vt_engine** 
     return set_err(nullptr, VT_ERR_RANGE, "sampling must be 0 (pcg) or 1 (sobol)");
   if (opts && opts->light_selection > 1)
     return set_err(nullptr, VT_ERR_RANGE, "light_selection must be 0 (light tree) or 1 (uniform)");
+  if (opts && opts->bvh_traversal > 1)
+    return set_err(nullptr, VT_ERR_RANGE, "bvh_traversal must be 0 (binary) or 1 (cwbvh-closest)");
   std::unique_ptr<vt_engine> e(new vt_engine());
   e->device = device_ordinal;
   if (opts) e->opts = *opts;
@@ -207,6 +209,7 @@ vt_status vt_destroy(vt_engine* eng) {
   eng->tmp_rays.release();
   eng->tmp_out.release();
   eng->tmp_counts.release();
+  engine_release_cwbvh(eng);
   for (auto& kv : eng->user_allocs) (void)hipFree(kv);
   engine_release_render(eng);
   engine_release_nrc(eng);
@@ -254,6 +257,7 @@ vt_status vt_set_scene(vt_engine* eng, const vt_scene_desc* scene) {
     nb.release();
     return s;
   }
+  engine_release_cwbvh(eng);
   eng->scene.release();
   eng->scene = nb;
   nb = SceneBuffers();
@@ -377,6 +381,16 @@ static vt_status trace_device(vt_engine* eng, const vt_ray* d_rays, uint32_t n, 
   p.defer_min = eng->defer_min;
   eng->last_ray_count = n;
   if (n == 0) return VT_OK;
+  // The cwbvh-closest mode covers closest hits; any-hit rays and the work
+  // counters (defined on the binary order) stay on the binary kernel.
+  if (!any && !d_counts && eng->opts.bvh_traversal == 1) {
+    HIP_TRY(eng, hipEventRecord(eng->ev_start, eng->stream));
+    if (vt_status s = engine_trace_cwbvh(eng, (const float4*)d_rays, n, nullptr, (uint32_t*)d_out, nullptr))
+      return s;
+    HIP_TRY(eng, hipEventRecord(eng->ev_stop, eng->stream));
+    eng->trace_timed = true;
+    return VT_OK;
+  }
   HIP_TRY(eng, hipEventRecord(eng->ev_start, eng->stream));
   HIP_TRY(eng, vtk::launch_trace(p, any, d_counts != nullptr, eng->trace_grid, eng->stream));
   HIP_TRY(eng, hipEventRecord(eng->ev_stop, eng->stream));

@@ -26,6 +26,7 @@ struct SceneBuffers {
 };
 
 struct RenderState;  // integrator state (render.hip)
+struct CwbvhState;   // wide-BVH traversal buffers (cwbvh.cpp)
 struct NrcState;     // neural radiance cache (nrc.hip)
 
 struct vt_engine {
@@ -58,6 +59,7 @@ struct vt_engine {
   uint32_t last_ray_count = 0;
   RenderState* render = nullptr;
   NrcState* nrc = nullptr;
+  CwbvhState* cwbvh = nullptr;
   std::string last_error;
 };
 
@@ -69,6 +71,20 @@ vt_status engine_build_trace_scene(vt_engine* e, const vt_scene_desc* scene, Sce
 vt_status engine_on_scene_changed(vt_engine* e);
 void engine_release_render(vt_engine* e);
 void engine_release_nrc(vt_engine* e);
+// Wide-BVH (cwbvh-closest) traversal state (cwbvh.cpp): built on first use
+// per scene, released on scene change.
+vt_status engine_cwbvh_ensure(vt_engine* e);
+void engine_release_cwbvh(vt_engine* e);
+// Device view of the wide forest (null before engine_cwbvh_ensure succeeds).
+namespace vtw {
+struct Scene;
+}
+const vtw::Scene* engine_cwbvh_view(vt_engine* e);
+// Closest hit in the cwbvh-closest mode: 2 x float4 per ray in, 5 words per
+// ray out, optional world hit positions; the ray count is read from d_n when
+// it is set (n is then the capacity).
+vt_status engine_trace_cwbvh(vt_engine* e, const float4* d_rays, uint32_t n, const uint32_t* d_n,
+                             uint32_t* d_hits, float* d_hit_pos);
 // Cache hooks for the integrator (nrc.cpp): counts are device words.
 vt_status engine_nrc_ensure(vt_engine* e);
 vt_status engine_nrc_infer_counted(vt_engine* e, const float* d_feat, const uint32_t* d_count, float* d_rgb,

@@ -489,7 +489,11 @@ vt_status vt_render_frame(vt_engine* e, const vt_frame* fr) {
     cp.hits = ps.hits;
     cp.hit_pos = ps.hit_pos;
     cp.n_ptr = ps.count;
-    HIP_CHECK(e, vtk::launch_trace(cp, false, false, e->trace_grid, e->stream));
+    if (e->opts.bvh_traversal == 1) {
+      if (vt_status s = engine_trace_cwbvh(e, ps.rays, cap, ps.count, ps.hits, ps.hit_pos)) return s;
+    } else {
+      HIP_CHECK(e, vtk::launch_trace(cp, false, false, e->trace_grid, e->stream));
+    }
     HIP_CHECK(e, vtr::launch_shade(f, sc, ps, v, cap, e->stream));
     if (v < f.bounce_limit) {
       vtk::TraceParams sp = tp;

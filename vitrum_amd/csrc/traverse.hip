@@ -29,6 +29,7 @@
 #include <cstdint>
 
 #include "traverse.h"
+#include "cwbvh.h"
 
 namespace vtk {
 
@@ -708,3 +709,403 @@ hipError_t launch_fp32_selftest(const float* a, const float* b, float* q, float*
 }
 
 }  // namespace vtk
+
+// ---------------------------------------------------------------------------
+// cwbvh-closest mode: pt-webgpu's opt-in wide traversal
+// (pathTrace/intersection.wgsl.ts:576-838).  One ray per lane over the
+// reference-layout TLAS; each BLAS is walked in its 8-wide compressed tree
+// (cwbvhIntersect.wgsl.ts:197-336) with the Moller-Trumbore core
+// (bvhIntersect.wgsl.ts:160-238); the proposed hit is accepted only when the
+// watertight test reproduces its distance bit for bit, and a disagreement, an
+// invalid wide layout or a stack overflow answers that BLAS with the binary
+// traversal (intersectionCore.wgsl.ts:461-596).  Any-hit rays never use it.
+// ---------------------------------------------------------------------------
+
+namespace vtw {
+
+using vtk::F4;
+using vtk::TriHit;
+using vtk::V3;
+using vtk::WtRay;
+
+constexpr uint32_t kWideStack = 64;  // cwbvhIntersect.wgsl.ts stack depth
+constexpr uint32_t kBinStack = 60;   // strides.ts:110
+constexpr uint32_t kTlasStack = 64;  // strides.ts:112
+constexpr uint32_t kLeafMask = 0xFFFF0000u;
+
+// The WGSL select-based min / max (NaN compares false).
+__device__ __forceinline__ float max2(float a, float b) { return a > b ? a : b; }
+__device__ __forceinline__ float min2(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ float cw_inf() { return 3.402823e38f; }
+
+__device__ __forceinline__ float decode(uint32_t q, float mn, float mx) {
+  if (!(mx > mn)) return mn;
+  if (q == 0) return mn;
+  if (q >= 65535u) return mx;
+  const float t = (float)q / 65535.0f;
+  return mn * (1.0f - t) + mx * t;
+}
+
+__device__ __forceinline__ bool bounds_valid(V3 mn, V3 mx) {
+  const float m = cw_inf();
+  return mn.x <= mx.x && mn.y <= mx.y && mn.z <= mx.z && vtk::fabs_(mn.x) <= m && vtk::fabs_(mn.y) <= m &&
+         vtk::fabs_(mn.z) <= m && vtk::fabs_(mx.x) <= m && vtk::fabs_(mx.y) <= m && vtk::fabs_(mx.z) <= m;
+}
+
+__device__ __forceinline__ bool aabb_entry(V3 o, V3 inv, V3 bmn, V3 bmx, float tmax) {
+  const float t0x = (bmn.x - o.x) * inv.x, t0y = (bmn.y - o.y) * inv.y, t0z = (bmn.z - o.z) * inv.z;
+  const float t1x = (bmx.x - o.x) * inv.x, t1y = (bmx.y - o.y) * inv.y, t1z = (bmx.z - o.z) * inv.z;
+  const float tn = max2(max2(min2(t0x, t1x), min2(t0y, t1y)), min2(t0z, t1z));
+  const float tf = min2(min2(max2(t0x, t1x), max2(t0y, t1y)), max2(t0z, t1z));
+  return !(tn > tf || tf < 0.0f || tn > tmax);
+}
+
+__device__ __forceinline__ V3 cross3(V3 a, V3 b) {
+  return vtk::v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ V3 scale_div(V3 a, float s) { return vtk::v3(a.x / s, a.y / s, a.z / s); }
+
+// mollerTrumboreCore: equilibrated edges and direction, angular determinant
+// rejection at 1e-7, barycentric tolerance 1e-6.  Returns t, or -1 for a miss.
+__device__ __noinline__ float mt_core(V3 o, V3 dir, V3 a, V3 b, V3 c, float tmin) {
+  using vtk::dot3;
+  using vtk::fabs_;
+  using vtk::nan_;
+  const V3 e1r = vtk::sub(b, a), e2r = vtk::sub(c, a);
+  const float es = max2(max2(max2(fabs_(e1r.x), fabs_(e1r.y)), fabs_(e1r.z)),
+                        max2(max2(fabs_(e2r.x), fabs_(e2r.y)), fabs_(e2r.z)));
+  const float ds = max2(fabs_(dir.x), max2(fabs_(dir.y), fabs_(dir.z)));
+  if (!(es > 0.0f) || es > cw_inf() || !(ds > 0.0f) || ds > cw_inf()) return -1.0f;
+  const V3 e1 = scale_div(e1r, es), e2 = scale_div(e2r, es), dn = scale_div(dir, ds);
+  const V3 ao = scale_div(vtk::sub(o, a), es);
+  const V3 n = cross3(e1, e2);
+  const float ns = max2(fabs_(n.x), max2(fabs_(n.y), fabs_(n.z)));
+  if (!(ns > 0.0f) || ns > cw_inf()) return -1.0f;
+  const V3 nd = scale_div(n, ns);
+  const float nl = __builtin_sqrtf(dot3(nd, nd));
+  const float dl = __builtin_sqrtf(dot3(dn, dn));
+  const V3 rd = scale_div(dn, dl);
+  const float det = -dot3(dn, n);
+  const V3 nn = scale_div(nd, nl);
+  const float ang = fabs_(dot3(rd, nn));
+  if (!(ang > 1e-7f) || det == 0.0f) return -1.0f;
+  const float inv_det = 1.0f / det;
+  const V3 dao = cross3(ao, dn);
+  const float u = dot3(e2, dao) * inv_det;
+  const float v = -dot3(e1, dao) * inv_det;
+  const float st = dot3(ao, n) * inv_det;
+  const float t = (st * es) / ds;
+  const float w = (1.0f - u) - v;
+  if (nan_(u) || nan_(v) || nan_(w) || nan_(t) || fabs_(u) > cw_inf() || fabs_(v) > cw_inf() ||
+      fabs_(w) > cw_inf() || fabs_(t) > cw_inf() || u < -1e-6f || v < -1e-6f || w < -1e-6f ||
+      t < max2(tmin, 0.0f))
+    return -1.0f;
+  return t;
+}
+
+__device__ __forceinline__ V3 vload(const Scene& s, uint32_t i) {
+  const float4 p = s.positions[i];
+  return vtk::v3(p.x, p.y, p.z);
+}
+
+struct LocalHit {
+  bool hit;
+  float t;
+  V3 pos;
+  uint32_t tri;
+  float u, v;
+};
+
+// traceMeshBvh over the reference node layout (8 words per node), left child
+// first, closest hit.
+__device__ __noinline__ LocalHit mesh_binary(const Scene& s, V3 o, V3 d, V3 inv, const WtRay& w, float tmin,
+                                             float tmax, uint32_t root) {
+  LocalHit h{false, tmax, vtk::v3(0, 0, 0), 0u, 0.0f, 0.0f};
+  if (s.bvh_node_count == 0 || root >= s.bvh_node_count) return h;
+  uint32_t stack[kBinStack];
+  uint32_t sp = 0;
+  stack[sp++] = root;
+  while (sp > 0) {
+    const uint32_t ni = stack[--sp];
+    if (ni >= s.bvh_node_count) continue;
+    const uint4* nd = reinterpret_cast<const uint4*>(s.bvh_nodes + 8 * (size_t)ni);
+    const uint4 lo = nd[0], hi = nd[1];
+    float tn;
+    if (!vtk::slab(o, inv, vtk::u2f(lo.x), vtk::u2f(lo.y), vtk::u2f(lo.z), vtk::u2f(lo.w), vtk::u2f(hi.x),
+                   vtk::u2f(hi.y), tmin, tn) ||
+        tn > h.t)
+      continue;
+    if ((hi.w & kLeafMask) == kLeafMask) {
+      const uint32_t count = hi.w & 0xFFFFu, start = hi.z;
+      for (uint32_t i = 0; i < count; ++i) {
+        const uint32_t t = start + i;
+        if (t >= s.tri_count) continue;
+        const uint4 ix = s.indices[t];
+        if (ix.x >= s.vertex_count || ix.y >= s.vertex_count || ix.z >= s.vertex_count) continue;
+        TriHit th;
+        if (vtk::tri_test(o, d, w, vload(s, ix.x), vload(s, ix.y), vload(s, ix.z), tmin, h.t, false, th)) {
+          h.hit = true;
+          h.t = th.t;
+          h.pos = th.point;
+          h.tri = t;
+          h.u = th.u;
+          h.v = th.v;
+        }
+      }
+    } else {
+      if (sp + 2 > kBinStack) return h;
+      stack[sp++] = ni + hi.z;
+      stack[sp++] = ni + 1;
+    }
+  }
+  return h;
+}
+
+// Wide walk.  status: 0 complete, 1 stack overflow, 2 invalid layout.
+struct WideHit {
+  uint32_t status;
+  bool hit;
+  float t;
+  uint32_t tri;
+};
+
+__device__ __noinline__ WideHit wide_walk(const Scene& s, V3 o, V3 d, V3 inv, float tmin, float tmax,
+                                          uint32_t root) {
+  WideHit best{0u, false, tmax, 0u};
+  const uint32_t nc = s.wide_count;
+  if (nc == 0) return best;
+  if (root >= nc) {
+    best.status = 2;
+    return best;
+  }
+  uint32_t stack[kWideStack];
+  uint32_t sp = 0;
+  stack[sp++] = root;
+  while (sp > 0) {
+    const uint32_t node = stack[--sp];
+    if (node >= nc) {
+      best.status = 2;
+      return best;
+    }
+    const uint32_t count = s.child_count[node];
+    if (count > kKids) {
+      best.status = 2;
+      return best;
+    }
+    const float* pb = s.node_bounds + 6 * (size_t)node;
+    const V3 pmn = vtk::v3(pb[0], pb[1], pb[2]), pmx = vtk::v3(pb[3], pb[4], pb[5]);
+    if (!bounds_valid(pmn, pmx)) {
+      best.status = 2;
+      return best;
+    }
+    for (uint32_t slot = 0; slot < count; ++slot) {
+      const size_t k = ((size_t)node * kKids + slot) * 3;
+      const uint32_t kind = s.child_meta[k], ref = s.child_meta[k + 1], cnt = s.child_meta[k + 2];
+      if (kind == 0) {
+        best.status = 2;
+        return best;
+      }
+      const uint32_t q0 = s.child_bounds[k], q1 = s.child_bounds[k + 1], q2 = s.child_bounds[k + 2];
+      const V3 bmn = vtk::v3(decode(q0 & 0xFFFFu, pmn.x, pmx.x), decode(q0 >> 16, pmn.y, pmx.y),
+                             decode(q1 & 0xFFFFu, pmn.z, pmx.z));
+      const V3 bmx = vtk::v3(decode(q1 >> 16, pmn.x, pmx.x), decode(q2 & 0xFFFFu, pmn.y, pmx.y),
+                             decode(q2 >> 16, pmn.z, pmx.z));
+      if (!bounds_valid(bmn, bmx)) {
+        best.status = 2;
+        return best;
+      }
+      if (!aabb_entry(o, inv, bmn, bmx, best.t)) continue;
+      if (kind == 1) {
+        if (ref >= nc) {
+          best.status = 2;
+          return best;
+        }
+        if (sp >= kWideStack) {
+          best.status = 1;
+          return best;
+        }
+        stack[sp++] = ref;
+      } else if (kind == 2) {
+        if (cnt == 0 || ref > s.tri_count || cnt > s.tri_count - ref) {
+          best.status = 2;
+          return best;
+        }
+        for (uint32_t i = 0; i < cnt; ++i) {
+          const uint32_t tri = ref + i;
+          const uint4 ix = s.indices[tri];
+          if (ix.x >= s.vertex_count || ix.y >= s.vertex_count || ix.z >= s.vertex_count) {
+            best.status = 2;
+            return best;
+          }
+          const float t = mt_core(o, d, vload(s, ix.x), vload(s, ix.y), vload(s, ix.z), tmin);
+          if (t >= 0.0f && t > tmin && t < best.t) {
+            best.hit = true;
+            best.t = t;
+            best.tri = tri;
+          }
+        }
+      } else {
+        best.status = 2;
+        return best;
+      }
+    }
+  }
+  return best;
+}
+
+// traceMeshCwbvhClosest.
+__device__ __forceinline__ LocalHit mesh_cwbvh(const Scene& s, V3 o, V3 d, V3 inv, const WtRay& w, float tmin,
+                                               float tmax, uint32_t wide_root, uint32_t binary_root) {
+  if (s.wide_count == 0 || wide_root >= s.wide_count) return mesh_binary(s, o, d, inv, w, tmin, tmax, binary_root);
+  const WideHit c = wide_walk(s, o, d, inv, tmin, tmax, wide_root);
+  if (c.status != 0) return mesh_binary(s, o, d, inv, w, tmin, tmax, binary_root);
+  LocalHit h{false, tmax, vtk::v3(0, 0, 0), 0u, 0.0f, 0.0f};
+  if (!c.hit || c.t <= tmin || c.t >= tmax) return h;
+  if (c.tri >= s.tri_count) return mesh_binary(s, o, d, inv, w, tmin, tmax, binary_root);
+  const uint4 ix = s.indices[c.tri];
+  TriHit e;
+  if (!vtk::tri_test(o, d, w, vload(s, ix.x), vload(s, ix.y), vload(s, ix.z), tmin, vtk::inf_const(), true, e) ||
+      vtk::f2u(e.t) != vtk::f2u(c.t))
+    return mesh_binary(s, o, d, inv, w, tmin, tmax, binary_root);
+  h.hit = true;
+  h.t = c.t;
+  h.pos = e.point;
+  h.tri = c.tri;
+  h.u = e.u;
+  h.v = e.v;
+  return h;
+}
+
+__device__ __forceinline__ void load_cols(const float* m, float4& c0, float4& c1, float4& c2, float4& c3) {
+  const float4* q = reinterpret_cast<const float4*>(m);
+  c0 = q[0];
+  c1 = q[1];
+  c2 = q[2];
+  c3 = q[3];
+}
+
+struct WorldHit {
+  float t;
+  uint32_t prim, inst;
+  float u, v;
+  V3 pos;
+};
+
+// One instance of traceTlasClosestCwbvh / tlasInstanceClosest: the local ray,
+// its [ltmin, ltmax] window, the BLAS walk and the world-distance acceptance.
+__device__ __forceinline__ void instance_closest(const Scene& s, uint32_t inst, bool wide, V3 ro, V3 rd,
+                                                 float tmin, WorldHit& b) {
+  if (inst >= s.inst_count) return;
+  float4 c0, c1, c2, c3;
+  load_cols(s.w2l + 16 * (size_t)inst, c0, c1, c2, c3);
+  const V3 lo = vtk::xform_point(c0, c1, c2, c3, ro);
+  const V3 ld = vtk::xform_dir(c0, c1, c2, rd);
+  const V3 linv = vtk::v3(vtk::safe_inv(ld.x), vtk::safe_inv(ld.y), vtk::safe_inv(ld.z));
+  const V3 sp = vtk::v3(ro.x + rd.x * tmin, ro.y + rd.y * tmin, ro.z + rd.z * tmin);
+  const float ltmin = max2(vtk::dot3(vtk::sub(vtk::xform_point(c0, c1, c2, c3, sp), lo), ld), 0.0f);
+  float ltmax = vtk::inf_const();
+  if (b.t < vtk::inf_const() * 0.5f) {
+    const V3 ep = vtk::v3(ro.x + rd.x * b.t, ro.y + rd.y * b.t, ro.z + rd.z * b.t);
+    ltmax = max2(vtk::dot3(vtk::sub(vtk::xform_point(c0, c1, c2, c3, ep), lo), ld), ltmin);
+  }
+  const WtRay w = vtk::wt_setup(ld);
+  const LocalHit lh = wide ? mesh_cwbvh(s, lo, ld, linv, w, ltmin, ltmax, s.wide_roots[inst], s.blas_roots[inst])
+                           : mesh_binary(s, lo, ld, linv, w, ltmin, ltmax, s.blas_roots[inst]);
+  if (!lh.hit || !(lh.t > ltmin)) return;
+  load_cols(s.l2w + 16 * (size_t)inst, c0, c1, c2, c3);
+  const V3 wpos = vtk::xform_point(c0, c1, c2, c3, lh.pos);
+  const float wd = vtk::dot3(vtk::sub(wpos, ro), rd);
+  // intersection.wgsl.ts:811 rejects with `<=` / `>=` (a NaN distance is kept),
+  // unlike tlasInstanceClosest's `>` / `<` acceptance.
+  if (wide && (wd <= tmin || wd >= b.t)) return;
+  if (!wide && !(wd > tmin && wd < b.t)) return;
+  b.t = wd;
+  b.prim = lh.tri;
+  b.inst = inst;
+  b.u = lh.u;
+  b.v = lh.v;
+  b.pos = wpos;
+}
+
+__global__ __launch_bounds__(128) void trace_cwbvh_kernel(Params p) {
+  const Scene& s = p.s;
+  // With a device count, p.n is the capacity of the ray and hit buffers.
+  const uint32_t n = p.n_ptr ? min(*p.n_ptr, p.n) : p.n;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const float4 ra = p.rays[2 * (size_t)i], rb = p.rays[2 * (size_t)i + 1];
+    const V3 ro = vtk::v3(ra.x, ra.y, ra.z), rd = vtk::v3(rb.x, rb.y, rb.z);
+    const float tmin = ra.w, tmax = rb.w;
+    const V3 rinv = vtk::v3(vtk::safe_inv(rd.x), vtk::safe_inv(rd.y), vtk::safe_inv(rd.z));
+    WorldHit b;
+    // Mode 0: wide BLAS walks; 1: binary BLAS walks (a TLAS stack overflow in
+    // mode 0 restarts here, as traceTlasClosestCwbvh falls back to
+    // traceTlasClosest); 2: every instance in permutation order
+    // (tlasClosestFallback, on a TLAS stack overflow in mode 1).
+    uint32_t mode = (s.tlas_node_count == 0 || s.inst_index_count == 0) ? 3u : 0u;
+    for (;;) {
+      b.t = tmax;
+      b.prim = 0;
+      b.inst = vtk::kMiss;
+      b.u = b.v = 0.0f;
+      b.pos = vtk::v3(0, 0, 0);
+      if (mode == 3) break;  // no TLAS: fail closed
+      if (mode == 2) {
+        for (uint32_t pi = 0; pi < s.inst_index_count; ++pi)
+          instance_closest(s, s.inst_indices[pi], false, ro, rd, tmin, b);
+        break;
+      }
+      uint32_t stack[kTlasStack];
+      uint32_t sp = 0;
+      stack[sp++] = 0;
+      bool overflow = false;
+      while (sp > 0 && !overflow) {
+        const uint32_t ni = stack[--sp];
+        if (ni >= s.tlas_node_count) continue;
+        const uint4* nd = reinterpret_cast<const uint4*>(s.tlas_nodes + 8 * (size_t)ni);
+        const uint4 lo = nd[0], hi = nd[1];
+        float tn;
+        if (!vtk::slab(ro, rinv, vtk::u2f(lo.x), vtk::u2f(lo.y), vtk::u2f(lo.z), vtk::u2f(lo.w),
+                       vtk::u2f(hi.x), vtk::u2f(hi.y), tmin, tn) ||
+            tn > b.t)
+          continue;
+        if ((hi.w & kLeafMask) == kLeafMask) {
+          const uint32_t count = hi.w & 0xFFFFu, start = hi.z;
+          for (uint32_t k = 0; k < count; ++k) {
+            const uint32_t pi = start + k;
+            if (pi >= s.inst_index_count) continue;
+            instance_closest(s, s.inst_indices[pi], mode == 0, ro, rd, tmin, b);
+          }
+        } else if (sp + 2 <= kTlasStack) {
+          stack[sp++] = ni + hi.z;
+          stack[sp++] = ni + 1;
+        } else {
+          overflow = true;
+        }
+      }
+      if (!overflow) break;
+      ++mode;
+    }
+    const bool hit = b.inst != vtk::kMiss;
+    uint32_t* h = p.hits + 5 * (size_t)i;
+    h[0] = hit ? b.prim : vtk::kMiss;
+    h[1] = hit ? b.inst : vtk::kMiss;
+    h[2] = hit ? vtk::f2u(b.t) : 0x7F800000u;
+    h[3] = hit ? vtk::f2u(b.u) : 0u;
+    h[4] = hit ? vtk::f2u(b.v) : 0u;
+    if (p.hit_pos) {
+      float* hp = p.hit_pos + 3 * (size_t)i;
+      hp[0] = b.pos.x;
+      hp[1] = b.pos.y;
+      hp[2] = b.pos.z;
+    }
+  }
+}
+
+hipError_t launch_trace(const Params& p, uint32_t grid, hipStream_t s) {
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL(trace_cwbvh_kernel, dim3(grid), dim3(128), 0, s, p);
+  return hipGetLastError();
+}
+
+}  // namespace vtw

@@ -369,11 +369,18 @@ function nrcSettings(enabled, config) {
 async function createPTEngine_HIP(opts) {
   const o = Object.assign({}, opts || {});
   const known = new Set(['device', 'maxBounces', 'maxSamplesPerPixel', 'sampling', 'sceneRadius', 'debug',
-    'onWarning', 'denoiser', 'extensions', 'nrcEnabled', 'nrcConfig', 'lightTreeImportanceSampling']);
+    'onWarning', 'denoiser', 'extensions', 'nrcEnabled', 'nrcConfig', 'lightTreeImportanceSampling',
+    'bvhTraversal']);
   for (const k of Object.keys(o)) {
     if (!known.has(k)) throw new TypeError(`pt-hip: unknown option '${k}'`);
   }
   if (o.denoiser && o.denoiser !== 'none') throw new RangeError("pt-hip has no denoiser (use denoiser: 'none')");
+  // bvhTraversal (ptWebgpuValidation.ts:75,507): 'binary' (default) or
+  // 'cwbvh-closest'.
+  if (o.bvhTraversal !== undefined && o.bvhTraversal !== 'binary' && o.bvhTraversal !== 'cwbvh-closest') {
+    throw new RangeError(`pt-hip: bvhTraversal is unsupported (got ${JSON.stringify(o.bvhTraversal)}); ` +
+      'expected one of "binary", "cwbvh-closest"');
+  }
   if (o.nrcEnabled) nrcSettings(true, o.nrcConfig);  // validated before any device work
   const native = loadAddon();
   const handle = native.create(o.device == null ? 0 : o.device, {
@@ -381,6 +388,7 @@ async function createPTEngine_HIP(opts) {
     maxSamplesPerPixel: Number.isFinite(o.maxSamplesPerPixel) ? o.maxSamplesPerPixel : 0,
     sampling: o.sampling === 'sobol' ? 1 : 0,
     lightSelection: o.lightTreeImportanceSampling === false ? 1 : 0,
+    bvhTraversal: o.bvhTraversal === 'cwbvh-closest' ? 1 : 0,
   });
   const engine = new HipPathTracerEngine(native, handle, o);
   if (typeof o.onWarning === 'function') engine.onWarning(o.onWarning);

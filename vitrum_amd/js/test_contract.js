@@ -45,6 +45,7 @@ assert.throws(() => pt.sceneBlob({ primitives: [], environment: { kind: 'hdri', 
   await assert.rejects(pt.createPTEngine_HIP({ denoiser: 'oidn-final' }), RangeError);
   await assert.rejects(pt.createPTEngine_HIP({ nrcEnabled: true, nrcConfig: { levels: 16 } }), TypeError);
   await assert.rejects(pt.createPTEngine_HIP({ nrcEnabled: true, nrcConfig: { recordCap: 2 } }), RangeError);
+  await assert.rejects(pt.createPTEngine_HIP({ bvhTraversal: 'cwbvh' }), RangeError);
   if (process.env.VITRUM_HIP_GPU !== '1') {
     await assert.rejects(pt.createPTEngine_HIP({}), (e) => e.code === 'VT_ERR_NO_DEVICE' || e.code === 'VT_ERR_HIP');
     console.log('pt-hip contract ok (no device)');
@@ -91,6 +92,14 @@ assert.throws(() => pt.sceneBlob({ primitives: [], environment: { kind: 'hdri', 
   const uc = await uni.captureFrame();
   assert.ok(uc.rgba.every((x) => Number.isFinite(x) && x >= 0), 'uniform-selection output finite');
   uni.dispose();
+  // the cwbvh-closest traversal renders the same quad
+  const cw = await pt.createPTEngine_HIP({ device: 0, bvhTraversal: 'cwbvh-closest' });
+  cw.setScene({ primitives: [quad], emitters: [], environment: { kind: 'hdri', hdri: { constant: [1, 1, 1] } } });
+  cw.renderFrame({ viewMatrix: view, projMatrix: proj, viewport: { width: 16, height: 16, devicePixelRatio: 1 },
+    frameIndex: 0, frameSeed: 7, quality: { bounces: 4 } });
+  const cc = await cw.captureFrame();
+  assert.ok(cc.rgba.every((x) => Number.isFinite(x) && x >= 0), 'cwbvh-closest output finite');
+  cw.dispose();
   assert.ok(nc.rgba.every((x) => Number.isFinite(x) && x >= 0), 'cache output finite and non-negative');
   nrc.setNeuralCache(false);
   nrc.dispose();

@@ -208,6 +208,7 @@ napi_value Create(napi_env env, napi_callback_info info) {
     if (!get_u32(env, a[1], "maxSamplesPerPixel", 0, &o.max_samples_per_pixel)) return nullptr;
     if (!get_u32(env, a[1], "sampling", 0, &o.sampling)) return nullptr;
     if (!get_u32(env, a[1], "lightSelection", 0, &o.light_selection)) return nullptr;
+    if (!get_u32(env, a[1], "bvhTraversal", 0, &o.bvh_traversal)) return nullptr;
   }
   vt_engine* eng = nullptr;
   if (throw_status(env, nullptr, vt_create(dev, &o, &eng))) return nullptr;

@@ -31,6 +31,7 @@ EXPORTED = [
     "vt_build_blas", "vt_build_tlas", "vt_pack_scene", "vt_pack_field_words",
     "vt_pack_copy_field", "vt_pack_free", "vt_version", "vt_render_frame", "vt_capture",
     "vt_reset", "vt_frame_time", "vt_capture_signatures", "vt_render_info", "vt_light_tree", "vt_invert_mat4",
+    "vt_set_bvh_traversal",
     "vt_nrc_param_count", "vt_nrc_reset", "vt_nrc_set_weights", "vt_nrc_get_weights", "vt_nrc_infer",
     "vt_nrc_infer_device", "vt_nrc_train_step", "vt_nrc_train_step_device", "vt_nrc_last_loss",
     "vt_nrc_gradient", "vt_nrc_render_config", "vt_nrc_timing", "vt_batch_create", "vt_batch_destroy",
@@ -83,7 +84,8 @@ class SceneDesc(C.Structure):
 
 class Options(C.Structure):
     _fields_ = [("max_bounces", C.c_uint32), ("max_samples_per_pixel", C.c_uint32),
-                ("sampling", C.c_uint32), ("light_selection", C.c_uint32), ("reserved", C.c_uint32 * 4)]
+                ("sampling", C.c_uint32), ("light_selection", C.c_uint32), ("bvh_traversal", C.c_uint32),
+                ("reserved", C.c_uint32 * 3)]
 
 
 class Frame(C.Structure):
@@ -239,11 +241,13 @@ def _material(m) -> Material:
 class Engine:
     """Thin owner of a vt_engine (one per GPU device)."""
 
-    def __init__(self, device: int = 0, max_bounces: int = 8, sampling: int = 0, light_selection: int = 0):
+    def __init__(self, device: int = 0, max_bounces: int = 8, sampling: int = 0, light_selection: int = 0,
+                 bvh_traversal: int = 0):
         """sampling: 0 PCG, 1 Owen-scrambled Sobol; light_selection: 0 light
-        tree (two or more lights), 1 uniform."""
+        tree (two or more lights), 1 uniform; bvh_traversal: 0 binary BVH,
+        1 cwbvh-closest (8-wide compressed BVH for closest hits)."""
         self._h = C.c_void_p()
-        opts = Options(max_bounces, 0, sampling, light_selection)
+        opts = Options(max_bounces, 0, sampling, light_selection, bvh_traversal)
         check(lib().vt_create(C.c_int32(device), C.byref(opts), C.byref(self._h)))
 
     def close(self):
@@ -293,6 +297,10 @@ class Engine:
         out = np.zeros(rays.size, np.uint32)
         check(lib().vt_trace_any(self._h, rays.ctypes.data, rays.size, out.ctypes.data), self._h)
         return out
+
+    def set_bvh_traversal(self, mode):
+        """0: binary BVH; 1: cwbvh-closest (closest-hit rays only)."""
+        check(lib().vt_set_bvh_traversal(self._h, C.c_uint32(mode)), self._h)
 
     def set_trace_tuning(self, refill_min, steps, near_first=1, defer_min=16):
         check(lib().vt_set_trace_tuning(self._h, C.c_uint32(refill_min), C.c_uint32(steps),
