@@ -225,6 +225,15 @@ vt_status vt_trace_closest_counts(vt_engine *eng, const vt_ray *rays, uint32_t n
 vt_status vt_set_trace_tuning(vt_engine *eng, uint32_t refill_min, uint32_t steps, uint32_t near_first,
                               uint32_t defer_min);
 
+/* Node width of the binary-BVH traversal (no counterpart in the reference):
+ * 4 (default) walks 4-wide trees collapsed from the reference's binary BVH
+ * and TLAS (a binary node is replaced by its children only when their boxes
+ * lie inside its box, so every box test the walk makes is one the reference
+ * makes, with the same arithmetic, and the hit records are unchanged); 2
+ * walks the binary trees.  Scenes whose 4-wide trees could need more stack
+ * than the kernel holds, and the work-counting launches, always use 2. */
+vt_status vt_set_node_width(vt_engine *eng, uint32_t width);
+
 /* bvhTraversal (packages/pt-webgpu/src/index.ts option; opt-in
  * 'cwbvh-closest', packages/pt-webgpu/src/wgsl/pathTrace/intersection.wgsl.ts:
  * 576-838): 0 = binary, 1 = wide.  Applies to vt_trace_closest* and to the

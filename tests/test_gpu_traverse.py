@@ -225,3 +225,36 @@ def test_spheres_ids_match_reference_cpu_walk(engine):
     same = walk_agreement(got, ts)
     print(f"spheres: {same.mean():.6f} of {sample.size} rays agree with the CPU walk")
     assert same.mean() >= 0.999, int((~same).sum())
+
+
+def test_node_widths_agree(engine):
+    """The 4-wide trees collapsed from the binary BVH and TLAS (default) and
+    the binary trees give the same hit records on the 1M-triangle scene (a
+    262,144-ray sample of primary + bounce rays), closest and any hit; the
+    oracle comparison of the default width is in the tests above."""
+    sc = W.spheres_scene()
+    engine.set_scene(sc)
+    pack, ps = oracle_scene(sc)
+    rays, st = W.primary_rays(sc["camera"], 1024, 1024, scene_radius=sc["scene_radius"])
+    ph = engine.trace_closest(rays)
+    bounce = W.bounce_rays(pack, rays, ph, st, sc["scene_radius"])
+    allr = np.concatenate([rays, bounce])
+    sample = allr[np.sort(np.random.default_rng(3).choice(allr.size, 262144, replace=False))]
+    try:
+        engine.set_node_width(2)
+        h2, a2 = engine.trace_closest(sample), engine.trace_any(sample)
+        for near in (0,):
+            engine.set_trace_tuning(8, 16, near, 16)
+            h2n = engine.trace_closest(sample)
+            engine.set_node_width(4)
+            assert_hits_equal(engine.trace_closest(sample), h2n, f"near_first={near}")
+            engine.set_node_width(2)
+        engine.set_trace_tuning(8, 16, 1, 16)
+        engine.set_node_width(4)
+        assert_hits_equal(engine.trace_closest(sample), h2, "width 4 vs 2")
+        np.testing.assert_array_equal(engine.trace_any(sample), a2)
+        with pytest.raises(N.VtError):
+            engine.set_node_width(3)
+    finally:
+        engine.set_trace_tuning(8, 16, 1, 16)
+        engine.set_node_width(4)

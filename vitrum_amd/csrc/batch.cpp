@@ -23,6 +23,7 @@ struct vt_batch {
   DeviceBuffer table, offsets;
   std::vector<uint32_t> host_offsets;
   bool table_dirty = true;
+  bool wide = false;  // every scene's Node128 trees fit the kernel's stack
   // Launch parameters (the autotuner's choice).
   uint32_t refill_min = 8, steps = 16, near_first = 1, defer_min = 16, blocks_per_cu = 1;
   // Captured graph of (counter reset, batched trace).
@@ -53,18 +54,13 @@ static void drop_graph(vt_batch* b) {
 static vt_status upload_table(vt_batch* b) {
   if (!b->table_dirty) return VT_OK;
   std::vector<vtk::DeviceScene> t(b->scenes.size());
+  bool all_wide = true;
   for (size_t k = 0; k < t.size(); ++k) {
-    const SceneBuffers& s = *b->scenes[k];
-    const vt::DeviceLayout& L = b->layouts[k];
-    t[k].blas_nodes = (const float4*)s.blas.ptr;
-    t[k].tlas_nodes = (const float4*)s.tlas.ptr;
-    t[k].tris = (const float4*)s.tris.ptr;
-    t[k].instances = (const float4*)s.instances.ptr;
-    t[k].tlas_root = L.tlas_root;
-    t[k].tlas_node_count = L.tlas_node_count;
-    t[k].tri_count = L.tri_count;
-    t[k].inst_count = L.inst_count;
+    bool wide = false;
+    t[k] = engine_device_scene(*b->scenes[k], b->layouts[k], &wide);
+    all_wide = all_wide && wide;
   }
+  b->wide = all_wide;
   if (vt_status s = engine_upload(b->eng, b->table, t.data(), t.size() * sizeof(vtk::DeviceScene))) return s;
   b->table_dirty = false;
   return VT_OK;
@@ -100,6 +96,7 @@ static vtk::TraceParams batch_params(vt_batch* b, const vt_ray* d_rays, vt_hit* 
   p.steps = b->steps;
   p.near_first = b->near_first;
   p.defer_min = b->defer_min;
+  p.wide = (b->wide && e->node_width == 4) ? 1u : 0u;
   p.scenes = (const vtk::DeviceScene*)b->table.ptr;
   p.scene_offsets = (const uint32_t*)b->offsets.ptr;
   p.scene_count = (uint32_t)b->scenes.size();

@@ -1,6 +1,7 @@
 // device_layout.cpp -- converts a packed scene into the Node64 HBM layout.
 #include "device_layout.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <unordered_map>
@@ -69,6 +70,106 @@ std::string convert_forest(const std::vector<uint32_t>& ref_nodes, const std::ve
   return err;
 }
 
+struct Slot {
+  uint32_t ref;
+  float box[6];
+};
+
+inline float area(const float* b) {
+  const float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
+  return dx * dy + dy * dz + dz * dx;
+}
+
+inline bool inside(const float* c, const float* p) {
+  return c[0] >= p[0] && c[1] >= p[1] && c[2] >= p[2] && c[3] <= p[3] && c[4] <= p[4] && c[5] <= p[5];
+}
+
+// Collapses a Node64 forest into Node128s (see device_layout.h).
+class Collapser {
+ public:
+  Collapser(const std::vector<uint32_t>& n64, std::vector<uint32_t>& out) : n64_(n64), out_(out) {}
+
+  // Node128 index of the tree under Node64 `node`; `need` receives the
+  // worst-case stack entries of a depth-first walk of that tree.
+  uint32_t build(uint32_t node, uint32_t* need) {
+    std::vector<Slot> slots;
+    children(node, slots);
+    while (slots.size() < 4) {
+      int pick = -1;
+      float best = -1.0f;
+      std::vector<Slot> kids;
+      for (size_t i = 0; i < slots.size(); ++i) {
+        if (slots[i].ref & kRefLeaf) continue;
+        std::vector<Slot> k;
+        children(slots[i].ref, k);
+        if (slots.size() - 1 + k.size() > 4) continue;
+        bool nested = true;
+        for (const Slot& c : k) nested = nested && inside(c.box, slots[i].box);
+        const float a = area(slots[i].box);
+        if (!nested || !(a > best)) continue;
+        best = a;
+        pick = (int)i;
+        kids = k;
+      }
+      if (pick < 0) break;
+      slots.erase(slots.begin() + pick);
+      slots.insert(slots.begin() + pick, kids.begin(), kids.end());
+    }
+    const uint32_t idx = (uint32_t)(out_.size() / kNode128Words);
+    out_.resize(out_.size() + kNode128Words, 0u);
+    uint32_t deepest = 0;
+    for (size_t k = 0; k < 4; ++k) {
+      uint32_t ref = kEmptyRef;
+      float box[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+      if (k < slots.size()) {
+        ref = slots[k].ref;
+        std::memcpy(box, slots[k].box, 24);
+        if (!(ref & kRefLeaf)) {
+          uint32_t sub = 0;
+          ref = build(ref, &sub);
+          deepest = std::max(deepest, sub);
+        }
+      }
+      uint32_t* w = &out_[(size_t)idx * kNode128Words];
+      for (int a = 0; a < 6; ++a) w[4 * a + k] = fbits(box[a]);
+      w[24 + k] = ref;
+    }
+    // Expanding this node pushes at most (slots - 1) entries.
+    *need = (slots.empty() ? 0u : (uint32_t)slots.size() - 1u) + deepest;
+    return idx;
+  }
+
+ private:
+  void children(uint32_t node, std::vector<Slot>& out) const {
+    const uint32_t* w = &n64_[(size_t)node * kNode64Words];
+    for (int c = 0; c < 2; ++c) {
+      const uint32_t ref = w[12 + c];
+      if (ref == kEmptyRef) continue;
+      Slot s;
+      s.ref = ref;
+      std::memcpy(s.box, w + 6 * c, 24);
+      out.push_back(s);
+    }
+  }
+
+  const std::vector<uint32_t>& n64_;
+  std::vector<uint32_t>& out_;
+};
+
+// Worst-case stack entries of a depth-first walk of a Node64 tree: one push
+// per expansion with two passing children.
+uint32_t stack_need2(const std::vector<uint32_t>& n64, uint32_t node) {
+  const uint32_t* w = &n64[(size_t)node * kNode64Words];
+  uint32_t deepest = 0, kids = 0;
+  for (int c = 0; c < 2; ++c) {
+    const uint32_t ref = w[12 + c];
+    if (ref == kEmptyRef) continue;
+    ++kids;
+    if (!(ref & kRefLeaf)) deepest = std::max(deepest, stack_need2(n64, ref));
+  }
+  return (kids > 0 ? kids - 1 : 0) + deepest;
+}
+
 }  // namespace
 
 std::string build_device_layout(const Pack& p, DeviceLayout& out) {
@@ -104,6 +205,30 @@ std::string build_device_layout(const Pack& p, DeviceLayout& out) {
     if (it == root_slot.end()) return "instance references an unknown BLAS root";
     w[32] = blas_wrapper[it->second];
     w[33] = inst;
+  }
+  // Node128 trees: one per BLAS wrapper, one for the TLAS.
+  std::vector<uint32_t> blas4_root(blas_wrapper.size(), 0);
+  uint32_t blas_need4 = 0, blas_need2 = 0;
+  {
+    Collapser cb(out.blas_nodes, out.blas4_nodes);
+    for (size_t k = 0; k < blas_wrapper.size(); ++k) {
+      uint32_t need = 0;
+      blas4_root[k] = cb.build(blas_wrapper[k], &need);
+      blas_need4 = std::max(blas_need4, need);
+      blas_need2 = std::max(blas_need2, stack_need2(out.blas_nodes, blas_wrapper[k]));
+    }
+  }
+  uint32_t tlas_need4 = 0, tlas_need2 = 0;
+  if (n_inst > 0) {
+    Collapser ct(out.tlas_nodes, out.tlas4_nodes);
+    out.tlas4_root = ct.build(out.tlas_root, &tlas_need4);
+    tlas_need2 = stack_need2(out.tlas_nodes, out.tlas_root);
+  }
+  out.stack4_need = tlas_need4 + blas_need4;
+  out.stack2_need = tlas_need2 + blas_need2;
+  for (uint32_t s = 0; s < (uint32_t)p.tlas_instance_indices.size(); ++s) {
+    const uint32_t inst = p.tlas_instance_indices[s];
+    out.instances[(size_t)s * kInstWords + 34] = blas4_root[root_slot[p.tlas_blas_roots[inst]]];
   }
   const uint32_t n_tri = (uint32_t)(p.indices.size() / 4);
   out.tri_count = n_tri;

@@ -24,7 +24,23 @@
 //
 // Instance record (144 bytes, TLAS permutation order): world-to-local
 // columns (4 x float4), local-to-world columns (4 x float4), then
-// {blas wrapper node, packed instance index, 0, 0}.
+// {blas wrapper node, packed instance index, blas Node128 root, 0}.
+//
+// Node128 (128 bytes, 4-wide): the same trees with levels collapsed.  A
+// Node128 made from a binary node holds up to four descendants in
+// depth-first order (a child is replaced by its two children, largest box
+// first, while fewer than four slots are used), stored axis-major so a
+// traversal step loads each coordinate of the four boxes as one float4:
+//
+//   words 0-3 min x, 4-7 min y, 8-11 min z, 12-15 max x, 16-19 max y,
+//   words 20-23 max z, 24-27 child refs (Node128 index or leaf ref),
+//   words 28-31 reserved (0); unused slots hold kEmptyRef and an empty box.
+//
+// A binary node is only replaced by its children when both child boxes lie
+// inside its box, so a ray that reaches a child's box test reaches it in
+// the binary tree too, and the boxes of the replaced nodes need no test of
+// their own (each slab-test bound of a parent is implied by the child's, as
+// the slab arithmetic is monotonic in the box coordinates).
 #pragma once
 
 #include <cstdint>
@@ -42,6 +58,7 @@ constexpr uint32_t kMaxLeafOffsetInRef = (1u << 27) - 1;
 constexpr uint32_t kNode64Words = 16;
 constexpr uint32_t kTriWords = 12;
 constexpr uint32_t kInstWords = 36;
+constexpr uint32_t kNode128Words = 32;
 
 struct DeviceLayout {
   std::vector<uint32_t> blas_nodes;   // Node64 words
@@ -50,6 +67,12 @@ struct DeviceLayout {
   std::vector<uint32_t> instances;    // kInstWords per instance slot
   uint32_t tlas_root = 0;             // wrapper Node64 index in tlas_nodes
   uint32_t blas_node_count = 0, tlas_node_count = 0, tri_count = 0, inst_count = 0;
+  std::vector<uint32_t> blas4_nodes;  // Node128 words
+  std::vector<uint32_t> tlas4_nodes;  // Node128 words
+  uint32_t tlas4_root = 0;
+  // Worst-case traversal stack entries (TLAS part + deepest BLAS part) of the
+  // Node128 trees, and of the Node64 trees.
+  uint32_t stack4_need = 0, stack2_need = 0;
 };
 
 // Returns an empty string on success, else the reason the pack cannot be

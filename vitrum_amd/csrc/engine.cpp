@@ -99,6 +99,8 @@ vt_status engine_upload(vt_engine* e, DeviceBuffer& dst, const void* src, size_t
 void SceneBuffers::release() {
   blas.release();
   tlas.release();
+  blas4.release();
+  tlas4.release();
   tris.release();
   instances.release();
   positions.release();
@@ -106,6 +108,49 @@ void SceneBuffers::release() {
   indices.release();
   tri_material.release();
   materials.release();
+}
+
+vt_status engine_upload_trace(vt_engine* e, const vt::DeviceLayout& layout, SceneBuffers& out) {
+  vt_status s = VT_OK;
+  s = s ? s : engine_upload(e, out.blas, layout.blas_nodes.data(), layout.blas_nodes.size() * 4);
+  s = s ? s : engine_upload(e, out.tlas, layout.tlas_nodes.data(), layout.tlas_nodes.size() * 4);
+  s = s ? s : engine_upload(e, out.blas4, layout.blas4_nodes.data(), layout.blas4_nodes.size() * 4);
+  s = s ? s : engine_upload(e, out.tlas4, layout.tlas4_nodes.data(), layout.tlas4_nodes.size() * 4);
+  s = s ? s : engine_upload(e, out.tris, layout.tris.data(), layout.tris.size() * 4);
+  s = s ? s : engine_upload(e, out.instances, layout.instances.data(), layout.instances.size() * 4);
+  return s;
+}
+
+vtk::DeviceScene engine_device_scene(const SceneBuffers& b, const vt::DeviceLayout& L, bool* wide) {
+  vtk::DeviceScene d{};
+  d.blas_nodes = (const float4*)b.blas.ptr;
+  d.tlas_nodes = (const float4*)b.tlas.ptr;
+  d.tris = (const float4*)b.tris.ptr;
+  d.instances = (const float4*)b.instances.ptr;
+  d.tlas_root = L.tlas_root;
+  d.tlas_node_count = L.tlas_node_count;
+  d.tri_count = L.tri_count;
+  d.inst_count = L.inst_count;
+  d.blas4_nodes = (const float4*)b.blas4.ptr;
+  d.tlas4_nodes = (const float4*)b.tlas4.ptr;
+  d.tlas4_root = L.tlas4_root;
+  if (wide) *wide = !L.blas4_nodes.empty() && L.stack4_need <= vtk::kStackCapacity;
+  return d;
+}
+
+vtk::TraceParams engine_trace_params(vt_engine* e) {
+  vtk::TraceParams p{};
+  bool wide = false;
+  p.scene = engine_device_scene(e->scene, *e->layout, &wide);
+  p.wide = (wide && e->node_width == 4) ? 1u : 0u;
+  p.counter = (uint32_t*)e->counter.ptr;
+  p.error = (uint32_t*)e->error_word.ptr;
+  p.spill = (uint2*)e->spill.ptr;
+  p.refill_min = e->refill_min;
+  p.steps = e->trace_steps;
+  p.near_first = e->near_first;
+  p.defer_min = e->defer_min;
+  return p;
 }
 
 // ---------------------------------------------------------------------------
@@ -127,11 +172,7 @@ vt_status engine_build_trace_scene(vt_engine* eng, const vt_scene_desc* scene, S
   std::string le = vt::build_device_layout(pack, layout);
   if (!le.empty()) return set_err(eng, VT_ERR_RANGE, le);
   HIP_TRY(eng, hipSetDevice(eng->device));
-  vt_status s = VT_OK;
-  s = s ? s : engine_upload(eng, out.blas, layout.blas_nodes.data(), layout.blas_nodes.size() * 4);
-  s = s ? s : engine_upload(eng, out.tlas, layout.tlas_nodes.data(), layout.tlas_nodes.size() * 4);
-  s = s ? s : engine_upload(eng, out.tris, layout.tris.data(), layout.tris.size() * 4);
-  s = s ? s : engine_upload(eng, out.instances, layout.instances.data(), layout.instances.size() * 4);
+  vt_status s = engine_upload_trace(eng, layout, out);
   if (!s) {
     hipError_t r = hipStreamSynchronize(eng->stream);
     if (r != hipSuccess) s = set_err(eng, VT_ERR_HIP, hipGetErrorString(r));
@@ -240,11 +281,7 @@ vt_status vt_set_scene(vt_engine* eng, const vt_scene_desc* scene) {
   HIP_TRY(eng, hipSetDevice(eng->device));
   // Upload into fresh buffers, then swap: a failure leaves the old scene bound.
   SceneBuffers nb;
-  vt_status s = VT_OK;
-  s = s ? s : engine_upload(eng, nb.blas, layout->blas_nodes.data(), layout->blas_nodes.size() * 4);
-  s = s ? s : engine_upload(eng, nb.tlas, layout->tlas_nodes.data(), layout->tlas_nodes.size() * 4);
-  s = s ? s : engine_upload(eng, nb.tris, layout->tris.data(), layout->tris.size() * 4);
-  s = s ? s : engine_upload(eng, nb.instances, layout->instances.data(), layout->instances.size() * 4);
+  vt_status s = engine_upload_trace(eng, *layout, nb);
   s = s ? s : engine_upload(eng, nb.positions, pack->positions.data(), pack->positions.size() * 4);
   s = s ? s : engine_upload(eng, nb.normals, pack->normals.data(), pack->normals.size() * 4);
   s = s ? s : engine_upload(eng, nb.indices, pack->indices.data(), pack->indices.size() * 4);
@@ -357,28 +394,12 @@ static vt_status trace_device(vt_engine* eng, const vt_ray* d_rays, uint32_t n, 
   if (n && (!d_rays || !d_out)) return set_err(eng, VT_ERR_INVALID_ARGUMENT, "NULL buffer");
   if (n > 0x7FFFFFFFu) return set_err(eng, VT_ERR_RANGE, "too many rays");
   HIP_TRY(eng, hipSetDevice(eng->device));
-  vtk::TraceParams p{};
-  const vt::DeviceLayout& L = *eng->layout;
-  p.scene.blas_nodes = (const float4*)eng->scene.blas.ptr;
-  p.scene.tlas_nodes = (const float4*)eng->scene.tlas.ptr;
-  p.scene.tris = (const float4*)eng->scene.tris.ptr;
-  p.scene.instances = (const float4*)eng->scene.instances.ptr;
-  p.scene.tlas_root = L.tlas_root;
-  p.scene.tlas_node_count = L.tlas_node_count;
-  p.scene.tri_count = L.tri_count;
-  p.scene.inst_count = L.inst_count;
+  vtk::TraceParams p = engine_trace_params(eng);
   p.rays = (const float4*)d_rays;
   p.hits = any ? nullptr : (uint32_t*)d_out;
   p.occluded = any ? (uint32_t*)d_out : nullptr;
   p.counts = d_counts;
   p.n = n;
-  p.counter = (uint32_t*)eng->counter.ptr;
-  p.error = (uint32_t*)eng->error_word.ptr;
-  p.spill = (uint2*)eng->spill.ptr;
-  p.refill_min = eng->refill_min;
-  p.steps = eng->trace_steps;
-  p.near_first = eng->near_first;
-  p.defer_min = eng->defer_min;
   eng->last_ray_count = n;
   if (n == 0) return VT_OK;
   // The cwbvh-closest mode covers closest hits; any-hit rays and the work
@@ -606,3 +627,10 @@ vt_status vt_pack_copy_field(const vt_pack* p, int32_t field, void* dst, uint64_
 void vt_pack_free(vt_pack* p) { delete p; }
 
 }  // extern "C"
+
+vt_status vt_set_node_width(vt_engine* eng, uint32_t width) {
+  if (!eng) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "engine is NULL");
+  if (width != 2 && width != 4) return set_err(eng, VT_ERR_RANGE, "node width must be 2 or 4");
+  eng->node_width = width;
+  return VT_OK;
+}

@@ -10,6 +10,7 @@
 #include "../../include/vitrum_hip.h"
 #include "builder.h"
 #include "device_layout.h"
+#include "traverse.h"
 
 struct DeviceBuffer {
   void* ptr = nullptr;
@@ -21,6 +22,7 @@ struct DeviceBuffer {
 
 struct SceneBuffers {
   DeviceBuffer blas, tlas, tris, instances;          // traversal layout
+  DeviceBuffer blas4, tlas4;                         // Node128 trees
   DeviceBuffer positions, normals, indices, tri_material, materials;  // shading data
   void release();
 };
@@ -43,6 +45,7 @@ struct vt_engine {
   uint32_t trace_steps = 16;
   uint32_t near_first = 1;
   uint32_t defer_min = 16;
+  uint32_t node_width = 4;  // 4: Node128 trees where the stack bound allows, 2: Node64
   bool counting = false;
   DeviceBuffer counter, error_word, spill;
   DeviceBuffer tmp_rays, tmp_out, tmp_counts;
@@ -64,6 +67,14 @@ struct vt_engine {
 };
 
 vt_status engine_upload(vt_engine* e, DeviceBuffer& dst, const void* src, size_t bytes);
+// Uploads the traversal buffers of a device layout (Node64 + Node128 trees,
+// triangles, instances).
+vt_status engine_upload_trace(vt_engine* e, const vt::DeviceLayout& layout, SceneBuffers& out);
+// Device view of one scene's traversal buffers; `wide` reports whether its
+// Node128 trees can be walked (their worst-case stack fits the kernel's).
+vtk::DeviceScene engine_device_scene(const SceneBuffers& b, const vt::DeviceLayout& layout, bool* wide);
+// Fills the scene and scheduling fields of a launch for the engine's scene.
+vtk::TraceParams engine_trace_params(vt_engine* e);
 // Packs, builds and uploads one scene's traversal buffers (engine.cpp).
 vt_status engine_build_trace_scene(vt_engine* e, const vt_scene_desc* scene, SceneBuffers& out,
                                    vt::DeviceLayout& layout);

@@ -370,26 +370,7 @@ static vt_status ensure_paths(vt_engine* e, uint32_t w, uint32_t h) {
   return VT_OK;
 }
 
-static vtk::TraceParams trace_params(vt_engine* e) {
-  vtk::TraceParams p{};
-  const vt::DeviceLayout& L = *e->layout;
-  p.scene.blas_nodes = (const float4*)e->scene.blas.ptr;
-  p.scene.tlas_nodes = (const float4*)e->scene.tlas.ptr;
-  p.scene.tris = (const float4*)e->scene.tris.ptr;
-  p.scene.instances = (const float4*)e->scene.instances.ptr;
-  p.scene.tlas_root = L.tlas_root;
-  p.scene.tlas_node_count = L.tlas_node_count;
-  p.scene.tri_count = L.tri_count;
-  p.scene.inst_count = L.inst_count;
-  p.counter = (uint32_t*)e->counter.ptr;
-  p.error = (uint32_t*)e->error_word.ptr;
-  p.spill = (uint2*)e->spill.ptr;
-  p.refill_min = e->refill_min;
-  p.steps = e->trace_steps;
-  p.near_first = e->near_first;
-  p.defer_min = e->defer_min;
-  return p;
-}
+static vtk::TraceParams trace_params(vt_engine* e) { return engine_trace_params(e); }
 
 extern "C" {
 

@@ -20,6 +20,10 @@ struct DeviceScene {
   uint32_t tlas_node_count;
   uint32_t tri_count;
   uint32_t inst_count;
+  const float4* blas4_nodes;  // Node128, 8 x float4 each (null: use the Node64 trees)
+  const float4* tlas4_nodes;
+  uint32_t tlas4_root;
+  uint32_t reserved;
 };
 
 struct TraceParams {
@@ -38,16 +42,20 @@ struct TraceParams {
   uint32_t steps;         // traversal steps between refill checks
   uint32_t near_first;    // closest hit: visit the nearer child first (ties as the reference)
   uint32_t defer_min;     // run a leaf / instance operation once this many lanes need it
+  uint32_t wide;          // 1: walk the Node128 trees (every scene must have them)
   // Batched launch: rays [offsets[k], offsets[k+1]) belong to scenes[k].
   const DeviceScene* scenes;
   const uint32_t* scene_offsets;
   uint32_t scene_count;
 };
 
+// Closest (any = false) or any hit; `count` selects the work-counting variant
+// (Node64 trees, the reference's visiting order).  Otherwise the Node128
+// trees are walked when p.wide is set.
 hipError_t launch_trace(const TraceParams& p, bool any, bool count, uint32_t grid, hipStream_t s);
 // Closest hit over a batch of scenes (p.scenes / p.scene_offsets / p.scene_count).
 hipError_t launch_trace_batch(const TraceParams& p, uint32_t grid, hipStream_t s);
-hipError_t trace_occupancy(bool any, int* blocks_per_cu);
+hipError_t trace_occupancy(bool any, int* blocks_per_cu);  // the smallest over the kernels launched
 hipError_t trace_occupancy_batch(int* blocks_per_cu);
 hipError_t launch_zero(uint32_t* p, uint32_t n, hipStream_t s);
 // Streaming copy of n16 16-byte words.

@@ -27,6 +27,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <initializer_list>
 
 #include "traverse.h"
 #include "cwbvh.h"
@@ -312,7 +313,7 @@ __device__ __forceinline__ V3 xform_dir(float4 c0, float4 c1, float4 c2, V3 d) {
 // instance geometry and exists for measurement only.  Node culling stays
 // `tNear > best` (equal distances are still visited).  The counting variant
 // always uses the reference order, so its work counts are the reference's.
-template <bool ANY, bool COUNT, bool MULTI, uint32_t BLOCK, uint32_t LDS_DEPTH>
+template <bool ANY, bool COUNT, bool MULTI, bool W4, uint32_t BLOCK, uint32_t LDS_DEPTH>
 // Register budget: four waves per SIMD (128 VGPRs; with the 20-entry LDS
 // stack, 8 blocks of 128 fit a CU), which beats three waves without spills.
 __global__ __launch_bounds__(BLOCK, 4) void trace_kernel(TraceParams p) {
@@ -339,11 +340,11 @@ __global__ __launch_bounds__(BLOCK, 4) void trace_kernel(TraceParams p) {
   };
   // Scene of the current ray: the launch's scene, or (batched) the one whose
   // ray range holds the ray, looked up when the lane takes a new ray.
-  const float4* __restrict__ blas = p.scene.blas_nodes;
-  const float4* __restrict__ tlas = p.scene.tlas_nodes;
+  const float4* __restrict__ blas = W4 ? p.scene.blas4_nodes : p.scene.blas_nodes;
+  const float4* __restrict__ tlas = W4 ? p.scene.tlas4_nodes : p.scene.tlas_nodes;
   const float4* __restrict__ tris = p.scene.tris;
   const float4* __restrict__ insts = p.scene.instances;
-  uint32_t tlas_root = p.scene.tlas_root, tlas_count = p.scene.tlas_node_count;
+  uint32_t tlas_root = W4 ? p.scene.tlas4_root : p.scene.tlas_root, tlas_count = p.scene.tlas_node_count;
   const uint32_t lane = __lane_id();
   const uint64_t lane_below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const bool near = !ANY && !COUNT && p.near_first != 0u;        // inside BLASes
@@ -393,7 +394,7 @@ __global__ __launch_bounds__(BLOCK, 4) void trace_kernel(TraceParams p) {
     inst_id = meta.y;
     level = 1;
     blas_base = sp;
-    cur = meta.x;
+    cur = W4 ? meta.z : meta.x;
     if (COUNT) ++c_inst;
   };
 
@@ -469,11 +470,11 @@ __global__ __launch_bounds__(BLOCK, 4) void trace_kernel(TraceParams p) {
               if (p.scene_offsets[mid] <= idx) lo_s = mid; else hi_s = mid;
             }
             const DeviceScene& S = p.scenes[lo_s];
-            blas = S.blas_nodes;
-            tlas = S.tlas_nodes;
+            blas = W4 ? S.blas4_nodes : S.blas_nodes;
+            tlas = W4 ? S.tlas4_nodes : S.tlas_nodes;
             tris = S.tris;
             insts = S.instances;
-            tlas_root = S.tlas_root;
+            tlas_root = W4 ? S.tlas4_root : S.tlas_root;
             tlas_count = S.tlas_node_count;
           }
           const float4 a = p.rays[2 * (size_t)idx], b = p.rays[2 * (size_t)idx + 1];
@@ -589,7 +590,81 @@ __global__ __launch_bounds__(BLOCK, 4) void trace_kernel(TraceParams p) {
         }
         if (!active || pend != kPendNone) continue;
       }
-      if ((cur & kLeafBit) == 0) {
+      if (W4 && (cur & kLeafBit) == 0) {
+        // ---- interior: one Node128, four child boxes tested axis by axis ----
+        const float4* nd = (level ? blas : tlas) + 8 * (size_t)cur;
+        const V3 o = level ? lo : ro;
+        const V3 inv = level ? linv : rinv;
+        const float tm = level ? ltmin : tmin;
+        const float bound = level ? lbest : (ANY ? tmax : best);
+        float4 a = nd[0], b = nd[3];
+        float n0, n1, n2, n3, f0, f1, f2, f3;
+        {
+          const float p0 = (a.x - o.x) * inv.x, q0 = (b.x - o.x) * inv.x;
+          const float p1 = (a.y - o.x) * inv.x, q1 = (b.y - o.x) * inv.x;
+          const float p2 = (a.z - o.x) * inv.x, q2 = (b.z - o.x) * inv.x;
+          const float p3 = (a.w - o.x) * inv.x, q3 = (b.w - o.x) * inv.x;
+          n0 = fminf(p0, q0), f0 = fmaxf(p0, q0);
+          n1 = fminf(p1, q1), f1 = fmaxf(p1, q1);
+          n2 = fminf(p2, q2), f2 = fmaxf(p2, q2);
+          n3 = fminf(p3, q3), f3 = fmaxf(p3, q3);
+        }
+        a = nd[1], b = nd[4];
+        {
+          const float p0 = (a.x - o.y) * inv.y, q0 = (b.x - o.y) * inv.y;
+          const float p1 = (a.y - o.y) * inv.y, q1 = (b.y - o.y) * inv.y;
+          const float p2 = (a.z - o.y) * inv.y, q2 = (b.z - o.y) * inv.y;
+          const float p3 = (a.w - o.y) * inv.y, q3 = (b.w - o.y) * inv.y;
+          n0 = fmaxf(n0, fminf(p0, q0)), f0 = fminf(f0, fmaxf(p0, q0));
+          n1 = fmaxf(n1, fminf(p1, q1)), f1 = fminf(f1, fmaxf(p1, q1));
+          n2 = fmaxf(n2, fminf(p2, q2)), f2 = fminf(f2, fmaxf(p2, q2));
+          n3 = fmaxf(n3, fminf(p3, q3)), f3 = fminf(f3, fmaxf(p3, q3));
+        }
+        a = nd[2], b = nd[5];
+        {
+          const float p0 = (a.x - o.z) * inv.z, q0 = (b.x - o.z) * inv.z;
+          const float p1 = (a.y - o.z) * inv.z, q1 = (b.y - o.z) * inv.z;
+          const float p2 = (a.z - o.z) * inv.z, q2 = (b.z - o.z) * inv.z;
+          const float p3 = (a.w - o.z) * inv.z, q3 = (b.w - o.z) * inv.z;
+          n0 = fmaxf(n0, fminf(p0, q0)), f0 = fminf(f0, fmaxf(p0, q0));
+          n1 = fmaxf(n1, fminf(p1, q1)), f1 = fminf(f1, fmaxf(p1, q1));
+          n2 = fmaxf(n2, fminf(p2, q2)), f2 = fminf(f2, fmaxf(p2, q2));
+          n3 = fmaxf(n3, fminf(p3, q3)), f3 = fminf(f3, fmaxf(p3, q3));
+        }
+        const uint4 r = *reinterpret_cast<const uint4*>(nd + 6);
+        // The slab test of each child: !(tNear > tFar || tFar < tMin), then
+        // the bound (the pop-time cull re-checks it against the best hit).
+        const bool k0 = r.x != kEmpty && !(n0 > f0 || f0 < tm) && !(n0 > bound);
+        const bool k1 = r.y != kEmpty && !(n1 > f1 || f1 < tm) && !(n1 > bound);
+        const bool k2 = r.z != kEmpty && !(n2 > f2 || f2 < tm) && !(n2 > bound);
+        const bool k3 = r.w != kEmpty && !(n3 > f3 || f3 < tm) && !(n3 > bound);
+        if (COUNT) {
+          const uint32_t tested = (r.x != kEmpty) + (r.y != kEmpty) + (r.z != kEmpty) + (r.w != kEmpty);
+          if (level) c_blas += tested; else c_tlas += tested;
+        }
+        // Visit first: the nearest passing child (inside BLASes with
+        // near_first, lowest slot on equal distances), else the first passing
+        // slot, which is the reference's depth-first order.  The others are
+        // pushed so that they pop in slot order.
+        uint32_t first = k0 ? 0u : (k1 ? 1u : (k2 ? 2u : (k3 ? 3u : 4u)));
+        if ((level ? near : near_tlas) && first < 4u) {
+          float bt = n0;
+          bt = first == 1u ? n1 : (first == 2u ? n2 : (first == 3u ? n3 : bt));
+          if (k1 && n1 < bt) first = 1u, bt = n1;
+          if (k2 && n2 < bt) first = 2u, bt = n2;
+          if (k3 && n3 < bt) first = 3u, bt = n3;
+        }
+        bool ok = true;
+        if (k3 && first != 3u) ok = push(r.w, n3);
+        if (ok && k2 && first != 2u) ok = push(r.z, n2);
+        if (ok && k1 && first != 1u) ok = push(r.y, n1);
+        if (ok && k0 && first != 0u) ok = push(r.x, n0);
+        if (!ok) {
+          finish(false);
+          continue;
+        }
+        cur = first == 0u ? r.x : (first == 1u ? r.y : (first == 2u ? r.z : (first == 3u ? r.w : kNone)));
+      } else if (!W4 && (cur & kLeafBit) == 0) {
         // ---- interior: one Node64 load, two child boxes ----
         const float4* nd = (level ? blas : tlas) + 4 * (size_t)cur;
         const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2], q3 = nd[3];
@@ -658,35 +733,59 @@ __global__ void fp32_selftest_kernel(const float* a, const float* b, float* q, f
 
 namespace vtk {
 
-template <bool ANY, bool COUNT, bool MULTI = false>
+template <bool ANY, bool COUNT, bool MULTI, bool W4>
 static hipError_t launch_one(const TraceParams& p, uint32_t grid, hipStream_t s) {
-  hipLaunchKernelGGL((trace_kernel<ANY, COUNT, MULTI, kTraceBlock, kLdsDepth>), dim3(grid),
+  hipLaunchKernelGGL((trace_kernel<ANY, COUNT, MULTI, W4, kTraceBlock, kLdsDepth>), dim3(grid),
                      dim3(kTraceBlock), 0, s, p);
   return hipGetLastError();
 }
 
 hipError_t launch_trace_batch(const TraceParams& p, uint32_t grid, hipStream_t s) {
   hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, p.counter, 1u);
-  return launch_one<false, false, true>(p, grid, s);
+  return p.wide ? launch_one<false, false, true, true>(p, grid, s)
+                             : launch_one<false, false, true, false>(p, grid, s);
 }
 
 hipError_t launch_trace(const TraceParams& p, bool any, bool count, uint32_t grid, hipStream_t s) {
   hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, p.counter, 1u);
-  if (any) return launch_one<true, false>(p, grid, s);
-  return count ? launch_one<false, true>(p, grid, s) : launch_one<false, false>(p, grid, s);
+  if (count) return launch_one<false, true, false, false>(p, grid, s);
+  const bool w4 = p.wide != 0;
+  if (any) return w4 ? launch_one<true, false, false, true>(p, grid, s) : launch_one<true, false, false, false>(p, grid, s);
+  return w4 ? launch_one<false, false, false, true>(p, grid, s) : launch_one<false, false, false, false>(p, grid, s);
+}
+
+template <bool ANY, bool COUNT, bool MULTI, bool W4>
+static int occupancy_of() {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, trace_kernel<ANY, COUNT, MULTI, W4, kTraceBlock, kLdsDepth>,
+                                                   kTraceBlock, 0) != hipSuccess)
+    return -1;
+  return n;
+}
+
+static hipError_t min_occupancy(std::initializer_list<int> v, int* out) {
+  int m = 1 << 30;
+  for (int x : v) {
+    if (x < 0) return hipErrorInvalidValue;
+    m = x < m ? x : m;
+  }
+  *out = m;
+  return hipSuccess;
 }
 
 hipError_t trace_occupancy_batch(int* blocks_per_cu) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      blocks_per_cu, trace_kernel<false, false, true, kTraceBlock, kLdsDepth>, kTraceBlock, 0);
+  return min_occupancy({occupancy_of<false, false, true, false>(), occupancy_of<false, false, true, true>()},
+                       blocks_per_cu);
 }
 
 hipError_t trace_occupancy(bool any, int* blocks_per_cu) {
   if (any)
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        blocks_per_cu, trace_kernel<true, false, false, kTraceBlock, kLdsDepth>, kTraceBlock, 0);
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      blocks_per_cu, trace_kernel<false, false, false, kTraceBlock, kLdsDepth>, kTraceBlock, 0);
+    return min_occupancy({occupancy_of<true, false, false, false>(), occupancy_of<true, false, false, true>()},
+                         blocks_per_cu);
+  // The counting variant (measurement only) is left out: its extra blocks
+  // simply start as others retire.
+  return min_occupancy({occupancy_of<false, false, false, false>(), occupancy_of<false, false, false, true>()},
+                       blocks_per_cu);
 }
 
 hipError_t launch_zero(uint32_t* p, uint32_t n, hipStream_t s) {
