@@ -94,7 +94,7 @@ class Collapser {
   uint32_t build(uint32_t node, uint32_t* need) {
     std::vector<Slot> slots;
     children(node, slots);
-    while (slots.size() < 4) {
+    while (slots.size() < kWideKids) {
       int pick = -1;
       float best = -1.0f;
       std::vector<Slot> kids;
@@ -102,7 +102,7 @@ class Collapser {
         if (slots[i].ref & kRefLeaf) continue;
         std::vector<Slot> k;
         children(slots[i].ref, k);
-        if (slots.size() - 1 + k.size() > 4) continue;
+        if (slots.size() - 1 + k.size() > kWideKids) continue;
         bool nested = true;
         for (const Slot& c : k) nested = nested && inside(c.box, slots[i].box);
         const float a = area(slots[i].box);
@@ -118,7 +118,7 @@ class Collapser {
     const uint32_t idx = (uint32_t)(out_.size() / kNode128Words);
     out_.resize(out_.size() + kNode128Words, 0u);
     uint32_t deepest = 0;
-    for (size_t k = 0; k < 4; ++k) {
+    for (size_t k = 0; k < kWideKids; ++k) {
       uint32_t ref = kEmptyRef;
       float box[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
       if (k < slots.size()) {
@@ -131,8 +131,8 @@ class Collapser {
         }
       }
       uint32_t* w = &out_[(size_t)idx * kNode128Words];
-      for (int a = 0; a < 6; ++a) w[4 * a + k] = fbits(box[a]);
-      w[24 + k] = ref;
+      for (int a = 0; a < 6; ++a) w[kWideKids * a + k] = fbits(box[a]);
+      w[6 * kWideKids + k] = ref;
     }
     // Expanding this node pushes at most (slots - 1) entries.
     *need = (slots.empty() ? 0u : (uint32_t)slots.size() - 1u) + deepest;

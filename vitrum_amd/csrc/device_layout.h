@@ -26,15 +26,15 @@
 // columns (4 x float4), local-to-world columns (4 x float4), then
 // {blas wrapper node, packed instance index, blas Node128 root, 0}.
 //
-// Node128 (128 bytes, 4-wide): the same trees with levels collapsed.  A
-// Node128 made from a binary node holds up to four descendants in
-// depth-first order (a child is replaced by its two children, largest box
-// first, while fewer than four slots are used), stored axis-major so a
-// traversal step loads each coordinate of the four boxes as one float4:
+// Wide node (kWideKids = K children, 8K words): the same trees with levels
+// collapsed.  A wide node made from a binary node holds up to K descendants
+// in depth-first order (a child is replaced by its two children, largest box
+// first, while fewer than K slots are used), stored axis-major so a
+// traversal step loads each coordinate of all K boxes as K/4 float4s:
 //
-//   words 0-3 min x, 4-7 min y, 8-11 min z, 12-15 max x, 16-19 max y,
-//   words 20-23 max z, 24-27 child refs (Node128 index or leaf ref),
-//   words 28-31 reserved (0); unused slots hold kEmptyRef and an empty box.
+//   rows of K words: min x, min y, min z, max x, max y, max z, child refs
+//   (wide node index or leaf ref), then one reserved row (0); unused slots
+//   hold kEmptyRef and an empty box.  With K = 4 this is 128 bytes.
 //
 // A binary node is only replaced by its children when both child boxes lie
 // inside its box, so a ray that reaches a child's box test reaches it in
@@ -58,7 +58,8 @@ constexpr uint32_t kMaxLeafOffsetInRef = (1u << 27) - 1;
 constexpr uint32_t kNode64Words = 16;
 constexpr uint32_t kTriWords = 12;
 constexpr uint32_t kInstWords = 36;
-constexpr uint32_t kNode128Words = 32;
+constexpr uint32_t kWideKids = 4;                    // children per wide node
+constexpr uint32_t kNode128Words = 8 * kWideKids;    // words per wide node
 
 struct DeviceLayout {
   std::vector<uint32_t> blas_nodes;   // Node64 words

@@ -634,3 +634,6 @@ vt_status vt_set_node_width(vt_engine* eng, uint32_t width) {
   eng->node_width = width;
   return VT_OK;
 }
+
+static_assert(vt::kWideKids == vtk::kWideKids && vt::kNode128Words == 4 * vtk::kWideNodeFloat4s,
+              "the wide-node layout of the builder and of the traversal kernel differ");

@@ -10,6 +10,11 @@ namespace vtk {
 constexpr uint32_t kTraceBlock = 128;     // threads per block (2 waves)
 constexpr uint32_t kLdsDepth = 20;        // stack entries per lane kept in LDS
 constexpr uint32_t kStackCapacity = 128;  // TLAS + BLAS entries per lane (>= 2 * 58 + 2)
+// Wide nodes: children per node (a multiple of 4) and node size in float4s:
+// six coordinate rows and one child-ref row of kWideKids words, padded to
+// 8 rows (device_layout.h).
+constexpr uint32_t kWideKids = 4;
+constexpr uint32_t kWideNodeFloat4s = 2 * kWideKids;
 
 struct DeviceScene {
   const float4* blas_nodes;  // Node64, 4 x float4 each

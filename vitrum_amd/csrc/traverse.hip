@@ -592,7 +592,8 @@ __global__ __launch_bounds__(BLOCK, 4) void trace_kernel(TraceParams p) {
       }
       if (W4 && (cur & kLeafBit) == 0) {
         // ---- interior: one Node128, four child boxes tested axis by axis ----
-        const float4* nd = (level ? blas : tlas) + 8 * (size_t)cur;
+        static_assert(kWideKids == 4, "the expansion below is written for 4-wide nodes");
+        const float4* nd = (level ? blas : tlas) + (size_t)kWideNodeFloat4s * cur;
         const V3 o = level ? lo : ro;
         const V3 inv = level ? linv : rinv;
         const float tm = level ? ltmin : tmin;
