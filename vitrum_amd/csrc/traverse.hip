@@ -531,34 +531,138 @@ __global__ __launch_bounds__(BLOCK, 4) void trace_kernel(TraceParams p) {
         enter_instance(slot);
         pend = kPendNone;
       }
-      if (m_leaf && (all || (uint32_t)__popcll(m_leaf) >= dm) && active && pend == kPendLeaf) {
-        const uint32_t count = (cur >> 27) & 15u, first = cur & 0x07FFFFFFu;
-        cur = kNone;
-        pend = kPendNone;
-        if (!lw_ready) {
-          lw = wt_setup(ld);
-          lw_ready = true;
-        }
-        for (uint32_t i = 0; i < count; ++i) {
-          const uint32_t t = first + i;
-          const float4* tr = tris + 3 * (size_t)t;
-          const float4 w0 = tr[0], w1 = tr[1], w2 = tr[2];
-          if (COUNT) ++c_tri;
-          TriHit h;
-          if (tri_test(lo, ld, lw, v3(w0.x, w0.y, w0.z), v3(w0.w, w1.x, w1.y), v3(w1.z, w1.w, w2.x), ltmin,
-                       lbest, near && lhit, h)) {
-            if (ANY) {
-              finish(true);
-              break;
-            }
-            if (h.t == lbest && lhit && !(t < lprim)) continue;  // equal t: lower index wins
-            lhit = true;
-            lbest = h.t;
-            lprim = t;
-            lu = h.u;
-            lv = h.v;
-            lpos = h.point;
+      if (m_leaf && (all || (uint32_t)__popcll(m_leaf) >= dm)) {
+        const bool mine = active && pend == kPendLeaf;
+        uint32_t cnt = 0, tfirst = 0;
+        if (mine) {
+          cnt = (cur >> 27) & 15u;
+          tfirst = cur & 0x07FFFFFFu;
+          cur = kNone;
+          pend = kPendNone;
+          if (!lw_ready) {
+            lw = wt_setup(ld);
+            lw_ready = true;
           }
+        }
+        if (COUNT) {
+          // Work counting: each lane tests its own leaf, one triangle at a time.
+          for (uint32_t i = 0; i < cnt; ++i) {
+            const uint32_t t = tfirst + i;
+            const float4* tr = tris + 3 * (size_t)t;
+            const float4 w0 = tr[0], w1 = tr[1], w2 = tr[2];
+            ++c_tri;
+            TriHit h;
+            if (tri_test(lo, ld, lw, v3(w0.x, w0.y, w0.z), v3(w0.w, w1.x, w1.y), v3(w1.z, w1.w, w2.x), ltmin,
+                         lbest, near && lhit, h)) {
+              if (h.t == lbest && lhit && !(t < lprim)) continue;  // equal t: lower index wins
+              lhit = true;
+              lbest = h.t;
+              lprim = t;
+              lu = h.u;
+              lv = h.v;
+              lpos = h.point;
+            }
+          }
+        } else {
+          // The leaves' (ray, triangle) pairs are spread over all 64 lanes,
+          // 64 pairs per round: pair q belongs to the lane whose exclusive
+          // prefix of triangle counts is the largest one <= q.  Every pair is
+          // tested against its ray's state at the start of the round, then
+          // each ray keeps the lexicographic minimum (t, triangle index) of
+          // its pairs -- exactly what testing them one after another gives
+          // (a later triangle replaces the best only with a smaller t or, at
+          // equal t, a lower index), so the hit records are unchanged.
+          uint32_t excl = 0, total = 0;
+#pragma unroll
+          for (uint32_t b = 0; b < 4; ++b) {  // counts are below 16: one ballot per bit
+            const uint64_t m = __ballot((cnt >> b) & 1u);
+            excl += (uint32_t)__popcll(m & lane_below) << b;
+            total += (uint32_t)__popcll(m) << b;
+          }
+          const uint32_t incl = excl + cnt;
+          bool occl = false;
+          for (uint32_t base = 0; base < total; base += 64u) {
+            const uint32_t q = base + lane;
+            const bool work = q < total;
+            // owner: the lowest lane whose inclusive prefix exceeds q
+            uint32_t o_lo = 0, o_hi = 63;
+#pragma unroll
+            for (uint32_t it = 0; it < 6; ++it) {
+              const uint32_t mid = (o_lo + o_hi) >> 1;
+              if ((uint32_t)__shfl((int)incl, (int)mid) > q) o_hi = mid; else o_lo = mid + 1u;
+            }
+            const int o = (int)min(o_lo, 63u);
+            const uint32_t o_first = (uint32_t)__shfl((int)tfirst, o), o_excl = (uint32_t)__shfl((int)excl, o);
+            const V3 wo = v3(__shfl(lo.x, o), __shfl(lo.y, o), __shfl(lo.z, o));
+            const V3 wd = v3(__shfl(ld.x, o), __shfl(ld.y, o), __shfl(ld.z, o));
+            WtRay ww;
+            ww.ds = __shfl(lw.ds, o);
+            ww.sx = __shfl(lw.sx, o);
+            ww.sy = __shfl(lw.sy, o);
+            ww.sz = __shfl(lw.sz, o);
+            ww.perm = (uint32_t)__shfl((int)lw.perm, o);
+            const float w_tmin = __shfl(ltmin, o), w_hi = __shfl(lbest, o);
+            const bool w_incl = __shfl((int)(near && lhit), o) != 0;
+            uint32_t kt = 0xFFFFFFFFu, ki = 0xFFFFFFFFu, kl = lane;
+            TriHit h;
+            h.t = h.u = h.v = 0.0f;
+            h.point = v3(0, 0, 0);
+            // A batched launch can hold rays of different scenes in one wave:
+            // the pair's triangle is in its owner's scene.
+            const float4* otris = tris;
+            if (MULTI) {
+              const uint64_t tp = (uint64_t)(uintptr_t)tris;
+              const uint32_t p_lo = (uint32_t)__shfl((int)(uint32_t)tp, o);
+              const uint32_t p_hi = (uint32_t)__shfl((int)(uint32_t)(tp >> 32), o);
+              otris = reinterpret_cast<const float4*>((uintptr_t)(((uint64_t)p_hi << 32) | p_lo));
+            }
+            if (work) {
+              const uint32_t t = o_first + (q - o_excl);
+              const float4* tr = otris + 3 * (size_t)t;
+              const float4 w0 = tr[0], w1 = tr[1], w2 = tr[2];
+              if (tri_test(wo, wd, ww, v3(w0.x, w0.y, w0.z), v3(w0.w, w1.x, w1.y), v3(w1.z, w1.w, w2.x), w_tmin,
+                           w_hi, w_incl, h)) {
+                kt = f2u(h.t);  // t > 0: the bit patterns order as the values
+                ki = t;
+              }
+            }
+            // Segmented minimum over each owner's run of lanes (at most 15).
+#pragma unroll
+            for (uint32_t d = 1; d < 16; d <<= 1) {
+              const uint32_t ot = (uint32_t)__shfl_down((int)kt, d), oi = (uint32_t)__shfl_down((int)ki, d);
+              const uint32_t ol = (uint32_t)__shfl_down((int)kl, d);
+              const int oo = __shfl_down(o, d);
+              if (lane + d < 64u && oo == o && (ot < kt || (ot == kt && oi < ki))) {
+                kt = ot;
+                ki = oi;
+                kl = ol;
+              }
+            }
+            // Each ray reads its run's minimum (its run in this round starts at
+            // lane max(excl, base) - base) and the winner's hit data.
+            const bool here = mine && cnt > 0u && excl < base + 64u && incl > base;
+            const int src = (int)((max(excl, base) - base) & 63u);
+            const uint32_t rt = (uint32_t)__shfl((int)kt, src), ri = (uint32_t)__shfl((int)ki, src);
+            const int rl = __shfl((int)kl, src);
+            const float ru = __shfl(h.u, rl), rv = __shfl(h.v, rl);
+            const V3 rp = v3(__shfl(h.point.x, rl), __shfl(h.point.y, rl), __shfl(h.point.z, rl));
+            if (here && rt != 0xFFFFFFFFu) {
+              if (ANY) {
+                occl = true;
+              } else {
+                const float t = u2f(rt);
+                if (!lhit || t < lbest || (t == lbest && ri < lprim)) {
+                  lhit = true;
+                  lbest = t;
+                  lprim = ri;
+                  lu = ru;
+                  lv = rv;
+                  lpos = rp;
+                }
+              }
+            }
+          }
+          if (ANY && occl) finish(true);
         }
       }
       if (!active || pend != kPendNone) continue;
