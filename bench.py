@@ -91,7 +91,7 @@ NRC_TRAIN_FLOPS = NRC_FWD_FLOPS + 2 * (3 * 64 + 3 * 64 * 64) + 2 * (4 * 64 * 64 
 BF16_PEAK_TFLOPS = 2500.0
 # Identifies the traversal kernel configuration the committed traffic
 # measurement belongs to (child order, refill threshold, steps per refill).
-KERNEL_KEY = "trace_kernel<closest> near_first=1 refill=8 steps=16 defer=16 lds=20 waves=4"
+KERNEL_KEY = "trace_kernel<closest> nodes=4-wide leaves=lane-compacted near_first=1 refill=8 steps=16 defer=12 lds=20 waves=4"
 
 
 def nrc_mlp(eng, reps=10):
@@ -235,6 +235,8 @@ def main():
     ap.add_argument("--near-first", type=int, default=-1, help="1: nearer child first, 0: left first (-1: default)")
     ap.add_argument("--defer-min", type=int, default=0,
                     help="lanes that must need a leaf/instance operation before a wave runs it (0: default)")
+    ap.add_argument("--node-width", type=int, default=4, choices=(2, 4),
+                    help="traversal node width: 4-wide trees collapsed from the binary BVH (default) or binary")
     ap.add_argument("--no-nrc-mlp", action="store_true", help="skip the cache MLP kernel measurement")
     ap.add_argument("--no-render-rmse", action="store_true",
                     help="skip the plain vs cache time-to-RMSE measurement (about a minute of GPU time)")
@@ -251,9 +253,10 @@ def main():
     from vitrum_amd import native, workloads as W
 
     eng = native.Engine(0)
+    eng.set_node_width(args.node_width)
     if args.refill_min or args.steps_per_refill or args.near_first >= 0 or args.defer_min:
         eng.set_trace_tuning(args.refill_min or 8, args.steps_per_refill or 16,
-                             1 if args.near_first < 0 else args.near_first, args.defer_min or 16)
+                             1 if args.near_first < 0 else args.near_first, args.defer_min or 12)
     if args.calibrate_copy_mib:
         nb = args.calibrate_copy_mib << 20
         a, b = eng.alloc(nb), eng.alloc(nb)
@@ -283,9 +286,6 @@ def main():
     # Reference layout: 32 B per binary node tested, 64 B per triangle tested
     # (index vec4u + 3 vertex vec4f; shared-bvh/src/strides.ts:24-71).
     ref_bytes = 32.0 * (box_tests + tlas_tests) + 64.0 * tri_tests
-    # This kernel's layout: one 64 B Node64 per expansion (two boxes), 48 B per
-    # triangle, 80 B per instance entry (W2L + metadata), 32 B ray + 20 B hit.
-    own_bytes = 32.0 * (box_tests + tlas_tests) + 48.0 * tri_tests + 80.0 * inst_entries + 52.0 * n
 
     d_rays = eng.alloc(rays.nbytes)
     d_hits = eng.alloc(n * W.HIT_DTYPE.itemsize)
@@ -335,7 +335,7 @@ def main():
             "avg_tri_tests_per_ray": round(tri_tests / n, 3),
             "avg_tlas_tests_per_ray": round(tlas_tests / n, 3),
             "avg_instances_per_ray": round(inst_entries / n, 3),
-            "own_layout_gbps": round(own_bytes / (avg_ms * 1e-3) / 1e9, 1),
+            "node_width": args.node_width,
         },
         "roofline": {
             "bound": "hbm",

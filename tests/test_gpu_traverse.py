@@ -249,12 +249,12 @@ def test_node_widths_agree(engine):
             engine.set_node_width(4)
             assert_hits_equal(engine.trace_closest(sample), h2n, f"near_first={near}")
             engine.set_node_width(2)
-        engine.set_trace_tuning(8, 16, 1, 16)
+        engine.set_trace_tuning(8, 16, 1, 12)
         engine.set_node_width(4)
         assert_hits_equal(engine.trace_closest(sample), h2, "width 4 vs 2")
         np.testing.assert_array_equal(engine.trace_any(sample), a2)
         with pytest.raises(N.VtError):
             engine.set_node_width(3)
     finally:
-        engine.set_trace_tuning(8, 16, 1, 16)
+        engine.set_trace_tuning(8, 16, 1, 12)
         engine.set_node_width(4)

@@ -25,7 +25,7 @@ struct vt_batch {
   bool table_dirty = true;
   bool wide = false;  // every scene's Node128 trees fit the kernel's stack
   // Launch parameters (the autotuner's choice).
-  uint32_t refill_min = 8, steps = 16, near_first = 1, defer_min = 16, blocks_per_cu = 1;
+  uint32_t refill_min = 8, steps = 16, near_first = 1, defer_min = 12, blocks_per_cu = 1;
   // Captured graph of (counter reset, batched trace).
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -237,7 +237,7 @@ vt_status vt_batch_autotune(vt_batch* b, const vt_ray* d_rays, const uint32_t* r
   if (n == 0 || !d_rays || !d_hits) return berr(b, VT_ERR_INVALID_ARGUMENT, "empty batch or NULL buffer");
   drop_graph(b);
   const uint32_t max_bpc = b->blocks_per_cu;
-  const uint32_t refills[] = {8, 16, 32}, steps[] = {16, 48}, orders[] = {1, 0}, defers[] = {1, 16};
+  const uint32_t refills[] = {8, 16, 32}, steps[] = {16, 48}, orders[] = {1, 0}, defers[] = {1, 12};
   vt_tuning best{};
   best.ms = -1.0f;
   for (uint32_t bpc = max_bpc; bpc >= 1 && bpc + 2 > max_bpc; --bpc)
