@@ -215,7 +215,10 @@ vt_status vt_trace_closest_counts(vt_engine *eng, const vt_ray *rays, uint32_t n
  * closest-hit traversal: 0 = the reference's left-first order; 1 (default) =
  * nearer child first inside each BLAS, equal triangle distances resolved as
  * the reference resolves them (lowest reordered index), instances in the
- * reference's TLAS order -- hit records identical to the reference's; 2 = also
+ * reference's TLAS order -- hit records identical to the reference's except
+ * where two distinct triangles of one BLAS are hit at exactly the same
+ * distance and the lower-indexed one's box test rounds above it (the
+ * order-dependent pop-time cull can then pick differently); 2 = also
  * nearest-first across instances (can differ from the reference on coplanar
  * geometry in different instances; for measurement).  A lane whose next
  * operation is a BLAS leaf's triangle tests, an instance entry or an instance
