@@ -237,6 +237,18 @@ vt_status vt_set_trace_tuning(vt_engine *eng, uint32_t refill_min, uint32_t step
  * than the kernel holds, and the work-counting launches, always use 2. */
 vt_status vt_set_node_width(vt_engine *eng, uint32_t width);
 
+/* Hit filter of vt_trace_closest* / vt_trace_any* (the path tracer always
+ * uses 1): 0 (default) = the raw traversal (traceClosestRaw and
+ * traceTlasAny); 1 = the path tracer's wrappers (traceClosest,
+ * packages/pt-webgpu/src/wgsl/pathTrace/intersection.wgsl.ts:462-490, and
+ * traceSurfaceVisibility, connect.wgsl.ts:37-103): a hit on the back face of
+ * a single-sided opaque material (materialAcceptsSidedHit,
+ * material.wgsl.ts:3259-3263; doubleSided defaults to false) is not a
+ * surface -- closest hit re-traces beyond it, any hit is not blocked by it.
+ * The front face follows the watertight test's side and the instance's
+ * orientation (intersection.wgsl.ts:45). */
+vt_status vt_set_hit_filter(vt_engine *eng, uint32_t filter);
+
 /* bvhTraversal (packages/pt-webgpu/src/index.ts option; opt-in
  * 'cwbvh-closest', packages/pt-webgpu/src/wgsl/pathTrace/intersection.wgsl.ts:
  * 576-838): 0 = binary, 1 = wide.  Applies to vt_trace_closest* and to the

@@ -399,6 +399,8 @@ typedef struct {
   mat_t *mats;
   env_t env;
   double bias, tmin;
+  uint8_t *single_sided; /* per packed triangle: back faces are not surfaces */
+  uint32_t hit_limit;    /* sceneSurfaceHitLimit (material.wgsl.ts:423-445) */
 } rctx;
 
 static float f32(double x) { return (float)x; }
@@ -412,10 +414,11 @@ static orc_ray mk_ray(d3 o, d3 d, double tmin, double tmax) {
   return r;
 }
 
+/* traceSurfaceVisibility (connect.wgsl.ts:37-103), opaque materials. */
 static int visible(const rctx *c, d3 o, d3 d, double tmax) {
   orc_ray r = mk_ray(o, d, c->tmin, tmax);
   uint32_t occ = 0;
-  orc_trace_any(c->d->scene, &r, 1, &occ, 1);
+  orc_trace_occluded_sided(c->d->scene, c->single_sided, c->hit_limit, &r, 1, &occ, 1);
   return occ == 0;
 }
 
@@ -427,12 +430,13 @@ typedef struct {
   float tpos[3];   /* the traversal's f32 world hit position */
 } surf_t;
 
-/* Closest hit and binary64 surface at the hit. */
+/* Closest hit (traceClosest, intersection.wgsl.ts:462-490: back faces of
+ * single-sided materials are skipped) and binary64 surface at the hit. */
 static int closest(const rctx *c, d3 o, d3 d, surf_t *s) {
   orc_ray r = mk_ray(o, d, c->tmin, F32_MAX);
   orc_hit h;
   float wpos[3];
-  orc_trace_closest_pos(c->d->scene, &r, 1, &h, wpos, 1);
+  orc_trace_closest_sided(c->d->scene, c->single_sided, &r, 1, &h, wpos, 1);
   if (h.inst == 0xFFFFFFFFu) {
     s->hit = 0;
     return 0;
@@ -805,6 +809,18 @@ int orc_render(const orc_render_desc *d, uint32_t frame_index, uint32_t x0, uint
   c.env.pdf = d->env_pdf;
   c.bias = (double)d->ray_bias;
   c.tmin = (double)f32(c.bias * 0.1);
+  /* materialAcceptsSidedHit (material.wgsl.ts:3259-3263): opaque materials
+   * that are not double-sided have no back-face surfaces. */
+  const uint32_t ntri = d->scene->triangle_count;
+  c.single_sided = (uint8_t *)calloc(ntri + 1, 1);
+  for (uint32_t t = 0; t < ntri; ++t) {
+    const uint32_t mid = d->tri_material[t];
+    c.single_sided[t] = mid < d->material_count ? (uint8_t)!c.mats[mid].two_sided : 1u;
+  }
+  {
+    const uint64_t lim = (uint64_t)ntri * (d->scene->tlas_node_count ? d->scene->instance_count : 1u);
+    c.hit_limit = lim > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)lim;
+  }
   if (threads < 1) threads = 1;
   if (threads > 64) threads = 64;
   pthread_t tid[64];
@@ -821,6 +837,7 @@ int orc_render(const orc_render_desc *d, uint32_t frame_index, uint32_t x0, uint
   for (uint32_t t = 0; t < threads; ++t)
     if (tid[t]) pthread_join(tid[t], NULL);
   free(c.mats);
+  free(c.single_sided);
   free(em);
   free(order);
   (void)cmp_em;

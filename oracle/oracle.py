@@ -262,6 +262,54 @@ def trace_any(scene: PackedScene, rays, *, threads=None):
     return out
 
 
+def single_sided_flags(scene: PackedScene, materials):
+    """One byte per packed triangle: 1 when its material is opaque and not
+    double-sided (materialAcceptsSidedHit, material.wgsl.ts:3259-3263);
+    `materials` is the per-primitive material list of the scene (dicts with
+    an optional doubleSided, default false)."""
+    tri_mat = np.asarray(scene.pack["tri_material"], np.uint32)
+    ds = np.array([1 if m.get("doubleSided", False) else 0 for m in materials] or [0], np.uint8)
+    ok = tri_mat < ds.size
+    out = np.ones(tri_mat.size, np.uint8)
+    out[ok] = 1 - ds[tri_mat[ok]]
+    return out
+
+
+def trace_closest_sided(scene: PackedScene, single_sided, rays, *, threads=None, with_pos=False):
+    """traceClosest (pt-webgpu intersection.wgsl.ts:462-490): back-face hits
+    on single-sided materials are skipped by re-tracing from the hit."""
+    rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+    n = rays.size
+    hits = np.zeros(n, HIT_DTYPE)
+    pos = np.zeros(3 * n, np.float32)
+    flags = np.ascontiguousarray(single_sided, np.uint8)
+    threads = threads or min(16, os.cpu_count() or 1)
+    lib().orc_trace_closest_sided(C.byref(scene.struct), flags.ctypes.data_as(C.c_void_p),
+                                  rays.ctypes.data_as(C.c_void_p), C.c_uint32(n),
+                                  hits.ctypes.data_as(C.c_void_p), pos.ctypes.data_as(C.c_void_p),
+                                  C.c_uint32(threads))
+    return (hits, pos.reshape(n, 3)) if with_pos else hits
+
+
+def trace_occluded_sided(scene: PackedScene, single_sided, rays, *, hit_limit=0xFFFFFFFF, threads=None):
+    """traceSurfaceVisibility (connect.wgsl.ts:37-103), opaque materials:
+    1 where an accepted surface blocks the ray."""
+    rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+    n = rays.size
+    out = np.zeros(n, np.uint32)
+    flags = np.ascontiguousarray(single_sided, np.uint8)
+    threads = threads or min(16, os.cpu_count() or 1)
+    lib().orc_trace_occluded_sided(C.byref(scene.struct), flags.ctypes.data_as(C.c_void_p),
+                                   C.c_uint32(hit_limit), rays.ctypes.data_as(C.c_void_p), C.c_uint32(n),
+                                   _p(out, U32P), C.c_uint32(threads))
+    return out
+
+
+def orientation_sign(l2w):
+    m = np.ascontiguousarray(l2w, np.float32).ravel()
+    return int(lib().orc_orientation_sign(m.ctypes.data_as(F32P)))
+
+
 HIT64_DTYPE = np.dtype([("hit", "<u4"), ("prim", "<u4"), ("inst", "<u4"), ("pad", "<u4"),
                         ("t", "<f8"), ("bary", "<f8", 3)])
 
@@ -367,7 +415,7 @@ class CwbvhView(C.Structure):
                 ("child_count", U32P), ("node_count", C.c_uint32), ("inst_root", U32P)]
 
 
-def trace_closest_cwbvh(scene: PackedScene, cw, inst_roots, rays, *, threads=None):
+def trace_closest_cwbvh(scene: PackedScene, cw, inst_roots, rays, *, threads=None, single_sided=None):
     """pt-webgpu's `cwbvh-closest` mode in binary32 (intersection.wgsl.ts:
     576-838): Moller-Trumbore proposal in the wide forest, watertight
     re-verification, binary fallback.  Same hit record as trace_closest."""
@@ -378,8 +426,11 @@ def trace_closest_cwbvh(scene: PackedScene, cw, inst_roots, rays, *, threads=Non
     v = CwbvhView(keep[0].ctypes.data_as(F32P), keep[1].ctypes.data_as(C.c_void_p), keep[2].ctypes.data_as(U32P),
                   keep[3].ctypes.data_as(U32P), cw.node_count, roots.ctypes.data_as(U32P))
     threads = threads or min(16, os.cpu_count() or 1)
-    lib().orc_trace_closest_cwbvh(C.byref(scene.struct), C.byref(v), rays.ctypes.data_as(C.c_void_p),
-                                  C.c_uint32(rays.size), hits.ctypes.data_as(C.c_void_p), C.c_uint32(threads))
+    flags = None if single_sided is None else np.ascontiguousarray(single_sided, np.uint8)
+    lib().orc_trace_closest_cwbvh_sided(C.byref(scene.struct), C.byref(v),
+                                        None if flags is None else flags.ctypes.data_as(C.c_void_p),
+                                        rays.ctypes.data_as(C.c_void_p), C.c_uint32(rays.size),
+                                        hits.ctypes.data_as(C.c_void_p), C.c_uint32(threads))
     return hits
 
 

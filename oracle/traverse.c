@@ -266,6 +266,7 @@ typedef struct {
   v3 position;
   uint32_t tri;
   float u, v;
+  int front; /* triHit.side > 0 (intersectionCore.wgsl.ts:547) */
 } local_hit;
 
 typedef struct {
@@ -284,6 +285,7 @@ static int trace_mesh_bvh(const orc_scene *s, const ray3 *r, float tmin, float t
   hit->position = (v3){0, 0, 0};
   hit->tri = 0;
   hit->u = hit->v = 0;
+  hit->front = 0;
   if (s->bvh_node_count == 0 || root >= s->bvh_node_count) return 0;
   uint32_t stack[BVH_STACK_DEPTH];
   uint32_t sp = 0;
@@ -315,6 +317,7 @@ static int trace_mesh_bvh(const orc_scene *s, const ray3 *r, float tmin, float t
           hit->tri = t;
           hit->u = th.bary.y;
           hit->v = th.bary.z;
+          hit->front = th.side > 0.0f;
         }
       }
     } else {
@@ -346,7 +349,27 @@ typedef struct {
   uint32_t tri, inst;
   float u, v;
   v3 pos; /* world hit position, L2W * local hit point (f32) */
+  int front;
 } world_hit;
+
+/* transformLinearOrientationSign (intersectionCore.wgsl.ts:96-112): the sign
+ * of the scaled 3x3 determinant of L2W; an orientation-reversing instance
+ * flips the local front face (intersection.wgsl.ts:45). */
+static float orientation_sign(const float *m) {
+  const float s0 = fmax2(fabsf_(m[0]), fmax2(fabsf_(m[1]), fabsf_(m[2])));
+  const float s1 = fmax2(fabsf_(m[4]), fmax2(fabsf_(m[5]), fabsf_(m[6])));
+  const float s2 = fmax2(fabsf_(m[8]), fmax2(fabsf_(m[9]), fabsf_(m[10])));
+  if (!(s0 > 0.0f) || s0 > 3.402823e38f || !(s1 > 0.0f) || s1 > 3.402823e38f || !(s2 > 0.0f) ||
+      s2 > 3.402823e38f)
+    return 1.0f;
+  const v3 a = {m[0] / s0, m[1] / s0, m[2] / s0}, b = {m[4] / s1, m[5] / s1, m[6] / s1};
+  const v3 c = {m[8] / s2, m[9] / s2, m[10] / s2};
+  const v3 x = {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+  const float det = (x.x * c.x + x.y * c.y) + x.z * c.z;
+  return det >= 0.0f ? 1.0f : -1.0f;
+}
+
+int orc_orientation_sign(const float *l2w) { return orientation_sign(l2w) > 0.0f ? 1 : -1; }
 
 static void tlas_instance_closest(const orc_scene *s, uint32_t inst, const ray3 *r, float tmin,
                                   world_hit *hit, counters *cnt) {
@@ -380,6 +403,7 @@ static void tlas_instance_closest(const orc_scene *s, uint32_t inst, const ray3 
       hit->u = lh.u;
       hit->v = lh.v;
       hit->pos = wpos;
+      hit->front = orientation_sign(l2w) > 0.0f ? lh.front : !lh.front;
     }
   }
 }
@@ -391,6 +415,7 @@ static void trace_tlas_closest(const orc_scene *s, const ray3 *r, float tmin, fl
   hit->tri = 0;
   hit->inst = 0xFFFFFFFFu;
   hit->u = hit->v = 0;
+  hit->front = 0;
   if (s->tlas_node_count == 0 || s->instance_index_count == 0) return; /* fail closed */
   uint32_t stack[TLAS_STACK_DEPTH];
   uint32_t sp = 0;
@@ -584,6 +609,122 @@ int orc_trace_any(const orc_scene *s, const orc_ray *rays, uint32_t n, uint32_t 
                   uint32_t threads) {
   job_t proto = {s, rays, 0, 0, NULL, NULL, out, NULL};
   return run_jobs(any_job, proto, n, threads);
+}
+
+/* ---- closest-hit and visibility wrappers (a8) ----------------------------
+ * traceClosest (pt-webgpu pathTrace/intersection.wgsl.ts:462-490): a hit on
+ * the back face of a single-sided opaque material is not a surface
+ * (materialAcceptsSidedHit, material.wgsl.ts:3259-3263); the trace resumes
+ * with the hit distance as the exclusive lower bound, and a cursor that does
+ * not advance, or reaches tMax, ends it as a miss.  traceSurfaceVisibility
+ * (connect.wgsl.ts:37-103), opaque materials: the first accepted surface
+ * blocks; skipped surfaces count against the surface-hit limit. */
+
+static void closest_sided(const orc_scene *s, const uint8_t *single_sided, const ray3 *r, float tmin,
+                          float tmax, world_hit *h) {
+  float cursor = tmin;
+  for (;;) {
+    trace_tlas_closest(s, r, cursor, tmax, h, NULL);
+    if (!h->did_hit) return;
+    if (h->front || !single_sided || h->tri >= s->triangle_count || !single_sided[h->tri]) return;
+    const float next = h->dist;
+    if (!(next > cursor) || !(next < tmax)) {
+      h->did_hit = 0;
+      return;
+    }
+    cursor = next;
+  }
+}
+
+static int visible_sided(const orc_scene *s, const uint8_t *single_sided, uint32_t hit_limit, const ray3 *r,
+                         float tmin, float tmax) {
+  float cursor = tmin;
+  uint32_t count = 0;
+  for (;;) {
+    world_hit h;
+    trace_tlas_closest(s, r, cursor, tmax, &h, NULL);
+    if (!h.did_hit) return 1;
+    if (count >= hit_limit) return 0;
+    ++count;
+    const int accepted =
+        h.front || !single_sided || h.tri >= s->triangle_count || !single_sided[h.tri];
+    if (accepted) return 0;
+    if (!(h.dist > cursor)) return 0;
+    cursor = h.dist;
+  }
+}
+
+typedef struct {
+  job_t base;
+  const uint8_t *single_sided;
+  uint32_t hit_limit;
+} sided_job_t;
+
+static void *closest_sided_job(void *arg) {
+  sided_job_t *sj = (sided_job_t *)arg;
+  job_t *j = &sj->base;
+  for (uint32_t i = j->begin; i < j->end; ++i) {
+    const orc_ray *q = &j->rays[i];
+    ray3 r = make_ray((v3){q->o[0], q->o[1], q->o[2]}, (v3){q->d[0], q->d[1], q->d[2]});
+    world_hit h;
+    closest_sided(j->s, sj->single_sided, &r, q->tmin, q->tmax, &h);
+    orc_hit *o = &j->hits[i];
+    o->prim = h.did_hit ? h.tri : 0xFFFFFFFFu;
+    o->inst = h.did_hit ? h.inst : 0xFFFFFFFFu;
+    o->t = h.did_hit ? h.dist : INFINITY;
+    o->u = h.did_hit ? h.u : 0.0f;
+    o->v = h.did_hit ? h.v : 0.0f;
+    if (j->pos) {
+      float *pp = j->pos + 3 * (size_t)i;
+      pp[0] = h.did_hit ? h.pos.x : 0.0f;
+      pp[1] = h.did_hit ? h.pos.y : 0.0f;
+      pp[2] = h.did_hit ? h.pos.z : 0.0f;
+    }
+  }
+  return NULL;
+}
+
+static void *visible_sided_job(void *arg) {
+  sided_job_t *sj = (sided_job_t *)arg;
+  job_t *j = &sj->base;
+  for (uint32_t i = j->begin; i < j->end; ++i) {
+    const orc_ray *q = &j->rays[i];
+    ray3 r = make_ray((v3){q->o[0], q->o[1], q->o[2]}, (v3){q->d[0], q->d[1], q->d[2]});
+    j->any[i] = visible_sided(j->s, sj->single_sided, sj->hit_limit, &r, q->tmin, q->tmax) ? 0u : 1u;
+  }
+  return NULL;
+}
+
+static int run_sided(void *(*fn)(void *), sided_job_t proto, uint32_t n, uint32_t threads) {
+  if (threads < 1) threads = 1;
+  if (threads > 256) threads = 256;
+  pthread_t tid[256];
+  sided_job_t jobs[256];
+  const uint32_t per = (n + threads - 1) / threads;
+  for (uint32_t t = 0; t < threads; ++t) {
+    jobs[t] = proto;
+    jobs[t].base.begin = t * per < n ? t * per : n;
+    jobs[t].base.end = (t + 1) * per < n ? (t + 1) * per : n;
+    if (pthread_create(&tid[t], NULL, fn, &jobs[t]) != 0) {
+      fn(&jobs[t]);
+      tid[t] = 0;
+    }
+  }
+  for (uint32_t t = 0; t < threads; ++t)
+    if (tid[t]) pthread_join(tid[t], NULL);
+  return 0;
+}
+
+int orc_trace_closest_sided(const orc_scene *s, const uint8_t *single_sided, const orc_ray *rays, uint32_t n,
+                            orc_hit *out, float *pos, uint32_t threads) {
+  sided_job_t proto = {{s, rays, 0, 0, out, NULL, NULL, pos}, single_sided, 0};
+  return run_sided(closest_sided_job, proto, n, threads);
+}
+
+int orc_trace_occluded_sided(const orc_scene *s, const uint8_t *single_sided, uint32_t hit_limit,
+                             const orc_ray *rays, uint32_t n, uint32_t *occluded, uint32_t threads) {
+  sided_job_t proto = {{s, rays, 0, 0, NULL, NULL, occluded, NULL}, single_sided, hit_limit};
+  return run_sided(visible_sided_job, proto, n, threads);
 }
 
 /* ---- FP64 closed-form check --------------------------------------------- */
@@ -824,6 +965,7 @@ static int trace_mesh_cwbvh(const orc_scene *s, const orc_cwbvh_view *cw, const 
   hit->position = (v3){0, 0, 0};
   hit->tri = 0;
   hit->u = hit->v = 0;
+  hit->front = 0;
   if (cw->node_count == 0 || wide_root >= cw->node_count)
     return trace_mesh_bvh(s, r, tmin, tmax, 1, hit, binary_root, cnt);
   const cw_hit c = cw_walk(s, cw, r->o, r->d, tmin, tmax, wide_root);
@@ -842,6 +984,7 @@ static int trace_mesh_cwbvh(const orc_scene *s, const orc_cwbvh_view *cw, const 
   hit->tri = c.tri;
   hit->u = e.bary.y;
   hit->v = e.bary.z;
+  hit->front = e.side > 0.0f;
   return 1;
 }
 
@@ -853,6 +996,7 @@ static void trace_tlas_closest_cwbvh(const orc_scene *s, const orc_cwbvh_view *c
   hit->tri = 0;
   hit->inst = 0xFFFFFFFFu;
   hit->u = hit->v = 0;
+  hit->front = 0;
   if (s->tlas_node_count == 0 || s->instance_index_count == 0) {
     trace_tlas_closest(s, r, tmin, tmax, hit, NULL);
     return;
@@ -898,6 +1042,7 @@ static void trace_tlas_closest_cwbvh(const orc_scene *s, const orc_cwbvh_view *c
           hit->u = lh.u;
           hit->v = lh.v;
           hit->pos = wpos;
+          hit->front = orientation_sign(l2w) > 0.0f ? lh.front : !lh.front;
         }
       }
     } else {
@@ -919,6 +1064,7 @@ typedef struct {
   const orc_ray *rays;
   orc_hit *out;
   uint32_t begin, end;
+  const uint8_t *single_sided;
 } cw_job;
 
 static void *cw_job_fn(void *arg) {
@@ -927,7 +1073,19 @@ static void *cw_job_fn(void *arg) {
     const orc_ray *q = &j->rays[i];
     const ray3 r = make_ray((v3){q->o[0], q->o[1], q->o[2]}, (v3){q->d[0], q->d[1], q->d[2]});
     world_hit h;
-    trace_tlas_closest_cwbvh(j->s, j->cw, &r, q->tmin, q->tmax, &h);
+    /* traceClosest's sided loop over traceClosestRaw in this mode */
+    float cursor = q->tmin;
+    for (;;) {
+      trace_tlas_closest_cwbvh(j->s, j->cw, &r, cursor, q->tmax, &h);
+      if (!j->single_sided || !h.did_hit || h.front || h.tri >= j->s->triangle_count ||
+          !j->single_sided[h.tri])
+        break;
+      if (!(h.dist > cursor) || !(h.dist < q->tmax)) {
+        h.did_hit = 0;
+        break;
+      }
+      cursor = h.dist;
+    }
     orc_hit *o = &j->out[i];
     o->prim = h.did_hit ? h.tri : 0xFFFFFFFFu;
     o->inst = h.did_hit ? h.inst : 0xFFFFFFFFu;
@@ -938,15 +1096,24 @@ static void *cw_job_fn(void *arg) {
   return NULL;
 }
 
+int orc_trace_closest_cwbvh_sided(const orc_scene *s, const orc_cwbvh_view *cw, const uint8_t *single_sided,
+                                  const orc_ray *rays, uint32_t n, orc_hit *out, uint32_t threads);
+
 int orc_trace_closest_cwbvh(const orc_scene *s, const orc_cwbvh_view *cw, const orc_ray *rays, uint32_t n,
                             orc_hit *out, uint32_t threads) {
+  return orc_trace_closest_cwbvh_sided(s, cw, NULL, rays, n, out, threads);
+}
+
+int orc_trace_closest_cwbvh_sided(const orc_scene *s, const orc_cwbvh_view *cw, const uint8_t *single_sided,
+                                  const orc_ray *rays, uint32_t n, orc_hit *out, uint32_t threads) {
   if (threads < 1) threads = 1;
   if (threads > 256) threads = 256;
   pthread_t tid[256];
   cw_job jobs[256];
   const uint32_t per = (n + threads - 1) / threads;
   for (uint32_t t = 0; t < threads; ++t) {
-    jobs[t] = (cw_job){s, cw, rays, out, t * per < n ? t * per : n, (t + 1) * per < n ? (t + 1) * per : n};
+    jobs[t] = (cw_job){s, cw, rays, out, t * per < n ? t * per : n, (t + 1) * per < n ? (t + 1) * per : n,
+                       single_sided};
     if (pthread_create(&tid[t], NULL, cw_job_fn, &jobs[t]) != 0) {
       cw_job_fn(&jobs[t]);
       tid[t] = 0;

@@ -167,6 +167,20 @@ int orc_trace_closest_pos(const orc_scene *s, const orc_ray *rays, uint32_t n,
 /* Any hit: traceTlasAny (tlasSceneHitTraversal.wgsl.ts:227). */
 int orc_trace_any(const orc_scene *s, const orc_ray *rays, uint32_t n,
                   uint32_t *out, uint32_t threads);
+/* The path tracer's wrappers over the closest hit (a8): traceClosest
+ * (intersection.wgsl.ts:462-490) skips hits on the back face of a
+ * single-sided opaque material by re-tracing from the hit distance;
+ * traceSurfaceVisibility (connect.wgsl.ts:37-103) reports a ray occluded by
+ * the first accepted surface, after at most hit_limit surfaces.
+ * single_sided: one byte per packed triangle (nonzero: back faces are not
+ * surfaces), or NULL (every hit accepted).  orc_orientation_sign: sign of an
+ * instance's local-to-world 3x3 determinant (transformLinearOrientationSign,
+ * intersectionCore.wgsl.ts:96-112); negative flips the local front face. */
+int orc_trace_closest_sided(const orc_scene *s, const uint8_t *single_sided, const orc_ray *rays, uint32_t n,
+                            orc_hit *out, float *pos, uint32_t threads);
+int orc_trace_occluded_sided(const orc_scene *s, const uint8_t *single_sided, uint32_t hit_limit,
+                             const orc_ray *rays, uint32_t n, uint32_t *occluded, uint32_t threads);
+int orc_orientation_sign(const float *l2w);
 
 /* FP64 ray/triangle intersection of the hit triangle in world space: the
  * instance's local vertices are carried to world space through its
@@ -223,6 +237,10 @@ typedef struct orc_cwbvh_view {
 int orc_trace_closest_cwbvh(const orc_scene *s, const orc_cwbvh_view *cw,
                             const orc_ray *rays, uint32_t n, orc_hit *out,
                             uint32_t threads);
+/* The same with traceClosest's sided loop (single_sided as for
+ * orc_trace_closest_sided; NULL accepts every hit). */
+int orc_trace_closest_cwbvh_sided(const orc_scene *s, const orc_cwbvh_view *cw, const uint8_t *single_sided,
+                                  const orc_ray *rays, uint32_t n, orc_hit *out, uint32_t threads);
 /* blas_wide_root: wide root of each packed instance's BLAS. */
 int orc_cwbvh_trace_closest(const orc_scene *s, const orc_cwbvh *c,
                             const uint32_t *blas_wide_root,

@@ -176,3 +176,101 @@ def test_fp64_parameters_close_on_cornell():
     # cancellation on small, distant triangles: bounded here at 2e-3 absolute.
     assert np.abs(hits["u"][m] - f[m, 1]).max() <= 2e-3
     assert np.abs(hits["v"][m] - f[m, 2]).max() <= 2e-3
+
+
+# ---- the path tracer's closest-hit and visibility wrappers (a8) ------------
+
+
+def _quad(y, up=True, material_id=0, transform=None):
+    """A unit quad in the plane y = const, wound so that its normal is +y
+    (up) or -y."""
+    pos = np.array([[0, y, 0], [1, y, 0], [1, y, 1], [0, y, 1]], np.float32)
+    idx = np.array([0, 2, 1, 0, 3, 2] if up else [0, 1, 2, 0, 2, 3], np.uint32)
+    p = dict(kind="mesh", positions=pos.ravel(), normals=np.tile([0, 1 if up else -1, 0], 4).astype(np.float32),
+             indices=idx, material_id=material_id)
+    if transform is not None:
+        p["transform"] = np.asarray(transform, np.float32)
+    return p
+
+
+def _sided_scene(mats, quads):
+    pack = O.pack_scene(quads)
+    ps = O.PackedScene(pack)
+    return ps, O.single_sided_flags(ps, mats)
+
+
+def _vertical_rays(ys, up):
+    r = np.zeros(len(ys), O.RAY_DTYPE)
+    r["o"] = [[0.3, y, 0.6] for y in ys]
+    r["d"] = [0, 1, 0] if up else [0, -1, 0]
+    r["tmin"] = 1e-4
+    r["tmax"] = W.TMAX_INFINITY
+    return r
+
+
+def test_single_sided_back_faces_are_skipped():
+    """materialAcceptsSidedHit: a single-sided (default) quad facing up at
+    y = 1 over a single-sided floor facing up at y = 0.  From above, the
+    quad's front face is the hit; from between them going up, the quad's back
+    face is not a surface (miss); from below going up both back faces are
+    skipped.  A double-sided quad is hit from either side.  The raw
+    traversal reports the geometric closest hit in every case."""
+    mats = [{}, {}]
+    ps, ss = _sided_scene(mats, [_quad(0.0, True, 0), _quad(1.0, True, 1)])
+    assert ss.tolist() == [1, 1, 1, 1]
+    down = _vertical_rays([2.0], up=False)
+    h = O.trace_closest_sided(ps, ss, down)
+    assert h["t"][0] == np.float32(1.0) and h["inst"][0] == 1
+    up = _vertical_rays([0.5, -1.0], up=True)
+    h = O.trace_closest_sided(ps, ss, up)
+    assert h["inst"].tolist() == [0xFFFFFFFF, 0xFFFFFFFF]
+    raw = O.trace_closest(ps, up)
+    assert raw["t"].tolist() == [0.5, 1.0]
+    # visibility: occluded only through front faces
+    vis_up = O.trace_occluded_sided(ps, ss, up)
+    assert vis_up.tolist() == [0, 0]
+    assert O.trace_occluded_sided(ps, ss, down).tolist() == [1]
+    # double-sided upper quad: hit from below too
+    ps2, ss2 = _sided_scene([{}, {"doubleSided": True}], [_quad(0.0, True, 0), _quad(1.0, True, 1)])
+    assert ss2.tolist() == [1, 1, 0, 0]
+    h = O.trace_closest_sided(ps2, ss2, up)
+    assert h["inst"].tolist() == [1, 1] and h["t"].tolist() == [0.5, 2.0]
+    assert O.trace_occluded_sided(ps2, ss2, up).tolist() == [1, 1]
+    # with no flags every hit is a surface
+    np.testing.assert_array_equal(O.trace_closest_sided(ps, np.zeros_like(ss), up)["t"], raw["t"])
+
+
+def test_front_face_follows_the_instance_orientation():
+    """intersection.wgsl.ts:45: an orientation-reversing instance flips the
+    local front face.  The quad facing up, mirrored in x, is seen from above
+    through its back face: the single-sided hit is skipped."""
+    mirror = np.eye(4, dtype=np.float32)
+    mirror[0, 0] = -1.0
+    assert O.orientation_sign(mirror.T.ravel()) == -1
+    assert O.orientation_sign(np.eye(4, dtype=np.float32).ravel()) == 1
+    ps, ss = _sided_scene([{}], [_quad(1.0, True, 0, transform=mirror.T.ravel())])
+    down = _vertical_rays([2.0], up=False)
+    down["o"] = [[-0.3, 2.0, 0.6]]
+    assert O.trace_closest(ps, down)["t"][0] == np.float32(1.0)
+    assert O.trace_closest_sided(ps, ss, down)["inst"][0] == 0xFFFFFFFF
+    up = _vertical_rays([0.0], up=True)
+    up["o"] = [[-0.3, 0.0, 0.6]]
+    assert O.trace_closest_sided(ps, ss, up)["t"][0] == np.float32(1.0)
+
+
+def test_sided_cursor_is_exclusive():
+    """The re-trace starts at the skipped hit's distance, exclusive: a front
+    face coplanar with a skipped back face (same t) is not reported -- the
+    reference's loop steps past every surface at that distance."""
+    ps, ss = _sided_scene([{}, {}], [_quad(1.0, False, 0), _quad(1.0, True, 1)])
+    up = _vertical_rays([0.0], up=True)
+    raw = O.trace_closest(ps, up)
+    assert raw["t"][0] == np.float32(1.0)
+    h = O.trace_closest_sided(ps, ss, up)
+    # the lower-indexed first hit is the back face of quad 1 (normal +y seen
+    # from below) or the front face of quad 0 (normal -y): whichever the raw
+    # traversal reports first decides
+    if raw["inst"][0] == 0:
+        assert h["inst"][0] == 0
+    else:
+        assert h["inst"][0] == 0xFFFFFFFF

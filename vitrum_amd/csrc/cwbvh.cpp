@@ -222,7 +222,7 @@ vt_status cw_err(vt_engine* e, vt_status code, const std::string& msg) {
 
 struct CwbvhState {
   DeviceBuffer tlas_nodes, inst_indices, w2l, l2w, blas_roots, wide_roots, bvh_nodes;
-  DeviceBuffer node_bounds, child_bounds, child_meta, child_count;
+  DeviceBuffer node_bounds, child_bounds, child_meta, child_count, single_sided, inst_reversed;
   vtw::Scene view{};
 };
 
@@ -231,7 +231,7 @@ void engine_release_cwbvh(vt_engine* e) {
   for (DeviceBuffer* b : {&e->cwbvh->tlas_nodes, &e->cwbvh->inst_indices, &e->cwbvh->w2l, &e->cwbvh->l2w,
                           &e->cwbvh->blas_roots, &e->cwbvh->wide_roots, &e->cwbvh->bvh_nodes,
                           &e->cwbvh->node_bounds, &e->cwbvh->child_bounds, &e->cwbvh->child_meta,
-                          &e->cwbvh->child_count})
+                          &e->cwbvh->child_count, &e->cwbvh->single_sided, &e->cwbvh->inst_reversed})
     b->release();
   delete e->cwbvh;
   e->cwbvh = nullptr;
@@ -273,6 +273,13 @@ vt_status engine_cwbvh_ensure(vt_engine* e) {
   s = s ? s : up(st->child_bounds, f.child_bounds.data(), f.child_bounds.size() * 4);
   s = s ? s : up(st->child_meta, f.child_meta.data(), f.child_meta.size() * 4);
   s = s ? s : up(st->child_count, f.child_count.data(), f.child_count.size() * 4);
+  std::vector<uint8_t> single(P.indices.size() / 4, 1u), reversed(std::max<uint32_t>(1, n_inst), 0u);
+  if (e->layout)
+    for (size_t t = 0; t < single.size() && t < e->layout->tri_count; ++t)
+      single[t] = e->layout->tris[t * vt::kTriWords + 9] > 0.5f ? 1u : 0u;
+  for (uint32_t i = 0; i < n_inst; ++i) reversed[i] = vt::orientation_reversed(&P.tlas_l2w[16 * (size_t)i]) ? 1u : 0u;
+  s = s ? s : up(st->single_sided, single.data(), single.size());
+  s = s ? s : up(st->inst_reversed, reversed.data(), reversed.size());
   if (s) {
     e->cwbvh = st.release();
     engine_release_cwbvh(e);
@@ -299,6 +306,8 @@ vt_status engine_cwbvh_ensure(vt_engine* e) {
   v.child_meta = (const uint32_t*)st->child_meta.ptr;
   v.child_count = (const uint32_t*)st->child_count.ptr;
   v.wide_count = (uint32_t)f.child_count.size();
+  v.single_sided = (const uint8_t*)st->single_sided.ptr;
+  v.inst_reversed = (const uint8_t*)st->inst_reversed.ptr;
   const hipError_t r = hipStreamSynchronize(e->stream);
   e->cwbvh = st.release();
   if (r != hipSuccess) return cw_err(e, VT_ERR_HIP, hipGetErrorString(r));
@@ -308,7 +317,7 @@ vt_status engine_cwbvh_ensure(vt_engine* e) {
 const vtw::Scene* engine_cwbvh_view(vt_engine* e) { return e->cwbvh ? &e->cwbvh->view : nullptr; }
 
 vt_status engine_trace_cwbvh(vt_engine* e, const float4* d_rays, uint32_t n, const uint32_t* d_n,
-                             uint32_t* d_hits, float* d_hit_pos) {
+                             uint32_t* d_hits, float* d_hit_pos, uint32_t sided) {
   if (n == 0) return VT_OK;
   if (vt_status s = engine_cwbvh_ensure(e)) return s;
   vtw::Params p{};
@@ -318,6 +327,7 @@ vt_status engine_trace_cwbvh(vt_engine* e, const float4* d_rays, uint32_t n, con
   p.hit_pos = d_hit_pos;
   p.n = n;
   p.n_ptr = d_n;
+  p.sided = sided;
   // One ray per lane, grid-stride: enough blocks to cover the rays, capped at
   // 16 blocks of 128 per CU.
   const uint32_t cap = std::max<uint32_t>(1, e->cu_count) * 16u;

@@ -52,6 +52,8 @@ struct Scene {
   const uint32_t* child_meta;
   const uint32_t* child_count;
   uint32_t wide_count;
+  const uint8_t* single_sided;   // per packed triangle: back faces are not surfaces
+  const uint8_t* inst_reversed;  // per packed instance: orientation-reversing L2W
 };
 
 struct Params {
@@ -61,6 +63,7 @@ struct Params {
   float* hit_pos;         // optional, 3 per ray
   uint32_t n;
   const uint32_t* n_ptr;  // when set, the ray count is read from device memory (n = capacity)
+  uint32_t sided;         // 1: traceClosest's sided re-trace (intersection.wgsl.ts:462-490)
 };
 
 hipError_t launch_trace(const Params& p, uint32_t grid, hipStream_t s);

@@ -70,6 +70,27 @@ std::string convert_forest(const std::vector<uint32_t>& ref_nodes, const std::ve
   return err;
 }
 
+}  // namespace
+
+bool orientation_reversed(const float* m) {
+  auto amax = [](float a, float b, float c) {
+    const float x = std::fabs(a), y = std::fabs(b), z = std::fabs(c);
+    const float yz = y > z ? y : z;
+    return x > yz ? x : yz;
+  };
+  const float s0 = amax(m[0], m[1], m[2]), s1 = amax(m[4], m[5], m[6]), s2 = amax(m[8], m[9], m[10]);
+  const float big = 3.402823e38f;
+  if (!(s0 > 0.0f) || s0 > big || !(s1 > 0.0f) || s1 > big || !(s2 > 0.0f) || s2 > big) return false;
+  const float ax = m[0] / s0, ay = m[1] / s0, az = m[2] / s0;
+  const float bx = m[4] / s1, by = m[5] / s1, bz = m[6] / s1;
+  const float cx = m[8] / s2, cy = m[9] / s2, cz = m[10] / s2;
+  const float xx = ay * bz - az * by, xy = az * bx - ax * bz, xz = ax * by - ay * bx;
+  const float det = (xx * cx + xy * cy) + xz * cz;
+  return !(det >= 0.0f);
+}
+
+namespace {
+
 struct Slot {
   uint32_t ref;
   float box[6];
@@ -205,6 +226,7 @@ std::string build_device_layout(const Pack& p, DeviceLayout& out) {
     if (it == root_slot.end()) return "instance references an unknown BLAS root";
     w[32] = blas_wrapper[it->second];
     w[33] = inst;
+    w[35] = orientation_reversed(&p.tlas_l2w[16 * (size_t)inst]) ? 1u : 0u;
   }
   // Node128 trees: one per BLAS wrapper, one for the TLAS.
   std::vector<uint32_t> blas4_root(blas_wrapper.size(), 0);

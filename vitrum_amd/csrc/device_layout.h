@@ -19,12 +19,15 @@
 //                            a BLAS, instance slot for the TLAS).
 //            kEmptyRef    -> no child (its box is empty: +inf..-inf).
 //
-// Triangle record (48 bytes): a.xyz b.x | b.yz c.xy | c.z 0 0 0, indexed by
-// the global packed (BVH-reordered) triangle index.
+// Triangle record (48 bytes): a.xyz b.x | b.yz c.xy | c.z flags 0 0, indexed
+// by the global packed (BVH-reordered) triangle index.
 //
 // Instance record (144 bytes, TLAS permutation order): world-to-local
 // columns (4 x float4), local-to-world columns (4 x float4), then
-// {blas wrapper node, packed instance index, blas Node128 root, 0}.
+// {blas wrapper node, packed instance index, blas Node128 root,
+// orientation-reversing (1) or not (0)}.  Triangle record word 9 is 1 for a
+// triangle whose material is single-sided (set by the engine, which knows the
+// materials), else 0.
 //
 // Wide node (kWideKids = K children, 8K words): the same trees with levels
 // collapsed.  A wide node made from a binary node holds up to K descendants
@@ -75,6 +78,11 @@ struct DeviceLayout {
   // Node128 trees, and of the Node64 trees.
   uint32_t stack4_need = 0, stack2_need = 0;
 };
+
+// transformLinearOrientationSign (pt-webgpu intersectionCore.wgsl.ts:96-112)
+// in binary32: true when the scaled 3x3 determinant of a column-major
+// local-to-world matrix is negative (the instance flips front faces).
+bool orientation_reversed(const float* l2w);
 
 // Returns an empty string on success, else the reason the pack cannot be
 // represented (e.g. a leaf larger than 15 primitives).

@@ -150,6 +150,7 @@ vtk::TraceParams engine_trace_params(vt_engine* e) {
   p.steps = e->trace_steps;
   p.near_first = e->near_first;
   p.defer_min = e->defer_min;
+  p.sided = e->hit_filter;
   return p;
 }
 
@@ -278,6 +279,14 @@ vt_status vt_set_scene(vt_engine* eng, const vt_scene_desc* scene) {
   auto layout = std::make_unique<vt::DeviceLayout>();
   std::string le = vt::build_device_layout(*pack, *layout);
   if (!le.empty()) return set_err(eng, VT_ERR_RANGE, le);
+  // Single-sided triangles (materialAcceptsSidedHit, material.wgsl.ts:
+  // 3259-3263): opaque materials without doubleSided (the default) have no
+  // back-face surfaces for the path tracer's closest-hit and visibility rays.
+  for (uint32_t t = 0; t < layout->tri_count; ++t) {
+    const uint32_t mid = pack->tri_material[t];
+    const bool single = mid < scene->material_count ? scene->materials[mid].double_sided == 0 : true;
+    layout->tris[(size_t)t * vt::kTriWords + 9] = single ? 1.0f : 0.0f;
+  }
   HIP_TRY(eng, hipSetDevice(eng->device));
   // Upload into fresh buffers, then swap: a failure leaves the old scene bound.
   SceneBuffers nb;
@@ -406,7 +415,8 @@ static vt_status trace_device(vt_engine* eng, const vt_ray* d_rays, uint32_t n, 
   // counters (defined on the binary order) stay on the binary kernel.
   if (!any && !d_counts && eng->opts.bvh_traversal == 1) {
     HIP_TRY(eng, hipEventRecord(eng->ev_start, eng->stream));
-    if (vt_status s = engine_trace_cwbvh(eng, (const float4*)d_rays, n, nullptr, (uint32_t*)d_out, nullptr))
+    if (vt_status s = engine_trace_cwbvh(eng, (const float4*)d_rays, n, nullptr, (uint32_t*)d_out, nullptr,
+                                         eng->hit_filter))
       return s;
     HIP_TRY(eng, hipEventRecord(eng->ev_stop, eng->stream));
     eng->trace_timed = true;
@@ -637,3 +647,10 @@ vt_status vt_set_node_width(vt_engine* eng, uint32_t width) {
 
 static_assert(vt::kWideKids == vtk::kWideKids && vt::kNode128Words == 4 * vtk::kWideNodeFloat4s,
               "the wide-node layout of the builder and of the traversal kernel differ");
+
+vt_status vt_set_hit_filter(vt_engine* eng, uint32_t filter) {
+  if (!eng) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "engine is NULL");
+  if (filter > 1) return set_err(eng, VT_ERR_RANGE, "hit filter must be 0 (raw) or 1 (sided wrappers)");
+  eng->hit_filter = filter;
+  return VT_OK;
+}

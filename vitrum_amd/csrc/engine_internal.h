@@ -46,6 +46,7 @@ struct vt_engine {
   uint32_t near_first = 1;
   uint32_t defer_min = 12;
   uint32_t node_width = 4;  // 4: Node128 trees where the stack bound allows, 2: Node64
+  uint32_t hit_filter = 0;  // vt_trace_*: 0 raw traversal, 1 the path tracer's sided wrappers
   bool counting = false;
   DeviceBuffer counter, error_word, spill;
   DeviceBuffer tmp_rays, tmp_out, tmp_counts;
@@ -93,9 +94,10 @@ struct Scene;
 const vtw::Scene* engine_cwbvh_view(vt_engine* e);
 // Closest hit in the cwbvh-closest mode: 2 x float4 per ray in, 5 words per
 // ray out, optional world hit positions; the ray count is read from d_n when
-// it is set (n is then the capacity).
+// it is set (n is then the capacity); sided = 1 applies traceClosest's
+// single-sided re-trace.
 vt_status engine_trace_cwbvh(vt_engine* e, const float4* d_rays, uint32_t n, const uint32_t* d_n,
-                             uint32_t* d_hits, float* d_hit_pos);
+                             uint32_t* d_hits, float* d_hit_pos, uint32_t sided);
 // Cache hooks for the integrator (nrc.cpp): counts are device words.
 vt_status engine_nrc_ensure(vt_engine* e);
 vt_status engine_nrc_infer_counted(vt_engine* e, const float* d_feat, const uint32_t* d_count, float* d_rgb,

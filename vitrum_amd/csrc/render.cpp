@@ -470,8 +470,9 @@ vt_status vt_render_frame(vt_engine* e, const vt_frame* fr) {
     cp.hits = ps.hits;
     cp.hit_pos = ps.hit_pos;
     cp.n_ptr = ps.count;
+    cp.sided = 1;  // traceClosest: back faces of single-sided materials are skipped
     if (e->opts.bvh_traversal == 1) {
-      if (vt_status s = engine_trace_cwbvh(e, ps.rays, cap, ps.count, ps.hits, ps.hit_pos)) return s;
+      if (vt_status s = engine_trace_cwbvh(e, ps.rays, cap, ps.count, ps.hits, ps.hit_pos, 1u)) return s;
     } else {
       HIP_CHECK(e, vtk::launch_trace(cp, false, false, e->trace_grid, e->stream));
     }
@@ -481,6 +482,7 @@ vt_status vt_render_frame(vt_engine* e, const vt_frame* fr) {
       sp.rays = ps.shadow_rays;
       sp.occluded = ps.shadow_occ;
       sp.n_ptr = ps.count + 2;
+      sp.sided = 1;  // traceSurfaceVisibility: back faces of single-sided materials do not block
       HIP_CHECK(e, vtk::launch_trace(sp, true, false, e->trace_grid, e->stream));
       HIP_CHECK(e, vtr::launch_shadow_resolve(ps, cap, e->stream));
     }

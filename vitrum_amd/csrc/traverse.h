@@ -48,6 +48,11 @@ struct TraceParams {
   uint32_t near_first;    // closest hit: visit the nearer child first (ties as the reference)
   uint32_t defer_min;     // run a leaf / instance operation once this many lanes need it
   uint32_t wide;          // 1: walk the Node128 trees (every scene must have them)
+  // 1: the path tracer's wrappers -- closest hit skips back faces of
+  // single-sided materials (re-tracing beyond them), any hit ignores them
+  // (triangle record word 9 = 1 marks a single-sided triangle, instance word
+  // 35 = 1 an orientation-reversing instance); 0: raw traversal.
+  uint32_t sided;
   // Batched launch: rays [offsets[k], offsets[k+1]) belong to scenes[k].
   const DeviceScene* scenes;
   const uint32_t* scene_offsets;

@@ -123,6 +123,7 @@ struct TriHit {
   float t;
   float u, v;  // bary.y, bary.z
   V3 point;
+  bool front;  // side > 0: the ray meets the authored front face
 };
 
 // Ray-only part of the watertight test (opticalWatertightTriangle.ts:106-124):
@@ -159,24 +160,26 @@ __device__ __forceinline__ WtRay wt_setup(V3 dir) {
   return w;
 }
 
-// Rare tail of an in-window hit (opticalWatertightTriangle.ts:164-200): the
-// authored-normal side test.
-__device__ __noinline__ bool wt_side_ok(V3 dir, float ds, V3 a, V3 b, V3 c) {
+// Rare tail of an in-window hit (opticalWatertightTriangle.ts:164-220): the
+// authored-normal side test.  Returns 0 for a rejected hit, 1 for the front
+// face (side > 0), -1 for the back face.
+__device__ __noinline__ int wt_side(V3 dir, float ds, V3 a, V3 b, V3 c) {
   const V3 e1r = sub(b, a), e2r = sub(c, a);
   const float es = jmax(max3abs(e1r.x, e1r.y, e1r.z), max3abs(e2r.x, e2r.y, e2r.z));
-  if (!(es > 0.0f) || !finite_(es)) return false;
+  if (!(es > 0.0f) || !finite_(es)) return 0;
   const V3 s1 = v3(e1r.x / es, e1r.y / es, e1r.z / es);
   const V3 s2 = v3(e2r.x / es, e2r.y / es, e2r.z / es);
   const V3 nr = v3(s1.y * s2.z - s1.z * s2.y, s1.z * s2.x - s1.x * s2.z, s1.x * s2.y - s1.y * s2.x);
   const float ns = max3abs(nr.x, nr.y, nr.z);
-  if (!(ns > 0.0f) || !finite_(ns)) return false;
+  if (!(ns > 0.0f) || !finite_(ns)) return 0;
   const V3 nd = v3(nr.x / ns, nr.y / ns, nr.z / ns);
   const float nl = __builtin_sqrtf((nd.x * nd.x + nd.y * nd.y) + nd.z * nd.z);
-  if (!(nl > 0.0f) || !finite_(nl)) return false;
+  if (!(nl > 0.0f) || !finite_(nl)) return 0;
   const V3 n = v3(nd.x / nl, nd.y / nl, nd.z / nl);
   const V3 d = v3(dir.x / ds, dir.y / ds, dir.z / ds);
   const float side = -((d.x * n.x + d.y * n.y) + d.z * n.z);
-  return !(side == 0.0f || !finite_(side));
+  if (side == 0.0f || !finite_(side)) return 0;
+  return side > 0.0f ? 1 : -1;
 }
 
 // Returns true and fills h when the triangle is hit with lo < t < hi (or
@@ -228,7 +231,9 @@ __device__ __forceinline__ bool tri_test(V3 o, V3 dir, const WtRay& w, V3 a, V3 
   if (!(t > tmin && (t < hi || (incl_hi && t == hi)))) return false;  // traversal acceptance window
   // opticalWatertightTriangle.ts:164-219: authored-normal side test, then
   // barycentrics and the canonical hit point.
-  if (!wt_side_ok(dir, w.ds, a, b, c)) return false;
+  const int side = wt_side(dir, w.ds, a, b, c);
+  if (side == 0) return false;
+  h.front = side > 0;
   h.t = t;
   h.u = e1 / det;
   h.v = e2 / det;
@@ -358,13 +363,15 @@ __global__ __launch_bounds__(BLOCK, 4) void trace_kernel(TraceParams p) {
   float tmin = 0, tmax = 0, best = 0, bu = 0, bv = 0;
   V3 bpos{};
   uint32_t bprim = 0, binst = kMiss, bslot = 0;
+  bool bfront = false;  // the best hit is on the front face (world orientation)
   // Instance-local state.
   V3 lo{}, ld{}, linv{}, lpos{};
   WtRay lw{};
   bool lw_ready = false;
   float ltmin = 0, lbest = 0, lu = 0, lv = 0;
   uint32_t lprim = 0, inst_id = 0;
-  bool lhit = false;
+  bool lhit = false, lfront = false;
+  bool lmirror = false;  // the instance's local-to-world map reverses orientation
   uint32_t slot = 0, slot_end = 0, enter_slot = 0;
   uint32_t level = 0, cur = kNone, sp = 0, blas_base = 0;
   uint32_t pend = 0;  // pending deferred operation (kPend*)
@@ -391,6 +398,8 @@ __global__ __launch_bounds__(BLOCK, 4) void trace_kernel(TraceParams p) {
     }
     lbest = ltmax;
     lhit = false;
+    lfront = false;
+    lmirror = meta.w != 0u;
     inst_id = meta.y;
     level = 1;
     blas_base = sp;
@@ -411,6 +420,7 @@ __global__ __launch_bounds__(BLOCK, 4) void trace_kernel(TraceParams p) {
         bslot = slot;
         bu = lu;
         bv = lv;
+        bfront = lfront != lmirror;
       }
     }
   };
@@ -561,6 +571,7 @@ __global__ __launch_bounds__(BLOCK, 4) void trace_kernel(TraceParams p) {
               lu = h.u;
               lv = h.v;
               lpos = h.point;
+              lfront = h.front;
             }
           }
         } else {
@@ -603,10 +614,12 @@ __global__ __launch_bounds__(BLOCK, 4) void trace_kernel(TraceParams p) {
             ww.perm = (uint32_t)__shfl((int)lw.perm, o);
             const float w_tmin = __shfl(ltmin, o), w_hi = __shfl(lbest, o);
             const bool w_incl = __shfl((int)(near && lhit), o) != 0;
+            const bool w_mirror = __shfl((int)lmirror, o) != 0;
             uint32_t kt = 0xFFFFFFFFu, ki = 0xFFFFFFFFu, kl = lane;
             TriHit h;
             h.t = h.u = h.v = 0.0f;
             h.point = v3(0, 0, 0);
+            h.front = false;
             // A batched launch can hold rays of different scenes in one wave:
             // the pair's triangle is in its owner's scene.
             const float4* otris = tris;
@@ -620,8 +633,12 @@ __global__ __launch_bounds__(BLOCK, 4) void trace_kernel(TraceParams p) {
               const uint32_t t = o_first + (q - o_excl);
               const float4* tr = otris + 3 * (size_t)t;
               const float4 w0 = tr[0], w1 = tr[1], w2 = tr[2];
+              // Any hit through the path tracer's wrappers (traceAny /
+              // traceSurfaceVisibility): a back face of a single-sided
+              // material does not block.
               if (tri_test(wo, wd, ww, v3(w0.x, w0.y, w0.z), v3(w0.w, w1.x, w1.y), v3(w1.z, w1.w, w2.x), w_tmin,
-                           w_hi, w_incl, h)) {
+                           w_hi, w_incl, h) &&
+                  (!ANY || !p.sided || h.front != w_mirror || !(w2.y > 0.5f))) {
                 kt = f2u(h.t);  // t > 0: the bit patterns order as the values
                 ki = t;
               }
@@ -646,6 +663,7 @@ __global__ __launch_bounds__(BLOCK, 4) void trace_kernel(TraceParams p) {
             const int rl = __shfl((int)kl, src);
             const float ru = __shfl(h.u, rl), rv = __shfl(h.v, rl);
             const V3 rp = v3(__shfl(h.point.x, rl), __shfl(h.point.y, rl), __shfl(h.point.z, rl));
+            const bool rf = __shfl((int)h.front, rl) != 0;
             if (here && rt != 0xFFFFFFFFu) {
               if (ANY) {
                 occl = true;
@@ -658,6 +676,7 @@ __global__ __launch_bounds__(BLOCK, 4) void trace_kernel(TraceParams p) {
                   lu = ru;
                   lv = rv;
                   lpos = rp;
+                  lfront = rf;
                 }
               }
             }
@@ -688,6 +707,25 @@ __global__ __launch_bounds__(BLOCK, 4) void trace_kernel(TraceParams p) {
               break;
             }
             continue;
+          }
+          // traceClosest (intersection.wgsl.ts:462-490): a back-face hit on a
+          // single-sided material is not a surface; the trace starts again
+          // with its distance as the exclusive lower bound, and ends as a
+          // miss once that bound stops advancing or reaches tMax.
+          if (!ANY && p.sided && binst != kMiss && !bfront && tris[3 * (size_t)bprim + 2].y > 0.5f) {
+            if (best > tmin && best < tmax) {
+              tmin = best;
+              best = tmax;
+              binst = kMiss;
+              bprim = 0;
+              bslot = 0;
+              bu = bv = 0.0f;
+              level = 0;
+              sp = 0;
+              cur = tlas_root;
+              break;
+            }
+            binst = kMiss;
           }
           finish(false);
           break;
@@ -1018,13 +1056,14 @@ struct LocalHit {
   V3 pos;
   uint32_t tri;
   float u, v;
+  bool front;  // local front face (the watertight test's side)
 };
 
 // traceMeshBvh over the reference node layout (8 words per node), left child
 // first, closest hit.
 __device__ __noinline__ LocalHit mesh_binary(const Scene& s, V3 o, V3 d, V3 inv, const WtRay& w, float tmin,
                                              float tmax, uint32_t root) {
-  LocalHit h{false, tmax, vtk::v3(0, 0, 0), 0u, 0.0f, 0.0f};
+  LocalHit h{false, tmax, vtk::v3(0, 0, 0), 0u, 0.0f, 0.0f, false};
   if (s.bvh_node_count == 0 || root >= s.bvh_node_count) return h;
   uint32_t stack[kBinStack];
   uint32_t sp = 0;
@@ -1054,6 +1093,7 @@ __device__ __noinline__ LocalHit mesh_binary(const Scene& s, V3 o, V3 d, V3 inv,
           h.tri = t;
           h.u = th.u;
           h.v = th.v;
+          h.front = th.front;
         }
       }
     } else {
@@ -1163,7 +1203,7 @@ __device__ __forceinline__ LocalHit mesh_cwbvh(const Scene& s, V3 o, V3 d, V3 in
   if (s.wide_count == 0 || wide_root >= s.wide_count) return mesh_binary(s, o, d, inv, w, tmin, tmax, binary_root);
   const WideHit c = wide_walk(s, o, d, inv, tmin, tmax, wide_root);
   if (c.status != 0) return mesh_binary(s, o, d, inv, w, tmin, tmax, binary_root);
-  LocalHit h{false, tmax, vtk::v3(0, 0, 0), 0u, 0.0f, 0.0f};
+  LocalHit h{false, tmax, vtk::v3(0, 0, 0), 0u, 0.0f, 0.0f, false};
   if (!c.hit || c.t <= tmin || c.t >= tmax) return h;
   if (c.tri >= s.tri_count) return mesh_binary(s, o, d, inv, w, tmin, tmax, binary_root);
   const uint4 ix = s.indices[c.tri];
@@ -1177,6 +1217,7 @@ __device__ __forceinline__ LocalHit mesh_cwbvh(const Scene& s, V3 o, V3 d, V3 in
   h.tri = c.tri;
   h.u = e.u;
   h.v = e.v;
+  h.front = e.front;
   return h;
 }
 
@@ -1193,6 +1234,7 @@ struct WorldHit {
   uint32_t prim, inst;
   float u, v;
   V3 pos;
+  bool front;  // world front face (flipped by an orientation-reversing instance)
 };
 
 // One instance of traceTlasClosestCwbvh / tlasInstanceClosest: the local ray,
@@ -1229,6 +1271,7 @@ __device__ __forceinline__ void instance_closest(const Scene& s, uint32_t inst, 
   b.u = lh.u;
   b.v = lh.v;
   b.pos = wpos;
+  b.front = lh.front != (s.inst_reversed[inst] != 0);
 }
 
 __global__ __launch_bounds__(128) void trace_cwbvh_kernel(Params p) {
@@ -1239,56 +1282,70 @@ __global__ __launch_bounds__(128) void trace_cwbvh_kernel(Params p) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const float4 ra = p.rays[2 * (size_t)i], rb = p.rays[2 * (size_t)i + 1];
     const V3 ro = vtk::v3(ra.x, ra.y, ra.z), rd = vtk::v3(rb.x, rb.y, rb.z);
-    const float tmin = ra.w, tmax = rb.w;
+    const float tmax = rb.w;
     const V3 rinv = vtk::v3(vtk::safe_inv(rd.x), vtk::safe_inv(rd.y), vtk::safe_inv(rd.z));
     WorldHit b;
-    // Mode 0: wide BLAS walks; 1: binary BLAS walks (a TLAS stack overflow in
-    // mode 0 restarts here, as traceTlasClosestCwbvh falls back to
-    // traceTlasClosest); 2: every instance in permutation order
-    // (tlasClosestFallback, on a TLAS stack overflow in mode 1).
-    uint32_t mode = (s.tlas_node_count == 0 || s.inst_index_count == 0) ? 3u : 0u;
+    // traceClosest (intersection.wgsl.ts:462-490) with p.sided: a back-face
+    // hit on a single-sided material is skipped by tracing again from its
+    // distance (exclusive); a bound that stops advancing or reaches tMax
+    // ends the ray as a miss.
+    float tmin = ra.w;
     for (;;) {
-      b.t = tmax;
-      b.prim = 0;
-      b.inst = vtk::kMiss;
-      b.u = b.v = 0.0f;
-      b.pos = vtk::v3(0, 0, 0);
-      if (mode == 3) break;  // no TLAS: fail closed
-      if (mode == 2) {
-        for (uint32_t pi = 0; pi < s.inst_index_count; ++pi)
-          instance_closest(s, s.inst_indices[pi], false, ro, rd, tmin, b);
+      // Mode 0: wide BLAS walks; 1: binary BLAS walks (a TLAS stack overflow in
+      // mode 0 restarts here, as traceTlasClosestCwbvh falls back to
+      // traceTlasClosest); 2: every instance in permutation order
+      // (tlasClosestFallback, on a TLAS stack overflow in mode 1).
+      uint32_t mode = (s.tlas_node_count == 0 || s.inst_index_count == 0) ? 3u : 0u;
+      for (;;) {
+        b.t = tmax;
+        b.prim = 0;
+        b.inst = vtk::kMiss;
+        b.u = b.v = 0.0f;
+        b.pos = vtk::v3(0, 0, 0);
+        b.front = false;
+        if (mode == 3) break;  // no TLAS: fail closed
+        if (mode == 2) {
+          for (uint32_t pi = 0; pi < s.inst_index_count; ++pi)
+            instance_closest(s, s.inst_indices[pi], false, ro, rd, tmin, b);
+          break;
+        }
+        uint32_t stack[kTlasStack];
+        uint32_t sp = 0;
+        stack[sp++] = 0;
+        bool overflow = false;
+        while (sp > 0 && !overflow) {
+          const uint32_t ni = stack[--sp];
+          if (ni >= s.tlas_node_count) continue;
+          const uint4* nd = reinterpret_cast<const uint4*>(s.tlas_nodes + 8 * (size_t)ni);
+          const uint4 lo = nd[0], hi = nd[1];
+          float tn;
+          if (!vtk::slab(ro, rinv, vtk::u2f(lo.x), vtk::u2f(lo.y), vtk::u2f(lo.z), vtk::u2f(lo.w),
+                         vtk::u2f(hi.x), vtk::u2f(hi.y), tmin, tn) ||
+              tn > b.t)
+            continue;
+          if ((hi.w & kLeafMask) == kLeafMask) {
+            const uint32_t count = hi.w & 0xFFFFu, start = hi.z;
+            for (uint32_t k = 0; k < count; ++k) {
+              const uint32_t pi = start + k;
+              if (pi >= s.inst_index_count) continue;
+              instance_closest(s, s.inst_indices[pi], mode == 0, ro, rd, tmin, b);
+            }
+          } else if (sp + 2 <= kTlasStack) {
+            stack[sp++] = ni + hi.z;
+            stack[sp++] = ni + 1;
+          } else {
+            overflow = true;
+          }
+        }
+        if (!overflow) break;
+        ++mode;
+      }
+      if (!p.sided || b.inst == vtk::kMiss || b.front || b.prim >= s.tri_count || !s.single_sided[b.prim]) break;
+      if (!(b.t > tmin) || !(b.t < tmax)) {
+        b.inst = vtk::kMiss;
         break;
       }
-      uint32_t stack[kTlasStack];
-      uint32_t sp = 0;
-      stack[sp++] = 0;
-      bool overflow = false;
-      while (sp > 0 && !overflow) {
-        const uint32_t ni = stack[--sp];
-        if (ni >= s.tlas_node_count) continue;
-        const uint4* nd = reinterpret_cast<const uint4*>(s.tlas_nodes + 8 * (size_t)ni);
-        const uint4 lo = nd[0], hi = nd[1];
-        float tn;
-        if (!vtk::slab(ro, rinv, vtk::u2f(lo.x), vtk::u2f(lo.y), vtk::u2f(lo.z), vtk::u2f(lo.w),
-                       vtk::u2f(hi.x), vtk::u2f(hi.y), tmin, tn) ||
-            tn > b.t)
-          continue;
-        if ((hi.w & kLeafMask) == kLeafMask) {
-          const uint32_t count = hi.w & 0xFFFFu, start = hi.z;
-          for (uint32_t k = 0; k < count; ++k) {
-            const uint32_t pi = start + k;
-            if (pi >= s.inst_index_count) continue;
-            instance_closest(s, s.inst_indices[pi], mode == 0, ro, rd, tmin, b);
-          }
-        } else if (sp + 2 <= kTlasStack) {
-          stack[sp++] = ni + hi.z;
-          stack[sp++] = ni + 1;
-        } else {
-          overflow = true;
-        }
-      }
-      if (!overflow) break;
-      ++mode;
+      tmin = b.t;
     }
     const bool hit = b.inst != vtk::kMiss;
     uint32_t* h = p.hits + 5 * (size_t)i;

@@ -31,7 +31,7 @@ EXPORTED = [
     "vt_build_blas", "vt_build_tlas", "vt_pack_scene", "vt_pack_field_words",
     "vt_pack_copy_field", "vt_pack_free", "vt_version", "vt_render_frame", "vt_capture",
     "vt_reset", "vt_frame_time", "vt_capture_signatures", "vt_render_info", "vt_light_tree", "vt_invert_mat4",
-    "vt_set_bvh_traversal", "vt_set_node_width",
+    "vt_set_bvh_traversal", "vt_set_node_width", "vt_set_hit_filter",
     "vt_nrc_param_count", "vt_nrc_reset", "vt_nrc_set_weights", "vt_nrc_get_weights", "vt_nrc_infer",
     "vt_nrc_infer_device", "vt_nrc_train_step", "vt_nrc_train_step_device", "vt_nrc_last_loss",
     "vt_nrc_gradient", "vt_nrc_render_config", "vt_nrc_timing", "vt_batch_create", "vt_batch_destroy",
@@ -301,6 +301,10 @@ class Engine:
     def set_node_width(self, width):
         """4: 4-wide trees collapsed from the binary BVH (default); 2: binary."""
         check(lib().vt_set_node_width(self._h, C.c_uint32(width)), self._h)
+
+    def set_hit_filter(self, sided):
+        """0: raw traversal (default); 1: the path tracer's sided wrappers."""
+        check(lib().vt_set_hit_filter(self._h, C.c_uint32(sided)), self._h)
 
     def set_bvh_traversal(self, mode):
         """0: binary BVH; 1: cwbvh-closest (closest-hit rays only)."""
