@@ -336,14 +336,17 @@ void vt_pack_free(vt_pack *p);
  * direction and normal as (theta/pi, phi/(2 pi)), roughness, diffuse albedo
  * (3), specular albedo (3), 2 zero floats; targets are RGB. */
 typedef struct vt_nrc_options {
-  float learning_rate; /* 1e-2 */
-  float beta1;         /* 0.9 */
-  float beta2;         /* 0.999 */
-  float epsilon;       /* 1e-8 */
-  uint32_t seed;       /* initial weights */
-  uint32_t reserved[3];
+  float learning_rate;       /* 1e-2 (MLP weights) */
+  float beta1;               /* 0.9 */
+  float beta2;               /* 0.999 */
+  float epsilon;             /* 1e-8 */
+  uint32_t seed;             /* initial weights */
+  float table_learning_rate; /* hash-grid tables; 0 = 1e-1 (nrcSubsystem.ts tableLearningRate) */
+  uint32_t reserved[2];
 } vt_nrc_options;
-/* Number of FP32 master weights (W0..W3 64x64 then W4 3x64, row-major [out][in]). */
+/* Number of FP32 master parameters: W0..W3 (64x64) then W4 (3x64), row-major
+ * [out][in], then the hash-grid tables (8 levels x 4096 rows x 2 features,
+ * row-major [level][row][feature]) -- 82,112 in all. */
 vt_status vt_nrc_param_count(uint32_t *count);
 /* Re-initialises weights (from opts->seed) and Adam state; NULL keeps options. */
 vt_status vt_nrc_reset(vt_engine *eng, const vt_nrc_options *opts);

@@ -1,4 +1,5 @@
-"""GPU neural radiance cache (vitrum_amd/csrc/nrc.hip) against the binary64 oracle.
+"""GPU neural radiance cache (vitrum_amd/csrc/nrc.hip) against the binary64 oracle:
+hash-grid + one-blob encoding, BF16 MLP, table and weight gradients, Adam.
 
 Tolerances (bf16 network, FP32 MFMA accumulation vs binary64 with the same
 bf16 roundings): an f32-vs-f64 difference in a pre-activation can flip one
@@ -13,13 +14,21 @@ from oracle import nrc as N
 pytestmark = pytest.mark.gpu
 
 
+def unit(rng, n):
+    d = rng.normal(size=(n, 3))
+    return d / np.linalg.norm(d, axis=1, keepdims=True)
+
+
 def features(n, seed):
+    """Raw cache records: position in [0,1]^3, unit direction and normal,
+    roughness, diffuse albedo."""
     rng = np.random.default_rng(seed)
     f = np.zeros((n, 16), np.float32)
     f[:, 0:3] = rng.random((n, 3))
-    f[:, 3:7] = rng.random((n, 4))
-    f[:, 7] = rng.random(n)
-    f[:, 8:14] = rng.random((n, 6))
+    f[:, 3:6] = unit(rng, n)
+    f[:, 6:9] = unit(rng, n)
+    f[:, 9] = rng.random(n)
+    f[:, 10:13] = rng.random((n, 3))
     return f
 
 
@@ -32,11 +41,11 @@ def smooth_target(f):
     radiance field for convergence tests)."""
     p = f[:, 0:3].astype(np.float64)
     base = 0.5 + 0.4 * np.sin(2 * np.pi * p[:, 0]) * np.cos(np.pi * p[:, 1]) + 0.3 * p[:, 2]
-    return (f[:, 8:11] * (base * (0.6 + 0.4 * f[:, 3]))[:, None] + 0.05).astype(np.float32)
+    return (f[:, 10:13] * (base * (0.8 + 0.2 * f[:, 3]))[:, None] + 0.05).astype(np.float32)
 
 
 def test_param_count(engine):
-    assert engine.nrc_param_count() == N.N_PARAMS == 16576
+    assert engine.nrc_param_count() == N.N_PARAMS == 16576 + 8 * 4096 * 2
 
 
 @pytest.mark.parametrize("n", [1, 63, 64, 1000, 4099])
@@ -68,7 +77,7 @@ def test_gradient_matches_oracle(engine, n):
     loss, g = engine.nrc_gradient(f, t)
     rloss, rg = N.gradient(w, f, t)
     assert loss == pytest.approx(rloss, rel=2e-2)
-    offs = [0, 4096, 8192, 12288, 16384, 16576]
+    offs = [0, 4096, 8192, 12288, 16384, 16576, N.N_PARAMS]
     for a, b in zip(offs[:-1], offs[1:]):
         d = np.linalg.norm(g[a:b] - rg[a:b]) / (np.linalg.norm(rg[a:b]) + 1e-30)
         cos = float(np.dot(g[a:b], rg[a:b]) / (np.linalg.norm(g[a:b]) * np.linalg.norm(rg[a:b]) + 1e-30))

@@ -58,6 +58,9 @@ static void init_weights(uint32_t seed, std::vector<float>& w) {
   };
   for (uint32_t l = 0; l < vtn::kHidden; ++l) fill((size_t)l * 4096, 64, 64);
   fill(16384, vtn::kOut, 64);
+  // Hash-grid tables: U(-1e-4, 1e-4) (nrcSubsystem.ts:541, Instant-NGP section 3).
+  for (size_t i = 0; i < vtn::kTableParams; ++i)
+    w[vtn::kMlpParams + i] = (float)((((double)next() / 4294967296.0) * 2.0 - 1.0) * 1e-4);
 }
 
 static vt_status nrc_ensure(vt_engine* e) {
@@ -68,6 +71,7 @@ static vt_status nrc_ensure(vt_engine* e) {
   s->opts.beta2 = 0.999f;
   s->opts.epsilon = 1e-8f;
   s->opts.seed = 1;
+  s->opts.table_learning_rate = 1e-1f;
   e->nrc = s;
   vt_status st = VT_OK;
   const size_t pb = vtn::kParams * sizeof(float);
@@ -111,7 +115,8 @@ vt_status engine_nrc_ensure(vt_engine* e) { return nrc_ensure(e); }
 vt_status engine_nrc_infer_counted(vt_engine* e, const float* d_feat, const uint32_t* d_count, float* d_rgb,
                                    uint32_t capacity) {
   if (vt_status s = nrc_ensure(e)) return s;
-  NRC_HIP(e, vtn::launch_infer(d_feat, capacity, d_count, (const uint4*)e->nrc->frags.ptr, d_rgb, e->stream));
+  NRC_HIP(e, vtn::launch_infer(d_feat, capacity, d_count, (const uint4*)e->nrc->frags.ptr,
+                               (const float*)e->nrc->master.ptr + vtn::kMlpParams, d_rgb, e->stream));
   return VT_OK;
 }
 
@@ -140,9 +145,11 @@ vt_status vt_nrc_reset(vt_engine* e, const vt_nrc_options* opts) {
   NrcState& s = *e->nrc;
   if (opts) {
     if (!(opts->learning_rate > 0) || !(opts->beta1 >= 0 && opts->beta1 < 1) ||
-        !(opts->beta2 >= 0 && opts->beta2 < 1) || !(opts->epsilon > 0))
-      return nrc_err(e, VT_ERR_RANGE, "learning_rate > 0, 0 <= beta1, beta2 < 1 and epsilon > 0 are required");
+        !(opts->beta2 >= 0 && opts->beta2 < 1) || !(opts->epsilon > 0) || !(opts->table_learning_rate >= 0))
+      return nrc_err(e, VT_ERR_RANGE,
+                     "learning_rate > 0, table_learning_rate >= 0, 0 <= beta1, beta2 < 1 and epsilon > 0 are required");
     s.opts = *opts;
+    if (s.opts.table_learning_rate == 0.0f) s.opts.table_learning_rate = 1e-1f;
   }
   std::vector<float> w;
   init_weights(s.opts.seed, w);
@@ -171,7 +178,8 @@ vt_status vt_nrc_infer_device(vt_engine* e, const float* d_features, uint32_t n,
   if (vt_status s = nrc_ensure(e)) return s;
   NRC_HIP(e, hipSetDevice(e->device));
   NRC_HIP(e, hipEventRecord(e->nrc->ev[0], e->stream));
-  NRC_HIP(e, vtn::launch_infer(d_features, n, nullptr, (const uint4*)e->nrc->frags.ptr, d_rgb, e->stream));
+  NRC_HIP(e, vtn::launch_infer(d_features, n, nullptr, (const uint4*)e->nrc->frags.ptr,
+                               (const float*)e->nrc->master.ptr + vtn::kMlpParams, d_rgb, e->stream));
   NRC_HIP(e, hipEventRecord(e->nrc->ev[1], e->stream));
   e->nrc->timed_infer = true;
   return VT_OK;
@@ -199,8 +207,9 @@ static vt_status nrc_accumulate(vt_engine* e, const float* d_features, const flo
   NRC_HIP(e, hipSetDevice(e->device));
   NRC_HIP(e, vtn::launch_fill((float*)s.loss.ptr, 1, 0.0f, e->stream));
   NRC_HIP(e, hipEventRecord(s.ev[2], e->stream));
-  NRC_HIP(e, vtn::launch_train(d_features, d_targets, n, d_count, (const uint4*)s.frags.ptr, (float*)s.grad.ptr,
-                               (float*)s.loss.ptr, e->stream));
+  NRC_HIP(e, vtn::launch_train(d_features, d_targets, n, d_count, (const uint4*)s.frags.ptr,
+                               (const float*)s.master.ptr + vtn::kMlpParams, (float*)s.grad.ptr, (float*)s.loss.ptr,
+                               e->stream));
   NRC_HIP(e, hipEventRecord(s.ev[3], e->stream));
   s.timed_train = true;
   return VT_OK;
@@ -211,6 +220,7 @@ static vt_status nrc_apply_adam(vt_engine* e) {
   s.step += 1;
   vtn::AdamParams p{};
   p.lr = s.opts.learning_rate;
+  p.lr_table = s.opts.table_learning_rate;
   p.beta1 = s.opts.beta1;
   p.beta2 = s.opts.beta2;
   p.eps = s.opts.epsilon;

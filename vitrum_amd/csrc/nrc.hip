@@ -2,6 +2,11 @@
 //
 // Network (Mueller et al. 2021, Real-time Neural Radiance Caching): encoded
 // input x (64) -> H1..H4 = ReLU(W_l H_{l-1}) (64 wide) -> y = W4 H4 (RGB).
+// Input encoding (walkaround-hybrid src/neural/nrc/nrcEncoding.ts): a
+// multiresolution hash grid of the position with trainable feature tables
+// (Instant-NGP), one-blob bins of the octahedral direction, raw surface
+// features.  The tables receive d loss / d input through W0^T, scattered with
+// the trilinear weights.
 // Loss: relative L2 of walkaround-hybrid/src/neural/nrc/wgsl/fusedMlp.wgsl.ts
 // :65-120 (luminance of the prediction as a stop-gradient denominator,
 // epsilon 1e-2); Adam as in walkaround-hybrid/src/neural/nrc/
@@ -59,26 +64,115 @@ __host__ __device__ __forceinline__ uint32_t nmap(uint32_t kb, uint32_t g, uint3
   return (2u * kb + (t >> 2)) * 16u + g * 4u + (t & 3u);
 }
 
-// Encoded input dimension d of a raw feature record (see nrc.h).
-//   0-35  sin(pi * 2^k * p_a), axis a = d / 12, k = d % 12 (12 frequencies)
-//   36-51 one-blob, 4 Gaussian bins (sigma = 1/4) for each of the 4 angles
-//   52    1 - exp(-roughness); 53-55 diffuse albedo; 56-58 specular albedo
-//   63    1 (bias input); others 0
-__device__ __forceinline__ float encode_dim(uint32_t d, const float* f) {
-  if (d < 36u) {
-    const float t = f[d / 12u] * (float)(1u << (d % 12u));
-    const float r = t - 2.0f * floorf(t * 0.5f);
-    return sinpif(r);
+// ---- input encoding (nrcEncoding.ts) --------------------------------------
+
+// Instant-NGP spatial hash (nrcEncoding.ts:spatialHash3D): XOR of the corner
+// coordinates times 1, 2654435761 and 805459861 (32-bit wraparound), modulo
+// the row count (a power of two).
+__device__ __forceinline__ uint32_t grid_hash(uint32_t ix, uint32_t iy, uint32_t iz) {
+  return (ix ^ (iy * 2654435761u) ^ (iz * 805459861u)) & (kRows - 1u);
+}
+
+// Trilinear corners of level l for a normalised position: row and weight.
+struct Corners {
+  uint32_t row[8];
+  float w[8];
+};
+
+__device__ __forceinline__ void grid_corners(uint32_t l, float nx, float ny, float nz, Corners& c) {
+  const float res = (float)(kMinRes << l);
+  const float px = nx * res, py = ny * res, pz = nz * res;
+  const float x0 = floorf(px), y0 = floorf(py), z0 = floorf(pz);
+  const float fx = px - x0, fy = py - y0, fz = pz - z0;
+  const uint32_t ix = (uint32_t)x0, iy = (uint32_t)y0, iz = (uint32_t)z0;
+#pragma unroll
+  for (uint32_t k = 0; k < 8; ++k) {
+    const uint32_t cx = k & 1u, cy = (k >> 1) & 1u, cz = (k >> 2) & 1u;
+    const float wx = cx ? fx : 1.0f - fx, wy = cy ? fy : 1.0f - fy, wz = cz ? fz : 1.0f - fz;
+    c.w[k] = (wx * wy) * wz;
+    c.row[k] = l * kRows + grid_hash(ix + cx, iy + cy, iz + cz);
   }
-  if (d < 52u) {
-    const uint32_t q = (d - 36u) >> 2, b = (d - 36u) & 3u;
-    const float e = (f[3u + q] - ((float)b + 0.5f) * 0.25f) * 4.0f;
-    return expf(-0.5f * (e * e));
+}
+
+// octEncodeDir (nrcEncoding.ts): octahedral map of a direction to [0,1]^2.
+__device__ __forceinline__ void oct_encode(float dx, float dy, float dz, float& u, float& v) {
+  const float sc = fmaxf(fabsf(dx), fmaxf(fabsf(dy), fabsf(dz)));
+  if (!(sc > 0.0f) || !(sc <= 3.402823e38f)) {
+    u = v = 0.5f;
+    return;
   }
-  if (d == 52u) return 1.0f - expf(-f[7]);
-  if (d < 56u) return f[8u + d - 53u];
-  if (d < 59u) return f[11u + d - 56u];
-  return d == 63u ? 1.0f : 0.0f;
+  const float sx = dx / sc, sy = dy / sc, sz = dz / sc;
+  const float inv = 1.0f / ((fabsf(sx) + fabsf(sy)) + fabsf(sz));
+  float px = sx * inv, py = sy * inv;
+  if (sz < 0.0f) {
+    const float ox = (1.0f - fabsf(py)) * (px >= 0.0f ? 1.0f : -1.0f);
+    const float oy = (1.0f - fabsf(px)) * (py >= 0.0f ? 1.0f : -1.0f);
+    px = ox;
+    py = oy;
+  }
+  u = px * 0.5f + 0.5f;
+  v = py * 0.5f + 0.5f;
+}
+
+// One-blob bins b0..b0+3 of a scalar (oneBlobEncodeScalar: Gaussian kernels
+// at (i + 0.5) / k, sigma 1/k, normalised to unit L1 over the k bins).
+__device__ __forceinline__ void one_blob4(float x, uint32_t b0, float* out) {
+  const float uc = fminf(fmaxf(x, 0.0f), 1.0f);
+  float a[kBlobBins], sum = 0.0f;
+#pragma unroll
+  for (uint32_t i = 0; i < kBlobBins; ++i) {
+    const float d = (uc - ((float)i + 0.5f) / (float)kBlobBins) * (float)kBlobBins;
+    a[i] = expf(-0.5f * (d * d));
+    sum += a[i];
+  }
+#pragma unroll
+  for (uint32_t j = 0; j < 4; ++j) {
+    float x4 = 0.0f;
+#pragma unroll
+    for (uint32_t i = 0; i < kBlobBins; ++i)
+      if (i == b0 + j) x4 = a[i];
+    out[j] = sum > 1e-20f ? x4 / sum : x4;
+  }
+}
+
+// The 16 encoded inputs a lane of group g supplies for one sample, in the
+// K order of nmap: kb 0 -> dims {4g..4g+3, 16+4g..16+4g+3}, kb 1 -> dims
+// {32+4g..32+4g+3, 48+4g..48+4g+3}.  Lane group g owns hash-grid levels 2g
+// and 2g + 1.
+__device__ __forceinline__ void encode_lane(uint32_t g, const float* f, const float* __restrict__ tables,
+                                            float (&v)[2][8]) {
+  // hash grid, levels 2g, 2g + 1
+#pragma unroll
+  for (uint32_t j = 0; j < 2; ++j) {
+    Corners c;
+    grid_corners(2u * g + j, f[0], f[1], f[2], c);
+    float f0 = 0.0f, f1 = 0.0f;
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k) {
+      const float2 t = *reinterpret_cast<const float2*>(tables + 2u * (size_t)c.row[k]);
+      f0 += c.w[k] * t.x;
+      f1 += c.w[k] * t.y;
+    }
+    v[0][2 * j] = f0;
+    v[0][2 * j + 1] = f1;
+  }
+  // one-blob of the direction: groups 0, 1 -> u bins 0-3, 4-7; groups 2, 3 -> v
+  float ou, ov;
+  oct_encode(f[3], f[4], f[5], ou, ov);
+  one_blob4(g < 2u ? ou : ov, (g & 1u) * 4u, &v[0][4]);
+  // raw features: dims 32-34 normal, 35 roughness, 36-38 albedo
+#pragma unroll
+  for (uint32_t t = 0; t < 8; ++t) v[1][t] = 0.0f;
+  if (g == 0u) {
+    v[1][0] = f[6];
+    v[1][1] = f[7];
+    v[1][2] = f[8];
+    v[1][3] = f[9];
+  } else if (g == 1u) {
+    v[1][0] = f[10];
+    v[1][1] = f[11];
+    v[1][2] = f[12];
+  }
 }
 
 // ---- packing: FP32 master weights -> permuted bf16 fragments --------------
@@ -97,6 +191,10 @@ __global__ void pack_kernel(const float* __restrict__ w, uint4* __restrict__ fra
     // forward, output layer: rows 0..2 of W4, zero padding to 16 rows
     const uint32_t kb = ((id - kHidden * kFragsHidden) / 64u) & 1u;
     for (uint32_t t = 0; t < 8; ++t) v[t] = r < kOut ? w[16384u + r * 64u + nmap(kb, g, t)] : 0.0f;
+  } else if (id >= kFragsFwd + kFragsBwd) {
+    // input gradient of dims 0..15: A = W0^T, row i = r, k = output neuron
+    const uint32_t kb = ((id - kFragsFwd - kFragsBwd) / 64u) & 1u;
+    for (uint32_t t = 0; t < 8; ++t) v[t] = w[nmap(kb, g, t) * 64u + r];
   } else {
     // backward through W_m (m = 1..4): A = W_m^T, row i = ib*16 + r, k = output neuron
     const uint32_t b = id - kFragsFwd;
@@ -117,30 +215,27 @@ __global__ void pack_kernel(const float* __restrict__ w, uint4* __restrict__ fra
 // ---- shared forward ------------------------------------------------------------
 
 // Builds the B fragments of the encoded inputs of 4 sample blocks.
-__device__ __forceinline__ void encode_tile(const float* __restrict__ feats, uint32_t n, uint32_t base,
-                                            uint32_t g, uint32_t r, uint4 (&B)[4][2]) {
+__device__ __forceinline__ void encode_tile(const float* __restrict__ feats, const float* __restrict__ tables,
+                                            uint32_t n, uint32_t base, uint32_t g, uint32_t r, uint4 (&B)[4][2]) {
 #pragma unroll
   for (uint32_t sb = 0; sb < 4; ++sb) {
     const uint32_t s = base + sb * 16u + r;
-    float f[kFeat];
+    float v[2][8];
     if (s < n) {
+      float f[kFeat];
       const float4* fp = reinterpret_cast<const float4*>(feats + (size_t)s * kFeat);
 #pragma unroll
       for (uint32_t q = 0; q < 4; ++q) {
         const float4 x = fp[q];
         f[4 * q] = x.x, f[4 * q + 1] = x.y, f[4 * q + 2] = x.z, f[4 * q + 3] = x.w;
       }
+      encode_lane(g, f, tables, v);
     } else {
 #pragma unroll
-      for (uint32_t q = 0; q < kFeat; ++q) f[q] = 0.0f;
+      for (uint32_t t = 0; t < 8; ++t) v[0][t] = v[1][t] = 0.0f;
     }
-#pragma unroll
-    for (uint32_t kb = 0; kb < 2; ++kb) {
-      float v[8];
-#pragma unroll
-      for (uint32_t t = 0; t < 8; ++t) v[t] = s < n ? encode_dim(nmap(kb, g, t), f) : 0.0f;
-      B[sb][kb] = pack8(v);
-    }
+    B[sb][0] = pack8(v[0]);
+    B[sb][1] = pack8(v[1]);
   }
 }
 
@@ -194,13 +289,14 @@ __device__ __forceinline__ void output_layer(const uint4* __restrict__ frags, ui
 
 __global__ __launch_bounds__(256) void infer_kernel(const float* __restrict__ feats, uint32_t cap,
                                                     const uint32_t* __restrict__ n_ptr,
-                                                    const uint4* __restrict__ frags, float* __restrict__ rgb) {
+                                                    const uint4* __restrict__ frags,
+                                                    const float* __restrict__ tables, float* __restrict__ rgb) {
   const uint32_t n = n_ptr ? min(*n_ptr, cap) : cap;
   const uint32_t lane = threadIdx.x & 63u, g = lane >> 4, r = lane & 15u;
   const uint32_t base = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64u;
   if (base >= n) return;  // whole wave past the end
   uint4 B[4][2];
-  encode_tile(feats, n, base, g, r, B);
+  encode_tile(feats, tables, n, base, g, r, B);
 #pragma unroll
   for (uint32_t l = 0; l < kHidden; ++l) (void)hidden_layer(frags + l * kFragsHidden, lane, B);
   f32x4 Y[4];
@@ -293,7 +389,7 @@ __device__ __forceinline__ void backward_layer(const uint4* __restrict__ fr, uin
 
 __global__ __launch_bounds__(64) void train_kernel(const float* __restrict__ feats, const float* __restrict__ targets,
                                                    uint32_t cap, const uint32_t* __restrict__ n_ptr,
-                                                   const uint4* __restrict__ frags,
+                                                   const uint4* __restrict__ frags, const float* __restrict__ tables,
                                                    float* __restrict__ grad, float* __restrict__ loss) {
   const uint32_t n = n_ptr ? min(*n_ptr, cap) : cap;
   if (blockIdx.x * 64u >= n) return;  // whole block past the end (uniform)
@@ -303,7 +399,7 @@ __global__ __launch_bounds__(64) void train_kernel(const float* __restrict__ fea
   const uint32_t base = blockIdx.x * 64u;
   uint4 B[4][2];
   uint64_t mask[kHidden];
-  encode_tile(feats, n, base, g, r, B);
+  encode_tile(feats, tables, n, base, g, r, B);
   store_t(HT, g, r, B);
 #pragma unroll
   for (uint32_t l = 0; l < kHidden; ++l) {
@@ -376,6 +472,39 @@ __global__ __launch_bounds__(64) void train_kernel(const float* __restrict__ fea
   store_t(DT, g, r, B);  // dH1
   __syncthreads();
   weight_grad(DT, HT, 4u, 64u, g, r, grad);
+  // d loss / d x of the hash-grid inputs (dims 0..15) = W0^T dH1: lane (g, r)
+  // receives dims 4g..4g+3 -- the features of levels 2g, 2g + 1 that it
+  // encoded -- of samples sb*16 + r, and scatters them into the table
+  // gradient with the trilinear weights (hashGridBackward: colliding rows
+  // accumulate).
+  f32x4 X[4];
+#pragma unroll
+  for (uint32_t sb = 0; sb < 4; ++sb) X[sb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (uint32_t kb = 0; kb < 2; ++kb) {
+    const bf16x8 a = frag_of(frags[kFragsFwd + kFragsBwd + kb * 64u + lane]);
+#pragma unroll
+    for (uint32_t sb = 0; sb < 4; ++sb) X[sb] = mfma(a, frag_of(B[sb][kb]), X[sb]);
+  }
+  float* __restrict__ tgrad = grad + kMlpParams;
+#pragma unroll
+  for (uint32_t sb = 0; sb < 4; ++sb) {
+    const uint32_t s = base + sb * 16u + r;
+    if (s >= n) continue;
+    const float* fp = feats + (size_t)s * kFeat;
+    const float nx = fp[0], ny = fp[1], nz = fp[2];
+#pragma unroll
+    for (uint32_t j = 0; j < 2; ++j) {
+      Corners c;
+      grid_corners(2u * g + j, nx, ny, nz, c);
+      const float d0 = X[sb][2 * j], d1 = X[sb][2 * j + 1];
+#pragma unroll
+      for (uint32_t k = 0; k < 8; ++k) {
+        unsafeAtomicAdd(tgrad + 2u * (size_t)c.row[k], c.w[k] * d0);
+        unsafeAtomicAdd(tgrad + 2u * (size_t)c.row[k] + 1u, c.w[k] * d1);
+      }
+    }
+  }
 }
 
 // ---- Adam (fusedMlpTrainer.ts:30-84) -------------------------------------------
@@ -399,7 +528,8 @@ __global__ void adam_kernel(float* __restrict__ w, float* __restrict__ grad, flo
   const float vi = p.beta2 * ov + (1.0f - p.beta2) * gr * gr;
   const float mhat = mi / p.bc1, vhat = vi / p.bc2;
   if (!finite_(mi) || !finite_(vi) || !finite_(mhat) || !finite_(vhat) || vi < 0.0f || vhat < 0.0f) return;
-  const float upd = cur - p.lr * mhat / (sqrtf(vhat) + p.eps);
+  const float lr = i < kMlpParams ? p.lr : p.lr_table;
+  const float upd = cur - lr * mhat / (sqrtf(vhat) + p.eps);
   if (!finite_(upd)) return;
   m[i] = mi;
   v[i] = vi;
@@ -418,19 +548,19 @@ hipError_t launch_pack(const float* master, uint4* frags, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_infer(const float* feats, uint32_t n, const uint32_t* n_ptr, const uint4* frags, float* rgb,
-                        hipStream_t s) {
+hipError_t launch_infer(const float* feats, uint32_t n, const uint32_t* n_ptr, const uint4* frags,
+                        const float* tables, float* rgb, hipStream_t s) {
   if (n == 0) return hipSuccess;
   const uint32_t waves = (n + 63u) / 64u;
-  hipLaunchKernelGGL(infer_kernel, dim3((waves + 3u) / 4u), dim3(256), 0, s, feats, n, n_ptr, frags, rgb);
+  hipLaunchKernelGGL(infer_kernel, dim3((waves + 3u) / 4u), dim3(256), 0, s, feats, n, n_ptr, frags, tables, rgb);
   return hipGetLastError();
 }
 
 hipError_t launch_train(const float* feats, const float* targets, uint32_t n, const uint32_t* n_ptr,
-                        const uint4* frags, float* grad, float* loss, hipStream_t s) {
+                        const uint4* frags, const float* tables, float* grad, float* loss, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(train_kernel, dim3((n + 63u) / 64u), dim3(64), 0, s, feats, targets, n, n_ptr, frags, grad,
-                     loss);
+  hipLaunchKernelGGL(train_kernel, dim3((n + 63u) / 64u), dim3(64), 0, s, feats, targets, n, n_ptr, frags, tables,
+                     grad, loss);
   return hipGetLastError();
 }
 

@@ -488,28 +488,28 @@ __device__ __forceinline__ uint32_t wave_append_active(uint32_t* counter, bool w
   return want ? base + (uint32_t)__popcll(m & below) : kMiss;
 }
 
-// (theta / pi, phi / (2 pi) + 1/2) of a unit vector.
-__device__ __forceinline__ void sph01(F3 d, float& a, float& b) {
-  a = acosf(clampf(d.y, -1.0f, 1.0f)) * kInvPi;
-  b = atan2f(d.z, d.x) * kInv2Pi + 0.5f;
-}
 
 // Cache input record of a surface vertex (layout in nrc.h).
+// Raw cache-query record (vtn feature layout): the position normalised by the
+// scene bounds, the direction towards the previous vertex, the shading normal,
+// roughness and diffuse albedo (nrcQueryRadiance(xs, ns, -wi, rough, albedo),
+// risGiNrc.wgsl.ts:477-480).
 __device__ void nrc_features(const FrameParams& f, F3 hp, F3 wo, F3 nrm, const Mat& m, float* out) {
   out[0] = clampf((hp.x - f.scene_min[0]) * f.scene_inv_extent[0], 0.0f, 1.0f);
   out[1] = clampf((hp.y - f.scene_min[1]) * f.scene_inv_extent[1], 0.0f, 1.0f);
   out[2] = clampf((hp.z - f.scene_min[2]) * f.scene_inv_extent[2], 0.0f, 1.0f);
-  sph01(wo, out[3], out[4]);
-  sph01(nrm, out[5], out[6]);
-  out[7] = m.rough;
+  out[3] = wo.x;
+  out[4] = wo.y;
+  out[5] = wo.z;
+  out[6] = nrm.x;
+  out[7] = nrm.y;
+  out[8] = nrm.z;
+  out[9] = m.rough;
   const float kd = 1.0f - m.metal;
-  out[8] = m.base.x * kd;
-  out[9] = m.base.y * kd;
-  out[10] = m.base.z * kd;
-  const F3 f0 = spec_f0(m);
-  out[11] = f0.x;
-  out[12] = f0.y;
-  out[13] = f0.z;
+  out[10] = m.base.x * kd;
+  out[11] = m.base.y * kd;
+  out[12] = m.base.z * kd;
+  out[13] = 0.0f;
   out[14] = 0.0f;
   out[15] = 0.0f;
 }

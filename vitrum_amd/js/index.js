@@ -347,8 +347,8 @@ class HipPathTracerEngine {
 function nrcSettings(enabled, config) {
   const c = Object.assign({}, config || {});
   for (const k of Object.keys(c)) {
-    if (k !== 'recordCap' && k !== 'learningRate' && k !== 'seed') {
-      throw new TypeError(`pt-hip: nrcConfig.${k} is not supported (recordCap, learningRate, seed)`);
+    if (k !== 'recordCap' && k !== 'learningRate' && k !== 'tableLearningRate' && k !== 'seed') {
+      throw new TypeError(`pt-hip: nrcConfig.${k} is not supported (recordCap, learningRate, tableLearningRate, seed)`);
     }
   }
   const cap = c.recordCap == null ? 4 * 65536 : c.recordCap;
@@ -359,7 +359,11 @@ function nrcSettings(enabled, config) {
     throw new RangeError('pt-hip: nrcConfig.learningRate must be a positive number');
   }
   const nrc = { enabled: enabled ? 1 : 0, trainPaths: Math.ceil(cap / 4) };
+  if (c.tableLearningRate != null && !(typeof c.tableLearningRate === 'number' && c.tableLearningRate >= 0)) {
+    throw new RangeError('pt-hip: nrcConfig.tableLearningRate must be a non-negative number');
+  }
   if (c.learningRate != null) nrc.learningRate = c.learningRate;
+  if (c.tableLearningRate != null) nrc.tableLearningRate = c.tableLearningRate;
   if (c.seed != null) nrc.seed = c.seed;
   return nrc;
 }

@@ -47,7 +47,8 @@ class Tuning(C.Structure):
 
 class NrcOptions(C.Structure):
     _fields_ = [("learning_rate", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float),
-                ("epsilon", C.c_float), ("seed", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+                ("epsilon", C.c_float), ("seed", C.c_uint32), ("table_learning_rate", C.c_float),
+                ("reserved", C.c_uint32 * 2)]
 
 
 NRC_FEATURES = 16
@@ -355,8 +356,9 @@ class Engine:
         check(lib().vt_nrc_param_count(C.byref(n)))
         return n.value
 
-    def nrc_reset(self, learning_rate=1e-2, beta1=0.9, beta2=0.999, epsilon=1e-8, seed=1):
-        o = NrcOptions(learning_rate, beta1, beta2, epsilon, seed)
+    def nrc_reset(self, learning_rate=1e-2, beta1=0.9, beta2=0.999, epsilon=1e-8, seed=1,
+                  table_learning_rate=1e-1):
+        o = NrcOptions(learning_rate, beta1, beta2, epsilon, seed, table_learning_rate)
         check(lib().vt_nrc_reset(self._h, C.byref(o)), self._h)
 
     def nrc_set_weights(self, w):
