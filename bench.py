@@ -147,7 +147,7 @@ def _accumulate(eng, ivp, width, height, first_frame, frames, seed):
     return eng.capture(width, height)[..., :3].astype(np.float64)
 
 
-def time_to_rmse(W, width, height, ref_spp, target, train_paths, max_frames):
+def time_to_rmse(W, width, height, ref_spp, target, train_paths, max_frames, teacher_vertices=4):
     """Plain vs cache-terminated path tracer on the Cornell scene: GPU time to
     a relative RMSE `target` against the converged image.
 
@@ -177,12 +177,14 @@ def time_to_rmse(W, width, height, ref_spp, target, train_paths, max_frames):
     ref_var = float(np.mean((lum(ref_a) - lum(ref_b)) ** 2) / 4.0)
     out = {"scene": "cornell (config-1 box, 32,768 tris)", "width": width, "height": height,
            "reference_spp": 2 * half, "target_rel_rmse": target, "train_paths_per_frame": train_paths,
+           "teacher_vertices": teacher_vertices,
            "reference_rel_rmse": round(float(np.sqrt(ref_var)) / mean_lum, 5),
            "method": "noise-corrected MSE vs split-half reference; fit e^2 = b^2 + v/N"}
     for name, nrc in (("plain", False), ("nrc", True)):
         eng.nrc_render_config(nrc, train_paths)
         if nrc:
             eng.nrc_reset(seed=11)
+            eng.nrc_teacher_vertices(teacher_vertices)
         eng.reset()
         gpu_ms, curve = 0.0, []
         check = 1
@@ -243,6 +245,8 @@ def main():
     ap.add_argument("--rmse-size", default="1920x1080")
     ap.add_argument("--rmse-ref-spp", type=int, default=1024)
     ap.add_argument("--rmse-target", type=float, default=1e-2)
+    ap.add_argument("--nrc-teacher-vertices", type=int, default=4,
+                    help="shaded vertices of a self-training path before its terminal cache query (0: full length)")
     ap.add_argument("--rmse-max-frames", type=int, default=192)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -358,7 +362,7 @@ def main():
         w, h = (int(x) for x in args.rmse_size.split("x"))
         try:
             result["time_to_rmse"] = time_to_rmse(W, w, h, args.rmse_ref_spp, args.rmse_target, 65536,
-                                                  args.rmse_max_frames)
+                                                  args.rmse_max_frames, args.nrc_teacher_vertices)
         except Exception as exc:  # the traversal line above stays valid
             result["time_to_rmse"] = {"error": repr(exc)}
     print(json.dumps(result), flush=True)

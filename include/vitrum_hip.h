@@ -373,9 +373,20 @@ vt_status vt_nrc_gradient(vt_engine *eng, const float *features, const float *ta
  * `enabled`, each image path ends at the vertex after its first
  * diffuse-dominant vertex (specular lobe probability < 1/2) and adds
  * throughput x cache prediction there; `train_paths` extra self-training
- * paths per frame (pseudo-random pixels, full length) record up to 4 vertices
- * each and drive one Adam step per frame.  Resets accumulation. */
+ * paths per frame (pseudo-random pixels) record up to 4 vertices each and
+ * drive one Adam step per frame.  Resets accumulation. */
 vt_status vt_nrc_render_config(vt_engine *eng, int32_t enabled, uint32_t train_paths);
+
+/* Length of the self-training paths' teacher suffix (NRC_TEACHER_MAX_VERTICES
+ * = 4, walkaround-hybrid shaders/nrcIndependentSuffix.wgsl.ts:23): a training
+ * path shades `vertices` vertices (all of them recorded for training) and
+ * ends at the next one with a cache query whose prediction x throughput is
+ * added to its radiance before the targets are formed (self-training
+ * termination, Mueller et al. 2021, sec. 5); the reference truncates there
+ * without an estimate of the remainder.  0 = training paths run to the
+ * bounce limit with no terminal query.  Range [0, 4], default 4.  Resets
+ * accumulation; VT_ERR_STATE without a scene. */
+vt_status vt_nrc_teacher_vertices(vt_engine *eng, uint32_t vertices);
 
 /* ---- batches: many scenes in one persistent launch --------------------------
  * No single counterpart in the reference (it renders one scene per engine);

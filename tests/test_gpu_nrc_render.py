@@ -7,6 +7,7 @@ high-spp plain reference, divided by the reference's mean luminance.
 import numpy as np
 import pytest
 
+from vitrum_amd import native as N
 from vitrum_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
@@ -73,3 +74,33 @@ def test_cache_trains_online_and_beats_plain_at_equal_spp(engine, cornell):
     bias = abs(np.mean(lum(nrc)) - np.mean(lum(ref))) / np.mean(lum(ref))
     assert bias < 0.1, bias
     engine.nrc_render_config(False)
+
+
+def test_teacher_suffix_ends_with_a_cache_query(engine, cornell):
+    """Self-training termination: with a one-vertex teacher each training path
+    records its first vertex and ends at the second with a cache query, so
+    the cache learns multi-bounce radiance by bootstrapping from its own
+    predictions.  The bootstrapped cache stays stable and its image mean
+    within 10% of the reference, as with full-length training paths (0); the
+    range of the setting is [0, 4]."""
+    sc, res, ivp, ref = cornell
+    with pytest.raises(N.VtError):
+        engine.nrc_teacher_vertices(5)
+    ref_lum = np.mean(lum(ref))
+    try:
+        for vertices in (1, 0):
+            engine.nrc_reset(seed=5)
+            engine.nrc_render_config(True, 8192)
+            engine.nrc_teacher_vertices(vertices)
+            engine.reset()
+            for f in range(160):
+                engine.render_frame(ivp, res, res, 3000 + f, SEED, 8)
+            loss = engine.nrc_last_loss()
+            nrc = render(engine, ivp, res, 16, start=4000)
+            assert np.isfinite(loss) and np.all(np.isfinite(nrc)) and nrc.min() >= 0
+            bias = (np.mean(lum(nrc)) - ref_lum) / ref_lum
+            print(f"teacher vertices {vertices}: relative bias {bias:+.4f}, loss {loss:.4f}", flush=True)
+            assert abs(bias) < 0.1, (vertices, bias)
+    finally:
+        engine.nrc_teacher_vertices(4)
+        engine.nrc_render_config(False)

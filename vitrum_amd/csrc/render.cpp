@@ -104,6 +104,7 @@ struct RenderState {
   // Neural radiance cache (used when nrc_enabled).
   bool nrc_enabled = false;
   uint32_t nrc_train_paths = 65536;
+  uint32_t nrc_train_vertices = 4;  // shaded teacher vertices before the terminal query (0: full length)
   float scene_min[3] = {0, 0, 0}, scene_inv_extent[3] = {1, 1, 1};
   DeviceBuffer nrc_count, q_feat, q_thr, q_path, q_rgb, train_rec, train_n, train_feat, train_tgt;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -349,10 +350,11 @@ static vt_status ensure_paths(vt_engine* e, uint32_t w, uint32_t h) {
   need(R.accum, pix * 16);
   if (train) {
     need(R.nrc_count, 64);
-    need(R.q_feat, pix * 64);
-    need(R.q_thr, pix * 12);
-    need(R.q_path, pix * 4);
-    need(R.q_rgb, pix * 12);
+    const size_t nq = pix + train;  // image-path queries + training-path terminal queries
+    need(R.q_feat, nq * 64);
+    need(R.q_thr, nq * 12);
+    need(R.q_path, nq * 4);
+    need(R.q_rgb, nq * 12);
     need(R.train_rec, (size_t)train * vtr::kTrainSlots * vtr::kTrainRec * 4);
     need(R.train_n, (size_t)train * 4);
     need(R.train_feat, (size_t)train * vtr::kTrainSlots * 64);
@@ -403,6 +405,7 @@ vt_status vt_render_frame(vt_engine* e, const vt_frame* fr) {
   f.sobol_dirs = (const uint16_t*)R.sobol_dirs.ptr;
   f.nrc = R.nrc_enabled ? 1u : 0u;
   f.train_paths = R.nrc_enabled ? R.nrc_train_paths : 0u;
+  f.train_vertices = R.nrc_train_vertices;
   for (int k = 0; k < 3; ++k) f.scene_min[k] = R.scene_min[k], f.scene_inv_extent[k] = R.scene_inv_extent[k];
   vtr::ShadeScene sc{};
   sc.positions = (const float4*)e->scene.positions.ptr;
@@ -492,8 +495,9 @@ vt_status vt_render_frame(vt_engine* e, const vt_frame* fr) {
   if (f.nrc) {
     // Cache queries of the image paths, then the online training step on the
     // self-training paths' recorded vertices.
-    if (vt_status s = engine_nrc_infer_counted(e, ps.q_feat, ps.nrc_count, ps.q_rgb, f.pixels)) return s;
-    HIP_CHECK(e, vtr::launch_nrc_apply(ps, f.pixels, e->stream));
+    if (vt_status s = engine_nrc_infer_counted(e, ps.q_feat, ps.nrc_count, ps.q_rgb, f.pixels + f.train_paths))
+      return s;
+    HIP_CHECK(e, vtr::launch_nrc_apply(ps, f.pixels + f.train_paths, e->stream));
   }
   HIP_CHECK(e, vtr::launch_accumulate(f, ps, e->stream));
   if (f.nrc) {
@@ -569,6 +573,16 @@ vt_status vt_nrc_render_config(vt_engine* e, int32_t enabled, uint32_t train_pat
   R.nrc_enabled = enabled != 0;
   if (enabled) R.nrc_train_paths = train_paths;
   R.width = R.height = 0;  // re-size path state on the next frame
+  e->samples_accumulated = 0;
+  return VT_OK;
+}
+
+vt_status vt_nrc_teacher_vertices(vt_engine* e, uint32_t vertices) {
+  if (!e) return VT_ERR_INVALID_ARGUMENT;
+  if (vertices > vtr::kTrainSlots)
+    return rerr(e, VT_ERR_RANGE, "teacher vertices must be in [0, 4] (0: full-length training paths)");
+  if (!e->render) return rerr(e, VT_ERR_STATE, "no scene is set");
+  e->render->nrc_train_vertices = vertices;
   e->samples_accumulated = 0;
   return VT_OK;
 }

@@ -45,6 +45,10 @@ struct FrameParams {
   // train_paths are self-training paths started at pseudo-random pixels.
   uint32_t nrc;
   uint32_t pixels, train_paths;
+  // Self-training termination: a training path that reaches vertex
+  // `train_vertices` ends there with a cache query, whose prediction stands
+  // in for the rest of the path (0: training paths run to bounce_limit).
+  uint32_t train_vertices;
   float scene_min[3], scene_inv_extent[3];
   // Sampler: PCG, or Owen-scrambled Sobol (frame key, direction table).
   uint32_t sobol;

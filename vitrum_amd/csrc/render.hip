@@ -685,12 +685,16 @@ __global__ __launch_bounds__(256) void shade_kernel(FrameParams f, ShadeScene sc
         // Neural radiance cache: an image path that has passed a
         // diffuse-dominant vertex ends here with a cache query (its
         // prediction times T is added after the bounce loop); a training path
-        // records the vertex (features, T, radiance so far) for its target.
+        // records the vertex (features, T, radiance so far) for its target,
+        // and at vertex f.train_vertices ends with a query of its own, so its
+        // targets carry the cache's estimate of the truncated remainder
+        // (self-training, Mueller et al. 2021, "Real-time Neural Radiance
+        // Caching for Path Tracing", sec. 5).
         bool query = false;
         if (f.nrc) {
           const bool train = (flags & 4u) != 0u;
-          query = !train && (flags & 2u) != 0u;
-          if (train && vertex < kTrainSlots) {
+          query = train ? (f.train_vertices != 0u && vertex >= f.train_vertices) : (flags & 2u) != 0u;
+          if (train && !query && vertex < kTrainSlots) {
             float* rec = s.train_rec + ((size_t)(p - f.pixels) * kTrainSlots + vertex) * kTrainRec;
             nrc_features(f, hp, wo, nrm, m, rec);
             store3(rec + 16, T);

@@ -20,7 +20,10 @@
 //  * Persistent waves: each wave pulls rays from a global counter.  Lanes that
 //    finish are refilled together once enough of the wave is idle: a ballot
 //    of idle lanes, one atomicAdd per refill, and a prefix count (mbcnt) to
-//    hand out consecutive ray indices.
+//    hand out consecutive ray indices.  A launch with fewer rays than 64 per
+//    resident wave hands each wave at most its even share, so small batches
+//    (late bounces, training paths) occupy every wave slot of every SIMD and
+//    the dependent node loads of one wave hide behind the others'.
 //  * One combined TLAS+BLAS stack per lane in LDS (entry-major so a wave's
 //    accesses at equal depth hit 64 distinct banks), spilling the rare deep
 //    entries to a per-thread global area.
@@ -356,6 +359,8 @@ __global__ __launch_bounds__(BLOCK, 4) void trace_kernel(TraceParams p) {
   const bool near_tlas = !ANY && !COUNT && p.near_first == 2u;  // also across instances
 
   const uint32_t n_rays = p.n_ptr ? *p.n_ptr : p.n;
+  const uint32_t n_waves = gridDim.x * (BLOCK / 64u);
+  const uint32_t take_cap = max(1u, min(64u, n_rays / n_waves + (n_rays % n_waves != 0u)));
   bool active = false;
   uint32_t ray_idx = 0;
   // World ray and best hit.
@@ -465,12 +470,14 @@ __global__ __launch_bounds__(BLOCK, 4) void trace_kernel(TraceParams p) {
     const uint64_t idle = __ballot(!active);
     const uint32_t n_idle = (uint32_t)__popcll(idle);
     if (!exhausted && (n_idle >= p.refill_min || idle == __ballot(1))) {
+      const uint32_t take = min(n_idle, take_cap);
       uint32_t base = 0;
-      if (lane == 0) base = atomicAdd(p.counter, n_idle);
+      if (lane == 0) base = atomicAdd(p.counter, take);
       base = __shfl(base, 0);
-      exhausted = base + n_idle >= n_rays;
-      if (!active) {
-        const uint32_t idx = base + (uint32_t)__popcll(idle & lane_below);
+      exhausted = base + take >= n_rays;
+      const uint32_t rank = (uint32_t)__popcll(idle & lane_below);
+      if (!active && rank < take) {
+        const uint32_t idx = base + rank;
         if (idx < n_rays) {
           ray_idx = idx;
           if (MULTI) {

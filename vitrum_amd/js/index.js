@@ -205,7 +205,7 @@ class HipPathTracerEngine {
     this._samples = 0;
     if (this._nrc) {
       this._n.nrcConfig(this._h, this._nrc);
-      this._nrc = { enabled: 1, trainPaths: this._nrc.trainPaths };  // seed / rate apply once
+      this._nrc = keptNrc(this._nrc);  // seed / rate apply once
     }
   }
 
@@ -220,7 +220,7 @@ class HipPathTracerEngine {
     if (this._scene != null) {
       this._n.nrcConfig(this._h, nrc);
       // seed / learning rate apply once; later scenes keep the trained cache
-      this._nrc = enabled ? { enabled: 1, trainPaths: nrc.trainPaths } : null;
+      this._nrc = enabled ? keptNrc(nrc) : null;
     } else {
       this._nrc = enabled ? nrc : null;  // applied by setScene
     }
@@ -347,8 +347,10 @@ class HipPathTracerEngine {
 function nrcSettings(enabled, config) {
   const c = Object.assign({}, config || {});
   for (const k of Object.keys(c)) {
-    if (k !== 'recordCap' && k !== 'learningRate' && k !== 'tableLearningRate' && k !== 'seed') {
-      throw new TypeError(`pt-hip: nrcConfig.${k} is not supported (recordCap, learningRate, tableLearningRate, seed)`);
+    if (k !== 'recordCap' && k !== 'learningRate' && k !== 'tableLearningRate' && k !== 'seed' &&
+        k !== 'teacherVertices') {
+      throw new TypeError(
+        `pt-hip: nrcConfig.${k} is not supported (recordCap, learningRate, tableLearningRate, seed, teacherVertices)`);
     }
   }
   const cap = c.recordCap == null ? 4 * 65536 : c.recordCap;
@@ -365,7 +367,21 @@ function nrcSettings(enabled, config) {
   if (c.learningRate != null) nrc.learningRate = c.learningRate;
   if (c.tableLearningRate != null) nrc.tableLearningRate = c.tableLearningRate;
   if (c.seed != null) nrc.seed = c.seed;
+  if (c.teacherVertices != null) {
+    if (!(Number.isInteger(c.teacherVertices) && c.teacherVertices >= 0 && c.teacherVertices <= 4)) {
+      throw new RangeError('pt-hip: nrcConfig.teacherVertices must be an integer in [0, 4] (0: full-length training paths)');
+    }
+    nrc.teacherVertices = c.teacherVertices;
+  }
   return nrc;
+}
+
+// The part of a cache configuration that is re-applied to every new scene
+// (the seed and learning rates initialise the cache once).
+function keptNrc(nrc) {
+  const kept = { enabled: 1, trainPaths: nrc.trainPaths };
+  if (nrc.teacherVertices != null) kept.teacherVertices = nrc.teacherVertices;
+  return kept;
 }
 
 // EngineFactory (packages/core/src/engine/factory.ts): opts.device is the HIP

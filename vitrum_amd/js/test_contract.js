@@ -45,6 +45,7 @@ assert.throws(() => pt.sceneBlob({ primitives: [], environment: { kind: 'hdri', 
   await assert.rejects(pt.createPTEngine_HIP({ denoiser: 'oidn-final' }), RangeError);
   await assert.rejects(pt.createPTEngine_HIP({ nrcEnabled: true, nrcConfig: { levels: 16 } }), TypeError);
   await assert.rejects(pt.createPTEngine_HIP({ nrcEnabled: true, nrcConfig: { recordCap: 2 } }), RangeError);
+  await assert.rejects(pt.createPTEngine_HIP({ nrcEnabled: true, nrcConfig: { teacherVertices: 5 } }), RangeError);
   await assert.rejects(pt.createPTEngine_HIP({ bvhTraversal: 'cwbvh' }), RangeError);
   if (process.env.VITRUM_HIP_GPU !== '1') {
     await assert.rejects(pt.createPTEngine_HIP({}), (e) => e.code === 'VT_ERR_NO_DEVICE' || e.code === 'VT_ERR_HIP');
@@ -77,7 +78,7 @@ assert.throws(() => pt.sceneBlob({ primitives: [], environment: { kind: 'hdri', 
   const hits = new Float32Array(addon.traceClosest(eng._h, new Float32Array([0.6, 0.3, 1, 0, 0, 0, -1, 1e30])));
   assert.ok(Math.abs(hits[2] - 1) < 1e-6, `t = ${hits[2]}`);
   // The radiance cache (nrcEnabled): trains online, finite non-negative output.
-  const nrc = await pt.createPTEngine_HIP({ device: 0, nrcEnabled: true, nrcConfig: { recordCap: 4096, seed: 7 } });
+  const nrc = await pt.createPTEngine_HIP({ device: 0, nrcEnabled: true, nrcConfig: { recordCap: 4096, seed: 7, teacherVertices: 2 } });
   nrc.setScene({ primitives: [quad], emitters: [], environment: { kind: 'hdri', hdri: { constant: [1, 1, 1] } } });
   for (let i = 0; i < 4; i += 1) {
     nrc.renderFrame({ viewMatrix: view, projMatrix: proj, viewport: { width: 16, height: 16, devicePixelRatio: 1 },

@@ -474,17 +474,19 @@ napi_value InvertMat4(napi_env env, napi_callback_info info) {
   return arr;
 }
 
-// nrcConfig(handle, {enabled, trainPaths, learningRate, tableLearningRate, seed}): the radiance
-// cache of the path tracer (nrcEnabled / nrcConfig of the reference's
-// WalkaroundGPUPipeline.ts:1860-1862).  A learningRate or seed re-initialises
-// the cache's weights and Adam state; needs a scene.
+// nrcConfig(handle, {enabled, trainPaths, learningRate, tableLearningRate, seed,
+// teacherVertices}): the radiance cache of the path tracer (nrcEnabled /
+// nrcConfig of the reference's WalkaroundGPUPipeline.ts:1860-1862).  A
+// learningRate or seed re-initialises the cache's weights and Adam state;
+// teacherVertices sets the training paths' suffix length; needs a scene.
 napi_value NrcConfig(napi_env env, napi_callback_info info) {
   auto a = args_of(env, info, 2);
   Handle* h = handle_of(env, a[0]);
   if (!h) return nullptr;
   uint32_t enabled = 0, train_paths = 65536, seed = 1;
   float lr = 1e-2f, table_lr = 1e-1f;
-  bool has_lr = false, has_seed = false, has_tlr = false;
+  uint32_t teacher = 4;
+  bool has_lr = false, has_seed = false, has_tlr = false, has_teacher = false;
   if (!is_nullish(env, a[1])) {
     if (!get_u32(env, a[1], "enabled", 0, &enabled)) return nullptr;
     if (!get_u32(env, a[1], "trainPaths", 65536, &train_paths)) return nullptr;
@@ -494,6 +496,8 @@ napi_value NrcConfig(napi_env env, napi_callback_info info) {
     if (!get_u32(env, a[1], "seed", 1, &seed)) return nullptr;
     napi_has_named_property(env, a[1], "tableLearningRate", &has_tlr);
     if (!get_f32(env, a[1], "tableLearningRate", 1e-1f, &table_lr)) return nullptr;
+    napi_has_named_property(env, a[1], "teacherVertices", &has_teacher);
+    if (!get_u32(env, a[1], "teacherVertices", 4, &teacher)) return nullptr;
   }
   if (has_lr || has_seed || has_tlr) {
     vt_nrc_options o{};
@@ -505,7 +509,8 @@ napi_value NrcConfig(napi_env env, napi_callback_info info) {
     o.table_learning_rate = table_lr;
     if (throw_status(env, h->eng, vt_nrc_reset(h->eng, &o))) return nullptr;
   }
-  throw_status(env, h->eng, vt_nrc_render_config(h->eng, enabled ? 1 : 0, train_paths));
+  if (throw_status(env, h->eng, vt_nrc_render_config(h->eng, enabled ? 1 : 0, train_paths))) return nullptr;
+  if (has_teacher) throw_status(env, h->eng, vt_nrc_teacher_vertices(h->eng, teacher));
   return nullptr;
 }
 

@@ -34,7 +34,7 @@ EXPORTED = [
     "vt_set_bvh_traversal", "vt_set_node_width", "vt_set_hit_filter",
     "vt_nrc_param_count", "vt_nrc_reset", "vt_nrc_set_weights", "vt_nrc_get_weights", "vt_nrc_infer",
     "vt_nrc_infer_device", "vt_nrc_train_step", "vt_nrc_train_step_device", "vt_nrc_last_loss",
-    "vt_nrc_gradient", "vt_nrc_render_config", "vt_nrc_timing", "vt_batch_create", "vt_batch_destroy",
+    "vt_nrc_gradient", "vt_nrc_render_config", "vt_nrc_teacher_vertices", "vt_nrc_timing", "vt_batch_create", "vt_batch_destroy",
     "vt_batch_add_scene", "vt_batch_scene_count", "vt_batch_trace_closest_device", "vt_batch_capture",
     "vt_batch_replay", "vt_batch_autotune", "vt_batch_set_tuning",
 ]
@@ -403,6 +403,11 @@ class Engine:
 
     def nrc_render_config(self, enabled, train_paths=65536):
         check(lib().vt_nrc_render_config(self._h, C.c_int32(1 if enabled else 0), C.c_uint32(train_paths)), self._h)
+
+    def nrc_teacher_vertices(self, vertices=4):
+        """Shaded vertices of a self-training path before its terminal cache
+        query (0: full-length training paths)."""
+        check(lib().vt_nrc_teacher_vertices(self._h, C.c_uint32(vertices)), self._h)
 
     def nrc_timing(self):
         a, b = C.c_float(), C.c_float()
