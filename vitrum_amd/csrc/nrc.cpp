@@ -8,6 +8,8 @@
 
 struct NrcState {
   DeviceBuffer master, m, v, grad, frags, loss;
+  DeviceBuffer part;  // per-workgroup partial MLP weight gradients of the training kernel
+  DeviceBuffer dx;    // per-sample gradients of the hash-grid inputs (16 floats per sample)
   DeviceBuffer feats, targets, out;  // staging for the host-buffer entry points
   vt_nrc_options opts{};
   uint32_t step = 0;
@@ -15,7 +17,7 @@ struct NrcState {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   bool timed_infer = false, timed_train = false;
   void release() {
-    for (DeviceBuffer* b : {&master, &m, &v, &grad, &frags, &loss, &feats, &targets, &out}) b->release();
+    for (DeviceBuffer* b : {&master, &m, &v, &grad, &frags, &loss, &part, &dx, &feats, &targets, &out}) b->release();
     for (hipEvent_t& x : ev)
       if (x) (void)hipEventDestroy(x), x = nullptr;
   }
@@ -78,6 +80,7 @@ static vt_status nrc_ensure(vt_engine* e) {
   for (DeviceBuffer* b : {&s->master, &s->m, &s->v, &s->grad}) st = st ? st : b->ensure(e, pb);
   st = st ? st : s->frags.ensure(e, vtn::kFrags * 16);
   st = st ? st : s->loss.ensure(e, 64);
+  st = st ? st : s->part.ensure(e, (size_t)vtn::kGradSlots * vtn::kMlpParams * sizeof(float));
   for (hipEvent_t& x : s->ev)
     if (!st && hipEventCreate(&x) != hipSuccess) st = VT_ERR_HIP;
   if (st) {
@@ -205,11 +208,12 @@ static vt_status nrc_accumulate(vt_engine* e, const float* d_features, const flo
                                 const uint32_t* d_count) {
   NrcState& s = *e->nrc;
   NRC_HIP(e, hipSetDevice(e->device));
+  if (vt_status st = s.dx.ensure(e, (size_t)n * 16u * sizeof(float))) return st;
   NRC_HIP(e, vtn::launch_fill((float*)s.loss.ptr, 1, 0.0f, e->stream));
   NRC_HIP(e, hipEventRecord(s.ev[2], e->stream));
   NRC_HIP(e, vtn::launch_train(d_features, d_targets, n, d_count, (const uint4*)s.frags.ptr,
-                               (const float*)s.master.ptr + vtn::kMlpParams, (float*)s.grad.ptr, (float*)s.loss.ptr,
-                               e->stream));
+                               (const float*)s.master.ptr + vtn::kMlpParams, (float*)s.grad.ptr, (float*)s.part.ptr,
+                               (float*)s.dx.ptr, (float*)s.loss.ptr, e->stream));
   NRC_HIP(e, hipEventRecord(s.ev[3], e->stream));
   s.timed_train = true;
   return VT_OK;

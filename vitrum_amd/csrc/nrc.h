@@ -51,15 +51,22 @@ struct AdamParams {
   float lr_table;  // learning rate of the hash-grid tables
 };
 
+// Per-workgroup partial MLP weight gradients of the training kernel (each
+// workgroup owns one slot of kMlpParams floats and accumulates its tiles there
+// with plain stores; a reduction kernel then adds the slots into grad).
+constexpr uint32_t kGradSlots = 1024;
+
 hipError_t launch_pack(const float* master, uint4* frags, hipStream_t s);
 // When n_ptr is set the sample count is read from device memory (at most n).
 // `tables` = master + kMlpParams (the FP32 hash-grid features).
 hipError_t launch_infer(const float* feats, uint32_t n, const uint32_t* n_ptr, const uint4* frags,
                         const float* tables, float* rgb, hipStream_t s);
 // grad += d loss / d parameters over the batch (kParams: MLP weights, then the
-// tables); loss[0] += relative-L2 loss of the batch.
+// tables); loss[0] += relative-L2 loss of the batch.  Scratch: `part` holds
+// kGradSlots * kMlpParams floats, `dx` 16 floats per sample (n).
 hipError_t launch_train(const float* feats, const float* targets, uint32_t n, const uint32_t* n_ptr,
-                        const uint4* frags, const float* tables, float* grad, float* loss, hipStream_t s);
+                        const uint4* frags, const float* tables, float* grad, float* part, float* dx, float* loss,
+                        hipStream_t s);
 hipError_t launch_adam(float* master, float* grad, float* m, float* v, AdamParams p, hipStream_t s);
 hipError_t launch_fill(float* p, uint32_t n, float value, hipStream_t s);
 

@@ -26,6 +26,7 @@
 // operands are staged transposed ([neuron][sample]) in LDS.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "nrc.h"
@@ -332,9 +333,13 @@ __device__ __forceinline__ void store_t(uint16_t* T, uint32_t g, uint32_t r, con
 }
 
 // dW (rows: `rows` output neurons in 16-blocks) += D^T H over the 64 samples,
-// with D = DT ([neuron][sample]) and H = HT; accumulated into grad[o*64 + i].
+// with D = DT ([neuron][sample]) and H = HT, into the workgroup's own slot
+// (slot[o*64 + i]; `first`: the slot's first tile, which stores instead of
+// adding).  Every element belongs to exactly one lane, so plain loads and
+// stores suffice.
 __device__ __forceinline__ void weight_grad(const uint16_t* DT, const uint16_t* HT, uint32_t row_blocks,
-                                            uint32_t rows, uint32_t g, uint32_t r, float* __restrict__ grad) {
+                                            uint32_t rows, uint32_t g, uint32_t r, float* __restrict__ slot,
+                                            bool first) {
   for (uint32_t ob = 0; ob < row_blocks; ++ob) {
     f32x4 C[4];
 #pragma unroll
@@ -353,7 +358,10 @@ __device__ __forceinline__ void weight_grad(const uint16_t* DT, const uint16_t* 
 #pragma unroll
       for (uint32_t j = 0; j < 4; ++j) {
         const uint32_t o = ob * 16u + g * 4u + j;
-        if (o < rows) unsafeAtomicAdd(grad + o * 64u + ib * 16u + r, C[ib][j]);
+        if (o < rows) {
+          float* q = slot + o * 64u + ib * 16u + r;
+          *q = first ? C[ib][j] : *q + C[ib][j];
+        }
       }
   }
 }
@@ -390,13 +398,19 @@ __device__ __forceinline__ void backward_layer(const uint4* __restrict__ fr, uin
 __global__ __launch_bounds__(64) void train_kernel(const float* __restrict__ feats, const float* __restrict__ targets,
                                                    uint32_t cap, const uint32_t* __restrict__ n_ptr,
                                                    const uint4* __restrict__ frags, const float* __restrict__ tables,
-                                                   float* __restrict__ grad, float* __restrict__ loss) {
+                                                   float* __restrict__ partials, float* __restrict__ dx,
+                                                   float* __restrict__ loss) {
   const uint32_t n = n_ptr ? min(*n_ptr, cap) : cap;
-  if (blockIdx.x * 64u >= n) return;  // whole block past the end (uniform)
+  const uint32_t tiles = (n + 63u) / 64u;
   __shared__ __align__(16) uint16_t HT[5 * kTile];  // inputs of W0..W4, transposed
   __shared__ __align__(16) uint16_t DT[kTile];      // deltas, transposed
   const uint32_t lane = threadIdx.x, g = lane >> 4, r = lane & 15u;
-  const uint32_t base = blockIdx.x * 64u;
+  float* __restrict__ slot = partials + (size_t)blockIdx.x * kMlpParams;
+  // Tiles of 64 samples, strided over the workgroups (uniform loop bounds).
+  for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+  const bool first = tile == blockIdx.x;
+  const uint32_t base = tile * 64u;
+  __syncthreads();  // the previous tile's last reads of HT/DT
   uint4 B[4][2];
   uint64_t mask[kHidden];
   encode_tile(feats, tables, n, base, g, r, B);
@@ -458,20 +472,20 @@ __global__ __launch_bounds__(64) void train_kernel(const float* __restrict__ fea
   if (lane == 0) unsafeAtomicAdd(loss, part / (3.0f * (float)n));
   __syncthreads();
   // dW4 (3 x 64) and dH4
-  weight_grad(DT, HT + 4u * kTile, 1u, kOut, g, r, grad + 16384u);
+  weight_grad(DT, HT + 4u * kTile, 1u, kOut, g, r, slot + 16384u, first);
   __syncthreads();
   const uint4* bw = frags + kFragsFwd;
   backward_layer(bw + 3u * kFragsHidden, lane, 1u, mask[3], B);
   for (uint32_t m = 3; m >= 1; --m) {
     store_t(DT, g, r, B);  // dH_{m+1} (pre-activation delta of layer m's output)
     __syncthreads();
-    weight_grad(DT, HT + m * kTile, 4u, 64u, g, r, grad + m * 4096u);
+    weight_grad(DT, HT + m * kTile, 4u, 64u, g, r, slot + m * 4096u, first);
     __syncthreads();
     backward_layer(bw + (m - 1u) * kFragsHidden, lane, 2u, mask[m - 1u], B);
   }
   store_t(DT, g, r, B);  // dH1
   __syncthreads();
-  weight_grad(DT, HT, 4u, 64u, g, r, grad);
+  weight_grad(DT, HT, 4u, 64u, g, r, slot, first);
   // d loss / d x of the hash-grid inputs (dims 0..15) = W0^T dH1: lane (g, r)
   // receives dims 4g..4g+3 -- the features of levels 2g, 2g + 1 that it
   // encoded -- of samples sb*16 + r, and scatters them into the table
@@ -486,25 +500,63 @@ __global__ __launch_bounds__(64) void train_kernel(const float* __restrict__ fea
 #pragma unroll
     for (uint32_t sb = 0; sb < 4; ++sb) X[sb] = mfma(a, frag_of(B[sb][kb]), X[sb]);
   }
-  float* __restrict__ tgrad = grad + kMlpParams;
+  // Per-sample input gradients of the hash-grid dims, scattered into the
+  // tables by table_grad_kernel.
 #pragma unroll
   for (uint32_t sb = 0; sb < 4; ++sb) {
     const uint32_t s = base + sb * 16u + r;
-    if (s >= n) continue;
+    if (s < n) *reinterpret_cast<float4*>(dx + (size_t)s * 16u + 4u * g) = make_float4(X[sb][0], X[sb][1], X[sb][2], X[sb][3]);
+  }
+  }  // tiles
+}
+
+// Hash-grid table gradient (hashGridBackward): each input gradient of a
+// level's two features, times the trilinear weight of each of the 8 corners,
+// accumulates into that corner's row.  One workgroup per (level, chunk of
+// samples) sums its chunk into a private copy of the level's 4096 x 2
+// gradient in LDS, then adds the copy to grad with one atomic per non-zero
+// element: the coarse levels have only a few hundred distinct rows, and
+// scattering every sample's contribution straight to memory serialises on
+// them.
+__global__ __launch_bounds__(256) void table_grad_kernel(const float* __restrict__ feats, const float* __restrict__ dx,
+                                                         uint32_t cap, const uint32_t* __restrict__ n_ptr,
+                                                         uint32_t per_chunk, float* __restrict__ tgrad) {
+  __shared__ float acc[kRows * kLevelFeat];
+  const uint32_t l = blockIdx.x, tid = threadIdx.x;
+  for (uint32_t i = tid; i < kRows * kLevelFeat; i += 256u) acc[i] = 0.0f;
+  __syncthreads();
+  const uint32_t n = n_ptr ? min(*n_ptr, cap) : cap;
+  const uint32_t lo = blockIdx.y * per_chunk, hi = min(n, lo + per_chunk);
+  for (uint32_t s = lo + tid; s < hi; s += 256u) {
     const float* fp = feats + (size_t)s * kFeat;
-    const float nx = fp[0], ny = fp[1], nz = fp[2];
+    Corners c;
+    grid_corners(l, fp[0], fp[1], fp[2], c);
+    const float d0 = dx[(size_t)s * 16u + 2u * l], d1 = dx[(size_t)s * 16u + 2u * l + 1u];
 #pragma unroll
-    for (uint32_t j = 0; j < 2; ++j) {
-      Corners c;
-      grid_corners(2u * g + j, nx, ny, nz, c);
-      const float d0 = X[sb][2 * j], d1 = X[sb][2 * j + 1];
-#pragma unroll
-      for (uint32_t k = 0; k < 8; ++k) {
-        unsafeAtomicAdd(tgrad + 2u * (size_t)c.row[k], c.w[k] * d0);
-        unsafeAtomicAdd(tgrad + 2u * (size_t)c.row[k] + 1u, c.w[k] * d1);
-      }
+    for (uint32_t k = 0; k < 8; ++k) {
+      const uint32_t row = c.row[k] - l * kRows;
+      atomicAdd(&acc[2u * row], c.w[k] * d0);
+      atomicAdd(&acc[2u * row + 1u], c.w[k] * d1);
     }
   }
+  __syncthreads();
+  float* __restrict__ out = tgrad + (size_t)l * kRows * kLevelFeat;
+  for (uint32_t i = tid; i < kRows * kLevelFeat; i += 256u)
+    if (acc[i] != 0.0f) unsafeAtomicAdd(out + i, acc[i]);
+}
+
+// grad[i] += sum of the partial MLP weight gradients of the workgroups that
+// processed at least one tile; blockIdx.y picks every gridDim.y-th slot.
+__global__ __launch_bounds__(256) void reduce_slots_kernel(const float* __restrict__ part, uint32_t slots,
+                                                           uint32_t cap, const uint32_t* __restrict__ n_ptr,
+                                                           float* __restrict__ grad) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  const uint32_t n = n_ptr ? min(*n_ptr, cap) : cap;
+  const uint32_t used = min(slots, (n + 63u) / 64u);
+  if (i >= kMlpParams) return;
+  float acc = 0.0f;
+  for (uint32_t k = blockIdx.y; k < used; k += gridDim.y) acc += part[(size_t)k * kMlpParams + i];
+  if (acc != 0.0f) unsafeAtomicAdd(grad + i, acc);
 }
 
 // ---- Adam (fusedMlpTrainer.ts:30-84) -------------------------------------------
@@ -557,10 +609,33 @@ hipError_t launch_infer(const float* feats, uint32_t n, const uint32_t* n_ptr, c
 }
 
 hipError_t launch_train(const float* feats, const float* targets, uint32_t n, const uint32_t* n_ptr,
-                        const uint4* frags, const float* tables, float* grad, float* loss, hipStream_t s) {
+                        const uint4* frags, const float* tables, float* grad, float* part, float* dx, float* loss,
+                        hipStream_t s) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(train_kernel, dim3((n + 63u) / 64u), dim3(64), 0, s, feats, targets, n, n_ptr, frags, tables,
-                     grad, loss);
+  // As many workgroups as are resident at once (bounded by the slots), each
+  // looping over tiles, so no workgroup waits for a second wave of launches.
+  static uint32_t resident = 0;
+  if (resident == 0) {
+    int dev = 0, cus = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, train_kernel, 64, 0);
+    if (e != hipSuccess) return e;
+    resident = (uint32_t)std::max(1, cus * std::max(per_cu, 1));
+  }
+  const uint32_t tiles = (n + 63u) / 64u;
+  const uint32_t blocks = std::min(std::min(tiles, resident), kGradSlots);
+  hipLaunchKernelGGL(train_kernel, dim3(blocks), dim3(64), 0, s, feats, targets, n, n_ptr, frags, tables, part, dx,
+                     loss);
+  if (hipError_t e = hipGetLastError()) return e;
+  hipLaunchKernelGGL(reduce_slots_kernel, dim3((kMlpParams + 255u) / 256u, 32u), dim3(256), 0, s, part, blocks, n,
+                     n_ptr, grad);
+  if (hipError_t e = hipGetLastError()) return e;
+  // About 4096 samples per workgroup, at most 64 chunks per level.
+  const uint32_t chunks = std::min(64u, (n + 4095u) / 4096u);
+  const uint32_t per_chunk = (n + chunks - 1u) / chunks;
+  hipLaunchKernelGGL(table_grad_kernel, dim3(kLevels, chunks), dim3(256), 0, s, feats, dx, n, n_ptr, per_chunk,
+                     grad + kMlpParams);
   return hipGetLastError();
 }
 
