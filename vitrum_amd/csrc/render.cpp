@@ -108,6 +108,12 @@ struct RenderState {
   float scene_min[3] = {0, 0, 0}, scene_inv_extent[3] = {1, 1, 1};
   DeviceBuffer nrc_count, q_feat, q_thr, q_path, q_rgb, train_rec, train_n, train_feat, train_tgt;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // Shadow rays of a bounce are traced on `side` while the main stream traces
+  // the next bounce's extension rays: fork after shading (ev_fork), join
+  // before the next shading (ev_join).  Default priority, like the main
+  // stream.
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   void release_paths() {
     for (DeviceBuffer* b : {&rng, &rng_dim, &throughput, &conn, &radiance, &prev_pdf, &flags, &signature, &count,
                             &queue[0], &queue[1], &rays[0], &rays[1], &hits, &hit_pos, &shadow_path,
@@ -126,8 +132,10 @@ void engine_release_render(vt_engine* e) {
   if (!e->render) return;
   e->render->release_paths();
   e->render->release_scene();
-  if (e->render->ev0) (void)hipEventDestroy(e->render->ev0);
-  if (e->render->ev1) (void)hipEventDestroy(e->render->ev1);
+  if (e->render->side) (void)hipStreamSynchronize(e->render->side);
+  for (hipEvent_t ev : {e->render->ev0, e->render->ev1, e->render->ev_fork, e->render->ev_join})
+    if (ev) (void)hipEventDestroy(ev);
+  if (e->render->side) (void)hipStreamDestroy(e->render->side);
   delete e->render;
   e->render = nullptr;
 }
@@ -147,6 +155,9 @@ vt_status engine_on_scene_changed(vt_engine* e) {
     e->render = new RenderState();
     HIP_CHECK(e, hipEventCreate(&e->render->ev0));
     HIP_CHECK(e, hipEventCreate(&e->render->ev1));
+    HIP_CHECK(e, hipEventCreateWithFlags(&e->render->ev_fork, hipEventDisableTiming));
+    HIP_CHECK(e, hipEventCreateWithFlags(&e->render->ev_join, hipEventDisableTiming));
+    HIP_CHECK(e, hipStreamCreateWithFlags(&e->render->side, hipStreamNonBlocking));
   }
   RenderState& R = *e->render;
   const vt::Pack& P = *e->pack;
@@ -463,6 +474,11 @@ vt_status vt_render_frame(vt_engine* e, const vt_frame* fr) {
   ps.rays = (float4*)R.rays[cur].ptr;
   HIP_CHECK(e, vtr::launch_camera(f, ps, e->stream));
   vtk::TraceParams tp = trace_params(e);
+  // From the second bounce on, a bounce's extension trace runs beside the
+  // previous bounce's shadow trace: each gets half of the persistent grid,
+  // its own ray counter and its own half of the stack-spill area.
+  const uint32_t half_grid = std::max(1u, e->trace_grid / 2u);
+  const size_t spill_half = (size_t)(vtk::kStackCapacity - vtk::kLdsDepth) * half_grid * vtk::kTraceBlock;
   for (uint32_t v = 0; v <= f.bounce_limit; ++v) {
     ps.queue = (uint32_t*)R.queue[cur].ptr;
     ps.rays = (float4*)R.rays[cur].ptr;
@@ -477,19 +493,28 @@ vt_status vt_render_frame(vt_engine* e, const vt_frame* fr) {
     if (e->opts.bvh_traversal == 1) {
       if (vt_status s = engine_trace_cwbvh(e, ps.rays, cap, ps.count, ps.hits, ps.hit_pos, 1u)) return s;
     } else {
-      HIP_CHECK(e, vtk::launch_trace(cp, false, false, e->trace_grid, e->stream));
+      HIP_CHECK(e, vtk::launch_trace(cp, false, false, v == 0 ? e->trace_grid : half_grid, e->stream));
     }
+    // The previous bounce's shadow rays are resolved (radiance, shadow
+    // queue and count) before this bounce's shading touches them.
+    if (v > 0) HIP_CHECK(e, hipStreamWaitEvent(e->stream, R.ev_join, 0));
     HIP_CHECK(e, vtr::launch_shade(f, sc, ps, v, cap, e->stream));
     if (v < f.bounce_limit) {
+      HIP_CHECK(e, hipEventRecord(R.ev_fork, e->stream));
+      HIP_CHECK(e, hipStreamWaitEvent(R.side, R.ev_fork, 0));
       vtk::TraceParams sp = tp;
       sp.rays = ps.shadow_rays;
       sp.occluded = ps.shadow_occ;
       sp.n_ptr = ps.count + 2;
       sp.sided = 1;  // traceSurfaceVisibility: back faces of single-sided materials do not block
-      HIP_CHECK(e, vtk::launch_trace(sp, true, false, e->trace_grid, e->stream));
-      HIP_CHECK(e, vtr::launch_shadow_resolve(ps, cap, e->stream));
+      sp.counter = tp.counter + 8;
+      sp.spill = tp.spill + spill_half;
+      HIP_CHECK(e, vtk::launch_trace(sp, true, false, half_grid, R.side));
+      HIP_CHECK(e, vtr::launch_shadow_resolve(ps, cap, R.side));
+      HIP_CHECK(e, vtr::launch_reset_counts(ps, 2u, R.side));  // shadow count
+      HIP_CHECK(e, hipEventRecord(R.ev_join, R.side));
     }
-    HIP_CHECK(e, vtr::launch_reset_counts(ps, e->stream));
+    HIP_CHECK(e, vtr::launch_reset_counts(ps, 1u, e->stream));  // extension counts
     cur ^= 1;
   }
   if (f.nrc) {

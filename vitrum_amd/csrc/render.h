@@ -102,7 +102,8 @@ hipError_t launch_accumulate(const FrameParams& f, const PathState& s, hipStream
 hipError_t launch_nrc_apply(const PathState& s, uint32_t capacity, hipStream_t st);
 // Turns the recorded vertices of the training paths into (features, target) samples.
 hipError_t launch_nrc_targets(const FrameParams& f, const PathState& s, hipStream_t st);
-hipError_t launch_reset_counts(const PathState& s, hipStream_t st);
+// which & 1: extension counts rotate; which & 2: the shadow count is cleared.
+hipError_t launch_reset_counts(const PathState& s, uint32_t which, hipStream_t st);
 hipError_t launch_zero_f32(float* p, uint32_t n, hipStream_t st);
 
 }  // namespace vtr

@@ -941,11 +941,15 @@ __global__ void accumulate_kernel(FrameParams f, PathState s) {
 
 // Promotes the next extension queue to the current one and clears the
 // shadow count, on the device so a frame needs no host round trip.
-__global__ void rotate_counts_kernel(uint32_t* count) {
+// which & 1: extension counts (next queue becomes current); which & 2:
+// shadow count.
+__global__ void rotate_counts_kernel(uint32_t* count, uint32_t which) {
   if (threadIdx.x == 0) {
-    count[0] = count[1];
-    count[1] = 0u;
-    count[2] = 0u;
+    if (which & 1u) {
+      count[0] = count[1];
+      count[1] = 0u;
+    }
+    if (which & 2u) count[2] = 0u;
   }
 }
 
@@ -991,8 +995,8 @@ hipError_t launch_accumulate(const FrameParams& f, const PathState& s, hipStream
   return hipGetLastError();
 }
 
-hipError_t launch_reset_counts(const PathState& s, hipStream_t st) {
-  hipLaunchKernelGGL(rotate_counts_kernel, dim3(1), dim3(64), 0, st, s.count);
+hipError_t launch_reset_counts(const PathState& s, uint32_t which, hipStream_t st) {
+  hipLaunchKernelGGL(rotate_counts_kernel, dim3(1), dim3(64), 0, st, s.count, which);
   return hipGetLastError();
 }
 
