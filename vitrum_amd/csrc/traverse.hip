@@ -464,7 +464,13 @@ __global__ __launch_bounds__(BLOCK, 4) void trace_kernel(TraceParams p) {
   // Once a refill hands out indices past the last ray the supply is gone for
   // good; the wave then only finishes its active lanes (no further atomics on
   // the shared counter, which every finishing wave would otherwise hit).
-  bool exhausted = false;
+  // The first fill is static -- wave w takes rays [w * take_cap, (w + 1) *
+  // take_cap) -- so a launch starts without every wave hitting the shared
+  // counter at once (and an empty launch touches it not at all); the counter
+  // hands out the rays after that range.
+  bool exhausted = false, first_fill = true;
+  const uint32_t wave_id = blockIdx.x * (BLOCK / 64u) + threadIdx.x / 64u;
+  const uint32_t static_end = n_waves * take_cap;
   for (;;) {
     // ---- wave-level refill: ballot idle lanes, one atomic, prefix ranks ----
     const uint64_t idle = __ballot(!active);
@@ -472,8 +478,13 @@ __global__ __launch_bounds__(BLOCK, 4) void trace_kernel(TraceParams p) {
     if (!exhausted && (n_idle >= p.refill_min || idle == __ballot(1))) {
       const uint32_t take = min(n_idle, take_cap);
       uint32_t base = 0;
-      if (lane == 0) base = atomicAdd(p.counter, take);
-      base = __shfl(base, 0);
+      if (first_fill) {
+        base = wave_id * take_cap;  // every lane is idle: take == take_cap
+      } else {
+        if (lane == 0) base = static_end + atomicAdd(p.counter, take);
+        base = __shfl(base, 0);
+      }
+      first_fill = false;
       exhausted = base + take >= n_rays;
       const uint32_t rank = (uint32_t)__popcll(idle & lane_below);
       if (!active && rank < take) {
