@@ -91,7 +91,7 @@ NRC_TRAIN_FLOPS = NRC_FWD_FLOPS + 2 * (3 * 64 + 3 * 64 * 64) + 2 * (4 * 64 * 64 
 BF16_PEAK_TFLOPS = 2500.0
 # Identifies the traversal kernel configuration the committed traffic
 # measurement belongs to (child order, refill threshold, steps per refill).
-KERNEL_KEY = "trace_kernel<closest> nodes=4-wide leaves=lane-compacted near_first=1 refill=8 steps=16 defer=12 lds=20 waves=4"
+KERNEL_KEY = "trace_kernel<closest> nodes=4-wide leaves=lane-compacted near_first=1 first_fill=static refill=8 steps=16 defer=12 lds=20 waves=4"
 
 
 def nrc_mlp(eng, reps=10):
