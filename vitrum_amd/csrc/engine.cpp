@@ -364,6 +364,7 @@ vt_status vt_memcpy_d2d(vt_engine* eng, void* d_dst, const void* d_src, size_t b
 
 vt_status vt_synchronize(vt_engine* eng) {
   if (!eng) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "engine is NULL");
+  if (vt_status s = engine_nrc_join(eng)) return s;
   HIP_TRY(eng, hipStreamSynchronize(eng->stream));
   return VT_OK;
 }

@@ -100,6 +100,11 @@ vt_status engine_trace_cwbvh(vt_engine* e, const float4* d_rays, uint32_t n, con
                              uint32_t* d_hits, float* d_hit_pos, uint32_t sided);
 // Cache hooks for the integrator (nrc.cpp): counts are device words.
 vt_status engine_nrc_ensure(vt_engine* e);
+// The path tracer's cache training step runs on a stream of its own: join
+// makes the engine stream wait for it, quiesce waits for it on the host
+// (before buffers it reads are freed).
+vt_status engine_nrc_join(vt_engine* e);
+vt_status engine_nrc_quiesce(vt_engine* e);
 vt_status engine_nrc_infer_counted(vt_engine* e, const float* d_feat, const uint32_t* d_count, float* d_rgb,
                                    uint32_t capacity);
 vt_status engine_nrc_train_counted(vt_engine* e, const float* d_feat, const float* d_tgt, const uint32_t* d_count,

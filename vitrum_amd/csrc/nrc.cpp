@@ -16,10 +16,20 @@ struct NrcState {
   // HIP events around the last inference and the last training launch.
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   bool timed_infer = false, timed_train = false;
+  // The path tracer's training step runs on `train` (default priority) so it
+  // overlaps the start of the next frame; `trained` marks its end, and every
+  // later use of the cache on the engine stream waits for it first.
+  hipStream_t train = nullptr;
+  hipEvent_t ready = nullptr, trained = nullptr;
+  bool train_pending = false;
   void release() {
+    if (train) (void)hipStreamSynchronize(train);
     for (DeviceBuffer* b : {&master, &m, &v, &grad, &frags, &loss, &part, &dx, &feats, &targets, &out}) b->release();
     for (hipEvent_t& x : ev)
       if (x) (void)hipEventDestroy(x), x = nullptr;
+    for (hipEvent_t* x : {&ready, &trained})
+      if (*x) (void)hipEventDestroy(*x), *x = nullptr;
+    if (train) (void)hipStreamDestroy(train), train = nullptr;
   }
 };
 
@@ -65,8 +75,25 @@ static void init_weights(uint32_t seed, std::vector<float>& w) {
     w[vtn::kMlpParams + i] = (float)((((double)next() / 4294967296.0) * 2.0 - 1.0) * 1e-4);
 }
 
+// Makes the engine stream wait for a training step still running on the
+// training stream.
+static vt_status nrc_join(vt_engine* e) {
+  NrcState* s = e->nrc;
+  if (!s || !s->train_pending) return VT_OK;
+  s->train_pending = false;
+  NRC_HIP(e, hipStreamWaitEvent(e->stream, s->trained, 0));
+  return VT_OK;
+}
+
+vt_status engine_nrc_join(vt_engine* e) { return nrc_join(e); }
+
+vt_status engine_nrc_quiesce(vt_engine* e) {
+  if (e->nrc && e->nrc->train) NRC_HIP(e, hipStreamSynchronize(e->nrc->train));
+  return VT_OK;
+}
+
 static vt_status nrc_ensure(vt_engine* e) {
-  if (e->nrc) return VT_OK;
+  if (e->nrc) return nrc_join(e);
   NrcState* s = new NrcState();
   s->opts.learning_rate = 1e-2f;
   s->opts.beta1 = 0.9f;
@@ -83,6 +110,9 @@ static vt_status nrc_ensure(vt_engine* e) {
   st = st ? st : s->part.ensure(e, (size_t)vtn::kGradSlots * vtn::kMlpParams * sizeof(float));
   for (hipEvent_t& x : s->ev)
     if (!st && hipEventCreate(&x) != hipSuccess) st = VT_ERR_HIP;
+  for (hipEvent_t* x : {&s->ready, &s->trained})
+    if (!st && hipEventCreateWithFlags(x, hipEventDisableTiming) != hipSuccess) st = VT_ERR_HIP;
+  if (!st && hipStreamCreateWithFlags(&s->train, hipStreamNonBlocking) != hipSuccess) st = VT_ERR_HIP;
   if (st) {
     engine_release_nrc(e);
     return st;
@@ -109,9 +139,9 @@ static bool finite_features(const float* f, size_t n) {
   return true;
 }
 
-static vt_status nrc_apply_adam(vt_engine* e);
+static vt_status nrc_apply_adam(vt_engine* e, hipStream_t st);
 static vt_status nrc_accumulate(vt_engine* e, const float* d_features, const float* d_targets, uint32_t n,
-                                const uint32_t* d_count = nullptr);
+                                const uint32_t* d_count, hipStream_t st);
 
 vt_status engine_nrc_ensure(vt_engine* e) { return nrc_ensure(e); }
 
@@ -127,8 +157,15 @@ vt_status engine_nrc_train_counted(vt_engine* e, const float* d_feat, const floa
                                    uint32_t capacity) {
   if (vt_status s = nrc_ensure(e)) return s;
   if (capacity == 0) return VT_OK;
-  if (vt_status s = nrc_accumulate(e, d_feat, d_tgt, capacity, d_count)) return s;
-  return nrc_apply_adam(e);
+  NrcState& n = *e->nrc;
+  // Fork: the step reads the records written so far on the engine stream.
+  NRC_HIP(e, hipEventRecord(n.ready, e->stream));
+  NRC_HIP(e, hipStreamWaitEvent(n.train, n.ready, 0));
+  if (vt_status s = nrc_accumulate(e, d_feat, d_tgt, capacity, d_count, n.train)) return s;
+  if (vt_status s = nrc_apply_adam(e, n.train)) return s;
+  NRC_HIP(e, hipEventRecord(n.trained, n.train));
+  n.train_pending = true;
+  return VT_OK;
 }
 
 extern "C" {
@@ -145,6 +182,7 @@ vt_status vt_nrc_reset(vt_engine* e, const vt_nrc_options* opts) {
     vt_status s = nrc_ensure(e);  // allocates, then re-enters with defaults
     if (s || !opts) return s;
   }
+  if (vt_status st = nrc_join(e)) return st;
   NrcState& s = *e->nrc;
   if (opts) {
     if (!(opts->learning_rate > 0) || !(opts->beta1 >= 0 && opts->beta1 < 1) ||
@@ -205,21 +243,21 @@ vt_status vt_nrc_infer(vt_engine* e, const float* features, uint32_t n, float* r
 // Accumulates the batch gradient into the engine's gradient buffer and the
 // batch loss into loss[0] (both device memory of the engine).
 static vt_status nrc_accumulate(vt_engine* e, const float* d_features, const float* d_targets, uint32_t n,
-                                const uint32_t* d_count) {
+                                const uint32_t* d_count, hipStream_t stream) {
   NrcState& s = *e->nrc;
   NRC_HIP(e, hipSetDevice(e->device));
   if (vt_status st = s.dx.ensure(e, (size_t)n * 16u * sizeof(float))) return st;
-  NRC_HIP(e, vtn::launch_fill((float*)s.loss.ptr, 1, 0.0f, e->stream));
-  NRC_HIP(e, hipEventRecord(s.ev[2], e->stream));
+  NRC_HIP(e, vtn::launch_fill((float*)s.loss.ptr, 1, 0.0f, stream));
+  NRC_HIP(e, hipEventRecord(s.ev[2], stream));
   NRC_HIP(e, vtn::launch_train(d_features, d_targets, n, d_count, (const uint4*)s.frags.ptr,
                                (const float*)s.master.ptr + vtn::kMlpParams, (float*)s.grad.ptr, (float*)s.part.ptr,
-                               (float*)s.dx.ptr, (float*)s.loss.ptr, e->stream));
-  NRC_HIP(e, hipEventRecord(s.ev[3], e->stream));
+                               (float*)s.dx.ptr, (float*)s.loss.ptr, stream));
+  NRC_HIP(e, hipEventRecord(s.ev[3], stream));
   s.timed_train = true;
   return VT_OK;
 }
 
-static vt_status nrc_apply_adam(vt_engine* e) {
+static vt_status nrc_apply_adam(vt_engine* e, hipStream_t stream) {
   NrcState& s = *e->nrc;
   s.step += 1;
   vtn::AdamParams p{};
@@ -231,8 +269,8 @@ static vt_status nrc_apply_adam(vt_engine* e) {
   p.bc1 = (float)(1.0 - std::pow((double)s.opts.beta1, (double)s.step));
   p.bc2 = (float)(1.0 - std::pow((double)s.opts.beta2, (double)s.step));
   NRC_HIP(e, vtn::launch_adam((float*)s.master.ptr, (float*)s.grad.ptr, (float*)s.m.ptr, (float*)s.v.ptr, p,
-                              e->stream));
-  NRC_HIP(e, vtn::launch_pack((const float*)s.master.ptr, (uint4*)s.frags.ptr, e->stream));
+                              stream));
+  NRC_HIP(e, vtn::launch_pack((const float*)s.master.ptr, (uint4*)s.frags.ptr, stream));
   return VT_OK;
 }
 
@@ -240,8 +278,8 @@ vt_status vt_nrc_train_step_device(vt_engine* e, const float* d_features, const 
   if (!e || (n && (!d_features || !d_targets))) return VT_ERR_INVALID_ARGUMENT;
   if (n == 0) return VT_OK;
   if (vt_status s = nrc_ensure(e)) return s;
-  if (vt_status s = nrc_accumulate(e, d_features, d_targets, n)) return s;
-  return nrc_apply_adam(e);
+  if (vt_status s = nrc_accumulate(e, d_features, d_targets, n, nullptr, e->stream)) return s;
+  return nrc_apply_adam(e, e->stream);
 }
 
 vt_status vt_nrc_timing(vt_engine* e, float* infer_ms, float* train_ms) {
@@ -301,7 +339,8 @@ vt_status vt_nrc_gradient(vt_engine* e, const float* features, const float* targ
   NRC_HIP(e, vtn::launch_fill((float*)s.grad.ptr, vtn::kParams, 0.0f, e->stream));
   if (n) {
     if (vt_status st = stage_batch(e, features, targets, n)) return st;
-    if (vt_status st = nrc_accumulate(e, (const float*)s.feats.ptr, (const float*)s.targets.ptr, n)) return st;
+    if (vt_status st = nrc_accumulate(e, (const float*)s.feats.ptr, (const float*)s.targets.ptr, n, nullptr, e->stream))
+      return st;
   } else {
     NRC_HIP(e, vtn::launch_fill((float*)s.loss.ptr, 1, 0.0f, e->stream));
   }

@@ -130,6 +130,7 @@ struct RenderState {
 
 void engine_release_render(vt_engine* e) {
   if (!e->render) return;
+  (void)engine_nrc_quiesce(e);
   e->render->release_paths();
   e->render->release_scene();
   if (e->render->side) (void)hipStreamSynchronize(e->render->side);
@@ -332,6 +333,7 @@ static vt_status ensure_paths(vt_engine* e, uint32_t w, uint32_t h) {
   const uint32_t train = R.nrc_enabled ? R.nrc_train_paths : 0u;
   const uint32_t paths = w * h + train;
   if (R.width == w && R.height == h && R.paths == paths) return VT_OK;
+  if (vt_status s = engine_nrc_quiesce(e)) return s;  // a training step may still read the records
   R.release_paths();
   const size_t n = (size_t)paths;
   const size_t pix = (size_t)w * h;
@@ -468,7 +470,9 @@ vt_status vt_render_frame(vt_engine* e, const vt_frame* fr) {
   }
   HIP_CHECK(e, hipEventRecord(R.ev0, e->stream));
   HIP_CHECK(e, vtk::launch_zero((uint32_t*)R.count.ptr, 16, e->stream));
-  if (f.nrc) HIP_CHECK(e, vtk::launch_zero((uint32_t*)R.nrc_count.ptr, 16, e->stream));
+  // Query count only: the previous frame's training step may still read the
+  // record count (word 1), which is cleared after the cache is joined below.
+  if (f.nrc) HIP_CHECK(e, vtk::launch_zero((uint32_t*)R.nrc_count.ptr, 1, e->stream));
   int cur = 0;
   ps.queue = (uint32_t*)R.queue[cur].ptr;
   ps.rays = (float4*)R.rays[cur].ptr;
@@ -526,6 +530,7 @@ vt_status vt_render_frame(vt_engine* e, const vt_frame* fr) {
   }
   HIP_CHECK(e, vtr::launch_accumulate(f, ps, e->stream));
   if (f.nrc) {
+    HIP_CHECK(e, vtk::launch_zero((uint32_t*)R.nrc_count.ptr + 1, 1, e->stream));
     HIP_CHECK(e, vtr::launch_nrc_targets(f, ps, e->stream));
     if (vt_status s = engine_nrc_train_counted(e, ps.train_feat, ps.train_tgt, ps.nrc_count + 1,
                                                f.train_paths * vtr::kTrainSlots))
