@@ -158,8 +158,11 @@ def time_to_rmse(W, width, height, ref_spp, target, train_paths, max_frames, tea
     Each estimator is rendered for up to max_frames samples; its measured
     curve e^2(N) is fitted by b^2 + v / N (b = bias, v = per-sample variance,
     least squares over N >= 8) and the time to `target` is
-    N* x (mean GPU frame time), N* = v / (target^2 - b^2); None when the bias
-    alone exceeds the target.  Frame times are HIP events around each frame."""
+    N* x (mean frame wall time), N* = v / (target^2 - b^2); None when the bias
+    alone exceeds the target.  Frame times are host wall-clock intervals over
+    frames enqueued back to back, each interval ending with a full engine
+    synchronisation (the cache's training step included) before an image is
+    read back; the read-backs are outside the intervals."""
     from vitrum_amd import native
     seed = 0x9E3779B9
     eng = native.Engine(0)
@@ -179,7 +182,7 @@ def time_to_rmse(W, width, height, ref_spp, target, train_paths, max_frames, tea
            "reference_spp": 2 * half, "target_rel_rmse": target, "train_paths_per_frame": train_paths,
            "teacher_vertices": teacher_vertices,
            "reference_rel_rmse": round(float(np.sqrt(ref_var)) / mean_lum, 5),
-           "method": "noise-corrected MSE vs split-half reference; fit e^2 = b^2 + v/N"}
+           "method": "noise-corrected MSE vs split-half reference; fit e^2 = b^2 + v/N; frame time = wall clock, frames back to back"}
     for name, nrc in (("plain", False), ("nrc", True)):
         eng.nrc_render_config(nrc, train_paths)
         if nrc:
@@ -188,16 +191,21 @@ def time_to_rmse(W, width, height, ref_spp, target, train_paths, max_frames, tea
         eng.reset()
         gpu_ms, curve = 0.0, []
         check = 1
+        eng.synchronize()
+        t_start = time.perf_counter()
         for f in range(max_frames):
             eng.render_frame(ivp, width, height, f, seed, 8)
-            gpu_ms += eng.frame_time_ms()
             if f + 1 == check or f + 1 == max_frames:
+                eng.synchronize()
+                gpu_ms += (time.perf_counter() - t_start) * 1e3
                 img = eng.capture(width, height)[..., :3].astype(np.float64)
                 mse = float(np.mean((lum(img) - lum(ref)) ** 2)) - ref_var
                 e = float(np.sqrt(max(mse, 0.0))) / mean_lum
                 curve.append([f + 1, round(gpu_ms, 2), round(e, 5)])
                 log(f"{name}: {f + 1} spp, {gpu_ms:.0f} ms, rel RMSE {e:.4f}")
                 check = max(check + 1, int(check * 1.5))
+                eng.synchronize()
+                t_start = time.perf_counter()
         pts = np.array([[n, e] for n, _, e in curve if n >= 8], np.float64)
         frame_ms = gpu_ms / max_frames
         entry = {"frames": max_frames, "frame_ms": round(frame_ms, 3), "curve": curve}
