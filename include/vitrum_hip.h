@@ -273,7 +273,10 @@ vt_status vt_memcpy_d2h(vt_engine *eng, void *dst, const void *d_src, size_t byt
  * sizes), so its HBM counters are those of a plain copy of known size. */
 vt_status vt_memcpy_d2d(vt_engine *eng, void *d_dst, const void *d_src, size_t bytes);
 vt_status vt_synchronize(vt_engine *eng);
-/* The engine's hipStream_t, for callers that order their own work with it. */
+/* The engine's hipStream_t, for callers that order their own work with it.
+ * The renderer also forks work onto streams of its own and joins it back
+ * before the frame's image is complete; a cache training step may still run
+ * after that until the next cache use or vt_synchronize. */
 vt_status vt_get_stream(vt_engine *eng, void **stream);
 
 /* Kernel timing: milliseconds of the last traversal launch measured with HIP
