@@ -104,3 +104,28 @@ def test_teacher_suffix_ends_with_a_cache_query(engine, cornell):
     finally:
         engine.nrc_teacher_vertices(4)
         engine.nrc_render_config(False)
+
+
+def test_cache_calls_see_the_finished_training_step(engine, cornell):
+    """The path tracer's training step runs on a stream of its own and may
+    still run when vt_render_frame returns; every cache entry point joins it
+    first.  Weights and loss read straight after a frame equal those read
+    after a full synchronisation, and the step did change the weights."""
+    sc, res, ivp, ref = cornell
+    try:
+        engine.nrc_reset(seed=9)
+        w0 = engine.nrc_get_weights()
+        engine.nrc_render_config(True, 8192)
+        engine.reset()
+        for f in range(3):
+            engine.render_frame(ivp, res, res, 7000 + f, SEED, 8)
+        w1 = engine.nrc_get_weights()
+        loss1 = engine.nrc_last_loss()
+        engine.synchronize()
+        w2 = engine.nrc_get_weights()
+        loss2 = engine.nrc_last_loss()
+        assert np.array_equal(w1.view(np.uint32), w2.view(np.uint32))
+        assert loss1 == loss2 and np.isfinite(loss1)
+        assert not np.array_equal(w0, w1)
+    finally:
+        engine.nrc_render_config(False)
