@@ -82,6 +82,15 @@ static uint32_t pcg_next(orng *r) {
   return orc_pcg_next(&r->s);
 }
 static double rand01(orng *s) { return (double)(pcg_next(s) >> 8) / 16777216.0; }
+/* Polar disc coordinates sqrt(u1)cos(2 pi u2), sqrt(u1)sin(2 pi u2) rounded
+ * once to binary32: the reference draws these in f32, and the product
+ * evaluates them in binary64 and rounds once, so both sides start a sampled
+ * direction from the same two values (bsdf.wgsl.ts:1648-1664, 1710-1735). */
+static void disc_polar(double u1, double u2, double *a, double *b) {
+  double phi = 2.0 * PI * u2, r = sqrt(u1);
+  *a = (double)(float)(r * cos(phi));
+  *b = (double)(float)(r * sin(phi));
+}
 
 /* represented_bernoulli_probability_f32 (pcg.wgsl.ts:31-40) */
 static double represented(double p) {
@@ -258,8 +267,8 @@ static d3 vndf_sample(d3 wo, double alpha, orng *rng) {
   d3 t1 = lensq > 1e-10 ? scl(D3(-vh.y, vh.x, 0), 1.0 / sqrt(lensq)) : D3(1, 0, 0);
   d3 t2 = cross(vh, t1);
   double u1 = rand01(rng), u2 = rand01(rng);
-  double r = sqrt(u1), phi = 2.0 * PI * u2;
-  double a = r * cos(phi), b = r * sin(phi);
+  double a, b;
+  disc_polar(u1, u2, &a, &b);
   double s = 0.5 * (1.0 + vh.z);
   b = (1.0 - s) * sqrt(maxd(0, 1.0 - a * a)) + s * b;
   d3 nh = add(add(scl(t1, a), scl(t2, b)), scl(vh, sqrt(maxd(0, 1.0 - a * a - b * b))));
@@ -363,9 +372,11 @@ static d3 env_cell_dir(uint32_t x, uint32_t y, uint32_t w, uint32_t h, double xa
   double phi0 = ((double)x / w - 0.5) * (2.0 * PI), phi1 = ((double)(x + 1) / w - 0.5) * (2.0 * PI);
   double th0 = (double)y / h * PI, th1 = (double)(y + 1) / h * PI;
   double phi = phi0 + (phi1 - phi0) * xa;
-  double ct = cos(th0) + (cos(th1) - cos(th0)) * xb;
+  double c0 = cos(th0), c1 = cos(th1);
+  double ct = c0 + (c1 - c0) * xb;
   double st = sqrt(maxd(0, 1 - ct * ct));
-  return D3(cos(phi) * st, ct, sin(phi) * st);
+  /* each component rounded once to binary32, as the product rounds it */
+  return D3((double)(float)(cos(phi) * st), (double)(float)ct, (double)(float)(sin(phi) * st));
 }
 
 /* ---- render ----------------------------------------------------------- */
@@ -651,8 +662,9 @@ static void render_path(const rctx *c, uint32_t px, uint32_t py, uint32_t frame_
       wi = safe_normalize(sub(inc, scl(hw, 2.0 * dot(hw, inc))));
     } else {
       double u1 = rand01(&rng), u2 = rand01(&rng);
-      double r = sqrt(u1), phi = 2.0 * PI * u2, ct = sqrt(maxd(0, 1 - u1));
-      wi = safe_normalize(add(add(scl(tb, r * cos(phi)), scl(bb, r * sin(phi))), scl(n, ct)));
+      double a, b, ct = (double)(float)sqrt(maxd(0, 1 - u1));
+      disc_polar(u1, u2, &a, &b);
+      wi = safe_normalize(add(add(scl(tb, a), scl(bb, b)), scl(n, ct)));
     }
     sig = mixlog(sig, glossy ? 0xB1 : 0xB0, ev);
     /* finalizeFiniteBounceSample (bsdf.wgsl.ts:1863-1926) */

@@ -92,3 +92,214 @@ O_LUT_E = [0.1375, 0.5617, 0.7546, 0.8522, 0.9111, 0.9505, 0.9788, 1.0,
            0.6669, 0.6137, 0.5657, 0.5286, 0.5, 0.478, 0.4611, 0.4483,
            0.6017, 0.537, 0.4773, 0.4296, 0.3905, 0.358, 0.3305, 0.3069]
 O_LUT_EAVG = [0.9106, 0.8931, 0.8629, 0.8094, 0.725, 0.6147, 0.4931, 0.3766]
+
+
+# ---------------------------------------------------------------------------
+# Mesh-area light closed forms
+# ---------------------------------------------------------------------------
+
+def _quad_mesh(p0, eu, ev, n_sub):
+    """Grid of n_sub x n_sub cells over p0 + s eu + t ev; triangles wound so
+    that cross(b - a, c - a) points along cross(eu, ev)."""
+    p0, eu, ev = (np.asarray(x, np.float64) for x in (p0, eu, ev))
+    g = np.arange(n_sub + 1) / n_sub
+    pts = p0 + g[:, None, None] * eu + g[None, :, None] * ev
+    pos = pts.reshape(-1, 3)
+    idx = []
+    for i in range(n_sub):
+        for j in range(n_sub):
+            a, b = i * (n_sub + 1) + j, (i + 1) * (n_sub + 1) + j
+            c, d = b + 1, a + 1
+            idx += [a, b, c, a, c, d]
+    n = np.cross(eu, ev)
+    nrm = np.tile(n / np.linalg.norm(n), (pos.shape[0], 1))
+    return pos, nrm, np.asarray(idx, np.uint32).reshape(-1, 3)
+
+
+def emissive_box(le, albedo, roughness=1.0, metallic=0.0, n_sub=2, viewed_wall_emits=True):
+    """A closed box [-1, 1]^3 whose walls emit Le toward the inside
+    (single-sided, the front faces point inward) and reflect with the base
+    BSDF; the camera sits at the centre and looks at the z = +1 wall head-on
+    through a 3-degree field, so wo is within 1.5 degrees of that wall's
+    normal.  Every direction from a point on that wall meets another wall's
+    front face, so the radiance arriving there is exactly Le and one bounce
+    adds Le * rho(wo), rho the BSDF's directional albedo.  With
+    `viewed_wall_emits` False the viewed wall reflects only: its own coplanar
+    triangles, which a light tree rates highest at zero distance but which
+    add nothing there, are then not emitters."""
+    faces = [((-1, -1, 1), (0, 2, 0), (2, 0, 0)), ((-1, -1, -1), (2, 0, 0), (0, 2, 0)),
+             ((1, -1, -1), (0, 0, 2), (0, 2, 0)), ((-1, -1, -1), (0, 2, 0), (0, 0, 2)),
+             ((-1, 1, -1), (2, 0, 0), (0, 0, 2)), ((-1, -1, -1), (0, 0, 2), (2, 0, 0))]
+    mat = {"baseColor": [albedo] * 3, "roughness": roughness, "metallic": metallic,
+           "emissive": list(le), "emissiveIntensity": 1.0}
+    prims = []
+    for wall, (p0, eu, ev) in enumerate(faces):
+        pos, nrm, idx = _quad_mesh(p0, eu, ev, n_sub)
+        centre = np.asarray(p0, np.float64) + 0.5 * (np.asarray(eu) + np.asarray(ev))
+        assert np.dot(nrm[0], -centre) > 0  # front faces point into the box
+        m = dict(mat)
+        if wall == 0 and not viewed_wall_emits:
+            m.pop("emissive")
+            m.pop("emissiveIntensity")
+        prims.append({"kind": "mesh", "positions": pos.astype(np.float32).ravel(),
+                      "normals": nrm.astype(np.float32).ravel(), "indices": idx.ravel(),
+                      "material": m})
+    cam = W.look_at_camera(eye=(0, 0, 0), target=(0, 0, 1), up=(0, 1, 0), vfov_deg=3)
+    return {"primitives": prims, "camera": cam, "scene_radius": 1.8, "sky": [0, 0, 0]}
+
+
+def disc_polygon(centre, radius, segments):
+    """Vertices of a regular polygon in the plane y = centre.y, counter-
+    clockwise seen from below (its triangle fan faces -y)."""
+    ang = 2 * np.pi * np.arange(segments) / segments
+    c = np.asarray(centre, np.float64)
+    return c + radius * np.stack([np.cos(ang), np.zeros_like(ang), -np.sin(ang)], axis=1)
+
+
+def disc_over_plane(le, albedo=0.5, roughness=1.0, metallic=0.0, height=1.0, radius=0.25,
+                    segments=48, view_deg=30.0):
+    """A small emitting polygon disc facing down at `height` above a floor
+    y = 0; the camera looks at the point under the disc's centre from
+    `view_deg` off the floor normal through a 0.4-degree field (the disc does
+    not block the view)."""
+    ring = disc_polygon((0, height, 0), radius, segments)
+    pos = np.concatenate([[[0, height, 0]], ring]).astype(np.float32)
+    idx = np.array([[0, 1 + (i + 1) % segments, 1 + i] for i in range(segments)], np.uint32)
+    a, b, c = (pos[idx[:, k]].astype(np.float64) for k in range(3))
+    assert (np.cross(b - a, c - a)[:, 1] < 0).all()  # the fan emits downward
+    disc = {"kind": "mesh", "positions": pos.ravel(), "normals": np.tile([0, -1, 0], len(pos)).astype(np.float32),
+            "indices": idx.ravel(),
+            "material": {"baseColor": [0, 0, 0], "roughness": 1.0, "metallic": 0.0,
+                         "emissive": list(le), "emissiveIntensity": 1.0}}
+    fpos = np.array([-200, 0, -200, 200, 0, -200, 200, 0, 200, -200, 0, 200], np.float32)
+    floor = {"kind": "mesh", "positions": fpos, "normals": np.tile([0, 1, 0], 4).astype(np.float32),
+             "indices": np.array([0, 2, 1, 0, 3, 2], np.uint32),
+             "material": {"baseColor": [albedo] * 3, "roughness": roughness, "metallic": metallic}}
+    t = np.radians(view_deg)
+    eye = (0.0, 5 * np.cos(t), 5 * np.sin(t))
+    cam = W.look_at_camera(eye=eye, target=(0, 0, 0), up=(0, 1, 0), vfov_deg=0.4)
+    return {"primitives": [floor, disc], "camera": cam, "scene_radius": 300.0, "sky": [0, 0, 0],
+            "_disc": pos.astype(np.float64), "_disc_idx": idx}
+
+
+def base_bsdf(albedo, roughness, metallic, n, wo, wi):
+    """The base BSDF f(wo, wi) (diffuse (1-F) kd/pi + GGX D G1 G1 F / (4 cos
+    cos) + Kulla-Conty multiscatter) for unit vectors, wi of shape (..., 3);
+    the formula the quadratures integrate."""
+    ndl = np.maximum(wi @ n, 0.0)
+    ndv = max(float(wo @ n), 0.0)
+    h = wi + wo
+    h = h / np.linalg.norm(h, axis=-1, keepdims=True)
+    ndh, vdh = np.maximum(h @ n, 0), np.maximum(h @ wo, 0)
+    f0 = 0.04 * (1 - metallic) + albedo * metallic
+    F = f0 + (1 - f0) * np.clip(1 - vdh, 0, 1) ** 5
+    alpha = max(roughness * roughness, 1e-3)
+    d = alpha ** 2 / (np.pi * ((1 - ndh ** 2) + ndh ** 2 * alpha ** 2) ** 2)
+
+    def g1(c):
+        c = np.asarray(c, np.float64)
+        t2 = np.maximum(0, 1 - c * c) / np.maximum(c * c, 1e-10)
+        return np.where(c <= 1e-6, 0.0, 2.0 / (1 + np.sqrt(1 + alpha * alpha * t2)))
+
+    spec = d * g1(ndv) * g1(ndl) * F / np.maximum(4 * ndv * ndl, 1e-6)
+    diff = (1 - F) * (1 - metallic) * albedo / np.pi
+    E = np.array(O_LUT_E).reshape(8, 8)
+    Eavg = np.array(O_LUT_EAVG)
+
+    def lut(c):
+        fr, fm = roughness * 7, np.clip(c, 0, 1) * 7
+        r0, m0 = int(np.floor(fr)), np.floor(fm).astype(int)
+        r1, m1 = min(r0 + 1, 7), np.minimum(m0 + 1, 7)
+        tr, tm = fr - r0, fm - m0
+        e0 = E[r0, m0] + (E[r0, m1] - E[r0, m0]) * tm
+        e1 = E[r1, m0] + (E[r1, m1] - E[r1, m0]) * tm
+        return np.clip(e0 + (e1 - e0) * tr, 0.02, 1)
+
+    fr_ = roughness * 7
+    r0 = int(np.floor(fr_))
+    eavg = np.clip(Eavg[r0] + (Eavg[min(r0 + 1, 7)] - Eavg[r0]) * (fr_ - r0), 0.3, 1)
+    favg = f0 + (1 - f0) / 21
+    fms = favg * favg * eavg / max(1 - favg * (1 - eavg), 1e-4)
+    ms = fms * (1 - lut(ndv)) * (1 - lut(ndl)) / max(np.pi * (1 - eavg), 1e-6)
+    ok = (ndl > 1e-5) & (ndv > 1e-5)
+    return np.where(ok, diff + spec + ms, 0.0)
+
+
+def _triangle_points(a, b, c, m):
+    """Centroids and areas of the m^2 congruent sub-triangles of (a, b, c)."""
+    pts, i = [], np.arange(m)
+    for r in range(m):
+        for s in range(m - r):
+            pts.append(a + (b - a) * (r + 1 / 3) / m + (c - a) * (s + 1 / 3) / m)
+            if r + s < m - 1:
+                pts.append(a + (b - a) * (r + 2 / 3) / m + (c - a) * (s + 2 / 3) / m)
+    area = 0.5 * np.linalg.norm(np.cross(b - a, c - a)) / (m * m)
+    del i
+    return np.asarray(pts), area
+
+
+def polygon_reflected_radiance(x, n, wo, tris, le, brdf=None, m=24):
+    """Reflected radiance at x toward wo from an emitting triangle set of
+    radiance le (front faces toward x), by area quadrature of
+    f(wo, wi) Le cos(theta_x) cos(theta_l) / d^2; with brdf None f = 1/pi, so
+    the result is the irradiance over pi."""
+    total = 0.0
+    for a, b, c in tris:
+        pts, area = _triangle_points(a, b, c, m)
+        nl = np.cross(b - a, c - a)
+        nl /= np.linalg.norm(nl)
+        v = pts - x
+        d2 = np.einsum("ij,ij->i", v, v)
+        wi = v / np.sqrt(d2)[:, None]
+        cx = np.maximum(wi @ n, 0)
+        cl = np.maximum(-(wi @ nl), 0)
+        f = np.full(len(pts), 1 / np.pi) if brdf is None else brdf(wi)
+        total += np.sum(f * cx * cl / d2) * area
+    return np.asarray(le, np.float64) * total
+
+
+def polygon_irradiance_lambert(x, n, ring):
+    """Irradiance at x (normal n) from a unit-radiance Lambertian polygon with
+    vertices `ring`: Lambert's closed form E = (1/2) sum_i beta_i (gamma_i . n),
+    beta_i the angle the edge (v_i, v_i+1) subtends at x and gamma_i the unit
+    normal of the plane through x and that edge (oriented by the polygon's
+    winding as seen from x)."""
+    r = ring - x
+    r /= np.linalg.norm(r, axis=1, keepdims=True)
+    e = 0.0
+    for i in range(len(r)):
+        p, q = r[i], r[(i + 1) % len(r)]
+        g = np.cross(p, q)
+        beta = np.arctan2(np.linalg.norm(g), p @ q)
+        e += beta * (g / np.linalg.norm(g)) @ n
+    return abs(0.5 * e)
+
+
+def camera_floor_points(scene, res):
+    """Where the pixel-centre rays of `scene`'s camera meet the floor y = 0."""
+    ivp = np.asarray(W.inv_view_proj(scene["camera"], res, res), np.float64).reshape(4, 4).T
+    pts = []
+    eye = scene["camera"]["eye"]
+    for py in range(res):
+        for px in range(res):
+            nx, ny = (px + 0.5) / res * 2 - 1, 1 - (py + 0.5) / res * 2
+            q = ivp @ np.array([nx, ny, 1.0, 1.0])
+            d = q[:3] / q[3] - eye
+            t = -eye[1] / d[1]
+            pts.append(eye + t * d)
+    return np.asarray(pts)
+
+
+def disc_expected_radiance(scene, le, albedo, roughness, metallic, res=8):
+    """Mean reflected radiance over the camera's footprint on the floor (an
+    8 x 8 grid of pixel-centre points), one bounce from the disc."""
+    pos, idx = scene["_disc"], scene["_disc_idx"]
+    tris = [(pos[i], pos[j], pos[k]) for i, j, k in idx]
+    n = np.array([0.0, 1.0, 0.0])
+    eye = scene["camera"]["eye"]
+    out = []
+    for x in camera_floor_points(scene, res):
+        wo = (eye - x) / np.linalg.norm(eye - x)
+        f = lambda wi: base_bsdf(albedo, roughness, metallic, n, wo, wi)  # noqa: E731
+        out.append(polygon_reflected_radiance(x, n, wo, tris, le, brdf=f))
+    return np.mean(out, axis=0)

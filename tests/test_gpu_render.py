@@ -4,9 +4,10 @@ Parity contract (same sampler seeds on both sides):
   * path match: >= 99.9% of paths take identical discrete decisions (hit
     triangles, light picks, lobes, roulette), compared by per-path hashes;
   * matched paths: per-pixel radiance within 2e-4 relative + 1e-6 absolute
-    for >= 99.8% of pixels and within 5e-3 relative for every pixel (f32
-    device arithmetic against binary64; glossy GGX lobes at roughness 0.05
-    amplify f32 direction rounding to a few 1e-4 on rare paths);
+    for >= 99.9% of pixels (f32 device arithmetic against binary64; the
+    sampled directions' polar coordinates are rounded once from binary64 on
+    both sides, so glossy lobes at roughness 0.05 start from the same local
+    directions);
   * every pixel of a multi-frame mean: |mu_gpu - mu_cpu| <= 2e-4 |mu_cpu| +
     4 sqrt(var / N) with >= 99.9% of pixels passing.
 """
@@ -43,9 +44,10 @@ def check_single_frame(engine, scene, res, sampling=0, light_tree=True):
     diff = np.abs(gpu - cpu)
     tol = 2e-4 * np.abs(cpu) + 1e-6
     ok = (diff <= tol).all(axis=2) | ~match
-    assert ok.mean() >= 0.998, f"matched-path radiance agreement {ok.mean():.5f}"
-    loose = (diff <= 5e-3 * np.abs(cpu) + 1e-6).all(axis=2) | ~match
-    assert loose.all(), f"matched paths beyond 5e-3: {np.argwhere(~loose)[:5].tolist()}"
+    rel = (diff / (np.abs(cpu) + 1e-6)).max(axis=2)[match]
+    print(f"path match {rate:.5f}; matched pixels within 1e-4: {np.mean(rel <= 1e-4):.5f}, "
+          f"2e-4: {np.mean(rel <= 2e-4):.5f}; max relative {rel.max():.3g}", flush=True)
+    assert ok.mean() >= 0.999, f"matched-path radiance agreement {ok.mean():.5f}"
     return gpu, cpu, rate
 
 
@@ -152,3 +154,41 @@ def test_directly_viewed_emitter_is_exact(engine):
     sc = {"primitives": [prim], "camera": cam, "scene_radius": 8.0, "sky": [0, 0, 0]}
     img, _ = gpu_frames(engine, sc, 16, 2, max_bounces=1)
     np.testing.assert_allclose(img.reshape(-1, 3), np.tile([2.0, 1.0, 0.5], (256, 1)), rtol=1e-6)
+
+
+# ---- mesh-area lights: closed forms (same scenes as the oracle's pins) ------
+
+def _image_bound(img, expect, rel=2e-3):
+    v = img.reshape(-1, 3)
+    return v.mean(axis=0), 4 * v.std(axis=0, ddof=1) / np.sqrt(v.shape[0]) + rel * np.abs(expect)
+
+
+@pytest.mark.parametrize("roughness,metallic,tree,viewed_wall_emits,n_sub", [
+    (1.0, 0.0, False, True, 2), (0.2, 1.0, False, True, 1), (0.3, 0.0, True, True, 2),
+    (1.0, 0.0, True, False, 2)])
+def test_emissive_enclosure_one_bounce(engine, uniform_engine, roughness, metallic, tree, viewed_wall_emits, n_sub):
+    """Inside a closed box of inward single-sided emitters the one-bounce
+    pixel is Le (viewed wall's emission) + Le rho(wo) exactly: light pick,
+    sqrt(r1) triangle sampling, area pdf with the light cosine and both MIS
+    sides on the device."""
+    from render_cases import emissive_box
+    le, a = np.array([1.0, 0.7, 0.4]), 0.6
+    sc = emissive_box(le.tolist(), a, roughness, metallic, n_sub=n_sub, viewed_wall_emits=viewed_wall_emits)
+    img, _ = gpu_frames(engine if tree else uniform_engine, sc, 64, 64, max_bounces=1)
+    expect = le * (float(viewed_wall_emits) + directional_albedo_at_normal(a, roughness, metallic))
+    mean, bound = _image_bound(img, expect)
+    assert (np.abs(mean - expect) <= bound).all(), (mean, expect, bound)
+
+
+@pytest.mark.parametrize("roughness,tree", [(1.0, True), (1.0, False), (0.4, True)])
+def test_disc_light_over_plane_matches_quadrature(engine, uniform_engine, roughness, tree):
+    """A 48-gon emitting disc over a floor seen 30 degrees off normal: the
+    device's one-bounce estimate converges to the area quadrature (pinned to
+    Lambert's polygon formula)."""
+    from render_cases import disc_expected_radiance, disc_over_plane
+    le, a = np.array([30.0, 20.0, 10.0]), 0.5
+    sc = disc_over_plane(le.tolist(), a, roughness, 0.0)
+    img, _ = gpu_frames(engine if tree else uniform_engine, sc, 64, 64, max_bounces=1)
+    expect = disc_expected_radiance(sc, le, a, roughness, 0.0)
+    mean, bound = _image_bound(img, expect)
+    assert (np.abs(mean - expect) <= bound).all(), (mean, expect, bound)

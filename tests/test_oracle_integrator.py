@@ -114,3 +114,111 @@ def test_light_tree_and_uniform_selection_share_the_expectation():
     um = u_img.reshape(frames, -1).mean(axis=1)
     se = np.sqrt(tm.var(ddof=1) / frames + um.var(ddof=1) / frames)
     assert abs(tm.mean() - um.mean()) <= 4 * se, (tm.mean(), um.mean(), se)
+
+
+# ---- mesh-area lights: closed forms ---------------------------------------
+
+def _mean_and_bound(imgs, expect, rel=2e-3):
+    """Image mean and its allowed deviation: 4 standard errors of the per-path
+    values (independent pixels and frames) plus `rel` for the quadrature."""
+    v = imgs.reshape(-1, 3)
+    se = v.std(axis=0, ddof=1) / np.sqrt(v.shape[0])
+    return v.mean(axis=0), 4 * se + rel * np.abs(expect)
+
+
+def test_polygon_quadrature_matches_lambert_formula():
+    """The area quadrature that the disc test integrates with reproduces
+    Lambert's closed-form irradiance of a polygon (on axis, and off axis with
+    a tilted receiver), and on axis tends to pi sin^2(alpha) for the disc."""
+    from render_cases import disc_polygon, polygon_irradiance_lambert, polygon_reflected_radiance
+    ring = disc_polygon((0, 1, 0), 0.25, 48)
+    tris = [(np.array([0, 1.0, 0]), ring[(i + 1) % 48], ring[i]) for i in range(48)]
+    for x, n in (((0, 0, 0), (0, 1, 0)), ((0.4, 0, -0.3), (0.2, 0.9, 0.1))):
+        x, n = np.asarray(x, float), np.asarray(n, float) / np.linalg.norm(n)
+        q = polygon_reflected_radiance(x, n, None, tris, [1.0])[0] * np.pi
+        assert q == pytest.approx(polygon_irradiance_lambert(x, n, ring), rel=2e-5)
+    fine = disc_polygon((0, 1, 0), 0.25, 4096)
+    e = polygon_irradiance_lambert(np.zeros(3), np.array([0, 1.0, 0]), fine)
+    assert e == pytest.approx(np.pi * 0.25 ** 2 / (0.25 ** 2 + 1.0), rel=1e-6)
+
+
+# (roughness, metallic, light tree, viewed wall emits).  Uniform selection
+# keeps the variance small.  The reference's light tree rates a node by the
+# distance to its bounding box, and the boxes of merged walls enclose the
+# whole interior, so inside an enclosure it picks walls near the shading point
+# almost always and the far wall about once in 10^4 draws (with the
+# matching 1/pdf): unbiased, but 30x noisier, and its bound is loose.
+# The last column is the wall tessellation: with one cell per wall the
+# triangles subtend large solid angles, their area pdfs are small, and a
+# glossy metal lobe gives the BSDF side of the MIS most of the weight.
+ENCLOSURE_CASES = [(1.0, 0.0, False, True, 2), (0.3, 0.0, False, True, 2), (0.5, 1.0, False, True, 2),
+                   (0.2, 1.0, False, True, 1), (1.0, 0.0, False, False, 2), (1.0, 0.0, True, False, 2),
+                   (0.3, 0.0, True, True, 2)]
+
+
+@pytest.mark.parametrize("roughness,metallic,light_tree,viewed_wall_emits,n_sub", ENCLOSURE_CASES)
+def test_emissive_enclosure_one_bounce(roughness, metallic, light_tree, viewed_wall_emits, n_sub):
+    """Inside a closed box of single-sided emitters (radiance Le, albedo a),
+    every direction sees Le, so with one bounce the pixel is Le (1 + rho(wo))
+    exactly (Le rho(wo) when the viewed wall does not emit), rho from the BSDF
+    quadrature.  The estimate runs through light selection (tree or uniform),
+    sqrt(r1) triangle sampling, the area to solid angle pdf with the
+    light-side cosine, and the power-heuristic MIS of NEE against BSDF samples
+    that hit an emitter (the glossy cases give the BSDF side most of the
+    weight near the lobe): a wrong area pdf, a missing cosine or a dropped
+    MIS weight moves the mean by far more than the bound."""
+    from render_cases import emissive_box
+    le, a = np.array([1.0, 0.7, 0.4]), 0.6
+    sc = emissive_box(le.tolist(), a, roughness, metallic, n_sub=n_sub, viewed_wall_emits=viewed_wall_emits)
+    bias, _ = W.ray_bias(sc["scene_radius"])
+    imgs, _ = cpu_frames(sc, 24, 16, float(bias), max_bounces=1, light_tree=light_tree)
+    expect = le * (float(viewed_wall_emits) + directional_albedo_at_normal(a, roughness, metallic))
+    mean, bound = _mean_and_bound(imgs, expect)
+    assert (np.abs(mean - expect) <= bound).all(), (mean, expect, bound)
+
+
+def test_emissive_enclosure_multi_bounce_within_albedo_bounds():
+    """With k bounces the pixel lies between Le sum_i rho_min^i and
+    Le sum_i rho_max^i (i = 0..k), rho_min/max the extremes of the directional
+    albedo over view angles; the k = 3 mean must also exceed the k = 2 mean
+    by at least Le rho_min^3."""
+    from render_cases import base_bsdf, emissive_box
+    le, a = np.array([1.0, 0.7, 0.4]), 0.6
+    n = np.array([0.0, 0.0, 1.0])
+    rhos = []
+    for th in np.linspace(0.0, 1.55, 32):
+        wo = np.array([np.sin(th), 0.0, np.cos(th)])
+        t = (np.arange(200) + 0.5) / 200 * (np.pi / 2)
+        p = (np.arange(400) + 0.5) / 400 * (2 * np.pi)
+        T, P = np.meshgrid(t, p, indexing="ij")
+        wi = np.stack([np.sin(T) * np.cos(P), np.sin(T) * np.sin(P), np.cos(T)], axis=-1)
+        f = base_bsdf(a, 1.0, 0.0, n, wo, wi)
+        rhos.append(float(np.sum(f * np.cos(T) * np.sin(T)) * (np.pi / 2 / 200) * (2 * np.pi / 400)))
+    lo, hi = min(rhos), max(rhos)
+    sc = emissive_box(le.tolist(), a)
+    bias, _ = W.ray_bias(sc["scene_radius"])
+    means = {}
+    for k in (2, 3):
+        imgs, _ = cpu_frames(sc, 24, 16, float(bias), max_bounces=k, light_tree=False)
+        mean, bound = _mean_and_bound(imgs, le, rel=0.0)
+        means[k] = mean
+        assert (mean >= le * sum(lo ** i for i in range(k + 1)) - bound).all(), (k, mean)
+        assert (mean <= le * sum(hi ** i for i in range(k + 1)) + bound).all(), (k, mean)
+    assert (means[3] - means[2] >= le * lo ** 3 * 0.9).all()
+
+
+@pytest.mark.parametrize("roughness,metallic,light_tree", [
+    (1.0, 0.0, True), (1.0, 0.0, False), (0.4, 0.0, True)])
+def test_disc_light_over_plane_matches_quadrature(roughness, metallic, light_tree):
+    """A small emitting disc (48-gon) 1 unit above a floor, the floor seen
+    30 degrees off its normal: one bounce of NEE + BSDF-hit MIS converges to
+    the area quadrature of f Le cos cos / d^2 over the disc's triangles (the
+    quadrature is pinned to Lambert's polygon formula)."""
+    from render_cases import disc_expected_radiance, disc_over_plane
+    le, a = np.array([30.0, 20.0, 10.0]), 0.5
+    sc = disc_over_plane(le.tolist(), a, roughness, metallic)
+    bias, _ = W.ray_bias(sc["scene_radius"])
+    imgs, _ = cpu_frames(sc, 24, 24, float(bias), max_bounces=1, light_tree=light_tree)
+    expect = disc_expected_radiance(sc, le, a, roughness, metallic)
+    mean, bound = _mean_and_bound(imgs, expect)
+    assert (np.abs(mean - expect) <= bound).all(), (mean, expect, bound)

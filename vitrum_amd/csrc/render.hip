@@ -111,6 +111,18 @@ __device__ __forceinline__ uint32_t rng_next(Rng& r) {
   return v;
 }
 __device__ __forceinline__ float rand01(Rng& r) { return (float)(rng_next(r) >> 8) / 16777216.0f; }
+// Polar disc coordinates sqrt(u1)·cos(2π·u2), sqrt(u1)·sin(2π·u2), evaluated
+// in binary64 and rounded once to binary32.  A phase rounded to binary32
+// first (2π·u2 near 6 carries up to 2.4e-7 rad) and a binary32 cosine would
+// tilt low-roughness GGX samples enough to move the next hit measurably; the
+// binary64 restatement of the integrator rounds the same two values once.
+__device__ __forceinline__ void disc_polar(float u1, float u2, float& a, float& b) {
+  double s, c;
+  sincos(6.283185307179586 * (double)u2, &s, &c);
+  const double r = __builtin_sqrt((double)u1);
+  a = (float)(r * c);
+  b = (float)(r * s);
+}
 __device__ float represented(float p) {
   if (!(p > 0.0f)) return 0.0f;
   if (p >= 1.0f) return 1.0f;
@@ -410,9 +422,8 @@ __device__ F3 vndf_sample(F3 wo, float alpha, Rng& rng) {
   const F3 t1 = lensq > 1e-10f ? f3(-vh.y * inv, vh.x * inv, 0.0f) : f3(1, 0, 0);
   const F3 t2 = cross(vh, t1);
   const float u1 = rand01(rng), u2 = rand01(rng);
-  const float r = __builtin_sqrtf(u1), phi = 2.0f * kPi * u2;
-  const float a = r * cosf(phi);
-  float b = r * sinf(phi);
+  float a, b;
+  disc_polar(u1, u2, a, b);
   const float s = 0.5f * (1.0f + vh.z);
   b = (1.0f - s) * __builtin_sqrtf(fmaxf(0.0f, 1.0f - a * a)) + s * b;
   const F3 nh = a * t1 + b * t2 + __builtin_sqrtf(fmaxf(0.0f, 1.0f - a * a - b * b)) * vh;
@@ -458,6 +469,22 @@ __device__ float env_pdf_dir(const ShadeScene& sc, F3 d) {
   const uint32_t x = min((uint32_t)floorf(u * (float)sc.env_w), sc.env_w - 1u);
   const uint32_t y = min((uint32_t)floorf(v * (float)sc.env_h), sc.env_h - 1u);
   return sc.env_pdf[y * sc.env_w + x];
+}
+
+// A direction inside environment texel (x, y), uniform in phi and cos(theta)
+// (environmentPacking.ts texel layout), evaluated in binary64 and rounded once
+// per component, as the binary64 restatement rounds it.
+__device__ F3 env_cell_dir(uint32_t x, uint32_t y, uint32_t w, uint32_t h, float xa, float xb) {
+  const double kTwoPi = 6.283185307179586, kPiD = 3.141592653589793;
+  const double phi0 = ((double)x / w - 0.5) * kTwoPi, phi1 = ((double)(x + 1u) / w - 0.5) * kTwoPi;
+  const double th0 = (double)y / h * kPiD, th1 = (double)(y + 1u) / h * kPiD;
+  const double phi = phi0 + (phi1 - phi0) * (double)xa;
+  const double c0 = cos(th0), c1 = cos(th1);
+  const double ct = c0 + (c1 - c0) * (double)xb;
+  const double st = __builtin_sqrt(fmax(0.0, 1.0 - ct * ct));
+  double sp, cp;
+  sincos(phi, &sp, &cp);
+  return f3((float)(cp * st), (float)ct, (float)(sp * st));
 }
 
 // ---- wave queue ------------------------------------------------------------
@@ -773,14 +800,7 @@ __global__ __launch_bounds__(256) void shade_kernel(FrameParams f, ShadeScene sc
             const uint32_t idx = min(lo, count - 1u);
             const float xa = rand01(rng), xb = rand01(rng);
             const uint32_t ex = idx % sc.env_w, ey = idx / sc.env_w;
-            const float iw = 1.0f / (float)sc.env_w, ih = 1.0f / (float)sc.env_h;
-            const float phi0 = ((float)ex * iw - 0.5f) * (2.0f * kPi), phi1 = ((float)(ex + 1u) * iw - 0.5f) * (2.0f * kPi);
-            const float th0 = (float)ey * ih * kPi, th1 = (float)(ey + 1u) * ih * kPi;
-            const float phi = phi0 + (phi1 - phi0) * xa;
-            const float c0 = cosf(th0), c1 = cosf(th1);
-            const float ct = c0 + (c1 - c0) * xb;
-            const float st = __builtin_sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
-            const F3 wi = safe_normalize(f3(cosf(phi) * st, ct, sinf(phi) * st));
+            const F3 wi = safe_normalize(env_cell_dir(ex, ey, sc.env_w, sc.env_h, xa, xb));
             const float epdf = sc.env_pdf[idx];
             const float ndl = fabsf(dot(nrm, wi));
             if (ndl > 0.0f) {
@@ -814,9 +834,10 @@ __global__ __launch_bounds__(256) void shade_kernel(FrameParams f, ShadeScene sc
           wi = safe_normalize(inc - hw * (2.0f * dot(hw, inc)));
         } else {
           const float u1 = rand01(rng), u2 = rand01(rng);
-          const float r = __builtin_sqrtf(u1), phi = 2.0f * kPi * u2;
+          float a, b;
+          disc_polar(u1, u2, a, b);
           const float ct = __builtin_sqrtf(fmaxf(0.0f, 1.0f - u1));
-          wi = safe_normalize((r * cosf(phi)) * tb + (r * sinf(phi)) * bb + ct * nrm);
+          wi = safe_normalize(a * tb + b * bb + ct * nrm);
         }
         sig = mix64(sig, glossy ? 0xB1ull : 0xB0ull);
         const float mpdf = bsdf_pdf(m, nrm, wo, wi);
