@@ -143,10 +143,31 @@ vt_status vt_destroy(vt_engine *eng);
 vt_status vt_last_error(const vt_engine *eng, char *buf, size_t len);
 
 /* Engine.setScene (packages/core/src/engine/index.ts:120;
- * packages/pt-webgpu/src/index.ts:1120).  Packs and builds on the host (BLAS
- * per primitive, TLAS over instances), converts to the device layout and
- * uploads.  Transactional: on failure the previous scene stays bound. */
+ * packages/pt-webgpu/src/index.ts:1120; atomic rebuild at :1229-1260).
+ * Validates (material indices, Le = emissive x emissiveIntensity finite and
+ * >= 0 in f32), packs and builds on the host (BLAS per primitive, TLAS over
+ * instances), converts to the device layout, and builds and uploads the
+ * traversal buffers and the shading data (materials, emitters, environment
+ * table, light tree) into fresh allocations.  Only when every step has
+ * succeeded is the new scene bound and accumulation reset.  Transactional:
+ * on any failure -- a validation or build error, an allocation failure on
+ * the host (VT_ERR_OUT_OF_MEMORY) or the device -- the previous scene stays
+ * bound and renders as before. */
 vt_status vt_set_scene(vt_engine *eng, const vt_scene_desc *scene);
+
+/* Host threads of the scene builder (no counterpart in the reference, whose
+ * builder is single-threaded JavaScript): 0 (default) = every hardware
+ * thread; a host that shares its CPUs passes its own share. */
+vt_status vt_set_build_threads(vt_engine *eng, uint32_t threads);
+
+/* Test hook (no counterpart in the reference): each bit of `flags` makes the
+ * next matching step fail once -- 1: scene packing throws std::bad_alloc
+ * (vt_set_scene on `eng`; vt_pack_scene when `eng` is NULL), 2: the
+ * shading-data upload of vt_set_scene reports VT_ERR_OUT_OF_MEMORY, 4: its
+ * preparation throws std::runtime_error.  Every entry point catches
+ * exceptions and returns VT_ERR_OUT_OF_MEMORY (std::bad_alloc) or
+ * VT_ERR_BUILD (any other), so none unwinds into the caller. */
+vt_status vt_debug_inject_failure(vt_engine *eng, uint32_t flags);
 
 /* ---- frames ---------------------------------------------------------------- */
 
@@ -170,7 +191,8 @@ typedef struct vt_frame {
 vt_status vt_render_frame(vt_engine *eng, const vt_frame *frame);
 /* captureFrame (packages/core/src/engine/index.ts:466; pt-webgpu
  * index.ts:2019): Float32 RGBA running mean, row-major, top-left origin,
- * alpha 1.  Blocking. */
+ * alpha 1.  Blocking.  If a traversal ran out of stack since the last check,
+ * returns VT_ERR_HIP once, clears the condition and leaves `rgba` unwritten. */
 vt_status vt_capture(vt_engine *eng, float *rgba, uint32_t width, uint32_t height);
 /* Engine.reset (packages/core/src/engine/index.ts:256): clears accumulation. */
 vt_status vt_reset(vt_engine *eng);

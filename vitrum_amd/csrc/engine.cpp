@@ -5,6 +5,7 @@
 #include <atomic>
 #include <cstdarg>
 #include <cstdio>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -28,6 +29,8 @@ vt_status set_err(vt_engine* e, vt_status code, const std::string& msg) {
     g_global_error = msg;
   return code;
 }
+
+thread_local uint32_t g_fail_inject = 0;  // kInject* bits for calls without an engine
 
 #define HIP_TRY(eng, expr)                                                               \
   do {                                                                                   \
@@ -62,6 +65,13 @@ vt_status check_prims(vt_engine* e, const vt_primitive* prims, uint32_t n) {
 }
 
 }  // namespace
+
+vt_status engine_error(vt_engine* e, vt_status code, const std::string& msg) { return set_err(e, code, msg); }
+
+uint32_t engine_build_threads(const vt_engine* e) {
+  const uint32_t hw = std::max(1u, std::thread::hardware_concurrency());
+  return e && e->build_threads ? std::min(e->build_threads, hw) : hw;
+}
 
 // ---------------------------------------------------------------------------
 // Device buffer helpers
@@ -167,8 +177,7 @@ vt_status engine_build_trace_scene(vt_engine* eng, const vt_scene_desc* scene, S
   std::vector<vt::PrimitiveIn> prims(scene->primitive_count);
   for (uint32_t i = 0; i < scene->primitive_count; ++i) prims[i] = to_prim(scene->primitives[i]);
   vt::Pack pack;
-  const uint32_t threads = std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
-  vt::BuildError be = vt::pack_scene(prims.data(), (uint32_t)prims.size(), threads, pack);
+  vt::BuildError be = vt::pack_scene(prims.data(), (uint32_t)prims.size(), engine_build_threads(eng), pack);
   if (be) return set_err(eng, be.code, be.message);
   std::string le = vt::build_device_layout(pack, layout);
   if (!le.empty()) return set_err(eng, VT_ERR_RANGE, le);
@@ -185,9 +194,11 @@ vt_status engine_build_trace_scene(vt_engine* eng, const vt_scene_desc* scene, S
 extern "C" {
 
 vt_status vt_invert_mat4(const float m[16], float out[16]) {
-  if (!m || !out) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "NULL argument");
-  if (!vt::invert_mat4(m, out)) return set_err(nullptr, VT_ERR_RANGE, "matrix is not invertible");
-  return VT_OK;
+  return vt_guarded(nullptr, [&]() -> vt_status {
+    if (!m || !out) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (!vt::invert_mat4(m, out)) return set_err(nullptr, VT_ERR_RANGE, "matrix is not invertible");
+    return VT_OK;
+  });
 }
 
 const char* vt_version(void) { return "vitrum-hip 0.1.0 gfx950"; }
@@ -200,197 +211,264 @@ vt_status vt_last_error(const vt_engine* eng, char* buf, size_t len) {
 }
 
 vt_status vt_create(int32_t device_ordinal, const vt_options* opts, vt_engine** out) {
-  if (!out) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "out is NULL");
-  *out = nullptr;
-  int count = 0;
-  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
-    return set_err(nullptr, VT_ERR_NO_DEVICE, "no HIP device is available");
-  if (device_ordinal < 0 || device_ordinal >= count)
-    return set_err(nullptr, VT_ERR_RANGE, "device ordinal out of range");
-  if (opts && opts->sampling > 1)
-    return set_err(nullptr, VT_ERR_RANGE, "sampling must be 0 (pcg) or 1 (sobol)");
-  if (opts && opts->light_selection > 1)
-    return set_err(nullptr, VT_ERR_RANGE, "light_selection must be 0 (light tree) or 1 (uniform)");
-  if (opts && opts->bvh_traversal > 1)
-    return set_err(nullptr, VT_ERR_RANGE, "bvh_traversal must be 0 (binary) or 1 (cwbvh-closest)");
-  std::unique_ptr<vt_engine> e(new vt_engine());
-  e->device = device_ordinal;
-  if (opts) e->opts = *opts;
-  if (e->opts.max_bounces == 0) e->opts.max_bounces = 8;
-  HIP_TRY(nullptr, hipSetDevice(device_ordinal));
-  HIP_TRY(nullptr, hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
-  HIP_TRY(nullptr, hipEventCreate(&e->ev_start));
-  HIP_TRY(nullptr, hipEventCreate(&e->ev_stop));
-  hipDeviceProp_t prop;
-  HIP_TRY(nullptr, hipGetDeviceProperties(&prop, device_ordinal));
-  e->cu_count = (uint32_t)prop.multiProcessorCount;
-  int per_cu = 0;
-  HIP_TRY(nullptr, vtk::trace_occupancy(false, &per_cu));
-  e->trace_blocks_per_cu = std::max(1, per_cu);
-  e->trace_grid = e->cu_count * e->trace_blocks_per_cu;
-  vt_status s = e->counter.ensure(e.get(), 64);
-  if (!s) s = e->error_word.ensure(e.get(), 64);
-  const size_t spill = (size_t)(vtk::kStackCapacity - vtk::kLdsDepth) * e->trace_grid *
-                       vtk::kTraceBlock * sizeof(uint2);
-  if (!s) s = e->spill.ensure(e.get(), spill);
-  if (s) return set_err(nullptr, s, e->last_error);
-  HIP_TRY(nullptr, vtk::launch_zero((uint32_t*)e->error_word.ptr, 16, e->stream));
-  HIP_TRY(nullptr, hipStreamSynchronize(e->stream));
-  *out = e.release();
-  return VT_OK;
+  return vt_guarded(nullptr, [&]() -> vt_status {
+    if (!out) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+      return set_err(nullptr, VT_ERR_NO_DEVICE, "no HIP device is available");
+    if (device_ordinal < 0 || device_ordinal >= count)
+      return set_err(nullptr, VT_ERR_RANGE, "device ordinal out of range");
+    if (opts && opts->sampling > 1)
+      return set_err(nullptr, VT_ERR_RANGE, "sampling must be 0 (pcg) or 1 (sobol)");
+    if (opts && opts->light_selection > 1)
+      return set_err(nullptr, VT_ERR_RANGE, "light_selection must be 0 (light tree) or 1 (uniform)");
+    if (opts && opts->bvh_traversal > 1)
+      return set_err(nullptr, VT_ERR_RANGE, "bvh_traversal must be 0 (binary) or 1 (cwbvh-closest)");
+    std::unique_ptr<vt_engine> e(new vt_engine());
+    e->device = device_ordinal;
+    if (opts) e->opts = *opts;
+    if (e->opts.max_bounces == 0) e->opts.max_bounces = 8;
+    HIP_TRY(nullptr, hipSetDevice(device_ordinal));
+    HIP_TRY(nullptr, hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    HIP_TRY(nullptr, hipEventCreate(&e->ev_start));
+    HIP_TRY(nullptr, hipEventCreate(&e->ev_stop));
+    hipDeviceProp_t prop;
+    HIP_TRY(nullptr, hipGetDeviceProperties(&prop, device_ordinal));
+    e->cu_count = (uint32_t)prop.multiProcessorCount;
+    int per_cu = 0;
+    HIP_TRY(nullptr, vtk::trace_occupancy(false, &per_cu));
+    e->trace_blocks_per_cu = std::max(1, per_cu);
+    e->trace_grid = e->cu_count * e->trace_blocks_per_cu;
+    vt_status s = e->counter.ensure(e.get(), 64);
+    if (!s) s = e->error_word.ensure(e.get(), 64);
+    const size_t spill = (size_t)(vtk::kStackCapacity - vtk::kLdsDepth) * e->trace_grid *
+                         vtk::kTraceBlock * sizeof(uint2);
+    if (!s) s = e->spill.ensure(e.get(), spill);
+    if (s) return set_err(nullptr, s, e->last_error);
+    HIP_TRY(nullptr, vtk::launch_zero((uint32_t*)e->error_word.ptr, 16, e->stream));
+    HIP_TRY(nullptr, hipStreamSynchronize(e->stream));
+    *out = e.release();
+    return VT_OK;
+  });
 }
 
 vt_status vt_destroy(vt_engine* eng) {
-  if (!eng) return VT_OK;
-  (void)hipSetDevice(eng->device);
-  (void)hipStreamSynchronize(eng->stream);
-  eng->scene.release();
-  eng->counter.release();
-  eng->error_word.release();
-  eng->spill.release();
-  eng->tmp_rays.release();
-  eng->tmp_out.release();
-  eng->tmp_counts.release();
-  engine_release_cwbvh(eng);
-  for (auto& kv : eng->user_allocs) (void)hipFree(kv);
-  engine_release_render(eng);
-  engine_release_nrc(eng);
-  (void)hipEventDestroy(eng->ev_start);
-  (void)hipEventDestroy(eng->ev_stop);
-  (void)hipStreamDestroy(eng->stream);
-  delete eng;
-  return VT_OK;
+  return vt_guarded(eng, [&]() -> vt_status {
+    if (!eng) return VT_OK;
+    (void)hipSetDevice(eng->device);
+    (void)hipStreamSynchronize(eng->stream);
+    eng->scene.release();
+    eng->counter.release();
+    eng->error_word.release();
+    eng->spill.release();
+    eng->tmp_rays.release();
+    eng->tmp_out.release();
+    eng->tmp_counts.release();
+    engine_release_cwbvh(eng);
+    for (auto& kv : eng->user_allocs) (void)hipFree(kv);
+    engine_release_render(eng);
+    engine_release_nrc(eng);
+    (void)hipEventDestroy(eng->ev_start);
+    (void)hipEventDestroy(eng->ev_stop);
+    (void)hipStreamDestroy(eng->stream);
+    delete eng;
+    return VT_OK;
+  });
 }
 
 vt_status vt_set_scene(vt_engine* eng, const vt_scene_desc* scene) {
+  return vt_guarded(eng, [&]() -> vt_status {
+    if (!eng) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "engine is NULL");
+    if (!scene) return set_err(eng, VT_ERR_INVALID_ARGUMENT, "scene is NULL");
+    if (vt_status s = check_prims(eng, scene->primitives, scene->primitive_count)) return s;
+    if (scene->material_count && !scene->materials)
+      return set_err(eng, VT_ERR_INVALID_ARGUMENT, "materials is NULL");
+    for (uint32_t i = 0; i < scene->primitive_count; ++i)
+      if (scene->primitives[i].material >= std::max<uint32_t>(1, scene->material_count) &&
+          scene->material_count > 0)
+        return set_err(eng, VT_ERR_RANGE, "primitive material index out of range");
+    for (uint32_t i = 0; i < scene->material_count; ++i) {
+      // Le = emissive x emissiveIntensity, rounded once to f32, must be finite
+      // and non-negative (an f32 overflow from finite JavaScript numbers would
+      // otherwise reach the light tree as an infinite power).
+      const vt_material& m = scene->materials[i];
+      for (int k = 0; k < 3; ++k) {
+        const float le = (float)((double)m.emissive[k] * (double)m.emissive_intensity);
+        if (!std::isfinite(le) || le < 0.0f)
+          return set_err(eng, VT_ERR_RANGE, "material " + std::to_string(i) +
+                                                ": emissive x emissiveIntensity must be finite and >= 0 in f32");
+      }
+    }
+    if (eng->fail_inject & kInjectPackThrow) {
+      eng->fail_inject &= ~kInjectPackThrow;
+      throw std::bad_alloc();
+    }
+    std::vector<vt::PrimitiveIn> prims(scene->primitive_count);
+    for (uint32_t i = 0; i < scene->primitive_count; ++i) prims[i] = to_prim(scene->primitives[i]);
+    auto pack = std::make_unique<vt::Pack>();
+    vt::BuildError be = vt::pack_scene(prims.data(), (uint32_t)prims.size(), engine_build_threads(eng), *pack);
+    if (be) return set_err(eng, be.code, be.message);
+    auto layout = std::make_unique<vt::DeviceLayout>();
+    std::string le = vt::build_device_layout(*pack, *layout);
+    if (!le.empty()) return set_err(eng, VT_ERR_RANGE, le);
+    // Single-sided triangles (materialAcceptsSidedHit, material.wgsl.ts:
+    // 3259-3263): opaque materials without doubleSided (the default) have no
+    // back-face surfaces for the path tracer's closest-hit and visibility rays.
+    for (uint32_t t = 0; t < layout->tri_count; ++t) {
+      const uint32_t mid = pack->tri_material[t];
+      const bool single = mid < scene->material_count ? scene->materials[mid].double_sided == 0 : true;
+      layout->tris[(size_t)t * vt::kTriWords + 9] = single ? 1.0f : 0.0f;
+    }
+    std::vector<vt_material> materials(scene->materials, scene->materials + scene->material_count);
+    HIP_TRY(eng, hipSetDevice(eng->device));
+    // Build and upload everything into fresh objects -- traversal buffers,
+    // then the shading data -- and bind them only when all of it succeeded:
+    // any failure leaves the previous scene bound and usable.
+    SceneBuffers nb;
+    RenderSceneData* rd = nullptr;
+    struct Unbound {  // frees what was not bound, also when an exception unwinds
+      SceneBuffers& nb;
+      RenderSceneData*& rd;
+      ~Unbound() {
+        nb.release();
+        engine_discard_render(rd);
+      }
+    } unbound{nb, rd};
+    vt_status s = engine_upload_trace(eng, *layout, nb);
+    s = s ? s : engine_upload(eng, nb.positions, pack->positions.data(), pack->positions.size() * 4);
+    s = s ? s : engine_upload(eng, nb.normals, pack->normals.data(), pack->normals.size() * 4);
+    s = s ? s : engine_upload(eng, nb.indices, pack->indices.data(), pack->indices.size() * 4);
+    s = s ? s : engine_upload(eng, nb.tri_material, pack->tri_material.data(), pack->tri_material.size() * 4);
+    s = s ? s : engine_prepare_render(eng, *pack, materials, scene->environment, scene->scene_radius, &rd);
+    if (!s) {
+      hipError_t r = hipStreamSynchronize(eng->stream);
+      if (r != hipSuccess) s = set_err(eng, VT_ERR_HIP, hipGetErrorString(r));
+    }
+    if (s) return s;
+    // Commit: nothing below can fail.
+    engine_release_cwbvh(eng);
+    eng->scene.release();
+    eng->scene = nb;
+    nb = SceneBuffers();
+    engine_commit_render(eng, rd);
+    rd = nullptr;
+    eng->layout = std::move(layout);
+    eng->pack = std::move(pack);
+    eng->scene_desc_env = scene->environment;
+    eng->materials = std::move(materials);
+    eng->scene_radius = scene->scene_radius;
+    eng->has_scene = true;
+    eng->samples_accumulated = 0;
+    return VT_OK;
+  });
+}
+
+vt_status vt_set_build_threads(vt_engine* eng, uint32_t threads) {
   if (!eng) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "engine is NULL");
-  if (!scene) return set_err(eng, VT_ERR_INVALID_ARGUMENT, "scene is NULL");
-  if (vt_status s = check_prims(eng, scene->primitives, scene->primitive_count)) return s;
-  for (uint32_t i = 0; i < scene->primitive_count; ++i)
-    if (scene->primitives[i].material >= std::max<uint32_t>(1, scene->material_count) &&
-        scene->material_count > 0)
-      return set_err(eng, VT_ERR_RANGE, "primitive material index out of range");
-  std::vector<vt::PrimitiveIn> prims(scene->primitive_count);
-  for (uint32_t i = 0; i < scene->primitive_count; ++i) prims[i] = to_prim(scene->primitives[i]);
-  auto pack = std::make_unique<vt::Pack>();
-  const uint32_t threads = std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
-  vt::BuildError be = vt::pack_scene(prims.data(), (uint32_t)prims.size(), threads, *pack);
-  if (be) return set_err(eng, be.code, be.message);
-  auto layout = std::make_unique<vt::DeviceLayout>();
-  std::string le = vt::build_device_layout(*pack, *layout);
-  if (!le.empty()) return set_err(eng, VT_ERR_RANGE, le);
-  // Single-sided triangles (materialAcceptsSidedHit, material.wgsl.ts:
-  // 3259-3263): opaque materials without doubleSided (the default) have no
-  // back-face surfaces for the path tracer's closest-hit and visibility rays.
-  for (uint32_t t = 0; t < layout->tri_count; ++t) {
-    const uint32_t mid = pack->tri_material[t];
-    const bool single = mid < scene->material_count ? scene->materials[mid].double_sided == 0 : true;
-    layout->tris[(size_t)t * vt::kTriWords + 9] = single ? 1.0f : 0.0f;
-  }
-  HIP_TRY(eng, hipSetDevice(eng->device));
-  // Upload into fresh buffers, then swap: a failure leaves the old scene bound.
-  SceneBuffers nb;
-  vt_status s = engine_upload_trace(eng, *layout, nb);
-  s = s ? s : engine_upload(eng, nb.positions, pack->positions.data(), pack->positions.size() * 4);
-  s = s ? s : engine_upload(eng, nb.normals, pack->normals.data(), pack->normals.size() * 4);
-  s = s ? s : engine_upload(eng, nb.indices, pack->indices.data(), pack->indices.size() * 4);
-  s = s ? s : engine_upload(eng, nb.tri_material, pack->tri_material.data(), pack->tri_material.size() * 4);
-  if (!s) {
-    hipError_t r = hipStreamSynchronize(eng->stream);
-    if (r != hipSuccess) s = set_err(eng, VT_ERR_HIP, hipGetErrorString(r));
-  }
-  if (s) {
-    nb.release();
-    return s;
-  }
-  engine_release_cwbvh(eng);
-  eng->scene.release();
-  eng->scene = nb;
-  nb = SceneBuffers();
-  eng->layout = std::move(layout);
-  eng->pack = std::move(pack);
-  eng->scene_desc_env = scene->environment;
-  eng->materials.assign(scene->materials, scene->materials + scene->material_count);
-  eng->scene_radius = scene->scene_radius;
-  eng->has_scene = true;
-  return engine_on_scene_changed(eng);
+  eng->build_threads = threads;
+  return VT_OK;
+}
+
+vt_status vt_debug_inject_failure(vt_engine* eng, uint32_t flags) {
+  if (eng)
+    eng->fail_inject = flags;
+  else
+    g_fail_inject = flags;
+  return VT_OK;
 }
 
 vt_status vt_device_alloc(vt_engine* eng, size_t bytes, void** d_ptr) {
-  if (!eng || !d_ptr) return set_err(eng, VT_ERR_INVALID_ARGUMENT, "NULL argument");
-  HIP_TRY(eng, hipSetDevice(eng->device));
-  HIP_TRY(eng, hipMalloc(d_ptr, std::max<size_t>(bytes, 16)));
-  eng->user_allocs.push_back(*d_ptr);
-  return VT_OK;
+  return vt_guarded(eng, [&]() -> vt_status {
+    if (!eng || !d_ptr) return set_err(eng, VT_ERR_INVALID_ARGUMENT, "NULL argument");
+    HIP_TRY(eng, hipSetDevice(eng->device));
+    HIP_TRY(eng, hipMalloc(d_ptr, std::max<size_t>(bytes, 16)));
+    eng->user_allocs.push_back(*d_ptr);
+    return VT_OK;
+  });
 }
 
 vt_status vt_device_free(vt_engine* eng, void* d_ptr) {
-  if (!eng) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "engine is NULL");
-  auto it = std::find(eng->user_allocs.begin(), eng->user_allocs.end(), d_ptr);
-  if (it == eng->user_allocs.end())
-    return set_err(eng, VT_ERR_INVALID_ARGUMENT, "pointer was not allocated by vt_device_alloc");
-  eng->user_allocs.erase(it);
-  HIP_TRY(eng, hipStreamSynchronize(eng->stream));
-  HIP_TRY(eng, hipFree(d_ptr));
-  return VT_OK;
+  return vt_guarded(eng, [&]() -> vt_status {
+    if (!eng) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "engine is NULL");
+    auto it = std::find(eng->user_allocs.begin(), eng->user_allocs.end(), d_ptr);
+    if (it == eng->user_allocs.end())
+      return set_err(eng, VT_ERR_INVALID_ARGUMENT, "pointer was not allocated by vt_device_alloc");
+    eng->user_allocs.erase(it);
+    HIP_TRY(eng, hipStreamSynchronize(eng->stream));
+    HIP_TRY(eng, hipFree(d_ptr));
+    return VT_OK;
+  });
 }
 
 vt_status vt_memcpy_h2d(vt_engine* eng, void* d_dst, const void* src, size_t bytes) {
-  if (!eng || (!src && bytes) || (!d_dst && bytes))
-    return set_err(eng, VT_ERR_INVALID_ARGUMENT, "NULL argument");
-  HIP_TRY(eng, hipMemcpyAsync(d_dst, src, bytes, hipMemcpyHostToDevice, eng->stream));
-  HIP_TRY(eng, hipStreamSynchronize(eng->stream));
-  return VT_OK;
+  return vt_guarded(eng, [&]() -> vt_status {
+    if (!eng || (!src && bytes) || (!d_dst && bytes))
+      return set_err(eng, VT_ERR_INVALID_ARGUMENT, "NULL argument");
+    HIP_TRY(eng, hipMemcpyAsync(d_dst, src, bytes, hipMemcpyHostToDevice, eng->stream));
+    HIP_TRY(eng, hipStreamSynchronize(eng->stream));
+    return VT_OK;
+  });
 }
 
 vt_status vt_memcpy_d2h(vt_engine* eng, void* dst, const void* d_src, size_t bytes) {
-  if (!eng || (!dst && bytes) || (!d_src && bytes))
-    return set_err(eng, VT_ERR_INVALID_ARGUMENT, "NULL argument");
-  HIP_TRY(eng, hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, eng->stream));
-  HIP_TRY(eng, hipStreamSynchronize(eng->stream));
-  return VT_OK;
+  return vt_guarded(eng, [&]() -> vt_status {
+    if (!eng || (!dst && bytes) || (!d_src && bytes))
+      return set_err(eng, VT_ERR_INVALID_ARGUMENT, "NULL argument");
+    HIP_TRY(eng, hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, eng->stream));
+    HIP_TRY(eng, hipStreamSynchronize(eng->stream));
+    return VT_OK;
+  });
 }
 
 vt_status vt_memcpy_d2d(vt_engine* eng, void* d_dst, const void* d_src, size_t bytes) {
-  if (!eng || (!d_dst && bytes) || (!d_src && bytes))
-    return set_err(eng, VT_ERR_INVALID_ARGUMENT, "NULL argument");
-  if ((bytes | (uintptr_t)d_dst | (uintptr_t)d_src) & 15u)
-    return set_err(eng, VT_ERR_INVALID_ARGUMENT, "pointers and size must be multiples of 16 bytes");
-  HIP_TRY(eng, hipSetDevice(eng->device));
-  HIP_TRY(eng, vtk::launch_copy16((uint4*)d_dst, (const uint4*)d_src, bytes / 16, eng->stream));
-  HIP_TRY(eng, hipStreamSynchronize(eng->stream));
-  return VT_OK;
+  return vt_guarded(eng, [&]() -> vt_status {
+    if (!eng || (!d_dst && bytes) || (!d_src && bytes))
+      return set_err(eng, VT_ERR_INVALID_ARGUMENT, "NULL argument");
+    if ((bytes | (uintptr_t)d_dst | (uintptr_t)d_src) & 15u)
+      return set_err(eng, VT_ERR_INVALID_ARGUMENT, "pointers and size must be multiples of 16 bytes");
+    HIP_TRY(eng, hipSetDevice(eng->device));
+    HIP_TRY(eng, vtk::launch_copy16((uint4*)d_dst, (const uint4*)d_src, bytes / 16, eng->stream));
+    HIP_TRY(eng, hipStreamSynchronize(eng->stream));
+    return VT_OK;
+  });
 }
 
 vt_status vt_synchronize(vt_engine* eng) {
-  if (!eng) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "engine is NULL");
-  if (vt_status s = engine_nrc_join(eng)) return s;
-  HIP_TRY(eng, hipStreamSynchronize(eng->stream));
-  return VT_OK;
+  return vt_guarded(eng, [&]() -> vt_status {
+    if (!eng) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "engine is NULL");
+    if (vt_status s = engine_nrc_join(eng)) return s;
+    HIP_TRY(eng, hipStreamSynchronize(eng->stream));
+    return VT_OK;
+  });
 }
 
 vt_status vt_get_stream(vt_engine* eng, void** stream) {
-  if (!eng || !stream) return set_err(eng, VT_ERR_INVALID_ARGUMENT, "NULL argument");
-  *stream = (void*)eng->stream;
-  return VT_OK;
+  return vt_guarded(eng, [&]() -> vt_status {
+    if (!eng || !stream) return set_err(eng, VT_ERR_INVALID_ARGUMENT, "NULL argument");
+    *stream = (void*)eng->stream;
+    return VT_OK;
+  });
 }
 
 vt_status vt_stats(vt_engine* eng, vt_frame_stats* out) {
-  if (!eng || !out) return set_err(eng, VT_ERR_INVALID_ARGUMENT, "NULL argument");
-  *out = vt_frame_stats{};
-  if (eng->trace_timed) {
-    HIP_TRY(eng, hipEventSynchronize(eng->ev_stop));
-    float ms = 0;
-    HIP_TRY(eng, hipEventElapsedTime(&ms, eng->ev_start, eng->ev_stop));
-    out->last_trace_ms = ms;
-  }
-  out->last_frame_ms = eng->last_frame_ms;
-  out->samples_accumulated = eng->samples_accumulated;
-  out->last_ray_count = eng->last_ray_count;
-  out->device_bytes = eng->scene.blas.bytes + eng->scene.tlas.bytes + eng->scene.tris.bytes +
-                      eng->scene.instances.bytes + eng->scene.positions.bytes +
-                      eng->scene.normals.bytes + eng->scene.indices.bytes;
-  return VT_OK;
+  return vt_guarded(eng, [&]() -> vt_status {
+    if (!eng || !out) return set_err(eng, VT_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = vt_frame_stats{};
+    if (eng->trace_timed) {
+      HIP_TRY(eng, hipEventSynchronize(eng->ev_stop));
+      float ms = 0;
+      HIP_TRY(eng, hipEventElapsedTime(&ms, eng->ev_start, eng->ev_stop));
+      out->last_trace_ms = ms;
+    }
+    out->last_frame_ms = eng->last_frame_ms;
+    out->samples_accumulated = eng->samples_accumulated;
+    out->last_ray_count = eng->last_ray_count;
+    out->device_bytes = eng->scene.blas.bytes + eng->scene.tlas.bytes + eng->scene.tris.bytes +
+                        eng->scene.instances.bytes + eng->scene.positions.bytes +
+                        eng->scene.normals.bytes + eng->scene.indices.bytes;
+    return VT_OK;
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -466,63 +544,79 @@ static vt_status trace_host(vt_engine* eng, const vt_ray* rays, uint32_t n, void
 }
 
 vt_status vt_trace_closest(vt_engine* eng, const vt_ray* rays, uint32_t n, vt_hit* hits) {
-  return trace_host(eng, rays, n, hits, sizeof(vt_hit), false, nullptr);
+  return vt_guarded(eng, [&]() -> vt_status {
+    return trace_host(eng, rays, n, hits, sizeof(vt_hit), false, nullptr);
+  });
 }
 
 vt_status vt_trace_any(vt_engine* eng, const vt_ray* rays, uint32_t n, uint32_t* occluded) {
-  return trace_host(eng, rays, n, occluded, 4, true, nullptr);
+  return vt_guarded(eng, [&]() -> vt_status {
+    return trace_host(eng, rays, n, occluded, 4, true, nullptr);
+  });
 }
 
 vt_status vt_trace_closest_counts(vt_engine* eng, const vt_ray* rays, uint32_t n, vt_hit* hits,
                                   uint32_t* counts) {
-  if (!counts) return set_err(eng, VT_ERR_INVALID_ARGUMENT, "counts is NULL");
-  return trace_host(eng, rays, n, hits, sizeof(vt_hit), false, counts);
+  return vt_guarded(eng, [&]() -> vt_status {
+    if (!counts) return set_err(eng, VT_ERR_INVALID_ARGUMENT, "counts is NULL");
+    return trace_host(eng, rays, n, hits, sizeof(vt_hit), false, counts);
+  });
 }
 
 vt_status vt_set_counting(vt_engine* eng, int32_t enabled) {
-  if (!eng) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "engine is NULL");
-  eng->counting = enabled != 0;
-  return VT_OK;
+  return vt_guarded(eng, [&]() -> vt_status {
+    if (!eng) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "engine is NULL");
+    eng->counting = enabled != 0;
+    return VT_OK;
+  });
 }
 
 vt_status vt_trace_closest_device(vt_engine* eng, const vt_ray* d_rays, uint32_t n, vt_hit* d_hits) {
-  return trace_device(eng, d_rays, n, d_hits, false, nullptr);
+  return vt_guarded(eng, [&]() -> vt_status {
+    return trace_device(eng, d_rays, n, d_hits, false, nullptr);
+  });
 }
 
 vt_status vt_trace_any_device(vt_engine* eng, const vt_ray* d_rays, uint32_t n, uint32_t* d_occluded) {
-  return trace_device(eng, d_rays, n, d_occluded, true, nullptr);
+  return vt_guarded(eng, [&]() -> vt_status {
+    return trace_device(eng, d_rays, n, d_occluded, true, nullptr);
+  });
 }
 
 vt_status vt_set_trace_tuning(vt_engine* eng, uint32_t refill_min, uint32_t steps, uint32_t near_first,
                               uint32_t defer_min) {
-  if (!eng) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "engine is NULL");
-  if (refill_min < 1 || refill_min > 64 || steps < 1 || near_first > 2 || defer_min < 1 || defer_min > 64)
-    return set_err(eng, VT_ERR_RANGE,
-                   "refill_min and defer_min must be in [1, 64], steps >= 1 and near_first 0, 1 or 2");
-  eng->refill_min = refill_min;
-  eng->trace_steps = steps;
-  eng->near_first = near_first;
-  eng->defer_min = defer_min;
-  return VT_OK;
+  return vt_guarded(eng, [&]() -> vt_status {
+    if (!eng) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "engine is NULL");
+    if (refill_min < 1 || refill_min > 64 || steps < 1 || near_first > 2 || defer_min < 1 || defer_min > 64)
+      return set_err(eng, VT_ERR_RANGE,
+                     "refill_min and defer_min must be in [1, 64], steps >= 1 and near_first 0, 1 or 2");
+    eng->refill_min = refill_min;
+    eng->trace_steps = steps;
+    eng->near_first = near_first;
+    eng->defer_min = defer_min;
+    return VT_OK;
+  });
 }
 
 vt_status vt_selftest_fp32(vt_engine* eng, const float* a, const float* b, float* q, float* r,
                            uint32_t n) {
-  if (!eng || (n && (!a || !b || !q || !r)))
-    return set_err(eng, VT_ERR_INVALID_ARGUMENT, "NULL argument");
-  if (n == 0) return VT_OK;
-  HIP_TRY(eng, hipSetDevice(eng->device));
-  DeviceBuffer buf;
-  if (vt_status s = buf.ensure(eng, (size_t)n * 16)) return s;
-  float* d = (float*)buf.ptr;
-  HIP_TRY(eng, hipMemcpyAsync(d, a, (size_t)n * 4, hipMemcpyHostToDevice, eng->stream));
-  HIP_TRY(eng, hipMemcpyAsync(d + n, b, (size_t)n * 4, hipMemcpyHostToDevice, eng->stream));
-  HIP_TRY(eng, vtk::launch_fp32_selftest(d, d + n, d + 2 * (size_t)n, d + 3 * (size_t)n, n, eng->stream));
-  HIP_TRY(eng, hipMemcpyAsync(q, d + 2 * (size_t)n, (size_t)n * 4, hipMemcpyDeviceToHost, eng->stream));
-  HIP_TRY(eng, hipMemcpyAsync(r, d + 3 * (size_t)n, (size_t)n * 4, hipMemcpyDeviceToHost, eng->stream));
-  HIP_TRY(eng, hipStreamSynchronize(eng->stream));
-  buf.release();
-  return VT_OK;
+  return vt_guarded(eng, [&]() -> vt_status {
+    if (!eng || (n && (!a || !b || !q || !r)))
+      return set_err(eng, VT_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (n == 0) return VT_OK;
+    HIP_TRY(eng, hipSetDevice(eng->device));
+    DeviceBuffer buf;
+    if (vt_status s = buf.ensure(eng, (size_t)n * 16)) return s;
+    float* d = (float*)buf.ptr;
+    HIP_TRY(eng, hipMemcpyAsync(d, a, (size_t)n * 4, hipMemcpyHostToDevice, eng->stream));
+    HIP_TRY(eng, hipMemcpyAsync(d + n, b, (size_t)n * 4, hipMemcpyHostToDevice, eng->stream));
+    HIP_TRY(eng, vtk::launch_fp32_selftest(d, d + n, d + 2 * (size_t)n, d + 3 * (size_t)n, n, eng->stream));
+    HIP_TRY(eng, hipMemcpyAsync(q, d + 2 * (size_t)n, (size_t)n * 4, hipMemcpyDeviceToHost, eng->stream));
+    HIP_TRY(eng, hipMemcpyAsync(r, d + 3 * (size_t)n, (size_t)n * 4, hipMemcpyDeviceToHost, eng->stream));
+    HIP_TRY(eng, hipStreamSynchronize(eng->stream));
+    buf.release();
+    return VT_OK;
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -535,23 +629,25 @@ vt_status vt_build_blas(const float* positions, uint32_t position_floats, uint32
                         uint32_t max_depth, uint32_t* out_node_words, uint32_t* out_node_count,
                         uint32_t* out_reordered_indices, uint32_t* out_reordered_mat,
                         uint32_t* out_reordered_to_source) {
-  if ((position_floats && !positions) || (index_words && !indices) || !out_node_words ||
-      !out_node_count || (index_words && !out_reordered_indices))
-    return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "NULL argument");
-  vt::BlasOut o;
-  vt::BuildError e = vt::build_blas(positions, position_floats, position_stride, indices, index_words,
-                                    index_stride, tri_material_ids, max_leaf, num_bins, max_depth, o);
-  if (e) return set_err(nullptr, e.code, e.message);
-  std::memcpy(out_node_words, o.nodes.data(), o.nodes.size() * 4);
-  *out_node_count = (uint32_t)(o.nodes.size() / 8);
-  if (!o.reordered_indices.empty())
-    std::memcpy(out_reordered_indices, o.reordered_indices.data(), o.reordered_indices.size() * 4);
-  if (out_reordered_mat && !o.reordered_mat.empty())
-    std::memcpy(out_reordered_mat, o.reordered_mat.data(), o.reordered_mat.size() * 4);
-  if (out_reordered_to_source && !o.reordered_to_source.empty())
-    std::memcpy(out_reordered_to_source, o.reordered_to_source.data(),
-                o.reordered_to_source.size() * 4);
-  return VT_OK;
+  return vt_guarded(nullptr, [&]() -> vt_status {
+    if ((position_floats && !positions) || (index_words && !indices) || !out_node_words ||
+        !out_node_count || (index_words && !out_reordered_indices))
+      return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "NULL argument");
+    vt::BlasOut o;
+    vt::BuildError e = vt::build_blas(positions, position_floats, position_stride, indices, index_words,
+                                      index_stride, tri_material_ids, max_leaf, num_bins, max_depth, o);
+    if (e) return set_err(nullptr, e.code, e.message);
+    std::memcpy(out_node_words, o.nodes.data(), o.nodes.size() * 4);
+    *out_node_count = (uint32_t)(o.nodes.size() / 8);
+    if (!o.reordered_indices.empty())
+      std::memcpy(out_reordered_indices, o.reordered_indices.data(), o.reordered_indices.size() * 4);
+    if (out_reordered_mat && !o.reordered_mat.empty())
+      std::memcpy(out_reordered_mat, o.reordered_mat.data(), o.reordered_mat.size() * 4);
+    if (out_reordered_to_source && !o.reordered_to_source.empty())
+      std::memcpy(out_reordered_to_source, o.reordered_to_source.data(),
+                  o.reordered_to_source.size() * 4);
+    return VT_OK;
+  });
 }
 
 vt_status vt_build_tlas(uint32_t n, const uint32_t* blas_ids, const double* aabb_min,
@@ -559,20 +655,22 @@ vt_status vt_build_tlas(uint32_t n, const uint32_t* blas_ids, const double* aabb
                         uint32_t num_bins, uint32_t max_depth, uint32_t* out_nodes,
                         uint32_t* out_node_count, uint32_t* out_instance_indices,
                         uint32_t* out_blas_roots, float* out_transforms) {
-  if (n && (!blas_ids || !aabb_min || !aabb_max || !world_to_local))
-    return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "NULL argument");
-  if (!out_nodes || !out_node_count || !out_instance_indices || !out_blas_roots || !out_transforms)
-    return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "NULL output");
-  vt::TlasOut o;
-  vt::BuildError e = vt::build_tlas(n, blas_ids, aabb_min, aabb_max, world_to_local, max_leaf,
-                                    num_bins, max_depth, o);
-  if (e) return set_err(nullptr, e.code, e.message);
-  std::memcpy(out_nodes, o.nodes.data(), o.nodes.size() * 4);
-  *out_node_count = (uint32_t)(o.nodes.size() / 8);
-  std::memcpy(out_instance_indices, o.instance_indices.data(), o.instance_indices.size() * 4);
-  std::memcpy(out_blas_roots, o.blas_roots.data(), o.blas_roots.size() * 4);
-  std::memcpy(out_transforms, o.transforms.data(), o.transforms.size() * 4);
-  return VT_OK;
+  return vt_guarded(nullptr, [&]() -> vt_status {
+    if (n && (!blas_ids || !aabb_min || !aabb_max || !world_to_local))
+      return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (!out_nodes || !out_node_count || !out_instance_indices || !out_blas_roots || !out_transforms)
+      return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "NULL output");
+    vt::TlasOut o;
+    vt::BuildError e = vt::build_tlas(n, blas_ids, aabb_min, aabb_max, world_to_local, max_leaf,
+                                      num_bins, max_depth, o);
+    if (e) return set_err(nullptr, e.code, e.message);
+    std::memcpy(out_nodes, o.nodes.data(), o.nodes.size() * 4);
+    *out_node_count = (uint32_t)(o.nodes.size() / 8);
+    std::memcpy(out_instance_indices, o.instance_indices.data(), o.instance_indices.size() * 4);
+    std::memcpy(out_blas_roots, o.blas_roots.data(), o.blas_roots.size() * 4);
+    std::memcpy(out_transforms, o.transforms.data(), o.transforms.size() * 4);
+    return VT_OK;
+  });
 }
 
 struct vt_pack {
@@ -581,16 +679,22 @@ struct vt_pack {
 
 vt_status vt_pack_scene(const vt_primitive* prims, uint32_t prim_count, uint32_t threads,
                         vt_pack** out) {
-  if (!out) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "out is NULL");
-  *out = nullptr;
-  if (vt_status s = check_prims(nullptr, prims, prim_count)) return s;
-  std::vector<vt::PrimitiveIn> v(prim_count);
-  for (uint32_t i = 0; i < prim_count; ++i) v[i] = to_prim(prims[i]);
-  auto pk = std::make_unique<vt_pack>();
-  vt::BuildError e = vt::pack_scene(v.data(), prim_count, threads, pk->p);
-  if (e) return set_err(nullptr, e.code, e.message);
-  *out = pk.release();
-  return VT_OK;
+  return vt_guarded(nullptr, [&]() -> vt_status {
+    if (!out) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    if (vt_status s = check_prims(nullptr, prims, prim_count)) return s;
+    if (g_fail_inject & kInjectPackThrow) {
+      g_fail_inject &= ~kInjectPackThrow;
+      throw std::bad_alloc();
+    }
+    std::vector<vt::PrimitiveIn> v(prim_count);
+    for (uint32_t i = 0; i < prim_count; ++i) v[i] = to_prim(prims[i]);
+    auto pk = std::make_unique<vt_pack>();
+    vt::BuildError e = vt::pack_scene(v.data(), prim_count, threads, pk->p);
+    if (e) return set_err(nullptr, e.code, e.message);
+    *out = pk.release();
+    return VT_OK;
+  });
 }
 
 static const void* pack_field(const vt::Pack& p, int32_t f, size_t* words) {
@@ -625,14 +729,16 @@ int64_t vt_pack_field_words(const vt_pack* p, int32_t field) {
 }
 
 vt_status vt_pack_copy_field(const vt_pack* p, int32_t field, void* dst, uint64_t dst_words) {
-  if (!p || field < 0 || field >= VT_PACK_FIELD_COUNT)
-    return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "bad pack or field");
-  size_t w = 0;
-  const void* src = pack_field(p->p, field, &w);
-  if (w > dst_words) return set_err(nullptr, VT_ERR_RANGE, "destination too small");
-  if (w && !dst) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "dst is NULL");
-  if (w) std::memcpy(dst, src, w * 4);
-  return VT_OK;
+  return vt_guarded(nullptr, [&]() -> vt_status {
+    if (!p || field < 0 || field >= VT_PACK_FIELD_COUNT)
+      return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "bad pack or field");
+    size_t w = 0;
+    const void* src = pack_field(p->p, field, &w);
+    if (w > dst_words) return set_err(nullptr, VT_ERR_RANGE, "destination too small");
+    if (w && !dst) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "dst is NULL");
+    if (w) std::memcpy(dst, src, w * 4);
+    return VT_OK;
+  });
 }
 
 void vt_pack_free(vt_pack* p) { delete p; }
@@ -640,18 +746,22 @@ void vt_pack_free(vt_pack* p) { delete p; }
 }  // extern "C"
 
 vt_status vt_set_node_width(vt_engine* eng, uint32_t width) {
-  if (!eng) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "engine is NULL");
-  if (width != 2 && width != 4) return set_err(eng, VT_ERR_RANGE, "node width must be 2 or 4");
-  eng->node_width = width;
-  return VT_OK;
+  return vt_guarded(eng, [&]() -> vt_status {
+    if (!eng) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "engine is NULL");
+    if (width != 2 && width != 4) return set_err(eng, VT_ERR_RANGE, "node width must be 2 or 4");
+    eng->node_width = width;
+    return VT_OK;
+  });
 }
 
 static_assert(vt::kWideKids == vtk::kWideKids && vt::kNode128Words == 4 * vtk::kWideNodeFloat4s,
               "the wide-node layout of the builder and of the traversal kernel differ");
 
 vt_status vt_set_hit_filter(vt_engine* eng, uint32_t filter) {
-  if (!eng) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "engine is NULL");
-  if (filter > 1) return set_err(eng, VT_ERR_RANGE, "hit filter must be 0 (raw) or 1 (sided wrappers)");
-  eng->hit_filter = filter;
-  return VT_OK;
+  return vt_guarded(eng, [&]() -> vt_status {
+    if (!eng) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "engine is NULL");
+    if (filter > 1) return set_err(eng, VT_ERR_RANGE, "hit filter must be 0 (raw) or 1 (sided wrappers)");
+    eng->hit_filter = filter;
+    return VT_OK;
+  });
 }

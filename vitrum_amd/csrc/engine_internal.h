@@ -3,7 +3,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <exception>
 #include <memory>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -11,6 +13,36 @@
 #include "builder.h"
 #include "device_layout.h"
 #include "traverse.h"
+
+// Records `msg` as the engine's last error (the thread's, without an engine)
+// and returns `code`.
+vt_status engine_error(vt_engine* e, vt_status code, const std::string& msg);
+
+// Runs the body of a C entry point.  An exception -- std::bad_alloc from a
+// builder's containers on a very large scene, std::length_error, anything a
+// library throws -- must not unwind through extern "C" into ctypes or N-API:
+// it becomes VT_ERR_OUT_OF_MEMORY (allocation) or VT_ERR_BUILD (any other
+// exception), with its message.
+template <class F>
+vt_status vt_guarded(vt_engine* e, F&& body) noexcept {
+  try {
+    return body();
+  } catch (const std::bad_alloc&) {
+    try {
+      return engine_error(e, VT_ERR_OUT_OF_MEMORY, "out of host memory");
+    } catch (...) {
+      return VT_ERR_OUT_OF_MEMORY;
+    }
+  } catch (const std::exception& ex) {
+    try {
+      return engine_error(e, VT_ERR_BUILD, std::string("internal error: ") + ex.what());
+    } catch (...) {
+      return VT_ERR_BUILD;
+    }
+  } catch (...) {
+    return VT_ERR_BUILD;
+  }
+}
 
 struct DeviceBuffer {
   void* ptr = nullptr;
@@ -27,7 +59,13 @@ struct SceneBuffers {
   void release();
 };
 
-struct RenderState;  // integrator state (render.hip)
+struct RenderState;      // integrator state (render.hip)
+struct RenderSceneData;  // one scene's shading data (render.cpp)
+
+// Test hooks (vt_debug_inject_failure): each bit makes the next matching step
+// fail once -- packing throws std::bad_alloc, the shading-data upload reports
+// an allocation failure, or its preparation throws std::runtime_error.
+enum : uint32_t { kInjectPackThrow = 1u, kInjectRenderAlloc = 2u, kInjectRenderThrow = 4u };
 struct CwbvhState;   // wide-BVH traversal buffers (cwbvh.cpp)
 struct NrcState;     // neural radiance cache (nrc.hip)
 
@@ -65,7 +103,12 @@ struct vt_engine {
   NrcState* nrc = nullptr;
   CwbvhState* cwbvh = nullptr;
   std::string last_error;
+  uint32_t fail_inject = 0;    // kInject* bits
+  uint32_t build_threads = 0;  // host threads of the scene builder (0: every hardware thread)
 };
+
+// Threads the scene builder uses for this engine.
+uint32_t engine_build_threads(const vt_engine* e);
 
 vt_status engine_upload(vt_engine* e, DeviceBuffer& dst, const void* src, size_t bytes);
 // Uploads the traversal buffers of a device layout (Node64 + Node128 trees,
@@ -79,8 +122,13 @@ vtk::TraceParams engine_trace_params(vt_engine* e);
 // Packs, builds and uploads one scene's traversal buffers (engine.cpp).
 vt_status engine_build_trace_scene(vt_engine* e, const vt_scene_desc* scene, SceneBuffers& out,
                                    vt::DeviceLayout& layout);
-// Hooks into the integrator (render.cpp): rebuild per-scene render data, free it.
-vt_status engine_on_scene_changed(vt_engine* e);
+// Hooks into the integrator (render.cpp).  prepare builds and uploads a new
+// scene's shading data without touching the bound scene; commit binds it
+// (cannot fail; takes ownership); discard frees an unbound set.
+vt_status engine_prepare_render(vt_engine* e, const vt::Pack& pack, const std::vector<vt_material>& materials,
+                                const vt_environment& env, float scene_radius, RenderSceneData** out);
+void engine_commit_render(vt_engine* e, RenderSceneData* d);
+void engine_discard_render(RenderSceneData* d);
 void engine_release_render(vt_engine* e);
 void engine_release_nrc(vt_engine* e);
 // Wide-BVH (cwbvh-closest) traversal state (cwbvh.cpp): built on first use
@@ -100,6 +148,8 @@ vt_status engine_trace_cwbvh(vt_engine* e, const float4* d_rays, uint32_t n, con
                              uint32_t* d_hits, float* d_hit_pos, uint32_t sided);
 // Cache hooks for the integrator (nrc.cpp): counts are device words.
 vt_status engine_nrc_ensure(vt_engine* e);
+// Creates the cache if it does not exist, without joining a training step.
+vt_status engine_nrc_create(vt_engine* e);
 // The path tracer's cache training step runs on a stream of its own: join
 // makes the engine stream wait for it, quiesce waits for it on the host
 // (before buffers it reads are freed).

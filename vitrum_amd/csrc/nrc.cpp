@@ -145,6 +145,9 @@ static vt_status nrc_accumulate(vt_engine* e, const float* d_features, const flo
 
 vt_status engine_nrc_ensure(vt_engine* e) { return nrc_ensure(e); }
 
+// A new cache has no training step in flight, so creating one joins nothing.
+vt_status engine_nrc_create(vt_engine* e) { return e->nrc ? VT_OK : nrc_ensure(e); }
+
 vt_status engine_nrc_infer_counted(vt_engine* e, const float* d_feat, const uint32_t* d_count, float* d_rgb,
                                    uint32_t capacity) {
   if (vt_status s = nrc_ensure(e)) return s;
@@ -171,73 +174,85 @@ vt_status engine_nrc_train_counted(vt_engine* e, const float* d_feat, const floa
 extern "C" {
 
 vt_status vt_nrc_param_count(uint32_t* count) {
-  if (!count) return VT_ERR_INVALID_ARGUMENT;
-  *count = vtn::kParams;
-  return VT_OK;
+  return vt_guarded(nullptr, [&]() -> vt_status {
+    if (!count) return VT_ERR_INVALID_ARGUMENT;
+    *count = vtn::kParams;
+    return VT_OK;
+  });
 }
 
 vt_status vt_nrc_reset(vt_engine* e, const vt_nrc_options* opts) {
-  if (!e) return VT_ERR_INVALID_ARGUMENT;
-  if (!e->nrc) {
-    vt_status s = nrc_ensure(e);  // allocates, then re-enters with defaults
-    if (s || !opts) return s;
-  }
-  if (vt_status st = nrc_join(e)) return st;
-  NrcState& s = *e->nrc;
-  if (opts) {
-    if (!(opts->learning_rate > 0) || !(opts->beta1 >= 0 && opts->beta1 < 1) ||
-        !(opts->beta2 >= 0 && opts->beta2 < 1) || !(opts->epsilon > 0) || !(opts->table_learning_rate >= 0))
-      return nrc_err(e, VT_ERR_RANGE,
-                     "learning_rate > 0, table_learning_rate >= 0, 0 <= beta1, beta2 < 1 and epsilon > 0 are required");
-    s.opts = *opts;
-    if (s.opts.table_learning_rate == 0.0f) s.opts.table_learning_rate = 1e-1f;
-  }
-  std::vector<float> w;
-  init_weights(s.opts.seed, w);
-  return upload_master(e, w.data());
+  return vt_guarded(e, [&]() -> vt_status {
+    if (!e) return VT_ERR_INVALID_ARGUMENT;
+    if (!e->nrc) {
+      vt_status s = nrc_ensure(e);  // allocates, then re-enters with defaults
+      if (s || !opts) return s;
+    }
+    if (vt_status st = nrc_join(e)) return st;
+    NrcState& s = *e->nrc;
+    if (opts) {
+      if (!(opts->learning_rate > 0) || !(opts->beta1 >= 0 && opts->beta1 < 1) ||
+          !(opts->beta2 >= 0 && opts->beta2 < 1) || !(opts->epsilon > 0) || !(opts->table_learning_rate >= 0))
+        return nrc_err(e, VT_ERR_RANGE,
+                       "learning_rate > 0, table_learning_rate >= 0, 0 <= beta1, beta2 < 1 and epsilon > 0 are required");
+      s.opts = *opts;
+      if (s.opts.table_learning_rate == 0.0f) s.opts.table_learning_rate = 1e-1f;
+    }
+    std::vector<float> w;
+    init_weights(s.opts.seed, w);
+    return upload_master(e, w.data());
+  });
 }
 
 vt_status vt_nrc_set_weights(vt_engine* e, const float* w, uint32_t count) {
-  if (!e || !w) return VT_ERR_INVALID_ARGUMENT;
-  if (count != vtn::kParams) return nrc_err(e, VT_ERR_RANGE, "weight count must equal vt_nrc_param_count()");
-  if (!finite_features(w, count)) return nrc_err(e, VT_ERR_RANGE, "weights must be finite");
-  if (vt_status s = nrc_ensure(e)) return s;
-  return upload_master(e, w);
+  return vt_guarded(e, [&]() -> vt_status {
+    if (!e || !w) return VT_ERR_INVALID_ARGUMENT;
+    if (count != vtn::kParams) return nrc_err(e, VT_ERR_RANGE, "weight count must equal vt_nrc_param_count()");
+    if (!finite_features(w, count)) return nrc_err(e, VT_ERR_RANGE, "weights must be finite");
+    if (vt_status s = nrc_ensure(e)) return s;
+    return upload_master(e, w);
+  });
 }
 
 vt_status vt_nrc_get_weights(vt_engine* e, float* w, uint32_t count) {
-  if (!e || !w) return VT_ERR_INVALID_ARGUMENT;
-  if (count != vtn::kParams) return nrc_err(e, VT_ERR_RANGE, "weight count must equal vt_nrc_param_count()");
-  if (vt_status s = nrc_ensure(e)) return s;
-  NRC_HIP(e, hipMemcpyAsync(w, e->nrc->master.ptr, count * sizeof(float), hipMemcpyDeviceToHost, e->stream));
-  NRC_HIP(e, hipStreamSynchronize(e->stream));
-  return VT_OK;
+  return vt_guarded(e, [&]() -> vt_status {
+    if (!e || !w) return VT_ERR_INVALID_ARGUMENT;
+    if (count != vtn::kParams) return nrc_err(e, VT_ERR_RANGE, "weight count must equal vt_nrc_param_count()");
+    if (vt_status s = nrc_ensure(e)) return s;
+    NRC_HIP(e, hipMemcpyAsync(w, e->nrc->master.ptr, count * sizeof(float), hipMemcpyDeviceToHost, e->stream));
+    NRC_HIP(e, hipStreamSynchronize(e->stream));
+    return VT_OK;
+  });
 }
 
 vt_status vt_nrc_infer_device(vt_engine* e, const float* d_features, uint32_t n, float* d_rgb) {
-  if (!e || (n && (!d_features || !d_rgb))) return VT_ERR_INVALID_ARGUMENT;
-  if (vt_status s = nrc_ensure(e)) return s;
-  NRC_HIP(e, hipSetDevice(e->device));
-  NRC_HIP(e, hipEventRecord(e->nrc->ev[0], e->stream));
-  NRC_HIP(e, vtn::launch_infer(d_features, n, nullptr, (const uint4*)e->nrc->frags.ptr,
-                               (const float*)e->nrc->master.ptr + vtn::kMlpParams, d_rgb, e->stream));
-  NRC_HIP(e, hipEventRecord(e->nrc->ev[1], e->stream));
-  e->nrc->timed_infer = true;
-  return VT_OK;
+  return vt_guarded(e, [&]() -> vt_status {
+    if (!e || (n && (!d_features || !d_rgb))) return VT_ERR_INVALID_ARGUMENT;
+    if (vt_status s = nrc_ensure(e)) return s;
+    NRC_HIP(e, hipSetDevice(e->device));
+    NRC_HIP(e, hipEventRecord(e->nrc->ev[0], e->stream));
+    NRC_HIP(e, vtn::launch_infer(d_features, n, nullptr, (const uint4*)e->nrc->frags.ptr,
+                                 (const float*)e->nrc->master.ptr + vtn::kMlpParams, d_rgb, e->stream));
+    NRC_HIP(e, hipEventRecord(e->nrc->ev[1], e->stream));
+    e->nrc->timed_infer = true;
+    return VT_OK;
+  });
 }
 
 vt_status vt_nrc_infer(vt_engine* e, const float* features, uint32_t n, float* rgb) {
-  if (!e || (n && (!features || !rgb))) return VT_ERR_INVALID_ARGUMENT;
-  if (!finite_features(features, (size_t)n * vtn::kFeat)) return nrc_err(e, VT_ERR_RANGE, "features must be finite");
-  if (vt_status s = nrc_ensure(e)) return s;
-  NrcState& s = *e->nrc;
-  if (vt_status st = s.feats.ensure(e, (size_t)n * vtn::kFeat * 4 + 64)) return st;
-  if (vt_status st = s.out.ensure(e, (size_t)n * 12 + 64)) return st;
-  NRC_HIP(e, hipMemcpyAsync(s.feats.ptr, features, (size_t)n * vtn::kFeat * 4, hipMemcpyHostToDevice, e->stream));
-  if (vt_status st = vt_nrc_infer_device(e, (const float*)s.feats.ptr, n, (float*)s.out.ptr)) return st;
-  NRC_HIP(e, hipMemcpyAsync(rgb, s.out.ptr, (size_t)n * 12, hipMemcpyDeviceToHost, e->stream));
-  NRC_HIP(e, hipStreamSynchronize(e->stream));
-  return VT_OK;
+  return vt_guarded(e, [&]() -> vt_status {
+    if (!e || (n && (!features || !rgb))) return VT_ERR_INVALID_ARGUMENT;
+    if (!finite_features(features, (size_t)n * vtn::kFeat)) return nrc_err(e, VT_ERR_RANGE, "features must be finite");
+    if (vt_status s = nrc_ensure(e)) return s;
+    NrcState& s = *e->nrc;
+    if (vt_status st = s.feats.ensure(e, (size_t)n * vtn::kFeat * 4 + 64)) return st;
+    if (vt_status st = s.out.ensure(e, (size_t)n * 12 + 64)) return st;
+    NRC_HIP(e, hipMemcpyAsync(s.feats.ptr, features, (size_t)n * vtn::kFeat * 4, hipMemcpyHostToDevice, e->stream));
+    if (vt_status st = vt_nrc_infer_device(e, (const float*)s.feats.ptr, n, (float*)s.out.ptr)) return st;
+    NRC_HIP(e, hipMemcpyAsync(rgb, s.out.ptr, (size_t)n * 12, hipMemcpyDeviceToHost, e->stream));
+    NRC_HIP(e, hipStreamSynchronize(e->stream));
+    return VT_OK;
+  });
 }
 
 // Accumulates the batch gradient into the engine's gradient buffer and the
@@ -275,35 +290,41 @@ static vt_status nrc_apply_adam(vt_engine* e, hipStream_t stream) {
 }
 
 vt_status vt_nrc_train_step_device(vt_engine* e, const float* d_features, const float* d_targets, uint32_t n) {
-  if (!e || (n && (!d_features || !d_targets))) return VT_ERR_INVALID_ARGUMENT;
-  if (n == 0) return VT_OK;
-  if (vt_status s = nrc_ensure(e)) return s;
-  if (vt_status s = nrc_accumulate(e, d_features, d_targets, n, nullptr, e->stream)) return s;
-  return nrc_apply_adam(e, e->stream);
+  return vt_guarded(e, [&]() -> vt_status {
+    if (!e || (n && (!d_features || !d_targets))) return VT_ERR_INVALID_ARGUMENT;
+    if (n == 0) return VT_OK;
+    if (vt_status s = nrc_ensure(e)) return s;
+    if (vt_status s = nrc_accumulate(e, d_features, d_targets, n, nullptr, e->stream)) return s;
+    return nrc_apply_adam(e, e->stream);
+  });
 }
 
 vt_status vt_nrc_timing(vt_engine* e, float* infer_ms, float* train_ms) {
-  if (!e || !infer_ms || !train_ms) return VT_ERR_INVALID_ARGUMENT;
-  *infer_ms = *train_ms = 0.0f;
-  if (!e->nrc) return VT_OK;
-  NrcState& s = *e->nrc;
-  if (s.timed_infer) {
-    NRC_HIP(e, hipEventSynchronize(s.ev[1]));
-    NRC_HIP(e, hipEventElapsedTime(infer_ms, s.ev[0], s.ev[1]));
-  }
-  if (s.timed_train) {
-    NRC_HIP(e, hipEventSynchronize(s.ev[3]));
-    NRC_HIP(e, hipEventElapsedTime(train_ms, s.ev[2], s.ev[3]));
-  }
-  return VT_OK;
+  return vt_guarded(e, [&]() -> vt_status {
+    if (!e || !infer_ms || !train_ms) return VT_ERR_INVALID_ARGUMENT;
+    *infer_ms = *train_ms = 0.0f;
+    if (!e->nrc) return VT_OK;
+    NrcState& s = *e->nrc;
+    if (s.timed_infer) {
+      NRC_HIP(e, hipEventSynchronize(s.ev[1]));
+      NRC_HIP(e, hipEventElapsedTime(infer_ms, s.ev[0], s.ev[1]));
+    }
+    if (s.timed_train) {
+      NRC_HIP(e, hipEventSynchronize(s.ev[3]));
+      NRC_HIP(e, hipEventElapsedTime(train_ms, s.ev[2], s.ev[3]));
+    }
+    return VT_OK;
+  });
 }
 
 vt_status vt_nrc_last_loss(vt_engine* e, float* loss) {
-  if (!e || !loss) return VT_ERR_INVALID_ARGUMENT;
-  if (vt_status s = nrc_ensure(e)) return s;
-  NRC_HIP(e, hipMemcpyAsync(loss, e->nrc->loss.ptr, 4, hipMemcpyDeviceToHost, e->stream));
-  NRC_HIP(e, hipStreamSynchronize(e->stream));
-  return VT_OK;
+  return vt_guarded(e, [&]() -> vt_status {
+    if (!e || !loss) return VT_ERR_INVALID_ARGUMENT;
+    if (vt_status s = nrc_ensure(e)) return s;
+    NRC_HIP(e, hipMemcpyAsync(loss, e->nrc->loss.ptr, 4, hipMemcpyDeviceToHost, e->stream));
+    NRC_HIP(e, hipStreamSynchronize(e->stream));
+    return VT_OK;
+  });
 }
 
 static vt_status stage_batch(vt_engine* e, const float* features, const float* targets, uint32_t n) {
@@ -318,37 +339,41 @@ static vt_status stage_batch(vt_engine* e, const float* features, const float* t
 }
 
 vt_status vt_nrc_train_step(vt_engine* e, const float* features, const float* targets, uint32_t n, float* loss) {
-  if (!e || (n && (!features || !targets))) return VT_ERR_INVALID_ARGUMENT;
-  if (vt_status s = nrc_ensure(e)) return s;
-  if (n == 0) return VT_OK;
-  if (vt_status s = stage_batch(e, features, targets, n)) return s;
-  NrcState& s = *e->nrc;
-  if (vt_status st = vt_nrc_train_step_device(e, (const float*)s.feats.ptr, (const float*)s.targets.ptr, n))
-    return st;
-  if (loss) return vt_nrc_last_loss(e, loss);
-  NRC_HIP(e, hipStreamSynchronize(e->stream));
-  return VT_OK;
+  return vt_guarded(e, [&]() -> vt_status {
+    if (!e || (n && (!features || !targets))) return VT_ERR_INVALID_ARGUMENT;
+    if (vt_status s = nrc_ensure(e)) return s;
+    if (n == 0) return VT_OK;
+    if (vt_status s = stage_batch(e, features, targets, n)) return s;
+    NrcState& s = *e->nrc;
+    if (vt_status st = vt_nrc_train_step_device(e, (const float*)s.feats.ptr, (const float*)s.targets.ptr, n))
+      return st;
+    if (loss) return vt_nrc_last_loss(e, loss);
+    NRC_HIP(e, hipStreamSynchronize(e->stream));
+    return VT_OK;
+  });
 }
 
 vt_status vt_nrc_gradient(vt_engine* e, const float* features, const float* targets, uint32_t n, float* grad,
                           float* loss) {
-  if (!e || !grad || (n && (!features || !targets))) return VT_ERR_INVALID_ARGUMENT;
-  if (vt_status s = nrc_ensure(e)) return s;
-  NrcState& s = *e->nrc;
-  NRC_HIP(e, hipSetDevice(e->device));
-  NRC_HIP(e, vtn::launch_fill((float*)s.grad.ptr, vtn::kParams, 0.0f, e->stream));
-  if (n) {
-    if (vt_status st = stage_batch(e, features, targets, n)) return st;
-    if (vt_status st = nrc_accumulate(e, (const float*)s.feats.ptr, (const float*)s.targets.ptr, n, nullptr, e->stream))
-      return st;
-  } else {
-    NRC_HIP(e, vtn::launch_fill((float*)s.loss.ptr, 1, 0.0f, e->stream));
-  }
-  NRC_HIP(e, hipMemcpyAsync(grad, s.grad.ptr, vtn::kParams * 4, hipMemcpyDeviceToHost, e->stream));
-  if (loss) NRC_HIP(e, hipMemcpyAsync(loss, s.loss.ptr, 4, hipMemcpyDeviceToHost, e->stream));
-  NRC_HIP(e, vtn::launch_fill((float*)s.grad.ptr, vtn::kParams, 0.0f, e->stream));
-  NRC_HIP(e, hipStreamSynchronize(e->stream));
-  return VT_OK;
+  return vt_guarded(e, [&]() -> vt_status {
+    if (!e || !grad || (n && (!features || !targets))) return VT_ERR_INVALID_ARGUMENT;
+    if (vt_status s = nrc_ensure(e)) return s;
+    NrcState& s = *e->nrc;
+    NRC_HIP(e, hipSetDevice(e->device));
+    NRC_HIP(e, vtn::launch_fill((float*)s.grad.ptr, vtn::kParams, 0.0f, e->stream));
+    if (n) {
+      if (vt_status st = stage_batch(e, features, targets, n)) return st;
+      if (vt_status st = nrc_accumulate(e, (const float*)s.feats.ptr, (const float*)s.targets.ptr, n, nullptr, e->stream))
+        return st;
+    } else {
+      NRC_HIP(e, vtn::launch_fill((float*)s.loss.ptr, 1, 0.0f, e->stream));
+    }
+    NRC_HIP(e, hipMemcpyAsync(grad, s.grad.ptr, vtn::kParams * 4, hipMemcpyDeviceToHost, e->stream));
+    if (loss) NRC_HIP(e, hipMemcpyAsync(loss, s.loss.ptr, 4, hipMemcpyDeviceToHost, e->stream));
+    NRC_HIP(e, vtn::launch_fill((float*)s.grad.ptr, vtn::kParams, 0.0f, e->stream));
+    NRC_HIP(e, hipStreamSynchronize(e->stream));
+    return VT_OK;
+  });
 }
 
 }  // extern "C"

@@ -6,6 +6,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -86,16 +88,25 @@ std::vector<float> represented_cdf(const std::vector<double>& w) {
 
 }  // namespace
 
-struct RenderState {
-  // Per-scene shading data.
+// Per-scene shading data.  vt_set_scene builds and uploads a complete set
+// before it binds anything, so a failure leaves the previous scene's set.
+struct RenderSceneData {
   DeviceBuffer materials, emitters, instances, env_cdf, env_pdf, tri_source, light_tree;
   uint32_t light_tree_nodes = 0;  // 0: uniform light selection
   std::vector<float> light_tree_host;
-  DeviceBuffer sobol_dirs;  // 512 x 16 Sobol direction components (built once)
   uint32_t emitter_count = 0;
   uint32_t env_present = 0, env_w = 64, env_h = 32;
   float env_radiance[3] = {0, 0, 0};
   float ray_bias = 0, tmin = 0;
+  float scene_min[3] = {0, 0, 0}, scene_inv_extent[3] = {1, 1, 1};
+  void release_scene() {
+    for (DeviceBuffer* b : {&materials, &emitters, &instances, &env_cdf, &env_pdf, &tri_source, &light_tree})
+      b->release();
+  }
+};
+
+struct RenderState : RenderSceneData {
+  DeviceBuffer sobol_dirs;  // 512 x 16 Sobol direction components (built once)
   // Per-resolution path state.
   uint32_t width = 0, height = 0, paths = 0;
   DeviceBuffer rng, rng_dim, throughput, conn, radiance, prev_pdf, flags, signature, count;
@@ -105,7 +116,6 @@ struct RenderState {
   bool nrc_enabled = false;
   uint32_t nrc_train_paths = 65536;
   uint32_t nrc_train_vertices = 4;  // shaded teacher vertices before the terminal query (0: full length)
-  float scene_min[3] = {0, 0, 0}, scene_inv_extent[3] = {1, 1, 1};
   DeviceBuffer nrc_count, q_feat, q_thr, q_path, q_rgb, train_rec, train_n, train_feat, train_tgt;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // Shadow rays of a bounce are traced on `side` while the main stream traces
@@ -122,10 +132,6 @@ struct RenderState {
       b->release();
     width = height = paths = 0;
   }
-  void release_scene() {
-    for (DeviceBuffer* b : {&materials, &emitters, &instances, &env_cdf, &env_pdf, &tri_source, &sobol_dirs, &light_tree})
-      b->release();
-  }
 };
 
 void engine_release_render(vt_engine* e) {
@@ -133,6 +139,7 @@ void engine_release_render(vt_engine* e) {
   (void)engine_nrc_quiesce(e);
   e->render->release_paths();
   e->render->release_scene();
+  e->render->sobol_dirs.release();
   if (e->render->side) (void)hipStreamSynchronize(e->render->side);
   for (hipEvent_t ev : {e->render->ev0, e->render->ev1, e->render->ev_fork, e->render->ev_join})
     if (ev) (void)hipEventDestroy(ev);
@@ -150,22 +157,50 @@ static float ray_origin_bias(double radius, const double* center) {
   return (float)std::max(radius * (1e-3 / std::sqrt(3.0)), 2.0 * ulp);
 }
 
-vt_status engine_on_scene_changed(vt_engine* e) {
+static vt_status ensure_render_state(vt_engine* e) {
+  if (e->render) return VT_OK;
+  std::unique_ptr<RenderState> r(new RenderState());
+  HIP_CHECK(e, hipEventCreate(&r->ev0));
+  HIP_CHECK(e, hipEventCreate(&r->ev1));
+  HIP_CHECK(e, hipEventCreateWithFlags(&r->ev_fork, hipEventDisableTiming));
+  HIP_CHECK(e, hipEventCreateWithFlags(&r->ev_join, hipEventDisableTiming));
+  HIP_CHECK(e, hipStreamCreateWithFlags(&r->side, hipStreamNonBlocking));
+  std::vector<uint16_t> dirs(vts::kDims * vts::kBits);
+  vts::build_directions(dirs.data());
+  if (vt_status s = engine_upload(e, r->sobol_dirs, dirs.data(), dirs.size() * 2)) return s;
+  HIP_CHECK(e, hipStreamSynchronize(e->stream));
+  e->render = r.release();
+  return VT_OK;
+}
+
+void engine_discard_render(RenderSceneData* d) {
+  if (!d) return;
+  d->release_scene();
+  delete d;
+}
+
+void engine_commit_render(vt_engine* e, RenderSceneData* d) {
+  RenderSceneData& cur = *e->render;
+  cur.release_scene();
+  cur = *d;
+  delete d;  // its device buffers now belong to the engine
   e->samples_accumulated = 0;
-  if (!e->render) {
-    e->render = new RenderState();
-    HIP_CHECK(e, hipEventCreate(&e->render->ev0));
-    HIP_CHECK(e, hipEventCreate(&e->render->ev1));
-    HIP_CHECK(e, hipEventCreateWithFlags(&e->render->ev_fork, hipEventDisableTiming));
-    HIP_CHECK(e, hipEventCreateWithFlags(&e->render->ev_join, hipEventDisableTiming));
-    HIP_CHECK(e, hipStreamCreateWithFlags(&e->render->side, hipStreamNonBlocking));
+}
+
+vt_status engine_prepare_render(vt_engine* e, const vt::Pack& P, const std::vector<vt_material>& materials,
+                                const vt_environment& env, float scene_radius, RenderSceneData** out) {
+  *out = nullptr;
+  if (vt_status s = ensure_render_state(e)) return s;
+  if (e->fail_inject & kInjectRenderThrow) {
+    e->fail_inject &= ~kInjectRenderThrow;
+    throw std::runtime_error("injected failure while preparing the scene's shading data");
   }
-  RenderState& R = *e->render;
-  const vt::Pack& P = *e->pack;
+  std::unique_ptr<RenderSceneData, void (*)(RenderSceneData*)> hold(new RenderSceneData(), engine_discard_render);
+  RenderSceneData& R = *hold;
   // Materials: base rgb, roughness, metallic, Le = emissive * intensity, twoSided.
-  std::vector<float> mats(std::max<size_t>(1, e->materials.size()) * vtr::kMatFloats, 0.0f);
-  for (size_t i = 0; i < e->materials.size(); ++i) {
-    const vt_material& m = e->materials[i];
+  std::vector<float> mats(std::max<size_t>(1, materials.size()) * vtr::kMatFloats, 0.0f);
+  for (size_t i = 0; i < materials.size(); ++i) {
+    const vt_material& m = materials[i];
     float* q = &mats[i * vtr::kMatFloats];
     // Le = emissive x emissiveIntensity in binary64, rounded once to f32 (the
     // reference multiplies JavaScript numbers and packs a Float32Array)
@@ -175,7 +210,7 @@ vt_status engine_on_scene_changed(vt_engine* e) {
     q[4] = m.metallic;
     q[8] = m.double_sided ? 1.0f : 0.0f;
   }
-  if (e->materials.empty()) mats[0] = mats[1] = mats[2] = 0.8f, mats[3] = 1.0f;
+  if (materials.empty()) mats[0] = mats[1] = mats[2] = 0.8f, mats[3] = 1.0f;
   // Mesh-area emitters, one per emissive triangle (scene/emitterPacking.ts:
   // 748-820): primitives in scene order, each primitive's instances in packed
   // order, triangles in source order; world vertices are the binary64
@@ -232,10 +267,8 @@ vt_status engine_on_scene_changed(vt_engine* e) {
   if (em.empty()) em.assign(vtr::kEmitFloats, 0.0f);
   // Environment: constant-radiance equirect map, luminance x texel solid
   // angle importance (scene/environmentPacking.ts:340-420).
-  R.env_present = e->scene_desc_env.kind == 1 &&
-                  (e->scene_desc_env.radiance[0] > 0 || e->scene_desc_env.radiance[1] > 0 ||
-                   e->scene_desc_env.radiance[2] > 0);
-  for (int k = 0; k < 3; ++k) R.env_radiance[k] = R.env_present ? e->scene_desc_env.radiance[k] : 0.0f;
+  R.env_present = env.kind == 1 && (env.radiance[0] > 0 || env.radiance[1] > 0 || env.radiance[2] > 0);
+  for (int k = 0; k < 3; ++k) R.env_radiance[k] = R.env_present ? env.radiance[k] : 0.0f;
   const uint32_t W = R.env_w, H = R.env_h;
   std::vector<double> weights((size_t)W * H), omega((size_t)W * H);
   const double lum = 0.2126 * R.env_radiance[0] + 0.7152 * R.env_radiance[1] + 0.0722 * R.env_radiance[2];
@@ -293,7 +326,7 @@ vt_status engine_on_scene_changed(vt_engine* e) {
   R.light_tree_host = tree;
   if (tree.empty()) tree.assign(vtl::kNodeFloats, 0.0f);
   // Ray-origin bias from the scene bounds (or the caller's radius).
-  double center[3], radius = e->scene_radius;
+  double center[3], radius = scene_radius;
   for (int k = 0; k < 3; ++k) center[k] = 0.5 * (P.world_min[k] + P.world_max[k]);
   if (!(radius > 0)) {
     double d2 = 0;
@@ -318,13 +351,12 @@ vt_status engine_on_scene_changed(vt_engine* e) {
   s = s ? s : engine_upload(e, R.env_pdf, pdf.data(), pdf.size() * 4);
   s = s ? s : engine_upload(e, R.tri_source, P.tri_source.data(), P.tri_source.size() * 4);
   s = s ? s : engine_upload(e, R.light_tree, tree.data(), tree.size() * 4);
-  if (!s && !R.sobol_dirs.ptr) {
-    std::vector<uint16_t> dirs(vts::kDims * vts::kBits);
-    vts::build_directions(dirs.data());
-    s = engine_upload(e, R.sobol_dirs, dirs.data(), dirs.size() * 2);
+  if (!s && (e->fail_inject & kInjectRenderAlloc)) {
+    e->fail_inject &= ~kInjectRenderAlloc;
+    s = rerr(e, VT_ERR_OUT_OF_MEMORY, "injected allocation failure while uploading the scene's shading data");
   }
   if (s) return s;
-  HIP_CHECK(e, hipStreamSynchronize(e->stream));
+  *out = hold.release();
   return VT_OK;
 }
 
@@ -390,250 +422,281 @@ static vtk::TraceParams trace_params(vt_engine* e) { return engine_trace_params(
 extern "C" {
 
 vt_status vt_render_frame(vt_engine* e, const vt_frame* fr) {
-  if (!e) return VT_ERR_INVALID_ARGUMENT;
-  if (!fr) return rerr(e, VT_ERR_INVALID_ARGUMENT, "frame is NULL");
-  if (!e->has_scene || !e->render) return rerr(e, VT_ERR_STATE, "no scene is set");
-  if (fr->width == 0 || fr->height == 0 || (uint64_t)fr->width * fr->height > (1u << 28))
-    return rerr(e, VT_ERR_RANGE, "viewport must be non-empty and at most 2^28 pixels");
-  for (int i = 0; i < 16; ++i)
-    if (!std::isfinite(fr->inv_view_proj[i])) return rerr(e, VT_ERR_RANGE, "inverse view-projection must be finite");
-  HIP_CHECK(e, hipSetDevice(e->device));
-  if (vt_status s = ensure_paths(e, fr->width, fr->height)) return s;
-  if (e->opts.max_samples_per_pixel && e->samples_accumulated >= e->opts.max_samples_per_pixel) return VT_OK;
-  if (fr->samples_target && e->samples_accumulated >= fr->samples_target) return VT_OK;
-  RenderState& R = *e->render;
-  vtr::FrameParams f{};
-  std::memcpy(f.inv_vp, fr->inv_view_proj, 64);
-  f.width = fr->width;
-  f.height = fr->height;
-  f.frame_seed = fr->frame_seed;
-  f.frame_index = fr->frame_index;
-  const uint32_t mb = fr->max_bounces ? fr->max_bounces : e->opts.max_bounces;
-  f.bounce_limit = std::max(1u, std::min(mb, 8u));
-  f.ray_bias = R.ray_bias;
-  f.tmin = R.tmin;
-  f.pixels = fr->width * fr->height;
-  f.sobol = e->opts.sampling == 1 ? 1u : 0u;
-  f.sobol_key = vts::frame_key(fr->frame_seed, fr->frame_index);
-  f.sobol_dirs = (const uint16_t*)R.sobol_dirs.ptr;
-  f.nrc = R.nrc_enabled ? 1u : 0u;
-  f.train_paths = R.nrc_enabled ? R.nrc_train_paths : 0u;
-  f.train_vertices = R.nrc_train_vertices;
-  for (int k = 0; k < 3; ++k) f.scene_min[k] = R.scene_min[k], f.scene_inv_extent[k] = R.scene_inv_extent[k];
-  vtr::ShadeScene sc{};
-  sc.positions = (const float4*)e->scene.positions.ptr;
-  sc.normals = (const float4*)e->scene.normals.ptr;
-  sc.indices = (const uint4*)e->scene.indices.ptr;
-  sc.tri_material = (const uint32_t*)e->scene.tri_material.ptr;
-  sc.tri_source = (const uint32_t*)R.tri_source.ptr;
-  sc.materials = (const float*)R.materials.ptr;
-  sc.emitters = (const float*)R.emitters.ptr;
-  sc.instances = (const float*)R.instances.ptr;
-  sc.env_cdf = (const float*)R.env_cdf.ptr;
-  sc.env_pdf = (const float*)R.env_pdf.ptr;
-  sc.emitter_count = R.emitter_count;
-  sc.env_present = R.env_present;
-  sc.env_w = R.env_w;
-  sc.env_h = R.env_h;
-  for (int k = 0; k < 3; ++k) sc.env_radiance[k] = R.env_radiance[k];
-  sc.light_tree = (const float*)R.light_tree.ptr;
-  sc.light_tree_nodes = R.light_tree_nodes;
-  vtr::PathState ps{};
-  ps.rng = (uint32_t*)R.rng.ptr;
-  ps.rng_dim = (uint32_t*)R.rng_dim.ptr;
-  ps.throughput = (float*)R.throughput.ptr;
-  ps.conn = (float*)R.conn.ptr;
-  ps.radiance = (float*)R.radiance.ptr;
-  ps.prev_pdf = (float*)R.prev_pdf.ptr;
-  ps.flags = (uint32_t*)R.flags.ptr;
-  ps.signature = (uint64_t*)R.signature.ptr;
-  ps.count = (uint32_t*)R.count.ptr;
-  ps.hits = (uint32_t*)R.hits.ptr;
-  ps.hit_pos = (float*)R.hit_pos.ptr;
-  ps.shadow_path = (uint32_t*)R.shadow_path.ptr;
-  ps.shadow_rays = (float4*)R.shadow_rays.ptr;
-  ps.shadow_contrib = (float*)R.shadow_contrib.ptr;
-  ps.shadow_occ = (uint32_t*)R.shadow_occ.ptr;
-  ps.accum = (float*)R.accum.ptr;
-  ps.nrc_count = (uint32_t*)R.nrc_count.ptr;
-  ps.q_feat = (float*)R.q_feat.ptr;
-  ps.q_thr = (float*)R.q_thr.ptr;
-  ps.q_path = (uint32_t*)R.q_path.ptr;
-  ps.q_rgb = (float*)R.q_rgb.ptr;
-  ps.train_rec = (float*)R.train_rec.ptr;
-  ps.train_n = (uint32_t*)R.train_n.ptr;
-  ps.train_feat = (float*)R.train_feat.ptr;
-  ps.train_tgt = (float*)R.train_tgt.ptr;
-  const uint32_t cap = R.paths;
-  if (f.nrc) {
-    if (vt_status s = engine_nrc_ensure(e)) return s;
-  }
-  HIP_CHECK(e, hipEventRecord(R.ev0, e->stream));
-  HIP_CHECK(e, vtk::launch_zero((uint32_t*)R.count.ptr, 16, e->stream));
-  // Query count only: the previous frame's training step may still read the
-  // record count (word 1), which is cleared after the cache is joined below.
-  if (f.nrc) HIP_CHECK(e, vtk::launch_zero((uint32_t*)R.nrc_count.ptr, 1, e->stream));
-  int cur = 0;
-  ps.queue = (uint32_t*)R.queue[cur].ptr;
-  ps.rays = (float4*)R.rays[cur].ptr;
-  HIP_CHECK(e, vtr::launch_camera(f, ps, e->stream));
-  vtk::TraceParams tp = trace_params(e);
-  // From the second bounce on, a bounce's extension trace runs beside the
-  // previous bounce's shadow trace: each gets half of the persistent grid,
-  // its own ray counter and its own half of the stack-spill area.
-  const uint32_t half_grid = std::max(1u, e->trace_grid / 2u);
-  const size_t spill_half = (size_t)(vtk::kStackCapacity - vtk::kLdsDepth) * half_grid * vtk::kTraceBlock;
-  for (uint32_t v = 0; v <= f.bounce_limit; ++v) {
+  return vt_guarded(e, [&]() -> vt_status {
+    if (!e) return VT_ERR_INVALID_ARGUMENT;
+    if (!fr) return rerr(e, VT_ERR_INVALID_ARGUMENT, "frame is NULL");
+    if (!e->has_scene || !e->render) return rerr(e, VT_ERR_STATE, "no scene is set");
+    if (fr->width == 0 || fr->height == 0 || (uint64_t)fr->width * fr->height > (1u << 28))
+      return rerr(e, VT_ERR_RANGE, "viewport must be non-empty and at most 2^28 pixels");
+    for (int i = 0; i < 16; ++i)
+      if (!std::isfinite(fr->inv_view_proj[i])) return rerr(e, VT_ERR_RANGE, "inverse view-projection must be finite");
+    HIP_CHECK(e, hipSetDevice(e->device));
+    if (vt_status s = ensure_paths(e, fr->width, fr->height)) return s;
+    if (e->opts.max_samples_per_pixel && e->samples_accumulated >= e->opts.max_samples_per_pixel) return VT_OK;
+    if (fr->samples_target && e->samples_accumulated >= fr->samples_target) return VT_OK;
+    RenderState& R = *e->render;
+    vtr::FrameParams f{};
+    std::memcpy(f.inv_vp, fr->inv_view_proj, 64);
+    f.width = fr->width;
+    f.height = fr->height;
+    f.frame_seed = fr->frame_seed;
+    f.frame_index = fr->frame_index;
+    const uint32_t mb = fr->max_bounces ? fr->max_bounces : e->opts.max_bounces;
+    f.bounce_limit = std::max(1u, std::min(mb, 8u));
+    f.ray_bias = R.ray_bias;
+    f.tmin = R.tmin;
+    f.pixels = fr->width * fr->height;
+    f.sobol = e->opts.sampling == 1 ? 1u : 0u;
+    f.sobol_key = vts::frame_key(fr->frame_seed, fr->frame_index);
+    f.sobol_dirs = (const uint16_t*)R.sobol_dirs.ptr;
+    f.nrc = R.nrc_enabled ? 1u : 0u;
+    f.train_paths = R.nrc_enabled ? R.nrc_train_paths : 0u;
+    f.train_vertices = R.nrc_train_vertices;
+    for (int k = 0; k < 3; ++k) f.scene_min[k] = R.scene_min[k], f.scene_inv_extent[k] = R.scene_inv_extent[k];
+    vtr::ShadeScene sc{};
+    sc.positions = (const float4*)e->scene.positions.ptr;
+    sc.normals = (const float4*)e->scene.normals.ptr;
+    sc.indices = (const uint4*)e->scene.indices.ptr;
+    sc.tri_material = (const uint32_t*)e->scene.tri_material.ptr;
+    sc.tri_source = (const uint32_t*)R.tri_source.ptr;
+    sc.materials = (const float*)R.materials.ptr;
+    sc.emitters = (const float*)R.emitters.ptr;
+    sc.instances = (const float*)R.instances.ptr;
+    sc.env_cdf = (const float*)R.env_cdf.ptr;
+    sc.env_pdf = (const float*)R.env_pdf.ptr;
+    sc.emitter_count = R.emitter_count;
+    sc.env_present = R.env_present;
+    sc.env_w = R.env_w;
+    sc.env_h = R.env_h;
+    for (int k = 0; k < 3; ++k) sc.env_radiance[k] = R.env_radiance[k];
+    sc.light_tree = (const float*)R.light_tree.ptr;
+    sc.light_tree_nodes = R.light_tree_nodes;
+    vtr::PathState ps{};
+    ps.rng = (uint32_t*)R.rng.ptr;
+    ps.rng_dim = (uint32_t*)R.rng_dim.ptr;
+    ps.throughput = (float*)R.throughput.ptr;
+    ps.conn = (float*)R.conn.ptr;
+    ps.radiance = (float*)R.radiance.ptr;
+    ps.prev_pdf = (float*)R.prev_pdf.ptr;
+    ps.flags = (uint32_t*)R.flags.ptr;
+    ps.signature = (uint64_t*)R.signature.ptr;
+    ps.count = (uint32_t*)R.count.ptr;
+    ps.hits = (uint32_t*)R.hits.ptr;
+    ps.hit_pos = (float*)R.hit_pos.ptr;
+    ps.shadow_path = (uint32_t*)R.shadow_path.ptr;
+    ps.shadow_rays = (float4*)R.shadow_rays.ptr;
+    ps.shadow_contrib = (float*)R.shadow_contrib.ptr;
+    ps.shadow_occ = (uint32_t*)R.shadow_occ.ptr;
+    ps.accum = (float*)R.accum.ptr;
+    ps.nrc_count = (uint32_t*)R.nrc_count.ptr;
+    ps.q_feat = (float*)R.q_feat.ptr;
+    ps.q_thr = (float*)R.q_thr.ptr;
+    ps.q_path = (uint32_t*)R.q_path.ptr;
+    ps.q_rgb = (float*)R.q_rgb.ptr;
+    ps.train_rec = (float*)R.train_rec.ptr;
+    ps.train_n = (uint32_t*)R.train_n.ptr;
+    ps.train_feat = (float*)R.train_feat.ptr;
+    ps.train_tgt = (float*)R.train_tgt.ptr;
+    const uint32_t cap = R.paths;
+    // The previous frame's cache training step keeps running on its own
+    // stream beside this frame's traces and shading: it reads only the
+    // training features, targets and record count, which this frame rewrites
+    // only after its cache inference has joined the step.  The frame-time
+    // events (ev0 -> ev1) therefore exclude the training step.
+    if (f.nrc) {
+      if (vt_status s = engine_nrc_create(e)) return s;
+    }
+    HIP_CHECK(e, hipEventRecord(R.ev0, e->stream));
+    HIP_CHECK(e, vtk::launch_zero((uint32_t*)R.count.ptr, 16, e->stream));
+    // Query count only: the previous frame's training step may still read the
+    // record count (word 1), which is cleared after the cache is joined below.
+    if (f.nrc) HIP_CHECK(e, vtk::launch_zero((uint32_t*)R.nrc_count.ptr, 1, e->stream));
+    int cur = 0;
     ps.queue = (uint32_t*)R.queue[cur].ptr;
     ps.rays = (float4*)R.rays[cur].ptr;
-    ps.next_queue = (uint32_t*)R.queue[cur ^ 1].ptr;
-    ps.next_rays = (float4*)R.rays[cur ^ 1].ptr;
-    vtk::TraceParams cp = tp;
-    cp.rays = ps.rays;
-    cp.hits = ps.hits;
-    cp.hit_pos = ps.hit_pos;
-    cp.n_ptr = ps.count;
-    cp.sided = 1;  // traceClosest: back faces of single-sided materials are skipped
-    if (e->opts.bvh_traversal == 1) {
-      if (vt_status s = engine_trace_cwbvh(e, ps.rays, cap, ps.count, ps.hits, ps.hit_pos, 1u)) return s;
-    } else {
-      HIP_CHECK(e, vtk::launch_trace(cp, false, false, v == 0 ? e->trace_grid : half_grid, e->stream));
+    HIP_CHECK(e, vtr::launch_camera(f, ps, e->stream));
+    vtk::TraceParams tp = trace_params(e);
+    // From the second bounce on, a bounce's extension trace runs beside the
+    // previous bounce's shadow trace: each gets half of the persistent grid,
+    // its own ray counter and its own half of the stack-spill area.
+    const uint32_t half_grid = std::max(1u, e->trace_grid / 2u);
+    const size_t spill_half = (size_t)(vtk::kStackCapacity - vtk::kLdsDepth) * half_grid * vtk::kTraceBlock;
+    for (uint32_t v = 0; v <= f.bounce_limit; ++v) {
+      ps.queue = (uint32_t*)R.queue[cur].ptr;
+      ps.rays = (float4*)R.rays[cur].ptr;
+      ps.next_queue = (uint32_t*)R.queue[cur ^ 1].ptr;
+      ps.next_rays = (float4*)R.rays[cur ^ 1].ptr;
+      vtk::TraceParams cp = tp;
+      cp.rays = ps.rays;
+      cp.hits = ps.hits;
+      cp.hit_pos = ps.hit_pos;
+      cp.n_ptr = ps.count;
+      cp.sided = 1;  // traceClosest: back faces of single-sided materials are skipped
+      if (e->opts.bvh_traversal == 1) {
+        if (vt_status s = engine_trace_cwbvh(e, ps.rays, cap, ps.count, ps.hits, ps.hit_pos, 1u)) return s;
+      } else {
+        HIP_CHECK(e, vtk::launch_trace(cp, false, false, v == 0 ? e->trace_grid : half_grid, e->stream));
+      }
+      // The previous bounce's shadow rays are resolved (radiance, shadow
+      // queue and count) before this bounce's shading touches them.
+      if (v > 0) HIP_CHECK(e, hipStreamWaitEvent(e->stream, R.ev_join, 0));
+      HIP_CHECK(e, vtr::launch_shade(f, sc, ps, v, cap, e->stream));
+      if (v < f.bounce_limit) {
+        HIP_CHECK(e, hipEventRecord(R.ev_fork, e->stream));
+        HIP_CHECK(e, hipStreamWaitEvent(R.side, R.ev_fork, 0));
+        vtk::TraceParams sp = tp;
+        sp.rays = ps.shadow_rays;
+        sp.occluded = ps.shadow_occ;
+        sp.n_ptr = ps.count + 2;
+        sp.sided = 1;  // traceSurfaceVisibility: back faces of single-sided materials do not block
+        sp.counter = tp.counter + 8;
+        sp.spill = tp.spill + spill_half;
+        HIP_CHECK(e, vtk::launch_trace(sp, true, false, half_grid, R.side));
+        HIP_CHECK(e, vtr::launch_shadow_resolve(ps, cap, R.side));
+        HIP_CHECK(e, vtr::launch_reset_counts(ps, 2u, R.side));  // shadow count
+        HIP_CHECK(e, hipEventRecord(R.ev_join, R.side));
+      }
+      HIP_CHECK(e, vtr::launch_reset_counts(ps, 1u, e->stream));  // extension counts
+      cur ^= 1;
     }
-    // The previous bounce's shadow rays are resolved (radiance, shadow
-    // queue and count) before this bounce's shading touches them.
-    if (v > 0) HIP_CHECK(e, hipStreamWaitEvent(e->stream, R.ev_join, 0));
-    HIP_CHECK(e, vtr::launch_shade(f, sc, ps, v, cap, e->stream));
-    if (v < f.bounce_limit) {
-      HIP_CHECK(e, hipEventRecord(R.ev_fork, e->stream));
-      HIP_CHECK(e, hipStreamWaitEvent(R.side, R.ev_fork, 0));
-      vtk::TraceParams sp = tp;
-      sp.rays = ps.shadow_rays;
-      sp.occluded = ps.shadow_occ;
-      sp.n_ptr = ps.count + 2;
-      sp.sided = 1;  // traceSurfaceVisibility: back faces of single-sided materials do not block
-      sp.counter = tp.counter + 8;
-      sp.spill = tp.spill + spill_half;
-      HIP_CHECK(e, vtk::launch_trace(sp, true, false, half_grid, R.side));
-      HIP_CHECK(e, vtr::launch_shadow_resolve(ps, cap, R.side));
-      HIP_CHECK(e, vtr::launch_reset_counts(ps, 2u, R.side));  // shadow count
-      HIP_CHECK(e, hipEventRecord(R.ev_join, R.side));
+    if (f.nrc) {
+      // Cache queries of the image paths, then the online training step on the
+      // self-training paths' recorded vertices.
+      if (vt_status s = engine_nrc_infer_counted(e, ps.q_feat, ps.nrc_count, ps.q_rgb, f.pixels + f.train_paths))
+        return s;
+      HIP_CHECK(e, vtr::launch_nrc_apply(ps, f.pixels + f.train_paths, e->stream));
     }
-    HIP_CHECK(e, vtr::launch_reset_counts(ps, 1u, e->stream));  // extension counts
-    cur ^= 1;
-  }
-  if (f.nrc) {
-    // Cache queries of the image paths, then the online training step on the
-    // self-training paths' recorded vertices.
-    if (vt_status s = engine_nrc_infer_counted(e, ps.q_feat, ps.nrc_count, ps.q_rgb, f.pixels + f.train_paths))
-      return s;
-    HIP_CHECK(e, vtr::launch_nrc_apply(ps, f.pixels + f.train_paths, e->stream));
-  }
-  HIP_CHECK(e, vtr::launch_accumulate(f, ps, e->stream));
-  if (f.nrc) {
-    HIP_CHECK(e, vtk::launch_zero((uint32_t*)R.nrc_count.ptr + 1, 1, e->stream));
-    HIP_CHECK(e, vtr::launch_nrc_targets(f, ps, e->stream));
-    if (vt_status s = engine_nrc_train_counted(e, ps.train_feat, ps.train_tgt, ps.nrc_count + 1,
-                                               f.train_paths * vtr::kTrainSlots))
-      return s;
-  }
-  HIP_CHECK(e, hipEventRecord(R.ev1, e->stream));
-  e->samples_accumulated += 1;
-  e->frame_timed = true;
-  return VT_OK;
+    HIP_CHECK(e, vtr::launch_accumulate(f, ps, e->stream));
+    if (f.nrc) {
+      HIP_CHECK(e, vtk::launch_zero((uint32_t*)R.nrc_count.ptr + 1, 1, e->stream));
+      HIP_CHECK(e, vtr::launch_nrc_targets(f, ps, e->stream));
+      if (vt_status s = engine_nrc_train_counted(e, ps.train_feat, ps.train_tgt, ps.nrc_count + 1,
+                                                 f.train_paths * vtr::kTrainSlots))
+        return s;
+    }
+    HIP_CHECK(e, hipEventRecord(R.ev1, e->stream));
+    e->samples_accumulated += 1;
+    e->frame_timed = true;
+    return VT_OK;
+  });
 }
 
 vt_status vt_capture(vt_engine* e, float* rgba, uint32_t width, uint32_t height) {
-  if (!e) return VT_ERR_INVALID_ARGUMENT;
-  if (!rgba) return rerr(e, VT_ERR_INVALID_ARGUMENT, "rgba is NULL");
-  if (!e->render || e->render->width != width || e->render->height != height)
-    return rerr(e, VT_ERR_STATE, "no frame of this size has been rendered");
-  HIP_CHECK(e, hipMemcpyAsync(rgba, e->render->accum.ptr, (size_t)width * height * 16,
-                              hipMemcpyDeviceToHost, e->stream));
-  HIP_CHECK(e, hipStreamSynchronize(e->stream));
-  const float inv = e->samples_accumulated ? 1.0f / (float)e->samples_accumulated : 0.0f;
-  for (size_t i = 0; i < (size_t)width * height; ++i) {
-    rgba[4 * i] *= inv;
-    rgba[4 * i + 1] *= inv;
-    rgba[4 * i + 2] *= inv;
-    rgba[4 * i + 3] = 1.0f;
-  }
-  uint32_t err = 0;
-  HIP_CHECK(e, hipMemcpy(&err, e->error_word.ptr, 4, hipMemcpyDeviceToHost));
-  if (err) return rerr(e, VT_ERR_HIP, "traversal stack capacity exceeded during rendering");
-  return VT_OK;
+  return vt_guarded(e, [&]() -> vt_status {
+    if (!e) return VT_ERR_INVALID_ARGUMENT;
+    if (!rgba) return rerr(e, VT_ERR_INVALID_ARGUMENT, "rgba is NULL");
+    if (!e->render || e->render->width != width || e->render->height != height)
+      return rerr(e, VT_ERR_STATE, "no frame of this size has been rendered");
+    // The traversal's error word is read (and cleared) first: after a stack
+    // overflow the capture reports it once and leaves `rgba` untouched, and
+    // later captures succeed again.
+    uint32_t err = 0;
+    HIP_CHECK(e, hipMemcpyAsync(&err, e->error_word.ptr, 4, hipMemcpyDeviceToHost, e->stream));
+    HIP_CHECK(e, hipStreamSynchronize(e->stream));
+    if (err) {
+      HIP_CHECK(e, vtk::launch_zero((uint32_t*)e->error_word.ptr, 1, e->stream));
+      HIP_CHECK(e, hipStreamSynchronize(e->stream));
+      return rerr(e, VT_ERR_HIP, "traversal stack capacity exceeded during rendering");
+    }
+    HIP_CHECK(e, hipMemcpyAsync(rgba, e->render->accum.ptr, (size_t)width * height * 16,
+                                hipMemcpyDeviceToHost, e->stream));
+    HIP_CHECK(e, hipStreamSynchronize(e->stream));
+    const float inv = e->samples_accumulated ? 1.0f / (float)e->samples_accumulated : 0.0f;
+    for (size_t i = 0; i < (size_t)width * height; ++i) {
+      rgba[4 * i] *= inv;
+      rgba[4 * i + 1] *= inv;
+      rgba[4 * i + 2] *= inv;
+      rgba[4 * i + 3] = 1.0f;
+    }
+    return VT_OK;
+  });
 }
 
 vt_status vt_capture_signatures(vt_engine* e, uint64_t* sig, uint32_t width, uint32_t height) {
-  if (!e || !sig) return VT_ERR_INVALID_ARGUMENT;
-  if (!e->render || e->render->width != width || e->render->height != height)
-    return rerr(e, VT_ERR_STATE, "no frame of this size has been rendered");
-  HIP_CHECK(e, hipMemcpyAsync(sig, e->render->signature.ptr, (size_t)width * height * 8,
-                              hipMemcpyDeviceToHost, e->stream));
-  HIP_CHECK(e, hipStreamSynchronize(e->stream));
-  return VT_OK;
+  return vt_guarded(e, [&]() -> vt_status {
+    if (!e || !sig) return VT_ERR_INVALID_ARGUMENT;
+    if (!e->render || e->render->width != width || e->render->height != height)
+      return rerr(e, VT_ERR_STATE, "no frame of this size has been rendered");
+    HIP_CHECK(e, hipMemcpyAsync(sig, e->render->signature.ptr, (size_t)width * height * 8,
+                                hipMemcpyDeviceToHost, e->stream));
+    HIP_CHECK(e, hipStreamSynchronize(e->stream));
+    return VT_OK;
+  });
 }
 
 vt_status vt_reset(vt_engine* e) {
-  if (!e) return VT_ERR_INVALID_ARGUMENT;
-  e->samples_accumulated = 0;
-  if (e->render && e->render->width) {
-    HIP_CHECK(e, vtr::launch_zero_f32((float*)e->render->accum.ptr,
-                                      e->render->width * e->render->height * 4, e->stream));
-    HIP_CHECK(e, hipStreamSynchronize(e->stream));
-  }
-  return VT_OK;
+  return vt_guarded(e, [&]() -> vt_status {
+    if (!e) return VT_ERR_INVALID_ARGUMENT;
+    e->samples_accumulated = 0;
+    if (e->render && e->render->width) {
+      HIP_CHECK(e, vtr::launch_zero_f32((float*)e->render->accum.ptr,
+                                        e->render->width * e->render->height * 4, e->stream));
+      HIP_CHECK(e, hipStreamSynchronize(e->stream));
+    }
+    return VT_OK;
+  });
 }
 
 vt_status vt_frame_time(vt_engine* e, float* ms) {
-  if (!e || !ms) return VT_ERR_INVALID_ARGUMENT;
-  *ms = 0;
-  if (!e->render || !e->frame_timed) return VT_OK;
-  HIP_CHECK(e, hipEventSynchronize(e->render->ev1));
-  HIP_CHECK(e, hipEventElapsedTime(ms, e->render->ev0, e->render->ev1));
-  e->last_frame_ms = *ms;
-  return VT_OK;
+  return vt_guarded(e, [&]() -> vt_status {
+    if (!e || !ms) return VT_ERR_INVALID_ARGUMENT;
+    *ms = 0;
+    if (!e->render || !e->frame_timed) return VT_OK;
+    HIP_CHECK(e, hipEventSynchronize(e->render->ev1));
+    HIP_CHECK(e, hipEventElapsedTime(ms, e->render->ev0, e->render->ev1));
+    e->last_frame_ms = *ms;
+    return VT_OK;
+  });
 }
 
 vt_status vt_nrc_render_config(vt_engine* e, int32_t enabled, uint32_t train_paths) {
-  if (!e) return VT_ERR_INVALID_ARGUMENT;
-  if (enabled && (train_paths == 0 || train_paths > (1u << 22)))
-    return rerr(e, VT_ERR_RANGE, "train_paths must be in [1, 2^22]");
-  if (!e->render) return rerr(e, VT_ERR_STATE, "no scene is set");
-  RenderState& R = *e->render;
-  R.nrc_enabled = enabled != 0;
-  if (enabled) R.nrc_train_paths = train_paths;
-  R.width = R.height = 0;  // re-size path state on the next frame
-  e->samples_accumulated = 0;
-  return VT_OK;
+  return vt_guarded(e, [&]() -> vt_status {
+    if (!e) return VT_ERR_INVALID_ARGUMENT;
+    if (enabled && (train_paths == 0 || train_paths > (1u << 22)))
+      return rerr(e, VT_ERR_RANGE, "train_paths must be in [1, 2^22]");
+    if (!e->render) return rerr(e, VT_ERR_STATE, "no scene is set");
+    RenderState& R = *e->render;
+    R.nrc_enabled = enabled != 0;
+    if (enabled) R.nrc_train_paths = train_paths;
+    R.width = R.height = 0;  // re-size path state on the next frame
+    e->samples_accumulated = 0;
+    return VT_OK;
+  });
 }
 
 vt_status vt_nrc_teacher_vertices(vt_engine* e, uint32_t vertices) {
-  if (!e) return VT_ERR_INVALID_ARGUMENT;
-  if (vertices > vtr::kTrainSlots)
-    return rerr(e, VT_ERR_RANGE, "teacher vertices must be in [0, 4] (0: full-length training paths)");
-  if (!e->render) return rerr(e, VT_ERR_STATE, "no scene is set");
-  e->render->nrc_train_vertices = vertices;
-  e->samples_accumulated = 0;
-  return VT_OK;
+  return vt_guarded(e, [&]() -> vt_status {
+    if (!e) return VT_ERR_INVALID_ARGUMENT;
+    if (vertices > vtr::kTrainSlots)
+      return rerr(e, VT_ERR_RANGE, "teacher vertices must be in [0, 4] (0: full-length training paths)");
+    if (!e->render) return rerr(e, VT_ERR_STATE, "no scene is set");
+    e->render->nrc_train_vertices = vertices;
+    e->samples_accumulated = 0;
+    return VT_OK;
+  });
 }
 
 vt_status vt_light_tree(vt_engine* e, float* nodes, uint32_t capacity, uint32_t* count) {
-  if (!e || !count) return VT_ERR_INVALID_ARGUMENT;
-  if (!e->render) return rerr(e, VT_ERR_STATE, "no scene is set");
-  const RenderState& R = *e->render;
-  *count = R.light_tree_nodes;
-  if (nodes && capacity) {
-    if (capacity < R.light_tree_nodes) return rerr(e, VT_ERR_RANGE, "capacity below the light tree's node count");
-    std::memcpy(nodes, R.light_tree_host.data(), (size_t)R.light_tree_nodes * vtl::kNodeFloats * sizeof(float));
-  }
-  return VT_OK;
+  return vt_guarded(e, [&]() -> vt_status {
+    if (!e || !count) return VT_ERR_INVALID_ARGUMENT;
+    if (!e->render) return rerr(e, VT_ERR_STATE, "no scene is set");
+    const RenderState& R = *e->render;
+    *count = R.light_tree_nodes;
+    if (nodes && capacity) {
+      if (capacity < R.light_tree_nodes) return rerr(e, VT_ERR_RANGE, "capacity below the light tree's node count");
+      std::memcpy(nodes, R.light_tree_host.data(), (size_t)R.light_tree_nodes * vtl::kNodeFloats * sizeof(float));
+    }
+    return VT_OK;
+  });
 }
 
 vt_status vt_render_info(vt_engine* e, uint32_t* emitter_count, float* ray_bias) {
-  if (!e || !e->render) return VT_ERR_STATE;
-  if (emitter_count) *emitter_count = e->render->emitter_count;
-  if (ray_bias) *ray_bias = e->render->ray_bias;
-  return VT_OK;
+  return vt_guarded(e, [&]() -> vt_status {
+    if (!e || !e->render) return VT_ERR_STATE;
+    if (emitter_count) *emitter_count = e->render->emitter_count;
+    if (ray_bias) *ray_bias = e->render->ray_bias;
+    return VT_OK;
+  });
 }
 
 }  // extern "C"

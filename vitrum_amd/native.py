@@ -169,6 +169,12 @@ def _prim_structs(primitives):
     return arr, keep
 
 
+def inject_failure(flags):
+    """Test hook without an engine: bit 1 makes the next vt_pack_scene on this
+    thread throw std::bad_alloc inside the library."""
+    check(lib().vt_debug_inject_failure(None, C.c_uint32(flags)))
+
+
 def pack_scene(primitives, threads=0):
     """packSceneFromCore through the product builder (vt_pack_scene)."""
     arr, _keep = _prim_structs(primitives)
@@ -243,13 +249,21 @@ class Engine:
     """Thin owner of a vt_engine (one per GPU device)."""
 
     def __init__(self, device: int = 0, max_bounces: int = 8, sampling: int = 0, light_selection: int = 0,
-                 bvh_traversal: int = 0):
+                 bvh_traversal: int = 0, build_threads: int = 16):
         """sampling: 0 PCG, 1 Owen-scrambled Sobol; light_selection: 0 light
         tree (two or more lights), 1 uniform; bvh_traversal: 0 binary BVH,
-        1 cwbvh-closest (8-wide compressed BVH for closest hits)."""
+        1 cwbvh-closest (8-wide compressed BVH for closest hits);
+        build_threads: host threads of the scene builder (0: all)."""
         self._h = C.c_void_p()
         opts = Options(max_bounces, 0, sampling, light_selection, bvh_traversal)
         check(lib().vt_create(C.c_int32(device), C.byref(opts), C.byref(self._h)))
+        check(lib().vt_set_build_threads(self._h, C.c_uint32(build_threads)), self._h)
+
+    def inject_failure(self, flags):
+        """Test hook: make the next vt_set_scene fail at the chosen step (bit
+        1 packing throws std::bad_alloc, 2 the shading-data upload fails, 4
+        its preparation throws)."""
+        check(lib().vt_debug_inject_failure(self._h, C.c_uint32(flags)), self._h)
 
     def close(self):
         if self._h:
