@@ -10,6 +10,7 @@ from HIP events recorded on the engine stream that runs the kernel.
 Prints one JSON line.
 """
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -28,17 +29,28 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def measured_traffic(kernel_key):
+def kernel_key(tuning):
+    """Identifies the traversal kernel a traffic measurement belongs to: a
+    hash of the kernel's source files and the launch tuning."""
+    h = hashlib.sha256()
+    for f in ("traverse.hip", "traverse.h"):
+        with open(os.path.join(ROOT, "vitrum_amd", "csrc", f), "rb") as fh:
+            h.update(fh.read())
+    return f"src={h.hexdigest()[:16]} {tuning}"
+
+
+def measured_traffic(key):
     """HBM bytes per launch of the closest-hit kernel from the committed
     rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE in KiB per dispatch,
-    each scaled by the reading of a plain copy of known size), or None."""
+    each scaled by the reading of a plain copy of known size), or None when
+    that measurement was made on other kernel sources or tuning."""
     path = os.path.join(ROOT, "profiles", "trace_closest_spheres1M.traffic.json")
     try:
         with open(path) as fh:
             t = json.load(fh)
     except (OSError, ValueError):
         return None
-    if t.get("kernel_key") != kernel_key:
+    if t.get("kernel_key") != key:
         return None
     return int(round((t["fetch_kib"] * t["fetch_scale"] + t["write_kib"] * t["write_scale"]) * 1024))
 
@@ -61,27 +73,55 @@ def _timed_passes(fn, rays, seconds):
     return n, passes, time.perf_counter() - t0
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_threads():
+    """The CPUs this process may run on (its affinity mask), at most 16: on a
+    shared GPU host one GPU's process gets a 16-CPU share whatever the machine
+    reports."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
 def cpu_baseline(scene, rays, seconds):
     """The reference's CPU closest hit -- the shared-bvh wide-BVH walk
     (intersectCompressedWideBvhFirstHit over tlasIntersect candidates, binary64),
-    restated in C in the oracle -- on a bounded sample of the same rays, on the
-    host's cores; the oracle's f32 watertight traversal (the GPU kernel's own
-    semantics) is timed beside it."""
+    restated in C in the oracle -- on bounded samples of the same rays, on one
+    host thread and on T threads; the oracle's f32 watertight traversal (the
+    GPU kernel's own semantics) is timed the same way beside it."""
     from oracle import oracle as O
     prims = [dict(p, material_id=i) for i, p in enumerate(scene["primitives"])]
     ps = O.PackedScene(O.pack_scene(prims))
-    threads = min(16, os.cpu_count() or 1)
+    threads = host_threads()
     t0 = time.perf_counter()
     cw, roots = O.scene_cwbvh(ps)
     collapse_s = time.perf_counter() - t0
-    n, passes, dt = _timed_passes(lambda r: O.cwbvh_trace_closest(ps, cw, roots, r, threads=threads),
-                                  rays, seconds / 2)
-    n2, _, dt2 = _timed_passes(lambda r: O.trace_closest(ps, r, threads=threads), rays, seconds / 2)
+    walk = lambda t: (lambda r: O.cwbvh_trace_closest(ps, cw, roots, r, threads=t))  # noqa: E731
+    tight = lambda t: (lambda r: O.trace_closest(ps, r, threads=t))  # noqa: E731
+    n, passes, dt = _timed_passes(walk(threads), rays, seconds * 0.3)
+    n1, _, dt1 = _timed_passes(walk(1), rays, seconds * 0.2)
+    n2, _, dt2 = _timed_passes(tight(threads), rays, seconds * 0.3)
+    n3, _, dt3 = _timed_passes(tight(1), rays, seconds * 0.2)
     return {"value": round(n / dt / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(),
             "sample": f"shared-bvh CPU walk (binary64, 8-wide collapsed BVH, {collapse_s:.2f} s collapse): "
                       f"{n} rays = {passes} pass(es) over the leading rays of the batch (primary rows "
-                      f"first), {dt:.1f} s on {threads} threads",
-            "watertight_f32_value": round(n2 / dt2 / 1e6, 4)}
+                      f"first), {dt:.1f} s on {threads} threads; 1 thread: {n1} rays in {dt1:.1f} s",
+            "value_1_thread": round(n1 / dt1 / 1e6, 4),
+            "watertight_f32_value": round(n2 / dt2 / 1e6, 4),
+            "watertight_f32_value_1_thread": round(n3 / dt3 / 1e6, 4)}
 
 
 # MLP FLOPs per sample: forward 2 (4*64*64 + 64*3); backward through
@@ -89,9 +129,6 @@ def cpu_baseline(scene, rays, seconds):
 NRC_FWD_FLOPS = 2 * (4 * 64 * 64 + 64 * 3)
 NRC_TRAIN_FLOPS = NRC_FWD_FLOPS + 2 * (3 * 64 + 3 * 64 * 64) + 2 * (4 * 64 * 64 + 3 * 64)
 BF16_PEAK_TFLOPS = 2500.0
-# Identifies the traversal kernel configuration the committed traffic
-# measurement belongs to (child order, refill threshold, steps per refill).
-KERNEL_KEY = "trace_kernel<closest> nodes=4-wide leaves=lane-compacted near_first=1 first_fill=static refill=8 steps=16 defer=12 lds=20 waves=4"
 
 
 def nrc_mlp(eng, reps=10):
@@ -147,22 +184,29 @@ def _accumulate(eng, ivp, width, height, first_frame, frames, seed):
     return eng.capture(width, height)[..., :3].astype(np.float64)
 
 
+def _tile_means(x, t=16):
+    h, w = (x.shape[0] // t) * t, (x.shape[1] // t) * t
+    return x[:h, :w].reshape(h // t, t, w // t, t).mean(axis=(1, 3))
+
+
 def time_to_rmse(W, width, height, ref_spp, target, train_paths, max_frames, teacher_vertices=4):
-    """Plain vs cache-terminated path tracer on the Cornell scene: GPU time to
-    a relative RMSE `target` against the converged image.
+    """Plain vs cache-terminated path tracer on the Cornell scene: wall time
+    until the image's relative RMSE first reaches `target`, measured.
 
     The reference is two independent plain halves A and B (ref_spp/2 samples
-    each); its own noise is removed from the error: with R = (A + B)/2,
+    each).  Its own noise is removed from the error: with R = (A + B)/2,
     E[(x - R)^2] = MSE(x) + Var(R) and Var(R) = E[(A - B)^2] / 4, so
-    MSE(x) = mean((x - R)^2) - mean((A - B)^2) / 4 (per-pixel luminance).
-    Each estimator is rendered for up to max_frames samples; its measured
-    curve e^2(N) is fitted by b^2 + v / N (b = bias, v = per-sample variance,
-    least squares over N >= 8) and the time to `target` is
-    N* x (mean frame wall time), N* = v / (target^2 - b^2); None when the bias
-    alone exceeds the target.  Frame times are host wall-clock intervals over
-    frames enqueued back to back, each interval ending with a full engine
-    synchronisation (the cache's training step included) before an image is
-    read back; the read-backs are outside the intervals."""
+    MSE(x) = mean((x - R)^2) - mean((A - B)^2) / 4 (per-pixel luminance,
+    averaged over the image; the correction's own sampling error over 2M
+    pixels is reported as `error_uncertainty`).  Each estimator renders frames
+    back to back; its error is checked on a geometric schedule and, once it is
+    within 1.5x of the target, every `fine_step` frames; the reported time is
+    the accumulated wall time at the first check at or below the target (no
+    extrapolation).  Timed intervals end with a full engine synchronisation
+    (the cache's training step included); image read-backs and error checks
+    are outside them.  A least-squares fit e^2 = b^2 + v/N is reported beside
+    the measurement, with b = 0 for the plain tracer (unbiased by
+    construction)."""
     from vitrum_amd import native
     seed = 0x9E3779B9
     eng = native.Engine(0)
@@ -176,52 +220,71 @@ def time_to_rmse(W, width, height, ref_spp, target, train_paths, max_frames, tea
     ref_b = _accumulate(eng, ivp, width, height, 60_000_000, half, seed)
     log(f"reference {2 * half} spp in {time.perf_counter() - t0:.0f} s")
     ref = 0.5 * (ref_a + ref_b)
-    mean_lum = float(np.mean(lum(ref)))
-    ref_var = float(np.mean((lum(ref_a) - lum(ref_b)) ** 2) / 4.0)
+    la, lb, lr = lum(ref_a), lum(ref_b), lum(ref)
+    mean_lum = float(np.mean(lr))
+    dd = (la - lb) ** 2 / 4.0
+    ref_var = float(np.mean(dd))
+    # standard error of the noise correction (per-pixel terms are independent)
+    corr_se = float(np.std(dd) / np.sqrt(dd.size))
     out = {"scene": "cornell (config-1 box, 32,768 tris)", "width": width, "height": height,
            "reference_spp": 2 * half, "target_rel_rmse": target, "train_paths_per_frame": train_paths,
            "teacher_vertices": teacher_vertices,
            "reference_rel_rmse": round(float(np.sqrt(ref_var)) / mean_lum, 5),
-           "method": "noise-corrected MSE vs split-half reference; fit e^2 = b^2 + v/N; frame time = wall clock, frames back to back"}
+           "error_uncertainty": round(corr_se / (2 * target * mean_lum ** 2), 6),
+           "method": "first measured crossing of the noise-corrected relative RMSE vs a split-half plain "
+                     "reference; frame time = wall clock, frames back to back"}
     for name, nrc in (("plain", False), ("nrc", True)):
         eng.nrc_render_config(nrc, train_paths)
         if nrc:
             eng.nrc_reset(seed=11)
             eng.nrc_teacher_vertices(teacher_vertices)
         eng.reset()
-        gpu_ms, curve = 0.0, []
-        check = 1
+        wall_ms, curve, crossing = 0.0, [], None
+        check, fine = 1, None
         eng.synchronize()
         t_start = time.perf_counter()
+        img = None
         for f in range(max_frames):
             eng.render_frame(ivp, width, height, f, seed, 8)
             if f + 1 == check or f + 1 == max_frames:
                 eng.synchronize()
-                gpu_ms += (time.perf_counter() - t_start) * 1e3
+                wall_ms += (time.perf_counter() - t_start) * 1e3
                 img = eng.capture(width, height)[..., :3].astype(np.float64)
-                mse = float(np.mean((lum(img) - lum(ref)) ** 2)) - ref_var
+                mse = float(np.mean((lum(img) - lr) ** 2)) - ref_var
                 e = float(np.sqrt(max(mse, 0.0))) / mean_lum
-                curve.append([f + 1, round(gpu_ms, 2), round(e, 5)])
-                log(f"{name}: {f + 1} spp, {gpu_ms:.0f} ms, rel RMSE {e:.4f}")
-                check = max(check + 1, int(check * 1.5))
+                curve.append([f + 1, round(wall_ms, 2), round(e, 5)])
+                if len(curve) % 8 == 1 or e <= target:
+                    log(f"{name}: {f + 1} spp, {wall_ms:.0f} ms, rel RMSE {e:.4f}")
+                if e <= target:
+                    crossing = (f + 1, wall_ms)
+                    break
+                if fine is None and e < 1.5 * target:
+                    fine = max(8, (f + 1) // 64)
+                check = check + fine if fine else max(check + 1, int(check * 1.5))
                 eng.synchronize()
                 t_start = time.perf_counter()
+        frames = curve[-1][0]
+        entry = {"frames": frames, "frame_ms": round(wall_ms / frames, 3), "curve": curve[::max(1, len(curve) // 24)]}
+        entry["spp_at_target"] = crossing[0] if crossing else None
+        entry["time_to_target_ms"] = round(crossing[1], 1) if crossing else None
         pts = np.array([[n, e] for n, _, e in curve if n >= 8], np.float64)
-        frame_ms = gpu_ms / max_frames
-        entry = {"frames": max_frames, "frame_ms": round(frame_ms, 3), "curve": curve}
         if pts.shape[0] >= 2:
-            A = np.stack([np.ones(pts.shape[0]), 1.0 / pts[:, 0]], axis=1)
-            (b2, v), *_ = np.linalg.lstsq(A, pts[:, 1] ** 2, rcond=None)
-            b2 = max(float(b2), 0.0)
+            if nrc:
+                A = np.stack([np.ones(pts.shape[0]), 1.0 / pts[:, 0]], axis=1)
+                (b2, v), *_ = np.linalg.lstsq(A, pts[:, 1] ** 2, rcond=None)
+                b2 = max(float(b2), 0.0)
+            else:
+                b2, v = 0.0, float(np.sum(pts[:, 1] ** 2 / pts[:, 0]) / np.sum(1.0 / pts[:, 0] ** 2))
             entry["fit_bias"] = round(float(np.sqrt(b2)), 5)
             entry["fit_sample_variance"] = round(float(v), 5)
-            if target ** 2 > b2 and v > 0:
-                n_star = v / (target ** 2 - b2)
-                entry["spp_to_target"] = round(float(n_star), 1)
-                entry["time_to_target_ms"] = round(float(n_star * frame_ms), 1)
-            else:
-                entry["spp_to_target"] = None
-                entry["time_to_target_ms"] = None
+        # bias at the crossing: signed image-mean luminance difference and the
+        # RMS of 16x16-tile mean differences (noise shrinks 256x in the tiles;
+        # the plain tracer's figure is the noise floor of this statistic)
+        d = (lum(img) - lr) / mean_lum
+        tiles = _tile_means(d)
+        entry["mean_rel_bias"] = round(float(d.mean()), 6)
+        entry["tile16_rel_diff_rms"] = round(float(np.sqrt(np.mean(tiles ** 2))), 6)
+        entry["tile16_rel_diff_p99"] = round(float(np.quantile(np.abs(tiles), 0.99)), 6)
         out[name] = entry
     p, n = out["plain"].get("time_to_target_ms"), out["nrc"].get("time_to_target_ms")
     out["nrc_speedup"] = round(p / n, 3) if p and n else None
@@ -249,13 +312,14 @@ def main():
                     help="traversal node width: 4-wide trees collapsed from the binary BVH (default) or binary")
     ap.add_argument("--no-nrc-mlp", action="store_true", help="skip the cache MLP kernel measurement")
     ap.add_argument("--no-render-rmse", action="store_true",
-                    help="skip the plain vs cache time-to-RMSE measurement (about a minute of GPU time)")
+                    help="skip the plain vs cache time-to-RMSE measurement (about three minutes of GPU time)")
     ap.add_argument("--rmse-size", default="1920x1080")
-    ap.add_argument("--rmse-ref-spp", type=int, default=1024)
+    ap.add_argument("--rmse-ref-spp", type=int, default=4096)
     ap.add_argument("--rmse-target", type=float, default=1e-2)
     ap.add_argument("--nrc-teacher-vertices", type=int, default=4,
                     help="shaded vertices of a self-training path before its terminal cache query (0: full length)")
-    ap.add_argument("--rmse-max-frames", type=int, default=192)
+    ap.add_argument("--rmse-max-frames", type=int, default=16384,
+                    help="frames after which an estimator that has not reached the target stops")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -269,6 +333,9 @@ def main():
     if args.refill_min or args.steps_per_refill or args.near_first >= 0 or args.defer_min:
         eng.set_trace_tuning(args.refill_min or 8, args.steps_per_refill or 16,
                              1 if args.near_first < 0 else args.near_first, args.defer_min or 12)
+    key = kernel_key(f"node_width={args.node_width} refill={args.refill_min or 8} "
+                     f"steps={args.steps_per_refill or 16} near_first={1 if args.near_first < 0 else args.near_first} "
+                     f"defer={args.defer_min or 12}")
     if args.calibrate_copy_mib:
         nb = args.calibrate_copy_mib << 20
         a, b = eng.alloc(nb), eng.alloc(nb)
@@ -355,7 +422,8 @@ def main():
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4),
-            "traffic": measured_traffic(KERNEL_KEY),
+            "traffic": measured_traffic(key),
+            "kernel_key": key,
             "bytes_per_launch": int(ref_bytes),
         },
         "cpu_baseline": None,
