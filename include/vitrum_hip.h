@@ -105,6 +105,35 @@ typedef struct vt_environment {
   float radiance[3];
 } vt_environment;
 
+/* Scene.emitters (packages/core/src/scene/emitters.ts:7-114): the analytic
+ * emitter kinds, packed as emitterPacking.ts:600-760 packs them.  Fields a
+ * kind does not use are ignored.  radiance = f32(color) x f32(intensity). */
+typedef enum vt_emitter_kind {
+  VT_EMITTER_DIRECTIONAL = 0, /* direction (the way the light travels), angular_diameter */
+  VT_EMITTER_POINT = 1,       /* position, distance (0: no cutoff), decay (2: inverse square) */
+  VT_EMITTER_SPOT = 2,        /* position, direction, angle (half cone), penumbra, distance, decay */
+  VT_EMITTER_RECT_AREA = 3,   /* position, u_axis, v_axis (half extents; u x v is the emitting side) */
+  VT_EMITTER_DISC_AREA = 4    /* position, normal, radius */
+} vt_emitter_kind;
+
+typedef struct vt_emitter {
+  uint32_t kind;
+  float color[3];
+  float intensity;
+  uint32_t cast_shadow; /* 0: no shadow rays (castShadow: false) */
+  float position[3];
+  float direction[3];
+  float normal[3];
+  float u_axis[3];
+  float v_axis[3];
+  float radius;
+  float angle;
+  float penumbra;
+  float distance;
+  float decay;
+  float angular_diameter;
+} vt_emitter;
+
 typedef struct vt_scene_desc {
   const vt_primitive *primitives;
   uint32_t primitive_count;
@@ -113,6 +142,8 @@ typedef struct vt_scene_desc {
   vt_environment environment;
   float scene_radius; /* ray-origin bias scale (sceneScalePolicy.ts:83-123);
                          0 = derive from the scene bounds */
+  const vt_emitter *emitters; /* analytic emitters; mesh-area light comes from emissive materials */
+  uint32_t emitter_count;
 } vt_scene_desc;
 
 /* ---- engine lifecycle ---------------------------------------------------- */
@@ -194,6 +225,30 @@ vt_status vt_render_frame(vt_engine *eng, const vt_frame *frame);
  * alpha 1.  Blocking.  If a traversal ran out of stack since the last check,
  * returns VT_ERR_HIP once, clears the condition and leaves `rgba` unwritten. */
 vt_status vt_capture(vt_engine *eng, float *rgba, uint32_t width, uint32_t height);
+
+/* FrameQualitySettings.tonemap / exposure / outputColorSpace
+ * (packages/core/src/frame.ts:32-105) as the present pass applies them
+ * (packages/pt-webgpu/src/wgsl/present.wgsl.ts:58-95). */
+typedef struct vt_present {
+  uint32_t tonemap;     /* 0 aces (default), 1 agx, 2 reinhard, 3 linear, 4 none */
+  float exposure;       /* linear multiplier applied first, >= 0 (default 1) */
+  uint32_t color_space; /* 0 srgb (IEC 61966-2-1 OETF, default), 1 linear */
+} vt_present;
+/* captureFrame({colorSpace: 'output'}) (packages/core/src/engine/index.ts:
+ * 466; pt-webgpu index.ts:2019-2035, which reads the present pass's
+ * rgba16float texture): the running mean tonemapped, OETF-encoded, clamped to
+ * [0, 65504] and rounded to binary16, returned as float32 RGBA (alpha 1).
+ * Blocking. */
+vt_status vt_capture_output(vt_engine *eng, float *rgba, uint32_t width, uint32_t height,
+                            const vt_present *present);
+/* FrameRendered.normalDepth / albedo / variance (packages/core/src/frame.ts:
+ * 350-421), written by accumulateFrame (packages/pt-webgpu/src/wgsl/
+ * pathTrace/kernel.wgsl.ts:920-960).  which 0: first-hit shading normal *
+ * 0.5 + 0.5 and hit distance (a miss gives (0.5, 1, 0.5, 0)); 1: first-hit
+ * base colour, 1 (a miss gives zeros); both from the latest frame.  2: the
+ * luminance variance of the accumulated samples, max(0, E[L^2] - E[L]^2), in
+ * the first of the four floats.  4 floats per pixel.  Blocking. */
+vt_status vt_capture_aux(vt_engine *eng, uint32_t which, float *out, uint32_t width, uint32_t height);
 /* Engine.reset (packages/core/src/engine/index.ts:256): clears accumulation. */
 vt_status vt_reset(vt_engine *eng);
 /* GPU milliseconds of the last vt_render_frame (HIP events on the engine

@@ -82,6 +82,7 @@ static uint32_t pcg_next(orng *r) {
   return orc_pcg_next(&r->s);
 }
 static double rand01(orng *s) { return (double)(pcg_next(s) >> 8) / 16777216.0; }
+static double rand01_(void *s) { return rand01((orng *)s); }
 /* Polar disc coordinates sqrt(u1)cos(2 pi u2), sqrt(u1)sin(2 pi u2) rounded
  * once to binary32: the reference draws these in f32, and the product
  * evaluates them in binary64 and rounds once, so both sides start a sampled
@@ -252,29 +253,6 @@ static double bsdf_pdf(const mat_t *m, d3 n, d3 wo, d3 wi) {
   return dp * ndl * INV_PI + sp * pspec;
 }
 
-/* buildOnb (bsdf.wgsl.ts:1632-1640) */
-static void build_onb(d3 n, d3 *t, d3 *b) {
-  d3 up = fabs(n.y) > 0.999 ? D3(1, 0, 0) : D3(0, 1, 0);
-  d3 c = cross(up, n);
-  *t = scl(c, 1.0 / sqrt(dot(c, c)));
-  *b = cross(n, *t);
-}
-
-/* sampleGgxVndfTangent (bsdf.wgsl.ts:1710-1735) */
-static d3 vndf_sample(d3 wo, double alpha, orng *rng) {
-  d3 vh = safe_normalize(D3(alpha * wo.x, alpha * wo.y, wo.z));
-  double lensq = vh.x * vh.x + vh.y * vh.y;
-  d3 t1 = lensq > 1e-10 ? scl(D3(-vh.y, vh.x, 0), 1.0 / sqrt(lensq)) : D3(1, 0, 0);
-  d3 t2 = cross(vh, t1);
-  double u1 = rand01(rng), u2 = rand01(rng);
-  double a, b;
-  disc_polar(u1, u2, &a, &b);
-  double s = 0.5 * (1.0 + vh.z);
-  b = (1.0 - s) * sqrt(maxd(0, 1.0 - a * a)) + s * b;
-  d3 nh = add(add(scl(t1, a), scl(t2, b)), scl(vh, sqrt(maxd(0, 1.0 - a * a - b * b))));
-  return safe_normalize(D3(alpha * nh.x, alpha * nh.y, maxd(1e-6, nh.z)));
-}
-
 /* ---- camera (kernelCore.wgsl.ts:22-26; common.wgsl.ts:412-469) --------- */
 
 
@@ -344,6 +322,91 @@ static double area_to_solid_angle(double dist, double cos_l, double area) {
   return dist * dist / (cos_l * area);
 }
 
+/* ---- analytic emitters (kernel.wgsl.ts:903-918, 1198-1497) --------------- */
+
+/* the area element of a rect (half axes u, v: 4|u x v|) or disc (the affine
+ * image of the unit disc: pi |u x v|) and its unit normal u x v */
+static int analytic_area(d3 u, d3 v, int disc, d3 *normal, double *area) {
+  d3 c = cross(u, v);
+  double l = sqrt(dot(c, c));
+  if (!(l > 0)) return 0;
+  *normal = scl(c, 1.0 / l);
+  *area = (disc ? PI : 4.0) * l;
+  return 1;
+}
+/* ptDirectionalConeIsDelta / ptDirectionalConeSinCos (material.wgsl.ts:
+ * 208-235) and sampleDirectionalCone (kernel.wgsl.ts:903-918) */
+static int cone_is_delta(double ad) { return !(ad > 0) || sin(ad * 0.25) < 7.666467e-20; }
+static d3 sample_directional_cone(d3 axis, double ad, orng *rng) {
+  d3 d = safe_normalize(axis);
+  if (cone_is_delta(ad)) return d;
+  double xi1 = rand01(rng), xi2 = rand01(rng);
+  double sq = sin(ad * 0.25), omc = 2.0 * sq * sq;
+  double q = (1.0 - clampd(xi1, 0, 1)) * omc;
+  double st = sqrt(maxd(q * (2.0 - q), 0)), ct = sqrt(maxd(1.0 - st * st, 0));
+  double phi = 6.28318530718 * xi2;
+  d3 tx = fabs(d.x) > 0.9 ? D3(0, 1, 0) : D3(1, 0, 0);
+  d3 by = safe_normalize(cross(d, tx));
+  d3 bx = cross(by, d);
+  return safe_normalize(add(add(scl(bx, st * cos(phi)), scl(by, st * sin(phi))), scl(d, ct)));
+}
+/* pointSpotDistanceAttenuation (material.wgsl.ts:2176-2204) */
+static double point_attenuation(double dist, double cutoff, double decay) {
+  if (!(dist > 0)) return 0;
+  double att = 1.0;
+  if (decay > 0.01) att = 1.0 / pow(dist, decay);
+  if (cutoff > 0) att *= clampd(1.0 - pow(dist / cutoff, 4.0), 0, 1);
+  return att;
+}
+static double smoothstep_d(double e0, double e1, double x) {
+  double t = clampd((x - e0) / (e1 - e0), 0, 1);
+  return t * t * (3.0 - 2.0 * t);
+}
+/* concentricDiscSample (kernelCore.wgsl.ts:73-85) */
+static void concentric_disc(double a, double b, double *x, double *y) {
+  if (a == 0 && b == 0) {
+    *x = *y = 0;
+    return;
+  }
+  double r, phi;
+  if (fabs(a) >= fabs(b)) {
+    r = a;
+    phi = (PI / 4.0) * (b / a);
+  } else {
+    r = b;
+    phi = PI / 2.0 - (PI / 4.0) * (a / b);
+  }
+  *x = r * cos(phi);
+  *y = r * sin(phi);
+}
+/* intersectRectAreaLightRay (connect.wgsl.ts:395-446), binary64 */
+static int intersect_rect_light(const float *q, d3 o, d3 d, double tmin, double *t_out, double *pdf_out) {
+  d3 c = D3(q[0], q[1], q[2]), u = D3(q[4], q[5], q[6]), v = D3(q[8], q[9], q[10]);
+  int disc = fabs(q[15] - 1.0f) < 0.5f;
+  d3 n;
+  double area;
+  if (!analytic_area(u, v, disc, &n, &area)) return 0;
+  double den = dot(n, d);
+  if (den == 0) return 0;
+  double t = dot(n, sub(c, o)) / den;
+  if (t <= tmin) return 0;
+  d3 rel = sub(add(o, scl(d, t)), c);
+  /* coordinates of rel in (u, v): least squares through the normal equations */
+  double uu = dot(u, u), uv = dot(u, v), vv = dot(v, v), ru = dot(rel, u), rv = dot(rel, v);
+  double det = uu * vv - uv * uv;
+  if (det == 0) return 0;
+  double a = (ru * vv - rv * uv) / det, b = (rv * uu - ru * uv) / det;
+  int inside = disc ? (a * a + b * b <= 1.0) : (fabs(a) <= 1.0 && fabs(b) <= 1.0);
+  if (!inside) return 0;
+  double cl = maxd(dot(n, scl(d, -1)), 0);
+  if (cl <= 0) return 0;
+  double pdf = area_to_solid_angle(t, cl, area);
+  if (!(pdf > 0)) return 0;
+  *t_out = t;
+  *pdf_out = pdf;
+  return 1;
+}
+
 /* ---- environment (connect.wgsl.ts:126-219; connectCore.wgsl.ts:46-61) --- */
 
 typedef struct {
@@ -402,6 +465,16 @@ typedef struct orc_render_desc {
   uint32_t sampling; /* 0 = PCG, 1 = Owen-scrambled Sobol */
   const float *light_tree;        /* packed light tree (light_tree.c), or NULL */
   uint32_t light_tree_nodes;      /* 0 = uniform light selection */
+  /* NULL, or 8 doubles per rendered pixel: the first hit's f32 shading
+   * normal * 0.5 + 0.5, its distance from the camera ray's origin, base
+   * colour, 1 -- (0.5, 1, 0.5, 0, 0, 0, 0, 0) for a miss (accumulateFrame,
+   * kernel.wgsl.ts:946-957). */
+  double *aux;
+  /* Analytic emitters, 16 floats each in light-selection order (directional,
+   * point, spot, rect/disc; the records of the product's render.h): they come
+   * before the mesh-area emitters in the selection index. */
+  const float *lights;
+  uint32_t n_dir, n_point, n_spot, n_rect;
 } orc_render_desc;
 
 typedef struct {
@@ -433,12 +506,83 @@ static int visible(const rctx *c, d3 o, d3 d, double tmax) {
   return occ == 0;
 }
 
+/* ---- binary32 ray generation ----------------------------------------------
+ * The reference computes the shading normal, the sampled direction and the
+ * next ray's origin in f32 (shadePrologue.wgsl.ts, bsdf.wgsl.ts:1632-1778,
+ * kernel.wgsl.ts:1993-2013), and every later decision depends on those bits:
+ * the next hit point feeds the light tree, whose 2^24-bucket split moves by
+ * one bucket for a relative change of 1e-7 in a child's importance, so a
+ * binary64 ray would reach a different light-selection pdf (by 1/buckets,
+ * ~1e-4 for a leaf holding a few thousand buckets).  These helpers restate
+ * that arithmetic operation for operation: one IEEE binary32 rounding per
+ * operation, sums left to right, sqrt and divisions correctly rounded.
+ * Shading values (BSDF, pdfs, MIS, radiance) stay binary64. */
+typedef struct {
+  float x, y, z;
+} f3_t;
+static f3_t F3f(float x, float y, float z) {
+  f3_t r = {x, y, z};
+  return r;
+}
+static float dotf3(f3_t a, f3_t b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+static f3_t crossf3(f3_t a, f3_t b) {
+  return F3f(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static f3_t addf3(f3_t a, f3_t b) { return F3f(a.x + b.x, a.y + b.y, a.z + b.z); }
+static f3_t subf3(f3_t a, f3_t b) { return F3f(a.x - b.x, a.y - b.y, a.z - b.z); }
+static f3_t sclf3(f3_t a, float s) { return F3f(a.x * s, a.y * s, a.z * s); }
+static float maxf_(float a, float b) { return a > b ? a : b; }
+/* safe_normalize (common.wgsl.ts): max-abs scaling, then unit length */
+static f3_t safe_normalize_f(f3_t v) {
+  const float s = maxf_(fabsf(v.x), maxf_(fabsf(v.y), fabsf(v.z)));
+  if (!(s > 0.0f) || s > 3.4028229765e38f) return F3f(0, 1, 0);
+  const f3_t q = F3f(v.x / s, v.y / s, v.z / s);
+  const float l = sqrtf(dotf3(q, q));
+  return F3f(q.x / l, q.y / l, q.z / l);
+}
+/* buildOnb (bsdf.wgsl.ts:1632-1640) */
+static void build_onb_f(f3_t n, f3_t *t, f3_t *b) {
+  const f3_t up = fabsf(n.y) > 0.999f ? F3f(1, 0, 0) : F3f(0, 1, 0);
+  const f3_t c = crossf3(up, n);
+  const float l = sqrtf(dotf3(c, c));
+  *t = F3f(c.x / l, c.y / l, c.z / l);
+  *b = crossf3(n, *t);
+}
+static d3 D3f(f3_t v) { return D3(v.x, v.y, v.z); }
+/* a shadow or connection ray's origin: the hit point pushed off along the
+ * normal on the side the ray leaves (hitPos + offsetNormal * bias, f32) */
+static d3 shadow_origin(f3_t hp, f3_t n, int upper, float bias) {
+  const f3_t off = upper ? n : F3f(-n.x, -n.y, -n.z);
+  return D3f(addf3(hp, sclf3(off, bias)));
+}
+
+static void disc_polar(double u1, double u2, double *a, double *b);
+static double rand01_(void *rng);
+/* sampleGgxVndfTangent (bsdf.wgsl.ts:1710-1735) in f32 */
+static f3_t vndf_sample_f(f3_t wo, float alpha, void *rng) {
+  const f3_t vh = safe_normalize_f(F3f(alpha * wo.x, alpha * wo.y, wo.z));
+  const float lensq = vh.x * vh.x + vh.y * vh.y;
+  const float inv = lensq > 1e-10f ? 1.0f / sqrtf(lensq) : 0.0f;
+  const f3_t t1 = lensq > 1e-10f ? F3f(-vh.y * inv, vh.x * inv, 0.0f) : F3f(1, 0, 0);
+  const f3_t t2 = crossf3(vh, t1);
+  const double u1 = rand01_(rng), u2 = rand01_(rng);
+  double ad, bd;
+  disc_polar(u1, u2, &ad, &bd);
+  const float a = (float)ad;
+  float b = (float)bd;
+  const float sw = 0.5f * (1.0f + vh.z);
+  b = (1.0f - sw) * sqrtf(maxf_(0.0f, 1.0f - a * a)) + sw * b;
+  const f3_t nh = addf3(addf3(sclf3(t1, a), sclf3(t2, b)), sclf3(vh, sqrtf(maxf_(0.0f, (1.0f - a * a) - b * b))));
+  return safe_normalize_f(F3f(alpha * nh.x, alpha * nh.y, maxf_(1e-6f, nh.z)));
+}
+
 typedef struct {
   int hit;
   uint32_t inst, prim;
-  d3 pos, normal;  /* shading normal oriented to the front face */
+  d3 pos, normal;  /* binary64 hit and the shading normal (nrm32 widened) */
   int front;
   float tpos[3];   /* the traversal's f32 world hit position */
+  f3_t nrm32;      /* f32 shading normal oriented to the front face */
 } surf_t;
 
 /* Closest hit (traceClosest, intersection.wgsl.ts:462-490: back faces of
@@ -491,9 +635,34 @@ static int closest(const rctx *c, d3 o, d3 d, surf_t *s) {
   d3 nw = safe_normalize(D3(w2l[0] * nl.x + w2l[1] * nl.y + w2l[2] * nl.z,
                             w2l[4] * nl.x + w2l[5] * nl.y + w2l[6] * nl.z,
                             w2l[8] * nl.x + w2l[9] * nl.y + w2l[10] * nl.z));
-  d3 ng = cross(e1, e2);
-  s->front = dot(ng, Dd) < 0;
-  s->normal = s->front ? nw : scl(nw, -1);
+  (void)nw;
+  /* f32 shading normal and front face, as the reference computes them
+   * (shadePrologue.wgsl.ts:55-130): barycentric blend of the local vertex
+   * normals, the inverse-transpose (W2L columns) to world, and the front face
+   * from the local geometric normal against the local ray direction, flipped
+   * by a mirroring instance (intersectionCore.wgsl.ts:96-112). */
+  {
+    const float bu = h.u, bv = h.v, ba = (1.0f - bu) - bv;
+    f3_t nn[3], pp[3];
+    for (int v = 0; v < 3; ++v) {
+      const float *q = c->d->normals + 4 * (size_t)tri[v];
+      const float *pq = sc->positions + 4 * (size_t)tri[v];
+      nn[v] = F3f(q[0], q[1], q[2]);
+      pp[v] = F3f(pq[0], pq[1], pq[2]);
+    }
+    const f3_t nl = safe_normalize_f(addf3(addf3(sclf3(nn[0], ba), sclf3(nn[1], bu)), sclf3(nn[2], bv)));
+    const f3_t nwf = safe_normalize_f(F3f(dotf3(F3f(w2l[0], w2l[1], w2l[2]), nl),
+                                          dotf3(F3f(w2l[4], w2l[5], w2l[6]), nl),
+                                          dotf3(F3f(w2l[8], w2l[9], w2l[10]), nl)));
+    const f3_t ng = crossf3(subf3(pp[1], pp[0]), subf3(pp[2], pp[0]));
+    const f3_t df = F3f(r.d[0], r.d[1], r.d[2]);
+    const f3_t ld = F3f(dotf3(F3f(w2l[0], w2l[4], w2l[8]), df), dotf3(F3f(w2l[1], w2l[5], w2l[9]), df),
+                        dotf3(F3f(w2l[2], w2l[6], w2l[10]), df));
+    const int mirrored = orc_orientation_sign(m) < 0;
+    s->front = (dotf3(ld, ng) < 0.0f) != mirrored;
+    s->nrm32 = s->front ? nwf : F3f(-nwf.x, -nwf.y, -nwf.z);
+    s->normal = D3f(s->nrm32);
+  }
   return 1;
 }
 
@@ -525,7 +694,7 @@ static uint64_t mixlog(uint64_t h, uint64_t v, uint64_t *ev) {
 }
 
 static void render_path(const rctx *c, uint32_t px, uint32_t py, uint32_t frame_index, double *out,
-                        uint64_t *sig_out, const int *sorted_to_orig, uint64_t *ev) {
+                        uint64_t *sig_out, const int *sorted_to_orig, uint64_t *ev, double *aux) {
   const orc_render_desc *d = c->d;
   orng rng;
   memset(&rng, 0, sizeof rng);
@@ -554,21 +723,41 @@ static void render_path(const rctx *c, uint32_t px, uint32_t py, uint32_t frame_
   }
   int prev_mis = 0;
   uint32_t limit = d->max_bounces < 1 ? 1 : (d->max_bounces > 8 ? 8 : d->max_bounces);
-  uint32_t n_light = d->emitter_count + (c->env.present ? 1u : 0u);
+  uint32_t n_light = d->n_dir + d->n_point + d->n_spot + d->n_rect + d->emitter_count + (c->env.present ? 1u : 0u);
   for (uint32_t bounce = 0; bounce < limit; ++bounce) {
     surf_t s;
     closest(c, o, dir, &s);
     sig = mixlog(sig, s.hit ? (((uint64_t)s.inst << 32) | s.prim) : 0xFFFFFFFFFFFFull, ev);
     if (!s.hit) {
       if (!prev_mis && c->env.present) L = add(L, mul(T, c->env.radiance));
+      if (aux && bounce == 0) {
+        const double miss_aux[8] = {0.5, 1.0, 0.5, 0.0, 0.0, 0.0, 0.0, 0.0};
+        memcpy(aux, miss_aux, sizeof miss_aux);
+      }
       break;
     }
     const mat_t *m = &c->mats[d->tri_material[s.prim]];
-    d3 n = s.normal, wo = scl(dir, -1), hp = s.pos;
+    /* shading runs at the traversal's f32 hit point with the f32 normal */
+    d3 n = s.normal, wo = scl(dir, -1), hp = D3(s.tpos[0], s.tpos[1], s.tpos[2]);
+    const f3_t hp32 = F3f(s.tpos[0], s.tpos[1], s.tpos[2]), n32 = s.nrm32;
+    const float bias32 = (float)c->bias;
+    if (aux && bounce == 0) {
+      /* firstHit* (shadePrologue.wgsl.ts:173-179), f32 as the reference */
+      const f3_t dv = subf3(hp32, F3f((float)o.x, (float)o.y, (float)o.z));
+      aux[0] = n32.x * 0.5f + 0.5f;
+      aux[1] = n32.y * 0.5f + 0.5f;
+      aux[2] = n32.z * 0.5f + 0.5f;
+      aux[3] = sqrtf(dotf3(dv, dv));
+      aux[4] = m->base.x;
+      aux[5] = m->base.y;
+      aux[6] = m->base.z;
+      aux[7] = 1.0;
+    }
     if (!prev_mis && (s.front || m->two_sided)) L = add(L, mul(T, m->emit));
     /* NEE: one light picked by the light tree (one 24-bit draw when the
      * root is interior), or uniformly when the tree is off or declines
      * (sampleCanonicalDirectLight, mediumNee.wgsl.ts:14-38). */
+    const uint32_t n_ana = d->n_dir + d->n_point + d->n_spot + d->n_rect;
     if (n_light > 0) {
       uint32_t picked = 0xFFFFFFFFu;
       double inv_sel = 0;
@@ -591,8 +780,70 @@ static void render_path(const rctx *c, uint32_t px, uint32_t py, uint32_t frame_
         inv_sel = (double)n_light;
       }
       sig = mixlog(sig, 0xA000000000ull | picked, ev);
-      if (picked < d->emitter_count) {
-        const emitter_t *e = &c->em[sorted_to_orig[picked]];
+      if (picked < n_ana) {
+        const float *q = d->lights + 16 * (size_t)picked;
+        if (picked < d->n_dir) {
+          /* directional: irradiance along a (cone-sampled) direction, no MIS */
+          d3 wi = sample_directional_cone(D3(q[0], q[1], q[2]), q[3], &rng);
+          double ndl = fabs(dot(n, wi));
+          if (q[7] > 0.5f || visible(c, shadow_origin(hp32, n32, dot(n, wi) > 0, bias32), wi, F32_MAX))
+            L = add(L, scl(mul(mul(T, bsdf_eval(m, n, wo, wi)), D3(q[4], q[5], q[6])), ndl * inv_sel));
+        } else if (picked < d->n_dir + d->n_point + d->n_spot) {
+          /* point / spot: delta position with the authored decay and cutoff */
+          int spot = picked >= d->n_dir + d->n_point;
+          double cutoff = q[spot ? 12 : 8], decay = q[spot ? 13 : 9];
+          d3 tl = sub(D3(q[0], q[1], q[2]), hp);
+          double dist = sqrt(dot(tl, tl));
+          if (dist > 0 && !(cutoff > 0 && dist > cutoff)) {
+            d3 wi = scl(tl, 1.0 / dist);
+            double soft = 1.0;
+            int lit = 1;
+            if (spot) {
+              d3 axis = safe_normalize(D3(q[4], q[5], q[6]));
+              double co = q[7], ci = q[11], cc = dot(scl(wi, -1), axis);
+              lit = cc >= co;
+              soft = smoothstep_d(co, maxd(ci, co + 1e-6), cc);
+            }
+            double tmax = maxd(dist - 2.0 * c->bias, c->tmin);
+            if (lit && (q[3] > 0.5f || visible(c, shadow_origin(hp32, n32, dot(n, wi) > 0, bias32), wi, tmax))) {
+              double ndl = fabs(dot(n, wi));
+              d3 rad = spot ? D3(q[8], q[9], q[10]) : D3(q[4], q[5], q[6]);
+              double att = point_attenuation(dist, cutoff, decay);
+              L = add(L, scl(mul(mul(T, bsdf_eval(m, n, wo, wi)), rad), ndl * soft * att * inv_sel));
+            }
+          }
+        } else {
+          /* rect / disc area: a point on the emitter, MIS against the BSDF */
+          d3 cc = D3(q[0], q[1], q[2]), u = D3(q[4], q[5], q[6]), v = D3(q[8], q[9], q[10]);
+          int disc = fabs(q[15] - 1.0f) < 0.5f;
+          d3 ln;
+          double area;
+          int ok = analytic_area(u, v, disc, &ln, &area);
+          double xi1 = rand01(&rng), xi2 = rand01(&rng), a, b;
+          if (disc) {
+            concentric_disc(xi1 * 2.0 - 1.0, xi2 * 2.0 - 1.0, &a, &b);
+          } else {
+            a = xi1 * 2.0 - 1.0;
+            b = xi2 * 2.0 - 1.0;
+          }
+          d3 lp = add(add(cc, scl(u, a)), scl(v, b));
+          d3 tl = sub(lp, hp);
+          double dist = sqrt(dot(tl, tl));
+          d3 wi = safe_normalize(tl);
+          double ndl = fabs(dot(n, wi));
+          if (dist > 0 && ndl > 0 && ok) {
+            double cl = maxd(dot(ln, scl(wi, -1)), 0);
+            double lpdf = cl > 0 ? area_to_solid_angle(dist, cl, area) : 0;
+            if (lpdf > 0) {
+              double w = power_heuristic(lpdf, bsdf_pdf(m, n, wo, wi));
+              double tmax = maxd(dist - 2.0 * c->bias, c->tmin);
+              if (q[3] > 0.5f || visible(c, shadow_origin(hp32, n32, dot(n, wi) > 0, bias32), wi, tmax))
+                L = add(L, scl(mul(mul(T, bsdf_eval(m, n, wo, wi)), D3(q[12], q[13], q[14])), ndl * w / lpdf * inv_sel));
+            }
+          }
+        }
+      } else if (picked - n_ana < d->emitter_count) {
+        const emitter_t *e = &c->em[sorted_to_orig[picked - n_ana]];
         double r1 = rand01(&rng), r2 = rand01(&rng);
         double su = sqrt(r1), uu = 1 - su, vv = r2 * su, ww = 1 - uu - vv;
         d3 lp = add(add(scl(e->a, uu), scl(e->b, vv)), scl(e->c, ww));
@@ -612,9 +863,8 @@ static void render_path(const rctx *c, uint32_t px, uint32_t py, uint32_t frame_
             if (lpdf > 0) {
               double bpdf = bsdf_pdf(m, n, wo, wi);
               double w = power_heuristic(lpdf, bpdf);
-              d3 off = dot(n, wi) > 0 ? n : scl(n, -1);
               double tmax = maxd(dist - 2.0 * c->bias, c->tmin);
-              if (visible(c, add(hp, scl(off, c->bias)), wi, tmax)) {
+              if (visible(c, shadow_origin(hp32, n32, dot(n, wi) > 0, bias32), wi, tmax)) {
                 double k = ndl * w / lpdf * inv_sel;
                 L = add(L, scl(mul(mul(T, f), e->le), k));
               }
@@ -634,8 +884,7 @@ static void render_path(const rctx *c, uint32_t px, uint32_t py, uint32_t frame_
         double epdf = (double)c->env.pdf[idx];
         double ndl = fabs(dot(n, wi));
         if (ndl > 0) {
-          d3 off = dot(n, wi) > 0 ? n : scl(n, -1);
-          if (visible(c, add(hp, scl(off, c->bias)), wi, F32_MAX)) {
+          if (visible(c, shadow_origin(hp32, n32, dot(n, wi) > 0, bias32), wi, F32_MAX)) {
             d3 f = bsdf_eval(m, n, wo, wi);
             double bpdf = bsdf_pdf(m, n, wo, wi);
             double w = power_heuristic(epdf, bpdf);
@@ -650,22 +899,26 @@ static void render_path(const rctx *c, uint32_t px, uint32_t py, uint32_t frame_
     double sp = spec_prob(m, ndv);
     (void)rand01(&rng); /* extension-lobe draw: P(base) = 1 */
     int glossy = rand01(&rng) < sp;
-    d3 tb, bb;
-    build_onb(n, &tb, &bb);
-    d3 wi;
+    /* the sampled direction in f32 (glossyReflectionSample / the cosine
+     * lobe, bsdf.wgsl.ts:1648-1778): it is the next ray's direction */
+    f3_t tb, bb, wi32;
+    build_onb_f(n32, &tb, &bb);
+    const f3_t wo32 = F3f(-(float)dir.x, -(float)dir.y, -(float)dir.z);
     if (glossy) {
-      double alpha = maxd(m->rough * m->rough, ALPHA_FLOOR);
-      d3 wl = D3(dot(wo, tb), dot(wo, bb), dot(wo, n));
-      d3 hl = vndf_sample(wl, alpha, &rng);
-      d3 hw = safe_normalize(add(add(scl(tb, hl.x), scl(bb, hl.y)), scl(n, hl.z)));
-      d3 inc = scl(wo, -1);
-      wi = safe_normalize(sub(inc, scl(hw, 2.0 * dot(hw, inc))));
+      const float rough = (float)m->rough, alpha = maxf_(rough * rough, 1e-3f);
+      const f3_t wl = F3f(dotf3(wo32, tb), dotf3(wo32, bb), dotf3(wo32, n32));
+      const f3_t hl = vndf_sample_f(wl, alpha, &rng);
+      const f3_t hw = safe_normalize_f(addf3(addf3(sclf3(tb, hl.x), sclf3(bb, hl.y)), sclf3(n32, hl.z)));
+      const f3_t inc = F3f(-wo32.x, -wo32.y, -wo32.z);
+      wi32 = safe_normalize_f(subf3(inc, sclf3(hw, 2.0f * dotf3(hw, inc))));
     } else {
-      double u1 = rand01(&rng), u2 = rand01(&rng);
-      double a, b, ct = (double)(float)sqrt(maxd(0, 1 - u1));
+      const double u1 = rand01(&rng), u2 = rand01(&rng);
+      double a, b;
       disc_polar(u1, u2, &a, &b);
-      wi = safe_normalize(add(add(scl(tb, a), scl(bb, b)), scl(n, ct)));
+      const float ct = sqrtf(maxf_(0.0f, 1.0f - (float)u1));
+      wi32 = safe_normalize_f(addf3(addf3(sclf3(tb, (float)a), sclf3(bb, (float)b)), sclf3(n32, ct)));
     }
+    d3 wi = D3f(wi32);
     sig = mixlog(sig, glossy ? 0xB1 : 0xB0, ev);
     /* finalizeFiniteBounceSample (bsdf.wgsl.ts:1863-1926) */
     double mpdf = bsdf_pdf(m, n, wo, wi);
@@ -682,16 +935,27 @@ static void render_path(const rctx *c, uint32_t px, uint32_t py, uint32_t frame_
     }
     d3 T_at = T;
     T = mul(T, tmul);
-    d3 new_o = add(hp, scl(n, c->bias));
+    d3 new_o = D3f(addf3(hp32, sclf3(n32, bias32)));
     /* BSDF -> area light and environment connections (connect.wgsl.ts:478-700). */
     if (allows) {
       double ndl = fabs(dot(n, wi));
       double bpdf = mpdf;
       if (ndl > 0 && bpdf > 0) {
-        d3 off = dot(n, wi) > 0 ? n : scl(n, -1);
-        d3 oo = add(hp, scl(off, c->bias));
+        d3 oo = shadow_origin(hp32, n32, dot(n, wi) > 0, bias32);
         double best_t = F32_MAX, best_pdf = 0;
         d3 best_le = D3(0, 0, 0);
+        /* analytic rect and disc lights first (connect.wgsl.ts:544-572) */
+        for (uint32_t k = 0; k < d->n_rect; ++k) {
+          const float *q = d->lights + 16 * (size_t)(d->n_dir + d->n_point + d->n_spot + k);
+          double t, lpdf;
+          if (!intersect_rect_light(q, oo, wi, c->tmin, &t, &lpdf)) continue;
+          if (!(q[3] > 0.5f) && !visible(c, oo, wi, maxd(t - 2.0 * c->bias, c->tmin))) continue;
+          if (t < best_t) {
+            best_t = t;
+            best_pdf = lpdf;
+            best_le = D3(q[12], q[13], q[14]);
+          }
+        }
         for (uint32_t li = 0; li < d->emitter_count; ++li) {
           const emitter_t *e = &c->em[sorted_to_orig[li]];
           /* intersectMeshAreaLightRay (connect.wgsl.ts:453-476), binary64 */
@@ -767,7 +1031,8 @@ static void *render_rows(void *arg) {
     for (uint32_t xx = 0; xx < j->w; ++xx) {
       size_t i = (size_t)yy * j->w + xx;
       render_path(j->c, j->x0 + xx, j->y0 + yy, j->frame_index, j->out + 3 * i, j->sig + i,
-                  j->sorted_to_orig, j->events ? j->events + ORC_EVENT_SLOTS * i : NULL);
+                  j->sorted_to_orig, j->events ? j->events + ORC_EVENT_SLOTS * i : NULL,
+                  j->c->d->aux ? j->c->d->aux + 8 * i : NULL);
     }
   return NULL;
 }

@@ -563,7 +563,9 @@ class RenderDesc(C.Structure):
         ("env_cdf", F32P), ("env_pdf", F32P), ("inv_view_proj", C.c_float * 16),
         ("width", C.c_uint32), ("height", C.c_uint32), ("frame_seed", C.c_uint32),
         ("max_bounces", C.c_uint32), ("ray_bias", C.c_float), ("sampling", C.c_uint32),
-        ("light_tree", F32P), ("light_tree_nodes", C.c_uint32),
+        ("light_tree", F32P), ("light_tree_nodes", C.c_uint32), ("aux", F64P),
+        ("lights", F32P), ("n_dir", C.c_uint32), ("n_point", C.c_uint32), ("n_spot", C.c_uint32),
+        ("n_rect", C.c_uint32),
     ]
 
 
@@ -643,6 +645,166 @@ def _hypot3_js(a, b, c):
     return np.sqrt(s) * m
 
 
+def _area_vector_f32(u, v, coefficient):
+    """classifyAreaVectorF32 (pt-webgpu/src/scene/areaEmitterGeometry.ts:
+    51-128): coefficient * |u x v| in f32 arithmetic; 0 when degenerate."""
+    f = np.float32
+    u = [f(x) for x in u]
+    v = [f(x) for x in v]
+    es = max(abs(x) for x in u + v)
+    if not (es > 0 and np.isfinite(es)):
+        return f(0)
+    su = [x / es for x in u]
+    sv = [x / es for x in v]
+    cx = su[1] * sv[2] - su[2] * sv[1]
+    cy = su[2] * sv[0] - su[0] * sv[2]
+    cz = su[0] * sv[1] - su[1] * sv[0]
+    cs = max(abs(cx), abs(cy), abs(cz))
+    if not (cs > 0 and np.isfinite(cs)):
+        return f(0)
+    dl = f(_hypot3_js(cx / cs, cy / cs, cz / cs))
+    if not (dl > 0):
+        return f(0)
+    with np.errstate(over="ignore"):
+        area = ((f(coefficient) * (cs * dl)) * es) * es
+        inv = f(1) / area if area > 0 else f(0)
+    return area if (area > 0 and np.isfinite(area) and inv > 0 and np.isfinite(inv)) else f(0)
+
+
+def _lum_power(rad, area=None):
+    """emitterPower (pt-webgpu/src/scene/emitterPacking.ts:900-934): f32
+    luminance of the f32 radiance, times the f32 area of a finite emitter."""
+    lum = np.float32(max(0.0, 0.2126 * float(rad[0]) + 0.7152 * float(rad[1]) + 0.0722 * float(rad[2])))
+    if area is None:
+        return float(lum)
+    return float(np.float32(float(lum) * float(area)))
+
+
+def _normalize_f32(v):
+    """normalizeDirectionF32 (areaEmitterGeometry.ts:181-198), or None."""
+    s = max(abs(float(x)) for x in v)
+    if not (s > 0 and math.isfinite(s)):
+        return None
+    q = [float(x) / s for x in v]
+    ln = _hypot3_js(*q)
+    if not (ln > 0):
+        return None
+    return [float(np.float32(x / ln)) for x in q]
+
+
+def pack_analytic(emitters):
+    """packEmitterArrays (pt-webgpu/src/scene/emitterPacking.ts:600-760) for
+    the analytic kinds, as records of 16 f32 in light-selection order --
+    directional (toward-light, angular diameter, irradiance, no-shadow),
+    point (position, no-shadow, radiance, 0, cutoff, decay), spot (position,
+    no-shadow, axis, cos outer, radiance, cos inner, cutoff, decay), then
+    rect and disc (centre, no-shadow, u, 0, v, 0, radiance, shape) -- and the
+    counts (n_dir, n_point, n_spot, n_rect).  Raises ValueError where the
+    reference throws RangeError."""
+    f = np.float32
+    recs = {k: [] for k in ("directional", "point", "spot", "rect-area", "disc-area")}
+    for e in emitters:
+        recs[e["kind"]].append(e)
+
+    def rad(e):
+        r = [float(f(f(c) * f(e.get("intensity", 1.0)))) for c in e["color"]]
+        if not all(math.isfinite(x) and x >= 0 for x in r):
+            raise ValueError("radiance must be finite and non-negative")
+        return r
+
+    def shadow(e):
+        return 1.0 if e.get("castShadow", True) is False else 0.0
+
+    out = []
+    for e in recs["directional"]:
+        x, y, z = (float(c) for c in e["direction"])
+        ln = _hypot3_js(x, y, z)
+        t = [0.0, 1.0, 0.0] if ln < 1e-8 else [-x / ln, -y / ln, -z / ln]
+        ad = e.get("angularDiameter", 0.0)
+        ad = ad if (ad is not None and math.isfinite(ad) and ad > 0) else 0.0
+        out.append(t + [ad] + rad(e) + [shadow(e)] + [0.0] * 8)
+    for e in recs["point"]:
+        dist = e.get("distance", 0.0)
+        dist = dist if dist and dist > 0 else 0.0
+        out.append(list(e["position"]) + [shadow(e)] + rad(e) + [0.0, dist, e.get("decay", 2.0)] + [0.0] * 6)
+    for e in recs["spot"]:
+        x, y, z = (float(c) for c in e["direction"])
+        ln = _hypot3_js(x, y, z)
+        d = [0.0, -1.0, 0.0] if ln < 1e-8 else [x / ln, y / ln, z / ln]
+        pen = min(1.0, max(0.0, e.get("penumbra", 0.0) or 0.0))
+        outer = e["angle"]
+        inner = outer * (1 - pen)
+        dist = e.get("distance", 0.0)
+        dist = dist if dist and dist > 0 else 0.0
+        out.append(list(e["position"]) + [shadow(e)] + d + [math.cos(outer)] + rad(e) + [math.cos(inner)] +
+                   [dist, e.get("decay", 2.0), 0.0, 0.0])
+    for e in recs["rect-area"]:
+        u = [float(f(c)) for c in e["uAxis"]]
+        v = [float(f(c)) for c in e["vAxis"]]
+        if not _area_vector_f32(u, v, 4.0) > 0:
+            raise ValueError("rect-area emitter does not retain positive f32 area")
+        out.append(list(e["position"]) + [shadow(e)] + u + [0.0] + v + [0.0] + rad(e) + [0.0])
+    for e in recs["disc-area"]:
+        r = e["radius"]
+        if not (math.isfinite(r) and r > 0):
+            raise ValueError("disc-area emitter has a non-positive radius")
+        n = _normalize_f32(e["normal"])
+        if n is None:
+            raise ValueError("disc-area emitter has a degenerate normal")
+        ax, ay, az = (1.0, 0.0, 0.0) if abs(n[1]) > 0.999 else (0.0, 1.0, 0.0)
+        ux, uy, uz = n
+        t = _normalize_f32([ay * uz - az * uy, az * ux - ax * uz, ax * uy - ay * ux])
+        if t is None:
+            raise ValueError("disc-area emitter has a degenerate tangent basis")
+        b = [uy * t[2] - uz * t[1], uz * t[0] - ux * t[2], ux * t[1] - uy * t[0]]
+        uax = [float(f(c * r)) for c in t]
+        vax = [float(f(c * r)) for c in b]
+        if not _area_vector_f32(uax, vax, math.pi) > 0:
+            raise ValueError("disc-area emitter does not retain positive f32 area")
+        out.append(list(e["position"]) + [shadow(e)] + uax + [0.0] + vax + [0.0] + rad(e) + [1.0])
+    counts = (len(recs["directional"]), len(recs["point"]), len(recs["spot"]),
+              len(recs["rect-area"]) + len(recs["disc-area"]))
+    arr = np.asarray(out, np.float32).reshape(-1, 16)
+    return arr, counts
+
+
+def _analytic_leaves(records, counts):
+    """Leaves of the positional analytic lights (flatEmitterWalk.ts:115-188 as
+    emitterPacking.ts:1090-1140 uses it): point / spot at their position
+    (spot: a cone of half angle acos(cos outer) about its axis), rect / disc
+    on their box with a hemisphere about u x v."""
+    nd, npt, ns, nr = counts
+    out = []
+    for k in range(nd, nd + npt + ns + nr):
+        q = [float(x) for x in records[k]]
+        p = q[0:3]
+        if k < nd + npt + ns:
+            spot = k >= nd + npt
+            rad = q[8:11] if spot else q[4:7]
+            cone = None
+            if spot:
+                ln = _hypot3_js(*q[4:7])
+                axis = [x / ln for x in q[4:7]] if ln > 1e-12 else [0.0, 0.0, 0.0]
+                cone = (axis, 0.0, math.acos(max(-1.0, min(1.0, q[7]))))
+            out.append((_lum_power(rad), list(p), list(p), list(p), cone))
+        else:
+            u, v = q[4:7], q[8:11]
+            disc = abs(q[15] - 1.0) < 0.5
+            area = _area_vector_f32(u, v, math.pi if disc else 4.0)
+            if disc:
+                r = _hypot3_js(*u)
+                ext = [r, r, r]
+            else:
+                ext = [abs(u[i]) + abs(v[i]) for i in range(3)]
+            lo = [p[i] - ext[i] for i in range(3)]
+            hi = [p[i] + ext[i] for i in range(3)]
+            n = [u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]]
+            ln = _hypot3_js(*n)
+            axis = [x / ln for x in n] if ln > 1e-12 else [0.0, 0.0, 0.0]
+            out.append((_lum_power(q[12:15], area), list(p), lo, hi, (axis, 0.0, math.pi / 2)))
+    return out
+
+
 def _triangle_area_f32(a, b, c):
     """measureTriangleAreaF32 (pt-webgpu/src/scene/areaEmitterGeometry.ts:
     37-173): equilibrated f32 cross product, one rounding per operation;
@@ -670,7 +832,7 @@ def _triangle_area_f32(a, b, c):
     return area if (area > 0 and np.isfinite(area) and inv > 0 and np.isfinite(inv)) else f(0)
 
 
-def scene_light_tree(emitters, env_radiance=None, env_size=(64, 32)):
+def scene_light_tree(emitters, env_radiance=None, env_size=(64, 32), analytic=None):
     """The scene's NEE light tree (emitterPacking.ts:1005-1230 leaves, then
     light_tree_build): one leaf per mesh emitter record in order -- power =
     f32 luminance of its radiance x its f32 area, bounds of its vertices,
@@ -679,10 +841,17 @@ def scene_light_tree(emitters, env_radiance=None, env_size=(64, 32)):
     of the emitters' bounds).  None below two lights."""
     em = np.asarray(emitters, np.float32).reshape(-1, 16)
     env_present = env_radiance is not None and any(v > 0 for v in env_radiance)
-    n = em.shape[0] + (1 if env_present else 0)
+    records, counts = analytic if analytic is not None else (np.zeros((0, 16), np.float32), (0, 0, 0, 0))
+    n = em.shape[0] + (1 if env_present else 0) + sum(counts)
     if n < 2:
         return None
     powers, cents, lo, hi, cones = [], [], [], [], []
+    for pw, c, l, h, cone in _analytic_leaves(records, counts):
+        powers.append(pw)
+        cents.append(c)
+        lo.append(np.asarray(l, np.float64))
+        hi.append(np.asarray(h, np.float64))
+        cones.append(cone)
     for r in em:
         a, b, c = r[0:3], r[3:6], r[6:9]
         le = r[9:12]
@@ -712,14 +881,26 @@ def scene_light_tree(emitters, env_radiance=None, env_size=(64, 32)):
             om = dphi * (math.cos(y / H * math.pi) - math.cos((y + 1) / H * math.pi))
             total += max(0.0, lum * om)
         pw = float(np.float32(total))
-        powers.append((pw if pw > 0 else 2.0 ** -149) if total > 0 else 0.0)
-        if em.shape[0]:
-            ulo, uhi = np.min(np.array(lo), axis=0), np.max(np.array(hi), axis=0)
-        else:
-            ulo = uhi = np.zeros(3)
+        env_power = (pw if pw > 0 else 2.0 ** -149) if total > 0 else 0.0
+    # directional lights and the environment sit on the union of the
+    # positional lights' bounds (the origin when there are none)
+    if lo:
+        ulo, uhi = np.min(np.array(lo), axis=0), np.max(np.array(hi), axis=0)
+    else:
+        ulo = uhi = np.zeros(3)
+    ucen = [(ulo[k] + uhi[k]) / 2 for k in range(3)]
+    nd = counts[0]
+    for k in range(nd - 1, -1, -1):
+        powers.insert(0, _lum_power([float(x) for x in records[k][4:7]]))
+        cents.insert(0, ucen)
+        lo.insert(0, ulo)
+        hi.insert(0, uhi)
+        cones.insert(0, None)
+    if env_present:
+        powers.append(env_power)
         lo.append(ulo)
         hi.append(uhi)
-        cents.append([(ulo[k] + uhi[k]) / 2 for k in range(3)])
+        cents.append(ucen)
         cones.append(None)
     return light_tree_build(powers, cents, lo, hi, cones)
 
@@ -792,11 +973,13 @@ def build_constant_env(radiance, width=64, height=32):
 
 def render(pack, materials, camera_inv_vp, width, height, *, frame_seed, frame_index, max_bounces,
            ray_bias, env_radiance=None, env_size=(64, 32), tile=None, threads=None, events=False,
-           sampling=0, light_tree=True):
+           sampling=0, light_tree=True, aux=False, emitters=None):
     """One frame (1 spp) of the binary64 integrator mirror.  Returns
     (radiance[h, w, 3] float64, signature[h, w] uint64), plus with `events`
     the decisions mixed into each signature ([h, w, 40] uint64: count, then
-    hit keys, light picks, lobes and roulette outcomes in path order)."""
+    hit keys, light picks, lobes and roulette outcomes in path order), plus
+    with `aux` the first-hit images ([h, w, 8]: normal * 0.5 + 0.5, depth,
+    base colour, 1)."""
     ps = PackedScene(pack)
     mt = material_table(materials)
     em = np.ascontiguousarray(build_emitters(pack, mt))
@@ -828,7 +1011,11 @@ def render(pack, materials, camera_inv_vp, width, height, *, frame_seed, frame_i
     d.max_bounces = max_bounces
     d.ray_bias = ray_bias
     d.sampling = sampling
-    tree = scene_light_tree(em, env_radiance, env_size) if light_tree else None
+    ana = pack_analytic(emitters or [])
+    recs = np.ascontiguousarray(ana[0] if ana[0].size else np.zeros((1, 16), np.float32))
+    d.lights = recs.ctypes.data_as(F32P)
+    d.n_dir, d.n_point, d.n_spot, d.n_rect = ana[1]
+    tree = scene_light_tree(em, env_radiance, env_size, analytic=ana) if light_tree else None
     if tree is not None:
         tree = np.ascontiguousarray(tree, np.float32)
         d.light_tree = tree.ctypes.data_as(F32P)
@@ -837,11 +1024,14 @@ def render(pack, materials, camera_inv_vp, width, height, *, frame_seed, frame_i
     out = np.zeros((h, w, 3), np.float64)
     sig = np.zeros((h, w), np.uint64)
     ev = np.zeros((h, w, 40), np.uint64) if events else None
+    aux_out = np.zeros((h, w, 8), np.float64) if aux else None
+    d.aux = aux_out.ctypes.data_as(F64P) if aux else None
     lib().orc_render(C.byref(d), C.c_uint32(frame_index), x0, y0, w, h,
                      out.ctypes.data_as(F64P), sig.ctypes.data_as(C.POINTER(C.c_uint64)),
                      ev.ctypes.data_as(C.POINTER(C.c_uint64)) if events else None,
                      C.c_uint32(threads or min(16, os.cpu_count() or 1)))
-    return (out, sig, ev) if events else (out, sig)
+    res = (out, sig) + ((ev,) if events else ()) + ((aux_out,) if aux else ())
+    return res
 
 
 # ---------------------------------------------------------------------------

@@ -11,9 +11,13 @@ def materials_of(scene):
     return [p.get("material", {}) for p in scene["primitives"]]
 
 
-def cpu_frames(scene, res, frames, bias, max_bounces=8, sampling=0, light_tree=True, first_frame=0):
+def cpu_frames(scene, res, frames, bias, max_bounces=8, sampling=0, light_tree=True, first_frame=0,
+               tile=None, pack=None):
+    """`frames` single-sample frames of the binary64 integrator at res x res
+    (or only the pixels of `tile` = (x0, y0, w, h)); returns (images,
+    signatures) stacked over frames."""
     prims = [dict(p, material_id=i) for i, p in enumerate(scene["primitives"])]
-    pack = O.pack_scene(prims)
+    pack = O.pack_scene(prims) if pack is None else pack
     ivp = W.inv_view_proj(scene["camera"], res, res)
     sky = scene.get("sky", [0, 0, 0])
     out, sigs = [], []
@@ -21,7 +25,7 @@ def cpu_frames(scene, res, frames, bias, max_bounces=8, sampling=0, light_tree=T
         img, sig = O.render(pack, materials_of(scene), ivp, res, res, frame_seed=SEED, frame_index=f,
                             max_bounces=max_bounces, ray_bias=bias,
                             env_radiance=sky if any(sky) else None, sampling=sampling,
-                            light_tree=light_tree)
+                            light_tree=light_tree, tile=tile, emitters=scene.get("emitters"))
         out.append(img)
         sigs.append(sig)
     return np.stack(out), np.stack(sigs)
@@ -303,3 +307,86 @@ def disc_expected_radiance(scene, le, albedo, roughness, metallic, res=8):
         f = lambda wi: base_bsdf(albedo, roughness, metallic, n, wo, wi)  # noqa: E731
         out.append(polygon_reflected_radiance(x, n, wo, tris, le, brdf=f))
     return np.mean(out, axis=0)
+
+
+# ---------------------------------------------------------------------------
+# Analytic emitters
+# ---------------------------------------------------------------------------
+
+def floor_with_emitters(emitters, albedo=0.5, roughness=1.0, metallic=0.0, view_deg=0.0, fov=0.4):
+    """A 400 x 400 floor at y = 0 lit only by `emitters`, looked at from 5
+    units away, `view_deg` off the normal, through a narrow field (ray-origin
+    offset 1.2e-3)."""
+    fpos = np.array([-200, 0, -200, 200, 0, -200, 200, 0, 200, -200, 0, 200], np.float32)
+    floor = {"kind": "mesh", "positions": fpos, "normals": np.tile([0, 1, 0], 4).astype(np.float32),
+             "indices": np.array([0, 2, 1, 0, 3, 2], np.uint32),
+             "material": {"baseColor": [albedo] * 3, "roughness": roughness, "metallic": metallic}}
+    t = np.radians(view_deg)
+    eye = (0.0, 5 * np.cos(t), 5 * np.sin(t) + (0.001 if view_deg == 0 else 0.0))
+    cam = W.look_at_camera(eye=eye, target=(0, 0, 0), up=(0, 1, 0) if view_deg else (0, 0, -1), vfov_deg=fov)
+    # scene_radius sets the ray-origin offset (radius * 1e-3 / sqrt(3)): the
+    # BSDF-side connection to a light sees it from the offset origin, so the
+    # offset is kept small against the light's height
+    return {"primitives": [floor], "camera": cam, "scene_radius": 2.0, "sky": [0, 0, 0],
+            "emitters": list(emitters)}
+
+
+def footprint_mean(scene, fn, res=8):
+    """Mean of fn(x, wo) over an 8 x 8 grid of the camera's pixel-centre hits
+    on the floor."""
+    eye = scene["camera"]["eye"]
+    vals = []
+    for x in camera_floor_points(scene, res):
+        wo = (eye - x) / np.linalg.norm(eye - x)
+        vals.append(fn(x, wo))
+    return np.mean(vals, axis=0)
+
+
+def point_light_radiance(x, wo, light, bsdf_args, n=np.array([0.0, 1.0, 0.0])):
+    """Reflected radiance at x from a point (or spot) light: f(wo, wi) I
+    cos(theta) / d^decay (the delta light's closed form), with the spot's
+    smoothstep cone factor."""
+    p = np.asarray(light["position"], np.float64)
+    d = p - x
+    dist = np.linalg.norm(d)
+    wi = d / dist
+    rad = np.float32(np.asarray(light["color"], np.float32) * np.float32(light.get("intensity", 1.0)))
+    att = dist ** -light.get("decay", 2.0)
+    soft = 1.0
+    if light["kind"] == "spot":
+        axis = np.asarray(light["direction"], np.float64)
+        axis /= np.linalg.norm(axis)
+        co = np.cos(light["angle"])
+        ci = np.cos(light["angle"] * (1 - light.get("penumbra", 0.0)))
+        cc = float(-wi @ axis)
+        if cc < co:
+            return np.zeros(3)
+        t = np.clip((cc - co) / (max(ci, co + 1e-6) - co), 0, 1)
+        soft = t * t * (3 - 2 * t)
+    f = base_bsdf(*bsdf_args, n, wo, wi[None, :])[0]
+    return f * rad.astype(np.float64) * max(float(wi @ n), 0.0) * att * soft
+
+
+def rect_light_radiance(x, wo, light, bsdf_args, n=np.array([0.0, 1.0, 0.0]), m=24):
+    """Reflected radiance at x from a rect (half axes u, v) or disc (normal,
+    radius; a fine polygon) area light, by area quadrature."""
+    p = np.asarray(light["position"], np.float64)
+    rad = (np.asarray(light["color"], np.float32) * np.float32(light.get("intensity", 1.0))).astype(np.float64)
+    if light["kind"] == "rect-area":
+        u, v = (np.asarray(light[k], np.float64) for k in ("uAxis", "vAxis"))
+        c = [p - u - v, p + u - v, p + u + v, p - u + v]
+        tris = [(c[0], c[1], c[2]), (c[0], c[2], c[3])]
+    else:
+        nn = np.asarray(light["normal"], np.float64)
+        nn /= np.linalg.norm(nn)
+        a = np.array([1.0, 0, 0]) if abs(nn[1]) > 0.999 else np.array([0, 1.0, 0])
+        t = np.cross(a, nn)
+        t /= np.linalg.norm(t)
+        b = np.cross(nn, t)
+        k = 512
+        ang = 2 * np.pi * np.arange(k) / k
+        ring = p + light["radius"] * (np.cos(ang)[:, None] * t + np.sin(ang)[:, None] * b)
+        tris = [(p, ring[i], ring[(i + 1) % k]) for i in range(k)]
+        m = 4
+    f = lambda wi: base_bsdf(*bsdf_args, n, wo, wi)  # noqa: E731
+    return polygon_reflected_radiance(x, n, wo, tris, rad, brdf=f, m=m)

@@ -340,7 +340,8 @@ vt_status vt_set_scene(vt_engine* eng, const vt_scene_desc* scene) {
     s = s ? s : engine_upload(eng, nb.normals, pack->normals.data(), pack->normals.size() * 4);
     s = s ? s : engine_upload(eng, nb.indices, pack->indices.data(), pack->indices.size() * 4);
     s = s ? s : engine_upload(eng, nb.tri_material, pack->tri_material.data(), pack->tri_material.size() * 4);
-    s = s ? s : engine_prepare_render(eng, *pack, materials, scene->environment, scene->scene_radius, &rd);
+    s = s ? s : engine_prepare_render(eng, *pack, materials, scene->environment, scene->scene_radius,
+                                      scene->emitters, scene->emitter_count, &rd);
     if (!s) {
       hipError_t r = hipStreamSynchronize(eng->stream);
       if (r != hipSuccess) s = set_err(eng, VT_ERR_HIP, hipGetErrorString(r));

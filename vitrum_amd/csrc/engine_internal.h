@@ -107,6 +107,7 @@ struct vt_engine {
   uint32_t build_threads = 0;  // host threads of the scene builder (0: every hardware thread)
 };
 
+
 // Threads the scene builder uses for this engine.
 uint32_t engine_build_threads(const vt_engine* e);
 
@@ -126,7 +127,8 @@ vt_status engine_build_trace_scene(vt_engine* e, const vt_scene_desc* scene, Sce
 // scene's shading data without touching the bound scene; commit binds it
 // (cannot fail; takes ownership); discard frees an unbound set.
 vt_status engine_prepare_render(vt_engine* e, const vt::Pack& pack, const std::vector<vt_material>& materials,
-                                const vt_environment& env, float scene_radius, RenderSceneData** out);
+                                const vt_environment& env, float scene_radius, const vt_emitter* emitters,
+                                uint32_t emitter_count, RenderSceneData** out);
 void engine_commit_render(vt_engine* e, RenderSceneData* d);
 void engine_discard_render(RenderSceneData* d);
 void engine_release_render(vt_engine* e);

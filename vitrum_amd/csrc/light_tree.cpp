@@ -228,12 +228,11 @@ std::vector<float> build_packed(const std::vector<Leaf>& leaves) {
   return out;
 }
 
-Leaf triangle_leaf(const float a[3], const float b[3], const float c[3], const float le[3], bool two_sided) {
-  Leaf L{};
-  // f32 triangle area (areaEmitterGeometry.ts:37-163): equilibrated edge
+double js_hypot3(double a, double b, double c) { return hypot3(a, b, c); }
+
+float area_f32(const float u[3], const float v[3], float coefficient) {
+  // classifyAreaVectorF32 (areaEmitterGeometry.ts:51-128): equilibrated edge
   // cross product, one rounding per operation.
-  float u[3], v[3];
-  for (int k = 0; k < 3; ++k) u[k] = b[k] - a[k], v[k] = c[k] - a[k];
   float area = 0;
   const float es = std::max({std::fabs(u[0]), std::fabs(u[1]), std::fabs(u[2]), std::fabs(v[0]),
                              std::fabs(v[1]), std::fabs(v[2])});
@@ -248,7 +247,7 @@ Leaf triangle_leaf(const float a[3], const float b[3], const float c[3], const f
       const float dl = (float)hypot3(cx / cs, cy / cs, cz / cs);
       if (dl > 0 && std::isfinite(dl)) {
         const float scaled = cs * dl;
-        const float weighted = 0.5f * scaled;
+        const float weighted = coefficient * scaled;
         const float once = weighted * es;
         const float ar = once * es;
         const float inv = 1.0f / ar;
@@ -256,11 +255,25 @@ Leaf triangle_leaf(const float a[3], const float b[3], const float c[3], const f
       }
     }
   }
-  // emitterPower: f32 luminance of the f32 radiance times the f32 area.
+  return area;
+}
+
+double power_of(const float le[3], double area) {
+  // emitterPower (emitterPacking.ts:900-934): f32 luminance of the f32
+  // radiance, times the f32 area for a finite emitter.
   const double lum_d = 0.2126 * (double)le[0] + 0.7152 * (double)le[1] + 0.0722 * (double)le[2];
   const float lum = (float)std::max(0.0, lum_d);
-  const float power = (float)((double)lum * (double)area);
-  L.power = power;
+  if (area < 0) return lum;  // delta light
+  return (float)((double)lum * area);
+}
+
+Leaf triangle_leaf(const float a[3], const float b[3], const float c[3], const float le[3], bool two_sided) {
+  Leaf L{};
+  // f32 triangle area (areaEmitterGeometry.ts:141-173)
+  float u[3], v[3];
+  for (int k = 0; k < 3; ++k) u[k] = b[k] - a[k], v[k] = c[k] - a[k];
+  const float area = area_f32(u, v, 0.5f);
+  L.power = power_of(le, area);
   for (int k = 0; k < 3; ++k) {
     L.centroid[k] = ((double)a[k] + (double)b[k] + (double)c[k]) / 3;
     L.lo[k] = std::min({(double)a[k], (double)b[k], (double)c[k]});

@@ -30,4 +30,13 @@ std::vector<float> build_packed(const std::vector<Leaf>& leaves);
 // the f32 triangle area of areaEmitterGeometry.ts:141-173).
 Leaf triangle_leaf(const float a[3], const float b[3], const float c[3], const float le[3], bool two_sided);
 
+// Math.hypot of three finite numbers as V8 computes it (max-scaled,
+// compensated sum of squares).
+double js_hypot3(double a, double b, double c);
+// classifyAreaVectorF32 (areaEmitterGeometry.ts:51-128): coefficient * |u x v|
+// in f32 arithmetic, 0 when degenerate or not finite.
+float area_f32(const float u[3], const float v[3], float coefficient);
+// emitterPower (emitterPacking.ts:900-934); area < 0 marks a delta light.
+double power_of(const float le[3], double area);
+
 }  // namespace vtl

@@ -11,6 +11,7 @@
 #include <string>
 #include <vector>
 
+#include "emitters.h"
 #include "engine_internal.h"
 #include "light_tree.h"
 #include "render.h"
@@ -91,7 +92,8 @@ std::vector<float> represented_cdf(const std::vector<double>& w) {
 // Per-scene shading data.  vt_set_scene builds and uploads a complete set
 // before it binds anything, so a failure leaves the previous scene's set.
 struct RenderSceneData {
-  DeviceBuffer materials, emitters, instances, env_cdf, env_pdf, tri_source, light_tree;
+  DeviceBuffer materials, emitters, instances, env_cdf, env_pdf, tri_source, light_tree, lights;
+  uint32_t n_dir = 0, n_point = 0, n_spot = 0, n_rect = 0;  // analytic emitters in `lights`
   uint32_t light_tree_nodes = 0;  // 0: uniform light selection
   std::vector<float> light_tree_host;
   uint32_t emitter_count = 0;
@@ -100,7 +102,7 @@ struct RenderSceneData {
   float ray_bias = 0, tmin = 0;
   float scene_min[3] = {0, 0, 0}, scene_inv_extent[3] = {1, 1, 1};
   void release_scene() {
-    for (DeviceBuffer* b : {&materials, &emitters, &instances, &env_cdf, &env_pdf, &tri_source, &light_tree})
+    for (DeviceBuffer* b : {&materials, &emitters, &instances, &env_cdf, &env_pdf, &tri_source, &light_tree, &lights})
       b->release();
   }
 };
@@ -112,6 +114,7 @@ struct RenderState : RenderSceneData {
   DeviceBuffer rng, rng_dim, throughput, conn, radiance, prev_pdf, flags, signature, count;
   DeviceBuffer queue[2], rays[2], hits, hit_pos;
   DeviceBuffer shadow_path, shadow_rays, shadow_contrib, shadow_occ, accum;
+  DeviceBuffer moments, aux_nd, aux_albedo, present;  // variance moments, first-hit images, capture staging
   // Neural radiance cache (used when nrc_enabled).
   bool nrc_enabled = false;
   uint32_t nrc_train_paths = 65536;
@@ -127,7 +130,8 @@ struct RenderState : RenderSceneData {
   void release_paths() {
     for (DeviceBuffer* b : {&rng, &rng_dim, &throughput, &conn, &radiance, &prev_pdf, &flags, &signature, &count,
                             &queue[0], &queue[1], &rays[0], &rays[1], &hits, &hit_pos, &shadow_path,
-                            &shadow_rays, &shadow_contrib, &shadow_occ, &accum, &nrc_count, &q_feat, &q_thr,
+                            &shadow_rays, &shadow_contrib, &shadow_occ, &accum, &moments, &aux_nd, &aux_albedo,
+                            &present, &nrc_count, &q_feat, &q_thr,
                             &q_path, &q_rgb, &train_rec, &train_n, &train_feat, &train_tgt})
       b->release();
     width = height = paths = 0;
@@ -188,7 +192,8 @@ void engine_commit_render(vt_engine* e, RenderSceneData* d) {
 }
 
 vt_status engine_prepare_render(vt_engine* e, const vt::Pack& P, const std::vector<vt_material>& materials,
-                                const vt_environment& env, float scene_radius, RenderSceneData** out) {
+                                const vt_environment& env, float scene_radius, const vt_emitter* emitters,
+                                uint32_t emitter_count, RenderSceneData** out) {
   *out = nullptr;
   if (vt_status s = ensure_render_state(e)) return s;
   if (e->fail_inject & kInjectRenderThrow) {
@@ -284,19 +289,40 @@ vt_status engine_prepare_render(vt_engine* e, const vt::Pack& P, const std::vect
   // environment -- built when there are two or more lights and the engine
   // uses tree selection (the reference's default; uploadSceneBuffers.ts:
   // 1850-1862, emitterPacking.ts:1005-1230).
+  // Analytic emitters: directional, point, spot, rect and disc records.
+  vte::Packed ana;
+  const std::string pe = vte::pack(emitters, emitter_count, ana);
+  if (!pe.empty()) return rerr(e, VT_ERR_RANGE, pe);
+  R.n_dir = ana.n_dir, R.n_point = ana.n_point, R.n_spot = ana.n_spot, R.n_rect = ana.n_rect;
   std::vector<float> tree;
-  const uint32_t n_lights = R.emitter_count + (R.env_present ? 1u : 0u);
+  const uint32_t n_lights = ana.count() + R.emitter_count + (R.env_present ? 1u : 0u);
   if (e->opts.light_selection == 0 && n_lights >= 2) {
-    std::vector<vtl::Leaf> leaves;
+    // Leaves in the NEE walk's order (buildLightTreeInputForScene,
+    // emitterPacking.ts:1005-1230): directional, point, spot, rect/disc,
+    // mesh triangles, environment.  Directional lights and the environment
+    // sit on the union of the positional lights' bounds.
+    std::vector<vtl::Leaf> leaves = vte::positional_leaves(ana);
     leaves.reserve(n_lights);
-    double ulo[3] = {INFINITY, INFINITY, INFINITY}, uhi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (uint32_t i = 0; i < R.emitter_count; ++i) {
       const float* q = &em[(size_t)i * vtr::kEmitFloats];
       leaves.push_back(vtl::triangle_leaf(q, q + 3, q + 6, q + 9, q[12] > 0.5f));
+    }
+    double ulo[3] = {INFINITY, INFINITY, INFINITY}, uhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (const vtl::Leaf& L : leaves)
       for (int k = 0; k < 3; ++k) {
-        ulo[k] = std::min(ulo[k], leaves.back().lo[k]);
-        uhi[k] = std::max(uhi[k], leaves.back().hi[k]);
+        ulo[k] = std::min(ulo[k], L.lo[k]);
+        uhi[k] = std::max(uhi[k], L.hi[k]);
       }
+    if (ana.n_dir) {
+      double lo[3], hi[3];
+      for (int k = 0; k < 3; ++k) {
+        const bool ok = std::isfinite(ulo[k]);
+        lo[k] = ok ? ulo[k] : 0.0;
+        hi[k] = ok ? uhi[k] : 0.0;
+      }
+      std::vector<vtl::Leaf> dirs;
+      for (uint32_t d = 0; d < ana.n_dir; ++d) dirs.push_back(vte::directional_leaf(ana, d, lo, hi));
+      leaves.insert(leaves.begin(), dirs.begin(), dirs.end());
     }
     if (R.env_present) {
       // environment leaf: luminance x solid angle over the map, rounded to
@@ -351,6 +377,8 @@ vt_status engine_prepare_render(vt_engine* e, const vt::Pack& P, const std::vect
   s = s ? s : engine_upload(e, R.env_pdf, pdf.data(), pdf.size() * 4);
   s = s ? s : engine_upload(e, R.tri_source, P.tri_source.data(), P.tri_source.size() * 4);
   s = s ? s : engine_upload(e, R.light_tree, tree.data(), tree.size() * 4);
+  if (ana.records.empty()) ana.records.assign(vtr::kLightFloats, 0.0f);
+  s = s ? s : engine_upload(e, R.lights, ana.records.data(), ana.records.size() * 4);
   if (!s && (e->fail_inject & kInjectRenderAlloc)) {
     e->fail_inject &= ~kInjectRenderAlloc;
     s = rerr(e, VT_ERR_OUT_OF_MEMORY, "injected allocation failure while uploading the scene's shading data");
@@ -393,6 +421,10 @@ static vt_status ensure_paths(vt_engine* e, uint32_t w, uint32_t h) {
   need(R.shadow_contrib, n * 12);
   need(R.shadow_occ, n * 4);
   need(R.accum, pix * 16);
+  need(R.moments, pix * 8);
+  need(R.aux_nd, pix * 16);
+  need(R.aux_albedo, pix * 16);
+  need(R.present, pix * 16);
   if (train) {
     need(R.nrc_count, 64);
     const size_t nq = pix + train;  // image-path queries + training-path terminal queries
@@ -412,8 +444,7 @@ static vt_status ensure_paths(vt_engine* e, uint32_t w, uint32_t h) {
   R.width = w;
   R.height = h;
   R.paths = paths;
-  HIP_CHECK(e, vtr::launch_zero_f32((float*)R.accum.ptr, (uint32_t)(pix * 4), e->stream));
-  e->samples_accumulated = 0;
+  e->samples_accumulated = 0;  // the next frame clears the accumulation
   return VT_OK;
 }
 
@@ -471,6 +502,11 @@ vt_status vt_render_frame(vt_engine* e, const vt_frame* fr) {
     for (int k = 0; k < 3; ++k) sc.env_radiance[k] = R.env_radiance[k];
     sc.light_tree = (const float*)R.light_tree.ptr;
     sc.light_tree_nodes = R.light_tree_nodes;
+    sc.lights = (const float*)R.lights.ptr;
+    sc.n_dir = R.n_dir;
+    sc.n_point = R.n_point;
+    sc.n_spot = R.n_spot;
+    sc.n_rect = R.n_rect;
     vtr::PathState ps{};
     ps.rng = (uint32_t*)R.rng.ptr;
     ps.rng_dim = (uint32_t*)R.rng_dim.ptr;
@@ -488,6 +524,15 @@ vt_status vt_render_frame(vt_engine* e, const vt_frame* fr) {
     ps.shadow_contrib = (float*)R.shadow_contrib.ptr;
     ps.shadow_occ = (uint32_t*)R.shadow_occ.ptr;
     ps.accum = (float*)R.accum.ptr;
+    ps.moments = (float*)R.moments.ptr;
+    ps.aux_nd = (float*)R.aux_nd.ptr;
+    ps.aux_albedo = (float*)R.aux_albedo.ptr;
+    // Accumulation restarts (reset, new scene, new viewport, cache settings)
+    // set the sample count to zero; the first frame after one clears the sums.
+    if (e->samples_accumulated == 0) {
+      HIP_CHECK(e, vtr::launch_zero_f32(ps.accum, f.pixels * 4, e->stream));
+      HIP_CHECK(e, vtr::launch_zero_f32(ps.moments, f.pixels * 2, e->stream));
+    }
     ps.nrc_count = (uint32_t*)R.nrc_count.ptr;
     ps.q_feat = (float*)R.q_feat.ptr;
     ps.q_thr = (float*)R.q_thr.ptr;
@@ -629,10 +674,54 @@ vt_status vt_reset(vt_engine* e) {
     if (!e) return VT_ERR_INVALID_ARGUMENT;
     e->samples_accumulated = 0;
     if (e->render && e->render->width) {
-      HIP_CHECK(e, vtr::launch_zero_f32((float*)e->render->accum.ptr,
-                                        e->render->width * e->render->height * 4, e->stream));
+      const uint32_t pix = e->render->width * e->render->height;
+      HIP_CHECK(e, vtr::launch_zero_f32((float*)e->render->accum.ptr, pix * 4, e->stream));
+      HIP_CHECK(e, vtr::launch_zero_f32((float*)e->render->moments.ptr, pix * 2, e->stream));
       HIP_CHECK(e, hipStreamSynchronize(e->stream));
     }
+    return VT_OK;
+  });
+}
+
+vt_status vt_capture_output(vt_engine* e, float* rgba, uint32_t width, uint32_t height, const vt_present* pr) {
+  return vt_guarded(e, [&]() -> vt_status {
+    if (!e) return VT_ERR_INVALID_ARGUMENT;
+    if (!rgba || !pr) return rerr(e, VT_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (pr->tonemap > 4) return rerr(e, VT_ERR_RANGE, "tonemap must be 0 (aces) .. 4 (none)");
+    if (!(pr->exposure >= 0.0f) || !std::isfinite(pr->exposure))
+      return rerr(e, VT_ERR_RANGE, "exposure must be finite and non-negative");
+    if (pr->color_space > 1) return rerr(e, VT_ERR_RANGE, "color_space must be 0 (srgb) or 1 (linear)");
+    if (!e->render || e->render->width != width || e->render->height != height || !e->samples_accumulated)
+      return rerr(e, VT_ERR_STATE, "no frame of this size has been rendered");
+    HIP_CHECK(e, hipSetDevice(e->device));
+    RenderState& R = *e->render;
+    const uint32_t pix = width * height;
+    HIP_CHECK(e, vtr::launch_present((const float*)R.accum.ptr, pix, e->samples_accumulated, pr->tonemap,
+                                     pr->exposure, pr->color_space, (float*)R.present.ptr, e->stream));
+    HIP_CHECK(e, hipMemcpyAsync(rgba, R.present.ptr, (size_t)pix * 16, hipMemcpyDeviceToHost, e->stream));
+    HIP_CHECK(e, hipStreamSynchronize(e->stream));
+    return VT_OK;
+  });
+}
+
+vt_status vt_capture_aux(vt_engine* e, uint32_t which, float* out, uint32_t width, uint32_t height) {
+  return vt_guarded(e, [&]() -> vt_status {
+    if (!e) return VT_ERR_INVALID_ARGUMENT;
+    if (!out) return rerr(e, VT_ERR_INVALID_ARGUMENT, "out is NULL");
+    if (which > 2) return rerr(e, VT_ERR_RANGE, "which must be 0 (normal-depth), 1 (albedo) or 2 (variance)");
+    if (!e->render || e->render->width != width || e->render->height != height || !e->samples_accumulated)
+      return rerr(e, VT_ERR_STATE, "no frame of this size has been rendered");
+    HIP_CHECK(e, hipSetDevice(e->device));
+    RenderState& R = *e->render;
+    const uint32_t pix = width * height;
+    const void* src = which == 0 ? R.aux_nd.ptr : R.aux_albedo.ptr;
+    if (which == 2) {
+      HIP_CHECK(e, vtr::launch_variance((const float*)R.moments.ptr, pix, e->samples_accumulated,
+                                        (float*)R.present.ptr, e->stream));
+      src = R.present.ptr;
+    }
+    HIP_CHECK(e, hipMemcpyAsync(out, src, (size_t)pix * 16, hipMemcpyDeviceToHost, e->stream));
+    HIP_CHECK(e, hipStreamSynchronize(e->stream));
     return VT_OK;
   });
 }

@@ -30,7 +30,22 @@ struct ShadeScene {
   // Light tree over the light slots (16 floats per node); 0 nodes = uniform.
   const float* light_tree;
   uint32_t light_tree_nodes;
+  // Analytic emitters (kLightFloats each) in light-selection order:
+  // directional, point, spot, then rect and disc areas; the mesh-area
+  // emitters follow them in the selection index, the environment comes last.
+  const float* lights;
+  uint32_t n_dir, n_point, n_spot, n_rect;
 };
+
+// Analytic emitter records (16 floats; no_shadow = castShadow false):
+//   directional: toward-light xyz, angular diameter, irradiance rgb, no_shadow
+//   point:       position xyz, no_shadow, radiance rgb, 0, cutoff distance, decay
+//   spot:        position xyz, no_shadow, axis xyz, cos outer, radiance rgb,
+//                cos inner, cutoff distance, decay
+//   rect / disc: centre xyz, no_shadow, u axis xyz, 0, v axis xyz, 0,
+//                radiance rgb, shape (0 rect with half-axes u, v; 1 disc, the
+//                affine image of the unit disc)
+constexpr uint32_t kLightFloats = 16;
 
 constexpr uint32_t kTrainSlots = 4;   // recorded vertices per self-training path
 constexpr uint32_t kTrainRec = 22;     // 16 feature floats + throughput (3) + radiance (3)
@@ -81,6 +96,12 @@ struct PathState {
   uint32_t* shadow_occ;
   // Accumulation.
   float* accum;           // 4 per pixel (rgb, unused)
+  float* moments;         // 2 per pixel: sums of sample luminance and its square
+  // First-hit auxiliary images of the latest frame (accumulateFrame,
+  // kernel.wgsl.ts:920-960): shading normal * 0.5 + 0.5 and the primary hit
+  // distance; base colour, 1 (a miss writes (0.5, 1, 0.5, 0) and zeros).
+  float* aux_nd;          // 4 per pixel
+  float* aux_albedo;      // 4 per pixel
   // Neural radiance cache.
   uint32_t* nrc_count;    // [0] cache queries, [1] training samples
   float* q_feat;          // 16 per query: features of the query vertex
@@ -98,6 +119,16 @@ hipError_t launch_shade(const FrameParams& f, const ShadeScene& sc, const PathSt
                         uint32_t vertex, uint32_t capacity, hipStream_t st);
 hipError_t launch_shadow_resolve(const PathState& s, uint32_t capacity, hipStream_t st);
 hipError_t launch_accumulate(const FrameParams& f, const PathState& s, hipStream_t st);
+// Present pass (present.wgsl.ts:58-95): running mean of `accum` (divided by
+// `samples`), exposure, tonemap operator `mode` (0 aces, 1 agx, 2 reinhard,
+// 3 linear, 4 none), sRGB OETF unless `linear_out`, clamped to [0, 65504] and
+// rounded to binary16 like the reference's rgba16float target; 4 floats per
+// pixel out (alpha 1).
+hipError_t launch_present(const float* accum, uint32_t pixels, uint32_t samples, uint32_t mode, float exposure,
+                          uint32_t linear_out, float* out, hipStream_t st);
+// Per-pixel luminance variance of the accumulated samples: max(0, E[L^2] -
+// E[L]^2) in the first of 4 floats per pixel.
+hipError_t launch_variance(const float* moments, uint32_t pixels, uint32_t samples, float* out, hipStream_t st);
 // Adds throughput x cache prediction to every queried path.
 hipError_t launch_nrc_apply(const PathState& s, uint32_t capacity, hipStream_t st);
 // Turns the recorded vertices of the training paths into (features, target) samples.

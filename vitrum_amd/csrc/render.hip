@@ -20,6 +20,7 @@
 //
 // Queues are appended with wave-aggregated atomics: a ballot of the lanes
 // that emit, one atomicAdd by the first of them, and mbcnt ranks.
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -430,8 +431,10 @@ __device__ F3 vndf_sample(F3 wo, float alpha, Rng& rng) {
   return safe_normalize(f3(alpha * nh.x, alpha * nh.y, fmaxf(1e-6f, nh.z)));
 }
 
-// measureAreaVector (common.wgsl.ts:310-362): unit normal and area.
-__device__ bool area_measure(F3 u, F3 v, F3& normal, float& area) {
+// measureAreaVector (common.wgsl.ts:310-362): unit normal and area
+// coefficient * |u x v| (0.5 for a triangle's edges, 4 for a rect's half
+// axes, pi for a disc's).
+__device__ bool area_measure(F3 u, F3 v, F3& normal, float& area, float coefficient = 0.5f) {
   const float es = fmaxf(fmaxf(fabsf(u.x), fmaxf(fabsf(u.y), fabsf(u.z))),
                          fmaxf(fabsf(v.x), fmaxf(fabsf(v.y), fabsf(v.z))));
   if (!(es > 0.0f) || es > safe_max()) return false;
@@ -441,7 +444,7 @@ __device__ bool area_measure(F3 u, F3 v, F3& normal, float& area) {
   const F3 ad = f3(av.x / cs, av.y / cs, av.z / cs);
   const float dl = __builtin_sqrtf(dot(ad, ad));
   normal = f3(ad.x / dl, ad.y / dl, ad.z / dl);
-  float a = 0.5f * (cs * dl);
+  float a = coefficient * (cs * dl);
   a = (a * es) * es;
   area = a;
   return a > 0.0f && a <= safe_max();
@@ -485,6 +488,132 @@ __device__ F3 env_cell_dir(uint32_t x, uint32_t y, uint32_t w, uint32_t h, float
   double sp, cp;
   sincos(phi, &sp, &cp);
   return f3((float)(cp * st), (float)ct, (float)(sp * st));
+}
+
+// ---- analytic emitters (kernel.wgsl.ts:903-918, 1198-1497) --------------------
+
+// cos and sin of a binary32 angle, evaluated in binary64 and rounded once (the
+// binary64 restatement rounds the same values).
+__device__ __forceinline__ void cos_sin_f(float a, float& c, float& s) {
+  double sd, cd;
+  sincos((double)a, &sd, &cd);
+  c = (float)cd;
+  s = (float)sd;
+}
+// x^y evaluated in binary64 and rounded once.
+__device__ __forceinline__ float pow_f(float x, float y) { return (float)pow((double)x, (double)y); }
+__device__ __forceinline__ F3 normalize_f(F3 v) {
+  const float l = __builtin_sqrtf(dot(v, v));
+  return f3(v.x / l, v.y / l, v.z / l);
+}
+// ptDirectionalConeIsDelta (material.wgsl.ts:208-216)
+__device__ bool cone_is_delta(float ad) {
+  if (!(ad > 0.0f)) return true;
+  float c, s;
+  cos_sin_f(ad * 0.25f, c, s);
+  return s < 7.666467e-20f;
+}
+// sampleDirectionalCone (kernel.wgsl.ts:903-918) with ptDirectionalConeSinCos
+// (material.wgsl.ts:224-235): uniform in solid angle inside the cone.
+__device__ F3 sample_directional_cone(F3 axis, float ad, Rng& rng) {
+  F3 d = safe_normalize(axis);
+  if (!cone_is_delta(ad)) {
+    const float xi1 = rand01(rng), xi2 = rand01(rng);
+    float cq, sq;
+    cos_sin_f(ad * 0.25f, cq, sq);
+    const float one_minus_cos_half = (2.0f * sq) * sq;
+    const float q = (1.0f - clampf(xi1, 0.0f, 1.0f)) * one_minus_cos_half;
+    const float sin_t = __builtin_sqrtf(fmaxf(q * (2.0f - q), 0.0f));
+    const float cos_t = __builtin_sqrtf(fmaxf(1.0f - sin_t * sin_t, 0.0f));
+    float cp, sp;
+    cos_sin_f(6.28318530718f * xi2, cp, sp);
+    const F3 tx = fabsf(d.x) > 0.9f ? f3(0, 1, 0) : f3(1, 0, 0);
+    const F3 by = normalize_f(cross(d, tx));
+    const F3 bx = cross(by, d);
+    d = normalize_f(((sin_t * cp) * bx + (sin_t * sp) * by) + cos_t * d);
+  }
+  return d;
+}
+// pointSpotDistanceAttenuation (material.wgsl.ts:2176-2204)
+__device__ float point_attenuation(float dist, float cutoff, float decay) {
+  if (!(dist > 0.0f) || dist > f32_max()) return 0.0f;
+  float att = 1.0f;
+  if (decay > 0.01f) {
+    const float den = pow_f(dist, decay);
+    if (!(den > 0.0f)) att = f32_max();
+    else if (den > f32_max()) return 0.0f;
+    else att = fminf(1.0f / den, f32_max());
+  }
+  if (cutoff > 0.0f) att = att * clampf(1.0f - pow_f(dist / cutoff, 4.0f), 0.0f, 1.0f);
+  return att;
+}
+__device__ float smoothstep_f(float e0, float e1, float x) {
+  const float t = clampf((x - e0) / (e1 - e0), 0.0f, 1.0f);
+  return t * t * (3.0f - 2.0f * t);
+}
+// concentricDiscSample (kernelCore.wgsl.ts:73-85), Shirley & Chiu 1997
+__device__ void concentric_disc(float a, float b, float& x, float& y) {
+  if (a == 0.0f && b == 0.0f) {
+    x = y = 0.0f;
+    return;
+  }
+  float cr, cphi;
+  if (fabsf(a) >= fabsf(b)) {
+    cr = a;
+    cphi = (kPi / 4.0f) * (b / a);
+  } else {
+    cr = b;
+    cphi = kPi / 2.0f - (kPi / 4.0f) * (a / b);
+  }
+  float c, s;
+  cos_sin_f(cphi, c, s);
+  x = cr * c;
+  y = cr * s;
+}
+// intersectRectAreaLightRay (connect.wgsl.ts:395-446) with
+// solveAreaVectorCoordinates (common.wgsl.ts:370-412): distance and solid-
+// angle pdf of the analytic rect or disc light `q` along a ray, or false.
+__device__ bool intersect_rect_light(const float* q, F3 o, F3 d, float tmin, float& t_out, float& pdf_out) {
+  const F3 c = load3(q), u = load3(q + 4), v = load3(q + 8);
+  const bool disc = fabsf(q[15] - 1.0f) < 0.5f;
+  F3 ln;
+  float area;
+  if (!area_measure(u, v, ln, area, disc ? kPi : 4.0f)) return false;
+  const float denom = dot(ln, d);
+  if (denom == 0.0f) return false;
+  const float t = dot(ln, c - o) / denom;
+  if (t <= tmin) return false;
+  const F3 rel = (o + d * t) - c;
+  const float es = edge_scale(u, v);
+  const F3 su = f3(u.x / es, u.y / es, u.z / es), sv = f3(v.x / es, v.y / es, v.z / es);
+  const F3 sr = f3(rel.x / es, rel.y / es, rel.z / es);
+  const F3 an = f3(fabsf(ln.x), fabsf(ln.y), fabsf(ln.z));
+  float det, sn, tn;
+  if (an.x >= an.y && an.x >= an.z) {
+    det = su.y * sv.z - su.z * sv.y;
+    sn = sr.y * sv.z - sr.z * sv.y;
+    tn = su.y * sr.z - su.z * sr.y;
+  } else if (an.y >= an.z) {
+    det = su.z * sv.x - su.x * sv.z;
+    sn = sr.z * sv.x - sr.x * sv.z;
+    tn = su.z * sr.x - su.x * sr.z;
+  } else {
+    det = su.x * sv.y - su.y * sv.x;
+    sn = sr.x * sv.y - sr.y * sv.x;
+    tn = su.x * sr.y - su.y * sr.x;
+  }
+  if (det == 0.0f) return false;
+  const float uc = sn / det, vc = tn / det;
+  if (!(fabsf(uc) <= f32_max()) || !(fabsf(vc) <= f32_max())) return false;
+  const bool inside = disc ? (uc * uc + vc * vc <= 1.0f) : (fabsf(uc) <= 1.0f && fabsf(vc) <= 1.0f);
+  if (!inside) return false;
+  const float cl = fmaxf(dot(ln, f3(-d.x, -d.y, -d.z)), 0.0f);
+  if (cl <= 0.0f) return false;
+  const float pdf = area_to_solid_angle(t, cl, area, es);
+  if (!(pdf > 0.0f)) return false;
+  t_out = t;
+  pdf_out = pdf;
+  return true;
 }
 
 // ---- wave queue ------------------------------------------------------------
@@ -665,9 +794,31 @@ __global__ __launch_bounds__(256) void shade_kernel(FrameParams f, ShadeScene sc
       const float* inst_rec = miss ? nullptr : sc.instances + (size_t)inst * kInstFloats;
       if (!miss) m = load_mat(sc.materials + (size_t)sc.tri_material[prim] * kMatFloats);
       if (prev_mis) {
-        if (miss) {
-          if (sc.env_present) L = L + Cn * env * power_heuristic(prev_pdf, env_pdf_dir(sc, dir));
-        } else if ((m.emit.x > 0.0f || m.emit.y > 0.0f || m.emit.z > 0.0f) && sc.emitter_count) {
+        // The closest visible area light along the previous BSDF sample
+        // (bsdfAreaLightConnectionContribution, connect.wgsl.ts:478-700):
+        // analytic rect and disc lights are intersected here (they are not in
+        // the BVH; one is visible when no surface lies before it, which the
+        // sample's own closest hit tells), a mesh emitter is the hit itself.
+        float best_t = f32_max(), best_pdf = 0.0f;
+        F3 best_le = f3(0, 0, 0);
+        if (sc.n_rect) {
+          const float4 ra = s.rays[2 * (size_t)i];
+          const F3 ro = f3(ra.x, ra.y, ra.z);
+          const uint32_t r0 = sc.n_dir + sc.n_point + sc.n_spot;
+          for (uint32_t k = 0; k < sc.n_rect; ++k) {
+            const float* q = sc.lights + (size_t)(r0 + k) * kLightFloats;
+            float tl, pl;
+            if (intersect_rect_light(q, ro, dir, f.tmin, tl, pl)) {
+              const bool visible = q[3] > 0.5f || miss || !(t_hit < fmaxf(tl - f.ray_bias * 2.0f, f.tmin));
+              if (visible && tl < best_t) {
+                best_t = tl;
+                best_pdf = pl;
+                best_le = load3(q + 12);
+              }
+            }
+          }
+        }
+        if (!miss && (m.emit.x > 0.0f || m.emit.y > 0.0f || m.emit.z > 0.0f) && sc.emitter_count) {
           const uint32_t ebase = __float_as_uint(inst_rec[33]);
           if (ebase != kMiss) {
             const float* e = sc.emitters + (size_t)(ebase + sc.tri_source[prim]) * kEmitFloats;
@@ -678,10 +829,16 @@ __global__ __launch_bounds__(256) void shade_kernel(FrameParams f, ShadeScene sc
               float cl = dot(ln, f3(-dir.x, -dir.y, -dir.z));
               cl = e[12] > 0.5f ? fabsf(cl) : fmaxf(cl, 0.0f);
               const float lpdf = area_to_solid_angle(t_hit, cl, area, edge_scale(b - a, c - a));
-              if (lpdf > 0.0f) L = L + Cn * load3(e + 9) * power_heuristic(prev_pdf, lpdf);
+              if (lpdf > 0.0f && t_hit < best_t) {
+                best_t = t_hit;
+                best_pdf = lpdf;
+                best_le = load3(e + 9);
+              }
             }
           }
         }
+        if (best_pdf > 0.0f) L = L + Cn * best_le * power_heuristic(prev_pdf, best_pdf);
+        if (miss && sc.env_present) L = L + Cn * env * power_heuristic(prev_pdf, env_pdf_dir(sc, dir));
       }
       const bool alive = (flags & 1u) != 0u;
       bool done = !alive || vertex >= f.bounce_limit;
@@ -689,6 +846,10 @@ __global__ __launch_bounds__(256) void shade_kernel(FrameParams f, ShadeScene sc
       if (!done && miss) {
         if (!prev_mis && sc.env_present) L = L + T * env;
         done = true;
+        if (vertex == 0 && p < f.pixels) {
+          *(float4*)(s.aux_nd + 4 * (size_t)p) = make_float4(0.5f, 1.0f, 0.5f, 0.0f);
+          *(float4*)(s.aux_albedo + 4 * (size_t)p) = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
       }
       if (!done) {
         // 2. Surface at the hit (shadePrologue.wgsl.ts:55-130).
@@ -708,6 +869,16 @@ __global__ __launch_bounds__(256) void shade_kernel(FrameParams f, ShadeScene sc
         const F3 nrm = front ? nw : f3(-nw.x, -nw.y, -nw.z);
         const F3 hp = load3(s.hit_pos + 3 * (size_t)i);
         const F3 wo = f3(-dir.x, -dir.y, -dir.z);
+        if (vertex == 0 && p < f.pixels) {
+          // firstHit* (shadePrologue.wgsl.ts:173-179): normal, base colour and
+          // the distance from the primary ray's origin.
+          const float4 ra = s.rays[2 * (size_t)i];
+          const F3 dv = hp - f3(ra.x, ra.y, ra.z);
+          float4* nd = (float4*)(s.aux_nd + 4 * (size_t)p);
+          *nd = make_float4(nrm.x * 0.5f + 0.5f, nrm.y * 0.5f + 0.5f, nrm.z * 0.5f + 0.5f,
+                            __builtin_sqrtf(dot(dv, dv)));
+          *(float4*)(s.aux_albedo + 4 * (size_t)p) = make_float4(m.base.x, m.base.y, m.base.z, 1.0f);
+        }
         if (!prev_mis && (front || m.two_sided)) L = L + T * m.emit;
         // Neural radiance cache: an image path that has passed a
         // diffuse-dominant vertex ends here with a cache query (its
@@ -740,7 +911,8 @@ __global__ __launch_bounds__(256) void shade_kernel(FrameParams f, ShadeScene sc
         // 3. NEE over one light picked by the light tree, or uniformly when
         //    the tree is off or declines (mediumNee.wgsl.ts:14-38;
         //    kernel.wgsl.ts:1496-1690).
-        const uint32_t n_light = sc.emitter_count + (sc.env_present ? 1u : 0u);
+        const uint32_t n_ana = sc.n_dir + sc.n_point + sc.n_spot + sc.n_rect;
+        const uint32_t n_light = n_ana + sc.emitter_count + (sc.env_present ? 1u : 0u);
         if (n_light > 0u) {
           uint32_t picked = 0xFFFFFFFFu;
           float inv_sel = 0.0f;
@@ -757,8 +929,94 @@ __global__ __launch_bounds__(256) void shade_kernel(FrameParams f, ShadeScene sc
             inv_sel = (float)n_light;
           }
           sig = mix64(sig, 0xA000000000ull | picked);
-          if (picked < sc.emitter_count) {
-            const float* e = sc.emitters + (size_t)picked * kEmitFloats;
+          // A shadow ray carries the contribution; a light with castShadow
+          // false adds it unoccluded.
+          auto queue_shadow = [&](F3 contrib, F3 origin, F3 wi, float tmax, bool no_shadow) {
+            if (no_shadow) {
+              L = L + contrib;
+              return;
+            }
+            shadow_c = contrib;
+            shadow_a = make_float4(origin.x, origin.y, origin.z, f.tmin);
+            shadow_b = make_float4(wi.x, wi.y, wi.z, tmax);
+            want_shadow = true;
+          };
+          if (picked < n_ana) {
+            const float* q = sc.lights + (size_t)picked * kLightFloats;
+            const F3 neg_n = f3(-nrm.x, -nrm.y, -nrm.z);
+            if (picked < sc.n_dir) {
+              // directional: irradiance along a (cone-sampled) direction
+              const F3 wi = sample_directional_cone(load3(q), q[3], rng);
+              const F3 off = dot(nrm, wi) > 0.0f ? nrm : neg_n;
+              const float ndl = fabsf(dot(nrm, wi));
+              const F3 fr = bsdf_eval(m, nrm, wo, wi);
+              queue_shadow(T * fr * load3(q + 4) * (ndl * inv_sel), hp + off * f.ray_bias, wi, f32_max(),
+                           q[7] > 0.5f);
+            } else if (picked < sc.n_dir + sc.n_point + sc.n_spot) {
+              // point / spot: delta position, authored decay and cutoff
+              const bool spot = picked >= sc.n_dir + sc.n_point;
+              const float cutoff = q[spot ? 12 : 8], decay = q[spot ? 13 : 9];
+              const F3 tl = load3(q) - hp;
+              const float dist = safe_length(tl);
+              if (dist > 0.0f && !(cutoff > 0.0f && dist > cutoff)) {
+                const F3 wi = f3(tl.x / dist, tl.y / dist, tl.z / dist);
+                float soft = 1.0f;
+                bool lit = true;
+                if (spot) {
+                  const F3 axis = safe_normalize(load3(q + 4));
+                  const float cos_outer = q[7], cos_inner = q[11];
+                  const float cone_cos = dot(f3(-wi.x, -wi.y, -wi.z), axis);
+                  lit = cone_cos >= cos_outer;
+                  soft = smoothstep_f(cos_outer, fmaxf(cos_inner, cos_outer + 1e-6f), cone_cos);
+                }
+                if (lit) {
+                  const F3 off = dot(nrm, wi) > 0.0f ? nrm : neg_n;
+                  const float ndl = fabsf(dot(nrm, wi));
+                  const F3 fr = bsdf_eval(m, nrm, wo, wi);
+                  const float att = point_attenuation(dist, cutoff, decay);
+                  const F3 rad = load3(q + (spot ? 8 : 4));
+                  queue_shadow(T * fr * rad * (ndl * soft * att * inv_sel), hp + off * f.ray_bias, wi,
+                               fmaxf(dist - f.ray_bias * 2.0f, f.tmin), q[3] > 0.5f);
+                }
+              }
+            } else {
+              // rect / disc area: a point on the emitter, area measure to
+              // solid angle, MIS against the BSDF
+              const F3 c = load3(q), u = load3(q + 4), v = load3(q + 8);
+              const bool disc = fabsf(q[15] - 1.0f) < 0.5f;
+              F3 ln;
+              float area;
+              const bool ok = area_measure(u, v, ln, area, disc ? kPi : 4.0f);
+              const float xi1 = rand01(rng), xi2 = rand01(rng);
+              F3 lp;
+              if (disc) {
+                float dx, dy;
+                concentric_disc(xi1 * 2.0f - 1.0f, xi2 * 2.0f - 1.0f, dx, dy);
+                lp = (c + u * dx) + v * dy;
+              } else {
+                lp = (c + u * (xi1 * 2.0f - 1.0f)) + v * (xi2 * 2.0f - 1.0f);
+              }
+              const F3 tl = lp - hp;
+              const float dist = safe_length(tl);
+              const F3 wi = safe_normalize(tl);
+              const float ndl = fabsf(dot(nrm, wi));
+              if (dist > 0.0f && ndl > 0.0f && ok) {
+                const float cl = fmaxf(dot(ln, f3(-wi.x, -wi.y, -wi.z)), 0.0f);
+                if (cl > 0.0f) {
+                  const float lpdf = area_to_solid_angle(dist, cl, area, edge_scale(u, v));
+                  if (lpdf > 0.0f) {
+                    const F3 fr = bsdf_eval(m, nrm, wo, wi);
+                    const float bpdf = bsdf_pdf(m, nrm, wo, wi);
+                    const float w = power_heuristic(lpdf, bpdf);
+                    const F3 off = dot(nrm, wi) > 0.0f ? nrm : neg_n;
+                    queue_shadow(T * fr * load3(q + 12) * (ndl * w / lpdf * inv_sel), hp + off * f.ray_bias, wi,
+                                 fmaxf(dist - f.ray_bias * 2.0f, f.tmin), q[3] > 0.5f);
+                  }
+                }
+              }
+            }
+          } else if (picked - n_ana < sc.emitter_count) {
+            const float* e = sc.emitters + (size_t)(picked - n_ana) * kEmitFloats;
             const F3 a = load3(e), b = load3(e + 3), c = load3(e + 6);
             const float r1 = rand01(rng), r2 = rand01(rng);
             const float su = __builtin_sqrtf(r1), uu = 1.0f - su, vv = r2 * su, ww = (1.0f - uu) - vv;
@@ -955,9 +1213,78 @@ __global__ void accumulate_kernel(FrameParams f, PathState s) {
   if (p >= f.width * f.height) return;
   const float* L = s.radiance + 3 * (size_t)p;
   float* a = s.accum + 4 * (size_t)p;
-  a[0] = a[0] + fmaxf(L[0], 0.0f);
-  a[1] = a[1] + fmaxf(L[1], 0.0f);
-  a[2] = a[2] + fmaxf(L[2], 0.0f);
+  const F3 c = f3(fmaxf(L[0], 0.0f), fmaxf(L[1], 0.0f), fmaxf(L[2], 0.0f));
+  a[0] = a[0] + c.x;
+  a[1] = a[1] + c.y;
+  a[2] = a[2] + c.z;
+  // variance moments (accumulateFrame, kernel.wgsl.ts:936-940)
+  const float l = luminance(c);
+  float* mo = s.moments + 2 * (size_t)p;
+  mo[0] = mo[0] + l;
+  mo[1] = mo[1] + l * l;
+}
+
+// ---- present pass (present.wgsl.ts; tonemap.wgsl.ts) -----------------------
+
+__device__ float tm_exposure(float c, float exposure) {  // vt_safeExposureScalar
+  if (c != c || !(exposure > 0.0f)) return 0.0f;
+  const float fmax = 3.402823466e38f;
+  const float mag = fminf(fabsf(c) * fminf(exposure, fmax), fmax);
+  return c < 0.0f ? -mag : mag;
+}
+__device__ float tm_aces(float x) {  // Narkowicz 2015, numerator and denominator over v^2
+  const float v = fmaxf(x, 0.0f);
+  const float inv = 1.0f / fmaxf(v, 1e-20f);
+  const float curved = (2.51f + 0.03f * inv) / ((2.43f + 0.59f * inv) + (0.14f * inv) * inv);
+  return v == 0.0f ? 0.0f : fminf(fmaxf(curved, 0.0f), 1.0f);
+}
+__device__ float tm_agx(float c) {
+  const float v = fmaxf(c, 1e-6f);
+  const float lx = fminf(fmaxf((log2f(v) + 12.47393f) / (12.47393f + 4.026069f), 0.0f), 1.0f);
+  const float x2 = lx * lx, x4 = x2 * x2;
+  const float y = (((((15.5f * x4 * x2 - 40.14f * x4 * lx) + 31.96f * x4) - 6.868f * x2 * lx) + 0.4298f * x2) +
+                   0.1191f * lx) - 0.00232f;
+  return fminf(fmaxf(y, 0.0f), 1.0f);
+}
+__device__ float tm_apply(float c, uint32_t mode, float exposure) {  // vitrumTonemap
+  const float x = tm_exposure(c, exposure);
+  if (mode == 1u) return tm_agx(x);
+  if (mode == 2u) {
+    const float v = fmaxf(x, 0.0f);
+    return v / (1.0f + v);
+  }
+  if (mode == 3u) return fminf(fmaxf(x, 0.0f), 1.0f);
+  if (mode == 4u) return x;
+  return tm_aces(x);
+}
+__device__ float srgb_oetf(float c) {  // vt_linearToSrgb (IEC 61966-2-1)
+  const float v = fmaxf(c, 0.0f);
+  return v <= 0.0031308f ? v * 12.92f : 1.055f * powf(v, 1.0f / 2.4f) - 0.055f;
+}
+
+__global__ void present_kernel(const float* accum, uint32_t pixels, float inv_samples, uint32_t mode,
+                               float exposure, uint32_t linear_out, float* out) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= pixels) return;
+  const float4 a = *(const float4*)(accum + 4 * (size_t)p);
+  const float hdr[3] = {a.x * inv_samples, a.y * inv_samples, a.z * inv_samples};
+  float o[3];
+  for (int k = 0; k < 3; ++k) {
+    float t = tm_apply(fmaxf(hdr[k], 0.0f), mode, exposure);
+    if (!linear_out) t = srgb_oetf(t);
+    t = fminf(fmaxf(t, 0.0f), 65504.0f);
+    o[k] = __half2float(__float2half_rn(t));  // the rgba16float store
+  }
+  *(float4*)(out + 4 * (size_t)p) = make_float4(o[0], o[1], o[2], 1.0f);
+}
+
+__global__ void variance_kernel(const float* moments, uint32_t pixels, float samples, float* out) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= pixels) return;
+  const float count = fmaxf(samples, 1.0f);
+  const float mean = moments[2 * (size_t)p] / count;
+  const float var = fmaxf(0.0f, moments[2 * (size_t)p + 1] / count - mean * mean);
+  *(float4*)(out + 4 * (size_t)p) = make_float4(var, 0.0f, 0.0f, 0.0f);
 }
 
 // Promotes the next extension queue to the current one and clears the
@@ -1013,6 +1340,23 @@ hipError_t launch_nrc_targets(const FrameParams& f, const PathState& s, hipStrea
 hipError_t launch_accumulate(const FrameParams& f, const PathState& s, hipStream_t st) {
   const uint32_t n = f.width * f.height;
   hipLaunchKernelGGL(accumulate_kernel, dim3((n + 255) / 256), dim3(256), 0, st, f, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_present(const float* accum, uint32_t pixels, uint32_t samples, uint32_t mode, float exposure,
+                          uint32_t linear_out, float* out, hipStream_t st) {
+  if (pixels == 0) return hipSuccess;
+  // accum.xyz / max(accum.w, 1) of the reference's running sum
+  const float inv = 1.0f / (float)(samples > 0 ? samples : 1u);
+  hipLaunchKernelGGL(present_kernel, dim3((pixels + 255) / 256), dim3(256), 0, st, accum, pixels, inv, mode,
+                     exposure, linear_out, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_variance(const float* moments, uint32_t pixels, uint32_t samples, float* out, hipStream_t st) {
+  if (pixels == 0) return hipSuccess;
+  hipLaunchKernelGGL(variance_kernel, dim3((pixels + 255) / 256), dim3(256), 0, st, moments, pixels,
+                     (float)samples, out);
   return hipGetLastError();
 }
 

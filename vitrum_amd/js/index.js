@@ -142,10 +142,70 @@ function sceneBlob(scene, opts) {
     materials,
     environment: environmentOf(scene.environment),
     sceneRadius: opts && opts.sceneRadius ? opts.sceneRadius : 0,
+    emitters: emittersOf(scene.emitters),
   };
 }
 
+// Scene.emitters (core/src/scene/emitters.ts:7-114): the analytic kinds, in
+// scene order (the library packs them by kind as pt-webgpu does).  Mesh-area
+// emitters are the emissive materials of mesh primitives.
+const EMITTER_KINDS = { directional: 0, point: 1, spot: 2, 'rect-area': 3, 'disc-area': 4 };
+
+function emittersOf(list) {
+  if (list == null) return [];
+  if (!Array.isArray(list)) throw new TypeError('scene.emitters must be an array');
+  const out = [];
+  list.forEach((e, i) => {
+    if (e == null || typeof e !== 'object') throw new TypeError(`scene.emitters[${i}] must be an object`);
+    if (e.kind === 'mesh-area') return;  // lit through the referenced mesh's emissive material
+    if (!(e.kind in EMITTER_KINDS)) throw new RangeError(`scene.emitters[${i}].kind '${e.kind}' is not supported`);
+    const vec = (v, name, def) => {
+      if (v == null) return def;
+      if (v.length !== 3 || !Array.prototype.every.call(v, Number.isFinite)) {
+        throw new TypeError(`scene.emitters[${i}].${name} must be three finite numbers`);
+      }
+      return [v[0], v[1], v[2]];
+    };
+    const num = (v, name, def) => {
+      if (v == null) return def;
+      if (!Number.isFinite(v)) throw new TypeError(`scene.emitters[${i}].${name} must be a finite number`);
+      return v;
+    };
+    out.push({
+      kind: EMITTER_KINDS[e.kind],
+      color: vec(e.color, 'color', [1, 1, 1]),
+      intensity: num(e.intensity, 'intensity', 1),
+      castShadow: e.castShadow !== false,
+      position: vec(e.position, 'position', [0, 0, 0]),
+      direction: vec(e.direction, 'direction', [0, -1, 0]),
+      normal: vec(e.normal, 'normal', [0, -1, 0]),
+      uAxis: vec(e.uAxis, 'uAxis', [0, 0, 0]),
+      vAxis: vec(e.vAxis, 'vAxis', [0, 0, 0]),
+      radius: num(e.radius, 'radius', 0),
+      angle: num(e.angle, 'angle', 0),
+      penumbra: num(e.penumbra, 'penumbra', 0),
+      distance: num(e.distance, 'distance', 0),
+      decay: num(e.decay, 'decay', 2),
+      angularDiameter: num(e.angularDiameter, 'angularDiameter', 0),
+    });
+  });
+  return out;
+}
+
 const SKIPPED = Object.freeze({ kind: 'skipped', samplesAccumulated: 0, isConverged: false });
+
+// FrameQualitySettings.tonemap / exposure / outputColorSpace (core/src/
+// frame.ts:32-105): defaults aces, 1, srgb (present.wgsl.ts:10-11).
+const TONEMAP_MODES = { aces: 0, agx: 1, reinhard: 2, linear: 3, none: 4 };
+function presentOf(q) {
+  const tm = q.tonemap == null ? 'aces' : q.tonemap;
+  if (!(tm in TONEMAP_MODES)) throw new RangeError(`quality.tonemap '${tm}' is not supported`);
+  const exposure = q.exposure == null ? 1 : q.exposure;
+  if (!Number.isFinite(exposure) || exposure < 0) throw new RangeError('quality.exposure must be finite and >= 0');
+  const cs = q.outputColorSpace == null ? 'srgb' : q.outputColorSpace;
+  if (cs !== 'srgb' && cs !== 'linear') throw new RangeError(`quality.outputColorSpace '${cs}' is not supported`);
+  return { tonemap: TONEMAP_MODES[tm], exposure, colorSpace: cs === 'srgb' ? 0 : 1 };
+}
 
 class HipPathTracerEngine {
   constructor(native, handle, opts) {
@@ -194,12 +254,6 @@ class HipPathTracerEngine {
   setScene(scene) {
     this._live();
     const blob = sceneBlob(scene, this._opts);
-    if (scene.emitters && scene.emitters.length) {
-      this._emit('warning', {
-        code: 'unsupported-emitter', backend: 'pt-hip', phase: 'setScene',
-        message: 'pt-hip lights scenes with emissive meshes and the environment; scene.emitters are ignored',
-      });
-    }
     this._n.setScene(this._h, blob);
     this._scene = scene;
     this._samples = 0;
@@ -246,6 +300,7 @@ class HipPathTracerEngine {
     if (inv == null) throw new RangeError('projMatrix * viewMatrix is not invertible');
     const q = input.quality || {};
     const target = q.samplesTarget == null ? 0 : q.samplesTarget;
+    this._present = presentOf(q);
     if (this._size && (this._size[0] !== width || this._size[1] !== height)) this._samples = 0;
     this._size = [width, height];
     if (target && this._samples >= target) {
@@ -270,6 +325,9 @@ class HipPathTracerEngine {
     const out = {
       kind: 'rendered',
       primaryRadiance: this._texture(),
+      normalDepth: this._texture('normalDepth'),
+      albedo: this._texture('albedo'),
+      variance: this._texture('variance'),
       samplesAccumulated: this._samples,
       isConverged: target > 0 && this._samples >= target,
     };
@@ -278,9 +336,11 @@ class HipPathTracerEngine {
     return out;
   }
 
-  // BackendTexture branded 'hip': the engine's accumulation image.
-  _texture() {
-    return Object.freeze({ backend: 'hip', engine: this, width: this._size[0], height: this._size[1], format: 'rgba32float' });
+  // BackendTexture branded 'hip': one of the engine's images (the
+  // accumulation by default; read with captureFrame / captureAux).
+  _texture(image) {
+    return Object.freeze({ backend: 'hip', engine: this, image: image || 'primaryRadiance', width: this._size[0],
+      height: this._size[1], format: 'rgba32float' });
   }
 
   reset() {
@@ -317,24 +377,41 @@ class HipPathTracerEngine {
   onWarning(cb) { return this._on('warning', cb); }
 
   // captureFrame (core/src/engine/index.ts:466): row-major, top-left origin,
-  // linear RGBA float32 running mean.
+  // float32 RGBA.  colorSpace 'linear' (default): the running mean;
+  // 'output': the present pass of the last frame's quality settings
+  // (tonemap, exposure, outputColorSpace; present.wgsl.ts), binary16 values.
   captureFrame(opts) {
     return new Promise((resolve, reject) => {
       try {
         this._live();
-        if (opts && opts.colorSpace && opts.colorSpace !== 'linear') {
-          throw new RangeError("pt-hip captures the linear accumulation only (colorSpace 'linear')");
-        }
+        const cs = opts && opts.colorSpace != null ? opts.colorSpace : 'linear';
+        if (cs !== 'linear' && cs !== 'output') throw new RangeError(`captureFrame colorSpace '${cs}' is not supported`);
         if (!this._size || this._samples === 0) {
           resolve(null);
           return;
         }
         const [w, h] = this._size;
+        if (cs === 'output') {
+          const p = this._present;
+          resolve({ width: w, height: h, rgba: this._n.captureOutput(this._h, w, h, p.tonemap, p.exposure, p.colorSpace) });
+          return;
+        }
         resolve({ width: w, height: h, rgba: this._n.capture(this._h, w, h) });
       } catch (err) {
         reject(err);
       }
     });
+  }
+
+  // The auxiliary images behind FrameRendered.normalDepth / albedo /
+  // variance (accumulateFrame, kernel.wgsl.ts:920-960): float32 RGBA.
+  captureAux(name) {
+    this._live();
+    const which = { normalDepth: 0, albedo: 1, variance: 2 }[name];
+    if (which == null) throw new RangeError(`captureAux: unknown image '${name}'`);
+    if (!this._size || this._samples === 0) return null;
+    const [w, h] = this._size;
+    return { width: w, height: h, rgba: this._n.captureAux(this._h, which, w, h) };
   }
 
   stats() {

@@ -36,6 +36,24 @@ assert.ok(blob.primitives[0].indices instanceof Uint32Array);
 assert.strictEqual(blob.environment.kind, 0);
 assert.throws(() => pt.sceneBlob({ primitives: [{ kind: 'sphere' }] }), RangeError);
 assert.throws(() => pt.sceneBlob({ primitives: [Object.assign({}, quad, { transform: [1, 2] })] }), TypeError);
+// Scene.emitters: analytic kinds become emitter records, mesh-area entries
+// light through their mesh's emissive material, unknown kinds are rejected.
+const lit = pt.sceneBlob({
+  primitives: [quad],
+  emitters: [
+    { kind: 'point', id: 'p', color: [1, 1, 1], intensity: 5, position: [0, 2, 0] },
+    { kind: 'rect-area', id: 'r', color: [1, 1, 1], intensity: 2, position: [0, 1, 0], uAxis: [1, 0, 0], vAxis: [0, 0, 1] },
+    { kind: 'mesh-area', id: 'm', meshId: 'q', color: [1, 1, 1], intensity: 1 },
+    { kind: 'spot', id: 's', color: [1, 1, 1], intensity: 1, position: [0, 1, 0], direction: [0, -1, 0], angle: 0.5, castShadow: false },
+  ],
+  environment: { kind: 'none' },
+});
+assert.strictEqual(lit.emitters.length, 3);
+assert.deepStrictEqual(lit.emitters.map((e) => e.kind), [1, 3, 2]);
+assert.strictEqual(lit.emitters[0].decay, 2);
+assert.strictEqual(lit.emitters[2].castShadow, false);
+assert.throws(() => pt.sceneBlob({ primitives: [], emitters: [{ kind: 'laser' }] }), RangeError);
+assert.throws(() => pt.sceneBlob({ primitives: [], emitters: [{ kind: 'point', position: [0, NaN, 0] }] }), TypeError);
 const sky = pt.sceneBlob({ primitives: [], environment: { kind: 'hdri', hdri: { constant: [1, 0.5, 0.25] }, intensity: 2 } });
 assert.deepStrictEqual(sky.environment, { kind: 1, radiance: [2, 1, 0.5] });
 assert.throws(() => pt.sceneBlob({ primitives: [], environment: { kind: 'hdri', hdri: { width: 2, height: 1, data: new Float32Array([1, 1, 1, 1, 0, 0, 0, 1]) } } }), RangeError);
@@ -74,6 +92,22 @@ assert.throws(() => pt.sceneBlob({ primitives: [], environment: { kind: 'hdri', 
   assert.strictEqual(cap.rgba.length, 16 * 16 * 4);
   const center = cap.rgba[(8 * 16 + 8) * 4];
   assert.ok(center > 0 && center < 1.5, `center radiance ${center}`);
+  // colorSpace 'output': the present pass (aces + sRGB by default), in [0, 1]
+  const outCap = await eng.captureFrame({ colorSpace: 'output' });
+  assert.ok(outCap.rgba.every((x) => x >= 0 && x <= 1), 'output capture in [0, 1]');
+  assert.ok(outCap.rgba[(8 * 16 + 8) * 4] > 0);
+  const nd = eng.captureAux('normalDepth');
+  assert.ok(Math.abs(nd.rgba[(8 * 16 + 8) * 4 + 2] - 1) < 1e-6, 'quad normal +z');
+  assert.ok(nd.rgba[(8 * 16 + 8) * 4 + 3] > 2.5, 'hit distance ~3');
+  // an analytic point light over the same quad
+  const pl = await pt.createPTEngine_HIP({ device: 0 });
+  pl.setScene({ primitives: [quad], environment: { kind: 'none' },
+    emitters: [{ kind: 'point', id: 'p', color: [1, 1, 1], intensity: 4, position: [0.6, 0.3, 1] }] });
+  pl.renderFrame({ viewMatrix: view, projMatrix: proj, viewport: { width: 16, height: 16, devicePixelRatio: 1 },
+    frameIndex: 0, frameSeed: 7, quality: { bounces: 2 } });
+  const pc = await pl.captureFrame();
+  assert.ok(pc.rgba[(8 * 16 + 8) * 4] > 0, 'point light lights the quad');
+  pl.dispose();
   // traceClosest through the addon: a ray straight at the triangle.
   const hits = new Float32Array(addon.traceClosest(eng._h, new Float32Array([0.6, 0.3, 1, 0, 0, 0, -1, 1e30])));
   assert.ok(Math.abs(hits[2] - 1) < 1e-6, `t = ${hits[2]}`);

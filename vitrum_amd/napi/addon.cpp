@@ -314,6 +314,55 @@ napi_value SetScene(napi_env env, napi_callback_info info) {
     if (!get_vec(env, envv, "radiance", 3, zero, sd.environment.radiance)) return nullptr;
   }
   if (!get_f32(env, desc, "sceneRadius", 0.0f, &sd.scene_radius)) return nullptr;
+  // emitters: [{kind, color, intensity, castShadow, position, direction, normal,
+  //             uAxis, vAxis, radius, angle, penumbra, distance, decay, angularDiameter}]
+  std::vector<vt_emitter> ve;
+  bool has_em = false;
+  napi_has_named_property(env, desc, "emitters", &has_em);
+  if (has_em) {
+    napi_value ems;
+    NAPI_OK(napi_get_named_property(env, desc, "emitters", &ems));
+    bool em_arr = false;
+    napi_is_array(env, ems, &em_arr);
+    if (!em_arr) {
+      throw_type(env, "'emitters' must be an array");
+      return nullptr;
+    }
+    uint32_t ne = 0;
+    napi_get_array_length(env, ems, &ne);
+    ve.resize(ne);
+    const float down[3] = {0, -1, 0};
+    for (uint32_t i = 0; i < ne; ++i) {
+      napi_value e;
+      NAPI_OK(napi_get_element(env, ems, i, &e));
+      vt_emitter& q = ve[i];
+      std::memset(&q, 0, sizeof q);
+      const float one[3] = {1, 1, 1};
+      if (!get_u32(env, e, "kind", 0, &q.kind)) return nullptr;
+      if (!get_vec(env, e, "color", 3, one, q.color)) return nullptr;
+      if (!get_f32(env, e, "intensity", 1.0f, &q.intensity)) return nullptr;
+      bool cs = true, has_cs = false;
+      napi_has_named_property(env, e, "castShadow", &has_cs);
+      if (has_cs) {
+        napi_value v;
+        if (napi_get_named_property(env, e, "castShadow", &v) == napi_ok) napi_get_value_bool(env, v, &cs);
+      }
+      q.cast_shadow = cs ? 1u : 0u;
+      if (!get_vec(env, e, "position", 3, zero, q.position)) return nullptr;
+      if (!get_vec(env, e, "direction", 3, down, q.direction)) return nullptr;
+      if (!get_vec(env, e, "normal", 3, down, q.normal)) return nullptr;
+      if (!get_vec(env, e, "uAxis", 3, zero, q.u_axis)) return nullptr;
+      if (!get_vec(env, e, "vAxis", 3, zero, q.v_axis)) return nullptr;
+      if (!get_f32(env, e, "radius", 0.0f, &q.radius)) return nullptr;
+      if (!get_f32(env, e, "angle", 0.0f, &q.angle)) return nullptr;
+      if (!get_f32(env, e, "penumbra", 0.0f, &q.penumbra)) return nullptr;
+      if (!get_f32(env, e, "distance", 0.0f, &q.distance)) return nullptr;
+      if (!get_f32(env, e, "decay", 2.0f, &q.decay)) return nullptr;
+      if (!get_f32(env, e, "angularDiameter", 0.0f, &q.angular_diameter)) return nullptr;
+    }
+  }
+  sd.emitters = ve.data();
+  sd.emitter_count = (uint32_t)ve.size();
   if (throw_status(env, h->eng, vt_set_scene(h->eng, &sd))) return nullptr;
   return nullptr;
 }
@@ -361,6 +410,52 @@ napi_value Capture(napi_env env, napi_callback_info info) {
   napi_value ab, out;
   NAPI_OK(napi_create_arraybuffer(env, floats * 4, &data, &ab));
   if (throw_status(env, h->eng, vt_capture(h->eng, static_cast<float*>(data), w, hh))) return nullptr;
+  NAPI_OK(napi_create_typedarray(env, napi_float32_array, floats, ab, 0, &out));
+  return out;
+}
+
+// captureOutput(handle, width, height, tonemap, exposure, colorSpace) ->
+// Float32Array: the present pass (tonemap, OETF, binary16 store).
+napi_value CaptureOutput(napi_env env, napi_callback_info info) {
+  auto a = args_of(env, info, 6);
+  Handle* h = handle_of(env, a[0]);
+  if (!h) return nullptr;
+  uint32_t w = 0, hh = 0;
+  vt_present pr{};
+  double ex = 1.0;
+  if (napi_get_value_uint32(env, a[1], &w) != napi_ok || napi_get_value_uint32(env, a[2], &hh) != napi_ok ||
+      napi_get_value_uint32(env, a[3], &pr.tonemap) != napi_ok || napi_get_value_double(env, a[4], &ex) != napi_ok ||
+      napi_get_value_uint32(env, a[5], &pr.color_space) != napi_ok) {
+    throw_type(env, "captureOutput(handle, width, height, tonemap, exposure, colorSpace)");
+    return nullptr;
+  }
+  pr.exposure = (float)ex;
+  const size_t floats = (size_t)w * hh * 4;
+  void* data = nullptr;
+  napi_value ab, out;
+  NAPI_OK(napi_create_arraybuffer(env, floats * 4, &data, &ab));
+  if (throw_status(env, h->eng, vt_capture_output(h->eng, static_cast<float*>(data), w, hh, &pr))) return nullptr;
+  NAPI_OK(napi_create_typedarray(env, napi_float32_array, floats, ab, 0, &out));
+  return out;
+}
+
+// captureAux(handle, which, width, height) -> Float32Array: 0 normal-depth,
+// 1 albedo, 2 variance.
+napi_value CaptureAux(napi_env env, napi_callback_info info) {
+  auto a = args_of(env, info, 4);
+  Handle* h = handle_of(env, a[0]);
+  if (!h) return nullptr;
+  uint32_t which = 0, w = 0, hh = 0;
+  if (napi_get_value_uint32(env, a[1], &which) != napi_ok || napi_get_value_uint32(env, a[2], &w) != napi_ok ||
+      napi_get_value_uint32(env, a[3], &hh) != napi_ok) {
+    throw_type(env, "captureAux(handle, which, width, height)");
+    return nullptr;
+  }
+  const size_t floats = (size_t)w * hh * 4;
+  void* data = nullptr;
+  napi_value ab, out;
+  NAPI_OK(napi_create_arraybuffer(env, floats * 4, &data, &ab));
+  if (throw_status(env, h->eng, vt_capture_aux(h->eng, which, static_cast<float*>(data), w, hh))) return nullptr;
   NAPI_OK(napi_create_typedarray(env, napi_float32_array, floats, ab, 0, &out));
   return out;
 }
@@ -529,7 +624,8 @@ napi_value Init(napi_env env, napi_value exports) {
              {"capture", Capture},       {"reset", Reset},
              {"stats", Stats},           {"traceClosest", TraceClosest},
              {"traceAny", TraceAny},     {"invertMat4", InvertMat4},
-             {"nrcConfig", NrcConfig},   {"version", Version}};
+             {"nrcConfig", NrcConfig},   {"version", Version},
+             {"captureOutput", CaptureOutput}, {"captureAux", CaptureAux}};
   for (const auto& f : fns) {
     napi_value fn;
     if (napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.cb, nullptr, &fn) != napi_ok) return nullptr;

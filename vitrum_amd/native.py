@@ -77,16 +77,42 @@ class Environment(C.Structure):
     _fields_ = [("kind", C.c_uint32), ("radiance", C.c_float * 3)]
 
 
+class Emitter(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("color", C.c_float * 3), ("intensity", C.c_float),
+                ("cast_shadow", C.c_uint32), ("position", C.c_float * 3), ("direction", C.c_float * 3),
+                ("normal", C.c_float * 3), ("u_axis", C.c_float * 3), ("v_axis", C.c_float * 3),
+                ("radius", C.c_float), ("angle", C.c_float), ("penumbra", C.c_float), ("distance", C.c_float),
+                ("decay", C.c_float), ("angular_diameter", C.c_float)]
+
+
+EMITTER_KINDS = {"directional": 0, "point": 1, "spot": 2, "rect-area": 3, "disc-area": 4}
+
+
+def _emitter(e) -> Emitter:
+    """A SceneEmitter dict (packages/core/src/scene/emitters.ts field names)."""
+    v3 = lambda k: (C.c_float * 3)(*e.get(k, (0.0, 0.0, 0.0)))  # noqa: E731
+    return Emitter(EMITTER_KINDS[e["kind"]], v3("color"), float(e.get("intensity", 1.0)),
+                   0 if e.get("castShadow", True) is False else 1, v3("position"), v3("direction"), v3("normal"),
+                   v3("uAxis"), v3("vAxis"), float(e.get("radius", 0.0)), float(e.get("angle", 0.0)),
+                   float(e.get("penumbra", 0.0)), float(e.get("distance", 0.0)), float(e.get("decay", 2.0)),
+                   float(e.get("angularDiameter", 0.0)))
+
+
 class SceneDesc(C.Structure):
     _fields_ = [("primitives", C.POINTER(Primitive)), ("primitive_count", C.c_uint32),
                 ("materials", C.POINTER(Material)), ("material_count", C.c_uint32),
-                ("environment", Environment), ("scene_radius", C.c_float)]
+                ("environment", Environment), ("scene_radius", C.c_float),
+                ("emitters", C.POINTER(Emitter)), ("emitter_count", C.c_uint32)]
 
 
 class Options(C.Structure):
     _fields_ = [("max_bounces", C.c_uint32), ("max_samples_per_pixel", C.c_uint32),
                 ("sampling", C.c_uint32), ("light_selection", C.c_uint32), ("bvh_traversal", C.c_uint32),
                 ("reserved", C.c_uint32 * 3)]
+
+
+class Present(C.Structure):
+    _fields_ = [("tonemap", C.c_uint32), ("exposure", C.c_float), ("color_space", C.c_uint32)]
 
 
 class Frame(C.Structure):
@@ -129,6 +155,8 @@ def lib():
             getattr(L, name).argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]
         L.vt_trace_closest_counts.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
                                               C.c_void_p]
+        L.vt_capture_output.argtypes = [C.c_void_p, F32P, C.c_uint32, C.c_uint32, C.c_void_p]
+        L.vt_capture_aux.argtypes = [C.c_void_p, C.c_uint32, F32P, C.c_uint32, C.c_uint32]
         _LIB = L
     return _LIB
 
@@ -281,16 +309,7 @@ class Engine:
         return self._h
 
     def set_scene(self, scene):
-        prims = []
-        mats = []
-        for i, p in enumerate(scene["primitives"]):
-            prims.append(dict(p, material_id=i))
-            mats.append(_material(p.get("material", {})))
-        arr, _keep = _prim_structs(prims)
-        marr = (Material * max(1, len(mats)))(*mats)
-        sky = scene.get("sky", [0.0, 0.0, 0.0])
-        env = Environment(1 if any(sky) else 0, (C.c_float * 3)(*sky))
-        desc = SceneDesc(arr, len(prims), marr, len(mats), env, float(scene.get("scene_radius", 0.0)))
+        desc, _keep = _scene_desc(scene)
         check(lib().vt_set_scene(self._h, C.byref(desc)), self._h)
 
     def trace_closest(self, rays):
@@ -348,6 +367,24 @@ class Engine:
     def capture(self, width, height):
         out = np.zeros((height, width, 4), np.float32)
         check(lib().vt_capture(self._h, out.ctypes.data_as(F32P), width, height), self._h)
+        return out
+
+    def capture_output(self, width, height, tonemap=0, exposure=1.0, color_space=0):
+        """captureFrame({colorSpace: 'output'}): tonemapped, OETF-encoded
+        running mean rounded to binary16 (tonemap 0 aces .. 4 none;
+        color_space 0 srgb, 1 linear)."""
+        out = np.zeros((height, width, 4), np.float32)
+        pr = Present(tonemap, exposure, color_space)
+        check(lib().vt_capture_output(self._h, out.ctypes.data_as(F32P), C.c_uint32(width), C.c_uint32(height),
+                                      C.byref(pr)), self._h)
+        return out
+
+    def capture_aux(self, which, width, height):
+        """First-hit normal-depth (0) or albedo (1) of the latest frame, or
+        the accumulated luminance variance (2); 4 floats per pixel."""
+        out = np.zeros((height, width, 4), np.float32)
+        check(lib().vt_capture_aux(self._h, C.c_uint32(which), out.ctypes.data_as(F32P), C.c_uint32(width),
+                                   C.c_uint32(height)), self._h)
         return out
 
     def capture_signatures(self, width, height):
@@ -494,8 +531,10 @@ def _scene_desc(scene):
     marr = (Material * max(1, len(mats)))(*mats)
     sky = scene.get("sky", [0.0, 0.0, 0.0])
     env = Environment(1 if any(sky) else 0, (C.c_float * 3)(*sky))
-    desc = SceneDesc(arr, len(prims), marr, len(mats), env, float(scene.get("scene_radius", 0.0)))
-    return desc, (arr, keep, marr)
+    ems = scene.get("emitters", [])
+    earr = (Emitter * max(1, len(ems)))(*[_emitter(e) for e in ems])
+    desc = SceneDesc(arr, len(prims), marr, len(mats), env, float(scene.get("scene_radius", 0.0)), earr, len(ems))
+    return desc, (arr, keep, marr, earr)
 
 
 class Batch:

@@ -28,15 +28,11 @@ def _grid_quad(origin, edge_u, edge_v, nu, nv, normal):
     ss, tt = np.meshgrid(s, t, indexing="ij")
     pts = origin + ss[..., None] * edge_u + tt[..., None] * edge_v
     pos = pts.reshape(-1, 3)
-    idx = []
-    for i in range(nu):
-        for j in range(nv):
-            a = i * (nv + 1) + j
-            b = (i + 1) * (nv + 1) + j
-            c = (i + 1) * (nv + 1) + j + 1
-            d = i * (nv + 1) + j + 1
-            idx += [a, b, c, a, c, d]
-    idx = np.asarray(idx, np.uint32).reshape(-1, 3)
+    i, j = np.meshgrid(np.arange(nu), np.arange(nv), indexing="ij")
+    a = (i * (nv + 1) + j).ravel()
+    b = ((i + 1) * (nv + 1) + j).ravel()
+    c, d = b + 1, a + 1
+    idx = np.stack([a, b, c, a, c, d], axis=1).reshape(-1, 3).astype(np.uint32)
     # orient triangles to the requested normal
     n = np.cross(edge_u, edge_v)
     if np.dot(n, normal) < 0:
@@ -274,6 +270,73 @@ def spheres_scene(instances: int = 256, seed: int = 2024, radius: float = 4.0,
             "sky": [0.3, 0.35, 0.45]}
 
 
+def finned_latlong_sphere(segments: int = 61, rings: int = 21):
+    """Unit lat-long sphere (pole fans plus quad bands: 2 * segments *
+    (rings - 1) triangles; 61 x 21 gives 2,440) with one extra fin triangle
+    standing on the equator, so the mesh has exactly 2,441 triangles.  A closed
+    two-manifold triangle mesh always has an even triangle count, so an odd
+    count needs an open or non-manifold part; the fin is that part."""
+    verts = [[0.0, 1.0, 0.0]]
+    for r in range(1, rings):
+        th = np.pi * r / rings
+        for s_ in range(segments):
+            ph = 2 * np.pi * s_ / segments
+            verts.append([np.sin(th) * np.cos(ph), np.cos(th), np.sin(th) * np.sin(ph)])
+    verts.append([0.0, -1.0, 0.0])
+    bottom = len(verts) - 1
+
+    def ring_v(r, s_):
+        return 1 + (r - 1) * segments + (s_ % segments)
+
+    tris = []
+    for s_ in range(segments):
+        tris.append((0, ring_v(1, s_ + 1), ring_v(1, s_)))
+    for r in range(1, rings - 1):
+        for s_ in range(segments):
+            a, b = ring_v(r, s_), ring_v(r, s_ + 1)
+            c, d = ring_v(r + 1, s_ + 1), ring_v(r + 1, s_)
+            tris += [(a, b, c), (a, c, d)]
+    for s_ in range(segments):
+        tris.append((bottom, ring_v(rings - 1, s_), ring_v(rings - 1, s_ + 1)))
+    eq = rings // 2
+    a, b = ring_v(eq, 0), ring_v(eq, 1)
+    verts.append([1.3, 0.0, 0.05])
+    tris.append((a, b, len(verts) - 1))
+    return np.asarray(verts, np.float64), np.asarray(tris, np.uint32)
+
+
+def large_scene(instances: int = 4096, seed: int = 77, radius: float = 4.0, lights: int = 32,
+                box: float = 252.0):
+    """Config-5-sized scene: `instances` distinct displaced 2,441-triangle
+    meshes (finned_latlong_sphere; 4,096 x 2,441 = 9,998,336 triangles) with
+    random rigid transforms uniform in a box^3 volume (box chosen so the
+    instance density matches the 256-instance scene's 100^3), emissive light
+    spheres and a constant sky.  Materials as in spheres_scene."""
+    rng = np.random.default_rng(seed)
+    base_v, base_i = finned_latlong_sphere()
+    prims = []
+    for k in range(instances):
+        v = _displace(base_v, rng) * radius
+        n = _vertex_normals(v, base_i)
+        mat = {"baseColor": rng.uniform(0.1, 0.9, 3).tolist(),
+               "roughness": float(rng.uniform(0.05, 1.0)), "metallic": 0.0}
+        xf = _mat4(_random_rotation(rng), rng.uniform(0.0, box, 3))
+        prims.append(_mesh(v, n, base_i, mat, xf))
+    light_v, light_i = octa_sphere(8, split=False)
+    for k in range(lights):
+        v = light_v * (0.5 * radius)
+        n = _vertex_normals(v, light_i)
+        col = rng.uniform(0.6, 1.0, 3)
+        mat = {"baseColor": [0.0, 0.0, 0.0], "roughness": 1.0, "metallic": 0.0,
+               "emissive": (col / col.max()).tolist(), "emissiveIntensity": 40.0}
+        xf = _mat4(np.eye(3), rng.uniform(0.0, box, 3))
+        prims.append(_mesh(v, n, light_i, mat, xf))
+    c = box * 0.5
+    camera = look_at_camera(eye=(c, c, -0.45 * box), target=(c, c, c), up=(0, 1, 0), vfov_deg=60.0)
+    return {"primitives": prims, "camera": camera, "scene_radius": box * math.sqrt(3) * 0.5 + radius,
+            "sky": [0.3, 0.35, 0.45]}
+
+
 # ---------------------------------------------------------------------------
 # Cameras and rays
 # ---------------------------------------------------------------------------
@@ -434,6 +497,21 @@ def bounce_rays(pack, rays, hits, rng_state, scene_radius):
     out["d"] = _unit_f32(wi.astype(F32))
     out["tmin"] = tmin
     out["tmax"] = TMAX_INFINITY
+    return out
+
+
+def batch_workload(count: int = 64, seed: int = 3, lo: int = 1_000, hi: int = 2_000_000):
+    """The batched-launch workload: `count` Cornell-style scenes whose
+    triangle counts are log-uniform in [lo, hi] (rounded to even; each scene
+    tessellated as cornell_scene), each with a camera resolution drawn from
+    {256^2, 512^2, 1024^2}.  Returns [(scene, resolution)]."""
+    rng = np.random.default_rng(seed)
+    tris = np.exp(rng.uniform(np.log(lo), np.log(hi), count))
+    res = rng.choice([256, 512, 1024], count)
+    out = []
+    for t, r in zip(tris, res):
+        n = max(64, int(round(t / 2)) * 2)
+        out.append((cornell_scene(n), int(r)))
     return out
 
 
