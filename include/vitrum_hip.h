@@ -98,11 +98,45 @@ typedef struct vt_material {
   uint32_t double_sided;
 } vt_material;
 
-/* Environment (packages/core/src/scene/environment.ts:16-37): kind 0 = none,
- * 1 = constant radiance (a constant-texel HDRI). */
+/* Environment (packages/core/src/scene/environment.ts:16-37).
+ *   VT_ENV_NONE:     no environment (misses are black).
+ *   VT_ENV_CONSTANT: constant radiance[] in every direction (a constant
+ *                    64 x 32 HDRI; the importance table is built for that).
+ *   VT_ENV_HDRI:     equirectangular map HdriEnvironment.hdri as linear
+ *                    binary32 texels, width x height x channels, row-major,
+ *                    row 0 at +Y (theta 0), column 0 at phi = -pi, phi
+ *                    measured from +X towards +Z (connect.wgsl.ts:126-153).
+ *                    channels 1 (grey), 2 (r, g; b = r), 3 (rgb) or 4
+ *                    (rgba, alpha ignored), as environmentMapPixels.ts:
+ *                    103-160 expands them.  intensity scales the texels
+ *                    (HdriEnvironment.intensity), rotation_y is the dome's
+ *                    counter-clockwise rotation about +Y in radians.  The map
+ *                    is copied; the pointer need not outlive vt_set_scene.  A
+ *                    map whose luminance is zero everywhere renders as none
+ *                    (environmentPacking.ts:420-440).
+ *   VT_ENV_PROCEDURAL_SKY: ProceduralSkyEnvironment (environment.ts:59-71),
+ *                    the Preetham daylight model baked on the host into a
+ *                    256 x 128 map that is then sampled as an HDRI
+ *                    (environmentPacking.ts:222-282, preethamSky.ts:161-313):
+ *                    sun_direction (normalised; [0, 1, 0] when ~0),
+ *                    turbidity (clamped to [1.5, 30]), rayleigh, mie_coefficient,
+ *                    mie_directional_g (clamped to +-0.999999) and intensity. */
+typedef enum vt_environment_kind {
+  VT_ENV_NONE = 0,
+  VT_ENV_CONSTANT = 1,
+  VT_ENV_HDRI = 2,
+  VT_ENV_PROCEDURAL_SKY = 3
+} vt_environment_kind;
+
 typedef struct vt_environment {
   uint32_t kind;
-  float radiance[3];
+  float radiance[3];     /* VT_ENV_CONSTANT */
+  const float *texels;   /* VT_ENV_HDRI: width * height * channels floats */
+  uint32_t width, height, channels;
+  float intensity;       /* VT_ENV_HDRI, VT_ENV_PROCEDURAL_SKY; >= 0 */
+  float rotation_y;      /* VT_ENV_HDRI; radians */
+  double sun_direction[3];                 /* VT_ENV_PROCEDURAL_SKY */
+  double turbidity, rayleigh, mie_coefficient, mie_directional_g;
 } vt_environment;
 
 /* Scene.emitters (packages/core/src/scene/emitters.ts:7-114): the analytic
@@ -145,6 +179,14 @@ typedef struct vt_scene_desc {
   const vt_emitter *emitters; /* analytic emitters; mesh-area light comes from emissive materials */
   uint32_t emitter_count;
 } vt_scene_desc;
+
+/* bakePreethamSkyEquirect (shared-samplers/src/preethamSky.ts:161-313) on
+ * the host, no device needed: the procedural sky of `env` (kind
+ * VT_ENV_PROCEDURAL_SKY) as width x height binary32 rgb texels (3 floats per
+ * texel into rgb_out, row 0 at +Y) and the texels' luminance x solid-angle
+ * integral.  VT_ERR_RANGE with the reason for invalid parameters. */
+vt_status vt_bake_procedural_sky(const vt_environment *env, uint32_t width, uint32_t height, float *rgb_out,
+                                 double *luminance_integral);
 
 /* ---- engine lifecycle ---------------------------------------------------- */
 

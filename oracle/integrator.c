@@ -411,14 +411,28 @@ static int intersect_rect_light(const float *q, d3 o, d3 d, double tmin, double 
 
 typedef struct {
   int present;
-  d3 radiance;
   uint32_t w, h;
-  const float *cdf; /* w*h + 1 entries, f32 */
-  const float *pdf; /* per texel, solid-angle density of the sampler */
+  const float *cdf;    /* w*h + 1 entries, f32 */
+  const float *texels; /* per texel: intensity-scaled rgb, solid-angle pdf of the sampler */
+  float rot_c, rot_s;  /* the dome's rotation about +Y */
 } env_t;
 
-/* environmentLookup's texel pdf (connect.wgsl.ts:126-153) in binary64. */
-static double env_pdf_dir(const env_t *e, d3 d) {
+/* RY(-rot) d: the unrotated map's direction for world direction d, and
+ * RY(+rot) d back (connectCore.wgsl.ts:20-44), in binary32. */
+static d3 env_to_map(const env_t *e, d3 d) {
+  float x = (float)d.x, y = (float)d.y, z = (float)d.z, c = e->rot_c, s = e->rot_s;
+  float a = c * x, b = s * z, p = s * x, q = c * z;
+  return D3((double)(a - b), (double)y, (double)(p + q));
+}
+static d3 env_to_world(const env_t *e, d3 d) {
+  float x = (float)d.x, y = (float)d.y, z = (float)d.z, c = e->rot_c, s = e->rot_s;
+  float a = c * x, b = s * z, p = -s * x, q = c * z;
+  return D3((double)(a + b), (double)y, (double)(p + q));
+}
+
+/* environmentLookup's texel (connect.wgsl.ts:126-153): angles in binary64. */
+static const float *env_lookup(const env_t *e, d3 wd) {
+  d3 d = env_to_map(e, wd);
   double phi = atan2(d.z, d.x);
   double theta = acos(d.y < -1 ? -1 : (d.y > 1 ? 1 : d.y));
   double u = phi * (0.5 / PI) + 0.5;
@@ -428,8 +442,9 @@ static double env_pdf_dir(const env_t *e, d3 d) {
   uint32_t x = (uint32_t)floor(u * e->w), y = (uint32_t)floor(v * e->h);
   if (x > e->w - 1) x = e->w - 1;
   if (y > e->h - 1) y = e->h - 1;
-  return (double)e->pdf[y * e->w + x];
+  return e->texels + 4 * ((size_t)y * e->w + x);
 }
+static d3 env_rgb(const float *t) { return D3(t[0], t[1], t[2]); }
 
 static d3 env_cell_dir(uint32_t x, uint32_t y, uint32_t w, uint32_t h, double xa, double xb) {
   double phi0 = ((double)x / w - 0.5) * (2.0 * PI), phi1 = ((double)(x + 1) / w - 0.5) * (2.0 * PI);
@@ -453,10 +468,10 @@ typedef struct orc_render_desc {
   const float *emitters;          /* 16 floats: a xyz, b xyz, c xyz, le rgb, two_sided, inst, prim, pad */
   uint32_t emitter_count;
   uint32_t env_present;
-  float env_radiance[3];
   uint32_t env_w, env_h;
   const float *env_cdf;
-  const float *env_pdf;
+  const float *env_texels;        /* 4 floats per texel: scaled rgb, pdf */
+  float env_rot[2];               /* cos, sin of the rotation about +Y */
   float inv_view_proj[16];        /* column-major, f32 */
   uint32_t width, height;
   uint32_t frame_seed;
@@ -729,7 +744,7 @@ static void render_path(const rctx *c, uint32_t px, uint32_t py, uint32_t frame_
     closest(c, o, dir, &s);
     sig = mixlog(sig, s.hit ? (((uint64_t)s.inst << 32) | s.prim) : 0xFFFFFFFFFFFFull, ev);
     if (!s.hit) {
-      if (!prev_mis && c->env.present) L = add(L, mul(T, c->env.radiance));
+      if (!prev_mis && c->env.present) L = add(L, mul(T, env_rgb(env_lookup(&c->env, dir))));
       if (aux && bounce == 0) {
         const double miss_aux[8] = {0.5, 1.0, 0.5, 0.0, 0.0, 0.0, 0.0, 0.0};
         memcpy(aux, miss_aux, sizeof miss_aux);
@@ -880,8 +895,10 @@ static void render_path(const rctx *c, uint32_t px, uint32_t py, uint32_t frame_
         }
         uint32_t idx = lo < count - 1 ? lo : count - 1;
         double xa = rand01(&rng), xb = rand01(&rng);
-        d3 wi = safe_normalize(env_cell_dir(idx % c->env.w, idx / c->env.w, c->env.w, c->env.h, xa, xb));
-        double epdf = (double)c->env.pdf[idx];
+        d3 wi = safe_normalize(env_to_world(&c->env, env_cell_dir(idx % c->env.w, idx / c->env.w, c->env.w,
+                                                                  c->env.h, xa, xb)));
+        const float *et = c->env.texels + 4 * (size_t)idx;
+        double epdf = (double)et[3];
         double ndl = fabs(dot(n, wi));
         if (ndl > 0) {
           if (visible(c, shadow_origin(hp32, n32, dot(n, wi) > 0, bias32), wi, F32_MAX)) {
@@ -889,7 +906,7 @@ static void render_path(const rctx *c, uint32_t px, uint32_t py, uint32_t frame_
             double bpdf = bsdf_pdf(m, n, wo, wi);
             double w = power_heuristic(epdf, bpdf);
             double k = ndl * w / maxd(epdf, 1e-8) * inv_sel;
-            L = add(L, scl(mul(mul(T, f), c->env.radiance), k));
+            L = add(L, scl(mul(mul(T, f), env_rgb(et)), k));
           }
         }
       }
@@ -990,8 +1007,9 @@ static void render_path(const rctx *c, uint32_t px, uint32_t py, uint32_t frame_
           L = add(L, scl(mul(mul(T_at, f), best_le), ndl * w / bpdf));
         }
         if (c->env.present && visible(c, oo, wi, F32_MAX)) {
-          double w = power_heuristic(bpdf, env_pdf_dir(&c->env, wi));
-          L = add(L, scl(mul(mul(T_at, f), c->env.radiance), ndl * w / bpdf));
+          const float *et = env_lookup(&c->env, wi);
+          double w = power_heuristic(bpdf, (double)et[3]);
+          L = add(L, scl(mul(mul(T_at, f), env_rgb(et)), ndl * w / bpdf));
         }
       }
     }
@@ -1079,11 +1097,12 @@ int orc_render(const orc_render_desc *d, uint32_t frame_index, uint32_t x0, uint
   for (uint32_t i = 0; i < d->emitter_count; ++i) order[i] = (int)i;
   c.em = em;
   c.env.present = d->env_present != 0;
-  c.env.radiance = D3(d->env_radiance[0], d->env_radiance[1], d->env_radiance[2]);
   c.env.w = d->env_w;
   c.env.h = d->env_h;
   c.env.cdf = d->env_cdf;
-  c.env.pdf = d->env_pdf;
+  c.env.texels = d->env_texels;
+  c.env.rot_c = d->env_rot[0];
+  c.env.rot_s = d->env_rot[1];
   c.bias = (double)d->ray_bias;
   c.tmin = (double)f32(c.bias * 0.1);
   /* materialAcceptsSidedHit (material.wgsl.ts:3259-3263): opaque materials

@@ -559,8 +559,9 @@ class RenderDesc(C.Structure):
         ("scene", C.POINTER(Scene)), ("normals", F32P), ("tri_material", U32P),
         ("materials", F32P), ("material_count", C.c_uint32), ("emitters", F32P),
         ("emitter_count", C.c_uint32), ("env_present", C.c_uint32),
-        ("env_radiance", C.c_float * 3), ("env_w", C.c_uint32), ("env_h", C.c_uint32),
-        ("env_cdf", F32P), ("env_pdf", F32P), ("inv_view_proj", C.c_float * 16),
+        ("env_w", C.c_uint32), ("env_h", C.c_uint32),
+        ("env_cdf", F32P), ("env_texels", F32P), ("env_rot", C.c_float * 2),
+        ("inv_view_proj", C.c_float * 16),
         ("width", C.c_uint32), ("height", C.c_uint32), ("frame_seed", C.c_uint32),
         ("max_bounces", C.c_uint32), ("ray_bias", C.c_float), ("sampling", C.c_uint32),
         ("light_tree", F32P), ("light_tree_nodes", C.c_uint32), ("aux", F64P),
@@ -832,7 +833,7 @@ def _triangle_area_f32(a, b, c):
     return area if (area > 0 and np.isfinite(area) and inv > 0 and np.isfinite(inv)) else f(0)
 
 
-def scene_light_tree(emitters, env_radiance=None, env_size=(64, 32), analytic=None):
+def scene_light_tree(emitters, env_radiance=None, env_size=(64, 32), analytic=None, env_map=None):
     """The scene's NEE light tree (emitterPacking.ts:1005-1230 leaves, then
     light_tree_build): one leaf per mesh emitter record in order -- power =
     f32 luminance of its radiance x its f32 area, bounds of its vertices,
@@ -840,7 +841,8 @@ def scene_light_tree(emitters, env_radiance=None, env_size=(64, 32), analytic=No
     (power = luminance x solid angle summed over the map, bounds = the union
     of the emitters' bounds).  None below two lights."""
     em = np.asarray(emitters, np.float32).reshape(-1, 16)
-    env_present = env_radiance is not None and any(v > 0 for v in env_radiance)
+    env = environment_table(env_radiance, env_size, env_map)
+    env_present = env is not None
     records, counts = analytic if analytic is not None else (np.zeros((0, 16), np.float32), (0, 0, 0, 0))
     n = em.shape[0] + (1 if env_present else 0) + sum(counts)
     if n < 2:
@@ -870,16 +872,7 @@ def scene_light_tree(emitters, env_radiance=None, env_size=(64, 32), analytic=No
             ln = _hypot3_js(*nrm)
             cones.append(([x / ln for x in nrm] if ln > 1e-12 else [0.0, 0.0, 0.0], 0.0, np.pi / 2))
     if env_present:
-        W, H = env_size
-        lum = 0.2126 * float(np.float32(env_radiance[0])) + 0.7152 * float(np.float32(env_radiance[1])) + \
-            0.0722 * float(np.float32(env_radiance[2]))
-        total = 0.0
-        dphi = 2.0 * math.pi / W
-        for i in range(W * H):
-            y = i // W
-            # libm cosine (the host code's), not numpy's vectorised one
-            om = dphi * (math.cos(y / H * math.pi) - math.cos((y + 1) / H * math.pi))
-            total += max(0.0, lum * om)
+        total = env["power"]
         pw = float(np.float32(total))
         env_power = (pw if pw > 0 else 2.0 ** -149) if total > 0 else 0.0
     # directional lights and the environment sit on the union of the
@@ -954,26 +947,66 @@ def represented_cdf(weights, bucket_bits=24, counts=False):
     return (cdf, buckets) if counts else cdf
 
 
-def build_constant_env(radiance, width=64, height=32):
-    """Equirect constant-radiance HDRI importance table
-    (pt-webgpu/src/scene/environmentPacking.ts:340-420): weights
-    luminance x texel solid angle, represented 24-bit CDF, texel pdf =
-    effective pmf / texel solid angle."""
-    r = [float(np.float32(v)) for v in radiance]  # the engine stores f32 radiance
-    lum = 0.2126 * r[0] + 0.7152 * r[1] + 0.0722 * r[2]
-    dphi = 2 * math.pi / width
+def _luminance(r, g, b):
+    return 0.2126 * r + 0.7152 * g + 0.0722 * b
+
+
+def environment_table(env_radiance=None, env_size=(64, 32), env_map=None):
+    """The environment's importance table and texels
+    (pt-webgpu/src/scene/environmentPacking.ts:283-440), or None for no
+    environment or a map that is black everywhere.  `env_map` = {"texels":
+    (H, W, C) float32 linear radiance, C in 1..4 as environmentMapPixels.ts:
+    138-142 expands it, "intensity", "rotation_y"}; otherwise `env_radiance`
+    is a constant [r, g, b] over an env_size map.  Weights = luminance of the
+    unscaled texel x its solid angle; represented 24-bit CDF; per texel the
+    binary32 intensity-scaled rgb and pdf = effective pmf / solid angle;
+    power = luminance x solid angle of the scaled map; rotation rounded to
+    f32 after reduction mod 2 pi (environmentRotation.ts:10-26), cos and sin
+    rounded once."""
+    if env_map is not None:
+        tex = np.asarray(env_map["texels"], np.float32)
+        if tex.ndim == 2:
+            tex = tex[:, :, None]
+        H, W, ch = tex.shape
+        rgb = np.stack([tex[:, :, 0], tex[:, :, 1] if ch > 1 else tex[:, :, 0],
+                        tex[:, :, 2] if ch > 2 else tex[:, :, 0]], axis=-1).reshape(-1, 3)
+        intensity = np.float32(env_map.get("intensity", 1.0))
+        rot = float(np.float32(math.fmod(float(np.float32(env_map.get("rotation_y", 0.0))), 2 * math.pi)))
+    else:
+        if env_radiance is None or not any(v > 0 for v in env_radiance):
+            return None
+        W, H = env_size
+        rgb = np.tile(np.asarray(env_radiance, np.float32)[:3], (W * H, 1))
+        intensity, rot = np.float32(1.0), 0.0
+    dphi = 2 * math.pi / W
     # libm cosines, as the host code's
-    rows = np.array([dphi * (math.cos(j / height * math.pi) - math.cos((j + 1) / height * math.pi))
-                     for j in range(height)])
-    omega = np.repeat(rows, width)
-    cdf = represented_cdf(np.maximum(0.0, lum * omega))
+    rows = np.array([dphi * (math.cos(j / H * math.pi) - math.cos((j + 1) / H * math.pi)) for j in range(H)])
+    omega = np.repeat(rows, W)
+    r64 = rgb.astype(np.float64)
+    weights = np.maximum(0.0, _luminance(r64[:, 0], r64[:, 1], r64[:, 2]) * omega)
+    total = float(np.sum(weights))
+    if not total > 0:
+        return None
+    scaled = (rgb * intensity).astype(np.float32)  # one binary32 product per channel
+    s64 = scaled.astype(np.float64)
+    # summed in texel order, as the host code sums
+    power = float(np.cumsum(np.maximum(0.0, _luminance(s64[:, 0], s64[:, 1], s64[:, 2]) * omega))[-1])
+    cdf = represented_cdf(weights)
     pdf = ((cdf[1:].astype(np.float64) - cdf[:-1].astype(np.float64)) / omega).astype(np.float32)
-    return cdf, pdf
+    texels = np.concatenate([scaled, pdf[:, None]], axis=1).astype(np.float32)
+    return {"w": W, "h": H, "cdf": cdf, "texels": texels, "power": power,
+            "rot": (float(np.float32(math.cos(rot))), float(np.float32(math.sin(rot))))}
+
+
+def build_constant_env(radiance, width=64, height=32):
+    """(cdf, pdf) of a constant-radiance map (environment_table)."""
+    t = environment_table(radiance, (width, height))
+    return t["cdf"], t["texels"][:, 3].copy()
 
 
 def render(pack, materials, camera_inv_vp, width, height, *, frame_seed, frame_index, max_bounces,
            ray_bias, env_radiance=None, env_size=(64, 32), tile=None, threads=None, events=False,
-           sampling=0, light_tree=True, aux=False, emitters=None):
+           sampling=0, light_tree=True, aux=False, emitters=None, env_map=None):
     """One frame (1 spp) of the binary64 integrator mirror.  Returns
     (radiance[h, w, 3] float64, signature[h, w] uint64), plus with `events`
     the decisions mixed into each signature ([h, w, 40] uint64: count, then
@@ -985,12 +1018,13 @@ def render(pack, materials, camera_inv_vp, width, height, *, frame_seed, frame_i
     em = np.ascontiguousarray(build_emitters(pack, mt))
     normals = np.ascontiguousarray(pack["normals"], np.float32)
     tri_mat = np.ascontiguousarray(pack["tri_material"], np.uint32)
-    env_present = env_radiance is not None and any(v > 0 for v in env_radiance)
+    env = environment_table(env_radiance, env_size, env_map)
+    env_present = env is not None
     if env_present:
-        cdf, pdf = build_constant_env(env_radiance, *env_size)
+        cdf, etex = env["cdf"], env["texels"]
     else:
-        cdf, pdf = np.zeros(2, np.float32), np.zeros(1, np.float32)
-    cdf, pdf = np.ascontiguousarray(cdf), np.ascontiguousarray(pdf)
+        cdf, etex = np.zeros(2, np.float32), np.zeros((1, 4), np.float32)
+    cdf, etex = np.ascontiguousarray(cdf), np.ascontiguousarray(etex)
     d = RenderDesc()
     d.scene = C.pointer(ps.struct)
     d.normals = normals.ctypes.data_as(F32P)
@@ -1001,10 +1035,10 @@ def render(pack, materials, camera_inv_vp, width, height, *, frame_seed, frame_i
     d.emitters = em.ctypes.data_as(F32P) if em.size else None
     d.emitter_count = em.shape[0]
     d.env_present = 1 if env_present else 0
-    d.env_radiance = (C.c_float * 3)(*(env_radiance or [0, 0, 0]))
-    d.env_w, d.env_h = (env_size if env_present else (0, 0))
+    d.env_w, d.env_h = ((env["w"], env["h"]) if env_present else (0, 0))
     d.env_cdf = cdf.ctypes.data_as(F32P)
-    d.env_pdf = pdf.ctypes.data_as(F32P)
+    d.env_texels = etex.ctypes.data_as(F32P)
+    d.env_rot = (C.c_float * 2)(*(env["rot"] if env_present else (1.0, 0.0)))
     d.inv_view_proj = (C.c_float * 16)(*np.asarray(camera_inv_vp, np.float32).ravel())
     d.width, d.height = width, height
     d.frame_seed = frame_seed
@@ -1015,7 +1049,7 @@ def render(pack, materials, camera_inv_vp, width, height, *, frame_seed, frame_i
     recs = np.ascontiguousarray(ana[0] if ana[0].size else np.zeros((1, 16), np.float32))
     d.lights = recs.ctypes.data_as(F32P)
     d.n_dir, d.n_point, d.n_spot, d.n_rect = ana[1]
-    tree = scene_light_tree(em, env_radiance, env_size, analytic=ana) if light_tree else None
+    tree = scene_light_tree(em, env_radiance, env_size, analytic=ana, env_map=env_map) if light_tree else None
     if tree is not None:
         tree = np.ascontiguousarray(tree, np.float32)
         d.light_tree = tree.ctypes.data_as(F32P)

@@ -25,31 +25,65 @@ def cpu_frames(scene, res, frames, bias, max_bounces=8, sampling=0, light_tree=T
         img, sig = O.render(pack, materials_of(scene), ivp, res, res, frame_seed=SEED, frame_index=f,
                             max_bounces=max_bounces, ray_bias=bias,
                             env_radiance=sky if any(sky) else None, sampling=sampling,
-                            light_tree=light_tree, tile=tile, emitters=scene.get("emitters"))
+                            light_tree=light_tree, tile=tile, emitters=scene.get("emitters"),
+                            env_map=env_map_of(scene))
         out.append(img)
         sigs.append(sig)
     return np.stack(out), np.stack(sigs)
 
 
-def plane_under_sky(albedo, sky, roughness=1.0, metallic=0.0):
-    """A 400x400 floor at y = 0 under a constant sky, seen from far above
-    through a narrow field of view (view directions within 1.5 degrees of the
-    normal)."""
+def sky_bake(sky):
+    """The oracle's Preetham map of a ProceduralSkyEnvironment dict."""
+    from oracle import sky as S
+    rgb, _ = S.bake(sun_direction=sky.get("sunDirection", (0.0, 1.0, 0.0)), turbidity=sky.get("turbidity", 2.0),
+                    rayleigh=sky.get("rayleigh", 1.0), mie_coefficient=sky.get("mieCoefficient", 0.005),
+                    mie_directional_g=sky.get("mieDirectionalG", 0.8), intensity=sky.get("intensity", 1.0))
+    return rgb
+
+
+def env_map_of(scene):
+    """The oracle's env_map for a scene: its HDRI, or its procedural sky baked."""
+    if scene.get("procedural_sky") is not None:
+        return {"texels": sky_bake(scene["procedural_sky"])}
+    return scene.get("hdri")
+
+
+def plane_under_sky(albedo, sky, roughness=1.0, metallic=0.0, hdri=None):
+    """A 400x400 floor at y = 0 under a constant sky (or the HDRI `hdri`),
+    seen from far above through a narrow field of view (view directions
+    within 1.5 degrees of the normal)."""
     pos = np.array([-200, 0, -200, 200, 0, -200, 200, 0, 200, -200, 0, 200], np.float32)
     nrm = np.tile([0, 1, 0], 4).astype(np.float32)
     idx = np.array([0, 2, 1, 0, 3, 2], np.uint32)
     prim = {"kind": "mesh", "positions": pos, "normals": nrm, "indices": idx,
             "material": {"baseColor": [albedo] * 3, "roughness": roughness, "metallic": metallic}}
     cam = W.look_at_camera(eye=(0, 60, 0.01), target=(0, 0, 0), up=(0, 0, -1), vfov_deg=3)
-    return {"primitives": [prim], "camera": cam, "scene_radius": 300.0, "sky": sky}
+    sc = {"primitives": [prim], "camera": cam, "scene_radius": 300.0, "sky": sky}
+    if hdri is not None:
+        sc["hdri"] = hdri
+    return sc
 
 
-def directional_albedo_at_normal(albedo, roughness, metallic, n_theta=4000):
+def directional_albedo_at_normal(albedo, roughness, metallic, n_theta=4000, radiance_of_theta=None):
     """Quadrature of the hemispherical integral of f(n, wi) cos(theta) for the
     base BSDF (diffuse (1-F) kd/pi + GGX D G F / (4 cos cos) + Kulla-Conty
-    multiscatter) with wo = n; the lobes are isotropic, so phi is trivial."""
+    multiscatter) with wo = n; the lobes are isotropic, so phi is trivial.
+    With `radiance_of_theta` (a function of the polar angle from n returning
+    [k, 3] radiance) the integrand carries that incident radiance and the
+    result is the reflected rgb radiance."""
     th = (np.arange(n_theta) + 0.5) / n_theta * (np.pi / 2)
     ct, st = np.cos(th), np.sin(th)
+    f = brdf_at_normal_view(albedo, roughness, metallic, ct)
+    if radiance_of_theta is not None:
+        Li = np.asarray(radiance_of_theta(th), np.float64)
+        return np.sum((f * ct * st)[:, None] * Li, axis=0) * (np.pi / 2 / n_theta) * 2 * np.pi
+    return float(np.sum(f * ct * st) * (np.pi / 2 / n_theta) * 2 * np.pi)
+
+
+def brdf_at_normal_view(albedo, roughness, metallic, ct):
+    """The base BSDF f(n, wo = n, wi) at cos(n, wi) = ct (vector)."""
+    ct = np.asarray(ct, np.float64)
+    th = np.arccos(np.clip(ct, -1.0, 1.0))
     ndl = ct
     h_ct = np.cos(th / 2)                      # n.h with wo = n
     vdh = h_ct
@@ -82,8 +116,7 @@ def directional_albedo_at_normal(albedo, roughness, metallic, n_theta=4000):
     favg = f0 + (1 - f0) / 21
     fms = favg * favg * eavg / max(1 - favg * (1 - eavg), 1e-4)
     ms = fms * (1 - lut(1.0, roughness)) * (1 - lut(ndl, roughness)) / max(np.pi * (1 - eavg), 1e-6)
-    f = np.where(ndl > 1e-5, diff + spec + ms, 0.0)
-    return float(np.sum(f * ct * st) * (np.pi / 2 / n_theta) * 2 * np.pi)
+    return np.where(ndl > 1e-5, diff + spec + ms, 0.0)
 
 
 # Kulla-Conty tables of pt-webgpu/src/wgsl/pathTrace/material.wgsl.ts:2852-2866.
@@ -390,3 +423,107 @@ def rect_light_radiance(x, wo, light, bsdf_args, n=np.array([0.0, 1.0, 0.0]), m=
         m = 4
     f = lambda wi: base_bsdf(*bsdf_args, n, wo, wi)  # noqa: E731
     return polygon_reflected_radiance(x, n, wo, tris, rad, brdf=f, m=m)
+
+
+# ---------------------------------------------------------------------------
+# HDRI environments
+# ---------------------------------------------------------------------------
+
+def env_texel_index(shape, rot, dirs):
+    """(row, column) of the texel each world direction looks up in an H x W
+    equirect map rotated by `rot` about +Y: the map direction is RY(-rot) d,
+    column from phi = atan2(z, x) with phi = -pi at column 0, row from theta =
+    acos(y) with +Y at row 0 (the documented HdriEnvironment convention)."""
+    H, W = shape
+    d = np.asarray(dirs, np.float64).reshape(-1, 3)
+    d = d / np.linalg.norm(d, axis=1, keepdims=True)
+    c, s = np.cos(rot), np.sin(rot)
+    mx, my, mz = c * d[:, 0] - s * d[:, 2], d[:, 1], s * d[:, 0] + c * d[:, 2]
+    u = (np.arctan2(mz, mx) / (2 * np.pi) + 0.5) % 1.0
+    v = np.clip(np.arccos(np.clip(my, -1, 1)) / np.pi, 0, 0.999999)
+    return np.minimum((v * H).astype(np.int64), H - 1), np.minimum((u * W).astype(np.int64), W - 1)
+
+
+def hdri_sun_map(H=16, W=32, sun_col=24, sun_row=5, sun=60.0, seed=3):
+    """A dim, seeded random map (0.05..0.35 per channel) with a 2 x 2 texel
+    'sun' of radiance `sun` x (1, 0.9, 0.7) at (sun_row, sun_col); column 24
+    of 32 is phi = +pi/2 (+Z in the map), row 5 of 16 is ~34 degrees above the
+    horizon."""
+    rng = np.random.default_rng(seed)
+    tex = (0.05 + 0.3 * rng.random((H, W, 3))).astype(np.float32)
+    tex[sun_row:sun_row + 2, sun_col:sun_col + 2] = np.float32(sun) * np.array([1.0, 0.9, 0.7], np.float32)
+    return tex
+
+
+def wall_under_hdri(hdri, albedo=0.5, roughness=1.0, metallic=0.0):
+    """A 400 x 400 wall at x = 0 facing +X under an HDRI environment
+    (`hdri` = {"texels", "intensity", "rotation_y"}), seen from 60 units
+    along +X through a 3 degree field (view directions within 1.5 degrees of
+    the normal)."""
+    pos = np.array([0, -200, -200, 0, 200, -200, 0, 200, 200, 0, -200, 200], np.float32)
+    nrm = np.tile([1, 0, 0], 4).astype(np.float32)
+    idx = np.array([0, 1, 2, 0, 2, 3], np.uint32)
+    prim = {"kind": "mesh", "positions": pos, "normals": nrm, "indices": idx,
+            "material": {"baseColor": [albedo] * 3, "roughness": roughness, "metallic": metallic}}
+    cam = W.look_at_camera(eye=(60, 0, 0.01), target=(0, 0, 0), up=(0, 1, 0), vfov_deg=3)
+    return {"primitives": [prim], "camera": cam, "scene_radius": 300.0, "sky": [0, 0, 0], "hdri": hdri}
+
+
+def hdri_reflected_radiance(hdri, n, albedo, roughness, metallic, n_theta=1024, n_phi=2048):
+    """Radiance reflected towards wo = n by a surface lit by the HDRI: midpoint
+    quadrature over the hemisphere about n of f(theta) cos(theta) L(w), L the
+    texel of w (env_texel_index) times the intensity.  Shares no code with any
+    sampling routine."""
+    tex = np.asarray(hdri["texels"], np.float64)
+    rot = float(np.float32(hdri.get("rotation_y", 0.0)))
+    inten = float(np.float32(hdri.get("intensity", 1.0)))
+    n = np.asarray(n, np.float64)
+    t = np.array([1.0, 0.0, 0.0]) if abs(n[0]) < 0.9 else np.array([0.0, 1.0, 0.0])
+    b1 = np.cross(n, t)
+    b1 /= np.linalg.norm(b1)
+    b2 = np.cross(n, b1)
+    th = (np.arange(n_theta) + 0.5) / n_theta * (np.pi / 2)
+    ph = (np.arange(n_phi) + 0.5) / n_phi * (2 * np.pi)
+    ct, st = np.cos(th), np.sin(th)
+    f = brdf_at_normal_view(albedo, roughness, metallic, ct)
+    total = np.zeros(3)
+    for i in range(n_theta):
+        d = (st[i] * np.cos(ph))[:, None] * b1 + (st[i] * np.sin(ph))[:, None] * b2 + ct[i] * n
+        r, c = env_texel_index(tex.shape[:2], rot, d)
+        total += tex[r, c].sum(axis=0) * (f[i] * ct[i] * st[i])
+    return total * inten * (np.pi / 2 / n_theta) * (2 * np.pi / n_phi)
+
+
+def sky_view(hdri, vfov=70.0, target=(1.0, 0.3, 0.2)):
+    """An HDRI seen directly: the camera at the origin looks at `target`; the
+    only geometry is a small card behind it, out of view."""
+    tgt = np.asarray(target, np.float64)
+    back = -tgt / np.linalg.norm(tgt) * 5
+    pos = (np.array([[-0.1, -0.1, 0], [0.1, -0.1, 0], [0.1, 0.1, 0], [-0.1, 0.1, 0]]) + back).astype(np.float32)
+    prim = {"kind": "mesh", "positions": pos.ravel(), "normals": np.tile([0, 0, 1], 4).astype(np.float32),
+            "indices": np.array([0, 1, 2, 0, 2, 3], np.uint32),
+            "material": {"baseColor": [0.5] * 3, "roughness": 1.0, "metallic": 0.0}}
+    cam = W.look_at_camera(eye=(0, 0, 0), target=tuple(tgt), up=(0, 1, 0), vfov_deg=vfov)
+    return {"primitives": [prim], "camera": cam, "scene_radius": 5.0, "sky": [0, 0, 0], "hdri": hdri}
+
+
+def sky_view_expected(scene, res):
+    """Per pixel the texel radiance the camera sees, and whether the pixel's
+    whole footprint (corners and centre) lies in that one texel."""
+    tex = np.asarray(scene["hdri"]["texels"], np.float32)
+    rot = float(np.float32(scene["hdri"].get("rotation_y", 0.0)))
+    inten = np.float32(scene["hdri"].get("intensity", 1.0))
+    ivp = np.asarray(W.inv_view_proj(scene["camera"], res, res), np.float64).reshape(4, 4).T
+    eye = np.asarray(scene["camera"]["eye"], np.float64)
+    offs = [(0.5, 0.5), (0.001, 0.001), (0.999, 0.001), (0.001, 0.999), (0.999, 0.999)]
+    idx = []
+    for ox, oy in offs:
+        px, py = np.meshgrid(np.arange(res) + ox, np.arange(res) + oy)
+        q = np.stack([px / res * 2 - 1, 1 - py / res * 2, np.ones_like(px), np.ones_like(px)], -1) @ ivp.T
+        d = q[..., :3] / q[..., 3:] - eye
+        r, c = env_texel_index(tex.shape[:2], rot, d)
+        idx.append(r * tex.shape[1] + c)
+    idx = np.stack(idx)
+    same = (idx == idx[0]).all(axis=0).reshape(res, res)
+    flat = tex.reshape(-1, 3)
+    return (flat[idx[0]] * inten).astype(np.float32).reshape(res, res, 3), same

@@ -201,6 +201,22 @@ vt_status vt_invert_mat4(const float m[16], float out[16]) {
   });
 }
 
+vt_status vt_bake_procedural_sky(const vt_environment* env, uint32_t width, uint32_t height, float* rgb_out,
+                                 double* luminance_integral) {
+  return vt_guarded(nullptr, [&]() -> vt_status {
+    if (!env || !rgb_out || !luminance_integral) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (env->kind != VT_ENV_PROCEDURAL_SKY)
+      return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "the environment is not a procedural sky");
+    std::vector<float> rgb;
+    double integral = 0;
+    const std::string err = engine_bake_sky(*env, width, height, rgb, &integral);
+    if (!err.empty()) return set_err(nullptr, VT_ERR_RANGE, err);
+    std::memcpy(rgb_out, rgb.data(), rgb.size() * sizeof(float));
+    *luminance_integral = integral;
+    return VT_OK;
+  });
+}
+
 const char* vt_version(void) { return "vitrum-hip 0.1.0 gfx950"; }
 
 vt_status vt_last_error(const vt_engine* eng, char* buf, size_t len) {
@@ -356,7 +372,6 @@ vt_status vt_set_scene(vt_engine* eng, const vt_scene_desc* scene) {
     rd = nullptr;
     eng->layout = std::move(layout);
     eng->pack = std::move(pack);
-    eng->scene_desc_env = scene->environment;
     eng->materials = std::move(materials);
     eng->scene_radius = scene->scene_radius;
     eng->has_scene = true;

@@ -93,7 +93,6 @@ struct vt_engine {
   std::unique_ptr<vt::Pack> pack;
   std::unique_ptr<vt::DeviceLayout> layout;
   std::vector<vt_material> materials;
-  vt_environment scene_desc_env{};
   float scene_radius = 0;
   bool has_scene = false;
   float last_frame_ms = 0;
@@ -123,6 +122,11 @@ vtk::TraceParams engine_trace_params(vt_engine* e);
 // Packs, builds and uploads one scene's traversal buffers (engine.cpp).
 vt_status engine_build_trace_scene(vt_engine* e, const vt_scene_desc* scene, SceneBuffers& out,
                                    vt::DeviceLayout& layout);
+// The procedural sky's baked map (ProceduralSkyEnvironment -> Preetham bake,
+// environmentPacking.ts:237-282): width x height rgb binary32 texels (render.cpp).
+constexpr uint32_t kSkyWidth = 256, kSkyHeight = 128;
+std::string engine_bake_sky(const vt_environment& env, uint32_t width, uint32_t height, std::vector<float>& rgb,
+                            double* luminance_integral);
 // Hooks into the integrator (render.cpp).  prepare builds and uploads a new
 // scene's shading data without touching the bound scene; commit binds it
 // (cannot fail; takes ownership); discard frees an unbound set.

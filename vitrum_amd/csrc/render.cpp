@@ -15,6 +15,7 @@
 #include "engine_internal.h"
 #include "light_tree.h"
 #include "render.h"
+#include "sky.h"
 #include "sobol.h"
 
 namespace {
@@ -39,6 +40,8 @@ std::vector<float> represented_cdf(const std::vector<double>& w) {
   std::vector<size_t> pos;
   for (size_t i = 0; i < w.size(); ++i)
     if (w[i] > 0) pos.push_back(i);
+  if ((int64_t)pos.size() > total)
+    throw std::range_error("the importance table has more positive entries than its 2^24 random buckets");
   if (!pos.empty()) {
     double mx = 0;
     for (size_t i : pos) mx = std::max(mx, w[i]);
@@ -87,22 +90,155 @@ std::vector<float> represented_cdf(const std::vector<double>& w) {
   return cdf;
 }
 
+// The environment's importance table and texels (environmentPacking.ts:
+// 283-440): texel weights luminance x solid angle of the unscaled map, an
+// f32 CDF over 2^24 represented buckets, and per texel the intensity-scaled
+// radiance with the represented pmf over the texel's solid angle.
+struct EnvTable {
+  bool present = false;
+  uint32_t w = 0, h = 0;
+  float rot_c = 1.0f, rot_s = 0.0f;
+  std::vector<float> cdf;     // w * h + 1
+  std::vector<float> texels;  // w * h x (r, g, b, pdf per steradian)
+  double power = 0;           // luminance x solid angle of the scaled map
+};
+
+double luminance(double r, double g, double b) { return 0.2126 * r + 0.7152 * g + 0.0722 * b; }
+
+}  // namespace
+
+// The procedural sky's parameters as the reference defaults and packs them
+// (environmentPacking.ts:237-262): intensity through the binary32 lane check
+// (environmentRadianceScale.ts:81-98), then the Preetham bake.
+std::string engine_bake_sky(const vt_environment& env, uint32_t W, uint32_t H, std::vector<float>& rgb,
+                            double* integral) {
+  const double in = env.intensity;
+  if (!std::isfinite(in) || in < 0) return "procedural-sky intensity must be finite and non-negative";
+  vts::SkyParams p{};
+  for (int k = 0; k < 3; ++k) p.sun[k] = env.sun_direction[k];
+  // the sun direction is normalised here and again by the bake
+  const double len = std::hypot(p.sun[0], p.sun[1], p.sun[2]);
+  if (std::isfinite(len)) {
+    if (len < 1e-8)
+      p.sun[0] = 0, p.sun[1] = 1, p.sun[2] = 0;
+    else
+      for (int k = 0; k < 3; ++k) p.sun[k] /= len;
+  }
+  p.turbidity = env.turbidity;
+  p.rayleigh = env.rayleigh;
+  p.mie_coefficient = env.mie_coefficient;
+  p.mie_g = env.mie_directional_g;
+  p.intensity = in;
+  return vts::bake_preetham(p, W, H, rgb, integral);
+}
+
+namespace {
+
+// Empty string on success (t.present = false for none or an all-black map).
+std::string build_environment(const vt_environment& env, EnvTable& t) {
+  if (env.kind == VT_ENV_NONE) return {};
+  uint32_t W = 64, H = 32, ch = 0;
+  float intensity = 1.0f, rot = 0.0f;
+  const float* texels = env.texels;
+  std::vector<float> sky;
+  if (env.kind == VT_ENV_PROCEDURAL_SKY) {
+    // the baked daylight map, sampled as an HDRI with intensity 1 and no
+    // rotation (the sun direction is in world space; environmentPacking.ts:
+    // 237-282)
+    double integral = 0;
+    const std::string se = engine_bake_sky(env, kSkyWidth, kSkyHeight, sky, &integral);
+    if (!se.empty()) return se;
+    W = kSkyWidth, H = kSkyHeight, ch = 3;
+    texels = sky.data();
+  } else if (env.kind == VT_ENV_CONSTANT) {
+    for (int k = 0; k < 3; ++k)
+      if (!(std::isfinite(env.radiance[k]) && env.radiance[k] >= 0.0f))
+        return "environment radiance must be finite and non-negative";
+  } else if (env.kind == VT_ENV_HDRI) {
+    if (!env.texels || env.width == 0 || env.height == 0) return "an HDRI environment needs texels, width and height";
+    if (env.channels < 1 || env.channels > 4) return "HDRI channels must be 1, 2, 3 or 4";
+    if ((uint64_t)env.width * env.height * 4u > 0x7fffffffull) return "the HDRI has too many texels";
+    if (!(std::isfinite(env.intensity) && env.intensity >= 0.0f)) return "HDRI intensity must be finite and non-negative";
+    if (!std::isfinite(env.rotation_y)) return "HDRI rotationY must be finite";
+    W = env.width, H = env.height, ch = env.channels;
+    intensity = env.intensity;
+    // canonicalizeEnvironmentRotationF32 (environmentRotation.ts:10-26)
+    rot = (float)std::fmod((double)env.rotation_y, 2.0 * M_PI);
+    if (rot == 0.0f) rot = 0.0f;
+  } else {
+    return "environment kind must be 0 (none), 1 (constant) or 2 (hdri)";
+  }
+  const size_t n = (size_t)W * H;
+  std::vector<double> weights(n), omega(H);
+  const double dphi = 2.0 * M_PI / W;
+  for (uint32_t y = 0; y < H; ++y)
+    omega[y] = dphi * (std::cos((double)y / H * M_PI) - std::cos((double)(y + 1) / H * M_PI));
+  t.texels.assign(n * 4, 0.0f);
+  double total = 0, scaled_total = 0;
+  for (size_t i = 0; i < n; ++i) {
+    float c[3];
+    if (ch == 0) {
+      c[0] = env.radiance[0], c[1] = env.radiance[1], c[2] = env.radiance[2];
+    } else {
+      // environmentMapPixels.ts:138-142: grey, (r, g) with b = r, rgb, rgba
+      const float* q = texels + i * ch;
+      c[0] = q[0];
+      c[1] = ch > 1 ? q[1] : q[0];
+      c[2] = ch > 2 ? q[2] : q[0];
+    }
+    for (int k = 0; k < 3; ++k)
+      if (!(std::isfinite(c[k]) && c[k] >= 0.0f))
+        return "HDRI texel " + std::to_string(i) + " must contain finite, non-negative radiance";
+    const double om = omega[i / W];
+    float sc[3];
+    for (int k = 0; k < 3; ++k) {
+      sc[k] = c[k] * intensity;  // one binary32 product (ptScaleEnvironmentRadiance)
+      if (!std::isfinite(sc[k]))
+        return "HDRI texel " + std::to_string(i) + " multiplied by its intensity must remain finite in binary32";
+      t.texels[i * 4 + k] = sc[k];
+    }
+    weights[i] = std::max(0.0, luminance(c[0], c[1], c[2]) * om);
+    total += weights[i];
+    scaled_total += std::max(0.0, luminance(sc[0], sc[1], sc[2]) * om);
+  }
+  if (!std::isfinite(total) || !std::isfinite(scaled_total)) return "the HDRI importance integral overflowed";
+  if (!(total > 0)) {
+    t.texels.clear();
+    return {};
+  }
+  if (intensity > 0.0f && scaled_total == 0)
+    return "positive HDRI radiance multiplied by its positive intensity underflows to zero in binary32";
+  try {
+    t.cdf = represented_cdf(weights);
+  } catch (const std::exception& ex) {
+    return ex.what();
+  }
+  for (size_t i = 0; i < n; ++i)
+    t.texels[i * 4 + 3] = (float)(((double)t.cdf[i + 1] - (double)t.cdf[i]) / omega[i / W]);
+  t.present = true;
+  t.w = W, t.h = H;
+  t.rot_c = (float)std::cos((double)rot);
+  t.rot_s = (float)std::sin((double)rot);
+  t.power = scaled_total;
+  return {};
+}
+
 }  // namespace
 
 // Per-scene shading data.  vt_set_scene builds and uploads a complete set
 // before it binds anything, so a failure leaves the previous scene's set.
 struct RenderSceneData {
-  DeviceBuffer materials, emitters, instances, env_cdf, env_pdf, tri_source, light_tree, lights;
+  DeviceBuffer materials, emitters, instances, env_cdf, env_texels, tri_source, light_tree, lights;
   uint32_t n_dir = 0, n_point = 0, n_spot = 0, n_rect = 0;  // analytic emitters in `lights`
   uint32_t light_tree_nodes = 0;  // 0: uniform light selection
   std::vector<float> light_tree_host;
   uint32_t emitter_count = 0;
-  uint32_t env_present = 0, env_w = 64, env_h = 32;
-  float env_radiance[3] = {0, 0, 0};
+  uint32_t env_present = 0, env_w = 0, env_h = 0;
+  float env_rot[2] = {1, 0};  // cos, sin of the dome's rotation about +Y
   float ray_bias = 0, tmin = 0;
   float scene_min[3] = {0, 0, 0}, scene_inv_extent[3] = {1, 1, 1};
   void release_scene() {
-    for (DeviceBuffer* b : {&materials, &emitters, &instances, &env_cdf, &env_pdf, &tri_source, &light_tree, &lights})
+    for (DeviceBuffer* b : {&materials, &emitters, &instances, &env_cdf, &env_texels, &tri_source, &light_tree, &lights})
       b->release();
   }
 };
@@ -270,21 +406,15 @@ vt_status engine_prepare_render(vt_engine* e, const vt::Pack& P, const std::vect
   }
   R.emitter_count = (uint32_t)(em.size() / vtr::kEmitFloats);
   if (em.empty()) em.assign(vtr::kEmitFloats, 0.0f);
-  // Environment: constant-radiance equirect map, luminance x texel solid
-  // angle importance (scene/environmentPacking.ts:340-420).
-  R.env_present = env.kind == 1 && (env.radiance[0] > 0 || env.radiance[1] > 0 || env.radiance[2] > 0);
-  for (int k = 0; k < 3; ++k) R.env_radiance[k] = R.env_present ? env.radiance[k] : 0.0f;
-  const uint32_t W = R.env_w, H = R.env_h;
-  std::vector<double> weights((size_t)W * H), omega((size_t)W * H);
-  const double lum = 0.2126 * R.env_radiance[0] + 0.7152 * R.env_radiance[1] + 0.0722 * R.env_radiance[2];
-  const double dphi = 2.0 * M_PI / W;
-  for (uint32_t i = 0; i < W * H; ++i) {
-    const uint32_t y = i / W;
-    omega[i] = dphi * (std::cos((double)y / H * M_PI) - std::cos((double)(y + 1) / H * M_PI));
-    weights[i] = std::max(0.0, lum * omega[i]);
+  // Environment importance table (environmentPacking.ts:283-440).
+  EnvTable et;
+  {
+    const std::string ee = build_environment(env, et);
+    if (!ee.empty()) return rerr(e, VT_ERR_RANGE, ee);
   }
-  std::vector<float> cdf = represented_cdf(weights), pdf((size_t)W * H);
-  for (uint32_t i = 0; i < W * H; ++i) pdf[i] = (float)(((double)cdf[i + 1] - (double)cdf[i]) / omega[i]);
+  R.env_present = et.present;
+  R.env_w = et.w, R.env_h = et.h;
+  R.env_rot[0] = et.rot_c, R.env_rot[1] = et.rot_s;
   // Light tree over the light slots -- mesh emitters in order, then the
   // environment -- built when there are two or more lights and the engine
   // uses tree selection (the reference's default; uploadSceneBuffers.ts:
@@ -325,11 +455,11 @@ vt_status engine_prepare_render(vt_engine* e, const vt::Pack& P, const std::vect
       leaves.insert(leaves.begin(), dirs.begin(), dirs.end());
     }
     if (R.env_present) {
-      // environment leaf: luminance x solid angle over the map, rounded to
-      // f32 (environmentPacking.ts:183-202), bounds = the union of the
-      // positional lights' bounds (the origin when there are none)
-      double total = 0;
-      for (uint32_t i = 0; i < W * H; ++i) total += weights[i];
+      // environment leaf: luminance x solid angle over the intensity-scaled
+      // map, rounded to f32 (environmentPacking.ts:183-202, 366-413), bounds =
+      // the union of the positional lights' bounds (the origin when none)
+      const double total = et.power;
+      if (!std::isfinite((float)total)) return rerr(e, VT_ERR_RANGE, "environment light-tree power exceeds binary32 range");
       vtl::Leaf L{};
       const float pw = (float)total;
       L.power = total > 0 ? (pw > 0 ? pw : 0x1p-149f) : 0.0;
@@ -373,8 +503,10 @@ vt_status engine_prepare_render(vt_engine* e, const vt::Pack& P, const std::vect
   s = s ? s : engine_upload(e, R.materials, mats.data(), mats.size() * 4);
   s = s ? s : engine_upload(e, R.emitters, em.data(), em.size() * 4);
   s = s ? s : engine_upload(e, R.instances, inst.data(), inst.size() * 4);
-  s = s ? s : engine_upload(e, R.env_cdf, cdf.data(), cdf.size() * 4);
-  s = s ? s : engine_upload(e, R.env_pdf, pdf.data(), pdf.size() * 4);
+  if (et.cdf.empty()) et.cdf.assign(2, 0.0f);
+  if (et.texels.empty()) et.texels.assign(4, 0.0f);
+  s = s ? s : engine_upload(e, R.env_cdf, et.cdf.data(), et.cdf.size() * 4);
+  s = s ? s : engine_upload(e, R.env_texels, et.texels.data(), et.texels.size() * 4);
   s = s ? s : engine_upload(e, R.tri_source, P.tri_source.data(), P.tri_source.size() * 4);
   s = s ? s : engine_upload(e, R.light_tree, tree.data(), tree.size() * 4);
   if (ana.records.empty()) ana.records.assign(vtr::kLightFloats, 0.0f);
@@ -494,12 +626,13 @@ vt_status vt_render_frame(vt_engine* e, const vt_frame* fr) {
     sc.emitters = (const float*)R.emitters.ptr;
     sc.instances = (const float*)R.instances.ptr;
     sc.env_cdf = (const float*)R.env_cdf.ptr;
-    sc.env_pdf = (const float*)R.env_pdf.ptr;
+    sc.env_texels = (const float4*)R.env_texels.ptr;
     sc.emitter_count = R.emitter_count;
     sc.env_present = R.env_present;
     sc.env_w = R.env_w;
     sc.env_h = R.env_h;
-    for (int k = 0; k < 3; ++k) sc.env_radiance[k] = R.env_radiance[k];
+    sc.env_rot_c = R.env_rot[0];
+    sc.env_rot_s = R.env_rot[1];
     sc.light_tree = (const float*)R.light_tree.ptr;
     sc.light_tree_nodes = R.light_tree_nodes;
     sc.lights = (const float*)R.lights.ptr;

@@ -23,10 +23,10 @@ struct ShadeScene {
   const float* emitters;        // kEmitFloats per emitter, light-selection order
   const float* instances;       // kInstFloats per packed instance
   const float* env_cdf;         // env_w * env_h + 1
-  const float* env_pdf;         // per texel
+  const float4* env_texels;     // per texel: scaled radiance rgb, pdf per steradian
   uint32_t emitter_count;
   uint32_t env_present, env_w, env_h;
-  float env_radiance[3];
+  float env_rot_c, env_rot_s;   // the dome's rotation about +Y
   // Light tree over the light slots (16 floats per node); 0 nodes = uniform.
   const float* light_tree;
   uint32_t light_tree_nodes;

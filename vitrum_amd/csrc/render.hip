@@ -463,7 +463,21 @@ __device__ float edge_scale(F3 u, F3 v) {
 
 // ---- environment (connect.wgsl.ts:126-219) -------------------------------
 
-__device__ float env_pdf_dir(const ShadeScene& sc, F3 d) {
+// The dome is rotated by rot about +Y: a world direction d looks up the
+// unrotated map at RY(-rot) d, and a direction sampled in the map leaves as
+// RY(+rot) d (connectCore.wgsl.ts:20-44).
+__device__ __forceinline__ F3 env_to_map(const ShadeScene& sc, F3 d) {
+  const float c = sc.env_rot_c, s = sc.env_rot_s;
+  return f3(c * d.x - s * d.z, d.y, s * d.x + c * d.z);
+}
+__device__ __forceinline__ F3 env_to_world(const ShadeScene& sc, F3 d) {
+  const float c = sc.env_rot_c, s = sc.env_rot_s;
+  return f3(c * d.x + s * d.z, d.y, -s * d.x + c * d.z);
+}
+
+// The texel a world direction looks up: (scaled radiance, pdf per steradian).
+__device__ float4 env_lookup(const ShadeScene& sc, F3 wd) {
+  const F3 d = env_to_map(sc, wd);
   const float phi = atan2f(d.z, d.x);
   const float theta = acosf(clampf(d.y, -1.0f, 1.0f));
   float u = phi * kInv2Pi + 0.5f;
@@ -471,7 +485,7 @@ __device__ float env_pdf_dir(const ShadeScene& sc, F3 d) {
   const float v = clampf(theta * kInvPi, 0.0f, 0.999999f);
   const uint32_t x = min((uint32_t)floorf(u * (float)sc.env_w), sc.env_w - 1u);
   const uint32_t y = min((uint32_t)floorf(v * (float)sc.env_h), sc.env_h - 1u);
-  return sc.env_pdf[y * sc.env_w + x];
+  return sc.env_texels[y * sc.env_w + x];
 }
 
 // A direction inside environment texel (x, y), uniform in phi and cos(theta)
@@ -788,7 +802,6 @@ __global__ __launch_bounds__(256) void shade_kernel(FrameParams f, ShadeScene sc
       uint32_t flags = s.flags[p];
       uint64_t sig = s.signature[p];
       const bool prev_mis = prev_pdf > 0.0f;
-      const F3 env = f3(sc.env_radiance[0], sc.env_radiance[1], sc.env_radiance[2]);
       // 1. BSDF -> light / environment connection of the previous sample.
       Mat m{};
       const float* inst_rec = miss ? nullptr : sc.instances + (size_t)inst * kInstFloats;
@@ -838,13 +851,19 @@ __global__ __launch_bounds__(256) void shade_kernel(FrameParams f, ShadeScene sc
           }
         }
         if (best_pdf > 0.0f) L = L + Cn * best_le * power_heuristic(prev_pdf, best_pdf);
-        if (miss && sc.env_present) L = L + Cn * env * power_heuristic(prev_pdf, env_pdf_dir(sc, dir));
+        if (miss && sc.env_present) {
+          const float4 et = env_lookup(sc, dir);
+          L = L + Cn * f3(et.x, et.y, et.z) * power_heuristic(prev_pdf, et.w);
+        }
       }
       const bool alive = (flags & 1u) != 0u;
       bool done = !alive || vertex >= f.bounce_limit;
       if (!done) sig = mix64(sig, miss ? 0xFFFFFFFFFFFFull : (((uint64_t)inst << 32) | prim));
       if (!done && miss) {
-        if (!prev_mis && sc.env_present) L = L + T * env;
+        if (!prev_mis && sc.env_present) {
+          const float4 et = env_lookup(sc, dir);
+          L = L + T * f3(et.x, et.y, et.z);
+        }
         done = true;
         if (vertex == 0 && p < f.pixels) {
           *(float4*)(s.aux_nd + 4 * (size_t)p) = make_float4(0.5f, 1.0f, 0.5f, 0.0f);
@@ -1058,8 +1077,10 @@ __global__ __launch_bounds__(256) void shade_kernel(FrameParams f, ShadeScene sc
             const uint32_t idx = min(lo, count - 1u);
             const float xa = rand01(rng), xb = rand01(rng);
             const uint32_t ex = idx % sc.env_w, ey = idx / sc.env_w;
-            const F3 wi = safe_normalize(env_cell_dir(ex, ey, sc.env_w, sc.env_h, xa, xb));
-            const float epdf = sc.env_pdf[idx];
+            const F3 wi = safe_normalize(env_to_world(sc, env_cell_dir(ex, ey, sc.env_w, sc.env_h, xa, xb)));
+            const float4 et = sc.env_texels[idx];
+            const float epdf = et.w;
+            const F3 env = f3(et.x, et.y, et.z);
             const float ndl = fabsf(dot(nrm, wi));
             if (ndl > 0.0f) {
               const F3 off = dot(nrm, wi) > 0.0f ? nrm : f3(-nrm.x, -nrm.y, -nrm.z);

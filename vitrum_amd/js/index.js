@@ -62,38 +62,130 @@ function toU32(v, label) {
   throw new TypeError(`${label} must be a Uint32Array, Uint16Array or array`);
 }
 
-// Constant-radiance environments: an HDRI whose texels are all equal (the
-// importance table of such a map is what the device samples).
-function environmentOf(env) {
-  if (env == null || env.kind === 'none') return { kind: 0, radiance: [0, 0, 0] };
-  if (env.kind !== 'hdri') {
-    throw new RangeError(`pt-hip renders 'none' and constant 'hdri' environments, not '${env.kind}'`);
+// Environment maps (packages/shared-samplers/src/environmentMapPixels.ts:
+// 103-160): raw {width, height, data} or DataTexture-shaped {image: {...}}
+// payloads, with optional channels / dataType / colorSpace hints (direct or
+// under __vitrum_hint__), decoded to linear RGB binary32 texels.
+function halfToFloat(h) {
+  const s = (h & 0x8000) ? -1 : 1;
+  const e = (h & 0x7c00) >> 10;
+  const f = h & 0x03ff;
+  if (e === 0) return s * 2 ** -14 * (f / 1024);
+  if (e === 0x1f) return f ? NaN : s * Infinity;
+  return s * 2 ** (e - 15) * (1 + f / 1024);
+}
+
+function srgbToLinear(c) {
+  return c <= 0.04045 ? c / 12.92 : Math.pow((c + 0.055) / 1.055, 2.4);
+}
+
+function mapHint(map) {
+  if (map.__vitrum_hint__ != null) return map.__vitrum_hint__;
+  const h = {};
+  if ([1, 2, 3, 4].includes(map.channels)) h.channels = map.channels;
+  if (['uint8', 'uint16', 'float16', 'half-float', 'float32'].includes(map.dataType)) h.dataType = map.dataType;
+  if (map.colorSpace === 'linear' || map.colorSpace === 'srgb') h.colorSpace = map.colorSpace;
+  return h;
+}
+
+function decodeScalar(src, i, hint) {
+  const raw = Number(src[i] == null ? 0 : src[i]);
+  switch (hint.dataType) {
+    case 'uint16': return raw / 65535;
+    case 'float16': case 'half-float': return halfToFloat(raw);
+    case 'float32': return raw;
+    case 'uint8': return raw / 255;
+    default: break;
   }
-  const map = env.hdri || {};
-  const intensity = env.intensity == null ? 1 : env.intensity;
-  if (!Number.isFinite(intensity) || intensity < 0) throw new RangeError('environment.intensity must be finite and >= 0');
-  if (env.rotationY && env.rotationY !== 0) {
-    // A constant map is rotation invariant.
-  }
-  let rgb;
-  if (Array.isArray(map.constant) || map.constant instanceof Float32Array) {
-    rgb = [map.constant[0], map.constant[1], map.constant[2]];
-  } else if (map.data && map.width > 0 && map.height > 0) {
-    const d = map.data;
-    const ch = d.length / (map.width * map.height);
-    if (ch !== 3 && ch !== 4) throw new TypeError('environment hdri data must be RGB or RGBA');
-    rgb = [d[0], d[1], d[2]];
-    for (let i = 0; i < d.length; i += ch) {
-      if (d[i] !== rgb[0] || d[i + 1] !== rgb[1] || d[i + 2] !== rgb[2]) {
-        throw new RangeError('pt-hip renders constant-radiance environment maps only');
-      }
-    }
-  } else {
+  if (src instanceof Uint16Array) return raw / 65535;
+  if (src instanceof Float32Array || src instanceof Float64Array || Array.isArray(src)) return raw;
+  const bpe = src.BYTES_PER_ELEMENT || 1;
+  return raw / (2 ** (8 * bpe) - 1);
+}
+
+function environmentMapTexels(map) {
+  const image = map.image || {};
+  const pick = (k) => (map[k] != null ? map[k] : image[k]);
+  const src = pick('data');
+  const width = Number(pick('width') || 0);
+  const height = Number(pick('height') || 0);
+  if (src == null || !Number.isSafeInteger(width) || !Number.isSafeInteger(height) || width <= 0 || height <= 0) {
     throw new TypeError('environment.hdri must carry {constant:[r,g,b]} or {width,height,data}');
   }
-  const radiance = rgb.map((c) => c * intensity);
-  if (!radiance.every((c) => Number.isFinite(c) && c >= 0)) throw new RangeError('environment radiance must be finite and >= 0');
-  return { kind: radiance.some((c) => c > 0) ? 1 : 0, radiance };
+  const pixels = width * height;
+  if (pixels * 4 > 0x7fffffff || src.length < pixels) throw new RangeError('environment.hdri data is too short or too large');
+  const hint = mapHint(map);
+  const per = src.length / pixels;
+  const ch = hint.channels != null ? hint.channels : (per >= 4 ? 4 : per >= 3 ? 3 : per >= 2 ? 2 : 1);
+  if (src.length < pixels * ch) throw new RangeError('environment.hdri data is shorter than width x height x channels');
+  const srgb = hint.colorSpace === 'srgb';
+  const texels = new Float32Array(pixels * 3);
+  for (let p = 0; p < pixels; p += 1) {
+    const b0 = p * ch;
+    const r = decodeScalar(src, b0, hint);
+    const rgb = [r, ch > 1 ? decodeScalar(src, b0 + 1, hint) : r, ch > 2 ? decodeScalar(src, b0 + 2, hint) : r];
+    for (let k = 0; k < 3; k += 1) {
+      const v = srgb ? srgbToLinear(rgb[k]) : rgb[k];
+      if (!Number.isFinite(v) || !Number.isFinite(Math.fround(v)) || v < 0) {
+        throw new RangeError(`environment.hdri texel ${p} must contain finite, non-negative radiance`);
+      }
+      texels[p * 3 + k] = v;
+    }
+  }
+  return { width, height, texels };
+}
+
+// HdriEnvironment (packages/core/src/scene/environment.ts:21-56): a constant
+// radiance {constant: [r, g, b]} (kind 1) or an equirectangular map (kind 2)
+// with intensity and rotationY.
+// An environment scale as the binary32 lane takes it
+// (environmentRadianceScale.ts:81-98).
+function environmentScale(value, label) {
+  const v = value == null ? 1 : value;
+  if (!Number.isFinite(v) || v < 0) throw new RangeError(`${label} must be finite and >= 0`);
+  const packed = Math.fround(v);
+  if (!Number.isFinite(packed) || (v > 0 && packed === 0)) {
+    throw new RangeError(`${label} must remain finite and positive after binary32 packing`);
+  }
+  return packed;
+}
+
+function environmentOf(env) {
+  if (env == null || env.kind === 'none') return { kind: 0, radiance: [0, 0, 0] };
+  if (env.kind === 'procedural-sky') {
+    // ProceduralSkyEnvironment (environment.ts:59-71); the addon fills the
+    // defaults of environmentPacking.ts:245-262 for absent fields
+    const out = { kind: 3, radiance: [0, 0, 0], intensity: environmentScale(env.intensity, 'procedural-sky intensity') };
+    for (const k of ['turbidity', 'rayleigh', 'mieCoefficient', 'mieDirectionalG']) {
+      if (env[k] != null) {
+        if (!Number.isFinite(env[k])) throw new RangeError(`procedural-sky ${k} must be finite`);
+        out[k] = env[k];
+      }
+    }
+    if (env.sunDirection != null) {
+      const d = Array.from(env.sunDirection);
+      if (d.length !== 3 || !d.every(Number.isFinite)) throw new RangeError('procedural-sky sunDirection must be 3 finite numbers');
+      out.sunDirection = d;
+    }
+    return out;
+  }
+  if (env.kind !== 'hdri') {
+    throw new RangeError(`pt-hip renders 'none', 'hdri' and 'procedural-sky' environments, not '${env.kind}'`);
+  }
+  const map = env.hdri || {};
+  const intensity = environmentScale(env.intensity, 'environment.intensity');
+  let rotationY = env.rotationY == null ? 0 : env.rotationY;
+  if (!Number.isFinite(rotationY)) throw new RangeError('environment.rotationY must be finite');
+  // canonicalizeEnvironmentRotationF32 (environmentRotation.ts:10-26)
+  rotationY = Math.fround(rotationY % (2 * Math.PI));
+  if (Object.is(rotationY, -0)) rotationY = 0;
+  if (Array.isArray(map.constant) || map.constant instanceof Float32Array) {
+    const radiance = [map.constant[0], map.constant[1], map.constant[2]].map((c) => c * intensity);
+    if (!radiance.every((c) => Number.isFinite(c) && c >= 0)) throw new RangeError('environment radiance must be finite and >= 0');
+    return { kind: radiance.some((c) => c > 0) ? 1 : 0, radiance };
+  }
+  const { width, height, texels } = environmentMapTexels(map);
+  return { kind: 2, radiance: [0, 0, 0], texels, width, height, channels: 3, intensity, rotationY };
 }
 
 // Scene (packages/core/src/scene/index.ts:42) -> the addon's scene blob.

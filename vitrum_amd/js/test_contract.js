@@ -56,7 +56,25 @@ assert.throws(() => pt.sceneBlob({ primitives: [], emitters: [{ kind: 'laser' }]
 assert.throws(() => pt.sceneBlob({ primitives: [], emitters: [{ kind: 'point', position: [0, NaN, 0] }] }), TypeError);
 const sky = pt.sceneBlob({ primitives: [], environment: { kind: 'hdri', hdri: { constant: [1, 0.5, 0.25] }, intensity: 2 } });
 assert.deepStrictEqual(sky.environment, { kind: 1, radiance: [2, 1, 0.5] });
-assert.throws(() => pt.sceneBlob({ primitives: [], environment: { kind: 'hdri', hdri: { width: 2, height: 1, data: new Float32Array([1, 1, 1, 1, 0, 0, 0, 1]) } } }), RangeError);
+// HDRI payloads decode to linear RGB binary32 texels (environmentMapPixels.ts).
+const rgba = pt.sceneBlob({ primitives: [], environment: { kind: 'hdri', hdri: { width: 2, height: 1, data: new Float32Array([1, 2, 3, 9, 0, 0.5, 0, 1]) }, intensity: 2, rotationY: 0.5 } });
+assert.strictEqual(rgba.environment.kind, 2);
+assert.deepStrictEqual(Array.from(rgba.environment.texels), [1, 2, 3, 0, 0.5, 0]);
+assert.strictEqual(rgba.environment.intensity, 2);
+assert.strictEqual(rgba.environment.rotationY, 0.5);
+const u8 = pt.sceneBlob({ primitives: [], environment: { kind: 'hdri', hdri: { width: 1, height: 1, data: new Uint8Array([255, 0, 51]), colorSpace: 'srgb' } } });
+assert.deepStrictEqual(Array.from(u8.environment.texels), [1, 0, Math.fround(Math.pow((0.2 + 0.055) / 1.055, 2.4))]);
+const grey = pt.sceneBlob({ primitives: [], environment: { kind: 'hdri', hdri: { image: { width: 2, height: 1, data: new Uint16Array([65535, 0]) }, __vitrum_hint__: { channels: 1 } } } });
+assert.deepStrictEqual(Array.from(grey.environment.texels), [1, 1, 1, 0, 0, 0]);
+const half = pt.sceneBlob({ primitives: [], environment: { kind: 'hdri', hdri: { width: 1, height: 1, data: new Uint16Array([0x3c00, 0x4000, 0x3800]), dataType: 'float16' } } });
+assert.deepStrictEqual(Array.from(half.environment.texels), [1, 2, 0.5]);
+assert.throws(() => pt.sceneBlob({ primitives: [], environment: { kind: 'hdri', hdri: { width: 1, height: 1, data: new Float32Array([1, -1, 0]) } } }), RangeError);
+assert.throws(() => pt.sceneBlob({ primitives: [], environment: { kind: 'hdri', hdri: { width: 1, height: 1, data: [1, 1, 1] }, rotationY: Infinity } }), RangeError);
+const skyBlob = pt.sceneBlob({ primitives: [], environment: { kind: 'procedural-sky', sunDirection: [0.3, 0.8, 0.1], turbidity: 3 } });
+assert.deepStrictEqual(skyBlob.environment, { kind: 3, radiance: [0, 0, 0], intensity: 1, turbidity: 3, sunDirection: [0.3, 0.8, 0.1] });
+assert.throws(() => pt.sceneBlob({ primitives: [], environment: { kind: 'procedural-sky', rayleigh: NaN } }), RangeError);
+assert.throws(() => pt.sceneBlob({ primitives: [], environment: { kind: 'hdri', hdri: { constant: [1, 1, 1] }, intensity: 1e-50 } }), RangeError);
+assert.throws(() => pt.sceneBlob({ primitives: [], environment: { kind: 'vacuum' } }), RangeError);
 
 (async () => {
   await assert.rejects(pt.createPTEngine_HIP({ bogus: 1 }), TypeError);
@@ -108,6 +126,21 @@ assert.throws(() => pt.sceneBlob({ primitives: [], environment: { kind: 'hdri', 
   const pc = await pl.captureFrame();
   assert.ok(pc.rgba[(8 * 16 + 8) * 4] > 0, 'point light lights the quad');
   pl.dispose();
+  // an HDRI map (rotated, uint8 sRGB payload) and a procedural sky light the quad
+  const envs = [{ kind: 'hdri', hdri: { width: 4, height: 2, data: new Uint8Array(4 * 2 * 3).fill(200), colorSpace: 'srgb' }, rotationY: 1.2, intensity: 2 },
+    { kind: 'procedural-sky', sunDirection: [0.2, 0.9, 0.4] }];
+  for (const environment of envs) {
+    const he = await pt.createPTEngine_HIP({ device: 0 });
+    he.setScene({ primitives: [quad], emitters: [], environment });
+    for (let i = 0; i < 2; i += 1) {
+      he.renderFrame({ viewMatrix: view, projMatrix: proj, viewport: { width: 16, height: 16, devicePixelRatio: 1 },
+        frameIndex: i, frameSeed: 7, quality: { bounces: 3 } });
+    }
+    const hc = await he.captureFrame();
+    assert.ok(hc.rgba.every((x) => Number.isFinite(x) && x >= 0), `${environment.kind} output finite`);
+    assert.ok(hc.rgba[(8 * 16 + 8) * 4] > 0, `${environment.kind} lights the quad`);
+    he.dispose();
+  }
   // traceClosest through the addon: a ray straight at the triangle.
   const hits = new Float32Array(addon.traceClosest(eng._h, new Float32Array([0.6, 0.3, 1, 0, 0, 0, -1, 1e30])));
   assert.ok(Math.abs(hits[2] - 1) < 1e-6, `t = ${hits[2]}`);

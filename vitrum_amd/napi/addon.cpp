@@ -146,6 +146,44 @@ bool get_vec(napi_env env, napi_value obj, const char* key, uint32_t n, const fl
   return true;
 }
 
+// Binary64 number / n-vector properties (JS numbers kept whole).
+bool get_f64(napi_env env, napi_value obj, const char* key, double def, double* out) {
+  napi_value v;
+  bool has = false;
+  napi_has_named_property(env, obj, key, &has);
+  if (has) napi_get_named_property(env, obj, key, &v);
+  if (!has || is_nullish(env, v)) {
+    *out = def;
+    return true;
+  }
+  if (napi_get_value_double(env, v, out) != napi_ok) {
+    std::string m = std::string("'") + key + "' must be a number";
+    throw_type(env, m.c_str());
+    return false;
+  }
+  return true;
+}
+
+bool get_dvec(napi_env env, napi_value obj, const char* key, uint32_t n, const double* def, double* out) {
+  napi_value v;
+  bool has = false;
+  napi_has_named_property(env, obj, key, &has);
+  if (has) napi_get_named_property(env, obj, key, &v);
+  if (!has || is_nullish(env, v)) {
+    for (uint32_t i = 0; i < n; ++i) out[i] = def[i];
+    return true;
+  }
+  for (uint32_t i = 0; i < n; ++i) {
+    napi_value e;
+    if (napi_get_element(env, v, i, &e) != napi_ok || napi_get_value_double(env, e, &out[i]) != napi_ok) {
+      std::string m = std::string("'") + key + "' must hold " + std::to_string(n) + " numbers";
+      throw_type(env, m.c_str());
+      return false;
+    }
+  }
+  return true;
+}
+
 // Borrows the backing store of a typed-array property of the given type.
 bool get_typed(napi_env env, napi_value obj, const char* key, napi_typedarray_type want, bool optional,
                void** data, size_t* length) {
@@ -310,8 +348,31 @@ napi_value SetScene(napi_env env, napi_callback_info info) {
   bool has_env = false;
   napi_has_named_property(env, desc, "environment", &has_env);
   if (has_env && napi_get_named_property(env, desc, "environment", &envv) == napi_ok && !is_nullish(env, envv)) {
+    // {kind: 1, radiance} or {kind: 2, texels: Float32Array, width, height,
+    //  channels, intensity, rotationY}; the engine copies the texels.
     if (!get_u32(env, envv, "kind", 0, &sd.environment.kind)) return nullptr;
     if (!get_vec(env, envv, "radiance", 3, zero, sd.environment.radiance)) return nullptr;
+    void* td = nullptr;
+    size_t tn = 0;
+    if (!get_typed(env, envv, "texels", napi_float32_array, true, &td, &tn)) return nullptr;
+    if (!get_u32(env, envv, "width", 0, &sd.environment.width)) return nullptr;
+    if (!get_u32(env, envv, "height", 0, &sd.environment.height)) return nullptr;
+    if (!get_u32(env, envv, "channels", 0, &sd.environment.channels)) return nullptr;
+    if (!get_f32(env, envv, "intensity", 1.0f, &sd.environment.intensity)) return nullptr;
+    if (!get_f32(env, envv, "rotationY", 0.0f, &sd.environment.rotation_y)) return nullptr;
+    if (td && (uint64_t)sd.environment.width * sd.environment.height * sd.environment.channels != tn) {
+      throw_type(env, "'texels' must hold width x height x channels floats");
+      return nullptr;
+    }
+    sd.environment.texels = (const float*)td;
+    // {kind: 3, sunDirection, turbidity, rayleigh, mieCoefficient,
+    //  mieDirectionalG, intensity} (defaults of environmentPacking.ts:245-262)
+    const double up[3] = {0.0, 1.0, 0.0};
+    if (!get_dvec(env, envv, "sunDirection", 3, up, sd.environment.sun_direction)) return nullptr;
+    if (!get_f64(env, envv, "turbidity", 2.0, &sd.environment.turbidity)) return nullptr;
+    if (!get_f64(env, envv, "rayleigh", 1.0, &sd.environment.rayleigh)) return nullptr;
+    if (!get_f64(env, envv, "mieCoefficient", 0.005, &sd.environment.mie_coefficient)) return nullptr;
+    if (!get_f64(env, envv, "mieDirectionalG", 0.8, &sd.environment.mie_directional_g)) return nullptr;
   }
   if (!get_f32(env, desc, "sceneRadius", 0.0f, &sd.scene_radius)) return nullptr;
   // emitters: [{kind, color, intensity, castShadow, position, direction, normal,

@@ -74,7 +74,54 @@ class Material(C.Structure):
 
 
 class Environment(C.Structure):
-    _fields_ = [("kind", C.c_uint32), ("radiance", C.c_float * 3)]
+    _fields_ = [("kind", C.c_uint32), ("radiance", C.c_float * 3), ("texels", F32P),
+                ("width", C.c_uint32), ("height", C.c_uint32), ("channels", C.c_uint32),
+                ("intensity", C.c_float), ("rotation_y", C.c_float), ("sun_direction", C.c_double * 3),
+                ("turbidity", C.c_double), ("rayleigh", C.c_double), ("mie_coefficient", C.c_double),
+                ("mie_directional_g", C.c_double)]
+
+
+def _sky_environment(sky) -> "Environment":
+    """ProceduralSkyEnvironment (environment.ts:59-71) with the reference's
+    defaults (environmentPacking.ts:245-262)."""
+    env = Environment()
+    env.kind = 3
+    env.sun_direction = (C.c_double * 3)(*sky.get("sunDirection", (0.0, 1.0, 0.0)))
+    env.turbidity = float(sky.get("turbidity", 2.0))
+    env.rayleigh = float(sky.get("rayleigh", 1.0))
+    env.mie_coefficient = float(sky.get("mieCoefficient", 0.005))
+    env.mie_directional_g = float(sky.get("mieDirectionalG", 0.8))
+    env.intensity = float(sky.get("intensity", 1.0))
+    return env
+
+
+def bake_procedural_sky(sky, width=256, height=128):
+    """vt_bake_procedural_sky: the Preetham map ((height, width, 3) float32)
+    and its luminance x solid-angle integral; host only, no device."""
+    env = _sky_environment(sky)
+    out = np.zeros((height, width, 3), np.float32)
+    integral = C.c_double(0.0)
+    check(lib().vt_bake_procedural_sky(C.byref(env), width, height, out.ctypes.data_as(F32P), C.byref(integral)))
+    return out, integral.value
+
+
+def _environment(scene):
+    """(Environment, keep-alive) of a scene dict: "hdri" = {"texels": (H, W, C)
+    float32 array, "intensity", "rotation_y"} (HdriEnvironment), else "sky" =
+    constant radiance [r, g, b]."""
+    if scene.get("procedural_sky") is not None:
+        return _sky_environment(scene["procedural_sky"]), None
+    hdri = scene.get("hdri")
+    if hdri is not None:
+        tex = np.ascontiguousarray(hdri["texels"], dtype=np.float32)
+        if tex.ndim == 2:
+            tex = tex[:, :, None]
+        h, w, ch = tex.shape
+        env = Environment(2, (C.c_float * 3)(0, 0, 0), tex.ctypes.data_as(F32P), w, h, ch,
+                          float(hdri.get("intensity", 1.0)), float(hdri.get("rotation_y", 0.0)))
+        return env, tex
+    sky = scene.get("sky", [0.0, 0.0, 0.0])
+    return Environment(1 if any(sky) else 0, (C.c_float * 3)(*sky)), None
 
 
 class Emitter(C.Structure):
@@ -157,6 +204,7 @@ def lib():
                                               C.c_void_p]
         L.vt_capture_output.argtypes = [C.c_void_p, F32P, C.c_uint32, C.c_uint32, C.c_void_p]
         L.vt_capture_aux.argtypes = [C.c_void_p, C.c_uint32, F32P, C.c_uint32, C.c_uint32]
+        L.vt_bake_procedural_sky.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, F32P, C.POINTER(C.c_double)]
         _LIB = L
     return _LIB
 
@@ -529,12 +577,11 @@ def _scene_desc(scene):
         mats.append(_material(p.get("material", {})))
     arr, keep = _prim_structs(prims)
     marr = (Material * max(1, len(mats)))(*mats)
-    sky = scene.get("sky", [0.0, 0.0, 0.0])
-    env = Environment(1 if any(sky) else 0, (C.c_float * 3)(*sky))
+    env, tex = _environment(scene)
     ems = scene.get("emitters", [])
     earr = (Emitter * max(1, len(ems)))(*[_emitter(e) for e in ems])
     desc = SceneDesc(arr, len(prims), marr, len(mats), env, float(scene.get("scene_radius", 0.0)), earr, len(ems))
-    return desc, (arr, keep, marr, earr)
+    return desc, (arr, keep, marr, earr, tex)
 
 
 class Batch:
