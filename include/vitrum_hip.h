@@ -321,6 +321,20 @@ vt_status vt_trace_closest_device(vt_engine *eng, const vt_ray *d_rays, uint32_t
                                   vt_hit *d_hits);
 vt_status vt_trace_any_device(vt_engine *eng, const vt_ray *d_rays, uint32_t n,
                               uint32_t *d_occluded);
+/* No counterpart in the reference.  vt_trace_closest_device for callers that
+ * cannot allocate device memory (code under stream capture, serving loops,
+ * harnesses that forbid hipMalloc): it allocates nothing.  The launch's ray
+ * counter, error word and stack-spill area live in a fixed 64 MiB device
+ * array of the library (one copy per GPU, looked up at vt_create and held
+ * until the process ends), which bounds the persistent grid at 606 blocks
+ * (512 on a 256-CU MI355X); rays and hits are the caller's.  Same hit
+ * records as vt_trace_closest_device (binary / 4-wide walk; VT_ERR_STATE in
+ * the cwbvh-closest mode); at most 2^31 - 1 rays per call (VT_ERR_RANGE).
+ * Enqueued on the engine stream.  Every call on a GPU shares that GPU's
+ * array: calls that overlap on one GPU (other engines' streams, other host
+ * threads) race and must be serialised by the caller. */
+vt_status vt_trace_closest_device_noalloc(vt_engine *eng, const vt_ray *d_rays, uint32_t n,
+                                          vt_hit *d_hits);
 /* Work counters of the last closest-hit launch made with counting enabled
  * (vt_set_counting).  counts: 4 u32 per ray (BLAS boxes tested, triangles
  * tested, TLAS boxes tested, instances entered), host buffer. */

@@ -264,6 +264,7 @@ vt_status vt_create(int32_t device_ordinal, const vt_options* opts, vt_engine** 
     if (s) return set_err(nullptr, s, e->last_error);
     HIP_TRY(nullptr, vtk::launch_zero((uint32_t*)e->error_word.ptr, 16, e->stream));
     HIP_TRY(nullptr, hipStreamSynchronize(e->stream));
+    if (engine_noalloc_setup(e.get())) return set_err(nullptr, VT_ERR_HIP, e->last_error);
     *out = e.release();
     return VT_OK;
   });

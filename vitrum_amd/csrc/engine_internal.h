@@ -104,8 +104,12 @@ struct vt_engine {
   std::string last_error;
   uint32_t fail_inject = 0;    // kInject* bits
   uint32_t build_threads = 0;  // host threads of the scene builder (0: every hardware thread)
+  void* noalloc_scratch = nullptr;  // this device's copy of noalloc.hip's scratch array
 };
 
+// Looks up the device address of noalloc.hip's scratch array for the engine's
+// device (noalloc.hip; called once by vt_create).
+vt_status engine_noalloc_setup(vt_engine* e);
 
 // Threads the scene builder uses for this engine.
 uint32_t engine_build_threads(const vt_engine* e);

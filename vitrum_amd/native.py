@@ -556,6 +556,10 @@ class Engine:
         check(lib().vt_trace_closest_device(self._h, C.c_void_p(d_rays), n, C.c_void_p(d_hits)),
               self._h)
 
+    def trace_closest_device_noalloc(self, d_rays, n, d_hits):
+        check(lib().vt_trace_closest_device_noalloc(self._h, C.c_void_p(d_rays), C.c_uint32(n),
+                                                    C.c_void_p(d_hits)), self._h)
+
     def trace_any_device(self, d_rays, n, d_out):
         check(lib().vt_trace_any_device(self._h, C.c_void_p(d_rays), n, C.c_void_p(d_out)), self._h)
 
