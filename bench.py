@@ -305,7 +305,8 @@ def main():
                     help="run one device-to-device copy of this size first (HBM counter calibration)")
     ap.add_argument("--refill-min", type=int, default=0, help="traversal wave refill threshold (0: default)")
     ap.add_argument("--steps-per-refill", type=int, default=0, help="traversal steps between refills (0: default)")
-    ap.add_argument("--near-first", type=int, default=-1, help="1: nearer child first, 0: left first (-1: default)")
+    ap.add_argument("--near-first", type=int, default=-1,
+                    help="2: nearest first in BLASes and the TLAS, 1: in BLASes only, 0: left first (-1: default)")
     ap.add_argument("--defer-min", type=int, default=0,
                     help="lanes that must need a leaf/instance operation before a wave runs it (0: default)")
     ap.add_argument("--node-width", type=int, default=4, choices=(2, 4),
@@ -330,11 +331,11 @@ def main():
 
     eng = native.Engine(0)
     eng.set_node_width(args.node_width)
+    near_first = 2 if args.near_first < 0 else args.near_first
     if args.refill_min or args.steps_per_refill or args.near_first >= 0 or args.defer_min:
-        eng.set_trace_tuning(args.refill_min or 8, args.steps_per_refill or 16,
-                             1 if args.near_first < 0 else args.near_first, args.defer_min or 12)
+        eng.set_trace_tuning(args.refill_min or 8, args.steps_per_refill or 16, near_first, args.defer_min or 12)
     key = kernel_key(f"node_width={args.node_width} refill={args.refill_min or 8} "
-                     f"steps={args.steps_per_refill or 16} near_first={1 if args.near_first < 0 else args.near_first} "
+                     f"steps={args.steps_per_refill or 16} near_first={near_first} "
                      f"defer={args.defer_min or 12}")
     if args.calibrate_copy_mib:
         nb = args.calibrate_copy_mib << 20

@@ -345,15 +345,17 @@ vt_status vt_trace_closest_counts(vt_engine *eng, const vt_ray *rays, uint32_t n
 /* Traversal scheduling knobs (no counterpart in the reference): a wave is
  * refilled once `refill_min` (1..64) of its lanes are idle, lanes run up to
  * `steps` traversal steps between refill checks, and `near_first` orders the
- * closest-hit traversal: 0 = the reference's left-first order; 1 (default) =
- * nearer child first inside each BLAS, equal triangle distances resolved as
- * the reference resolves them (lowest reordered index), instances in the
+ * closest-hit traversal: 0 = the reference's left-first order; 1 = nearer
+ * child first inside each BLAS, equal triangle distances resolved as the
+ * reference resolves them (lowest reordered index), instances in the
  * reference's TLAS order -- hit records identical to the reference's except
  * where two distinct triangles of one BLAS are hit at exactly the same
  * distance and the lower-indexed one's box test rounds above it (the
- * order-dependent pop-time cull can then pick differently); 2 = also
- * nearest-first across instances (can differ from the reference on coplanar
- * geometry in different instances; for measurement).  A lane whose next
+ * order-dependent pop-time cull can then pick differently); 2 (default) =
+ * also nearest-first across instances, with every world bound widened by a
+ * near-tie margin and a ray whose hits in two instances come within that
+ * margin walked again in the reference's TLAS order, so instances add no
+ * further difference.  A lane whose next
  * operation is a BLAS leaf's triangle tests, an instance entry or an instance
  * exit waits until `defer_min` (1..64) lanes of its wave need the same
  * operation (or no lane has node work left), so the wave runs it once for

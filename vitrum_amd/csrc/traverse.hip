@@ -315,12 +315,20 @@ __device__ __forceinline__ V3 xform_dir(float4 c0, float4 c1, float4 c2, V3 d) {
 // wins); instances are still visited in the reference's TLAS order, because
 // with coplanar geometry in different instances the reference's choice
 // depends on that order through the rounding of the instance transforms.
-// near_first = 2 also orders the TLAS by distance (lower TLAS slot wins equal
-// world distances; an instance below the current best's slot gets its local
-// bound widened by 2^-18); it can differ from the reference on such coplanar
-// instance geometry and exists for measurement only.  Node culling stays
-// `tNear > best` (equal distances are still visited).  The counting variant
-// always uses the reference order, so its work counts are the reference's.
+// near_first = 2 also orders the TLAS by distance.  The reference's choice
+// between instances depends on its order only through near-ties: an
+// instance's local search is bounded by the best world distance found so far,
+// mapped into the instance's space, and where two instances' hits lie within
+// the rounding of that mapping the earlier-visited one can exclude the other.
+// So in this mode every world bound is widened by a margin (2^-13 of the
+// distance plus the origin's magnitude, far above the transforms' rounding),
+// which keeps every hit near the current best visible, and a lane that sees
+// two instances' hits within that margin of each other walks its ray again in
+// the reference's TLAS order.  Without such a near-tie the minimum world
+// distance is what any order finds, so the hit records equal the reference's.
+// Node culling stays `tNear > bound` (equal distances are still visited).  The
+// counting variant always uses the reference order, so its work counts are
+// the reference's.
 template <bool ANY, bool COUNT, bool MULTI, bool W4, uint32_t BLOCK, uint32_t LDS_DEPTH>
 // Register budget: four waves per SIMD (128 VGPRs; with the 20-entry LDS
 // stack, 8 blocks of 128 fit a CU), which beats three waves without spills.
@@ -369,6 +377,10 @@ __global__ __launch_bounds__(BLOCK, 4) void trace_kernel(TraceParams p) {
   V3 bpos{};
   uint32_t bprim = 0, binst = kMiss, bslot = 0;
   bool bfront = false;  // the best hit is on the front face (world orientation)
+  // near_first = 2: the lane walks in the reference's TLAS order (after a
+  // near-tie), a near-tie was seen, and the magnitude of the ray origin.
+  bool ref_order = false, recheck = false;
+  float rscale = 0.0f;
   // Instance-local state.
   V3 lo{}, ld{}, linv{}, lpos{};
   WtRay lw{};
@@ -382,6 +394,14 @@ __global__ __launch_bounds__(BLOCK, 4) void trace_kernel(TraceParams p) {
   uint32_t pend = 0;  // pending deferred operation (kPend*)
   uint32_t c_blas = 0, c_tri = 0, c_tlas = 0, c_inst = 0;
 
+  // The world-space bound of the TLAS walk: the best distance, widened by the
+  // near-tie margin while the lane orders the TLAS by distance.
+  auto world_bound = [&]() -> float {
+    if (ANY) return tmax;
+    if (near_tlas && !ref_order && binst != kMiss) return best + (best + rscale) * 0x1p-13f;
+    return best;
+  };
+
   auto enter_instance = [&](uint32_t s) {
     const float4* ir = insts + 9 * (size_t)s;
     const float4 c0 = ir[0], c1 = ir[1], c2 = ir[2], c3 = ir[3];
@@ -394,12 +414,11 @@ __global__ __launch_bounds__(BLOCK, 4) void trace_kernel(TraceParams p) {
     ltmin = fmaxf(dot3(sub(xform_point(c0, c1, c2, c3, start), lo), ld), 0.0f);
     // Closest: localTMax starts at INFINITY; any-hit: at the ray's tMax
     // (tlasSceneHitTraversal.wgsl.ts:103-108 and :168-173).
-    const float bound = ANY ? tmax : best;
+    const float bound = world_bound();
     float ltmax = ANY ? tmax : inf_const();
     if (bound < inf_const() * 0.5f) {
       const V3 end = v3(ro.x + rd.x * bound, ro.y + rd.y * bound, ro.z + rd.z * bound);
       ltmax = fmaxf(dot3(sub(xform_point(c0, c1, c2, c3, end), lo), ld), ltmin);
-      if (near_tlas && binst != kMiss && s < bslot) ltmax = ltmax * (1.0f + 0x1p-18f);
     }
     lbest = ltmax;
     lhit = false;
@@ -417,7 +436,9 @@ __global__ __launch_bounds__(BLOCK, 4) void trace_kernel(TraceParams p) {
       const float4* ir = insts + 9 * (size_t)slot + 4;
       const V3 w = xform_point(ir[0], ir[1], ir[2], ir[3], lpos);
       const float wd = dot3(sub(w, ro), rd);
-      if (wd > tmin && (wd < best || (near_tlas && wd == best && binst != kMiss && slot < bslot))) {
+      if (near_tlas && !ref_order && binst != kMiss && fabsf(wd - best) <= (best + rscale) * 0x1p-13f)
+        recheck = true;
+      if (wd > tmin && wd < best) {
         best = wd;
         bpos = w;
         bprim = lprim;
@@ -511,6 +532,9 @@ __global__ __launch_bounds__(BLOCK, 4) void trace_kernel(TraceParams p) {
           rd = v3(b.x, b.y, b.z);
           tmax = b.w;
           rinv = v3(safe_inv(rd.x), safe_inv(rd.y), safe_inv(rd.z));
+          rscale = fmaxf(fabs_(ro.x), fmaxf(fabs_(ro.y), fabs_(ro.z)));
+          ref_order = false;
+          recheck = false;
           best = tmax;
           binst = kMiss;
           bprim = 0;
@@ -720,11 +744,29 @@ __global__ __launch_bounds__(BLOCK, 4) void trace_kernel(TraceParams p) {
           }
           if (sp > 0) {
             const uint2 e = stk_load(--sp);
-            if (!(u2f(e.y) > (ANY ? tmax : best))) {
+            if (!(u2f(e.y) > world_bound())) {
               cur = e.x;
               break;
             }
             continue;
+          }
+          if (recheck) {
+            // A near-tie between instances: walk the ray again, from its
+            // original interval, in the reference's TLAS order.
+            const float4 a = p.rays[2 * (size_t)ray_idx], b = p.rays[2 * (size_t)ray_idx + 1];
+            tmin = a.w;
+            tmax = b.w;
+            best = tmax;
+            binst = kMiss;
+            bprim = 0;
+            bslot = 0;
+            bu = bv = 0.0f;
+            level = 0;
+            sp = 0;
+            ref_order = true;
+            recheck = false;
+            cur = tlas_root;
+            break;
           }
           // traceClosest (intersection.wgsl.ts:462-490): a back-face hit on a
           // single-sided material is not a surface; the trace starts again
@@ -757,7 +799,7 @@ __global__ __launch_bounds__(BLOCK, 4) void trace_kernel(TraceParams p) {
         const V3 o = level ? lo : ro;
         const V3 inv = level ? linv : rinv;
         const float tm = level ? ltmin : tmin;
-        const float bound = level ? lbest : (ANY ? tmax : best);
+        const float bound = level ? lbest : world_bound();
         float4 a = nd[0], b = nd[3];
         float n0, n1, n2, n3, f0, f1, f2, f3;
         {
@@ -808,7 +850,7 @@ __global__ __launch_bounds__(BLOCK, 4) void trace_kernel(TraceParams p) {
         // slot, which is the reference's depth-first order.  The others are
         // pushed so that they pop in slot order.
         uint32_t first = k0 ? 0u : (k1 ? 1u : (k2 ? 2u : (k3 ? 3u : 4u)));
-        if ((level ? near : near_tlas) && first < 4u) {
+        if ((level ? near : (near_tlas && !ref_order)) && first < 4u) {
           float bt = n0;
           bt = first == 1u ? n1 : (first == 2u ? n2 : (first == 3u ? n3 : bt));
           if (k1 && n1 < bt) first = 1u, bt = n1;
@@ -833,7 +875,7 @@ __global__ __launch_bounds__(BLOCK, 4) void trace_kernel(TraceParams p) {
         const V3 o = level ? lo : ro;
         const V3 inv = level ? linv : rinv;
         const float tm = level ? ltmin : tmin;
-        const float bound = level ? lbest : (ANY ? tmax : best);
+        const float bound = level ? lbest : world_bound();
         float tn_l, tn_r;
         const bool pass_l = ref_l != kEmpty && slab(o, inv, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, tm, tn_l);
         const bool pass_r = ref_r != kEmpty && slab(o, inv, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, tm, tn_r);
@@ -843,7 +885,7 @@ __global__ __launch_bounds__(BLOCK, 4) void trace_kernel(TraceParams p) {
         }
         const bool take_l = pass_l && !(tn_l > bound), take_r = pass_r && !(tn_r > bound);
         if (take_l && take_r) {
-          const bool right_first = (level ? near : near_tlas) && tn_r < tn_l;
+          const bool right_first = (level ? near : (near_tlas && !ref_order)) && tn_r < tn_l;
           if (!push(right_first ? ref_l : ref_r, right_first ? tn_l : tn_r)) {
             finish(false);
             continue;
