@@ -140,17 +140,26 @@ __device__ __forceinline__ void one_blob4(float x, uint32_t b0, float* out) {
 // K order of nmap: kb 0 -> dims {4g..4g+3, 16+4g..16+4g+3}, kb 1 -> dims
 // {32+4g..32+4g+3, 48+4g..48+4g+3}.  Lane group g owns hash-grid levels 2g
 // and 2g + 1.
+// Levels kLdsFirstLevel.. of the tables are read from a workgroup copy in LDS
+// when `lds` is given (the inference kernel stages them: their rows are hit
+// at random, while the coarse levels' few rows stay in the caches).
+constexpr uint32_t kLdsFirstLevel = 3;
+constexpr uint32_t kLdsRows = (kLevels - kLdsFirstLevel) * kRows;  // 20,480 float2 = 160 KiB
+
 __device__ __forceinline__ void encode_lane(uint32_t g, const float* f, const float* __restrict__ tables,
-                                            float (&v)[2][8]) {
+                                            float (&v)[2][8], const float2* lds = nullptr) {
   // hash grid, levels 2g, 2g + 1
 #pragma unroll
   for (uint32_t j = 0; j < 2; ++j) {
     Corners c;
-    grid_corners(2u * g + j, f[0], f[1], f[2], c);
+    const uint32_t level = 2u * g + j;
+    grid_corners(level, f[0], f[1], f[2], c);
     float f0 = 0.0f, f1 = 0.0f;
 #pragma unroll
     for (uint32_t k = 0; k < 8; ++k) {
-      const float2 t = *reinterpret_cast<const float2*>(tables + 2u * (size_t)c.row[k]);
+      const float2 t = (lds && level >= kLdsFirstLevel)
+                           ? lds[c.row[k] - kLdsFirstLevel * kRows]
+                           : *reinterpret_cast<const float2*>(tables + 2u * (size_t)c.row[k]);
       f0 += c.w[k] * t.x;
       f1 += c.w[k] * t.y;
     }
@@ -217,7 +226,8 @@ __global__ void pack_kernel(const float* __restrict__ w, uint4* __restrict__ fra
 
 // Builds the B fragments of the encoded inputs of 4 sample blocks.
 __device__ __forceinline__ void encode_tile(const float* __restrict__ feats, const float* __restrict__ tables,
-                                            uint32_t n, uint32_t base, uint32_t g, uint32_t r, uint4 (&B)[4][2]) {
+                                            uint32_t n, uint32_t base, uint32_t g, uint32_t r, uint4 (&B)[4][2],
+                                            const float2* lds = nullptr) {
 #pragma unroll
   for (uint32_t sb = 0; sb < 4; ++sb) {
     const uint32_t s = base + sb * 16u + r;
@@ -230,7 +240,7 @@ __device__ __forceinline__ void encode_tile(const float* __restrict__ feats, con
         const float4 x = fp[q];
         f[4 * q] = x.x, f[4 * q + 1] = x.y, f[4 * q + 2] = x.z, f[4 * q + 3] = x.w;
       }
-      encode_lane(g, f, tables, v);
+      encode_lane(g, f, tables, v, lds);
     } else {
 #pragma unroll
       for (uint32_t t = 0; t < 8; ++t) v[0][t] = v[1][t] = 0.0f;
@@ -287,152 +297,203 @@ __device__ __forceinline__ void output_layer(const uint4* __restrict__ frags, ui
 }
 
 // ---- inference: one wave = 64 samples -------------------------------------------
+//
+// Persistent: one 1024-thread workgroup per CU copies levels 3-7 of the
+// tables (160 KiB) into LDS once, then each of its 16 waves takes tiles of 64
+// samples, grid-strided.  The encoding reads the same f32 table values as from
+// memory, so the outputs are those of a direct gather.
 
-__global__ __launch_bounds__(256) void infer_kernel(const float* __restrict__ feats, uint32_t cap,
-                                                    const uint32_t* __restrict__ n_ptr,
-                                                    const uint4* __restrict__ frags,
-                                                    const float* __restrict__ tables, float* __restrict__ rgb) {
+__global__ __launch_bounds__(1024) void infer_kernel(const float* __restrict__ feats, uint32_t cap,
+                                                     const uint32_t* __restrict__ n_ptr,
+                                                     const uint4* __restrict__ frags,
+                                                     const float* __restrict__ tables, float* __restrict__ rgb) {
+  __shared__ float2 tab[kLdsRows];
+  const float2* src = reinterpret_cast<const float2*>(tables) + kLdsFirstLevel * kRows;
+  for (uint32_t i = threadIdx.x; i < kLdsRows; i += 1024u) tab[i] = src[i];
+  __syncthreads();
   const uint32_t n = n_ptr ? min(*n_ptr, cap) : cap;
   const uint32_t lane = threadIdx.x & 63u, g = lane >> 4, r = lane & 15u;
-  const uint32_t base = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64u;
-  if (base >= n) return;  // whole wave past the end
-  uint4 B[4][2];
-  encode_tile(feats, tables, n, base, g, r, B);
+  const uint32_t tiles = (n + 63u) / 64u, waves = gridDim.x * 16u;
+  for (uint32_t tile = blockIdx.x * 16u + (threadIdx.x >> 6); tile < tiles; tile += waves) {
+    const uint32_t base = tile * 64u;
+    // re-read the fragments every tile (an opaque pointer copy keeps them
+    // from being hoisted out of the loop into spilled registers)
+    const uint4* fw = frags;
+    asm volatile("" : "+s"(fw));
+    uint4 B[4][2];
+    encode_tile(feats, tables, n, base, g, r, B, tab);
 #pragma unroll
-  for (uint32_t l = 0; l < kHidden; ++l) (void)hidden_layer(frags + l * kFragsHidden, lane, B);
-  f32x4 Y[4];
-  output_layer(frags, lane, B, Y);
-  if (g == 0u) {
+    for (uint32_t l = 0; l < kHidden; ++l) (void)hidden_layer(fw + l * kFragsHidden, lane, B);
+    f32x4 Y[4];
+    output_layer(fw, lane, B, Y);
+    if (g == 0u) {
 #pragma unroll
-    for (uint32_t sb = 0; sb < 4; ++sb) {
-      const uint32_t s = base + sb * 16u + r;
-      if (s < n) {
-        rgb[3 * (size_t)s] = Y[sb][0];
-        rgb[3 * (size_t)s + 1] = Y[sb][1];
-        rgb[3 * (size_t)s + 2] = Y[sb][2];
+      for (uint32_t sb = 0; sb < 4; ++sb) {
+        const uint32_t s = base + sb * 16u + r;
+        if (s < n) {
+          rgb[3 * (size_t)s] = Y[sb][0];
+          rgb[3 * (size_t)s + 1] = Y[sb][1];
+          rgb[3 * (size_t)s + 2] = Y[sb][2];
+        }
       }
     }
   }
 }
 
 // ---- training: forward, relative-L2 gradient, backward, dW ------------------------
+//
+// A workgroup of four waves trains on tiles of 64 samples; wave w owns the
+// samples w*16 .. w*16+15 of a tile (one 16-column block of every MFMA), so a
+// wave's activations are 2 fragments per layer and its registers leave room
+// for three workgroups per CU.  The layer inputs are staged transposed
+// ([neuron][sample], bf16) in the workgroup's LDS for the weight gradients,
+// dW = D^T H over the tile's 64 samples, which wave w computes for output
+// rows w*16 .. w*16+15 into accumulators that stay in registers across all of
+// the workgroup's tiles; they are written to the workgroup's slot once.
 
 constexpr uint32_t kTile = 64u * 64u;  // one transposed [neuron][sample] bf16 tile
 
-// Stores the B fragments of a layer input transposed: T[n * 64 + s].
-__device__ __forceinline__ void store_t(uint16_t* T, uint32_t g, uint32_t r, const uint4 (&B)[4][2]) {
+// Encoded inputs of sample s for lane group g (zero past n).
+__device__ __forceinline__ void encode_block(const float* __restrict__ feats, const float* __restrict__ tables,
+                                             uint32_t n, uint32_t s, uint32_t g, uint4 (&B)[2]) {
+  float v[2][8];
+  if (s < n) {
+    float f[kFeat];
+    const float4* fp = reinterpret_cast<const float4*>(feats + (size_t)s * kFeat);
 #pragma unroll
-  for (uint32_t sb = 0; sb < 4; ++sb)
-#pragma unroll
-    for (uint32_t kb = 0; kb < 2; ++kb) {
-      const uint32_t w[4] = {B[sb][kb].x, B[sb][kb].y, B[sb][kb].z, B[sb][kb].w};
-#pragma unroll
-      for (uint32_t t = 0; t < 8; ++t)
-        T[nmap(kb, g, t) * 64u + sb * 16u + r] = (uint16_t)((w[t >> 1] >> ((t & 1u) * 16u)) & 0xFFFFu);
+    for (uint32_t q = 0; q < 4; ++q) {
+      const float4 x = fp[q];
+      f[4 * q] = x.x, f[4 * q + 1] = x.y, f[4 * q + 2] = x.z, f[4 * q + 3] = x.w;
     }
+    encode_lane(g, f, tables, v);
+  } else {
+#pragma unroll
+    for (uint32_t t = 0; t < 8; ++t) v[0][t] = v[1][t] = 0.0f;
+  }
+  B[0] = pack8(v[0]);
+  B[1] = pack8(v[1]);
 }
 
-// dW (rows: `rows` output neurons in 16-blocks) += D^T H over the 64 samples,
-// with D = DT ([neuron][sample]) and H = HT, into the workgroup's own slot
-// (slot[o*64 + i]; `first`: the slot's first tile, which stores instead of
-// adding).  Every element belongs to exactly one lane, so plain loads and
-// stores suffice.
-__device__ __forceinline__ void weight_grad(const uint16_t* DT, const uint16_t* HT, uint32_t row_blocks,
-                                            uint32_t rows, uint32_t g, uint32_t r, float* __restrict__ slot,
-                                            bool first) {
-  for (uint32_t ob = 0; ob < row_blocks; ++ob) {
-    f32x4 C[4];
+// One hidden layer of one 16-sample block: B -> ReLU(W B) in bf16; returns
+// the positive-output mask (bit kb*8 + t).
+__device__ __forceinline__ uint32_t hidden_block(const uint4* __restrict__ frags, uint32_t lane, uint4 (&B)[2]) {
+  f32x4 C[4];
 #pragma unroll
-    for (uint32_t ib = 0; ib < 4; ++ib) C[ib] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (uint32_t ob = 0; ob < 4; ++ob) C[ob] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (uint32_t ks = 0; ks < 2; ++ks) {
-      const uint4 a = *reinterpret_cast<const uint4*>(DT + (ob * 16u + r) * 64u + ks * 32u + g * 8u);
+  for (uint32_t ob = 0; ob < 4; ++ob)
 #pragma unroll
-      for (uint32_t ib = 0; ib < 4; ++ib) {
-        const uint4 b = *reinterpret_cast<const uint4*>(HT + (ib * 16u + r) * 64u + ks * 32u + g * 8u);
-        C[ib] = mfma(frag_of(a), frag_of(b), C[ib]);
-      }
+    for (uint32_t kb = 0; kb < 2; ++kb) C[ob] = mfma(frag_of(frags[(ob * 2u + kb) * 64u + lane]), frag_of(B[kb]), C[ob]);
+  uint32_t mask = 0;
+#pragma unroll
+  for (uint32_t kb = 0; kb < 2; ++kb) {
+    float v[8];
+#pragma unroll
+    for (uint32_t t = 0; t < 8; ++t) {
+      const float x = C[2u * kb + (t >> 2)][t & 3u];
+      v[t] = x > 0.0f ? x : 0.0f;
+      if (x > 0.0f) mask |= 1u << (kb * 8u + t);
     }
+    B[kb] = pack8(v);
+  }
+  return mask;
+}
+
+// Backward through W_m^T for one block: dH = (W_m^T dNext) masked by ReLU'.
+__device__ __forceinline__ void backward_block(const uint4* __restrict__ fr, uint32_t lane, uint32_t kbs,
+                                               uint32_t mask, uint4 (&B)[2]) {
+  f32x4 C[4];
 #pragma unroll
-    for (uint32_t ib = 0; ib < 4; ++ib)
+  for (uint32_t ib = 0; ib < 4; ++ib) C[ib] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-      for (uint32_t j = 0; j < 4; ++j) {
-        const uint32_t o = ob * 16u + g * 4u + j;
-        if (o < rows) {
-          float* q = slot + o * 64u + ib * 16u + r;
-          *q = first ? C[ib][j] : *q + C[ib][j];
-        }
-      }
+  for (uint32_t ib = 0; ib < 4; ++ib)
+    for (uint32_t kb = 0; kb < kbs; ++kb) C[ib] = mfma(frag_of(fr[(ib * 2u + kb) * 64u + lane]), frag_of(B[kb]), C[ib]);
+#pragma unroll
+  for (uint32_t kb = 0; kb < 2; ++kb) {
+    float v[8];
+#pragma unroll
+    for (uint32_t t = 0; t < 8; ++t) v[t] = ((mask >> (kb * 8u + t)) & 1u) ? C[2u * kb + (t >> 2)][t & 3u] : 0.0f;
+    B[kb] = pack8(v);
   }
 }
 
-// Backward through W_m^T: dH (C layout) = W_m^T dNext, masked by ReLU'.
-__device__ __forceinline__ void backward_layer(const uint4* __restrict__ fr, uint32_t lane, uint32_t kbs,
-                                               uint64_t mask, uint4 (&B)[4][2]) {
-  f32x4 C[4][4];
+// Stores a block's fragments transposed into column block w: T[n * 64 + w*16 + r].
+__device__ __forceinline__ void store_block(uint16_t* T, uint32_t w, uint32_t g, uint32_t r, const uint4 (&B)[2]) {
 #pragma unroll
-  for (uint32_t ib = 0; ib < 4; ++ib)
+  for (uint32_t kb = 0; kb < 2; ++kb) {
+    const uint32_t q[4] = {B[kb].x, B[kb].y, B[kb].z, B[kb].w};
 #pragma unroll
-    for (uint32_t sb = 0; sb < 4; ++sb) C[ib][sb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-  for (uint32_t ib = 0; ib < 4; ++ib)
-    for (uint32_t kb = 0; kb < kbs; ++kb) {
-      const bf16x8 a = frag_of(fr[(ib * 2u + kb) * 64u + lane]);
-#pragma unroll
-      for (uint32_t sb = 0; sb < 4; ++sb) C[ib][sb] = mfma(a, frag_of(B[sb][kb]), C[ib][sb]);
-    }
-#pragma unroll
-  for (uint32_t sb = 0; sb < 4; ++sb)
-#pragma unroll
-    for (uint32_t kb = 0; kb < 2; ++kb) {
-      float v[8];
-#pragma unroll
-      for (uint32_t t = 0; t < 8; ++t) {
-        const bool on = (mask >> (sb * 16u + kb * 8u + t)) & 1ull;
-        v[t] = on ? C[2u * kb + (t >> 2)][sb][t & 3u] : 0.0f;
-      }
-      B[sb][kb] = pack8(v);
-    }
+    for (uint32_t t = 0; t < 8; ++t)
+      T[nmap(kb, g, t) * 64u + w * 16u + r] = (uint16_t)((q[t >> 1] >> ((t & 1u) * 16u)) & 0xFFFFu);
+  }
 }
 
-__global__ __launch_bounds__(64) void train_kernel(const float* __restrict__ feats, const float* __restrict__ targets,
-                                                   uint32_t cap, const uint32_t* __restrict__ n_ptr,
-                                                   const uint4* __restrict__ frags, const float* __restrict__ tables,
-                                                   float* __restrict__ partials, float* __restrict__ dx,
-                                                   float* __restrict__ loss) {
+// acc[ib] += D^T H for output rows ob*16.. (A from DT) and input columns
+// ib*16.. (B from HT), K = the tile's 64 samples.
+__device__ __forceinline__ void weight_grad_rows(const uint16_t* DT, const uint16_t* HT, uint32_t ob, uint32_t g,
+                                                 uint32_t r, f32x4 (&acc)[4]) {
+#pragma unroll
+  for (uint32_t ks = 0; ks < 2; ++ks) {
+    const uint4 a = *reinterpret_cast<const uint4*>(DT + (ob * 16u + r) * 64u + ks * 32u + g * 8u);
+#pragma unroll
+    for (uint32_t ib = 0; ib < 4; ++ib) {
+      const uint4 b = *reinterpret_cast<const uint4*>(HT + (ib * 16u + r) * 64u + ks * 32u + g * 8u);
+      acc[ib] = mfma(frag_of(a), frag_of(b), acc[ib]);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256, 3) void train_kernel(const float* __restrict__ feats, const float* __restrict__ targets,
+                                                    uint32_t cap, const uint32_t* __restrict__ n_ptr,
+                                                    const uint4* __restrict__ frags, const float* __restrict__ tables,
+                                                    float* __restrict__ partials, float* __restrict__ dx,
+                                                    float* __restrict__ loss) {
   const uint32_t n = n_ptr ? min(*n_ptr, cap) : cap;
   const uint32_t tiles = (n + 63u) / 64u;
   __shared__ __align__(16) uint16_t HT[5 * kTile];  // inputs of W0..W4, transposed
   __shared__ __align__(16) uint16_t DT[kTile];      // deltas, transposed
-  const uint32_t lane = threadIdx.x, g = lane >> 4, r = lane & 15u;
-  float* __restrict__ slot = partials + (size_t)blockIdx.x * kMlpParams;
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6, g = lane >> 4, r = lane & 15u;
+  f32x4 acc[kHidden][4];  // dW_0..dW_3, rows w*16.., all 64 inputs
+  f32x4 acc_out;          // dW_4 (rows 0..2 of a 16-row block), inputs w*16..
+#pragma unroll
+  for (uint32_t m = 0; m < kHidden; ++m)
+#pragma unroll
+    for (uint32_t ib = 0; ib < 4; ++ib) acc[m][ib] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  acc_out = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  float part = 0.0f;
   // Tiles of 64 samples, strided over the workgroups (uniform loop bounds).
   for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
-  const bool first = tile == blockIdx.x;
-  const uint32_t base = tile * 64u;
-  __syncthreads();  // the previous tile's last reads of HT/DT
-  uint4 B[4][2];
-  uint64_t mask[kHidden];
-  encode_tile(feats, tables, n, base, g, r, B);
-  store_t(HT, g, r, B);
+    const uint32_t s = tile * 64u + w * 16u + r;
+    // The fragments are re-read from the cache every tile: an opaque copy of
+    // the pointer keeps the compiler from hoisting all 68 KB of them out of
+    // the loop into (spilled) registers.
+    const uint4* fw = frags;
+    asm volatile("" : "+s"(fw));
+    const uint4* bw = fw + kFragsFwd;
+    __syncthreads();  // the previous tile's last reads of HT/DT
+    uint4 B[2];
+    uint32_t mask[kHidden];
+    encode_block(feats, tables, n, s, g, B);
+    store_block(HT, w, g, r, B);
 #pragma unroll
-  for (uint32_t l = 0; l < kHidden; ++l) {
-    mask[l] = hidden_layer(frags + l * kFragsHidden, lane, B);
-    store_t(HT + (l + 1u) * kTile, g, r, B);
-  }
-  f32x4 Y[4];
-  output_layer(frags, lane, B, Y);
+    for (uint32_t l = 0; l < kHidden; ++l) {
+      // one layer's fragments at a time: loads are not hoisted across layers
+      __builtin_amdgcn_sched_barrier(0);
+      mask[l] = hidden_block(fw + l * kFragsHidden, lane, B);
+      store_block(HT + (l + 1u) * kTile, w, g, r, B);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    f32x4 Y = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (uint32_t kb = 0; kb < 2; ++kb)
+      Y = mfma(frag_of(fw[kHidden * kFragsHidden + kb * 64u + lane]), frag_of(B[kb]), Y);
 
-  // Relative L2 (fusedMlp.wgsl.ts:65-120): d/dy = 2 (y - t) / (3 n (L^2 + eps)),
-  // L = luminance of the prediction, normalised by its largest component.
-  float part = 0.0f;
-  for (uint32_t i = lane; i < 16u * 64u; i += 64u) DT[i] = 0;  // rows 0..15 of dY^T
-#pragma unroll
-  for (uint32_t sb = 0; sb < 4; ++sb) {
+    // Relative L2 (fusedMlp.wgsl.ts:65-120): d/dy = 2 (y - t) / (3 n (L^2 + eps)),
+    // L = luminance of the prediction, normalised by its largest component.
+    // Lanes of group 0 hold outputs 0..2 of their sample.
     float d[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    const uint32_t s = base + sb * 16u + r;
     if (g == 0u && s < n) {
-      const float p0 = Y[sb][0], p1 = Y[sb][1], p2 = Y[sb][2];
+      const float p0 = Y[0], p1 = Y[1], p2 = Y[2];
       const float t0 = targets[3 * (size_t)s], t1 = targets[3 * (size_t)s + 1], t2 = targets[3 * (size_t)s + 2];
       const float sc = fmaxf(fabsf(p0), fmaxf(fabsf(p1), fabsf(p2)));
       float lum = 0.0f;
@@ -453,80 +514,85 @@ __global__ __launch_bounds__(64) void train_kernel(const float* __restrict__ fea
         if (e == e && den == den) part += e * e / den;
       }
     }
-    B[sb][0] = pack8(d);
-    const float z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    B[sb][1] = pack8(z);
-  }
-  __syncthreads();
-  if (g == 0u) {
+    B[0] = pack8(d);
+    B[1] = make_uint4(0u, 0u, 0u, 0u);
+    // dY^T rows 0..15 of this wave's columns: lane (g, r) writes rows 4g..4g+3
+    // (the outputs it holds; rows 3..15 are zero padding).
+    {
+      const uint32_t q0 = B[0].x, q1 = B[0].y;
 #pragma unroll
-    for (uint32_t sb = 0; sb < 4; ++sb) {
-      const uint32_t w[2] = {B[sb][0].x, B[sb][0].y};
-      DT[0u * 64u + sb * 16u + r] = (uint16_t)(w[0] & 0xFFFFu);
-      DT[1u * 64u + sb * 16u + r] = (uint16_t)(w[0] >> 16);
-      DT[2u * 64u + sb * 16u + r] = (uint16_t)(w[1] & 0xFFFFu);
+      for (uint32_t j = 0; j < 4; ++j) {
+        const uint32_t bits = g == 0u ? ((j < 2u ? q0 : q1) >> ((j & 1u) * 16u)) & 0xFFFFu : 0u;
+        DT[(4u * g + j) * 64u + w * 16u + r] = (uint16_t)(j < 3u ? bits : 0u);
+      }
     }
+    __syncthreads();
+    // dW4 (rows 0..2 padded to 16) for inputs w*16..: one 16x16 block per wave
+#pragma unroll
+    for (uint32_t ks = 0; ks < 2; ++ks) {
+      const uint4 a = *reinterpret_cast<const uint4*>(DT + r * 64u + ks * 32u + g * 8u);
+      const uint4 b = *reinterpret_cast<const uint4*>(HT + 4u * kTile + (w * 16u + r) * 64u + ks * 32u + g * 8u);
+      acc_out = mfma(frag_of(a), frag_of(b), acc_out);
+    }
+    // dH4 through W4^T
+    backward_block(bw + 3u * kFragsHidden, lane, 1u, mask[3], B);
+#pragma unroll
+    for (uint32_t m = 3; m >= 1; --m) {
+      __builtin_amdgcn_sched_barrier(0);
+      __syncthreads();  // every wave's reads of DT are done
+      store_block(DT, w, g, r, B);  // dH_{m+1} (pre-activation delta of layer m's output)
+      __syncthreads();
+      weight_grad_rows(DT, HT + m * kTile, w, g, r, acc[m]);
+      backward_block(bw + (m - 1u) * kFragsHidden, lane, 2u, mask[m - 1u], B);
+    }
+    __syncthreads();
+    store_block(DT, w, g, r, B);  // dH1
+    __syncthreads();
+    weight_grad_rows(DT, HT, w, g, r, acc[0]);
+    // d loss / d x of the hash-grid inputs (dims 0..15) = W0^T dH1: lane
+    // (g, r) receives dims 4g..4g+3 -- the features of levels 2g, 2g + 1 that
+    // it encoded -- of its sample; table_grad_kernel scatters them into the
+    // tables with the trilinear weights (hashGridBackward).
+    f32x4 X = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (uint32_t kb = 0; kb < 2; ++kb) X = mfma(frag_of(fw[kFragsFwd + kFragsBwd + kb * 64u + lane]), frag_of(B[kb]), X);
+    if (s < n) *reinterpret_cast<float4*>(dx + (size_t)s * 16u + 4u * g) = make_float4(X[0], X[1], X[2], X[3]);
   }
-  // loss: wave sum, one atomic
+  // loss: wave sum, one atomic per wave
   for (uint32_t off = 32; off > 0; off >>= 1) part += __shfl_xor(part, (int)off);
-  if (lane == 0) unsafeAtomicAdd(loss, part / (3.0f * (float)n));
-  __syncthreads();
-  // dW4 (3 x 64) and dH4
-  weight_grad(DT, HT + 4u * kTile, 1u, kOut, g, r, slot + 16384u, first);
-  __syncthreads();
-  const uint4* bw = frags + kFragsFwd;
-  backward_layer(bw + 3u * kFragsHidden, lane, 1u, mask[3], B);
-  for (uint32_t m = 3; m >= 1; --m) {
-    store_t(DT, g, r, B);  // dH_{m+1} (pre-activation delta of layer m's output)
-    __syncthreads();
-    weight_grad(DT, HT + m * kTile, 4u, 64u, g, r, slot + m * 4096u, first);
-    __syncthreads();
-    backward_layer(bw + (m - 1u) * kFragsHidden, lane, 2u, mask[m - 1u], B);
-  }
-  store_t(DT, g, r, B);  // dH1
-  __syncthreads();
-  weight_grad(DT, HT, 4u, 64u, g, r, slot, first);
-  // d loss / d x of the hash-grid inputs (dims 0..15) = W0^T dH1: lane (g, r)
-  // receives dims 4g..4g+3 -- the features of levels 2g, 2g + 1 that it
-  // encoded -- of samples sb*16 + r, and scatters them into the table
-  // gradient with the trilinear weights (hashGridBackward: colliding rows
-  // accumulate).
-  f32x4 X[4];
+  if (lane == 0 && part != 0.0f) unsafeAtomicAdd(loss, part / (3.0f * (float)n));
+  // The workgroup's slot: every element is written by exactly one lane.
+  float* __restrict__ slot = partials + (size_t)blockIdx.x * kMlpParams;
 #pragma unroll
-  for (uint32_t sb = 0; sb < 4; ++sb) X[sb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (uint32_t m = 0; m < kHidden; ++m)
 #pragma unroll
-  for (uint32_t kb = 0; kb < 2; ++kb) {
-    const bf16x8 a = frag_of(frags[kFragsFwd + kFragsBwd + kb * 64u + lane]);
+    for (uint32_t ib = 0; ib < 4; ++ib)
 #pragma unroll
-    for (uint32_t sb = 0; sb < 4; ++sb) X[sb] = mfma(a, frag_of(B[sb][kb]), X[sb]);
-  }
-  // Per-sample input gradients of the hash-grid dims, scattered into the
-  // tables by table_grad_kernel.
+      for (uint32_t j = 0; j < 4; ++j) slot[m * 4096u + (w * 16u + g * 4u + j) * 64u + ib * 16u + r] = acc[m][ib][j];
 #pragma unroll
-  for (uint32_t sb = 0; sb < 4; ++sb) {
-    const uint32_t s = base + sb * 16u + r;
-    if (s < n) *reinterpret_cast<float4*>(dx + (size_t)s * 16u + 4u * g) = make_float4(X[sb][0], X[sb][1], X[sb][2], X[sb][3]);
-  }
-  }  // tiles
+  for (uint32_t j = 0; j < 4; ++j)
+    if (g * 4u + j < kOut) slot[16384u + (g * 4u + j) * 64u + w * 16u + r] = acc_out[j];
 }
 
 // Hash-grid table gradient (hashGridBackward): each input gradient of a
 // level's two features, times the trilinear weight of each of the 8 corners,
 // accumulates into that corner's row.  One workgroup per (level, chunk of
 // samples) sums its chunk into a private copy of the level's 4096 x 2
-// gradient in LDS, then adds the copy to grad with one atomic per non-zero
-// element: the coarse levels have only a few hundred distinct rows, and
-// scattering every sample's contribution straight to memory serialises on
-// them.
+// gradient in LDS and stores the copy whole into its own partial slice
+// (part[(chunk * kLevels + level) * 8192 ..]); table_reduce_kernel then sums
+// the chunks' slices.  The coarse levels have only a few hundred distinct
+// rows: scattering every sample's contribution straight to memory, or even
+// every chunk's copy with atomics, serialises on them.
 __global__ __launch_bounds__(256) void table_grad_kernel(const float* __restrict__ feats, const float* __restrict__ dx,
                                                          uint32_t cap, const uint32_t* __restrict__ n_ptr,
-                                                         uint32_t per_chunk, float* __restrict__ tgrad) {
+                                                         uint32_t per_chunk, float* __restrict__ part) {
   __shared__ float acc[kRows * kLevelFeat];
   const uint32_t l = blockIdx.x, tid = threadIdx.x;
   for (uint32_t i = tid; i < kRows * kLevelFeat; i += 256u) acc[i] = 0.0f;
   __syncthreads();
   const uint32_t n = n_ptr ? min(*n_ptr, cap) : cap;
   const uint32_t lo = blockIdx.y * per_chunk, hi = min(n, lo + per_chunk);
+#pragma unroll 2
   for (uint32_t s = lo + tid; s < hi; s += 256u) {
     const float* fp = feats + (size_t)s * kFeat;
     Corners c;
@@ -540,9 +606,21 @@ __global__ __launch_bounds__(256) void table_grad_kernel(const float* __restrict
     }
   }
   __syncthreads();
-  float* __restrict__ out = tgrad + (size_t)l * kRows * kLevelFeat;
-  for (uint32_t i = tid; i < kRows * kLevelFeat; i += 256u)
-    if (acc[i] != 0.0f) unsafeAtomicAdd(out + i, acc[i]);
+  float4* __restrict__ out = reinterpret_cast<float4*>(part + ((size_t)blockIdx.y * kLevels + l) * (kRows * kLevelFeat));
+  const float4* a4 = reinterpret_cast<const float4*>(acc);
+  for (uint32_t i = tid; i < kRows * kLevelFeat / 4u; i += 256u) out[i] = a4[i];
+}
+
+// grad[kMlpParams + e] += the sum over the chunks of element e of the
+// partial table gradients (one thread per element).
+__global__ __launch_bounds__(256) void table_reduce_kernel(const float* __restrict__ part, uint32_t chunks,
+                                                           float* __restrict__ tgrad) {
+  const uint32_t e = blockIdx.x * 256u + threadIdx.x;
+  if (e >= kTableParams) return;
+  const uint32_t l = e / (kRows * kLevelFeat), i = e % (kRows * kLevelFeat);
+  float sum = 0.0f;
+  for (uint32_t c = 0; c < chunks; ++c) sum += part[((size_t)c * kLevels + l) * (kRows * kLevelFeat) + i];
+  tgrad[e] += sum;
 }
 
 // grad[i] += sum of the partial MLP weight gradients of the workgroups that
@@ -603,8 +681,18 @@ hipError_t launch_pack(const float* master, uint4* frags, hipStream_t s) {
 hipError_t launch_infer(const float* feats, uint32_t n, const uint32_t* n_ptr, const uint4* frags,
                         const float* tables, float* rgb, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  const uint32_t waves = (n + 63u) / 64u;
-  hipLaunchKernelGGL(infer_kernel, dim3((waves + 3u) / 4u), dim3(256), 0, s, feats, n, n_ptr, frags, tables, rgb);
+  // one workgroup per CU (each holds 160 KiB of LDS), fewer for small batches
+  static uint32_t cus = 0;
+  if (cus == 0) {
+    int dev = 0, c = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    cus = (uint32_t)std::max(1, c);
+  }
+  const uint32_t tiles = (n + 63u) / 64u;
+  const uint32_t blocks = std::min(cus, (tiles + 15u) / 16u);
+  hipLaunchKernelGGL(infer_kernel, dim3(blocks), dim3(1024), 0, s, feats, n, n_ptr, frags, tables, rgb);
   return hipGetLastError();
 }
 
@@ -619,22 +707,29 @@ hipError_t launch_train(const float* feats, const float* targets, uint32_t n, co
     int dev = 0, cus = 0, per_cu = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, train_kernel, 64, 0);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, train_kernel, 256, 0);
     if (e != hipSuccess) return e;
     resident = (uint32_t)std::max(1, cus * std::max(per_cu, 1));
   }
   const uint32_t tiles = (n + 63u) / 64u;
   const uint32_t blocks = std::min(std::min(tiles, resident), kGradSlots);
-  hipLaunchKernelGGL(train_kernel, dim3(blocks), dim3(64), 0, s, feats, targets, n, n_ptr, frags, tables, part, dx,
+  hipLaunchKernelGGL(train_kernel, dim3(blocks), dim3(256), 0, s, feats, targets, n, n_ptr, frags, tables, part, dx,
                      loss);
   if (hipError_t e = hipGetLastError()) return e;
   hipLaunchKernelGGL(reduce_slots_kernel, dim3((kMlpParams + 255u) / 256u, 32u), dim3(256), 0, s, part, blocks, n,
                      n_ptr, grad);
   if (hipError_t e = hipGetLastError()) return e;
-  // About 4096 samples per workgroup, at most 64 chunks per level.
-  const uint32_t chunks = std::min(64u, (n + 4095u) / 4096u);
+  // About 2048 samples per workgroup, at most 128 chunks per level (enough
+  // workgroups to keep 16 waves on every CU); the chunks' partial tables go
+  // to `part` (free again once reduce_slots_kernel has read the MLP slots:
+  // 128 x 65,536 floats <= kGradSlots x kMlpParams).
+  static_assert((size_t)128 * kTableParams <= (size_t)kGradSlots * kMlpParams, "table partials fit the slots");
+  const uint32_t chunks = std::min(128u, (n + 2047u) / 2048u);
   const uint32_t per_chunk = (n + chunks - 1u) / chunks;
   hipLaunchKernelGGL(table_grad_kernel, dim3(kLevels, chunks), dim3(256), 0, s, feats, dx, n, n_ptr, per_chunk,
+                     part);
+  if (hipError_t e = hipGetLastError()) return e;
+  hipLaunchKernelGGL(table_reduce_kernel, dim3((kTableParams + 255u) / 256u), dim3(256), 0, s, part, chunks,
                      grad + kMlpParams);
   return hipGetLastError();
 }
