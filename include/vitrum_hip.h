@@ -342,6 +342,12 @@ vt_status vt_set_counting(vt_engine *eng, int32_t enabled);
 vt_status vt_trace_closest_counts(vt_engine *eng, const vt_ray *rays, uint32_t n, vt_hit *hits,
                                   uint32_t *counts);
 
+/* Shading order (no counterpart in the reference; the north-star design's
+ * material sort): 1 = each workgroup of the shading kernel counting-sorts its
+ * 256 queue entries by material in LDS (misses first) before shading them;
+ * 0 (default) = queue order.  Images are identical either way. */
+vt_status vt_set_shade_sort(vt_engine *eng, uint32_t enabled);
+
 /* Traversal scheduling knobs (no counterpart in the reference): a wave is
  * refilled once `refill_min` (1..64) of its lanes are idle, lanes run up to
  * `steps` traversal steps between refill checks, and `near_first` orders the

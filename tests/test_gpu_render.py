@@ -243,3 +243,17 @@ def test_first_hit_aux_and_variance_match_oracle(engine):
     scale = (lum ** 2).mean(axis=0)
     ok = np.abs(var - want) <= 1e-5 * scale + 1e-9
     assert ok.mean() >= 0.999, ok.mean()
+
+
+def test_material_sorted_shading_gives_identical_images(engine):
+    """vt_set_shade_sort: shading each workgroup's queue entries in
+    material-sorted order changes which lanes shade which path, not any
+    path's result: images and path signatures are bit-identical."""
+    for sc in (W.cornell_scene(), W.spheres_scene(instances=48, lights=4)):
+        a, sa = gpu_frames(engine, sc, 48, 2)
+        engine.set_shade_sort(True)
+        try:
+            b, sb = gpu_frames(engine, sc, 48, 2)
+        finally:
+            engine.set_shade_sort(False)
+        assert np.array_equal(a, b) and np.array_equal(sa, sb)

@@ -600,6 +600,15 @@ vt_status vt_trace_any_device(vt_engine* eng, const vt_ray* d_rays, uint32_t n, 
   });
 }
 
+vt_status vt_set_shade_sort(vt_engine* eng, uint32_t enabled) {
+  return vt_guarded(eng, [&]() -> vt_status {
+    if (!eng) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "engine is NULL");
+    if (enabled > 1) return set_err(eng, VT_ERR_RANGE, "enabled must be 0 or 1");
+    eng->shade_sort = enabled;
+    return VT_OK;
+  });
+}
+
 vt_status vt_set_trace_tuning(vt_engine* eng, uint32_t refill_min, uint32_t steps, uint32_t near_first,
                               uint32_t defer_min) {
   return vt_guarded(eng, [&]() -> vt_status {

@@ -82,6 +82,7 @@ struct vt_engine {
   uint32_t refill_min = 8;
   uint32_t trace_steps = 16;
   uint32_t near_first = 2;
+  uint32_t shade_sort = 0;  // material-sorted shading order (vt_set_shade_sort)
   uint32_t defer_min = 12;
   uint32_t node_width = 4;  // 4: Node128 trees where the stack bound allows, 2: Node64
   uint32_t hit_filter = 0;  // vt_trace_*: 0 raw traversal, 1 the path tracer's sided wrappers

@@ -580,18 +580,28 @@ __global__ __launch_bounds__(256, 3) void train_kernel(const float* __restrict__
 // samples) sums its chunk into a private copy of the level's 4096 x 2
 // gradient in LDS and stores the copy whole into its own partial slice
 // (part[(chunk * kLevels + level) * 8192 ..]); table_reduce_kernel then sums
-// the chunks' slices.  The coarse levels have only a few hundred distinct
-// rows: scattering every sample's contribution straight to memory, or even
-// every chunk's copy with atomics, serialises on them.
+// the chunks' slices.  The LDS copy holds fixed-point integers (2^46 per
+// unit): LDS integer atomics run about four times faster than float ones
+// here, and integer sums do not depend on the order of the additions.  A
+// contribution is clamped to +-4 (far beyond any gradient of a training
+// step; 16,384 contributions per element per chunk then stay below 2^63); a
+// non-finite one makes the chunk's whole slice NaN, as a float sum would
+// carry it, so Adam skips that step for the level.
+constexpr float kGradFix = 0x1p46f;
+constexpr float kGradFixClamp = 4.0f;
+
 __global__ __launch_bounds__(256) void table_grad_kernel(const float* __restrict__ feats, const float* __restrict__ dx,
                                                          uint32_t cap, const uint32_t* __restrict__ n_ptr,
                                                          uint32_t per_chunk, float* __restrict__ part) {
-  __shared__ float acc[kRows * kLevelFeat];
+  __shared__ unsigned long long acc[kRows * kLevelFeat];  // two's complement fixed point
+  __shared__ uint32_t nonfinite;
   const uint32_t l = blockIdx.x, tid = threadIdx.x;
-  for (uint32_t i = tid; i < kRows * kLevelFeat; i += 256u) acc[i] = 0.0f;
+  for (uint32_t i = tid; i < kRows * kLevelFeat; i += 256u) acc[i] = 0ull;
+  if (tid == 0) nonfinite = 0u;
   __syncthreads();
   const uint32_t n = n_ptr ? min(*n_ptr, cap) : cap;
   const uint32_t lo = blockIdx.y * per_chunk, hi = min(n, lo + per_chunk);
+  bool bad = false;
 #pragma unroll 2
   for (uint32_t s = lo + tid; s < hi; s += 256u) {
     const float* fp = feats + (size_t)s * kFeat;
@@ -601,14 +611,20 @@ __global__ __launch_bounds__(256) void table_grad_kernel(const float* __restrict
 #pragma unroll
     for (uint32_t k = 0; k < 8; ++k) {
       const uint32_t row = c.row[k] - l * kRows;
-      atomicAdd(&acc[2u * row], c.w[k] * d0);
-      atomicAdd(&acc[2u * row + 1u], c.w[k] * d1);
+      const float v0 = c.w[k] * d0, v1 = c.w[k] * d1;
+      bad = bad || !(fabsf(v0) <= 3.402823e38f) || !(fabsf(v1) <= 3.402823e38f);
+      const float q0 = fminf(fmaxf(v0, -kGradFixClamp), kGradFixClamp) * kGradFix;
+      const float q1 = fminf(fmaxf(v1, -kGradFixClamp), kGradFixClamp) * kGradFix;
+      atomicAdd(&acc[2u * row], (unsigned long long)__float2ll_rn(q0));
+      atomicAdd(&acc[2u * row + 1u], (unsigned long long)__float2ll_rn(q1));
     }
   }
+  if (bad) nonfinite = 1u;
   __syncthreads();
-  float4* __restrict__ out = reinterpret_cast<float4*>(part + ((size_t)blockIdx.y * kLevels + l) * (kRows * kLevelFeat));
-  const float4* a4 = reinterpret_cast<const float4*>(acc);
-  for (uint32_t i = tid; i < kRows * kLevelFeat / 4u; i += 256u) out[i] = a4[i];
+  float* __restrict__ out = part + ((size_t)blockIdx.y * kLevels + l) * (kRows * kLevelFeat);
+  const bool poisoned = nonfinite != 0u;
+  for (uint32_t i = tid; i < kRows * kLevelFeat; i += 256u)
+    out[i] = poisoned ? __builtin_nanf("") : (float)((double)(long long)acc[i] * (1.0 / 70368744177664.0));
 }
 
 // grad[kMlpParams + e] += the sum over the chunks of element e of the
@@ -719,12 +735,11 @@ hipError_t launch_train(const float* feats, const float* targets, uint32_t n, co
   hipLaunchKernelGGL(reduce_slots_kernel, dim3((kMlpParams + 255u) / 256u, 32u), dim3(256), 0, s, part, blocks, n,
                      n_ptr, grad);
   if (hipError_t e = hipGetLastError()) return e;
-  // About 2048 samples per workgroup, at most 128 chunks per level (enough
-  // workgroups to keep 16 waves on every CU); the chunks' partial tables go
-  // to `part` (free again once reduce_slots_kernel has read the MLP slots:
-  // 128 x 65,536 floats <= kGradSlots x kMlpParams).
-  static_assert((size_t)128 * kTableParams <= (size_t)kGradSlots * kMlpParams, "table partials fit the slots");
-  const uint32_t chunks = std::min(128u, (n + 2047u) / 2048u);
+  // About 4096 samples per workgroup, at most 64 chunks per level; the
+  // chunks' partial tables go to `part` (free again once reduce_slots_kernel
+  // has read the MLP slots: 64 x 65,536 floats <= kGradSlots x kMlpParams).
+  static_assert((size_t)64 * kTableParams <= (size_t)kGradSlots * kMlpParams, "table partials fit the slots");
+  const uint32_t chunks = std::min(64u, (n + 4095u) / 4096u);
   const uint32_t per_chunk = (n + chunks - 1u) / chunks;
   hipLaunchKernelGGL(table_grad_kernel, dim3(kLevels, chunks), dim3(256), 0, s, feats, dx, n, n_ptr, per_chunk,
                      part);

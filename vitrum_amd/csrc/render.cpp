@@ -615,6 +615,7 @@ vt_status vt_render_frame(vt_engine* e, const vt_frame* fr) {
     f.nrc = R.nrc_enabled ? 1u : 0u;
     f.train_paths = R.nrc_enabled ? R.nrc_train_paths : 0u;
     f.train_vertices = R.nrc_train_vertices;
+    f.shade_sort = e->shade_sort;
     for (int k = 0; k < 3; ++k) f.scene_min[k] = R.scene_min[k], f.scene_inv_extent[k] = R.scene_inv_extent[k];
     vtr::ShadeScene sc{};
     sc.positions = (const float4*)e->scene.positions.ptr;

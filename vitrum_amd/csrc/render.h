@@ -69,6 +69,8 @@ struct FrameParams {
   uint32_t sobol;
   uint32_t sobol_key;
   const uint16_t* sobol_dirs;
+  // Shade each workgroup's queue entries in material-sorted order.
+  uint32_t shade_sort;
 };
 
 // Structure-of-arrays path state, one path per pixel.

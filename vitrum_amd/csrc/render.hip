@@ -769,12 +769,46 @@ __global__ void camera_kernel(FrameParams f, PathState s) {
 // One vertex of every queued path: MIS connection of the previous BSDF sample,
 // emission, NEE (enqueues a shadow ray), BSDF sample (enqueues the next
 // extension ray) and Russian roulette.
+// Material sort (f.shade_sort).  Each workgroup shades its 256 queue entries
+// in the order of a counting sort by a small key -- misses first, then hits
+// grouped by material (ids past 13 share the last group), entries past the
+// queue's end last -- so a wave's lanes mostly take the same branches and
+// read the same material records.  The ranks come from per-wave ballots and
+// a fixed wave order, so the order is deterministic; each path's result does
+// not depend on it.  Off by default: with one uber-BSDF the branches barely
+// diverge, and on the 1080p Cornell frames the sort cost 2-3% of the frame.
+constexpr uint32_t kSortBins = 16;
+
 __global__ __launch_bounds__(256) void shade_kernel(FrameParams f, ShadeScene sc, PathState s,
                                                      uint32_t vertex) {
+  __shared__ uint32_t s_count[4][kSortBins];
+  __shared__ uint16_t s_order[256];
   const uint32_t n = s.count[0];
   const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t li = threadIdx.x, wave = li >> 6, lane = li & 63u;
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64u - lane));
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
-    const uint32_t i = base + threadIdx.x;
+    if (f.shade_sort) {
+      uint32_t bin = kSortBins - 1u;
+      if (base + li < n) {
+        const uint32_t* h = s.hits + 5 * (size_t)(base + li);
+        bin = h[1] == kMiss ? 0u : 1u + min(sc.tri_material[h[0]], kSortBins - 3u);
+      }
+      uint32_t rank = 0;
+#pragma unroll
+      for (uint32_t b = 0; b < kSortBins; ++b) {
+        const uint64_t m = __ballot(bin == b);
+        if (bin == b) rank = (uint32_t)__popcll(m & below);
+        if (lane == 0) s_count[wave][b] = (uint32_t)__popcll(m);
+      }
+      __syncthreads();
+      uint32_t off = rank;
+      for (uint32_t b = 0; b < bin; ++b) off += s_count[0][b] + s_count[1][b] + s_count[2][b] + s_count[3][b];
+      for (uint32_t w = 0; w < wave; ++w) off += s_count[w][bin];
+      s_order[off] = (uint16_t)li;
+      __syncthreads();
+    }
+    const uint32_t i = base + (f.shade_sort ? (uint32_t)s_order[li] : li);
     bool want_shadow = false, want_ext = false;
     float4 shadow_a, shadow_b, ext_a, ext_b;
     F3 shadow_c = f3(0, 0, 0);

@@ -392,7 +392,10 @@ class Engine:
         """0: binary BVH; 1: cwbvh-closest (closest-hit rays only)."""
         check(lib().vt_set_bvh_traversal(self._h, C.c_uint32(mode)), self._h)
 
-    def set_trace_tuning(self, refill_min, steps, near_first=1, defer_min=12):
+    def set_shade_sort(self, enabled):
+        check(lib().vt_set_shade_sort(self._h, C.c_uint32(1 if enabled else 0)), self._h)
+
+    def set_trace_tuning(self, refill_min, steps, near_first=2, defer_min=12):
         check(lib().vt_set_trace_tuning(self._h, C.c_uint32(refill_min), C.c_uint32(steps),
                                         C.c_uint32(near_first), C.c_uint32(defer_min)), self._h)
 
