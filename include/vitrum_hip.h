@@ -262,6 +262,30 @@ typedef struct vt_frame {
  * into the running mean.  Enqueued on the engine stream; returns without
  * waiting.  A changed viewport restarts accumulation. */
 vt_status vt_render_frame(vt_engine *eng, const vt_frame *frame);
+
+/* Batched frames (no counterpart in the reference, whose engines render one
+ * scene each): several engines, each with its own
+ * scene, camera and viewport, render one frame each with every bounce's
+ * closest-hit rays of all of them traced in ONE persistent launch over the
+ * scenes' table (the shadow rays in another), all on the host engine's
+ * stream.  A member's image is bit-identical to rendering the same frame on
+ * the member alone.  vt_render_batch_capture records a frame into a HIP
+ * graph; vt_render_batch_replay replays it for another frame index (each
+ * member keeps its captured camera, viewport and seed; the frame's seed,
+ * index and Sobol key are read from device memory), so a frame costs one
+ * graph launch.  Members: engines on the host's device with a scene set,
+ * the binary traversal and no radiance cache.  A member's own stream is
+ * ordered after each batched frame. */
+typedef struct vt_render_batch vt_render_batch;
+vt_status vt_render_batch_create(vt_engine *host, vt_render_batch **out);
+vt_status vt_render_batch_destroy(vt_render_batch *batch);
+vt_status vt_render_batch_add(vt_render_batch *batch, vt_engine *member);
+/* frames[k] is member k's frame (vt_render_frame's arguments). */
+vt_status vt_render_batch_frame(vt_render_batch *batch, const vt_frame *frames);
+vt_status vt_render_batch_capture(vt_render_batch *batch, const vt_frame *frames);
+/* VT_ERR_STATE when nothing was captured or a member's scene or viewport
+ * changed since. */
+vt_status vt_render_batch_replay(vt_render_batch *batch, uint32_t frame_index);
 /* captureFrame (packages/core/src/engine/index.ts:466; pt-webgpu
  * index.ts:2019): Float32 RGBA running mean, row-major, top-left origin,
  * alpha 1.  Blocking.  If a traversal ran out of stack since the last check,
@@ -342,8 +366,8 @@ vt_status vt_set_counting(vt_engine *eng, int32_t enabled);
 vt_status vt_trace_closest_counts(vt_engine *eng, const vt_ray *rays, uint32_t n, vt_hit *hits,
                                   uint32_t *counts);
 
-/* Shading order (no counterpart in the reference; the north-star design's
- * material sort): 1 = each workgroup of the shading kernel counting-sorts its
+/* Shading order (no counterpart in the reference; a material sort ahead of
+ * shading): 1 = each workgroup of the shading kernel counting-sorts its
  * 256 queue entries by material in LDS (misses first) before shading them;
  * 0 (default) = queue order.  Images are identical either way. */
 vt_status vt_set_shade_sort(vt_engine *eng, uint32_t enabled);

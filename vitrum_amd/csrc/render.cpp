@@ -582,85 +582,111 @@ static vt_status ensure_paths(vt_engine* e, uint32_t w, uint32_t h) {
 
 static vtk::TraceParams trace_params(vt_engine* e) { return engine_trace_params(e); }
 
+// A frame's launch parameters (validated), shared by vt_render_frame and the
+// batched frames of vt_render_batch_frame.
+struct FrameLaunch {
+  vtr::FrameParams f{};
+  vtr::ShadeScene sc{};
+  vtr::PathState ps{};
+  bool skip = false;   // the sample target is reached: render nothing
+};
+
+static vt_status frame_setup(vt_engine* e, const vt_frame* fr, FrameLaunch& L) {
+  if (!e->has_scene || !e->render) return rerr(e, VT_ERR_STATE, "no scene is set");
+  if (!fr) return rerr(e, VT_ERR_INVALID_ARGUMENT, "frame is NULL");
+  if (fr->width == 0 || fr->height == 0 || (uint64_t)fr->width * fr->height > (1u << 28))
+    return rerr(e, VT_ERR_RANGE, "viewport must be non-empty and at most 2^28 pixels");
+  for (int i = 0; i < 16; ++i)
+    if (!std::isfinite(fr->inv_view_proj[i])) return rerr(e, VT_ERR_RANGE, "inverse view-projection must be finite");
+  HIP_CHECK(e, hipSetDevice(e->device));
+  if (vt_status s = ensure_paths(e, fr->width, fr->height)) return s;
+  L.skip = (e->opts.max_samples_per_pixel && e->samples_accumulated >= e->opts.max_samples_per_pixel) ||
+           (fr->samples_target && e->samples_accumulated >= fr->samples_target);
+  if (L.skip) return VT_OK;
+  RenderState& R = *e->render;
+  vtr::FrameParams& f = L.f;
+  f = vtr::FrameParams{};
+  std::memcpy(f.inv_vp, fr->inv_view_proj, 64);
+  f.width = fr->width;
+  f.height = fr->height;
+  f.frame_seed = fr->frame_seed;
+  f.frame_index = fr->frame_index;
+  const uint32_t mb = fr->max_bounces ? fr->max_bounces : e->opts.max_bounces;
+  f.bounce_limit = std::max(1u, std::min(mb, 8u));
+  f.ray_bias = R.ray_bias;
+  f.tmin = R.tmin;
+  f.pixels = fr->width * fr->height;
+  f.sobol = e->opts.sampling == 1 ? 1u : 0u;
+  f.sobol_key = vts::frame_key(fr->frame_seed, fr->frame_index);
+  f.sobol_dirs = (const uint16_t*)R.sobol_dirs.ptr;
+  f.nrc = R.nrc_enabled ? 1u : 0u;
+  f.train_paths = R.nrc_enabled ? R.nrc_train_paths : 0u;
+  f.train_vertices = R.nrc_train_vertices;
+  f.shade_sort = e->shade_sort;
+  for (int k = 0; k < 3; ++k) f.scene_min[k] = R.scene_min[k], f.scene_inv_extent[k] = R.scene_inv_extent[k];
+  vtr::ShadeScene& sc = L.sc;
+  sc = vtr::ShadeScene{};
+  sc.positions = (const float4*)e->scene.positions.ptr;
+  sc.normals = (const float4*)e->scene.normals.ptr;
+  sc.indices = (const uint4*)e->scene.indices.ptr;
+  sc.tri_material = (const uint32_t*)e->scene.tri_material.ptr;
+  sc.tri_source = (const uint32_t*)R.tri_source.ptr;
+  sc.materials = (const float*)R.materials.ptr;
+  sc.emitters = (const float*)R.emitters.ptr;
+  sc.instances = (const float*)R.instances.ptr;
+  sc.env_cdf = (const float*)R.env_cdf.ptr;
+  sc.env_texels = (const float4*)R.env_texels.ptr;
+  sc.emitter_count = R.emitter_count;
+  sc.env_present = R.env_present;
+  sc.env_w = R.env_w;
+  sc.env_h = R.env_h;
+  sc.env_rot_c = R.env_rot[0];
+  sc.env_rot_s = R.env_rot[1];
+  sc.light_tree = (const float*)R.light_tree.ptr;
+  sc.light_tree_nodes = R.light_tree_nodes;
+  sc.lights = (const float*)R.lights.ptr;
+  sc.n_dir = R.n_dir;
+  sc.n_point = R.n_point;
+  sc.n_spot = R.n_spot;
+  sc.n_rect = R.n_rect;
+  vtr::PathState& ps = L.ps;
+  ps = vtr::PathState{};
+  ps.rng = (uint32_t*)R.rng.ptr;
+  ps.rng_dim = (uint32_t*)R.rng_dim.ptr;
+  ps.throughput = (float*)R.throughput.ptr;
+  ps.conn = (float*)R.conn.ptr;
+  ps.radiance = (float*)R.radiance.ptr;
+  ps.prev_pdf = (float*)R.prev_pdf.ptr;
+  ps.flags = (uint32_t*)R.flags.ptr;
+  ps.signature = (uint64_t*)R.signature.ptr;
+  ps.count = (uint32_t*)R.count.ptr;
+  ps.hits = (uint32_t*)R.hits.ptr;
+  ps.hit_pos = (float*)R.hit_pos.ptr;
+  ps.shadow_path = (uint32_t*)R.shadow_path.ptr;
+  ps.shadow_rays = (float4*)R.shadow_rays.ptr;
+  ps.shadow_contrib = (float*)R.shadow_contrib.ptr;
+  ps.shadow_occ = (uint32_t*)R.shadow_occ.ptr;
+  ps.accum = (float*)R.accum.ptr;
+  ps.moments = (float*)R.moments.ptr;
+  ps.aux_nd = (float*)R.aux_nd.ptr;
+  ps.aux_albedo = (float*)R.aux_albedo.ptr;
+  return VT_OK;
+}
+
+
 extern "C" {
 
 vt_status vt_render_frame(vt_engine* e, const vt_frame* fr) {
   return vt_guarded(e, [&]() -> vt_status {
     if (!e) return VT_ERR_INVALID_ARGUMENT;
     if (!fr) return rerr(e, VT_ERR_INVALID_ARGUMENT, "frame is NULL");
-    if (!e->has_scene || !e->render) return rerr(e, VT_ERR_STATE, "no scene is set");
-    if (fr->width == 0 || fr->height == 0 || (uint64_t)fr->width * fr->height > (1u << 28))
-      return rerr(e, VT_ERR_RANGE, "viewport must be non-empty and at most 2^28 pixels");
-    for (int i = 0; i < 16; ++i)
-      if (!std::isfinite(fr->inv_view_proj[i])) return rerr(e, VT_ERR_RANGE, "inverse view-projection must be finite");
-    HIP_CHECK(e, hipSetDevice(e->device));
-    if (vt_status s = ensure_paths(e, fr->width, fr->height)) return s;
-    if (e->opts.max_samples_per_pixel && e->samples_accumulated >= e->opts.max_samples_per_pixel) return VT_OK;
-    if (fr->samples_target && e->samples_accumulated >= fr->samples_target) return VT_OK;
+    FrameLaunch L;
+    if (vt_status s = frame_setup(e, fr, L)) return s;
+    if (L.skip) return VT_OK;
     RenderState& R = *e->render;
-    vtr::FrameParams f{};
-    std::memcpy(f.inv_vp, fr->inv_view_proj, 64);
-    f.width = fr->width;
-    f.height = fr->height;
-    f.frame_seed = fr->frame_seed;
-    f.frame_index = fr->frame_index;
-    const uint32_t mb = fr->max_bounces ? fr->max_bounces : e->opts.max_bounces;
-    f.bounce_limit = std::max(1u, std::min(mb, 8u));
-    f.ray_bias = R.ray_bias;
-    f.tmin = R.tmin;
-    f.pixels = fr->width * fr->height;
-    f.sobol = e->opts.sampling == 1 ? 1u : 0u;
-    f.sobol_key = vts::frame_key(fr->frame_seed, fr->frame_index);
-    f.sobol_dirs = (const uint16_t*)R.sobol_dirs.ptr;
-    f.nrc = R.nrc_enabled ? 1u : 0u;
-    f.train_paths = R.nrc_enabled ? R.nrc_train_paths : 0u;
-    f.train_vertices = R.nrc_train_vertices;
-    f.shade_sort = e->shade_sort;
-    for (int k = 0; k < 3; ++k) f.scene_min[k] = R.scene_min[k], f.scene_inv_extent[k] = R.scene_inv_extent[k];
-    vtr::ShadeScene sc{};
-    sc.positions = (const float4*)e->scene.positions.ptr;
-    sc.normals = (const float4*)e->scene.normals.ptr;
-    sc.indices = (const uint4*)e->scene.indices.ptr;
-    sc.tri_material = (const uint32_t*)e->scene.tri_material.ptr;
-    sc.tri_source = (const uint32_t*)R.tri_source.ptr;
-    sc.materials = (const float*)R.materials.ptr;
-    sc.emitters = (const float*)R.emitters.ptr;
-    sc.instances = (const float*)R.instances.ptr;
-    sc.env_cdf = (const float*)R.env_cdf.ptr;
-    sc.env_texels = (const float4*)R.env_texels.ptr;
-    sc.emitter_count = R.emitter_count;
-    sc.env_present = R.env_present;
-    sc.env_w = R.env_w;
-    sc.env_h = R.env_h;
-    sc.env_rot_c = R.env_rot[0];
-    sc.env_rot_s = R.env_rot[1];
-    sc.light_tree = (const float*)R.light_tree.ptr;
-    sc.light_tree_nodes = R.light_tree_nodes;
-    sc.lights = (const float*)R.lights.ptr;
-    sc.n_dir = R.n_dir;
-    sc.n_point = R.n_point;
-    sc.n_spot = R.n_spot;
-    sc.n_rect = R.n_rect;
-    vtr::PathState ps{};
-    ps.rng = (uint32_t*)R.rng.ptr;
-    ps.rng_dim = (uint32_t*)R.rng_dim.ptr;
-    ps.throughput = (float*)R.throughput.ptr;
-    ps.conn = (float*)R.conn.ptr;
-    ps.radiance = (float*)R.radiance.ptr;
-    ps.prev_pdf = (float*)R.prev_pdf.ptr;
-    ps.flags = (uint32_t*)R.flags.ptr;
-    ps.signature = (uint64_t*)R.signature.ptr;
-    ps.count = (uint32_t*)R.count.ptr;
-    ps.hits = (uint32_t*)R.hits.ptr;
-    ps.hit_pos = (float*)R.hit_pos.ptr;
-    ps.shadow_path = (uint32_t*)R.shadow_path.ptr;
-    ps.shadow_rays = (float4*)R.shadow_rays.ptr;
-    ps.shadow_contrib = (float*)R.shadow_contrib.ptr;
-    ps.shadow_occ = (uint32_t*)R.shadow_occ.ptr;
-    ps.accum = (float*)R.accum.ptr;
-    ps.moments = (float*)R.moments.ptr;
-    ps.aux_nd = (float*)R.aux_nd.ptr;
-    ps.aux_albedo = (float*)R.aux_albedo.ptr;
+    vtr::FrameParams& f = L.f;
+    vtr::ShadeScene& sc = L.sc;
+    vtr::PathState& ps = L.ps;
     // Accumulation restarts (reset, new scene, new viewport, cache settings)
     // set the sample count to zero; the first frame after one clears the sums.
     if (e->samples_accumulated == 0) {
@@ -919,6 +945,341 @@ vt_status vt_render_info(vt_engine* e, uint32_t* emitter_count, float* ray_bias)
     if (emitter_count) *emitter_count = e->render->emitter_count;
     if (ray_bias) *ray_bias = e->render->ray_bias;
     return VT_OK;
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched frames: several engines (each with its own scene, camera and
+// viewport) render one frame each, with every bounce's closest-hit rays of
+// all of them traced in one persistent launch over the scenes' table (and
+// the shadow rays in another), on the batch's stream; a frame can be recorded
+// once into a HIP graph and replayed.
+// ---------------------------------------------------------------------------
+
+struct vt_render_batch {
+  vt_engine* host = nullptr;  // its stream, traversal scratch and tuning run the batch
+  std::vector<vt_engine*> members;
+  DeviceBuffer scenes, table, offsets, rays, hits, hit_pos, occ, words;
+  // Graph of one frame and what it was recorded against.
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<vt_frame> frames;
+  std::vector<uintptr_t> state;  // each member's viewport, path-state and scene buffers at recording
+  uint32_t* staged = nullptr;      // pinned frame words (seed, index, Sobol key per member)
+  hipEvent_t staged_done = nullptr;
+  hipEvent_t frame_done = nullptr;  // members' own streams wait for it
+};
+
+namespace {
+
+vt_status batch_err(vt_render_batch* b, vt_status s, const std::string& m) { return rerr(b->host, s, m); }
+
+void batch_drop_graph(vt_render_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->frames.clear();
+  b->state.clear();
+}
+
+// Everything a batched frame needs before its launches: each member's frame
+// parameters, the device tables, buffers sized for the members' queues.
+struct BatchLaunch {
+  std::vector<FrameLaunch> L;
+  std::vector<vtr::BatchMember> bm;
+  uint32_t capacity = 0, max_limit = 0;
+  bool wide = true;
+};
+
+vt_status batch_prepare(vt_render_batch* b, const vt_frame* frames, bool device_words, BatchLaunch& B) {
+  vt_engine* e = b->host;
+  const uint32_t K = (uint32_t)b->members.size();
+  if (K == 0) return batch_err(b, VT_ERR_STATE, "the batch has no members");
+  if (!frames) return batch_err(b, VT_ERR_INVALID_ARGUMENT, "frames is NULL");
+  B.L.assign(K, FrameLaunch{});
+  B.bm.assign(K, vtr::BatchMember{});
+  std::vector<vtk::DeviceScene> ds(K);
+  uint64_t cap = 0;
+  for (uint32_t k = 0; k < K; ++k) {
+    vt_engine* m = b->members[k];
+    if (vt_status s = frame_setup(m, &frames[k], B.L[k])) return batch_err(b, s, "member " + std::to_string(k) + ": " + m->last_error);
+    if (B.L[k].skip) return batch_err(b, VT_ERR_STATE, "member " + std::to_string(k) + " has reached its sample target");
+    RenderState& R = *m->render;
+    if (R.nrc_enabled) return batch_err(b, VT_ERR_STATE, "batched frames do not run the radiance cache");
+    vtr::BatchMember& q = B.bm[k];
+    q.rays[0] = (const float4*)R.rays[0].ptr;
+    q.rays[1] = (const float4*)R.rays[1].ptr;
+    q.hits = (uint32_t*)R.hits.ptr;
+    q.hit_pos = (float*)R.hit_pos.ptr;
+    q.shadow_rays = (const float4*)R.shadow_rays.ptr;
+    q.shadow_occ = (uint32_t*)R.shadow_occ.ptr;
+    q.count = (const uint32_t*)R.count.ptr;
+    bool w = false;
+    ds[k] = engine_device_scene(m->scene, *m->layout, &w);
+    B.wide = B.wide && w;
+    cap += R.paths;
+    B.max_limit = std::max(B.max_limit, B.L[k].f.bounce_limit);
+    if (device_words) B.L[k].f.frame_dev = (const uint32_t*)b->words.ptr + 3u * k;
+  }
+  if (cap > 0x7FFFFFFFull) return batch_err(b, VT_ERR_RANGE, "the members' queues exceed 2^31 - 1 rays");
+  B.capacity = (uint32_t)cap;
+  vt_status s = VT_OK;
+  s = s ? s : b->rays.ensure(e, cap * 32);
+  s = s ? s : b->hits.ensure(e, cap * 20);
+  s = s ? s : b->hit_pos.ensure(e, cap * 12);
+  s = s ? s : b->occ.ensure(e, cap * 4);
+  s = s ? s : b->offsets.ensure(e, (K + 1) * 4);
+  s = s ? s : b->words.ensure(e, (size_t)K * 12);
+  s = s ? s : engine_upload(e, b->scenes, ds.data(), ds.size() * sizeof(vtk::DeviceScene));
+  s = s ? s : engine_upload(e, b->table, B.bm.data(), B.bm.size() * sizeof(vtr::BatchMember));
+  if (s) return s;
+  HIP_CHECK(e, hipStreamSynchronize(e->stream));  // the uploads read host vectors of this call
+  if (device_words)  // the addresses above may have moved
+    for (uint32_t k = 0; k < K; ++k) B.L[k].f.frame_dev = (const uint32_t*)b->words.ptr + 3u * k;
+  return VT_OK;
+}
+
+// The launches of one batched frame on the host engine's stream (recordable:
+// kernels only, no allocation, no memset, no host synchronisation).
+vt_status batch_enqueue(vt_render_batch* b, BatchLaunch& B) {
+  vt_engine* e = b->host;
+  hipStream_t st = e->stream;
+  const uint32_t K = (uint32_t)b->members.size();
+  const vtr::BatchMember* table = (const vtr::BatchMember*)b->table.ptr;
+  uint32_t* offsets = (uint32_t*)b->offsets.ptr;
+  for (uint32_t k = 0; k < K; ++k) {
+    RenderState& R = *b->members[k]->render;
+    vtr::PathState& ps = B.L[k].ps;
+    HIP_CHECK(e, vtk::launch_zero((uint32_t*)R.count.ptr, 16, st));
+    ps.queue = (uint32_t*)R.queue[0].ptr;
+    ps.rays = (float4*)R.rays[0].ptr;
+    HIP_CHECK(e, vtr::launch_camera(B.L[k].f, ps, st));
+  }
+  // The host engine's traversal scratch and tuning (its own scene, if any,
+  // plays no part: every ray takes its scene from the table).
+  vtk::TraceParams tp{};
+  tp.counter = (uint32_t*)e->counter.ptr;
+  tp.error = (uint32_t*)e->error_word.ptr;
+  tp.spill = (uint2*)e->spill.ptr;
+  tp.refill_min = e->refill_min;
+  tp.steps = e->trace_steps;
+  tp.near_first = e->near_first;
+  tp.defer_min = e->defer_min;
+  tp.scenes = (const vtk::DeviceScene*)b->scenes.ptr;
+  tp.scene_offsets = offsets;
+  tp.scene_count = K;
+  tp.wide = (B.wide && e->node_width == 4) ? 1u : 0u;
+  tp.n_ptr = offsets + K;
+  tp.sided = 1;  // traceClosest / traceSurfaceVisibility
+  int cur = 0;
+  for (uint32_t v = 0; v <= B.max_limit; ++v) {
+    // extension rays of every member: gather, one traversal launch, scatter
+    HIP_CHECK(e, vtr::launch_batch_prefix(table, K, 0u, offsets, st));
+    HIP_CHECK(e, vtr::launch_batch_gather(table, K, offsets, 0u, (uint32_t)cur, (float4*)b->rays.ptr, B.capacity, st));
+    vtk::TraceParams cp = tp;
+    cp.rays = (const float4*)b->rays.ptr;
+    cp.hits = (uint32_t*)b->hits.ptr;
+    cp.hit_pos = (float*)b->hit_pos.ptr;
+    HIP_CHECK(e, vtk::launch_trace_batch(cp, e->trace_grid, st));
+    HIP_CHECK(e, vtr::launch_batch_scatter(table, K, offsets, 0u, (const uint32_t*)b->hits.ptr,
+                                           (const float*)b->hit_pos.ptr, B.capacity, st));
+    for (uint32_t k = 0; k < K; ++k) {
+      if (v > B.L[k].f.bounce_limit) continue;
+      RenderState& R = *b->members[k]->render;
+      vtr::PathState& ps = B.L[k].ps;
+      ps.queue = (uint32_t*)R.queue[cur].ptr;
+      ps.rays = (float4*)R.rays[cur].ptr;
+      ps.next_queue = (uint32_t*)R.queue[cur ^ 1].ptr;
+      ps.next_rays = (float4*)R.rays[cur ^ 1].ptr;
+      HIP_CHECK(e, vtr::launch_shade(B.L[k].f, B.L[k].sc, ps, v, R.paths, st));
+    }
+    if (v < B.max_limit) {
+      // shadow rays of every member
+      HIP_CHECK(e, vtr::launch_batch_prefix(table, K, 2u, offsets, st));
+      HIP_CHECK(e, vtr::launch_batch_gather(table, K, offsets, 1u, 0u, (float4*)b->rays.ptr, B.capacity, st));
+      vtk::TraceParams sp = tp;
+      sp.rays = (const float4*)b->rays.ptr;
+      sp.occluded = (uint32_t*)b->occ.ptr;
+      HIP_CHECK(e, vtk::launch_trace_batch_any(sp, e->trace_grid, st));
+      HIP_CHECK(e, vtr::launch_batch_scatter(table, K, offsets, 1u, (const uint32_t*)b->occ.ptr, nullptr,
+                                             B.capacity, st));
+      for (uint32_t k = 0; k < K; ++k) {
+        if (v >= B.L[k].f.bounce_limit) continue;
+        const RenderState& R = *b->members[k]->render;
+        HIP_CHECK(e, vtr::launch_shadow_resolve(B.L[k].ps, R.paths, st));
+        HIP_CHECK(e, vtr::launch_reset_counts(B.L[k].ps, 2u, st));
+      }
+    }
+    for (uint32_t k = 0; k < K; ++k)
+      if (v <= B.L[k].f.bounce_limit) HIP_CHECK(e, vtr::launch_reset_counts(B.L[k].ps, 1u, st));
+    cur ^= 1;
+  }
+  for (uint32_t k = 0; k < K; ++k) HIP_CHECK(e, vtr::launch_accumulate(B.L[k].f, B.L[k].ps, st));
+  return VT_OK;
+}
+
+// Clears the accumulation of members whose sample count restarted, after
+// their own streams' work.
+vt_status batch_begin(vt_render_batch* b, const BatchLaunch& B) {
+  vt_engine* e = b->host;
+  for (size_t k = 0; k < b->members.size(); ++k) {
+    vt_engine* m = b->members[k];
+    if (m != e) HIP_CHECK(e, hipStreamSynchronize(m->stream));
+    if (m->render->side) HIP_CHECK(e, hipStreamSynchronize(m->render->side));
+    if (m->samples_accumulated == 0) {
+      HIP_CHECK(e, vtr::launch_zero_f32(B.L[k].ps.accum, B.L[k].f.pixels * 4, e->stream));
+      HIP_CHECK(e, vtr::launch_zero_f32(B.L[k].ps.moments, B.L[k].f.pixels * 2, e->stream));
+    }
+  }
+  return VT_OK;
+}
+
+// Orders each member's own stream after the batch's frame (its captures and
+// later frames then see the result) and counts the sample.
+vt_status batch_end(vt_render_batch* b) {
+  vt_engine* e = b->host;
+  HIP_CHECK(e, hipEventRecord(b->frame_done, e->stream));
+  for (vt_engine* m : b->members) {
+    if (m != e) HIP_CHECK(e, hipStreamWaitEvent(m->stream, b->frame_done, 0));
+    m->samples_accumulated += 1;
+    m->frame_timed = false;
+  }
+  return VT_OK;
+}
+
+// What a recorded graph depends on: each member's viewport, path-state and
+// scene buffers.
+std::vector<uintptr_t> batch_state(vt_render_batch* b) {
+  std::vector<uintptr_t> v;
+  for (vt_engine* m : b->members) {
+    const RenderState& R = *m->render;
+    v.push_back(((uintptr_t)R.width << 32) | R.height);
+    for (const void* p : {R.rays[0].ptr, R.rays[1].ptr, R.hits.ptr, R.accum.ptr, R.materials.ptr, R.lights.ptr,
+                          R.env_texels.ptr, m->scene.tris.ptr, m->scene.blas4.ptr})
+      v.push_back((uintptr_t)p);
+  }
+  return v;
+}
+
+}  // namespace
+
+extern "C" {
+
+vt_status vt_render_batch_create(vt_engine* host, vt_render_batch** out) {
+  return vt_guarded(host, [&]() -> vt_status {
+    if (!host || !out) return VT_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    std::unique_ptr<vt_render_batch> b(new vt_render_batch());
+    b->host = host;
+    HIP_CHECK(host, hipSetDevice(host->device));
+    HIP_CHECK(host, hipEventCreateWithFlags(&b->staged_done, hipEventDisableTiming));
+    HIP_CHECK(host, hipEventCreateWithFlags(&b->frame_done, hipEventDisableTiming));
+    *out = b.release();
+    return VT_OK;
+  });
+}
+
+vt_status vt_render_batch_destroy(vt_render_batch* b) {
+  if (!b) return VT_OK;
+  return vt_guarded(b->host, [&]() -> vt_status {
+    (void)hipStreamSynchronize(b->host->stream);
+    batch_drop_graph(b);
+    for (DeviceBuffer* d : {&b->scenes, &b->table, &b->offsets, &b->rays, &b->hits, &b->hit_pos, &b->occ, &b->words})
+      d->release();
+    if (b->staged) (void)hipHostFree(b->staged);
+    if (b->staged_done) (void)hipEventDestroy(b->staged_done);
+    if (b->frame_done) (void)hipEventDestroy(b->frame_done);
+    delete b;
+    return VT_OK;
+  });
+}
+
+vt_status vt_render_batch_add(vt_render_batch* b, vt_engine* member) {
+  if (!b) return VT_ERR_INVALID_ARGUMENT;
+  return vt_guarded(b->host, [&]() -> vt_status {
+    if (!member) return batch_err(b, VT_ERR_INVALID_ARGUMENT, "member is NULL");
+    if (member->device != b->host->device) return batch_err(b, VT_ERR_INVALID_ARGUMENT, "the member runs on another device");
+    if (!member->has_scene || !member->render) return batch_err(b, VT_ERR_STATE, "the member has no scene");
+    if (member->opts.bvh_traversal != 0) return batch_err(b, VT_ERR_STATE, "batched frames walk the binary BVH");
+    if (b->members.size() >= 1024) return batch_err(b, VT_ERR_RANGE, "at most 1024 members");
+    for (vt_engine* m : b->members)
+      if (m == member) return batch_err(b, VT_ERR_INVALID_ARGUMENT, "the engine is already a member");
+    batch_drop_graph(b);
+    b->members.push_back(member);
+    return VT_OK;
+  });
+}
+
+vt_status vt_render_batch_frame(vt_render_batch* b, const vt_frame* frames) {
+  if (!b) return VT_ERR_INVALID_ARGUMENT;
+  return vt_guarded(b->host, [&]() -> vt_status {
+    HIP_CHECK(b->host, hipSetDevice(b->host->device));
+    BatchLaunch B;
+    if (vt_status s = batch_prepare(b, frames, false, B)) return s;
+    if (vt_status s = batch_begin(b, B)) return s;
+    if (vt_status s = batch_enqueue(b, B)) return s;
+    return batch_end(b);
+  });
+}
+
+vt_status vt_render_batch_capture(vt_render_batch* b, const vt_frame* frames) {
+  if (!b) return VT_ERR_INVALID_ARGUMENT;
+  return vt_guarded(b->host, [&]() -> vt_status {
+    vt_engine* e = b->host;
+    HIP_CHECK(e, hipSetDevice(e->device));
+    batch_drop_graph(b);
+    BatchLaunch B;
+    if (vt_status s = batch_prepare(b, frames, true, B)) return s;
+    HIP_CHECK(e, hipStreamSynchronize(e->stream));  // the table uploads
+    HIP_CHECK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    vt_status s = batch_enqueue(b, B);
+    hipGraph_t g = nullptr;
+    const hipError_t r = hipStreamEndCapture(e->stream, &g);
+    if (s) {
+      if (g) (void)hipGraphDestroy(g);
+      return s;
+    }
+    if (r != hipSuccess || !g) return batch_err(b, VT_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(r));
+    b->graph = g;
+    HIP_CHECK(e, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->frames.assign(frames, frames + b->members.size());
+    b->state = batch_state(b);
+    if (!b->staged) HIP_CHECK(e, hipHostMalloc((void**)&b->staged, 3 * 1024 * sizeof(uint32_t), hipHostMallocDefault));
+    return VT_OK;
+  });
+}
+
+vt_status vt_render_batch_replay(vt_render_batch* b, uint32_t frame_index) {
+  if (!b) return VT_ERR_INVALID_ARGUMENT;
+  return vt_guarded(b->host, [&]() -> vt_status {
+    vt_engine* e = b->host;
+    if (!b->exec) return batch_err(b, VT_ERR_STATE, "no batched frame has been captured");
+    if (batch_state(b) != b->state) return batch_err(b, VT_ERR_STATE, "a member's scene or viewport changed; capture again");
+    HIP_CHECK(e, hipSetDevice(e->device));
+    const uint32_t K = (uint32_t)b->members.size();
+    BatchLaunch B;
+    B.L.resize(K);
+    for (uint32_t k = 0; k < K; ++k) {
+      vt_engine* m = b->members[k];
+      B.L[k].ps.accum = (float*)m->render->accum.ptr;
+      B.L[k].ps.moments = (float*)m->render->moments.ptr;
+      B.L[k].f.pixels = b->frames[k].width * b->frames[k].height;
+    }
+    if (vt_status s = batch_begin(b, B)) return s;
+    // this frame's words, staged in pinned memory once the previous copy is done
+    HIP_CHECK(e, hipEventSynchronize(b->staged_done));
+    for (uint32_t k = 0; k < K; ++k) {
+      b->staged[3 * k] = b->frames[k].frame_seed;
+      b->staged[3 * k + 1] = frame_index;
+      b->staged[3 * k + 2] = vts::frame_key(b->frames[k].frame_seed, frame_index);
+    }
+    HIP_CHECK(e, hipMemcpyAsync(b->words.ptr, b->staged, (size_t)K * 12, hipMemcpyHostToDevice, e->stream));
+    HIP_CHECK(e, hipEventRecord(b->staged_done, e->stream));
+    HIP_CHECK(e, hipGraphLaunch(b->exec, e->stream));
+    return batch_end(b);
   });
 }
 

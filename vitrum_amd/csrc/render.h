@@ -71,6 +71,10 @@ struct FrameParams {
   const uint16_t* sobol_dirs;
   // Shade each workgroup's queue entries in material-sorted order.
   uint32_t shade_sort;
+  // When set, the frame's seed, index and Sobol frame key are read from these
+  // three device words instead (launches recorded once in a HIP graph and
+  // replayed frame after frame).
+  const uint32_t* frame_dev;
 };
 
 // Structure-of-arrays path state, one path per pixel.
@@ -137,6 +141,22 @@ hipError_t launch_nrc_apply(const PathState& s, uint32_t capacity, hipStream_t s
 hipError_t launch_nrc_targets(const FrameParams& f, const PathState& s, hipStream_t st);
 // which & 1: extension counts rotate; which & 2: the shadow count is cleared.
 hipError_t launch_reset_counts(const PathState& s, uint32_t which, hipStream_t st);
+
+// Batched frames: the device view of one member engine's queues.
+struct BatchMember {
+  const float4* rays[2];      // extension rays of the two ping-pong queues
+  uint32_t* hits;             // 5 words per queue entry
+  float* hit_pos;             // 3 floats per queue entry
+  const float4* shadow_rays;
+  uint32_t* shadow_occ;
+  const uint32_t* count;      // the member's queue counts (word 0: current, 2: shadow)
+};
+hipError_t launch_batch_prefix(const BatchMember* m, uint32_t count, uint32_t word, uint32_t* offsets,
+                               hipStream_t st);
+hipError_t launch_batch_gather(const BatchMember* m, uint32_t count, const uint32_t* offsets, uint32_t shadow,
+                               uint32_t cur, float4* out, uint32_t capacity, hipStream_t st);
+hipError_t launch_batch_scatter(const BatchMember* m, uint32_t count, const uint32_t* offsets, uint32_t shadow,
+                                const uint32_t* words, const float* pos, uint32_t capacity, hipStream_t st);
 hipError_t launch_zero_f32(float* p, uint32_t n, hipStream_t st);
 
 }  // namespace vtr

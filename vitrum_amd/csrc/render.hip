@@ -711,7 +711,20 @@ __device__ __forceinline__ Rng path_rng(const FrameParams& f, uint32_t px, uint3
 }
 
 // generatePrimaryRay / unproject_ray_common (kernelCore.wgsl.ts:22-26; common.wgsl.ts:412-469)
-__global__ void camera_kernel(FrameParams f, PathState s) {
+// The frame's parameters with the per-frame words taken from device memory
+// when the launch carries frame_dev.
+__device__ __forceinline__ FrameParams frame_of(const FrameParams& f0) {
+  FrameParams f = f0;
+  if (f0.frame_dev) {
+    f.frame_seed = f0.frame_dev[0];
+    f.frame_index = f0.frame_dev[1];
+    f.sobol_key = f0.frame_dev[2];
+  }
+  return f;
+}
+
+__global__ void camera_kernel(FrameParams f0, PathState s) {
+  const FrameParams f = frame_of(f0);
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t total = f.pixels + (f.nrc ? f.train_paths : 0u);
   if (p >= total) return;
@@ -779,8 +792,9 @@ __global__ void camera_kernel(FrameParams f, PathState s) {
 // diverge, and on the 1080p Cornell frames the sort cost 2-3% of the frame.
 constexpr uint32_t kSortBins = 16;
 
-__global__ __launch_bounds__(256) void shade_kernel(FrameParams f, ShadeScene sc, PathState s,
+__global__ __launch_bounds__(256) void shade_kernel(FrameParams f0, ShadeScene sc, PathState s,
                                                      uint32_t vertex) {
+  const FrameParams f = frame_of(f0);
   __shared__ uint32_t s_count[4][kSortBins];
   __shared__ uint16_t s_order[256];
   const uint32_t n = s.count[0];
@@ -1356,6 +1370,64 @@ __global__ void rotate_counts_kernel(uint32_t* count, uint32_t which) {
   }
 }
 
+// ---- batched frames: one traversal launch over the queues of several engines ----
+
+// offsets[k] = the rays of members 0..k-1 in queue word `word` (exclusive
+// prefix; offsets[count] = the total).  One thread: a batch has few members.
+__global__ void batch_prefix_kernel(const BatchMember* __restrict__ m, uint32_t count, uint32_t word,
+                                    uint32_t* __restrict__ offsets) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    uint32_t total = 0;
+    for (uint32_t k = 0; k < count; ++k) {
+      offsets[k] = total;
+      total += m[k].count[word];
+    }
+    offsets[count] = total;
+  }
+}
+
+__device__ __forceinline__ uint32_t batch_member_of(const uint32_t* offsets, uint32_t count, uint32_t i) {
+  uint32_t lo = 0, hi = count;
+  while (lo + 1u < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (offsets[mid] <= i) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// Copies each member's extension (shadow = 0, from rays[cur]) or shadow rays
+// into the batch's ray array, member k's at offsets[k].
+__global__ void batch_gather_kernel(const BatchMember* __restrict__ m, uint32_t count,
+                                    const uint32_t* __restrict__ offsets, uint32_t shadow, uint32_t cur,
+                                    float4* __restrict__ out) {
+  const uint32_t total = offsets[count];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+    const uint32_t k = batch_member_of(offsets, count, i), li = i - offsets[k];
+    const float4* src = shadow ? m[k].shadow_rays : m[k].rays[cur];
+    out[2 * (size_t)i] = src[2 * (size_t)li];
+    out[2 * (size_t)i + 1] = src[2 * (size_t)li + 1];
+  }
+}
+
+// Copies the batch's closest-hit records and hit positions (shadow = 0) or
+// occlusion words (shadow = 1) back to each member.
+__global__ void batch_scatter_kernel(const BatchMember* __restrict__ m, uint32_t count,
+                                     const uint32_t* __restrict__ offsets, uint32_t shadow,
+                                     const uint32_t* __restrict__ words, const float* __restrict__ pos) {
+  const uint32_t total = offsets[count];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+    const uint32_t k = batch_member_of(offsets, count, i), li = i - offsets[k];
+    if (shadow) {
+      m[k].shadow_occ[li] = words[i];
+    } else {
+#pragma unroll
+      for (uint32_t w = 0; w < 5; ++w) m[k].hits[5 * (size_t)li + w] = words[5 * (size_t)i + w];
+#pragma unroll
+      for (uint32_t c = 0; c < 3; ++c) m[k].hit_pos[3 * (size_t)li + c] = pos[3 * (size_t)i + c];
+    }
+  }
+}
+
 __global__ void zero_f32_kernel(float* p, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0.0f;
@@ -1412,6 +1484,28 @@ hipError_t launch_variance(const float* moments, uint32_t pixels, uint32_t sampl
   if (pixels == 0) return hipSuccess;
   hipLaunchKernelGGL(variance_kernel, dim3((pixels + 255) / 256), dim3(256), 0, st, moments, pixels,
                      (float)samples, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_prefix(const BatchMember* m, uint32_t count, uint32_t word, uint32_t* offsets,
+                               hipStream_t st) {
+  hipLaunchKernelGGL(batch_prefix_kernel, dim3(1), dim3(64), 0, st, m, count, word, offsets);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_gather(const BatchMember* m, uint32_t count, const uint32_t* offsets, uint32_t shadow,
+                               uint32_t cur, float4* out, uint32_t capacity, hipStream_t st) {
+  const uint32_t want = (capacity + 255u) / 256u, blocks = want < 4096u ? want : 4096u;
+  hipLaunchKernelGGL(batch_gather_kernel, dim3(blocks ? blocks : 1u), dim3(256), 0, st, m, count, offsets, shadow,
+                     cur, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_scatter(const BatchMember* m, uint32_t count, const uint32_t* offsets, uint32_t shadow,
+                                const uint32_t* words, const float* pos, uint32_t capacity, hipStream_t st) {
+  const uint32_t want = (capacity + 255u) / 256u, blocks = want < 4096u ? want : 4096u;
+  hipLaunchKernelGGL(batch_scatter_kernel, dim3(blocks ? blocks : 1u), dim3(256), 0, st, m, count, offsets, shadow,
+                     words, pos);
   return hipGetLastError();
 }
 

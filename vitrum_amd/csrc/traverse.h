@@ -65,6 +65,8 @@ struct TraceParams {
 hipError_t launch_trace(const TraceParams& p, bool any, bool count, uint32_t grid, hipStream_t s);
 // Closest hit over a batch of scenes (p.scenes / p.scene_offsets / p.scene_count).
 hipError_t launch_trace_batch(const TraceParams& p, uint32_t grid, hipStream_t s);
+// Any hit over a batch of scenes (occluded words into p.occluded).
+hipError_t launch_trace_batch_any(const TraceParams& p, uint32_t grid, hipStream_t s);
 hipError_t trace_occupancy(bool any, int* blocks_per_cu);  // the smallest over the kernels launched
 hipError_t trace_occupancy_batch(int* blocks_per_cu);
 hipError_t launch_zero(uint32_t* p, uint32_t n, hipStream_t s);

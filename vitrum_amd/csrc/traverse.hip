@@ -949,6 +949,11 @@ hipError_t launch_trace_batch(const TraceParams& p, uint32_t grid, hipStream_t s
                              : launch_one<false, false, true, false>(p, grid, s);
 }
 
+hipError_t launch_trace_batch_any(const TraceParams& p, uint32_t grid, hipStream_t s) {
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, p.counter, 1u);
+  return p.wide ? launch_one<true, false, true, true>(p, grid, s) : launch_one<true, false, true, false>(p, grid, s);
+}
+
 hipError_t launch_trace(const TraceParams& p, bool any, bool count, uint32_t grid, hipStream_t s) {
   hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, p.counter, 1u);
   if (count) return launch_one<false, true, false, false>(p, grid, s);

@@ -205,6 +205,12 @@ def lib():
         L.vt_capture_output.argtypes = [C.c_void_p, F32P, C.c_uint32, C.c_uint32, C.c_void_p]
         L.vt_capture_aux.argtypes = [C.c_void_p, C.c_uint32, F32P, C.c_uint32, C.c_uint32]
         L.vt_bake_procedural_sky.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, F32P, C.POINTER(C.c_double)]
+        L.vt_render_batch_create.argtypes = [C.c_void_p, C.c_void_p]
+        L.vt_render_batch_destroy.argtypes = [C.c_void_p]
+        L.vt_render_batch_add.argtypes = [C.c_void_p, C.c_void_p]
+        L.vt_render_batch_frame.argtypes = [C.c_void_p, C.c_void_p]
+        L.vt_render_batch_capture.argtypes = [C.c_void_p, C.c_void_p]
+        L.vt_render_batch_replay.argtypes = [C.c_void_p, C.c_uint32]
         _LIB = L
     return _LIB
 
@@ -648,3 +654,53 @@ class Batch:
                                       C.byref(t)), self._eng.handle)
         return {"refill_min": t.refill_min, "steps": t.steps, "near_first": t.near_first,
                 "defer_min": t.defer_min, "blocks_per_cu": t.blocks_per_cu, "ms": t.ms}
+
+
+class RenderBatch:
+    """Several engines' frames rendered together (vt_render_batch_*): every
+    bounce's rays of all members traced in one persistent launch, on the
+    host engine's stream; capture() records a frame into a HIP graph that
+    replay() launches for later frame indices."""
+
+    def __init__(self, host):
+        self._host = host
+        self._h = C.c_void_p()
+        self._members = []
+        check(lib().vt_render_batch_create(host.handle, C.byref(self._h)), host.handle)
+
+    def close(self):
+        if self._h:
+            lib().vt_render_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, engine):
+        check(lib().vt_render_batch_add(self._h, engine.handle), self._host.handle)
+        self._members.append(engine)
+
+    @staticmethod
+    def _frames(frames):
+        arr = (Frame * len(frames))()
+        for k, fr in enumerate(frames):
+            arr[k] = Frame((C.c_float * 16)(*np.asarray(fr["inv_view_proj"], np.float32).ravel()), fr["width"],
+                           fr["height"], fr["frame_index"], fr.get("frame_seed", 0x9E3779B9),
+                           fr.get("max_bounces", 0), fr.get("samples_target", 0))
+        return arr
+
+    def frame(self, frames):
+        """frames: one dict per member (inv_view_proj, width, height,
+        frame_index, frame_seed, max_bounces)."""
+        arr = self._frames(frames)
+        check(lib().vt_render_batch_frame(self._h, C.cast(arr, C.c_void_p)), self._host.handle)
+
+    def capture(self, frames):
+        arr = self._frames(frames)
+        check(lib().vt_render_batch_capture(self._h, C.cast(arr, C.c_void_p)), self._host.handle)
+
+    def replay(self, frame_index):
+        check(lib().vt_render_batch_replay(self._h, C.c_uint32(frame_index)), self._host.handle)
