@@ -380,8 +380,22 @@ class HipPathTracerEngine {
   // Engine.renderFrame (core/src/engine/index.ts:251): one sample per pixel,
   // accumulated into the running mean until quality.samplesTarget.
   renderFrame(input) {
+    const a = this._frameArgs(input);
+    if (a.done) return a.done;
+    try {
+      this._n.renderFrame(this._h, a.frame);
+    } catch (err) {
+      this._fail(err);
+      throw err;
+    }
+    return this._frameDone(a.target);
+  }
+
+  // Validates a FrameInput; {done} when nothing is to be rendered, else the
+  // addon's frame description and the sample target.
+  _frameArgs(input) {
     this._live();
-    if (this.state === 'paused' || this._scene == null) return SKIPPED;
+    if (this.state === 'paused' || this._scene == null) return { done: SKIPPED };
     if (input == null || !isMat4(input.viewMatrix) || !isMat4(input.projMatrix)) {
       throw new TypeError('FrameInput.viewMatrix and projMatrix must be mat4s');
     }
@@ -396,11 +410,11 @@ class HipPathTracerEngine {
     if (this._size && (this._size[0] !== width || this._size[1] !== height)) this._samples = 0;
     this._size = [width, height];
     if (target && this._samples >= target) {
-      const out = { kind: 'rendered', primaryRadiance: this._texture(), samplesAccumulated: this._samples, isConverged: true };
-      return out;
+      return { done: { kind: 'rendered', primaryRadiance: this._texture(), samplesAccumulated: this._samples, isConverged: true } };
     }
-    try {
-      this._n.renderFrame(this._h, {
+    return {
+      target,
+      frame: {
         invViewProj: inv,
         width,
         height,
@@ -408,11 +422,11 @@ class HipPathTracerEngine {
         frameSeed: input.frameSeed >>> 0,
         maxBounces: q.bounces == null ? 0 : q.bounces,
         samplesTarget: target,
-      });
-    } catch (err) {
-      this._fail(err);
-      throw err;
-    }
+      },
+    };
+  }
+
+  _frameDone(target) {
     this._samples += 1;
     const out = {
       kind: 'rendered',
@@ -593,10 +607,35 @@ async function constructPathTracerHipForDispatch(opts, scene /* , aabb, needsTla
   return engine;
 }
 
+// Batched frames (no counterpart in the reference): one renderFrame of each
+// engine, with every bounce's rays of all of them traced in one persistent
+// launch (vt_render_batch_*).  Each engine's result is what its own
+// renderFrame would give.  Engines with the radiance cache, the
+// cwbvh-closest traversal or nothing to render this frame cannot join.
+function renderFrames(engines, inputs) {
+  if (!Array.isArray(engines) || !Array.isArray(inputs) || engines.length === 0 || engines.length !== inputs.length) {
+    throw new TypeError('renderFrames(engines, inputs) takes equally long, non-empty arrays');
+  }
+  const args = engines.map((eng, k) => {
+    if (!(eng instanceof HipPathTracerEngine)) throw new TypeError(`engines[${k}] is not a pt-hip engine`);
+    const a = eng._frameArgs(inputs[k]);
+    if (a.done) throw new RangeError(`engines[${k}] has nothing to render this frame`);
+    return a;
+  });
+  try {
+    engines[0]._n.renderFrames(engines.map((e) => e._h), args.map((a) => a.frame));
+  } catch (err) {
+    for (const eng of engines) eng._fail(err);
+    throw err;
+  }
+  return engines.map((eng, k) => eng._frameDone(args[k].target));
+}
+
 module.exports = {
   createPTEngine_HIP,
   constructPathTracerHipForDispatch,
   multiplyMat4,
   sceneBlob,
   loadAddon,
+  renderFrames,
 };

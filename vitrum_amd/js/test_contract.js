@@ -141,6 +141,32 @@ assert.throws(() => pt.sceneBlob({ primitives: [], environment: { kind: 'vacuum'
     assert.ok(hc.rgba[(8 * 16 + 8) * 4] > 0, `${environment.kind} lights the quad`);
     he.dispose();
   }
+  // batched frames: two engines' frames in one call equal their own renderFrame
+  {
+    const scenes = [{ primitives: [quad], emitters: [], environment: { kind: 'hdri', hdri: { constant: [1, 1, 1] } } },
+      { primitives: [quad], environment: { kind: 'procedural-sky', sunDirection: [0.3, 0.8, 0.2] },
+        emitters: [{ kind: 'point', color: [1, 1, 1], intensity: 2, position: [0.4, 0.4, 1] }] }];
+    const solo = [], both = [];
+    for (let k = 0; k < 2; k += 1) {
+      solo.push(await pt.createPTEngine_HIP({ device: 0 }));
+      both.push(await pt.createPTEngine_HIP({ device: 0 }));
+      solo[k].setScene(scenes[k]);
+      both[k].setScene(scenes[k]);
+    }
+    const inputs = [0, 1].map((k) => ({ viewMatrix: view, projMatrix: proj, viewport: { width: 16 + 8 * k, height: 16, devicePixelRatio: 1 },
+      frameIndex: 0, frameSeed: 5 + k, quality: { bounces: 3 + k } }));
+    for (let k = 0; k < 2; k += 1) solo[k].renderFrame(inputs[k]);
+    const outs = pt.renderFrames(both, inputs);
+    assert.strictEqual(outs.length, 2);
+    for (let k = 0; k < 2; k += 1) {
+      assert.strictEqual(outs[k].samplesAccumulated, 1);
+      const a = await solo[k].captureFrame(), b = await both[k].captureFrame();
+      assert.deepStrictEqual(Array.from(a.rgba), Array.from(b.rgba), `batched frame ${k} equals its solo frame`);
+      solo[k].dispose();
+      both[k].dispose();
+    }
+    assert.throws(() => pt.renderFrames([], []), TypeError);
+  }
   // traceClosest through the addon: a ray straight at the triangle.
   const hits = new Float32Array(addon.traceClosest(eng._h, new Float32Array([0.6, 0.3, 1, 0, 0, 0, -1, 1e30])));
   assert.ok(Math.abs(hits[2] - 1) < 1e-6, `t = ${hits[2]}`);

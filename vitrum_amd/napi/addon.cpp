@@ -456,6 +456,65 @@ napi_value RenderFrame(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// Reads one frame description (renderFrame's second argument).
+bool frame_of(napi_env env, napi_value v, vt_frame* f) {
+  if (is_nullish(env, v)) {
+    throw_type(env, "frame description is required");
+    return false;
+  }
+  *f = vt_frame{};
+  void* d;
+  size_t n;
+  if (!get_typed(env, v, "invViewProj", napi_float32_array, false, &d, &n)) return false;
+  if (n != 16) {
+    throw_type(env, "'invViewProj' must be a Float32Array(16)");
+    return false;
+  }
+  std::memcpy(f->inv_view_proj, d, 64);
+  return get_u32(env, v, "width", 0, &f->width) && get_u32(env, v, "height", 0, &f->height) &&
+         get_u32(env, v, "frameIndex", 0, &f->frame_index) && get_u32(env, v, "frameSeed", 0, &f->frame_seed) &&
+         get_u32(env, v, "maxBounces", 0, &f->max_bounces) && get_u32(env, v, "samplesTarget", 0, &f->samples_target);
+}
+
+// renderFrames([handle...], [frame...]): one frame of every engine, batched
+// (vt_render_batch_*: each bounce's rays of all engines in one traversal
+// launch, on the first engine's stream).
+napi_value RenderFrames(napi_env env, napi_callback_info info) {
+  auto a = args_of(env, info, 2);
+  bool ha = false, fa = false;
+  napi_is_array(env, a[0], &ha);
+  napi_is_array(env, a[1], &fa);
+  uint32_t nh = 0, nf = 0;
+  if (!ha || !fa || napi_get_array_length(env, a[0], &nh) != napi_ok || napi_get_array_length(env, a[1], &nf) != napi_ok ||
+      nh == 0 || nh != nf) {
+    throw_type(env, "renderFrames([handle...], [frame...]) takes equally long, non-empty arrays");
+    return nullptr;
+  }
+  std::vector<vt_engine*> engs(nh);
+  std::vector<vt_frame> frames(nh);
+  for (uint32_t i = 0; i < nh; ++i) {
+    napi_value hv, fv;
+    NAPI_OK(napi_get_element(env, a[0], i, &hv));
+    NAPI_OK(napi_get_element(env, a[1], i, &fv));
+    Handle* h = handle_of(env, hv);
+    if (!h) return nullptr;
+    engs[i] = h->eng;
+    if (!frame_of(env, fv, &frames[i])) return nullptr;
+  }
+  vt_render_batch* b = nullptr;
+  if (throw_status(env, engs[0], vt_render_batch_create(engs[0], &b))) return nullptr;
+  vt_status s = VT_OK;
+  for (uint32_t i = 0; i < nh && !s; ++i) s = vt_render_batch_add(b, engs[i]);
+  if (!s) s = vt_render_batch_frame(b, frames.data());
+  if (s) {
+    throw_status(env, engs[0], s);
+    vt_render_batch_destroy(b);
+    return nullptr;
+  }
+  vt_render_batch_destroy(b);
+  return nullptr;
+}
+
 // capture(handle, width, height) -> Float32Array (row-major RGBA, top-left origin)
 napi_value Capture(napi_env env, napi_callback_info info) {
   auto a = args_of(env, info, 3);
@@ -686,7 +745,8 @@ napi_value Init(napi_env env, napi_value exports) {
              {"stats", Stats},           {"traceClosest", TraceClosest},
              {"traceAny", TraceAny},     {"invertMat4", InvertMat4},
              {"nrcConfig", NrcConfig},   {"version", Version},
-             {"captureOutput", CaptureOutput}, {"captureAux", CaptureAux}};
+             {"captureOutput", CaptureOutput}, {"captureAux", CaptureAux},
+             {"renderFrames", RenderFrames}};
   for (const auto& f : fns) {
     napi_value fn;
     if (napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.cb, nullptr, &fn) != napi_ok) return nullptr;
