@@ -87,8 +87,12 @@ typedef struct vt_primitive {
   uint32_t transform_count;
 } vt_primitive;
 
-/* MaterialSpec subset rendered by the Lambert + GGX base BSDF
- * (packages/core/src/scene/material.ts:167-458). */
+/* MaterialSpec subset rendered by the opaque BSDF: the Lambert + GGX base
+ * and the clearcoat, sheen and anisotropy extension lobes
+ * (packages/core/src/scene/material.ts:167-458; bsdf.wgsl.ts:392-916).
+ * The extension scalars are clamped as materialPacking.ts:249-316 packs
+ * them; 0 turns a lobe off, so a zero-initialised extension block renders
+ * the base BSDF.  Non-finite values are VT_ERR_RANGE at vt_set_scene. */
 typedef struct vt_material {
   float base_color[3];
   float roughness;
@@ -96,6 +100,13 @@ typedef struct vt_material {
   float emissive[3];
   float emissive_intensity;
   uint32_t double_sided;
+  float clearcoat, clearcoat_roughness;
+  float sheen, sheen_roughness, sheen_color[3];
+  float anisotropy, anisotropy_rotation; /* strength in [0,1]; radians */
+  /* KHR_materials_specular: used when has_specular is non-zero, else the
+   * defaults specularColor (1,1,1) and specularIntensity 1. */
+  uint32_t has_specular;
+  float specular_color[3], specular_intensity;
 } vt_material;
 
 /* Environment (packages/core/src/scene/environment.ts:16-37).
@@ -493,6 +504,46 @@ vt_status vt_pack_scene(const vt_primitive *prims, uint32_t prim_count, uint32_t
 int64_t vt_pack_field_words(const vt_pack *p, int32_t field);
 vt_status vt_pack_copy_field(const vt_pack *p, int32_t field, void *dst, uint64_t dst_words);
 void vt_pack_free(vt_pack *p);
+
+/* Inspection of the bound scene: the VT_PACK_* fields of its pack (host
+ * copies, kept current by refits) and its device traversal trees
+ * (VT_SCENE_BLAS_NODE64 / _NODE128: the BLAS forests in the Node64 and
+ * Node128 layouts; VT_SCENE_TRIANGLES: the 12-word triangle records).
+ * *words receives the field's size in 32-bit words; dst may be NULL to ask
+ * for it.  VT_ERR_RANGE when dst_words is too small. */
+enum { VT_SCENE_BLAS_NODE64 = 64, VT_SCENE_BLAS_NODE128 = 65, VT_SCENE_TRIANGLES = 66 };
+vt_status vt_scene_copy_field(vt_engine *eng, int32_t field, void *dst, uint64_t dst_words, uint64_t *words);
+
+/* refitBvhBounds (packages/shared-bvh/src/refitBvhBounds.ts:80-179) on
+ * device buffers: recomputes, in place, every box of the 32-byte preorder
+ * BVH in d_nodes (node_count nodes, root at node 0) from its triangles
+ * (index_stride words per triangle, 3 or 4; position_stride floats per
+ * vertex, 3 or 4; the fourth lane is ignored), keeping the topology.  Boxes
+ * equal the reference's bit for bit.  The preflight of refitBvhBounds.ts:3-63
+ * runs first on the device: a corrupt or unreachable node, a bad triangle
+ * range or vertex index, or a non-finite vertex returns VT_ERR_BUILD without
+ * writing the nodes; a stride other than 3 or 4 is VT_ERR_RANGE.  Ordered on
+ * the engine's stream (the refit itself completes asynchronously). */
+vt_status vt_refit_bvh_bounds_device(vt_engine *eng, uint32_t *d_nodes, uint32_t node_count,
+                                     const uint32_t *d_indices, uint32_t index_words, uint32_t index_stride,
+                                     const float *d_positions, uint32_t position_floats,
+                                     uint32_t position_stride);
+
+/* Moves the vertices of a bound scene's primitive (scene order index) with
+ * the topology kept, as the reference's refit path for moved geometry
+ * (walkaround-hybrid HybridEnginePrimitiveUpdates.ts:1482-1560): positions
+ * (3 floats per vertex, the primitive's vertex count) and optionally normals
+ * replace its vertices, its BLAS is refitted on the device
+ * (refitBvhBounds), the TLAS over its instances' new world boxes
+ * (refitTlasTransforms, scenePack.ts:1310-1420; refitTlasInstances,
+ * tlas.ts:1141-1209), the ray bias follows the new scene bounds, and
+ * accumulation restarts.  Errors leave the scene unchanged: VT_ERR_RANGE for
+ * a primitive without packed geometry, another vertex count or non-finite
+ * data; VT_ERR_STATE without a scene or for an emissive primitive (its
+ * mesh-area emitters would move: set the scene again). */
+vt_status vt_refit_primitive(vt_engine *eng, uint32_t primitive, const float *positions,
+                             uint32_t position_floats, const float *normals, uint32_t normal_floats);
+
 
 /* ---- neural radiance cache ------------------------------------------------
  * The online-trained cache of walkaround-hybrid's NRC subsystem

@@ -122,21 +122,38 @@ typedef struct {
   double rough, metal;
   d3 emit;
   int two_sided;
+  /* extension lobes (bsdf.wgsl.ts:392-808) */
+  double cc, cc_rough, sheen, sheen_rough;
+  d3 sheen_color;
+  double aniso, aniso_c, aniso_s; /* strength; cos, sin of the rotation (binary32 table values) */
+  d3 spec_color;
+  double spec_int;
 } mat_t;
 
-/* materialSpecularF0 (bsdf.wgsl.ts:192-209), ior 1.5, white specular. */
+/* materialSpecularF0 (bsdf.wgsl.ts:192-209), ior 1.5: the IOR F0 tinted by
+ * specularColor (clamped to 1), scaled by specularIntensity, mixed towards the
+ * base colour by metallic. */
 static d3 spec_f0(const mat_t *m) {
   double ratio = (1.5 - 1.0) / (1.5 + 1.0);
   double f = ratio * ratio;
+  double si = clampd(m->spec_int, 0, 1);
+  d3 d = D3(mind(f * maxd(m->spec_color.x, 0), 1) * si, mind(f * maxd(m->spec_color.y, 0), 1) * si,
+            mind(f * maxd(m->spec_color.z, 0), 1) * si);
   double mt = clampd(m->metal, 0, 1);
-  return D3(f + (m->base.x - f) * mt, f + (m->base.y - f) * mt, f + (m->base.z - f) * mt);
+  return D3(d.x + (m->base.x - d.x) * mt, d.y + (m->base.y - d.y) * mt, d.z + (m->base.z - d.z) * mt);
+}
+/* F90 of materialSpecularFresnelSchlick: specularIntensity for dielectrics,
+ * 1 for metals. */
+static double spec_f90(const mat_t *m) {
+  double d = clampd(m->spec_int, 0, 1);
+  return d + (1 - d) * clampd(m->metal, 0, 1);
 }
 
 /* materialSpecularFresnelSchlick (bsdf.wgsl.ts:211-222) */
-static d3 schlick(double cos_t, d3 f0) {
+static d3 schlick(double cos_t, d3 f0, double f90) {
   double x = clampd(1.0 - cos_t, 0, 1);
   double x5 = x * x * x * x * x;
-  return D3(f0.x + (1 - f0.x) * x5, f0.y + (1 - f0.y) * x5, f0.z + (1 - f0.z) * x5);
+  return D3(f0.x + (f90 - f0.x) * x5, f0.y + (f90 - f0.y) * x5, f0.z + (f90 - f0.z) * x5);
 }
 static d3 clamp01v(d3 v) { return D3(clampd(v.x, 0, 1), clampd(v.y, 0, 1), clampd(v.z, 0, 1)); }
 
@@ -196,10 +213,11 @@ static double lut_eavg(double rough) {
   double tr = fr - r0;
   return clampd(GGX_EAVG_LUT[r0] + (GGX_EAVG_LUT[r1] - GGX_EAVG_LUT[r0]) * tr, 0.3, 1.0);
 }
-static d3 ggx_ms(d3 f0, double rough, double ndv, double ndl) {
-  double eavg = lut_eavg(rough), om = 1.0 - eavg;
+/* ggxMultiscatterLobeRoughness (material.wgsl.ts:2906-2926) */
+static d3 ggx_ms(d3 f0, double rough_v, double rough_l, double rough_avg, double ndv, double ndl) {
+  double eavg = lut_eavg(rough_avg), om = 1.0 - eavg;
   if (om < 1e-4) return D3(0, 0, 0);
-  double eo = lut_e(ndv, rough), ei = lut_e(ndl, rough);
+  double eo = lut_e(ndv, rough_v), ei = lut_e(ndl, rough_l);
   double shape = (1.0 - eo) * (1.0 - ei) / maxd(PI * om, 1e-6);
   double c[3] = {f0.x, f0.y, f0.z}, o[3];
   for (int k = 0; k < 3; ++k) {
@@ -210,47 +228,240 @@ static d3 ggx_ms(d3 f0, double rough, double ndv, double ndl) {
   return D3(o[0], o[1], o[2]);
 }
 
-/* evaluateBrdfFullWithClearcoatNormal (bsdf.wgsl.ts:809-916), base lobes only. */
+/* buildOnb (bsdf.wgsl.ts:1632-1640) */
+static void build_onb(d3 n, d3 *t, d3 *b) {
+  d3 up = fabs(n.y) > 0.999 ? D3(1, 0, 0) : D3(0, 1, 0);
+  d3 c = cross(up, n);
+  double l = sqrt(dot(c, c));
+  *t = D3(c.x / l, c.y / l, c.z / l);
+  *b = cross(n, *t);
+}
+/* the frame rotated by anisotropyRotation (bsdf.wgsl.ts:2582-2595) */
+static void aniso_frame(const mat_t *m, d3 n, d3 *t, d3 *b) {
+  d3 t0, b0;
+  build_onb(n, &t0, &b0);
+  *t = add(scl(t0, m->aniso_c), scl(b0, m->aniso_s));
+  *b = add(scl(t0, -m->aniso_s), scl(b0, m->aniso_c));
+}
+static double smoothstep_d(double e0, double e1, double x) {
+  double t = clampd((x - e0) / (e1 - e0), 0, 1);
+  return t * t * (3 - 2 * t);
+}
+/* computeAnisotropicAxes (bsdf.wgsl.ts:1928-1933) */
+static void aniso_axes(double alpha, double aniso, double *ax, double *ay) {
+  double aspect = sqrt(maxd(1.0 - 0.9 * aniso, 1e-4));
+  *ax = maxd(alpha / aspect, 1e-4);
+  *ay = maxd(alpha * aspect, 1e-4);
+}
+/* ggxDAnis, smithG1Anis (bsdf.wgsl.ts:597-608) */
+static double ggx_d_aniso(double ht, double hb, double hn, double ax, double ay) {
+  double d = (ht / ax) * (ht / ax) + (hb / ay) * (hb / ay) + hn * hn;
+  return 1.0 / maxd(PI * ax * ay * d * d, 1e-10);
+}
+static double smith_g1_aniso(double vt, double vb, double vn, double ax, double ay) {
+  double vn2 = maxd(vn * vn, 1e-10);
+  return (2.0 * vn) / maxd(vn + sqrt(vn2 + (vt * ax) * (vt * ax) + (vb * ay) * (vb * ay)), 1e-6);
+}
+/* anisotropicProjectedRoughness / anisotropicAverageRoughness /
+ * anisotropicMultiscatterScale (bsdf.wgsl.ts:755-796) */
+static double aniso_projected_rough(d3 dir, d3 t, d3 b, double rough, double aniso) {
+  double ax, ay;
+  aniso_axes(maxd(rough * rough, ALPHA_FLOOR), aniso, &ax, &ay);
+  double dt = dot(dir, t), db = dot(dir, b), tl2 = dt * dt + db * db;
+  double blend = clampd(0.15 * aniso, 0, 0.15), projected;
+  if (tl2 <= 1e-6)
+    projected = sqrt(clampd(0.5 * (ax + ay), 1e-4, 1));
+  else
+    projected = sqrt(clampd(sqrt(((dt * ax) * (dt * ax) + (db * ay) * (db * ay)) / tl2), 1e-4, 1));
+  return rough + (projected - rough) * blend;
+}
+static double aniso_average_rough(double rough, double aniso) {
+  double ax, ay;
+  aniso_axes(maxd(rough * rough, ALPHA_FLOOR), aniso, &ax, &ay);
+  double projected = sqrt(clampd(sqrt(0.5 * (ax * ax + ay * ay)), 1e-4, 1));
+  return rough + (projected - rough) * clampd(0.15 * aniso, 0, 0.15);
+}
+static double aniso_ms_scale(double aniso, double rough) {
+  double reduction = smoothstep_d(0, 0.35, clampd(aniso, 0, 1));
+  double scale = 0.6 * smoothstep_d(0.35, 0.9, rough);
+  return 1.0 + (scale - 1.0) * reduction;
+}
+/* brdfAnisotropicSpecPdf (bsdf.wgsl.ts:727-753) */
+static double aniso_spec_pdf(const mat_t *m, d3 n, d3 t, d3 b, d3 wo, d3 wi) {
+  double ndv = maxd(dot(n, wo), 1e-6), ndl = maxd(dot(n, wi), 0);
+  if (ndl <= 1e-5) return 0;
+  d3 h = safe_normalize(add(wo, wi));
+  double ax, ay;
+  aniso_axes(maxd(m->rough * m->rough, ALPHA_FLOOR), m->aniso, &ax, &ay);
+  double d = ggx_d_aniso(dot(h, t), dot(h, b), maxd(dot(h, n), 0), ax, ay);
+  double g1 = smith_g1_aniso(dot(wo, t), dot(wo, b), maxd(dot(wo, n), 1e-6), ax, ay);
+  return (d * g1) / maxd(4.0 * ndv, 1e-6);
+}
+/* clearcoatLayerWeight / evalClearcoatLobe / clearcoatPdf (bsdf.wgsl.ts:398-472) */
+static double cc_weight(const mat_t *m, d3 n, d3 wo) {
+  double x = clampd(1.0 - clampd(fabs(dot(n, wo)), 0, 1), 0, 1);
+  double fcc = 0.04 + (1.0 - 0.04) * (x * x * x * x * x);
+  return clampd(m->cc, 0, 1) * fcc;
+}
+static double cc_eval(const mat_t *m, d3 n, d3 wo, d3 wi) {
+  if (m->cc <= 0) return 0;
+  double ndl = maxd(dot(n, wi), 0), ndv = maxd(dot(n, wo), 0);
+  if (ndl <= 1e-5 || ndv <= 1e-5) return 0;
+  d3 h = safe_normalize(add(wi, wo));
+  double d = ggx_d(maxd(dot(n, h), 0), maxd(m->cc_rough * m->cc_rough, ALPHA_FLOOR));
+  double g = smith_g1(ndv, m->cc_rough) * smith_g1(ndl, m->cc_rough);
+  return cc_weight(m, n, wo) * ((d * g) / maxd(4.0 * ndv * ndl, 1e-6));
+}
+static double cc_pdf(const mat_t *m, d3 n, d3 wo, d3 wi) {
+  if (m->cc <= 0) return 0;
+  double ndv = maxd(dot(n, wo), 0);
+  if (ndv <= 1e-5) return 0;
+  double ndl = maxd(dot(n, wi), 0);
+  if (ndl <= 1e-5) return 0;
+  d3 h = safe_normalize(add(wi, wo));
+  double d = ggx_d(maxd(dot(n, h), 0), maxd(m->cc_rough * m->cc_rough, ALPHA_FLOOR));
+  return (d * smith_g1(ndv, m->cc_rough)) / maxd(4.0 * ndv, 1e-6);
+}
+/* charlieD, sheenVisibility, evalSheenLobe, sheenDirectionalAlbedo,
+ * sheenBaseAttenuation, charlieSheenPdf (bsdf.wgsl.ts:474-596) */
+static double charlie_d(double ndh, double alpha) {
+  double inv = 1.0 / maxd(alpha, 1e-4);
+  double sin_h = sqrt(maxd(0, 1 - ndh * ndh));
+  return (2.0 + inv) * pow(sin_h, inv) / (2.0 * PI);
+}
+static double sheen_vis(double ndl, double ndv) { return 1.0 / maxd(4.0 * (ndl + ndv - ndl * ndv), 1e-6); }
+static d3 sheen_eval(const mat_t *m, d3 n, d3 wo, d3 wi) {
+  if (m->sheen <= 0) return D3(0, 0, 0);
+  double ndl = maxd(dot(n, wi), 0), ndv = maxd(dot(n, wo), 0);
+  if (ndl <= 1e-5 || ndv <= 1e-5) return D3(0, 0, 0);
+  d3 h = safe_normalize(add(wi, wo));
+  double alpha = maxd(m->sheen_rough * m->sheen_rough, ALPHA_FLOOR);
+  return scl(m->sheen_color, m->sheen * charlie_d(maxd(dot(n, h), 0), alpha) * sheen_vis(ndl, ndv));
+}
+static double sheen_albedo(double c_raw, double alpha) {
+  double c = 1.0 - clampd(c_raw, 0, 1);
+  return 0.65584461 * (c * c * c) + 1.0 / (4.16526551 + exp(-7.97291361 * sqrt(alpha) + 6.33516894));
+}
+static double sheen_attenuation(const mat_t *m, d3 n, d3 wo, d3 wi) {
+  if (m->sheen <= 0) return 1;
+  double ndl = clampd(fabs(dot(n, wi)), 0, 1), ndv = clampd(fabs(dot(n, wo)), 0, 1);
+  double a = maxd(m->sheen_rough, 0.07), a2 = a * a;
+  double mc = maxd(maxd(m->sheen_color.x, m->sheen_color.y), m->sheen_color.z);
+  double full = mind(1.0 - mc * sheen_albedo(ndv, a2), 1.0 - mc * sheen_albedo(ndl, a2));
+  return clampd(1.0 + (full - 1.0) * clampd(m->sheen, 0, 1), 0, 1);
+}
+static double sheen_pdf(const mat_t *m, d3 n, d3 wo, d3 wi) {
+  if (m->sheen <= 0) return 0;
+  double ndl = maxd(dot(n, wi), 0), ndv = maxd(dot(n, wo), 0);
+  if (ndl <= 1e-5 || ndv <= 1e-5) return 0;
+  d3 h = safe_normalize(add(wi, wo));
+  double ndh = maxd(dot(n, h), 0), vdh = maxd(dot(wo, h), 1e-6);
+  double alpha = maxd(m->sheen_rough * m->sheen_rough, ALPHA_FLOOR);
+  return (charlie_d(ndh, alpha) * ndh) / maxd(4.0 * vdh, 1e-6);
+}
+
+/* evaluateBrdfFullWithClearcoatNormal (bsdf.wgsl.ts:809-916): base lobes
+ * (Lambert, isotropic or anisotropic GGX with evalBrdfSpecAnisotropic,
+ * Kulla-Conty), under sheen, under clearcoat:
+ *   (base * sheenAttenuation + sheen) * clearcoatAttenuation + clearcoat. */
 static d3 bsdf_eval(const mat_t *m, d3 n, d3 wo, d3 wi) {
   double ndl = maxd(dot(n, wi), 0), ndv = maxd(dot(n, wo), 0);
   if (ndl <= 1e-5 || ndv <= 1e-5) return D3(0, 0, 0);
   d3 h = safe_normalize(add(wi, wo));
   double ndh = maxd(dot(n, h), 0), vdh = maxd(dot(wo, h), 0);
   d3 f0 = spec_f0(m);
-  d3 F = clamp01v(schlick(vdh, f0));
+  d3 F = clamp01v(schlick(vdh, f0, spec_f90(m)));
   double kdm = 1.0 - m->metal;
   d3 diff = D3((1 - F.x) * kdm * m->base.x * INV_PI, (1 - F.y) * kdm * m->base.y * INV_PI,
                (1 - F.z) * kdm * m->base.z * INV_PI);
-  double alpha = maxd(m->rough * m->rough, ALPHA_FLOOR);
-  double d = ggx_d(ndh, alpha);
-  double g = smith_g1(ndv, m->rough) * smith_g1(ndl, m->rough);
-  double k = (d * g) / maxd(4.0 * ndv * ndl, 1e-6);
-  d3 ms = ggx_ms(clamp01v(f0), m->rough, ndv, ndl);
-  return add(add(diff, scl(F, k)), ms);
+  d3 spec, ms;
+  if (m->aniso > 0) {
+    d3 t, b;
+    aniso_frame(m, n, &t, &b);
+    double ax, ay;
+    aniso_axes(maxd(m->rough * m->rough, ALPHA_FLOOR), m->aniso, &ax, &ay);
+    double d = ggx_d_aniso(dot(h, t), dot(h, b), maxd(dot(h, n), 0), ax, ay);
+    double g = smith_g1_aniso(dot(wo, t), dot(wo, b), maxd(dot(wo, n), 1e-6), ax, ay) *
+               smith_g1_aniso(dot(wi, t), dot(wi, b), maxd(dot(wi, n), 1e-6), ax, ay);
+    spec = scl(F, (d * g) / maxd(4.0 * maxd(ndv, 1e-6) * ndl, 1e-6));
+    double ravg = aniso_average_rough(m->rough, m->aniso);
+    ms = scl(ggx_ms(clamp01v(f0), aniso_projected_rough(wo, t, b, m->rough, m->aniso),
+                    aniso_projected_rough(wi, t, b, m->rough, m->aniso), ravg, ndv, ndl),
+             aniso_ms_scale(m->aniso, ravg));
+  } else {
+    double alpha = maxd(m->rough * m->rough, ALPHA_FLOOR);
+    double d = ggx_d(ndh, alpha);
+    double g = smith_g1(ndv, m->rough) * smith_g1(ndl, m->rough);
+    spec = scl(F, (d * g) / maxd(4.0 * ndv * ndl, 1e-6));
+    ms = ggx_ms(clamp01v(f0), m->rough, m->rough, m->rough, ndv, ndl);
+  }
+  d3 base = add(add(diff, spec), ms);
+  if (m->cc <= 0 && m->sheen <= 0) return base;
+  double cc_att = m->cc > 0 ? clampd(1.0 - cc_weight(m, n, wo), 0, 1) : 1.0;
+  d3 layered = scl(add(scl(base, sheen_attenuation(m, n, wo, wi)), sheen_eval(m, n, wo, wi)), cc_att);
+  double cc = cc_eval(m, n, wo, wi);
+  return D3(layered.x + cc, layered.y + cc, layered.z + cc);
 }
 
 /* brdfFiniteBaseLobeWeights (bsdf.wgsl.ts:1172-1210): specular probability. */
 static double spec_prob(const mat_t *m, double ndv) {
-  d3 F = clamp01v(schlick(ndv, spec_f0(m)));
+  d3 F = clamp01v(schlick(ndv, spec_f0(m), spec_f90(m)));
   double x = maxd(luminance(F), m->metal);
   return represented(clampd(0.04 + (0.96 - 0.04) * x, 0.04, 0.96));
 }
 
-/* brdfDirectionalPdfWithIridescence (bsdf.wgsl.ts:1555-1631). */
+/* brdfRepresentedExtensionLobeProbabilities (bsdf.wgsl.ts:1219-1262):
+ * p[0..3] = joint base, joint clearcoat, joint sheen, P(clearcoat | ext). */
+static void ext_probs(const mat_t *m, double p[4]) {
+  double cc = maxd(m->cc, 0), sh = maxd(m->sheen, 0), ext = cc + sh;
+  if (!(ext > 0)) {
+    p[0] = 1, p[1] = p[2] = p[3] = 0;
+    return;
+  }
+  double total = 1.0 + ext, pb, pe, pc;
+  if (ext <= 1.0) {
+    pe = represented(ext / total);
+    pb = 1.0 - pe;
+  } else {
+    pb = represented(1.0 / total);
+    pe = 1.0 - pb;
+  }
+  if (!(cc > 0))
+    pc = 0;
+  else if (!(sh > 0))
+    pc = 1;
+  else if (cc <= sh)
+    pc = represented(cc / ext);
+  else
+    pc = 1.0 - represented(sh / ext);
+  p[0] = pb, p[1] = pe * pc, p[2] = pe - pe * pc, p[3] = pc;
+}
+
+/* brdfDirectionalPdfFullWithClearcoatNormal (bsdf.wgsl.ts:1370-1489) over
+ * brdfDirectionalPdfWithIridescence (:1555-1631) or the anisotropic VNDF. */
 static double bsdf_pdf(const mat_t *m, d3 n, d3 wo, d3 wi) {
   double wdn = dot(n, wi), odn = dot(n, wo);
   double ndv = maxd(odn, 0);
   if (ndv <= 1e-5) return 0;
-  d3 h = safe_normalize(add(wi, wo));
-  double ndh = maxd(dot(n, h), 0);
   double sp = spec_prob(m, ndv), dp = 1.0 - sp;
   if (!(wdn * odn > 0)) return 0;
   double ndl = maxd(wdn, 0);
   if (ndl <= 1e-5) return 0;
-  double alpha = maxd(m->rough * m->rough, ALPHA_FLOOR);
-  double d = ggx_d(ndh, alpha);
-  double pspec = (d * smith_g1(ndv, m->rough)) / maxd(4.0 * ndv, 1e-6);
-  return dp * ndl * INV_PI + sp * pspec;
+  double pspec;
+  if (m->aniso > 0) {
+    d3 t, b;
+    aniso_frame(m, n, &t, &b);
+    pspec = aniso_spec_pdf(m, n, t, b, wo, wi);
+  } else {
+    d3 h = safe_normalize(add(wi, wo));
+    double d = ggx_d(maxd(dot(n, h), 0), maxd(m->rough * m->rough, ALPHA_FLOOR));
+    pspec = (d * smith_g1(ndv, m->rough)) / maxd(4.0 * ndv, 1e-6);
+  }
+  double base = dp * ndl * INV_PI + sp * pspec;
+  if (m->cc <= 0 && m->sheen <= 0) return base;
+  double p[4];
+  ext_probs(m, p);
+  return (p[0] * base + p[1] * cc_pdf(m, n, wo, wi)) + p[2] * sheen_pdf(m, n, wo, wi);
 }
 
 /* ---- camera (kernelCore.wgsl.ts:22-26; common.wgsl.ts:412-469) --------- */
@@ -357,10 +568,6 @@ static double point_attenuation(double dist, double cutoff, double decay) {
   if (decay > 0.01) att = 1.0 / pow(dist, decay);
   if (cutoff > 0) att *= clampd(1.0 - pow(dist / cutoff, 4.0), 0, 1);
   return att;
-}
-static double smoothstep_d(double e0, double e1, double x) {
-  double t = clampd((x - e0) / (e1 - e0), 0, 1);
-  return t * t * (3.0 - 2.0 * t);
 }
 /* concentricDiscSample (kernelCore.wgsl.ts:73-85) */
 static void concentric_disc(double a, double b, double *x, double *y) {
@@ -573,9 +780,10 @@ static d3 shadow_origin(f3_t hp, f3_t n, int upper, float bias) {
 
 static void disc_polar(double u1, double u2, double *a, double *b);
 static double rand01_(void *rng);
-/* sampleGgxVndfTangent (bsdf.wgsl.ts:1710-1735) in f32 */
-static f3_t vndf_sample_f(f3_t wo, float alpha, void *rng) {
-  const f3_t vh = safe_normalize_f(F3f(alpha * wo.x, alpha * wo.y, wo.z));
+/* sampleGgxVndfTangent (bsdf.wgsl.ts:1710-1735) and, with ax != ay,
+ * sampleGgxVndfAnisTangent (:612-637), in f32 */
+static f3_t vndf_sample_f(f3_t wo, float ax, float ay, void *rng) {
+  const f3_t vh = safe_normalize_f(F3f(ax * wo.x, ay * wo.y, wo.z));
   const float lensq = vh.x * vh.x + vh.y * vh.y;
   const float inv = lensq > 1e-10f ? 1.0f / sqrtf(lensq) : 0.0f;
   const f3_t t1 = lensq > 1e-10f ? F3f(-vh.y * inv, vh.x * inv, 0.0f) : F3f(1, 0, 0);
@@ -588,7 +796,94 @@ static f3_t vndf_sample_f(f3_t wo, float alpha, void *rng) {
   const float sw = 0.5f * (1.0f + vh.z);
   b = (1.0f - sw) * sqrtf(maxf_(0.0f, 1.0f - a * a)) + sw * b;
   const f3_t nh = addf3(addf3(sclf3(t1, a), sclf3(t2, b)), sclf3(vh, sqrtf(maxf_(0.0f, (1.0f - a * a) - b * b))));
-  return safe_normalize_f(F3f(alpha * nh.x, alpha * nh.y, maxf_(1e-6f, nh.z)));
+  return safe_normalize_f(F3f(ax * nh.x, ay * nh.y, maxf_(1e-6f, nh.z)));
+}
+
+/* A material from its 24-float table record (see oracle.material_table). */
+static void mat_from(const float *q, mat_t *m) {
+  m->base = D3(q[0], q[1], q[2]);
+  m->rough = q[3];
+  m->metal = q[4];
+  m->emit = D3(q[5], q[6], q[7]);
+  m->two_sided = q[8] > 0.5f;
+  m->cc = q[9];
+  m->cc_rough = q[10];
+  m->sheen = q[11];
+  m->sheen_rough = q[12];
+  m->sheen_color = D3(q[13], q[14], q[15]);
+  m->aniso = q[16];
+  m->aniso_c = q[17];
+  m->aniso_s = q[18];
+  m->spec_color = D3(q[19], q[20], q[21]);
+  m->spec_int = q[22];
+}
+
+/* The opaque branch of sampleNextBounceDirectionWithClearcoatNormal
+ * (bsdf.wgsl.ts:2920-2997): the base lobes or an extension lobe (clearcoat,
+ * sheen) by their represented probabilities, then the direction in f32
+ * (glossyReflectionSample and its anisotropic form, the cosine lobe,
+ * charlieSheenSample; bsdf.wgsl.ts:640-691, 1648-1778): it is the next ray's
+ * direction.  sp: the specular probability at the vertex. */
+static f3_t bsdf_sample_f32(const mat_t *m, f3_t n32, f3_t wo32, double sp, orng *rng, unsigned *lobe_out) {
+  double ep[4];
+  ext_probs(m, ep);
+  f3_t tb, bb, wi32;
+  build_onb_f(n32, &tb, &bb);
+  const float ac = (float)m->aniso_c, as = (float)m->aniso_s;
+  if (m->aniso > 0) {
+    const f3_t t0 = tb, b0 = bb;
+    tb = addf3(sclf3(t0, ac), sclf3(b0, as));
+    bb = addf3(sclf3(t0, -as), sclf3(b0, ac));
+  }
+  const f3_t inc = F3f(-wo32.x, -wo32.y, -wo32.z);
+  unsigned lobe; /* 0 diffuse, 1 specular, 2 clearcoat, 3 sheen */
+  if (rand01(rng) < ep[0]) {
+    lobe = rand01(rng) < sp ? 1u : 0u;
+    if (lobe == 1u) {
+      const float rough = (float)m->rough, alpha = maxf_(rough * rough, 1e-3f);
+      float ax = alpha, ay = alpha;
+      if (m->aniso > 0) {
+        const float aspect = sqrtf(maxf_(1.0f - 0.9f * (float)m->aniso, 1e-4f));
+        ax = maxf_(alpha / aspect, 1e-4f);
+        ay = maxf_(alpha * aspect, 1e-4f);
+      }
+      const f3_t wl = F3f(dotf3(wo32, tb), dotf3(wo32, bb), dotf3(wo32, n32));
+      const f3_t hl = vndf_sample_f(wl, ax, ay, rng);
+      const f3_t hw = safe_normalize_f(addf3(addf3(sclf3(tb, hl.x), sclf3(bb, hl.y)), sclf3(n32, hl.z)));
+      wi32 = safe_normalize_f(subf3(inc, sclf3(hw, 2.0f * dotf3(hw, inc))));
+    } else {
+      f3_t tc, bc;
+      build_onb_f(n32, &tc, &bc);
+      const double u1 = rand01(rng), u2 = rand01(rng);
+      double a, b;
+      disc_polar(u1, u2, &a, &b);
+      const float ct = sqrtf(maxf_(0.0f, 1.0f - (float)u1));
+      wi32 = safe_normalize_f(addf3(addf3(sclf3(tc, (float)a), sclf3(bc, (float)b)), sclf3(n32, ct)));
+    }
+  } else if (rand01(rng) < ep[3]) {
+    lobe = 2u;
+    f3_t tc, bc;
+    build_onb_f(n32, &tc, &bc);
+    const float cr = (float)m->cc_rough, alpha = maxf_(cr * cr, 1e-3f);
+    const f3_t wl = F3f(dotf3(wo32, tc), dotf3(wo32, bc), dotf3(wo32, n32));
+    const f3_t hl = vndf_sample_f(wl, alpha, alpha, rng);
+    const f3_t hw = safe_normalize_f(addf3(addf3(sclf3(tc, hl.x), sclf3(bc, hl.y)), sclf3(n32, hl.z)));
+    wi32 = safe_normalize_f(subf3(inc, sclf3(hw, 2.0f * dotf3(hw, inc))));
+  } else {
+    lobe = 3u;
+    const double u1 = rand01(rng), u2 = rand01(rng);
+    const float sr = (float)m->sheen_rough, alpha = maxf_(sr * sr, 1e-3f);
+    const float inv = 1.0f / maxf_(alpha, 1e-4f);
+    const float sin_h = (float)pow(u1, 1.0 / (double)(inv + 2.0f));
+    const float cos_h = sqrtf(maxf_(0.0f, 1.0f - sin_h * sin_h));
+    double cd, sd;
+    disc_polar(1.0, u2, &cd, &sd);
+    const f3_t hw = safe_normalize_f(addf3(addf3(sclf3(tb, sin_h * (float)cd), sclf3(bb, sin_h * (float)sd)),
+                                           sclf3(n32, cos_h)));
+    wi32 = safe_normalize_f(subf3(inc, sclf3(hw, 2.0f * dotf3(hw, inc))));
+  }
+  *lobe_out = lobe;
+  return wi32;
 }
 
 typedef struct {
@@ -911,32 +1206,17 @@ static void render_path(const rctx *c, uint32_t px, uint32_t py, uint32_t frame_
         }
       }
     }
-    /* BSDF sample, opaque branch (bsdf.wgsl.ts:2920-2997). */
+    /* BSDF sample, opaque branch (bsdf.wgsl.ts:2920-2997): base lobes or
+     * an extension lobe (clearcoat, sheen) by their represented
+     * probabilities; the sampled direction in f32 (glossyReflectionSample
+     * and its anisotropic form, the cosine lobe, charlieSheenSample,
+     * bsdf.wgsl.ts:640-691, 1648-1778): it is the next ray's direction. */
     double ndv = maxd(dot(n, wo), 0);
     double sp = spec_prob(m, ndv);
-    (void)rand01(&rng); /* extension-lobe draw: P(base) = 1 */
-    int glossy = rand01(&rng) < sp;
-    /* the sampled direction in f32 (glossyReflectionSample / the cosine
-     * lobe, bsdf.wgsl.ts:1648-1778): it is the next ray's direction */
-    f3_t tb, bb, wi32;
-    build_onb_f(n32, &tb, &bb);
-    const f3_t wo32 = F3f(-(float)dir.x, -(float)dir.y, -(float)dir.z);
-    if (glossy) {
-      const float rough = (float)m->rough, alpha = maxf_(rough * rough, 1e-3f);
-      const f3_t wl = F3f(dotf3(wo32, tb), dotf3(wo32, bb), dotf3(wo32, n32));
-      const f3_t hl = vndf_sample_f(wl, alpha, &rng);
-      const f3_t hw = safe_normalize_f(addf3(addf3(sclf3(tb, hl.x), sclf3(bb, hl.y)), sclf3(n32, hl.z)));
-      const f3_t inc = F3f(-wo32.x, -wo32.y, -wo32.z);
-      wi32 = safe_normalize_f(subf3(inc, sclf3(hw, 2.0f * dotf3(hw, inc))));
-    } else {
-      const double u1 = rand01(&rng), u2 = rand01(&rng);
-      double a, b;
-      disc_polar(u1, u2, &a, &b);
-      const float ct = sqrtf(maxf_(0.0f, 1.0f - (float)u1));
-      wi32 = safe_normalize_f(addf3(addf3(sclf3(tb, (float)a), sclf3(bb, (float)b)), sclf3(n32, ct)));
-    }
+    unsigned lobe;
+    const f3_t wi32 = bsdf_sample_f32(m, n32, F3f(-(float)dir.x, -(float)dir.y, -(float)dir.z), sp, &rng, &lobe);
     d3 wi = D3f(wi32);
-    sig = mixlog(sig, glossy ? 0xB1 : 0xB0, ev);
+    sig = mixlog(sig, 0xB0 + lobe, ev);
     /* finalizeFiniteBounceSample (bsdf.wgsl.ts:1863-1926) */
     double mpdf = bsdf_pdf(m, n, wo, wi);
     d3 f = bsdf_eval(m, n, wo, wi);
@@ -1073,12 +1353,7 @@ int orc_render(const orc_render_desc *d, uint32_t frame_index, uint32_t x0, uint
   c.d = d;
   c.mats = (mat_t *)calloc(d->material_count + 1, sizeof(mat_t));
   for (uint32_t i = 0; i < d->material_count; ++i) {
-    const float *q = d->materials + 12 * (size_t)i;
-    c.mats[i].base = D3(q[0], q[1], q[2]);
-    c.mats[i].rough = q[3];
-    c.mats[i].metal = q[4];
-    c.mats[i].emit = D3(q[5], q[6], q[7]);
-    c.mats[i].two_sided = q[8] > 0.5f;
+    mat_from(d->materials + 24 * (size_t)i, &c.mats[i]);
   }
   /* Emitters: the record list is in light-selection order; a sorted copy
    * (by instance, triangle) serves hit lookups. */
@@ -1140,4 +1415,33 @@ int orc_render(const orc_render_desc *d, uint32_t frame_index, uint32_t x0, uint
   (void)find_emitter;
   (void)mind;
   return 0;
+}
+
+/* ---- BSDF test hooks ------------------------------------------------------
+ * Over a 24-float material record: the evaluation (the kernel without N.L),
+ * the directional density, and one sampled direction from a PCG stream
+ * (the state advances). */
+void orc_bsdf_eval(const float *mat, const double *n, const double *wo, const double *wi, double *out) {
+  mat_t m;
+  mat_from(mat, &m);
+  const d3 f = bsdf_eval(&m, D3(n[0], n[1], n[2]), D3(wo[0], wo[1], wo[2]), D3(wi[0], wi[1], wi[2]));
+  out[0] = f.x, out[1] = f.y, out[2] = f.z;
+}
+double orc_bsdf_pdf(const float *mat, const double *n, const double *wo, const double *wi) {
+  mat_t m;
+  mat_from(mat, &m);
+  return bsdf_pdf(&m, D3(n[0], n[1], n[2]), D3(wo[0], wo[1], wo[2]), D3(wi[0], wi[1], wi[2]));
+}
+void orc_bsdf_sample(const float *mat, const float *n, const float *wo, uint32_t *pcg_state, float *wi,
+                     uint32_t *lobe) {
+  mat_t m;
+  mat_from(mat, &m);
+  orng r = {*pcg_state, ORC_PCG_ONLY, 0, 0, 0};
+  const d3 nd = D3(n[0], n[1], n[2]);
+  const double sp = spec_prob(&m, maxd(dot(nd, D3(wo[0], wo[1], wo[2])), 0));
+  unsigned lb = 0;
+  const f3_t w = bsdf_sample_f32(&m, F3f(n[0], n[1], n[2]), F3f(wo[0], wo[1], wo[2]), sp, &r, &lb);
+  wi[0] = w.x, wi[1] = w.y, wi[2] = w.z;
+  *lobe = lb;
+  *pcg_state = r.s;
 }

@@ -23,6 +23,10 @@ class OracleError(RuntimeError):
     pass
 
 
+class RangeError(OracleError):
+    """A rejection the reference raises as RangeError."""
+
+
 class Primitive(C.Structure):
     _fields_ = [
         ("kind", C.c_uint32),
@@ -178,6 +182,121 @@ def invert_mat4(m):
     out = np.zeros(16, np.float32)
     ok = lib().orc_invert_mat4(_p(m, F32P), _p(out, F32P))
     return out if ok else None
+
+
+def refit_bvh_bounds(nodes, indices, positions, *, position_stride=4, index_stride=4):
+    """refitBvhBounds (packages/shared-bvh/src/refitBvhBounds.ts:80-179) of the
+    tree rooted at node 0 of `nodes`; returns the refitted copy."""
+    out = np.array(nodes, dtype=np.uint32).ravel().copy()
+    idx = np.ascontiguousarray(indices, dtype=np.uint32).ravel()
+    pos = np.ascontiguousarray(positions, dtype=np.float32).ravel()
+    rc = lib().orc_refit_bvh_bounds(_p(out, U32P), C.c_uint32(out.size // 8), _p(idx, U32P),
+                                    C.c_uint32(idx.size), C.c_uint32(index_stride), _p(pos, F32P),
+                                    C.c_uint32(pos.size), C.c_uint32(position_stride))
+    if rc == -2:
+        raise RangeError(_err())
+    if rc != 0:
+        raise OracleError(_err())
+    return out
+
+
+def refit_tlas_instances(nodes, instance_indices, aabb_min, aabb_max, changed):
+    """refitTlasInstances (packages/shared-bvh/src/tlas.ts:1141-1209);
+    returns (refitted copy, affected node indices children-first)."""
+    out = np.array(nodes, dtype=np.uint32).ravel().copy()
+    perm = np.ascontiguousarray(instance_indices, dtype=np.uint32)
+    mn = np.ascontiguousarray(aabb_min, dtype=np.float64).reshape(-1, 3)
+    mx = np.ascontiguousarray(aabb_max, dtype=np.float64).reshape(-1, 3)
+    ch = np.ascontiguousarray(changed, dtype=np.uint32)
+    aff = np.zeros(max(out.size // 8, 1), np.uint32)
+    n = C.c_uint32(0)
+    rc = lib().orc_refit_tlas_instances(_p(out, U32P), C.c_uint32(out.size // 8), _p(perm, U32P),
+                                        C.c_uint32(perm.size), C.c_uint32(mn.shape[0]), _p(mn, F64P),
+                                        _p(mx, F64P), _p(ch, U32P), C.c_uint32(ch.size), _p(aff, U32P),
+                                        C.byref(n))
+    if rc != 0:
+        raise OracleError(_err())
+    return out, aff[: n.value].copy()
+
+
+def js_min(a, b):
+    """Math.min of two non-NaN numbers (a negative zero wins a tie)."""
+    if a < b or b < a:
+        return a if a < b else b
+    return -0.0 if (a == 0 and (math.copysign(1, a) < 0 or math.copysign(1, b) < 0)) else a
+
+
+def js_max(a, b):
+    if a > b or b > a:
+        return a if a > b else b
+    return 0.0 if (a == 0 and not (math.copysign(1, a) < 0 and math.copysign(1, b) < 0)) else a
+
+
+def local_aabb(positions4):
+    """computeLocalAabb (scenePack.ts:172-195) of stride-4 positions, binary64."""
+    mn, mx = [math.inf] * 3, [-math.inf] * 3
+    for v in np.asarray(positions4, np.float32).reshape(-1, 4)[:, :3].astype(np.float64).tolist():
+        for k in range(3):
+            mn[k], mx[k] = js_min(mn[k], v[k]), js_max(mx[k], v[k])
+    return mn, mx
+
+
+def transform_aabb(l2w, mn, mx):
+    """transformAabb (scenePack.ts:197-223): the 8 corners through the binary32
+    column-major matrix in binary64 (transformPoint, :161-170)."""
+    m = [float(x) for x in np.asarray(l2w, np.float32).ravel()]
+    wmn, wmx = [math.inf] * 3, [-math.inf] * 3
+    for c in range(8):
+        p = [mx[0] if c & 1 else mn[0], mx[1] if c & 2 else mn[1], mx[2] if c & 4 else mn[2]]
+        t = [m[r] * p[0] + m[4 + r] * p[1] + m[8 + r] * p[2] + m[12 + r] for r in range(4)]
+        q = [t[0] / t[3], t[1] / t[3], t[2] / t[3]] if abs(t[3]) > 1e-8 else t[:3]
+        for k in range(3):
+            wmn[k], wmx[k] = js_min(wmn[k], q[k]), js_max(wmx[k], q[k])
+    return wmn, wmx
+
+
+def refit_primitive(pack, primitive, positions):
+    """A packed scene after one mesh primitive's vertices move with its
+    topology kept, as the reference's refit path does it
+    (walkaround-hybrid HybridEnginePrimitiveUpdates.ts:1482-1560): the new
+    positions replace the primitive's vertex slice, refitBvhBounds refits its
+    BLAS, and refitTlasTransforms with localBoundsChanged (scenePack.ts:
+    1310-1420) refits the TLAS over the primitive's instances from their new
+    world boxes.  Returns (new pack, world_min, world_max) with the world
+    bounds the binary64 union of every instance's box."""
+    out = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in pack.items()}
+    b = out["bindings"].reshape(-1, 8)
+    rows = np.nonzero(b[:, 6] == primitive)[0]
+    if rows.size != 1:
+        raise OracleError(f"primitive {primitive} has no packed geometry")
+    r = int(rows[0])
+    root, _, vstart, vcount = (int(x) for x in b[r, :4])
+    new = np.asarray(positions, np.float32).reshape(-1, 3)
+    if new.shape[0] != vcount:
+        raise OracleError("vertex count differs")
+    pos = out["positions"].reshape(-1, 4)
+    pos[vstart:vstart + vcount, :3] = new
+    roots = sorted(int(x) for x in b[:, 0])
+    total = out["bvh_nodes"].size // 8
+    end = next((x for x in roots if x > root), total)
+    nodes = out["bvh_nodes"].reshape(-1, 8)
+    nodes[root:end] = refit_bvh_bounds(nodes[root:end], out["indices"], out["positions"]).reshape(-1, 8)
+    lmn, lmx = local_aabb(pos[vstart:vstart + vcount])
+    out["binding_aabb"].reshape(-1, 6)[r] = np.array(lmn + lmx, np.float32)
+    inst_prim = out["instance_primitive"]
+    n_inst = inst_prim.size
+    boxes = {}
+    for row in range(b.shape[0]):
+        p, vs, vc = int(b[row, 6]), int(b[row, 2]), int(b[row, 3])
+        boxes[p] = local_aabb(pos[vs:vs + vc])
+    amn = np.zeros((n_inst, 3)); amx = np.zeros((n_inst, 3))
+    l2w = out["tlas_l2w"].reshape(-1, 16)
+    for i in range(n_inst):
+        wmn, wmx = transform_aabb(l2w[i], *boxes[int(inst_prim[i])])
+        amn[i], amx[i] = wmn, wmx
+    changed = np.nonzero(inst_prim == primitive)[0]
+    out["tlas_nodes"], _ = refit_tlas_instances(out["tlas_nodes"], out["tlas_instance_indices"], amn, amx, changed)
+    return out, amn.min(axis=0), amx.max(axis=0)
 
 
 def pack_scene(primitives):
@@ -571,9 +690,14 @@ class RenderDesc(C.Structure):
 
 
 def material_table(materials):
-    """12 floats per material: base rgb, roughness, metallic, Le rgb
-    (emissive x emissiveIntensity), doubleSided, 3 pad."""
-    out = np.zeros((max(1, len(materials)), 12), np.float32)
+    """24 floats per material: base rgb, roughness, metallic, Le rgb
+    (emissive x emissiveIntensity), doubleSided, then the extension lobes as
+    materialPacking.ts:259-316 packs them (clamped to [0, 1]): clearcoat,
+    clearcoatRoughness, sheen, sheenRoughness, sheenColor rgb, anisotropy, the
+    cosine and sine of anisotropyRotation (binary64, rounded once), then
+    specularColor rgb (>= 0), specularIntensity, 1 pad."""
+    out = np.zeros((max(1, len(materials)), 24), np.float32)
+    c01 = lambda x: min(1.0, max(0.0, float(np.float32(x))))
     for i, m in enumerate(materials):
         # the engine receives f32 emissive and intensity; Le is their binary64
         # product rounded once to f32
@@ -584,7 +708,71 @@ def material_table(materials):
         out[i, 4] = m.get("metallic", 0.0)
         out[i, 5:8] = (em * inten).astype(np.float32)
         out[i, 8] = 1.0 if m.get("doubleSided", False) else 0.0
+        out[i, 9] = c01(m.get("clearcoat", 0.0))
+        out[i, 10] = c01(m.get("clearcoatRoughness", 0.0))
+        out[i, 11] = c01(m.get("sheen", 0.0))
+        out[i, 12] = c01(m.get("sheenRoughness", 0.0))
+        out[i, 13:16] = [c01(c) for c in m.get("sheenColor", [0.0, 0.0, 0.0])]
+        out[i, 16] = c01(m.get("anisotropy", 0.0))
+        rot = float(np.float32(m.get("anisotropyRotation", 0.0)))
+        out[i, 17], out[i, 18] = math.cos(rot), math.sin(rot)
+        out[i, 19:22] = [max(0.0, float(np.float32(c))) for c in m.get("specularColor", [1.0, 1.0, 1.0])]
+        out[i, 22] = c01(m.get("specularIntensity", 1.0))
+    if not materials:
+        out[0, 0:4] = [0.8, 0.8, 0.8, 1.0]
+        out[0, 17] = 1.0
+        out[0, 19:23] = 1.0
     return out
+
+
+def _bsdf_lib():
+    L = lib()
+    if not getattr(L, "_bsdf_ready", False):
+        D3P = C.POINTER(C.c_double)
+        L.orc_bsdf_eval.argtypes = [F32P, D3P, D3P, D3P, D3P]
+        L.orc_bsdf_eval.restype = None
+        L.orc_bsdf_pdf.argtypes = [F32P, D3P, D3P, D3P]
+        L.orc_bsdf_pdf.restype = C.c_double
+        L.orc_bsdf_sample.argtypes = [F32P, F32P, F32P, U32P, F32P, U32P]
+        L.orc_bsdf_sample.restype = None
+        L._bsdf_ready = True
+    return L
+
+
+def bsdf_eval(mat_row, n, wo, wi):
+    """evaluateBrdfFullWithClearcoatNormal (the kernel without N.L) of one
+    material_table row."""
+    q = np.ascontiguousarray(mat_row, np.float32)
+    v = [np.ascontiguousarray(x, np.float64) for x in (n, wo, wi)]
+    out = np.zeros(3)
+    _bsdf_lib().orc_bsdf_eval(_p(q, F32P), *[_p(x, F64P) for x in v], _p(out, F64P))
+    return out
+
+
+def bsdf_pdf(mat_row, n, wo, wi):
+    """brdfDirectionalPdfFullWithClearcoatNormal of one material_table row."""
+    q = np.ascontiguousarray(mat_row, np.float32)
+    v = [np.ascontiguousarray(x, np.float64) for x in (n, wo, wi)]
+    return _bsdf_lib().orc_bsdf_pdf(_p(q, F32P), *[_p(x, F64P) for x in v])
+
+
+def bsdf_sample(mat_row, n, wo, count, seed=1):
+    """`count` sampled directions (f32) and lobes (0 diffuse, 1 specular,
+    2 clearcoat, 3 sheen) from one PCG stream."""
+    q = np.ascontiguousarray(mat_row, np.float32)
+    nf = np.ascontiguousarray(n, np.float32)
+    wf = np.ascontiguousarray(wo, np.float32)
+    st = np.array([seed], np.uint32)
+    wi = np.zeros((count, 3), np.float32)
+    lobe = np.zeros(count, np.uint32)
+    L = _bsdf_lib()
+    w = np.zeros(3, np.float32)
+    lb = np.zeros(1, np.uint32)
+    for k in range(count):
+        L.orc_bsdf_sample(_p(q, F32P), _p(nf, F32P), _p(wf, F32P), _p(st, U32P), _p(w, F32P), _p(lb, U32P))
+        wi[k] = w
+        lobe[k] = lb[0]
+    return wi, lobe
 
 
 def build_emitters(pack, mat_table):

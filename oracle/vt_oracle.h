@@ -62,6 +62,23 @@ int orc_build_tlas(uint32_t n, const uint32_t *blas_ids, const double *aabb_min,
 /* ---- invertMat4: packages/shared-bvh/src/mathUtils.ts:21 (1 = invertible) */
 int orc_invert_mat4(const float *m, float *out);
 
+/* ---- refitBvhBounds: packages/shared-bvh/src/refitBvhBounds.ts:3-179 -----
+ * In place over the tree rooted at node 0 of `node_words` (pass a view at a
+ * BLAS root to refit one tree of a forest; leaf triangle offsets index
+ * `indices`).  -2: a stride other than 3 or 4; -1: rejected topology or
+ * geometry (nothing is written).                                            */
+int orc_refit_bvh_bounds(uint32_t *node_words, uint32_t node_count, const uint32_t *indices,
+                         uint32_t index_words, uint32_t index_stride, const float *positions,
+                         uint32_t position_floats, uint32_t position_stride);
+
+/* ---- refitTlasInstances: packages/shared-bvh/src/tlas.ts:1012-1209 -------
+ * aabb_min / aabb_max: binary64, 3 per original instance.  out_affected
+ * (capacity node_count) receives the refitted nodes, children first.        */
+int orc_refit_tlas_instances(uint32_t *node_words, uint32_t node_count, const uint32_t *perm,
+                             uint32_t perm_count, uint32_t instance_count, const double *aabb_min,
+                             const double *aabb_max, const uint32_t *changed, uint32_t changed_count,
+                             uint32_t *out_affected, uint32_t *out_affected_count);
+
 /* ---- packSceneFromCore: packages/shared-bvh/src/scenePack.ts:1104 -------
  * A primitive is a mesh (kind 0, transform_count 0 or 1; NULL transforms =
  * identity) or an instanced mesh (kind 1, transform_count instances).       */

@@ -749,12 +749,7 @@ void build_slice(const PrimitiveIn& p, Slice& s) {
       lidx[4 * (size_t)t + k] = p.indices ? p.indices[3 * (size_t)t + k] : 3 * t + k;
   s.err = build_blas(s.lpos.data(), vcount * 4, 4, lidx.data(), tcount * 4, 4, lmat.data(), 4, 16,
                      kMaxBuildDepth, s.blas);
-  for (int k = 0; k < 3; ++k) s.lmin[k] = INFINITY, s.lmax[k] = -INFINITY;
-  for (uint32_t i = 0; i < vcount; ++i)
-    for (int k = 0; k < 3; ++k) {
-      s.lmin[k] = jmin(s.lmin[k], s.lpos[4 * (size_t)i + k]);
-      s.lmax[k] = jmax(s.lmax[k], s.lpos[4 * (size_t)i + k]);
-    }
+  local_aabb(s.lpos.data(), vcount, s.lmin, s.lmax);
   s.present = true;
 }
 
@@ -770,6 +765,88 @@ void xform_point(const float* m, const double* p, double* q) {
 }
 
 }  // namespace
+
+void local_aabb(const float* pos4, uint32_t count, double* mn, double* mx) {
+  for (int k = 0; k < 3; ++k) mn[k] = INFINITY, mx[k] = -INFINITY;
+  for (uint32_t i = 0; i < count; ++i)
+    for (int k = 0; k < 3; ++k) {
+      mn[k] = jmin(mn[k], pos4[4 * (size_t)i + k]);
+      mx[k] = jmax(mx[k], pos4[4 * (size_t)i + k]);
+    }
+}
+
+void world_aabb(const float* l2w, const double* lmn, const double* lmx, double* mn, double* mx) {
+  for (int k = 0; k < 3; ++k) mn[k] = INFINITY, mx[k] = -INFINITY;
+  for (int c = 0; c < 8; ++c) {
+    const double corner[3] = {(c & 1) ? lmx[0] : lmn[0], (c & 2) ? lmx[1] : lmn[1], (c & 4) ? lmx[2] : lmn[2]};
+    double q[3];
+    xform_point(l2w, corner, q);
+    for (int k = 0; k < 3; ++k) mn[k] = jmin(mn[k], q[k]), mx[k] = jmax(mx[k], q[k]);
+  }
+}
+
+BuildError refit_tlas_instances(std::vector<uint32_t>& nodes, const std::vector<uint32_t>& perm,
+                                uint32_t inst_count, const double* aabb_min, const double* aabb_max,
+                                const std::vector<uint32_t>& changed) {
+  auto fail = [](int code, const std::string& m) { return BuildError{code, m}; };
+  for (uint32_t i : perm) {
+    if (i >= inst_count) return fail(VT_ERR_BUILD, "refitTlas: instance reference out of range");
+    const double* a = aabb_min + 3 * (size_t)i;
+    const double* b = aabb_max + 3 * (size_t)i;
+    if (b[0] < a[0] || b[1] < a[1] || b[2] < a[2])
+      return fail(VT_ERR_BUILD, "refitTlas: instance " + std::to_string(i) + " has inverted AABB.");
+    for (int k = 0; k < 3; ++k)
+      if (!std::isfinite(a[k]) || !std::isfinite(b[k]))
+        return fail(VT_ERR_RANGE, "refitTlas: instance " + std::to_string(i) + " has non-finite AABB.");
+  }
+  if (changed.empty()) return {};
+  const uint32_t n = (uint32_t)(nodes.size() / 8);
+  std::vector<uint8_t> hit(inst_count, 0), affected(n, 0);
+  for (uint32_t i : changed) {
+    if (i >= inst_count) return fail(VT_ERR_RANGE, "refitTlas: changed instance out of range");
+    hit[i] = 1;
+  }
+  // Preorder: children follow their parent, so a parent array built in one
+  // pass lets every marked leaf flag its ancestors.
+  std::vector<uint32_t> parent(n, UINT32_MAX);
+  for (uint32_t node = 0; node < n; ++node) {
+    const uint32_t* q = &nodes[8 * (size_t)node];
+    if ((q[7] >> 16) == 0xFFFFu) {
+      const uint32_t off = q[6], cnt = q[7] & 0xFFFFu;
+      bool touched = false;
+      for (uint32_t k = 0; k < cnt && off + k < perm.size(); ++k) touched = touched || hit[perm[off + k]];
+      for (uint32_t c = node; touched && c != UINT32_MAX && !affected[c]; c = parent[c]) affected[c] = 1;
+      continue;
+    }
+    parent[node + 1] = node;
+    parent[node + q[6]] = node;
+  }
+  // Children carry higher indices than their parent: walking down from the
+  // last node refits every child before its parent.
+  for (uint32_t node = n; node-- > 0;) {
+    if (!affected[node]) continue;
+    uint32_t* q = &nodes[8 * (size_t)node];
+    if ((q[7] >> 16) == 0xFFFFu) {
+      double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+      for (uint32_t k = 0; k < (q[7] & 0xFFFFu); ++k) {
+        const uint32_t orig = perm[q[6] + k];
+        for (int a = 0; a < 3; ++a) {
+          if (aabb_min[3 * (size_t)orig + a] < mn[a]) mn[a] = aabb_min[3 * (size_t)orig + a];
+          if (aabb_max[3 * (size_t)orig + a] > mx[a]) mx[a] = aabb_max[3 * (size_t)orig + a];
+        }
+      }
+      for (int a = 0; a < 3; ++a) q[a] = fbits((float)mn[a]), q[3 + a] = fbits((float)mx[a]);
+    } else {
+      const uint32_t* l = &nodes[8 * ((size_t)node + 1)];
+      const uint32_t* r = &nodes[8 * ((size_t)node + q[6])];
+      for (int a = 0; a < 3; ++a) {
+        q[a] = fbits(jminf(bitsf(l[a]), bitsf(r[a])));
+        q[3 + a] = fbits(jmaxf(bitsf(l[3 + a]), bitsf(r[3 + a])));
+      }
+    }
+  }
+  return {};
+}
 
 BuildError pack_scene(const PrimitiveIn* prims, uint32_t prim_count, uint32_t threads, Pack& out) {
   out = Pack();
@@ -818,14 +895,8 @@ BuildError pack_scene(const PrimitiveIn* prims, uint32_t prim_count, uint32_t th
                                " has non-invertible instance transform; skipping this instance.");
         continue;
       }
-      double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-      for (int c = 0; c < 8; ++c) {
-        const double corner[3] = {(c & 1) ? s.lmax[0] : s.lmin[0], (c & 2) ? s.lmax[1] : s.lmin[1],
-                                  (c & 4) ? s.lmax[2] : s.lmin[2]};
-        double q[3];
-        xform_point(l2w, corner, q);
-        for (int k = 0; k < 3; ++k) mn[k] = jmin(mn[k], q[k]), mx[k] = jmax(mx[k], q[k]);
-      }
+      double mn[3], mx[3];
+      world_aabb(l2w, s.lmin, s.lmax, mn, mx);
       for (int k = 0; k < 3; ++k) {
         inst_mn.push_back(mn[k]);
         inst_mx.push_back(mx[k]);

@@ -91,4 +91,20 @@ struct Pack {
 
 BuildError pack_scene(const PrimitiveIn* prims, uint32_t prim_count, uint32_t threads, Pack& out);
 
+// computeLocalAabb (scenePack.ts:172-195) of `count` stride-4 vertices, in
+// binary64 with Math.min / Math.max.
+void local_aabb(const float* pos4, uint32_t count, double* mn, double* mx);
+// transformAabb (scenePack.ts:197-223): the 8 corners of a local box through a
+// column-major local-to-world matrix, binary64.
+void world_aabb(const float* l2w, const double* lmn, const double* lmx, double* mn, double* mx);
+
+// refitTlasInstances (tlas.ts:1141-1209) in place over reference TLAS nodes:
+// the leaves holding a changed instance take its binary64 box (rounded to
+// binary32), their ancestors the union of their children.  aabb_min/max hold
+// 3 doubles per original instance; every instance's box must be ordered and
+// finite (validateTlasRefitAabbs, tlas.ts:1012-1039).
+BuildError refit_tlas_instances(std::vector<uint32_t>& nodes, const std::vector<uint32_t>& perm,
+                                uint32_t inst_count, const double* aabb_min, const double* aabb_max,
+                                const std::vector<uint32_t>& changed);
+
 }  // namespace vt

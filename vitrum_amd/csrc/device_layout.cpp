@@ -18,9 +18,11 @@ inline uint32_t fbits(float f) {
 }
 
 // Converts one preorder 32-byte node forest (roots given) into Node64s.
-// leaf_offset_limit bounds the offsets a leaf ref can carry.
+// src64, when given, receives the reference node behind each Node64 child
+// slot (2 per Node64, kNoNode for an empty slot).
 std::string convert_forest(const std::vector<uint32_t>& ref_nodes, const std::vector<uint32_t>& roots,
-                           std::vector<uint32_t>& out, std::vector<uint32_t>& wrapper_of_root) {
+                           std::vector<uint32_t>& out, std::vector<uint32_t>& wrapper_of_root,
+                           std::vector<uint32_t>* src64 = nullptr) {
   const uint32_t count = (uint32_t)(ref_nodes.size() / 8);
   std::vector<uint32_t> map(count, ~0u);
   uint32_t next = 0;
@@ -28,6 +30,7 @@ std::string convert_forest(const std::vector<uint32_t>& ref_nodes, const std::ve
     if ((ref_nodes[8 * (size_t)i + 7] >> 16) != 0xFFFFu) map[i] = next++;
   const uint32_t interior = next;
   out.assign((size_t)(interior + roots.size()) * kNode64Words, 0u);
+  if (src64) src64->assign((size_t)(interior + roots.size()) * 2, kNoNode);
   std::string err;
   auto child_ref = [&](uint32_t j) -> uint32_t {
     const uint32_t* q = &ref_nodes[8 * (size_t)j];
@@ -56,6 +59,7 @@ std::string convert_forest(const std::vector<uint32_t>& ref_nodes, const std::ve
     put_box(w + 6, r);
     w[12] = child_ref(l);
     w[13] = child_ref(r);
+    if (src64) (*src64)[2 * (size_t)map[i]] = l, (*src64)[2 * (size_t)map[i] + 1] = r;
   }
   wrapper_of_root.resize(roots.size());
   for (size_t k = 0; k < roots.size(); ++k) {
@@ -65,6 +69,7 @@ std::string convert_forest(const std::vector<uint32_t>& ref_nodes, const std::ve
     put_empty_box(w + 6);
     w[12] = child_ref(roots[k]);
     w[13] = kEmptyRef;
+    if (src64) (*src64)[2 * (size_t)idx] = roots[k];
     wrapper_of_root[k] = idx;
   }
   return err;
@@ -94,6 +99,7 @@ namespace {
 struct Slot {
   uint32_t ref;
   float box[6];
+  uint32_t src;  // reference node of the box (kNoNode when not tracked)
 };
 
 inline float area(const float* b) {
@@ -105,10 +111,14 @@ inline bool inside(const float* c, const float* p) {
   return c[0] >= p[0] && c[1] >= p[1] && c[2] >= p[2] && c[3] <= p[3] && c[4] <= p[4] && c[5] <= p[5];
 }
 
-// Collapses a Node64 forest into Node128s (see device_layout.h).
+// Collapses a Node64 forest into Node128s (see device_layout.h).  With
+// src64 (the reference node of each Node64 child slot) it records in dst128
+// the Node128 slot that holds each reference node's box.
 class Collapser {
  public:
-  Collapser(const std::vector<uint32_t>& n64, std::vector<uint32_t>& out) : n64_(n64), out_(out) {}
+  Collapser(const std::vector<uint32_t>& n64, std::vector<uint32_t>& out,
+            const std::vector<uint32_t>* src64 = nullptr, std::vector<uint32_t>* dst128 = nullptr)
+      : n64_(n64), out_(out), src64_(src64), dst128_(dst128) {}
 
   // Node128 index of the tree under Node64 `node`; `need` receives the
   // worst-case stack entries of a depth-first walk of that tree.
@@ -154,6 +164,8 @@ class Collapser {
       uint32_t* w = &out_[(size_t)idx * kNode128Words];
       for (int a = 0; a < 6; ++a) w[kWideKids * a + k] = fbits(box[a]);
       w[6 * kWideKids + k] = ref;
+      if (dst128_ && k < slots.size() && slots[k].src != kNoNode)
+        (*dst128_)[slots[k].src] = idx * kWideKids + (uint32_t)k;
     }
     // Expanding this node pushes at most (slots - 1) entries.
     *need = (slots.empty() ? 0u : (uint32_t)slots.size() - 1u) + deepest;
@@ -169,12 +181,15 @@ class Collapser {
       Slot s;
       s.ref = ref;
       std::memcpy(s.box, w + 6 * c, 24);
+      s.src = src64_ ? (*src64_)[2 * (size_t)node + c] : kNoNode;
       out.push_back(s);
     }
   }
 
   const std::vector<uint32_t>& n64_;
   std::vector<uint32_t>& out_;
+  const std::vector<uint32_t>* src64_;
+  std::vector<uint32_t>* dst128_;
 };
 
 // Worst-case stack entries of a depth-first walk of a Node64 tree: one push
@@ -193,6 +208,30 @@ uint32_t stack_need2(const std::vector<uint32_t>& n64, uint32_t node) {
 
 }  // namespace
 
+std::string build_tlas_layout(const std::vector<uint32_t>& tlas_ref_nodes, uint32_t inst_count, TlasLayout& out) {
+  out = TlasLayout();
+  if (inst_count == 0) return {};
+  std::vector<uint32_t> wrapper;
+  std::string err = convert_forest(tlas_ref_nodes, {0u}, out.nodes, wrapper);
+  if (!err.empty()) return err;
+  out.root = wrapper[0];
+  out.node_count = (uint32_t)(out.nodes.size() / kNode64Words);
+  Collapser ct(out.nodes, out.nodes4);
+  out.root4 = ct.build(out.root, &out.need4);
+  out.need2 = stack_need2(out.nodes, out.root);
+  return {};
+}
+
+void set_tlas_layout(TlasLayout&& t, DeviceLayout& out) {
+  out.tlas_nodes = std::move(t.nodes);
+  out.tlas4_nodes = std::move(t.nodes4);
+  out.tlas_root = t.root;
+  out.tlas4_root = t.root4;
+  out.tlas_node_count = t.node_count;
+  out.stack4_need = t.need4 + out.blas_need4;
+  out.stack2_need = t.need2 + out.blas_need2;
+}
+
 std::string build_device_layout(const Pack& p, DeviceLayout& out) {
   out = DeviceLayout();
   // BLAS forest: one wrapper per distinct root.
@@ -202,19 +241,25 @@ std::string build_device_layout(const Pack& p, DeviceLayout& out) {
     root_slot[b.blas_root] = (uint32_t)roots.size();
     roots.push_back(b.blas_root);
   }
-  std::vector<uint32_t> blas_wrapper;
-  std::string err = convert_forest(p.bvh_nodes, roots, out.blas_nodes, blas_wrapper);
+  std::vector<uint32_t> blas_wrapper, src64;
+  std::string err = convert_forest(p.bvh_nodes, roots, out.blas_nodes, blas_wrapper, &src64);
   if (!err.empty()) return err;
   out.blas_node_count = (uint32_t)(out.blas_nodes.size() / kNode64Words);
+  // Refit maps: parents and the Node64 slot of every reference node.
+  const uint32_t n_ref = (uint32_t)(p.bvh_nodes.size() / 8);
+  out.blas_parent.assign(n_ref, kNoNode);
+  out.blas_dst64.assign(n_ref, kNoNode);
+  out.blas_dst128.assign(n_ref, kNoNode);
+  for (uint32_t i = 0; i < n_ref; ++i) {
+    const uint32_t* q = &p.bvh_nodes[8 * (size_t)i];
+    if ((q[7] >> 16) == 0xFFFFu) continue;
+    out.blas_parent[i + 1] = i;
+    out.blas_parent[i + q[6]] = i;
+  }
+  for (size_t slot = 0; slot < src64.size(); ++slot)
+    if (src64[slot] != kNoNode) out.blas_dst64[src64[slot]] = (uint32_t)slot;
 
   const uint32_t n_inst = (uint32_t)p.tlas_blas_roots.size();
-  if (n_inst > 0) {
-    std::vector<uint32_t> tlas_wrapper;
-    err = convert_forest(p.tlas_nodes, {0u}, out.tlas_nodes, tlas_wrapper);
-    if (!err.empty()) return err;
-    out.tlas_root = tlas_wrapper[0];
-    out.tlas_node_count = (uint32_t)(out.tlas_nodes.size() / kNode64Words);
-  }
   out.inst_count = n_inst;
   out.instances.assign((size_t)n_inst * kInstWords, 0u);
   for (uint32_t s = 0; s < (uint32_t)p.tlas_instance_indices.size(); ++s) {
@@ -230,24 +275,19 @@ std::string build_device_layout(const Pack& p, DeviceLayout& out) {
   }
   // Node128 trees: one per BLAS wrapper, one for the TLAS.
   std::vector<uint32_t> blas4_root(blas_wrapper.size(), 0);
-  uint32_t blas_need4 = 0, blas_need2 = 0;
   {
-    Collapser cb(out.blas_nodes, out.blas4_nodes);
+    Collapser cb(out.blas_nodes, out.blas4_nodes, &src64, &out.blas_dst128);
     for (size_t k = 0; k < blas_wrapper.size(); ++k) {
       uint32_t need = 0;
       blas4_root[k] = cb.build(blas_wrapper[k], &need);
-      blas_need4 = std::max(blas_need4, need);
-      blas_need2 = std::max(blas_need2, stack_need2(out.blas_nodes, blas_wrapper[k]));
+      out.blas_need4 = std::max(out.blas_need4, need);
+      out.blas_need2 = std::max(out.blas_need2, stack_need2(out.blas_nodes, blas_wrapper[k]));
     }
   }
-  uint32_t tlas_need4 = 0, tlas_need2 = 0;
-  if (n_inst > 0) {
-    Collapser ct(out.tlas_nodes, out.tlas4_nodes);
-    out.tlas4_root = ct.build(out.tlas_root, &tlas_need4);
-    tlas_need2 = stack_need2(out.tlas_nodes, out.tlas_root);
-  }
-  out.stack4_need = tlas_need4 + blas_need4;
-  out.stack2_need = tlas_need2 + blas_need2;
+  TlasLayout tl;
+  err = build_tlas_layout(p.tlas_nodes, n_inst, tl);
+  if (!err.empty()) return err;
+  set_tlas_layout(std::move(tl), out);
   for (uint32_t s = 0; s < (uint32_t)p.tlas_instance_indices.size(); ++s) {
     const uint32_t inst = p.tlas_instance_indices[s];
     out.instances[(size_t)s * kInstWords + 34] = blas4_root[root_slot[p.tlas_blas_roots[inst]]];

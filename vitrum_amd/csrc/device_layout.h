@@ -77,7 +77,16 @@ struct DeviceLayout {
   // Worst-case traversal stack entries (TLAS part + deepest BLAS part) of the
   // Node128 trees, and of the Node64 trees.
   uint32_t stack4_need = 0, stack2_need = 0;
+  uint32_t blas_need4 = 0, blas_need2 = 0;
+  // Where each reference BLAS node (the pack's 32-byte preorder nodes) lives
+  // in the derived trees, for refits that recompute its box on the device:
+  // its parent (kNoNode for a root), the Node64 child slot holding its box
+  // (Node64 index * 2 + side), and the Node128 slot holding it (node * 4 +
+  // slot, or kNoNode when the collapse replaced it by its children).
+  std::vector<uint32_t> blas_parent, blas_dst64, blas_dst128;
 };
+
+constexpr uint32_t kNoNode = 0xFFFFFFFFu;
 
 // transformLinearOrientationSign (pt-webgpu intersectionCore.wgsl.ts:96-112)
 // in binary32: true when the scaled 3x3 determinant of a column-major
@@ -87,5 +96,16 @@ bool orientation_reversed(const float* l2w);
 // Returns an empty string on success, else the reason the pack cannot be
 // represented (e.g. a leaf larger than 15 primitives).
 std::string build_device_layout(const Pack& p, DeviceLayout& out);
+
+// The TLAS part of a layout: Node64 and Node128 trees and their roots and
+// stack bounds, derived from the reference TLAS nodes alone (the instance
+// records do not depend on the TLAS boxes).
+struct TlasLayout {
+  std::vector<uint32_t> nodes, nodes4;
+  uint32_t root = 0, root4 = 0, node_count = 0, need4 = 0, need2 = 0;
+};
+std::string build_tlas_layout(const std::vector<uint32_t>& tlas_ref_nodes, uint32_t inst_count, TlasLayout& out);
+// Installs a TLAS part into a layout (stack bounds combined with its BLAS part).
+void set_tlas_layout(TlasLayout&& t, DeviceLayout& out);
 
 }  // namespace vt

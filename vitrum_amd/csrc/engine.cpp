@@ -125,6 +125,9 @@ vt_status engine_upload_trace(vt_engine* e, const vt::DeviceLayout& layout, Scen
   s = s ? s : engine_upload(e, out.blas, layout.blas_nodes.data(), layout.blas_nodes.size() * 4);
   s = s ? s : engine_upload(e, out.tlas, layout.tlas_nodes.data(), layout.tlas_nodes.size() * 4);
   s = s ? s : engine_upload(e, out.blas4, layout.blas4_nodes.data(), layout.blas4_nodes.size() * 4);
+  // A refit can collapse the TLAS differently; its Node128 tree never has
+  // more nodes than the Node64 tree it comes from.
+  s = s ? s : out.tlas4.ensure(e, std::max<size_t>((size_t)layout.tlas_node_count * vt::kNode128Words * 4, 16));
   s = s ? s : engine_upload(e, out.tlas4, layout.tlas4_nodes.data(), layout.tlas4_nodes.size() * 4);
   s = s ? s : engine_upload(e, out.tris, layout.tris.data(), layout.tris.size() * 4);
   s = s ? s : engine_upload(e, out.instances, layout.instances.data(), layout.instances.size() * 4);
@@ -283,6 +286,7 @@ vt_status vt_destroy(vt_engine* eng) {
     eng->tmp_out.release();
     eng->tmp_counts.release();
     engine_release_cwbvh(eng);
+    engine_release_refit(eng);
     for (auto& kv : eng->user_allocs) (void)hipFree(kv);
     engine_release_render(eng);
     engine_release_nrc(eng);
@@ -310,6 +314,12 @@ vt_status vt_set_scene(vt_engine* eng, const vt_scene_desc* scene) {
       // and non-negative (an f32 overflow from finite JavaScript numbers would
       // otherwise reach the light tree as an infinite power).
       const vt_material& m = scene->materials[i];
+      const float ext[] = {m.clearcoat, m.clearcoat_roughness, m.sheen, m.sheen_roughness, m.sheen_color[0],
+                           m.sheen_color[1], m.sheen_color[2], m.anisotropy, m.anisotropy_rotation,
+                           m.specular_color[0], m.specular_color[1], m.specular_color[2], m.specular_intensity};
+      for (float x : ext)
+        if (!std::isfinite(x))
+          return set_err(eng, VT_ERR_RANGE, "material " + std::to_string(i) + ": extension lobe parameters must be finite");
       for (int k = 0; k < 3; ++k) {
         const float le = (float)((double)m.emissive[k] * (double)m.emissive_intensity);
         if (!std::isfinite(le) || le < 0.0f)
@@ -366,6 +376,8 @@ vt_status vt_set_scene(vt_engine* eng, const vt_scene_desc* scene) {
     if (s) return s;
     // Commit: nothing below can fail.
     engine_release_cwbvh(eng);
+    engine_refit_scene_changed(eng);
+    ++eng->scene_generation;
     eng->scene.release();
     eng->scene = nb;
     nb = SceneBuffers();
@@ -768,6 +780,42 @@ vt_status vt_pack_copy_field(const vt_pack* p, int32_t field, void* dst, uint64_
 }
 
 void vt_pack_free(vt_pack* p) { delete p; }
+
+vt_status vt_scene_copy_field(vt_engine* eng, int32_t field, void* dst, uint64_t dst_words, uint64_t* words) {
+  return vt_guarded(eng, [&]() -> vt_status {
+    if (!eng) return VT_ERR_INVALID_ARGUMENT;
+    if (!eng->has_scene || !eng->pack || !eng->layout) return set_err(eng, VT_ERR_STATE, "no scene is set");
+    const DeviceBuffer* dev = nullptr;
+    size_t w = 0;
+    const void* src = nullptr;
+    switch (field) {
+      case VT_SCENE_BLAS_NODE64:
+        dev = &eng->scene.blas, w = eng->layout->blas_nodes.size();
+        break;
+      case VT_SCENE_BLAS_NODE128:
+        dev = &eng->scene.blas4, w = eng->layout->blas4_nodes.size();
+        break;
+      case VT_SCENE_TRIANGLES:
+        dev = &eng->scene.tris, w = eng->layout->tris.size();
+        break;
+      default:
+        if (field < 0 || field >= VT_PACK_FIELD_COUNT) return set_err(eng, VT_ERR_INVALID_ARGUMENT, "bad field");
+        src = pack_field(*eng->pack, field, &w);
+    }
+    if (words) *words = w;
+    if (!dst) return VT_OK;
+    if (w > dst_words) return set_err(eng, VT_ERR_RANGE, "destination too small");
+    if (!w) return VT_OK;
+    if (dev) {
+      HIP_TRY(eng, hipSetDevice(eng->device));
+      HIP_TRY(eng, hipMemcpyAsync(dst, dev->ptr, w * 4, hipMemcpyDeviceToHost, eng->stream));
+      HIP_TRY(eng, hipStreamSynchronize(eng->stream));
+    } else {
+      std::memcpy(dst, src, w * 4);
+    }
+    return VT_OK;
+  });
+}
 
 }  // extern "C"
 

@@ -67,6 +67,7 @@ struct RenderSceneData;  // one scene's shading data (render.cpp)
 // an allocation failure, or its preparation throws std::runtime_error.
 enum : uint32_t { kInjectPackThrow = 1u, kInjectRenderAlloc = 2u, kInjectRenderThrow = 4u };
 struct CwbvhState;   // wide-BVH traversal buffers (cwbvh.cpp)
+struct RefitState;   // device refit buffers (refit.cpp)
 struct NrcState;     // neural radiance cache (nrc.hip)
 
 struct vt_engine {
@@ -106,6 +107,10 @@ struct vt_engine {
   uint32_t fail_inject = 0;    // kInject* bits
   uint32_t build_threads = 0;  // host threads of the scene builder (0: every hardware thread)
   void* noalloc_scratch = nullptr;  // this device's copy of noalloc.hip's scratch array
+  RefitState* refit = nullptr;
+  // Counts scene changes (vt_set_scene, refits): recorded frame graphs
+  // depend on the scene they were recorded with.
+  uint64_t scene_generation = 0;
 };
 
 // Looks up the device address of noalloc.hip's scratch array for the engine's
@@ -141,6 +146,13 @@ vt_status engine_prepare_render(vt_engine* e, const vt::Pack& pack, const std::v
 void engine_commit_render(vt_engine* e, RenderSceneData* d);
 void engine_discard_render(RenderSceneData* d);
 void engine_release_render(vt_engine* e);
+// Recomputes the bound scene's ray bias and normalisation box from the
+// pack's world bounds after a refit moved them (render.cpp).
+void engine_render_rebound(vt_engine* e);
+// Refit state (refit.cpp): per-scene device copies are dropped when the
+// scene changes; everything is freed with the engine.
+void engine_refit_scene_changed(vt_engine* e);
+void engine_release_refit(vt_engine* e);
 void engine_release_nrc(vt_engine* e);
 // Wide-BVH (cwbvh-closest) traversal state (cwbvh.cpp): built on first use
 // per scene, released on scene change.

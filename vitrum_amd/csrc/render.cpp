@@ -297,6 +297,32 @@ static float ray_origin_bias(double radius, const double* center) {
   return (float)std::max(radius * (1e-3 / std::sqrt(3.0)), 2.0 * ulp);
 }
 
+// Ray-origin bias, tMin and the normalisation box of the scene, from its
+// world bounds (or the caller's radius).
+static void scene_bounds(RenderSceneData& R, const vt::Pack& P, float scene_radius) {
+  double center[3], radius = scene_radius;
+  for (int k = 0; k < 3; ++k) center[k] = 0.5 * (P.world_min[k] + P.world_max[k]);
+  if (!(radius > 0)) {
+    double d2 = 0;
+    for (int k = 0; k < 3; ++k) d2 += (P.world_max[k] - P.world_min[k]) * (P.world_max[k] - P.world_min[k]);
+    radius = 0.5 * std::sqrt(d2);
+    if (!(radius > 0)) radius = std::sqrt(3.0);
+  }
+  if (!std::isfinite(center[0])) center[0] = center[1] = center[2] = 0;
+  for (int k = 0; k < 3; ++k) {
+    const double lo = P.world_min[k], hi = P.world_max[k];
+    const bool ok = std::isfinite(lo) && std::isfinite(hi) && hi > lo;
+    R.scene_min[k] = ok ? (float)lo : 0.0f;
+    R.scene_inv_extent[k] = ok ? (float)(1.0 / (hi - lo)) : 1.0f;
+  }
+  R.ray_bias = ray_origin_bias(radius, center);
+  R.tmin = std::max(R.ray_bias * 0.1f, 1.175494351e-38f);
+}
+
+void engine_render_rebound(vt_engine* e) {
+  if (e->render && e->pack) scene_bounds(*e->render, *e->pack, e->scene_radius);
+}
+
 static vt_status ensure_render_state(vt_engine* e) {
   if (e->render) return VT_OK;
   std::unique_ptr<RenderState> r(new RenderState());
@@ -350,8 +376,26 @@ vt_status engine_prepare_render(vt_engine* e, const vt::Pack& P, const std::vect
     q[3] = m.roughness;
     q[4] = m.metallic;
     q[8] = m.double_sided ? 1.0f : 0.0f;
+    // Extension lobes, clamped as materialPacking.ts:259-316 packs them; the
+    // anisotropy rotation is stored as its cosine and sine (binary64, rounded
+    // once).
+    auto c01 = [](float x) { return std::min(1.0f, std::max(0.0f, x)); };
+    q[9] = c01(m.clearcoat);
+    q[10] = c01(m.clearcoat_roughness);
+    q[11] = c01(m.sheen);
+    q[12] = c01(m.sheen_roughness);
+    for (int k = 0; k < 3; ++k) q[13 + k] = c01(m.sheen_color[k]);
+    q[16] = c01(m.anisotropy);
+    q[17] = (float)std::cos((double)m.anisotropy_rotation);
+    q[18] = (float)std::sin((double)m.anisotropy_rotation);
+    for (int k = 0; k < 3; ++k) q[19 + k] = m.has_specular ? std::max(0.0f, m.specular_color[k]) : 1.0f;
+    q[22] = m.has_specular ? c01(m.specular_intensity) : 1.0f;
   }
-  if (materials.empty()) mats[0] = mats[1] = mats[2] = 0.8f, mats[3] = 1.0f;
+  if (materials.empty()) {
+    mats[0] = mats[1] = mats[2] = 0.8f, mats[3] = 1.0f;
+    mats[17] = 1.0f;
+    mats[19] = mats[20] = mats[21] = mats[22] = 1.0f;
+  }
   // Mesh-area emitters, one per emissive triangle (scene/emitterPacking.ts:
   // 748-820): primitives in scene order, each primitive's instances in packed
   // order, triangles in source order; world vertices are the binary64
@@ -481,24 +525,7 @@ vt_status engine_prepare_render(vt_engine* e, const vt::Pack& P, const std::vect
   R.light_tree_nodes = (uint32_t)(tree.size() / vtl::kNodeFloats);
   R.light_tree_host = tree;
   if (tree.empty()) tree.assign(vtl::kNodeFloats, 0.0f);
-  // Ray-origin bias from the scene bounds (or the caller's radius).
-  double center[3], radius = scene_radius;
-  for (int k = 0; k < 3; ++k) center[k] = 0.5 * (P.world_min[k] + P.world_max[k]);
-  if (!(radius > 0)) {
-    double d2 = 0;
-    for (int k = 0; k < 3; ++k) d2 += (P.world_max[k] - P.world_min[k]) * (P.world_max[k] - P.world_min[k]);
-    radius = 0.5 * std::sqrt(d2);
-    if (!(radius > 0)) radius = std::sqrt(3.0);
-  }
-  if (!std::isfinite(center[0])) center[0] = center[1] = center[2] = 0;
-  for (int k = 0; k < 3; ++k) {
-    const double lo = P.world_min[k], hi = P.world_max[k];
-    const bool ok = std::isfinite(lo) && std::isfinite(hi) && hi > lo;
-    R.scene_min[k] = ok ? (float)lo : 0.0f;
-    R.scene_inv_extent[k] = ok ? (float)(1.0 / (hi - lo)) : 1.0f;
-  }
-  R.ray_bias = ray_origin_bias(radius, center);
-  R.tmin = std::max(R.ray_bias * 0.1f, 1.175494351e-38f);
+  scene_bounds(R, P, scene_radius);
   vt_status s = VT_OK;
   s = s ? s : engine_upload(e, R.materials, mats.data(), mats.size() * 4);
   s = s ? s : engine_upload(e, R.emitters, em.data(), em.size() * 4);
@@ -1157,6 +1184,7 @@ std::vector<uintptr_t> batch_state(vt_render_batch* b) {
   for (vt_engine* m : b->members) {
     const RenderState& R = *m->render;
     v.push_back(((uintptr_t)R.width << 32) | R.height);
+    v.push_back((uintptr_t)m->scene_generation);
     for (const void* p : {R.rays[0].ptr, R.rays[1].ptr, R.hits.ptr, R.accum.ptr, R.materials.ptr, R.lights.ptr,
                           R.env_texels.ptr, m->scene.tris.ptr, m->scene.blas4.ptr})
       v.push_back((uintptr_t)p);

@@ -9,7 +9,11 @@
 
 namespace vtr {
 
-constexpr uint32_t kMatFloats = 12;   // base rgb, roughness, metallic, Le rgb, twoSided, pad x3
+// Material record: base rgb, roughness, metallic, Le rgb, twoSided,
+// clearcoat, clearcoatRoughness, sheen, sheenRoughness, sheenColor rgb,
+// anisotropy, cos and sin of anisotropyRotation, specularColor rgb,
+// specularIntensity, pad.
+constexpr uint32_t kMatFloats = 24;
 constexpr uint32_t kEmitFloats = 16;  // a, b, c, Le, twoSided, packed instance, packed tri, pad
 constexpr uint32_t kInstFloats = 36;  // W2L (16), L2W (16), mirrored, emitter base, pad x2
 

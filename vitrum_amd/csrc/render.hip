@@ -270,6 +270,12 @@ struct Mat {
   float rough, metal;
   F3 emit;
   bool two_sided;
+  // extension lobes (bsdf.wgsl.ts:392-808): clearcoat, sheen, anisotropy
+  float cc, cc_rough, sheen, sheen_rough;
+  F3 sheen_color;
+  float aniso, aniso_c, aniso_s;  // strength, cos / sin of the rotation
+  F3 spec_color;                  // KHR_materials_specular
+  float spec_int;
 };
 
 __device__ Mat load_mat(const float* m) {
@@ -279,28 +285,58 @@ __device__ Mat load_mat(const float* m) {
   r.metal = m[4];
   r.emit = load3(m + 5);
   r.two_sided = m[8] > 0.5f;
+  r.cc = m[9];
+  r.cc_rough = m[10];
+  r.sheen = m[11];
+  r.sheen_rough = m[12];
+  r.sheen_color = load3(m + 13);
+  r.aniso = m[16];
+  r.aniso_c = m[17];
+  r.aniso_s = m[18];
+  r.spec_color = load3(m + 19);
+  r.spec_int = m[22];
   return r;
 }
 
-// materialSpecularF0 with ior 1.5 and white specular (bsdf.wgsl.ts:192-209).
+// materialSpecularF0 with ior 1.5 (bsdf.wgsl.ts:192-209): the IOR F0 tinted by
+// specularColor (clamped to 1) and scaled by specularIntensity, mixed towards
+// the base colour by metallic.
 __device__ F3 spec_f0(const Mat& m) {
   const float ratio = (1.5f - 1.0f) / (1.5f + 1.0f);
   const float f = ratio * ratio;
+  const float si = clampf(m.spec_int, 0.0f, 1.0f);
+  const F3 d = f3(fminf(f * fmaxf(m.spec_color.x, 0.0f), 1.0f) * si, fminf(f * fmaxf(m.spec_color.y, 0.0f), 1.0f) * si,
+                  fminf(f * fmaxf(m.spec_color.z, 0.0f), 1.0f) * si);
   const float mt = clampf(m.metal, 0.0f, 1.0f);
-  return f3(f + (m.base.x - f) * mt, f + (m.base.y - f) * mt, f + (m.base.z - f) * mt);
+  return f3(d.x + (m.base.x - d.x) * mt, d.y + (m.base.y - d.y) * mt, d.z + (m.base.z - d.z) * mt);
 }
-// materialSpecularFresnelSchlick (bsdf.wgsl.ts:211-222), clamped to [0, 1].
-__device__ F3 schlick01(float c, F3 f0) {
+// The Schlick F90 of materialSpecularFresnelSchlick: specularIntensity for
+// dielectrics, 1 for metals (bsdf.wgsl.ts:211-222).
+__device__ float spec_f90(const Mat& m) {
+  const float d = clampf(m.spec_int, 0.0f, 1.0f);
+  return d + (1.0f - d) * clampf(m.metal, 0.0f, 1.0f);
+}
+// materialSpecularFresnelSchlick, clamped to [0, 1] as the layered interface
+// response clamps it (bsdf.wgsl.ts:283-292).
+__device__ F3 schlick01(float c, F3 f0, float f90) {
   const float x = clampf(1.0f - c, 0.0f, 1.0f);
   const float x2 = x * x;
   const float x5 = x2 * x2 * x;
-  return f3(clampf(f0.x + (1.0f - f0.x) * x5, 0.0f, 1.0f), clampf(f0.y + (1.0f - f0.y) * x5, 0.0f, 1.0f),
-            clampf(f0.z + (1.0f - f0.z) * x5, 0.0f, 1.0f));
+  return f3(clampf(f0.x + (f90 - f0.x) * x5, 0.0f, 1.0f), clampf(f0.y + (f90 - f0.y) * x5, 0.0f, 1.0f),
+            clampf(f0.z + (f90 - f0.z) * x5, 0.0f, 1.0f));
 }
-__device__ float ggx_d(float ndh, float alpha) {  // material.wgsl.ts:2797
+// ggxD (material.wgsl.ts:2797-2804) at the half vector h:
+// a^2 / (pi ((1 - cos^2) + cos^2 a^2)^2), with 1 - cos^2 taken as |n x h|^2.
+// Near the lobe's peak 1 - cos^2 is smaller than a^2; from a binary32 cosine
+// it would carry an absolute error of ~1e-7 (relative 1e-7 / a^2, 4e-4 for a
+// coat of roughness 0.15), from the cross product its error shrinks with the
+// angle.
+__device__ float ggx_d(F3 n, F3 h, float alpha) {
   const float a2 = alpha * alpha;
-  const float n2 = clampf(ndh * ndh, 0.0f, 1.0f);
-  const float d = (1.0f - n2) + n2 * a2;
+  const float c = fmaxf(dot(n, h), 0.0f);
+  const F3 x = cross(n, h);
+  const float s2 = c > 0.0f ? fminf(dot(x, x), 1.0f) : 1.0f;
+  const float d = s2 + (c * c) * a2;
   return a2 / (kPi * d * d);
 }
 __device__ float smith_g1(float ndv, float rough) {  // roughDielectric.ts:52-60
@@ -347,11 +383,12 @@ __device__ float lut_eavg(float rough) {
   const float tr = fr - (float)r0;
   return clampf(kGgxEavg[r0] + (kGgxEavg[r1] - kGgxEavg[r0]) * tr, 0.3f, 1.0f);
 }
-// ggxMultiscatterLobe (material.wgsl.ts:2906-2931)
-__device__ F3 ggx_ms(F3 f0, float rough, float ndv, float ndl) {
-  const float eavg = lut_eavg(rough), om = 1.0f - eavg;
+// ggxMultiscatterLobeRoughness (material.wgsl.ts:2906-2926): view and light
+// lookups at their own roughness, the average albedo at a third.
+__device__ F3 ggx_ms(F3 f0, float rough_v, float rough_l, float rough_avg, float ndv, float ndl) {
+  const float eavg = lut_eavg(rough_avg), om = 1.0f - eavg;
   if (om < 1e-4f) return f3(0, 0, 0);
-  const float eo = lut_e(ndv, rough), ei = lut_e(ndl, rough);
+  const float eo = lut_e(ndv, rough_v), ei = lut_e(ndl, rough_l);
   const float shape = (1.0f - eo) * (1.0f - ei) / fmaxf(kPi * om, 1e-6f);
   const float c[3] = {f0.x, f0.y, f0.z};
   float o[3];
@@ -364,48 +401,6 @@ __device__ F3 ggx_ms(F3 f0, float rough, float ndv, float ndl) {
   return f3(o[0], o[1], o[2]);
 }
 
-// evaluateBrdfFullWithClearcoatNormal, base lobes (bsdf.wgsl.ts:809-916).
-__device__ F3 bsdf_eval(const Mat& m, F3 n, F3 wo, F3 wi) {
-  const float ndl = fmaxf(dot(n, wi), 0.0f), ndv = fmaxf(dot(n, wo), 0.0f);
-  if (ndl <= 1e-5f || ndv <= 1e-5f) return f3(0, 0, 0);
-  const F3 h = safe_normalize(wi + wo);
-  const float ndh = fmaxf(dot(n, h), 0.0f), vdh = fmaxf(dot(wo, h), 0.0f);
-  const F3 f0 = spec_f0(m);
-  const F3 F = schlick01(vdh, f0);
-  const float kdm = 1.0f - m.metal;
-  const F3 diff = f3((1.0f - F.x) * kdm * m.base.x * kInvPi, (1.0f - F.y) * kdm * m.base.y * kInvPi,
-                     (1.0f - F.z) * kdm * m.base.z * kInvPi);
-  const float alpha = fmaxf(m.rough * m.rough, 1e-3f);
-  const float d = ggx_d(ndh, alpha);
-  const float g = smith_g1(ndv, m.rough) * smith_g1(ndl, m.rough);
-  const F3 spec = (d * g) * F * (1.0f / fmaxf(4.0f * ndv * ndl, 1e-6f));
-  const F3 f0c = f3(clampf(f0.x, 0, 1), clampf(f0.y, 0, 1), clampf(f0.z, 0, 1));
-  return diff + spec + ggx_ms(f0c, m.rough, ndv, ndl);
-}
-
-// brdfFiniteBaseLobeWeights specular probability (bsdf.wgsl.ts:1172-1210).
-__device__ float spec_prob(const Mat& m, float ndv) {
-  const F3 F = schlick01(ndv, spec_f0(m));
-  const float x = fmaxf(luminance(F), m.metal);
-  return represented(clampf(0.04f + (0.96f - 0.04f) * x, 0.04f, 0.96f));
-}
-
-// brdfDirectionalPdfWithIridescence (bsdf.wgsl.ts:1555-1631).
-__device__ float bsdf_pdf(const Mat& m, F3 n, F3 wo, F3 wi) {
-  const float wdn = dot(n, wi), odn = dot(n, wo);
-  const float ndv = fmaxf(odn, 0.0f);
-  if (ndv <= 1e-5f) return 0.0f;
-  const F3 h = safe_normalize(wi + wo);
-  const float ndh = fmaxf(dot(n, h), 0.0f);
-  const float sp = spec_prob(m, ndv), dp = 1.0f - sp;
-  if (!(wdn * odn > 0.0f)) return 0.0f;
-  const float ndl = fmaxf(wdn, 0.0f);
-  if (ndl <= 1e-5f) return 0.0f;
-  const float alpha = fmaxf(m.rough * m.rough, 1e-3f);
-  const float pspec = (ggx_d(ndh, alpha) * smith_g1(ndv, m.rough)) / fmaxf(4.0f * ndv, 1e-6f);
-  return dp * (ndl * kInvPi) + sp * pspec;
-}
-
 // buildOnb (bsdf.wgsl.ts:1632-1640)
 __device__ void build_onb(F3 n, F3& t, F3& b) {
   const F3 up = fabsf(n.y) > 0.999f ? f3(1, 0, 0) : f3(0, 1, 0);
@@ -414,10 +409,261 @@ __device__ void build_onb(F3 n, F3& t, F3& b) {
   t = f3(c.x / l, c.y / l, c.z / l);
   b = cross(n, t);
 }
+// The tangent frame rotated by anisotropyRotation (bsdf.wgsl.ts:2582-2595);
+// the rotation's cos and sin come from the material table.
+__device__ void aniso_frame(const Mat& m, F3 n, F3& t, F3& b) {
+  F3 t0, b0;
+  build_onb(n, t0, b0);
+  t = m.aniso_c * t0 + m.aniso_s * b0;
+  b = (-m.aniso_s) * t0 + m.aniso_c * b0;
+}
 
-// sampleGgxVndfTangent (bsdf.wgsl.ts:1710-1735)
-__device__ F3 vndf_sample(F3 wo, float alpha, Rng& rng) {
-  const F3 vh = safe_normalize(f3(alpha * wo.x, alpha * wo.y, wo.z));
+__device__ float smoothstep_unit(float e0, float e1, float x) {
+  const float t = clampf((x - e0) / (e1 - e0), 0.0f, 1.0f);
+  return t * t * (3.0f - 2.0f * t);
+}
+
+// ---- anisotropic GGX (bsdf.wgsl.ts:597-808) ----
+// computeAnisotropicAxes (bsdf.wgsl.ts:1928-1933)
+__device__ void aniso_axes(float alpha, float aniso, float& ax, float& ay) {
+  const float aspect = __builtin_sqrtf(fmaxf(1.0f - 0.9f * aniso, 1e-4f));
+  ax = fmaxf(alpha / aspect, 1e-4f);
+  ay = fmaxf(alpha * aspect, 1e-4f);
+}
+__device__ float ggx_d_aniso(float ht, float hb, float hn, float ax, float ay) {
+  const float d = (ht / ax) * (ht / ax) + (hb / ay) * (hb / ay) + hn * hn;
+  return 1.0f / fmaxf(kPi * ax * ay * d * d, 1e-10f);
+}
+__device__ float smith_g1_aniso(float vt, float vb, float vn, float ax, float ay) {
+  const float vn2 = fmaxf(vn * vn, 1e-10f);
+  const float den = vn + __builtin_sqrtf(vn2 + (vt * ax) * (vt * ax) + (vb * ay) * (vb * ay));
+  return (2.0f * vn) / fmaxf(den, 1e-6f);
+}
+// anisotropicProjectedRoughness / anisotropicAverageRoughness (bsdf.wgsl.ts:755-783)
+__device__ float aniso_projected_rough(F3 dir, F3 t, F3 b, float rough, float aniso) {
+  float ax, ay;
+  aniso_axes(fmaxf(rough * rough, 1e-3f), aniso, ax, ay);
+  const float dt = dot(dir, t), db = dot(dir, b);
+  const float tl2 = dt * dt + db * db;
+  const float blend = clampf(0.15f * aniso, 0.0f, 0.15f);
+  float projected;
+  if (tl2 <= 1e-6f) {
+    projected = __builtin_sqrtf(clampf(0.5f * (ax + ay), 1e-4f, 1.0f));
+  } else {
+    const float eff = __builtin_sqrtf(((dt * ax) * (dt * ax) + (db * ay) * (db * ay)) / tl2);
+    projected = __builtin_sqrtf(clampf(eff, 1e-4f, 1.0f));
+  }
+  return rough + (projected - rough) * blend;
+}
+__device__ float aniso_average_rough(float rough, float aniso) {
+  float ax, ay;
+  aniso_axes(fmaxf(rough * rough, 1e-3f), aniso, ax, ay);
+  const float rms = __builtin_sqrtf(0.5f * (ax * ax + ay * ay));
+  const float projected = __builtin_sqrtf(clampf(rms, 1e-4f, 1.0f));
+  return rough + (projected - rough) * clampf(0.15f * aniso, 0.0f, 0.15f);
+}
+// anisotropicMultiscatterScale (bsdf.wgsl.ts:785-796)
+__device__ float aniso_ms_scale(float aniso, float rough) {
+  const float reduction = smoothstep_unit(0.0f, 0.35f, clampf(aniso, 0.0f, 1.0f));
+  const float scale = 0.6f * smoothstep_unit(0.35f, 0.9f, rough);
+  return 1.0f + (scale - 1.0f) * reduction;
+}
+// brdfAnisotropicSpecPdf (bsdf.wgsl.ts:727-753)
+__device__ float aniso_spec_pdf(const Mat& m, F3 n, F3 t, F3 b, F3 wo, F3 wi) {
+  const float ndv = fmaxf(dot(n, wo), 1e-6f), ndl = fmaxf(dot(n, wi), 0.0f);
+  if (ndl <= 1e-5f) return 0.0f;
+  const F3 h = safe_normalize(wo + wi);
+  float ax, ay;
+  aniso_axes(fmaxf(m.rough * m.rough, 1e-3f), m.aniso, ax, ay);
+  const float d = ggx_d_aniso(dot(h, t), dot(h, b), fmaxf(dot(h, n), 0.0f), ax, ay);
+  const float g1 = smith_g1_aniso(dot(wo, t), dot(wo, b), fmaxf(dot(wo, n), 1e-6f), ax, ay);
+  return (d * g1) / fmaxf(4.0f * ndv, 1e-6f);
+}
+
+// ---- clearcoat and sheen layers (bsdf.wgsl.ts:398-596) ----
+// clearcoatLayerWeight: the coat's fixed-IOR (F0 0.04) Fresnel at |V.N|.
+__device__ float cc_weight(const Mat& m, F3 n, F3 wo) {
+  const float x = clampf(1.0f - clampf(fabsf(dot(n, wo)), 0.0f, 1.0f), 0.0f, 1.0f);
+  const float x2 = x * x;
+  const float fcc = 0.04f + (1.0f - 0.04f) * (x2 * x2 * x);
+  return clampf(m.cc, 0.0f, 1.0f) * fcc;
+}
+// evalClearcoatLobe (the kernel without N.L)
+__device__ float cc_eval(const Mat& m, F3 n, F3 wo, F3 wi) {
+  if (m.cc <= 0.0f) return 0.0f;
+  const float ndl = fmaxf(dot(n, wi), 0.0f), ndv = fmaxf(dot(n, wo), 0.0f);
+  if (ndl <= 1e-5f || ndv <= 1e-5f) return 0.0f;
+  const F3 h = safe_normalize(wi + wo);
+  const float alpha = fmaxf(m.cc_rough * m.cc_rough, 1e-3f);
+  const float d = ggx_d(n, h, alpha);
+  const float g = smith_g1(ndv, m.cc_rough) * smith_g1(ndl, m.cc_rough);
+  return cc_weight(m, n, wo) * ((d * g) / fmaxf(4.0f * ndv * ndl, 1e-6f));
+}
+__device__ float cc_pdf(const Mat& m, F3 n, F3 wo, F3 wi) {  // clearcoatPdf
+  if (m.cc <= 0.0f) return 0.0f;
+  const float ndv = fmaxf(dot(n, wo), 0.0f);
+  if (ndv <= 1e-5f) return 0.0f;
+  const float ndl = fmaxf(dot(n, wi), 0.0f);
+  if (ndl <= 1e-5f) return 0.0f;
+  const F3 h = safe_normalize(wi + wo);
+  const float alpha = fmaxf(m.cc_rough * m.cc_rough, 1e-3f);
+  return (ggx_d(n, h, alpha) * smith_g1(ndv, m.cc_rough)) / fmaxf(4.0f * ndv, 1e-6f);
+}
+// charlieD (Estevez and Kulla 2017) at the half vector h; sin(theta_h) is
+// |n x h| (see ggx_d), the power evaluated in binary64 and rounded once.
+__device__ float charlie_d(F3 n, F3 h, float alpha) {
+  const float inv = 1.0f / fmaxf(alpha, 1e-4f);
+  const F3 x = cross(n, h);
+  const float sin_h = fmaxf(dot(n, h), 0.0f) > 0.0f ? __builtin_sqrtf(fminf(dot(x, x), 1.0f)) : 1.0f;
+  return (2.0f + inv) * (float)pow((double)sin_h, (double)inv) / (2.0f * kPi);
+}
+__device__ float sheen_vis(float ndl, float ndv) { return 1.0f / fmaxf(4.0f * (ndl + ndv - ndl * ndv), 1e-6f); }
+// evalSheenLobe (the kernel without N.L)
+__device__ F3 sheen_eval(const Mat& m, F3 n, F3 wo, F3 wi) {
+  if (m.sheen <= 0.0f) return f3(0, 0, 0);
+  const float ndl = fmaxf(dot(n, wi), 0.0f), ndv = fmaxf(dot(n, wo), 0.0f);
+  if (ndl <= 1e-5f || ndv <= 1e-5f) return f3(0, 0, 0);
+  const F3 h = safe_normalize(wi + wo);
+  const float alpha = fmaxf(m.sheen_rough * m.sheen_rough, 1e-3f);
+  const float k = charlie_d(n, h, alpha) * sheen_vis(ndl, ndv);
+  return (m.sheen * m.sheen_color) * k;
+}
+// sheenDirectionalAlbedo (Estevez and Kulla 2017, section 5)
+__device__ float sheen_albedo(float c_raw, float alpha) {
+  const float c = 1.0f - clampf(c_raw, 0.0f, 1.0f);
+  return 0.65584461f * (c * c * c) + 1.0f / (4.16526551f + expf(-7.97291361f * __builtin_sqrtf(alpha) + 6.33516894f));
+}
+__device__ float sheen_attenuation(const Mat& m, F3 n, F3 wo, F3 wi) {  // sheenBaseAttenuation
+  if (m.sheen <= 0.0f) return 1.0f;
+  const float ndl = clampf(fabsf(dot(n, wi)), 0.0f, 1.0f), ndv = clampf(fabsf(dot(n, wo)), 0.0f, 1.0f);
+  const float a = fmaxf(m.sheen_rough, 0.07f), a2 = a * a;
+  const float mc = fmaxf(fmaxf(m.sheen_color.x, m.sheen_color.y), m.sheen_color.z);
+  const float full = fminf(1.0f - mc * sheen_albedo(ndv, a2), 1.0f - mc * sheen_albedo(ndl, a2));
+  return clampf(1.0f + (full - 1.0f) * clampf(m.sheen, 0.0f, 1.0f), 0.0f, 1.0f);
+}
+__device__ float sheen_pdf(const Mat& m, F3 n, F3 wo, F3 wi) {  // charlieSheenPdf
+  if (m.sheen <= 0.0f) return 0.0f;
+  const float ndl = fmaxf(dot(n, wi), 0.0f), ndv = fmaxf(dot(n, wo), 0.0f);
+  if (ndl <= 1e-5f || ndv <= 1e-5f) return 0.0f;
+  const F3 h = safe_normalize(wi + wo);
+  const float ndh = fmaxf(dot(n, h), 0.0f), vdh = fmaxf(dot(wo, h), 1e-6f);
+  const float alpha = fmaxf(m.sheen_rough * m.sheen_rough, 1e-3f);
+  return (charlie_d(n, h, alpha) * ndh) / fmaxf(4.0f * vdh, 1e-6f);
+}
+
+// evaluateBrdfFullWithClearcoatNormal (bsdf.wgsl.ts:809-916): the base lobes
+// (Lambert, isotropic or anisotropic GGX, Kulla-Conty) under the sheen layer
+// under the clearcoat layer.
+__device__ F3 bsdf_eval(const Mat& m, F3 n, F3 wo, F3 wi) {
+  const float ndl = fmaxf(dot(n, wi), 0.0f), ndv = fmaxf(dot(n, wo), 0.0f);
+  if (ndl <= 1e-5f || ndv <= 1e-5f) return f3(0, 0, 0);
+  const F3 h = safe_normalize(wi + wo);
+  const float vdh = fmaxf(dot(wo, h), 0.0f);
+  const F3 f0 = spec_f0(m);
+  const F3 F = schlick01(vdh, f0, spec_f90(m));
+  const float kdm = 1.0f - m.metal;
+  const F3 diff = f3((1.0f - F.x) * kdm * m.base.x * kInvPi, (1.0f - F.y) * kdm * m.base.y * kInvPi,
+                     (1.0f - F.z) * kdm * m.base.z * kInvPi);
+  const F3 f0c = f3(clampf(f0.x, 0, 1), clampf(f0.y, 0, 1), clampf(f0.z, 0, 1));
+  F3 spec, ms;
+  if (m.aniso > 0.0f) {
+    F3 t, b;
+    aniso_frame(m, n, t, b);
+    // evalBrdfSpecAnisotropic (bsdf.wgsl.ts:693-725)
+    float ax, ay;
+    aniso_axes(fmaxf(m.rough * m.rough, 1e-3f), m.aniso, ax, ay);
+    const float d = ggx_d_aniso(dot(h, t), dot(h, b), fmaxf(dot(h, n), 0.0f), ax, ay);
+    const float g = smith_g1_aniso(dot(wo, t), dot(wo, b), fmaxf(dot(wo, n), 1e-6f), ax, ay) *
+                    smith_g1_aniso(dot(wi, t), dot(wi, b), fmaxf(dot(wi, n), 1e-6f), ax, ay);
+    spec = (d * g) * F * (1.0f / fmaxf(4.0f * fmaxf(ndv, 1e-6f) * ndl, 1e-6f));
+    const float ravg = aniso_average_rough(m.rough, m.aniso);
+    ms = ggx_ms(f0c, aniso_projected_rough(wo, t, b, m.rough, m.aniso),
+                aniso_projected_rough(wi, t, b, m.rough, m.aniso), ravg, ndv, ndl) *
+         aniso_ms_scale(m.aniso, ravg);
+  } else {
+    const float alpha = fmaxf(m.rough * m.rough, 1e-3f);
+    const float d = ggx_d(n, h, alpha);
+    const float g = smith_g1(ndv, m.rough) * smith_g1(ndl, m.rough);
+    spec = (d * g) * F * (1.0f / fmaxf(4.0f * ndv * ndl, 1e-6f));
+    ms = ggx_ms(f0c, m.rough, m.rough, m.rough, ndv, ndl);
+  }
+  const F3 base = diff + spec + ms;
+  if (m.cc <= 0.0f && m.sheen <= 0.0f) return base;
+  const float cc_att = m.cc > 0.0f ? clampf(1.0f - cc_weight(m, n, wo), 0.0f, 1.0f) : 1.0f;
+  const F3 layered = (base * sheen_attenuation(m, n, wo, wi) + sheen_eval(m, n, wo, wi)) * cc_att;
+  const float cc = cc_eval(m, n, wo, wi);
+  return f3(layered.x + cc, layered.y + cc, layered.z + cc);
+}
+
+// brdfFiniteBaseLobeWeights specular probability (bsdf.wgsl.ts:1172-1210).
+__device__ float spec_prob(const Mat& m, float ndv) {
+  const F3 F = schlick01(ndv, spec_f0(m), spec_f90(m));
+  const float x = fmaxf(luminance(F), m.metal);
+  return represented(clampf(0.04f + (0.96f - 0.04f) * x, 0.04f, 0.96f));
+}
+
+// brdfRepresentedExtensionLobeProbabilities (bsdf.wgsl.ts:1219-1262): joint
+// P(base), P(clearcoat), P(sheen), and P(clearcoat | extension).
+struct ExtProbs {
+  float base, cc, sheen, cc_given_ext;
+};
+__device__ ExtProbs ext_probs(const Mat& m) {
+  const float cc = fmaxf(m.cc, 0.0f), sh = fmaxf(m.sheen, 0.0f), ext = cc + sh;
+  if (!(ext > 0.0f)) return ExtProbs{1.0f, 0.0f, 0.0f, 0.0f};
+  const float total = 1.0f + ext;
+  float pb, pe;
+  if (ext <= 1.0f) {
+    pe = represented(ext / total);
+    pb = 1.0f - pe;
+  } else {
+    pb = represented(1.0f / total);
+    pe = 1.0f - pb;
+  }
+  float pc;
+  if (!(cc > 0.0f))
+    pc = 0.0f;
+  else if (!(sh > 0.0f))
+    pc = 1.0f;
+  else if (cc <= sh)
+    pc = represented(cc / ext);
+  else
+    pc = 1.0f - represented(sh / ext);
+  const float jc = pe * pc;
+  return ExtProbs{pb, jc, pe - jc, pc};
+}
+
+// brdfDirectionalPdfFullWithClearcoatNormal (bsdf.wgsl.ts:1370-1489): the
+// base lobes' density (brdfDirectionalPdfWithIridescence, :1555-1631, or its
+// anisotropic VNDF form) and the extension lobes', by their joint
+// probabilities.
+__device__ float bsdf_pdf(const Mat& m, F3 n, F3 wo, F3 wi) {
+  const float wdn = dot(n, wi), odn = dot(n, wo);
+  const float ndv = fmaxf(odn, 0.0f);
+  if (ndv <= 1e-5f) return 0.0f;
+  const float sp = spec_prob(m, ndv), dp = 1.0f - sp;
+  if (!(wdn * odn > 0.0f)) return 0.0f;
+  const float ndl = fmaxf(wdn, 0.0f);
+  if (ndl <= 1e-5f) return 0.0f;
+  float pspec;
+  if (m.aniso > 0.0f) {
+    F3 t, b;
+    aniso_frame(m, n, t, b);
+    pspec = aniso_spec_pdf(m, n, t, b, wo, wi);
+  } else {
+    const F3 h = safe_normalize(wi + wo);
+    const float alpha = fmaxf(m.rough * m.rough, 1e-3f);
+    pspec = (ggx_d(n, h, alpha) * smith_g1(ndv, m.rough)) / fmaxf(4.0f * ndv, 1e-6f);
+  }
+  const float base = dp * (ndl * kInvPi) + sp * pspec;
+  if (m.cc <= 0.0f && m.sheen <= 0.0f) return base;
+  const ExtProbs e = ext_probs(m);
+  return (e.base * base + e.cc * cc_pdf(m, n, wo, wi)) + e.sheen * sheen_pdf(m, n, wo, wi);
+}
+
+// sampleGgxVndfTangent (bsdf.wgsl.ts:1710-1735) and, with ax != ay,
+// sampleGgxVndfAnisTangent (bsdf.wgsl.ts:612-637).
+__device__ F3 vndf_sample(F3 wo, float ax, float ay, Rng& rng) {
+  const F3 vh = safe_normalize(f3(ax * wo.x, ay * wo.y, wo.z));
   const float lensq = vh.x * vh.x + vh.y * vh.y;
   const float inv = lensq > 1e-10f ? 1.0f / __builtin_sqrtf(lensq) : 0.0f;
   const F3 t1 = lensq > 1e-10f ? f3(-vh.y * inv, vh.x * inv, 0.0f) : f3(1, 0, 0);
@@ -428,7 +674,7 @@ __device__ F3 vndf_sample(F3 wo, float alpha, Rng& rng) {
   const float s = 0.5f * (1.0f + vh.z);
   b = (1.0f - s) * __builtin_sqrtf(fmaxf(0.0f, 1.0f - a * a)) + s * b;
   const F3 nh = a * t1 + b * t2 + __builtin_sqrtf(fmaxf(0.0f, 1.0f - a * a - b * b)) * vh;
-  return safe_normalize(f3(alpha * nh.x, alpha * nh.y, fmaxf(1e-6f, nh.z)));
+  return safe_normalize(f3(ax * nh.x, ay * nh.y, fmaxf(1e-6f, nh.z)));
 }
 
 // measureAreaVector (common.wgsl.ts:310-362): unit normal and area
@@ -1147,26 +1393,60 @@ __global__ __launch_bounds__(256) void shade_kernel(FrameParams f0, ShadeScene s
         //    finalizer (bsdf.wgsl.ts:1863-1926).
         const float ndv = fmaxf(dot(nrm, wo), 0.0f);
         const float sp = spec_prob(m, ndv);
-        (void)rand01(rng);  // extension-lobe draw; P(base lobes) = 1
-        const bool glossy = rand01(rng) < sp;
+        const ExtProbs ep = ext_probs(m);
         F3 tb, bb;
-        build_onb(nrm, tb, bb);
+        if (m.aniso > 0.0f)
+          aniso_frame(m, nrm, tb, bb);
+        else
+          build_onb(nrm, tb, bb);
+        const F3 inc = f3(-wo.x, -wo.y, -wo.z);
         F3 wi;
-        if (glossy) {
-          const float alpha = fmaxf(m.rough * m.rough, 1e-3f);
-          const F3 wl = f3(dot(wo, tb), dot(wo, bb), dot(wo, nrm));
-          const F3 hl = vndf_sample(wl, alpha, rng);
-          const F3 hw = safe_normalize(hl.x * tb + hl.y * bb + hl.z * nrm);
-          const F3 inc = f3(-wo.x, -wo.y, -wo.z);
+        uint32_t lobe;  // 0 diffuse, 1 specular, 2 clearcoat, 3 sheen
+        if (rand01(rng) < ep.base) {
+          lobe = rand01(rng) < sp ? 1u : 0u;
+          if (lobe == 1u) {
+            // glossyReflectionSample / its anisotropic form (bsdf.wgsl.ts:640-691, 1752-1778)
+            const float alpha = fmaxf(m.rough * m.rough, 1e-3f);
+            float ax = alpha, ay = alpha;
+            if (m.aniso > 0.0f) aniso_axes(alpha, m.aniso, ax, ay);
+            const F3 wl = f3(dot(wo, tb), dot(wo, bb), dot(wo, nrm));
+            const F3 hl = vndf_sample(wl, ax, ay, rng);
+            const F3 hw = safe_normalize(hl.x * tb + hl.y * bb + hl.z * nrm);
+            wi = safe_normalize(inc - hw * (2.0f * dot(hw, inc)));
+          } else {
+            // cosineHemisphereSample (bsdf.wgsl.ts:1648-1664), its own frame
+            F3 tc, bc;
+            build_onb(nrm, tc, bc);
+            const float u1 = rand01(rng), u2 = rand01(rng);
+            float a, b;
+            disc_polar(u1, u2, a, b);
+            const float ct = __builtin_sqrtf(fmaxf(0.0f, 1.0f - u1));
+            wi = safe_normalize(a * tc + b * bc + ct * nrm);
+          }
+        } else if (rand01(rng) < ep.cc_given_ext) {
+          // the clearcoat lobe: an isotropic VNDF sample at the coat roughness
+          lobe = 2u;
+          F3 tc, bc;
+          build_onb(nrm, tc, bc);
+          const float alpha = fmaxf(m.cc_rough * m.cc_rough, 1e-3f);
+          const F3 wl = f3(dot(wo, tc), dot(wo, bc), dot(wo, nrm));
+          const F3 hl = vndf_sample(wl, alpha, alpha, rng);
+          const F3 hw = safe_normalize(hl.x * tc + hl.y * bc + hl.z * nrm);
           wi = safe_normalize(inc - hw * (2.0f * dot(hw, inc)));
         } else {
+          // charlieSheenSample (bsdf.wgsl.ts:1665-1700): sin(theta_h) = u1^(1/(1/alpha + 2))
+          lobe = 3u;
           const float u1 = rand01(rng), u2 = rand01(rng);
-          float a, b;
-          disc_polar(u1, u2, a, b);
-          const float ct = __builtin_sqrtf(fmaxf(0.0f, 1.0f - u1));
-          wi = safe_normalize(a * tb + b * bb + ct * nrm);
+          const float alpha = fmaxf(m.sheen_rough * m.sheen_rough, 1e-3f);
+          const float inv = 1.0f / fmaxf(alpha, 1e-4f);
+          const float sin_h = (float)pow((double)u1, 1.0 / (double)(inv + 2.0f));
+          const float cos_h = __builtin_sqrtf(fmaxf(0.0f, 1.0f - sin_h * sin_h));
+          float c, s;
+          disc_polar(1.0f, u2, c, s);
+          const F3 hw = safe_normalize((sin_h * c) * tb + (sin_h * s) * bb + cos_h * nrm);
+          wi = safe_normalize(inc - hw * (2.0f * dot(hw, inc)));
         }
-        sig = mix64(sig, glossy ? 0xB1ull : 0xB0ull);
+        sig = mix64(sig, 0xB0ull + lobe);
         const float mpdf = bsdf_pdf(m, nrm, wo, wi);
         const F3 fr = bsdf_eval(m, nrm, wo, wi);
         const float cosine = fabsf(dot(nrm, wi));

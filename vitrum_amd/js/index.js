@@ -206,6 +206,14 @@ function sceneBlob(scene, opts) {
       emissiveIntensity: m.emissiveIntensity == null ? 1 : m.emissiveIntensity,
       doubleSided: !!m.doubleSided,
     });
+    // extension lobes (MaterialSpec clearcoat / sheen / anisotropy / specular)
+    for (const k of ['clearcoat', 'clearcoatRoughness', 'sheen', 'sheenRoughness', 'sheenColor', 'anisotropy',
+      'anisotropyRotation', 'specularColor', 'specularIntensity']) {
+      if (m[k] != null) materials[materials.length - 1][k] = m[k];
+    }
+    for (const k of ['transmission', 'iridescence']) {
+      if (m[k] != null && m[k] > 0) throw new RangeError(`pt-hip renders opaque materials: ${k} is not supported (primitive ${i})`);
+    }
     let transforms = null;
     if (p.kind === 'mesh') {
       if (p.transform != null) {
@@ -286,6 +294,19 @@ function emittersOf(list) {
 
 const SKIPPED = Object.freeze({ kind: 'skipped', samplesAccumulated: 0, isConverged: false });
 
+// Fields of a ScenePrimitivePatch this backend renders, per primitive kind
+// (core/src/scene/primitives.ts:167-227).
+const PRIMITIVE_PATCH_KEYS = {
+  mesh: new Set(['positions', 'normals', 'indices', 'transform', 'material']),
+  'instanced-mesh': new Set(['positions', 'normals', 'indices', 'instances', 'material']),
+};
+
+function primitiveIndexOf(scene, id, method) {
+  const i = scene.primitives.findIndex((p) => String(p.id) === String(id));
+  if (i < 0) throw new Error(`${method}: primitive "${id}" not found in current scene`);
+  return i;
+}
+
 // FrameQualitySettings.tonemap / exposure / outputColorSpace (core/src/
 // frame.ts:32-105): defaults aces, 1, srgb (present.wgsl.ts:10-11).
 const TONEMAP_MODES = { aces: 0, agx: 1, reinhard: 2, linear: 3, none: 4 };
@@ -312,8 +333,8 @@ class HipPathTracerEngine {
     this.state = 'ready';
     this.backendId = 'pt-hip';
     this.capabilities = Object.freeze({
-      supportsIncrementalScene: false,
-      supportsAddRemovePrimitive: false,
+      supportsIncrementalScene: true,
+      supportsAddRemovePrimitive: true,
       supportsAuxBuffers: false,
       accumulates: true,
       maxSamplesPerPixel: opts.maxSamplesPerPixel || Infinity,
@@ -374,6 +395,101 @@ class HipPathTracerEngine {
   }
 
   getScene() {
+    return this._scene;
+  }
+
+  // Scene mutations (core/src/engine/index.ts:155-214).  Each validates its
+  // arguments, derives the next scene copy-on-write (patchPrimitiveInScene,
+  // core/src/scene/patchScene.ts:381: a material patch merges into the
+  // material, other fields replace) and binds it through setScene, whose
+  // result is what pt-webgpu's incremental paths produce (they equal a full
+  // repack of the patched scene); accumulation restarts.  With
+  // primitiveUpdates: 'refit', a patch of only positions (and normals) that
+  // keeps the vertex count moves the vertices with the BVH topology kept
+  // (the walkaround-hybrid refit path) instead; emissive primitives and
+  // other patches take the setScene path.
+  updatePrimitive(id, patch) {
+    this._live();
+    const scene = this._requireScene('updatePrimitive');
+    const i = primitiveIndexOf(scene, id, 'updatePrimitive');
+    if (patch == null || typeof patch !== 'object') throw new TypeError('updatePrimitive: patch must be an object');
+    const cur = scene.primitives[i];
+    const keys = [];
+    for (const k of Object.keys(patch)) {
+      if (k === 'id' || k === 'kind') {
+        if (patch[k] !== undefined && patch[k] !== cur[k]) throw new TypeError(`updatePrimitive: '${k}' cannot change`);
+        continue;
+      }
+      if (!PRIMITIVE_PATCH_KEYS[cur.kind] || !PRIMITIVE_PATCH_KEYS[cur.kind].has(k)) {
+        throw new TypeError(`updatePrimitive: '${k}' is not a patchable field of a ${cur.kind} primitive`);
+      }
+      keys.push(k);
+    }
+    if (keys.length === 0) return;  // a validated empty patch is a no-op
+    const next = Object.assign({}, cur, patch);
+    if (patch.material != null) next.material = Object.assign({}, cur.material || {}, patch.material);
+    const nextScene = Object.assign({}, scene, {
+      primitives: scene.primitives.map((p, k) => (k === i ? next : p)),
+    });
+    if (this._opts.primitiveUpdates === 'refit' && keys.every((k) => k === 'positions' || k === 'normals')) {
+      const positions = toF32(next.positions, 'positions');
+      const normals = patch.normals == null ? null : toF32(next.normals, 'normals');
+      try {
+        this._n.refitPrimitive(this._h, i, { positions, normals });
+        this._scene = nextScene;
+        this._samples = 0;
+        return;
+      } catch (err) {
+        if (err.code !== 'VT_ERR_RANGE' && err.code !== 'VT_ERR_STATE') throw err;
+        this._emit('warning', { code: 'pt-hip.update-primitive-rebuild', backend: 'pt-hip', phase: 'mutation',
+          method: 'updatePrimitive', message: `pt-hip: updatePrimitive("${id}") rebuilds the scene: ${err.message}` });
+      }
+    }
+    this.setScene(nextScene);
+  }
+
+  addPrimitive(primitive) {
+    this._live();
+    const scene = this._requireScene('addPrimitive');
+    if (primitive == null || typeof primitive !== 'object') throw new TypeError('addPrimitive: primitive must be an object');
+    if (scene.primitives.some((p) => String(p.id) === String(primitive.id))) {
+      throw new Error(`addPrimitive: primitive "${primitive.id}" already exists`);
+    }
+    this.setScene(Object.assign({}, scene, { primitives: scene.primitives.concat([primitive]) }));
+  }
+
+  removePrimitive(id) {
+    this._live();
+    const scene = this._requireScene('removePrimitive');
+    const i = primitiveIndexOf(scene, id, 'removePrimitive');
+    this.setScene(Object.assign({}, scene, { primitives: scene.primitives.filter((_, k) => k !== i) }));
+  }
+
+  updateEmitter(id, patch) {
+    this._live();
+    const scene = this._requireScene('updateEmitter');
+    const list = scene.emitters || [];
+    const i = list.findIndex((e) => String(e.id) === String(id));
+    if (i < 0) throw new Error(`updateEmitter: emitter "${id}" not found in current scene`);
+    if (patch == null || typeof patch !== 'object') throw new TypeError('updateEmitter: patch must be an object');
+    const cur = list[i];
+    if (patch.kind !== undefined && patch.kind !== cur.kind) {
+      throw new Error(`updateEmitter: emitter "${id}" kind cannot change from "${cur.kind}" to "${patch.kind}"`);
+    }
+    if (patch.id !== undefined && patch.id !== cur.id) throw new Error(`updateEmitter: emitter "${id}" id cannot be changed`);
+    if (Object.keys(patch).every((k) => k === 'id' || k === 'kind')) return;
+    const next = Object.assign({}, cur, patch);
+    this.setScene(Object.assign({}, scene, { emitters: list.map((e, k) => (k === i ? next : e)) }));
+  }
+
+  updateEnvironment(env) {
+    this._live();
+    const scene = this._requireScene('updateEnvironment');
+    this.setScene(Object.assign({}, scene, { environment: env == null ? { kind: 'none' } : env }));
+  }
+
+  _requireScene(method) {
+    if (this._scene == null) throw new Error(`${method}: no scene is set`);
     return this._scene;
   }
 
@@ -573,7 +689,7 @@ async function createPTEngine_HIP(opts) {
   const o = Object.assign({}, opts || {});
   const known = new Set(['device', 'maxBounces', 'maxSamplesPerPixel', 'sampling', 'sceneRadius', 'debug',
     'onWarning', 'denoiser', 'extensions', 'nrcEnabled', 'nrcConfig', 'lightTreeImportanceSampling',
-    'bvhTraversal']);
+    'bvhTraversal', 'primitiveUpdates']);
   for (const k of Object.keys(o)) {
     if (!known.has(k)) throw new TypeError(`pt-hip: unknown option '${k}'`);
   }
@@ -583,6 +699,9 @@ async function createPTEngine_HIP(opts) {
   if (o.bvhTraversal !== undefined && o.bvhTraversal !== 'binary' && o.bvhTraversal !== 'cwbvh-closest') {
     throw new RangeError(`pt-hip: bvhTraversal is unsupported (got ${JSON.stringify(o.bvhTraversal)}); ` +
       'expected one of "binary", "cwbvh-closest"');
+  }
+  if (o.primitiveUpdates !== undefined && o.primitiveUpdates !== 'rebuild' && o.primitiveUpdates !== 'refit') {
+    throw new RangeError(`pt-hip: primitiveUpdates must be "rebuild" or "refit" (got ${JSON.stringify(o.primitiveUpdates)})`);
   }
   if (o.nrcEnabled) nrcSettings(true, o.nrcConfig);  // validated before any device work
   const native = loadAddon();

@@ -167,6 +167,53 @@ assert.throws(() => pt.sceneBlob({ primitives: [], environment: { kind: 'vacuum'
     }
     assert.throws(() => pt.renderFrames([], []), TypeError);
   }
+  // Scene mutations: each equals setScene of the patched scene; the refit
+  // path of a one-triangle primitive builds the same trees as a rebuild.
+  {
+    const inp = { viewMatrix: view, projMatrix: proj, viewport: { width: 16, height: 16, devicePixelRatio: 1 },
+      frameIndex: 0, frameSeed: 7, quality: { bounces: 2 } };
+    const point = { kind: 'point', id: 'p', color: [1, 1, 1], intensity: 4, position: [0.6, 0.3, 1] };
+    const base = { primitives: [quad], emitters: [point], environment: { kind: 'none' } };
+    const moved = new Float32Array([0, 0, 0.2, 1, 0, 0.2, 1, 1, 0.2]);
+    const image = async (e) => { e.renderFrame(inp); return Array.from((await e.captureFrame()).rgba); };
+    const a = await pt.createPTEngine_HIP({ device: 0 });
+    const b = await pt.createPTEngine_HIP({ device: 0 });
+    const r = await pt.createPTEngine_HIP({ device: 0, primitiveUpdates: 'refit' });
+    assert.ok(a.capabilities.supportsIncrementalScene && a.capabilities.supportsAddRemovePrimitive);
+    a.setScene(base);
+    r.setScene(base);
+    const before = await image(a);
+    a.updatePrimitive('q', { positions: moved });
+    r.updatePrimitive('q', { positions: moved });
+    b.setScene({ primitives: [Object.assign({}, quad, { positions: moved })], emitters: [point], environment: { kind: 'none' } });
+    const want = await image(b);
+    assert.deepStrictEqual(await image(a), want, 'updatePrimitive equals setScene of the patched scene');
+    assert.deepStrictEqual(await image(r), want, 'the refit path equals it for a one-triangle primitive');
+    assert.notDeepStrictEqual(want, before);
+    assert.strictEqual(a.getScene().primitives[0].positions, moved);
+    assert.throws(() => a.updatePrimitive('nope', { positions: moved }), /not found/);
+    assert.throws(() => a.updatePrimitive('q', { kind: 'instanced-mesh' }), TypeError);
+    assert.throws(() => a.updatePrimitive('q', { bogus: 1 }), TypeError);
+    a.updatePrimitive('q', { material: { roughness: 0.3 } });
+    assert.strictEqual(a.getScene().primitives[0].material.baseColor, quad.material.baseColor, 'material patch merges');
+    const q2 = Object.assign({}, quad, { id: 'q2', positions: new Float32Array([0, 0, -0.5, 1, 0, -0.5, 1, 1, -0.5]) });
+    a.addPrimitive(q2);
+    assert.strictEqual(a.getScene().primitives.length, 2);
+    assert.throws(() => a.addPrimitive(q2), /already exists/);
+    a.removePrimitive('q2');
+    assert.strictEqual(a.getScene().primitives.length, 1);
+    assert.throws(() => a.removePrimitive('q2'), /not found/);
+    a.updatePrimitive('q', { material: quad.material });
+    a.updateEmitter('p', { intensity: 8 });
+    b.setScene({ primitives: [Object.assign({}, quad, { positions: moved })], emitters: [Object.assign({}, point, { intensity: 8 })],
+      environment: { kind: 'none' } });
+    assert.deepStrictEqual(await image(a), await image(b), 'updateEmitter equals setScene of the patched scene');
+    assert.throws(() => a.updateEmitter('p', { kind: 'spot' }), /cannot change/);
+    a.updateEnvironment({ kind: 'hdri', hdri: { constant: [0.5, 0.5, 0.5] } });
+    assert.strictEqual(a.getScene().environment.kind, 'hdri');
+    for (const e of [a, b, r]) e.dispose();
+    await assert.rejects(pt.createPTEngine_HIP({ device: 0, primitiveUpdates: 'sometimes' }), RangeError);
+  }
   // traceClosest through the addon: a ray straight at the triangle.
   const hits = new Float32Array(addon.traceClosest(eng._h, new Float32Array([0.6, 0.3, 1, 0, 0, 0, -1, 1e30])));
   assert.ok(Math.abs(hits[2] - 1) < 1e-6, `t = ${hits[2]}`);

@@ -338,6 +338,18 @@ napi_value SetScene(napi_env env, napi_callback_info info) {
     napi_has_named_property(env, m, "doubleSided", &has);
     if (has && napi_get_named_property(env, m, "doubleSided", &v) == napi_ok) napi_get_value_bool(env, v, &ds);
     q.double_sided = ds ? 1u : 0u;
+    // extension lobes (MaterialSpec clearcoat / sheen / anisotropy / specular)
+    if (!get_f32(env, m, "clearcoat", 0.0f, &q.clearcoat)) return nullptr;
+    if (!get_f32(env, m, "clearcoatRoughness", 0.0f, &q.clearcoat_roughness)) return nullptr;
+    if (!get_f32(env, m, "sheen", 0.0f, &q.sheen)) return nullptr;
+    if (!get_f32(env, m, "sheenRoughness", 0.0f, &q.sheen_roughness)) return nullptr;
+    if (!get_vec(env, m, "sheenColor", 3, zero, q.sheen_color)) return nullptr;
+    if (!get_f32(env, m, "anisotropy", 0.0f, &q.anisotropy)) return nullptr;
+    if (!get_f32(env, m, "anisotropyRotation", 0.0f, &q.anisotropy_rotation)) return nullptr;
+    const float one[3] = {1, 1, 1};
+    q.has_specular = 1u;
+    if (!get_vec(env, m, "specularColor", 3, one, q.specular_color)) return nullptr;
+    if (!get_f32(env, m, "specularIntensity", 1.0f, &q.specular_intensity)) return nullptr;
   }
   vt_scene_desc sd{};
   sd.primitives = vp.data();
@@ -588,6 +600,26 @@ napi_value Reset(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// refitPrimitive(handle, primitiveIndex, {positions, normals?}): moves one
+// primitive's vertices with its BVH topology kept (vt_refit_primitive).
+napi_value RefitPrimitive(napi_env env, napi_callback_info info) {
+  auto a = args_of(env, info, 3);
+  Handle* h = handle_of(env, a[0]);
+  if (!h) return nullptr;
+  uint32_t index = 0;
+  if (napi_get_value_uint32(env, a[1], &index) != napi_ok) {
+    throw_type(env, "primitive index must be a number");
+    return nullptr;
+  }
+  void *pos = nullptr, *nrm = nullptr;
+  size_t npos = 0, nnrm = 0;
+  if (!get_typed(env, a[2], "positions", napi_float32_array, false, &pos, &npos)) return nullptr;
+  if (!get_typed(env, a[2], "normals", napi_float32_array, true, &nrm, &nnrm)) return nullptr;
+  throw_status(env, h->eng,
+               vt_refit_primitive(h->eng, index, (const float*)pos, (uint32_t)npos, (const float*)nrm, (uint32_t)nnrm));
+  return nullptr;
+}
+
 napi_value Stats(napi_env env, napi_callback_info info) {
   auto a = args_of(env, info, 1);
   Handle* h = handle_of(env, a[0]);
@@ -746,7 +778,7 @@ napi_value Init(napi_env env, napi_value exports) {
              {"traceAny", TraceAny},     {"invertMat4", InvertMat4},
              {"nrcConfig", NrcConfig},   {"version", Version},
              {"captureOutput", CaptureOutput}, {"captureAux", CaptureAux},
-             {"renderFrames", RenderFrames}};
+             {"renderFrames", RenderFrames}, {"refitPrimitive", RefitPrimitive}};
   for (const auto& f : fns) {
     napi_value fn;
     if (napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.cb, nullptr, &fn) != napi_ok) return nullptr;

@@ -37,7 +37,14 @@ EXPORTED = [
     "vt_nrc_gradient", "vt_nrc_render_config", "vt_nrc_teacher_vertices", "vt_nrc_timing", "vt_batch_create", "vt_batch_destroy",
     "vt_batch_add_scene", "vt_batch_scene_count", "vt_batch_trace_closest_device", "vt_batch_capture",
     "vt_batch_replay", "vt_batch_autotune", "vt_batch_set_tuning",
+    "vt_refit_bvh_bounds_device", "vt_refit_primitive", "vt_scene_copy_field",
 ]
+
+SCENE_BLAS_NODE64, SCENE_BLAS_NODE128, SCENE_TRIANGLES = 64, 65, 66
+PACK_FIELDS = ["positions", "normals", "indices", "tri_material", "tri_source", "tri_primitive",
+               "bvh_nodes", "tlas_nodes", "tlas_instance_indices", "tlas_blas_roots", "tlas_w2l",
+               "tlas_l2w", "instance_primitive", "instance_source"]
+FLOAT_PACK_FIELDS = {"positions", "normals", "tlas_w2l", "tlas_l2w"}
 
 
 class Tuning(C.Structure):
@@ -70,7 +77,10 @@ class Primitive(C.Structure):
 class Material(C.Structure):
     _fields_ = [("base_color", C.c_float * 3), ("roughness", C.c_float), ("metallic", C.c_float),
                 ("emissive", C.c_float * 3), ("emissive_intensity", C.c_float),
-                ("double_sided", C.c_uint32)]
+                ("double_sided", C.c_uint32), ("clearcoat", C.c_float), ("clearcoat_roughness", C.c_float),
+                ("sheen", C.c_float), ("sheen_roughness", C.c_float), ("sheen_color", C.c_float * 3),
+                ("anisotropy", C.c_float), ("anisotropy_rotation", C.c_float), ("has_specular", C.c_uint32),
+                ("specular_color", C.c_float * 3), ("specular_intensity", C.c_float)]
 
 
 class Environment(C.Structure):
@@ -193,6 +203,11 @@ def lib():
         L.vt_pack_copy_field.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_uint64]
         L.vt_pack_free.argtypes = [C.c_void_p]
         L.vt_device_alloc.argtypes = [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]
+        L.vt_refit_bvh_bounds_device.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
+                                                 C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32]
+        L.vt_refit_primitive.argtypes = [C.c_void_p, C.c_uint32, F32P, C.c_uint32, F32P, C.c_uint32]
+        L.vt_scene_copy_field.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_uint64,
+                                          C.POINTER(C.c_uint64)]
         L.vt_device_free.argtypes = [C.c_void_p, C.c_void_p]
         L.vt_memcpy_h2d.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
         L.vt_memcpy_d2h.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
@@ -324,7 +339,13 @@ def _material(m) -> Material:
     return Material((C.c_float * 3)(*m.get("baseColor", [0.8, 0.8, 0.8])),
                     float(m.get("roughness", 1.0)), float(m.get("metallic", 0.0)),
                     (C.c_float * 3)(*em), float(m.get("emissiveIntensity", 1.0 if any(em) else 0.0)),
-                    1 if m.get("doubleSided", False) else 0)
+                    1 if m.get("doubleSided", False) else 0,
+                    float(m.get("clearcoat", 0.0)), float(m.get("clearcoatRoughness", 0.0)),
+                    float(m.get("sheen", 0.0)), float(m.get("sheenRoughness", 0.0)),
+                    (C.c_float * 3)(*m.get("sheenColor", [0.0, 0.0, 0.0])),
+                    float(m.get("anisotropy", 0.0)), float(m.get("anisotropyRotation", 0.0)), 1,
+                    (C.c_float * 3)(*m.get("specularColor", [1.0, 1.0, 1.0])),
+                    float(m.get("specularIntensity", 1.0)))
 
 
 class Engine:
@@ -397,6 +418,35 @@ class Engine:
     def set_bvh_traversal(self, mode):
         """0: binary BVH; 1: cwbvh-closest (closest-hit rays only)."""
         check(lib().vt_set_bvh_traversal(self._h, C.c_uint32(mode)), self._h)
+
+    def refit_primitive(self, primitive, positions, normals=None):
+        """Moves primitive `primitive`'s vertices (same count) with its BVH
+        topology kept: the device refit of its BLAS and the TLAS over its
+        instances (vt_refit_primitive)."""
+        pos = np.ascontiguousarray(positions, np.float32).ravel()
+        nrm = None if normals is None else np.ascontiguousarray(normals, np.float32).ravel()
+        check(lib().vt_refit_primitive(self._h, C.c_uint32(primitive), _p(pos, F32P), C.c_uint32(pos.size),
+                                       _p(nrm, F32P), C.c_uint32(0 if nrm is None else nrm.size)), self._h)
+
+    def refit_bvh_bounds_device(self, d_nodes, node_count, d_indices, index_words, index_stride, d_positions,
+                                position_floats, position_stride):
+        check(lib().vt_refit_bvh_bounds_device(self._h, C.c_void_p(d_nodes), C.c_uint32(node_count),
+                                               C.c_void_p(d_indices), C.c_uint32(index_words),
+                                               C.c_uint32(index_stride), C.c_void_p(d_positions),
+                                               C.c_uint32(position_floats), C.c_uint32(position_stride)),
+              self._h)
+
+    def scene_field(self, field):
+        """A field of the bound scene: a pack field name (host copy) or one of
+        SCENE_BLAS_NODE64 / SCENE_BLAS_NODE128 / SCENE_TRIANGLES (device)."""
+        fid = PACK_FIELDS.index(field) if isinstance(field, str) else field
+        words = C.c_uint64(0)
+        check(lib().vt_scene_copy_field(self._h, C.c_int32(fid), None, C.c_uint64(0), C.byref(words)), self._h)
+        dtype = np.float32 if (field in FLOAT_PACK_FIELDS or fid == SCENE_TRIANGLES) else np.uint32
+        out = np.zeros(max(words.value, 1), dtype)
+        check(lib().vt_scene_copy_field(self._h, C.c_int32(fid), out.ctypes.data_as(C.c_void_p),
+                                        C.c_uint64(out.size), C.byref(words)), self._h)
+        return out[: words.value]
 
     def set_shade_sort(self, enabled):
         check(lib().vt_set_shade_sort(self._h, C.c_uint32(1 if enabled else 0)), self._h)
