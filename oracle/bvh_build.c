@@ -142,9 +142,11 @@ static uint32_t blas_build_rec(bctx *x, uint32_t *subset, uint32_t n,
     }
   if (n <= x->max_leaf) return blas_leaf(x, node, subset, n, index);
   if (depth >= x->max_depth) {
+    /* buildArrayBvh.ts:537-546, with the 16-bit leaf-count suffix */
     x->failed = orc_fail("maxDepth=%u prevents enforcing maxLeafTriangles=%u: depth %u "
-                         "still contains %u triangles",
-                         x->max_depth, x->max_leaf, depth, n);
+                         "still contains %u triangles%s.",
+                         x->max_depth, x->max_leaf, depth, n,
+                         n > 0xFFFFu ? " and the packed 16-bit limit (0xFFFF = 65535)" : "");
     return index;
   }
   double cmin[3] = {INFINITY, INFINITY, INFINITY};
