@@ -479,6 +479,20 @@ vt_status vt_build_blas(const float *positions, uint32_t position_floats,
                         uint32_t *out_node_count, uint32_t *out_reordered_indices,
                         uint32_t *out_reordered_mat, uint32_t *out_reordered_to_source);
 
+/* buildArrayBvh (packages/shared-bvh/src/buildArrayBvh.ts:342-780) with the
+ * binned-SAH work on the engine's device: the same arguments, checks, error
+ * messages and outputs as vt_build_blas, byte for byte.  The tree grows one
+ * depth per round (every node of a depth in one launch); the host keeps the
+ * node list, sorts the median-fallback ranges and writes the preorder
+ * layout.  Allocates and frees its device scratch per call. */
+vt_status vt_build_blas_device(vt_engine *eng, const float *positions, uint32_t position_floats,
+                               uint32_t position_stride, const uint32_t *indices,
+                               uint32_t index_words, uint32_t index_stride,
+                               const uint32_t *tri_material_ids, uint32_t max_leaf,
+                               uint32_t num_bins, uint32_t max_depth, uint32_t *out_node_words,
+                               uint32_t *out_node_count, uint32_t *out_reordered_indices,
+                               uint32_t *out_reordered_mat, uint32_t *out_reordered_to_source);
+
 /* buildTlas (packages/shared-bvh/src/tlas.ts:859).  AABBs are binary64, as
  * the reference's JS numbers; out_nodes holds (2n-1)*8 words. */
 vt_status vt_build_tlas(uint32_t n, const uint32_t *blas_ids, const double *aabb_min,

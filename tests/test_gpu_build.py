@@ -1,0 +1,121 @@
+"""The device BLAS builder (vt_build_blas_device) against the oracle's
+buildArrayBvh restatement (oracle/bvh_build.c, buildArrayBvh.ts:342-780) and
+the host builder: nodes, reordered indices, materials and the permutation
+equal word for word, for triangle soups, grids full of tied centroids, flat
+and degenerate meshes that take the median fallback, 3- and 4-word strides,
+every leaf size and bin count the reference tests use, one triangle to 300k;
+argument and depth-limit errors carry the host builder's code and message.
+"""
+import time
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from vitrum_amd import native as N
+
+from test_oracle_refit import soup
+
+pytestmark = pytest.mark.gpu
+
+
+def grid(nx, ny, z_jitter=0.0, seed=0):
+    """A tessellated plane: two triangles per cell, many equal centroids."""
+    rng = np.random.default_rng(seed)
+    xs, ys = np.meshgrid(np.arange(nx + 1, dtype=np.float32), np.arange(ny + 1, dtype=np.float32), indexing="ij")
+    z = (rng.standard_normal(xs.shape) * z_jitter).astype(np.float32)
+    pos = np.stack([xs, ys, z, np.zeros_like(xs)], -1).reshape(-1, 4)
+    v = lambda i, j: i * (ny + 1) + j
+    tris = []
+    for i in range(nx):
+        for j in range(ny):
+            tris.append((v(i, j), v(i + 1, j), v(i, j + 1), 0))
+            tris.append((v(i + 1, j), v(i + 1, j + 1), v(i, j + 1), 0))
+    return pos, np.array(tris, np.uint32)
+
+
+def assert_same(a, b, what):
+    for k in ("node_count", "nodes", "reordered_indices", "reordered_mat", "reordered_to_source"):
+        x, y = a[k], b[k]
+        if k == "node_count":
+            assert x == y, (what, k, x, y)
+        else:
+            assert np.array_equal(np.asarray(x, np.uint32), np.asarray(y, np.uint32)), (what, k)
+
+
+def cases():
+    p, i = soup(1, 1)
+    yield "one triangle", p, i, {}
+    p, i = soup(257, 2)
+    yield "soup 257 leaf1", p, i, {"max_leaf": 1}
+    p, i = soup(3000, 3, zeros=True)
+    yield "soup 3000 signed zeros", p, i, {}
+    p, i = grid(40, 37)
+    yield "flat grid", p, i, {"max_leaf": 2}
+    p, i = grid(23, 31, 0.25, 4)
+    yield "bumpy grid bins 7", p, i, {"num_bins": 7}
+    # every triangle at one point: no centroid span, median fallback all the way down
+    p = np.zeros((3 * 200, 4), np.float32)
+    p[:, :3] = 1.5
+    i = np.concatenate([np.arange(600, dtype=np.uint32).reshape(-1, 3), np.zeros((200, 1), np.uint32)], 1)
+    yield "coincident", p, i, {}
+    # two clusters far apart plus identical copies: SAH then fallback
+    p, i = soup(500, 5)
+    p[: 3 * 250, :3] *= 1e-7
+    yield "tiny cluster", p, i, {"num_bins": 2, "max_leaf": 3}
+    p, i = soup(4096, 6)
+    yield "soup 4096 bins 256 leaf 16", p, i, {"num_bins": 256, "max_leaf": 16}
+    p, i = soup(999, 7)
+    yield "stride 3", np.ascontiguousarray(p[:, :3]), np.ascontiguousarray(i[:, :3]), \
+        {"position_stride": 3, "index_stride": 3}
+
+
+@pytest.mark.parametrize("case", list(range(9)))
+def test_device_build_matches_oracle(engine, case):
+    name, p, i, kw = list(cases())[case]
+    n = i.shape[0]
+    mats = (np.arange(n, dtype=np.uint32) * 7) % 5
+    print("device build case", name, n, kw, flush=True)
+    ref = O.build_blas(p, i, mats, **kw)
+    got = N.build_blas(p, i, mats, engine=engine, **kw)
+    assert_same(got, ref, name)
+
+
+def test_device_build_large_matches_host(engine):
+    p, i = soup(300_000, 11)
+    print("device build 300k soup", flush=True)
+    t0 = time.perf_counter()
+    host = N.build_blas(p, i)
+    t1 = time.perf_counter()
+    dev = N.build_blas(p, i, engine=engine)
+    t2 = time.perf_counter()
+    assert_same(dev, host, "300k soup")
+    print(f"host build {1e3 * (t1 - t0):.1f} ms, device build {1e3 * (t2 - t1):.1f} ms", flush=True)
+
+
+def test_device_build_errors_match_host(engine):
+    p, i = soup(64, 12)
+    for kw in ({"max_leaf": 0}, {"num_bins": 1}, {"max_depth": 59}, {"max_depth": 2}, {"max_depth": 0}):
+        with pytest.raises(N.VtError) as host:
+            N.build_blas(p, i, **kw)
+        with pytest.raises(N.VtError) as dev:
+            N.build_blas(p, i, engine=engine, **kw)
+        assert str(dev.value) == str(host.value), kw
+    bad = i.copy()
+    bad[5, 1] = 10_000
+    with pytest.raises(N.VtError, match="out-of-range vertex"):
+        N.build_blas(p, bad, engine=engine)
+    nan = p.copy()
+    nan[7, 2] = np.nan
+    with pytest.raises(N.VtError, match="must be finite"):
+        N.build_blas(nan, i, engine=engine)
+    # the engine stays usable after a rejected build
+    got = N.build_blas(p, i, engine=engine)
+    assert_same(got, N.build_blas(p, i), "after errors")
+
+
+def test_device_build_empty(engine):
+    got = N.build_blas(np.zeros(0, np.float32), np.zeros(0, np.uint32), engine=engine)
+    host = N.build_blas(np.zeros(0, np.float32), np.zeros(0, np.uint32))
+    assert got["node_count"] == host["node_count"] == 1
+    assert np.array_equal(got["nodes"], host["nodes"])

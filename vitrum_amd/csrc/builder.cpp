@@ -75,10 +75,9 @@ struct BlasJob {
 
 }  // namespace
 
-BuildError build_blas(const float* positions, uint32_t position_floats, uint32_t position_stride,
-                      const uint32_t* indices, uint32_t index_words, uint32_t index_stride,
-                      const uint32_t* tri_material_ids, uint32_t max_leaf, uint32_t num_bins,
-                      uint32_t max_depth, BlasOut& out) {
+BuildError check_blas_inputs(const float* positions, uint32_t position_floats, uint32_t position_stride,
+                             const uint32_t* indices, uint32_t index_words, uint32_t index_stride,
+                             uint32_t max_leaf, uint32_t num_bins, uint32_t max_depth) {
   if (position_stride != 3 && position_stride != 4)
     return fail(VT_ERR_RANGE, "positionStride must be exactly 3 or 4.");
   if (index_stride != 3 && index_stride != 4)
@@ -96,6 +95,23 @@ BuildError build_blas(const float* positions, uint32_t position_floats, uint32_t
   for (uint32_t i = 0; i < position_floats; ++i)
     if (!std::isfinite(positions[i]))
       return fail(VT_ERR_RANGE, "positions[" + std::to_string(i) + "] must be finite.");
+  const uint32_t n_vert = position_floats / position_stride;
+  for (uint32_t t = 0; t < index_words / index_stride; ++t) {
+    const uint32_t* tri = indices + (size_t)t * index_stride;
+    if (tri[0] >= n_vert || tri[1] >= n_vert || tri[2] >= n_vert)
+      return fail(VT_ERR_RANGE,
+                  "triangle " + std::to_string(t) + " references an out-of-range vertex.");
+  }
+  return {};
+}
+
+BuildError build_blas(const float* positions, uint32_t position_floats, uint32_t position_stride,
+                      const uint32_t* indices, uint32_t index_words, uint32_t index_stride,
+                      const uint32_t* tri_material_ids, uint32_t max_leaf, uint32_t num_bins,
+                      uint32_t max_depth, BlasOut& out) {
+  if (BuildError e = check_blas_inputs(positions, position_floats, position_stride, indices, index_words,
+                                      index_stride, max_leaf, num_bins, max_depth))
+    return e;
   const uint32_t n_tri = index_words / index_stride;
   out.nodes.clear();
   if (n_tri == 0) {

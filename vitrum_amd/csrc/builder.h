@@ -38,6 +38,12 @@ BuildError build_blas(const float* positions, uint32_t position_floats, uint32_t
                       const uint32_t* tri_material_ids, uint32_t max_leaf, uint32_t num_bins,
                       uint32_t max_depth, BlasOut& out);
 
+// The argument checks of build_blas (buildArrayBvh.ts:342-420), shared with
+// the device builder so both reject the same inputs with the same messages.
+BuildError check_blas_inputs(const float* positions, uint32_t position_floats, uint32_t position_stride,
+                             const uint32_t* indices, uint32_t index_words, uint32_t index_stride,
+                             uint32_t max_leaf, uint32_t num_bins, uint32_t max_depth);
+
 struct TlasOut {
   std::vector<uint32_t> nodes;
   std::vector<uint32_t> instance_indices;

@@ -1,0 +1,254 @@
+// gpubuild.cpp -- the device BLAS builder behind the C ABI.
+//
+// Drives the level kernels of gpubuild.hip: the tree grows one depth per
+// round, every node of a depth in one launch.  A node's result depends only
+// on the multiset of its triangles and their order inside its range, and the
+// range a node owns is the one the host builder's depth-first loop gives it
+// (partitions are stable and touch only the node's own range), so the nodes,
+// the decisions and the final permutation equal the host builder's; the host
+// then lays the nodes out in the same preorder (left child next, right child
+// at a relative offset; buildArrayBvh.ts:342-780).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "engine_internal.h"
+#include "gpubuild.h"
+
+namespace {
+
+#define GB_HIP(eng, expr)                                                                  \
+  do {                                                                                     \
+    hipError_t _e = (expr);                                                                \
+    if (_e != hipSuccess)                                                                  \
+      return engine_error((eng), VT_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+struct GNode {
+  uint32_t begin, end, depth;
+  uint32_t left = ~0u, right = ~0u, axis = 0;
+  float box[6];
+};
+
+// Device buffers of one build, freed when it ends.
+struct Scratch {
+  DeviceBuffer pos, idx, bmin, bmax, cen, perm, tmp, bin_of;
+  DeviceBuffer lbegin, lend, lwant, nbox, cbox, decision, nleft, pnodes;
+  ~Scratch() {
+    for (DeviceBuffer* b : {&pos, &idx, &bmin, &bmax, &cen, &perm, &tmp, &bin_of, &lbegin, &lend, &lwant, &nbox,
+                            &cbox, &decision, &nleft, &pnodes})
+      b->release();
+  }
+};
+
+}  // namespace
+
+// Same arguments, checks, outputs and errors as vt_build_blas; the SAH work
+// runs on the engine's device.
+vt_status vt_build_blas_device(vt_engine* eng, const float* positions, uint32_t position_floats,
+                               uint32_t position_stride, const uint32_t* indices, uint32_t index_words,
+                               uint32_t index_stride, const uint32_t* tri_material_ids, uint32_t max_leaf,
+                               uint32_t num_bins, uint32_t max_depth, uint32_t* out_node_words,
+                               uint32_t* out_node_count, uint32_t* out_reordered_indices,
+                               uint32_t* out_reordered_mat, uint32_t* out_reordered_to_source) {
+  return vt_guarded(eng, [&]() -> vt_status {
+    if (!eng) return engine_error(nullptr, VT_ERR_INVALID_ARGUMENT, "NULL engine");
+    if ((position_floats && !positions) || (index_words && !indices) || !out_node_words || !out_node_count ||
+        (index_words && !out_reordered_indices))
+      return engine_error(eng, VT_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (vt::BuildError e = vt::check_blas_inputs(positions, position_floats, position_stride, indices,
+                                                 index_words, index_stride, max_leaf, num_bins, max_depth))
+      return engine_error(eng, e.code, e.message);
+    const uint32_t n_tri = index_words / index_stride;
+    if (n_tri == 0) {
+      std::memset(out_node_words, 0, 32);
+      out_node_words[7] = vt::kLeafFlag;
+      *out_node_count = 1;
+      return VT_OK;
+    }
+    hipStream_t s = eng->stream;
+    GB_HIP(eng, hipSetDevice(eng->device));
+    Scratch b;
+    const size_t n = n_tri;
+    // A level holds at most n_tri nodes: every node owns at least one triangle.
+    struct Sz {
+      DeviceBuffer* buf;
+      size_t bytes;
+    };
+    for (const Sz& z : {Sz{&b.pos, (size_t)position_floats * 4}, Sz{&b.idx, (size_t)index_words * 4},
+                        Sz{&b.bmin, 12 * n}, Sz{&b.bmax, 12 * n}, Sz{&b.cen, 24 * n}, Sz{&b.perm, 4 * n},
+                        Sz{&b.tmp, 4 * n}, Sz{&b.bin_of, 3 * n}, Sz{&b.lbegin, 4 * n}, Sz{&b.lend, 4 * n},
+                        Sz{&b.lwant, n}, Sz{&b.nbox, 24 * n}, Sz{&b.cbox, 48 * n}, Sz{&b.decision, 12 * n},
+                        Sz{&b.nleft, 4 * n}, Sz{&b.pnodes, 4 * n}})
+      if (vt_status st = z.buf->ensure(eng, std::max<size_t>(z.bytes, 16))) return st;
+    GB_HIP(eng, hipMemcpyAsync(b.pos.ptr, positions, (size_t)position_floats * 4, hipMemcpyHostToDevice, s));
+    GB_HIP(eng, hipMemcpyAsync(b.idx.ptr, indices, (size_t)index_words * 4, hipMemcpyHostToDevice, s));
+    std::vector<uint32_t> perm(n_tri);
+    for (uint32_t t = 0; t < n_tri; ++t) perm[t] = t;
+    GB_HIP(eng, hipMemcpyAsync(b.perm.ptr, perm.data(), 4 * n, hipMemcpyHostToDevice, s));
+    float* d_bmin = (float*)b.bmin.ptr;
+    float* d_bmax = (float*)b.bmax.ptr;
+    double* d_cen = (double*)b.cen.ptr;
+    uint32_t* d_perm = (uint32_t*)b.perm.ptr;
+    GB_HIP(eng, vtg::launch_tri_records((const float*)b.pos.ptr, position_stride, (const uint32_t*)b.idx.ptr,
+                                        index_stride, n_tri, d_bmin, d_bmax, d_cen, s));
+
+    vtg::LevelArgs la{};
+    la.begin = (const uint32_t*)b.lbegin.ptr;
+    la.end = (const uint32_t*)b.lend.ptr;
+    la.split_wanted = (const uint8_t*)b.lwant.ptr;
+    la.perm = d_perm;
+    la.bmin = d_bmin;
+    la.bmax = d_bmax;
+    la.cen = d_cen;
+    la.bin_of = (uint8_t*)b.bin_of.ptr;
+    la.node_box = (float*)b.nbox.ptr;
+    la.cen_box = (double*)b.cbox.ptr;
+    la.decision = (uint32_t*)b.decision.ptr;
+    la.n_tri = n_tri;
+    la.num_bins = num_bins;
+    vtg::PartitionArgs pa{};
+    pa.nodes = (const uint32_t*)b.pnodes.ptr;
+    pa.begin = la.begin;
+    pa.end = la.end;
+    pa.decision = la.decision;
+    pa.bin_of = la.bin_of;
+    pa.perm = d_perm;
+    pa.tmp = (uint32_t*)b.tmp.ptr;
+    pa.n_left = (uint32_t*)b.nleft.ptr;
+    pa.n_tri = n_tri;
+
+    std::vector<GNode> nodes;
+    nodes.reserve(2 * n);
+    nodes.push_back({0, n_tri, 0});
+    std::vector<uint32_t> level{0}, next, hb, he, hdec, hnl, psel;
+    std::vector<uint8_t> hw;
+    std::vector<float> hbox;
+    std::vector<double> hcbox, hcen;  // hcen: centroids, fetched on the first median fallback
+    while (!level.empty()) {
+      const uint32_t L = (uint32_t)level.size();
+      hb.resize(L), he.resize(L), hw.resize(L);
+      bool any_split = false;
+      for (uint32_t i = 0; i < L; ++i) {
+        const GNode& g = nodes[level[i]];
+        const uint32_t cnt = g.end - g.begin;
+        hb[i] = g.begin, he[i] = g.end;
+        hw[i] = cnt > max_leaf ? 1 : 0;
+        any_split |= hw[i] != 0;
+        if (cnt > max_leaf && g.depth >= max_depth) {
+          // Every node of a level has the same depth, so the first such node
+          // in level order is the first the depth-first builder meets.
+          std::string extra = cnt > 0xFFFFu ? " and the packed 16-bit limit (0xFFFF = 65535)" : "";
+          return engine_error(eng, VT_ERR_BUILD,
+                              "maxDepth=" + std::to_string(max_depth) + " prevents enforcing maxLeafTriangles=" +
+                                  std::to_string(max_leaf) + ": depth " + std::to_string(g.depth) +
+                                  " still contains " + std::to_string(cnt) + " triangles" + extra + ".");
+        }
+      }
+      GB_HIP(eng, hipMemcpyAsync(b.lbegin.ptr, hb.data(), 4 * (size_t)L, hipMemcpyHostToDevice, s));
+      GB_HIP(eng, hipMemcpyAsync(b.lend.ptr, he.data(), 4 * (size_t)L, hipMemcpyHostToDevice, s));
+      GB_HIP(eng, hipMemcpyAsync(b.lwant.ptr, hw.data(), (size_t)L, hipMemcpyHostToDevice, s));
+      GB_HIP(eng, vtg::launch_level(la, L, s));
+      hbox.resize(6 * (size_t)L);
+      hdec.resize(3 * (size_t)L);
+      hcbox.resize(6 * (size_t)L);
+      GB_HIP(eng, hipMemcpyAsync(hbox.data(), b.nbox.ptr, 24 * (size_t)L, hipMemcpyDeviceToHost, s));
+      if (any_split) {
+        GB_HIP(eng, hipMemcpyAsync(hdec.data(), b.decision.ptr, 12 * (size_t)L, hipMemcpyDeviceToHost, s));
+        GB_HIP(eng, hipMemcpyAsync(hcbox.data(), b.cbox.ptr, 48 * (size_t)L, hipMemcpyDeviceToHost, s));
+      }
+      GB_HIP(eng, hipStreamSynchronize(s));
+      psel.clear();
+      for (uint32_t i = 0; i < L; ++i) {
+        std::memcpy(nodes[level[i]].box, &hbox[6 * (size_t)i], 24);
+        if (hw[i] && hdec[3 * (size_t)i + 2]) psel.push_back(i);
+      }
+      hnl.assign(L, 0u);
+      if (!psel.empty()) {
+        GB_HIP(eng, hipMemcpyAsync(b.pnodes.ptr, psel.data(), 4 * psel.size(), hipMemcpyHostToDevice, s));
+        GB_HIP(eng, vtg::launch_partition(pa, (uint32_t)psel.size(), s));
+        GB_HIP(eng, hipMemcpyAsync(hnl.data(), b.nleft.ptr, 4 * (size_t)L, hipMemcpyDeviceToHost, s));
+        GB_HIP(eng, hipStreamSynchronize(s));
+      }
+      next.clear();
+      for (uint32_t i = 0; i < L; ++i) {
+        if (!hw[i]) continue;
+        const uint32_t id = level[i];
+        const uint32_t nb = nodes[id].begin, ne = nodes[id].end, cnt = ne - nb;
+        uint32_t axis = hdec[3 * (size_t)i], n_left = hnl[i];
+        const bool partitioned = hdec[3 * (size_t)i + 2] && n_left != 0 && n_left != cnt;
+        if (!partitioned) {
+          // Median fallback (buildArrayBvh.ts:677-694): widest centroid axis,
+          // sort by centroid then triangle id, split at n/2.
+          const double* cb = &hcbox[6 * (size_t)i];
+          axis = 0;
+          double widest = cb[3] - cb[0];
+          for (uint32_t k = 1; k < 3; ++k)
+            if (cb[3 + k] - cb[k] > widest) axis = k, widest = cb[3 + k] - cb[k];
+          if (hcen.empty()) {
+            hcen.resize(3 * n);
+            GB_HIP(eng, hipMemcpyAsync(hcen.data(), d_cen, 24 * n, hipMemcpyDeviceToHost, s));
+          }
+          std::vector<uint32_t> seg(cnt);
+          GB_HIP(eng, hipMemcpyAsync(seg.data(), d_perm + nb, 4 * (size_t)cnt, hipMemcpyDeviceToHost, s));
+          GB_HIP(eng, hipStreamSynchronize(s));
+          const double* cax = hcen.data() + (size_t)axis * n;
+          std::sort(seg.begin(), seg.end(),
+                    [&](uint32_t x, uint32_t y) { return cax[x] != cax[y] ? cax[x] < cax[y] : x < y; });
+          GB_HIP(eng, hipMemcpyAsync(d_perm + nb, seg.data(), 4 * (size_t)cnt, hipMemcpyHostToDevice, s));
+          GB_HIP(eng, hipStreamSynchronize(s));
+          n_left = cnt / 2;
+        }
+        const uint32_t d = nodes[id].depth + 1;
+        const uint32_t lc = (uint32_t)nodes.size();
+        nodes.push_back({nb, nb + n_left, d});
+        nodes.push_back({nb + n_left, ne, d});
+        nodes[id].left = lc;
+        nodes[id].right = lc + 1;
+        nodes[id].axis = axis;
+        next.push_back(lc);
+        next.push_back(lc + 1);
+      }
+      level.swap(next);
+    }
+    GB_HIP(eng, hipMemcpyAsync(perm.data(), d_perm, 4 * n, hipMemcpyDeviceToHost, s));
+    GB_HIP(eng, hipStreamSynchronize(s));
+
+    // Preorder layout: left child next, right child at a relative offset.
+    const uint32_t total = (uint32_t)nodes.size();
+    std::vector<uint32_t> words(8 * (size_t)total, 0u);
+    std::vector<std::pair<uint32_t, uint32_t>> st{{0u, ~0u}};  // (node, preorder parent whose right child it is)
+    uint32_t slot = 0;
+    while (!st.empty()) {
+      auto [id, parent] = st.back();
+      st.pop_back();
+      const uint32_t at = slot++;
+      if (parent != ~0u) words[8 * (size_t)parent + 6] = at - parent;
+      const GNode& g = nodes[id];
+      uint32_t* w = &words[8 * (size_t)at];
+      std::memcpy(w, g.box, 24);
+      if (g.left == ~0u) {
+        w[6] = g.begin;
+        w[7] = vt::kLeafFlag | (g.end - g.begin);
+        continue;
+      }
+      w[7] = g.axis;
+      st.push_back({g.right, at});
+      st.push_back({g.left, ~0u});
+    }
+    if (vt::BuildError e = vt::validate_bvh(words.data(), total, nullptr, 0, n_tri, vt::kBvhStackDepth))
+      return engine_error(eng, e.code, e.message);
+    std::memcpy(out_node_words, words.data(), words.size() * 4);
+    *out_node_count = total;
+    for (uint32_t nt = 0; nt < n_tri; ++nt) {
+      const uint32_t ot = perm[nt];
+      for (uint32_t k = 0; k < index_stride; ++k)
+        out_reordered_indices[(size_t)nt * index_stride + k] =
+            k < 3 ? indices[(size_t)ot * index_stride + k] : 0u;
+      if (out_reordered_mat) out_reordered_mat[nt] = tri_material_ids ? tri_material_ids[ot] : 0u;
+      if (out_reordered_to_source) out_reordered_to_source[nt] = ot;
+    }
+    return VT_OK;
+  });
+}

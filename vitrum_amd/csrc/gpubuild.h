@@ -1,0 +1,58 @@
+// gpubuild.h -- binned-SAH BLAS construction on the device.
+//
+// The same trees as the host builder (builder.cpp, buildArrayBvh.ts:342-780),
+// byte for byte, built one tree level per round for every BLAS of a scene at
+// once: per level one workgroup per node computes its box and centroid box
+// and, for a node that is to be split, bins its triangles (16 bins per axis,
+// binary64 bin index) with integer atomics on order keys in LDS and sweeps
+// the bins on one thread in the host builder's order (first strict cost <,
+// axis-major); a second kernel partitions the node's range of the
+// triangle permutation stably (bin <= split first).  Box minima and maxima
+// are JavaScript Math.min / Math.max over order keys, so they do not depend
+// on the order the atomics land.  The host keeps the tree's shape (a few
+// words per node per level), sorts the rare median-fallback ranges
+// (buildArrayBvh.ts:677-694) as the host builder does, and writes the
+// preorder node array at the end.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace vtg {
+
+struct LevelArgs {
+  const uint32_t* begin;     // per level node: range of perm
+  const uint32_t* end;
+  const uint8_t* split_wanted;  // 1: more triangles than a leaf holds, depth allows a split
+  const uint32_t* perm;      // triangle permutation (global ids)
+  const float* bmin;         // 3 x n_tri, axis-major
+  const float* bmax;
+  const double* cen;         // 3 x n_tri centroids
+  uint8_t* bin_of;           // 3 x n_tri bin of each triangle on each axis
+  float* node_box;           // out: 6 per level node
+  double* cen_box;           // out: 6 per level node
+  uint32_t* decision;        // out: axis, split bin, SAH split taken
+  uint32_t n_tri, num_bins;
+};
+
+struct PartitionArgs {
+  const uint32_t* nodes;     // level nodes to partition
+  const uint32_t* begin;
+  const uint32_t* end;
+  const uint32_t* decision;
+  const uint8_t* bin_of;
+  uint32_t* perm;
+  uint32_t* tmp;
+  uint32_t* n_left;          // out per level node
+  uint32_t n_tri;
+};
+
+size_t level_lds_bytes(uint32_t num_bins);
+// Vertex indices must be below the vertex count (checked on the host).
+hipError_t launch_tri_records(const float* pos, uint32_t pstride, const uint32_t* idx, uint32_t istride, uint32_t n,
+                              float* bmin, float* bmax, double* cen, hipStream_t s);
+hipError_t launch_level(const LevelArgs& a, uint32_t nodes, hipStream_t s);
+hipError_t launch_partition(const PartitionArgs& a, uint32_t nodes, hipStream_t s);
+
+}  // namespace vtg

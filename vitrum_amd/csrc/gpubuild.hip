@@ -1,0 +1,253 @@
+// gpubuild.hip -- binned-SAH BLAS construction on the device, one tree level
+// per round (see gpubuild.h).
+#include "gpubuild.h"
+
+#include <cfloat>
+
+namespace vtg {
+namespace {
+
+constexpr uint32_t kBlock = 256;
+constexpr uint32_t kKeyPosInf = 0xFF800000u;  // fkey(+inf)
+constexpr uint32_t kKeyNegInf = 0x007FFFFFu;  // fkey(-inf)
+constexpr unsigned long long kDKeyPosInf = 0xFFF0000000000000ull;  // dkey(+inf)
+constexpr unsigned long long kDKeyNegInf = 0x000FFFFFFFFFFFFFull;  // dkey(-inf)
+
+// Math.min / Math.max of binary32 values (-0 is below +0), as order keys: an
+// unsigned key that sorts like the IEEE total order, so an integer minimum
+// of keys is the JavaScript minimum of the values.
+__device__ inline uint32_t fkey(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ inline float unkey(uint32_t k) { return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k); }
+__device__ inline uint64_t dkey(double d) {
+  const uint64_t u = (uint64_t)__double_as_longlong(d);
+  return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ inline double undkey(uint64_t k) {
+  return __longlong_as_double((long long)((k & 0x8000000000000000ull) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k));
+}
+__device__ inline float jminf(float a, float b) { return fkey(a) <= fkey(b) ? a : b; }
+__device__ inline float jmaxf(float a, float b) { return fkey(a) >= fkey(b) ? a : b; }
+__device__ inline double jmax(double a, double b) { return dkey(a) >= dkey(b) ? a : b; }
+
+// surfaceArea with extents clamped at zero (buildArrayBvh.ts:223-231), binary64.
+__device__ inline double half_area(const float* b) {
+  const double dx = jmax(0.0, (double)b[3] - (double)b[0]);
+  const double dy = jmax(0.0, (double)b[4] - (double)b[1]);
+  const double dz = jmax(0.0, (double)b[5] - (double)b[2]);
+  return dx * dy + dy * dz + dz * dx;
+}
+
+// Per-triangle records: binary32 bounds by Math.min / Math.max of the three
+// vertices, binary64 centroid 0.5 (lo + hi) (buildArrayBvh.ts:165-190).
+__global__ __launch_bounds__(kBlock) void tri_records_kernel(const float* __restrict__ pos, uint32_t pstride,
+                                                             const uint32_t* __restrict__ idx, uint32_t istride,
+                                                             uint32_t n, float* __restrict__ bmin,
+                                                             float* __restrict__ bmax, double* __restrict__ cen) {
+  const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n) return;
+  const float* v[3];
+  for (int j = 0; j < 3; ++j) v[j] = pos + pstride * (size_t)idx[istride * (size_t)t + j];
+  for (int k = 0; k < 3; ++k) {
+    const float lo = jminf(jminf(v[0][k], v[1][k]), v[2][k]);
+    const float hi = jmaxf(jmaxf(v[0][k], v[1][k]), v[2][k]);
+    bmin[(size_t)k * n + t] = lo;
+    bmax[(size_t)k * n + t] = hi;
+    cen[(size_t)k * n + t] = 0.5 * ((double)lo + (double)hi);
+  }
+}
+
+// One workgroup per node of the level: the node's box and centroid box, and
+// for a node that is to be split the binned SAH decision.  Bins live in LDS
+// as order keys updated with integer atomics; the sweeps run on one thread
+// in the host builder's order, so ties resolve the same way (first strict <,
+// axis-major, then split order).
+__global__ __launch_bounds__(kBlock) void level_kernel(LevelArgs a) {
+  extern __shared__ uint32_t lds[];
+  const uint32_t node = blockIdx.x;
+  const uint32_t begin = a.begin[node], end = a.end[node];
+  const uint32_t n = end - begin;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t K = a.num_bins;
+  // Dynamic LDS: per axis K bins of (3 min keys, 3 max keys, count), then
+  // the sweep arrays of thread 0 (6K + 6K floats, 2K counts).
+  __shared__ uint32_t box_key[6];
+  __shared__ unsigned long long cen_key[6];
+  __shared__ double s_cmin[3], s_span[3];
+  if (tid < 3) box_key[tid] = kKeyPosInf, box_key[3 + tid] = kKeyNegInf;
+  if (tid < 3) cen_key[tid] = kDKeyPosInf, cen_key[3 + tid] = kDKeyNegInf;
+  uint32_t* bins = lds;  // 3 * K * 7 words
+  for (uint32_t i = tid; i < 3 * K * 7; i += kBlock) {
+    const uint32_t f = i % 7;
+    bins[i] = f < 3 ? kKeyPosInf : (f < 6 ? kKeyNegInf : 0u);
+  }
+  __syncthreads();
+  // node box and centroid box
+  uint32_t lk[6] = {kKeyPosInf, kKeyPosInf, kKeyPosInf, kKeyNegInf, kKeyNegInf, kKeyNegInf};
+  unsigned long long ck[6] = {kDKeyPosInf, kDKeyPosInf, kDKeyPosInf, kDKeyNegInf, kDKeyNegInf, kDKeyNegInf};
+  for (uint32_t i = begin + tid; i < end; i += kBlock) {
+    const uint32_t t = a.perm[i];
+    for (int k = 0; k < 3; ++k) {
+      lk[k] = min(lk[k], fkey(a.bmin[(size_t)k * a.n_tri + t]));
+      lk[3 + k] = max(lk[3 + k], fkey(a.bmax[(size_t)k * a.n_tri + t]));
+      const unsigned long long c = dkey(a.cen[(size_t)k * a.n_tri + t]);
+      ck[k] = min(ck[k], c);
+      ck[3 + k] = max(ck[3 + k], c);
+    }
+  }
+  for (int k = 0; k < 3; ++k) {
+    atomicMin(&box_key[k], lk[k]);
+    atomicMax(&box_key[3 + k], lk[3 + k]);
+    atomicMin(&cen_key[k], ck[k]);
+    atomicMax(&cen_key[3 + k], ck[3 + k]);
+  }
+  __syncthreads();
+  float* nb = a.node_box + 6 * (size_t)node;
+  if (tid < 6) nb[tid] = unkey(box_key[tid]);
+  if (tid < 3) {
+    const double lo = undkey(cen_key[tid]), hi = undkey(cen_key[3 + tid]);
+    a.cen_box[6 * (size_t)node + tid] = lo;
+    a.cen_box[6 * (size_t)node + 3 + tid] = hi;
+    s_cmin[tid] = lo;
+    s_span[tid] = hi - lo;
+  }
+  __syncthreads();
+  if (!a.split_wanted[node]) return;
+  // binning on every axis with a centroid span above 1e-9
+  for (uint32_t i = begin + tid; i < end; i += kBlock) {
+    const uint32_t t = a.perm[i];
+    for (int axis = 0; axis < 3; ++axis) {
+      if (!(s_span[axis] > 1e-9)) continue;
+      const double f = floor((a.cen[(size_t)axis * a.n_tri + t] - s_cmin[axis]) / s_span[axis] * (double)K);
+      const uint32_t b = f < (double)(K - 1) ? (uint32_t)f : K - 1;
+      a.bin_of[(size_t)axis * a.n_tri + t] = (uint8_t)b;
+      uint32_t* bb = bins + (axis * K + b) * 7;
+      for (int k = 0; k < 3; ++k) {
+        atomicMin(&bb[k], fkey(a.bmin[(size_t)k * a.n_tri + t]));
+        atomicMax(&bb[3 + k], fkey(a.bmax[(size_t)k * a.n_tri + t]));
+      }
+      atomicAdd(&bb[6], 1u);
+    }
+  }
+  __syncthreads();
+  if (tid != 0) return;
+  float pbox[6];
+  for (int k = 0; k < 6; ++k) pbox[k] = nb[k];
+  const double parent_sa = half_area(pbox);
+  double best = INFINITY;
+  uint32_t best_axis = 0, best_split = 0;
+  float* pre = reinterpret_cast<float*>(lds + 3 * K * 7);                  // 6K floats
+  float* suf = pre + 6 * K;                                                  // 6K floats
+  int32_t* pre_n = reinterpret_cast<int32_t*>(suf + 6 * K);                 // K
+  int32_t* suf_n = pre_n + K;                                               // K
+  for (uint32_t axis = 0; axis < 3; ++axis) {
+    if (!(s_span[axis] > 1e-9)) continue;
+    const uint32_t* B = bins + axis * K * 7;
+    float acc[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    int32_t cnt = 0;
+    for (uint32_t b = 0; b < K; ++b) {
+      for (int k = 0; k < 3; ++k) {
+        acc[k] = jminf(acc[k], unkey(B[b * 7 + k]));
+        acc[3 + k] = jmaxf(acc[3 + k], unkey(B[b * 7 + 3 + k]));
+      }
+      for (int k = 0; k < 6; ++k) pre[6 * b + k] = acc[k];
+      cnt += (int32_t)B[b * 7 + 6];
+      pre_n[b] = cnt;
+    }
+    for (int k = 0; k < 3; ++k) acc[k] = INFINITY, acc[3 + k] = -INFINITY;
+    cnt = 0;
+    for (int32_t b = (int32_t)K - 1; b >= 0; --b) {
+      for (int k = 0; k < 3; ++k) {
+        acc[k] = jminf(acc[k], unkey(B[b * 7 + k]));
+        acc[3 + k] = jmaxf(acc[3 + k], unkey(B[b * 7 + 3 + k]));
+      }
+      for (int k = 0; k < 6; ++k) suf[6 * b + k] = acc[k];
+      cnt += (int32_t)B[b * 7 + 6];
+      suf_n[b] = cnt;
+    }
+    for (uint32_t s = 0; s + 1 < K; ++s) {
+      const int32_t nl = pre_n[s], nr = suf_n[s + 1];
+      if (nl == 0 || nr == 0) continue;
+      const double raw = half_area(pre + 6 * s) * (double)nl + half_area(suf + 6 * (s + 1)) * (double)nr;
+      const double cost = parent_sa > 0 ? raw / parent_sa : raw;
+      if (cost < best) best = cost, best_axis = axis, best_split = s;
+    }
+  }
+  a.decision[3 * (size_t)node + 0] = best_axis;
+  a.decision[3 * (size_t)node + 1] = best_split;
+  a.decision[3 * (size_t)node + 2] = (best < (double)n && best != INFINITY) ? 1u : 0u;
+}
+
+// One workgroup per node with an SAH split: a stable partition of its
+// segment of the permutation (bin <= split first), through tmp.
+__global__ __launch_bounds__(kBlock) void partition_kernel(PartitionArgs a) {
+  __shared__ uint32_t scan[kBlock];
+  __shared__ uint32_t total_left;
+  const uint32_t node = a.nodes[blockIdx.x];
+  const uint32_t begin = a.begin[node], end = a.end[node], n = end - begin;
+  const uint32_t axis = a.decision[3 * (size_t)node], split = a.decision[3 * (size_t)node + 1];
+  const uint32_t tid = threadIdx.x;
+  const uint8_t* bin = a.bin_of + (size_t)axis * a.n_tri;
+  if (tid == 0) total_left = 0;
+  __syncthreads();
+  uint32_t mine = 0;
+  for (uint32_t i = begin + tid; i < end; i += kBlock) mine += bin[a.perm[i]] <= split ? 1u : 0u;
+  atomicAdd(&total_left, mine);
+  __syncthreads();
+  const uint32_t n_left = total_left;
+  uint32_t base_l = 0, base_r = 0;
+  for (uint32_t c = 0; c < n; c += kBlock) {
+    const uint32_t i = begin + c + tid;
+    const bool in = c + tid < n;
+    const uint32_t t = in ? a.perm[i] : 0u;
+    const uint32_t left = in && bin[t] <= split ? 1u : 0u;
+    scan[tid] = left;
+    __syncthreads();
+    for (uint32_t off = 1; off < kBlock; off <<= 1) {  // inclusive Hillis-Steele scan
+      const uint32_t v = tid >= off ? scan[tid - off] : 0u;
+      __syncthreads();
+      scan[tid] += v;
+      __syncthreads();
+    }
+    const uint32_t incl = scan[tid];
+    const uint32_t chunk_left = scan[kBlock - 1];
+    if (in) {
+      const uint32_t l_before = incl - left;
+      const uint32_t pos = left ? base_l + l_before : n_left + base_r + (tid - l_before);
+      a.tmp[begin + pos] = t;
+    }
+    const uint32_t chunk_n = min(kBlock, n - c);
+    base_l += chunk_left;
+    base_r += chunk_n - chunk_left;
+    __syncthreads();
+  }
+  for (uint32_t i = begin + tid; i < end; i += kBlock) a.perm[i] = a.tmp[i];
+  if (tid == 0) a.n_left[node] = n_left;
+}
+
+}  // namespace
+
+size_t level_lds_bytes(uint32_t num_bins) { return ((size_t)3 * num_bins * 7 + 14 * (size_t)num_bins) * 4; }
+
+hipError_t launch_tri_records(const float* pos, uint32_t pstride, const uint32_t* idx, uint32_t istride, uint32_t n,
+                              float* bmin, float* bmax, double* cen, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  tri_records_kernel<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(pos, pstride, idx, istride, n, bmin, bmax, cen);
+  return hipGetLastError();
+}
+
+hipError_t launch_level(const LevelArgs& a, uint32_t nodes, hipStream_t s) {
+  if (nodes == 0) return hipSuccess;
+  level_kernel<<<nodes, kBlock, level_lds_bytes(a.num_bins), s>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t launch_partition(const PartitionArgs& a, uint32_t nodes, hipStream_t s) {
+  if (nodes == 0) return hipSuccess;
+  partition_kernel<<<nodes, kBlock, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+}  // namespace vtg

@@ -37,7 +37,7 @@ EXPORTED = [
     "vt_nrc_gradient", "vt_nrc_render_config", "vt_nrc_teacher_vertices", "vt_nrc_timing", "vt_batch_create", "vt_batch_destroy",
     "vt_batch_add_scene", "vt_batch_scene_count", "vt_batch_trace_closest_device", "vt_batch_capture",
     "vt_batch_replay", "vt_batch_autotune", "vt_batch_set_tuning",
-    "vt_refit_bvh_bounds_device", "vt_refit_primitive", "vt_scene_copy_field",
+    "vt_refit_bvh_bounds_device", "vt_refit_primitive", "vt_scene_copy_field", "vt_build_blas_device",
 ]
 
 SCENE_BLAS_NODE64, SCENE_BLAS_NODE128, SCENE_TRIANGLES = 64, 65, 66
@@ -290,8 +290,9 @@ def pack_scene(primitives, threads=0):
 
 
 def build_blas(positions, indices, tri_material_ids=None, *, position_stride=4, index_stride=4,
-               max_leaf=4, num_bins=16, max_depth=58):
-    """buildArrayBvh through the product builder (vt_build_blas)."""
+               max_leaf=4, num_bins=16, max_depth=58, engine=None):
+    """buildArrayBvh through the product builder: vt_build_blas on the host,
+    or vt_build_blas_device on `engine`'s GPU when an Engine is given."""
     positions = np.ascontiguousarray(positions, np.float32).ravel()
     indices = np.ascontiguousarray(indices, np.uint32).ravel()
     n = indices.size // index_stride
@@ -302,12 +303,14 @@ def build_blas(positions, indices, tri_material_ids=None, *, position_stride=4, 
     ridx = np.zeros(max(indices.size, 1), np.uint32)
     rmat = np.zeros(max(n, 1), np.uint32)
     rsrc = np.zeros(max(n, 1), np.uint32)
-    check(lib().vt_build_blas(_p(positions, F32P), C.c_uint32(positions.size),
-                              C.c_uint32(position_stride), _p(indices, U32P),
-                              C.c_uint32(indices.size), C.c_uint32(index_stride), _p(mats, U32P),
-                              C.c_uint32(max_leaf), C.c_uint32(num_bins), C.c_uint32(max_depth),
-                              _p(nodes, U32P), C.byref(count), _p(ridx, U32P), _p(rmat, U32P),
-                              _p(rsrc, U32P)))
+    args = (_p(positions, F32P), C.c_uint32(positions.size), C.c_uint32(position_stride), _p(indices, U32P),
+            C.c_uint32(indices.size), C.c_uint32(index_stride), _p(mats, U32P), C.c_uint32(max_leaf),
+            C.c_uint32(num_bins), C.c_uint32(max_depth), _p(nodes, U32P), C.byref(count), _p(ridx, U32P),
+            _p(rmat, U32P), _p(rsrc, U32P))
+    if engine is None:
+        check(lib().vt_build_blas(*args))
+    else:
+        check(lib().vt_build_blas_device(engine.handle, *args), engine.handle)
     c = count.value
     return {"nodes": nodes[: c * 8].copy(), "node_count": c,
             "reordered_indices": ridx[: indices.size].copy(), "reordered_mat": rmat[:n].copy(),
