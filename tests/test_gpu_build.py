@@ -68,9 +68,22 @@ def cases():
     p, i = soup(999, 7)
     yield "stride 3", np.ascontiguousarray(p[:, :3]), np.ascontiguousarray(i[:, :3]), \
         {"position_stride": 3, "index_stride": 3}
+    # ranges above 16384 triangles are reduced, binned and partitioned by
+    # several workgroups: ties, flat axes and the fallback there too
+    p, i = grid(130, 131)
+    yield "wide flat grid", p, i, {"max_leaf": 2}
+    p = np.zeros((3 * 40000, 4), np.float32)
+    p[:, :3] = -2.0
+    i = np.concatenate([np.arange(120000, dtype=np.uint32).reshape(-1, 3), np.zeros((40000, 1), np.uint32)], 1)
+    yield "wide coincident", p, i, {"max_leaf": 8}
+    p, i = soup(70000, 8, zeros=True)
+    yield "wide soup bins 32", p, i, {"num_bins": 32}
+    p, i = soup(50000, 9)
+    p[:25000, :3] = np.round(p[:25000, :3])
+    yield "wide rounded soup leaf 1", p, i, {"max_leaf": 1, "num_bins": 5}
 
 
-@pytest.mark.parametrize("case", list(range(9)))
+@pytest.mark.parametrize("case", list(range(13)))
 def test_device_build_matches_oracle(engine, case):
     name, p, i, kw = list(cases())[case]
     n = i.shape[0]

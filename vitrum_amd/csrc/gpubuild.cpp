@@ -35,12 +35,20 @@ struct GNode {
 struct Scratch {
   DeviceBuffer pos, idx, bmin, bmax, cen, perm, tmp, bin_of;
   DeviceBuffer lbegin, lend, lwant, nbox, cbox, decision, nleft, pnodes;
+  DeviceBuffer wslot, wbox, wcen, wbins, wfirst, wcount, chunks, cleft;
   ~Scratch() {
     for (DeviceBuffer* b : {&pos, &idx, &bmin, &bmax, &cen, &perm, &tmp, &bin_of, &lbegin, &lend, &lwant, &nbox,
-                            &cbox, &decision, &nleft, &pnodes})
+                            &cbox, &decision, &nleft, &pnodes, &wslot, &wbox, &wcen, &wbins, &wfirst, &wcount,
+                            &chunks, &cleft})
       b->release();
   }
 };
+
+// A node with more triangles than kWideMin is reduced, binned and
+// partitioned by several workgroups, kChunk triangles each; smaller nodes
+// take one workgroup.
+constexpr uint32_t kWideMin = 16384;
+constexpr uint32_t kChunk = 2048;
 
 }  // namespace
 
@@ -71,7 +79,9 @@ vt_status vt_build_blas_device(vt_engine* eng, const float* positions, uint32_t 
     GB_HIP(eng, hipSetDevice(eng->device));
     Scratch b;
     const size_t n = n_tri;
-    // A level holds at most n_tri nodes: every node owns at least one triangle.
+    // A level holds at most n_tri nodes: every node owns at least one
+    // triangle; wide nodes own more than kWideMin each.
+    const size_t max_wide = n / kWideMin + 1, max_chunks = n / kChunk + max_wide;
     struct Sz {
       DeviceBuffer* buf;
       size_t bytes;
@@ -80,7 +90,11 @@ vt_status vt_build_blas_device(vt_engine* eng, const float* positions, uint32_t 
                         Sz{&b.bmin, 12 * n}, Sz{&b.bmax, 12 * n}, Sz{&b.cen, 24 * n}, Sz{&b.perm, 4 * n},
                         Sz{&b.tmp, 4 * n}, Sz{&b.bin_of, 3 * n}, Sz{&b.lbegin, 4 * n}, Sz{&b.lend, 4 * n},
                         Sz{&b.lwant, n}, Sz{&b.nbox, 24 * n}, Sz{&b.cbox, 48 * n}, Sz{&b.decision, 12 * n},
-                        Sz{&b.nleft, 4 * n}, Sz{&b.pnodes, 4 * n}})
+                        Sz{&b.nleft, 4 * n}, Sz{&b.pnodes, 4 * n}, Sz{&b.wslot, 4 * n},
+                        Sz{&b.wbox, 24 * max_wide}, Sz{&b.wcen, 48 * max_wide},
+                        Sz{&b.wbins, (size_t)84 * num_bins * max_wide}, Sz{&b.wfirst, 4 * max_wide},
+                        Sz{&b.wcount, 4 * max_wide}, Sz{&b.chunks, 16 * max_chunks},
+                        Sz{&b.cleft, 4 * max_chunks}})
       if (vt_status st = z.buf->ensure(eng, std::max<size_t>(z.bytes, 16))) return st;
     GB_HIP(eng, hipMemcpyAsync(b.pos.ptr, positions, (size_t)position_floats * 4, hipMemcpyHostToDevice, s));
     GB_HIP(eng, hipMemcpyAsync(b.idx.ptr, indices, (size_t)index_words * 4, hipMemcpyHostToDevice, s));
@@ -118,6 +132,34 @@ vt_status vt_build_blas_device(vt_engine* eng, const float* positions, uint32_t 
     pa.tmp = (uint32_t*)b.tmp.ptr;
     pa.n_left = (uint32_t*)b.nleft.ptr;
     pa.n_tri = n_tri;
+    la.wide_slot = (const uint32_t*)b.wslot.ptr;
+    la.w_box_key = (const uint32_t*)b.wbox.ptr;
+    la.w_cen_key = (const unsigned long long*)b.wcen.ptr;
+    la.w_bins = (const uint32_t*)b.wbins.ptr;
+    vtg::WideArgs wa{};
+    uint32_t* d_chunks = (uint32_t*)b.chunks.ptr;  // begin, end, slot, node: max_chunks each
+    wa.c_begin = d_chunks;
+    wa.c_end = d_chunks + max_chunks;
+    wa.c_slot = d_chunks + 2 * max_chunks;
+    wa.c_node = d_chunks + 3 * max_chunks;
+    wa.w_first = (const uint32_t*)b.wfirst.ptr;
+    wa.w_count = (const uint32_t*)b.wcount.ptr;
+    wa.level_begin = la.begin;
+    wa.decision = la.decision;
+    wa.perm = d_perm;
+    wa.tmp = pa.tmp;
+    wa.bmin = d_bmin;
+    wa.bmax = d_bmax;
+    wa.cen = d_cen;
+    wa.bin_of = la.bin_of;
+    wa.w_box_key = (uint32_t*)b.wbox.ptr;
+    wa.w_cen_key = (unsigned long long*)b.wcen.ptr;
+    wa.w_bins = (uint32_t*)b.wbins.ptr;
+    wa.chunk_left = (uint32_t*)b.cleft.ptr;
+    wa.n_left = pa.n_left;
+    wa.n_tri = n_tri;
+    wa.num_bins = num_bins;
+    std::vector<uint32_t> hslot, hchunk, hwfirst, hwcount;
 
     std::vector<GNode> nodes;
     nodes.reserve(2 * n);
@@ -146,9 +188,39 @@ vt_status vt_build_blas_device(vt_engine* eng, const float* positions, uint32_t 
                                   " still contains " + std::to_string(cnt) + " triangles" + extra + ".");
         }
       }
+      // wide nodes and their chunks
+      hslot.assign(L, ~0u);
+      hwfirst.clear(), hwcount.clear();
+      std::vector<uint32_t> cb, ce, cs, cn;
+      for (uint32_t i = 0; i < L; ++i) {
+        if (!hw[i] || he[i] - hb[i] <= kWideMin) continue;
+        const uint32_t w = (uint32_t)hwfirst.size();
+        hslot[i] = w;
+        hwfirst.push_back((uint32_t)cb.size());
+        for (uint32_t x = hb[i]; x < he[i]; x += kChunk) {
+          cb.push_back(x);
+          ce.push_back(std::min(he[i], x + kChunk));
+          cs.push_back(w);
+          cn.push_back(i);
+        }
+        hwcount.push_back((uint32_t)cb.size() - hwfirst.back());
+      }
+      const uint32_t W = (uint32_t)hwfirst.size(), NC = (uint32_t)cb.size();
       GB_HIP(eng, hipMemcpyAsync(b.lbegin.ptr, hb.data(), 4 * (size_t)L, hipMemcpyHostToDevice, s));
       GB_HIP(eng, hipMemcpyAsync(b.lend.ptr, he.data(), 4 * (size_t)L, hipMemcpyHostToDevice, s));
       GB_HIP(eng, hipMemcpyAsync(b.lwant.ptr, hw.data(), (size_t)L, hipMemcpyHostToDevice, s));
+      GB_HIP(eng, hipMemcpyAsync(b.wslot.ptr, hslot.data(), 4 * (size_t)L, hipMemcpyHostToDevice, s));
+      if (W) {
+        hchunk.resize(4 * max_chunks);
+        std::copy(cb.begin(), cb.end(), hchunk.begin());
+        std::copy(ce.begin(), ce.end(), hchunk.begin() + max_chunks);
+        std::copy(cs.begin(), cs.end(), hchunk.begin() + 2 * max_chunks);
+        std::copy(cn.begin(), cn.end(), hchunk.begin() + 3 * max_chunks);
+        GB_HIP(eng, hipMemcpyAsync(d_chunks, hchunk.data(), 16 * max_chunks, hipMemcpyHostToDevice, s));
+        GB_HIP(eng, hipMemcpyAsync(b.wfirst.ptr, hwfirst.data(), 4 * (size_t)W, hipMemcpyHostToDevice, s));
+        GB_HIP(eng, hipMemcpyAsync(b.wcount.ptr, hwcount.data(), 4 * (size_t)W, hipMemcpyHostToDevice, s));
+        GB_HIP(eng, vtg::launch_wide_reduce(wa, W, NC, s));
+      }
       GB_HIP(eng, vtg::launch_level(la, L, s));
       hbox.resize(6 * (size_t)L);
       hdec.resize(3 * (size_t)L);
@@ -162,12 +234,13 @@ vt_status vt_build_blas_device(vt_engine* eng, const float* positions, uint32_t 
       psel.clear();
       for (uint32_t i = 0; i < L; ++i) {
         std::memcpy(nodes[level[i]].box, &hbox[6 * (size_t)i], 24);
-        if (hw[i] && hdec[3 * (size_t)i + 2]) psel.push_back(i);
+        if (hw[i] && hdec[3 * (size_t)i + 2] && hslot[i] == ~0u) psel.push_back(i);
       }
       hnl.assign(L, 0u);
-      if (!psel.empty()) {
+      if (!psel.empty() || W) {
         GB_HIP(eng, hipMemcpyAsync(b.pnodes.ptr, psel.data(), 4 * psel.size(), hipMemcpyHostToDevice, s));
         GB_HIP(eng, vtg::launch_partition(pa, (uint32_t)psel.size(), s));
+        GB_HIP(eng, vtg::launch_wide_partition(wa, NC, s));
         GB_HIP(eng, hipMemcpyAsync(hnl.data(), b.nleft.ptr, 4 * (size_t)L, hipMemcpyDeviceToHost, s));
         GB_HIP(eng, hipStreamSynchronize(s));
       }

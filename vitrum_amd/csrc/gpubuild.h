@@ -33,6 +33,40 @@ struct LevelArgs {
   float* node_box;           // out: 6 per level node
   double* cen_box;           // out: 6 per level node
   uint32_t* decision;        // out: axis, split bin, SAH split taken
+  // Wide nodes (ranges split over several workgroups): wide_slot[node] is
+  // the node's slot in the wide arrays, ~0u for a node the level kernel
+  // reduces and bins alone.  A wide node's box keys, centroid keys and bins
+  // come from the wide kernels.
+  const uint32_t* wide_slot;
+  const uint32_t* w_box_key;             // 6 per wide node
+  const unsigned long long* w_cen_key;   // 6 per wide node
+  const uint32_t* w_bins;                // 3 * num_bins * 7 per wide node
+  uint32_t n_tri, num_bins;
+};
+
+// Chunks of the wide nodes of a level: chunk c covers [c_begin[c], c_end[c])
+// of wide node c_slot[c] (level node c_node[c]); a wide node's chunks are
+// consecutive, from w_first[slot], w_count[slot] of them.
+struct WideArgs {
+  const uint32_t* c_begin;
+  const uint32_t* c_end;
+  const uint32_t* c_slot;
+  const uint32_t* c_node;
+  const uint32_t* w_first;
+  const uint32_t* w_count;
+  const uint32_t* level_begin;  // per level node
+  const uint32_t* decision;     // per level node
+  uint32_t* perm;
+  uint32_t* tmp;
+  const float* bmin;
+  const float* bmax;
+  const double* cen;
+  uint8_t* bin_of;
+  uint32_t* w_box_key;
+  unsigned long long* w_cen_key;
+  uint32_t* w_bins;
+  uint32_t* chunk_left;         // per chunk
+  uint32_t* n_left;             // per level node
   uint32_t n_tri, num_bins;
 };
 
@@ -54,5 +88,11 @@ hipError_t launch_tri_records(const float* pos, uint32_t pstride, const uint32_t
                               float* bmin, float* bmax, double* cen, hipStream_t s);
 hipError_t launch_level(const LevelArgs& a, uint32_t nodes, hipStream_t s);
 hipError_t launch_partition(const PartitionArgs& a, uint32_t nodes, hipStream_t s);
+// Box keys, centroid keys and bins of `wide` wide nodes over `chunks` chunks
+// (before launch_level).
+hipError_t launch_wide_reduce(const WideArgs& a, uint32_t wide, uint32_t chunks, hipStream_t s);
+// Stable partition of the wide nodes whose decision takes the SAH split
+// (after launch_level).
+hipError_t launch_wide_partition(const WideArgs& a, uint32_t chunks, hipStream_t s);
 
 }  // namespace vtg

@@ -40,6 +40,75 @@ __device__ inline double half_area(const float* b) {
   return dx * dy + dy * dz + dz * dx;
 }
 
+__device__ inline void init_bins(uint32_t* bins, uint32_t K) {
+  for (uint32_t i = threadIdx.x; i < 3 * K * 7; i += blockDim.x) {
+    const uint32_t f = i % 7;
+    bins[i] = f < 3 ? kKeyPosInf : (f < 6 ? kKeyNegInf : 0u);
+  }
+}
+
+// Box and centroid-box order keys of perm[i], i = first, first + step, ... < end.
+__device__ inline void range_keys(const uint32_t* perm, const float* bmin, const float* bmax, const double* cen,
+                                  uint32_t n_tri, uint32_t first, uint32_t end, uint32_t step, uint32_t lk[6],
+                                  unsigned long long ck[6]) {
+  for (int k = 0; k < 3; ++k) {
+    lk[k] = kKeyPosInf, lk[3 + k] = kKeyNegInf;
+    ck[k] = kDKeyPosInf, ck[3 + k] = kDKeyNegInf;
+  }
+  for (uint32_t i = first; i < end; i += step) {
+    const uint32_t t = perm[i];
+    for (int k = 0; k < 3; ++k) {
+      lk[k] = min(lk[k], fkey(bmin[(size_t)k * n_tri + t]));
+      lk[3 + k] = max(lk[3 + k], fkey(bmax[(size_t)k * n_tri + t]));
+      const unsigned long long c = dkey(cen[(size_t)k * n_tri + t]);
+      ck[k] = min(ck[k], c);
+      ck[3 + k] = max(ck[3 + k], c);
+    }
+  }
+}
+
+// Bins perm[i] on every axis whose centroid span exceeds 1e-9: bin index
+// floor((c - cmin) / span * K) in binary64, clamped to K - 1
+// (buildArrayBvh.ts:342-780); LDS bins hold order keys and a count.
+__device__ inline void bin_range(const uint32_t* perm, const float* bmin, const float* bmax, const double* cen,
+                                 uint32_t n_tri, uint32_t K, const double* cmin, const double* span, uint32_t first,
+                                 uint32_t end, uint32_t step, uint32_t* bins, uint8_t* bin_of) {
+  for (uint32_t i = first; i < end; i += step) {
+    const uint32_t t = perm[i];
+    for (int axis = 0; axis < 3; ++axis) {
+      if (!(span[axis] > 1e-9)) continue;
+      const double f = floor((cen[(size_t)axis * n_tri + t] - cmin[axis]) / span[axis] * (double)K);
+      const uint32_t b = f < (double)(K - 1) ? (uint32_t)f : K - 1;
+      bin_of[(size_t)axis * n_tri + t] = (uint8_t)b;
+      uint32_t* bb = bins + (axis * K + b) * 7;
+      for (int k = 0; k < 3; ++k) {
+        atomicMin(&bb[k], fkey(bmin[(size_t)k * n_tri + t]));
+        atomicMax(&bb[3 + k], fkey(bmax[(size_t)k * n_tri + t]));
+      }
+      atomicAdd(&bb[6], 1u);
+    }
+  }
+}
+
+// Exclusive prefix count of `flag` over a 256-thread workgroup (four 64-lane
+// waves, ballot per wave); *total receives the workgroup's count.  Every
+// thread must call it.
+__device__ inline uint32_t block_excl_count(bool flag, uint32_t* wave_tot, uint32_t* total) {
+  const uint64_t m = __ballot(flag);
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t before = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+  if (lane == 0) wave_tot[wave] = (uint32_t)__popcll(m);
+  __syncthreads();
+  uint32_t off = 0, sum = 0;
+  for (uint32_t w = 0; w < kBlock / 64; ++w) {
+    off += w < wave ? wave_tot[w] : 0u;
+    sum += wave_tot[w];
+  }
+  __syncthreads();
+  *total = sum;
+  return off + before;
+}
+
 // Per-triangle records: binary32 bounds by Math.min / Math.max of the three
 // vertices, binary64 centroid 0.5 (lo + hi) (buildArrayBvh.ts:165-190).
 __global__ __launch_bounds__(kBlock) void tri_records_kernel(const float* __restrict__ pos, uint32_t pstride,
@@ -76,34 +145,32 @@ __global__ __launch_bounds__(kBlock) void level_kernel(LevelArgs a) {
   __shared__ uint32_t box_key[6];
   __shared__ unsigned long long cen_key[6];
   __shared__ double s_cmin[3], s_span[3];
-  if (tid < 3) box_key[tid] = kKeyPosInf, box_key[3 + tid] = kKeyNegInf;
-  if (tid < 3) cen_key[tid] = kDKeyPosInf, cen_key[3 + tid] = kDKeyNegInf;
   uint32_t* bins = lds;  // 3 * K * 7 words
-  for (uint32_t i = tid; i < 3 * K * 7; i += kBlock) {
-    const uint32_t f = i % 7;
-    bins[i] = f < 3 ? kKeyPosInf : (f < 6 ? kKeyNegInf : 0u);
-  }
-  __syncthreads();
-  // node box and centroid box
-  uint32_t lk[6] = {kKeyPosInf, kKeyPosInf, kKeyPosInf, kKeyNegInf, kKeyNegInf, kKeyNegInf};
-  unsigned long long ck[6] = {kDKeyPosInf, kDKeyPosInf, kDKeyPosInf, kDKeyNegInf, kDKeyNegInf, kDKeyNegInf};
-  for (uint32_t i = begin + tid; i < end; i += kBlock) {
-    const uint32_t t = a.perm[i];
+  const uint32_t ws = a.wide_slot[node];
+  const bool wide = ws != ~0u;  // uniform over the workgroup
+  if (wide) {
+    // reduced and binned by the wide kernels
+    if (tid < 6) box_key[tid] = a.w_box_key[6 * (size_t)ws + tid], cen_key[tid] = a.w_cen_key[6 * (size_t)ws + tid];
+    const uint32_t* gb = a.w_bins + (size_t)ws * 3 * K * 7;
+    for (uint32_t i = tid; i < 3 * K * 7; i += kBlock) bins[i] = gb[i];
+    __syncthreads();
+  } else {
+    if (tid < 3) box_key[tid] = kKeyPosInf, box_key[3 + tid] = kKeyNegInf;
+    if (tid < 3) cen_key[tid] = kDKeyPosInf, cen_key[3 + tid] = kDKeyNegInf;
+    init_bins(bins, K);
+    __syncthreads();
+    // node box and centroid box
+    uint32_t lk[6];
+    unsigned long long ck[6];
+    range_keys(a.perm, a.bmin, a.bmax, a.cen, a.n_tri, begin + tid, end, kBlock, lk, ck);
     for (int k = 0; k < 3; ++k) {
-      lk[k] = min(lk[k], fkey(a.bmin[(size_t)k * a.n_tri + t]));
-      lk[3 + k] = max(lk[3 + k], fkey(a.bmax[(size_t)k * a.n_tri + t]));
-      const unsigned long long c = dkey(a.cen[(size_t)k * a.n_tri + t]);
-      ck[k] = min(ck[k], c);
-      ck[3 + k] = max(ck[3 + k], c);
+      atomicMin(&box_key[k], lk[k]);
+      atomicMax(&box_key[3 + k], lk[3 + k]);
+      atomicMin(&cen_key[k], ck[k]);
+      atomicMax(&cen_key[3 + k], ck[3 + k]);
     }
+    __syncthreads();
   }
-  for (int k = 0; k < 3; ++k) {
-    atomicMin(&box_key[k], lk[k]);
-    atomicMax(&box_key[3 + k], lk[3 + k]);
-    atomicMin(&cen_key[k], ck[k]);
-    atomicMax(&cen_key[3 + k], ck[3 + k]);
-  }
-  __syncthreads();
   float* nb = a.node_box + 6 * (size_t)node;
   if (tid < 6) nb[tid] = unkey(box_key[tid]);
   if (tid < 3) {
@@ -115,23 +182,10 @@ __global__ __launch_bounds__(kBlock) void level_kernel(LevelArgs a) {
   }
   __syncthreads();
   if (!a.split_wanted[node]) return;
-  // binning on every axis with a centroid span above 1e-9
-  for (uint32_t i = begin + tid; i < end; i += kBlock) {
-    const uint32_t t = a.perm[i];
-    for (int axis = 0; axis < 3; ++axis) {
-      if (!(s_span[axis] > 1e-9)) continue;
-      const double f = floor((a.cen[(size_t)axis * a.n_tri + t] - s_cmin[axis]) / s_span[axis] * (double)K);
-      const uint32_t b = f < (double)(K - 1) ? (uint32_t)f : K - 1;
-      a.bin_of[(size_t)axis * a.n_tri + t] = (uint8_t)b;
-      uint32_t* bb = bins + (axis * K + b) * 7;
-      for (int k = 0; k < 3; ++k) {
-        atomicMin(&bb[k], fkey(a.bmin[(size_t)k * a.n_tri + t]));
-        atomicMax(&bb[3 + k], fkey(a.bmax[(size_t)k * a.n_tri + t]));
-      }
-      atomicAdd(&bb[6], 1u);
-    }
+  if (!wide) {
+    bin_range(a.perm, a.bmin, a.bmax, a.cen, a.n_tri, K, s_cmin, s_span, begin + tid, end, kBlock, bins, a.bin_of);
+    __syncthreads();
   }
-  __syncthreads();
   if (tid != 0) return;
   float pbox[6];
   for (int k = 0; k < 6; ++k) pbox[k] = nb[k];
@@ -180,13 +234,35 @@ __global__ __launch_bounds__(kBlock) void level_kernel(LevelArgs a) {
   a.decision[3 * (size_t)node + 2] = (best < (double)n && best != INFINITY) ? 1u : 0u;
 }
 
+// Stable scatter of perm[first, last) of a node with split (axis, split):
+// triangles with bin <= split go to tmp[left_at...], the rest to
+// tmp[right_at...], both in their order.  Every thread of the workgroup
+// calls it.
+__device__ inline void scatter_range(const uint32_t* perm, const uint8_t* bin, uint32_t split, uint32_t first,
+                                     uint32_t last, uint32_t left_at, uint32_t right_at, uint32_t* tmp,
+                                     uint32_t* wave_tot) {
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t c = first; c < last; c += kBlock) {
+    const uint32_t i = c + tid;
+    const bool in = i < last;
+    const uint32_t t = in ? perm[i] : 0u;
+    const bool left = in && bin[t] <= split;
+    uint32_t chunk_left;
+    const uint32_t l_before = block_excl_count(left, wave_tot, &chunk_left);
+    if (in) tmp[left ? left_at + l_before : right_at + (tid - l_before)] = t;
+    const uint32_t chunk_n = min(kBlock, last - c);
+    left_at += chunk_left;
+    right_at += chunk_n - chunk_left;
+  }
+}
+
 // One workgroup per node with an SAH split: a stable partition of its
 // segment of the permutation (bin <= split first), through tmp.
 __global__ __launch_bounds__(kBlock) void partition_kernel(PartitionArgs a) {
-  __shared__ uint32_t scan[kBlock];
+  __shared__ uint32_t wave_tot[kBlock / 64];
   __shared__ uint32_t total_left;
   const uint32_t node = a.nodes[blockIdx.x];
-  const uint32_t begin = a.begin[node], end = a.end[node], n = end - begin;
+  const uint32_t begin = a.begin[node], end = a.end[node];
   const uint32_t axis = a.decision[3 * (size_t)node], split = a.decision[3 * (size_t)node + 1];
   const uint32_t tid = threadIdx.x;
   const uint8_t* bin = a.bin_of + (size_t)axis * a.n_tri;
@@ -197,34 +273,132 @@ __global__ __launch_bounds__(kBlock) void partition_kernel(PartitionArgs a) {
   atomicAdd(&total_left, mine);
   __syncthreads();
   const uint32_t n_left = total_left;
-  uint32_t base_l = 0, base_r = 0;
-  for (uint32_t c = 0; c < n; c += kBlock) {
-    const uint32_t i = begin + c + tid;
-    const bool in = c + tid < n;
-    const uint32_t t = in ? a.perm[i] : 0u;
-    const uint32_t left = in && bin[t] <= split ? 1u : 0u;
-    scan[tid] = left;
-    __syncthreads();
-    for (uint32_t off = 1; off < kBlock; off <<= 1) {  // inclusive Hillis-Steele scan
-      const uint32_t v = tid >= off ? scan[tid - off] : 0u;
-      __syncthreads();
-      scan[tid] += v;
-      __syncthreads();
-    }
-    const uint32_t incl = scan[tid];
-    const uint32_t chunk_left = scan[kBlock - 1];
-    if (in) {
-      const uint32_t l_before = incl - left;
-      const uint32_t pos = left ? base_l + l_before : n_left + base_r + (tid - l_before);
-      a.tmp[begin + pos] = t;
-    }
-    const uint32_t chunk_n = min(kBlock, n - c);
-    base_l += chunk_left;
-    base_r += chunk_n - chunk_left;
-    __syncthreads();
-  }
+  scatter_range(a.perm, bin, split, begin, end, begin, begin + n_left, a.tmp, wave_tot);
+  __syncthreads();
   for (uint32_t i = begin + tid; i < end; i += kBlock) a.perm[i] = a.tmp[i];
   if (tid == 0) a.n_left[node] = n_left;
+}
+
+// ---- wide nodes: one range over many workgroups ---------------------------
+
+__global__ __launch_bounds__(kBlock) void wide_init_kernel(WideArgs a) {
+  const uint32_t w = blockIdx.x, K = a.num_bins;
+  if (threadIdx.x < 3) {
+    a.w_box_key[6 * (size_t)w + threadIdx.x] = kKeyPosInf;
+    a.w_box_key[6 * (size_t)w + 3 + threadIdx.x] = kKeyNegInf;
+    a.w_cen_key[6 * (size_t)w + threadIdx.x] = kDKeyPosInf;
+    a.w_cen_key[6 * (size_t)w + 3 + threadIdx.x] = kDKeyNegInf;
+  }
+  init_bins(a.w_bins + (size_t)w * 3 * K * 7, K);
+}
+
+__global__ __launch_bounds__(kBlock) void wide_box_kernel(WideArgs a) {
+  __shared__ uint32_t box_key[6];
+  __shared__ unsigned long long cen_key[6];
+  const uint32_t c = blockIdx.x, tid = threadIdx.x;
+  if (tid < 3) box_key[tid] = kKeyPosInf, box_key[3 + tid] = kKeyNegInf;
+  if (tid < 3) cen_key[tid] = kDKeyPosInf, cen_key[3 + tid] = kDKeyNegInf;
+  __syncthreads();
+  uint32_t lk[6];
+  unsigned long long ck[6];
+  range_keys(a.perm, a.bmin, a.bmax, a.cen, a.n_tri, a.c_begin[c] + tid, a.c_end[c], kBlock, lk, ck);
+  for (int k = 0; k < 3; ++k) {
+    atomicMin(&box_key[k], lk[k]);
+    atomicMax(&box_key[3 + k], lk[3 + k]);
+    atomicMin(&cen_key[k], ck[k]);
+    atomicMax(&cen_key[3 + k], ck[3 + k]);
+  }
+  __syncthreads();
+  const uint32_t w = a.c_slot[c];
+  if (tid < 3) {
+    atomicMin(&a.w_box_key[6 * (size_t)w + tid], box_key[tid]);
+    atomicMax(&a.w_box_key[6 * (size_t)w + 3 + tid], box_key[3 + tid]);
+    atomicMin(&a.w_cen_key[6 * (size_t)w + tid], cen_key[tid]);
+    atomicMax(&a.w_cen_key[6 * (size_t)w + 3 + tid], cen_key[3 + tid]);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void wide_bin_kernel(WideArgs a) {
+  extern __shared__ uint32_t lds[];
+  __shared__ double s_cmin[3], s_span[3];
+  const uint32_t c = blockIdx.x, tid = threadIdx.x, K = a.num_bins;
+  const uint32_t w = a.c_slot[c];
+  if (tid < 3) {
+    const double lo = undkey(a.w_cen_key[6 * (size_t)w + tid]), hi = undkey(a.w_cen_key[6 * (size_t)w + 3 + tid]);
+    s_cmin[tid] = lo;
+    s_span[tid] = hi - lo;
+  }
+  init_bins(lds, K);
+  __syncthreads();
+  bin_range(a.perm, a.bmin, a.bmax, a.cen, a.n_tri, K, s_cmin, s_span, a.c_begin[c] + tid, a.c_end[c], kBlock, lds,
+            a.bin_of);
+  __syncthreads();
+  uint32_t* gb = a.w_bins + (size_t)w * 3 * K * 7;
+  for (uint32_t i = tid; i < 3 * K * 7; i += kBlock) {
+    const uint32_t f = i % 7;
+    if (lds[i - f + 6] == 0) continue;  // empty bin: nothing to merge
+    if (f < 3)
+      atomicMin(&gb[i], lds[i]);
+    else if (f < 6)
+      atomicMax(&gb[i], lds[i]);
+    else
+      atomicAdd(&gb[i], lds[i]);
+  }
+}
+
+__device__ inline bool wide_split(const WideArgs& a, uint32_t node, uint32_t* axis, uint32_t* split) {
+  *axis = a.decision[3 * (size_t)node];
+  *split = a.decision[3 * (size_t)node + 1];
+  return a.decision[3 * (size_t)node + 2] != 0;
+}
+
+__global__ __launch_bounds__(kBlock) void wide_count_kernel(WideArgs a) {
+  __shared__ uint32_t total;
+  const uint32_t c = blockIdx.x, tid = threadIdx.x;
+  uint32_t axis, split;
+  if (!wide_split(a, a.c_node[c], &axis, &split)) return;
+  const uint8_t* bin = a.bin_of + (size_t)axis * a.n_tri;
+  if (tid == 0) total = 0;
+  __syncthreads();
+  uint32_t mine = 0;
+  for (uint32_t i = a.c_begin[c] + tid; i < a.c_end[c]; i += kBlock) mine += bin[a.perm[i]] <= split ? 1u : 0u;
+  atomicAdd(&total, mine);
+  __syncthreads();
+  if (tid == 0) a.chunk_left[c] = total;
+}
+
+__global__ __launch_bounds__(kBlock) void wide_scatter_kernel(WideArgs a) {
+  __shared__ uint32_t wave_tot[kBlock / 64];
+  __shared__ uint32_t s_before, s_all;
+  const uint32_t c = blockIdx.x, tid = threadIdx.x;
+  const uint32_t node = a.c_node[c], w = a.c_slot[c];
+  uint32_t axis, split;
+  if (!wide_split(a, node, &axis, &split)) return;
+  if (tid == 0) s_before = 0, s_all = 0;
+  __syncthreads();
+  const uint32_t first = a.w_first[w], count = a.w_count[w];
+  uint32_t before = 0, all = 0;
+  for (uint32_t j = tid; j < count; j += kBlock) {
+    const uint32_t v = a.chunk_left[first + j];
+    all += v;
+    before += first + j < c ? v : 0u;
+  }
+  atomicAdd(&s_before, before);
+  atomicAdd(&s_all, all);
+  __syncthreads();
+  const uint32_t nb = a.level_begin[node], cb = a.c_begin[c];
+  const uint32_t n_left = s_all, left_at = nb + s_before;
+  const uint32_t right_at = nb + n_left + (cb - nb - s_before);
+  scatter_range(a.perm, a.bin_of + (size_t)axis * a.n_tri, split, cb, a.c_end[c], left_at, right_at, a.tmp,
+                wave_tot);
+  if (c == first && tid == 0) a.n_left[node] = n_left;
+}
+
+__global__ __launch_bounds__(kBlock) void wide_copy_kernel(WideArgs a) {
+  const uint32_t c = blockIdx.x;
+  uint32_t axis, split;
+  if (!wide_split(a, a.c_node[c], &axis, &split)) return;
+  for (uint32_t i = a.c_begin[c] + threadIdx.x; i < a.c_end[c]; i += kBlock) a.perm[i] = a.tmp[i];
 }
 
 }  // namespace
@@ -247,6 +421,22 @@ hipError_t launch_level(const LevelArgs& a, uint32_t nodes, hipStream_t s) {
 hipError_t launch_partition(const PartitionArgs& a, uint32_t nodes, hipStream_t s) {
   if (nodes == 0) return hipSuccess;
   partition_kernel<<<nodes, kBlock, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t launch_wide_reduce(const WideArgs& a, uint32_t wide, uint32_t chunks, hipStream_t s) {
+  if (wide == 0) return hipSuccess;
+  wide_init_kernel<<<wide, kBlock, 0, s>>>(a);
+  wide_box_kernel<<<chunks, kBlock, 0, s>>>(a);
+  wide_bin_kernel<<<chunks, kBlock, (size_t)3 * a.num_bins * 7 * 4, s>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t launch_wide_partition(const WideArgs& a, uint32_t chunks, hipStream_t s) {
+  if (chunks == 0) return hipSuccess;
+  wide_count_kernel<<<chunks, kBlock, 0, s>>>(a);
+  wide_scatter_kernel<<<chunks, kBlock, 0, s>>>(a);
+  wide_copy_kernel<<<chunks, kBlock, 0, s>>>(a);
   return hipGetLastError();
 }
 
