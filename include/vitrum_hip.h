@@ -244,6 +244,13 @@ vt_status vt_set_scene(vt_engine *eng, const vt_scene_desc *scene);
  * thread; a host that shares its CPUs passes its own share. */
 vt_status vt_set_build_threads(vt_engine *eng, uint32_t threads);
 
+/* Meshes of at least `min_triangles` triangles get their BLAS from the device
+ * builder (vt_build_blas_device) when a scene is bound; 0 (default) builds
+ * every BLAS on the host.  The bound scene is the same either way (no
+ * counterpart in the reference, whose BLAS build is buildArrayBvh.ts:342 in
+ * JavaScript). */
+vt_status vt_set_device_build(vt_engine *eng, uint32_t min_triangles);
+
 /* Test hook (no counterpart in the reference): each bit of `flags` makes the
  * next matching step fail once -- 1: scene packing throws std::bad_alloc
  * (vt_set_scene on `eng`; vt_pack_scene when `eng` is NULL), 2: the

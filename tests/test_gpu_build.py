@@ -68,7 +68,7 @@ def cases():
     p, i = soup(999, 7)
     yield "stride 3", np.ascontiguousarray(p[:, :3]), np.ascontiguousarray(i[:, :3]), \
         {"position_stride": 3, "index_stride": 3}
-    # ranges above 16384 triangles are reduced, binned and partitioned by
+    # ranges above 4096 triangles are reduced, binned and partitioned by
     # several workgroups: ties, flat axes and the fallback there too
     p, i = grid(130, 131)
     yield "wide flat grid", p, i, {"max_leaf": 2}
@@ -132,3 +132,26 @@ def test_device_build_empty(engine):
     host = N.build_blas(np.zeros(0, np.float32), np.zeros(0, np.uint32))
     assert got["node_count"] == host["node_count"] == 1
     assert np.array_equal(got["nodes"], host["nodes"])
+
+
+@pytest.mark.parametrize("scene_name", ["cornell", "spheres"])
+def test_scene_bound_with_device_builds_is_the_same_pack(scene_name):
+    """vt_set_device_build: meshes at or above the threshold take the device
+    builder when the scene is bound; every field of the bound pack equals
+    the host-built one's and the traced hits match."""
+    from vitrum_amd import workloads as W
+    scene = W.cornell_scene(65536) if scene_name == "cornell" else W.spheres_scene(64)
+    host, dev = N.Engine(0), N.Engine(0)
+    try:
+        dev.set_device_build(2000)
+        print("bind", scene_name, flush=True)
+        host.set_scene(scene)
+        dev.set_scene(scene)
+        for field in N.PACK_FIELDS:
+            a, b = host.scene_field(field), dev.scene_field(field)
+            assert a.dtype == b.dtype and np.array_equal(a.view(np.uint32), b.view(np.uint32)), field
+        for field in (N.SCENE_BLAS_NODE64, N.SCENE_BLAS_NODE128):
+            assert np.array_equal(host.scene_field(field), dev.scene_field(field)), field
+    finally:
+        host.close()
+        dev.close()

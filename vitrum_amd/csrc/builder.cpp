@@ -741,9 +741,11 @@ struct Slice {
   BlasOut blas;
   double lmin[3], lmax[3];
   BuildError err;
+  bool deferred = false;  // BLAS left to the large-mesh builder
+  std::vector<uint32_t> lidx, lmat;
 };
 
-void build_slice(const PrimitiveIn& p, Slice& s) {
+void build_slice(const PrimitiveIn& p, Slice& s, uint32_t defer_min) {
   const uint32_t vcount = p.position_floats / 3;
   if (vcount < 3) return;
   const uint32_t idx_len = p.indices ? p.index_count : vcount;
@@ -763,8 +765,14 @@ void build_slice(const PrimitiveIn& p, Slice& s) {
   for (uint32_t t = 0; t < tcount; ++t)
     for (int k = 0; k < 3; ++k)
       lidx[4 * (size_t)t + k] = p.indices ? p.indices[3 * (size_t)t + k] : 3 * t + k;
-  s.err = build_blas(s.lpos.data(), vcount * 4, 4, lidx.data(), tcount * 4, 4, lmat.data(), 4, 16,
-                     kMaxBuildDepth, s.blas);
+  if (defer_min && tcount >= defer_min) {
+    s.deferred = true;
+    s.lidx.swap(lidx);
+    s.lmat.swap(lmat);
+  } else {
+    s.err = build_blas(s.lpos.data(), vcount * 4, 4, lidx.data(), tcount * 4, 4, lmat.data(), 4, 16,
+                       kMaxBuildDepth, s.blas);
+  }
   local_aabb(s.lpos.data(), vcount, s.lmin, s.lmax);
   s.present = true;
 }
@@ -864,14 +872,16 @@ BuildError refit_tlas_instances(std::vector<uint32_t>& nodes, const std::vector<
   return {};
 }
 
-BuildError pack_scene(const PrimitiveIn* prims, uint32_t prim_count, uint32_t threads, Pack& out) {
+BuildError pack_scene(const PrimitiveIn* prims, uint32_t prim_count, uint32_t threads, Pack& out,
+                      const BlasBuilder* large, uint32_t large_min_triangles) {
   out = Pack();
   std::vector<Slice> slices(prim_count);
   if (threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
   threads = std::min<uint32_t>(threads, std::max<uint32_t>(1, prim_count));
   std::atomic<uint32_t> next{0};
   auto worker = [&]() {
-    for (uint32_t i; (i = next.fetch_add(1)) < prim_count;) build_slice(prims[i], slices[i]);
+    for (uint32_t i; (i = next.fetch_add(1)) < prim_count;)
+      build_slice(prims[i], slices[i], large ? std::max(1u, large_min_triangles) : 0u);
   };
   if (threads <= 1) {
     worker();
@@ -879,6 +889,12 @@ BuildError pack_scene(const PrimitiveIn* prims, uint32_t prim_count, uint32_t th
     std::vector<std::thread> pool;
     for (uint32_t t = 0; t < threads; ++t) pool.emplace_back(worker);
     for (auto& th : pool) th.join();
+  }
+  for (Slice& s : slices) {
+    if (!s.deferred) continue;
+    s.err = (*large)(s.lpos.data(), s.vcount * 4, s.lidx.data(), s.tcount * 4, s.lmat.data(), s.blas);
+    std::vector<uint32_t>().swap(s.lidx);
+    std::vector<uint32_t>().swap(s.lmat);
   }
   std::vector<double> inst_mn, inst_mx;
   std::vector<uint32_t> inst_root, roots;

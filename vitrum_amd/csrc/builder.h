@@ -10,6 +10,7 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -95,7 +96,17 @@ struct Pack {
   double world_min[3], world_max[3];
 };
 
-BuildError pack_scene(const PrimitiveIn* prims, uint32_t prim_count, uint32_t threads, Pack& out);
+// A BLAS builder for large meshes (the device builder) with build_blas's
+// contract, called with the stride-4 inputs, 4-triangle leaves, 16 bins and
+// maxDepth 58 that packing uses.
+using BlasBuilder = std::function<BuildError(const float* pos4, uint32_t position_floats, const uint32_t* idx4,
+                                             uint32_t index_words, const uint32_t* tri_material_ids, BlasOut& out)>;
+
+// Meshes of at least large_min_triangles triangles are built by `large`
+// (when given) on the calling thread after the host threads finish the rest;
+// the pack is the same either way.
+BuildError pack_scene(const PrimitiveIn* prims, uint32_t prim_count, uint32_t threads, Pack& out,
+                      const BlasBuilder* large = nullptr, uint32_t large_min_triangles = 0);
 
 // computeLocalAabb (scenePack.ts:172-195) of `count` stride-4 vertices, in
 // binary64 with Math.min / Math.max.

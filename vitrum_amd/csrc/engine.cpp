@@ -16,6 +16,7 @@
 #include "builder.h"
 #include "device_layout.h"
 #include "engine_internal.h"
+#include "gpubuild.h"
 #include "traverse.h"
 
 namespace {
@@ -71,6 +72,15 @@ vt_status engine_error(vt_engine* e, vt_status code, const std::string& msg) { r
 uint32_t engine_build_threads(const vt_engine* e) {
   const uint32_t hw = std::max(1u, std::thread::hardware_concurrency());
   return e && e->build_threads ? std::min(e->build_threads, hw) : hw;
+}
+
+vt::BuildError engine_pack_scene(vt_engine* e, const vt::PrimitiveIn* prims, uint32_t count, vt::Pack& out) {
+  if (!e->device_build_min) return vt::pack_scene(prims, count, engine_build_threads(e), out);
+  const vt::BlasBuilder dev = [e](const float* pos4, uint32_t floats, const uint32_t* idx4, uint32_t words,
+                                  const uint32_t* mats, vt::BlasOut& o) {
+    return vtg::build_blas_device(e, pos4, floats, 4, idx4, words, 4, mats, 4, 16, vt::kMaxBuildDepth, o);
+  };
+  return vt::pack_scene(prims, count, engine_build_threads(e), out, &dev, e->device_build_min);
 }
 
 // ---------------------------------------------------------------------------
@@ -180,7 +190,7 @@ vt_status engine_build_trace_scene(vt_engine* eng, const vt_scene_desc* scene, S
   std::vector<vt::PrimitiveIn> prims(scene->primitive_count);
   for (uint32_t i = 0; i < scene->primitive_count; ++i) prims[i] = to_prim(scene->primitives[i]);
   vt::Pack pack;
-  vt::BuildError be = vt::pack_scene(prims.data(), (uint32_t)prims.size(), engine_build_threads(eng), pack);
+  vt::BuildError be = engine_pack_scene(eng, prims.data(), (uint32_t)prims.size(), pack);
   if (be) return set_err(eng, be.code, be.message);
   std::string le = vt::build_device_layout(pack, layout);
   if (!le.empty()) return set_err(eng, VT_ERR_RANGE, le);
@@ -334,7 +344,7 @@ vt_status vt_set_scene(vt_engine* eng, const vt_scene_desc* scene) {
     std::vector<vt::PrimitiveIn> prims(scene->primitive_count);
     for (uint32_t i = 0; i < scene->primitive_count; ++i) prims[i] = to_prim(scene->primitives[i]);
     auto pack = std::make_unique<vt::Pack>();
-    vt::BuildError be = vt::pack_scene(prims.data(), (uint32_t)prims.size(), engine_build_threads(eng), *pack);
+    vt::BuildError be = engine_pack_scene(eng, prims.data(), (uint32_t)prims.size(), *pack);
     if (be) return set_err(eng, be.code, be.message);
     auto layout = std::make_unique<vt::DeviceLayout>();
     std::string le = vt::build_device_layout(*pack, *layout);
@@ -396,6 +406,12 @@ vt_status vt_set_scene(vt_engine* eng, const vt_scene_desc* scene) {
 vt_status vt_set_build_threads(vt_engine* eng, uint32_t threads) {
   if (!eng) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "engine is NULL");
   eng->build_threads = threads;
+  return VT_OK;
+}
+
+vt_status vt_set_device_build(vt_engine* eng, uint32_t min_triangles) {
+  if (!eng) return set_err(nullptr, VT_ERR_INVALID_ARGUMENT, "engine is NULL");
+  eng->device_build_min = min_triangles;
   return VT_OK;
 }
 

@@ -106,6 +106,7 @@ struct vt_engine {
   std::string last_error;
   uint32_t fail_inject = 0;    // kInject* bits
   uint32_t build_threads = 0;  // host threads of the scene builder (0: every hardware thread)
+  uint32_t device_build_min = 0;  // meshes with this many triangles build on the device (0: none)
   void* noalloc_scratch = nullptr;  // this device's copy of noalloc.hip's scratch array
   RefitState* refit = nullptr;
   // Counts scene changes (vt_set_scene, refits): recorded frame graphs
@@ -119,6 +120,9 @@ vt_status engine_noalloc_setup(vt_engine* e);
 
 // Threads the scene builder uses for this engine.
 uint32_t engine_build_threads(const vt_engine* e);
+// pack_scene with the engine's builder settings: host threads, and the
+// device builder for meshes of at least device_build_min triangles.
+vt::BuildError engine_pack_scene(vt_engine* e, const vt::PrimitiveIn* prims, uint32_t count, vt::Pack& out);
 
 vt_status engine_upload(vt_engine* e, DeviceBuffer& dst, const void* src, size_t bytes);
 // Uploads the traversal buffers of a device layout (Node64 + Node128 trees,

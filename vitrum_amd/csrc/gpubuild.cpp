@@ -21,8 +21,7 @@ namespace {
 #define GB_HIP(eng, expr)                                                                  \
   do {                                                                                     \
     hipError_t _e = (expr);                                                                \
-    if (_e != hipSuccess)                                                                  \
-      return engine_error((eng), VT_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    if (_e != hipSuccess) return vt::BuildError{VT_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)}; \
   } while (0)
 
 struct GNode {
@@ -47,33 +46,29 @@ struct Scratch {
 // A node with more triangles than kWideMin is reduced, binned and
 // partitioned by several workgroups, kChunk triangles each; smaller nodes
 // take one workgroup.
-constexpr uint32_t kWideMin = 16384;
-constexpr uint32_t kChunk = 2048;
+constexpr uint32_t kWideMin = 4096;
+constexpr uint32_t kChunk = 1024;
 
 }  // namespace
 
-// Same arguments, checks, outputs and errors as vt_build_blas; the SAH work
-// runs on the engine's device.
-vt_status vt_build_blas_device(vt_engine* eng, const float* positions, uint32_t position_floats,
+namespace vtg {
+
+vt::BuildError build_blas_device(vt_engine* eng, const float* positions, uint32_t position_floats,
                                uint32_t position_stride, const uint32_t* indices, uint32_t index_words,
                                uint32_t index_stride, const uint32_t* tri_material_ids, uint32_t max_leaf,
-                               uint32_t num_bins, uint32_t max_depth, uint32_t* out_node_words,
-                               uint32_t* out_node_count, uint32_t* out_reordered_indices,
-                               uint32_t* out_reordered_mat, uint32_t* out_reordered_to_source) {
-  return vt_guarded(eng, [&]() -> vt_status {
-    if (!eng) return engine_error(nullptr, VT_ERR_INVALID_ARGUMENT, "NULL engine");
-    if ((position_floats && !positions) || (index_words && !indices) || !out_node_words || !out_node_count ||
-        (index_words && !out_reordered_indices))
-      return engine_error(eng, VT_ERR_INVALID_ARGUMENT, "NULL argument");
+                               uint32_t num_bins, uint32_t max_depth, vt::BlasOut& out) {
     if (vt::BuildError e = vt::check_blas_inputs(positions, position_floats, position_stride, indices,
                                                  index_words, index_stride, max_leaf, num_bins, max_depth))
-      return engine_error(eng, e.code, e.message);
+      return e;
     const uint32_t n_tri = index_words / index_stride;
+    out.nodes.clear();
     if (n_tri == 0) {
-      std::memset(out_node_words, 0, 32);
-      out_node_words[7] = vt::kLeafFlag;
-      *out_node_count = 1;
-      return VT_OK;
+      out.nodes.assign(8, 0u);
+      out.nodes[7] = vt::kLeafFlag;
+      out.reordered_indices.assign(indices, indices + index_words);
+      out.reordered_mat.clear();
+      out.reordered_to_source.clear();
+      return {};
     }
     hipStream_t s = eng->stream;
     GB_HIP(eng, hipSetDevice(eng->device));
@@ -95,7 +90,8 @@ vt_status vt_build_blas_device(vt_engine* eng, const float* positions, uint32_t 
                         Sz{&b.wbins, (size_t)84 * num_bins * max_wide}, Sz{&b.wfirst, 4 * max_wide},
                         Sz{&b.wcount, 4 * max_wide}, Sz{&b.chunks, 16 * max_chunks},
                         Sz{&b.cleft, 4 * max_chunks}})
-      if (vt_status st = z.buf->ensure(eng, std::max<size_t>(z.bytes, 16))) return st;
+      if (vt_status st = z.buf->ensure(eng, std::max<size_t>(z.bytes, 16)))
+        return vt::BuildError{st, "device BLAS build: hipMalloc(" + std::to_string(z.bytes) + ") failed"};
     GB_HIP(eng, hipMemcpyAsync(b.pos.ptr, positions, (size_t)position_floats * 4, hipMemcpyHostToDevice, s));
     GB_HIP(eng, hipMemcpyAsync(b.idx.ptr, indices, (size_t)index_words * 4, hipMemcpyHostToDevice, s));
     std::vector<uint32_t> perm(n_tri);
@@ -182,10 +178,10 @@ vt_status vt_build_blas_device(vt_engine* eng, const float* positions, uint32_t 
           // Every node of a level has the same depth, so the first such node
           // in level order is the first the depth-first builder meets.
           std::string extra = cnt > 0xFFFFu ? " and the packed 16-bit limit (0xFFFF = 65535)" : "";
-          return engine_error(eng, VT_ERR_BUILD,
+          return vt::BuildError{VT_ERR_BUILD,
                               "maxDepth=" + std::to_string(max_depth) + " prevents enforcing maxLeafTriangles=" +
                                   std::to_string(max_leaf) + ": depth " + std::to_string(g.depth) +
-                                  " still contains " + std::to_string(cnt) + " triangles" + extra + ".");
+                                  " still contains " + std::to_string(cnt) + " triangles" + extra + "."};
         }
       }
       // wide nodes and their chunks
@@ -310,18 +306,46 @@ vt_status vt_build_blas_device(vt_engine* eng, const float* positions, uint32_t 
       st.push_back({g.right, at});
       st.push_back({g.left, ~0u});
     }
-    if (vt::BuildError e = vt::validate_bvh(words.data(), total, nullptr, 0, n_tri, vt::kBvhStackDepth))
-      return engine_error(eng, e.code, e.message);
-    std::memcpy(out_node_words, words.data(), words.size() * 4);
-    *out_node_count = total;
+    out.nodes.swap(words);
+    out.reordered_indices.assign((size_t)n_tri * index_stride, 0u);
+    out.reordered_mat.resize(n_tri);
     for (uint32_t nt = 0; nt < n_tri; ++nt) {
       const uint32_t ot = perm[nt];
-      for (uint32_t k = 0; k < index_stride; ++k)
-        out_reordered_indices[(size_t)nt * index_stride + k] =
-            k < 3 ? indices[(size_t)ot * index_stride + k] : 0u;
-      if (out_reordered_mat) out_reordered_mat[nt] = tri_material_ids ? tri_material_ids[ot] : 0u;
-      if (out_reordered_to_source) out_reordered_to_source[nt] = ot;
+      for (int k = 0; k < 3; ++k)
+        out.reordered_indices[(size_t)nt * index_stride + k] = indices[(size_t)ot * index_stride + k];
+      out.reordered_mat[nt] = tri_material_ids ? tri_material_ids[ot] : 0u;
     }
+    out.reordered_to_source.swap(perm);
+    return vt::validate_bvh(out.nodes.data(), total, nullptr, 0, n_tri, vt::kBvhStackDepth);
+}
+
+}  // namespace vtg
+
+// Same arguments, checks, outputs and errors as vt_build_blas; the SAH work
+// runs on the engine's device.
+vt_status vt_build_blas_device(vt_engine* eng, const float* positions, uint32_t position_floats,
+                               uint32_t position_stride, const uint32_t* indices, uint32_t index_words,
+                               uint32_t index_stride, const uint32_t* tri_material_ids, uint32_t max_leaf,
+                               uint32_t num_bins, uint32_t max_depth, uint32_t* out_node_words,
+                               uint32_t* out_node_count, uint32_t* out_reordered_indices,
+                               uint32_t* out_reordered_mat, uint32_t* out_reordered_to_source) {
+  return vt_guarded(eng, [&]() -> vt_status {
+    if (!eng) return engine_error(nullptr, VT_ERR_INVALID_ARGUMENT, "NULL engine");
+    if ((position_floats && !positions) || (index_words && !indices) || !out_node_words || !out_node_count ||
+        (index_words && !out_reordered_indices))
+      return engine_error(eng, VT_ERR_INVALID_ARGUMENT, "NULL argument");
+    vt::BlasOut o;
+    vt::BuildError e = vtg::build_blas_device(eng, positions, position_floats, position_stride, indices, index_words,
+                                              index_stride, tri_material_ids, max_leaf, num_bins, max_depth, o);
+    if (e) return engine_error(eng, e.code, e.message);
+    std::memcpy(out_node_words, o.nodes.data(), o.nodes.size() * 4);
+    *out_node_count = (uint32_t)(o.nodes.size() / 8);
+    if (!o.reordered_indices.empty())
+      std::memcpy(out_reordered_indices, o.reordered_indices.data(), o.reordered_indices.size() * 4);
+    if (out_reordered_mat && !o.reordered_mat.empty())
+      std::memcpy(out_reordered_mat, o.reordered_mat.data(), o.reordered_mat.size() * 4);
+    if (out_reordered_to_source && !o.reordered_to_source.empty())
+      std::memcpy(out_reordered_to_source, o.reordered_to_source.data(), o.reordered_to_source.size() * 4);
     return VT_OK;
   });
 }

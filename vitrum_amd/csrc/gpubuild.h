@@ -19,6 +19,10 @@
 
 #include <cstdint>
 
+#include "builder.h"
+
+struct vt_engine;
+
 namespace vtg {
 
 struct LevelArgs {
@@ -94,5 +98,12 @@ hipError_t launch_wide_reduce(const WideArgs& a, uint32_t wide, uint32_t chunks,
 // Stable partition of the wide nodes whose decision takes the SAH split
 // (after launch_level).
 hipError_t launch_wide_partition(const WideArgs& a, uint32_t chunks, hipStream_t s);
+
+// vt::build_blas with the SAH work on `eng`'s device and stream: the same
+// checks, errors and output, byte for byte.
+vt::BuildError build_blas_device(vt_engine* eng, const float* positions, uint32_t position_floats,
+                                 uint32_t position_stride, const uint32_t* indices, uint32_t index_words,
+                                 uint32_t index_stride, const uint32_t* tri_material_ids, uint32_t max_leaf,
+                                 uint32_t num_bins, uint32_t max_depth, vt::BlasOut& out);
 
 }  // namespace vtg

@@ -141,7 +141,7 @@ __global__ __launch_bounds__(kBlock) void level_kernel(LevelArgs a) {
   const uint32_t tid = threadIdx.x;
   const uint32_t K = a.num_bins;
   // Dynamic LDS: per axis K bins of (3 min keys, 3 max keys, count), then
-  // the sweep arrays of thread 0 (6K + 6K floats, 2K counts).
+  // the 3 (K - 1) candidate costs.
   __shared__ uint32_t box_key[6];
   __shared__ unsigned long long cen_key[6];
   __shared__ double s_cmin[3], s_span[3];
@@ -186,49 +186,44 @@ __global__ __launch_bounds__(kBlock) void level_kernel(LevelArgs a) {
     bin_range(a.perm, a.bmin, a.bmax, a.cen, a.n_tri, K, s_cmin, s_span, begin + tid, end, kBlock, bins, a.bin_of);
     __syncthreads();
   }
-  if (tid != 0) return;
+  // SAH cost of every (axis, split) candidate, one thread each: the left
+  // box is the Math.min / Math.max union of bins 0..split (exact, so equal
+  // to the host builder's running prefix), the right box that of the rest.
   float pbox[6];
-  for (int k = 0; k < 6; ++k) pbox[k] = nb[k];
+  for (int k = 0; k < 6; ++k) pbox[k] = unkey(box_key[k]);
   const double parent_sa = half_area(pbox);
+  const uint32_t nc = 3 * (K - 1);
+  double* cost = reinterpret_cast<double*>(lds + ((3 * K * 7 + 1) & ~1u));
+  for (uint32_t c = tid; c < nc; c += kBlock) {
+    const uint32_t axis = c / (K - 1), sp = c % (K - 1);
+    double v = INFINITY;
+    if (s_span[axis] > 1e-9) {
+      const uint32_t* B = bins + axis * K * 7;
+      float lb[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+      float rb[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+      int32_t nl = 0, nr = 0;
+      for (uint32_t b = 0; b < K; ++b) {
+        float* box = b <= sp ? lb : rb;
+        for (int k = 0; k < 3; ++k) {
+          box[k] = jminf(box[k], unkey(B[b * 7 + k]));
+          box[3 + k] = jmaxf(box[3 + k], unkey(B[b * 7 + 3 + k]));
+        }
+        (b <= sp ? nl : nr) += (int32_t)B[b * 7 + 6];
+      }
+      if (nl != 0 && nr != 0) {
+        const double raw = half_area(lb) * (double)nl + half_area(rb) * (double)nr;
+        v = parent_sa > 0 ? raw / parent_sa : raw;
+      }
+    }
+    cost[c] = v;
+  }
+  __syncthreads();
+  if (tid != 0) return;
+  // first strictly smaller cost in axis-major, split order
   double best = INFINITY;
   uint32_t best_axis = 0, best_split = 0;
-  float* pre = reinterpret_cast<float*>(lds + 3 * K * 7);                  // 6K floats
-  float* suf = pre + 6 * K;                                                  // 6K floats
-  int32_t* pre_n = reinterpret_cast<int32_t*>(suf + 6 * K);                 // K
-  int32_t* suf_n = pre_n + K;                                               // K
-  for (uint32_t axis = 0; axis < 3; ++axis) {
-    if (!(s_span[axis] > 1e-9)) continue;
-    const uint32_t* B = bins + axis * K * 7;
-    float acc[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-    int32_t cnt = 0;
-    for (uint32_t b = 0; b < K; ++b) {
-      for (int k = 0; k < 3; ++k) {
-        acc[k] = jminf(acc[k], unkey(B[b * 7 + k]));
-        acc[3 + k] = jmaxf(acc[3 + k], unkey(B[b * 7 + 3 + k]));
-      }
-      for (int k = 0; k < 6; ++k) pre[6 * b + k] = acc[k];
-      cnt += (int32_t)B[b * 7 + 6];
-      pre_n[b] = cnt;
-    }
-    for (int k = 0; k < 3; ++k) acc[k] = INFINITY, acc[3 + k] = -INFINITY;
-    cnt = 0;
-    for (int32_t b = (int32_t)K - 1; b >= 0; --b) {
-      for (int k = 0; k < 3; ++k) {
-        acc[k] = jminf(acc[k], unkey(B[b * 7 + k]));
-        acc[3 + k] = jmaxf(acc[3 + k], unkey(B[b * 7 + 3 + k]));
-      }
-      for (int k = 0; k < 6; ++k) suf[6 * b + k] = acc[k];
-      cnt += (int32_t)B[b * 7 + 6];
-      suf_n[b] = cnt;
-    }
-    for (uint32_t s = 0; s + 1 < K; ++s) {
-      const int32_t nl = pre_n[s], nr = suf_n[s + 1];
-      if (nl == 0 || nr == 0) continue;
-      const double raw = half_area(pre + 6 * s) * (double)nl + half_area(suf + 6 * (s + 1)) * (double)nr;
-      const double cost = parent_sa > 0 ? raw / parent_sa : raw;
-      if (cost < best) best = cost, best_axis = axis, best_split = s;
-    }
-  }
+  for (uint32_t c = 0; c < nc; ++c)
+    if (cost[c] < best) best = cost[c], best_axis = c / (K - 1), best_split = c % (K - 1);
   a.decision[3 * (size_t)node + 0] = best_axis;
   a.decision[3 * (size_t)node + 1] = best_split;
   a.decision[3 * (size_t)node + 2] = (best < (double)n && best != INFINITY) ? 1u : 0u;
@@ -403,7 +398,9 @@ __global__ __launch_bounds__(kBlock) void wide_copy_kernel(WideArgs a) {
 
 }  // namespace
 
-size_t level_lds_bytes(uint32_t num_bins) { return ((size_t)3 * num_bins * 7 + 14 * (size_t)num_bins) * 4; }
+size_t level_lds_bytes(uint32_t num_bins) {
+  return (size_t)((3 * num_bins * 7 + 1) & ~1u) * 4 + (size_t)3 * (num_bins - 1) * 8;
+}
 
 hipError_t launch_tri_records(const float* pos, uint32_t pstride, const uint32_t* idx, uint32_t istride, uint32_t n,
                               float* bmin, float* bmax, double* cen, hipStream_t s) {

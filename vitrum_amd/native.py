@@ -38,6 +38,7 @@ EXPORTED = [
     "vt_batch_add_scene", "vt_batch_scene_count", "vt_batch_trace_closest_device", "vt_batch_capture",
     "vt_batch_replay", "vt_batch_autotune", "vt_batch_set_tuning",
     "vt_refit_bvh_bounds_device", "vt_refit_primitive", "vt_scene_copy_field", "vt_build_blas_device",
+    "vt_set_device_build",
 ]
 
 SCENE_BLAS_NODE64, SCENE_BLAS_NODE128, SCENE_TRIANGLES = 64, 65, 66
@@ -450,6 +451,11 @@ class Engine:
         check(lib().vt_scene_copy_field(self._h, C.c_int32(fid), out.ctypes.data_as(C.c_void_p),
                                         C.c_uint64(out.size), C.byref(words)), self._h)
         return out[: words.value]
+
+    def set_device_build(self, min_triangles):
+        """Build the BLAS of meshes with at least `min_triangles` triangles on
+        the GPU when a scene is bound (0: every BLAS on the host)."""
+        check(lib().vt_set_device_build(self._h, C.c_uint32(min_triangles)), self._h)
 
     def set_shade_sort(self, enabled):
         check(lib().vt_set_shade_sort(self._h, C.c_uint32(1 if enabled else 0)), self._h)
