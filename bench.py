@@ -131,6 +131,30 @@ NRC_TRAIN_FLOPS = NRC_FWD_FLOPS + 2 * (3 * 64 + 3 * 64 * 64) + 2 * (4 * 64 * 64 
 BF16_PEAK_TFLOPS = 2500.0
 
 
+def blas_build(eng, native, triangles=1_000_000, reps=3):
+    """Binned-SAH BLAS build of a synthetic triangle soup (standard-normal
+    vertices, seeded): the device builder (vt_build_blas_device, best of
+    `reps`, host buffers in and out) beside the host builder (vt_build_blas,
+    one build); both produce the same nodes and permutation, checked here."""
+    rng = np.random.default_rng(11)
+    pos = np.concatenate([rng.standard_normal((3 * triangles, 3)).astype(np.float32),
+                          np.zeros((3 * triangles, 1), np.float32)], 1)
+    idx = np.concatenate([np.arange(3 * triangles, dtype=np.uint32).reshape(-1, 3),
+                          np.zeros((triangles, 1), np.uint32)], 1)
+    best = float("inf")
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        dev = native.build_blas(pos, idx, engine=eng)
+        best = min(best, time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    host = native.build_blas(pos, idx)
+    host_s = time.perf_counter() - t0
+    same = bool(np.array_equal(dev["nodes"], host["nodes"]) and
+                np.array_equal(dev["reordered_to_source"], host["reordered_to_source"]))
+    return {"triangles": triangles, "device_ms": round(1e3 * best, 2), "host_ms": round(1e3 * host_s, 2),
+            "speedup": round(host_s / best, 2), "identical": same, "nodes": int(dev["node_count"])}
+
+
 def nrc_mlp(eng, reps=10):
     """Cache MLP kernels on their own: 1920x1080 inference queries and one
     training step of 65,536 paths x 4 recorded vertices, HIP-event timed."""
@@ -312,6 +336,7 @@ def main():
     ap.add_argument("--node-width", type=int, default=4, choices=(2, 4),
                     help="traversal node width: 4-wide trees collapsed from the binary BVH (default) or binary")
     ap.add_argument("--no-nrc-mlp", action="store_true", help="skip the cache MLP kernel measurement")
+    ap.add_argument("--no-blas-build", action="store_true", help="skip the BLAS build measurement")
     ap.add_argument("--no-render-rmse", action="store_true",
                     help="skip the plain vs cache time-to-RMSE measurement (about three minutes of GPU time)")
     ap.add_argument("--rmse-size", default="1920x1080")
@@ -432,6 +457,9 @@ def main():
     if not args.no_nrc_mlp:
         result["nrc_mlp"] = nrc_mlp(eng)
         log(f"cache MLP: {result['nrc_mlp']}")
+    if not args.no_blas_build:
+        result["blas_build"] = blas_build(eng, native)
+        log(f"BLAS build: {result['blas_build']}")
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(sc, rays, args.cpu_seconds)
     eng.close()
