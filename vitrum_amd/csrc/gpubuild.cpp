@@ -34,11 +34,11 @@ struct GNode {
 struct Scratch {
   DeviceBuffer pos, idx, bmin, bmax, cen, perm, tmp, bin_of;
   DeviceBuffer lbegin, lend, lwant, nbox, cbox, decision, nleft, pnodes;
-  DeviceBuffer wslot, wbox, wcen, wbins, wfirst, wcount, chunks, cleft;
+  DeviceBuffer wslot, wbox, wcen, wbins, wfirst, wcount, chunks, cleft, lnodes;
   ~Scratch() {
     for (DeviceBuffer* b : {&pos, &idx, &bmin, &bmax, &cen, &perm, &tmp, &bin_of, &lbegin, &lend, &lwant, &nbox,
                             &cbox, &decision, &nleft, &pnodes, &wslot, &wbox, &wcen, &wbins, &wfirst, &wcount,
-                            &chunks, &cleft})
+                            &chunks, &cleft, &lnodes})
       b->release();
   }
 };
@@ -48,6 +48,10 @@ struct Scratch {
 // take one workgroup.
 constexpr uint32_t kWideMin = 4096;
 constexpr uint32_t kChunk = 1024;
+// Nodes of at most kSmallMax triangles share a workgroup, 64 threads each,
+// when four nodes' bins fit the LDS budget.
+constexpr uint32_t kSmallMax = 256;
+constexpr size_t kSmallLdsBudget = 64 * 1024;
 
 }  // namespace
 
@@ -89,7 +93,7 @@ vt::BuildError build_blas_device(vt_engine* eng, const float* positions, uint32_
                         Sz{&b.wbox, 24 * max_wide}, Sz{&b.wcen, 48 * max_wide},
                         Sz{&b.wbins, (size_t)84 * num_bins * max_wide}, Sz{&b.wfirst, 4 * max_wide},
                         Sz{&b.wcount, 4 * max_wide}, Sz{&b.chunks, 16 * max_chunks},
-                        Sz{&b.cleft, 4 * max_chunks}})
+                        Sz{&b.cleft, 4 * max_chunks}, Sz{&b.lnodes, 4 * n}})
       if (vt_status st = z.buf->ensure(eng, std::max<size_t>(z.bytes, 16)))
         return vt::BuildError{st, "device BLAS build: hipMalloc(" + std::to_string(z.bytes) + ") failed"};
     GB_HIP(eng, hipMemcpyAsync(b.pos.ptr, positions, (size_t)position_floats * 4, hipMemcpyHostToDevice, s));
@@ -129,6 +133,8 @@ vt::BuildError build_blas_device(vt_engine* eng, const float* positions, uint32_
     pa.n_left = (uint32_t*)b.nleft.ptr;
     pa.n_tri = n_tri;
     la.wide_slot = (const uint32_t*)b.wslot.ptr;
+    const bool small_ok = vtg::level_lds_bytes(num_bins, 64) <= kSmallLdsBudget;
+    std::vector<uint32_t> hlist;
     la.w_box_key = (const uint32_t*)b.wbox.ptr;
     la.w_cen_key = (const unsigned long long*)b.wcen.ptr;
     la.w_bins = (const uint32_t*)b.wbins.ptr;
@@ -217,7 +223,19 @@ vt::BuildError build_blas_device(vt_engine* eng, const float* positions, uint32_
         GB_HIP(eng, hipMemcpyAsync(b.wcount.ptr, hwcount.data(), 4 * (size_t)W, hipMemcpyHostToDevice, s));
         GB_HIP(eng, vtg::launch_wide_reduce(wa, W, NC, s));
       }
-      GB_HIP(eng, vtg::launch_level(la, L, s));
+      // one workgroup per large or wide node, four small nodes per workgroup
+      hlist.clear();
+      for (uint32_t i = 0; i < L; ++i)
+        if (!small_ok || hslot[i] != ~0u || he[i] - hb[i] > kSmallMax) hlist.push_back(i);
+      const uint32_t n_big = (uint32_t)hlist.size();
+      for (uint32_t i = 0; i < L; ++i)
+        if (small_ok && hslot[i] == ~0u && he[i] - hb[i] <= kSmallMax) hlist.push_back(i);
+      GB_HIP(eng, hipMemcpyAsync(b.lnodes.ptr, hlist.data(), 4 * (size_t)L, hipMemcpyHostToDevice, s));
+      vtg::LevelArgs lb = la;
+      lb.node_list = (const uint32_t*)b.lnodes.ptr;
+      GB_HIP(eng, vtg::launch_level(lb, n_big, 256, s));
+      lb.node_list += n_big;
+      GB_HIP(eng, vtg::launch_level(lb, L - n_big, 64, s));
       hbox.resize(6 * (size_t)L);
       hdec.resize(3 * (size_t)L);
       hcbox.resize(6 * (size_t)L);

@@ -26,6 +26,7 @@ struct vt_engine;
 namespace vtg {
 
 struct LevelArgs {
+  const uint32_t* node_list; // level nodes this launch covers
   const uint32_t* begin;     // per level node: range of perm
   const uint32_t* end;
   const uint8_t* split_wanted;  // 1: more triangles than a leaf holds, depth allows a split
@@ -86,11 +87,14 @@ struct PartitionArgs {
   uint32_t n_tri;
 };
 
-size_t level_lds_bytes(uint32_t num_bins);
+// Dynamic LDS of a level launch with threads_per_node threads per node.
+size_t level_lds_bytes(uint32_t num_bins, uint32_t threads_per_node);
 // Vertex indices must be below the vertex count (checked on the host).
 hipError_t launch_tri_records(const float* pos, uint32_t pstride, const uint32_t* idx, uint32_t istride, uint32_t n,
                               float* bmin, float* bmax, double* cen, hipStream_t s);
-hipError_t launch_level(const LevelArgs& a, uint32_t nodes, hipStream_t s);
+// threads_per_node: 256 (one node per workgroup; wide nodes only here) or
+// 64 (four nodes per workgroup, for small nodes).
+hipError_t launch_level(const LevelArgs& a, uint32_t nodes, uint32_t threads_per_node, hipStream_t s);
 hipError_t launch_partition(const PartitionArgs& a, uint32_t nodes, hipStream_t s);
 // Box keys, centroid keys and bins of `wide` wide nodes over `chunks` chunks
 // (before launch_level).

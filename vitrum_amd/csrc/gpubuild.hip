@@ -40,8 +40,8 @@ __device__ inline double half_area(const float* b) {
   return dx * dy + dy * dz + dz * dx;
 }
 
-__device__ inline void init_bins(uint32_t* bins, uint32_t K) {
-  for (uint32_t i = threadIdx.x; i < 3 * K * 7; i += blockDim.x) {
+__device__ inline void init_bins(uint32_t* bins, uint32_t K, uint32_t first, uint32_t step) {
+  for (uint32_t i = first; i < 3 * K * 7; i += step) {
     const uint32_t f = i % 7;
     bins[i] = f < 3 ? kKeyPosInf : (f < 6 ? kKeyNegInf : 0u);
   }
@@ -128,97 +128,114 @@ __global__ __launch_bounds__(kBlock) void tri_records_kernel(const float* __rest
   }
 }
 
-// One workgroup per node of the level: the node's box and centroid box, and
-// for a node that is to be split the binned SAH decision.  Bins live in LDS
-// as order keys updated with integer atomics; the sweeps run on one thread
-// in the host builder's order, so ties resolve the same way (first strict <,
-// axis-major, then split order).
-__global__ __launch_bounds__(kBlock) void level_kernel(LevelArgs a) {
-  extern __shared__ uint32_t lds[];
-  const uint32_t node = blockIdx.x;
-  const uint32_t begin = a.begin[node], end = a.end[node];
+// LDS words of one node's slot: per axis K bins of (3 min keys, 3 max keys,
+// count), then 3 (K - 1) binary64 candidate costs.
+__host__ __device__ inline uint32_t slot_words(uint32_t K) { return ((3 * K * 7 + 1) & ~1u) + 6 * (K - 1); }
+
+// TPN threads per node, kBlock / TPN nodes per workgroup (node_list[i] for
+// i = blockIdx.x * kBlock / TPN + slot, i < count): the node's box and
+// centroid box, and for a node that is to be split the binned SAH decision.
+// Bins live in LDS as order keys updated with integer atomics; every
+// (axis, split) candidate is costed on its own thread and the first strict
+// minimum is taken in the host builder's order (axis-major, then split).
+// Barriers are uniform over the workgroup: a slot without a node, or whose
+// node is a leaf, runs through them idle.
+template <uint32_t TPN>
+__global__ __launch_bounds__(kBlock) void level_kernel(LevelArgs a, uint32_t count) {
+  constexpr uint32_t kSlots = kBlock / TPN;
+  extern __shared__ unsigned long long lds_raw[];  // 8-byte aligned: the costs are binary64
+  uint32_t* lds = reinterpret_cast<uint32_t*>(lds_raw);
+  __shared__ uint32_t box_key_s[kSlots][6];
+  __shared__ unsigned long long cen_key_s[kSlots][6];
+  __shared__ double cmin_s[kSlots][3], span_s[kSlots][3];
+  const uint32_t slot = threadIdx.x / TPN, lane = threadIdx.x % TPN;
+  const uint32_t li = blockIdx.x * kSlots + slot;
+  const bool live = li < count;
+  const uint32_t node = live ? a.node_list[li] : 0u;
+  const uint32_t begin = live ? a.begin[node] : 0u, end = live ? a.end[node] : 0u;
   const uint32_t n = end - begin;
-  const uint32_t tid = threadIdx.x;
   const uint32_t K = a.num_bins;
-  // Dynamic LDS: per axis K bins of (3 min keys, 3 max keys, count), then
-  // the 3 (K - 1) candidate costs.
-  __shared__ uint32_t box_key[6];
-  __shared__ unsigned long long cen_key[6];
-  __shared__ double s_cmin[3], s_span[3];
-  uint32_t* bins = lds;  // 3 * K * 7 words
-  const uint32_t ws = a.wide_slot[node];
-  const bool wide = ws != ~0u;  // uniform over the workgroup
+  uint32_t* box_key = box_key_s[slot];
+  unsigned long long* cen_key = cen_key_s[slot];
+  double* s_cmin = cmin_s[slot];
+  double* s_span = span_s[slot];
+  uint32_t* bins = lds + slot * slot_words(K);
+  const uint32_t ws = live ? a.wide_slot[node] : ~0u;
+  const bool wide = ws != ~0u;  // only in the one-node-per-workgroup launch
   if (wide) {
     // reduced and binned by the wide kernels
-    if (tid < 6) box_key[tid] = a.w_box_key[6 * (size_t)ws + tid], cen_key[tid] = a.w_cen_key[6 * (size_t)ws + tid];
+    if (lane < 6) box_key[lane] = a.w_box_key[6 * (size_t)ws + lane], cen_key[lane] = a.w_cen_key[6 * (size_t)ws + lane];
     const uint32_t* gb = a.w_bins + (size_t)ws * 3 * K * 7;
-    for (uint32_t i = tid; i < 3 * K * 7; i += kBlock) bins[i] = gb[i];
-    __syncthreads();
+    for (uint32_t i = lane; i < 3 * K * 7; i += TPN) bins[i] = gb[i];
   } else {
-    if (tid < 3) box_key[tid] = kKeyPosInf, box_key[3 + tid] = kKeyNegInf;
-    if (tid < 3) cen_key[tid] = kDKeyPosInf, cen_key[3 + tid] = kDKeyNegInf;
-    init_bins(bins, K);
-    __syncthreads();
+    if (lane < 3) box_key[lane] = kKeyPosInf, box_key[3 + lane] = kKeyNegInf;
+    if (lane < 3) cen_key[lane] = kDKeyPosInf, cen_key[3 + lane] = kDKeyNegInf;
+    init_bins(bins, K, lane, TPN);
+  }
+  __syncthreads();
+  if (!wide) {
     // node box and centroid box
     uint32_t lk[6];
     unsigned long long ck[6];
-    range_keys(a.perm, a.bmin, a.bmax, a.cen, a.n_tri, begin + tid, end, kBlock, lk, ck);
+    range_keys(a.perm, a.bmin, a.bmax, a.cen, a.n_tri, begin + lane, end, TPN, lk, ck);
     for (int k = 0; k < 3; ++k) {
       atomicMin(&box_key[k], lk[k]);
       atomicMax(&box_key[3 + k], lk[3 + k]);
       atomicMin(&cen_key[k], ck[k]);
       atomicMax(&cen_key[3 + k], ck[3 + k]);
     }
-    __syncthreads();
-  }
-  float* nb = a.node_box + 6 * (size_t)node;
-  if (tid < 6) nb[tid] = unkey(box_key[tid]);
-  if (tid < 3) {
-    const double lo = undkey(cen_key[tid]), hi = undkey(cen_key[3 + tid]);
-    a.cen_box[6 * (size_t)node + tid] = lo;
-    a.cen_box[6 * (size_t)node + 3 + tid] = hi;
-    s_cmin[tid] = lo;
-    s_span[tid] = hi - lo;
   }
   __syncthreads();
-  if (!a.split_wanted[node]) return;
-  if (!wide) {
-    bin_range(a.perm, a.bmin, a.bmax, a.cen, a.n_tri, K, s_cmin, s_span, begin + tid, end, kBlock, bins, a.bin_of);
-    __syncthreads();
-  }
-  // SAH cost of every (axis, split) candidate, one thread each: the left
-  // box is the Math.min / Math.max union of bins 0..split (exact, so equal
-  // to the host builder's running prefix), the right box that of the rest.
-  float pbox[6];
-  for (int k = 0; k < 6; ++k) pbox[k] = unkey(box_key[k]);
-  const double parent_sa = half_area(pbox);
-  const uint32_t nc = 3 * (K - 1);
-  double* cost = reinterpret_cast<double*>(lds + ((3 * K * 7 + 1) & ~1u));
-  for (uint32_t c = tid; c < nc; c += kBlock) {
-    const uint32_t axis = c / (K - 1), sp = c % (K - 1);
-    double v = INFINITY;
-    if (s_span[axis] > 1e-9) {
-      const uint32_t* B = bins + axis * K * 7;
-      float lb[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-      float rb[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-      int32_t nl = 0, nr = 0;
-      for (uint32_t b = 0; b < K; ++b) {
-        float* box = b <= sp ? lb : rb;
-        for (int k = 0; k < 3; ++k) {
-          box[k] = jminf(box[k], unkey(B[b * 7 + k]));
-          box[3 + k] = jmaxf(box[3 + k], unkey(B[b * 7 + 3 + k]));
-        }
-        (b <= sp ? nl : nr) += (int32_t)B[b * 7 + 6];
-      }
-      if (nl != 0 && nr != 0) {
-        const double raw = half_area(lb) * (double)nl + half_area(rb) * (double)nr;
-        v = parent_sa > 0 ? raw / parent_sa : raw;
-      }
+  if (live && lane < 6) a.node_box[6 * (size_t)node + lane] = unkey(box_key[lane]);
+  if (lane < 3) {
+    const double lo = undkey(cen_key[lane]), hi = undkey(cen_key[3 + lane]);
+    if (live) {
+      a.cen_box[6 * (size_t)node + lane] = lo;
+      a.cen_box[6 * (size_t)node + 3 + lane] = hi;
     }
-    cost[c] = v;
+    s_cmin[lane] = lo;
+    s_span[lane] = hi - lo;
   }
   __syncthreads();
-  if (tid != 0) return;
+  const bool want = live && a.split_wanted[node];
+  if (want && !wide)
+    bin_range(a.perm, a.bmin, a.bmax, a.cen, a.n_tri, K, s_cmin, s_span, begin + lane, end, TPN, bins, a.bin_of);
+  __syncthreads();
+  // SAH cost of every (axis, split) candidate: the left box is the
+  // Math.min / Math.max union of bins 0..split (exact, so equal to the host
+  // builder's running prefix), the right box that of the rest.
+  const uint32_t nc = 3 * (K - 1);
+  double* cost = reinterpret_cast<double*>(bins + ((3 * K * 7 + 1) & ~1u));
+  if (want) {
+    float pbox[6];
+    for (int k = 0; k < 6; ++k) pbox[k] = unkey(box_key[k]);
+    const double parent_sa = half_area(pbox);
+    for (uint32_t c = lane; c < nc; c += TPN) {
+      const uint32_t axis = c / (K - 1), sp = c % (K - 1);
+      double v = INFINITY;
+      if (s_span[axis] > 1e-9) {
+        const uint32_t* B = bins + axis * K * 7;
+        float lb[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+        float rb[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+        int32_t nl = 0, nr = 0;
+        for (uint32_t b = 0; b < K; ++b) {
+          float* box = b <= sp ? lb : rb;
+          for (int k = 0; k < 3; ++k) {
+            box[k] = jminf(box[k], unkey(B[b * 7 + k]));
+            box[3 + k] = jmaxf(box[3 + k], unkey(B[b * 7 + 3 + k]));
+          }
+          (b <= sp ? nl : nr) += (int32_t)B[b * 7 + 6];
+        }
+        if (nl != 0 && nr != 0) {
+          const double raw = half_area(lb) * (double)nl + half_area(rb) * (double)nr;
+          v = parent_sa > 0 ? raw / parent_sa : raw;
+        }
+      }
+      cost[c] = v;
+    }
+  }
+  __syncthreads();
+  if (!want || lane != 0) return;
   // first strictly smaller cost in axis-major, split order
   double best = INFINITY;
   uint32_t best_axis = 0, best_split = 0;
@@ -284,7 +301,7 @@ __global__ __launch_bounds__(kBlock) void wide_init_kernel(WideArgs a) {
     a.w_cen_key[6 * (size_t)w + threadIdx.x] = kDKeyPosInf;
     a.w_cen_key[6 * (size_t)w + 3 + threadIdx.x] = kDKeyNegInf;
   }
-  init_bins(a.w_bins + (size_t)w * 3 * K * 7, K);
+  init_bins(a.w_bins + (size_t)w * 3 * K * 7, K, threadIdx.x, kBlock);
 }
 
 __global__ __launch_bounds__(kBlock) void wide_box_kernel(WideArgs a) {
@@ -323,7 +340,7 @@ __global__ __launch_bounds__(kBlock) void wide_bin_kernel(WideArgs a) {
     s_cmin[tid] = lo;
     s_span[tid] = hi - lo;
   }
-  init_bins(lds, K);
+  init_bins(lds, K, threadIdx.x, kBlock);
   __syncthreads();
   bin_range(a.perm, a.bmin, a.bmax, a.cen, a.n_tri, K, s_cmin, s_span, a.c_begin[c] + tid, a.c_end[c], kBlock, lds,
             a.bin_of);
@@ -398,8 +415,8 @@ __global__ __launch_bounds__(kBlock) void wide_copy_kernel(WideArgs a) {
 
 }  // namespace
 
-size_t level_lds_bytes(uint32_t num_bins) {
-  return (size_t)((3 * num_bins * 7 + 1) & ~1u) * 4 + (size_t)3 * (num_bins - 1) * 8;
+size_t level_lds_bytes(uint32_t num_bins, uint32_t threads_per_node) {
+  return (size_t)(kBlock / threads_per_node) * slot_words(num_bins) * 4;
 }
 
 hipError_t launch_tri_records(const float* pos, uint32_t pstride, const uint32_t* idx, uint32_t istride, uint32_t n,
@@ -409,9 +426,16 @@ hipError_t launch_tri_records(const float* pos, uint32_t pstride, const uint32_t
   return hipGetLastError();
 }
 
-hipError_t launch_level(const LevelArgs& a, uint32_t nodes, hipStream_t s) {
+hipError_t launch_level(const LevelArgs& a, uint32_t nodes, uint32_t threads_per_node, hipStream_t s) {
   if (nodes == 0) return hipSuccess;
-  level_kernel<<<nodes, kBlock, level_lds_bytes(a.num_bins), s>>>(a);
+  if (threads_per_node == kBlock) {
+    level_kernel<kBlock><<<nodes, kBlock, level_lds_bytes(a.num_bins, kBlock), s>>>(a, nodes);
+  } else if (threads_per_node == 64) {
+    constexpr uint32_t kSlots = kBlock / 64;
+    level_kernel<64><<<(nodes + kSlots - 1) / kSlots, kBlock, level_lds_bytes(a.num_bins, 64), s>>>(a, nodes);
+  } else {
+    return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
