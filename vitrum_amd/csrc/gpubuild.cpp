@@ -240,10 +240,9 @@ vt::BuildError build_blas_device(vt_engine* eng, const float* positions, uint32_
       hdec.resize(3 * (size_t)L);
       hcbox.resize(6 * (size_t)L);
       GB_HIP(eng, hipMemcpyAsync(hbox.data(), b.nbox.ptr, 24 * (size_t)L, hipMemcpyDeviceToHost, s));
-      if (any_split) {
+      if (any_split)
         GB_HIP(eng, hipMemcpyAsync(hdec.data(), b.decision.ptr, 12 * (size_t)L, hipMemcpyDeviceToHost, s));
-        GB_HIP(eng, hipMemcpyAsync(hcbox.data(), b.cbox.ptr, 48 * (size_t)L, hipMemcpyDeviceToHost, s));
-      }
+      bool have_cbox = false;  // centroid boxes: fetched for a median fallback only
       GB_HIP(eng, hipStreamSynchronize(s));
       psel.clear();
       for (uint32_t i = 0; i < L; ++i) {
@@ -268,6 +267,11 @@ vt::BuildError build_blas_device(vt_engine* eng, const float* positions, uint32_
         if (!partitioned) {
           // Median fallback (buildArrayBvh.ts:677-694): widest centroid axis,
           // sort by centroid then triangle id, split at n/2.
+          if (!have_cbox) {
+            GB_HIP(eng, hipMemcpyAsync(hcbox.data(), b.cbox.ptr, 48 * (size_t)L, hipMemcpyDeviceToHost, s));
+            GB_HIP(eng, hipStreamSynchronize(s));
+            have_cbox = true;
+          }
           const double* cb = &hcbox[6 * (size_t)i];
           axis = 0;
           double widest = cb[3] - cb[0];
