@@ -155,3 +155,25 @@ def test_scene_bound_with_device_builds_is_the_same_pack(scene_name):
     finally:
         host.close()
         dev.close()
+
+
+def test_device_build_10m_triangles_matches_host(engine):
+    """Ten million triangles in one mesh (the scale of the largest
+    benchmark scene's triangle count) build on the device to the host
+    builder's output, word for word."""
+    rng = np.random.default_rng(13)
+    n = 10_000_000
+    pos = np.zeros((3 * n, 4), np.float32)
+    centers = rng.random((n, 3), dtype=np.float32) * 100.0
+    for k in range(3):
+        pos[k::3, :3] = centers + rng.standard_normal((n, 3), dtype=np.float32) * 0.05
+    idx = np.zeros((n, 4), np.uint32)
+    idx[:, :3] = np.arange(3 * n, dtype=np.uint32).reshape(-1, 3)
+    print("device build 10M", flush=True)
+    t0 = time.perf_counter()
+    dev = N.build_blas(pos, idx, engine=engine)
+    t1 = time.perf_counter()
+    print(f"device build {1e3 * (t1 - t0):.0f} ms, {dev['node_count']} nodes", flush=True)
+    host = N.build_blas(pos, idx)
+    print(f"host build {1e3 * (time.perf_counter() - t1):.0f} ms", flush=True)
+    assert_same(dev, host, "10M")
