@@ -134,16 +134,18 @@ def test_device_build_empty(engine):
     assert np.array_equal(got["nodes"], host["nodes"])
 
 
-@pytest.mark.parametrize("scene_name", ["cornell", "spheres"])
-def test_scene_bound_with_device_builds_is_the_same_pack(scene_name):
-    """vt_set_device_build: meshes at or above the threshold take the device
-    builder when the scene is bound; every field of the bound pack equals
-    the host-built one's and the traced hits match."""
+@pytest.mark.parametrize("scene_name,threshold", [("cornell", 2000), ("spheres", 2000), ("cornell", 1),
+                                                  ("spheres", 1)])
+def test_scene_bound_with_device_builds_is_the_same_pack(scene_name, threshold):
+    """vt_set_device_build: meshes at or above the threshold are built
+    together on the device (one tree per mesh, grown level by level) when
+    the scene is bound; every field of the bound pack and the device trees
+    equal the host-built ones."""
     from vitrum_amd import workloads as W
     scene = W.cornell_scene(65536) if scene_name == "cornell" else W.spheres_scene(64)
     host, dev = N.Engine(0), N.Engine(0)
     try:
-        dev.set_device_build(2000)
+        dev.set_device_build(threshold)
         print("bind", scene_name, flush=True)
         host.set_scene(scene)
         dev.set_scene(scene)
@@ -152,6 +154,35 @@ def test_scene_bound_with_device_builds_is_the_same_pack(scene_name):
             assert a.dtype == b.dtype and np.array_equal(a.view(np.uint32), b.view(np.uint32)), field
         for field in (N.SCENE_BLAS_NODE64, N.SCENE_BLAS_NODE128):
             assert np.array_equal(host.scene_field(field), dev.scene_field(field)), field
+    finally:
+        host.close()
+        dev.close()
+
+
+def test_forest_of_many_meshes_matches_host():
+    """Several hundred meshes of different sizes (1 to 5000 triangles, some
+    empty or degenerate) bound with every BLAS built on the device: the pack
+    equals the host-built one."""
+    rng = np.random.default_rng(21)
+    prims = []
+    for m in range(300):
+        n = int(rng.integers(1, 5000)) if m % 37 else 1
+        p, i = soup(n, 100 + m)
+        if m % 11 == 0:
+            p[:, :3] = 0.25  # every centroid equal: median fallback
+        pos = np.ascontiguousarray(p[:, :3]).ravel()
+        prims.append({"kind": "mesh", "material": {"baseColor": [0.5, 0.5, 0.5]}, "positions": pos,
+                      "normals": np.zeros_like(pos), "indices": np.ascontiguousarray(i[:, :3]).ravel()})
+    scene = {"primitives": prims}
+    host, dev = N.Engine(0), N.Engine(0)
+    try:
+        dev.set_device_build(1)
+        print("bind 300 meshes", flush=True)
+        host.set_scene(scene)
+        dev.set_scene(scene)
+        for field in N.PACK_FIELDS:
+            a, b = host.scene_field(field), dev.scene_field(field)
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), field
     finally:
         host.close()
         dev.close()

@@ -890,11 +890,27 @@ BuildError pack_scene(const PrimitiveIn* prims, uint32_t prim_count, uint32_t th
     for (uint32_t t = 0; t < threads; ++t) pool.emplace_back(worker);
     for (auto& th : pool) th.join();
   }
-  for (Slice& s : slices) {
-    if (!s.deferred) continue;
-    s.err = (*large)(s.lpos.data(), s.vcount * 4, s.lidx.data(), s.tcount * 4, s.lmat.data(), s.blas);
-    std::vector<uint32_t>().swap(s.lidx);
-    std::vector<uint32_t>().swap(s.lmat);
+  std::vector<Slice*> deferred;
+  for (Slice& s : slices)
+    if (s.deferred) deferred.push_back(&s);
+  if (!deferred.empty()) {
+    std::vector<BlasJobIn> jobs;
+    for (Slice* s : deferred)
+      jobs.push_back({s->lpos.data(), s->vcount * 4, s->lidx.data(), s->tcount * 4, s->lmat.data()});
+    std::vector<BlasOut> outs;
+    const bool ok = !(*large)(jobs, outs) && outs.size() == deferred.size();
+    for (size_t i = 0; i < deferred.size(); ++i) {
+      Slice& s = *deferred[i];
+      if (ok)
+        s.blas.nodes.swap(outs[i].nodes), s.blas.reordered_indices.swap(outs[i].reordered_indices),
+            s.blas.reordered_mat.swap(outs[i].reordered_mat),
+            s.blas.reordered_to_source.swap(outs[i].reordered_to_source);
+      else
+        s.err = build_blas(s.lpos.data(), s.vcount * 4, 4, s.lidx.data(), s.tcount * 4, 4, s.lmat.data(), 4, 16,
+                           kMaxBuildDepth, s.blas);
+      std::vector<uint32_t>().swap(s.lidx);
+      std::vector<uint32_t>().swap(s.lmat);
+    }
   }
   std::vector<double> inst_mn, inst_mx;
   std::vector<uint32_t> inst_root, roots;

@@ -96,15 +96,23 @@ struct Pack {
   double world_min[3], world_max[3];
 };
 
-// A BLAS builder for large meshes (the device builder) with build_blas's
-// contract, called with the stride-4 inputs, 4-triangle leaves, 16 bins and
+// One mesh for a BLAS builder: stride-4 positions and indices.
+struct BlasJobIn {
+  const float* pos4;
+  uint32_t position_floats;
+  const uint32_t* idx4;
+  uint32_t index_words;
+  const uint32_t* mats;
+};
+// A builder of many BLASes at once (the device builder) with build_blas's
+// contract per mesh, called with the 4-triangle leaves, 16 bins and
 // maxDepth 58 that packing uses.
-using BlasBuilder = std::function<BuildError(const float* pos4, uint32_t position_floats, const uint32_t* idx4,
-                                             uint32_t index_words, const uint32_t* tri_material_ids, BlasOut& out)>;
+using BlasBuilder = std::function<BuildError(const std::vector<BlasJobIn>& meshes, std::vector<BlasOut>& outs)>;
 
 // Meshes of at least large_min_triangles triangles are built by `large`
-// (when given) on the calling thread after the host threads finish the rest;
-// the pack is the same either way.
+// (when given), all in one call on the calling thread after the host threads
+// finish the rest; should it fail, they are built on the host.  The pack is
+// the same either way.
 BuildError pack_scene(const PrimitiveIn* prims, uint32_t prim_count, uint32_t threads, Pack& out,
                       const BlasBuilder* large = nullptr, uint32_t large_min_triangles = 0);
 

@@ -76,9 +76,8 @@ uint32_t engine_build_threads(const vt_engine* e) {
 
 vt::BuildError engine_pack_scene(vt_engine* e, const vt::PrimitiveIn* prims, uint32_t count, vt::Pack& out) {
   if (!e->device_build_min) return vt::pack_scene(prims, count, engine_build_threads(e), out);
-  const vt::BlasBuilder dev = [e](const float* pos4, uint32_t floats, const uint32_t* idx4, uint32_t words,
-                                  const uint32_t* mats, vt::BlasOut& o) {
-    return vtg::build_blas_device(e, pos4, floats, 4, idx4, words, 4, mats, 4, 16, vt::kMaxBuildDepth, o);
+  const vt::BlasBuilder dev = [e](const std::vector<vt::BlasJobIn>& meshes, std::vector<vt::BlasOut>& outs) {
+    return vtg::build_blas_forest(e, meshes, 4, 16, vt::kMaxBuildDepth, outs);
   };
   return vt::pack_scene(prims, count, engine_build_threads(e), out, &dev, e->device_build_min);
 }

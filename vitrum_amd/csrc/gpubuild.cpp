@@ -53,27 +53,16 @@ constexpr uint32_t kChunk = 1024;
 constexpr uint32_t kSmallMax = 256;
 constexpr size_t kSmallLdsBudget = 64 * 1024;
 
-}  // namespace
-
-namespace vtg {
-
-vt::BuildError build_blas_device(vt_engine* eng, const float* positions, uint32_t position_floats,
-                               uint32_t position_stride, const uint32_t* indices, uint32_t index_words,
-                               uint32_t index_stride, const uint32_t* tri_material_ids, uint32_t max_leaf,
-                               uint32_t num_bins, uint32_t max_depth, vt::BlasOut& out) {
-    if (vt::BuildError e = vt::check_blas_inputs(positions, position_floats, position_stride, indices,
-                                                 index_words, index_stride, max_leaf, num_bins, max_depth))
-      return e;
+// The level loop over a forest: triangles [roots[r].first, roots[r].second)
+// of the (already checked) inputs form tree r, whose root is nodes[r].  On
+// return `nodes` holds every node and `perm` the final permutation (global
+// triangle ids).
+vt::BuildError build_core(vt_engine* eng, const float* positions, uint32_t position_floats, uint32_t position_stride,
+                          const uint32_t* indices, uint32_t index_words, uint32_t index_stride,
+                          const std::vector<std::pair<uint32_t, uint32_t>>& roots, uint32_t max_leaf,
+                          uint32_t num_bins, uint32_t max_depth, std::vector<GNode>& nodes,
+                          std::vector<uint32_t>& perm) {
     const uint32_t n_tri = index_words / index_stride;
-    out.nodes.clear();
-    if (n_tri == 0) {
-      out.nodes.assign(8, 0u);
-      out.nodes[7] = vt::kLeafFlag;
-      out.reordered_indices.assign(indices, indices + index_words);
-      out.reordered_mat.clear();
-      out.reordered_to_source.clear();
-      return {};
-    }
     hipStream_t s = eng->stream;
     GB_HIP(eng, hipSetDevice(eng->device));
     Scratch b;
@@ -98,7 +87,7 @@ vt::BuildError build_blas_device(vt_engine* eng, const float* positions, uint32_
         return vt::BuildError{st, "device BLAS build: hipMalloc(" + std::to_string(z.bytes) + ") failed"};
     GB_HIP(eng, hipMemcpyAsync(b.pos.ptr, positions, (size_t)position_floats * 4, hipMemcpyHostToDevice, s));
     GB_HIP(eng, hipMemcpyAsync(b.idx.ptr, indices, (size_t)index_words * 4, hipMemcpyHostToDevice, s));
-    std::vector<uint32_t> perm(n_tri);
+    perm.resize(n_tri);
     for (uint32_t t = 0; t < n_tri; ++t) perm[t] = t;
     GB_HIP(eng, hipMemcpyAsync(b.perm.ptr, perm.data(), 4 * n, hipMemcpyHostToDevice, s));
     float* d_bmin = (float*)b.bmin.ptr;
@@ -163,10 +152,13 @@ vt::BuildError build_blas_device(vt_engine* eng, const float* positions, uint32_
     wa.num_bins = num_bins;
     std::vector<uint32_t> hslot, hchunk, hwfirst, hwcount;
 
-    std::vector<GNode> nodes;
+    nodes.clear();
     nodes.reserve(2 * n);
-    nodes.push_back({0, n_tri, 0});
-    std::vector<uint32_t> level{0}, next, hb, he, hdec, hnl, psel;
+    std::vector<uint32_t> level, next, hb, he, hdec, hnl, psel;
+    for (const auto& r : roots) {
+      level.push_back((uint32_t)nodes.size());
+      nodes.push_back({r.first, r.second, 0});
+    }
     std::vector<uint8_t> hw;
     std::vector<float> hbox;
     std::vector<double> hcbox, hcen;  // hcen: centroids, fetched on the first median fallback
@@ -306,21 +298,32 @@ vt::BuildError build_blas_device(vt_engine* eng, const float* positions, uint32_
     GB_HIP(eng, hipMemcpyAsync(perm.data(), d_perm, 4 * n, hipMemcpyDeviceToHost, s));
     GB_HIP(eng, hipStreamSynchronize(s));
 
-    // Preorder layout: left child next, right child at a relative offset.
-    const uint32_t total = (uint32_t)nodes.size();
-    std::vector<uint32_t> words(8 * (size_t)total, 0u);
-    std::vector<std::pair<uint32_t, uint32_t>> st{{0u, ~0u}};  // (node, preorder parent whose right child it is)
+    return {};
+}
+
+// Tree `root` of a forest built by build_core as the host builder's output
+// for its mesh: preorder nodes (left child next, right child at a relative
+// offset), leaf offsets and permutation relative to the mesh's first
+// triangle tri_base, reordered indices and materials from the mesh's own
+// arrays.
+vt::BuildError emit_tree(const std::vector<GNode>& nodes, uint32_t root, const std::vector<uint32_t>& perm,
+                         uint32_t tri_base, uint32_t n_tri, const uint32_t* indices, uint32_t index_stride,
+                         const uint32_t* tri_material_ids, vt::BlasOut& out) {
+    std::vector<uint32_t> words;
+    words.reserve(16 * (size_t)n_tri);
+    std::vector<std::pair<uint32_t, uint32_t>> st{{root, ~0u}};  // (node, preorder parent whose right child it is)
     uint32_t slot = 0;
     while (!st.empty()) {
       auto [id, parent] = st.back();
       st.pop_back();
       const uint32_t at = slot++;
+      words.resize(words.size() + 8, 0u);
       if (parent != ~0u) words[8 * (size_t)parent + 6] = at - parent;
       const GNode& g = nodes[id];
       uint32_t* w = &words[8 * (size_t)at];
       std::memcpy(w, g.box, 24);
       if (g.left == ~0u) {
-        w[6] = g.begin;
+        w[6] = g.begin - tri_base;
         w[7] = vt::kLeafFlag | (g.end - g.begin);
         continue;
       }
@@ -328,21 +331,100 @@ vt::BuildError build_blas_device(vt_engine* eng, const float* positions, uint32_
       st.push_back({g.right, at});
       st.push_back({g.left, ~0u});
     }
+    const uint32_t total = slot;
     out.nodes.swap(words);
     out.reordered_indices.assign((size_t)n_tri * index_stride, 0u);
     out.reordered_mat.resize(n_tri);
+    out.reordered_to_source.resize(n_tri);
     for (uint32_t nt = 0; nt < n_tri; ++nt) {
-      const uint32_t ot = perm[nt];
+      const uint32_t ot = perm[tri_base + nt] - tri_base;
       for (int k = 0; k < 3; ++k)
         out.reordered_indices[(size_t)nt * index_stride + k] = indices[(size_t)ot * index_stride + k];
       out.reordered_mat[nt] = tri_material_ids ? tri_material_ids[ot] : 0u;
+      out.reordered_to_source[nt] = ot;
     }
-    out.reordered_to_source.swap(perm);
     return vt::validate_bvh(out.nodes.data(), total, nullptr, 0, n_tri, vt::kBvhStackDepth);
 }
 
-}  // namespace vtg
+}  // namespace
 
+namespace vtg {
+
+vt::BuildError build_blas_device(vt_engine* eng, const float* positions, uint32_t position_floats,
+                                 uint32_t position_stride, const uint32_t* indices, uint32_t index_words,
+                                 uint32_t index_stride, const uint32_t* tri_material_ids, uint32_t max_leaf,
+                                 uint32_t num_bins, uint32_t max_depth, vt::BlasOut& out) {
+  if (vt::BuildError e = vt::check_blas_inputs(positions, position_floats, position_stride, indices, index_words,
+                                               index_stride, max_leaf, num_bins, max_depth))
+    return e;
+  const uint32_t n_tri = index_words / index_stride;
+  out.nodes.clear();
+  if (n_tri == 0) {
+    out.nodes.assign(8, 0u);
+    out.nodes[7] = vt::kLeafFlag;
+    out.reordered_indices.assign(indices, indices + index_words);
+    out.reordered_mat.clear();
+    out.reordered_to_source.clear();
+    return {};
+  }
+  std::vector<GNode> nodes;
+  std::vector<uint32_t> perm;
+  if (vt::BuildError e = build_core(eng, positions, position_floats, position_stride, indices, index_words,
+                                    index_stride, {{0u, n_tri}}, max_leaf, num_bins, max_depth, nodes, perm))
+    return e;
+  return emit_tree(nodes, 0, perm, 0, n_tri, indices, index_stride, tri_material_ids, out);
+}
+
+vt::BuildError build_blas_forest(vt_engine* eng, const std::vector<vt::BlasJobIn>& meshes, uint32_t max_leaf,
+                                 uint32_t num_bins, uint32_t max_depth, std::vector<vt::BlasOut>& outs) {
+  outs.assign(meshes.size(), vt::BlasOut());
+  std::vector<float> pos;
+  std::vector<uint32_t> idx;
+  std::vector<std::pair<uint32_t, uint32_t>> roots;
+  std::vector<uint32_t> root_of(meshes.size(), ~0u), tri_base(meshes.size(), 0u);
+  size_t vsum = 0, tsum = 0;
+  for (const vt::BlasJobIn& m : meshes) vsum += m.position_floats, tsum += m.index_words;
+  if (vsum / 4 > 0xFFFFFFFFull || tsum / 4 > 0xFFFFFFFFull)
+    return vt::BuildError{VT_ERR_RANGE, "device BLAS build: the meshes exceed 2^32 vertices or triangles"};
+  pos.reserve(vsum);
+  idx.reserve(tsum);
+  for (size_t i = 0; i < meshes.size(); ++i) {
+    const vt::BlasJobIn& m = meshes[i];
+    if (vt::BuildError e = vt::check_blas_inputs(m.pos4, m.position_floats, 4, m.idx4, m.index_words, 4, max_leaf,
+                                                 num_bins, max_depth))
+      return e;
+    const uint32_t nt = m.index_words / 4, vbase = (uint32_t)(pos.size() / 4);
+    if (nt == 0) {
+      outs[i].nodes.assign(8, 0u);
+      outs[i].nodes[7] = vt::kLeafFlag;
+      continue;
+    }
+    tri_base[i] = (uint32_t)(idx.size() / 4);
+    root_of[i] = (uint32_t)roots.size();
+    roots.push_back({tri_base[i], tri_base[i] + nt});
+    pos.insert(pos.end(), m.pos4, m.pos4 + m.position_floats);
+    for (uint32_t t = 0; t < nt; ++t) {
+      for (int k = 0; k < 3; ++k) idx.push_back(m.idx4[4 * (size_t)t + k] + vbase);
+      idx.push_back(0u);
+    }
+  }
+  if (roots.empty()) return {};
+  std::vector<GNode> nodes;
+  std::vector<uint32_t> perm;
+  if (vt::BuildError e = build_core(eng, pos.data(), (uint32_t)pos.size(), 4, idx.data(), (uint32_t)idx.size(), 4,
+                                    roots, max_leaf, num_bins, max_depth, nodes, perm))
+    return e;
+  for (size_t i = 0; i < meshes.size(); ++i) {
+    if (root_of[i] == ~0u) continue;
+    const vt::BlasJobIn& m = meshes[i];
+    if (vt::BuildError e = emit_tree(nodes, root_of[i], perm, tri_base[i], m.index_words / 4, m.idx4, 4, m.mats,
+                                     outs[i]))
+      return e;
+  }
+  return {};
+}
+
+}  // namespace vtg
 // Same arguments, checks, outputs and errors as vt_build_blas; the SAH work
 // runs on the engine's device.
 vt_status vt_build_blas_device(vt_engine* eng, const float* positions, uint32_t position_floats,

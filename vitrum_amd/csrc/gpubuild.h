@@ -110,4 +110,10 @@ vt::BuildError build_blas_device(vt_engine* eng, const float* positions, uint32_
                                  uint32_t index_stride, const uint32_t* tri_material_ids, uint32_t max_leaf,
                                  uint32_t num_bins, uint32_t max_depth, vt::BlasOut& out);
 
+// build_blas_device over several meshes at once (stride-4 positions and
+// indices): one tree per mesh, all grown together level by level; outs[i]
+// equals vt::build_blas of mesh i.
+vt::BuildError build_blas_forest(vt_engine* eng, const std::vector<vt::BlasJobIn>& meshes, uint32_t max_leaf,
+                                 uint32_t num_bins, uint32_t max_depth, std::vector<vt::BlasOut>& outs);
+
 }  // namespace vtg
