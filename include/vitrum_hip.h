@@ -491,7 +491,8 @@ vt_status vt_build_blas(const float *positions, uint32_t position_floats,
  * messages and outputs as vt_build_blas, byte for byte.  The tree grows one
  * depth per round (every node of a depth in one launch); the host keeps the
  * node list, sorts the median-fallback ranges and writes the preorder
- * layout.  Allocates and frees its device scratch per call. */
+ * layout.  Its device scratch stays with the engine between calls (it only
+ * grows) and is freed by vt_destroy. */
 vt_status vt_build_blas_device(vt_engine *eng, const float *positions, uint32_t position_floats,
                                uint32_t position_stride, const uint32_t *indices,
                                uint32_t index_words, uint32_t index_stride,

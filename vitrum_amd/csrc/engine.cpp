@@ -296,6 +296,7 @@ vt_status vt_destroy(vt_engine* eng) {
     eng->tmp_counts.release();
     engine_release_cwbvh(eng);
     engine_release_refit(eng);
+    engine_release_gpubuild(eng);
     for (auto& kv : eng->user_allocs) (void)hipFree(kv);
     engine_release_render(eng);
     engine_release_nrc(eng);

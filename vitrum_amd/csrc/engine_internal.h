@@ -68,6 +68,7 @@ struct RenderSceneData;  // one scene's shading data (render.cpp)
 enum : uint32_t { kInjectPackThrow = 1u, kInjectRenderAlloc = 2u, kInjectRenderThrow = 4u };
 struct CwbvhState;   // wide-BVH traversal buffers (cwbvh.cpp)
 struct RefitState;   // device refit buffers (refit.cpp)
+struct GpuBuildScratch;  // device BLAS builder buffers (gpubuild.cpp)
 struct NrcState;     // neural radiance cache (nrc.hip)
 
 struct vt_engine {
@@ -109,6 +110,7 @@ struct vt_engine {
   uint32_t device_build_min = 0;  // meshes with this many triangles build on the device (0: none)
   void* noalloc_scratch = nullptr;  // this device's copy of noalloc.hip's scratch array
   RefitState* refit = nullptr;
+  GpuBuildScratch* gpubuild = nullptr;  // kept between builds, freed with the engine
   // Counts scene changes (vt_set_scene, refits): recorded frame graphs
   // depend on the scene they were recorded with.
   uint64_t scene_generation = 0;
@@ -157,6 +159,7 @@ void engine_render_rebound(vt_engine* e);
 // scene changes; everything is freed with the engine.
 void engine_refit_scene_changed(vt_engine* e);
 void engine_release_refit(vt_engine* e);
+void engine_release_gpubuild(vt_engine* e);
 void engine_release_nrc(vt_engine* e);
 // Wide-BVH (cwbvh-closest) traversal state (cwbvh.cpp): built on first use
 // per scene, released on scene change.

@@ -16,6 +16,27 @@
 #include "engine_internal.h"
 #include "gpubuild.h"
 
+// Device buffers of the builder, kept by the engine between builds (they
+// only grow) and freed with it.
+struct GpuBuildScratch {
+  DeviceBuffer pos, idx, bmin, bmax, cen, perm, tmp, bin_of;
+  DeviceBuffer lbegin, lend, lwant, nbox, cbox, decision, nleft, pnodes;
+  DeviceBuffer wslot, wbox, wcen, wbins, wfirst, wcount, chunks, cleft, lnodes;
+  void release() {
+    for (DeviceBuffer* b : {&pos, &idx, &bmin, &bmax, &cen, &perm, &tmp, &bin_of, &lbegin, &lend, &lwant, &nbox,
+                            &cbox, &decision, &nleft, &pnodes, &wslot, &wbox, &wcen, &wbins, &wfirst, &wcount,
+                            &chunks, &cleft, &lnodes})
+      b->release();
+  }
+};
+
+void engine_release_gpubuild(vt_engine* e) {
+  if (!e->gpubuild) return;
+  e->gpubuild->release();
+  delete e->gpubuild;
+  e->gpubuild = nullptr;
+}
+
 namespace {
 
 #define GB_HIP(eng, expr)                                                                  \
@@ -28,19 +49,6 @@ struct GNode {
   uint32_t begin, end, depth;
   uint32_t left = ~0u, right = ~0u, axis = 0;
   float box[6];
-};
-
-// Device buffers of one build, freed when it ends.
-struct Scratch {
-  DeviceBuffer pos, idx, bmin, bmax, cen, perm, tmp, bin_of;
-  DeviceBuffer lbegin, lend, lwant, nbox, cbox, decision, nleft, pnodes;
-  DeviceBuffer wslot, wbox, wcen, wbins, wfirst, wcount, chunks, cleft, lnodes;
-  ~Scratch() {
-    for (DeviceBuffer* b : {&pos, &idx, &bmin, &bmax, &cen, &perm, &tmp, &bin_of, &lbegin, &lend, &lwant, &nbox,
-                            &cbox, &decision, &nleft, &pnodes, &wslot, &wbox, &wcen, &wbins, &wfirst, &wcount,
-                            &chunks, &cleft, &lnodes})
-      b->release();
-  }
 };
 
 // A node with more triangles than kWideMin is reduced, binned and
@@ -65,7 +73,8 @@ vt::BuildError build_core(vt_engine* eng, const float* positions, uint32_t posit
     const uint32_t n_tri = index_words / index_stride;
     hipStream_t s = eng->stream;
     GB_HIP(eng, hipSetDevice(eng->device));
-    Scratch b;
+    if (!eng->gpubuild) eng->gpubuild = new GpuBuildScratch();
+    GpuBuildScratch& b = *eng->gpubuild;
     const size_t n = n_tri;
     // A level holds at most n_tri nodes: every node owns at least one
     // triangle; wide nodes own more than kWideMin each.
